@@ -1,0 +1,1338 @@
+// =====================================================================================================================
+// spb6_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement (C++17 + OpenMP, FP64) of the semi-implicit IMEX-RK time step of diffusionIMEX's SPB6_PC_IMEX /
+// SPB6_PC_IMEX_omp solvers, used as the parity oracle of the HIP path and as the timed CPU baseline. Nothing in the
+// product library links, includes or calls this file; only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+// leg load it (through ctypes).
+//
+// Scope: stiffened-gas phases, relaxation level 0 (instantaneous p closure, CL_P), MUSCL-minmod + Rusanov (local speed),
+// LSDIRK2 / backward Euler, transmissive / wall / periodic boundaries -- the hot path. Each function cites the reference
+// function it restates ("PC" = src/SPB6_PC_IMEX.cpp, "PCO" = src/SPB6_PC_IMEX_omp.cpp, "PHY" = src/spb6_physics.h,
+// "PHO" = src/spb6_physics_omp.h). Arithmetic is written operation-for-operation in the reference's evaluation order and
+// must be compiled with -ffp-contract=off (no FMA contraction), like the reference's x86-64 -O2 build.
+//
+// Selectable variants (OraOpts):
+//   closure_path : 0 = PHY closure_p (bracket first, PHY:462-494), 1 = PHO closure_p (Newton from the guess, PHO:472-548)
+//   share_start  : 0 = PHY shock_share bracket [-40, 40] (PHY:363), 1 = PHO weak-shock-centred bracket (PHO:365-374)
+//   dot_order    : 0 = block_sum B=4096 (PC:717-722), 1 = B=1024 (PCO:744-749), 2 = the HIP kernels' fixed tree order
+//   portable_math: 0 = libm exp/log/pow/hypot (as the reference), 1 = the project's portable routines (bit-identical on
+//                  host and device; used for GPU-vs-oracle bit parity)
+//
+// Parity pins (tests/test_oracle_*.py): the reference's archived full-precision profiles under data/ (cases 4, 5, 80-88),
+// closed forms (EOS, weak-shock share, Hugoniot), invariants (closure root, energy consistency, conservation), brute-force
+// bisection for closure_p, and a dense solve for the Thomas / BiCGStab linear algebra.
+// =====================================================================================================================
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <cstdint>
+#include <vector>
+#include <algorithm>
+#include <limits>
+#include <chrono>
+#include <string>
+#include <omp.h>
+
+namespace ora {
+
+static const double INF = std::numeric_limits<double>::infinity();
+enum { IA1 = 0, IM1 = 1, IM2 = 2, IMU = 3, IMV = 4, IE1 = 5, IE2 = 6, IEN = 7 };   // PHY:25-26
+static const int NV = 8, GH = 2;                                                   // PHY:25, PC:55
+enum { BC_TRANS = 0, BC_WALL = 1, BC_PERIODIC = 2 };                               // PHY:906
+
+// std::max / std::min semantics (first argument returned on ties and on NaN in the second comparison operand)
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+
+// ------------------------------------------------------------------------------------------------ portable math
+// Used in place of libm where the HIP kernels must produce the same bits (shock_share, reconstruction scales).
+// exp: Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, degree-13 Taylor polynomial in Horner form, exact 2^k scaling.
+// log: x = 2^e m, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(f/(2+f)) by its odd series, plus e ln2 in two parts.
+// pow(x, y) = exp(y log x) for x > 0. All accurate to a few ulp (checked against libm in tests).
+static const double LN2_HI = 6.93147180369123816490e-01;   // ln 2 rounded to 32 significant bits
+static const double LN2_LO = 1.90821492927058770002e-10;   // ln 2 - LN2_HI
+static const double INV_LN2 = 1.44269504088896338700e+00;
+static double pexp(double x) {
+	if (x != x) return x;
+	if (x > 709.782712893384) return INF;
+	if (x < -745.1332191019412) return 0.0;
+	const double k = std::floor(x * INV_LN2 + 0.5);
+	const double r = (x - k * LN2_HI) - k * LN2_LO;
+	double p = 1.0 / 6227020800.0;                 // 1/13!
+	p = 1.0 / 479001600.0 + r * p;                 // 1/12!
+	p = 1.0 / 39916800.0 + r * p;
+	p = 1.0 / 3628800.0 + r * p;
+	p = 1.0 / 362880.0 + r * p;
+	p = 1.0 / 40320.0 + r * p;
+	p = 1.0 / 5040.0 + r * p;
+	p = 1.0 / 720.0 + r * p;
+	p = 1.0 / 120.0 + r * p;
+	p = 1.0 / 24.0 + r * p;
+	p = 1.0 / 6.0 + r * p;
+	p = 0.5 + r * p;
+	p = 1.0 + r * p;
+	p = 1.0 + r * p;
+	const int ki = (int)k;
+	if (ki > 1000) return std::ldexp(std::ldexp(p, ki - 1000), 1000);
+	if (ki < -1000) return std::ldexp(std::ldexp(p, ki + 1000), -1000);
+	return std::ldexp(p, ki);
+}
+static double plog(double x) {
+	if (x != x) return x;
+	if (x < 0.0) return std::numeric_limits<double>::quiet_NaN();
+	if (x == 0.0) return -INF;
+	if (x == INF) return INF;
+	int e; double m = std::frexp(x, &e);           // x = m 2^e, m in [0.5, 1)
+	if (m < 0.70710678118654752440) { m = m + m; e = e - 1; }
+	const double f = m - 1.0, s = f / (2.0 + f), z = s * s;
+	double R = 1.0 / 23.0;
+	R = 1.0 / 21.0 + z * R;
+	R = 1.0 / 19.0 + z * R;
+	R = 1.0 / 17.0 + z * R;
+	R = 1.0 / 15.0 + z * R;
+	R = 1.0 / 13.0 + z * R;
+	R = 1.0 / 11.0 + z * R;
+	R = 1.0 / 9.0 + z * R;
+	R = 1.0 / 7.0 + z * R;
+	R = 1.0 / 5.0 + z * R;
+	R = 1.0 / 3.0 + z * R;
+	R = z * R;                                     // log m = 2 s + 2 s R, and 2 s = f - s f since s (2 + f) = f
+	const double logm = f - s * (f - 2.0 * R);
+	const double de = (double)e;
+	return de * LN2_HI + (logm + de * LN2_LO);
+}
+static double ppow(double x, double y) { return pexp(y * plog(x)); }
+static double phypot(double a, double b) { return std::sqrt(a * a + b * b); }
+
+// ------------------------------------------------------------------------------------------------ options / model
+struct Phase { double g, pi, q, qp, cv, b; };   // PHY:49 (SG / NASG subset)
+
+struct Model {
+	Phase PH[2];
+	int closure_path = 1, share_start = 1, dot_order = 2, portable = 1;
+	double exp_(double x) const { return portable ? pexp(x) : std::exp(x); }
+	double log_(double x) const { return portable ? plog(x) : std::log(x); }
+	double pow_(double x, double y) const { return portable ? ppow(x, y) : std::pow(x, y); }
+	double hypot_(double a, double b) const { return portable ? phypot(a, b) : std::hypot(a, b); }
+};
+
+// ------------------------------------------------------------------------------------------------ EOS (PHY:69-109)
+static inline double eps_amp(const Model& M, int k, double a, double m, double p) { const Phase& P = M.PH[k];
+	return (p + P.g * P.pi) * (a - m * P.b) / (P.g - 1.0) + m * P.q; }                               // PHY:69-71
+static inline double p_ame(const Model& M, int k, double a, double m, double e) { const Phase& P = M.PH[k];
+	return (e - m * P.q) * (P.g - 1.0) / (a - m * P.b) - P.g * P.pi; }                               // PHY:72-74
+static inline double T_amp(const Model& M, int k, double a, double m, double p) { const Phase& P = M.PH[k];
+	return (a / m - P.b) * (p + P.pi) / ((P.g - 1.0) * P.cv); }                                      // PHY:75-77
+static inline double c2_rp(const Model& M, int k, double r, double p) { const Phase& P = M.PH[k];
+	return P.g * (p + P.pi) / (r * (1.0 - r * P.b)); }                                               // PHY:83-85
+static inline double dedp_am(const Model& M, int k, double a, double m) { const Phase& P = M.PH[k];
+	return (a - m * P.b) / (P.g - 1.0); }                                                             // PHY:86-88
+static double p_mix(const Model& M, double a1, double m1, double m2, double rhoe) {                 // PHY:90-99
+	const double a[2] = {a1, 1.0 - a1}, m[2] = {m1, m2};
+	double num = rhoe, den = 0.0;
+	for (int k = 0; k < 2; k++) {
+		num -= m[k] * M.PH[k].q + M.PH[k].g * M.PH[k].pi * dedp_am(M, k, a[k], m[k]);
+		den += dedp_am(M, k, a[k], m[k]);
+	}
+	return num / den;
+}
+static inline bool p_admissible(const Model& M, int k, double p) { return p + M.PH[k].pi > 0; }     // PHY:101-104 (SG)
+static inline double eps_round(const Model& M, int k, double m) { return std::fabs(m * M.PH[k].q); } // PHY:106-109 (SG)
+
+// ------------------------------------------------------------------------------------------------ primitives (PHY:835-857)
+struct Prim { double a1, r1, r2, u, v, p1, p2, p, rho, cf; };
+static bool prim_of(const Model& M, const double* Q, Prim& w) {                                     // PHY:837-847
+	const double a1 = Q[IA1], a2 = 1 - a1, m1 = Q[IM1], m2 = Q[IM2];
+	if (!(a1 > 0 && a2 > 0 && m1 > 0 && m2 > 0)) return false;
+	w.a1 = a1; w.rho = m1 + m2; w.r1 = m1 / a1; w.r2 = m2 / a2; w.u = Q[IMU] / w.rho; w.v = Q[IMV] / w.rho;
+	w.p = p_mix(M, a1, m1, m2, Q[IEN] - 0.5 * w.rho * (w.u * w.u + w.v * w.v));
+	w.p1 = p_ame(M, 0, a1, m1, Q[IE1]); w.p2 = p_ame(M, 1, a2, m2, Q[IE2]);
+	if (!(p_admissible(M, 0, w.p1) && p_admissible(M, 1, w.p2))) return false;
+	w.cf = std::sqrt((m1 * c2_rp(M, 0, w.r1, w.p1) + m2 * c2_rp(M, 1, w.r2, w.p2)) / w.rho);
+	return true;
+}
+static void prim2cell(const Model& M, const Prim& w, double* Q) {                                   // PHY:849-854
+	const double a2 = 1 - w.a1, m1 = w.a1 * w.r1, m2 = a2 * w.r2, rho = m1 + m2;
+	Q[IA1] = w.a1; Q[IM1] = m1; Q[IM2] = m2; Q[IMU] = rho * w.u; Q[IMV] = rho * w.v;
+	Q[IE1] = eps_amp(M, 0, w.a1, m1, w.p1); Q[IE2] = eps_amp(M, 1, a2, m2, w.p2);
+	Q[IEN] = Q[IE1] + Q[IE2] + 0.5 * rho * (w.u * w.u + w.v * w.v);
+}
+static inline double pbar_of(const Model& M, const double* Q) {                                     // PHY:855-857
+	const double a1 = Q[IA1];
+	return a1 * p_ame(M, 0, a1, Q[IM1], Q[IE1]) + (1 - a1) * p_ame(M, 1, 1 - a1, Q[IM2], Q[IE2]);
+}
+
+// ------------------------------------------------------------------------------------------------ shock budget (PHY:260-418)
+static inline void eos_ep_ev(const Model& M, int k, double p, double v, double& ep, double& evp) {  // PHY:266-270 (SG)
+	const Phase& P = M.PH[k]; ep = (v - P.b) / (P.g - 1.0); evp = P.g * (p + P.pi) / (P.g - 1.0); }
+static inline bool eos_ok(const Model& M, int k, double p, double v) {                             // PHY:271-275 (SG)
+	if (!(v > 0) || !std::isfinite(p)) return false;
+	return v > M.PH[k].b && p + M.PH[k].pi > 0; }
+static double hug_v(const Model& M, int k, double p0, double p1, double v0) {                      // PHY:277-282 (SG)
+	const Phase& P = M.PH[k]; const double x0 = p0 + P.pi, x1 = p1 + P.pi;
+	return P.b + (v0 - P.b) * ((P.g - 1.0) * x1 + (P.g + 1.0) * x0) / ((P.g + 1.0) * x1 + (P.g - 1.0) * x0);
+}
+static void K_G(const Model& M, int k, double p, double v, double& K, double& G) {                 // PHY:294-299 (SG)
+	K = c2_rp(M, k, 1.0 / v, p) / v;
+	G = 0.5 * (M.PH[k].g + 1.0); if (M.PH[k].b != 0.0) G *= v / (v - M.PH[k].b);
+}
+struct ShockProf { double Y[2], vk0[2], v0, p0, j2, vend; };
+static bool prof_rhs(const Model& M, const ShockProf& S, double c, double v, const double* y, double* dy) {   // PHY:302-310
+	const double p = y[0], v1 = y[1], v2 = (v - S.Y[0] * v1) / S.Y[1], P = S.p0 + S.j2 * (S.v0 - v);
+	if (!eos_ok(M, 0, p, v1) || !eos_ok(M, 1, p, v2)) return false;
+	const double lam[2] = {c, 1.0 - c}, vk[2] = {v1, v2};
+	double num = -1.0, den = 0.0, r[2], ep[2], evp[2];
+	for (int k = 0; k < 2; k++) {
+		eos_ep_ev(M, k, p, vk[k], ep[k], evp[k]);
+		r[k] = lam[k] / S.Y[k] * (p - P); num += S.Y[k] * r[k] / evp[k]; den += S.Y[k] * ep[k] / evp[k];
+	}
+	dy[0] = num / den; dy[1] = (r[0] - ep[0] * dy[0]) / evp[0]; dy[2] = p - P;
+	return std::isfinite(dy[0]) && std::isfinite(dy[1]);
+}
+// Cash-Karp embedded RK4(5) coefficients (Cash & Karp 1990)
+static const double CK_A[6] = {0, 0.2, 0.3, 0.6, 1.0, 0.875};
+static const double CK_B[6][5] = {{0}, {0.2}, {3.0 / 40, 9.0 / 40}, {0.3, -0.9, 1.2}, {-11.0 / 54, 2.5, -70.0 / 27, 35.0 / 27},
+	{1631.0 / 55296, 175.0 / 512, 575.0 / 13824, 44275.0 / 110592, 253.0 / 4096}};
+static const double CK_C5[6] = {37.0 / 378, 0, 250.0 / 621, 125.0 / 594, 0, 512.0 / 1771};
+static const double CK_C4[6] = {2825.0 / 27648, 0, 18575.0 / 48384, 13525.0 / 55296, 277.0 / 14336, 0.25};
+static bool prof_integrate(const Model& M, const ShockProf& S, double c, double& v1end, double& qend) {   // PHY:311-341
+	double y[3] = {S.p0, S.vk0[0], 0.0}, v = S.v0;
+	const double L = S.vend - S.v0, sc[3] = {std::fabs(S.p0) + S.j2 * std::fabs(L), S.vk0[0], S.j2 * L * L + 1e-300};
+	double h = L / 32.0;
+	for (int it = 0; it < 4000; it++) {
+		const bool last = (L < 0 && v + h <= S.vend) || (L > 0 && v + h >= S.vend);
+		if (last) h = S.vend - v;
+		double k[6][3], yt[3];
+		bool ok = true;
+		for (int st = 0; st < 6 && ok; st++) {
+			for (int n = 0; n < 3; n++) { yt[n] = y[n]; for (int q = 0; q < st; q++) yt[n] += h * CK_B[st][q] * k[q][n]; }
+			ok = prof_rhs(M, S, c, v + CK_A[st] * h, yt, k[st]);
+		}
+		double err = 0.0, y5[3];
+		if (ok) for (int n = 0; n < 3; n++) {
+			double d5 = 0, d4 = 0; for (int st = 0; st < 6; st++) { d5 += CK_C5[st] * k[st][n]; d4 += CK_C4[st] * k[st][n]; }
+			y5[n] = y[n] + h * d5; err = smax(err, std::fabs(h * (d5 - d4)) / sc[n]);
+		}
+		if (!ok || !(err <= 1e-8)) {
+			h *= ok ? smax(0.1, 0.9 * M.pow_(1e-8 / err, 0.25)) : 0.25;
+			if (std::fabs(h) < 1e-14 * std::fabs(L)) return false;
+			continue;
+		}
+		for (int n = 0; n < 3; n++) y[n] = y5[n];
+		if (last) { v1end = y[1]; qend = y[2]; return true; }
+		v += h;
+		h *= smin(5.0, 0.9 * M.pow_(1e-8 / smax(err, 1e-30), 0.2));
+	}
+	return false;
+}
+static bool shock_share(const Model& M, const double Y[2], const double vpre[2], double p0, double p1, double lam[2], double& q) {   // PHY:343-383, PHO:365-374
+	double K[2], G[2], w[2], sw = 0.0, Kmin = INF;
+	for (int k = 0; k < 2; k++) { K_G(M, k, p0, vpre[k], K[k], G[k]); if (!(K[k] > 0)) return false; Kmin = smin(Kmin, K[k]); }
+	const double dp = p1 - p0;
+	if (dp < 1e-2 * Kmin) {   // weak shock: closed form
+		q = 0.0;
+		for (int k = 0; k < 2; k++) { w[k] = Y[k] * vpre[k] * G[k] / (K[k] * K[k]); sw += w[k]; }
+		for (int k = 0; k < 2; k++) lam[k] = w[k] / sw;
+		q = sw * dp * dp * dp / 6.0;
+		return true;
+	}
+	ShockProf S; S.p0 = p0;
+	double vH[2];
+	for (int k = 0; k < 2; k++) { S.Y[k] = Y[k]; S.vk0[k] = vpre[k]; vH[k] = hug_v(M, k, p0, p1, vpre[k]); if (!std::isfinite(vH[k])) return false; }
+	S.v0 = Y[0] * vpre[0] + Y[1] * vpre[1]; S.vend = Y[0] * vH[0] + Y[1] * vH[1];
+	if (!(S.vend < S.v0)) return false;
+	S.j2 = dp / (S.v0 - S.vend);
+	auto F = [&](double t, double& qq) -> double {
+		const double c = 1.0 / (1.0 + M.exp_(-t)); double v1e;
+		if (!prof_integrate(M, S, c, v1e, qq)) return NAN;
+		return (v1e - vH[0]) / vpre[0]; };
+	double ta, tb, qa, qb, fa, fb;
+	if (M.share_start == 0) { ta = -40.0; tb = 40.0; fa = F(ta, qa); fb = F(tb, qb); if (!std::isfinite(fa) || !std::isfinite(fb)) return false; }
+	else {
+		double t0;
+		{ double ww[2]; for (int k = 0; k < 2; k++) ww[k] = Y[k] * vpre[k] * G[k] / (K[k] * K[k]);
+		  const double c0 = smin(1.0 - 1e-12, smax(1e-12, ww[0] / (ww[0] + ww[1])));
+		  t0 = std::isfinite(c0) ? M.log_(c0 / (1.0 - c0)) : 0.0; }
+		ta = smax(-40.0, t0 - 1.0); tb = smin(40.0, t0 + 1.0); fa = F(ta, qa); fb = F(tb, qb);
+		if (!std::isfinite(fa) || !std::isfinite(fb)) return false;
+		for (double st = 2.0; fa >= 0 && ta > -40.0; st *= 2.0) { tb = ta; fb = fa; qb = qa; ta = smax(-40.0, ta - st); fa = F(ta, qa); if (!std::isfinite(fa)) return false; }
+		for (double st = 2.0; fb <= 0 && tb < 40.0; st *= 2.0) { ta = tb; fa = fb; qa = qb; tb = smin(40.0, tb + st); fb = F(tb, qb); if (!std::isfinite(fb)) return false; }
+	}
+	double t, qt;
+	if (fa >= 0) { t = ta; qt = qa; }
+	else if (fb <= 0) { t = tb; qt = qb; }
+	else {   // Illinois on the logit of the share
+		int side = 0; t = ta; qt = qa;
+		for (int it = 0; it < 100; it++) {
+			t = (ta * fb - tb * fa) / (fb - fa);
+			if (!(t > ta && t < tb)) t = 0.5 * (ta + tb);
+			const double ft = F(t, qt);
+			if (!std::isfinite(ft)) return false;
+			if (std::fabs(ft) < 1e-10 || tb - ta < 1e-10) break;
+			if (ft < 0) { ta = t; fa = ft; if (side == -1) fb *= 0.5; side = -1; }
+			else { tb = t; fb = ft; if (side == 1) fa *= 0.5; side = 1; }
+		}
+	}
+	const double c = 1.0 / (1.0 + M.exp_(-t));
+	lam[0] = c; lam[1] = 1.0 - c; q = qt;
+	return q > 0;
+}
+static void stencil_state(const Model& M, const double* c, double* st) {                          // PHY:385-390
+	const double a1 = c[IA1], m1 = c[IM1], m2 = c[IM2], r = m1 + m2;
+	st[0] = a1; st[1] = 1.0 - a1; st[2] = m1; st[3] = m2; st[4] = NAN;
+	st[5] = r > 0 ? c[IMU] / r : 0.0; st[6] = r > 0 ? c[IMV] / r : 0.0;
+	if (a1 > 0 && a1 < 1 && m1 > 0 && m2 > 0) {
+		const double p = p_mix(M, a1, m1, m2, c[IE1] + c[IE2]);
+		if (p_admissible(M, 0, p) && p_admissible(M, 1, p)) st[4] = p;
+	}
+}
+// cellptr(di, dj) returns a pointer to the 8 conserved values of the stencil cell
+template <class F> static void shock_budget(const Model& M, F cellptr, bool twoD, double Q[2]) {  // PHY:395-418
+	Q[0] = Q[1] = 0.0;
+	double c0[7], st[7], lo[7] = {0, 0, 0, 0, 0, 0, 0}, plo = 1e300, phi = -1e300, div = 0.0;
+	stencil_state(M, cellptr(0, 0), c0);
+	for (int d = 0; d < (twoD ? 2 : 1); d++) for (int q = -1; q <= 1; q++) {
+		if (d == 1 && q == 0) continue;
+		stencil_state(M, cellptr(d == 0 ? q : 0, d == 1 ? q : 0), st);
+		if (q == 1) div += st[5 + d]; else if (q == -1) div -= st[5 + d];
+		if (!std::isfinite(st[4])) continue;
+		if (st[4] < plo) { plo = st[4]; for (int n = 0; n < 7; n++) lo[n] = st[n]; }
+		if (st[4] > phi) phi = st[4];
+	}
+	if (!(div < 0.0) || !(phi > plo)) return;
+	double Y[2], vpre[2], lam[2], q;
+	const double rho = c0[2] + c0[3];
+	if (!(rho > 0)) return;
+	for (int k = 0; k < 2; k++) {
+		Y[k] = c0[2 + k] / rho;
+		vpre[k] = (lo[k] > 0 && lo[2 + k] > 0) ? lo[k] / lo[2 + k] : (c0[2 + k] > 0 ? c0[k] / c0[2 + k] : 0.0);
+	}
+	if (!(Y[0] > 0) || !(Y[1] > 0) || !(vpre[0] > 0) || !(vpre[1] > 0)) return;
+	if (!shock_share(M, Y, vpre, plo, phi, lam, q)) return;
+	for (int k = 0; k < 2; k++) Q[k] = lam[k] * rho * q;
+}
+
+// ------------------------------------------------------------------------------------------------ p closure (PHY:420-494, PHO:472-548)
+struct ClIn { double atr[2], mtr[2], etr[2], s, rhoeC, pguess, Qph[2]; };
+struct ClOut { double a[2], m[2], e[2], p[2], T[2], pbar; };
+struct PCoef { double A[2], B[2], N[2], lam[2]; };
+static double E_above(const Model& M, const ClIn& in, int k) {                                     // PHY:420-423 (SG)
+	return smax(0.0, in.etr[k] - in.mtr[k] * M.PH[k].q - (in.atr[k] - in.mtr[k] * M.PH[k].b) * M.PH[k].pi); }
+static void defect_split(const Model& M, const ClIn& in, double delta, double lam[2], double extra[2]) {   // PHY:428-439 (SHARE_MODE 0)
+	double E[2], sE = 0.0; for (int k = 0; k < 2; k++) { E[k] = E_above(M, in, k); sE += E[k]; }
+	double lE[2];
+	if (sE > 0) for (int k = 0; k < 2; k++) lE[k] = E[k] / sE;
+	else { const double sm = in.mtr[0] + in.mtr[1]; for (int k = 0; k < 2; k++) lE[k] = sm > 0 ? in.mtr[k] / sm : 0.5; }
+	const double Q = in.Qph[0] + in.Qph[1];
+	extra[0] = extra[1] = 0.0;
+	if (!(delta > 0.0) || !(Q > 0.0)) { lam[0] = lE[0]; lam[1] = lE[1]; return; }
+	if (delta < Q) { lam[0] = in.Qph[0] / Q; lam[1] = in.Qph[1] / Q; return; }
+	for (int k = 0; k < 2; k++) { lam[k] = lE[k]; extra[k] = in.Qph[k] - Q * lE[k]; }
+}
+static bool p_coef(const Model& M, const ClIn& in, PCoef& c, double pseg) {                       // PHY:443-454
+	if (!(in.mtr[0] + in.mtr[1] > 0)) return false;
+	const double D0 = in.rhoeC - in.etr[0] - in.etr[1];
+	double ex[2] = {0.0, 0.0};
+	defect_split(M, in, D0 + pseg * in.s, c.lam, ex);
+	for (int k = 0; k < 2; k++) {
+		c.A[k] = in.etr[k] - in.mtr[k] * M.PH[k].q + c.lam[k] * D0 + ex[k];
+		c.B[k] = in.atr[k] + c.lam[k] * in.s;
+		c.N[k] = c.A[k] - (c.B[k] - in.mtr[k] * M.PH[k].b) * M.PH[k].pi;
+	}
+	return true;
+}
+static inline double alpha_p(const Model& M, const PCoef& c, const ClIn& in, int k, double p) { const Phase& P = M.PH[k];   // PHY:455
+	return ((P.g - 1.0) * (c.A[k] + c.B[k] * p) + (p + P.g * P.pi) * in.mtr[k] * P.b) / (P.g * (p + P.pi)); }
+static inline double dalpha_p(const Model& M, const PCoef& c, int k, double p) { const Phase& P = M.PH[k];                   // PHY:456
+	return -(P.g - 1.0) * c.N[k] / (P.g * (p + P.pi) * (p + P.pi)); }
+static inline double plo_mix(const Model& M) { return smax(-M.PH[0].pi, -M.PH[1].pi); }            // PHY:458
+
+static bool closure_out(const Model& M, const ClIn& in, double p, ClOut& o) {                      // PHY:487-493
+	PCoef c; p_coef(M, in, c, p);
+	for (int k = 0; k < 2; k++) {
+		o.a[k] = alpha_p(M, c, in, k, p); o.m[k] = in.mtr[k];
+		if (!(o.a[k] > o.m[k] * M.PH[k].b) || !(p + M.PH[k].pi > 0)) return false;
+		o.e[k] = eps_amp(M, k, o.a[k], o.m[k], p); o.p[k] = p; o.T[k] = T_amp(M, k, o.a[k], o.m[k], p);
+	}
+	o.pbar = p; return std::isfinite(p);
+}
+static bool closure_p_phy(const Model& M, const ClIn& in, ClOut& o) {                              // PHY:462-494
+	PCoef c; if (!p_coef(M, in, c, in.pguess)) return false;
+	const double plo = plo_mix(M);
+	auto Phi = [&](double p) { PCoef q; p_coef(M, in, q, p); return alpha_p(M, q, in, 0, p) + alpha_p(M, q, in, 1, p) - 1.0; };
+	auto dPhi = [&](double p) { PCoef q; p_coef(M, in, q, p); return dalpha_p(M, q, 0, p) + dalpha_p(M, q, 1, p); };
+	double d = smax(in.pguess - plo, 1e-3 * (std::fabs(plo) + 1.0));
+	double lo = plo, hi = plo + d; double fh = Phi(hi);
+	if (fh > 0) { lo = hi; for (int k = 0; k < 200 && fh > 0; k++) { d *= 4.0; hi = plo + d; fh = Phi(hi); } if (fh > 0) return false; }
+	else { bool found = false; for (int k = 0; k < 200; k++) { const double dd = d * 0.25, pl = plo + dd, fl = Phi(pl); if (fl > 0) { lo = pl; found = true; break; } hi = pl; fh = fl; d = dd; } if (!found) return false; }
+	double p = 0.5 * (lo + hi); if (in.pguess > lo && in.pguess < hi) p = in.pguess;
+	for (int it = 0; it < 200; it++) {
+		const double f = Phi(p);
+		if (f == 0.0) break;
+		if (f > 0) lo = p; else hi = p;
+		const double df = dPhi(p);
+		double pn = df < 0 ? p - f / df : 0.5 * (lo + hi);
+		if (!(pn > lo && pn < hi)) pn = (hi - plo > 1e3 * (lo - plo + 1e-300) && lo > plo) ? plo + std::sqrt((lo - plo) * (hi - plo)) : 0.5 * (lo + hi);
+		if (std::fabs(pn - p) <= 2e-16 * (std::fabs(p) + std::fabs(plo)) || hi - lo <= 4e-16 * (std::fabs(hi) + std::fabs(lo))) { p = pn; break; }
+		p = pn;
+	}
+	return closure_out(M, in, p, o);
+}
+static bool closure_p_pho(const Model& M, const ClIn& in, ClOut& o) {                              // PHO:472-548
+	if (!(in.mtr[0] + in.mtr[1] > 0)) return false;
+	const double plo = plo_mix(M);
+	const double D0 = in.rhoeC - in.etr[0] - in.etr[1];
+	double lE[2], A0[2], sm = in.mtr[0] + in.mtr[1];
+	{ double E[2], sE = 0.0; for (int k = 0; k < 2; k++) { E[k] = E_above(M, in, k); sE += E[k]; }
+	  if (sE > 0) for (int k = 0; k < 2; k++) lE[k] = E[k] / sE;
+	  else for (int k = 0; k < 2; k++) lE[k] = sm > 0 ? in.mtr[k] / sm : 0.5; }
+	const double Qb = in.Qph[0] + in.Qph[1];
+	for (int k = 0; k < 2; k++) A0[k] = in.etr[k] - in.mtr[k] * M.PH[k].q;
+	auto PhiD = [&](double p, double& df) {
+		double lam[2], ex[2] = {0.0, 0.0};
+		const double delta = D0 + p * in.s;
+		if (!(delta > 0.0) || !(Qb > 0.0)) { lam[0] = lE[0]; lam[1] = lE[1]; }
+		else if (delta < Qb) { lam[0] = in.Qph[0] / Qb; lam[1] = in.Qph[1] / Qb; }
+		else for (int k = 0; k < 2; k++) { lam[k] = lE[k]; ex[k] = in.Qph[k] - Qb * lE[k]; }
+		double f = 0.0; df = 0.0;
+		for (int k = 0; k < 2; k++) {
+			const Phase& P = M.PH[k];
+			const double A = A0[k] + lam[k] * D0 + ex[k], B = in.atr[k] + lam[k] * in.s, N = A - (B - in.mtr[k] * P.b) * P.pi;
+			f += ((P.g - 1.0) * (A + B * p) + (p + P.g * P.pi) * in.mtr[k] * P.b) / (P.g * (p + P.pi));
+			df += -(P.g - 1.0) * N / (P.g * (p + P.pi) * (p + P.pi));
+		}
+		return f - 1.0; };
+	double p = plo + smax(in.pguess - plo, 1e-3 * (std::fabs(plo) + 1.0));
+	double lo = plo, hi = INF;
+	bool conv = false;
+	for (int it = 0; it < 400; it++) {
+		double df; const double f = PhiD(p, df);
+		if (f == 0.0) { conv = true; break; }
+		if (f > 0) lo = p; else hi = p;
+		double pn = df < 0 ? p - f / df : NAN;
+		if (std::fabs(pn - p) <= 2e-16 * (std::fabs(p) + std::fabs(plo))) { p = pn; conv = true; break; }
+		if (!(pn > lo && pn < hi)) {
+			if (!std::isfinite(hi)) pn = plo + 4.0 * (p - plo);
+			else if (lo == plo) pn = plo + 0.25 * (hi - plo);
+			else pn = (hi - plo > 1e3 * (lo - plo + 1e-300) && lo > plo) ? plo + std::sqrt((lo - plo) * (hi - plo)) : 0.5 * (lo + hi);
+		}
+		if (std::fabs(pn - p) <= 2e-16 * (std::fabs(p) + std::fabs(plo)) || (hi < INF && hi - lo <= 4e-16 * (std::fabs(hi) + std::fabs(lo)))) { p = pn; conv = true; break; }
+		p = pn;
+	}
+	if (!conv) return false;
+	return closure_out(M, in, p, o);
+}
+static bool closure(const Model& M, const ClIn& in, ClOut& o) {                                    // PHY:786-810 (CL_P)
+	for (int k = 0; k < 2; k++) if (!(in.atr[k] > 0) || !(in.mtr[k] > 0) || !std::isfinite(in.etr[k])) return false;
+	if (!std::isfinite(in.rhoeC) || !std::isfinite(in.s)) return false;
+	const bool ok = M.closure_path == 0 ? closure_p_phy(M, in, o) : closure_p_pho(M, in, o);
+	return ok && std::isfinite(o.pbar);
+}
+static double closure_roundoff(const Model& M, const ClIn& in, double p) {                         // PC:631-638
+	PCoef c; if (!p_coef(M, in, c, p)) return 0.0;
+	double dphi = 0.0, sens = 0.0;
+	for (int k = 0; k < 2; k++) { dphi += dalpha_p(M, c, k, p); sens += std::fabs(c.lam[k]) * (M.PH[k].g - 1.0) / (M.PH[k].g * (p + M.PH[k].pi)); }
+	const double en = 2.220446049250313e-16 * (std::fabs(in.rhoeC) + std::fabs(in.etr[0]) + std::fabs(in.etr[1]) + std::fabs(in.mtr[0] * M.PH[0].q) + std::fabs(in.mtr[1] * M.PH[1].q));
+	return dphi < 0 ? en * sens / -dphi : 0.0;
+}
+
+// ------------------------------------------------------------------------------------------------ fixed-order sums
+// Device tree order: blocks of 1024 consecutive terms; in a block, thread t (0..255) adds terms t, t+256, t+512, t+768 in
+// turn (starting from 0.0); each 64-lane wavefront combines with xor-butterflies (offsets 32..1); the block total is
+// (w0 + w1) + (w2 + w3). The block totals are combined the same way by one 256-thread block (thread t adds totals
+// t, t+256, ... in turn). The HIP reduction kernels use exactly this order.
+static double tree256(const double* v) {
+	double w[4];
+	for (int q = 0; q < 4; q++) {
+		double a[64]; for (int l = 0; l < 64; l++) a[l] = v[q * 64 + l];
+		for (int off = 32; off >= 1; off >>= 1) { double b[64]; for (int l = 0; l < 64; l++) b[l] = a[l] + a[l ^ off]; std::memcpy(a, b, sizeof a); }
+		w[q] = a[0];
+	}
+	return (w[0] + w[1]) + (w[2] + w[3]);
+}
+template <class F> static double dev_sum(int n, F term) {
+	const int nb = (n + 1023) / 1024; std::vector<double> part(nb > 0 ? nb : 1, 0.0);
+	#pragma omp parallel for schedule(static)
+	for (int b = 0; b < nb; b++) {
+		double v[256];
+		for (int t = 0; t < 256; t++) { double s = 0.0; for (int it = 0; it < 4; it++) { const int i = b * 1024 + it * 256 + t; if (i < n) s += term(i); } v[t] = s; }
+		part[b] = tree256(v);
+	}
+	double v[256];
+	for (int t = 0; t < 256; t++) { double s = 0.0; for (int c = t; c < nb; c += 256) s += part[c]; v[t] = s; }
+	return tree256(v);
+}
+template <class F> static double blk_sum(int n, int B, F term) {                                   // PC:717-722 / PCO:744-749
+	const int nb = (n + B - 1) / B; std::vector<double> part(nb, 0.0);
+	#pragma omp parallel for schedule(static)
+	for (int c = 0; c < nb; c++) { double s = 0; const int e = std::min(n, (c + 1) * B); for (int r = c * B; r < e; r++) s += term(r); part[c] = s; }
+	double s = 0; for (int c = 0; c < nb; c++) s += part[c]; return s;
+}
+template <class F> static double osum(const Model& M, int n, F term) {
+	if (M.dot_order == 2) return dev_sum(n, term);
+	return blk_sum(n, M.dot_order == 0 ? 4096 : 1024, term);
+}
+
+// ================================================================================================ solver context
+struct Cfg {
+	int nx, ny; double x0, x1, y0, y1;
+	int bc[4]; double bcu[4];
+	double ph[2][6];
+	int rk; double cfl, cfl_ac; int ramp; double dtfix; int acdt;
+	double newton_tol; int newton_iter; double krylov_tol; int krylov_iter;
+	int limiter, order, rus_local;
+	int closure_path, share_start, dot_order, portable_math;
+	int nthreads;
+};
+struct Stats { long steps, rk1, rejects, floor_acc, stages, newton_it, krylov_it, linsolves; double maxres, t, dt_last; int status; };
+
+struct SpRow { int n; int col[5]; double val[5]; };                                                  // PC:708
+
+struct Ctx {
+	Cfg C; Model M;
+	int nx, ny, NXT, NYT; double dx, dy;
+	bool twoD;
+	// grids (AoS, PC:112-131), arrays (PC:133-152)
+	std::vector<double> U, Un, Qe, Qs, Ust, wxL, wxR, wyL, wyR, H[3];
+	std::vector<double> phx, phy, a2s, ubx, uby;
+	// stage data (PC:565-582)
+	double tau, rx, ry;
+	std::vector<double> ust, vst, ubsx, kpx, wLx, pvx, ubsy, kpy, wLy, pvy, Kw, Qph, cz, Pold;
+	// pressure
+	std::vector<double> P, Pst; double P_REF = 0.0;
+	double t = 0.0, dtprev = -1.0; bool startup = true;
+	Stats st{};
+	double Ae[3][3], Ai[3][3];
+	int last_newton_iters = 0, last_krylov = 0;
+	inline size_t id(int i, int j) const { return (size_t)i + (size_t)j * NXT; }
+	inline double& u(std::vector<double>& g, int i, int j, int k) { return g[NV * id(i, j) + k]; }
+	inline double u(const std::vector<double>& g, int i, int j, int k) const { return g[NV * id(i, j) + k]; }
+};
+
+static void init_ctx(Ctx& X, const Cfg& C) {
+	X.C = C;
+	for (int k = 0; k < 2; k++) X.M.PH[k] = Phase{C.ph[k][0], C.ph[k][1], C.ph[k][2], C.ph[k][3], C.ph[k][4], C.ph[k][5]};
+	X.M.closure_path = C.closure_path; X.M.share_start = C.share_start; X.M.dot_order = C.dot_order; X.M.portable = C.portable_math;
+	X.nx = C.nx; X.ny = C.ny; X.NXT = C.nx + 2 * GH; X.NYT = C.ny + 2 * GH;
+	X.dx = (C.x1 - C.x0) / C.nx; X.dy = (C.y1 - C.y0) / C.ny; X.twoD = C.ny > 1;
+	const size_t nc = (size_t)X.NXT * X.NYT;
+	for (auto* g : {&X.U, &X.Un, &X.Qe, &X.Qs, &X.Ust, &X.wxL, &X.wxR, &X.wyL, &X.wyR, &X.H[0], &X.H[1], &X.H[2]}) g->assign(nc * NV, 0.0);
+	for (auto* a : {&X.phx, &X.phy, &X.a2s, &X.ubx, &X.uby, &X.ust, &X.vst, &X.ubsx, &X.kpx, &X.pvx, &X.ubsy, &X.kpy, &X.pvy, &X.Kw, &X.cz, &X.Pold}) a->assign(nc, 0.0);
+	X.wLx.assign(nc, 0.5); X.wLy.assign(nc, 0.5); X.Qph.assign(nc * 2, 0.0);
+	X.P.assign((size_t)C.nx * C.ny, 0.0); X.Pst = X.P;
+}
+
+// ------------------------------------------------------------------------------------------------ boundaries
+static void bc_apply(Ctx& X, std::vector<double>& g) {                                              // PHY:926-957 (TRANS/WALL/PERIODIC)
+	const int nx = X.nx, ny = X.ny, G = GH; const int* BCS = X.C.bc; const double* BCU = X.C.bcu;
+	auto ghost = [&](int i, int j, int si, int sj, int side) {
+		for (int k = 0; k < 8; k++) X.u(g, i, j, k) = X.u(g, si, sj, k);
+		if (BCS[side] != BC_WALL) return;
+		const int kn = side < 2 ? IMU : IMV; const double rho = X.u(g, i, j, IM1) + X.u(g, i, j, IM2);
+		if (BCU[side] == 0.0) { X.u(g, i, j, kn) = -X.u(g, i, j, kn); return; }
+		const double mn = 2.0 * rho * BCU[side] - X.u(g, si, sj, kn);
+		X.u(g, i, j, IEN) += 0.5 * (mn * mn - X.u(g, si, sj, kn) * X.u(g, si, sj, kn)) / rho; X.u(g, i, j, kn) = mn; };
+	for (int j = 0; j < ny + 2 * G; j++)
+		for (int q = 0; q < G; q++) {
+			const int iL = G - 1 - q, iR = nx + G + q;
+			const int sL = BCS[0] == BC_PERIODIC ? nx + G - 1 - q : (BCS[0] == BC_WALL ? G + q : G);
+			const int sR = BCS[1] == BC_PERIODIC ? G + q : (BCS[1] == BC_WALL ? nx + G - 1 - q : nx + G - 1);
+			ghost(iL, j, sL, j, 0); ghost(iR, j, sR, j, 1);
+		}
+	if (ny > 1 || BCS[2] == BC_PERIODIC) {
+		for (int i = 0; i < nx + 2 * G; i++)
+			for (int q = 0; q < G; q++) {
+				const int jL = G - 1 - q, jR = ny + G + q;
+				const int sL = BCS[2] == BC_PERIODIC ? ny + G - 1 - q : (BCS[2] == BC_WALL ? G + q : G);
+				const int sR = BCS[3] == BC_PERIODIC ? G + q : (BCS[3] == BC_WALL ? ny + G - 1 - q : ny + G - 1);
+				ghost(i, jL, i, sL, 2); ghost(i, jR, i, sR, 3);
+			}
+	} else {
+		for (int i = 0; i < nx + 2 * G; i++)
+			for (int q = 0; q < G; q++)
+				for (int k = 0; k < 8; k++) { X.u(g, i, G - 1 - q, k) = X.u(g, i, G, k); X.u(g, i, G + 1 + q, k) = X.u(g, i, G, k); }
+	}
+}
+static void bound_cond_array(Ctx& X, std::vector<double>& a) {                                      // PC:490-511 (signs +1)
+	const int G = GH, nx = X.nx, ny = X.ny; const int* BCS = X.C.bc;
+	auto src = [&](int side, int q, int n, int& sL) {
+		if (BCS[side] == BC_PERIODIC) sL = (side % 2 == 0) ? n + G - 1 - q : G + q;
+		else if (BCS[side] == BC_WALL) sL = (side % 2 == 0) ? G + q : n + G - 1 - q;
+		else sL = (side % 2 == 0) ? G : n + G - 1; };
+	auto A = [&](int i, int j) -> double& { return a[X.id(i, j)]; };
+	for (int j = 0; j < ny + 2 * G; j++) for (int q = 0; q < G; q++) { int sL, sR; src(0, q, nx, sL); src(1, q, nx, sR); A(G - 1 - q, j) = A(sL, j); A(nx + G + q, j) = A(sR, j); }
+	for (int i = 0; i < nx + 2 * G; i++) for (int q = 0; q < G; q++) {
+		int sL, sR;
+		if (ny == 1 && BCS[2] != BC_PERIODIC) { sL = G; sR = G; } else { src(2, q, ny, sL); src(3, q, ny, sR); }
+		A(i, G - 1 - q) = A(i, sL); A(i, ny + G + q) = A(i, sR);
+	}
+}
+
+// ------------------------------------------------------------------------------------------------ explicit stage
+static void tvd_reconstruction(Ctx& X, const std::vector<double>& Q) {                             // PC:349-446
+	const Model& M = X.M; const int NXT = X.NXT, NYT = X.NYT;
+	X.wxL = Q; X.wxR = Q; X.wyL = Q; X.wyR = Q;
+	std::fill(X.phx.begin(), X.phx.end(), 0.0); std::fill(X.phy.begin(), X.phy.end(), 0.0);
+	if (X.C.order < 2) return;
+	std::vector<double> vsc((size_t)NXT * NYT, 0.0), esc((size_t)NXT * NYT, 0.0);
+	#pragma omp parallel for schedule(static)
+	for (int c = 0; c < NXT * NYT; c++) {
+		const double* q = &Q[NV * (size_t)c]; Prim w;
+		const double rho = q[IM1] + q[IM2], mm = M.hypot_(q[IMU], q[IMV]);
+		vsc[c] = (rho > 0 ? mm / rho : 0.0) + (prim_of(M, q, w) ? w.cf : 0.0);
+		esc[c] = std::fabs(q[IE1]) + std::fabs(q[IE2]) + (rho > 0 ? 0.5 * mm * mm / rho : 0.0);
+	}
+	const int VOL[5] = {IA1, IM1, IM2, IE1, IE2};
+	const bool lim = X.C.limiter != 0;
+	for (int dir = 0; dir < (X.ny > 1 ? 2 : 1); dir++) {
+		std::vector<double>& wL = dir == 0 ? X.wxL : X.wyL; std::vector<double>& wR = dir == 0 ? X.wxR : X.wyR;
+		std::vector<double>& phis = dir == 0 ? X.phx : X.phy;
+		#pragma omp parallel for schedule(static)
+		for (int c = 0; c < NXT * NYT; c++) {
+			const int i = c % NXT, j = c / NXT;
+			int ia = i, ja = j, ib = i, jb = j;
+			if (dir == 0) { ia = i - 1; ib = i + 1; if (ia < 0 || ib >= NXT) continue; }
+			else { ja = j - 1; jb = j + 1; if (ja < 0 || jb >= NYT) continue; }
+			const size_t ca = X.id(ia, ja), cc = X.id(i, j), cb = X.id(ib, jb);
+			auto U = [&](size_t cell, int k) { return Q[NV * cell + k]; };
+			double phi = 1.0;
+			for (int q = 0; q < 5; q++) {
+				const int k = VOL[q];
+				const double xa = U(ca, k), xc = U(cc, k), xb = U(cb, k);
+				double sc;
+				if (k == IA1) sc = 3.0;
+				else if (k == IM1 || k == IM2) sc = std::fabs(xa) + std::fabs(xc) + std::fabs(xb);
+				else sc = esc[ca] + esc[cc] + esc[cb];
+				const double a = xc - xa, b = xb - xc, tiny = 1e-12 * sc + 1e-300;
+				if (!lim) break;
+				if (std::fabs(a) <= tiny && std::fabs(b) <= tiny) continue;
+				if (a * b <= 0) { phi = 0.0; break; }
+				const double cm = 0.5 * (a + b), m = std::fabs(a) < std::fabs(b) ? a : b;
+				phi = smin(phi, m / cm);
+			}
+			if (lim && phi > 0.0) {   // face admissibility cap (SG/NASG phases)
+				const size_t cs[3] = {ca, cc, cb};
+				for (int ph = 0; ph < 2; ph++) {
+					double L1[3], L2[3];
+					for (int n = 0; n < 3; n++) {
+						const double a = ph == 0 ? U(cs[n], IA1) : 1.0 - U(cs[n], IA1), m = U(cs[n], ph == 0 ? IM1 : IM2), e = U(cs[n], ph == 0 ? IE1 : IE2);
+						L1[n] = a - m * M.PH[ph].b; L2[n] = e - m * M.PH[ph].q - L1[n] * M.PH[ph].pi;
+					}
+					const double* Ls[2] = {L1, L2};
+					for (int l = 0; l < 2; l++) {
+						const double dL = std::fabs(Ls[l][2] - Ls[l][0]);
+						if (Ls[l][1] > 0.0 && dL > 0.0) phi = smin(phi, 2.0 * Ls[l][1] / dL);
+					}
+				}
+			}
+			for (int q = 0; q < 5; q++) {
+				const int k = VOL[q];
+				const double sig = phi * 0.5 * (U(cb, k) - U(ca, k));
+				wL[NV * cc + k] = U(cc, k) - 0.5 * sig;
+				wR[NV * cc + k] = U(cc, k) + 0.5 * sig;
+			}
+			const double ra = U(ca, IM1) + U(ca, IM2), rc = U(cc, IM1) + U(cc, IM2), rb = U(cb, IM1) + U(cb, IM2);
+			const double tiny = 1e-12 * (vsc[ca] + vsc[cc] + vsc[cb]) + 1e-300;
+			double vL[2], vR[2], sm = 1.0;
+			for (int d = 0; d < 2; d++) {
+				const int k = d == 0 ? IMU : IMV;
+				const double va = U(ca, k) / ra, vc = U(cc, k) / rc, vb = U(cb, k) / rb, a = vc - va, b = vb - vc;
+				double sl = 0.0;
+				if (!lim) sl = 0.5 * (a + b);
+				else if (!(std::fabs(a) <= tiny && std::fabs(b) <= tiny)) {
+					if (a * b > 0) { sl = std::fabs(a) < std::fabs(b) ? a : b; sm = smin(sm, sl / (0.5 * (a + b))); }
+					else sm = 0.0;
+				}
+				vL[d] = vc - 0.5 * sl; vR[d] = vc + 0.5 * sl;
+			}
+			double* L = &wL[NV * cc]; double* R = &wR[NV * cc];
+			const double rL = L[IM1] + L[IM2], rR = R[IM1] + R[IM2];
+			L[IMU] = rL * vL[0]; L[IMV] = rL * vL[1]; L[IEN] = L[IE1] + L[IE2] + 0.5 * rL * (vL[0] * vL[0] + vL[1] * vL[1]);
+			R[IMU] = rR * vR[0]; R[IMV] = rR * vR[1]; R[IEN] = R[IE1] + R[IE2] + 0.5 * rR * (vR[0] * vR[0] + vR[1] * vR[1]);
+			phis[cc] = smin(phi, sm);
+		}
+	}
+}
+static void rusanov(const double* QL, const double* QR, int dirk, double* flux) {                 // PC:197-209, PC:224-246 (local speed)
+	const double uL = QL[dirk] / (QL[IM1] + QL[IM2]), uR = QR[dirk] / (QR[IM1] + QR[IM2]);
+	double fL[NV], fR[NV];
+	for (int k = 0; k < NV; k++) { fL[k] = QL[k] * uL; fR[k] = QR[k] * uR; }
+	const double a = smax(std::fabs(QL[dirk] / (QL[IM1] + QL[IM2])), std::fabs(QR[dirk] / (QR[IM1] + QR[IM2])));
+	for (int k = 0; k < NV; k++) flux[k] = 0.5 * (fL[k] + fR[k]) - 0.5 * a * (QR[k] - QL[k]);
+}
+static bool explicit_update(Ctx& X, double adt) {                                                    // PC:251-326 (rus=1)
+	const int g = GH, nx = X.nx, ny = X.ny, NXT = X.NXT, NYT = X.NYT;
+	bool rok = true;
+	#pragma omp parallel for schedule(static) reduction(&&:rok)
+	for (int c = 0; c < NXT * NYT; c++) { const double rho = X.Qe[NV * (size_t)c + IM1] + X.Qe[NV * (size_t)c + IM2]; if (!(rho > 0)) rok = false; }
+	if (!rok) return false;
+	for (int j = g; j < ny + g; j++) for (int i = g - 1; i < nx + g; i++) {
+		const double uL = X.u(X.wxR, i, j, IMU) / (X.u(X.wxR, i, j, IM1) + X.u(X.wxR, i, j, IM2)), uR = X.u(X.wxL, i + 1, j, IMU) / (X.u(X.wxL, i + 1, j, IM1) + X.u(X.wxL, i + 1, j, IM2));
+		X.ubx[X.id(i, j)] = 0.5 * (uL + uR);
+	}
+	if (ny > 1) for (int j = g - 1; j < ny + g; j++) for (int i = g; i < nx + g; i++) {
+		const double vL = X.u(X.wyR, i, j, IMV) / (X.u(X.wyR, i, j, IM1) + X.u(X.wyR, i, j, IM2)), vR = X.u(X.wyL, i, j + 1, IMV) / (X.u(X.wyL, i, j + 1, IM1) + X.u(X.wyL, i, j + 1, IM2));
+		X.uby[X.id(i, j)] = 0.5 * (vL + vR);
+	}
+	const double rxd = adt / X.dx, ryd = adt / X.dy;
+	#pragma omp parallel for schedule(static)
+	for (int q = 0; q < nx * ny; q++) {
+		const int i = q % nx + g, j = q / nx + g;
+		double fp[NV], fm[NV];
+		rusanov(&X.wxR[NV * X.id(i, j)], &X.wxL[NV * X.id(i + 1, j)], IMU, fp);
+		rusanov(&X.wxR[NV * X.id(i - 1, j)], &X.wxL[NV * X.id(i, j)], IMU, fm);
+		double* o = &X.Ust[NV * X.id(i, j)];
+		for (int k = 0; k < NV; k++) o[k] = X.u(X.Qs, i, j, k) - rxd * (fp[k] - fm[k]);
+		o[IA1] += rxd * X.u(X.Qe, i, j, IA1) * (X.ubx[X.id(i, j)] - X.ubx[X.id(i - 1, j)]);
+		if (ny > 1) {
+			double gp[NV], gm[NV];
+			rusanov(&X.wyR[NV * X.id(i, j)], &X.wyL[NV * X.id(i, j + 1)], IMV, gp);
+			rusanov(&X.wyR[NV * X.id(i, j - 1)], &X.wyL[NV * X.id(i, j)], IMV, gm);
+			for (int k = 0; k < NV; k++) o[k] -= ryd * (gp[k] - gm[k]);
+			o[IA1] += ryd * X.u(X.Qe, i, j, IA1) * (X.uby[X.id(i, j)] - X.uby[X.id(i, j - 1)]);
+		}
+		X.a2s[X.id(i, j)] = 1.0 - o[IA1];
+	}
+	bc_apply(X, X.Ust);
+	bound_cond_array(X, X.a2s);
+	return true;
+}
+
+// ------------------------------------------------------------------------------------------------ implicit stage
+static int M_P(const Ctx& X, int i, int j) {                                                        // PC:688-697
+	int I, J; const bool xper = X.C.bc[0] == BC_PERIODIC, yper = X.C.bc[2] == BC_PERIODIC;
+	if (i < GH) I = xper ? X.nx + i : GH;
+	else if (i > X.nx + GH - 1) I = xper ? i - X.nx : X.nx + GH - 1;
+	else I = i;
+	if (j < GH) J = (yper && X.ny > 1) ? X.ny + j : GH;
+	else if (j > X.ny + GH - 1) J = (yper && X.ny > 1) ? j - X.ny : X.ny + GH - 1;
+	else J = j;
+	return (I - GH) + X.nx * (J - GH);
+}
+static bool stage_prep(Ctx& X) {                                                                    // PC:851-969
+	const Model& M = X.M; const int NXT = X.NXT, NYT = X.NYT, nx = X.nx, ny = X.ny;
+	const std::vector<double>& Xs = X.Ust;
+	auto XS = [&](int i, int j, int k) { return Xs[NV * X.id(i, j) + k]; };
+	bool sok = true;
+	#pragma omp parallel for schedule(static) reduction(&&:sok)
+	for (int c = 0; c < NXT * NYT; c++) {
+		const int i = c % NXT, j = c / NXT;
+		const double sa = XS(i, j, IA1) + X.a2s[c], rho = XS(i, j, IM1) + XS(i, j, IM2);
+		if (!(sa > 0 && rho > 0)) { sok = false; continue; }
+		X.ust[c] = XS(i, j, IMU) / rho; X.vst[c] = XS(i, j, IMV) / rho;
+	}
+	if (!sok) return false;
+	// frozen sound speed and residual scale K + P_E (PC:875-898)
+	#pragma omp parallel for schedule(static)
+	for (int c = 0; c < NXT * NYT; c++) {
+		const int i = c % NXT, j = c / NXT;
+		const double a1 = XS(i, j, IA1), a2 = X.a2s[c], m1 = XS(i, j, IM1), m2 = XS(i, j, IM2);
+		double cz = INF;
+		if (a1 > 0 && a2 > 0 && m1 > 0 && m2 > 0) {
+			const double p1 = p_ame(M, 0, a1, m1, XS(i, j, IE1)), p2 = p_ame(M, 1, a2, m2, XS(i, j, IE2));
+			if (p_admissible(M, 0, p1) && p_admissible(M, 1, p2)) {
+				const double c21 = c2_rp(M, 0, m1 / a1, p1), c22 = c2_rp(M, 1, m2 / a2, p2);
+				const double c2 = (m1 * c21 + m2 * c22) / (m1 + m2);
+				cz = c2 > 0 ? std::sqrt(c2) : INF;
+			}
+		}
+		X.cz[c] = cz;
+		const double sa = a1 + a2;
+		double K = smax(M.PH[0].pi, M.PH[1].pi), PE = 0.0;
+		if (a1 > 0 && a2 > 0 && m1 > 0 && m2 > 0) {
+			const double p1 = p_ame(M, 0, a1, m1, XS(i, j, IE1)), p2 = p_ame(M, 1, a2, m2, XS(i, j, IE2));
+			if (p_admissible(M, 0, p1) && p_admissible(M, 1, p2)) K = 1.0 / ((a1 / sa) / (m1 / a1 * c2_rp(M, 0, m1 / a1, p1)) + (a2 / sa) / (m2 / a2 * c2_rp(M, 1, m2 / a2, p2)));
+			const double dd = dedp_am(M, 0, a1, m1) + dedp_am(M, 1, a2, m2);
+			if (dd > 0) PE = (std::fabs(XS(i, j, IE1)) + eps_round(M, 0, m1) + std::fabs(XS(i, j, IE2)) + eps_round(M, 1, m2)) / dd;
+		}
+		X.Kw[c] = K + PE;
+		X.Pold[c] = X.Pst[M_P(X, i, j)];   // PC:899-900: the stage pressure (Newton guess)
+	}
+	const double tau = X.tau, dx = X.dx, dy = X.dy;
+	auto rhoS = [&](int i, int j) { return XS(i, j, IM1) + XS(i, j, IM2); };
+	auto minmod = [](double a, double b) { return a * b <= 0 ? 0.0 : (std::fabs(a) < std::fabs(b) ? a : b); };
+	const bool lim = X.C.limiter != 0; const int order = X.C.order;
+	auto psi_face = [&](double xm, double xL, double xR, double xp) {                               // PC:901-905
+		if (order < 2) return 1.0;
+		const double D = xR - xL; if (D == 0.0) return 1.0;
+		if (!lim) return smin(1.0, smax(0.0, 1.0 - 0.25 * ((xR - xm) + (xp - xL)) / D));
+		return 1.0 - 0.5 * (minmod(xL - xm, D) + minmod(D, xp - xR)) / D; };
+	auto UST = [&](int i, int j) { return X.ust[X.id(i, j)]; };
+	auto VST = [&](int i, int j) { return X.vst[X.id(i, j)]; };
+	auto dil_jump = [&](int i, int j, bool ydir) -> double {                                         // PC:912-925
+		if (!ydir) {
+			const double d = UST(i + 1, j) - UST(i, j);
+			if (ny == 1 || j < 1 || j + 1 >= NYT) return d;
+			return 0.25 * (2.0 * d + ((UST(i + 1, j + 1) - UST(i, j + 1)) + (UST(i + 1, j - 1) - UST(i, j - 1))))
+				+ 0.25 * (dx / dy) * ((VST(i, j + 1) - VST(i, j - 1)) + (VST(i + 1, j + 1) - VST(i + 1, j - 1)));
+		}
+		const double d = VST(i, j + 1) - VST(i, j);
+		if (i < 1 || i + 1 >= NXT) return d;
+		return 0.25 * (2.0 * d + ((VST(i + 1, j + 1) - VST(i + 1, j)) + (VST(i - 1, j + 1) - VST(i - 1, j))))
+			+ 0.25 * (dy / dx) * ((UST(i + 1, j) - UST(i - 1, j)) + (UST(i + 1, j + 1) - UST(i - 1, j + 1)));
+	};
+	auto set_face = [&](int iL, int jL, int iR, int jR, double hh, bool ydir, double& ubs, double& kp, double& wL, double& pv) {   // PC:926-948
+		const double rL = rhoS(iL, jL), rR = rhoS(iR, jR);
+		const double uL = ydir ? VST(iL, jL) : UST(iL, jL), uR = ydir ? VST(iR, jR) : UST(iR, jR);
+		ubs = 0.5 * (uL + uR); kp = tau / hh / (0.5 * (rL + rR)); wL = rR / (rL + rR); pv = 0.0;
+		const double cL = X.cz[X.id(iL, jL)], cR = X.cz[X.id(iR, jR)];
+		if (!(std::isfinite(cL) && std::isfinite(cR))) return;
+		const double phi = smax(0.0, 1.0 - tau * smin(cL, cR) / hh);
+		if (phi == 0.0) return;
+		const int di = iR - iL, dj = jR - jL, im = iL - di, jm = jL - dj, ip = iR + di, jp = jR + dj;
+		double psiP = 1.0, psiU = 1.0;
+		if (im >= 0 && jm >= 0 && ip < NXT && jp < NYT) {
+			const std::vector<double>& ph = ydir ? X.phy : X.phx;
+			const double phis = smin(ph[X.id(iL, jL)], ph[X.id(iR, jR)]);
+			auto PO = [&](int i, int j) { return X.Pold[X.id(i, j)]; };
+			psiP = 1.0 - (1.0 - psi_face(PO(im, jm), PO(iL, jL), PO(iR, jR), PO(ip, jp))) * phis;
+			const double vm = ydir ? VST(im, jm) : UST(im, jm), vp = ydir ? VST(ip, jp) : UST(ip, jp);
+			psiU = psi_face(vm, uL, uR, vp);
+		}
+		const double ZL = rL * cL, ZR = rR * cR;
+		kp += phi * psiP / (ZL + ZR); pv = phi * psiU * ZL * ZR / (ZL + ZR) * dil_jump(iL, jL, ydir);
+	};
+	#pragma omp parallel for schedule(static)
+	for (int c = 0; c < NXT * NYT; c++) {
+		const int i = c % NXT, j = c / NXT;
+		if (i < NXT - 1) set_face(i, j, i + 1, j, dx, false, X.ubsx[c], X.kpx[c], X.wLx[c], X.pvx[c]);
+		if (ny > 1 && j < NYT - 1) set_face(i, j, i, j + 1, dy, true, X.ubsy[c], X.kpy[c], X.wLy[c], X.pvy[c]);
+	}
+	// shock budget on the +-1 cross stencil of X* (PC:961-969)
+	#pragma omp parallel for schedule(dynamic, 8)
+	for (int q = 0; q < nx * ny; q++) {
+		const int i = q % nx + GH, j = q / nx + GH;
+		auto cell = [&](int di, int dj) { return &Xs[NV * X.id(i + di, j + dj)]; };
+		shock_budget(M, cell, X.twoD, &X.Qph[X.id(i, j) * 2]);
+	}
+	return true;
+}
+
+struct FaceVals { double uf, du, pf, wL, wR, utL, utR; const double* cL; const double* cR; };     // PC:590
+struct FaceFlux { double M, Mt, E; };
+static inline void face_weights(FaceVals& f, double ube) { f.wL = smax(f.uf, 0.0) - smax(ube, 0.0); f.wR = smin(f.uf, 0.0) - smin(ube, 0.0); }   // PC:592
+static inline int wall_x(const Ctx& X, int i) { if (i == GH - 1 && X.C.bc[0] == BC_WALL) return 0; if (i == X.nx + GH - 1 && X.C.bc[1] == BC_WALL) return 1; return -1; }   // PC:594
+static inline int wall_y(const Ctx& X, int j) { if (j == GH - 1 && X.C.bc[2] == BC_WALL) return 2; if (j == X.ny + GH - 1 && X.C.bc[3] == BC_WALL) return 3; return -1; }   // PC:595
+// donor face states: x face (i,j) = i+1/2 takes its left state from cell i's right face value (wxR) and its right
+// state from cell i+1's left face value (wxL) (PC:866-873); the donors are read in place from the face grids
+static inline void face_x(const Ctx& X, int i, int j, double PL, double PR, FaceVals& f) {           // PC:597-606
+	const size_t c = X.id(i, j);
+	f.pf = X.wLx[c] * PL + (1.0 - X.wLx[c]) * PR - X.pvx[c];
+	f.cL = &X.wxR[NV * c]; f.cR = &X.wxL[NV * X.id(i + 1, j)]; f.utL = X.vst[c]; f.utR = X.vst[X.id(i + 1, j)];
+	if (const int w = wall_x(X, i); w >= 0) {
+		if (X.C.bcu[w] == 0.0) { f.uf = 0.0; f.du = 0.0; f.wL = f.wR = 0.0; return; }
+		f.uf = X.C.bcu[w]; f.du = f.uf - X.ubx[c]; face_weights(f, X.ubx[c]); return; }
+	f.uf = X.ubsx[c] - X.kpx[c] * (PR - PL);
+	f.du = f.uf - X.ubx[c];
+	face_weights(f, X.ubx[c]);
+}
+static inline void face_y(const Ctx& X, int i, int j, double PL, double PR, FaceVals& f) {           // PC:607-616
+	const size_t c = X.id(i, j);
+	f.pf = X.wLy[c] * PL + (1.0 - X.wLy[c]) * PR - X.pvy[c];
+	f.cL = &X.wyR[NV * c]; f.cR = &X.wyL[NV * X.id(i, j + 1)]; f.utL = X.ust[c]; f.utR = X.ust[X.id(i, j + 1)];
+	if (const int w = wall_y(X, j); w >= 0) {
+		if (X.C.bcu[w] == 0.0) { f.uf = 0.0; f.du = 0.0; f.wL = f.wR = 0.0; return; }
+		f.uf = X.C.bcu[w]; f.du = f.uf - X.uby[c]; face_weights(f, X.uby[c]); return; }
+	f.uf = X.ubsy[c] - X.kpy[c] * (PR - PL);
+	f.du = f.uf - X.uby[c];
+	face_weights(f, X.uby[c]);
+}
+// donor component k of the reference's 6-vector (alpha1, 1 - alpha1, m1, m2, e1, e2) read from an 8-variable face state
+static inline double dk(const double* w, int k) { return k == 2 ? w[IM1] : k == 3 ? w[IM2] : k == 4 ? w[IE1] : w[IE2]; }
+static inline double inc(const FaceVals& f, int k) { return dk(f.cL, k) * f.wL + dk(f.cR, k) * f.wR; }   // PC:617
+static inline FaceFlux face_flux(const Ctx& X, const FaceVals& f) {                                 // PC:620-625
+	const double ML = (f.cL[IM1] + f.cL[IM2]) * f.wL, MR = (f.cR[IM1] + f.cR[IM2]) * f.wR, Mm = ML + MR;
+	FaceFlux F; F.M = Mm; F.Mt = f.utL * ML + f.utR * MR;
+	F.E = inc(f, 4) + inc(f, 5) + 0.5 * f.uf * f.uf * Mm + 0.5 * (f.utL * f.utL * ML + f.utR * f.utR * MR) + (X.P_REF + f.pf) * f.uf;
+	return F;
+}
+struct CellOut { ClOut cl; double mx, my, E, s, sig; };
+static bool cell_residual(const Ctx& X, int i, int j, double Pc, double PW, double PE, double PS, double PN, double& G, CellOut* out) {   // PC:640-679
+	const Model& M = X.M;
+	FaceVals fe, fw, fn, fs;
+	face_x(X, i, j, Pc, PE, fe); face_x(X, i - 1, j, PW, Pc, fw);
+	const bool twoD = X.twoD;
+	if (twoD) { face_y(X, i, j, Pc, PN, fn); face_y(X, i, j - 1, PS, Pc, fs); }
+	const size_t c = X.id(i, j); const double* xs = &X.Ust[NV * c];
+	double Xv[6] = {xs[IA1], X.a2s[c], xs[IM1], xs[IM2], xs[IE1], xs[IE2]};
+	for (int k = 2; k < 6; k++) {
+		Xv[k] -= X.rx * (inc(fe, k) - inc(fw, k));
+		if (twoD) Xv[k] -= X.ry * (inc(fn, k) - inc(fs, k));
+	}
+	Xv[1] = 1.0 - Xv[0];
+	const double s = X.rx * (fe.uf - fw.uf) + (twoD ? X.ry * (fn.uf - fs.uf) : 0.0);
+	Xv[0] *= 1.0 - s; Xv[1] *= 1.0 - s;
+	const FaceFlux FEe = face_flux(X, fe), FEw = face_flux(X, fw);
+	double mx = xs[IMU] - X.rx * (fe.pf - fw.pf) - X.rx * (fe.uf * FEe.M - fw.uf * FEw.M);
+	double my = xs[IMV] - X.rx * (FEe.Mt - FEw.Mt);
+	double E = xs[IEN] - X.rx * (FEe.E - FEw.E);
+	if (twoD) {
+		const FaceFlux FEn = face_flux(X, fn), FEs = face_flux(X, fs);
+		my -= X.ry * (fn.pf - fs.pf) + X.ry * (fn.uf * FEn.M - fs.uf * FEs.M);
+		mx -= X.ry * (FEn.Mt - FEs.Mt);
+		E -= X.ry * (FEn.E - FEs.E);
+	}
+	const double rho = Xv[2] + Xv[3];
+	if (!(rho > 0)) return false;
+	ClIn in; in.atr[0] = Xv[0]; in.atr[1] = Xv[1]; in.mtr[0] = Xv[2]; in.mtr[1] = Xv[3]; in.etr[0] = Xv[4]; in.etr[1] = Xv[5];
+	in.s = s; in.rhoeC = E - 0.5 * (mx * mx + my * my) / rho; in.pguess = X.P_REF + Pc; in.Qph[0] = X.Qph[c * 2]; in.Qph[1] = X.Qph[c * 2 + 1];
+	ClOut o; if (!closure(M, in, o)) return false;
+	G = (o.pbar - X.P_REF) - Pc;
+	if (out) { out->cl = o; out->mx = mx; out->my = my; out->E = E; out->s = s; out->sig = closure_roundoff(M, in, o.pbar); }
+	return std::isfinite(G);
+}
+static inline double Pnb(const Ctx& X, const std::vector<double>& P, int i, int j, int di, int dj) { return P[M_P(X, i + GH + di, j + GH + dj)]; }   // PC:698
+static bool residual_at(const Ctx& X, const std::vector<double>& P, int i, int j, double dc, int which, double dn, double& G, CellOut* out = nullptr) {   // PC:699-705
+	const double Pc = P[(size_t)i + (size_t)j * X.nx] + dc;
+	double PW = Pnb(X, P, i, j, -1, 0), PE = Pnb(X, P, i, j, 1, 0), PS = Pnb(X, P, i, j, 0, -1), PN = Pnb(X, P, i, j, 0, 1);
+	if (which == 0) PW += dn;
+	if (which == 1) PE += dn;
+	if (which == 2) PS += dn;
+	if (which == 3) PN += dn;
+	return cell_residual(X, i + GH, j + GH, Pc, PW, PE, PS, PN, G, out);
+}
+
+// ------------------------------------------------------------------------------------------------ linear solvers
+static void spmv(const std::vector<SpRow>& A, const std::vector<double>& x, std::vector<double>& y) {   // PC:711-715
+	const int n = (int)A.size();
+	#pragma omp parallel for schedule(static)
+	for (int r = 0; r < n; r++) { double s = 0; for (int q = 0; q < A[r].n; q++) s += A[r].val[q] * x[A[r].col[q]]; y[r] = s; }
+}
+static bool bicgstab(Ctx& X, std::vector<double>& x, const std::vector<double>& b, const std::vector<SpRow>& A, int& iters) {   // PC:725-758
+	const Model& M = X.M; const int n = (int)b.size(); const double tol = X.C.krylov_tol; const int maxit = X.C.krylov_iter;
+	std::vector<double> dinv(n, 1.0);
+	for (int r = 0; r < n; r++) if (A[r].n > 0 && A[r].val[0] != 0) dinv[r] = 1.0 / A[r].val[0];
+	auto dot = [&](const std::vector<double>& a, const std::vector<double>& c) { return osum(M, n, [&](int r) { return a[r] * c[r]; }); };
+	const double E = osum(M, n, [&](int r) { double e = 0; for (int q = 0; q < A[r].n; q++) e += A[r].val[q]; return e; });
+	const double Einv = (E != 0.0 && std::isfinite(E)) ? 1.0 / E : 0.0;
+	auto prec = [&](const std::vector<double>& in, std::vector<double>& out) {
+		const double cc = osum(M, n, [&](int r) { return in[r]; }) * Einv;
+		#pragma omp parallel for schedule(static)
+		for (int r = 0; r < n; r++) out[r] = dinv[r] * in[r] + cc; };
+	x.assign(n, 0.0); prec(b, x);
+	std::vector<double> r(n), r0(n), p(n, 0.0), v(n, 0.0), s(n), t(n), ph(n), sh(n), Ax(n);
+	spmv(A, x, Ax); for (int q = 0; q < n; q++) r[q] = b[q] - Ax[q]; r0 = r;
+	const double bn = std::sqrt(dot(b, b)) + 1e-300; double rho = 1, alpha = 1, omega = 1;
+	iters = 0;
+	for (int it = 0; it < maxit; it++) {
+		iters = it;
+		if (std::sqrt(dot(r, r)) <= tol * bn) return true;
+		const double rho1 = dot(r0, r); if (rho1 == 0) return false;
+		const double beta = (rho1 / rho) * (alpha / omega); rho = rho1;
+		#pragma omp parallel for schedule(static)
+		for (int q = 0; q < n; q++) p[q] = r[q] + beta * (p[q] - omega * v[q]);
+		prec(p, ph);
+		spmv(A, ph, v); alpha = rho / dot(r0, v);
+		#pragma omp parallel for schedule(static)
+		for (int q = 0; q < n; q++) s[q] = r[q] - alpha * v[q];
+		if (std::sqrt(dot(s, s)) <= tol * bn) { for (int q = 0; q < n; q++) x[q] += alpha * ph[q]; iters = it + 1; return true; }
+		prec(s, sh);
+		spmv(A, sh, t); omega = dot(t, s) / dot(t, t);
+		#pragma omp parallel for schedule(static)
+		for (int q = 0; q < n; q++) { x[q] += alpha * ph[q] + omega * sh[q]; r[q] = s[q] - omega * t[q]; }
+	}
+	iters = maxit;
+	return std::sqrt(dot(r, r)) <= 1e3 * tol * bn;
+}
+static void thomas(const std::vector<SpRow>& A, const std::vector<double>& rhs, std::vector<double>& dP) {   // PC:807-812
+	const int n = (int)A.size();
+	std::vector<double> lo(n), di(n), up(n), cp(n), dp(n);
+	for (int r = 0; r < n; r++) { lo[r] = up[r] = di[r] = 0; for (int q = 0; q < A[r].n; q++) { const int c = A[r].col[q]; if (c == r) di[r] += A[r].val[q]; else if (c == r - 1) lo[r] = A[r].val[q]; else if (c == r + 1) up[r] = A[r].val[q]; } }
+	cp[0] = up[0] / di[0]; dp[0] = rhs[0] / di[0];
+	for (int r = 1; r < n; r++) { const double m = di[r] - lo[r] * cp[r - 1]; cp[r] = up[r] / m; dp[r] = (rhs[r] - lo[r] * dp[r - 1]) / m; }
+	dP[n - 1] = dp[n - 1]; for (int r = n - 2; r >= 0; r--) dP[r] = dp[r] - cp[r] * dP[r + 1];
+}
+
+// ------------------------------------------------------------------------------------------------ Newton (PC:761-847, PCO:799-904 with jreuse=0)
+struct NewtonTrace { std::vector<double>* J5; };   // optional dump of the first Jacobian (tests)
+static bool newton(Ctx& X, std::vector<double>& P, int& iters, double& resfinal) {
+	const int nx = X.nx, ny = X.ny, n = nx * ny; const double tol = X.C.newton_tol;
+	std::vector<double> G(n), Gn(n), dP(n), rhs(n), sc(n), Pn = P;
+	std::vector<SpRow> A(n);
+	auto KW = [&](int q) { return X.Kw[X.id(q % nx + GH, q / nx + GH)]; };
+	auto evalAll = [&](const std::vector<double>& Q, std::vector<double>& R) -> bool {
+		bool ok = true;
+		#pragma omp parallel for schedule(static) reduction(&&:ok)
+		for (int q = 0; q < n; q++) { double g; if (!residual_at(X, Q, q % nx, q / nx, 0, -1, 0, g)) ok = false; else R[q] = g; }
+		return ok; };
+	auto norm = [&](const std::vector<double>& R, const std::vector<double>& Q) {
+		double m = 0;
+		#pragma omp parallel for schedule(static) reduction(max:m)
+		for (int q = 0; q < n; q++) m = smax(m, std::fabs(R[q]) / (std::fabs(X.P_REF + Q[q]) + KW(q) + 1.0));
+		return m; };
+	if (!evalAll(P, G)) { iters = 0; resfinal = INF; return false; }
+	double res = norm(G, P);
+	const bool tri = (ny == 1 && X.C.bc[0] != BC_PERIODIC);
+	for (int it = 0; it < X.C.newton_iter; it++) {
+		iters = it; resfinal = res;
+		if (res < tol) return true;
+		#pragma omp parallel for schedule(static)
+		for (int q = 0; q < n; q++) {
+			const int I = q % nx + GH, J = q / nx + GH; const double K = X.Kw[X.id(I, J)];
+			double D = X.rx * (X.kpx[X.id(I, J)] + X.kpx[X.id(I - 1, J)]);
+			if (ny > 1) D += X.ry * (X.kpy[X.id(I, J)] + X.kpy[X.id(I, J - 1)]);
+			sc[q] = (std::fabs(X.P_REF + P[q]) + K + 1.0) / (1.0 + K * D);
+		}
+		bool jok = true;
+		#pragma omp parallel for schedule(static) reduction(&&:jok)
+		for (int r = 0; r < n; r++) {
+			const int i = r % nx, j = r / nx; const double h = 1e-7 * sc[r];
+			double g1; if (!residual_at(X, P, i, j, h, -1, 0, g1)) { jok = false; continue; }
+			SpRow& row = A[r]; row.n = 0; row.col[row.n] = r; row.val[row.n++] = (g1 - G[r]) / h;
+			const int dd[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+			for (int w = 0; w < 4; w++) {
+				if (ny == 1 && w >= 2) break;
+				const int c = M_P(X, i + GH + dd[w][0], j + GH + dd[w][1]);
+				const double hn = 1e-7 * sc[c];
+				double g2; if (!residual_at(X, P, i, j, 0, w, hn, g2)) { jok = false; break; }
+				row.col[row.n] = c; row.val[row.n++] = (g2 - G[r]) / hn;
+			}
+		}
+		if (!jok) return false;
+		for (int q = 0; q < n; q++) rhs[q] = -G[q];
+		X.st.linsolves++;
+		if (tri) thomas(A, rhs, dP);
+		else { int kit; const bool ok = bicgstab(X, dP, rhs, A, kit); X.st.krylov_it += kit; X.last_krylov = kit; if (!ok) return false; }
+		double sn = 0.0;
+		for (int q = 0; q < n; q++) sn = smax(sn, std::fabs(dP[q]) / (std::fabs(P[q]) + sc[q]));
+		if (sn < tol) {
+			for (int q = 0; q < n; q++) P[q] += dP[q];
+			X.st.newton_it++; iters = it + 1;
+			if (!evalAll(P, G)) return false;
+			resfinal = norm(G, P); return true;
+		}
+		double lam = 1.0; bool acc = false;
+		for (int ls = 0; ls < 40; ls++) {
+			for (int q = 0; q < n; q++) Pn[q] = P[q] + lam * dP[q];
+			if (evalAll(Pn, Gn)) { const double rn = norm(Gn, Pn); if (rn < res * (1.0 - 1e-4 * lam) || rn < tol) { acc = true; res = rn; break; } }
+			lam *= 0.5;
+		}
+		if (!acc) {   // round-off floor acceptance (PC:834-842)
+			resfinal = res; bool floor_ok = true;
+			#pragma omp parallel for schedule(static) reduction(&&:floor_ok)
+			for (int q = 0; q < n; q++) {
+				double g; CellOut o;
+				if (!residual_at(X, P, q % nx, q / nx, 0, -1, 0, g, &o)) { floor_ok = false; continue; }
+				if (std::fabs(g) > tol * (std::fabs(X.P_REF + P[q]) + KW(q) + 1.0) + o.sig) floor_ok = false;
+			}
+			if (floor_ok) X.st.floor_acc++;
+			return floor_ok;
+		}
+		std::swap(P, Pn); std::swap(G, Gn);
+		X.st.newton_it++;
+	}
+	iters = X.C.newton_iter; resfinal = res; return res < tol;
+}
+
+static bool pressure_subsystem(Ctx& X, double h) {                                                  // PC:851-993
+	X.tau = h; X.rx = h / X.dx; X.ry = h / X.dy;
+	if (!stage_prep(X)) return false;
+	int iters; double res;
+	const bool ok = newton(X, X.Pst, iters, res);
+	X.last_newton_iters = iters;
+	if (ok) X.st.maxres = smax(X.st.maxres, res);
+	if (!ok) return false;
+	bool fok = true; const int nx = X.nx;
+	std::vector<double> out((size_t)nx * X.ny * NV);
+	#pragma omp parallel for schedule(static) reduction(&&:fok)
+	for (int q = 0; q < nx * X.ny; q++) {
+		double Gv; CellOut o; if (!residual_at(X, X.Pst, q % nx, q / nx, 0, -1, 0, Gv, &o)) { fok = false; continue; }
+		double* w = &out[(size_t)q * NV];
+		w[IA1] = o.cl.a[0]; w[IM1] = o.cl.m[0]; w[IM2] = o.cl.m[1]; w[IMU] = o.mx; w[IMV] = o.my; w[IE1] = o.cl.e[0]; w[IE2] = o.cl.e[1]; w[IEN] = o.E;
+	}
+	if (!fok) return false;
+	for (int q = 0; q < nx * X.ny; q++) for (int k = 0; k < NV; k++) X.u(X.Ust, q % nx + GH, q / nx + GH, k) = out[(size_t)q * NV + k];
+	bc_apply(X, X.Ust);
+	X.st.stages++;
+	return true;
+}
+
+// ------------------------------------------------------------------------------------------------ driver
+static double caldt(const Ctx& X, bool acoustic) {                                                  // PC:477-488
+	double ux = 0, uy = 0;
+	for (int j = GH; j < X.ny + GH; j++) for (int i = GH; i < X.nx + GH; i++) {
+		Prim w; if (!prim_of(X.M, &X.U[NV * X.id(i, j)], w)) continue;
+		const double c = acoustic ? w.cf : 0.0;
+		ux = smax(ux, std::fabs(w.u) + c); uy = smax(uy, std::fabs(w.v) + c);
+	}
+	const double rate = ux / X.dx + (X.ny > 1 ? uy / X.dy : 0.0);
+	return rate > 0 ? 1.0 / rate : INF;
+}
+static void set_tab(Ctx& X, int rk) {                                                               // PC:1195-1218
+	for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) X.Ae[a][b] = X.Ai[a][b] = 0;
+	if (rk == 1) X.Ai[0][0] = 1.0;
+	else if (rk == 2) { const double bb = 1.0 - 1.0 / std::sqrt(2.0), cc = 1.0 / (2.0 * bb); X.Ae[1][0] = cc; X.Ai[0][0] = bb; X.Ai[1][0] = 1 - bb; X.Ai[1][1] = bb; }
+	else if (rk == 3) { const double r12 = 0.5, r13 = 1.0 / 3.0, r14 = 1.0 / 4.0; X.Ae[1][0] = r12; X.Ae[2][0] = r12; X.Ae[2][1] = r12; X.Ai[0][0] = r14; X.Ai[1][1] = r14; X.Ai[2][0] = r13; X.Ai[2][1] = r13; X.Ai[2][2] = r13; }
+}
+static void init_pressure(Ctx& X) {                                                                 // PC:1222-1227
+	X.P_REF = INF;
+	for (int j = 0; j < X.ny; j++) for (int i = 0; i < X.nx; i++) X.P_REF = smin(X.P_REF, pbar_of(X.M, &X.U[NV * X.id(i + GH, j + GH)]));
+	if (!std::isfinite(X.P_REF)) X.P_REF = 0.0;
+	for (int j = 0; j < X.ny; j++) for (int i = 0; i < X.nx; i++) X.P[(size_t)i + (size_t)j * X.nx] = pbar_of(X.M, &X.U[NV * X.id(i + GH, j + GH)]) - X.P_REF;
+}
+// one attempt of a step of size dt with an rk-stage table; false = inadmissible (PC:1243-1285)
+static bool attempt(Ctx& X, double dt, int rk) {
+	set_tab(X, rk);
+	X.Un = X.U; X.Pst = X.P;
+	const int nx = X.nx, ny = X.ny;
+	bool ok = true;
+	for (int rks = 0; rks < rk && ok; rks++) {
+		X.Qe = X.Un; X.Qs = X.Un;
+		for (int r = 0; r < rks; r++) {
+			const double ae = X.Ae[rks][r], ai = X.Ai[rks][r];
+			#pragma omp parallel for schedule(static)
+			for (int q = 0; q < nx * ny; q++) {
+				const size_t c = X.id(q % nx + GH, q / nx + GH);
+				for (int k = 0; k < NV; k++) {
+					X.Qe[NV * c + k] = X.Qe[NV * c + k] + ae * dt * X.H[r][NV * c + k];
+					X.Qs[NV * c + k] = X.Qs[NV * c + k] + ai * dt * X.H[r][NV * c + k];
+				}
+			}
+		}
+		bc_apply(X, X.Qe); bc_apply(X, X.Qs);
+		const double adt = X.Ai[rks][rks] * dt;
+		tvd_reconstruction(X, X.Qe);
+		ok = explicit_update(X, adt);
+		ok = ok && pressure_subsystem(X, adt);
+		if (ok) {
+			#pragma omp parallel for schedule(static)
+			for (int q = 0; q < nx * ny; q++) {
+				const size_t c = X.id(q % nx + GH, q / nx + GH);
+				for (int k = 0; k < NV; k++) X.H[rks][NV * c + k] = (X.Ust[NV * c + k] - X.Qs[NV * c + k]) / adt;
+			}
+		}
+	}
+	return ok;
+}
+// time loop (PC:1230-1296): runs until t_end or max_steps accepted steps; returns 0 ok, 2 failure
+static int run(Ctx& X, double t_end, long max_steps) {
+	const double ramp_factor = 1.2;
+	long done_steps = 0;
+	while (X.t < t_end * (1 - 1e-12) && (max_steps < 0 || done_steps < max_steps)) {
+		bool ramping = false; double dt;
+		if (X.C.dtfix > 0) dt = X.C.dtfix;
+		else {
+			dt = X.C.cfl * caldt(X, X.C.acdt != 0);
+			if (X.C.ramp) { if (X.dtprev < 0) X.dtprev = X.C.cfl_ac * caldt(X, true) / ramp_factor; if (dt > ramp_factor * X.dtprev) { dt = ramp_factor * X.dtprev; ramping = true; } }
+			if (!std::isfinite(dt)) dt = X.C.cfl_ac * caldt(X, true);
+		}
+		dt = smin(dt, t_end - X.t);
+		if (!(dt > 1e-12 * t_end)) { X.st.status = 3; return 2; }
+		bool ok = false;
+		for (int att = 0; att < 30 && !ok; att++) {
+			const bool first_order = X.C.ramp && ramping && X.startup;
+			const int rk = first_order ? 1 : X.C.rk; if (rk == 1) X.st.rk1++;
+			ok = attempt(X, dt, rk);
+			if (ok) { X.U = X.Ust; X.P = X.Pst; }
+			else { X.st.rejects++; dt *= 0.5; }
+		}
+		if (!ok) { X.st.status = 2; return 2; }
+		X.t = X.t + dt; X.dtprev = dt; X.st.steps++; X.st.dt_last = dt; done_steps++; if (!ramping) X.startup = false;
+	}
+	X.st.t = X.t;
+	return 0;
+}
+
+// ------------------------------------------------------------------------------------------------ reference cases (PHY:961-1100 subset)
+static void set_SG(Cfg& C, int k, double g, double pi) { C.ph[k][0] = g; C.ph[k][1] = pi; C.ph[k][2] = 0; C.ph[k][3] = 0; C.ph[k][4] = 1000.0; C.ph[k][5] = 0; }   // PHY:958
+static bool case_setup(int test, Cfg& C, double& tend) {                                           // PHY:961-1022 (SG cases)
+	C.x0 = 0; C.x1 = 1; C.y0 = 0; C.y1 = 1; for (int q = 0; q < 4; q++) { C.bc[q] = BC_TRANS; C.bcu[q] = 0; }
+	switch (test) {
+	case 0: set_SG(C, 0, 4.4, 6e8); set_SG(C, 1, 1.4, 0); tend = 1.85e-3; break;
+	case 4: set_SG(C, 0, 4.4, 6e8); set_SG(C, 1, 1.4, 0); tend = 2.79e-3; break;
+	case 5: set_SG(C, 0, 4.4, 6e8); set_SG(C, 1, 1.4, 0); tend = 240e-6; break;
+	case 80: case 82: case 83: case 84: set_SG(C, 0, 1.4, 0); set_SG(C, 1, 1.4, 0); tend = test == 80 ? 0.5 : test == 82 ? 0.14 : test == 83 ? 2.5e-6 : 0.15; break;
+	case 81: case 85: set_SG(C, 0, 4.4, 6e8); set_SG(C, 1, 4.4, 6e8); tend = test == 81 ? 1e-4 : 0.15e-3; break;
+	case 86: case 87: set_SG(C, 0, 4.4, 6e8); set_SG(C, 1, 1.4, 0); tend = test == 86 ? 0.25 : 241e-6; break;
+	case 88: set_SG(C, 0, 2.43, 5.3e9); set_SG(C, 1, 1.62, 141e9); tend = 30e-6; break;
+	default: return false;
+	}
+	return true;
+}
+static void case_state(const Model& M, int test, double x, double* Q) {                           // PHY:1028-1100 (SG 1D cases)
+	auto setcell = [&](double a1, double r1, double r2, double u, double v, double p1, double p2) {
+		Prim w; w.a1 = a1; w.r1 = r1; w.r2 = r2; w.u = u; w.v = v; w.p1 = p1; w.p2 = p2; prim2cell(M, w, Q); };
+	switch (test) {
+	case 0: { const double u = x < 0.5 ? -100.0 : 100.0; setcell(1 - 1e-2, 1000.0, 1.0, u, 0, 1e5, 1e5); break; }
+	case 4: { const bool L = x < 0.5; setcell(L ? 1 - 1e-8 : 1e-8, 1000.0, 10.0, 100.0, 0, 1e5, 1e5); break; }
+	case 5: { const bool L = x < 0.75; setcell(L ? 1 - 1e-6 : 1e-6, 1000.0, 1.0, 0, 0, L ? 1e9 : 1e5, L ? 1e9 : 1e5); break; }
+	case 80: case 81: case 82: case 83: case 84: case 85: {
+		double xi = 0.5, rL = 0, uL = 0, pL = 0, rR = 0, uR = 0, pR = 0;
+		switch (test) {
+		case 80: xi = 0.25; rL = 1000; uL = 1; pL = 1e5; rR = 0.01; uR = 1; pR = 1e5; break;
+		case 81: rL = 1000; uL = 0; pL = 1e8; rR = 1000; uR = 15; pR = 0.98e8; break;
+		case 82: rL = 0.445; uL = 0.698; pL = 3.528; rR = 0.5; uR = 0; pR = 0.571; break;
+		case 83: rL = 1; uL = 0; pL = 1e10; rR = 0.125; uR = 0; pR = 0.1; break;
+		case 84: rL = 1; uL = -1; pL = 0.4; rR = 1; uR = 1; pR = 0.4; break;
+		default: rL = 1000; uL = 0; pL = 1e6; rR = 1000; uR = 0; pR = 1e5; break; }
+		const bool L = x < xi; const double r = L ? rL : rR, u = L ? uL : uR, p = L ? pL : pR;
+		setcell(1 - 1e-8, r, r, u, 0, p, p); break; }
+	case 86: case 87: {
+		const bool adv = test == 86, L = x < (adv ? 0.25 : 0.7);
+		setcell(L ? 1 - 1e-7 : 1e-7, 1000.0, adv ? 0.01 : 50.0, adv ? 1.0 : 0.0, 0, adv ? 1e5 : (L ? 1e9 : 1e5), adv ? 1e5 : (L ? 1e9 : 1e5)); break; }
+	case 88: { const double p = x < 0.6 ? 1e10 : 1e5; setcell(0.5954, 1185.0, 3622.0, 0, 0, p, p); break; }
+	}
+}
+static void cell_diag(const Model& M, const double* Q, double* o) {                                 // PHY:1105-1116 (columns p u v rho alpha2 Y2 rho1 rho2 cf cw T1 T2 p1 p2)
+	Prim w; const bool ok = prim_of(M, Q, w);
+	const double a1 = Q[IA1], m1 = Q[IM1], m2 = Q[IM2], rho = m1 + m2;
+	double cw = NAN, T1 = NAN, T2 = NAN;
+	if (ok) {
+		cw = std::sqrt(1.0 / (rho * (a1 / (w.r1 * c2_rp(M, 0, w.r1, w.p1)) + (1 - a1) / (w.r2 * c2_rp(M, 1, w.r2, w.p2)))));
+		T1 = T_amp(M, 0, a1, m1, w.p1); T2 = T_amp(M, 1, 1 - a1, m2, w.p2);
+	}
+	o[0] = ok ? a1 * w.p1 + (1 - a1) * w.p2 : NAN; o[1] = ok ? w.u : NAN; o[2] = ok ? w.v : NAN; o[3] = rho; o[4] = 1 - a1; o[5] = m2 / rho;
+	o[6] = ok ? w.r1 : NAN; o[7] = ok ? w.r2 : NAN; o[8] = ok ? w.cf : NAN; o[9] = cw; o[10] = T1; o[11] = T2; o[12] = ok ? w.p1 : NAN; o[13] = ok ? w.p2 : NAN;
+}
+
+}   // namespace ora
+
+// ================================================================================================ C ABI (tests only)
+using namespace ora;
+extern "C" {
+
+typedef struct { double ph[2][6]; int closure_path, share_start, dot_order, portable_math; } OraModel;
+static Model mk_model(const OraModel* m) {
+	Model M; for (int k = 0; k < 2; k++) M.PH[k] = Phase{m->ph[k][0], m->ph[k][1], m->ph[k][2], m->ph[k][3], m->ph[k][4], m->ph[k][5]};
+	M.closure_path = m->closure_path; M.share_start = m->share_start; M.dot_order = m->dot_order; M.portable = m->portable_math; return M;
+}
+
+int ora_version(void) { return 1; }
+int ora_set_threads(int n) { if (n > 0) omp_set_num_threads(n); return omp_get_max_threads(); }
+
+void* ora_create(const Cfg* C) { Ctx* X = new Ctx(); init_ctx(*X, *C); return X; }
+void ora_destroy(void* h) { delete (Ctx*)h; }
+size_t ora_cfg_size(void) { return sizeof(Cfg); }
+size_t ora_stats_size(void) { return sizeof(Stats); }
+
+// interior state in AoS order U[(i + j nx) * 8 + k]; applies the boundary fill and the initial pressure (PC:1157-1227)
+void ora_set_state(void* h, const double* Uin) {
+	Ctx& X = *(Ctx*)h;
+	for (int j = 0; j < X.ny; j++) for (int i = 0; i < X.nx; i++) for (int k = 0; k < NV; k++) X.u(X.U, i + GH, j + GH, k) = Uin[((size_t)i + (size_t)j * X.nx) * NV + k];
+	bc_apply(X, X.U); init_pressure(X);
+	X.t = 0.0; X.dtprev = -1.0; X.startup = true; X.st = Stats{};
+}
+void ora_get_state(void* h, double* Uout) {
+	Ctx& X = *(Ctx*)h;
+	for (int j = 0; j < X.ny; j++) for (int i = 0; i < X.nx; i++) for (int k = 0; k < NV; k++) Uout[((size_t)i + (size_t)j * X.nx) * NV + k] = X.u(X.U, i + GH, j + GH, k);
+}
+void ora_get_pressure(void* h, double* P, double* pref) { Ctx& X = *(Ctx*)h; std::memcpy(P, X.P.data(), X.P.size() * sizeof(double)); *pref = X.P_REF; }
+int ora_run(void* h, double t_end, long max_steps) { return run(*(Ctx*)h, t_end, max_steps); }
+void ora_get_stats(void* h, Stats* s) { *s = ((Ctx*)h)->st; }
+double ora_time(void* h) { return ((Ctx*)h)->t; }
+
+// one stage of explicit + implicit work on the current state with step adt (rk=1 stage): exposes the intermediate fields
+int ora_stage_explicit(void* h, double adt) {
+	Ctx& X = *(Ctx*)h; X.Un = X.U; X.Qe = X.U; X.Qs = X.U; X.Pst = X.P;
+	tvd_reconstruction(X, X.Qe);
+	return explicit_update(X, adt) ? 0 : 1;
+}
+int ora_stage_prep(void* h, double adt) { Ctx& X = *(Ctx*)h; X.tau = adt; X.rx = adt / X.dx; X.ry = adt / X.dy; return stage_prep(X) ? 0 : 1; }
+int ora_stage_newton(void* h, int* iters, double* res) { Ctx& X = *(Ctx*)h; int it; double r; const bool ok = newton(X, X.Pst, it, r); *iters = it; *res = r; return ok ? 0 : 1; }
+// residual of every cell at pressure P (interior, nx*ny); returns the number of failed cells
+int ora_residual_all(void* h, const double* P, double* G) {
+	Ctx& X = *(Ctx*)h; const int n = X.nx * X.ny; std::vector<double> Pv(P, P + n); int bad = 0;
+	for (int q = 0; q < n; q++) { double g; if (!residual_at(X, Pv, q % X.nx, q / X.nx, 0, -1, 0, g)) { G[q] = NAN; bad++; } else G[q] = g; }
+	return bad;
+}
+// finite-difference Jacobian rows at P with residual G: val[5*n] (self, W, E, S, N), sc[n]
+int ora_jacobian(void* h, const double* P, const double* G, double* val, double* scout) {
+	Ctx& X = *(Ctx*)h; const int nx = X.nx, ny = X.ny, n = nx * ny; std::vector<double> Pv(P, P + n); int bad = 0;
+	for (int q = 0; q < n; q++) {
+		const int I = q % nx + GH, J = q / nx + GH; const double K = X.Kw[X.id(I, J)];
+		double D = X.rx * (X.kpx[X.id(I, J)] + X.kpx[X.id(I - 1, J)]);
+		if (ny > 1) D += X.ry * (X.kpy[X.id(I, J)] + X.kpy[X.id(I, J - 1)]);
+		scout[q] = (std::fabs(X.P_REF + Pv[q]) + K + 1.0) / (1.0 + K * D);
+	}
+	for (int r = 0; r < n; r++) {
+		const int i = r % nx, j = r / nx; const double hh = 1e-7 * scout[r];
+		for (int w = 0; w < 5; w++) val[5 * r + w] = 0.0;
+		double g1; if (!residual_at(X, Pv, i, j, hh, -1, 0, g1)) { bad++; continue; }
+		val[5 * r] = (g1 - G[r]) / hh;
+		const int dd[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+		for (int w = 0; w < 4; w++) {
+			if (ny == 1 && w >= 2) break;
+			const int c = M_P(X, i + GH + dd[w][0], j + GH + dd[w][1]);
+			const double hn = 1e-7 * scout[c];
+			double g2; if (!residual_at(X, Pv, i, j, 0, w, hn, g2)) { bad++; break; }
+			val[5 * r + 1 + w] = (g2 - G[r]) / hn;
+		}
+	}
+	return bad;
+}
+// BiCGStab / Thomas on the 5-point (2D) or 3-point (1D) pattern of the current grid: val[5n] as from ora_jacobian
+int ora_linsolve(void* h, const double* val, const double* b, double* x, int use_thomas, int* iters) {
+	Ctx& X = *(Ctx*)h; const int nx = X.nx, ny = X.ny, n = nx * ny; std::vector<SpRow> A(n);
+	const int dd[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+	for (int r = 0; r < n; r++) {
+		const int i = r % nx, j = r / nx; SpRow& row = A[r]; row.n = 0; row.col[row.n] = r; row.val[row.n++] = val[5 * r];
+		for (int w = 0; w < 4; w++) { if (ny == 1 && w >= 2) break; row.col[row.n] = M_P(X, i + GH + dd[w][0], j + GH + dd[w][1]); row.val[row.n++] = val[5 * r + 1 + w]; }
+	}
+	std::vector<double> bv(b, b + n), xv(n);
+	int it = 0; bool ok = true;
+	if (use_thomas) thomas(A, bv, xv); else ok = bicgstab(X, xv, bv, A, it);
+	std::memcpy(x, xv.data(), n * sizeof(double)); *iters = it; return ok ? 0 : 1;
+}
+// named intermediate fields (with ghosts, NXT*NYT, or NXT*NYT*8 for grids)
+const double* ora_field(void* h, const char* name) {
+	Ctx& X = *(Ctx*)h; const std::string s(name);
+	if (s == "U") return X.U.data(); if (s == "Ust") return X.Ust.data(); if (s == "Qe") return X.Qe.data(); if (s == "Qs") return X.Qs.data();
+	if (s == "wxL") return X.wxL.data(); if (s == "wxR") return X.wxR.data(); if (s == "wyL") return X.wyL.data(); if (s == "wyR") return X.wyR.data();
+	if (s == "phx") return X.phx.data(); if (s == "phy") return X.phy.data(); if (s == "a2s") return X.a2s.data(); if (s == "ubx") return X.ubx.data(); if (s == "uby") return X.uby.data();
+	if (s == "ust") return X.ust.data(); if (s == "vst") return X.vst.data(); if (s == "cz") return X.cz.data(); if (s == "Kw") return X.Kw.data();
+	if (s == "ubsx") return X.ubsx.data(); if (s == "kpx") return X.kpx.data(); if (s == "wLx") return X.wLx.data(); if (s == "pvx") return X.pvx.data();
+	if (s == "ubsy") return X.ubsy.data(); if (s == "kpy") return X.kpy.data(); if (s == "wLy") return X.wLy.data(); if (s == "pvy") return X.pvy.data();
+	if (s == "Qph") return X.Qph.data(); if (s == "Pst") return X.Pst.data(); if (s == "P") return X.P.data();
+	return nullptr;
+}
+double ora_pref(void* h) { return ((Ctx*)h)->P_REF; }
+
+// reference test cases (1D SG): fills cfg geometry/EOS and the interior state U[nx*8]; returns t_end (<0: unknown case)
+double ora_case_init(int test, Cfg* C, double* U) {
+	double tend; if (!case_setup(test, *C, tend)) return -1.0;
+	Model M; for (int k = 0; k < 2; k++) M.PH[k] = Phase{C->ph[k][0], C->ph[k][1], C->ph[k][2], C->ph[k][3], C->ph[k][4], C->ph[k][5]};
+	const double dx = (C->x1 - C->x0) / C->nx;
+	for (int i = 0; i < C->nx; i++) case_state(M, test, C->x0 + (i + 0.5) * dx, &U[(size_t)i * NV]);
+	return tend;
+}
+// profile columns p u v rho alpha2 Y2 rho1 rho2 cf cw T1 T2 p1 p2 (14 per cell)
+void ora_cell_diag(const OraModel* m, const double* U, int ncell, double* out) { const Model M = mk_model(m); for (int c = 0; c < ncell; c++) cell_diag(M, &U[(size_t)c * NV], &out[(size_t)c * 14]); }
+
+// ---- per-function entry points (unit tests)
+double ora_p_mix(const OraModel* m, double a1, double m1, double m2, double rhoe) { return p_mix(mk_model(m), a1, m1, m2, rhoe); }
+// in: atr0 atr1 mtr0 mtr1 etr0 etr1 s rhoeC pguess Qph0 Qph1; out: a0 a1 m0 m1 e0 e1 pbar sig
+int ora_closure(const OraModel* m, const double* in, double* out) {
+	const Model M = mk_model(m); ClIn c; c.atr[0] = in[0]; c.atr[1] = in[1]; c.mtr[0] = in[2]; c.mtr[1] = in[3]; c.etr[0] = in[4]; c.etr[1] = in[5];
+	c.s = in[6]; c.rhoeC = in[7]; c.pguess = in[8]; c.Qph[0] = in[9]; c.Qph[1] = in[10];
+	ClOut o; if (!closure(M, c, o)) return 0;
+	out[0] = o.a[0]; out[1] = o.a[1]; out[2] = o.m[0]; out[3] = o.m[1]; out[4] = o.e[0]; out[5] = o.e[1]; out[6] = o.pbar; out[7] = closure_roundoff(M, c, o.pbar);
+	return 1;
+}
+// Phi(p) = sum alpha_k(p) - 1 of the closure input (for brute-force root checks)
+double ora_closure_phi(const OraModel* m, const double* in, double p) {
+	const Model M = mk_model(m); ClIn c; c.atr[0] = in[0]; c.atr[1] = in[1]; c.mtr[0] = in[2]; c.mtr[1] = in[3]; c.etr[0] = in[4]; c.etr[1] = in[5];
+	c.s = in[6]; c.rhoeC = in[7]; c.pguess = in[8]; c.Qph[0] = in[9]; c.Qph[1] = in[10];
+	PCoef q; p_coef(M, c, q, p); return alpha_p(M, q, c, 0, p) + alpha_p(M, q, c, 1, p) - 1.0;
+}
+int ora_shock_share(const OraModel* m, const double* Y, const double* vpre, double p0, double p1, double* lam, double* q) { return shock_share(mk_model(m), Y, vpre, p0, p1, lam, *q) ? 1 : 0; }
+double ora_hug_v(const OraModel* m, int k, double p0, double p1, double v0) { return hug_v(mk_model(m), k, p0, p1, v0); }
+int ora_prof_integrate(const OraModel* m, const double* Y, const double* vpre, double p0, double p1, double c, double* v1end, double* qend) {
+	const Model M = mk_model(m); ShockProf S; S.p0 = p0; double vH[2];
+	for (int k = 0; k < 2; k++) { S.Y[k] = Y[k]; S.vk0[k] = vpre[k]; vH[k] = hug_v(M, k, p0, p1, vpre[k]); }
+	S.v0 = Y[0] * vpre[0] + Y[1] * vpre[1]; S.vend = Y[0] * vH[0] + Y[1] * vH[1]; S.j2 = (p1 - p0) / (S.v0 - S.vend);
+	return prof_integrate(M, S, c, *v1end, *qend) ? 1 : 0;
+}
+// shock budget of one cell from its cross stencil: cells[5*8] = centre, W, E, S, N (S/N ignored when twoD = 0)
+void ora_shock_budget(const OraModel* m, const double* cells, int twoD, double* Q) {
+	const Model M = mk_model(m);
+	auto cell = [&](int di, int dj) { const int k = (di == 0 && dj == 0) ? 0 : di == -1 ? 1 : di == 1 ? 2 : dj == -1 ? 3 : 4; return &cells[k * NV]; };
+	shock_budget(M, cell, twoD != 0, Q);
+}
+double ora_pexp(double x) { return pexp(x); }
+double ora_plog(double x) { return plog(x); }
+double ora_ppow(double x, double y) { return ppow(x, y); }
+// fixed-order sums (dot_order 0/1/2) of a vector
+double ora_sum(int dot_order, const double* a, int n) { Model M; M.dot_order = dot_order; return osum(M, n, [&](int r) { return a[r]; }); }
+
+}   // extern "C"
