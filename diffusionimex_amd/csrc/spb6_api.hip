@@ -1,0 +1,593 @@
+// spb6_api.hip -- host driver and C ABI of the gfx950 SPB 6+1 IMEX step (include/spb6_imex.h).
+//
+// The driver follows the reference's control flow exactly (time loop PC:1230-1296, attempt loop PC:1243-1289,
+// PressureSubsystem PC:851-993, Newton_Solver PC:761-847 with PCO:799-904 at jreuse=0, GMRES_Solver = BiCGStab PC:725-758,
+// Thomas PC:807-812); every data-parallel loop runs as a HIP kernel on the context's stream, and the host only takes
+// the scalar decisions the reference takes (norms, step criterion, line-search acceptance, reject-and-halve).
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "spb6_kernels.h"
+#include "../../include/spb6_imex.h"
+
+using namespace spb;
+
+static thread_local std::string g_err;
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { g_err = std::string(#x) + ": " + hipGetErrorString(e_); return SPB6_HIP_ERROR; } } while (0)
+
+namespace {
+
+struct HostScratch { unsigned long long nrm[8]; int flags[NFLAGS]; KS ks; };
+
+enum TimerCat { T_EXPLICIT = 0, T_PREP, T_SHOCK, T_RESIDUAL, T_JACOBIAN, T_KRYLOV, T_FINAL, T_OTHER, T_NCAT };
+
+struct Timer {
+	bool on = false;
+	std::vector<hipEvent_t> pool;
+	struct Pend { int cat; hipEvent_t a, b; };
+	std::vector<Pend> pend;
+	double ms[T_NCAT] = {0}; long cnt[T_NCAT] = {0};
+	size_t next = 0;
+	hipEvent_t get() {
+		if (next >= pool.size()) { hipEvent_t e; hipEventCreate(&e); pool.push_back(e); }
+		return pool[next++];
+	}
+	void resolve() {
+		if (pend.empty()) return;
+		hipEventSynchronize(pend.back().b);
+		for (auto& p : pend) { float t = 0; hipEventElapsedTime(&t, p.a, p.b); ms[p.cat] += t; cnt[p.cat]++; }
+		pend.clear(); next = 0;
+	}
+	~Timer() { for (auto e : pool) hipEventDestroy(e); }
+};
+
+}   // namespace
+
+struct spb6_ctx {
+	spb6_config_t cfg;
+	int device = 0;
+	hipStream_t st = nullptr;
+	Geo g; Phys ph;
+	// ghosted grids (8 fields each) and arrays
+	double *U = nullptr, *Ust = nullptr, *Qe = nullptr, *Qs = nullptr, *H[2] = {nullptr, nullptr};
+	double *wxL, *wxR, *wyL, *wyR, *phx, *phy, *a2s, *ubx, *uby, *vsc, *esc;
+	double *ust, *vst, *cz, *Kw, *Pold, *ubsx, *kpx, *wLx, *pvx, *ubsy, *kpy, *wLy, *pvy, *Qph;
+	// pressure system (interior)
+	double *P, *Pst, *Pn, *G, *Gn, *dP, *rhs, *sc, *J, *cpw, *dpw;
+	double *kr, *kr0, *kp, *kv, *ks_, *kt;
+	double* part = nullptr; int part_stride = 0; unsigned* counter = nullptr; KS* ksd = nullptr;
+	unsigned long long* nrm = nullptr; int* flags = nullptr;
+	HostScratch* hs = nullptr;
+	std::vector<void*> allocs;
+	// driver state
+	double P_REF = 0.0, t = 0.0, dtprev = -1.0; bool startup = true;
+	double Ae[3][3], Ai[3][3];
+	spb6_stats_t stats;
+	int last_krylov = 0;
+	double tau_last = 0.0;
+	Timer timer;
+};
+
+namespace {
+
+int dalloc(spb6_ctx* c, double** p, size_t count) {
+	HIPCHK(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(double)));
+	c->allocs.push_back(*p);
+	HIPCHK(hipMemset(*p, 0, std::max<size_t>(count, 1) * sizeof(double)));
+	return SPB6_OK;
+}
+
+Dev make_dev(const spb6_ctx* c) {
+	Dev D;
+	D.g = c->g; D.ph = c->ph; D.P_REF = c->P_REF; D.rx = 0; D.ry = 0; D.tau = 0;
+	D.Ust_c = c->Ust; D.Ust = c->Ust; D.a2s = c->a2s; D.ubx = c->ubx; D.uby = c->uby; D.phx = c->phx; D.phy = c->phy;
+	D.vsc = c->vsc; D.esc = c->esc; D.wxL = c->wxL; D.wxR = c->wxR; D.wyL = c->wyL; D.wyR = c->wyR;
+	D.ust = c->ust; D.vst = c->vst; D.cz = c->cz; D.Kw = c->Kw; D.Pold = c->Pold;
+	D.ubsx = c->ubsx; D.kpx = c->kpx; D.wLx = c->wLx; D.pvx = c->pvx; D.ubsy = c->ubsy; D.kpy = c->kpy; D.wLy = c->wLy; D.pvy = c->pvy;
+	D.Qph = c->Qph; D.flags = c->flags;
+	return D;
+}
+
+// timed launch region
+struct TRegion {
+	spb6_ctx* c; int cat; hipEvent_t a = nullptr;
+	TRegion(spb6_ctx* c_, int cat_) : c(c_), cat(cat_) { if (c->timer.on) { a = c->timer.get(); hipEventRecord(a, c->st); } }
+	~TRegion() { if (c->timer.on) { hipEvent_t b = c->timer.get(); hipEventRecord(b, c->st); c->timer.pend.push_back({cat, a, b}); if (c->timer.pend.size() > 512) c->timer.resolve(); } }
+};
+
+int reset_scalars(spb6_ctx* c) {
+	HIPCHK(hipMemsetAsync(c->nrm, 0, 8 * sizeof(unsigned long long), c->st));
+	HIPCHK(hipMemsetAsync(c->flags, 0, NFLAGS * sizeof(int), c->st));
+	return SPB6_OK;
+}
+// copy norms, flags and Krylov scalars to pinned host memory and wait
+int fetch(spb6_ctx* c, bool with_ks = false) {
+	HIPCHK(hipMemcpyAsync(c->hs->nrm, c->nrm, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipMemcpyAsync(c->hs->flags, c->flags, NFLAGS * sizeof(int), hipMemcpyDeviceToHost, c->st));
+	if (with_ks) HIPCHK(hipMemcpyAsync(&c->hs->ks, c->ksd, sizeof(KS), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	return SPB6_OK;
+}
+inline double nrm_val(const spb6_ctx* c, int k) { double v; std::memcpy(&v, &c->hs->nrm[k], sizeof v); return v; }
+
+// BiCGStab (PC:725-758) on the Jacobian c->J with right-hand side c->rhs; solution in c->dP
+int krylov(spb6_ctx* c, int& iters, bool& ok) {
+	const Geo& g = c->g;
+	KrylovBufs B{c->dP, c->kr, c->kr0, c->kp, c->kv, c->ks_, c->kt, c->J, c->part, c->part_stride, c->counter, c->ksd};
+	const double tol = c->cfg.krylov_tol; const int maxit = c->cfg.krylov_iter;
+	{
+		TRegion tr(c, T_KRYLOV);
+		launch_krylov_setup(g, B, c->rhs, c->st, tol, maxit);
+	}
+	int chunk = std::max(4, c->last_krylov), launched = 0;
+	for (;;) {
+		{
+			TRegion tr(c, T_KRYLOV);
+			for (int k = 0; k < chunk && launched < maxit; k++, launched++) launch_krylov_iter(g, B, c->st, tol, maxit);
+		}
+		HIPCHK(hipMemcpyAsync(&c->hs->ks, c->ksd, sizeof(KS), hipMemcpyDeviceToHost, c->st));
+		HIPCHK(hipStreamSynchronize(c->st));
+		if (c->hs->ks.done || launched >= maxit) break;
+		chunk = 4;
+	}
+	iters = c->hs->ks.iters; ok = c->hs->ks.done && c->hs->ks.status == 0;
+	c->last_krylov = iters;
+	return SPB6_OK;
+}
+
+// evalAll + norm (PC:764-770) at P + lam dP (dP may be null); trial pressure to Pout, residual to G
+int eval_all(spb6_ctx* c, const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, bool& ok, double& res) {
+	int rc = reset_scalars(c); if (rc) return rc;
+	{
+		TRegion tr(c, T_RESIDUAL);
+		launch_residual(D, P, dP, lam, Pout, G, &c->nrm[0], FLAG_RES, c->st);
+	}
+	rc = fetch(c); if (rc) return rc;
+	ok = c->hs->flags[FLAG_RES] == 0; res = nrm_val(c, 0);
+	return SPB6_OK;
+}
+
+// Newton_Solver (PC:761-847): unknown c->Pst; status SPB6_OK or SPB6_NEWTON_FAIL / SPB6_KRYLOV_FAIL
+int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal) {
+	const int n = c->g.n; const double tol = c->cfg.newton_tol;
+	const bool tri = (c->g.ny == 1 && c->cfg.bc[0] != SPB6_BC_PERIODIC);
+	bool ok; double res;
+	int rc = eval_all(c, D, c->Pst, nullptr, 0.0, nullptr, c->G, ok, res); if (rc) return rc;
+	if (!ok) { iters = 0; resfinal = INFINITY; return SPB6_NEWTON_FAIL; }
+	for (int it = 0; it < c->cfg.newton_iter; it++) {
+		iters = it; resfinal = res;
+		if (res < tol) return SPB6_OK;
+		rc = reset_scalars(c); if (rc) return rc;
+		{
+			TRegion tr(c, T_JACOBIAN);
+			launch_jacobian(D, c->Pst, c->G, c->sc, c->J, c->rhs, c->st);
+		}
+		c->stats.linsolves++;
+		bool kok = true; int kit = 0;
+		if (tri) {
+			TRegion tr(c, T_OTHER);
+			launch_thomas(n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
+		} else {
+			rc = krylov(c, kit, kok); if (rc) return rc;
+			c->stats.krylov_it += kit;
+		}
+		launch_stepcrit(n, c->dP, c->Pst, c->sc, &c->nrm[1], c->st);
+		rc = fetch(c); if (rc) return rc;
+		if (c->hs->flags[FLAG_JAC]) return SPB6_NEWTON_FAIL;
+		if (!kok) return SPB6_KRYLOV_FAIL;
+		const double sn = nrm_val(c, 1);
+		if (sn < tol) {   // step criterion (PC:819-826): accept P + dP
+			rc = eval_all(c, D, c->Pst, c->dP, 1.0, c->Pn, c->Gn, ok, res); if (rc) return rc;
+			std::swap(c->Pst, c->Pn); std::swap(c->G, c->Gn);
+			c->stats.newton_it++; iters = it + 1;
+			if (!ok) return SPB6_NEWTON_FAIL;
+			resfinal = res; return SPB6_OK;
+		}
+		double lam = 1.0; bool acc = false;
+		for (int ls = 0; ls < 40; ls++) {   // Armijo line search (PC:827-832)
+			double rn;
+			rc = eval_all(c, D, c->Pst, c->dP, lam, c->Pn, c->Gn, ok, rn); if (rc) return rc;
+			if (ok && (rn < res * (1.0 - 1e-4 * lam) || rn < tol)) { acc = true; res = rn; break; }
+			lam *= 0.5;
+		}
+		if (!acc) {   // round-off floor acceptance (PC:834-842)
+			resfinal = res;
+			rc = reset_scalars(c); if (rc) return rc;
+			launch_floor(D, c->Pst, tol, c->st);
+			rc = fetch(c); if (rc) return rc;
+			if (c->hs->flags[FLAG_FLOOR]) return SPB6_NEWTON_FAIL;
+			c->stats.floor_acc++;
+			return SPB6_OK;
+		}
+		std::swap(c->Pst, c->Pn); std::swap(c->G, c->Gn);
+		c->stats.newton_it++;
+	}
+	iters = c->cfg.newton_iter; resfinal = res;
+	return res < tol ? SPB6_OK : SPB6_NEWTON_FAIL;
+}
+
+Dev stage_dev(spb6_ctx* c, double adt) {
+	Dev D = make_dev(c); D.tau = adt; D.rx = adt / c->g.dx; D.ry = adt / c->g.dy; return D;
+}
+
+// explicit part of a stage from (Qe, Qs) into Ust / a2s / face data (TVDReconstruction + Update)
+int explicit_part(spb6_ctx* c, const double* Qe, const double* Qs, double adt) {
+	Dev D = stage_dev(c, adt);
+	TRegion tr(c, T_EXPLICIT);
+	launch_explicit(D, c->cfg.limiter, c->cfg.order, Qe, Qs, adt, c->st);
+	return SPB6_OK;
+}
+int prep_part(spb6_ctx* c, double adt) {
+	c->tau_last = adt;
+	Dev D = stage_dev(c, adt);
+	{ TRegion tr(c, T_PREP); launch_prep(D, c->cfg.limiter, c->cfg.order, adt, c->Pst, c->st); }
+	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->st); }
+	return SPB6_OK;
+}
+// PressureSubsystem (PC:851-993) after the explicit part; writes the stage state into Ust and H_rks
+int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& iters, double& res) {
+	int rc = prep_part(c, adt); if (rc) return rc;
+	rc = fetch(c); if (rc) return rc;
+	if (c->hs->flags[FLAG_STAGE]) return SPB6_STAGE_INADMISSIBLE;
+	Dev D = stage_dev(c, adt);
+	rc = newton(c, D, iters, res);
+	if (rc != SPB6_OK) return rc;
+	c->stats.maxres = std::max(c->stats.maxres, res);
+	rc = reset_scalars(c); if (rc) return rc;
+	{ TRegion tr(c, T_FINAL); launch_final(D, c->Pst, Qs, adt, Hk, c->st); }
+	launch_bc_grid(c->g, c->Ust, c->st);
+	rc = fetch(c); if (rc) return rc;
+	if (c->hs->flags[FLAG_FINAL]) return SPB6_CLOSURE_FAIL;
+	c->stats.stages++;
+	return SPB6_OK;
+}
+
+void set_tab(spb6_ctx* c, int rk) {   // PC:1195-1218
+	for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) c->Ae[a][b] = c->Ai[a][b] = 0;
+	if (rk == 1) c->Ai[0][0] = 1.0;
+	else if (rk == 2) { const double bb = 1.0 - 1.0 / std::sqrt(2.0), cc = 1.0 / (2.0 * bb); c->Ae[1][0] = cc; c->Ai[0][0] = bb; c->Ai[1][0] = 1 - bb; c->Ai[1][1] = bb; }
+}
+
+// one attempt (PC:1243-1285): on success Ust / Pst hold the new state and pressure
+int attempt(spb6_ctx* c, double dt, int rk) {
+	set_tab(c, rk);
+	const size_t n = c->g.n;
+	HIPCHK(hipMemcpyAsync(c->Pst, c->P, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+	for (int rks = 0; rks < rk; rks++) {
+		const double *Qe = c->U, *Qs = c->U;
+		if (rks > 0) {
+			TRegion tr(c, T_OTHER);
+			launch_combine(c->g, c->U, c->H[0], c->H[1], rks, c->Ae[rks][0] * dt, rks > 1 ? c->Ae[rks][1] * dt : 0.0,
+			               c->Ai[rks][0] * dt, rks > 1 ? c->Ai[rks][1] * dt : 0.0, c->Qe, c->Qs, c->st);
+			launch_bc_grid(c->g, c->Qe, c->st); launch_bc_grid(c->g, c->Qs, c->st);
+			Qe = c->Qe; Qs = c->Qs;
+		}
+		const double adt = c->Ai[rks][rks] * dt;
+		int rc = reset_scalars(c); if (rc) return rc;
+		rc = explicit_part(c, Qe, Qs, adt); if (rc) return rc;
+		rc = fetch(c); if (rc) return rc;
+		if (c->hs->flags[FLAG_RHO]) return SPB6_RHO_NONPOSITIVE;
+		int iters; double res;
+		rc = implicit_part(c, adt, Qs, c->H[rks], iters, res);
+		if (rc != SPB6_OK) return rc;
+	}
+	return SPB6_OK;
+}
+
+int caldt(spb6_ctx* c, bool acoustic, double& inv_rate) {   // PC:477-488
+	HIPCHK(hipMemsetAsync(c->nrm, 0, 2 * sizeof(unsigned long long), c->st));
+	launch_caldt(c->g, c->ph, c->U, acoustic ? 1 : 0, c->nrm, c->st);
+	int rc = fetch(c); if (rc) return rc;
+	const double ux = nrm_val(c, 0), uy = nrm_val(c, 1);
+	const double rate = ux / c->g.dx + (c->g.ny > 1 ? uy / c->g.dy : 0.0);
+	inv_rate = rate > 0 ? 1.0 / rate : INFINITY;
+	return SPB6_OK;
+}
+
+// pbar_of (PHY:855-857) on the host
+double pbar_host(const Phys& P, const double* Q) {
+	const double a1 = Q[IA1];
+	return a1 * p_ame(P, 0, a1, Q[IM1], Q[IE1]) + (1 - a1) * p_ame(P, 1, 1 - a1, Q[IM2], Q[IE2]);
+}
+
+}   // namespace
+
+// ================================================================================================ C ABI
+extern "C" {
+
+const char* spb6_last_error(void) { return g_err.c_str(); }
+
+spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
+	if (!cfg || cfg->nx < 2 || cfg->ny < 1) { g_err = "bad config"; return nullptr; }
+	if (cfg->rus_local != 1) { g_err = "only the local Rusanov speed (rus_local = 1) is supported"; return nullptr; }
+	if (cfg->rk < 1 || cfg->rk > 2) { g_err = "rk must be 1 or 2"; return nullptr; }
+	spb6_ctx* c = new spb6_ctx();
+	c->cfg = *cfg; c->device = device;
+	std::memset(&c->stats, 0, sizeof c->stats);
+	auto fail = [&](const char* what) -> spb6_ctx* { if (g_err.empty()) g_err = what; spb6_destroy(c); return nullptr; };
+	if (hipSetDevice(device) != hipSuccess) return fail("hipSetDevice failed");
+	if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return fail("stream");
+	Geo& g = c->g;
+	g.nx = cfg->nx; g.ny = cfg->ny; g.NXT = cfg->nx + 2 * GH; g.NYT = cfg->ny + 2 * GH; g.NC = g.NXT * g.NYT; g.n = cfg->nx * cfg->ny;
+	g.twoD = cfg->ny > 1; g.xper = cfg->bc[0] == SPB6_BC_PERIODIC; g.yper = cfg->bc[2] == SPB6_BC_PERIODIC;
+	for (int q = 0; q < 4; q++) { g.bc[q] = cfg->bc[q]; g.bcu[q] = cfg->bcu[q]; }
+	g.dx = (cfg->x1 - cfg->x0) / cfg->nx; g.dy = (cfg->y1 - cfg->y0) / cfg->ny;
+	for (int k = 0; k < 2; k++) { c->ph.g[k] = cfg->ph[k].gamma; c->ph.pi[k] = cfg->ph[k].pi_inf; c->ph.q[k] = cfg->ph[k].q; c->ph.b[k] = cfg->ph[k].b; c->ph.cv[k] = cfg->ph[k].cv; }
+	const size_t NC = g.NC, n = g.n;
+	g_err.clear();
+	double** grids[] = {&c->U, &c->Ust, &c->Qe, &c->Qs, &c->H[0], &c->H[1], &c->wxL, &c->wxR};
+	for (auto p : grids) if (dalloc(c, p, NC * NV)) return fail("alloc");
+	if (g.twoD) { if (dalloc(c, &c->wyL, NC * NV) || dalloc(c, &c->wyR, NC * NV)) return fail("alloc"); }
+	else { c->wyL = c->wxL; c->wyR = c->wxR; }
+	double** arrs[] = {&c->phx, &c->phy, &c->a2s, &c->ubx, &c->uby, &c->vsc, &c->esc, &c->ust, &c->vst, &c->cz, &c->Kw, &c->Pold,
+	                   &c->ubsx, &c->kpx, &c->wLx, &c->pvx, &c->ubsy, &c->kpy, &c->wLy, &c->pvy};
+	for (auto p : arrs) if (dalloc(c, p, NC)) return fail("alloc");
+	if (dalloc(c, &c->Qph, 2 * NC)) return fail("alloc");
+	double** vecs[] = {&c->P, &c->Pst, &c->Pn, &c->G, &c->Gn, &c->dP, &c->rhs, &c->sc, &c->cpw, &c->dpw, &c->kr, &c->kr0, &c->kp, &c->kv, &c->ks_, &c->kt};
+	for (auto p : vecs) if (dalloc(c, p, n)) return fail("alloc");
+	if (dalloc(c, &c->J, 5 * n)) return fail("alloc");
+	c->part_stride = (int)((n + 1023) / 1024) + 1;
+	if (dalloc(c, &c->part, 4 * (size_t)c->part_stride)) return fail("alloc");
+	if (hipMalloc((void**)&c->counter, 64) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->counter);
+	hipMemset(c->counter, 0, 64);
+	if (hipMalloc((void**)&c->ksd, sizeof(KS)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->ksd);
+	if (hipMalloc((void**)&c->nrm, 8 * sizeof(unsigned long long)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->nrm);
+	if (hipMalloc((void**)&c->flags, NFLAGS * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->flags);
+	if (hipHostMalloc((void**)&c->hs, sizeof(HostScratch), hipHostMallocDefault) != hipSuccess) return fail("host alloc");
+	if (hipDeviceSynchronize() != hipSuccess) return fail("sync");
+	return c;
+}
+
+void spb6_destroy(spb6_ctx* c) {
+	if (!c) return;
+	if (c->st) hipStreamSynchronize(c->st);
+	for (void* p : c->allocs) hipFree(p);
+	if (c->hs) hipHostFree(c->hs);
+	if (c->st) hipStreamDestroy(c->st);
+	delete c;
+}
+
+int spb6_set_state(spb6_ctx* c, const double* U_aos) {
+	if (!c || !U_aos) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	const Geo& g = c->g;
+	std::vector<double> h((size_t)g.NC * NV, 0.0);
+	for (int j = 0; j < g.ny; j++) for (int i = 0; i < g.nx; i++) {
+		const size_t cc = (size_t)(i + GH) + (size_t)(j + GH) * g.NXT;
+		for (int k = 0; k < NV; k++) h[(size_t)k * g.NC + cc] = U_aos[((size_t)i + (size_t)j * g.nx) * NV + k];
+	}
+	HIPCHK(hipMemcpyAsync(c->U, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+	launch_bc_grid(g, c->U, c->st);
+	// P_REF = min pbar, P = pbar - P_REF (PC:1222-1227)
+	std::vector<double> P((size_t)g.n);
+	double pref = INFINITY;
+	for (int q = 0; q < g.n; q++) { const double pb = pbar_host(c->ph, &U_aos[(size_t)q * NV]); pref = (pb < pref) ? pb : pref; }
+	if (!std::isfinite(pref)) pref = 0.0;
+	for (int q = 0; q < g.n; q++) P[q] = pbar_host(c->ph, &U_aos[(size_t)q * NV]) - pref;
+	c->P_REF = pref;
+	HIPCHK(hipMemcpyAsync(c->P, P.data(), P.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+	HIPCHK(hipMemcpyAsync(c->Pst, P.data(), P.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	c->t = 0.0; c->dtprev = -1.0; c->startup = true; c->last_krylov = 0;
+	std::memset(&c->stats, 0, sizeof c->stats);
+	return SPB6_OK;
+}
+
+int spb6_get_state(spb6_ctx* c, double* U_aos) {
+	if (!c || !U_aos) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	const Geo& g = c->g;
+	std::vector<double> h((size_t)g.NC * NV);
+	HIPCHK(hipMemcpyAsync(h.data(), c->U, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	for (int j = 0; j < g.ny; j++) for (int i = 0; i < g.nx; i++) {
+		const size_t cc = (size_t)(i + GH) + (size_t)(j + GH) * g.NXT;
+		for (int k = 0; k < NV; k++) U_aos[((size_t)i + (size_t)j * g.nx) * NV + k] = h[(size_t)k * g.NC + cc];
+	}
+	return SPB6_OK;
+}
+
+int spb6_get_pressure(spb6_ctx* c, double* P, double* p_ref) {
+	if (!c) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	if (P) { HIPCHK(hipMemcpyAsync(P, c->P, (size_t)c->g.n * sizeof(double), hipMemcpyDeviceToHost, c->st)); HIPCHK(hipStreamSynchronize(c->st)); }
+	if (p_ref) *p_ref = c->P_REF;
+	return SPB6_OK;
+}
+int spb6_set_pressure(spb6_ctx* c, const double* P, double p_ref) {
+	if (!c || !P) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	c->P_REF = p_ref;
+	HIPCHK(hipMemcpyAsync(c->P, P, (size_t)c->g.n * sizeof(double), hipMemcpyHostToDevice, c->st));
+	HIPCHK(hipMemcpyAsync(c->Pst, P, (size_t)c->g.n * sizeof(double), hipMemcpyHostToDevice, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	return SPB6_OK;
+}
+
+int spb6_caldt(spb6_ctx* c, int acoustic, double* inv_rate) {
+	if (!c || !inv_rate) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	return caldt(c, acoustic != 0, *inv_rate);
+}
+int spb6_bc_fill(spb6_ctx* c) {
+	if (!c) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	launch_bc_grid(c->g, c->U, c->st);
+	HIPCHK(hipStreamSynchronize(c->st));
+	return SPB6_OK;
+}
+
+int spb6_explicit_stage(spb6_ctx* c, double adt) {
+	if (!c) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	HIPCHK(hipMemcpyAsync(c->Pst, c->P, (size_t)c->g.n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+	int rc = reset_scalars(c); if (rc) return rc;
+	rc = explicit_part(c, c->U, c->U, adt); if (rc) return rc;
+	rc = fetch(c); if (rc) return rc;
+	return c->hs->flags[FLAG_RHO] ? SPB6_RHO_NONPOSITIVE : SPB6_OK;
+}
+int spb6_stage_prep(spb6_ctx* c, double adt) {
+	if (!c) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	int rc = reset_scalars(c); if (rc) return rc;
+	rc = prep_part(c, adt); if (rc) return rc;
+	rc = fetch(c); if (rc) return rc;
+	return c->hs->flags[FLAG_STAGE] ? SPB6_STAGE_INADMISSIBLE : SPB6_OK;
+}
+int spb6_implicit_stage(spb6_ctx* c, double adt, int* newton_iters, double* residual) {
+	if (!c) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	int it = 0; double res = 0;
+	const int rc = implicit_part(c, adt, c->U, c->H[0], it, res);
+	if (newton_iters) *newton_iters = it;
+	if (residual) *residual = res;
+	return rc;
+}
+int spb6_step(spb6_ctx* c, double dt, int rk) {
+	if (!c || !(dt > 0) || rk < 1 || rk > 2) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	const int rc = attempt(c, dt, rk);
+	if (rc == SPB6_OK) { std::swap(c->U, c->Ust); std::swap(c->P, c->Pst); c->t += dt; c->stats.steps++; }
+	return rc;
+}
+
+int spb6_run(spb6_ctx* c, double t_end, long max_steps) {
+	if (!c) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	const double ramp_factor = 1.2;
+	long done_steps = 0;
+	while (c->t < t_end * (1 - 1e-12) && (max_steps < 0 || done_steps < max_steps)) {
+		bool ramping = false; double dt, ir;
+		int rc;
+		if (c->cfg.dtfix > 0) dt = c->cfg.dtfix;
+		else {
+			rc = caldt(c, c->cfg.acdt != 0, ir); if (rc) return rc;
+			dt = c->cfg.cfl * ir;
+			if (c->cfg.ramp) {
+				if (c->dtprev < 0) { rc = caldt(c, true, ir); if (rc) return rc; c->dtprev = c->cfg.cfl_ac * ir / ramp_factor; }
+				if (dt > ramp_factor * c->dtprev) { dt = ramp_factor * c->dtprev; ramping = true; }
+			}
+			if (!std::isfinite(dt)) { rc = caldt(c, true, ir); if (rc) return rc; dt = c->cfg.cfl_ac * ir; }
+		}
+		dt = ((t_end - c->t) < dt) ? (t_end - c->t) : dt;
+		if (!(dt > 1e-12 * t_end)) { c->stats.status = SPB6_DT_UNDERFLOW; return SPB6_DT_UNDERFLOW; }
+		bool ok = false;
+		for (int att = 0; att < 30 && !ok; att++) {
+			const bool first_order = c->cfg.ramp && ramping && c->startup;
+			const int rk = first_order ? 1 : c->cfg.rk; if (rk == 1) c->stats.rk1++;
+			rc = attempt(c, dt, rk);
+			if (rc == SPB6_HIP_ERROR || rc == SPB6_BAD_ARG) return rc;
+			if (rc == SPB6_OK) { std::swap(c->U, c->Ust); std::swap(c->P, c->Pst); ok = true; }
+			else { c->stats.rejects++; dt *= 0.5; }
+		}
+		if (!ok) { c->stats.status = SPB6_STEP_FAILED; return SPB6_STEP_FAILED; }
+		c->t = c->t + dt; c->dtprev = dt; c->stats.steps++; c->stats.dt_last = dt; done_steps++;
+		if (!ramping) c->startup = false;
+	}
+	c->stats.t = c->t;
+	return SPB6_OK;
+}
+
+int spb6_get_stats(spb6_ctx* c, spb6_stats_t* s) { if (!c || !s) return SPB6_BAD_ARG; *s = c->stats; s->t = c->t; return SPB6_OK; }
+double spb6_time(spb6_ctx* c) { return c ? c->t : 0.0; }
+
+int spb6_residual(spb6_ctx* c, const double* P, double* G) {
+	if (!c || !P || !G) return -1;
+	hipSetDevice(c->device);
+	const size_t n = c->g.n;
+	if (hipMemcpyAsync(c->Pn, P, n * sizeof(double), hipMemcpyHostToDevice, c->st) != hipSuccess) return -1;
+	Dev D = stage_dev(c, c->tau_last);
+	if (reset_scalars(c)) return -1;
+	launch_residual(D, c->Pn, nullptr, 0.0, nullptr, c->Gn, &c->nrm[0], FLAG_RES, c->st);
+	if (hipMemcpyAsync(G, c->Gn, n * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess) return -1;
+	if (hipStreamSynchronize(c->st) != hipSuccess) return -1;
+	int bad = 0; for (size_t q = 0; q < n; q++) if (G[q] != G[q]) bad++;
+	return bad;
+}
+int spb6_jacobian(spb6_ctx* c, const double* P, const double* G, double* val, double* sc) {
+	if (!c || !P || !G || !val || !sc) return -1;
+	hipSetDevice(c->device);
+	const size_t n = c->g.n;
+	if (hipMemcpyAsync(c->Pn, P, n * sizeof(double), hipMemcpyHostToDevice, c->st) != hipSuccess) return -1;
+	if (hipMemcpyAsync(c->Gn, G, n * sizeof(double), hipMemcpyHostToDevice, c->st) != hipSuccess) return -1;
+	Dev D = stage_dev(c, c->tau_last);
+	if (reset_scalars(c)) return -1;
+	launch_jacobian(D, c->Pn, c->Gn, c->sc, c->J, c->rhs, c->st);
+	std::vector<double> J(5 * n);
+	if (hipMemcpyAsync(J.data(), c->J, 5 * n * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess) return -1;
+	if (hipMemcpyAsync(sc, c->sc, n * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess) return -1;
+	if (fetch(c)) return -1;
+	for (size_t r = 0; r < n; r++) for (int w = 0; w < 5; w++) val[5 * r + w] = J[w * n + r];
+	return c->hs->flags[FLAG_JAC] ? 1 : 0;
+}
+int spb6_linsolve(spb6_ctx* c, const double* val, const double* b, double* x, int use_thomas, int* iters) {
+	if (!c || !val || !b || !x) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	const size_t n = c->g.n;
+	std::vector<double> J(5 * n);
+	for (size_t r = 0; r < n; r++) for (int w = 0; w < 5; w++) J[w * n + r] = val[5 * r + w];
+	HIPCHK(hipMemcpyAsync(c->J, J.data(), 5 * n * sizeof(double), hipMemcpyHostToDevice, c->st));
+	HIPCHK(hipMemcpyAsync(c->rhs, b, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+	int it = 0; bool ok = true;
+	if (use_thomas) launch_thomas((int)n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
+	else { int rc = krylov(c, it, ok); if (rc) return rc; }
+	HIPCHK(hipMemcpyAsync(x, c->dP, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	if (iters) *iters = it;
+	return ok ? SPB6_OK : SPB6_KRYLOV_FAIL;
+}
+
+long spb6_get_field(spb6_ctx* c, const char* name, double* out, long capacity) {
+	if (!c || !name || !out) return -1;
+	hipSetDevice(c->device);
+	const std::string s(name);
+	const Geo& g = c->g;
+	const double* grid = nullptr; const double* arr = nullptr; long cnt = 0;
+	if (s == "U") grid = c->U; else if (s == "Ust") grid = c->Ust; else if (s == "Qe") grid = c->Qe; else if (s == "Qs") grid = c->Qs;
+	else if (s == "wxL") grid = c->wxL; else if (s == "wxR") grid = c->wxR; else if (s == "wyL") grid = c->wyL; else if (s == "wyR") grid = c->wyR;
+	else if (s == "phx") arr = c->phx; else if (s == "phy") arr = c->phy; else if (s == "a2s") arr = c->a2s; else if (s == "ubx") arr = c->ubx; else if (s == "uby") arr = c->uby;
+	else if (s == "ust") arr = c->ust; else if (s == "vst") arr = c->vst; else if (s == "cz") arr = c->cz; else if (s == "Kw") arr = c->Kw;
+	else if (s == "ubsx") arr = c->ubsx; else if (s == "kpx") arr = c->kpx; else if (s == "wLx") arr = c->wLx; else if (s == "pvx") arr = c->pvx;
+	else if (s == "ubsy") arr = c->ubsy; else if (s == "kpy") arr = c->kpy; else if (s == "wLy") arr = c->wLy; else if (s == "pvy") arr = c->pvy;
+	else if (s == "Qph") { arr = c->Qph; cnt = 2L * g.NC; }
+	else if (s == "P") { arr = c->P; cnt = g.n; } else if (s == "Pst") { arr = c->Pst; cnt = g.n; }
+	else return -1;
+	if (grid) {
+		const long tot = (long)g.NC * NV; if (capacity < tot) return -1;
+		std::vector<double> h(tot);
+		if (hipMemcpyAsync(h.data(), grid, tot * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess) return -1;
+		if (hipStreamSynchronize(c->st) != hipSuccess) return -1;
+		for (long cc = 0; cc < g.NC; cc++) for (int k = 0; k < NV; k++) out[cc * NV + k] = h[(size_t)k * g.NC + cc];
+		return tot;
+	}
+	if (!cnt) cnt = g.NC;
+	if (capacity < cnt) return -1;
+	if (hipMemcpyAsync(out, arr, cnt * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess) return -1;
+	if (hipStreamSynchronize(c->st) != hipSuccess) return -1;
+	return cnt;
+}
+
+int spb6_enable_timing(spb6_ctx* c, int on) {
+	if (!c) return SPB6_BAD_ARG;
+	c->timer.resolve();
+	c->timer.on = on != 0;
+	for (int k = 0; k < T_NCAT; k++) { c->timer.ms[k] = 0; c->timer.cnt[k] = 0; }
+	return SPB6_OK;
+}
+int spb6_get_kernel_times(spb6_ctx* c, spb6_kernel_times_t* t) {
+	if (!c || !t) return SPB6_BAD_ARG;
+	c->timer.resolve();
+	const Timer& T = c->timer;
+	t->explicit_ms = T.ms[T_EXPLICIT]; t->prep_ms = T.ms[T_PREP]; t->shock_ms = T.ms[T_SHOCK]; t->residual_ms = T.ms[T_RESIDUAL];
+	t->jacobian_ms = T.ms[T_JACOBIAN]; t->krylov_ms = T.ms[T_KRYLOV]; t->final_ms = T.ms[T_FINAL]; t->other_ms = T.ms[T_OTHER];
+	t->explicit_n = T.cnt[T_EXPLICIT]; t->prep_n = T.cnt[T_PREP]; t->shock_n = T.cnt[T_SHOCK]; t->residual_n = T.cnt[T_RESIDUAL];
+	t->jacobian_n = T.cnt[T_JACOBIAN]; t->krylov_n = T.cnt[T_KRYLOV]; t->final_n = T.cnt[T_FINAL];
+	t->krylov_iters = c->stats.krylov_it;
+	return SPB6_OK;
+}
+
+}   // extern "C"

@@ -1,0 +1,358 @@
+// spb6_device.h -- per-cell FP64 physics of the SPB 6+1 IMEX step for gfx950 (stiffened-gas phases, p closure).
+//
+// Every function evaluates its operations in the reference's order (src/spb6_physics.h "PHY", src/spb6_physics_omp.h
+// "PHO", src/SPB6_PC_IMEX.cpp "PC") and the library is compiled with -ffp-contract=off, so each result is the same IEEE
+// double the reference's x86-64 build produces (division and square root are correctly rounded on both). The only
+// transcendental functions of the path (shock_share's exp / log / pow) use the portable routines below, which run the
+// same operations on host and device.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#define SPB_HD __host__ __device__ __forceinline__
+
+namespace spb {
+
+enum { IA1 = 0, IM1 = 1, IM2 = 2, IMU = 3, IMV = 4, IE1 = 5, IE2 = 6, IEN = 7 };
+constexpr int NV = 8, GH = 2;
+enum { BC_TRANS = 0, BC_WALL = 1, BC_PERIODIC = 2 };
+
+SPB_HD double dinf() { return __builtin_huge_val(); }
+SPB_HD double dnan() { return __builtin_nan(""); }
+// std::max / std::min semantics: max(a, b) = (a < b) ? b : a, min(a, b) = (b < a) ? b : a
+SPB_HD double smax(double a, double b) { return (a < b) ? b : a; }
+SPB_HD double smin(double a, double b) { return (b < a) ? b : a; }
+SPB_HD bool isfin(double x) { return __builtin_isfinite(x); }
+
+// ------------------------------------------------------------------ portable exp / log / pow / hypot
+// exp: x = k ln2 + r (Cody-Waite, ln2 split in a 32-bit head and a tail), |r| <= ln2/2, Taylor degree 13, exact 2^k.
+// log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), f = m - 1, s = f/(2+f), log m = 2 atanh s = f - s (f - 2 R),
+//      R = sum_{j>=1} s^{2j}/(2j+1); plus e ln2 in two parts.
+SPB_HD double pexp(double x) {
+	if (x != x) return x;
+	if (x > 709.782712893384) return dinf();
+	if (x < -745.1332191019412) return 0.0;
+	const double k = floor(x * 1.44269504088896338700e+00 + 0.5);
+	const double r = (x - k * 6.93147180369123816490e-01) - k * 1.90821492927058770002e-10;
+	double p = 1.0 / 6227020800.0;
+	p = 1.0 / 479001600.0 + r * p;
+	p = 1.0 / 39916800.0 + r * p;
+	p = 1.0 / 3628800.0 + r * p;
+	p = 1.0 / 362880.0 + r * p;
+	p = 1.0 / 40320.0 + r * p;
+	p = 1.0 / 5040.0 + r * p;
+	p = 1.0 / 720.0 + r * p;
+	p = 1.0 / 120.0 + r * p;
+	p = 1.0 / 24.0 + r * p;
+	p = 1.0 / 6.0 + r * p;
+	p = 0.5 + r * p;
+	p = 1.0 + r * p;
+	p = 1.0 + r * p;
+	const int ki = (int)k;
+	if (ki > 1000) return ldexp(ldexp(p, ki - 1000), 1000);
+	if (ki < -1000) return ldexp(ldexp(p, ki + 1000), -1000);
+	return ldexp(p, ki);
+}
+SPB_HD double plog(double x) {
+	if (x != x) return x;
+	if (x < 0.0) return dnan();
+	if (x == 0.0) return -dinf();
+	if (x == dinf()) return dinf();
+	int e; double m = frexp(x, &e);
+	if (m < 0.70710678118654752440) { m = m + m; e = e - 1; }
+	const double f = m - 1.0, s = f / (2.0 + f), z = s * s;
+	double R = 1.0 / 23.0;
+	R = 1.0 / 21.0 + z * R;
+	R = 1.0 / 19.0 + z * R;
+	R = 1.0 / 17.0 + z * R;
+	R = 1.0 / 15.0 + z * R;
+	R = 1.0 / 13.0 + z * R;
+	R = 1.0 / 11.0 + z * R;
+	R = 1.0 / 9.0 + z * R;
+	R = 1.0 / 7.0 + z * R;
+	R = 1.0 / 5.0 + z * R;
+	R = 1.0 / 3.0 + z * R;
+	R = z * R;
+	const double logm = f - s * (f - 2.0 * R);
+	const double de = (double)e;
+	return de * 6.93147180369123816490e-01 + (logm + de * 1.90821492927058770002e-10);
+}
+SPB_HD double ppow(double x, double y) { return pexp(y * plog(x)); }
+SPB_HD double phypot(double a, double b) {
+	if (b == 0.0) return fabs(a);
+	if (a == 0.0) return fabs(b);
+	return sqrt(a * a + b * b);
+}
+
+// ------------------------------------------------------------------ phases (stiffened gas / NASG covolume b)
+struct Phys {
+	double g[2], pi[2], q[2], b[2], cv[2];
+};
+
+SPB_HD double eps_amp(const Phys& P, int k, double a, double m, double p) {          // PHY:69-71
+	return (p + P.g[k] * P.pi[k]) * (a - m * P.b[k]) / (P.g[k] - 1.0) + m * P.q[k]; }
+SPB_HD double p_ame(const Phys& P, int k, double a, double m, double e) {            // PHY:72-74
+	return (e - m * P.q[k]) * (P.g[k] - 1.0) / (a - m * P.b[k]) - P.g[k] * P.pi[k]; }
+SPB_HD double c2_rp(const Phys& P, int k, double r, double p) {                      // PHY:83-85
+	return P.g[k] * (p + P.pi[k]) / (r * (1.0 - r * P.b[k])); }
+SPB_HD double dedp_am(const Phys& P, int k, double a, double m) {                    // PHY:86-88
+	return (a - m * P.b[k]) / (P.g[k] - 1.0); }
+SPB_HD double p_mix(const Phys& P, double a1, double m1, double m2, double rhoe) {  // PHY:90-99
+	const double a0 = a1, aa1 = 1.0 - a1;
+	double num = rhoe, den = 0.0;
+	const double d0 = dedp_am(P, 0, a0, m1);
+	num -= m1 * P.q[0] + P.g[0] * P.pi[0] * d0;
+	den += d0;
+	const double d1 = dedp_am(P, 1, aa1, m2);
+	num -= m2 * P.q[1] + P.g[1] * P.pi[1] * d1;
+	den += d1;
+	return num / den;
+}
+SPB_HD bool p_adm(const Phys& P, int k, double p) { return p + P.pi[k] > 0; }       // PHY:101-104 (SG)
+
+// frozen sound speed of prim_of (PHY:837-847); false where prim_of fails
+SPB_HD bool prim_cf(const Phys& P, double a1, double m1, double m2, double e1, double e2, double& cf) {
+	const double a2 = 1 - a1;
+	if (!(a1 > 0 && a2 > 0 && m1 > 0 && m2 > 0)) return false;
+	const double rho = m1 + m2, r1 = m1 / a1, r2 = m2 / a2;
+	const double p1 = p_ame(P, 0, a1, m1, e1), p2 = p_ame(P, 1, a2, m2, e2);
+	if (!(p_adm(P, 0, p1) && p_adm(P, 1, p2))) return false;
+	cf = sqrt((m1 * c2_rp(P, 0, r1, p1) + m2 * c2_rp(P, 1, r2, p2)) / rho);
+	return true;
+}
+
+// ------------------------------------------------------------------ shock budget (PHY:260-418 with PHO:365-374)
+struct Prof { double Y0, Y1, vk00, v0, p0, j2, vend; };
+SPB_HD bool eos_ok(const Phys& P, int k, double p, double v) {                       // PHY:271-275 (SG)
+	if (!(v > 0) || !isfin(p)) return false;
+	return v > P.b[k] && p + P.pi[k] > 0; }
+SPB_HD double hug_v(const Phys& P, int k, double p0, double p1, double v0) {         // PHY:277-282 (SG)
+	const double x0 = p0 + P.pi[k], x1 = p1 + P.pi[k];
+	return P.b[k] + (v0 - P.b[k]) * ((P.g[k] - 1.0) * x1 + (P.g[k] + 1.0) * x0) / ((P.g[k] + 1.0) * x1 + (P.g[k] - 1.0) * x0);
+}
+SPB_HD bool prof_rhs(const Phys& P, const Prof& S, double c, double v, const double* y, double* dy) {   // PHY:302-310
+	const double p = y[0], v1 = y[1], v2 = (v - S.Y0 * v1) / S.Y1, Pr = S.p0 + S.j2 * (S.v0 - v);
+	if (!eos_ok(P, 0, p, v1) || !eos_ok(P, 1, p, v2)) return false;
+	const double lam0 = c, lam1 = 1.0 - c;
+	double num = -1.0, den = 0.0;
+	const double ep0 = (v1 - P.b[0]) / (P.g[0] - 1.0), evp0 = P.g[0] * (p + P.pi[0]) / (P.g[0] - 1.0);
+	const double r0 = lam0 / S.Y0 * (p - Pr);
+	num += S.Y0 * r0 / evp0; den += S.Y0 * ep0 / evp0;
+	const double ep1 = (v2 - P.b[1]) / (P.g[1] - 1.0), evp1 = P.g[1] * (p + P.pi[1]) / (P.g[1] - 1.0);
+	const double r1 = lam1 / S.Y1 * (p - Pr);
+	num += S.Y1 * r1 / evp1; den += S.Y1 * ep1 / evp1;
+	dy[0] = num / den; dy[1] = (r0 - ep0 * dy[0]) / evp0; dy[2] = p - Pr;
+	return isfin(dy[0]) && isfin(dy[1]);
+}
+// adaptive Cash-Karp RK4(5) integration of the relaxed shock profile (PHY:311-341)
+__device__ __forceinline__ bool prof_integrate(const Phys& P, const Prof& S, double c, double& v1end, double& qend) {
+	constexpr double A1 = 0.2, A2 = 0.3, A3 = 0.6, A4 = 1.0, A5 = 0.875;
+	constexpr double B10 = 0.2;
+	constexpr double B20 = 3.0 / 40, B21 = 9.0 / 40;
+	constexpr double B30 = 0.3, B31 = -0.9, B32 = 1.2;
+	constexpr double B40 = -11.0 / 54, B41 = 2.5, B42 = -70.0 / 27, B43 = 35.0 / 27;
+	constexpr double B50 = 1631.0 / 55296, B51 = 175.0 / 512, B52 = 575.0 / 13824, B53 = 44275.0 / 110592, B54 = 253.0 / 4096;
+	constexpr double C50 = 37.0 / 378, C52 = 250.0 / 621, C53 = 125.0 / 594, C55 = 512.0 / 1771;
+	constexpr double C40 = 2825.0 / 27648, C42 = 18575.0 / 48384, C43 = 13525.0 / 55296, C44 = 277.0 / 14336, C45 = 0.25;
+	double y[3] = {S.p0, S.vk00, 0.0}, v = S.v0;
+	const double L = S.vend - S.v0;
+	const double sc[3] = {fabs(S.p0) + S.j2 * fabs(L), S.vk00, S.j2 * L * L + 1e-300};
+	double h = L / 32.0;
+	for (int it = 0; it < 4000; it++) {
+		const bool last = (L < 0 && v + h <= S.vend) || (L > 0 && v + h >= S.vend);
+		if (last) h = S.vend - v;
+		double k0[3], k1[3], k2[3], k3[3], k4[3], k5[3], yt[3];
+		bool ok = prof_rhs(P, S, c, v + 0.0 * h, y, k0);
+		// stage inputs are accumulated term by term in the same order as the reference's loop over q
+		if (ok) { for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B10 * k0[n]; } ok = prof_rhs(P, S, c, v + A1 * h, yt, k1); }
+		if (ok) { for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B20 * k0[n]; yt[n] += h * B21 * k1[n]; } ok = prof_rhs(P, S, c, v + A2 * h, yt, k2); }
+		if (ok) { for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B30 * k0[n]; yt[n] += h * B31 * k1[n]; yt[n] += h * B32 * k2[n]; } ok = prof_rhs(P, S, c, v + A3 * h, yt, k3); }
+		if (ok) { for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B40 * k0[n]; yt[n] += h * B41 * k1[n]; yt[n] += h * B42 * k2[n]; yt[n] += h * B43 * k3[n]; } ok = prof_rhs(P, S, c, v + A4 * h, yt, k4); }
+		if (ok) { for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B50 * k0[n]; yt[n] += h * B51 * k1[n]; yt[n] += h * B52 * k2[n]; yt[n] += h * B53 * k3[n]; yt[n] += h * B54 * k4[n]; } ok = prof_rhs(P, S, c, v + A5 * h, yt, k5); }
+		double err = 0.0, y5[3];
+		if (ok) for (int n = 0; n < 3; n++) {
+			double d5 = 0, d4 = 0;
+			d5 += C50 * k0[n]; d4 += C40 * k0[n];
+			d5 += 0.0 * k1[n]; d4 += 0.0 * k1[n];
+			d5 += C52 * k2[n]; d4 += C42 * k2[n];
+			d5 += C53 * k3[n]; d4 += C43 * k3[n];
+			d5 += 0.0 * k4[n]; d4 += C44 * k4[n];
+			d5 += C55 * k5[n]; d4 += C45 * k5[n];
+			y5[n] = y[n] + h * d5; err = smax(err, fabs(h * (d5 - d4)) / sc[n]);
+		}
+		if (!ok || !(err <= 1e-8)) {
+			h *= ok ? smax(0.1, 0.9 * ppow(1e-8 / err, 0.25)) : 0.25;
+			if (fabs(h) < 1e-14 * fabs(L)) return false;
+			continue;
+		}
+		for (int n = 0; n < 3; n++) y[n] = y5[n];
+		if (last) { v1end = y[1]; qend = y[2]; return true; }
+		v += h;
+		h *= smin(5.0, 0.9 * ppow(1e-8 / smax(err, 1e-30), 0.2));
+	}
+	return false;
+}
+__device__ __forceinline__ double share_F(const Phys& P, const Prof& S, double vH0, double vpre0, double t, double& qq) {
+	const double c = 1.0 / (1.0 + pexp(-t)); double v1e;
+	if (!prof_integrate(P, S, c, v1e, qq)) return dnan();
+	return (v1e - vH0) / vpre0;
+}
+// phase shares lam and mixture dissipation q of a shock from (p0, vpre, Y) to p1 (PHY:343-383, PHO start bracket)
+__device__ __noinline__ bool shock_share(const Phys& P, const double Y[2], const double vpre[2], double p0, double p1, double lam[2], double& q) {
+	double K[2], G[2], Kmin = dinf();
+	for (int k = 0; k < 2; k++) {
+		K[k] = c2_rp(P, k, 1.0 / vpre[k], p0) / vpre[k];
+		G[k] = 0.5 * (P.g[k] + 1.0); if (P.b[k] != 0.0) G[k] *= vpre[k] / (vpre[k] - P.b[k]);
+		if (!(K[k] > 0)) return false; Kmin = smin(Kmin, K[k]);
+	}
+	const double dp = p1 - p0;
+	if (dp < 1e-2 * Kmin) {
+		double w[2], sw = 0.0;
+		for (int k = 0; k < 2; k++) { w[k] = Y[k] * vpre[k] * G[k] / (K[k] * K[k]); sw += w[k]; }
+		for (int k = 0; k < 2; k++) lam[k] = w[k] / sw;
+		q = sw * dp * dp * dp / 6.0;
+		return true;
+	}
+	Prof S; S.p0 = p0; S.Y0 = Y[0]; S.Y1 = Y[1]; S.vk00 = vpre[0];
+	const double vH0 = hug_v(P, 0, p0, p1, vpre[0]), vH1 = hug_v(P, 1, p0, p1, vpre[1]);
+	if (!isfin(vH0) || !isfin(vH1)) return false;
+	S.v0 = Y[0] * vpre[0] + Y[1] * vpre[1]; S.vend = Y[0] * vH0 + Y[1] * vH1;
+	if (!(S.vend < S.v0)) return false;
+	S.j2 = dp / (S.v0 - S.vend);
+	double t0;
+	{ double ww0 = Y[0] * vpre[0] * G[0] / (K[0] * K[0]), ww1 = Y[1] * vpre[1] * G[1] / (K[1] * K[1]);
+	  const double c0 = smin(1.0 - 1e-12, smax(1e-12, ww0 / (ww0 + ww1)));
+	  t0 = isfin(c0) ? plog(c0 / (1.0 - c0)) : 0.0; }
+	double ta = smax(-40.0, t0 - 1.0), tb = smin(40.0, t0 + 1.0), qa, qb;
+	double fa = share_F(P, S, vH0, vpre[0], ta, qa), fb = share_F(P, S, vH0, vpre[0], tb, qb);
+	if (!isfin(fa) || !isfin(fb)) return false;
+	for (double st = 2.0; fa >= 0 && ta > -40.0; st *= 2.0) { tb = ta; fb = fa; qb = qa; ta = smax(-40.0, ta - st); fa = share_F(P, S, vH0, vpre[0], ta, qa); if (!isfin(fa)) return false; }
+	for (double st = 2.0; fb <= 0 && tb < 40.0; st *= 2.0) { ta = tb; fa = fb; qa = qb; tb = smin(40.0, tb + st); fb = share_F(P, S, vH0, vpre[0], tb, qb); if (!isfin(fb)) return false; }
+	double t, qt;
+	if (fa >= 0) { t = ta; qt = qa; }
+	else if (fb <= 0) { t = tb; qt = qb; }
+	else {
+		int side = 0; t = ta; qt = qa;
+		for (int it = 0; it < 100; it++) {
+			t = (ta * fb - tb * fa) / (fb - fa);
+			if (!(t > ta && t < tb)) t = 0.5 * (ta + tb);
+			const double ft = share_F(P, S, vH0, vpre[0], t, qt);
+			if (!isfin(ft)) return false;
+			if (fabs(ft) < 1e-10 || tb - ta < 1e-10) break;
+			if (ft < 0) { ta = t; fa = ft; if (side == -1) fb *= 0.5; side = -1; }
+			else { tb = t; fb = ft; if (side == 1) fa *= 0.5; side = 1; }
+		}
+	}
+	const double c = 1.0 / (1.0 + pexp(-t));
+	lam[0] = c; lam[1] = 1.0 - c; q = qt;
+	return q > 0;
+}
+// stencil_state (PHY:385-390) from the 8 conserved values
+SPB_HD void stencil_state(const Phys& P, const double* c, double* st) {
+	const double a1 = c[IA1], m1 = c[IM1], m2 = c[IM2], r = m1 + m2;
+	st[0] = a1; st[1] = 1.0 - a1; st[2] = m1; st[3] = m2; st[4] = dnan();
+	st[5] = r > 0 ? c[IMU] / r : 0.0; st[6] = r > 0 ? c[IMV] / r : 0.0;
+	if (a1 > 0 && a1 < 1 && m1 > 0 && m2 > 0) {
+		const double p = p_mix(P, a1, m1, m2, c[IE1] + c[IE2]);
+		if (p_adm(P, 0, p) && p_adm(P, 1, p)) st[4] = p;
+	}
+}
+
+// ------------------------------------------------------------------ p closure (PHY:420-494, PHO:472-548)
+struct ClIn { double atr0, atr1, mtr0, mtr1, etr0, etr1, s, rhoeC, pguess, Q0, Q1; };
+struct ClOut { double a0, a1, e0, e1, pbar; };
+
+SPB_HD double E_above(const Phys& P, int k, double atr, double mtr, double etr) {    // PHY:420-423 (SG)
+	return smax(0.0, etr - mtr * P.q[k] - (atr - mtr * P.b[k]) * P.pi[k]); }
+
+// segment of the piecewise-affine defect share at delta (PHY:428-439): lam, extra
+SPB_HD void defect_seg(double delta, double Qb, double Q0, double Q1, double lE0, double lE1,
+                       double& lam0, double& lam1, double& ex0, double& ex1) {
+	ex0 = 0.0; ex1 = 0.0;
+	if (!(delta > 0.0) || !(Qb > 0.0)) { lam0 = lE0; lam1 = lE1; }
+	else if (delta < Qb) { lam0 = Q0 / Qb; lam1 = Q1 / Qb; }
+	else { lam0 = lE0; lam1 = lE1; ex0 = Q0 - Qb * lE0; ex1 = Q1 - Qb * lE1; }
+}
+// closure_p along the PHO iteration path: safeguarded Newton from the guess, Phi and Phi' together (PHO:472-548).
+// Returns false where the reference's closure returns false; on success fills o (alpha, eps per phase and pbar).
+SPB_HD bool closure_p(const Phys& P, const ClIn& in, ClOut& o) {
+	// closure() input checks (PHY:787-790)
+	if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0)) return false;
+	if (!(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
+	if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
+	if (!(in.mtr0 + in.mtr1 > 0)) return false;
+	const double plo = smax(-P.pi[0], -P.pi[1]);
+	const double D0 = in.rhoeC - in.etr0 - in.etr1;
+	const double sm = in.mtr0 + in.mtr1;
+	double lE0, lE1;
+	{ const double E0 = E_above(P, 0, in.atr0, in.mtr0, in.etr0), E1 = E_above(P, 1, in.atr1, in.mtr1, in.etr1);
+	  double sE = 0.0; sE += E0; sE += E1;
+	  if (sE > 0) { lE0 = E0 / sE; lE1 = E1 / sE; }
+	  else { lE0 = sm > 0 ? in.mtr0 / sm : 0.5; lE1 = sm > 0 ? in.mtr1 / sm : 0.5; } }
+	const double Qb = in.Q0 + in.Q1;
+	const double A00 = in.etr0 - in.mtr0 * P.q[0], A01 = in.etr1 - in.mtr1 * P.q[1];
+	const double g0 = P.g[0], g1 = P.g[1], pi0 = P.pi[0], pi1 = P.pi[1], b0 = P.b[0], b1 = P.b[1];
+	double p = plo + smax(in.pguess - plo, 1e-3 * (fabs(plo) + 1.0));
+	double lo = plo, hi = dinf();
+	bool conv = false;
+	for (int it = 0; it < 400; it++) {
+		double lam0, lam1, ex0, ex1;
+		defect_seg(D0 + p * in.s, Qb, in.Q0, in.Q1, lE0, lE1, lam0, lam1, ex0, ex1);
+		const double A0 = A00 + lam0 * D0 + ex0, B0 = in.atr0 + lam0 * in.s, N0 = A0 - (B0 - in.mtr0 * b0) * pi0;
+		const double A1 = A01 + lam1 * D0 + ex1, B1 = in.atr1 + lam1 * in.s, N1 = A1 - (B1 - in.mtr1 * b1) * pi1;
+		double f = 0.0, df = 0.0;
+		f += ((g0 - 1.0) * (A0 + B0 * p) + (p + g0 * pi0) * in.mtr0 * b0) / (g0 * (p + pi0));
+		df += -(g0 - 1.0) * N0 / (g0 * (p + pi0) * (p + pi0));
+		f += ((g1 - 1.0) * (A1 + B1 * p) + (p + g1 * pi1) * in.mtr1 * b1) / (g1 * (p + pi1));
+		df += -(g1 - 1.0) * N1 / (g1 * (p + pi1) * (p + pi1));
+		f = f - 1.0;
+		if (f == 0.0) { conv = true; break; }
+		if (f > 0) lo = p; else hi = p;
+		double pn = df < 0 ? p - f / df : dnan();
+		if (fabs(pn - p) <= 2e-16 * (fabs(p) + fabs(plo))) { p = pn; conv = true; break; }
+		if (!(pn > lo && pn < hi)) {
+			if (!isfin(hi)) pn = plo + 4.0 * (p - plo);
+			else if (lo == plo) pn = plo + 0.25 * (hi - plo);
+			else pn = (hi - plo > 1e3 * (lo - plo + 1e-300) && lo > plo) ? plo + sqrt((lo - plo) * (hi - plo)) : 0.5 * (lo + hi);
+		}
+		if (fabs(pn - p) <= 2e-16 * (fabs(p) + fabs(plo)) || (hi < dinf() && hi - lo <= 4e-16 * (fabs(hi) + fabs(lo)))) { p = pn; conv = true; break; }
+		p = pn;
+	}
+	if (!conv) return false;
+	// outputs (PHY:487-493)
+	double lam0, lam1, ex0, ex1;
+	defect_seg(D0 + p * in.s, Qb, in.Q0, in.Q1, lE0, lE1, lam0, lam1, ex0, ex1);
+	const double A0 = in.etr0 - in.mtr0 * P.q[0] + lam0 * D0 + ex0, B0 = in.atr0 + lam0 * in.s;
+	const double A1 = in.etr1 - in.mtr1 * P.q[1] + lam1 * D0 + ex1, B1 = in.atr1 + lam1 * in.s;
+	o.a0 = ((g0 - 1.0) * (A0 + B0 * p) + (p + g0 * pi0) * in.mtr0 * b0) / (g0 * (p + pi0));
+	if (!(o.a0 > in.mtr0 * b0) || !(p + pi0 > 0)) return false;
+	o.e0 = eps_amp(P, 0, o.a0, in.mtr0, p);
+	o.a1 = ((g1 - 1.0) * (A1 + B1 * p) + (p + g1 * pi1) * in.mtr1 * b1) / (g1 * (p + pi1));
+	if (!(o.a1 > in.mtr1 * b1) || !(p + pi1 > 0)) return false;
+	o.e1 = eps_amp(P, 1, o.a1, in.mtr1, p);
+	o.pbar = p;
+	return isfin(p);
+}
+// round-off floor of the closure pressure (PC:631-638)
+SPB_HD double closure_roundoff(const Phys& P, const ClIn& in, double p) {
+	const double D0 = in.rhoeC - in.etr0 - in.etr1;
+	const double sm = in.mtr0 + in.mtr1;
+	const double E0 = E_above(P, 0, in.atr0, in.mtr0, in.etr0), E1 = E_above(P, 1, in.atr1, in.mtr1, in.etr1);
+	double sE = 0.0; sE += E0; sE += E1;
+	double lE0, lE1;
+	if (sE > 0) { lE0 = E0 / sE; lE1 = E1 / sE; }
+	else { lE0 = sm > 0 ? in.mtr0 / sm : 0.5; lE1 = sm > 0 ? in.mtr1 / sm : 0.5; }
+	double lam0, lam1, ex0, ex1;
+	defect_seg(D0 + p * in.s, in.Q0 + in.Q1, in.Q0, in.Q1, lE0, lE1, lam0, lam1, ex0, ex1);
+	const double A0 = in.etr0 - in.mtr0 * P.q[0] + lam0 * D0 + ex0, B0 = in.atr0 + lam0 * in.s, N0 = A0 - (B0 - in.mtr0 * P.b[0]) * P.pi[0];
+	const double A1 = in.etr1 - in.mtr1 * P.q[1] + lam1 * D0 + ex1, B1 = in.atr1 + lam1 * in.s, N1 = A1 - (B1 - in.mtr1 * P.b[1]) * P.pi[1];
+	double dphi = 0.0, sens = 0.0;
+	dphi += -(P.g[0] - 1.0) * N0 / (P.g[0] * (p + P.pi[0]) * (p + P.pi[0]));
+	sens += fabs(lam0) * (P.g[0] - 1.0) / (P.g[0] * (p + P.pi[0]));
+	dphi += -(P.g[1] - 1.0) * N1 / (P.g[1] * (p + P.pi[1]) * (p + P.pi[1]));
+	sens += fabs(lam1) * (P.g[1] - 1.0) / (P.g[1] * (p + P.pi[1]));
+	const double en = 2.220446049250313e-16 * (fabs(in.rhoeC) + fabs(in.etr0) + fabs(in.etr1) + fabs(in.mtr0 * P.q[0]) + fabs(in.mtr1 * P.q[1]));
+	return dphi < 0 ? en * sens / -dphi : 0.0;
+}
+
+}   // namespace spb

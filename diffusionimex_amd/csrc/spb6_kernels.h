@@ -1,0 +1,59 @@
+// spb6_kernels.h -- device-side structures and kernel launchers of the SPB 6+1 IMEX step (internal to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "spb6_device.h"
+
+namespace spb {
+
+// error / event flags written by kernels (int slots in device memory)
+enum { FLAG_RHO = 0, FLAG_STAGE = 1, FLAG_RES = 2, FLAG_JAC = 3, FLAG_FLOOR = 4, FLAG_FINAL = 5, FLAG_RES2 = 6, NFLAGS = 8 };
+
+struct Geo {
+	int nx, ny, NXT, NYT, NC, n;   // interior sizes, ghosted sizes, ghosted cell count, interior cell count
+	int twoD, xper, yper;
+	int bc[4]; double bcu[4];
+	double dx, dy;
+};
+
+// BiCGStab loop scalars kept on the device between kernels
+struct KS {
+	double Einv, cc_p, cc_s, bn, rho, alpha, omega, beta;
+	int done, iters, early, status;
+};
+
+// all device pointers of one stage (passed by value to the residual-type kernels)
+struct Dev {
+	Geo g; Phys ph;
+	double P_REF, rx, ry, tau;
+	const double *Ust_c;
+	double *Ust, *a2s, *ubx, *uby, *phx, *phy, *vsc, *esc;
+	double *wxL, *wxR, *wyL, *wyR;
+	double *ust, *vst, *cz, *Kw, *Pold;
+	double *ubsx, *kpx, *wLx, *pvx, *ubsy, *kpy, *wLy, *pvy, *Qph;
+	int* flags;
+};
+
+struct KrylovBufs {
+	double *x, *r, *r0, *p, *v, *s, *t;
+	const double* J;
+	double* part; int stride; unsigned* counter; KS* ks;
+};
+
+void launch_bc_grid(const Geo& g, double* u, hipStream_t st);
+void launch_bc_array(const Geo& g, double* a, hipStream_t st);
+void launch_combine(const Geo& g, const double* Un, const double* H0, const double* H1, int nr, double ae0, double ae1, double ai0, double ai1, double* Qe, double* Qs, hipStream_t st);
+void launch_explicit(const Dev& D, int limiter, int order, const double* Qe, const double* Qs, double adt, hipStream_t st);
+void launch_prep(const Dev& D, int limiter, int order, double tau, const double* Pst, hipStream_t st);
+void launch_shock(const Dev& D, hipStream_t st);
+void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st);
+void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, hipStream_t st);
+void launch_fdscale(const Dev& D, const double* P, double* sc, hipStream_t st);
+void launch_stepcrit(int n, const double* dP, const double* P, const double* sc, unsigned long long* nrm, hipStream_t st);
+void launch_floor(const Dev& D, const double* P, double tol, hipStream_t st);
+void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, hipStream_t st);
+void launch_caldt(const Geo& g, const Phys& P, const double* U, int acoustic, unsigned long long* nrm, hipStream_t st);
+void launch_thomas(int n, const double* J, const double* rhs, double* x, double* cp, double* dp, hipStream_t st);
+void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hipStream_t st, double tol, int maxit);
+void launch_krylov_iter(const Geo& g, const KrylovBufs& B, hipStream_t st, double tol, int maxit);
+
+}   // namespace spb

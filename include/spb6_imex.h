@@ -1,0 +1,143 @@
+/*
+ * spb6_imex.h -- C ABI of the MI355X (gfx950) semi-implicit IMEX-RK time step of the SPB 6+1-equation two-phase model.
+ *
+ * Drop-in boundary for diffusionIMEX's SPB6_PC_IMEX stepper. The reference has no FFI; its "interface" is the set of
+ * free functions, globals and grid types of src/SPB6_PC_IMEX.cpp ("PC") and src/spb6_physics.h ("PHY"). Each entry point
+ * below names the reference function or global it replaces (file:line). Plain C types only: pointers and sizes.
+ *
+ * Ownership: the caller owns host arrays; the context owns every device buffer (allocated once in spb6_create, the
+ * analogue of PC_omp's reused static buffers, src/SPB6_PC_IMEX_omp.cpp:802, :912-918). No allocation per stage or step.
+ * Threading: one context per host thread; calls on one context are not re-entrant. Results are run-to-run deterministic
+ * (fixed-order reductions).
+ * Layout at the boundary: the reference's AoS cell order U[(i + j*nx)*8 + k], interior cells only (PC:104, PC:124);
+ * on the device the state is structure-of-arrays with two ghost layers.
+ */
+#ifndef SPB6_IMEX_H
+#define SPB6_IMEX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Conserved variables per cell (PHY:25-26): alpha1, m1, m2, rho u, rho v, eps1, eps2, E */
+#define SPB6_NUM_VARS 8
+
+/* boundary kinds per side (left, right, bottom, top) -- PHY:906 BC_TRANS / BC_WALL / BC_PERIODIC */
+enum spb6_bc { SPB6_BC_TRANSMISSIVE = 0, SPB6_BC_WALL = 1, SPB6_BC_PERIODIC = 2 };
+
+/* status codes: the reference returns bool false for every one of these and halves dt (PC:1287-1290) */
+enum spb6_status {
+    SPB6_OK = 0,
+    SPB6_RHO_NONPOSITIVE = 1,     /* rho <= 0 in Qe (PC:257) */
+    SPB6_STAGE_INADMISSIBLE = 2,  /* sum alpha* <= 0 or rho* <= 0 (PC:862) */
+    SPB6_CLOSURE_FAIL = 3,        /* inadmissible closure input/output in the final state (PHY:787-790, PHY:490) */
+    SPB6_NEWTON_FAIL = 4,         /* no descent and no round-off floor, or no convergence (PC:842, PC:846) */
+    SPB6_KRYLOV_FAIL = 5,         /* BiCGStab breakdown or no convergence (PC:743, PC:757) */
+    SPB6_DT_UNDERFLOW = 6,        /* dt < 1e-12 t_end (PC:1240) */
+    SPB6_STEP_FAILED = 7,         /* 30 halvings without an admissible step (PC:1290) */
+    SPB6_BAD_ARG = 8,
+    SPB6_HIP_ERROR = 9
+};
+
+/* Phase EOS parameters (PHY:49-54 Phase{g, pi, q, qp, cv, b}); stiffened gas when b = 0 */
+typedef struct spb6_phase {
+    double gamma, pi_inf, q, qp, cv, b;
+} spb6_phase_t;
+
+/* Replaces the reference's configuration globals (PC:57-80, PHY:33-37, PHY:906-913). */
+typedef struct spb6_config {
+    int nx, ny;                      /* interior cells (ny = 1: 1D, PHY:952-956) */
+    double x0, x1, y0, y1;           /* domain (TestSetup, PHY:898) */
+    int bc[4];                       /* BCS[4] (PHY:908) */
+    double bcu[4];                   /* BCU[4]: wall normal velocity (PHY:908) */
+    spb6_phase_t ph[2];              /* PH[2] (PHY:54) */
+    int rk;                          /* RKs: 1 backward Euler, 2 LSDIRK2 (PC:57) */
+    double cfl, cfl_ac;              /* CFL, CFL_ac (PC:59-60) */
+    int ramp;                        /* RAMP (PC:63) */
+    double dtfix;                    /* DTFIX (PC:65) */
+    int acdt;                        /* caltimestep == Fullydt (PC:50) */
+    double newton_tol; int newton_iter;   /* NewtonTol, NewtonIter (PC:67-70) */
+    double krylov_tol; int krylov_iter;   /* GMREStol, GMRESiter (PC:71-72) */
+    int limiter;                     /* LIMITER (PHY:913) */
+    int order;                       /* SpaceOrder (PC:61) */
+    int rus_local;                   /* RUS_LOCAL (PC:80); only 1 is supported on the device */
+    /* additions without a reference counterpart */
+    int krylov_fp32;                 /* 1: BiCGStab vectors/matrix in FP32 inside the FP64 Newton (defect correction) */
+    int learned_guess;               /* 1: Newton initial guess from the MLP (spb6_mlp_*), plain guess as fallback */
+} spb6_config_t;
+
+/* run statistics: the counters of the reference's stats line (PC:1337-1339) plus Krylov counts */
+typedef struct spb6_stats {
+    long steps, rk1, rejects, floor_acc, stages, newton_it, krylov_it, linsolves;
+    double maxres, t, dt_last;
+    int status;
+    long guess_used, guess_fallback;  /* learned-guess stages, and stages where the plain guess was kept */
+} spb6_stats_t;
+
+typedef struct spb6_ctx spb6_ctx;
+
+/* Create a context on HIP device `device`: allocates all device buffers for the grid (main's Grid/Array/PField
+ * allocations, PC:1157-1163, and StageData, PC:565-582). Returns NULL on error (spb6_last_error tells why). */
+spb6_ctx* spb6_create(const spb6_config_t* cfg, int device);
+void spb6_destroy(spb6_ctx* ctx);
+const char* spb6_last_error(void);
+
+/* Upload interior AoS state (u_initialization + bound_cond_u, PC:450-475) and initialise the pressure unknown
+ * P = pbar - P_REF with P_REF = min pbar (PC:1222-1227). Resets time and counters. */
+int spb6_set_state(spb6_ctx* ctx, const double* U_aos);
+/* Download interior AoS state. */
+int spb6_get_state(spb6_ctx* ctx, double* U_aos);
+/* Pressure unknown P (interior, nx*ny, PField PC:155) and the scalar P_REF (PC:68). */
+int spb6_get_pressure(spb6_ctx* ctx, double* P, double* p_ref);
+int spb6_set_pressure(spb6_ctx* ctx, const double* P, double p_ref);
+
+/* caldt (PC:477-488): returns 1 / (max(|u|+c)/dx + max(|v|+c)/dy) in *inv_rate (c = frozen speed if acoustic). */
+int spb6_caldt(spb6_ctx* ctx, int acoustic, double* inv_rate);
+/* bound_cond_u on the state (PHY:926-957). */
+int spb6_bc_fill(spb6_ctx* ctx);
+
+/* Explicit part of one stage with Qe = Qs = U and step adt: TVDReconstruction (PC:349) + Update (PC:251). */
+int spb6_explicit_stage(spb6_ctx* ctx, double adt);
+/* Implicit part of that stage: PressureSubsystem (PC:851-993) = stage prep, shock budget, Newton_Solver (PC:761) with
+ * Thomas (PC:807-812) or BiCGStab (PC:725-758), final state. The stage's Newton guess is the current pressure. */
+int spb6_implicit_stage(spb6_ctx* ctx, double adt, int* newton_iters, double* residual);
+/* Only the stage preparation and shock budget of PressureSubsystem (PC:859-969), for per-kernel checks. */
+int spb6_stage_prep(spb6_ctx* ctx, double adt);
+/* One attempt of a step of size dt with an rk-stage table (body of the attempt loop, PC:1243-1285); on success the
+ * state and pressure advance. */
+int spb6_step(spb6_ctx* ctx, double dt, int rk);
+/* The time loop (PC:1230-1296): dt from caldt with the start-up ramp, reject-and-halve, until t_end or max_steps
+ * accepted steps (max_steps < 0: no cap). */
+int spb6_run(spb6_ctx* ctx, double t_end, long max_steps);
+int spb6_get_stats(spb6_ctx* ctx, spb6_stats_t* stats);
+double spb6_time(spb6_ctx* ctx);
+
+/* Residual G_i(P) = pbar_cl,i - P_i of every interior cell at the given P (residual_at, PC:699-705) after
+ * spb6_stage_prep; returns the number of cells whose closure failed (-1 on error). */
+int spb6_residual(spb6_ctx* ctx, const double* P, double* G);
+/* Finite-difference Jacobian rows (Newton_Solver's SpRow, PC:790-804): val[5*n] = (self, W, E, S, N), scale sc[n]. */
+int spb6_jacobian(spb6_ctx* ctx, const double* P, const double* G, double* val, double* sc);
+/* Linear solve with the 5-point (2D) / 3-point (1D) rows val[5*n]: BiCGStab (use_thomas = 0, PC:725) or Thomas
+ * (PC:807-812). */
+int spb6_linsolve(spb6_ctx* ctx, const double* val, const double* b, double* x, int use_thomas, int* iters);
+
+/* Copy a named device field to the host (ghosted fields: (nx+4)*(ny+4) values, grids 8x that in AoS order).
+ * Names: U Ust Qe Qs wxL wxR wyL wyR phx phy a2s ubx uby ust vst cz Kw ubsx kpx wLx pvx ubsy kpy wLy pvy Qph P Pst.
+ * Returns the number of doubles written, or -1. */
+long spb6_get_field(spb6_ctx* ctx, const char* name, double* out, long capacity);
+
+/* ---- per-launch timing of the dominant kernels (HIP events on the context's stream) */
+typedef struct spb6_kernel_times {
+    double explicit_ms, prep_ms, shock_ms, residual_ms, jacobian_ms, krylov_ms, final_ms, other_ms;
+    long explicit_n, prep_n, shock_n, residual_n, jacobian_n, krylov_n, final_n;
+    long krylov_iters;
+} spb6_kernel_times_t;
+int spb6_enable_timing(spb6_ctx* ctx, int on);
+int spb6_get_kernel_times(spb6_ctx* ctx, spb6_kernel_times_t* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,33 @@
+"""Extracts the reference's archived full-precision 1D profiles into tests/golden/ref_profiles.npz.
+
+Source files (data, not code) under the reference tree: data/Schropff2026_Riemann_problems/c{80..88}_200.dat,
+data/SPB2009_4.3.1_water_air/water_air_N100_t240us.dat (case 5), data/SPB2009_4.1_interface_advection/
+interface_advection_t2.79ms.dat (case 4, N = 200). Columns kept (17 significant digits in the source): p u v rho alpha2
+Y2 rho1 rho2 p1 p2 (write_profile, PC:1002-1013). Run: python tests/golden/make_ref_profiles.py /root/reference
+"""
+import os
+import sys
+
+import numpy as np
+
+COLS = {"p": 2, "u": 3, "v": 4, "rho": 5, "alpha2": 6, "Y2": 7, "rho1": 8, "rho2": 9, "p1": 14, "p2": 15}
+FILES = {f"c{c}_200": (c, 200, f"data/Schropff2026_Riemann_problems/c{c}_200.dat") for c in range(80, 89)}
+FILES["c5_100"] = (5, 100, "data/SPB2009_4.3.1_water_air/water_air_N100_t240us.dat")
+FILES["c4_200"] = (4, 200, "data/SPB2009_4.1_interface_advection/interface_advection_t2.79ms.dat")
+
+
+def main(ref):
+    out = {}
+    for key, (case, n, rel) in FILES.items():
+        a = np.loadtxt(os.path.join(ref, rel))
+        assert a.shape[0] == n, (key, a.shape)
+        for name, col in COLS.items():
+            out[f"{key}/{name}"] = a[:, col]
+        out[f"{key}/meta"] = np.array([case, n], dtype=np.int64)
+    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_profiles.npz")
+    np.savez_compressed(dst, **out)
+    print(dst, len(out))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
