@@ -1,0 +1,149 @@
+"""Benchmark: 2D IMEX cell-updates/s of the SPB 6+1 semi-implicit time step on one MI355X.
+
+Workload: 2D shock-bubble, 2048 x 1024 cells on 0.2 x 0.1 m (Mach-1.22 shock in gas gamma 1.4 onto a gamma-1.648
+light-gas bubble, reflective top/bottom walls), FP64, material CFL 0.5, LSDIRK2, Newton tol 1e-13, BiCGStab tol 1e-13.
+A "step" is one accepted LSDIRK2 time step (2 stages with all their Newton and BiCGStab work). --warmup steps run first
+(they cover the start-up ramp), then exactly --steps steps are timed; every stepper call returns after its HIP stream
+synchronises, so the timed region is bracketed by device synchronisation on both sides.
+
+Prints one JSON line with the metric, the roofline of the dominant kernel (algorithmic bytes per launch / average launch
+duration from HIP events on the stepper's stream) and the CPU baseline (the oracle's PC_omp-path restatement on the host
+cores, a bounded sample of the same workload resumed from the GPU's warmed state).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12   # MI355X HBM3E, B/s
+# algorithmic bytes per interior cell per launch (FP64, 2D): compulsory reads + writes of each kernel's data flow
+BYTES_2D = {"explicit": 232, "prep": 184, "residual": 256, "jacobian": 304, "krylov_iter": 296, "final": 376}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=12)
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); multi-GPU sharding is design-only", file=sys.stderr)
+        sys.exit(2)
+    from diffusionimex_amd import spb6, cases
+    cfg, U = cases.shock_bubble(a.nx, a.ny, seed=a.seed, cfl=0.5)
+    n = a.nx * a.ny
+    st = spb6.Stepper(cfg, device=0)
+    st.set_state(U)
+    rc = st.run(1.0, a.warmup)
+    if rc != 0:
+        raise SystemExit(f"warmup failed: {spb6.STATUS.get(rc, rc)}")
+    s0 = st.stats()
+    st.enable_timing(True)
+    t0 = time.perf_counter()
+    rc = st.run(1.0, a.steps)
+    t1 = time.perf_counter()
+    if rc != 0:
+        raise SystemExit(f"timed run failed: {spb6.STATUS.get(rc, rc)}")
+    s1 = st.stats()
+    kt = st.kernel_times()
+    wall = t1 - t0
+    steps = s1["steps"] - s0["steps"]
+    stages = s1["stages"] - s0["stages"]
+    newton = s1["newton_it"] - s0["newton_it"]
+    kry = s1["krylov_it"] - s0["krylov_it"]
+    lins = s1["linsolves"] - s0["linsolves"]
+    value = n * steps / wall
+
+    # dominant kernel by accumulated time, its algorithmic bytes per launch and average launch duration
+    cats = {
+        "explicit": (kt["explicit_ms"], kt["explicit_n"], BYTES_2D["explicit"] * n),
+        "prep": (kt["prep_ms"], kt["prep_n"], BYTES_2D["prep"] * n),
+        "residual": (kt["residual_ms"], kt["residual_n"], BYTES_2D["residual"] * n),
+        "jacobian": (kt["jacobian_ms"], kt["jacobian_n"], BYTES_2D["jacobian"] * n),
+        "krylov_iter": (kt["krylov_ms"], kry, BYTES_2D["krylov_iter"] * n),
+        "final": (kt["final_ms"], kt["final_n"], BYTES_2D["final"] * n),
+    }
+    kernel_ms = {k: round(v[0], 3) for k, v in cats.items()}
+    kernel_ms["shock"] = round(kt["shock_ms"], 3)
+    kernel_ms["other"] = round(kt["other_ms"], 3)
+    dom = max(cats, key=lambda k: cats[k][0])
+    ms, cnt, byts = cats[dom]
+    avg_s = (ms / 1e3) / max(cnt, 1)
+    achieved = byts / avg_s if avg_s > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None, "bytes_per_launch": byts, "avg_launch_ms": avg_s * 1e3,
+                "launches": cnt}
+    # whole-step algorithmic HBM rate
+    step_bytes = (BYTES_2D["explicit"] + BYTES_2D["prep"] + BYTES_2D["final"]) * stages + BYTES_2D["jacobian"] * newton \
+        + BYTES_2D["residual"] * (stages + newton) + BYTES_2D["krylov_iter"] * kry
+    out = {
+        "metric": "2D IMEX cell-updates/s + BiCGStab iters/step, learned+mixed vs plain FP64",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / max(steps, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"2D shock-bubble {a.nx}x{a.ny} plain FP64 (Mach 1.22, gamma 1.4 / 1.648, walls top/bottom)",
+                   "cells": n, "seed": a.seed, "cfl": 0.5, "newton_tol": cfg.newton_tol, "krylov_tol": cfg.krylov_tol},
+        "newton_per_stage": newton / max(stages, 1), "krylov_per_newton": kry / max(lins, 1), "krylov_per_step": kry / max(steps, 1),
+        "hbm_step_frac": step_bytes * n / wall / HBM_PEAK,
+        "kernel_ms": kernel_ms,
+        "roofline": roofline,
+    }
+    if not a.no_cpu and not a.profile:
+        out["cpu_baseline"] = cpu_baseline(a, cfg, st)
+    print(json.dumps(out))
+
+
+def cpu_baseline(a, cfg, st):
+    """The oracle (PC_omp paths, B = 1024 block sums, libm) on the host cores: cpu-steps LSDIRK2 steps of the same grid
+    resumed from the GPU's warmed state (same t, dt history, pressure)."""
+    from oracle import oracle as O
+    from tests.helpers import oracle_cfg_from
+    threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    L = O.lib()
+    L.ora_set_threads(threads)
+    oc = oracle_cfg_from(cfg, O)
+    O.reference_mode(oc, "PC_omp")
+    o = O.Oracle(oc)
+    o.set_state(st.state())
+    P, pref = st.pressure()
+    o.set_pressure(P, pref)
+    s = st.stats()
+    o.set_clock(st.t, s["dt_last"], 0)
+    t0 = time.perf_counter()
+    rc = o.run(1.0, a.cpu_steps)
+    t1 = time.perf_counter()
+    n = cfg.nx * cfg.ny
+    so = o.stats()
+    return {"value": n * so["steps"] / (t1 - t0), "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{so['steps']} LSDIRK2 steps of the {cfg.nx}x{cfg.ny} workload resumed from the GPU state after "
+                      f"{a.warmup} warm-up steps ({t1 - t0:.1f} s, rc {rc}, {so['newton_it']} Newton, {so['krylov_it']} BiCGStab)",
+            "cpu": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+if __name__ == "__main__":
+    main()

@@ -21,7 +21,7 @@ static thread_local std::string g_err;
 
 namespace {
 
-struct HostScratch { unsigned long long nrm[8]; int flags[NFLAGS]; KS ks; };
+struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; };
 
 enum TimerCat { T_EXPLICIT = 0, T_PREP, T_SHOCK, T_RESIDUAL, T_JACOBIAN, T_KRYLOV, T_FINAL, T_OTHER, T_NCAT };
 
@@ -59,7 +59,7 @@ struct spb6_ctx {
 	// pressure system (interior)
 	double *P, *Pst, *Pn, *G, *Gn, *dP, *rhs, *sc, *J, *cpw, *dpw;
 	double *kr, *kr0, *kp, *kv, *ks_, *kt;
-	double* part = nullptr; int part_stride = 0; unsigned* counter = nullptr; KS* ksd = nullptr;
+	double* part = nullptr; int part_stride = 0; KS* ksd = nullptr;
 	unsigned long long* nrm = nullptr; int* flags = nullptr;
 	HostScratch* hs = nullptr;
 	std::vector<void*> allocs;
@@ -100,38 +100,44 @@ struct TRegion {
 };
 
 int reset_scalars(spb6_ctx* c) {
-	HIPCHK(hipMemsetAsync(c->nrm, 0, 8 * sizeof(unsigned long long), c->st));
+	HIPCHK(hipMemsetAsync(c->nrm, 0, 8 * 64 * sizeof(unsigned long long), c->st));
 	HIPCHK(hipMemsetAsync(c->flags, 0, NFLAGS * sizeof(int), c->st));
 	return SPB6_OK;
 }
 // copy norms, flags and Krylov scalars to pinned host memory and wait
 int fetch(spb6_ctx* c, bool with_ks = false) {
-	HIPCHK(hipMemcpyAsync(c->hs->nrm, c->nrm, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipMemcpyAsync(c->hs->nrm, c->nrm, 8 * 64 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
 	HIPCHK(hipMemcpyAsync(c->hs->flags, c->flags, NFLAGS * sizeof(int), hipMemcpyDeviceToHost, c->st));
 	if (with_ks) HIPCHK(hipMemcpyAsync(&c->hs->ks, c->ksd, sizeof(KS), hipMemcpyDeviceToHost, c->st));
 	HIPCHK(hipStreamSynchronize(c->st));
 	return SPB6_OK;
 }
-inline double nrm_val(const spb6_ctx* c, int k) { double v; std::memcpy(&v, &c->hs->nrm[k], sizeof v); return v; }
+// maximum of the 64 striped slots of norm k (non-negative doubles compare like their bit patterns)
+inline double nrm_val(const spb6_ctx* c, int k) {
+	unsigned long long m = 0;
+	for (int q = 0; q < 64; q++) m = std::max(m, c->hs->nrm[64 * k + q]);
+	double v; std::memcpy(&v, &m, sizeof v); return v;
+}
 
 // BiCGStab (PC:725-758) on the Jacobian c->J with right-hand side c->rhs; solution in c->dP
 int krylov(spb6_ctx* c, int& iters, bool& ok) {
 	const Geo& g = c->g;
-	KrylovBufs B{c->dP, c->kr, c->kr0, c->kp, c->kv, c->ks_, c->kt, c->J, c->part, c->part_stride, c->counter, c->ksd};
+	KrylovBufs B{c->dP, c->kr, c->kr0, c->kp, c->kv, c->ks_, c->kt, c->J, c->part, c->part_stride, c->ksd};
 	const double tol = c->cfg.krylov_tol; const int maxit = c->cfg.krylov_iter;
 	{
 		TRegion tr(c, T_KRYLOV);
-		launch_krylov_setup(g, B, c->rhs, c->st, tol, maxit);
+		launch_krylov_setup(g, B, c->rhs, c->st);
 	}
-	int chunk = std::max(4, c->last_krylov), launched = 0;
+	// the loop top of iteration `launched` runs in its first pass, so maxit + 1 passes decide every outcome
+	int chunk = std::max(4, c->last_krylov + 1), launched = 0;
 	for (;;) {
 		{
 			TRegion tr(c, T_KRYLOV);
-			for (int k = 0; k < chunk && launched < maxit; k++, launched++) launch_krylov_iter(g, B, c->st, tol, maxit);
+			for (int k = 0; k < chunk && launched <= maxit; k++, launched++) launch_krylov_iter(g, B, c->st, launched, tol, maxit);
 		}
 		HIPCHK(hipMemcpyAsync(&c->hs->ks, c->ksd, sizeof(KS), hipMemcpyDeviceToHost, c->st));
 		HIPCHK(hipStreamSynchronize(c->st));
-		if (c->hs->ks.done || launched >= maxit) break;
+		if (c->hs->ks.done || launched > maxit) break;
 		chunk = 4;
 	}
 	iters = c->hs->ks.iters; ok = c->hs->ks.done && c->hs->ks.status == 0;
@@ -175,7 +181,7 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal) {
 			rc = krylov(c, kit, kok); if (rc) return rc;
 			c->stats.krylov_it += kit;
 		}
-		launch_stepcrit(n, c->dP, c->Pst, c->sc, &c->nrm[1], c->st);
+		launch_stepcrit(n, c->dP, c->Pst, c->sc, &c->nrm[64], c->st);
 		rc = fetch(c); if (rc) return rc;
 		if (c->hs->flags[FLAG_JAC]) return SPB6_NEWTON_FAIL;
 		if (!kok) return SPB6_KRYLOV_FAIL;
@@ -279,7 +285,7 @@ int attempt(spb6_ctx* c, double dt, int rk) {
 }
 
 int caldt(spb6_ctx* c, bool acoustic, double& inv_rate) {   // PC:477-488
-	HIPCHK(hipMemsetAsync(c->nrm, 0, 2 * sizeof(unsigned long long), c->st));
+	HIPCHK(hipMemsetAsync(c->nrm, 0, 2 * 64 * sizeof(unsigned long long), c->st));
 	launch_caldt(c->g, c->ph, c->U, acoustic ? 1 : 0, c->nrm, c->st);
 	int rc = fetch(c); if (rc) return rc;
 	const double ux = nrm_val(c, 0), uy = nrm_val(c, 1);
@@ -331,11 +337,9 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	for (auto p : vecs) if (dalloc(c, p, n)) return fail("alloc");
 	if (dalloc(c, &c->J, 5 * n)) return fail("alloc");
 	c->part_stride = (int)((n + 1023) / 1024) + 1;
-	if (dalloc(c, &c->part, 4 * (size_t)c->part_stride)) return fail("alloc");
-	if (hipMalloc((void**)&c->counter, 64) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->counter);
-	hipMemset(c->counter, 0, 64);
+	if (dalloc(c, &c->part, (size_t)KRYLOV_SLOTS * c->part_stride)) return fail("alloc");
 	if (hipMalloc((void**)&c->ksd, sizeof(KS)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->ksd);
-	if (hipMalloc((void**)&c->nrm, 8 * sizeof(unsigned long long)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->nrm);
+	if (hipMalloc((void**)&c->nrm, 8 * 64 * sizeof(unsigned long long)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->nrm);
 	if (hipMalloc((void**)&c->flags, NFLAGS * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->flags);
 	if (hipHostMalloc((void**)&c->hs, sizeof(HostScratch), hipHostMallocDefault) != hipSuccess) return fail("host alloc");
 	if (hipDeviceSynchronize() != hipSuccess) return fail("sync");

@@ -15,9 +15,9 @@ struct Geo {
 	double dx, dy;
 };
 
-// BiCGStab loop scalars kept on the device between kernels
+// BiCGStab loop scalars kept on the device between kernels (rho, alpha, omega double-buffered by iteration parity)
 struct KS {
-	double Einv, cc_p, cc_s, bn, rho, alpha, omega, beta;
+	double Einv, cc_p, cc_s, bn, rho[2], alpha[2], omega[2];
 	int done, iters, early, status;
 };
 
@@ -33,11 +33,15 @@ struct Dev {
 	int* flags;
 };
 
-struct KrylovBufs {
-	double *x, *r, *r0, *p, *v, *s, *t;
-	const double* J;
-	double* part; int stride; unsigned* counter; KS* ks;
+template <typename T> struct KrylovBufsT {
+	T *x, *r, *r0, *p, *v, *s, *t;
+	const T* J;
+	double* part; int stride; KS* ks;
 };
+using KrylovBufs = KrylovBufsT<double>;
+using KrylovBufs32 = KrylovBufsT<float>;
+constexpr int KRYLOV_SLOTS = 11;   // partial-sum slots of the Krylov passes
+int krylov_items(int n);
 
 void launch_bc_grid(const Geo& g, double* u, hipStream_t st);
 void launch_bc_array(const Geo& g, double* a, hipStream_t st);
@@ -53,7 +57,9 @@ void launch_floor(const Dev& D, const double* P, double tol, hipStream_t st);
 void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, hipStream_t st);
 void launch_caldt(const Geo& g, const Phys& P, const double* U, int acoustic, unsigned long long* nrm, hipStream_t st);
 void launch_thomas(int n, const double* J, const double* rhs, double* x, double* cp, double* dp, hipStream_t st);
-void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hipStream_t st, double tol, int maxit);
-void launch_krylov_iter(const Geo& g, const KrylovBufs& B, hipStream_t st, double tol, int maxit);
+void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hipStream_t st);
+void launch_krylov_iter(const Geo& g, const KrylovBufs& B, hipStream_t st, int it, double tol, int maxit);
+void launch_krylov_setup32(const Geo& g, const KrylovBufs32& B, const float* b, hipStream_t st);
+void launch_krylov_iter32(const Geo& g, const KrylovBufs32& B, hipStream_t st, int it, double tol, int maxit);
 
 }   // namespace spb

@@ -3,7 +3,7 @@
 // Device layout: every cell field is its own array over the ghosted grid, index c = i + j*NXT (NXT = nx+4, NYT = ny+4,
 // two ghost layers); conserved grids hold 8 such arrays back to back (field k at base + k*NC). Pressure-system vectors
 // (P, G, Krylov vectors, Jacobian diagonals) are interior-only, row r = ii + jj*nx. One thread per cell or face row;
-// 256-thread blocks (4 wavefronts of 64). Reductions use one fixed tree order (see block_sum256 / finish_sum), so every
+// 256-thread blocks (4 wavefronts of 64). Reductions use one fixed tree order (see block_sum256 / grid_total), so every
 // sum is bit-reproducible and equal to the CPU oracle's "device order".
 #include "spb6_device.h"
 #include "spb6_kernels.h"
@@ -29,29 +29,18 @@ __device__ __forceinline__ double block_sum256(double v, double* sh) {
 	__syncthreads();
 	return (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
-// non-negative double max through its bit pattern (monotone for x >= 0)
+// non-negative double max through its bit pattern (monotone for x >= 0): block maximum, then one atomic per block on one
+// of 64 striped slots (dst[0..63]); the host takes the maximum of the 64 slots
 __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* dst, double v) {
+	__shared__ double shm[4];
 	v = wave_max(v);
-	if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(v));
-}
-// store this block's partial sums, then return true in the last block to arrive (the grid's reduction finisher)
-__device__ __forceinline__ bool arrive(double* part, int slots, int stride, const double* vals, unsigned* counter) {
-	__shared__ bool last;
-	if (threadIdx.x == 0) {
-		for (int s = 0; s < slots; s++) __hip_atomic_store(&part[s * stride + blockIdx.x], vals[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		__threadfence();
-		const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-		last = (prev == gridDim.x - 1);
-	}
 	__syncthreads();
-	if (last) __threadfence();
-	return last;
-}
-// total of one slot of partials in the fixed order (thread t adds partials t, t+256, ... then block_sum256)
-__device__ __forceinline__ double finish_sum(const double* part, int nb, double* sh) {
-	double s = 0.0;
-	for (int c = threadIdx.x; c < nb; c += 256) s += __hip_atomic_load(&part[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	return block_sum256(s, sh);
+	if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		const double m = smax(smax(shm[0], shm[1]), smax(shm[2], shm[3]));
+		if (m > 0.0) atomicMax(&dst[blockIdx.x & 63], (unsigned long long)__double_as_longlong(m));
+	}
 }
 
 // ------------------------------------------------------------------------------------------------ boundaries (PHY:926-957)
@@ -606,7 +595,7 @@ __global__ void __launch_bounds__(256) k_caldt(Geo g, Phys P, const double* __re
 		}
 	}
 	atomic_max_nonneg(&nrm[0], ux);
-	atomic_max_nonneg(&nrm[1], uy);
+	atomic_max_nonneg(&nrm[64], uy);
 }
 
 // ------------------------------------------------------------------------------------------------ Thomas (PC:807-812), one thread
@@ -629,11 +618,16 @@ __global__ void __launch_bounds__(256) k_thomas(int n, const double* __restrict_
 }
 
 // ------------------------------------------------------------------------------------------------ BiCGStab (PC:725-758)
-// Right-preconditioned BiCGStab with Jacobi + constant coarse space, run as five fused passes per iteration with the
-// loop scalars kept on the device (KS). Every kernel exits at once when the solve is finished, so the host launches
-// iterations in chunks and reads KS::done once per chunk. Rows: each thread owns rows b*1024 + it*256 + t (it = 0..3),
-// the fixed order of the sums.
+// Right-preconditioned BiCGStab with Jacobi + constant coarse space, five fused passes per iteration. No atomics and no
+// grid fences: each pass writes one partial sum per block, and every block of the next pass re-reduces those partials
+// itself in the fixed order (the kernel boundary makes them visible), so all blocks derive the same loop scalars. Scalars
+// that outlive a pass (rho, alpha, omega, cc) are kept in KS, double-buffered by the parity of the iteration index the
+// host passes in, so no pass reads a value that another block of the same pass writes. Every pass returns at once when
+// the solve has finished, so the host launches iterations in chunks and reads KS::done once per chunk.
+// Rows: block b, thread t owns rows b*256*items + it*256 + t (it = 0..items-1), the fixed order of the sums.
 template <typename T> struct KV { T *x, *r, *r0, *p, *v, *s, *t; const T* J; };
+struct KP { double* part; int stride; int nb; int items; };
+enum { SL_RR = 0, SL_R0R, SL_SP, SL_R0V, SL_SS, SL_SSUM, SL_TS, SL_TT, SL_E, SL_SB, SL_BB, NSLOTS };
 
 template <typename T>
 __device__ __forceinline__ T dinv_of(const T* __restrict__ J, int r) { const T d = J[r]; return d != T(0) ? T(1) / d : T(1); }
@@ -659,12 +653,24 @@ __device__ __forceinline__ T spmv_plain(const Geo& g, const T* __restrict__ J, c
 	if (g.twoD) { s += J[3 * n + r] * z[b.S]; s += J[4 * n + r] * z[b.N]; }
 	return s;
 }
-#define ROWS_BEGIN for (int it_ = 0; it_ < 4; it_++) { const int r = blockIdx.x * 1024 + it_ * 256 + threadIdx.x; if (r >= n) continue;
+#define ROWS_BEGIN for (int it_ = 0; it_ < P.items; it_++) { const int r = blockIdx.x * 256 * P.items + it_ * 256 + threadIdx.x; if (r >= n) continue;
 #define ROWS_END }
+// this block's partial of one slot (thread 0 writes)
+__device__ __forceinline__ void put_partial(const KP& P, int slot, double v, double* sh) {
+	const double tot = block_sum256(v, sh);
+	if (threadIdx.x == 0) P.part[slot * P.stride + blockIdx.x] = tot;
+}
+// grid total of one slot written by the previous pass, in the fixed order (identical in every block)
+__device__ __forceinline__ double grid_total(const KP& P, int slot, double* sh) {
+	const double* q = P.part + slot * P.stride;
+	double s = 0.0;
+	for (int c = threadIdx.x; c < P.nb; c += 256) s += q[c];
+	return block_sum256(s, sh);
+}
 
-// setup 1: E = sum of all entries, sum(b), b.b  ->  Einv, cc_b, bn (PC:726-738)
+// setup 1: partials of E = sum of all entries, sum(b), b.b (PC:726-738)
 template <typename T>
-__global__ void __launch_bounds__(256) kb_setup1(Geo g, const T* __restrict__ J, const T* __restrict__ b, double* part, int stride, unsigned* counter, KS* ks) {
+__global__ void __launch_bounds__(256) kb_setup1(Geo g, const T* __restrict__ J, const T* __restrict__ b, KP P) {
 	__shared__ double sh[4];
 	const int n = g.n; const int nent = g.twoD ? 5 : 3;
 	double sE = 0.0, sb = 0.0, sbb = 0.0;
@@ -672,38 +678,27 @@ __global__ void __launch_bounds__(256) kb_setup1(Geo g, const T* __restrict__ J,
 		T e = T(0); for (int qq = 0; qq < nent; qq++) e += J[qq * n + r];
 		sE += (double)e; sb += (double)b[r]; sbb += (double)(b[r] * b[r]);
 	ROWS_END
-	double v[3] = {block_sum256(sE, sh), block_sum256(sb, sh), block_sum256(sbb, sh)};
-	if (!arrive(part, 3, stride, v, counter)) return;
-	const int nb = gridDim.x;
-	const double E = finish_sum(part, nb, sh), Sb = finish_sum(part + stride, nb, sh), bb = finish_sum(part + 2 * stride, nb, sh);
-	if (threadIdx.x == 0) {
-		const T Et = (T)E;
-		const T Einv = (Et != T(0) && isfin((double)Et)) ? T(1) / Et : T(0);
-		ks->Einv = (double)Einv; ks->cc_p = (double)((T)Sb * Einv);
-		ks->bn = (double)((T)sqrt((double)(T)bb) + (T)1e-300);
-		ks->rho = 1; ks->alpha = 1; ks->omega = 1; ks->done = 0; ks->iters = 0; ks->early = 0; ks->status = 0;
-		*counter = 0;
-	}
+	put_partial(P, SL_E, sE, sh); put_partial(P, SL_SB, sb, sh); put_partial(P, SL_BB, sbb, sh);
 }
-// setup 2: x = M^-1 b, p = v = 0
+// setup 2: Einv, cc_b, bn; x = M^-1 b, p = v = 0
 template <typename T>
-__global__ void __launch_bounds__(256) kb_setup2(Geo g, const T* __restrict__ J, const T* __restrict__ b, KV<T> V, const KS* ks) {
-	const int n = g.n; const T cc = (T)ks->cc_p;
+__global__ void __launch_bounds__(256) kb_setup2(Geo g, const T* __restrict__ J, const T* __restrict__ b, KV<T> V, KS* ks, KP P) {
+	__shared__ double sh[4];
+	const int n = g.n;
+	const double E = grid_total(P, SL_E, sh), Sb = grid_total(P, SL_SB, sh), bb = grid_total(P, SL_BB, sh);
+	const T Et = (T)E;
+	const T Einv = (Et != T(0) && isfin((double)Et)) ? T(1) / Et : T(0);
+	const T cc = (T)Sb * Einv;
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		ks->Einv = (double)Einv; ks->bn = (double)((T)sqrt((double)(T)bb) + (T)1e-300);
+		ks->rho[0] = 1; ks->alpha[0] = 1; ks->omega[0] = 1;
+		ks->done = 0; ks->iters = 0; ks->early = 0; ks->status = 0;
+	}
 	ROWS_BEGIN V.x[r] = dinv_of(J, r) * b[r] + cc; V.p[r] = T(0); V.v[r] = T(0); ROWS_END
 }
-// the top of the loop (PC:741-744) for the next iteration, in the finishing block: convergence, breakdown, beta
+// setup 3: r = b - A x, r0 = r; partials of r.r, r0.r
 template <typename T>
-__device__ __forceinline__ void loop_top(KS* ks, double rr, double r0r, double tol, int maxit) {
-	const T bn = (T)ks->bn;
-	if ((T)sqrt((double)(T)rr) <= (T)tol * bn) { ks->done = 1; ks->status = 0; return; }
-	if (ks->iters >= maxit) { ks->done = 1; ks->status = ((T)sqrt((double)(T)rr) <= (T)1e3 * (T)tol * bn) ? 0 : 1; return; }
-	const T rho1 = (T)r0r;
-	if (rho1 == T(0)) { ks->done = 1; ks->status = 1; return; }
-	ks->beta = (double)((rho1 / (T)ks->rho) * ((T)ks->alpha / (T)ks->omega)); ks->rho = (double)rho1;
-}
-// setup 3: r = b - A x, r0 = r, then the loop top with r.r and r0.r
-template <typename T>
-__global__ void __launch_bounds__(256) kb_setup3(Geo g, const T* __restrict__ b, KV<T> V, double* part, int stride, unsigned* counter, KS* ks, double tol, int maxit) {
+__global__ void __launch_bounds__(256) kb_setup3(Geo g, const T* __restrict__ b, KV<T> V, KP P) {
 	__shared__ double sh[4];
 	const int n = g.n;
 	double srr = 0.0, sr0r = 0.0;
@@ -712,83 +707,86 @@ __global__ void __launch_bounds__(256) kb_setup3(Geo g, const T* __restrict__ b,
 		const T rv = b[r] - ax; V.r[r] = rv; V.r0[r] = rv;
 		srr += (double)(rv * rv); sr0r += (double)(rv * rv);
 	ROWS_END
-	double v[2] = {block_sum256(srr, sh), block_sum256(sr0r, sh)};
-	if (!arrive(part, 2, stride, v, counter)) return;
-	const int nb = gridDim.x;
-	const double rr = finish_sum(part, nb, sh), r0r = finish_sum(part + stride, nb, sh);
-	if (threadIdx.x == 0) { loop_top<T>(ks, rr, r0r, tol, maxit); *counter = 0; }
+	put_partial(P, SL_RR, srr, sh); put_partial(P, SL_R0R, sr0r, sh);
 }
-// I1: p = r + beta (p - omega v); sum(p) -> cc_p
+// I1 (iteration it): loop top (PC:741-744) from r.r and r0.r, then p = r + beta (p - omega v); partials of sum(p)
 template <typename T>
-__global__ void __launch_bounds__(256) kb_p(Geo g, KV<T> V, double* part, int stride, unsigned* counter, KS* ks) {
+__global__ void __launch_bounds__(256) kb_p(Geo g, KV<T> V, KS* ks, KP P, int it, double tol, int maxit) {
 	if (ks->done) return;
 	__shared__ double sh[4];
-	const int n = g.n; const T beta = (T)ks->beta, omega = (T)ks->omega;
+	const int n = g.n, cur = it & 1, nxt = (it + 1) & 1;
+	const double rr = grid_total(P, SL_RR, sh), r0r = grid_total(P, SL_R0R, sh);
+	const T bn = (T)ks->bn;
+	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+	if ((T)sqrt((double)(T)rr) <= (T)tol * bn) { if (lead) { ks->done = 1; ks->status = 0; } return; }
+	if (it >= maxit) { if (lead) { ks->done = 1; ks->status = ((T)sqrt((double)(T)rr) <= (T)1e3 * (T)tol * bn) ? 0 : 1; } return; }
+	const T rho1 = (T)r0r;
+	if (rho1 == T(0)) { if (lead) { ks->done = 1; ks->status = 1; } return; }
+	const T beta = (rho1 / (T)ks->rho[cur]) * ((T)ks->alpha[cur] / (T)ks->omega[cur]);
+	const T omega = (T)ks->omega[cur];
+	if (lead) ks->rho[nxt] = (double)rho1;
 	double sp = 0.0;
 	ROWS_BEGIN const T pv = V.r[r] + beta * (V.p[r] - omega * V.v[r]); V.p[r] = pv; sp += (double)pv; ROWS_END
-	double v[1] = {block_sum256(sp, sh)};
-	if (!arrive(part, 1, stride, v, counter)) return;
-	const double Sp = finish_sum(part, gridDim.x, sh);
-	if (threadIdx.x == 0) { ks->cc_p = (double)((T)Sp * (T)ks->Einv); *counter = 0; }
+	put_partial(P, SL_SP, sp, sh);
 }
-// I2: v = A M^-1 p; r0.v -> alpha
+// I2: cc_p = sum(p) Einv; v = A M^-1 p; partials of r0.v
 template <typename T>
-__global__ void __launch_bounds__(256) kb_v(Geo g, KV<T> V, double* part, int stride, unsigned* counter, KS* ks) {
-	if (ks->done) return;
-	__shared__ double sh[4];
-	const int n = g.n; const T cc = (T)ks->cc_p;
-	double s = 0.0;
-	ROWS_BEGIN const T vv = spmv_prec(g, V.J, (const T*)V.p, cc, r); V.v[r] = vv; s += (double)(V.r0[r] * vv); ROWS_END
-	double v[1] = {block_sum256(s, sh)};
-	if (!arrive(part, 1, stride, v, counter)) return;
-	const double r0v = finish_sum(part, gridDim.x, sh);
-	if (threadIdx.x == 0) { ks->alpha = (double)((T)ks->rho / (T)r0v); *counter = 0; }
-}
-// I3: s = r - alpha v; s.s, sum(s) -> early exit test or cc_s
-template <typename T>
-__global__ void __launch_bounds__(256) kb_s(Geo g, KV<T> V, double* part, int stride, unsigned* counter, KS* ks, double tol) {
-	if (ks->done) return;
-	__shared__ double sh[4];
-	const int n = g.n; const T alpha = (T)ks->alpha;
-	double sss = 0.0, ss = 0.0;
-	ROWS_BEGIN const T sv = V.r[r] - alpha * V.v[r]; V.s[r] = sv; sss += (double)(sv * sv); ss += (double)sv; ROWS_END
-	double v[2] = {block_sum256(sss, sh), block_sum256(ss, sh)};
-	if (!arrive(part, 2, stride, v, counter)) return;
-	const double s2 = finish_sum(part, gridDim.x, sh), Ss = finish_sum(part + stride, gridDim.x, sh);
-	if (threadIdx.x == 0) {
-		if ((T)sqrt((double)(T)s2) <= (T)tol * (T)ks->bn) ks->early = 1;
-		else ks->cc_s = (double)((T)Ss * (T)ks->Einv);
-		*counter = 0;
-	}
-}
-// I4: early exit x += alpha M^-1 p; otherwise t = A M^-1 s; t.s, t.t -> omega
-template <typename T>
-__global__ void __launch_bounds__(256) kb_t(Geo g, KV<T> V, double* part, int stride, unsigned* counter, KS* ks) {
+__global__ void __launch_bounds__(256) kb_v(Geo g, KV<T> V, KS* ks, KP P, int it) {
 	if (ks->done) return;
 	__shared__ double sh[4];
 	const int n = g.n;
-	if (ks->early) {
-		const T alpha = (T)ks->alpha, ccp = (T)ks->cc_p;
-		ROWS_BEGIN V.x[r] += alpha * (dinv_of(V.J, r) * V.p[r] + ccp); ROWS_END
-		double v[1] = {0.0};
-		if (!arrive(part, 1, stride, v, counter)) return;
-		if (threadIdx.x == 0) { ks->iters += 1; ks->done = 1; ks->status = 0; *counter = 0; }
-		return;
-	}
-	const T cc = (T)ks->cc_s;
-	double sts = 0.0, stt = 0.0;
-	ROWS_BEGIN const T tv = spmv_prec(g, V.J, (const T*)V.s, cc, r); V.t[r] = tv; sts += (double)(tv * V.s[r]); stt += (double)(tv * tv); ROWS_END
-	double v[2] = {block_sum256(sts, sh), block_sum256(stt, sh)};
-	if (!arrive(part, 2, stride, v, counter)) return;
-	const double ts = finish_sum(part, gridDim.x, sh), tt = finish_sum(part + stride, gridDim.x, sh);
-	if (threadIdx.x == 0) { ks->omega = (double)((T)ts / (T)tt); *counter = 0; }
+	const T cc = (T)grid_total(P, SL_SP, sh) * (T)ks->Einv;
+	if (blockIdx.x == 0 && threadIdx.x == 0) ks->cc_p = (double)cc;
+	double s = 0.0;
+	ROWS_BEGIN const T vv = spmv_prec(g, V.J, (const T*)V.p, cc, r); V.v[r] = vv; s += (double)(V.r0[r] * vv); ROWS_END
+	put_partial(P, SL_R0V, s, sh);
 }
-// I5: x += alpha M^-1 p + omega M^-1 s; r = s - omega t; r.r, r0.r -> loop top
+// I3: alpha = rho / r0.v; s = r - alpha v; partials of s.s and sum(s)
 template <typename T>
-__global__ void __launch_bounds__(256) kb_xr(Geo g, KV<T> V, double* part, int stride, unsigned* counter, KS* ks, double tol, int maxit) {
+__global__ void __launch_bounds__(256) kb_s(Geo g, KV<T> V, KS* ks, KP P, int it) {
 	if (ks->done) return;
 	__shared__ double sh[4];
-	const int n = g.n; const T alpha = (T)ks->alpha, omega = (T)ks->omega, ccp = (T)ks->cc_p, ccs = (T)ks->cc_s;
+	const int n = g.n, nxt = (it + 1) & 1;
+	const T alpha = (T)ks->rho[nxt] / (T)grid_total(P, SL_R0V, sh);
+	if (blockIdx.x == 0 && threadIdx.x == 0) ks->alpha[nxt] = (double)alpha;
+	double sss = 0.0, ssum = 0.0;
+	ROWS_BEGIN const T sv = V.r[r] - alpha * V.v[r]; V.s[r] = sv; sss += (double)(sv * sv); ssum += (double)sv; ROWS_END
+	put_partial(P, SL_SS, sss, sh); put_partial(P, SL_SSUM, ssum, sh);
+}
+// I4: early exit if |s| <= tol |b| (x += alpha M^-1 p, PC:751); otherwise cc_s = sum(s) Einv, t = A M^-1 s; partials of
+// t.s and t.t
+template <typename T>
+__global__ void __launch_bounds__(256) kb_t(Geo g, KV<T> V, KS* ks, KP P, int it, double tol) {
+	if (ks->done) return;
+	__shared__ double sh[4];
+	const int n = g.n, nxt = (it + 1) & 1;
+	const double s2 = grid_total(P, SL_SS, sh), Ss = grid_total(P, SL_SSUM, sh);
+	const T alpha = (T)ks->alpha[nxt];
+	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+	if ((T)sqrt((double)(T)s2) <= (T)tol * (T)ks->bn) {
+		const T ccp = (T)ks->cc_p;
+		ROWS_BEGIN V.x[r] += alpha * (dinv_of(V.J, r) * V.p[r] + ccp); ROWS_END
+		if (lead) { ks->early = 1; ks->iters = it + 1; ks->status = 0; }
+		return;
+	}
+	const T cc = (T)Ss * (T)ks->Einv;
+	if (lead) ks->cc_s = (double)cc;
+	double sts = 0.0, stt = 0.0;
+	ROWS_BEGIN const T tv = spmv_prec(g, V.J, (const T*)V.s, cc, r); V.t[r] = tv; sts += (double)(tv * V.s[r]); stt += (double)(tv * tv); ROWS_END
+	put_partial(P, SL_TS, sts, sh); put_partial(P, SL_TT, stt, sh);
+}
+// I5: omega = t.s / t.t; x += alpha M^-1 p + omega M^-1 s; r = s - omega t; partials of r.r and r0.r
+template <typename T>
+__global__ void __launch_bounds__(256) kb_xr(Geo g, KV<T> V, KS* ks, KP P, int it) {
+	if (ks->done) return;
+	const int nxt = (it + 1) & 1;
+	if (ks->early) { if (blockIdx.x == 0 && threadIdx.x == 0) ks->done = 1; return; }
+	__shared__ double sh[4];
+	const int n = g.n;
+	const double ts = grid_total(P, SL_TS, sh), tt = grid_total(P, SL_TT, sh);
+	const T omega = (T)ts / (T)tt;
+	const T alpha = (T)ks->alpha[nxt], ccp = (T)ks->cc_p, ccs = (T)ks->cc_s;
+	if (blockIdx.x == 0 && threadIdx.x == 0) { ks->omega[nxt] = (double)omega; ks->iters = it + 1; }
 	double srr = 0.0, sr0r = 0.0;
 	ROWS_BEGIN
 		const T di = dinv_of(V.J, r);
@@ -797,19 +795,40 @@ __global__ void __launch_bounds__(256) kb_xr(Geo g, KV<T> V, double* part, int s
 		const T rv = V.s[r] - omega * V.t[r]; V.r[r] = rv;
 		srr += (double)(rv * rv); sr0r += (double)(V.r0[r] * rv);
 	ROWS_END
-	double v[2] = {block_sum256(srr, sh), block_sum256(sr0r, sh)};
-	if (!arrive(part, 2, stride, v, counter)) return;
-	const double rr = finish_sum(part, gridDim.x, sh), r0r = finish_sum(part + stride, gridDim.x, sh);
-	if (threadIdx.x == 0) { ks->iters += 1; loop_top<T>(ks, rr, r0r, tol, maxit); *counter = 0; }
+	put_partial(P, SL_RR, srr, sh); put_partial(P, SL_R0R, sr0r, sh);
 }
-template __global__ void kb_setup1<double>(Geo, const double*, const double*, double*, int, unsigned*, KS*);
-template __global__ void kb_setup2<double>(Geo, const double*, const double*, KV<double>, const KS*);
-template __global__ void kb_setup3<double>(Geo, const double*, KV<double>, double*, int, unsigned*, KS*, double, int);
-template __global__ void kb_p<double>(Geo, KV<double>, double*, int, unsigned*, KS*);
-template __global__ void kb_v<double>(Geo, KV<double>, double*, int, unsigned*, KS*);
-template __global__ void kb_s<double>(Geo, KV<double>, double*, int, unsigned*, KS*, double);
-template __global__ void kb_t<double>(Geo, KV<double>, double*, int, unsigned*, KS*);
-template __global__ void kb_xr<double>(Geo, KV<double>, double*, int, unsigned*, KS*, double, int);
+
+// rows per thread of the Krylov passes: about a thousand blocks whatever n (4 .. 32 rows per thread)
+int krylov_items(int n) {
+	int it = 4;
+	while (it < 32 && (long)n > (long)256 * it * 2048) it *= 2;
+	return it;
+}
+
+template <typename T>
+static void krylov_setup_t(const Geo& g, const KrylovBufsT<T>& B, const T* b, hipStream_t st) {
+	const int items = krylov_items(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
+	KP P{B.part, B.stride, nb, items};
+	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J};
+	hipLaunchKernelGGL(kb_setup1<T>, dim3(nb), dim3(256), 0, st, g, B.J, b, P);
+	hipLaunchKernelGGL(kb_setup2<T>, dim3(nb), dim3(256), 0, st, g, B.J, b, V, B.ks, P);
+	hipLaunchKernelGGL(kb_setup3<T>, dim3(nb), dim3(256), 0, st, g, b, V, P);
+}
+template <typename T>
+static void krylov_iter_t(const Geo& g, const KrylovBufsT<T>& B, hipStream_t st, int it, double tol, int maxit) {
+	const int items = krylov_items(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
+	KP P{B.part, B.stride, nb, items};
+	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J};
+	hipLaunchKernelGGL(kb_p<T>, dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol, maxit);
+	hipLaunchKernelGGL(kb_v<T>, dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+	hipLaunchKernelGGL(kb_s<T>, dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+	hipLaunchKernelGGL(kb_t<T>, dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol);
+	hipLaunchKernelGGL(kb_xr<T>, dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+}
+void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hipStream_t st) { krylov_setup_t<double>(g, B, b, st); }
+void launch_krylov_iter(const Geo& g, const KrylovBufs& B, hipStream_t st, int it, double tol, int maxit) { krylov_iter_t<double>(g, B, st, it, tol, maxit); }
+void launch_krylov_setup32(const Geo& g, const KrylovBufs32& B, const float* b, hipStream_t st) { krylov_setup_t<float>(g, B, b, st); }
+void launch_krylov_iter32(const Geo& g, const KrylovBufs32& B, hipStream_t st, int it, double tol, int maxit) { krylov_iter_t<float>(g, B, st, it, tol, maxit); }
 
 // ------------------------------------------------------------------------------------------------ host launchers
 static inline int nblk(long n, int b = 256) { return (int)((n + b - 1) / b); }
@@ -867,21 +886,4 @@ void launch_caldt(const Geo& g, const Phys& P, const double* U, int acoustic, un
 void launch_thomas(int n, const double* J, const double* rhs, double* x, double* cp, double* dp, hipStream_t st) {
 	hipLaunchKernelGGL(k_thomas, dim3(1), dim3(64), 0, st, n, J, rhs, x, cp, dp);
 }
-void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hipStream_t st, double tol, int maxit) {
-	const int nb = nblk(g.n, 1024);
-	KV<double> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J};
-	hipLaunchKernelGGL(kb_setup1<double>, dim3(nb), dim3(256), 0, st, g, B.J, b, B.part, B.stride, B.counter, B.ks);
-	hipLaunchKernelGGL(kb_setup2<double>, dim3(nb), dim3(256), 0, st, g, B.J, b, V, (const KS*)B.ks);
-	hipLaunchKernelGGL(kb_setup3<double>, dim3(nb), dim3(256), 0, st, g, b, V, B.part, B.stride, B.counter, B.ks, tol, maxit);
-}
-void launch_krylov_iter(const Geo& g, const KrylovBufs& B, hipStream_t st, double tol, int maxit) {
-	const int nb = nblk(g.n, 1024);
-	KV<double> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J};
-	hipLaunchKernelGGL(kb_p<double>, dim3(nb), dim3(256), 0, st, g, V, B.part, B.stride, B.counter, B.ks);
-	hipLaunchKernelGGL(kb_v<double>, dim3(nb), dim3(256), 0, st, g, V, B.part, B.stride, B.counter, B.ks);
-	hipLaunchKernelGGL(kb_s<double>, dim3(nb), dim3(256), 0, st, g, V, B.part, B.stride, B.counter, B.ks, tol);
-	hipLaunchKernelGGL(kb_t<double>, dim3(nb), dim3(256), 0, st, g, V, B.part, B.stride, B.counter, B.ks);
-	hipLaunchKernelGGL(kb_xr<double>, dim3(nb), dim3(256), 0, st, g, V, B.part, B.stride, B.counter, B.ks, tol, maxit);
-}
-
 }   // namespace spb

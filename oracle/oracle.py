@@ -71,6 +71,8 @@ def lib():
         L.ora_get_pressure.argtypes = [vp, dp, dp]
         L.ora_run.argtypes = [vp, C.c_double, C.c_long]
         L.ora_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.ora_set_pressure.argtypes = [vp, dp, C.c_double]
+        L.ora_set_clock.argtypes = [vp, C.c_double, C.c_double, C.c_int]
         L.ora_time.restype = C.c_double
         L.ora_time.argtypes = [vp]
         L.ora_stage_explicit.argtypes = [vp, C.c_double]
@@ -193,6 +195,13 @@ class Oracle:
         pref = np.zeros(1)
         self.L.ora_get_pressure(self.h, _dp(P), _dp(pref))
         return P.reshape(self.ny, self.nx), float(pref[0])
+
+    def set_pressure(self, P, pref):
+        P = np.ascontiguousarray(P, dtype=np.float64).ravel()
+        self.L.ora_set_pressure(self.h, _dp(P), float(pref))
+
+    def set_clock(self, t, dtprev, startup):
+        self.L.ora_set_clock(self.h, float(t), float(dtprev), int(bool(startup)))
 
     def run(self, t_end=1e300, max_steps=-1):
         return self.L.ora_run(self.h, float(t_end), int(max_steps))

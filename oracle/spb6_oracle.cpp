@@ -438,10 +438,11 @@ static double closure_roundoff(const Model& M, const ClIn& in, double p) {      
 }
 
 // ------------------------------------------------------------------------------------------------ fixed-order sums
-// Device tree order: blocks of 1024 consecutive terms; in a block, thread t (0..255) adds terms t, t+256, t+512, t+768 in
-// turn (starting from 0.0); each 64-lane wavefront combines with xor-butterflies (offsets 32..1); the block total is
-// (w0 + w1) + (w2 + w3). The block totals are combined the same way by one 256-thread block (thread t adds totals
-// t, t+256, ... in turn). The HIP reduction kernels use exactly this order.
+// Device tree order: the terms are cut into blocks of 256*items consecutive terms, items = 4, 8, 16 or 32 growing with n
+// (about a thousand blocks); in a block, thread t (0..255) adds terms t, t+256, t+512, ... in turn (starting from 0.0);
+// each 64-lane wavefront combines with xor-butterflies (offsets 32..1); the block total is (w0 + w1) + (w2 + w3). The
+// block totals are combined the same way by one 256-thread block (thread t adds totals t, t+256, ... in turn). The HIP
+// reduction kernels use exactly this order.
 static double tree256(const double* v) {
 	double w[4];
 	for (int q = 0; q < 4; q++) {
@@ -451,12 +452,13 @@ static double tree256(const double* v) {
 	}
 	return (w[0] + w[1]) + (w[2] + w[3]);
 }
+static int dev_items(int n) { int it = 4; while (it < 32 && (long)n > (long)256 * it * 2048) it *= 2; return it; }
 template <class F> static double dev_sum(int n, F term) {
-	const int nb = (n + 1023) / 1024; std::vector<double> part(nb > 0 ? nb : 1, 0.0);
+	const int items = dev_items(n), B = 256 * items, nb = (n + B - 1) / B; std::vector<double> part(nb > 0 ? nb : 1, 0.0);
 	#pragma omp parallel for schedule(static)
 	for (int b = 0; b < nb; b++) {
 		double v[256];
-		for (int t = 0; t < 256; t++) { double s = 0.0; for (int it = 0; it < 4; it++) { const int i = b * 1024 + it * 256 + t; if (i < n) s += term(i); } v[t] = s; }
+		for (int t = 0; t < 256; t++) { double s = 0.0; for (int it = 0; it < items; it++) { const int i = b * B + it * 256 + t; if (i < n) s += term(i); } v[t] = s; }
 		part[b] = tree256(v);
 	}
 	double v[256];
@@ -1219,6 +1221,9 @@ void ora_get_state(void* h, double* Uout) {
 }
 void ora_get_pressure(void* h, double* P, double* pref) { Ctx& X = *(Ctx*)h; std::memcpy(P, X.P.data(), X.P.size() * sizeof(double)); *pref = X.P_REF; }
 int ora_run(void* h, double t_end, long max_steps) { return run(*(Ctx*)h, t_end, max_steps); }
+// resume from another solver's state: pressure unknown and P_REF, and the driver clock (t, previous dt, start-up flag)
+void ora_set_pressure(void* h, const double* P, double pref) { Ctx& X = *(Ctx*)h; std::memcpy(X.P.data(), P, X.P.size() * sizeof(double)); X.P_REF = pref; }
+void ora_set_clock(void* h, double t, double dtprev, int startup) { Ctx& X = *(Ctx*)h; X.t = t; X.dtprev = dtprev; X.startup = startup != 0; }
 void ora_get_stats(void* h, Stats* s) { *s = ((Ctx*)h)->st; }
 double ora_time(void* h) { return ((Ctx*)h)->t; }
 
