@@ -36,46 +36,56 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
+    ap.add_argument("--forms", default="plain,mixed", help="comma list of plain, mixed (FP32 Krylov), learned (MLP guess + FP32 Krylov)")
     return ap.parse_args()
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); multi-GPU sharding is design-only", file=sys.stderr)
-        sys.exit(2)
+def run_form(a, form, keep_state=False):
+    """warm up and time one form of the stepper on the workload; returns (result dict, stepper, cfg)"""
     from diffusionimex_amd import spb6, cases
-    cfg, U = cases.shock_bubble(a.nx, a.ny, seed=a.seed, cfl=0.5)
+    kw = {}
+    if form in ("mixed", "learned"):
+        kw["krylov_fp32"] = 1
+    if form == "learned":
+        kw["learned_guess"] = 1
+    cfg, U = cases.shock_bubble(a.nx, a.ny, seed=a.seed, cfl=0.5, **kw)
     n = a.nx * a.ny
     st = spb6.Stepper(cfg, device=0)
+    if form == "learned":
+        from diffusionimex_amd import mlp
+        mlp.load_into(st)
     st.set_state(U)
     rc = st.run(1.0, a.warmup)
     if rc != 0:
-        raise SystemExit(f"warmup failed: {spb6.STATUS.get(rc, rc)}")
+        raise SystemExit(f"{form}: warmup failed: {spb6.STATUS.get(rc, rc)}")
     s0 = st.stats()
     st.enable_timing(True)
     t0 = time.perf_counter()
     rc = st.run(1.0, a.steps)
     t1 = time.perf_counter()
     if rc != 0:
-        raise SystemExit(f"timed run failed: {spb6.STATUS.get(rc, rc)}")
+        raise SystemExit(f"{form}: timed run failed: {spb6.STATUS.get(rc, rc)}")
     s1 = st.stats()
     kt = st.kernel_times()
     wall = t1 - t0
-    steps = s1["steps"] - s0["steps"]
-    stages = s1["stages"] - s0["stages"]
-    newton = s1["newton_it"] - s0["newton_it"]
-    kry = s1["krylov_it"] - s0["krylov_it"]
-    lins = s1["linsolves"] - s0["linsolves"]
-    value = n * steps / wall
+    d = {k: s1[k] - s0[k] for k in ("steps", "stages", "newton_it", "krylov_it", "linsolves", "guess_used", "guess_fallback")}
+    res = {"value": n * d["steps"] / wall, "wall": wall, "ms_per_step": 1e3 * wall / max(d["steps"], 1),
+           "newton_per_stage": d["newton_it"] / max(d["stages"], 1), "krylov_per_newton": d["krylov_it"] / max(d["linsolves"], 1),
+           "krylov_per_step": d["krylov_it"] / max(d["steps"], 1), "d": d, "kt": kt}
+    if form == "learned":
+        res["guess_used"], res["guess_fallback"] = d["guess_used"], d["guess_fallback"]
+    return res, st, cfg
 
-    # dominant kernel by accumulated time, its algorithmic bytes per launch and average launch duration
+
+def roofline_of(res, n, fp32):
+    kt, d = res["kt"], res["d"]
+    kb = BYTES_2D["krylov_iter"] // (2 if fp32 else 1)
     cats = {
         "explicit": (kt["explicit_ms"], kt["explicit_n"], BYTES_2D["explicit"] * n),
         "prep": (kt["prep_ms"], kt["prep_n"], BYTES_2D["prep"] * n),
         "residual": (kt["residual_ms"], kt["residual_n"], BYTES_2D["residual"] * n),
         "jacobian": (kt["jacobian_ms"], kt["jacobian_n"], BYTES_2D["jacobian"] * n),
-        "krylov_iter": (kt["krylov_ms"], kry, BYTES_2D["krylov_iter"] * n),
+        "krylov_iter": (kt["krylov_ms"], d["krylov_it"], kb * n),
         "final": (kt["final_ms"], kt["final_n"], BYTES_2D["final"] * n),
     }
     kernel_ms = {k: round(v[0], 3) for k, v in cats.items()}
@@ -85,26 +95,62 @@ def main():
     ms, cnt, byts = cats[dom]
     avg_s = (ms / 1e3) / max(cnt, 1)
     achieved = byts / avg_s if avg_s > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None, "bytes_per_launch": byts, "avg_launch_ms": avg_s * 1e3,
-                "launches": cnt}
-    # whole-step algorithmic HBM rate
-    step_bytes = (BYTES_2D["explicit"] + BYTES_2D["prep"] + BYTES_2D["final"]) * stages + BYTES_2D["jacobian"] * newton \
-        + BYTES_2D["residual"] * (stages + newton) + BYTES_2D["krylov_iter"] * kry
+    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": None, "bytes_per_launch": byts, "avg_launch_ms": avg_s * 1e3,
+            "launches": cnt}
+    step_bytes = (BYTES_2D["explicit"] + BYTES_2D["prep"] + BYTES_2D["final"]) * d["stages"] + BYTES_2D["jacobian"] * d["newton_it"] \
+        + BYTES_2D["residual"] * (d["stages"] + d["newton_it"]) + kb * d["krylov_it"]
+    return roof, kernel_ms, step_bytes * n / res["wall"] / HBM_PEAK
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); multi-GPU sharding is design-only", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    n = a.nx * a.ny
+    forms = [f for f in a.forms.split(",") if f]
+    results, states = {}, {}
+    st_plain = cfg = None
+    for form in forms:
+        res, st, c = run_form(a, form)
+        states[form] = st.state()
+        results[form] = res
+        if form == "plain":
+            st_plain, cfg = st, c
+        else:
+            st.close()
+        if cfg is None:
+            cfg = c
+    head = "learned" if "learned" in results else ("mixed" if "mixed" in results and "plain" not in results else "plain")
+    r = results[head]
+    roof, kernel_ms, hbm_frac = roofline_of(r, n, head != "plain")
+    forms_out = {}
+    for form, res in results.items():
+        fo = {k: res[k] for k in ("value", "ms_per_step", "newton_per_stage", "krylov_per_newton", "krylov_per_step")}
+        if "plain" in states and form != "plain":
+            A, B = states[form].reshape(-1, 8), states["plain"].reshape(-1, 8)
+            fo["field_l1_vs_fp64"] = [float(np.abs(A[:, k] - B[:, k]).sum() / max(np.abs(B[:, k]).sum(), 1e-300)) for k in range(8)]
+            fo["speedup_vs_plain"] = res["value"] / results["plain"]["value"]
+        if form == "learned":
+            fo["guess_used"], fo["guess_fallback"] = res["guess_used"], res["guess_fallback"]
+        forms_out[form] = fo
     out = {
         "metric": "2D IMEX cell-updates/s + BiCGStab iters/step, learned+mixed vs plain FP64",
-        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
-        "ms_per_step": 1e3 * wall / max(steps, 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"2D shock-bubble {a.nx}x{a.ny} plain FP64 (Mach 1.22, gamma 1.4 / 1.648, walls top/bottom)",
-                   "cells": n, "seed": a.seed, "cfl": 0.5, "newton_tol": cfg.newton_tol, "krylov_tol": cfg.krylov_tol},
-        "newton_per_stage": newton / max(stages, 1), "krylov_per_newton": kry / max(lins, 1), "krylov_per_step": kry / max(steps, 1),
-        "hbm_step_frac": step_bytes * n / wall / HBM_PEAK,
-        "kernel_ms": kernel_ms,
-        "roofline": roofline,
+        "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": r["d"]["steps"], "warmup": a.warmup,
+        "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if head == "plain" else "f64 (FP32 Krylov)", "data": "synthetic",
+        "config": {"workload": f"2D shock-bubble {a.nx}x{a.ny} ({head} form; Mach 1.22, gamma 1.4 / 1.648, walls top/bottom)",
+                   "cells": n, "seed": a.seed, "cfl": 0.5, "newton_tol": 1e-13, "krylov_tol": 1e-13, "form": head},
+        "newton_per_stage": r["newton_per_stage"], "krylov_per_newton": r["krylov_per_newton"], "krylov_per_step": r["krylov_per_step"],
+        "hbm_step_frac": hbm_frac, "kernel_ms": kernel_ms, "forms": forms_out,
+        "roofline": roof,
     }
     if not a.no_cpu and not a.profile:
-        out["cpu_baseline"] = cpu_baseline(a, cfg, st)
+        if st_plain is None:
+            _, st_plain, cfg = run_form(a, "plain")
+        out["cpu_baseline"] = cpu_baseline(a, cfg, st_plain)
     print(json.dumps(out))
 
 

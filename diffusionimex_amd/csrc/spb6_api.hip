@@ -59,6 +59,8 @@ struct spb6_ctx {
 	// pressure system (interior)
 	double *P, *Pst, *Pn, *G, *Gn, *dP, *rhs, *sc, *J, *cpw, *dpw;
 	double *kr, *kr0, *kp, *kv, *ks_, *kt;
+	// FP32 Krylov form: matrix, right-hand side and vectors in single precision
+	float *J32 = nullptr, *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *r032 = nullptr, *p32 = nullptr, *v32 = nullptr, *s32 = nullptr, *t32 = nullptr;
 	double* part = nullptr; int part_stride = 0; KS* ksd = nullptr;
 	unsigned long long* nrm = nullptr; int* flags = nullptr;
 	HostScratch* hs = nullptr;
@@ -119,21 +121,25 @@ inline double nrm_val(const spb6_ctx* c, int k) {
 	double v; std::memcpy(&v, &m, sizeof v); return v;
 }
 
-// BiCGStab (PC:725-758) on the Jacobian c->J with right-hand side c->rhs; solution in c->dP
-int krylov(spb6_ctx* c, int& iters, bool& ok) {
-	const Geo& g = c->g;
-	KrylovBufs B{c->dP, c->kr, c->kr0, c->kp, c->kv, c->ks_, c->kt, c->J, c->part, c->part_stride, c->ksd};
-	const double tol = c->cfg.krylov_tol; const int maxit = c->cfg.krylov_iter;
+// BiCGStab (PC:725-758) on the Jacobian c->J with right-hand side c->rhs; solution in c->dP. With krylov_fp32 the
+// iteration runs on FP32 copies of the matrix and vectors (inner tolerance at least 1e-6 |b|, the FP32 floor) and the
+// step is widened back to FP64; the Newton residual stays FP64 and corrects the inexact step (defect correction).
+template <typename T>
+int krylov_run(spb6_ctx* c, const KrylovBufsT<T>& B, const T* b, double tol, int& iters, bool& ok) {
+	const Geo& g = c->g; const int maxit = c->cfg.krylov_iter;
 	{
 		TRegion tr(c, T_KRYLOV);
-		launch_krylov_setup(g, B, c->rhs, c->st);
+		if constexpr (sizeof(T) == 8) launch_krylov_setup(g, B, b, c->st); else launch_krylov_setup32(g, B, b, c->st);
 	}
 	// the loop top of iteration `launched` runs in its first pass, so maxit + 1 passes decide every outcome
 	int chunk = std::max(4, c->last_krylov + 1), launched = 0;
 	for (;;) {
 		{
 			TRegion tr(c, T_KRYLOV);
-			for (int k = 0; k < chunk && launched <= maxit; k++, launched++) launch_krylov_iter(g, B, c->st, launched, tol, maxit);
+			for (int k = 0; k < chunk && launched <= maxit; k++, launched++) {
+				if constexpr (sizeof(T) == 8) launch_krylov_iter(g, B, c->st, launched, tol, maxit);
+				else launch_krylov_iter32(g, B, c->st, launched, tol, maxit);
+			}
 		}
 		HIPCHK(hipMemcpyAsync(&c->hs->ks, c->ksd, sizeof(KS), hipMemcpyDeviceToHost, c->st));
 		HIPCHK(hipStreamSynchronize(c->st));
@@ -142,6 +148,20 @@ int krylov(spb6_ctx* c, int& iters, bool& ok) {
 	}
 	iters = c->hs->ks.iters; ok = c->hs->ks.done && c->hs->ks.status == 0;
 	c->last_krylov = iters;
+	return SPB6_OK;
+}
+int krylov(spb6_ctx* c, int& iters, bool& ok) {
+	const long n = c->g.n;
+	if (!c->cfg.krylov_fp32) {
+		KrylovBufs B{c->dP, c->kr, c->kr0, c->kp, c->kv, c->ks_, c->kt, c->J, c->part, c->part_stride, c->ksd};
+		return krylov_run<double>(c, B, c->rhs, c->cfg.krylov_tol, iters, ok);
+	}
+	launch_to_f32(5 * n, c->J, c->J32, c->st);
+	launch_to_f32(n, c->rhs, c->b32, c->st);
+	KrylovBufs32 B{c->x32, c->r32, c->r032, c->p32, c->v32, c->s32, c->t32, c->J32, c->part, c->part_stride, c->ksd};
+	const int rc = krylov_run<float>(c, B, c->b32, std::max(c->cfg.krylov_tol, 1e-6), iters, ok);
+	if (rc) return rc;
+	launch_to_f64(n, c->x32, c->dP, c->st);
 	return SPB6_OK;
 }
 
@@ -336,6 +356,11 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	double** vecs[] = {&c->P, &c->Pst, &c->Pn, &c->G, &c->Gn, &c->dP, &c->rhs, &c->sc, &c->cpw, &c->dpw, &c->kr, &c->kr0, &c->kp, &c->kv, &c->ks_, &c->kt};
 	for (auto p : vecs) if (dalloc(c, p, n)) return fail("alloc");
 	if (dalloc(c, &c->J, 5 * n)) return fail("alloc");
+	if (cfg->krylov_fp32) {
+		float** fv[] = {&c->b32, &c->x32, &c->r32, &c->r032, &c->p32, &c->v32, &c->s32, &c->t32};
+		for (auto p : fv) { if (hipMalloc((void**)p, n * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(*p); }
+		if (hipMalloc((void**)&c->J32, 5 * n * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->J32);
+	}
 	c->part_stride = (int)((n + 1023) / 1024) + 1;
 	if (dalloc(c, &c->part, (size_t)KRYLOV_SLOTS * c->part_stride)) return fail("alloc");
 	if (hipMalloc((void**)&c->ksd, sizeof(KS)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->ksd);

@@ -798,6 +798,18 @@ __global__ void __launch_bounds__(256) kb_xr(Geo g, KV<T> V, KS* ks, KP P, int i
 	put_partial(P, SL_RR, srr, sh); put_partial(P, SL_R0R, sr0r, sh);
 }
 
+// precision conversions around the FP32 Krylov solve (the Newton system stays FP64)
+__global__ void __launch_bounds__(256) k_to_f32(long n, const double* __restrict__ a, float* __restrict__ b) {
+	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) b[i] = (float)a[i];
+}
+__global__ void __launch_bounds__(256) k_to_f64(long n, const float* __restrict__ a, double* __restrict__ b) {
+	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) b[i] = (double)a[i];
+}
+void launch_to_f32(long n, const double* a, float* b, hipStream_t st) { hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b); }
+void launch_to_f64(long n, const float* a, double* b, hipStream_t st) { hipLaunchKernelGGL(k_to_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b); }
+
 // rows per thread of the Krylov passes: about a thousand blocks whatever n (4 .. 32 rows per thread)
 int krylov_items(int n) {
 	int it = 4;
