@@ -7,11 +7,13 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 #include <algorithm>
 #include "spb6_kernels.h"
+#include "spb6_mlp.h"
 #include "../../include/spb6_imex.h"
 
 using namespace spb;
@@ -72,6 +74,23 @@ struct spb6_ctx {
 	int last_krylov = 0;
 	double tau_last = 0.0;
 	Timer timer;
+	std::vector<double> hpart;          // host copy of block partials (FP64 defect norms)
+	// learned guess and training dumps
+	spb6_mlp* model = nullptr;
+	float* F = nullptr;                 // 7 per-cell feature fields over the ghosted grid
+	double* Pplain = nullptr;           // the plain guess of the current stage
+	float *dX = nullptr, *dY = nullptr; unsigned long long* dcount = nullptr; long dcap = 0; int dstride = 0; unsigned dseq = 0;
+};
+
+struct spb6_mlp {
+	int device = 0; hipStream_t st = nullptr;
+	float *P = nullptr, *G = nullptr, *m = nullptr, *v = nullptr, *mean = nullptr, *istd = nullptr, *loss = nullptr;
+	__bf16 *W1t = nullptr, *W2t = nullptr;
+	float mu_t = 0.f, sd_t = 1.f, b3 = 0.f;
+	float *X = nullptr, *Y = nullptr; long ndata = 0;
+	float *Xb = nullptr, *Yb = nullptr, *ws = nullptr, *yb = nullptr; int* idx = nullptr; int bcap = 0;
+	int step = 0;
+	std::vector<void*> allocs;
 };
 
 namespace {
@@ -150,18 +169,49 @@ int krylov_run(spb6_ctx* c, const KrylovBufsT<T>& B, const T* b, double tol, int
 	c->last_krylov = iters;
 	return SPB6_OK;
 }
-int krylov(spb6_ctx* c, int& iters, bool& ok) {
+// FP32 Krylov form: inexact Newton step. The step solves J dP = b to a relative FP64 residual
+// eta = min(1e-2, max(res / 2, 0.1 tol / res)) -- res / 2 keeps Newton's quadratic convergence, 0.1 tol / res is all the
+// last step needs to bring the Newton residual under tol -- by sweeps of FP32 BiCGStab on the FP64 defect
+// r = b - J dP (iterative refinement); each sweep asks its FP32 solve for max(eta, 1e-6) relative accuracy. A step
+// whose defect misses eta but has decreased is still returned: Newton's residual test and line search judge it.
+int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res) {
 	const long n = c->g.n;
 	if (!c->cfg.krylov_fp32) {
 		KrylovBufs B{c->dP, c->kr, c->kr0, c->kp, c->kv, c->ks_, c->kt, c->J, c->part, c->part_stride, c->ksd};
 		return krylov_run<double>(c, B, c->rhs, c->cfg.krylov_tol, iters, ok);
 	}
+	const double tol = c->cfg.newton_tol;
+	const double eta = std::min(1e-2, std::max(0.5 * newton_res, 0.1 * tol / std::max(newton_res, 1e-300)));
 	launch_to_f32(5 * n, c->J, c->J32, c->st);
 	launch_to_f32(n, c->rhs, c->b32, c->st);
+	HIPCHK(hipMemsetAsync(c->dP, 0, n * sizeof(double), c->st));
 	KrylovBufs32 B{c->x32, c->r32, c->r032, c->p32, c->v32, c->s32, c->t32, c->J32, c->part, c->part_stride, c->ksd};
-	const int rc = krylov_run<float>(c, B, c->b32, std::max(c->cfg.krylov_tol, 1e-6), iters, ok);
-	if (rc) return rc;
-	launch_to_f64(n, c->x32, c->dP, c->st);
+	iters = 0; ok = false;
+	const int nbl = (int)((n + 255) / 256);
+	for (int sweep = 0; sweep < 6; sweep++) {
+		int it = 0; bool kok = false;
+		const int rc = krylov_run<float>(c, B, c->b32, std::max(eta, 1e-6), it, kok);
+		if (rc) return rc;
+		iters += it;
+		launch_add_f32(n, c->x32, c->dP, c->st);
+		// FP64 defect of the accumulated step
+		{
+			TRegion tr(c, T_KRYLOV);
+			launch_linres(c->g, c->J, c->rhs, c->dP, c->kr, c->part, c->part_stride, c->st);
+		}
+		std::vector<double>& hp = c->hpart;
+		hp.resize(2 * (size_t)c->part_stride);
+		HIPCHK(hipMemcpyAsync(hp.data(), c->part, 2 * (size_t)c->part_stride * sizeof(double), hipMemcpyDeviceToHost, c->st));
+		HIPCHK(hipStreamSynchronize(c->st));
+		double rr = 0.0, bb = 0.0;
+		for (int q = 0; q < nbl; q++) { rr += hp[q]; bb += hp[c->part_stride + q]; }
+		if (getenv("SPB6_VERBOSE") && atoi(getenv("SPB6_VERBOSE")) >= 3)
+			fprintf(stderr, "        sweep %d krylov %d defect %.3e eta %.3e\n", sweep, it, std::sqrt(rr / bb), eta);
+		if (std::sqrt(rr) <= eta * std::sqrt(bb) || !(bb > 0)) { ok = true; break; }
+		ok = std::isfinite(rr) && rr < bb;   // progress so far
+		if (!kok && sweep > 0) break;
+		launch_to_f32(n, c->kr, c->b32, c->st);
+	}
 	return SPB6_OK;
 }
 
@@ -178,14 +228,17 @@ int eval_all(spb6_ctx* c, const Dev& D, const double* P, const double* dP, doubl
 }
 
 // Newton_Solver (PC:761-847): unknown c->Pst; status SPB6_OK or SPB6_NEWTON_FAIL / SPB6_KRYLOV_FAIL
-int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal) {
+int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G = false, bool ok0 = false, double res0 = 0.0) {
 	const int n = c->g.n; const double tol = c->cfg.newton_tol;
 	const bool tri = (c->g.ny == 1 && c->cfg.bc[0] != SPB6_BC_PERIODIC);
-	bool ok; double res;
-	int rc = eval_all(c, D, c->Pst, nullptr, 0.0, nullptr, c->G, ok, res); if (rc) return rc;
+	bool ok = ok0; double res = res0;
+	int rc = SPB6_OK;
+	if (!have_G) { rc = eval_all(c, D, c->Pst, nullptr, 0.0, nullptr, c->G, ok, res); if (rc) return rc; }
 	if (!ok) { iters = 0; resfinal = INFINITY; return SPB6_NEWTON_FAIL; }
+	const int verbose = getenv("SPB6_VERBOSE") ? atoi(getenv("SPB6_VERBOSE")) : 0;
 	for (int it = 0; it < c->cfg.newton_iter; it++) {
 		iters = it; resfinal = res;
+		if (verbose >= 3) fprintf(stderr, "    newton it %d res %.3e\n", it, res);   // the reference's verbose >= 3 trace (PC:781)
 		if (res < tol) return SPB6_OK;
 		rc = reset_scalars(c); if (rc) return rc;
 		{
@@ -198,7 +251,7 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal) {
 			TRegion tr(c, T_OTHER);
 			launch_thomas(n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
 		} else {
-			rc = krylov(c, kit, kok); if (rc) return rc;
+			rc = krylov(c, kit, kok, res); if (rc) return rc;
 			c->stats.krylov_it += kit;
 		}
 		launch_stepcrit(n, c->dP, c->Pst, c->sc, &c->nrm[64], c->st);
@@ -206,6 +259,7 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal) {
 		if (c->hs->flags[FLAG_JAC]) return SPB6_NEWTON_FAIL;
 		if (!kok) return SPB6_KRYLOV_FAIL;
 		const double sn = nrm_val(c, 1);
+		if (verbose >= 3) fprintf(stderr, "      step %.3e krylov %d\n", sn, kit);
 		if (sn < tol) {   // step criterion (PC:819-826): accept P + dP
 			rc = eval_all(c, D, c->Pst, c->dP, 1.0, c->Pn, c->Gn, ok, res); if (rc) return rc;
 			std::swap(c->Pst, c->Pn); std::swap(c->G, c->Gn);
@@ -217,6 +271,7 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal) {
 		for (int ls = 0; ls < 40; ls++) {   // Armijo line search (PC:827-832)
 			double rn;
 			rc = eval_all(c, D, c->Pst, c->dP, lam, c->Pn, c->Gn, ok, rn); if (rc) return rc;
+			if (verbose >= 3) fprintf(stderr, "      line search lam %.3e res %.3e ok %d\n", lam, rn, (int)ok);
 			if (ok && (rn < res * (1.0 - 1e-4 * lam) || rn < tol)) { acc = true; res = rn; break; }
 			lam *= 0.5;
 		}
@@ -260,8 +315,34 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 	rc = fetch(c); if (rc) return rc;
 	if (c->hs->flags[FLAG_STAGE]) return SPB6_STAGE_INADMISSIBLE;
 	Dev D = stage_dev(c, adt);
-	rc = newton(c, D, iters, res);
+	const long n = c->g.n;
+	const bool dumping = c->dX && c->dcap > 0;
+	bool have_G = false, ok0 = false; double res0 = 0.0;
+	if (dumping || (c->cfg.learned_guess && c->model)) {
+		HIPCHK(hipMemcpyAsync(c->Pplain, c->Pst, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+		mlp_feat_cell(D, c->Pplain, c->F, c->st);
+	}
+	if (c->cfg.learned_guess && c->model) {
+		// the plain guess judges the learned one: keep whichever has the smaller Newton residual norm
+		rc = eval_all(c, D, c->Pst, nullptr, 0.0, nullptr, c->G, ok0, res0); if (rc) return rc;
+		const spb6_mlp* M = c->model;
+		MlpDev W{M->P, M->W1t, M->W2t, M->mean, M->istd, M->b3, M->mu_t, M->sd_t};
+		mlp_infer(&D, c->F, adt, (int)n, nullptr, W, c->Pplain, c->Pn, nullptr, c->st);
+		bool okl; double resl;
+		rc = eval_all(c, D, c->Pn, nullptr, 0.0, nullptr, c->Gn, okl, resl); if (rc) return rc;
+		if (okl && (!ok0 || resl < res0)) {
+			if (ok0 && resl > 0) c->stats.guess_log10_gain += std::log10(res0 / resl);
+			std::swap(c->Pst, c->Pn); std::swap(c->G, c->Gn); ok0 = true; res0 = resl; c->stats.guess_used++;
+		}
+		else c->stats.guess_fallback++;
+		have_G = true;
+	}
+	rc = newton(c, D, iters, res, have_G, ok0, res0);
 	if (rc != SPB6_OK) return rc;
+	if (dumping) {
+		const int off = (int)((c->dseq++ * 2654435761u) % (unsigned)c->dstride);
+		mlp_dump(D, c->F, adt, c->Pst, c->Pplain, off, c->dstride, c->dcap, c->dX, c->dY, c->dcount, c->st);
+	}
 	c->stats.maxres = std::max(c->stats.maxres, res);
 	rc = reset_scalars(c); if (rc) return rc;
 	{ TRegion tr(c, T_FINAL); launch_final(D, c->Pst, Qs, adt, Hk, c->st); }
@@ -361,12 +442,14 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 		for (auto p : fv) { if (hipMalloc((void**)p, n * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(*p); }
 		if (hipMalloc((void**)&c->J32, 5 * n * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->J32);
 	}
-	c->part_stride = (int)((n + 1023) / 1024) + 1;
+	c->part_stride = (int)((n + 255) / 256) + 1;   // partial-sum slots sized for 256-row blocks
 	if (dalloc(c, &c->part, (size_t)KRYLOV_SLOTS * c->part_stride)) return fail("alloc");
 	if (hipMalloc((void**)&c->ksd, sizeof(KS)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->ksd);
 	if (hipMalloc((void**)&c->nrm, 8 * 64 * sizeof(unsigned long long)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->nrm);
 	if (hipMalloc((void**)&c->flags, NFLAGS * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->flags);
 	if (hipHostMalloc((void**)&c->hs, sizeof(HostScratch), hipHostMallocDefault) != hipSuccess) return fail("host alloc");
+	if (dalloc(c, &c->Pplain, n)) return fail("alloc");
+	if (hipMalloc((void**)&c->F, 7 * NC * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->F);
 	if (hipDeviceSynchronize() != hipSuccess) return fail("sync");
 	return c;
 }
@@ -375,6 +458,7 @@ void spb6_destroy(spb6_ctx* c) {
 	if (!c) return;
 	if (c->st) hipStreamSynchronize(c->st);
 	for (void* p : c->allocs) hipFree(p);
+	if (c->dX) { hipFree(c->dX); hipFree(c->dY); hipFree(c->dcount); }
 	if (c->hs) hipHostFree(c->hs);
 	if (c->st) hipStreamDestroy(c->st);
 	delete c;
@@ -563,7 +647,7 @@ int spb6_linsolve(spb6_ctx* c, const double* val, const double* b, double* x, in
 	HIPCHK(hipMemcpyAsync(c->rhs, b, n * sizeof(double), hipMemcpyHostToDevice, c->st));
 	int it = 0; bool ok = true;
 	if (use_thomas) launch_thomas((int)n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
-	else { int rc = krylov(c, it, ok); if (rc) return rc; }
+	else { int rc = krylov(c, it, ok, 1.0); if (rc) return rc; }
 	HIPCHK(hipMemcpyAsync(x, c->dP, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
 	HIPCHK(hipStreamSynchronize(c->st));
 	if (iters) *iters = it;
@@ -616,6 +700,182 @@ int spb6_get_kernel_times(spb6_ctx* c, spb6_kernel_times_t* t) {
 	t->explicit_n = T.cnt[T_EXPLICIT]; t->prep_n = T.cnt[T_PREP]; t->shock_n = T.cnt[T_SHOCK]; t->residual_n = T.cnt[T_RESIDUAL];
 	t->jacobian_n = T.cnt[T_JACOBIAN]; t->krylov_n = T.cnt[T_KRYLOV]; t->final_n = T.cnt[T_FINAL];
 	t->krylov_iters = c->stats.krylov_it;
+	return SPB6_OK;
+}
+
+
+// ---- learned guess: dumps of (features, target) pairs from the running solver
+int spb6_dump_enable(spb6_ctx* c, long capacity, int stride) {
+	if (!c || capacity < 0 || stride < 1) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	if (c->dX) { hipFree(c->dX); hipFree(c->dY); hipFree(c->dcount); c->dX = nullptr; c->dY = nullptr; c->dcount = nullptr; }
+	c->dcap = capacity; c->dstride = stride;
+	if (capacity == 0) return SPB6_OK;
+	HIPCHK(hipMalloc((void**)&c->dX, capacity * MLP_IN * sizeof(float)));
+	HIPCHK(hipMalloc((void**)&c->dY, capacity * sizeof(float)));
+	HIPCHK(hipMalloc((void**)&c->dcount, sizeof(unsigned long long)));
+	HIPCHK(hipMemset(c->dcount, 0, sizeof(unsigned long long)));
+	return SPB6_OK;
+}
+long spb6_dump_fetch(spb6_ctx* c, float* X, float* Y, long max_rows) {
+	if (!c || !c->dX) return -1;
+	hipSetDevice(c->device);
+	unsigned long long cnt = 0;
+	if (hipMemcpy(&cnt, c->dcount, sizeof cnt, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+	long rows = (long)std::min<unsigned long long>(cnt, (unsigned long long)c->dcap);
+	rows = std::min(rows, max_rows);
+	if (X && rows > 0 && hipMemcpy(X, c->dX, rows * MLP_IN * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+	if (Y && rows > 0 && hipMemcpy(Y, c->dY, rows * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+	hipMemset(c->dcount, 0, sizeof(unsigned long long));
+	return rows;
+}
+int spb6_set_guess_model(spb6_ctx* c, spb6_mlp* m) {
+	if (!c) return SPB6_BAD_ARG;
+	c->model = m;
+	return SPB6_OK;
+}
+
+// ---- the network: FP32 training on the device, BF16 MFMA inference
+static int mlp_alloc(spb6_mlp* m, void** p, size_t bytes) {
+	HIPCHK(hipMalloc(p, std::max<size_t>(bytes, 4)));
+	m->allocs.push_back(*p);
+	HIPCHK(hipMemset(*p, 0, std::max<size_t>(bytes, 4)));
+	return SPB6_OK;
+}
+spb6_mlp* spb6_mlp_create(int device) {
+	spb6_mlp* m = new spb6_mlp();
+	m->device = device;
+	auto fail = [&]() -> spb6_mlp* { spb6_mlp_destroy(m); return nullptr; };
+	if (hipSetDevice(device) != hipSuccess) return fail();
+	if (hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking) != hipSuccess) return fail();
+	if (mlp_alloc(m, (void**)&m->P, MLP_NPARAM * 4) || mlp_alloc(m, (void**)&m->G, MLP_NPARAM * 4) || mlp_alloc(m, (void**)&m->m, MLP_NPARAM * 4) ||
+	    mlp_alloc(m, (void**)&m->v, MLP_NPARAM * 4) || mlp_alloc(m, (void**)&m->mean, MLP_IN * 4) || mlp_alloc(m, (void**)&m->istd, MLP_IN * 4) ||
+	    mlp_alloc(m, (void**)&m->loss, 4) || mlp_alloc(m, (void**)&m->W1t, MLP_H * MLP_KPAD * 2) || mlp_alloc(m, (void**)&m->W2t, MLP_H * MLP_H * 2))
+		return fail();
+	return m;
+}
+void spb6_mlp_destroy(spb6_mlp* m) {
+	if (!m) return;
+	if (m->st) hipStreamSynchronize(m->st);
+	for (void* p : m->allocs) hipFree(p);
+	if (m->st) hipStreamDestroy(m->st);
+	delete m;
+}
+static int mlp_sync_infer(spb6_mlp* m) {   // refresh the BF16 copy and the host scalars after the FP32 parameters change
+	mlp_pack(m->P, m->W1t, m->W2t, m->st);
+	HIPCHK(hipMemcpyAsync(&m->b3, m->P + MLP_OFF_B3, sizeof(float), hipMemcpyDeviceToHost, m->st));
+	HIPCHK(hipStreamSynchronize(m->st));
+	return SPB6_OK;
+}
+int spb6_mlp_set_params(spb6_mlp* m, const float* params, const float* norm) {
+	if (!m || !params) return SPB6_BAD_ARG;
+	hipSetDevice(m->device);
+	HIPCHK(hipMemcpy(m->P, params, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+	if (norm) {
+		std::vector<float> istd(MLP_IN);
+		for (int k = 0; k < MLP_IN; k++) istd[k] = 1.f / std::max(norm[MLP_IN + k], 1e-12f);
+		HIPCHK(hipMemcpy(m->mean, norm, MLP_IN * sizeof(float), hipMemcpyHostToDevice));
+		HIPCHK(hipMemcpy(m->istd, istd.data(), MLP_IN * sizeof(float), hipMemcpyHostToDevice));
+		m->mu_t = norm[2 * MLP_IN]; m->sd_t = norm[2 * MLP_IN + 1];
+	}
+	HIPCHK(hipMemset(m->m, 0, MLP_NPARAM * 4)); HIPCHK(hipMemset(m->v, 0, MLP_NPARAM * 4)); m->step = 0;
+	return mlp_sync_infer(m);
+}
+int spb6_mlp_get_params(spb6_mlp* m, float* params) {
+	if (!m || !params) return SPB6_BAD_ARG;
+	hipSetDevice(m->device);
+	HIPCHK(hipMemcpy(params, m->P, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+	return SPB6_OK;
+}
+int spb6_mlp_nparams(void) { return MLP_NPARAM; }
+static int mlp_batch_bufs(spb6_mlp* m, int Bn) {
+	if (Bn <= m->bcap) return SPB6_OK;
+	for (float** p : {&m->Xb, &m->Yb, &m->ws, &m->yb}) if (*p) { hipFree(*p); *p = nullptr; }
+	if (m->idx) { hipFree(m->idx); m->idx = nullptr; }
+	HIPCHK(hipMalloc((void**)&m->Xb, (size_t)Bn * MLP_IN * 4)); HIPCHK(hipMalloc((void**)&m->Yb, (size_t)Bn * 4));
+	HIPCHK(hipMalloc((void**)&m->ws, (size_t)mlp_ws_floats(Bn) * 4)); HIPCHK(hipMalloc((void**)&m->yb, (size_t)Bn * 4));
+	HIPCHK(hipMalloc((void**)&m->idx, (size_t)Bn * 4));
+	m->bcap = Bn;
+	return SPB6_OK;
+}
+int spb6_mlp_set_data(spb6_mlp* m, const float* X, const float* Y, long n) {
+	if (!m || !X || !Y || n <= 0) return SPB6_BAD_ARG;
+	hipSetDevice(m->device);
+	if (m->X) { hipFree(m->X); hipFree(m->Y); m->X = m->Y = nullptr; }
+	HIPCHK(hipMalloc((void**)&m->X, (size_t)n * MLP_IN * 4)); HIPCHK(hipMalloc((void**)&m->Y, (size_t)n * 4));
+	HIPCHK(hipMemcpy(m->X, X, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice));
+	HIPCHK(hipMemcpy(m->Y, Y, (size_t)n * 4, hipMemcpyHostToDevice));
+	m->ndata = n;
+	return SPB6_OK;
+}
+// epochs of Adam over the device data set (standardised features and targets), mini-batches in a fresh random order
+// each epoch (SplitMix64-seeded Fisher-Yates on the host); loss_out[e] = mean squared error of epoch e
+int spb6_mlp_train(spb6_mlp* m, int epochs, int batch, float lr, unsigned long long seed, float* loss_out) {
+	if (!m || !m->X || epochs < 1 || batch < 1) return SPB6_BAD_ARG;
+	hipSetDevice(m->device);
+	int rc = mlp_batch_bufs(m, batch); if (rc) return rc;
+	std::vector<int> perm(m->ndata);
+	for (long i = 0; i < m->ndata; i++) perm[i] = (int)i;
+	unsigned long long s = seed;
+	auto rnd = [&]() { s += 0x9E3779B97F4A7C15ull; unsigned long long z = s; z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull; z = (z ^ (z >> 27)) * 0x94D049BB133111EBull; return z ^ (z >> 31); };
+	int* didx = nullptr;
+	HIPCHK(hipMalloc((void**)&didx, (size_t)m->ndata * 4));
+	for (int e = 0; e < epochs; e++) {
+		for (long i = m->ndata - 1; i > 0; i--) { const long j = (long)(rnd() % (unsigned long long)(i + 1)); std::swap(perm[i], perm[j]); }
+		HIPCHK(hipMemcpyAsync(didx, perm.data(), (size_t)m->ndata * 4, hipMemcpyHostToDevice, m->st));
+		HIPCHK(hipMemsetAsync(m->loss, 0, 4, m->st));
+		long seen = 0;
+		for (long b0 = 0; b0 + batch <= m->ndata; b0 += batch) {
+			mlp_gather(batch, m->X, m->Y, didx + b0, m->Xb, m->Yb, m->st);
+			mlp_fwd_bwd(m->P, m->Xb, m->Yb, batch, m->ws, m->G, m->yb, m->loss, m->st);
+			mlp_adam(m->P, m->G, m->m, m->v, lr, ++m->step, m->st);
+			seen += batch;
+		}
+		float l = 0.f;
+		HIPCHK(hipMemcpyAsync(&l, m->loss, 4, hipMemcpyDeviceToHost, m->st));
+		HIPCHK(hipStreamSynchronize(m->st));
+		if (loss_out) loss_out[e] = seen ? l / (float)seen : 0.f;
+	}
+	hipFree(didx);
+	return mlp_sync_infer(m);
+}
+// FP32 forward on host features (standardised), y[n]
+int spb6_mlp_forward(spb6_mlp* m, const float* X, int n, float* y) {
+	if (!m || !X || !y || n < 1) return SPB6_BAD_ARG;
+	hipSetDevice(m->device);
+	int rc = mlp_batch_bufs(m, n); if (rc) return rc;
+	HIPCHK(hipMemcpyAsync(m->Xb, X, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice, m->st));
+	mlp_fwd_bwd(m->P, m->Xb, nullptr, n, m->ws, nullptr, m->yb, nullptr, m->st);
+	HIPCHK(hipMemcpyAsync(y, m->yb, (size_t)n * 4, hipMemcpyDeviceToHost, m->st));
+	HIPCHK(hipStreamSynchronize(m->st));
+	return SPB6_OK;
+}
+// gradient of the batch loss mean((y - t)^2) with respect to every parameter (FP32 backward kernels)
+int spb6_mlp_gradients(spb6_mlp* m, const float* X, const float* Y, int n, float* grad, float* loss) {
+	if (!m || !X || !Y || !grad || n < 1) return SPB6_BAD_ARG;
+	hipSetDevice(m->device);
+	int rc = mlp_batch_bufs(m, n); if (rc) return rc;
+	HIPCHK(hipMemcpyAsync(m->Xb, X, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice, m->st));
+	HIPCHK(hipMemcpyAsync(m->Yb, Y, (size_t)n * 4, hipMemcpyHostToDevice, m->st));
+	HIPCHK(hipMemsetAsync(m->loss, 0, 4, m->st));
+	mlp_fwd_bwd(m->P, m->Xb, m->Yb, n, m->ws, m->G, m->yb, m->loss, m->st);
+	HIPCHK(hipMemcpyAsync(grad, m->G, MLP_NPARAM * 4, hipMemcpyDeviceToHost, m->st));
+	float l = 0.f;
+	HIPCHK(hipMemcpyAsync(&l, m->loss, 4, hipMemcpyDeviceToHost, m->st));
+	HIPCHK(hipStreamSynchronize(m->st));
+	if (loss) *loss = l / (float)n;
+	return SPB6_OK;
+}
+// BF16 MFMA inference on host features (standardised), y[n]
+int spb6_mlp_infer_bf16(spb6_mlp* m, const float* X, int n, float* y) {
+	if (!m || !X || !y || n < 1) return SPB6_BAD_ARG;
+	hipSetDevice(m->device);
+	int rc = mlp_batch_bufs(m, n); if (rc) return rc;
+	HIPCHK(hipMemcpyAsync(m->Xb, X, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice, m->st));
+	MlpDev W{m->P, m->W1t, m->W2t, m->mean, m->istd, m->b3, m->mu_t, m->sd_t};
+	mlp_infer(nullptr, nullptr, 0.0, n, m->Xb, W, nullptr, nullptr, m->yb, m->st);
+	HIPCHK(hipMemcpyAsync(y, m->yb, (size_t)n * 4, hipMemcpyDeviceToHost, m->st));
+	HIPCHK(hipStreamSynchronize(m->st));
 	return SPB6_OK;
 }
 

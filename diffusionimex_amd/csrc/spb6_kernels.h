@@ -59,6 +59,8 @@ void launch_caldt(const Geo& g, const Phys& P, const double* U, int acoustic, un
 void launch_thomas(int n, const double* J, const double* rhs, double* x, double* cp, double* dp, hipStream_t st);
 void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hipStream_t st);
 void launch_krylov_iter(const Geo& g, const KrylovBufs& B, hipStream_t st, int it, double tol, int maxit);
+void launch_linres(const Geo& g, const double* J, const double* b, const double* x, double* r, double* part, int stride, hipStream_t st);
+void launch_add_f32(long n, const float* d, double* x, hipStream_t st);
 void launch_to_f32(long n, const double* a, float* b, hipStream_t st);
 void launch_to_f64(long n, const float* a, double* b, hipStream_t st);
 void launch_krylov_setup32(const Geo& g, const KrylovBufs32& B, const float* b, hipStream_t st);

@@ -798,6 +798,27 @@ __global__ void __launch_bounds__(256) kb_xr(Geo g, KV<T> V, KS* ks, KP P, int i
 	put_partial(P, SL_RR, srr, sh); put_partial(P, SL_R0R, sr0r, sh);
 }
 
+// FP64 defect of the linear system r = b - J x, with partials of r.r (slot SL_RR) and b.b (slot SL_BB) per block
+__global__ void __launch_bounds__(256) k_linres(Geo g, const double* __restrict__ J, const double* __restrict__ b, const double* __restrict__ x,
+                                               double* __restrict__ r, double* __restrict__ part, int stride) {
+	__shared__ double sh[4];
+	const int n = g.n;
+	const int q = blockIdx.x * 256 + threadIdx.x;
+	double rr = 0.0, bb = 0.0;
+	if (q < n) { const double v = b[q] - spmv_plain(g, J, x, q); r[q] = v; rr = v * v; bb = b[q] * b[q]; }
+	const double t1 = block_sum256(rr, sh), t2 = block_sum256(bb, sh);
+	if (threadIdx.x == 0) { part[blockIdx.x] = t1; part[stride + blockIdx.x] = t2; }
+}
+// x += (double) d
+__global__ void __launch_bounds__(256) k_add_f32(long n, const float* __restrict__ d, double* __restrict__ x) {
+	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) x[i] += (double)d[i];
+}
+void launch_linres(const Geo& g, const double* J, const double* b, const double* x, double* r, double* part, int stride, hipStream_t st) {
+	hipLaunchKernelGGL(k_linres, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, st, g, J, b, x, r, part, stride);
+}
+void launch_add_f32(long n, const float* d, double* x, hipStream_t st) { hipLaunchKernelGGL(k_add_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, d, x); }
+
 // precision conversions around the FP32 Krylov solve (the Newton system stays FP64)
 __global__ void __launch_bounds__(256) k_to_f32(long n, const double* __restrict__ a, float* __restrict__ b) {
 	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;

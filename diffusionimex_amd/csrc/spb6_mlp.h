@@ -1,0 +1,30 @@
+// spb6_mlp.h -- learned Newton guess: network constants and launchers (internal to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "spb6_kernels.h"
+
+namespace spb {
+
+constexpr int MLP_IN = 175, MLP_H = 128, MLP_KPAD = 192;
+// parameter vector layout (FP32): W1[175 x 128] (row k, column n), b1[128], W2[128 x 128], b2[128], w3[128], b3
+constexpr int MLP_OFF_W1 = 0, MLP_OFF_B1 = MLP_IN * MLP_H, MLP_OFF_W2 = MLP_OFF_B1 + MLP_H, MLP_OFF_B2 = MLP_OFF_W2 + MLP_H * MLP_H,
+              MLP_OFF_W3 = MLP_OFF_B2 + MLP_H, MLP_OFF_B3 = MLP_OFF_W3 + MLP_H, MLP_NPARAM = MLP_OFF_B3 + 1;
+
+// device copy of a trained network for inference
+struct MlpDev {
+	const float* params; const __bf16* W1t; const __bf16* W2t; const float* mean; const float* istd;
+	float b3, mu_t, sd_t;
+};
+
+void mlp_feat_cell(const Dev& D, const double* Pplain, float* F, hipStream_t st);
+void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, const double* Pplain, int off, int stride, long cap,
+              float* X, float* Y, unsigned long long* count, hipStream_t st);
+void mlp_pack(const float* params, __bf16* W1t, __bf16* W2t, hipStream_t st);
+void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain, double* Pout,
+               float* yout, hipStream_t st);
+void mlp_fwd_bwd(const float* P, const float* Xb, const float* Yb, int Bn, float* ws, float* G, float* yout, float* loss, hipStream_t st);
+void mlp_adam(float* P, const float* G, float* m, float* v, float lr, int t, hipStream_t st);
+void mlp_gather(int Bn, const float* X, const float* Y, const int* idx, float* Xb, float* Yb, hipStream_t st);
+inline long mlp_ws_floats(int Bn) { return (long)Bn * MLP_H * 6 + 2L * Bn; }
+
+}   // namespace spb

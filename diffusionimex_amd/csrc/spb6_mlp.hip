@@ -1,0 +1,386 @@
+// spb6_mlp.hip -- learned Newton initial guess: features, FP32 training kernels, BF16 MFMA inference (gfx950).
+//
+// Network: 175 -> 128 (GELU) -> 128 (GELU) -> 1. Input = the 5x5 stencil of 7 per-cell fields of the stage (explicit
+// predictor X*, plain guess P, frozen sound speed), expressed relative to the centre cell and standardised; output =
+// the standardised pressure change (P_converged - P_plain) / K_w of the centre cell. Training runs in FP32 with
+// hand-written kernels (tiled SGEMM, GELU forward/backward, bias sums, Adam); inference runs the two hidden layers on
+// MFMA (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32 accumulation), persistent blocks with the weights in LDS.
+#include <hip/hip_runtime.h>
+#include "spb6_device.h"
+#include "spb6_kernels.h"
+#include "spb6_mlp.h"
+#include <algorithm>
+
+namespace spb {
+
+// ------------------------------------------------------------------------------------------------ features
+__device__ __forceinline__ int mapP(const Geo& g, int i, int j) {   // M_P (PC:688-697)
+	int I, J;
+	if (i < GH) I = g.xper ? g.nx + i : GH;
+	else if (i > g.nx + GH - 1) I = g.xper ? i - g.nx : g.nx + GH - 1;
+	else I = i;
+	if (j < GH) J = (g.yper && g.ny > 1) ? g.ny + j : GH;
+	else if (j > g.ny + GH - 1) J = (g.yper && g.ny > 1) ? j - g.ny : g.ny + GH - 1;
+	else J = j;
+	return (I - GH) + g.nx * (J - GH);
+}
+// per-cell raw fields over the ghosted grid: F[f*NC + c], f = alpha1, log rho, u*, v*, pbar(X*) - P_REF, plain P, c_frozen
+__global__ void __launch_bounds__(256) k_feat_cell(Geo g, Phys P, double P_REF, const double* __restrict__ Xs, const double* __restrict__ Pg,
+                                                   const double* __restrict__ ust, const double* __restrict__ vst, const double* __restrict__ cz,
+                                                   float* __restrict__ F) {
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= g.NC) return;
+	const int NC = g.NC, i = c % g.NXT, j = c / g.NXT;
+	const double a1 = Xs[IA1 * NC + c], m1 = Xs[IM1 * NC + c], m2 = Xs[IM2 * NC + c], rho = m1 + m2;
+	const double pg = Pg[mapP(g, i, j)];
+	double pb = pg;
+	if (a1 > 0 && a1 < 1 && m1 > 0 && m2 > 0 && rho > 0) {
+		const double mu = Xs[IMU * NC + c], mv = Xs[IMV * NC + c];
+		const double pm = p_mix(P, a1, m1, m2, Xs[IEN * NC + c] - 0.5 * (mu * mu + mv * mv) / rho) - P_REF;
+		if (isfin(pm)) pb = pm;
+	}
+	const double cf = isfin(cz[c]) ? cz[c] : 0.0;
+	F[0 * NC + c] = (float)a1;
+	F[1 * NC + c] = (float)(rho > 0 ? log(rho) : -50.0);
+	F[2 * NC + c] = (float)ust[c];
+	F[3 * NC + c] = (float)vst[c];
+	F[4 * NC + c] = (float)pb;
+	F[5 * NC + c] = (float)pg;
+	F[6 * NC + c] = (float)cf;
+}
+struct FeatCtx { Geo g; const float* F; const double* Kw; const double* cz; double tau_dx; };
+// raw feature k (0..174) of interior row q: stencil offset s = k / 7 (5x5, row-major from (-2,-2)), field f = k % 7
+__device__ __forceinline__ float feature(const FeatCtx& X, int q, int k, float Kc, float czc, float F1c, float F2c, float F3c, float F5c) {
+	const Geo& g = X.g;
+	const int s = k / 7, f = k - 7 * s, dxo = s % 5 - 2, dyo = s / 5 - 2;
+	const int c = (q % g.nx + GH + dxo) + (q / g.nx + GH + dyo) * g.NXT;
+	const float* F = X.F; const int NC = g.NC;
+	switch (f) {
+	case 0: return F[c];
+	case 1: return F[NC + c] - F1c;
+	case 2: return (F[2 * NC + c] - F2c) / czc;
+	case 3: return (F[3 * NC + c] - F3c) / czc;
+	case 4: return (F[4 * NC + c] - F5c) / Kc;
+	case 5: return (F[5 * NC + c] - F5c) / Kc;
+	default: return log1pf((float)X.tau_dx * F[6 * NC + c]);
+	}
+}
+struct Centre { float Kc, czc, F1c, F2c, F3c, F5c; };
+__device__ __forceinline__ Centre centre_of(const FeatCtx& X, int q) {
+	const Geo& g = X.g; const int c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT, NC = g.NC;
+	Centre C;
+	C.Kc = (float)X.Kw[c]; const double cz = X.cz[c]; C.czc = (float)(isfin(cz) && cz > 0 ? cz : 1.0);
+	C.F1c = X.F[NC + c]; C.F2c = X.F[2 * NC + c]; C.F3c = X.F[3 * NC + c]; C.F5c = X.F[5 * NC + c];
+	return C;
+}
+// training dump: raw features of interior rows q = off, off + stride, ... and the target (P_conv - P_plain) / K_w
+__global__ void __launch_bounds__(256) k_dump(FeatCtx X, const double* __restrict__ Pconv, const double* __restrict__ Pplain, int off, int stride,
+                                              long cap, float* __restrict__ Xout, float* __restrict__ Yout, unsigned long long* __restrict__ count) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int q = off + t * stride;
+	__shared__ unsigned long long base;
+	const bool act = q < X.g.n;
+	// one counter update per block reserves the block's rows
+	__shared__ int nact;
+	if (threadIdx.x == 0) nact = 0;
+	__syncthreads();
+	int slot = -1;
+	if (act) slot = atomicAdd(&nact, 1);
+	__syncthreads();
+	if (threadIdx.x == 0) base = atomicAdd(count, (unsigned long long)nact);
+	__syncthreads();
+	if (!act) return;
+	const long row = (long)base + slot;
+	if (row >= cap) return;
+	const Centre C = centre_of(X, q);
+	for (int k = 0; k < MLP_IN; k++) Xout[row * MLP_IN + k] = feature(X, q, k, C.Kc, C.czc, C.F1c, C.F2c, C.F3c, C.F5c);
+	Yout[row] = (float)((Pconv[q] - Pplain[q]) / X.Kw[(q % X.g.nx + GH) + (q / X.g.nx + GH) * X.g.NXT]);
+}
+
+// ------------------------------------------------------------------------------------------------ FP32 training kernels
+// C[M x N] = op(A)[M x K] op(B)[K x N] (+ beta C): op(A)(m, k) = ta ? A[k*lda + m] : A[m*lda + k], likewise B.
+// 64x64 output tiles, K chunks of 16 staged in LDS, 256 threads with a 4x4 register tile each.
+__global__ void __launch_bounds__(256) k_sgemm(int M, int N, int K, const float* __restrict__ A, int lda, int ta,
+                                              const float* __restrict__ B, int ldb, int tb, float* __restrict__ C, int ldc, float beta) {
+	__shared__ float As[16][64 + 4], Bs[16][64 + 4];
+	const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+	const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+	float acc[4][4] = {};
+	for (int k0 = 0; k0 < K; k0 += 16) {
+		for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+			const int kk = e / 64, mm = e % 64;
+			const int m = m0 + mm, k = k0 + kk;
+			As[kk][mm] = (m < M && k < K) ? (ta ? A[(long)k * lda + m] : A[(long)m * lda + k]) : 0.f;
+			const int n = n0 + mm;
+			Bs[kk][mm] = (n < N && k < K) ? (tb ? B[(long)n * ldb + k] : B[(long)k * ldb + n]) : 0.f;
+		}
+		__syncthreads();
+		#pragma unroll
+		for (int kk = 0; kk < 16; kk++) {
+			float a[4], b[4];
+			#pragma unroll
+			for (int r = 0; r < 4; r++) { a[r] = As[kk][ty * 4 + r]; b[r] = Bs[kk][tx * 4 + r]; }
+			#pragma unroll
+			for (int r = 0; r < 4; r++)
+				#pragma unroll
+				for (int s = 0; s < 4; s++) acc[r][s] += a[r] * b[s];
+		}
+		__syncthreads();
+	}
+	#pragma unroll
+	for (int r = 0; r < 4; r++)
+		#pragma unroll
+		for (int s = 0; s < 4; s++) {
+			const int m = m0 + ty * 4 + r, n = n0 + tx * 4 + s;
+			if (m < M && n < N) { const long o = (long)m * ldc + n; C[o] = acc[r][s] + (beta != 0.f ? beta * C[o] : 0.f); }
+		}
+}
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float dgelu(float x) { return 0.5f * (1.f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x); }
+// Z += bias (per column), H = GELU(Z)
+__global__ void __launch_bounds__(256) k_bias_gelu(long rows, int N, float* __restrict__ Z, const float* __restrict__ b, float* __restrict__ H) {
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= rows * N) return;
+	const float z = Z[e] + b[e % N];
+	Z[e] = z; H[e] = gelu(z);
+}
+// y = H w + b (one output), loss partial and dy = 2 (y - t) / B
+__global__ void __launch_bounds__(256) k_head(int Bn, const float* __restrict__ H, const float* __restrict__ w, const float* __restrict__ bp,
+                                              const float* __restrict__ t, float* __restrict__ y, float* __restrict__ dy, float* __restrict__ loss) {
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	float l = 0.f;
+	if (r < Bn) {
+		float s = *bp;
+		for (int k = 0; k < MLP_H; k++) s += H[(long)r * MLP_H + k] * w[k];
+		y[r] = s;
+		if (t) { const float d = s - t[r]; l = d * d; if (dy) dy[r] = 2.f * d / (float)Bn; }
+	}
+	if (loss) {
+		for (int off = 32; off >= 1; off >>= 1) l += __shfl_xor(l, off, 64);
+		if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
+	}
+}
+// dZ2 = (dy w^T) * GELU'(Z2), and the head gradients dw = H^T dy, db = sum dy
+__global__ void __launch_bounds__(256) k_head_back(int Bn, const float* __restrict__ dy, const float* __restrict__ w, const float* __restrict__ Z,
+                                                   float* __restrict__ dZ) {
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)Bn * MLP_H) return;
+	const int r = (int)(e / MLP_H), k = (int)(e % MLP_H);
+	dZ[e] = dy[r] * w[k] * dgelu(Z[e]);
+}
+__global__ void __launch_bounds__(256) k_mul_dgelu(long n, float* __restrict__ dH, const float* __restrict__ Z) {
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < n) dH[e] *= dgelu(Z[e]);
+}
+// column sums of a rows x N matrix (bias gradients), one block per column, fixed order
+__global__ void __launch_bounds__(256) k_colsum(int rows, int N, const float* __restrict__ A, float* __restrict__ out) {
+	__shared__ float sh[256];
+	const int col = blockIdx.x;
+	float s = 0.f;
+	for (int r = threadIdx.x; r < rows; r += 256) s += A[(long)r * N + col];
+	sh[threadIdx.x] = s;
+	__syncthreads();
+	for (int h = 128; h > 0; h >>= 1) { if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h]; __syncthreads(); }
+	if (threadIdx.x == 0) out[col] = sh[0];
+}
+// Adam (Kingma & Ba 2015) with bias correction
+__global__ void __launch_bounds__(256) k_adam(int n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                                              float lr, float b1, float b2, float eps, float c1, float c2) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const float gi = g[i];
+	const float mi = b1 * m[i] + (1.f - b1) * gi, vi = b2 * v[i] + (1.f - b2) * gi * gi;
+	m[i] = mi; v[i] = vi;
+	p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+__global__ void __launch_bounds__(256) k_gather_rows(int Bn, int K, const float* __restrict__ X, const float* __restrict__ Y, const int* __restrict__ idx,
+                                                     float* __restrict__ Xb, float* __restrict__ Yb) {
+	const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= (long)Bn * K) return;
+	const int r = (int)(e / K), k = (int)(e % K);
+	const long src = idx[r];
+	Xb[e] = X[src * K + k];
+	if (k == 0) Yb[r] = Y[src];
+}
+
+// ------------------------------------------------------------------------------------------------ BF16 MFMA inference
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+union BF8 { bf16x8 v; __bf16 e[8]; uint4 u; };
+
+// W1t[n][k] (k padded to 192) and W2t[n][k]: transposed BF16 weights so one lane's 8 consecutive k are one 16-byte read
+__global__ void __launch_bounds__(256) k_pack_bf16(const float* __restrict__ params, __bf16* __restrict__ W1t, __bf16* __restrict__ W2t) {
+	const int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < MLP_H * MLP_KPAD) {
+		const int n = e / MLP_KPAD, k = e % MLP_KPAD;
+		W1t[e] = (__bf16)(k < MLP_IN ? params[MLP_OFF_W1 + k * MLP_H + n] : 0.f);
+	}
+	if (e < MLP_H * MLP_H) {
+		const int n = e / MLP_H, k = e % MLP_H;
+		W2t[e] = (__bf16)params[MLP_OFF_W2 + k * MLP_H + n];
+	}
+}
+// MFMA 16x16x32 operand layout: lane l holds A rows m = l % 16 and k = 8 (l / 16) .. +7 of a 32-wide k step, B columns
+// n = l % 16 with the same k; the 16x16 FP32 result gives lane l column n = l % 16, rows 4 (l / 16) .. +3.
+// mode 0: features from the stage (guess); mode 1: standardised features given in Xin[rows][175] (tests)
+__global__ void __launch_bounds__(256) k_mlp_infer(FeatCtx X, int nrows, const float* __restrict__ Xin, int mode,
+                                                   const float* __restrict__ mean, const float* __restrict__ istd,
+                                                   const __bf16* __restrict__ W1t, const float* __restrict__ b1,
+                                                   const __bf16* __restrict__ W2t, const float* __restrict__ b2,
+                                                   const float* __restrict__ w3, float b3, float mu_t, float sd_t,
+                                                   const double* __restrict__ Pplain, double* __restrict__ Pout, float* __restrict__ yout) {
+	__shared__ uint4 sW1[MLP_H * MLP_KPAD / 8];     // 48 KB
+	__shared__ uint4 sW2[MLP_H * MLP_H / 8];        // 32 KB
+	__shared__ __bf16 sH[4][16][MLP_H + 8];         // per-wave hidden activations (A layout source)
+	__shared__ float sMean[MLP_IN], sIstd[MLP_IN];
+	for (int e = threadIdx.x; e < MLP_H * MLP_KPAD / 8; e += 256) sW1[e] = ((const uint4*)W1t)[e];
+	for (int e = threadIdx.x; e < MLP_H * MLP_H / 8; e += 256) sW2[e] = ((const uint4*)W2t)[e];
+	for (int e = threadIdx.x; e < MLP_IN; e += 256) { sMean[e] = mean[e]; sIstd[e] = istd[e]; }
+	__syncthreads();
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+	const int ntiles = (nrows + 15) / 16;
+	for (int tile = blockIdx.x * 4 + wave; tile < ntiles; tile += gridDim.x * 4) {
+		const int q = tile * 16 + lr;           // this lane's A row (cell)
+		const bool valid = q < nrows;
+		Centre C{};
+		if (valid && mode == 0) C = centre_of(X, q);
+		// layer 1: Z1[16 x 128] = X[16 x 192] W1[192 x 128]
+		f32x4 acc1[8];
+		#pragma unroll
+		for (int nt = 0; nt < 8; nt++) acc1[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+		#pragma unroll
+		for (int ks = 0; ks < MLP_KPAD / 32; ks++) {
+			BF8 a;
+			#pragma unroll
+			for (int e = 0; e < 8; e++) {
+				const int k = ks * 32 + lk * 8 + e;
+				float x = 0.f;
+				if (valid && k < MLP_IN) {
+					const float raw = mode == 0 ? feature(X, q, k, C.Kc, C.czc, C.F1c, C.F2c, C.F3c, C.F5c) : Xin[(long)q * MLP_IN + k];
+					x = mode == 0 ? (raw - sMean[k]) * sIstd[k] : raw;
+				}
+				a.e[e] = (__bf16)x;
+			}
+			#pragma unroll
+			for (int nt = 0; nt < 8; nt++) {
+				BF8 b; b.u = sW1[((nt * 16 + lr) * MLP_KPAD + ks * 32 + lk * 8) / 8];
+				acc1[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc1[nt], 0, 0, 0);
+			}
+		}
+		// bias + GELU, to LDS in row-major [m][n] for the next layer's A fragments
+		#pragma unroll
+		for (int nt = 0; nt < 8; nt++)
+			#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				const int n = nt * 16 + lr, m = lk * 4 + r;
+				sH[wave][m][n] = (__bf16)gelu(acc1[nt][r] + b1[n]);
+			}
+		__builtin_amdgcn_s_waitcnt(0);
+		__builtin_amdgcn_wave_barrier();
+		// layer 2: Z2[16 x 128] = H1[16 x 128] W2[128 x 128]
+		f32x4 acc2[8];
+		#pragma unroll
+		for (int nt = 0; nt < 8; nt++) acc2[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+		#pragma unroll
+		for (int ks = 0; ks < MLP_H / 32; ks++) {
+			BF8 a;
+			#pragma unroll
+			for (int e = 0; e < 8; e++) a.e[e] = sH[wave][lr][ks * 32 + lk * 8 + e];
+			#pragma unroll
+			for (int nt = 0; nt < 8; nt++) {
+				BF8 b; b.u = sW2[((nt * 16 + lr) * MLP_H + ks * 32 + lk * 8) / 8];
+				acc2[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc2[nt], 0, 0, 0);
+			}
+		}
+		__builtin_amdgcn_wave_barrier();
+		// head: y_m = sum_n GELU(Z2[m][n] + b2[n]) w3[n] + b3; lanes with equal l / 16 hold the same rows
+		float part[4] = {0.f, 0.f, 0.f, 0.f};
+		#pragma unroll
+		for (int nt = 0; nt < 8; nt++) {
+			const int n = nt * 16 + lr;
+			#pragma unroll
+			for (int r = 0; r < 4; r++) part[r] += gelu(acc2[nt][r] + b2[n]) * w3[n];
+		}
+		#pragma unroll
+		for (int r = 0; r < 4; r++)
+			#pragma unroll
+			for (int off = 1; off < 16; off <<= 1) part[r] += __shfl_xor(part[r], off, 64);
+		if (lr == 0) {
+			#pragma unroll
+			for (int r = 0; r < 4; r++) {
+				const int qq = tile * 16 + lk * 4 + r;
+				if (qq < nrows) {
+					const float y = part[r] + b3;
+					if (yout) yout[qq] = y;
+					if (Pout) {
+						const Geo& g = X.g;
+						const double Kc = X.Kw[(qq % g.nx + GH) + (qq / g.nx + GH) * g.NXT];
+						Pout[qq] = Pplain[qq] + Kc * (double)(mu_t + sd_t * y);
+					}
+				}
+			}
+		}
+	}
+}
+
+// ------------------------------------------------------------------------------------------------ launchers
+static inline unsigned nb(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+void mlp_feat_cell(const Dev& D, const double* Pplain, float* F, hipStream_t st) {
+	hipLaunchKernelGGL(k_feat_cell, dim3(nb(D.g.NC)), dim3(256), 0, st, D.g, D.ph, D.P_REF, (const double*)D.Ust, Pplain, (const double*)D.ust,
+	                   (const double*)D.vst, (const double*)D.cz, F);
+}
+void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, const double* Pplain, int off, int stride, long cap,
+              float* X, float* Y, unsigned long long* count, hipStream_t st) {
+	FeatCtx fx{D.g, F, D.Kw, D.cz, tau / D.g.dx};
+	const long rows = (D.g.n - off + stride - 1) / stride;
+	if (rows <= 0) return;
+	hipLaunchKernelGGL(k_dump, dim3(nb(rows)), dim3(256), 0, st, fx, Pconv, Pplain, off, stride, cap, X, Y, count);
+}
+void mlp_pack(const float* params, __bf16* W1t, __bf16* W2t, hipStream_t st) {
+	hipLaunchKernelGGL(k_pack_bf16, dim3(nb(MLP_H * MLP_KPAD)), dim3(256), 0, st, params, W1t, W2t);
+}
+void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain, double* Pout, float* yout, hipStream_t st) {
+	FeatCtx fx{};
+	if (D) { fx.g = D->g; fx.F = F; fx.Kw = D->Kw; fx.cz = D->cz; fx.tau_dx = tau / D->g.dx; }
+	int dev = 0; hipGetDevice(&dev);
+	int cus = 256; hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+	const int tiles = (nrows + 15) / 16;
+	const int blocks = std::max(1, std::min(cus, (tiles + 3) / 4));
+	hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, fx, nrows, Xin, Xin ? 1 : 0, W.mean, W.istd, W.W1t, W.params + MLP_OFF_B1,
+	                   W.W2t, W.params + MLP_OFF_B2, W.params + MLP_OFF_W3, W.b3, W.mu_t, W.sd_t, Pplain, Pout, yout);
+}
+
+// forward (and optionally backward) of the FP32 network on a batch Xb[Bn x 175]; grads into G (same layout as params)
+void mlp_fwd_bwd(const float* P, const float* Xb, const float* Yb, int Bn, float* ws, float* G, float* yout, float* loss, hipStream_t st) {
+	float* Z1 = ws; float* H1 = Z1 + (long)Bn * MLP_H; float* Z2 = H1 + (long)Bn * MLP_H; float* H2 = Z2 + (long)Bn * MLP_H;
+	float* dy = H2 + (long)Bn * MLP_H; float* dZ2 = dy + Bn; float* dZ1 = dZ2 + (long)Bn * MLP_H; float* y = dZ1 + (long)Bn * MLP_H;
+	const dim3 g1(nb(MLP_H, 64), nb(Bn, 64));
+	hipLaunchKernelGGL(k_sgemm, g1, dim3(256), 0, st, Bn, MLP_H, MLP_IN, Xb, MLP_IN, 0, P + MLP_OFF_W1, MLP_H, 0, Z1, MLP_H, 0.f);
+	hipLaunchKernelGGL(k_bias_gelu, dim3(nb((long)Bn * MLP_H)), dim3(256), 0, st, (long)Bn, MLP_H, Z1, P + MLP_OFF_B1, H1);
+	hipLaunchKernelGGL(k_sgemm, g1, dim3(256), 0, st, Bn, MLP_H, MLP_H, (const float*)H1, MLP_H, 0, P + MLP_OFF_W2, MLP_H, 0, Z2, MLP_H, 0.f);
+	hipLaunchKernelGGL(k_bias_gelu, dim3(nb((long)Bn * MLP_H)), dim3(256), 0, st, (long)Bn, MLP_H, Z2, P + MLP_OFF_B2, H2);
+	hipLaunchKernelGGL(k_head, dim3(nb(Bn)), dim3(256), 0, st, Bn, (const float*)H2, P + MLP_OFF_W3, P + MLP_OFF_B3, Yb, yout ? yout : y, G ? dy : nullptr, loss);
+	if (!G) return;
+	// head gradients
+	hipLaunchKernelGGL(k_sgemm, dim3(1, nb(MLP_H, 64)), dim3(256), 0, st, MLP_H, 1, Bn, (const float*)H2, MLP_H, 1, (const float*)dy, 1, 0, G + MLP_OFF_W3, 1, 0.f);
+	hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, st, Bn, 1, (const float*)dy, G + MLP_OFF_B3);
+	hipLaunchKernelGGL(k_head_back, dim3(nb((long)Bn * MLP_H)), dim3(256), 0, st, Bn, (const float*)dy, P + MLP_OFF_W3, (const float*)Z2, dZ2);
+	// layer 2 gradients: dW2 = H1^T dZ2, db2 = colsum dZ2, dH1 = dZ2 W2^T
+	hipLaunchKernelGGL(k_sgemm, dim3(nb(MLP_H, 64), nb(MLP_H, 64)), dim3(256), 0, st, MLP_H, MLP_H, Bn, (const float*)H1, MLP_H, 1, (const float*)dZ2, MLP_H, 0, G + MLP_OFF_W2, MLP_H, 0.f);
+	hipLaunchKernelGGL(k_colsum, dim3(MLP_H), dim3(256), 0, st, Bn, MLP_H, (const float*)dZ2, G + MLP_OFF_B2);
+	hipLaunchKernelGGL(k_sgemm, g1, dim3(256), 0, st, Bn, MLP_H, MLP_H, (const float*)dZ2, MLP_H, 0, P + MLP_OFF_W2, MLP_H, 1, dZ1, MLP_H, 0.f);
+	hipLaunchKernelGGL(k_mul_dgelu, dim3(nb((long)Bn * MLP_H)), dim3(256), 0, st, (long)Bn * MLP_H, dZ1, (const float*)Z1);
+	// layer 1 gradients: dW1 = X^T dZ1, db1 = colsum dZ1
+	hipLaunchKernelGGL(k_sgemm, dim3(nb(MLP_H, 64), nb(MLP_IN, 64)), dim3(256), 0, st, MLP_IN, MLP_H, Bn, Xb, MLP_IN, 1, (const float*)dZ1, MLP_H, 0, G + MLP_OFF_W1, MLP_H, 0.f);
+	hipLaunchKernelGGL(k_colsum, dim3(MLP_H), dim3(256), 0, st, Bn, MLP_H, (const float*)dZ1, G + MLP_OFF_B1);
+}
+void mlp_adam(float* P, const float* G, float* m, float* v, float lr, int t, hipStream_t st) {
+	const float b1 = 0.9f, b2 = 0.999f;
+	const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+	hipLaunchKernelGGL(k_adam, dim3(nb(MLP_NPARAM)), dim3(256), 0, st, MLP_NPARAM, P, G, m, v, lr, b1, b2, 1e-8f, c1, c2);
+}
+void mlp_gather(int Bn, const float* X, const float* Y, const int* idx, float* Xb, float* Yb, hipStream_t st) {
+	hipLaunchKernelGGL(k_gather_rows, dim3(nb((long)Bn * MLP_IN)), dim3(256), 0, st, Bn, MLP_IN, X, Y, idx, Xb, Yb);
+}
+
+}   // namespace spb

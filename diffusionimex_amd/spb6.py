@@ -40,7 +40,7 @@ class Stats(C.Structure):
     _fields_ = [("steps", C.c_long), ("rk1", C.c_long), ("rejects", C.c_long), ("floor_acc", C.c_long),
                 ("stages", C.c_long), ("newton_it", C.c_long), ("krylov_it", C.c_long), ("linsolves", C.c_long),
                 ("maxres", C.c_double), ("t", C.c_double), ("dt_last", C.c_double), ("status", C.c_int),
-                ("guess_used", C.c_long), ("guess_fallback", C.c_long)]
+                ("guess_used", C.c_long), ("guess_fallback", C.c_long), ("guess_log10_gain", C.c_double)]
 
 
 class KernelTimes(C.Structure):
@@ -54,7 +54,10 @@ class KernelTimes(C.Structure):
 EXPORTS = ["spb6_create", "spb6_destroy", "spb6_last_error", "spb6_set_state", "spb6_get_state", "spb6_get_pressure",
            "spb6_set_pressure", "spb6_caldt", "spb6_bc_fill", "spb6_explicit_stage", "spb6_implicit_stage",
            "spb6_stage_prep", "spb6_step", "spb6_run", "spb6_get_stats", "spb6_time", "spb6_residual",
-           "spb6_jacobian", "spb6_linsolve", "spb6_get_field", "spb6_enable_timing", "spb6_get_kernel_times"]
+           "spb6_jacobian", "spb6_linsolve", "spb6_get_field", "spb6_enable_timing", "spb6_get_kernel_times",
+           "spb6_mlp_create", "spb6_mlp_destroy", "spb6_mlp_nparams", "spb6_mlp_set_params", "spb6_mlp_get_params",
+           "spb6_mlp_set_data", "spb6_mlp_train", "spb6_mlp_forward", "spb6_mlp_gradients", "spb6_mlp_infer_bf16",
+           "spb6_set_guess_model", "spb6_dump_enable", "spb6_dump_fetch"]
 
 _lib = None
 
@@ -93,6 +96,21 @@ def load():
     L.spb6_get_field.argtypes = [vp, C.c_char_p, dp, C.c_long]
     L.spb6_enable_timing.argtypes = [vp, C.c_int]
     L.spb6_get_kernel_times.argtypes = [vp, C.POINTER(KernelTimes)]
+    fp = C.POINTER(C.c_float)
+    L.spb6_mlp_create.restype = vp
+    L.spb6_mlp_create.argtypes = [C.c_int]
+    L.spb6_mlp_destroy.argtypes = [vp]
+    L.spb6_mlp_set_params.argtypes = [vp, fp, fp]
+    L.spb6_mlp_get_params.argtypes = [vp, fp]
+    L.spb6_mlp_set_data.argtypes = [vp, fp, fp, C.c_long]
+    L.spb6_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp]
+    L.spb6_mlp_forward.argtypes = [vp, fp, C.c_int, fp]
+    L.spb6_mlp_gradients.argtypes = [vp, fp, fp, C.c_int, fp, fp]
+    L.spb6_mlp_infer_bf16.argtypes = [vp, fp, C.c_int, fp]
+    L.spb6_set_guess_model.argtypes = [vp, vp]
+    L.spb6_dump_enable.argtypes = [vp, C.c_long, C.c_int]
+    L.spb6_dump_fetch.restype = C.c_long
+    L.spb6_dump_fetch.argtypes = [vp, fp, fp, C.c_long]
     _lib = L
     return L
 

@@ -74,6 +74,7 @@ typedef struct spb6_stats {
     double maxres, t, dt_last;
     int status;
     long guess_used, guess_fallback;  /* learned-guess stages, and stages where the plain guess was kept */
+    double guess_log10_gain;          /* sum over stages of log10(plain residual norm / chosen guess's residual norm) */
 } spb6_stats_t;
 
 typedef struct spb6_ctx spb6_ctx;
@@ -136,6 +137,33 @@ typedef struct spb6_kernel_times {
 } spb6_kernel_times_t;
 int spb6_enable_timing(spb6_ctx* ctx, int on);
 int spb6_get_kernel_times(spb6_ctx* ctx, spb6_kernel_times_t* t);
+
+/* ---- learned Newton guess (no counterpart in the reference; the reference's guess is the previous stage's pressure,
+ * PC:1250, PC:1286). A 175-128-128-1 GELU network maps the 5x5 stencil of 7 stage fields to the stage's pressure change;
+ * with cfg.learned_guess the stepper evaluates the Newton residual at the plain and at the learned guess and starts
+ * Newton from the one with the smaller residual norm (the plain guess is the fallback and the judge). */
+typedef struct spb6_mlp spb6_mlp;
+spb6_mlp* spb6_mlp_create(int device);
+void spb6_mlp_destroy(spb6_mlp* m);
+int spb6_mlp_nparams(void);
+/* params: FP32 parameter vector (W1[175x128], b1, W2[128x128], b2, w3[128], b3); norm: feature means[175], feature
+ * standard deviations[175], target mean, target standard deviation (NULL keeps the current ones) */
+int spb6_mlp_set_params(spb6_mlp* m, const float* params, const float* norm);
+int spb6_mlp_get_params(spb6_mlp* m, float* params);
+/* training set of standardised features X[n x 175] and targets Y[n], copied to the device */
+int spb6_mlp_set_data(spb6_mlp* m, const float* X, const float* Y, long n);
+/* Adam epochs on the device data (hand-written FP32 forward/backward kernels); loss_out[epochs] */
+int spb6_mlp_train(spb6_mlp* m, int epochs, int batch, float lr, unsigned long long seed, float* loss_out);
+/* FP32 forward, FP32 gradients of mean((y - t)^2), and the BF16 MFMA inference path on host features */
+int spb6_mlp_forward(spb6_mlp* m, const float* X, int n, float* y);
+int spb6_mlp_gradients(spb6_mlp* m, const float* X, const float* Y, int n, float* grad, float* loss);
+int spb6_mlp_infer_bf16(spb6_mlp* m, const float* X, int n, float* y);
+/* stepper side: use this network for the learned guess (the network must outlive the context's use of it) */
+int spb6_set_guess_model(spb6_ctx* ctx, spb6_mlp* m);
+/* record raw (features, target) pairs of every stage's cells q = off, off + stride, ... into a device buffer of
+ * `capacity` rows (capacity 0 disables); spb6_dump_fetch copies up to max_rows rows to the host and empties it */
+int spb6_dump_enable(spb6_ctx* ctx, long capacity, int stride);
+long spb6_dump_fetch(spb6_ctx* ctx, float* X, float* Y, long max_rows);
 
 #ifdef __cplusplus
 }
