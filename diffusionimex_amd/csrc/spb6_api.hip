@@ -79,6 +79,7 @@ struct spb6_ctx {
 	spb6_mlp* model = nullptr;
 	float* F = nullptr;                 // 7 per-cell feature fields over the ghosted grid
 	double* Pplain = nullptr;           // the plain guess of the current stage
+	double* Gplain = nullptr;           // its residual
 	float *dX = nullptr, *dY = nullptr; unsigned long long* dcount = nullptr; long dcap = 0; int dstride = 0; unsigned dseq = 0;
 };
 
@@ -170,18 +171,20 @@ int krylov_run(spb6_ctx* c, const KrylovBufsT<T>& B, const T* b, double tol, int
 	return SPB6_OK;
 }
 // FP32 Krylov form: inexact Newton step. The step solves J dP = b to a relative FP64 residual
-// eta = min(1e-2, max(res / 2, 0.1 tol / res)) -- res / 2 keeps Newton's quadratic convergence, 0.1 tol / res is all the
-// last step needs to bring the Newton residual under tol -- by sweeps of FP32 BiCGStab on the FP64 defect
-// r = b - J dP (iterative refinement); each sweep asks its FP32 solve for max(eta, 1e-6) relative accuracy. A step
-// whose defect misses eta but has decreased is still returned: Newton's residual test and line search judge it.
+// eta = min(1e-6, 0.1 tol / res) (res = the Newton residual norm, tol = the Newton tolerance), i.e. the step's own error
+// is held near a tenth of the tolerance. A looser eta leaves errors in the slow (smooth) modes of J that barely show in
+// the residual but move P by ~1e-10 in quiescent regions, where the upwind split (u_f)^+ / (u_f)^- of the face
+// increments has its kink at u_f = 0: the following Newton steps then converge only linearly. The solve runs as sweeps
+// of FP32 BiCGStab on the FP64 defect r = b - J dP (iterative refinement), each sweep asking its FP32 solve for
+// max(eta, 1e-6) relative accuracy (the FP32 floor). A step whose defect misses eta but has decreased is still
+// returned: Newton's residual test and line search judge it.
 int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res) {
 	const long n = c->g.n;
 	if (!c->cfg.krylov_fp32) {
 		KrylovBufs B{c->dP, c->kr, c->kr0, c->kp, c->kv, c->ks_, c->kt, c->J, c->part, c->part_stride, c->ksd};
 		return krylov_run<double>(c, B, c->rhs, c->cfg.krylov_tol, iters, ok);
 	}
-	const double tol = c->cfg.newton_tol;
-	const double eta = std::min(1e-2, std::max(0.5 * newton_res, 0.1 * tol / std::max(newton_res, 1e-300)));
+	const double eta = std::min(1e-6, 0.1 * c->cfg.newton_tol / std::max(newton_res, 1e-300));
 	launch_to_f32(5 * n, c->J, c->J32, c->st);
 	launch_to_f32(n, c->rhs, c->b32, c->st);
 	HIPCHK(hipMemsetAsync(c->dP, 0, n * sizeof(double), c->st));
@@ -317,17 +320,22 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 	Dev D = stage_dev(c, adt);
 	const long n = c->g.n;
 	const bool dumping = c->dX && c->dcap > 0;
+	const bool learned = c->cfg.learned_guess && c->model;
 	bool have_G = false, ok0 = false; double res0 = 0.0;
-	if (dumping || (c->cfg.learned_guess && c->model)) {
+	if (dumping || learned) {
+		// the plain guess and its residual: features, the activity mask and the judge of the learned guess
+		rc = eval_all(c, D, c->Pst, nullptr, 0.0, nullptr, c->G, ok0, res0); if (rc) return rc;
+		have_G = true;
 		HIPCHK(hipMemcpyAsync(c->Pplain, c->Pst, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+		HIPCHK(hipMemcpyAsync(c->Gplain, c->G, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 		mlp_feat_cell(D, c->Pplain, c->F, c->st);
 	}
-	if (c->cfg.learned_guess && c->model) {
+	const double thr = ok0 ? GUESS_ACTIVE * res0 : 0.0;
+	if (learned && !(ok0 && res0 < c->cfg.newton_tol)) {
 		// the plain guess judges the learned one: keep whichever has the smaller Newton residual norm
-		rc = eval_all(c, D, c->Pst, nullptr, 0.0, nullptr, c->G, ok0, res0); if (rc) return rc;
 		const spb6_mlp* M = c->model;
 		MlpDev W{M->P, M->W1t, M->W2t, M->mean, M->istd, M->b3, M->mu_t, M->sd_t};
-		mlp_infer(&D, c->F, adt, (int)n, nullptr, W, c->Pplain, c->Pn, nullptr, c->st);
+		mlp_infer(&D, c->F, adt, (int)n, nullptr, W, c->Pplain, ok0 ? c->Gplain : nullptr, thr, c->Pn, nullptr, c->st);
 		bool okl; double resl;
 		rc = eval_all(c, D, c->Pn, nullptr, 0.0, nullptr, c->Gn, okl, resl); if (rc) return rc;
 		if (okl && (!ok0 || resl < res0)) {
@@ -335,13 +343,13 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 			std::swap(c->Pst, c->Pn); std::swap(c->G, c->Gn); ok0 = true; res0 = resl; c->stats.guess_used++;
 		}
 		else c->stats.guess_fallback++;
-		have_G = true;
 	}
+	else if (learned) c->stats.guess_fallback++;   // the plain guess has already converged
 	rc = newton(c, D, iters, res, have_G, ok0, res0);
 	if (rc != SPB6_OK) return rc;
 	if (dumping) {
 		const int off = (int)((c->dseq++ * 2654435761u) % (unsigned)c->dstride);
-		mlp_dump(D, c->F, adt, c->Pst, c->Pplain, off, c->dstride, c->dcap, c->dX, c->dY, c->dcount, c->st);
+		mlp_dump(D, c->F, adt, c->Pst, c->Pplain, c->Gplain, thr, off, c->dstride, c->dcap, c->dX, c->dY, c->dcount, c->st);
 	}
 	c->stats.maxres = std::max(c->stats.maxres, res);
 	rc = reset_scalars(c); if (rc) return rc;
@@ -448,7 +456,7 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (hipMalloc((void**)&c->nrm, 8 * 64 * sizeof(unsigned long long)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->nrm);
 	if (hipMalloc((void**)&c->flags, NFLAGS * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->flags);
 	if (hipHostMalloc((void**)&c->hs, sizeof(HostScratch), hipHostMallocDefault) != hipSuccess) return fail("host alloc");
-	if (dalloc(c, &c->Pplain, n)) return fail("alloc");
+	if (dalloc(c, &c->Pplain, n) || dalloc(c, &c->Gplain, n)) return fail("alloc");
 	if (hipMalloc((void**)&c->F, 7 * NC * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->F);
 	if (hipDeviceSynchronize() != hipSuccess) return fail("sync");
 	return c;
@@ -873,7 +881,7 @@ int spb6_mlp_infer_bf16(spb6_mlp* m, const float* X, int n, float* y) {
 	int rc = mlp_batch_bufs(m, n); if (rc) return rc;
 	HIPCHK(hipMemcpyAsync(m->Xb, X, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice, m->st));
 	MlpDev W{m->P, m->W1t, m->W2t, m->mean, m->istd, m->b3, m->mu_t, m->sd_t};
-	mlp_infer(nullptr, nullptr, 0.0, n, m->Xb, W, nullptr, nullptr, m->yb, m->st);
+	mlp_infer(nullptr, nullptr, 0.0, n, m->Xb, W, nullptr, nullptr, 0.0, nullptr, m->yb, m->st);
 	HIPCHK(hipMemcpyAsync(y, m->yb, (size_t)n * 4, hipMemcpyDeviceToHost, m->st));
 	HIPCHK(hipStreamSynchronize(m->st));
 	return SPB6_OK;

@@ -17,11 +17,14 @@ struct MlpDev {
 };
 
 void mlp_feat_cell(const Dev& D, const double* Pplain, float* F, hipStream_t st);
-void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, const double* Pplain, int off, int stride, long cap,
-              float* X, float* Y, unsigned long long* count, hipStream_t st);
+void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, const double* Pplain, const double* Gplain, double thr,
+              int off, int stride, long cap, float* X, float* Y, unsigned long long* count, hipStream_t st);
 void mlp_pack(const float* params, __bf16* W1t, __bf16* W2t, hipStream_t st);
-void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain, double* Pout,
-               float* yout, hipStream_t st);
+void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
+               const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st);
+// a cell takes the network's guess (and is dumped for training) when its plain-guess residual is at least this fraction
+// of the plain residual's max norm: elsewhere the plain guess is kept, so quiescent cells are never perturbed
+constexpr double GUESS_ACTIVE = 1e-3;
 void mlp_fwd_bwd(const float* P, const float* Xb, const float* Yb, int Bn, float* ws, float* G, float* yout, float* loss, hipStream_t st);
 void mlp_adam(float* P, const float* G, float* m, float* v, float lr, int t, hipStream_t st);
 void mlp_gather(int Bn, const float* X, const float* Y, const int* idx, float* Xb, float* Yb, hipStream_t st);

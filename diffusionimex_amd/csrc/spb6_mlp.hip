@@ -48,7 +48,17 @@ __global__ void __launch_bounds__(256) k_feat_cell(Geo g, Phys P, double P_REF, 
 	F[5 * NC + c] = (float)pg;
 	F[6 * NC + c] = (float)cf;
 }
-struct FeatCtx { Geo g; const float* F; const double* Kw; const double* cz; double tau_dx; };
+struct FeatCtx {
+	Geo g; const float* F; const double* Kw; const double* cz; double tau_dx;
+	// activity mask: a cell takes part (is dumped, or takes the network's guess) iff |G_q| / (|P_REF + P_q| + K_q + 1)
+	// >= thr for the plain guess's residual G and pressure P; G == nullptr marks every cell active
+	const double* G; const double* Pplain; double P_REF; double thr;
+};
+__device__ __forceinline__ bool cell_active(const FeatCtx& X, int q) {
+	if (!X.G) return true;
+	const double K = X.Kw[(q % X.g.nx + GH) + (q / X.g.nx + GH) * X.g.NXT];
+	return fabs(X.G[q]) >= X.thr * (fabs(X.P_REF + X.Pplain[q]) + K + 1.0);
+}
 // raw feature k (0..174) of interior row q: stencil offset s = k / 7 (5x5, row-major from (-2,-2)), field f = k % 7
 __device__ __forceinline__ float feature(const FeatCtx& X, int q, int k, float Kc, float czc, float F1c, float F2c, float F3c, float F5c) {
 	const Geo& g = X.g;
@@ -79,7 +89,7 @@ __global__ void __launch_bounds__(256) k_dump(FeatCtx X, const double* __restric
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	const int q = off + t * stride;
 	__shared__ unsigned long long base;
-	const bool act = q < X.g.n;
+	const bool act = q < X.g.n && cell_active(X, q);
 	// one counter update per block reserves the block's rows
 	__shared__ int nact;
 	if (threadIdx.x == 0) nact = 0;
@@ -242,6 +252,11 @@ __global__ void __launch_bounds__(256) k_mlp_infer(FeatCtx X, int nrows, const f
 	for (int tile = blockIdx.x * 4 + wave; tile < ntiles; tile += gridDim.x * 4) {
 		const int q = tile * 16 + lr;           // this lane's A row (cell)
 		const bool valid = q < nrows;
+		// a tile none of whose cells is active keeps the plain guess and skips the network (wave-uniform branch)
+		if (mode == 0 && __ballot(valid && cell_active(X, q)) == 0ull) {
+			if (lk == 0 && valid && Pout) Pout[q] = Pplain[q];
+			continue;
+		}
 		Centre C{};
 		if (valid && mode == 0) C = centre_of(X, q);
 		// layer 1: Z1[16 x 128] = X[16 x 192] W1[192 x 128]
@@ -315,7 +330,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(FeatCtx X, int nrows, const f
 					if (Pout) {
 						const Geo& g = X.g;
 						const double Kc = X.Kw[(qq % g.nx + GH) + (qq / g.nx + GH) * g.NXT];
-						Pout[qq] = Pplain[qq] + Kc * (double)(mu_t + sd_t * y);
+						Pout[qq] = cell_active(X, qq) ? Pplain[qq] + Kc * (double)(mu_t + sd_t * y) : Pplain[qq];
 					}
 				}
 			}
@@ -329,9 +344,9 @@ void mlp_feat_cell(const Dev& D, const double* Pplain, float* F, hipStream_t st)
 	hipLaunchKernelGGL(k_feat_cell, dim3(nb(D.g.NC)), dim3(256), 0, st, D.g, D.ph, D.P_REF, (const double*)D.Ust, Pplain, (const double*)D.ust,
 	                   (const double*)D.vst, (const double*)D.cz, F);
 }
-void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, const double* Pplain, int off, int stride, long cap,
-              float* X, float* Y, unsigned long long* count, hipStream_t st) {
-	FeatCtx fx{D.g, F, D.Kw, D.cz, tau / D.g.dx};
+void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, const double* Pplain, const double* Gplain, double thr,
+              int off, int stride, long cap, float* X, float* Y, unsigned long long* count, hipStream_t st) {
+	FeatCtx fx{D.g, F, D.Kw, D.cz, tau / D.g.dx, Gplain, Pplain, D.P_REF, thr};
 	const long rows = (D.g.n - off + stride - 1) / stride;
 	if (rows <= 0) return;
 	hipLaunchKernelGGL(k_dump, dim3(nb(rows)), dim3(256), 0, st, fx, Pconv, Pplain, off, stride, cap, X, Y, count);
@@ -339,9 +354,10 @@ void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, con
 void mlp_pack(const float* params, __bf16* W1t, __bf16* W2t, hipStream_t st) {
 	hipLaunchKernelGGL(k_pack_bf16, dim3(nb(MLP_H * MLP_KPAD)), dim3(256), 0, st, params, W1t, W2t);
 }
-void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain, double* Pout, float* yout, hipStream_t st) {
+void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
+               const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st) {
 	FeatCtx fx{};
-	if (D) { fx.g = D->g; fx.F = F; fx.Kw = D->Kw; fx.cz = D->cz; fx.tau_dx = tau / D->g.dx; }
+	if (D) { fx.g = D->g; fx.F = F; fx.Kw = D->Kw; fx.cz = D->cz; fx.tau_dx = tau / D->g.dx; fx.G = Gplain; fx.Pplain = Pplain; fx.P_REF = D->P_REF; fx.thr = thr; }
 	int dev = 0; hipGetDevice(&dev);
 	int cus = 256; hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
 	const int tiles = (nrows + 15) / 16;
