@@ -36,7 +36,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
-    ap.add_argument("--forms", default="plain,mixed", help="comma list of plain, mixed (FP32 Krylov), learned (MLP guess + FP32 Krylov)")
+    ap.add_argument("--forms", default=None,
+                    help="comma list of plain, mixed (FP32 Krylov), learned (MLP guess + FP32 Krylov); default all three "
+                         "when the trained network is present, else plain,mixed")
     return ap.parse_args()
 
 
@@ -110,6 +112,9 @@ def main():
         sys.exit(2)
     import numpy as np
     n = a.nx * a.ny
+    if a.forms is None:
+        from diffusionimex_amd import mlp
+        a.forms = "plain,mixed,learned" if os.path.exists(mlp.WEIGHTS) else "plain,mixed"
     forms = [f for f in a.forms.split(",") if f]
     results, states = {}, {}
     st_plain = cfg = None

@@ -5,21 +5,21 @@
 
 namespace spb {
 
-constexpr int MLP_IN = 175, MLP_H = 128, MLP_KPAD = 192;
+constexpr int MLP_IN = 175, MLP_H = 128, MLP_KPAD = 224;   // layer-1 k padded for the MFMA slot order
 // parameter vector layout (FP32): W1[175 x 128] (row k, column n), b1[128], W2[128 x 128], b2[128], w3[128], b3
 constexpr int MLP_OFF_W1 = 0, MLP_OFF_B1 = MLP_IN * MLP_H, MLP_OFF_W2 = MLP_OFF_B1 + MLP_H, MLP_OFF_B2 = MLP_OFF_W2 + MLP_H * MLP_H,
               MLP_OFF_W3 = MLP_OFF_B2 + MLP_H, MLP_OFF_B3 = MLP_OFF_W3 + MLP_H, MLP_NPARAM = MLP_OFF_B3 + 1;
 
 // device copy of a trained network for inference
 struct MlpDev {
-	const float* params; const __bf16* W1t; const __bf16* W2t; const float* mean; const float* istd;
+	const float* params; const __bf16* W1t; const __bf16* W2t; const float* smean; const float* sistd;   // per MFMA slot
 	float b3, mu_t, sd_t;
 };
 
 void mlp_feat_cell(const Dev& D, const double* Pplain, float* F, hipStream_t st);
 void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, const double* Pplain, const double* Gplain, double thr,
               int off, int stride, long cap, float* X, float* Y, unsigned long long* count, hipStream_t st);
-void mlp_pack(const float* params, __bf16* W1t, __bf16* W2t, hipStream_t st);
+void mlp_pack(const float* params, const float* mean, const float* istd, __bf16* W1t, __bf16* W2t, float* smean, float* sistd, hipStream_t st);
 void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
                const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st);
 // a cell takes the network's guess (and is dumped for training) when its plain-guess residual is at least this fraction

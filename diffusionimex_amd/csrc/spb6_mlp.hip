@@ -24,10 +24,11 @@ __device__ __forceinline__ int mapP(const Geo& g, int i, int j) {   // M_P (PC:6
 	else J = j;
 	return (I - GH) + g.nx * (J - GH);
 }
-// per-cell raw fields over the ghosted grid: F[f*NC + c], f = alpha1, log rho, u*, v*, pbar(X*) - P_REF, plain P, c_frozen
+// per-cell raw fields over the ghosted grid: F[f*NC + c], f = alpha1, log rho, u*, v*, pbar(X*) - P_REF, plain P,
+// log(1 + tau c_frozen / dx)
 __global__ void __launch_bounds__(256) k_feat_cell(Geo g, Phys P, double P_REF, const double* __restrict__ Xs, const double* __restrict__ Pg,
                                                    const double* __restrict__ ust, const double* __restrict__ vst, const double* __restrict__ cz,
-                                                   float* __restrict__ F) {
+                                                   float tau_dx, float* __restrict__ F) {
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c >= g.NC) return;
 	const int NC = g.NC, i = c % g.NXT, j = c / g.NXT;
@@ -46,7 +47,7 @@ __global__ void __launch_bounds__(256) k_feat_cell(Geo g, Phys P, double P_REF, 
 	F[3 * NC + c] = (float)vst[c];
 	F[4 * NC + c] = (float)pb;
 	F[5 * NC + c] = (float)pg;
-	F[6 * NC + c] = (float)cf;
+	F[6 * NC + c] = log1pf(tau_dx * (float)cf);   // acoustic CFL of the cell, log-compressed
 }
 struct FeatCtx {
 	Geo g; const float* F; const double* Kw; const double* cz; double tau_dx;
@@ -59,29 +60,33 @@ __device__ __forceinline__ bool cell_active(const FeatCtx& X, int q) {
 	const double K = X.Kw[(q % X.g.nx + GH) + (q / X.g.nx + GH) * X.g.NXT];
 	return fabs(X.G[q]) >= X.thr * (fabs(X.P_REF + X.Pplain[q]) + K + 1.0);
 }
-// raw feature k (0..174) of interior row q: stencil offset s = k / 7 (5x5, row-major from (-2,-2)), field f = k % 7
-__device__ __forceinline__ float feature(const FeatCtx& X, int q, int k, float Kc, float czc, float F1c, float F2c, float F3c, float F5c) {
-	const Geo& g = X.g;
-	const int s = k / 7, f = k - 7 * s, dxo = s % 5 - 2, dyo = s / 5 - 2;
-	const int c = (q % g.nx + GH + dxo) + (q / g.nx + GH + dyo) * g.NXT;
-	const float* F = X.F; const int NC = g.NC;
-	switch (f) {
-	case 0: return F[c];
-	case 1: return F[NC + c] - F1c;
-	case 2: return (F[2 * NC + c] - F2c) / czc;
-	case 3: return (F[3 * NC + c] - F3c) / czc;
-	case 4: return (F[4 * NC + c] - F5c) / Kc;
-	case 5: return (F[5 * NC + c] - F5c) / Kc;
-	default: return log1pf((float)X.tau_dx * F[6 * NC + c]);
-	}
-}
-struct Centre { float Kc, czc, F1c, F2c, F3c, F5c; };
+// centre-cell values a row's features are taken relative to
+struct Centre { float iK, icz, F1c, F2c, F3c, F5c; };
 __device__ __forceinline__ Centre centre_of(const FeatCtx& X, int q) {
 	const Geo& g = X.g; const int c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT, NC = g.NC;
 	Centre C;
-	C.Kc = (float)X.Kw[c]; const double cz = X.cz[c]; C.czc = (float)(isfin(cz) && cz > 0 ? cz : 1.0);
+	C.iK = 1.f / (float)X.Kw[c]; const double cz = X.cz[c]; C.icz = 1.f / (float)(isfin(cz) && cz > 0 ? cz : 1.0);
 	C.F1c = X.F[NC + c]; C.F2c = X.F[2 * NC + c]; C.F3c = X.F[3 * NC + c]; C.F5c = X.F[5 * NC + c];
 	return C;
+}
+// feature of field f from the field value v at a stencil cell: alpha1 and log(1 + tau c / dx) as they are, log rho
+// relative to the centre, velocities relative to the centre in units of the centre's frozen sound speed, pressures
+// relative to the centre's plain guess in units of the centre's K_w
+__device__ __forceinline__ float feat_tr(int f, float v, const Centre& C) {
+	switch (f) {
+	case 1: return v - C.F1c;
+	case 2: return (v - C.F2c) * C.icz;
+	case 3: return (v - C.F3c) * C.icz;
+	case 4: case 5: return (v - C.F5c) * C.iK;
+	default: return v;
+	}
+}
+// raw feature k (0..174) of interior row q: stencil offset s = k / 7 (5x5, row-major from (-2,-2)), field f = k % 7
+__device__ __forceinline__ float feature(const FeatCtx& X, int q, int k, const Centre& C) {
+	const Geo& g = X.g;
+	const int s = k / 7, f = k - 7 * s, dxo = s % 5 - 2, dyo = s / 5 - 2;
+	const int c = (q % g.nx + GH + dxo) + (q / g.nx + GH + dyo) * g.NXT;
+	return feat_tr(f, X.F[f * g.NC + c], C);
 }
 // training dump: raw features of interior rows q = off, off + stride, ... and the target (P_conv - P_plain) / K_w
 __global__ void __launch_bounds__(256) k_dump(FeatCtx X, const double* __restrict__ Pconv, const double* __restrict__ Pplain, int off, int stride,
@@ -103,7 +108,7 @@ __global__ void __launch_bounds__(256) k_dump(FeatCtx X, const double* __restric
 	const long row = (long)base + slot;
 	if (row >= cap) return;
 	const Centre C = centre_of(X, q);
-	for (int k = 0; k < MLP_IN; k++) Xout[row * MLP_IN + k] = feature(X, q, k, C.Kc, C.czc, C.F1c, C.F2c, C.F3c, C.F5c);
+	for (int k = 0; k < MLP_IN; k++) Xout[row * MLP_IN + k] = feature(X, q, k, C);
 	Yout[row] = (float)((Pconv[q] - Pplain[q]) / X.Kw[(q % X.g.nx + GH) + (q / X.g.nx + GH) * X.g.NXT]);
 }
 
@@ -218,12 +223,22 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 union BF8 { bf16x8 v; __bf16 e[8]; uint4 u; };
 
-// W1t[n][k] (k padded to 192) and W2t[n][k]: transposed BF16 weights so one lane's 8 consecutive k are one 16-byte read
-__global__ void __launch_bounds__(256) k_pack_bf16(const float* __restrict__ params, __bf16* __restrict__ W1t, __bf16* __restrict__ W2t) {
+// MFMA k order of layer 1: slot t = 32 ks + 8 lk + e (ks = k step, lk = lane / 16, e = element of the lane's 8) holds
+// field f = (8 ks + e) / 7 at stencil point s = 4 ((8 ks + e) % 7) + lk (s < 25; f = 7 and s >= 25 are zero padding).
+// The field is then a compile-time constant of the unrolled (ks, e) loops and only the stencil offset varies by lane.
+__host__ __device__ constexpr int slot_feature(int t) {
+	const int ks = t / 32, lk = (t / 8) % 4, e = t % 8, p = 8 * ks + e, f = p / 7, s = 4 * (p % 7) + lk;
+	return (f < 7 && s < 25) ? 7 * s + f : -1;
+}
+// W1t[n][t] (t in the slot order above) and W2t[n][k]: transposed BF16 weights, one lane's 8 consecutive k = 16 bytes;
+// smean / sistd: the standardisation of each slot (0 for padding)
+__global__ void __launch_bounds__(256) k_pack_bf16(const float* __restrict__ params, const float* __restrict__ mean, const float* __restrict__ istd,
+                                                   __bf16* __restrict__ W1t, __bf16* __restrict__ W2t, float* __restrict__ smean, float* __restrict__ sistd) {
 	const int e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e < MLP_H * MLP_KPAD) {
-		const int n = e / MLP_KPAD, k = e % MLP_KPAD;
-		W1t[e] = (__bf16)(k < MLP_IN ? params[MLP_OFF_W1 + k * MLP_H + n] : 0.f);
+		const int n = e / MLP_KPAD, t = e % MLP_KPAD, k = slot_feature(t);
+		W1t[e] = (__bf16)(k >= 0 ? params[MLP_OFF_W1 + k * MLP_H + n] : 0.f);
+		if (n == 0) { smean[t] = k >= 0 ? mean[k] : 0.f; sistd[t] = k >= 0 ? istd[k] : 0.f; }
 	}
 	if (e < MLP_H * MLP_H) {
 		const int n = e / MLP_H, k = e % MLP_H;
@@ -232,24 +247,26 @@ __global__ void __launch_bounds__(256) k_pack_bf16(const float* __restrict__ par
 }
 // MFMA 16x16x32 operand layout: lane l holds A rows m = l % 16 and k = 8 (l / 16) .. +7 of a 32-wide k step, B columns
 // n = l % 16 with the same k; the 16x16 FP32 result gives lane l column n = l % 16, rows 4 (l / 16) .. +3.
-// mode 0: features from the stage (guess); mode 1: standardised features given in Xin[rows][175] (tests)
-__global__ void __launch_bounds__(256) k_mlp_infer(FeatCtx X, int nrows, const float* __restrict__ Xin, int mode,
-                                                   const float* __restrict__ mean, const float* __restrict__ istd,
+// Persistent blocks of 8 waves share the BF16 weights in LDS; each wave takes 16-row tiles. mode 0: features from the
+// stage (guess); mode 1: standardised features given in Xin[rows][175] (tests).
+constexpr int INF_WAVES = 8;
+__global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(FeatCtx X, int nrows, const float* __restrict__ Xin, int mode,
+                                                   const float* __restrict__ smean, const float* __restrict__ sistd,
                                                    const __bf16* __restrict__ W1t, const float* __restrict__ b1,
                                                    const __bf16* __restrict__ W2t, const float* __restrict__ b2,
                                                    const float* __restrict__ w3, float b3, float mu_t, float sd_t,
                                                    const double* __restrict__ Pplain, double* __restrict__ Pout, float* __restrict__ yout) {
-	__shared__ uint4 sW1[MLP_H * MLP_KPAD / 8];     // 48 KB
+	__shared__ uint4 sW1[MLP_H * MLP_KPAD / 8];     // 56 KB
 	__shared__ uint4 sW2[MLP_H * MLP_H / 8];        // 32 KB
-	__shared__ __bf16 sH[4][16][MLP_H + 8];         // per-wave hidden activations (A layout source)
-	__shared__ float sMean[MLP_IN], sIstd[MLP_IN];
-	for (int e = threadIdx.x; e < MLP_H * MLP_KPAD / 8; e += 256) sW1[e] = ((const uint4*)W1t)[e];
-	for (int e = threadIdx.x; e < MLP_H * MLP_H / 8; e += 256) sW2[e] = ((const uint4*)W2t)[e];
-	for (int e = threadIdx.x; e < MLP_IN; e += 256) { sMean[e] = mean[e]; sIstd[e] = istd[e]; }
+	__shared__ __bf16 sH[INF_WAVES][16][MLP_H + 8];  // per-wave hidden activations (A layout source), 34 KB
+	__shared__ float sMean[MLP_KPAD], sIstd[MLP_KPAD];
+	for (int e = threadIdx.x; e < MLP_H * MLP_KPAD / 8; e += 64 * INF_WAVES) sW1[e] = ((const uint4*)W1t)[e];
+	for (int e = threadIdx.x; e < MLP_H * MLP_H / 8; e += 64 * INF_WAVES) sW2[e] = ((const uint4*)W2t)[e];
+	for (int e = threadIdx.x; e < MLP_KPAD; e += 64 * INF_WAVES) { sMean[e] = smean[e]; sIstd[e] = sistd[e]; }
 	__syncthreads();
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
 	const int ntiles = (nrows + 15) / 16;
-	for (int tile = blockIdx.x * 4 + wave; tile < ntiles; tile += gridDim.x * 4) {
+	for (int tile = blockIdx.x * INF_WAVES + wave; tile < ntiles; tile += gridDim.x * INF_WAVES) {
 		const int q = tile * 16 + lr;           // this lane's A row (cell)
 		const bool valid = q < nrows;
 		// a tile none of whose cells is active keeps the plain guess and skips the network (wave-uniform branch)
@@ -258,8 +275,9 @@ __global__ void __launch_bounds__(256) k_mlp_infer(FeatCtx X, int nrows, const f
 			continue;
 		}
 		Centre C{};
-		if (valid && mode == 0) C = centre_of(X, q);
-		// layer 1: Z1[16 x 128] = X[16 x 192] W1[192 x 128]
+		int cq = 0;                              // ghosted index of the row's cell
+		if (valid && mode == 0) { C = centre_of(X, q); cq = (q % X.g.nx + GH) + (q / X.g.nx + GH) * X.g.NXT; }
+		// layer 1: Z1[16 x 128] = X[16 x 224] W1[224 x 128]
 		f32x4 acc1[8];
 		#pragma unroll
 		for (int nt = 0; nt < 8; nt++) acc1[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -268,11 +286,13 @@ __global__ void __launch_bounds__(256) k_mlp_infer(FeatCtx X, int nrows, const f
 			BF8 a;
 			#pragma unroll
 			for (int e = 0; e < 8; e++) {
-				const int k = ks * 32 + lk * 8 + e;
+				const int p = 8 * ks + e, f = p / 7, sp = 4 * (p % 7) + lk, t = ks * 32 + lk * 8 + e;
 				float x = 0.f;
-				if (valid && k < MLP_IN) {
-					const float raw = mode == 0 ? feature(X, q, k, C.Kc, C.czc, C.F1c, C.F2c, C.F3c, C.F5c) : Xin[(long)q * MLP_IN + k];
-					x = mode == 0 ? (raw - sMean[k]) * sIstd[k] : raw;
+				if (f < 7 && sp < 25 && valid) {
+					if (mode == 0) {
+						const int c = cq + (sp % 5 - 2) + (sp / 5 - 2) * X.g.NXT;
+						x = (feat_tr(f, X.F[f * X.g.NC + c], C) - sMean[t]) * sIstd[t];
+					} else x = Xin[(long)q * MLP_IN + 7 * sp + f];
 				}
 				a.e[e] = (__bf16)x;
 			}
@@ -342,7 +362,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(FeatCtx X, int nrows, const f
 static inline unsigned nb(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 void mlp_feat_cell(const Dev& D, const double* Pplain, float* F, hipStream_t st) {
 	hipLaunchKernelGGL(k_feat_cell, dim3(nb(D.g.NC)), dim3(256), 0, st, D.g, D.ph, D.P_REF, (const double*)D.Ust, Pplain, (const double*)D.ust,
-	                   (const double*)D.vst, (const double*)D.cz, F);
+	                   (const double*)D.vst, (const double*)D.cz, (float)(D.tau / D.g.dx), F);
 }
 void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, const double* Pplain, const double* Gplain, double thr,
               int off, int stride, long cap, float* X, float* Y, unsigned long long* count, hipStream_t st) {
@@ -351,8 +371,8 @@ void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, con
 	if (rows <= 0) return;
 	hipLaunchKernelGGL(k_dump, dim3(nb(rows)), dim3(256), 0, st, fx, Pconv, Pplain, off, stride, cap, X, Y, count);
 }
-void mlp_pack(const float* params, __bf16* W1t, __bf16* W2t, hipStream_t st) {
-	hipLaunchKernelGGL(k_pack_bf16, dim3(nb(MLP_H * MLP_KPAD)), dim3(256), 0, st, params, W1t, W2t);
+void mlp_pack(const float* params, const float* mean, const float* istd, __bf16* W1t, __bf16* W2t, float* smean, float* sistd, hipStream_t st) {
+	hipLaunchKernelGGL(k_pack_bf16, dim3(nb(MLP_H * MLP_KPAD)), dim3(256), 0, st, params, mean, istd, W1t, W2t, smean, sistd);
 }
 void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
                const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st) {
@@ -361,9 +381,9 @@ void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float*
 	int dev = 0; hipGetDevice(&dev);
 	int cus = 256; hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
 	const int tiles = (nrows + 15) / 16;
-	const int blocks = std::max(1, std::min(cus, (tiles + 3) / 4));
-	hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, st, fx, nrows, Xin, Xin ? 1 : 0, W.mean, W.istd, W.W1t, W.params + MLP_OFF_B1,
-	                   W.W2t, W.params + MLP_OFF_B2, W.params + MLP_OFF_W3, W.b3, W.mu_t, W.sd_t, Pplain, Pout, yout);
+	const int blocks = std::max(1, std::min(cus, (tiles + INF_WAVES - 1) / INF_WAVES));
+	hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * INF_WAVES), 0, st, fx, nrows, Xin, Xin ? 1 : 0, W.smean, W.sistd, W.W1t,
+	                   W.params + MLP_OFF_B1, W.W2t, W.params + MLP_OFF_B2, W.params + MLP_OFF_W3, W.b3, W.mu_t, W.sd_t, Pplain, Pout, yout);
 }
 
 // forward (and optionally backward) of the FP32 network on a batch Xb[Bn x 175]; grads into G (same layout as params)
