@@ -138,3 +138,30 @@ def perturbed_interface(nx=4096, ny=2048, seed=0, cfl=0.5, **kw):
     U = prim2cell(ph, a1, 1000.0, np.where(post, r1, r0), np.where(post, u1, 0.0), 0.0,
                   np.where(post, p1, p0), np.where(post, p1, p0))
     return cfg, U
+
+
+def riemann_batch(n=4096, seed=0, lmin=256, lmax=8192, cfl=0.5, newton_tol=1e-10, **kw):
+    """Batched 1D two-phase Riemann problems on [0, 1] m, transmissive: lengths uniform in [lmin, lmax] cells; interface
+    position uniform in [0.3, 0.7]; liquid left (alpha_liquid = 1 - 1e-6, rho 1000, p_L) and gas right (alpha_liquid =
+    1e-6, rho 50, p_R), u = 0; p_L, p_R log-uniform in [1e5, 1e9] Pa; liquid stiffened gas gamma ~ U[2, 6],
+    p_inf ~ U[3e8, 8e8] Pa; gas gamma 1.4. Returns (scheme Config, [BatchProblem], [U_p[nx_p, 8]], draws)."""
+    from .batch import problem
+    rng = SplitMix64(seed)
+    probs, Us, draws = [], [], []
+    for _ in range(n):
+        nx = lmin + int(rng.uniform(0.0, lmax - lmin + 1))
+        nx = min(nx, lmax)
+        xi = rng.uniform(0.3, 0.7)
+        pL = 10.0 ** rng.uniform(5.0, 9.0)
+        pR = 10.0 ** rng.uniform(5.0, 9.0)
+        gl = rng.uniform(2.0, 6.0)
+        pil = rng.uniform(3e8, 8e8)
+        ph = ((gl, pil), (1.4, 0.0))
+        x = (np.arange(nx) + 0.5) / nx
+        L = x < xi
+        U = prim2cell(ph, np.where(L, 1 - 1e-6, 1e-6), 1000.0, 50.0, 0.0, 0.0, np.where(L, pL, pR), np.where(L, pL, pR))
+        probs.append(problem(nx, ph))
+        Us.append(U.reshape(nx, 8))
+        draws.append({"nx": nx, "x_interface": xi, "pL": pL, "pR": pR, "gamma_liquid": gl, "pinf_liquid": pil})
+    scheme = make_config(2, 1, cfl=cfl, newton_tol=newton_tol, **kw)
+    return scheme, probs, Us, draws

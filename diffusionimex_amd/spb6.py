@@ -57,7 +57,10 @@ EXPORTS = ["spb6_create", "spb6_destroy", "spb6_last_error", "spb6_set_state", "
            "spb6_jacobian", "spb6_linsolve", "spb6_get_field", "spb6_enable_timing", "spb6_get_kernel_times",
            "spb6_mlp_create", "spb6_mlp_destroy", "spb6_mlp_nparams", "spb6_mlp_set_params", "spb6_mlp_get_params",
            "spb6_mlp_set_data", "spb6_mlp_train", "spb6_mlp_forward", "spb6_mlp_gradients", "spb6_mlp_infer_bf16",
-           "spb6_set_guess_model", "spb6_dump_enable", "spb6_dump_fetch"]
+           "spb6_set_guess_model", "spb6_dump_enable", "spb6_dump_fetch",
+           "spb6_batch_create", "spb6_batch_destroy", "spb6_batch_last_error", "spb6_batch_cells", "spb6_batch_set_state",
+           "spb6_batch_get_state", "spb6_batch_get_pressure", "spb6_batch_run", "spb6_batch_get_stats",
+           "spb6_batch_set_launch", "spb6_batch_autotune"]
 
 _lib = None
 
@@ -111,6 +114,20 @@ def load():
     L.spb6_dump_enable.argtypes = [vp, C.c_long, C.c_int]
     L.spb6_dump_fetch.restype = C.c_long
     L.spb6_dump_fetch.argtypes = [vp, fp, fp, C.c_long]
+    L.spb6_batch_create.restype = vp
+    L.spb6_batch_create.argtypes = [vp, C.c_int, C.POINTER(Config), C.c_int]
+    L.spb6_batch_destroy.argtypes = [vp]
+    L.spb6_batch_last_error.restype = C.c_char_p
+    L.spb6_batch_last_error.argtypes = [vp]
+    L.spb6_batch_cells.restype = C.c_long
+    L.spb6_batch_cells.argtypes = [vp]
+    L.spb6_batch_set_state.argtypes = [vp, dp]
+    L.spb6_batch_get_state.argtypes = [vp, dp]
+    L.spb6_batch_get_pressure.argtypes = [vp, dp, dp]
+    L.spb6_batch_run.argtypes = [vp, C.c_double, C.c_long]
+    L.spb6_batch_get_stats.argtypes = [vp, C.POINTER(Stats), C.POINTER(Stats)]
+    L.spb6_batch_set_launch.argtypes = [vp, C.c_int, C.c_int]
+    L.spb6_batch_autotune.argtypes = [vp, C.c_long, vp]
     _lib = L
     return L
 

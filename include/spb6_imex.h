@@ -165,6 +165,51 @@ int spb6_set_guess_model(spb6_ctx* ctx, spb6_mlp* m);
 int spb6_dump_enable(spb6_ctx* ctx, long capacity, int stride);
 long spb6_dump_fetch(spb6_ctx* ctx, float* X, float* Y, long max_rows);
 
+/* ---- batched 1D problems: many independent runs of the reference's 1D path (main's time loop PC:1230-1296 with
+ * PressureSubsystem PC:851-993 and its Thomas solve PC:807-812, one run per problem, each with its own length, domain,
+ * boundaries and phases) advanced together on one GPU. Each problem keeps its own dt, start-up ramp, first-order ramp
+ * steps and reject-and-halve loop and gives the bits of a single-problem run of the same configuration. Scheme settings
+ * (rk, CFL, ramp, tolerances, limiter, order) are shared and taken from a spb6_config_t whose grid, boundary and phase
+ * fields are ignored. Periodic boundaries are not accepted (the reference solves periodic 1D rows with BiCGStab). */
+typedef struct spb6_batch spb6_batch;
+typedef struct spb6_batch_problem {
+    int nx;                      /* interior cells */
+    double x0, x1;               /* domain */
+    int bc[2];                   /* left, right: SPB6_BC_TRANSMISSIVE or SPB6_BC_WALL (BCS[0..1], PHY:908) */
+    double bcu[2];               /* wall normal velocity (BCU[0..1]) */
+    spb6_phase_t ph[2];          /* the problem's phases (PH[2], PHY:54) */
+} spb6_batch_problem_t;
+/* launch shapes tried by spb6_batch_autotune and the one kept (no counterpart in the reference) */
+typedef struct spb6_batch_tune {
+    int threads, per_cu;         /* workgroup size (64, 128, 256) and persistent workgroups per compute unit */
+    double best_ms;              /* wall time of the trial at the kept shape */
+    int ntried; int tried_threads[16], tried_per_cu[16]; double tried_ms[16];
+} spb6_batch_tune_t;
+
+/* Allocate the batch (every device buffer, once); problems keep their index order at the boundary. */
+spb6_batch* spb6_batch_create(const spb6_batch_problem_t* problems, int nproblems, const spb6_config_t* scheme, int device);
+void spb6_batch_destroy(spb6_batch* batch);
+const char* spb6_batch_last_error(spb6_batch* batch);
+/* total interior cells (sum of nx) */
+long spb6_batch_cells(spb6_batch* batch);
+/* U: the problems' AoS states back to back (sum nx * 8 doubles); per problem P_REF = min pbar and P = pbar - P_REF
+ * (PC:1222-1227), t = 0, start-up ramp armed, counters cleared */
+int spb6_batch_set_state(spb6_batch* batch, const double* U);
+int spb6_batch_get_state(spb6_batch* batch, double* U);
+/* P: sum nx doubles; pref: one per problem (may be NULL) */
+int spb6_batch_get_pressure(spb6_batch* batch, double* P, double* pref);
+/* Advance every problem by `steps` accepted steps or to t_end, whichever comes first (spb6_run per problem). Returns
+ * SPB6_STEP_FAILED if any problem failed (its status is in its stats; the others still ran). */
+int spb6_batch_run(spb6_batch* batch, double t_end, long steps);
+/* per-problem counters (array of nproblems, may be NULL) and their totals (may be NULL) */
+int spb6_batch_get_stats(spb6_batch* batch, spb6_stats_t* per_problem, spb6_stats_t* total);
+/* launch shape: threads per workgroup (64, 128, 256) and persistent workgroups per compute unit (1..16); results do
+ * not depend on it (no counterpart in the reference) */
+int spb6_batch_set_launch(spb6_batch* batch, int threads, int per_cu);
+/* time `steps` steps at each candidate shape on a snapshot of the current state, restore the state, keep the fastest
+ * shape (no counterpart in the reference) */
+int spb6_batch_autotune(spb6_batch* batch, long steps, spb6_batch_tune_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,21 +32,25 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_python_structs_match_header_sizes():
-    """ctypes mirrors of spb6_config_t / spb6_stats_t have the C layout (checked against a compiled probe)."""
+    """ctypes mirrors of the header structs have the C layout (checked against a compiled probe)."""
     from diffusionimex_amd import spb6
     src = os.path.join(ROOT, "include", "spb6_imex.h")
     import subprocess
     import tempfile
     with tempfile.TemporaryDirectory() as d:
         c = os.path.join(d, "p.c")
-        open(c, "w").write('#include "%s"\n#include <stdio.h>\nint main(){printf("%%zu %%zu %%zu\\n", sizeof(spb6_config_t), '
-                           'sizeof(spb6_stats_t), sizeof(spb6_kernel_times_t));return 0;}\n' % src)
+        open(c, "w").write('#include "%s"\n#include <stdio.h>\nint main(){printf("%%zu %%zu %%zu %%zu %%zu\\n", sizeof(spb6_config_t), '
+                           'sizeof(spb6_stats_t), sizeof(spb6_kernel_times_t), sizeof(spb6_batch_problem_t), '
+                           'sizeof(spb6_batch_tune_t));return 0;}\n' % src)
         exe = os.path.join(d, "p")
         subprocess.run(["gcc", c, "-o", exe], check=True)
         out = subprocess.run([exe], stdout=subprocess.PIPE, check=True).stdout.decode().split()
     assert int(out[0]) == ctypes.sizeof(spb6.Config)
     assert int(out[1]) == ctypes.sizeof(spb6.Stats)
     assert int(out[2]) == ctypes.sizeof(spb6.KernelTimes)
+    from diffusionimex_amd import batch
+    assert int(out[3]) == ctypes.sizeof(batch.BatchProblem)
+    assert int(out[4]) == ctypes.sizeof(batch.Tune)
 
 
 def test_missing_library_fails_loudly(monkeypatch):
