@@ -39,6 +39,8 @@ struct BProb {                 // immutable description of one problem
 struct BState {                // the problem's time loop (device resident)
 	double t, dt, dtprev, P_REF, maxres, dt_last;
 	long steps, target, rejects, newton_it, linsolves, floor_acc, stages, rk1;
+	long nt0;                  // newton_it when the current step began
+	double cost;               // work of the last completed step: (Newton iterations + 2 stages) * cells
 	int phase;                 // 0: the next attempt starts a new step (compute dt), 1: an attempt of this step is pending
 	int ramping, startup, rk, att, par, ppar, stage_ok, status, done;
 };
@@ -286,33 +288,44 @@ __device__ void wg_problem(const BArgs& A, int s, int p, double* red, int* ired)
 			dt = ((A.t_end - S.t) < dt) ? (A.t_end - S.t) : dt;
 			__syncthreads();
 			if (threadIdx.x == 0) {
-				S.dtprev = dtprev; S.dt = dt; S.ramping = ramping; S.att = 0; S.phase = 1;
+				S.dtprev = dtprev; S.dt = dt; S.ramping = ramping; S.att = 0; S.phase = 1; S.nt0 = S.newton_it;
 				S.rk = (A.ramp && ramping && S.startup) ? 1 : A.rk;
 				if (!(dt > 1e-12 * A.t_end)) S.status = SPB6_DT_UNDERFLOW;
 			}
 			__syncthreads();
 			if (S.status != SPB6_OK) { if (threadIdx.x == 0) { Sg = S; atomicAdd(A.nfin, 1); } return; }
 		}
-		if (threadIdx.x == 0 && S.rk == 1) S.rk1++;
-		for (int q = threadIdx.x; q < v.g.n; q += T) v.Pst[q] = v.P[q];   // the attempt starts from the step's P
-		__syncthreads();
 	}
-	const int rc = wg_stage<T>(A, v, S, s, S.dt, red, ired);
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		if (rc != SPB6_OK) {   // reject and halve (PC:1286-1290)
-			S.rejects++; S.dt *= 0.5; S.att++; S.stage_ok = 0;
-			if (S.att >= 30) { S.status = SPB6_STEP_FAILED; atomicAdd(A.nfin, 1); }
-		} else if (s == 0 && S.rk == 2) S.stage_ok = 1;
-		else {                 // accept (PC:1285-1296)
-			S.par ^= 1; S.ppar ^= 1; S.stage_ok = 0; S.phase = 0;
-			S.t = S.t + S.dt; S.dtprev = S.dt; S.dt_last = S.dt; S.steps++;
-			if (!S.ramping) S.startup = 0;
-			if (S.steps >= S.target || !(S.t < A.t_end * (1 - 1e-12))) { S.done = 1; atomicAdd(A.nfin, 1); }
+	// Attempts (PC:1243-1290). A rejected attempt is retried at once in this launch with dt / 2 -- from stage 1, and on
+	// through stage 2 when this is the stage-2 launch -- so each launch pair completes one accepted step per problem.
+	int st = s;
+	for (;;) {
+		if (st == 0) {
+			if (threadIdx.x == 0 && S.rk == 1) S.rk1++;
+			for (int q = threadIdx.x; q < v.g.n; q += T) v.Pst[q] = v.P[q];   // the attempt starts from the step's P
+			__syncthreads();
 		}
-		Sg = S;
+		const int rc = wg_stage<T>(A, v, S, st, S.dt, red, ired);
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			if (rc != SPB6_OK) {   // reject and halve (PC:1286-1290)
+				S.rejects++; S.dt *= 0.5; S.att++; S.stage_ok = 0;
+				if (S.att >= 30) { S.status = SPB6_STEP_FAILED; atomicAdd(A.nfin, 1); }
+			} else if (st == 0 && S.rk == 2) S.stage_ok = 1;
+			else {                 // accept (PC:1285-1296)
+				S.par ^= 1; S.ppar ^= 1; S.stage_ok = 0; S.phase = 0;
+				S.t = S.t + S.dt; S.dtprev = S.dt; S.dt_last = S.dt; S.steps++;
+				S.cost = (double)(S.newton_it - S.nt0 + 2) * (double)B.L;
+				if (!S.ramping) S.startup = 0;
+				if (S.steps >= S.target || !(S.t < A.t_end * (1 - 1e-12))) { S.done = 1; atomicAdd(A.nfin, 1); }
+			}
+			Sg = S;
+		}
+		__syncthreads();
+		if (S.status != SPB6_OK || S.phase == 0) break;   // failed, or accepted
+		if (S.stage_ok) { if (s == 0) break; st = 1; }     // stage 1 done: stage 2 belongs to the stage-2 launch
+		else st = 0;                                       // rejected: the attempt restarts at stage 1
 	}
-	__syncthreads();
 }
 
 template <int T>
@@ -395,6 +408,7 @@ int shape_graph(spb6_batch* b, int threads, int per_cu, hipGraphExec_t& exec) {
 	return SPB6_OK;
 }
 int use_shape(spb6_batch* b, int threads, int per_cu) {
+	b->tune.threads = threads; b->tune.per_cu = per_cu;
 	for (size_t k = 0; k < b->shapes.size(); k++)
 		if (b->shapes[k].threads == threads && b->shapes[k].per_cu == per_cu) { b->cur = (int)k; return SPB6_OK; }
 	hipGraphExec_t ex = nullptr;
@@ -432,14 +446,23 @@ int run_steps(spb6_batch* b, double t_end, long steps) {
 	}
 	b->graph_t_end = t_end;
 	const int np = (int)b->hprob.size();
-	const int per_check = 8;
+	const int per_check = 4;
 	long replays = 0;
+	std::vector<std::pair<double, int>> key(np);
 	while (fin < np) {
 		for (int k = 0; k < per_check; k++) BCHK(hipGraphLaunch(b->shapes[b->cur].exec, b->st));
 		replays += per_check;
 		BCHK(hipMemcpyAsync(b->hnfin, b->A.nfin, sizeof(int), hipMemcpyDeviceToHost, b->st));
 		BCHK(hipStreamSynchronize(b->st));
 		fin = *b->hnfin;
+		if (fin >= np) break;
+		// longest-processing-time-first on the measured cost of each problem's last step (the queue order is read by
+		// the captured launches from device memory, so no re-capture is needed)
+		s = read_states(b);
+		for (int p = 0; p < np; p++) key[p] = {(s[p].done || s[p].status != SPB6_OK) ? -1.0 : (s[p].cost > 0 ? s[p].cost : (double)b->hprob[p].L), p};
+		std::stable_sort(key.begin(), key.end(), [](const std::pair<double, int>& a, const std::pair<double, int>& c) { return a.first > c.first; });
+		for (int p = 0; p < np; p++) b->horder[p] = key[p].second;
+		BCHK(hipMemcpyAsync((void*)b->A.order, b->horder.data(), np * sizeof(int), hipMemcpyHostToDevice, b->st));
 		if (replays > 64 * (steps + 64)) return bfail(b, "batch run did not finish");   // every step takes at most 30 attempts
 	}
 	return SPB6_OK;
