@@ -16,6 +16,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -36,6 +38,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
+    ap.add_argument("--batch-steps", type=int, default=8, help="timed steps of the batched 1D leg (0: skip the leg)")
+    ap.add_argument("--batch-warmup", type=int, default=12)
+    ap.add_argument("--batch-cpu-problems", type=int, default=64)
     ap.add_argument("--forms", default=None,
                     help="comma list of plain, mixed (FP32 Krylov), learned (MLP guess + FP32 Krylov); default all three "
                          "when the trained network is present, else plain,mixed")
@@ -156,7 +161,81 @@ def main():
         if st_plain is None:
             _, st_plain, cfg = run_form(a, "plain")
         out["cpu_baseline"] = cpu_baseline(a, cfg, st_plain)
+    if a.batch_steps > 0 and not a.profile:
+        if st_plain is not None:
+            st_plain.close()
+        out["batch"] = run_batch(a)
     print(json.dumps(out))
+
+
+def run_batch(a):
+    """Batched 1D leg: 4096 two-phase Riemann problems (lengths 256..8192, per-problem stiffened-gas liquid, p_L, p_R
+    log-uniform in [1e5, 1e9] Pa) advanced by one persistent launch per IMEX stage, replayed as a hipGraph, with the
+    autotuned launch shape; cell-updates/s over the timed steps after the warm-up, and the oracle on the host cores over
+    a sample of the problems resumed from the same warmed state."""
+    from diffusionimex_amd import cases
+    from diffusionimex_amd.batch import Batch
+    scheme, probs, Us, _ = cases.riemann_batch(4096, seed=a.seed)
+    b = Batch(probs, scheme)
+    b.set_state(Us)
+    if b.run(1.0, a.batch_warmup) != 0:
+        raise SystemExit("batch warm-up failed")
+    tune = b.autotune(steps=4)
+    per0, tot0 = b.stats()
+    t0 = time.perf_counter()
+    rc = b.run(1.0, a.batch_steps)
+    wall = time.perf_counter() - t0
+    per1, tot1 = b.stats()
+    if rc != 0:
+        raise SystemExit("batch timed run failed")
+    cells = b.cells
+    d = {k: tot1[k] - tot0[k] for k in ("steps", "stages", "newton_it", "rejects")}
+    res = {"workload": f"{len(probs)} batched 1D Riemann problems, {cells} cells (lengths 256..8192), Newton tol 1e-10, CFL 0.5",
+           "metric": "cell-updates/s", "value": cells * a.batch_steps / wall, "steps": a.batch_steps, "warmup": a.batch_warmup,
+           "ms_per_step": 1e3 * wall / a.batch_steps, "newton_per_stage": d["newton_it"] / max(d["stages"], 1),
+           "rejects": d["rejects"], "launch": {"threads": tune["threads"], "per_cu": tune["per_cu"]},
+           "autotune_ms": {f"{t}x{c}": round(ms, 1) for t, c, ms in tune["tried"]}}
+    if not a.no_cpu:
+        res["cpu_baseline"] = batch_cpu_baseline(a, scheme, probs, b, per1)
+    b.close()
+    return res
+
+
+def batch_cpu_baseline(a, scheme, probs, b, per):
+    """The oracle (PC_omp paths) on the host cores, one problem per thread: a sample of the batch's problems resumed from
+    the GPU's warmed state, batch-steps steps each."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    from tests.helpers import oracle_cfg_from
+    from diffusionimex_amd import spb6
+    threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    O.lib().ora_set_threads(1)
+    U = b.state()
+    P, pref = b.pressure()
+    pick = np.linspace(0, len(probs) - 1, min(a.batch_cpu_problems, len(probs))).astype(int)
+
+    def one(p):
+        pr = probs[p]
+        ph = [tuple(getattr(pr.ph[k], f) for f in ("gamma", "pi_inf", "q", "qp", "cv", "b")) for k in range(2)]
+        cfg = spb6.make_config(pr.nx, 1, phases=ph, domain=(pr.x0, pr.x1, 0.0, 1.0), cfl=scheme.cfl, newton_tol=scheme.newton_tol)
+        oc = oracle_cfg_from(cfg, O)
+        O.reference_mode(oc, "PC_omp")
+        o = O.Oracle(oc)
+        o.set_state(b.problem_state(U, p).reshape(1, -1, 8))
+        o.set_pressure(P[b.off[p]:b.off[p + 1]].reshape(1, -1), pref[p])
+        s = per[p]
+        o.set_clock(s["t"], s["dt_last"], 1 if s["rk1"] >= s["steps"] else 0)
+        rc = o.run(1.0, a.batch_steps)
+        return pr.nx * o.stats()["steps"], rc
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        outs = list(ex.map(one, pick))
+    wall = time.perf_counter() - t0
+    cu = sum(c for c, _ in outs)
+    return {"value": cu / wall, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{len(pick)} of the {len(probs)} problems, {a.batch_steps} steps each from the GPU's warmed state, "
+                      f"one problem per host thread ({wall:.1f} s, {sum(1 for _, r in outs if r != 0)} failed)"}
 
 
 def cpu_baseline(a, cfg, st):

@@ -480,7 +480,9 @@ __device__ __forceinline__ void caldt_at(const Geo& g, const Phys& P, const doub
 	}
 }
 // Thomas (PC:807-812) on the 3-point rows J[r] (self), J[n + r] (W: column r - 1, or r at the left boundary), J[2n + r]
-// (E: column r + 1, or r at the right); serial, one thread
+// (E: column r + 1, or r at the right); serial, one thread. The recurrences are the reference's, operation for
+// operation; rows are taken in blocks of eight whose coefficient loads are issued together, so the dependent chain
+// waits on memory once per block instead of once per row.
 __device__ __forceinline__ void thomas_serial(int n, const double* __restrict__ J, const double* __restrict__ rhs, double* __restrict__ x,
                                               double* __restrict__ cp, double* __restrict__ dp) {
 	auto coef = [&](int r, double& lo, double& di, double& up) {
@@ -490,12 +492,33 @@ __device__ __forceinline__ void thomas_serial(int n, const double* __restrict__ 
 		if (cW == r) di += J[n + r]; else lo = J[n + r];
 		if (cE == r) di += J[2 * n + r]; else up = J[2 * n + r];
 	};
-	double lo, di, up;
-	coef(0, lo, di, up);
-	cp[0] = up / di; dp[0] = rhs[0] / di;
-	for (int r = 1; r < n; r++) { coef(r, lo, di, up); const double m = di - lo * cp[r - 1]; cp[r] = up / m; dp[r] = (rhs[r] - lo * dp[r - 1]) / m; }
-	x[n - 1] = dp[n - 1];
-	for (int r = n - 2; r >= 0; r--) x[r] = dp[r] - cp[r] * x[r + 1];
+	constexpr int BLK = 8;
+	double lo0, di0, up0;
+	coef(0, lo0, di0, up0);
+	double cpp = up0 / di0, dpp = rhs[0] / di0;
+	cp[0] = cpp; dp[0] = dpp;
+	for (int r = 1; r < n; r += BLK) {
+		const int m = n - r < BLK ? n - r : BLK;
+		double lo[BLK], di[BLK], up[BLK], b[BLK];
+		#pragma unroll
+		for (int k = 0; k < BLK; k++) if (k < m) { coef(r + k, lo[k], di[k], up[k]); b[k] = rhs[r + k]; }
+		#pragma unroll
+		for (int k = 0; k < BLK; k++) if (k < m) {
+			const double mm = di[k] - lo[k] * cpp;
+			cpp = up[k] / mm; dpp = (b[k] - lo[k] * dpp) / mm;
+			cp[r + k] = cpp; dp[r + k] = dpp;
+		}
+	}
+	double xn = dpp;   // dp[n - 1]
+	x[n - 1] = xn;
+	for (int r = n - 2; r >= 0; r -= BLK) {
+		const int m = r + 1 < BLK ? r + 1 : BLK;
+		double c[BLK], d[BLK];
+		#pragma unroll
+		for (int k = 0; k < BLK; k++) if (k < m) { c[k] = cp[r - k]; d[k] = dp[r - k]; }
+		#pragma unroll
+		for (int k = 0; k < BLK; k++) if (k < m) { xn = d[k] - c[k] * xn; x[r - k] = xn; }
+	}
 }
 
 }   // namespace spb
