@@ -80,6 +80,8 @@ struct spb6_ctx {
 	float* F = nullptr;                 // 7 per-cell feature fields over the ghosted grid
 	double* Pplain = nullptr;           // the plain guess of the current stage
 	double* Gplain = nullptr;           // its residual
+	int *slist = nullptr, *scount = nullptr;   // strong-shock cells of the shock budget's second pass
+	int cus = 256;                      // compute units of the device
 	float *dX = nullptr, *dY = nullptr; unsigned long long* dcount = nullptr; long dcap = 0; int dstride = 0; unsigned dseq = 0;
 };
 
@@ -310,7 +312,7 @@ int prep_part(spb6_ctx* c, double adt) {
 	c->tau_last = adt;
 	Dev D = stage_dev(c, adt);
 	{ TRegion tr(c, T_PREP); launch_prep(D, c->cfg.limiter, c->cfg.order, adt, c->Pst, c->st); }
-	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->st); }
+	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->slist, c->scount, 16 * c->cus, c->st); }
 	return SPB6_OK;
 }
 // PressureSubsystem (PC:851-993) after the explicit part; writes the stage state into Ust and H_rks
@@ -458,6 +460,9 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (hipMalloc((void**)&c->flags, NFLAGS * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->flags);
 	if (hipHostMalloc((void**)&c->hs, sizeof(HostScratch), hipHostMallocDefault) != hipSuccess) return fail("host alloc");
 	if (dalloc(c, &c->Pplain, n) || dalloc(c, &c->Gplain, n)) return fail("alloc");
+	if (hipMalloc((void**)&c->slist, (n + 1) * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->slist);
+	c->scount = c->slist + n;
+	(void)hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device);
 	if (hipMalloc((void**)&c->F, 7 * NC * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->F);
 	if (hipDeviceSynchronize() != hipSuccess) return fail("sync");
 	return c;

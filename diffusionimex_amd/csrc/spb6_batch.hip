@@ -50,6 +50,7 @@ struct BArgs {
 	double *U0, *U1, *H0, *H1, *Qe, *Qs, *wxL, *wxR;
 	double *vsc, *esc, *phx, *a2s, *ubx, *ust, *vst, *cz, *Kw, *Pold, *ubsx, *kpx, *wLx, *pvx, *Qph;
 	double *P0, *P1, *Pn, *G, *Gn, *dP, *sc, *rhs, *J, *cpw, *dpw;
+	int* slist;                // strong-shock cells of each problem (sum L ints)
 	// scheme (shared by all problems)
 	int limiter, order_, rk, ramp, acdt, newton_iter;
 	double cfl, cfl_ac, dtfix, tol, t_end;
@@ -93,7 +94,7 @@ __device__ __forceinline__ int wg_any(int f, int* red) {
 // per-problem geometry and device view: pointers shifted so that the grid path's ghosted index c = i + GH * NXT
 // (interior row j = GH of a one-row grid) addresses the problem's cells; field arrays carry a front pad so the shift
 // stays inside the allocation
-struct View { Geo g; Dev D; double *U, *Ust, *H0, *H1, *Qe, *Qs; double *P, *Pst, *Pn, *G, *Gn, *dP, *sc, *rhs, *J, *cpw, *dpw; };
+struct View { Geo g; Dev D; double *U, *Ust, *H0, *H1, *Qe, *Qs; double *P, *Pst, *Pn, *G, *Gn, *dP, *sc, *rhs, *J, *cpw, *dpw; int* slist; };
 __device__ __forceinline__ View make_view(const BArgs& A, const BProb& B, const BState& S) {
 	View v;
 	Geo& g = v.g;
@@ -116,7 +117,7 @@ __device__ __forceinline__ View make_view(const BArgs& A, const BProb& B, const 
 	const long po = B.poff;
 	v.P = (S.ppar ? A.P1 : A.P0) + po; v.Pst = (S.ppar ? A.P0 : A.P1) + po;
 	v.Pn = A.Pn + po; v.G = A.G + po; v.Gn = A.Gn + po; v.dP = A.dP + po; v.sc = A.sc + po; v.rhs = A.rhs + po;
-	v.J = A.J + 3 * po; v.cpw = A.cpw + po; v.dpw = A.dpw + po;
+	v.J = A.J + 3 * po; v.cpw = A.cpw + po; v.dpw = A.dpw + po; v.slist = A.slist + po;
 	return v;
 }
 
@@ -236,7 +237,15 @@ __device__ int wg_stage(const BArgs& A, View& v, BState& S, int s, double dt, do
 	for (int i = threadIdx.x; i < NXT; i += T)
 		prep_face_at(g, A.order_, A.limiter, tau, D.Ust, D.ust, D.vst, D.cz, D.Pold, D.phx, D.phy, D.ubsx, D.kpx, D.wLx, D.pvx,
 		             D.ubsy, D.kpy, D.wLy, D.pvy, i + GH * NXT);
-	for (int q = threadIdx.x; q < n; q += T) shock_at(g, D.ph, D.Ust, D.Qph, q);
+	// shock budget in two passes: cells without a strong shock are settled at once, the strong-shock cells are gathered
+	// and then run with every lane busy
+	__shared__ int scnt;
+	if (threadIdx.x == 0) scnt = 0;
+	__syncthreads();
+	for (int q = threadIdx.x; q < n; q += T) if (shock_classify_at(g, D.ph, D.Ust, D.Qph, q)) v.slist[atomicAdd(&scnt, 1)] = q;
+	__syncthreads();
+	const int nstrong = scnt;
+	for (int k = threadIdx.x; k < nstrong; k += T) shock_strong_at(g, D.ph, D.Ust, D.Qph, v.slist[k]);
 	if (wg_any<T>(bad, ired)) return SPB6_STAGE_INADMISSIBLE;   // (barrier inside)
 	double res;
 	const int rc = wg_newton<T>(A, v, S, red, ired, res);
@@ -511,6 +520,7 @@ spb6_batch* spb6_batch_create(const spb6_batch_problem_t* probs, int nprob, cons
 	double** vecs[] = {&A.P0, &A.P1, &A.Pn, &A.G, &A.Gn, &A.dP, &A.sc, &A.rhs, &A.cpw, &A.dpw};
 	for (auto p : vecs) if (balloc(b, p, b->NI)) return fail();
 	if (balloc(b, &A.J, 3 * b->NI)) return fail();
+	{ void* sl = nullptr; if (hipMalloc(&sl, b->NI * sizeof(int)) != hipSuccess) return fail(); b->allocs.push_back(sl); A.slist = (int*)sl; }
 	void* q = nullptr;
 	if (hipMalloc(&q, nprob * sizeof(BProb)) != hipSuccess) return fail(); b->allocs.push_back(q); A.prob = (const BProb*)q;
 	if (hipMemcpy(q, b->hprob.data(), nprob * sizeof(BProb), hipMemcpyHostToDevice) != hipSuccess) return fail();

@@ -274,12 +274,15 @@ __device__ __forceinline__ void prep_face_at(const Geo& g, int order, int limite
 		else { ubsy[c] = ubs; kpy[c] = kp; wLy[c] = wL; pvy[c] = pv; }
 	}
 }
-// shock budget of interior row q on the +-1 cross stencil of X* (PHY:395-418)
-__device__ __forceinline__ void shock_at(const Geo& g, const Phys& P, const double* __restrict__ Xs, double* __restrict__ Qph, int q) {
+// inputs of the shock budget of interior row q on the +-1 cross stencil of X* (PHY:395-418): true for a compressive
+// cell with a pressure jump over its stencil, then Y (mass fractions), vpre (pre-shock phase volumes), the low and high
+// stencil pressures and the cell's density are set
+__device__ __forceinline__ bool shock_inputs(const Geo& g, const Phys& P, const double* __restrict__ Xs, int q, double Y[2], double vpre[2],
+                                             double& plo, double& phi, double& rho) {
 	const int NC = g.NC, c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
 	auto load = [&](int cc, double* u) { for (int k = 0; k < NV; k++) u[k] = Xs[k * NC + cc]; };
-	double Q0 = 0.0, Q1 = 0.0;
-	double u[NV], c0[7], st[7], lo[7] = {0, 0, 0, 0, 0, 0, 0}, plo = 1e300, phi = -1e300, div = 0.0;
+	double u[NV], c0[7], st[7], lo[7] = {0, 0, 0, 0, 0, 0, 0}, div = 0.0;
+	plo = 1e300; phi = -1e300;
 	load(c, u); stencil_state(P, u, c0);
 	for (int d = 0; d < (g.twoD ? 2 : 1); d++) for (int s = -1; s <= 1; s++) {
 		if (d == 1 && s == 0) continue;
@@ -289,20 +292,57 @@ __device__ __forceinline__ void shock_at(const Geo& g, const Phys& P, const doub
 		if (st[4] < plo) { plo = st[4]; for (int n = 0; n < 7; n++) lo[n] = st[n]; }
 		if (st[4] > phi) phi = st[4];
 	}
-	if ((div < 0.0) && (phi > plo)) {
-		const double rho = c0[2] + c0[3];
-		if (rho > 0) {
-			double Y[2], vpre[2], lam[2], qq;
-			for (int k = 0; k < 2; k++) {
-				Y[k] = c0[2 + k] / rho;
-				vpre[k] = (lo[k] > 0 && lo[2 + k] > 0) ? lo[k] / lo[2 + k] : (c0[2 + k] > 0 ? c0[k] / c0[2 + k] : 0.0);
-			}
-			if ((Y[0] > 0) && (Y[1] > 0) && (vpre[0] > 0) && (vpre[1] > 0) && shock_share(P, Y, vpre, plo, phi, lam, qq)) {
-				Q0 = lam[0] * rho * qq; Q1 = lam[1] * rho * qq;
-			}
-		}
+	if (!((div < 0.0) && (phi > plo))) return false;
+	rho = c0[2] + c0[3];
+	if (!(rho > 0)) return false;
+	for (int k = 0; k < 2; k++) {
+		Y[k] = c0[2 + k] / rho;
+		vpre[k] = (lo[k] > 0 && lo[2 + k] > 0) ? lo[k] / lo[2 + k] : (c0[2 + k] > 0 ? c0[k] / c0[2 + k] : 0.0);
+	}
+	return (Y[0] > 0) && (Y[1] > 0) && (vpre[0] > 0) && (vpre[1] > 0);
+}
+// shock budget Q_k = lam_k rho q of interior row q (PHY:395-418), in one pass
+__device__ __forceinline__ void shock_at(const Geo& g, const Phys& P, const double* __restrict__ Xs, double* __restrict__ Qph, int q) {
+	const int NC = g.NC, c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
+	double Q0 = 0.0, Q1 = 0.0, Y[2], vpre[2], plo, phi, rho, lam[2], qq;
+	if (shock_inputs(g, P, Xs, q, Y, vpre, plo, phi, rho) && shock_share(P, Y, vpre, plo, phi, lam, qq)) {
+		Q0 = lam[0] * rho * qq; Q1 = lam[1] * rho * qq;
 	}
 	Qph[c] = Q0; Qph[NC + c] = Q1;
+}
+// first of the two passes of the shock budget: settles every cell without a strong shock (no share, or the weak-shock
+// closed form) and returns true for a cell that needs the strong branch (its Qph is then left to shock_strong_at)
+__device__ __forceinline__ bool shock_classify_at(const Geo& g, const Phys& P, const double* __restrict__ Xs, double* __restrict__ Qph, int q) {
+	const int NC = g.NC, c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
+	double Q0 = 0.0, Q1 = 0.0, Y[2], vpre[2], plo, phi, rho, lam[2], qq, K[2], G[2];
+	if (shock_inputs(g, P, Xs, q, Y, vpre, plo, phi, rho)) {
+		const int m = share_pre(P, Y, vpre, plo, phi, K, G, lam, qq);
+		if (m == 2) return true;
+		if (m == 1) { Q0 = lam[0] * rho * qq; Q1 = lam[1] * rho * qq; }
+	}
+	Qph[c] = Q0; Qph[NC + c] = Q1;
+	return false;
+}
+// second pass: the strong branch of a cell shock_classify_at returned true for
+__device__ __forceinline__ void shock_strong_at(const Geo& g, const Phys& P, const double* __restrict__ Xs, double* __restrict__ Qph, int q) {
+	const int NC = g.NC, c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
+	double Q0 = 0.0, Q1 = 0.0, Y[2], vpre[2], plo, phi, rho, lam[2], qq, K[2], G[2];
+	if (shock_inputs(g, P, Xs, q, Y, vpre, plo, phi, rho) && share_pre(P, Y, vpre, plo, phi, K, G, lam, qq) == 2 &&
+	    share_strong(P, Y, vpre, plo, phi, K, G, lam, qq)) {
+		Q0 = lam[0] * rho * qq; Q1 = lam[1] * rho * qq;
+	}
+	Qph[c] = Q0; Qph[NC + c] = Q1;
+}
+// append the cells of this wavefront whose flag is set to list (one atomic per wavefront); the list order is not
+// deterministic but each cell's result does not depend on it
+__device__ __forceinline__ void wave_append(bool flag, int q, int* __restrict__ list, int* __restrict__ count) {
+	const unsigned long long m = __ballot(flag);
+	if (m == 0ull) return;
+	const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
+	int base = 0;
+	if (lane == leader) base = atomicAdd(count, __popcll(m));
+	base = __shfl(base, leader, 64);
+	if (flag) list[base + __popcll(m & ((1ull << lane) - 1ull))] = q;
 }
 
 // ------------------------------------------------------------------------------------------------ cell residual (PC:584-705)

@@ -197,13 +197,17 @@ __device__ __forceinline__ double share_F(const Phys& P, const Prof& S, double v
 	if (!prof_integrate(P, S, c, v1e, qq)) return dnan();
 	return (v1e - vH0) / vpre0;
 }
-// phase shares lam and mixture dissipation q of a shock from (p0, vpre, Y) to p1 (PHY:343-383, PHO start bracket)
-__device__ __noinline__ bool shock_share(const Phys& P, const double Y[2], const double vpre[2], double p0, double p1, double lam[2], double& q) {
-	double K[2], G[2], Kmin = dinf();
+// phase shares lam and mixture dissipation q of a shock from (p0, vpre, Y) to p1 (PHY:343-383, PHO start bracket),
+// in two parts: share_pre computes the phase stiffnesses K, G and settles the cheap cases -- 0: no share (shock_share
+// false), 1: the weak-shock closed form (lam, q set; shock_share true) -- or returns 2 when the strong branch
+// (share_strong: shooting over the adaptive RK45 profile) is needed
+SPB_HD int share_pre(const Phys& P, const double Y[2], const double vpre[2], double p0, double p1, double K[2], double G[2],
+                     double lam[2], double& q) {
+	double Kmin = dinf();
 	for (int k = 0; k < 2; k++) {
 		K[k] = c2_rp(P, k, 1.0 / vpre[k], p0) / vpre[k];
 		G[k] = 0.5 * (P.g[k] + 1.0); if (P.b[k] != 0.0) G[k] *= vpre[k] / (vpre[k] - P.b[k]);
-		if (!(K[k] > 0)) return false; Kmin = smin(Kmin, K[k]);
+		if (!(K[k] > 0)) return 0; Kmin = smin(Kmin, K[k]);
 	}
 	const double dp = p1 - p0;
 	if (dp < 1e-2 * Kmin) {
@@ -211,8 +215,13 @@ __device__ __noinline__ bool shock_share(const Phys& P, const double Y[2], const
 		for (int k = 0; k < 2; k++) { w[k] = Y[k] * vpre[k] * G[k] / (K[k] * K[k]); sw += w[k]; }
 		for (int k = 0; k < 2; k++) lam[k] = w[k] / sw;
 		q = sw * dp * dp * dp / 6.0;
-		return true;
+		return 1;
 	}
+	return 2;
+}
+__device__ __noinline__ bool share_strong(const Phys& P, const double Y[2], const double vpre[2], double p0, double p1, const double K[2],
+                                          const double G[2], double lam[2], double& q) {
+	const double dp = p1 - p0;
 	Prof S; S.p0 = p0; S.Y0 = Y[0]; S.Y1 = Y[1]; S.vk00 = vpre[0];
 	const double vH0 = hug_v(P, 0, p0, p1, vpre[0]), vH1 = hug_v(P, 1, p0, p1, vpre[1]);
 	if (!isfin(vH0) || !isfin(vH1)) return false;
@@ -246,6 +255,12 @@ __device__ __noinline__ bool shock_share(const Phys& P, const double Y[2], const
 	const double c = 1.0 / (1.0 + pexp(-t));
 	lam[0] = c; lam[1] = 1.0 - c; q = qt;
 	return q > 0;
+}
+__device__ __forceinline__ bool shock_share(const Phys& P, const double Y[2], const double vpre[2], double p0, double p1, double lam[2], double& q) {
+	double K[2], G[2];
+	const int m = share_pre(P, Y, vpre, p0, p1, K, G, lam, q);
+	if (m != 2) return m == 1;
+	return share_strong(P, Y, vpre, p0, p1, K, G, lam, q);
 }
 // stencil_state (PHY:385-390) from the 8 conserved values
 SPB_HD void stencil_state(const Phys& P, const double* c, double* st) {

@@ -48,7 +48,9 @@ void launch_bc_array(const Geo& g, double* a, hipStream_t st);
 void launch_combine(const Geo& g, const double* Un, const double* H0, const double* H1, int nr, double ae0, double ae1, double ai0, double ai1, double* Qe, double* Qs, hipStream_t st);
 void launch_explicit(const Dev& D, int limiter, int order, const double* Qe, const double* Qs, double adt, hipStream_t st);
 void launch_prep(const Dev& D, int limiter, int order, double tau, const double* Pst, hipStream_t st);
-void launch_shock(const Dev& D, hipStream_t st);
+// shock budget in two passes; list holds n ints, count one int (both device scratch); strong_blocks: blocks of 64 for
+// the strong-shock pass (a few per compute unit)
+void launch_shock(const Dev& D, int* list, int* count, int strong_blocks, hipStream_t st);
 void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st);
 void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, hipStream_t st);
 void launch_fdscale(const Dev& D, const double* P, double* sc, hipStream_t st);

@@ -8,6 +8,7 @@
 #include "spb6_device.h"
 #include "spb6_kernels.h"
 #include "spb6_cell.h"
+#include <algorithm>
 
 namespace spb {
 
@@ -157,11 +158,19 @@ __global__ void __launch_bounds__(256) k_prep_face(Geo g, int order, int limiter
 	if (c >= g.NC) return;
 	prep_face_at(g, order, limiter, tau, Xs, ust, vst, cz, Pold, phx, phy, ubsx, kpx, wLx, pvx, ubsy, kpy, wLy, pvy, c);
 }
-// shock budget of each interior cell on the +-1 cross stencil of X* (PHY:395-418)
-__global__ void __launch_bounds__(64) k_shock(Geo g, Phys P, const double* __restrict__ Xs, double* __restrict__ Qph) {
+// shock budget of each interior cell on the +-1 cross stencil of X* (PHY:395-418), in two passes: every cell is
+// classified and all but the strong-shock cells are settled; the strong-shock cells (adaptive RK45 shooting) are then
+// taken from a compact list, so a wavefront's lanes all run the expensive branch instead of waiting on one that does
+__global__ void __launch_bounds__(256) k_shock_classify(Geo g, Phys P, const double* __restrict__ Xs, double* __restrict__ Qph,
+                                                       int* __restrict__ list, int* __restrict__ count) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	if (q >= g.n) return;
-	shock_at(g, P, Xs, Qph, q);
+	const bool strong = q < g.n && shock_classify_at(g, P, Xs, Qph, q);
+	wave_append(strong, q, list, count);
+}
+__global__ void __launch_bounds__(64) k_shock_strong(Geo g, Phys P, const double* __restrict__ Xs, double* __restrict__ Qph,
+                                                    const int* __restrict__ list, const int* __restrict__ count) {
+	const int m = *count;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) shock_strong_at(g, P, Xs, Qph, list[i]);
 }
 
 // ------------------------------------------------------------------------------------------------ cell residual (PC:584-705)
@@ -442,9 +451,17 @@ int krylov_items(int n) {
 	return it;
 }
 
+// FP32 form: no summation order to match; about 1k blocks measured best at 2M rows (fewer blocks lose memory
+// parallelism faster than they save partial re-reductions)
+static int krylov_items32(int n) {
+	int it = 4;
+	while (it < 128 && (long)n > (long)256 * it * 1024) it *= 2;
+	return it;
+}
+template <typename T> static int items_of(int n) { return sizeof(T) == 8 ? krylov_items(n) : krylov_items32(n); }
 template <typename T>
 static void krylov_setup_t(const Geo& g, const KrylovBufsT<T>& B, const T* b, hipStream_t st) {
-	const int items = krylov_items(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
+	const int items = items_of<T>(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
 	KP P{B.part, B.stride, nb, items};
 	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J};
 	hipLaunchKernelGGL(kb_setup1<T>, dim3(nb), dim3(256), 0, st, g, B.J, b, P);
@@ -453,7 +470,7 @@ static void krylov_setup_t(const Geo& g, const KrylovBufsT<T>& B, const T* b, hi
 }
 template <typename T>
 static void krylov_iter_t(const Geo& g, const KrylovBufsT<T>& B, hipStream_t st, int it, double tol, int maxit) {
-	const int items = krylov_items(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
+	const int items = items_of<T>(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
 	KP P{B.part, B.stride, nb, items};
 	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J};
 	hipLaunchKernelGGL(kb_p<T>, dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol, maxit);
@@ -495,8 +512,11 @@ void launch_prep(const Dev& D, int limiter, int order, double tau, const double*
 	hipLaunchKernelGGL(k_prep_face, dim3(nblk(g.NC)), dim3(256), 0, st, g, order, limiter, tau, D.Ust, D.ust, D.vst, D.cz, D.Pold, D.phx, D.phy,
 	                   D.ubsx, D.kpx, D.wLx, D.pvx, D.ubsy, D.kpy, D.wLy, D.pvy);
 }
-void launch_shock(const Dev& D, hipStream_t st) {
-	hipLaunchKernelGGL(k_shock, dim3(nblk(D.g.n, 64)), dim3(64), 0, st, D.g, D.ph, D.Ust, D.Qph);
+void launch_shock(const Dev& D, int* list, int* count, int strong_blocks, hipStream_t st) {
+	(void)hipMemsetAsync(count, 0, sizeof(int), st);
+	hipLaunchKernelGGL(k_shock_classify, dim3(nblk(D.g.n)), dim3(256), 0, st, D.g, D.ph, D.Ust, D.Qph, list, count);
+	hipLaunchKernelGGL(k_shock_strong, dim3(std::max(1, std::min(nblk(D.g.n, 64), strong_blocks))), dim3(64), 0, st, D.g, D.ph,
+	                   (const double*)D.Ust, D.Qph, (const int*)list, (const int*)count);
 }
 void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st) {
 	hipLaunchKernelGGL(k_residual, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, dP, lam, Pout, G, nrm, D.flags, flag_slot);
