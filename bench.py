@@ -92,12 +92,14 @@ def roofline_of(res, n, fp32):
         "prep": (kt["prep_ms"], kt["prep_n"], BYTES_2D["prep"] * n),
         "residual": (kt["residual_ms"], kt["residual_n"], BYTES_2D["residual"] * n),
         "jacobian": (kt["jacobian_ms"], kt["jacobian_n"], BYTES_2D["jacobian"] * n),
-        "krylov_iter": (kt["krylov_ms"], d["krylov_it"], kb * n),
+        "krylov_iter": (kt["krylov_ms"], kt["krylov_launched"], kb * n),
         "final": (kt["final_ms"], kt["final_n"], BYTES_2D["final"] * n),
     }
     kernel_ms = {k: round(v[0], 3) for k, v in cats.items()}
     kernel_ms["shock"] = round(kt["shock_ms"], 3)
     kernel_ms["other"] = round(kt["other_ms"], 3)
+    kernel_ms["krylov_aux"] = round(kt["krylov_aux_ms"], 3)
+    kernel_ms["guess"] = round(kt["guess_ms"], 3)
     dom = max(cats, key=lambda k: cats[k][0])
     ms, cnt, byts = cats[dom]
     avg_s = (ms / 1e3) / max(cnt, 1)

@@ -25,7 +25,9 @@ namespace {
 
 struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; };
 
-enum TimerCat { T_EXPLICIT = 0, T_PREP, T_SHOCK, T_RESIDUAL, T_JACOBIAN, T_KRYLOV, T_FINAL, T_OTHER, T_NCAT };
+// T_KRYLOV: BiCGStab iteration passes only; T_KAUX: the solves' setup, precision conversions and FP64 defects;
+// T_GUESS: the learned guess's features and inference
+enum TimerCat { T_EXPLICIT = 0, T_PREP, T_SHOCK, T_RESIDUAL, T_JACOBIAN, T_KRYLOV, T_FINAL, T_OTHER, T_KAUX, T_GUESS, T_NCAT };
 
 struct Timer {
 	bool on = false;
@@ -72,6 +74,7 @@ struct spb6_ctx {
 	double Ae[3][3], Ai[3][3];
 	spb6_stats_t stats;
 	int last_krylov = 0;
+	long krylov_launched = 0;          // BiCGStab iteration passes launched since timing was (re)enabled
 	double tau_last = 0.0;
 	Timer timer;
 	std::vector<double> hpart;          // host copy of block partials (FP64 defect norms)
@@ -151,7 +154,7 @@ template <typename T>
 int krylov_run(spb6_ctx* c, const KrylovBufsT<T>& B, const T* b, double tol, int& iters, bool& ok) {
 	const Geo& g = c->g; const int maxit = c->cfg.krylov_iter;
 	{
-		TRegion tr(c, T_KRYLOV);
+		TRegion tr(c, T_KAUX);
 		if constexpr (sizeof(T) == 8) launch_krylov_setup(g, B, b, c->st); else launch_krylov_setup32(g, B, b, c->st);
 	}
 	// the loop top of iteration `launched` runs in its first pass, so maxit + 1 passes decide every outcome
@@ -171,6 +174,7 @@ int krylov_run(spb6_ctx* c, const KrylovBufsT<T>& B, const T* b, double tol, int
 	}
 	iters = c->hs->ks.iters; ok = c->hs->ks.done && c->hs->ks.status == 0;
 	c->last_krylov = iters;
+	c->krylov_launched += launched;
 	return SPB6_OK;
 }
 // FP32 Krylov form: inexact Newton step. The step solves J dP = b to a relative FP64 residual
@@ -188,9 +192,12 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res) {
 		return krylov_run<double>(c, B, c->rhs, c->cfg.krylov_tol, iters, ok);
 	}
 	const double eta = std::min(1e-6, 0.1 * c->cfg.newton_tol / std::max(newton_res, 1e-300));
-	launch_to_f32(5 * n, c->J, c->J32, c->st);
-	launch_to_f32(n, c->rhs, c->b32, c->st);
-	HIPCHK(hipMemsetAsync(c->dP, 0, n * sizeof(double), c->st));
+	{
+		TRegion tr(c, T_KAUX);
+		launch_to_f32(5 * n, c->J, c->J32, c->st);
+		launch_to_f32(n, c->rhs, c->b32, c->st);
+		HIPCHK(hipMemsetAsync(c->dP, 0, n * sizeof(double), c->st));
+	}
 	KrylovBufs32 B{c->x32, c->r32, c->r032, c->p32, c->v32, c->s32, c->t32, c->J32, c->part, c->part_stride, c->ksd};
 	iters = 0; ok = false;
 	const int nbl = (int)((n + 255) / 256);
@@ -199,10 +206,10 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res) {
 		const int rc = krylov_run<float>(c, B, c->b32, std::max(eta, 1e-6), it, kok);
 		if (rc) return rc;
 		iters += it;
-		launch_add_f32(n, c->x32, c->dP, c->st);
 		// FP64 defect of the accumulated step
 		{
-			TRegion tr(c, T_KRYLOV);
+			TRegion tr(c, T_KAUX);
+			launch_add_f32(n, c->x32, c->dP, c->st);
 			launch_linres(c->g, c->J, c->rhs, c->dP, c->kr, c->part, c->part_stride, c->st);
 		}
 		std::vector<double>& hp = c->hpart;
@@ -216,7 +223,7 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res) {
 		if (std::sqrt(rr) <= eta * std::sqrt(bb) || !(bb > 0)) { ok = true; break; }
 		ok = std::isfinite(rr) && rr < bb;   // progress so far
 		if (!kok && sweep > 0) break;
-		launch_to_f32(n, c->kr, c->b32, c->st);
+		{ TRegion tr(c, T_KAUX); launch_to_f32(n, c->kr, c->b32, c->st); }
 	}
 	return SPB6_OK;
 }
@@ -331,6 +338,7 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 		have_G = true;
 		HIPCHK(hipMemcpyAsync(c->Pplain, c->Pst, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 		HIPCHK(hipMemcpyAsync(c->Gplain, c->G, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+		TRegion tr(c, T_GUESS);
 		mlp_feat_cell(D, c->Pplain, c->F, c->st);
 	}
 	const double thr = ok0 ? GUESS_ACTIVE * res0 : 0.0;
@@ -338,7 +346,10 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 		// the plain guess judges the learned one: keep whichever has the smaller Newton residual norm
 		const spb6_mlp* M = c->model;
 		MlpDev W{M->P, M->W1t, M->W2t, M->smean, M->sistd, M->b3, M->mu_t, M->sd_t};
-		mlp_infer(&D, c->F, adt, (int)n, nullptr, W, c->Pplain, ok0 ? c->Gplain : nullptr, thr, c->Pn, nullptr, c->st);
+		{
+			TRegion tr(c, T_GUESS);
+			mlp_infer(&D, c->F, adt, (int)n, nullptr, W, c->Pplain, ok0 ? c->Gplain : nullptr, thr, c->Pn, nullptr, c->st);
+		}
 		bool okl; double resl;
 		rc = eval_all(c, D, c->Pn, nullptr, 0.0, nullptr, c->Gn, okl, resl); if (rc) return rc;
 		if (okl && (!ok0 || resl < res0)) {
@@ -703,6 +714,7 @@ int spb6_enable_timing(spb6_ctx* c, int on) {
 	c->timer.resolve();
 	c->timer.on = on != 0;
 	for (int k = 0; k < T_NCAT; k++) { c->timer.ms[k] = 0; c->timer.cnt[k] = 0; }
+	c->krylov_launched = 0;
 	return SPB6_OK;
 }
 int spb6_get_kernel_times(spb6_ctx* c, spb6_kernel_times_t* t) {
@@ -714,6 +726,7 @@ int spb6_get_kernel_times(spb6_ctx* c, spb6_kernel_times_t* t) {
 	t->explicit_n = T.cnt[T_EXPLICIT]; t->prep_n = T.cnt[T_PREP]; t->shock_n = T.cnt[T_SHOCK]; t->residual_n = T.cnt[T_RESIDUAL];
 	t->jacobian_n = T.cnt[T_JACOBIAN]; t->krylov_n = T.cnt[T_KRYLOV]; t->final_n = T.cnt[T_FINAL];
 	t->krylov_iters = c->stats.krylov_it;
+	t->krylov_aux_ms = T.ms[T_KAUX]; t->guess_ms = T.ms[T_GUESS]; t->krylov_launched = c->krylov_launched;
 	return SPB6_OK;
 }
 

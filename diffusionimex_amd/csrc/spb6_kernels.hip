@@ -9,6 +9,7 @@
 #include "spb6_kernels.h"
 #include "spb6_cell.h"
 #include <algorithm>
+#include <type_traits>
 
 namespace spb {
 
@@ -237,37 +238,47 @@ __global__ void __launch_bounds__(256) k_thomas(int n, const double* __restrict_
 // that outlive a pass (rho, alpha, omega, cc) are kept in KS, double-buffered by the parity of the iteration index the
 // host passes in, so no pass reads a value that another block of the same pass writes. Every pass returns at once when
 // the solve has finished, so the host launches iterations in chunks and reads KS::done once per chunk.
-// Rows: block b, thread t owns rows b*256*items + it*256 + t (it = 0..items-1), the fixed order of the sums.
+// Rows: block b, thread t owns rows b*256*IT + it*256 + t (it = 0..IT-1), the fixed order of the sums. IT is a
+// compile-time count, and each thread loads the inputs of a chunk of its rows into registers before it computes them
+// (the first chunk before the partial re-reduction that opens the pass), so several loads per thread are in flight.
 template <typename T> struct KV { T *x, *r, *r0, *p, *v, *s, *t; const T* J; };
-struct KP { double* part; int stride; int nb; int items; };
+struct KP { double* part; int stride; int nb; };
 enum { SL_RR = 0, SL_R0R, SL_SP, SL_R0V, SL_SS, SL_SSUM, SL_TS, SL_TT, SL_E, SL_SB, SL_BB, NSLOTS };
 
 template <typename T>
-__device__ __forceinline__ T dinv_of(const T* __restrict__ J, int r) { const T d = J[r]; return d != T(0) ? T(1) / d : T(1); }
+__device__ __forceinline__ T dinv_v(T d) { return d != T(0) ? T(1) / d : T(1); }
+template <typename T>
+__device__ __forceinline__ T dinv_of(const T* __restrict__ J, int r) { return dinv_v(J[r]); }
+// the inputs of one row's product: entries (self, W, E, S, N), the vector and the diagonal at those columns
+template <typename T> struct SpIn { T j[5], z[5], d[5]; };
+template <typename T>
+__device__ __forceinline__ void sp_load(const Geo& g, const T* __restrict__ J, const T* __restrict__ z, int r, bool diag, SpIn<T>& s) {
+	const int n = g.n; const Nb b = neighbours(g, r);
+	const int col[5] = {r, b.W, b.E, b.S, b.N};
+	const int ne = g.twoD ? 5 : 3;
+	#pragma unroll
+	for (int k = 0; k < 5; k++) if (k < ne) { s.j[k] = J[k * n + r]; s.z[k] = z[col[k]]; if (diag) s.d[k] = J[col[k]]; }
+}
 // y_r = sum_q A_rq (dinv_c z_c + cc) over the row's entries in the order self, W, E, S, N
 template <typename T>
-__device__ __forceinline__ T spmv_prec(const Geo& g, const T* __restrict__ J, const T* __restrict__ z, T cc, int r) {
-	const int n = g.n; const Nb b = neighbours(g, r);
-	auto ph = [&](int c) { return dinv_of(J, c) * z[c] + cc; };
-	T s = T(0);
-	s += J[r] * ph(r);
-	s += J[n + r] * ph(b.W);
-	s += J[2 * n + r] * ph(b.E);
-	if (g.twoD) { s += J[3 * n + r] * ph(b.S); s += J[4 * n + r] * ph(b.N); }
-	return s;
+__device__ __forceinline__ T sp_prec(const SpIn<T>& s, int ne, T cc) {
+	T y = T(0);
+	#pragma unroll
+	for (int k = 0; k < 5; k++) if (k < ne) y += s.j[k] * (dinv_v(s.d[k]) * s.z[k] + cc);
+	return y;
+}
+template <typename T>
+__device__ __forceinline__ T sp_plain(const SpIn<T>& s, int ne) {
+	T y = T(0);
+	#pragma unroll
+	for (int k = 0; k < 5; k++) if (k < ne) y += s.j[k] * s.z[k];
+	return y;
 }
 template <typename T>
 __device__ __forceinline__ T spmv_plain(const Geo& g, const T* __restrict__ J, const T* __restrict__ z, int r) {
-	const int n = g.n; const Nb b = neighbours(g, r);
-	T s = T(0);
-	s += J[r] * z[r];
-	s += J[n + r] * z[b.W];
-	s += J[2 * n + r] * z[b.E];
-	if (g.twoD) { s += J[3 * n + r] * z[b.S]; s += J[4 * n + r] * z[b.N]; }
-	return s;
+	SpIn<T> s; sp_load(g, J, z, r, false, s); return sp_plain(s, g.twoD ? 5 : 3);
 }
-#define ROWS_BEGIN for (int it_ = 0; it_ < P.items; it_++) { const int r = blockIdx.x * 256 * P.items + it_ * 256 + threadIdx.x; if (r >= n) continue;
-#define ROWS_END }
+__device__ __forceinline__ int krow(int IT, int k) { return blockIdx.x * 256 * IT + k * 256 + threadIdx.x; }
 // this block's partial of one slot (thread 0 writes)
 __device__ __forceinline__ void put_partial(const KP& P, int slot, double v, double* sh) {
 	const double tot = block_sum256(v, sh);
@@ -280,21 +291,27 @@ __device__ __forceinline__ double grid_total(const KP& P, int slot, double* sh) 
 	for (int c = threadIdx.x; c < P.nb; c += 256) s += q[c];
 	return block_sum256(s, sh);
 }
+// rows per register chunk of the vector passes and of the product passes: FP32 passes gain from several rows' loads in
+// flight per thread; the FP64 passes measured best row by row (their per-row loads are already twice as wide)
+template <typename T> constexpr int chv() { return sizeof(T) == 4 ? 4 : 1; }
+template <typename T> constexpr int chs() { return sizeof(T) == 4 ? 2 : 1; }
 
 // setup 1: partials of E = sum of all entries, sum(b), b.b (PC:726-738)
-template <typename T>
+template <typename T, int IT>
 __global__ void __launch_bounds__(256) kb_setup1(Geo g, const T* __restrict__ J, const T* __restrict__ b, KP P) {
 	__shared__ double sh[4];
 	const int n = g.n; const int nent = g.twoD ? 5 : 3;
 	double sE = 0.0, sb = 0.0, sbb = 0.0;
-	ROWS_BEGIN
+	#pragma unroll
+	for (int it = 0; it < IT; it++) {
+		const int r = krow(IT, it); if (r >= n) continue;
 		T e = T(0); for (int qq = 0; qq < nent; qq++) e += J[qq * n + r];
 		sE += (double)e; sb += (double)b[r]; sbb += (double)(b[r] * b[r]);
-	ROWS_END
+	}
 	put_partial(P, SL_E, sE, sh); put_partial(P, SL_SB, sb, sh); put_partial(P, SL_BB, sbb, sh);
 }
 // setup 2: Einv, cc_b, bn; x = M^-1 b, p = v = 0
-template <typename T>
+template <typename T, int IT>
 __global__ void __launch_bounds__(256) kb_setup2(Geo g, const T* __restrict__ J, const T* __restrict__ b, KV<T> V, KS* ks, KP P) {
 	__shared__ double sh[4];
 	const int n = g.n;
@@ -307,27 +324,34 @@ __global__ void __launch_bounds__(256) kb_setup2(Geo g, const T* __restrict__ J,
 		ks->rho[0] = 1; ks->alpha[0] = 1; ks->omega[0] = 1;
 		ks->done = 0; ks->iters = 0; ks->early = 0; ks->status = 0;
 	}
-	ROWS_BEGIN V.x[r] = dinv_of(J, r) * b[r] + cc; V.p[r] = T(0); V.v[r] = T(0); ROWS_END
+	#pragma unroll
+	for (int it = 0; it < IT; it++) { const int r = krow(IT, it); if (r < n) { V.x[r] = dinv_of(J, r) * b[r] + cc; V.p[r] = T(0); V.v[r] = T(0); } }
 }
 // setup 3: r = b - A x, r0 = r; partials of r.r, r0.r
-template <typename T>
+template <typename T, int IT>
 __global__ void __launch_bounds__(256) kb_setup3(Geo g, const T* __restrict__ b, KV<T> V, KP P) {
 	__shared__ double sh[4];
 	const int n = g.n;
 	double srr = 0.0, sr0r = 0.0;
-	ROWS_BEGIN
+	#pragma unroll
+	for (int it = 0; it < IT; it++) {
+		const int r = krow(IT, it); if (r >= n) continue;
 		const T ax = spmv_plain(g, V.J, (const T*)V.x, r);
 		const T rv = b[r] - ax; V.r[r] = rv; V.r0[r] = rv;
 		srr += (double)(rv * rv); sr0r += (double)(rv * rv);
-	ROWS_END
+	}
 	put_partial(P, SL_RR, srr, sh); put_partial(P, SL_R0R, sr0r, sh);
 }
 // I1 (iteration it): loop top (PC:741-744) from r.r and r0.r, then p = r + beta (p - omega v); partials of sum(p)
-template <typename T>
+template <typename T, int IT>
 __global__ void __launch_bounds__(256) kb_p(Geo g, KV<T> V, KS* ks, KP P, int it, double tol, int maxit) {
 	if (ks->done) return;
 	__shared__ double sh[4];
 	const int n = g.n, cur = it & 1, nxt = (it + 1) & 1;
+	constexpr int CHV = chv<T>();
+	T rv[CHV], pv[CHV], vv[CHV];
+	#pragma unroll
+	for (int k = 0; k < CHV; k++) { const int r = krow(IT, k); if (k < IT && r < n) { rv[k] = V.r[r]; pv[k] = V.p[r]; vv[k] = V.v[r]; } }
 	const double rr = grid_total(P, SL_RR, sh), r0r = grid_total(P, SL_R0R, sh);
 	const T bn = (T)ks->bn;
 	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -339,75 +363,148 @@ __global__ void __launch_bounds__(256) kb_p(Geo g, KV<T> V, KS* ks, KP P, int it
 	const T omega = (T)ks->omega[cur];
 	if (lead) ks->rho[nxt] = (double)rho1;
 	double sp = 0.0;
-	ROWS_BEGIN const T pv = V.r[r] + beta * (V.p[r] - omega * V.v[r]); V.p[r] = pv; sp += (double)pv; ROWS_END
+	#pragma unroll
+	for (int c0 = 0; c0 < IT; c0 += CHV) {
+		if (c0 > 0) {
+			#pragma unroll
+			for (int k = 0; k < CHV; k++) { const int r = krow(IT, c0 + k); if (c0 + k < IT && r < n) { rv[k] = V.r[r]; pv[k] = V.p[r]; vv[k] = V.v[r]; } }
+		}
+		#pragma unroll
+		for (int k = 0; k < CHV; k++) {
+			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
+			const T pn = rv[k] + beta * (pv[k] - omega * vv[k]); V.p[r] = pn; sp += (double)pn;
+		}
+	}
 	put_partial(P, SL_SP, sp, sh);
 }
 // I2: cc_p = sum(p) Einv; v = A M^-1 p; partials of r0.v
-template <typename T>
+template <typename T, int IT>
 __global__ void __launch_bounds__(256) kb_v(Geo g, KV<T> V, KS* ks, KP P, int it) {
 	if (ks->done) return;
 	__shared__ double sh[4];
-	const int n = g.n;
+	const int n = g.n, ne = g.twoD ? 5 : 3;
+	constexpr int CHS = chs<T>();
+	SpIn<T> q[CHS]; T r0v[CHS];
+	#pragma unroll
+	for (int k = 0; k < CHS; k++) { const int r = krow(IT, k); if (k < IT && r < n) { sp_load(g, V.J, (const T*)V.p, r, true, q[k]); r0v[k] = V.r0[r]; } }
 	const T cc = (T)grid_total(P, SL_SP, sh) * (T)ks->Einv;
 	if (blockIdx.x == 0 && threadIdx.x == 0) ks->cc_p = (double)cc;
 	double s = 0.0;
-	ROWS_BEGIN const T vv = spmv_prec(g, V.J, (const T*)V.p, cc, r); V.v[r] = vv; s += (double)(V.r0[r] * vv); ROWS_END
+	#pragma unroll
+	for (int c0 = 0; c0 < IT; c0 += CHS) {
+		if (c0 > 0) {
+			#pragma unroll
+			for (int k = 0; k < CHS; k++) { const int r = krow(IT, c0 + k); if (c0 + k < IT && r < n) { sp_load(g, V.J, (const T*)V.p, r, true, q[k]); r0v[k] = V.r0[r]; } }
+		}
+		#pragma unroll
+		for (int k = 0; k < CHS; k++) {
+			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
+			const T vv = sp_prec(q[k], ne, cc); V.v[r] = vv; s += (double)(r0v[k] * vv);
+		}
+	}
 	put_partial(P, SL_R0V, s, sh);
 }
 // I3: alpha = rho / r0.v; s = r - alpha v; partials of s.s and sum(s)
-template <typename T>
+template <typename T, int IT>
 __global__ void __launch_bounds__(256) kb_s(Geo g, KV<T> V, KS* ks, KP P, int it) {
 	if (ks->done) return;
 	__shared__ double sh[4];
 	const int n = g.n, nxt = (it + 1) & 1;
+	constexpr int CHV = chv<T>();
+	T rv[CHV], vv[CHV];
+	#pragma unroll
+	for (int k = 0; k < CHV; k++) { const int r = krow(IT, k); if (k < IT && r < n) { rv[k] = V.r[r]; vv[k] = V.v[r]; } }
 	const T alpha = (T)ks->rho[nxt] / (T)grid_total(P, SL_R0V, sh);
 	if (blockIdx.x == 0 && threadIdx.x == 0) ks->alpha[nxt] = (double)alpha;
 	double sss = 0.0, ssum = 0.0;
-	ROWS_BEGIN const T sv = V.r[r] - alpha * V.v[r]; V.s[r] = sv; sss += (double)(sv * sv); ssum += (double)sv; ROWS_END
+	#pragma unroll
+	for (int c0 = 0; c0 < IT; c0 += CHV) {
+		if (c0 > 0) {
+			#pragma unroll
+			for (int k = 0; k < CHV; k++) { const int r = krow(IT, c0 + k); if (c0 + k < IT && r < n) { rv[k] = V.r[r]; vv[k] = V.v[r]; } }
+		}
+		#pragma unroll
+		for (int k = 0; k < CHV; k++) {
+			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
+			const T sv = rv[k] - alpha * vv[k]; V.s[r] = sv; sss += (double)(sv * sv); ssum += (double)sv;
+		}
+	}
 	put_partial(P, SL_SS, sss, sh); put_partial(P, SL_SSUM, ssum, sh);
 }
 // I4: early exit if |s| <= tol |b| (x += alpha M^-1 p, PC:751); otherwise cc_s = sum(s) Einv, t = A M^-1 s; partials of
 // t.s and t.t
-template <typename T>
+template <typename T, int IT>
 __global__ void __launch_bounds__(256) kb_t(Geo g, KV<T> V, KS* ks, KP P, int it, double tol) {
 	if (ks->done) return;
 	__shared__ double sh[4];
-	const int n = g.n, nxt = (it + 1) & 1;
+	const int n = g.n, nxt = (it + 1) & 1, ne = g.twoD ? 5 : 3;
+	constexpr int CHS = chs<T>();
+	SpIn<T> q[CHS];
+	#pragma unroll
+	for (int k = 0; k < CHS; k++) { const int r = krow(IT, k); if (k < IT && r < n) sp_load(g, V.J, (const T*)V.s, r, true, q[k]); }
 	const double s2 = grid_total(P, SL_SS, sh), Ss = grid_total(P, SL_SSUM, sh);
 	const T alpha = (T)ks->alpha[nxt];
 	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
 	if ((T)sqrt((double)(T)s2) <= (T)tol * (T)ks->bn) {
 		const T ccp = (T)ks->cc_p;
-		ROWS_BEGIN V.x[r] += alpha * (dinv_of(V.J, r) * V.p[r] + ccp); ROWS_END
+		#pragma unroll
+		for (int k = 0; k < IT; k++) { const int r = krow(IT, k); if (r < n) V.x[r] += alpha * (dinv_of(V.J, r) * V.p[r] + ccp); }
 		if (lead) { ks->early = 1; ks->iters = it + 1; ks->status = 0; }
 		return;
 	}
 	const T cc = (T)Ss * (T)ks->Einv;
 	if (lead) ks->cc_s = (double)cc;
 	double sts = 0.0, stt = 0.0;
-	ROWS_BEGIN const T tv = spmv_prec(g, V.J, (const T*)V.s, cc, r); V.t[r] = tv; sts += (double)(tv * V.s[r]); stt += (double)(tv * tv); ROWS_END
+	#pragma unroll
+	for (int c0 = 0; c0 < IT; c0 += CHS) {
+		if (c0 > 0) {
+			#pragma unroll
+			for (int k = 0; k < CHS; k++) { const int r = krow(IT, c0 + k); if (c0 + k < IT && r < n) sp_load(g, V.J, (const T*)V.s, r, true, q[k]); }
+		}
+		#pragma unroll
+		for (int k = 0; k < CHS; k++) {
+			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
+			const T tv = sp_prec(q[k], ne, cc); V.t[r] = tv; sts += (double)(tv * q[k].z[0]); stt += (double)(tv * tv);
+		}
+	}
 	put_partial(P, SL_TS, sts, sh); put_partial(P, SL_TT, stt, sh);
 }
 // I5: omega = t.s / t.t; x += alpha M^-1 p + omega M^-1 s; r = s - omega t; partials of r.r and r0.r
-template <typename T>
+template <typename T, int IT>
 __global__ void __launch_bounds__(256) kb_xr(Geo g, KV<T> V, KS* ks, KP P, int it) {
 	if (ks->done) return;
 	const int nxt = (it + 1) & 1;
 	if (ks->early) { if (blockIdx.x == 0 && threadIdx.x == 0) ks->done = 1; return; }
 	__shared__ double sh[4];
 	const int n = g.n;
+	constexpr int CHV = chv<T>();
+	T dv[CHV], pv[CHV], sv[CHV], tv[CHV], xv[CHV], r0v[CHV];
+	auto load = [&](int c0) {
+		#pragma unroll
+		for (int k = 0; k < CHV; k++) {
+			const int r = krow(IT, c0 + k);
+			if (c0 + k < IT && r < n) { dv[k] = V.J[r]; pv[k] = V.p[r]; sv[k] = V.s[r]; tv[k] = V.t[r]; xv[k] = V.x[r]; r0v[k] = V.r0[r]; }
+		}
+	};
+	load(0);
 	const double ts = grid_total(P, SL_TS, sh), tt = grid_total(P, SL_TT, sh);
 	const T omega = (T)ts / (T)tt;
 	const T alpha = (T)ks->alpha[nxt], ccp = (T)ks->cc_p, ccs = (T)ks->cc_s;
 	if (blockIdx.x == 0 && threadIdx.x == 0) { ks->omega[nxt] = (double)omega; ks->iters = it + 1; }
 	double srr = 0.0, sr0r = 0.0;
-	ROWS_BEGIN
-		const T di = dinv_of(V.J, r);
-		const T ph = di * V.p[r] + ccp, shv = di * V.s[r] + ccs;
-		V.x[r] += alpha * ph + omega * shv;
-		const T rv = V.s[r] - omega * V.t[r]; V.r[r] = rv;
-		srr += (double)(rv * rv); sr0r += (double)(V.r0[r] * rv);
-	ROWS_END
+	#pragma unroll
+	for (int c0 = 0; c0 < IT; c0 += CHV) {
+		if (c0 > 0) load(c0);
+		#pragma unroll
+		for (int k = 0; k < CHV; k++) {
+			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
+			const T di = dinv_v(dv[k]);
+			const T ph = di * pv[k] + ccp, shv = di * sv[k] + ccs;
+			V.x[r] = xv[k] + (alpha * ph + omega * shv);
+			const T rv = sv[k] - omega * tv[k]; V.r[r] = rv;
+			srr += (double)(rv * rv); sr0r += (double)(r0v[k] * rv);
+		}
+	}
 	put_partial(P, SL_RR, srr, sh); put_partial(P, SL_R0R, sr0r, sh);
 }
 
@@ -450,34 +547,48 @@ int krylov_items(int n) {
 	while (it < 32 && (long)n > (long)256 * it * 2048) it *= 2;
 	return it;
 }
-
-// FP32 form: no summation order to match; about 1k blocks measured best at 2M rows (fewer blocks lose memory
-// parallelism faster than they save partial re-reductions)
+// FP32 form: no summation order to match; about 1k blocks measured best at 2M rows
 static int krylov_items32(int n) {
 	int it = 4;
-	while (it < 128 && (long)n > (long)256 * it * 1024) it *= 2;
+	while (it < 64 && (long)n > (long)256 * it * 1024) it *= 2;
 	return it;
 }
 template <typename T> static int items_of(int n) { return sizeof(T) == 8 ? krylov_items(n) : krylov_items32(n); }
+// calls f with the rows-per-thread count as a compile-time constant
+template <typename F> static void with_items(int items, F f) {
+	switch (items) {
+	case 4: f(std::integral_constant<int, 4>{}); break;
+	case 8: f(std::integral_constant<int, 8>{}); break;
+	case 16: f(std::integral_constant<int, 16>{}); break;
+	case 32: f(std::integral_constant<int, 32>{}); break;
+	default: f(std::integral_constant<int, 64>{}); break;
+	}
+}
 template <typename T>
 static void krylov_setup_t(const Geo& g, const KrylovBufsT<T>& B, const T* b, hipStream_t st) {
 	const int items = items_of<T>(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
-	KP P{B.part, B.stride, nb, items};
+	KP P{B.part, B.stride, nb};
 	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J};
-	hipLaunchKernelGGL(kb_setup1<T>, dim3(nb), dim3(256), 0, st, g, B.J, b, P);
-	hipLaunchKernelGGL(kb_setup2<T>, dim3(nb), dim3(256), 0, st, g, B.J, b, V, B.ks, P);
-	hipLaunchKernelGGL(kb_setup3<T>, dim3(nb), dim3(256), 0, st, g, b, V, P);
+	with_items(items, [&](auto I) {
+		constexpr int IT = decltype(I)::value;
+		hipLaunchKernelGGL((kb_setup1<T, IT>), dim3(nb), dim3(256), 0, st, g, B.J, b, P);
+		hipLaunchKernelGGL((kb_setup2<T, IT>), dim3(nb), dim3(256), 0, st, g, B.J, b, V, B.ks, P);
+		hipLaunchKernelGGL((kb_setup3<T, IT>), dim3(nb), dim3(256), 0, st, g, b, V, P);
+	});
 }
 template <typename T>
 static void krylov_iter_t(const Geo& g, const KrylovBufsT<T>& B, hipStream_t st, int it, double tol, int maxit) {
 	const int items = items_of<T>(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
-	KP P{B.part, B.stride, nb, items};
+	KP P{B.part, B.stride, nb};
 	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J};
-	hipLaunchKernelGGL(kb_p<T>, dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol, maxit);
-	hipLaunchKernelGGL(kb_v<T>, dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
-	hipLaunchKernelGGL(kb_s<T>, dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
-	hipLaunchKernelGGL(kb_t<T>, dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol);
-	hipLaunchKernelGGL(kb_xr<T>, dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+	with_items(items, [&](auto I) {
+		constexpr int IT = decltype(I)::value;
+		hipLaunchKernelGGL((kb_p<T, IT>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol, maxit);
+		hipLaunchKernelGGL((kb_v<T, IT>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+		hipLaunchKernelGGL((kb_s<T, IT>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+		hipLaunchKernelGGL((kb_t<T, IT>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol);
+		hipLaunchKernelGGL((kb_xr<T, IT>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+	});
 }
 void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hipStream_t st) { krylov_setup_t<double>(g, B, b, st); }
 void launch_krylov_iter(const Geo& g, const KrylovBufs& B, hipStream_t st, int it, double tol, int maxit) { krylov_iter_t<double>(g, B, st, it, tol, maxit); }

@@ -134,6 +134,9 @@ typedef struct spb6_kernel_times {
     double explicit_ms, prep_ms, shock_ms, residual_ms, jacobian_ms, krylov_ms, final_ms, other_ms;
     long explicit_n, prep_n, shock_n, residual_n, jacobian_n, krylov_n, final_n;
     long krylov_iters;
+    double krylov_aux_ms;        /* BiCGStab setup, FP32 conversions and FP64 defects (krylov_ms: iteration passes only) */
+    double guess_ms;             /* learned-guess features and inference */
+    long krylov_launched;        /* BiCGStab iterations launched (a pass launched after convergence returns at once) */
 } spb6_kernel_times_t;
 int spb6_enable_timing(spb6_ctx* ctx, int on);
 int spb6_get_kernel_times(spb6_ctx* ctx, spb6_kernel_times_t* t);
