@@ -260,12 +260,14 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(FeatCtx X, int nro
 	__shared__ uint4 sW2[MLP_H * MLP_H / 8];        // 32 KB
 	__shared__ __bf16 sH[INF_WAVES][16][MLP_H + 8];  // per-wave hidden activations (A layout source), 34 KB
 	__shared__ float sMean[MLP_KPAD], sIstd[MLP_KPAD];
+	__shared__ float sF[INF_WAVES][7][5][20];       // per-wave 5x5 stencils of a 16-cell tile (20 x 5 cells x 7 fields), 22 KB
 	for (int e = threadIdx.x; e < MLP_H * MLP_KPAD / 8; e += 64 * INF_WAVES) sW1[e] = ((const uint4*)W1t)[e];
 	for (int e = threadIdx.x; e < MLP_H * MLP_H / 8; e += 64 * INF_WAVES) sW2[e] = ((const uint4*)W2t)[e];
 	for (int e = threadIdx.x; e < MLP_KPAD; e += 64 * INF_WAVES) { sMean[e] = smean[e]; sIstd[e] = sistd[e]; }
 	__syncthreads();
 	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
 	const int ntiles = (nrows + 15) / 16;
+	const bool staged = mode == 0 && X.g.nx % 16 == 0 && nrows % 16 == 0;
 	for (int tile = blockIdx.x * INF_WAVES + wave; tile < ntiles; tile += gridDim.x * INF_WAVES) {
 		const int q = tile * 16 + lr;           // this lane's A row (cell)
 		const bool valid = q < nrows;
@@ -276,7 +278,21 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(FeatCtx X, int nro
 		}
 		Centre C{};
 		int cq = 0;                              // ghosted index of the row's cell
-		if (valid && mode == 0) { C = centre_of(X, q); cq = (q % X.g.nx + GH) + (q / X.g.nx + GH) * X.g.NXT; }
+		if (staged) {
+			// the tile's 16 cells lie in one grid row: stage the 20 x 5 cells x 7 fields of their stencils in LDS with
+			// coalesced loads, one round trip, and take every feature from there
+			const int q0 = tile * 16, i0 = q0 % X.g.nx, j0 = q0 / X.g.nx;
+			for (int e = lane; e < 700; e += 64) {
+				const int f = e / 100, rem = e - 100 * f, dy = rem / 20, dx = rem - 20 * dy;
+				sF[wave][f][dy][dx] = X.F[f * X.g.NC + (i0 + dx) + (j0 + dy) * X.g.NXT];   // ghosted (i0 - 2 + dx + GH, j0 - 2 + dy + GH)
+			}
+			cq = (q % X.g.nx + GH) + (q / X.g.nx + GH) * X.g.NXT;
+			C.iK = 1.f / (float)X.Kw[cq]; const double cz = X.cz[cq]; C.icz = 1.f / (float)(isfin(cz) && cz > 0 ? cz : 1.0);
+			__builtin_amdgcn_s_waitcnt(0);
+			__builtin_amdgcn_wave_barrier();
+			C.F1c = sF[wave][1][2][lr + 2]; C.F2c = sF[wave][2][2][lr + 2]; C.F3c = sF[wave][3][2][lr + 2]; C.F5c = sF[wave][5][2][lr + 2];
+		}
+		else if (valid && mode == 0) { C = centre_of(X, q); cq = (q % X.g.nx + GH) + (q / X.g.nx + GH) * X.g.NXT; }
 		// layer 1: Z1[16 x 128] = X[16 x 224] W1[224 x 128]
 		f32x4 acc1[8];
 		#pragma unroll
@@ -289,7 +305,8 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(FeatCtx X, int nro
 				const int p = 8 * ks + e, f = p / 7, sp = 4 * (p % 7) + lk, t = ks * 32 + lk * 8 + e;
 				float x = 0.f;
 				if (f < 7 && sp < 25 && valid) {
-					if (mode == 0) {
+					if (staged) x = (feat_tr(f, sF[wave][f][sp / 5][lr + sp % 5], C) - sMean[t]) * sIstd[t];
+					else if (mode == 0) {
 						const int c = cq + (sp % 5 - 2) + (sp / 5 - 2) * X.g.NXT;
 						x = (feat_tr(f, X.F[f * X.g.NC + c], C) - sMean[t]) * sIstd[t];
 					} else x = Xin[(long)q * MLP_IN + 7 * sp + f];
@@ -378,8 +395,8 @@ void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float*
                const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st) {
 	FeatCtx fx{};
 	if (D) { fx.g = D->g; fx.F = F; fx.Kw = D->Kw; fx.cz = D->cz; fx.tau_dx = tau / D->g.dx; fx.G = Gplain; fx.Pplain = Pplain; fx.P_REF = D->P_REF; fx.thr = thr; }
-	int dev = 0; hipGetDevice(&dev);
-	int cus = 256; hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+	int dev = 0; (void)hipGetDevice(&dev);
+	int cus = 256; (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
 	const int tiles = (nrows + 15) / 16;
 	const int blocks = std::max(1, std::min(cus, (tiles + INF_WAVES - 1) / INF_WAVES));
 	hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * INF_WAVES), 0, st, fx, nrows, Xin, Xin ? 1 : 0, W.smean, W.sistd, W.W1t,
