@@ -37,7 +37,12 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU leg)")
+    ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU legs; legs default to the 2D one)")
+    ap.add_argument("--legs", default=None,
+                    help="comma list of shock-bubble (the headline 2048x1024 leg), interface, batch; default all three "
+                         "(shock-bubble only with --profile)")
+    ap.add_argument("--interface-steps", type=int, default=6,
+                    help="timed steps of the 4096x2048 perturbed-interface leg, plain vs learned + FP32 Krylov (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=8, help="timed steps of the batched 1D leg (0: skip the leg)")
     ap.add_argument("--batch-warmup", type=int, default=12)
     ap.add_argument("--batch-cpu-problems", type=int, default=64)
@@ -84,7 +89,13 @@ def run_form(a, form, keep_state=False):
     return res, st, cfg
 
 
-def roofline_of(res, n, fp32):
+# HBM bytes per BiCGStab iteration measured by rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over the five passes,
+# calibrated on the FP32 conversion kernel of the 5n-entry Jacobian (known bytes: FETCH_SIZE reads x 2.00, WRITE_SIZE
+# x 1.00); commands and per-pass counts in profiles/krylov_fp32_iteration_2048x1024_traffic.md
+TRAFFIC = {(True, 2048, 1024): 320248722}
+
+
+def roofline_of(res, n, fp32, nx=2048, ny=1024):
     kt, d = res["kt"], res["d"]
     kb = BYTES_2D["krylov_iter"] // (2 if fp32 else 1)
     cats = {
@@ -104,8 +115,9 @@ def roofline_of(res, n, fp32):
     ms, cnt, byts = cats[dom]
     avg_s = (ms / 1e3) / max(cnt, 1)
     achieved = byts / avg_s if avg_s > 0 else 0.0
+    traffic = TRAFFIC.get((fp32, nx, ny)) if dom == "krylov_iter" else None
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None, "bytes_per_launch": byts, "avg_launch_ms": avg_s * 1e3,
+            "frac": achieved / HBM_PEAK, "traffic": traffic, "bytes_per_launch": byts, "avg_launch_ms": avg_s * 1e3,
             "launches": cnt}
     step_bytes = (BYTES_2D["explicit"] + BYTES_2D["prep"] + BYTES_2D["final"]) * d["stages"] + BYTES_2D["jacobian"] * d["newton_it"] \
         + BYTES_2D["residual"] * (d["stages"] + d["newton_it"]) + kb * d["krylov_it"]
@@ -114,10 +126,23 @@ def roofline_of(res, n, fp32):
 
 def main():
     a = parse()
+    if a.legs is None:
+        a.legs = "shock-bubble" if a.profile else "shock-bubble,interface,batch"
+    legs = set(x for x in a.legs.split(",") if x)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); multi-GPU sharding is design-only", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
+    out = run_headline(a) if "shock-bubble" in legs else {"legs": sorted(legs)}
+    if a.interface_steps > 0 and "interface" in legs:
+        out["interface"] = run_interface(a)
+    if a.batch_steps > 0 and "batch" in legs:
+        out["batch"] = run_batch(a)
+    print(json.dumps(out))
+
+
+def run_headline(a):
+    """The 2048x1024 shock-bubble leg in each form (plain FP64, FP32 Krylov, learned guess + FP32 Krylov): the JSON
+    contract fields, with the head form's dominant-kernel roofline and the oracle CPU baseline."""
     n = a.nx * a.ny
     if a.forms is None:
         from diffusionimex_amd import mlp
@@ -137,7 +162,7 @@ def main():
             cfg = c
     head = "learned" if "learned" in results else ("mixed" if "mixed" in results and "plain" not in results else "plain")
     r = results[head]
-    roof, kernel_ms, hbm_frac = roofline_of(r, n, head != "plain")
+    roof, kernel_ms, hbm_frac = roofline_of(r, n, head != "plain", a.nx, a.ny)
     forms_out = {}
     for form, res in results.items():
         fo = {k: res[k] for k in ("value", "ms_per_step", "newton_per_stage", "krylov_per_newton", "krylov_per_step")}
@@ -163,11 +188,49 @@ def main():
         if st_plain is None:
             _, st_plain, cfg = run_form(a, "plain")
         out["cpu_baseline"] = cpu_baseline(a, cfg, st_plain)
-    if a.batch_steps > 0 and not a.profile:
-        if st_plain is not None:
-            st_plain.close()
-        out["batch"] = run_batch(a)
-    print(json.dumps(out))
+    if st_plain is not None:
+        st_plain.close()
+    return out
+
+
+def run_interface(a):
+    """4096x2048 Mach-1.5 shock on a perturbed water-gas interface (seeded amplitude and mode), Newton tolerance 1e-10:
+    the plain FP64 form against the learned guess + FP32 Krylov form, same warm-up and timed steps."""
+    from diffusionimex_amd import spb6, cases, mlp
+    res, states = {}, {}
+    forms = ["plain", "learned"] if os.path.exists(mlp.WEIGHTS) else ["plain", "mixed"]
+    for form in forms:
+        kw = {} if form == "plain" else {"krylov_fp32": 1}
+        if form == "learned":
+            kw["learned_guess"] = 1
+        cfg, U = cases.perturbed_interface(4096, 2048, seed=a.seed, cfl=0.5, newton_tol=1e-10, **kw)
+        st = spb6.Stepper(cfg, device=0)
+        if form == "learned":
+            mlp.load_into(st)
+        st.set_state(U.reshape(cfg.ny, cfg.nx, 8))
+        if st.run(1.0, 10) != 0:
+            raise SystemExit(f"interface {form}: warm-up failed")
+        s0 = st.stats()
+        t0 = time.perf_counter()
+        rc = st.run(1.0, a.interface_steps)
+        wall = time.perf_counter() - t0
+        s1 = st.stats()
+        if rc != 0:
+            raise SystemExit(f"interface {form}: timed run failed")
+        d = {k: s1[k] - s0[k] for k in ("steps", "stages", "newton_it", "krylov_it", "linsolves")}
+        res[form] = {"value": cfg.nx * cfg.ny * d["steps"] / wall, "ms_per_step": 1e3 * wall / d["steps"],
+                     "newton_per_stage": d["newton_it"] / max(d["stages"], 1),
+                     "krylov_per_newton": d["krylov_it"] / max(d["linsolves"], 1),
+                     "krylov_per_step": d["krylov_it"] / max(d["steps"], 1)}
+        states[form] = st.state().reshape(-1, 8)
+        st.close()
+    other = forms[1]
+    A, B = states[other], states["plain"]
+    res[other]["field_l1_vs_plain"] = [float(np.abs(A[:, k] - B[:, k]).sum() / max(np.abs(B[:, k]).sum(), 1e-300)) for k in range(8)]
+    res[other]["speedup_vs_plain"] = res[other]["value"] / res["plain"]["value"]
+    res["workload"] = "4096x2048 shock on a perturbed water-gas interface, Newton tol 1e-10, 10 warm-up steps"
+    res["steps"] = a.interface_steps
+    return res
 
 
 def run_batch(a):
@@ -197,7 +260,7 @@ def run_batch(a):
            "ms_per_step": 1e3 * wall / a.batch_steps, "newton_per_stage": d["newton_it"] / max(d["stages"], 1),
            "rejects": d["rejects"], "launch": {"threads": tune["threads"], "per_cu": tune["per_cu"]},
            "autotune_ms": {f"{t}x{c}": round(ms, 1) for t, c, ms in tune["tried"]}}
-    if not a.no_cpu:
+    if not a.no_cpu and not a.profile:
         res["cpu_baseline"] = batch_cpu_baseline(a, scheme, probs, b, per1)
     b.close()
     return res
