@@ -146,27 +146,43 @@ def collect(runs, steps, stride, warm=0, capacity=4_000_000):
     return np.concatenate(Xs), np.concatenate(Ys)
 
 
-def standardise(X, Y):
-    """Feature and target statistics: norm = [mean(175), std(175), mean_t, std_t]; returns (Xs, Ys, norm)."""
-    mu = X.mean(axis=0, dtype=np.float64)
-    sd = X.std(axis=0, dtype=np.float64)
+def standardise(X, Y, sample=1_000_000, seed=0, chunk=1_000_000):
+    """Feature and target statistics, norm = [mean(175), std(175), mean_t, std_t], from at most `sample` rows; X and Y
+    are standardised in place (chunk by chunk, FP32). Returns (X, Y, norm)."""
+    rs = np.random.default_rng(seed)
+    idx = rs.choice(X.shape[0], size=min(sample, X.shape[0]), replace=False) if X.shape[0] > sample else slice(None)
+    Xs, Ys = X[idx].astype(np.float64), Y[idx].astype(np.float64)
+    mu, sd = Xs.mean(axis=0), Xs.std(axis=0)
     sd = np.where(sd > 1e-12, sd, 1.0)
-    mt, st = float(Y.mean(dtype=np.float64)), float(Y.std(dtype=np.float64))
+    mt, st = float(Ys.mean()), float(Ys.std())
     st = st if st > 0 else 1.0
     norm = np.concatenate([mu, sd, [mt, st]]).astype(np.float32)
-    Xs = ((X - mu.astype(np.float32)) / sd.astype(np.float32)).astype(np.float32)
-    Ys = ((Y - mt) / st).astype(np.float32)
-    return Xs, Ys, norm
+    m32, s32 = mu.astype(np.float32), sd.astype(np.float32)
+    for a in range(0, X.shape[0], chunk):
+        X[a:a + chunk] -= m32
+        X[a:a + chunk] /= s32
+    Y -= np.float32(mt)
+    Y /= np.float32(st)
+    return X, Y, norm
 
 
-def train_network(X, Y, epochs=20, batch=8192, lr=1e-3, seed=0, device=0):
-    """Standardise, then train a fresh network on the GPU; returns (network, norm, loss per epoch)."""
-    Xs, Ys, norm = standardise(X, Y)
+def train_network(X, Y, epochs=20, batch=8192, lr=1e-3, seed=0, device=0, log=None, rounds=None):
+    """Standardise (in place), then train a fresh network on the GPU; returns (network, norm, loss per epoch). With
+    `log`, training runs in rounds of a few epochs and log(message) is called after each."""
+    Xs, Ys, norm = standardise(X, Y, seed=seed)
     net = Network(device)
     net.set_params(init_params(seed), norm)
     net.set_data(Xs, Ys)
-    loss = net.train(epochs, batch=batch, lr=lr, seed=seed)
-    return net, norm, loss
+    if log is None:
+        return net, norm, net.train(epochs, batch=batch, lr=lr, seed=seed)
+    rounds = rounds or max(1, epochs // 4)
+    losses, done = [], 0
+    while done < epochs:
+        k = min(rounds, epochs - done)
+        losses.append(net.train(k, batch=batch, lr=lr, seed=seed + done))
+        done += k
+        log(f"epochs {done}/{epochs}: loss {losses[-1][-1]:.4e}")
+    return net, norm, np.concatenate(losses)
 
 
 def save(net, norm, path=WEIGHTS, **meta):

@@ -32,13 +32,13 @@ def main(argv=None):
     p.add_argument("--out", default=mlp.WEIGHTS)
     p.add_argument("--nx", type=int, default=2048)
     p.add_argument("--ny", type=int, default=1024)
-    p.add_argument("--bubble-seeds", type=lambda s: [int(x) for x in s.split(",") if x], default=[1, 2, 3])
-    p.add_argument("--interface-seeds", type=lambda s: [int(x) for x in s.split(",") if x], default=[])
+    p.add_argument("--bubble-seeds", type=lambda s: [int(x) for x in s.split(",") if x], default=[1, 2, 3, 4, 5, 6, 7, 8])
+    p.add_argument("--interface-seeds", type=lambda s: [int(x) for x in s.split(",") if x], default=[1, 2, 3, 4])
     p.add_argument("--warm", type=int, default=12, help="undumped steps first (start-up ramp)")
-    p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--stride", type=int, default=2)
-    p.add_argument("--capacity", type=int, default=3_000_000, help="rows per run")
-    p.add_argument("--epochs", type=int, default=30)
+    p.add_argument("--steps", type=int, default=30)
+    p.add_argument("--stride", type=int, default=1)
+    p.add_argument("--capacity", type=int, default=6_000_000, help="rows per run")
+    p.add_argument("--epochs", type=int, default=16)
     p.add_argument("--batch", type=int, default=8192)
     p.add_argument("--lr", type=float, default=1e-3)
     p.add_argument("--seed", type=int, default=0)
@@ -52,8 +52,10 @@ def main(argv=None):
         Xs.append(X); Ys.append(Y); names.append(name)
     X, Y = np.concatenate(Xs), np.concatenate(Ys)
     del Xs, Ys
+    print(f"{X.shape[0]} rows ({X.nbytes / 2**30:.1f} GiB of features)", flush=True)
     t1 = time.perf_counter()
-    net, norm, loss = mlp.train_network(X, Y, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed)
+    net, norm, loss = mlp.train_network(X, Y, epochs=a.epochs, batch=a.batch, lr=a.lr, seed=a.seed,
+                                        log=lambda m: print(f"{m} ({time.perf_counter() - t1:.1f} s)", flush=True), rounds=2)
     t2 = time.perf_counter()
     for e in range(0, a.epochs, max(1, a.epochs // 10)):
         print(f"epoch {e}: loss {loss[e]:.4e}", flush=True)
