@@ -348,7 +348,8 @@ __device__ __forceinline__ void wave_append(bool flag, int q, int* __restrict__ 
 // ------------------------------------------------------------------------------------------------ cell residual (PC:584-705)
 struct FaceIn { double wl, pv, ubs, kp, ube, cl[4], cr[4], utl, utr, bw; int wall; };
 struct CellIn { double xs[NV], a2s, Q0, Q1, Kw; FaceIn fe, fw, fn, fs; };
-struct FaceVals { double uf, pf, wL, wR; };
+template <class T> struct FaceValsT { T uf, pf, wL, wR; };
+using FaceVals = FaceValsT<double>;
 struct ResOut { double a0, a1, m0, m1, e0, e1, mx, my, E, sig; };
 
 __device__ __forceinline__ void load_face(const Dev& D, bool ydir, int c, int idx, FaceIn& f) {
@@ -375,46 +376,59 @@ __device__ __forceinline__ void load_cell(const Dev& D, int q, CellIn& C) {
 	load_face(D, false, c, i, C.fe); load_face(D, false, c - 1, i - 1, C.fw);
 	if (g.twoD) { load_face(D, true, c, j, C.fn); load_face(D, true, c - g.NXT, j - 1, C.fs); }
 }
-__device__ __forceinline__ void face_eval(const FaceIn& f, double PL, double PR, FaceVals& v) {   // face_x / face_y (PC:597-616)
+// The face and cell assembly below is templated on the scalar: double for the residual (the reference's operations,
+// bit for bit), D1 for the analytic Jacobian's directional derivatives.
+template <class T>
+SPB_HD void face_eval(const FaceIn& f, T PL, T PR, FaceValsT<T>& v) {   // face_x / face_y (PC:597-616)
 	v.pf = f.wl * PL + (1.0 - f.wl) * PR - f.pv;
-	if (f.wall >= 0 && f.bw == 0.0) { v.uf = 0.0; v.wL = 0.0; v.wR = 0.0; return; }
-	v.uf = f.wall >= 0 ? f.bw : f.ubs - f.kp * (PR - PL);
+	if (f.wall >= 0 && f.bw == 0.0) { v.uf = T(0.0); v.wL = T(0.0); v.wR = T(0.0); return; }
+	v.uf = f.wall >= 0 ? T(f.bw) : f.ubs - f.kp * (PR - PL);
 	v.wL = smax(v.uf, 0.0) - smax(f.ube, 0.0); v.wR = smin(v.uf, 0.0) - smin(f.ube, 0.0);   // face_weights (PC:592)
 }
-__device__ __forceinline__ double finc(const FaceIn& f, const FaceVals& v, int k) { return f.cl[k] * v.wL + f.cr[k] * v.wR; }   // inc (PC:617), k in m1 m2 e1 e2
+template <class T>
+SPB_HD T finc(const FaceIn& f, const FaceValsT<T>& v, int k) { return f.cl[k] * v.wL + f.cr[k] * v.wR; }   // inc (PC:617), k in m1 m2 e1 e2
 // face_flux (PC:620-625): increment mass, tangential momentum and energy fluxes
-__device__ __forceinline__ void fflux(const FaceIn& f, const FaceVals& v, double P_REF, double& M, double& Mt, double& E) {
-	const double ML = (f.cl[0] + f.cl[1]) * v.wL, MR = (f.cr[0] + f.cr[1]) * v.wR;
+template <class T>
+SPB_HD void fflux(const FaceIn& f, const FaceValsT<T>& v, double P_REF, T& M, T& Mt, T& E) {
+	const T ML = (f.cl[0] + f.cl[1]) * v.wL, MR = (f.cr[0] + f.cr[1]) * v.wR;
 	M = ML + MR; Mt = f.utl * ML + f.utr * MR;
 	E = finc(f, v, 2) + finc(f, v, 3) + 0.5 * v.uf * v.uf * M + 0.5 * (f.utl * f.utl * ML + f.utr * f.utr * MR) + (P_REF + v.pf) * v.uf;
 }
-// G = (pbar_cl - P_REF) - Pc of one cell (cell_residual, PC:640-679)
-__device__ __forceinline__ bool cell_res(const Phys& P, double P_REF, double rx, double ry, bool twoD, const CellIn& C,
-                                         double Pc, double PW, double PE, double PS, double PN, double& G, ResOut* out) {
-	FaceVals fe, fw, fn, fs;
+// closure inputs of one cell at the given pressures (cell_residual, PC:640-672): X = transported (alpha_1, alpha_2, m1,
+// m2, eps_1, eps_2), the volume change s, the momenta and the total energy
+template <class T>
+SPB_HD void cell_assemble(double P_REF, double rx, double ry, bool twoD, const CellIn& C, T Pc, T PW, T PE,
+                                              T PS, T PN, T X[6], T& s, T& mx, T& my, T& E) {
+	FaceValsT<T> fe, fw, fn, fs;
 	face_eval(C.fe, Pc, PE, fe); face_eval(C.fw, PW, Pc, fw);
 	if (twoD) { face_eval(C.fn, Pc, PN, fn); face_eval(C.fs, PS, Pc, fs); }
-	double X[6] = {C.xs[IA1], C.a2s, C.xs[IM1], C.xs[IM2], C.xs[IE1], C.xs[IE2]};
+	X[0] = T(C.xs[IA1]); X[1] = T(C.a2s); X[2] = T(C.xs[IM1]); X[3] = T(C.xs[IM2]); X[4] = T(C.xs[IE1]); X[5] = T(C.xs[IE2]);
 	#pragma unroll
 	for (int k = 2; k < 6; k++) {
 		X[k] -= rx * (finc(C.fe, fe, k - 2) - finc(C.fw, fw, k - 2));
 		if (twoD) X[k] -= ry * (finc(C.fn, fn, k - 2) - finc(C.fs, fs, k - 2));
 	}
 	X[1] = 1.0 - X[0];
-	const double s = rx * (fe.uf - fw.uf) + (twoD ? ry * (fn.uf - fs.uf) : 0.0);
+	s = rx * (fe.uf - fw.uf) + (twoD ? ry * (fn.uf - fs.uf) : T(0.0));
 	X[0] *= 1.0 - s; X[1] *= 1.0 - s;
-	double Me, Mte, Ee, Mw, Mtw, Ew;
+	T Me, Mte, Ee, Mw, Mtw, Ew;
 	fflux(C.fe, fe, P_REF, Me, Mte, Ee); fflux(C.fw, fw, P_REF, Mw, Mtw, Ew);
-	double mx = C.xs[IMU] - rx * (fe.pf - fw.pf) - rx * (fe.uf * Me - fw.uf * Mw);
-	double my = C.xs[IMV] - rx * (Mte - Mtw);
-	double E = C.xs[IEN] - rx * (Ee - Ew);
+	mx = C.xs[IMU] - rx * (fe.pf - fw.pf) - rx * (fe.uf * Me - fw.uf * Mw);
+	my = C.xs[IMV] - rx * (Mte - Mtw);
+	E = C.xs[IEN] - rx * (Ee - Ew);
 	if (twoD) {
-		double Mn, Mtn, En, Ms, Mts, Es;
+		T Mn, Mtn, En, Ms, Mts, Es;
 		fflux(C.fn, fn, P_REF, Mn, Mtn, En); fflux(C.fs, fs, P_REF, Ms, Mts, Es);
 		my -= ry * (fn.pf - fs.pf) + ry * (fn.uf * Mn - fs.uf * Ms);
 		mx -= ry * (Mtn - Mts);
 		E -= ry * (En - Es);
 	}
+}
+// G = (pbar_cl - P_REF) - Pc of one cell (cell_residual, PC:640-679)
+SPB_HD bool cell_res(const Phys& P, double P_REF, double rx, double ry, bool twoD, const CellIn& C,
+                                         double Pc, double PW, double PE, double PS, double PN, double& G, ResOut* out) {
+	double X[6], s, mx, my, E;
+	cell_assemble(P_REF, rx, ry, twoD, C, Pc, PW, PE, PS, PN, X, s, mx, my, E);
 	const double rho = X[2] + X[3];
 	if (!(rho > 0)) return false;
 	ClIn in; in.atr0 = X[0]; in.atr1 = X[1]; in.mtr0 = X[2]; in.mtr1 = X[3]; in.etr0 = X[4]; in.etr1 = X[5];
@@ -482,6 +496,35 @@ __device__ __forceinline__ bool jacobian_at(const Dev& D, const double* __restri
 	}
 	J[q] = v0; J[n + q] = vW; J[2 * n + q] = vE;
 	if (twoD) { J[3 * n + q] = vS; J[4 * n + q] = vN; }
+	return ok;
+}
+// analytic Jacobian row q (the fast forms): the residual's closure is already solved at P, so each entry follows from
+// the implicit function theorem at that root, dG/dP_j = -(dPhi/dP_j) / (dPhi/dp) - [j = self], with dPhi/dP_j carried
+// through the face and cell assembly in D1 and no closure iterations. Same entries (self, W, E, S, N), layout and
+// rhs = -G as jacobian_at; a kinked flux (upwind switch at u_f = 0) takes the forward difference's side. False where
+// dPhi/dp is not negative or an entry is not finite.
+__device__ __forceinline__ bool jacobian_analytic_at(const Dev& D, const double* __restrict__ P, const double* __restrict__ G,
+                                                     double* __restrict__ J, double* __restrict__ rhs, int q) {
+	const int n = D.g.n; const bool twoD = D.g.twoD;
+	const Nb b = neighbours(D.g, q);
+	CellIn C; load_cell(D, q, C);
+	const double Pv[5] = {P[q], P[b.W], P[b.E], twoD ? P[b.S] : 0.0, twoD ? P[b.N] : 0.0};
+	const double Gq = G[q];
+	rhs[q] = -Gq;
+	const double p = (Gq + Pv[0]) + D.P_REF;   // the closure's root at P: G = (pbar - P_REF) - Pc
+	bool ok = true;
+	#pragma unroll 1
+	for (int j = 0; j < (twoD ? 5 : 3); j++) {
+		D1 X[6], s, mx, my, E;
+		cell_assemble(D.P_REF, D.rx, D.ry, twoD, C, D1(Pv[0], j == 0), D1(Pv[1], j == 1), D1(Pv[2], j == 2), D1(Pv[3], j == 3),
+		              D1(Pv[4], j == 4), X, s, mx, my, E);
+		const D1 rhoeC = E - 0.5 * (mx * mx + my * my) / (X[2] + X[3]);
+		double dphidp;
+		const D1 phi = closure_phi(D.ph, X, s, rhoeC, C.Q0, C.Q1, p, dphidp);
+		const double v = -phi.d / dphidp - (j == 0 ? 1.0 : 0.0);
+		ok = ok && dphidp < 0.0 && isfin(v);
+		J[j * n + q] = v;
+	}
 	return ok;
 }
 // round-off floor acceptance test of row q (PC:834-842): false where the residual exceeds tol N_i + sigma_i

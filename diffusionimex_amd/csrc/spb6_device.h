@@ -24,6 +24,30 @@ SPB_HD double smax(double a, double b) { return (a < b) ? b : a; }
 SPB_HD double smin(double a, double b) { return (b < a) ? b : a; }
 SPB_HD bool isfin(double x) { return __builtin_isfinite(x); }
 
+// forward-mode derivative along one direction (value v, derivative d), for the analytic Newton Jacobian. smax / smin
+// with a constant take the right derivative at the kink (the limit of a forward difference with a positive step).
+struct D1 {
+	double v, d;
+	SPB_HD D1(double x = 0.0, double dx = 0.0) : v(x), d(dx) {}
+};
+SPB_HD D1 operator+(D1 a, D1 b) { return D1(a.v + b.v, a.d + b.d); }
+SPB_HD D1 operator-(D1 a, D1 b) { return D1(a.v - b.v, a.d - b.d); }
+SPB_HD D1 operator*(D1 a, D1 b) { return D1(a.v * b.v, a.d * b.v + a.v * b.d); }
+SPB_HD D1 operator/(D1 a, D1 b) { const double q = a.v / b.v; return D1(q, (a.d - q * b.d) / b.v); }
+SPB_HD D1 operator+(D1 a, double b) { return D1(a.v + b, a.d); }
+SPB_HD D1 operator+(double a, D1 b) { return D1(a + b.v, b.d); }
+SPB_HD D1 operator-(D1 a, double b) { return D1(a.v - b, a.d); }
+SPB_HD D1 operator-(double a, D1 b) { return D1(a - b.v, -b.d); }
+SPB_HD D1 operator*(D1 a, double b) { return D1(a.v * b, a.d * b); }
+SPB_HD D1 operator*(double a, D1 b) { return D1(a * b.v, a * b.d); }
+SPB_HD D1 operator/(D1 a, double b) { return D1(a.v / b, a.d / b); }
+SPB_HD D1& operator-=(D1& a, D1 b) { a = a - b; return a; }
+SPB_HD D1& operator*=(D1& a, D1 b) { a = a * b; return a; }
+SPB_HD D1 smax(D1 a, double b) { return a.v < b ? D1(b) : (a.v > b ? a : D1(a.v, smax(a.d, 0.0))); }
+SPB_HD D1 smin(D1 a, double b) { return b < a.v ? D1(b) : (a.v < b ? a : D1(a.v, smin(a.d, 0.0))); }
+SPB_HD double val(double x) { return x; }
+SPB_HD double val(D1 x) { return x.v; }
+
 // ------------------------------------------------------------------ portable exp / log / pow / hypot
 // exp: x = k ln2 + r (Cody-Waite, ln2 split in a 32-bit head and a tail), |r| <= ln2/2, Taylor degree 13, exact 2^k.
 // log: x = 2^e m with m in [sqrt(1/2), sqrt(2)), f = m - 1, s = f/(2+f), log m = 2 atanh s = f - s (f - 2 R),
@@ -347,6 +371,34 @@ SPB_HD bool closure_p(const Phys& P, const ClIn& in, ClOut& o) {
 	o.e1 = eps_amp(P, 1, o.a1, in.mtr1, p);
 	o.pbar = p;
 	return isfin(p);
+}
+// closure_p's Newton function Phi(p) = sum_k alpha_k(p) - 1 (PHO:472-548) at a fixed p, with the inputs X (transported
+// alpha_1, alpha_2, m1, m2, eps_1, eps_2), s and (rho e)^C carrying one directional derivative: returns Phi and its
+// derivative along that direction, and dphidp = dPhi/dp at fixed inputs (closure_p's df). The defect segment is the one
+// closure_p selects at p, so within it lam and the extra share do not depend on p.
+SPB_HD D1 closure_phi(const Phys& P, const D1 X[6], D1 s, D1 rhoeC, double Q0, double Q1, double p, double& dphidp) {
+	const D1 D0 = rhoeC - X[4] - X[5];
+	const D1 sm = X[2] + X[3];
+	const D1 E0 = smax(X[4] - X[2] * P.q[0] - (X[0] - X[2] * P.b[0]) * P.pi[0], 0.0);   // E_above (PHY:420-423)
+	const D1 E1 = smax(X[5] - X[3] * P.q[1] - (X[1] - X[3] * P.b[1]) * P.pi[1], 0.0);
+	const D1 sE = E0 + E1;
+	D1 lE0, lE1;
+	if (sE.v > 0) { lE0 = E0 / sE; lE1 = E1 / sE; }
+	else if (sm.v > 0) { lE0 = X[2] / sm; lE1 = X[3] / sm; }
+	else { lE0 = D1(0.5); lE1 = D1(0.5); }
+	const double Qb = Q0 + Q1;
+	const D1 delta = D0 + p * s;
+	D1 lam0, lam1, ex0, ex1;   // defect_seg (PHY:428-439)
+	if (!(delta.v > 0.0) || !(Qb > 0.0)) { lam0 = lE0; lam1 = lE1; }
+	else if (delta.v < Qb) { lam0 = D1(Q0 / Qb); lam1 = D1(Q1 / Qb); }
+	else { lam0 = lE0; lam1 = lE1; ex0 = Q0 - Qb * lE0; ex1 = Q1 - Qb * lE1; }
+	const double g0 = P.g[0], g1 = P.g[1], pi0 = P.pi[0], pi1 = P.pi[1], b0 = P.b[0], b1 = P.b[1];
+	const D1 A0 = (X[4] - X[2] * P.q[0]) + lam0 * D0 + ex0, B0 = X[0] + lam0 * s, N0 = A0 - (B0 - X[2] * b0) * pi0;
+	const D1 A1 = (X[5] - X[3] * P.q[1]) + lam1 * D0 + ex1, B1 = X[1] + lam1 * s, N1 = A1 - (B1 - X[3] * b1) * pi1;
+	const D1 f = ((g0 - 1.0) * (A0 + B0 * p) + (p + g0 * pi0) * X[2] * b0) / (g0 * (p + pi0))
+	           + ((g1 - 1.0) * (A1 + B1 * p) + (p + g1 * pi1) * X[3] * b1) / (g1 * (p + pi1));
+	dphidp = -(g0 - 1.0) * N0.v / (g0 * (p + pi0) * (p + pi0)) - (g1 - 1.0) * N1.v / (g1 * (p + pi1) * (p + pi1));
+	return f - 1.0;
 }
 // round-off floor of the closure pressure (PC:631-638)
 SPB_HD double closure_roundoff(const Phys& P, const ClIn& in, double p) {

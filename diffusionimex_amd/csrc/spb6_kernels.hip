@@ -197,6 +197,13 @@ __global__ void __launch_bounds__(256) k_jacobian(Dev D, const double* __restric
 	if (q >= D.g.n) return;
 	if (!jacobian_at(D, P, G, sc, J, rhs, q)) flags[FLAG_JAC] = 1;
 }
+// analytic Jacobian rows (the fast forms): one thread per row, no closure iterations
+__global__ void __launch_bounds__(256) k_jacobian_an(Dev D, const double* __restrict__ P, const double* __restrict__ G,
+                           double* __restrict__ J, double* __restrict__ rhs, int* __restrict__ flags) {
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= D.g.n) return;
+	if (!jacobian_analytic_at(D, P, G, J, rhs, q)) flags[FLAG_JAC] = 1;
+}
 // Newton step criterion max |dP| / (|P| + s) (PC:819-823)
 __global__ void __launch_bounds__(256) k_stepcrit(int n, const double* __restrict__ dP, const double* __restrict__ P, const double* __restrict__ sc, unsigned long long* __restrict__ nrm) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -632,9 +639,10 @@ void launch_shock(const Dev& D, int* list, int* count, int strong_blocks, hipStr
 void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st) {
 	hipLaunchKernelGGL(k_residual, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, dP, lam, Pout, G, nrm, D.flags, flag_slot);
 }
-void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, hipStream_t st) {
+void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, bool analytic, hipStream_t st) {
 	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc);
-	hipLaunchKernelGGL(k_jacobian, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, (const double*)sc, J, rhs, D.flags);
+	if (analytic) hipLaunchKernelGGL(k_jacobian_an, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, J, rhs, D.flags);
+	else hipLaunchKernelGGL(k_jacobian, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, (const double*)sc, J, rhs, D.flags);
 }
 void launch_fdscale(const Dev& D, const double* P, double* sc, hipStream_t st) {
 	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc);

@@ -32,7 +32,7 @@ class Config(C.Structure):
         ("acdt", C.c_int), ("newton_tol", C.c_double), ("newton_iter", C.c_int),
         ("krylov_tol", C.c_double), ("krylov_iter", C.c_int),
         ("limiter", C.c_int), ("order", C.c_int), ("rus_local", C.c_int),
-        ("krylov_fp32", C.c_int), ("learned_guess", C.c_int),
+        ("krylov_fp32", C.c_int), ("learned_guess", C.c_int), ("analytic_jacobian", C.c_int),
     ]
 
 
@@ -139,7 +139,8 @@ def _dp(a):
 
 def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0, 1.0), bc=(0, 0, 0, 0), bcu=(0, 0, 0, 0),
                 rk=2, cfl=0.2, cfl_ac=0.4, ramp=1, dtfix=0.0, acdt=0, newton_tol=1e-13, newton_iter=60,
-                krylov_tol=1e-13, krylov_iter=4000, limiter=1, order=2, rus_local=1, krylov_fp32=0, learned_guess=0):
+                krylov_tol=1e-13, krylov_iter=4000, limiter=1, order=2, rus_local=1, krylov_fp32=0, learned_guess=0,
+                analytic_jacobian=0):
     """Config with the reference defaults (PC:57-80): phases are (gamma, p_inf[, q, qp, cv, b])."""
     c = Config()
     c.nx, c.ny = nx, ny
@@ -155,6 +156,7 @@ def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0
     c.rk, c.cfl, c.cfl_ac, c.ramp, c.dtfix, c.acdt = rk, cfl, cfl_ac, ramp, dtfix, acdt
     c.newton_tol, c.newton_iter, c.krylov_tol, c.krylov_iter = newton_tol, newton_iter, krylov_tol, krylov_iter
     c.limiter, c.order, c.rus_local, c.krylov_fp32, c.learned_guess = limiter, order, rus_local, krylov_fp32, learned_guess
+    c.analytic_jacobian = analytic_jacobian
     return c
 
 

@@ -54,17 +54,24 @@ def test_fp32_gradients_match_fp64_reference(B):
         assert err <= 2e-5, (a, b, err)
 
 
+def _gelu_tanh(x):
+    """the inference kernel's GELU: the tanh form x sigma(2u), u = sqrt(2/pi) (x + 0.044715 x^3)"""
+    u = np.sqrt(2 / np.pi) * (x + 0.044715 * x ** 3)
+    return x / (1 + np.exp(-2 * u))
+
+
 def test_bf16_mfma_inference_matches_its_rounding_model():
-    """Inputs, W1, W2 and the first hidden layer are rounded to BF16; products accumulate in FP32 on MFMA; the second
-    GELU and the head run in FP32. The kernel must agree with that model up to FP32 accumulation order and rare BF16
-    ties of the hidden layer, and stay within BF16 accuracy of the FP64 network."""
+    """Inputs, W1, W2 and the first hidden layer are rounded to BF16; products accumulate in FP32 on MFMA; both GELUs
+    take the tanh form (within 4.8e-4 of the erf form the network is trained with) and the second GELU and the head run
+    in FP32. The kernel must agree with that model up to FP32 accumulation order, the native exp2 / reciprocal and rare
+    BF16 ties of the hidden layer, and stay within BF16 accuracy of the FP64 network."""
     from oracle import mlp_ref as R
     net, P, X, _ = _net_and_data(5, 1000)
     y = net.infer_bf16(X)
     W1, b1, W2, b2, w3, b3 = R.unpack(P.astype(np.float64))
     Z1 = _bf16(X) @ _bf16(W1) + b1
-    H1 = _bf16(R.gelu(Z1).astype(np.float32))
-    H2 = R.gelu(H1 @ _bf16(W2) + b2)
+    H1 = _bf16(_gelu_tanh(Z1).astype(np.float32))
+    H2 = _gelu_tanh(H1 @ _bf16(W2) + b2)
     ym = H2 @ w3 + b3
     scale = np.abs(H2) @ np.abs(w3) + abs(b3)
     dev = np.abs(y - ym) / scale
