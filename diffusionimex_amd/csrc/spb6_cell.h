@@ -513,8 +513,10 @@ __device__ __forceinline__ bool jacobian_analytic_at(const Dev& D, const double*
 	rhs[q] = -Gq;
 	const double p = (Gq + Pv[0]) + D.P_REF;   // the closure's root at P: G = (pbar - P_REF) - Pc
 	bool ok = true;
-	#pragma unroll 1
-	for (int j = 0; j < (twoD ? 5 : 3); j++) {
+	// unrolled: the value parts of the five passes are identical and computed once
+	#pragma unroll
+	for (int j = 0; j < 5; j++) {
+		if (j >= 3 && !twoD) break;
 		D1 X[6], s, mx, my, E;
 		cell_assemble(D.P_REF, D.rx, D.ry, twoD, C, D1(Pv[0], j == 0), D1(Pv[1], j == 1), D1(Pv[2], j == 2), D1(Pv[3], j == 3),
 		              D1(Pv[4], j == 4), X, s, mx, my, E);

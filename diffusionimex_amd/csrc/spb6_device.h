@@ -24,8 +24,9 @@ SPB_HD double smax(double a, double b) { return (a < b) ? b : a; }
 SPB_HD double smin(double a, double b) { return (b < a) ? b : a; }
 SPB_HD bool isfin(double x) { return __builtin_isfinite(x); }
 
-// forward-mode derivative along one direction (value v, derivative d), for the analytic Newton Jacobian. smax / smin
-// with a constant take the right derivative at the kink (the limit of a forward difference with a positive step).
+// forward-mode derivative along one direction (value v, derivative d), for the analytic Newton Jacobian. Division takes
+// one reciprocal (values need not match the double path bit for bit here); smax / smin with a constant take the right
+// derivative at the kink (the limit of a forward difference with a positive step).
 struct D1 {
 	double v, d;
 	SPB_HD D1(double x = 0.0, double dx = 0.0) : v(x), d(dx) {}
@@ -33,14 +34,14 @@ struct D1 {
 SPB_HD D1 operator+(D1 a, D1 b) { return D1(a.v + b.v, a.d + b.d); }
 SPB_HD D1 operator-(D1 a, D1 b) { return D1(a.v - b.v, a.d - b.d); }
 SPB_HD D1 operator*(D1 a, D1 b) { return D1(a.v * b.v, a.d * b.v + a.v * b.d); }
-SPB_HD D1 operator/(D1 a, D1 b) { const double q = a.v / b.v; return D1(q, (a.d - q * b.d) / b.v); }
+SPB_HD D1 operator/(D1 a, D1 b) { const double r = 1.0 / b.v, q = a.v * r; return D1(q, (a.d - q * b.d) * r); }
 SPB_HD D1 operator+(D1 a, double b) { return D1(a.v + b, a.d); }
 SPB_HD D1 operator+(double a, D1 b) { return D1(a + b.v, b.d); }
 SPB_HD D1 operator-(D1 a, double b) { return D1(a.v - b, a.d); }
 SPB_HD D1 operator-(double a, D1 b) { return D1(a - b.v, -b.d); }
 SPB_HD D1 operator*(D1 a, double b) { return D1(a.v * b, a.d * b); }
 SPB_HD D1 operator*(double a, D1 b) { return D1(a * b.v, a * b.d); }
-SPB_HD D1 operator/(D1 a, double b) { return D1(a.v / b, a.d / b); }
+SPB_HD D1 operator/(D1 a, double b) { const double r = 1.0 / b; return D1(a.v * r, a.d * r); }
 SPB_HD D1& operator-=(D1& a, D1 b) { a = a - b; return a; }
 SPB_HD D1& operator*=(D1& a, D1 b) { a = a * b; return a; }
 SPB_HD D1 smax(D1 a, double b) { return a.v < b ? D1(b) : (a.v > b ? a : D1(a.v, smax(a.d, 0.0))); }
