@@ -377,7 +377,7 @@ __device__ __forceinline__ void load_cell(const Dev& D, int q, CellIn& C) {
 	if (g.twoD) { load_face(D, true, c, j, C.fn); load_face(D, true, c - g.NXT, j - 1, C.fs); }
 }
 // The face and cell assembly below is templated on the scalar: double for the residual (the reference's operations,
-// bit for bit), D1 for the analytic Jacobian's directional derivatives.
+// bit for bit), Dual<2> for a face's derivatives along its two pressures in the analytic Jacobian.
 template <class T>
 SPB_HD void face_eval(const FaceIn& f, T PL, T PR, FaceValsT<T>& v) {   // face_x / face_y (PC:597-616)
 	v.pf = f.wl * PL + (1.0 - f.wl) * PR - f.pv;
@@ -498,35 +498,74 @@ __device__ __forceinline__ bool jacobian_at(const Dev& D, const double* __restri
 	if (twoD) { J[3 * n + q] = vS; J[4 * n + q] = vN; }
 	return ok;
 }
+// One face's share of the closure inputs of a cell (the terms of cell_assemble), with the derivatives along the face's
+// two pressures (PL, PR) scattered into the cell's directions iL, iR (0 self, 1 W, 2 E, 3 S, 4 N). c = -rx (-ry) for the
+// cell's east (north) face and +rx (+ry) for its west (south) face; X[2..5], s, the normal and tangential momenta and E
+// accumulate.
+SPB_HD void jac_face(const FaceIn& f, double PL, double PR, int iL, int iR, double c, double P_REF,
+                     Dual<5> X[6], Dual<5>& s, Dual<5>& mn, Dual<5>& mt, Dual<5>& E) {
+	using T2 = Dual<2>;
+	FaceValsT<T2> v;
+	face_eval(f, T2::seed(PL, 0), T2::seed(PR, 1), v);
+	T2 M, Mt, Ef;
+	fflux(f, v, P_REF, M, Mt, Ef);
+	auto add = [&](Dual<5>& acc, const T2& q, double w) { acc.v += w * q.v; acc.d[iL] += w * q.d[0]; acc.d[iR] += w * q.d[1]; };
+	#pragma unroll
+	for (int k = 0; k < 4; k++) add(X[k + 2], finc(f, v, k), c);
+	add(s, v.uf, -c);
+	add(mn, v.pf + v.uf * M, c);
+	add(mt, Mt, c);
+	add(E, Ef, c);
+}
 // analytic Jacobian row q (the fast forms): the residual's closure is already solved at P, so each entry follows from
-// the implicit function theorem at that root, dG/dP_j = -(dPhi/dP_j) / (dPhi/dp) - [j = self], with dPhi/dP_j carried
-// through the face and cell assembly in D1 and no closure iterations. Same entries (self, W, E, S, N), layout and
-// rhs = -G as jacobian_at; a kinked flux (upwind switch at u_f = 0) takes the forward difference's side. False where
-// dPhi/dp is not negative or an entry is not finite.
+// the implicit function theorem at that root, dG/dP_j = -(dPhi/dP_j) / (dPhi/dp) - [j = self]. The closure inputs and
+// their derivatives along the five pressures (self, W, E, S, N) are accumulated face by face (each face loaded when it is
+// used, its derivatives along its own two pressures in Dual<2>), then the closure function is evaluated once in Dual<5>;
+// no closure iterations. Same entries, layout and rhs = -G as jacobian_at; a kinked flux (upwind switch at u_f = 0)
+// takes the forward difference's side. False where dPhi/dp is not negative or an entry is not finite.
+template <class FaceAt>
+SPB_HD bool jacobian_analytic_row(const Phys& Ph, double P_REF, double rx, double ry, bool twoD, const double xs[NV], double Q0, double Q1,
+                                  const double Pv[5], double Gq, FaceAt face_at, double J[5]) {
+	Dual<5> X[6], s(0.0), mx(xs[IMU]), my(xs[IMV]), E(xs[IEN]);
+	X[2] = Dual<5>(xs[IM1]); X[3] = Dual<5>(xs[IM2]); X[4] = Dual<5>(xs[IE1]); X[5] = Dual<5>(xs[IE2]);
+	FaceIn f;
+	face_at(0, f); jac_face(f, Pv[0], Pv[2], 0, 2, -rx, P_REF, X, s, mx, my, E);   // east: (Pc, PE)
+	face_at(1, f); jac_face(f, Pv[1], Pv[0], 1, 0, rx, P_REF, X, s, mx, my, E);    // west: (PW, Pc)
+	if (twoD) {
+		face_at(2, f); jac_face(f, Pv[0], Pv[4], 0, 4, -ry, P_REF, X, s, my, mx, E);   // north: (Pc, PN)
+		face_at(3, f); jac_face(f, Pv[3], Pv[0], 3, 0, ry, P_REF, X, s, my, mx, E);    // south: (PS, Pc)
+	}
+	X[0] = xs[IA1] * (1.0 - s); X[1] = (1.0 - xs[IA1]) * (1.0 - s);
+	const Dual<5> rhoeC = E - 0.5 * (mx * mx + my * my) / (X[2] + X[3]);
+	const double p = (Gq + Pv[0]) + P_REF;   // the closure's root at P: G = (pbar - P_REF) - Pc
+	double dphidp;
+	const Dual<5> phi = closure_phi(Ph, X, s, rhoeC, Q0, Q1, p, dphidp);
+	bool ok = dphidp < 0.0;
+	#pragma unroll
+	for (int j = 0; j < 5; j++) { J[j] = -phi.d[j] / dphidp - (j == 0 ? 1.0 : 0.0); ok = ok && isfin(J[j]); }
+	return ok;
+}
 __device__ __forceinline__ bool jacobian_analytic_at(const Dev& D, const double* __restrict__ P, const double* __restrict__ G,
                                                      double* __restrict__ J, double* __restrict__ rhs, int q) {
-	const int n = D.g.n; const bool twoD = D.g.twoD;
-	const Nb b = neighbours(D.g, q);
-	CellIn C; load_cell(D, q, C);
+	const Geo& g = D.g; const int n = g.n, NC = g.NC, i = q % g.nx + GH, j = q / g.nx + GH, c = i + j * g.NXT;
+	const bool twoD = g.twoD;
+	const Nb b = neighbours(g, q);
 	const double Pv[5] = {P[q], P[b.W], P[b.E], twoD ? P[b.S] : 0.0, twoD ? P[b.N] : 0.0};
+	double xs[NV];
+	#pragma unroll
+	for (int k = 0; k < NV; k++) xs[k] = D.Ust[k * NC + c];
 	const double Gq = G[q];
 	rhs[q] = -Gq;
-	const double p = (Gq + Pv[0]) + D.P_REF;   // the closure's root at P: G = (pbar - P_REF) - Pc
-	bool ok = true;
-	// unrolled: the value parts of the five passes are identical and computed once
-	#pragma unroll
-	for (int j = 0; j < 5; j++) {
-		if (j >= 3 && !twoD) break;
-		D1 X[6], s, mx, my, E;
-		cell_assemble(D.P_REF, D.rx, D.ry, twoD, C, D1(Pv[0], j == 0), D1(Pv[1], j == 1), D1(Pv[2], j == 2), D1(Pv[3], j == 3),
-		              D1(Pv[4], j == 4), X, s, mx, my, E);
-		const D1 rhoeC = E - 0.5 * (mx * mx + my * my) / (X[2] + X[3]);
-		double dphidp;
-		const D1 phi = closure_phi(D.ph, X, s, rhoeC, C.Q0, C.Q1, p, dphidp);
-		const double v = -phi.d / dphidp - (j == 0 ? 1.0 : 0.0);
-		ok = ok && dphidp < 0.0 && isfin(v);
-		J[j * n + q] = v;
-	}
+	auto face_at = [&](int w, FaceIn& f) {
+		if (w == 0) load_face(D, false, c, i, f);
+		else if (w == 1) load_face(D, false, c - 1, i - 1, f);
+		else if (w == 2) load_face(D, true, c, j, f);
+		else load_face(D, true, c - g.NXT, j - 1, f);
+	};
+	double Jr[5];
+	const bool ok = jacobian_analytic_row(D.ph, D.P_REF, D.rx, D.ry, twoD, xs, D.Qph[c], D.Qph[NC + c], Pv, Gq, face_at, Jr);
+	J[q] = Jr[0]; J[n + q] = Jr[1]; J[2 * n + q] = Jr[2];
+	if (twoD) { J[3 * n + q] = Jr[3]; J[4 * n + q] = Jr[4]; }
 	return ok;
 }
 // round-off floor acceptance test of row q (PC:834-842): false where the residual exceeds tol N_i + sigma_i

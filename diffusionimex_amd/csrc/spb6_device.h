@@ -24,30 +24,54 @@ SPB_HD double smax(double a, double b) { return (a < b) ? b : a; }
 SPB_HD double smin(double a, double b) { return (b < a) ? b : a; }
 SPB_HD bool isfin(double x) { return __builtin_isfinite(x); }
 
-// forward-mode derivative along one direction (value v, derivative d), for the analytic Newton Jacobian. Division takes
+// forward-mode derivatives along N directions (value v, derivatives d), for the analytic Newton Jacobian. Division takes
 // one reciprocal (values need not match the double path bit for bit here); smax / smin with a constant take the right
 // derivative at the kink (the limit of a forward difference with a positive step).
-struct D1 {
-	double v, d;
-	SPB_HD D1(double x = 0.0, double dx = 0.0) : v(x), d(dx) {}
+template <int N> struct Dual {
+	double v, d[N];
+	SPB_HD Dual(double x = 0.0) : v(x) {
+		#pragma unroll
+		for (int i = 0; i < N; i++) d[i] = 0.0;
+	}
+	SPB_HD static Dual seed(double x, int i) { Dual r(x); r.d[i] = 1.0; return r; }
 };
-SPB_HD D1 operator+(D1 a, D1 b) { return D1(a.v + b.v, a.d + b.d); }
-SPB_HD D1 operator-(D1 a, D1 b) { return D1(a.v - b.v, a.d - b.d); }
-SPB_HD D1 operator*(D1 a, D1 b) { return D1(a.v * b.v, a.d * b.v + a.v * b.d); }
-SPB_HD D1 operator/(D1 a, D1 b) { const double r = 1.0 / b.v, q = a.v * r; return D1(q, (a.d - q * b.d) * r); }
-SPB_HD D1 operator+(D1 a, double b) { return D1(a.v + b, a.d); }
-SPB_HD D1 operator+(double a, D1 b) { return D1(a + b.v, b.d); }
-SPB_HD D1 operator-(D1 a, double b) { return D1(a.v - b, a.d); }
-SPB_HD D1 operator-(double a, D1 b) { return D1(a - b.v, -b.d); }
-SPB_HD D1 operator*(D1 a, double b) { return D1(a.v * b, a.d * b); }
-SPB_HD D1 operator*(double a, D1 b) { return D1(a * b.v, a * b.d); }
-SPB_HD D1 operator/(D1 a, double b) { const double r = 1.0 / b; return D1(a.v * r, a.d * r); }
-SPB_HD D1& operator-=(D1& a, D1 b) { a = a - b; return a; }
-SPB_HD D1& operator*=(D1& a, D1 b) { a = a * b; return a; }
-SPB_HD D1 smax(D1 a, double b) { return a.v < b ? D1(b) : (a.v > b ? a : D1(a.v, smax(a.d, 0.0))); }
-SPB_HD D1 smin(D1 a, double b) { return b < a.v ? D1(b) : (a.v < b ? a : D1(a.v, smin(a.d, 0.0))); }
-SPB_HD double val(double x) { return x; }
-SPB_HD double val(D1 x) { return x.v; }
+#define SPB_DUAL_OP(expr_v, expr_d)                                                       \
+	{ Dual<N> o_; o_.v = (expr_v);                                                          \
+	  _Pragma("unroll") for (int i = 0; i < N; i++) o_.d[i] = (expr_d);                     \
+	  return o_; }
+template <int N> SPB_HD Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) SPB_DUAL_OP(a.v + b.v, a.d[i] + b.d[i])
+template <int N> SPB_HD Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) SPB_DUAL_OP(a.v - b.v, a.d[i] - b.d[i])
+template <int N> SPB_HD Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) SPB_DUAL_OP(a.v * b.v, a.d[i] * b.v + a.v * b.d[i])
+template <int N> SPB_HD Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) {
+	const double r = 1.0 / b.v, q = a.v * r;
+	SPB_DUAL_OP(q, (a.d[i] - q * b.d[i]) * r)
+}
+template <int N> SPB_HD Dual<N> operator+(const Dual<N>& a, double b) SPB_DUAL_OP(a.v + b, a.d[i])
+template <int N> SPB_HD Dual<N> operator+(double a, const Dual<N>& b) SPB_DUAL_OP(a + b.v, b.d[i])
+template <int N> SPB_HD Dual<N> operator-(const Dual<N>& a, double b) SPB_DUAL_OP(a.v - b, a.d[i])
+template <int N> SPB_HD Dual<N> operator-(double a, const Dual<N>& b) SPB_DUAL_OP(a - b.v, -b.d[i])
+template <int N> SPB_HD Dual<N> operator*(const Dual<N>& a, double b) SPB_DUAL_OP(a.v * b, a.d[i] * b)
+template <int N> SPB_HD Dual<N> operator*(double a, const Dual<N>& b) SPB_DUAL_OP(a * b.v, a * b.d[i])
+template <int N> SPB_HD Dual<N> operator/(const Dual<N>& a, double b) { const double r = 1.0 / b; SPB_DUAL_OP(a.v * r, a.d[i] * r) }
+#undef SPB_DUAL_OP
+template <int N> SPB_HD Dual<N>& operator-=(Dual<N>& a, const Dual<N>& b) { a = a - b; return a; }
+template <int N> SPB_HD Dual<N>& operator*=(Dual<N>& a, const Dual<N>& b) { a = a * b; return a; }
+template <int N> SPB_HD Dual<N> smax(const Dual<N>& a, double b) {
+	if (a.v < b) return Dual<N>(b);
+	if (a.v > b) return a;
+	Dual<N> r(a.v);
+	#pragma unroll
+	for (int i = 0; i < N; i++) r.d[i] = smax(a.d[i], 0.0);
+	return r;
+}
+template <int N> SPB_HD Dual<N> smin(const Dual<N>& a, double b) {
+	if (b < a.v) return Dual<N>(b);
+	if (a.v < b) return a;
+	Dual<N> r(a.v);
+	#pragma unroll
+	for (int i = 0; i < N; i++) r.d[i] = smin(a.d[i], 0.0);
+	return r;
+}
 
 // ------------------------------------------------------------------ portable exp / log / pow / hypot
 // exp: x = k ln2 + r (Cody-Waite, ln2 split in a 32-bit head and a tail), |r| <= ln2/2, Taylor degree 13, exact 2^k.
@@ -377,26 +401,27 @@ SPB_HD bool closure_p(const Phys& P, const ClIn& in, ClOut& o) {
 // alpha_1, alpha_2, m1, m2, eps_1, eps_2), s and (rho e)^C carrying one directional derivative: returns Phi and its
 // derivative along that direction, and dphidp = dPhi/dp at fixed inputs (closure_p's df). The defect segment is the one
 // closure_p selects at p, so within it lam and the extra share do not depend on p.
-SPB_HD D1 closure_phi(const Phys& P, const D1 X[6], D1 s, D1 rhoeC, double Q0, double Q1, double p, double& dphidp) {
-	const D1 D0 = rhoeC - X[4] - X[5];
-	const D1 sm = X[2] + X[3];
-	const D1 E0 = smax(X[4] - X[2] * P.q[0] - (X[0] - X[2] * P.b[0]) * P.pi[0], 0.0);   // E_above (PHY:420-423)
-	const D1 E1 = smax(X[5] - X[3] * P.q[1] - (X[1] - X[3] * P.b[1]) * P.pi[1], 0.0);
-	const D1 sE = E0 + E1;
-	D1 lE0, lE1;
+template <class T>
+SPB_HD T closure_phi(const Phys& P, const T X[6], const T& s, const T& rhoeC, double Q0, double Q1, double p, double& dphidp) {
+	const T D0 = rhoeC - X[4] - X[5];
+	const T sm = X[2] + X[3];
+	const T E0 = smax(X[4] - X[2] * P.q[0] - (X[0] - X[2] * P.b[0]) * P.pi[0], 0.0);   // E_above (PHY:420-423)
+	const T E1 = smax(X[5] - X[3] * P.q[1] - (X[1] - X[3] * P.b[1]) * P.pi[1], 0.0);
+	const T sE = E0 + E1;
+	T lE0, lE1;
 	if (sE.v > 0) { lE0 = E0 / sE; lE1 = E1 / sE; }
 	else if (sm.v > 0) { lE0 = X[2] / sm; lE1 = X[3] / sm; }
-	else { lE0 = D1(0.5); lE1 = D1(0.5); }
+	else { lE0 = T(0.5); lE1 = T(0.5); }
 	const double Qb = Q0 + Q1;
-	const D1 delta = D0 + p * s;
-	D1 lam0, lam1, ex0, ex1;   // defect_seg (PHY:428-439)
+	const T delta = D0 + p * s;
+	T lam0, lam1, ex0, ex1;   // defect_seg (PHY:428-439)
 	if (!(delta.v > 0.0) || !(Qb > 0.0)) { lam0 = lE0; lam1 = lE1; }
-	else if (delta.v < Qb) { lam0 = D1(Q0 / Qb); lam1 = D1(Q1 / Qb); }
+	else if (delta.v < Qb) { lam0 = T(Q0 / Qb); lam1 = T(Q1 / Qb); }
 	else { lam0 = lE0; lam1 = lE1; ex0 = Q0 - Qb * lE0; ex1 = Q1 - Qb * lE1; }
 	const double g0 = P.g[0], g1 = P.g[1], pi0 = P.pi[0], pi1 = P.pi[1], b0 = P.b[0], b1 = P.b[1];
-	const D1 A0 = (X[4] - X[2] * P.q[0]) + lam0 * D0 + ex0, B0 = X[0] + lam0 * s, N0 = A0 - (B0 - X[2] * b0) * pi0;
-	const D1 A1 = (X[5] - X[3] * P.q[1]) + lam1 * D0 + ex1, B1 = X[1] + lam1 * s, N1 = A1 - (B1 - X[3] * b1) * pi1;
-	const D1 f = ((g0 - 1.0) * (A0 + B0 * p) + (p + g0 * pi0) * X[2] * b0) / (g0 * (p + pi0))
+	const T A0 = (X[4] - X[2] * P.q[0]) + lam0 * D0 + ex0, B0 = X[0] + lam0 * s, N0 = A0 - (B0 - X[2] * b0) * pi0;
+	const T A1 = (X[5] - X[3] * P.q[1]) + lam1 * D0 + ex1, B1 = X[1] + lam1 * s, N1 = A1 - (B1 - X[3] * b1) * pi1;
+	const T f = ((g0 - 1.0) * (A0 + B0 * p) + (p + g0 * pi0) * X[2] * b0) / (g0 * (p + pi0))
 	           + ((g1 - 1.0) * (A1 + B1 * p) + (p + g1 * pi1) * X[3] * b1) / (g1 * (p + pi1));
 	dphidp = -(g0 - 1.0) * N0.v / (g0 * (p + pi0) * (p + pi0)) - (g1 - 1.0) * N1.v / (g1 * (p + pi1) * (p + pi1));
 	return f - 1.0;
