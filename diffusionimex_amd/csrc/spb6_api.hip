@@ -91,8 +91,8 @@ struct spb6_ctx {
 struct spb6_mlp {
 	int device = 0; hipStream_t st = nullptr;
 	float *P = nullptr, *G = nullptr, *m = nullptr, *v = nullptr, *mean = nullptr, *istd = nullptr, *loss = nullptr;
-	float *smean = nullptr, *sistd = nullptr;   // standardisation per MFMA slot of the BF16 layer 1
-	__bf16 *W1t = nullptr, *W2t = nullptr;
+	float *mub1 = nullptr;                      // per-field input shift and folded layer-1 bias of the BF16 inference
+	__bf16 *W1s = nullptr, *W1f = nullptr, *W2p = nullptr;
 	float mu_t = 0.f, sd_t = 1.f, b3 = 0.f;
 	float *X = nullptr, *Y = nullptr; long ndata = 0;
 	float *Xb = nullptr, *Yb = nullptr, *ws = nullptr, *yb = nullptr; int* idx = nullptr; int bcap = 0;
@@ -347,7 +347,7 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 	if (learned && !(ok0 && res0 < c->cfg.newton_tol)) {
 		// the plain guess judges the learned one: keep whichever has the smaller Newton residual norm
 		const spb6_mlp* M = c->model;
-		MlpDev W{M->P, M->W1t, M->W2t, M->smean, M->sistd, M->b3, M->mu_t, M->sd_t};
+		MlpDev W{M->P, M->W1s, M->W1f, M->W2p, M->mub1, M->b3, M->mu_t, M->sd_t};
 		{
 			TRegion tr(c, T_GUESS);
 			mlp_infer(&D, c->F, adt, (int)n, nullptr, W, c->Pplain, ok0 ? c->Gplain : nullptr, thr, c->Pn, nullptr, c->st);
@@ -779,8 +779,8 @@ spb6_mlp* spb6_mlp_create(int device) {
 	if (hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking) != hipSuccess) return fail();
 	if (mlp_alloc(m, (void**)&m->P, MLP_NPARAM * 4) || mlp_alloc(m, (void**)&m->G, MLP_NPARAM * 4) || mlp_alloc(m, (void**)&m->m, MLP_NPARAM * 4) ||
 	    mlp_alloc(m, (void**)&m->v, MLP_NPARAM * 4) || mlp_alloc(m, (void**)&m->mean, MLP_IN * 4) || mlp_alloc(m, (void**)&m->istd, MLP_IN * 4) ||
-	    mlp_alloc(m, (void**)&m->loss, 4) || mlp_alloc(m, (void**)&m->W1t, MLP_H * MLP_KPAD * 2) || mlp_alloc(m, (void**)&m->W2t, MLP_H * MLP_H * 2) ||
-	    mlp_alloc(m, (void**)&m->smean, MLP_KPAD * 4) || mlp_alloc(m, (void**)&m->sistd, MLP_KPAD * 4))
+	    mlp_alloc(m, (void**)&m->loss, 4) || mlp_alloc(m, (void**)&m->W1s, MLP_H * MLP_KPAD * 2) || mlp_alloc(m, (void**)&m->W1f, MLP_H * MLP_KPAD * 2) ||
+	    mlp_alloc(m, (void**)&m->W2p, MLP_H * MLP_H * 2) || mlp_alloc(m, (void**)&m->mub1, (7 + MLP_H) * 4))
 		return fail();
 	return m;
 }
@@ -792,7 +792,7 @@ void spb6_mlp_destroy(spb6_mlp* m) {
 	delete m;
 }
 static int mlp_sync_infer(spb6_mlp* m) {   // refresh the BF16 copy and the host scalars after the FP32 parameters change
-	mlp_pack(m->P, m->mean, m->istd, m->W1t, m->W2t, m->smean, m->sistd, m->st);
+	mlp_pack(m->P, m->mean, m->istd, m->W1s, m->W1f, m->W2p, m->mub1, m->st);
 	HIPCHK(hipMemcpyAsync(&m->b3, m->P + MLP_OFF_B3, sizeof(float), hipMemcpyDeviceToHost, m->st));
 	HIPCHK(hipStreamSynchronize(m->st));
 	return SPB6_OK;
@@ -902,7 +902,7 @@ int spb6_mlp_infer_bf16(spb6_mlp* m, const float* X, int n, float* y) {
 	hipSetDevice(m->device);
 	int rc = mlp_batch_bufs(m, n); if (rc) return rc;
 	HIPCHK(hipMemcpyAsync(m->Xb, X, (size_t)n * MLP_IN * 4, hipMemcpyHostToDevice, m->st));
-	MlpDev W{m->P, m->W1t, m->W2t, m->smean, m->sistd, m->b3, m->mu_t, m->sd_t};
+	MlpDev W{m->P, m->W1s, m->W1f, m->W2p, m->mub1, m->b3, m->mu_t, m->sd_t};
 	mlp_infer(nullptr, nullptr, 0.0, n, m->Xb, W, nullptr, nullptr, 0.0, nullptr, m->yb, m->st);
 	HIPCHK(hipMemcpyAsync(y, m->yb, (size_t)n * 4, hipMemcpyDeviceToHost, m->st));
 	HIPCHK(hipStreamSynchronize(m->st));

@@ -197,8 +197,9 @@ __global__ void __launch_bounds__(256) k_jacobian(Dev D, const double* __restric
 	if (q >= D.g.n) return;
 	if (!jacobian_at(D, P, G, sc, J, rhs, q)) flags[FLAG_JAC] = 1;
 }
-// analytic Jacobian rows (the fast forms): one thread per row, no closure iterations
-__global__ void __launch_bounds__(256) k_jacobian_an(Dev D, const double* __restrict__ P, const double* __restrict__ G,
+// analytic Jacobian rows (the fast forms): one thread per row, no closure iterations. Two waves per SIMD (a few
+// registers spilled) measured 252 us at 2M rows against 289 us at the compiler's default of one.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_jacobian_an(Dev D, const double* __restrict__ P, const double* __restrict__ G,
                            double* __restrict__ J, double* __restrict__ rhs, int* __restrict__ flags) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= D.g.n) return;

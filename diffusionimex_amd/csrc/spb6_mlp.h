@@ -12,14 +12,16 @@ constexpr int MLP_OFF_W1 = 0, MLP_OFF_B1 = MLP_IN * MLP_H, MLP_OFF_W2 = MLP_OFF_
 
 // device copy of a trained network for inference
 struct MlpDev {
-	const float* params; const __bf16* W1t; const __bf16* W2t; const float* smean; const float* sistd;   // per MFMA slot
+	const float* params;
+	const __bf16* W1s; const __bf16* W1f; const __bf16* W2p;   // BF16 layer 1 for standardised / shifted inputs, layer 2
+	const float* mub1;                                          // per-field input shift mu[7] and the folded bias b1f[128]
 	float b3, mu_t, sd_t;
 };
 
 void mlp_feat_cell(const Dev& D, const double* Pplain, float* F, hipStream_t st);
 void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, const double* Pplain, const double* Gplain, double thr,
               int off, int stride, long cap, float* X, float* Y, unsigned long long* count, hipStream_t st);
-void mlp_pack(const float* params, const float* mean, const float* istd, __bf16* W1t, __bf16* W2t, float* smean, float* sistd, hipStream_t st);
+void mlp_pack(const float* params, const float* mean, const float* istd, __bf16* W1s, __bf16* W1f, __bf16* W2p, float* mub1, hipStream_t st);
 void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
                const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st);
 // a cell takes the network's guess (and is dumped for training) when its plain-guess residual is at least this fraction

@@ -229,61 +229,94 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 union BF8 { bf16x8 v; __bf16 e[8]; uint4 u; };
 
-// MFMA k order of layer 1: slot t = 32 ks + 8 lk + e (ks = k step, lk = lane / 16, e = element of the lane's 8) holds
-// field f = (8 ks + e) / 7 at stencil point s = 4 ((8 ks + e) % 7) + lk (s < 25; f = 7 and s >= 25 are zero padding).
-// The field is then a compile-time constant of the unrolled (ks, e) loops and only the stencil offset varies by lane.
+// MFMA k order of layer 1: slot t = 32 ks + 8 g + e (k step ks, lane group g = lane / 16, element e of the lane's 8)
+// holds field f = (8 ks + e) / 7 at stencil point s = 4 ((8 ks + e) % 7) + g (s < 25). The field is then a compile-time
+// constant of the unrolled (ks, e) loops and only the stencil offset varies by lane. f = 7 and s >= 25 are padding;
+// slot MLP_BIAS_SLOT (ks 6, g 0, e 1: a field-7 slot) takes the constant input 1 and carries layer 1's bias.
+constexpr int MLP_BIAS_SLOT = 6 * 32 + 1;
 __host__ __device__ constexpr int slot_feature(int t) {
 	const int ks = t / 32, lk = (t / 8) % 4, e = t % 8, p = 8 * ks + e, f = p / 7, s = 4 * (p % 7) + lk;
 	return (f < 7 && s < 25) ? 7 * s + f : -1;
 }
-// W1t[n][t] (t in the slot order above) and W2t[n][k]: transposed BF16 weights, one lane's 8 consecutive k = 16 bytes;
-// smean / sistd: the standardisation of each slot (0 for padding)
-__global__ void __launch_bounds__(256) k_pack_bf16(const float* __restrict__ params, const float* __restrict__ mean, const float* __restrict__ istd,
-                                                   __bf16* __restrict__ W1t, __bf16* __restrict__ W2t, float* __restrict__ smean, float* __restrict__ sistd) {
+// layer-2 k order: element s of lane group g in k step j is hidden unit h2k(j, g, s), i.e. the rows 4g .. 4g + 3 of
+// layer 1's output tiles 2j and 2j + 1 that the lane already holds, so layer 1's results feed layer 2 from registers
+__host__ __device__ constexpr int h2k(int j, int g, int s) { return 32 * j + (s < 4 ? 4 * g + s : 16 + 4 * g + (s - 4)); }
+// Folded standardisation for the stage's features: W1 (x - mean) istd + b1 = (W1 istd) (x - mu_f) + b1f with the
+// per-field shift mu_f (the mean of field f's slot means) and b1f[n] = b1[n] + sum_k W1[k][n] (mu_f(k) - mean_k) istd_k.
+// The BF16 input is then x - mu_f (the field centred, as precise as the standardised value); mub1 = (mu[7], b1f[128]).
+__global__ void __launch_bounds__(MLP_H) k_pack_fold(const float* __restrict__ params, const float* __restrict__ mean,
+                                                     const float* __restrict__ istd, float* __restrict__ mub1) {
+	__shared__ float mu[7];
+	const int n = threadIdx.x;
+	if (n < 7) {
+		double s = 0.0;
+		for (int sp = 0; sp < 25; sp++) s += mean[7 * sp + n];
+		mu[n] = (float)(s / 25.0);
+		mub1[n] = mu[n];
+	}
+	__syncthreads();
+	double b = params[MLP_OFF_B1 + n];
+	for (int k = 0; k < MLP_IN; k++) b += (double)params[MLP_OFF_W1 + k * MLP_H + n] * ((double)mu[k % 7] - (double)mean[k]) * (double)istd[k];
+	mub1[7 + n] = (float)b;
+}
+// transposed BF16 weights, one lane's 8 consecutive k = 16 bytes: W1s[n][t] for standardised inputs and W1f[n][t] for
+// inputs x - mu_f (istd folded in), each with its bias in MLP_BIAS_SLOT; W2p[m][32 j + 8 g + s] = W2[h2k(j, g, s)][m]
+__global__ void __launch_bounds__(256) k_pack_bf16(const float* __restrict__ params, const float* __restrict__ istd, const float* __restrict__ mub1,
+                                                   __bf16* __restrict__ W1s, __bf16* __restrict__ W1f, __bf16* __restrict__ W2p) {
 	const int e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e < MLP_H * MLP_KPAD) {
 		const int n = e / MLP_KPAD, t = e % MLP_KPAD, k = slot_feature(t);
-		W1t[e] = (__bf16)(k >= 0 ? params[MLP_OFF_W1 + k * MLP_H + n] : 0.f);
-		if (n == 0) { smean[t] = k >= 0 ? mean[k] : 0.f; sistd[t] = k >= 0 ? istd[k] : 0.f; }
+		float ws = 0.f, wf = 0.f;
+		if (k >= 0) { ws = params[MLP_OFF_W1 + k * MLP_H + n]; wf = ws * istd[k]; }
+		else if (t == MLP_BIAS_SLOT) { ws = params[MLP_OFF_B1 + n]; wf = mub1[7 + n]; }
+		W1s[e] = (__bf16)ws; W1f[e] = (__bf16)wf;
 	}
 	if (e < MLP_H * MLP_H) {
-		const int n = e / MLP_H, k = e % MLP_H;
-		W2t[e] = (__bf16)params[MLP_OFF_W2 + k * MLP_H + n];
+		const int m = e / MLP_H, kk = e % MLP_H;
+		W2p[e] = (__bf16)params[MLP_OFF_W2 + h2k(kk / 32, (kk / 8) % 4, kk % 8) * MLP_H + m];
 	}
 }
-// MFMA 16x16x32 operand layout: lane l holds A rows m = l % 16 and k = 8 (l / 16) .. +7 of a 32-wide k step, B columns
-// n = l % 16 with the same k; the 16x16 FP32 result gives lane l column n = l % 16, rows 4 (l / 16) .. +3.
-// Persistent blocks of 8 waves share the BF16 weights in LDS; each wave takes 16-row tiles. mode 0: features from the
-// stage (guess); mode 1: standardised features given in Xin[rows][175] (tests).
-constexpr int INF_WAVES = 8;
+// MFMA 16x16x32 operand layout: lane l holds A rows i = l % 16 with k = 8 (l / 16) .. +7 of a 32-wide k step, B columns
+// j = l % 16 with the same k; the 16x16 FP32 result gives lane l column j = l % 16, rows 4 (l / 16) .. +3.
+// Both layers run transposed, Z^T = W^T X^T: the weights are the A operand (rows = hidden units) and a 16-cell tile
+// is the B operand (columns = cells), so lane l works on cell l % 16 throughout and layer 1's FP32 results, after GELU
+// and BF16 rounding, are layer 2's B operand in place. Persistent blocks of 16 waves share the BF16 weights in LDS;
+// each wave takes 16-cell tiles. mode 0: features from the stage (guess); mode 1: standardised features given in
+// Xin[rows][175] (tests).
+constexpr int INF_WAVES = 16;
 __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(FeatCtx X, int nrows, const float* __restrict__ Xin, int mode,
-                                                   const float* __restrict__ smean, const float* __restrict__ sistd,
-                                                   const __bf16* __restrict__ W1t, const float* __restrict__ b1,
-                                                   const __bf16* __restrict__ W2t, const float* __restrict__ b2,
-                                                   const float* __restrict__ w3, float b3, float mu_t, float sd_t,
+                                                   const float* __restrict__ mu_f, const __bf16* __restrict__ W1, const __bf16* __restrict__ W2p,
+                                                   const float* __restrict__ b2, const float* __restrict__ w3, float b3, float mu_t, float sd_t,
                                                    const double* __restrict__ Pplain, double* __restrict__ Pout, float* __restrict__ yout) {
 	__shared__ uint4 sW1[MLP_H * MLP_KPAD / 8];     // 56 KB
 	__shared__ uint4 sW2[MLP_H * MLP_H / 8];        // 32 KB
-	__shared__ __bf16 sH[INF_WAVES][16][MLP_H + 8];  // per-wave hidden activations (A layout source), 34 KB
-	__shared__ float sMean[MLP_KPAD], sIstd[MLP_KPAD];
-	__shared__ float sF[INF_WAVES][7][5][20];       // per-wave 5x5 stencils of a 16-cell tile (20 x 5 cells x 7 fields), 22 KB
-	for (int e = threadIdx.x; e < MLP_H * MLP_KPAD / 8; e += 64 * INF_WAVES) sW1[e] = ((const uint4*)W1t)[e];
-	for (int e = threadIdx.x; e < MLP_H * MLP_H / 8; e += 64 * INF_WAVES) sW2[e] = ((const uint4*)W2t)[e];
-	for (int e = threadIdx.x; e < MLP_KPAD; e += 64 * INF_WAVES) { sMean[e] = smean[e]; sIstd[e] = sistd[e]; }
+	__shared__ float4 sB2[MLP_H / 4], sW3[MLP_H / 4];
+	__shared__ float sF[INF_WAVES][7][5][20];       // per-wave 5x5 stencils of a 16-cell tile (20 x 5 cells x 7 fields), 44 KB
+	for (int e = threadIdx.x; e < MLP_H * MLP_KPAD / 8; e += 64 * INF_WAVES) sW1[e] = ((const uint4*)W1)[e];
+	for (int e = threadIdx.x; e < MLP_H * MLP_H / 8; e += 64 * INF_WAVES) sW2[e] = ((const uint4*)W2p)[e];
+	for (int e = threadIdx.x; e < MLP_H / 4; e += 64 * INF_WAVES) { sB2[e] = ((const float4*)b2)[e]; sW3[e] = ((const float4*)w3)[e]; }
 	__syncthreads();
-	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, lk = lane >> 4;
+	const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, lr = lane & 15, g = lane >> 4;
+	float mu[7];
+	#pragma unroll
+	for (int f = 0; f < 7; f++) mu[f] = mode == 0 ? mu_f[f] : 0.f;
+	// this lane group's stencil points s = 4 m + g (m = 0..6) as offsets into a staged tile, and whether they exist
+	int soff[7]; bool sin[7];
+	#pragma unroll
+	for (int m = 0; m < 7; m++) { const int sp = 4 * m + g; sin[m] = sp < 25; soff[m] = sin[m] ? (sp / 5) * 20 + sp % 5 : 0; }
 	const int ntiles = (nrows + 15) / 16;
 	const bool staged = mode == 0 && X.g.nx % 16 == 0 && nrows % 16 == 0;
+	const float* tileF = &sF[wave][0][0][0] + lr;
 	for (int tile = blockIdx.x * INF_WAVES + wave; tile < ntiles; tile += gridDim.x * INF_WAVES) {
-		const int q = tile * 16 + lr;           // this lane's A row (cell)
+		const int q = tile * 16 + lr;           // this lane's cell
 		const bool valid = q < nrows;
 		// a tile none of whose cells is active keeps the plain guess and skips the network (wave-uniform branch)
 		if (mode == 0 && __ballot(valid && cell_active(X, q)) == 0ull) {
-			if (lk == 0 && valid && Pout) Pout[q] = Pplain[q];
+			if (g == 0 && valid && Pout) Pout[q] = Pplain[q];
 			continue;
 		}
 		Centre C{};
-		int cq = 0;                              // ghosted index of the row's cell
+		int cq = 0;                              // ghosted index of the lane's cell
 		if (staged) {
 			// the tile's 16 cells lie in one grid row: stage the 20 x 5 cells x 7 fields of their stencils in LDS with
 			// coalesced loads, one round trip, and take every feature from there
@@ -296,86 +329,71 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(FeatCtx X, int nro
 			C.iK = 1.f / (float)X.Kw[cq]; const double cz = X.cz[cq]; C.icz = 1.f / (float)(isfin(cz) && cz > 0 ? cz : 1.0);
 			__builtin_amdgcn_s_waitcnt(0);
 			__builtin_amdgcn_wave_barrier();
-			C.F1c = sF[wave][1][2][lr + 2]; C.F2c = sF[wave][2][2][lr + 2]; C.F3c = sF[wave][3][2][lr + 2]; C.F5c = sF[wave][5][2][lr + 2];
+			C.F1c = tileF[100 + 42]; C.F2c = tileF[200 + 42]; C.F3c = tileF[300 + 42]; C.F5c = tileF[500 + 42];   // (dy, dx) = (2, 2)
 		}
 		else if (valid && mode == 0) { C = centre_of(X, q); cq = (q % X.g.nx + GH) + (q / X.g.nx + GH) * X.g.NXT; }
-		// layer 1: Z1[16 x 128] = X[16 x 224] W1[224 x 128]
+		// layer 1: Z1^T[128 x 16] = W1^T[128 x 224] X^T[224 x 16]; acc1[nt][r] = hidden unit 16 nt + 4 g + r of cell lr
 		f32x4 acc1[8];
 		#pragma unroll
 		for (int nt = 0; nt < 8; nt++) acc1[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 		#pragma unroll
 		for (int ks = 0; ks < MLP_KPAD / 32; ks++) {
-			BF8 a;
+			BF8 b;
 			#pragma unroll
 			for (int e = 0; e < 8; e++) {
-				const int p = 8 * ks + e, f = p / 7, sp = 4 * (p % 7) + lk, t = ks * 32 + lk * 8 + e;
+				const int p = 8 * ks + e, f = p / 7, m = p % 7;
 				float x = 0.f;
-				if (f < 7 && sp < 25 && valid) {
-					if (staged) x = (feat_tr(f, sF[wave][f][sp / 5][lr + sp % 5], C) - sMean[t]) * sIstd[t];
-					else if (mode == 0) {
-						const int c = cq + (sp % 5 - 2) + (sp / 5 - 2) * X.g.NXT;
-						x = (feat_tr(f, X.F[f * X.g.NC + c], C) - sMean[t]) * sIstd[t];
-					} else x = Xin[(long)q * MLP_IN + 7 * sp + f];
-				}
-				a.e[e] = (__bf16)x;
+				if (f < 7) {
+					if (valid && sin[m]) {
+						if (staged) x = feat_tr(f, tileF[100 * f + soff[m]], C) - mu[f];
+						else if (mode == 0) {
+							const int sp = 4 * m + g;
+							x = feat_tr(f, X.F[f * X.g.NC + cq + (sp % 5 - 2) + (sp / 5 - 2) * X.g.NXT], C) - mu[f];
+						} else x = Xin[(long)q * MLP_IN + 7 * (4 * m + g) + f];
+					}
+				} else if (ks * 32 + e == MLP_BIAS_SLOT && g == 0) x = 1.f;
+				b.e[e] = (__bf16)x;
 			}
 			#pragma unroll
 			for (int nt = 0; nt < 8; nt++) {
-				BF8 b; b.u = sW1[((nt * 16 + lr) * MLP_KPAD + ks * 32 + lk * 8) / 8];
+				BF8 a; a.u = sW1[((nt * 16 + lr) * MLP_KPAD + ks * 32 + g * 8) / 8];
 				acc1[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc1[nt], 0, 0, 0);
 			}
 		}
-		// bias + GELU, to LDS in row-major [m][n] for the next layer's A fragments
+		// GELU (the bias came in through MLP_BIAS_SLOT) and BF16: layer 2's B fragments in the h2k order
+		BF8 h[4];
 		#pragma unroll
-		for (int nt = 0; nt < 8; nt++)
+		for (int j = 0; j < 4; j++)
 			#pragma unroll
-			for (int r = 0; r < 4; r++) {
-				const int n = nt * 16 + lr, m = lk * 4 + r;
-				sH[wave][m][n] = (__bf16)gelu_fast(acc1[nt][r] + b1[n]);
-			}
-		__builtin_amdgcn_s_waitcnt(0);
-		__builtin_amdgcn_wave_barrier();
-		// layer 2: Z2[16 x 128] = H1[16 x 128] W2[128 x 128]
+			for (int r = 0; r < 4; r++) { h[j].e[r] = (__bf16)gelu_fast(acc1[2 * j][r]); h[j].e[4 + r] = (__bf16)gelu_fast(acc1[2 * j + 1][r]); }
+		// layer 2: Z2^T[128 x 16] = W2^T[128 x 128] H1^T[128 x 16]; acc2[mt][r] = hidden unit 16 mt + 4 g + r of cell lr
 		f32x4 acc2[8];
 		#pragma unroll
-		for (int nt = 0; nt < 8; nt++) acc2[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+		for (int mt = 0; mt < 8; mt++) acc2[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
 		#pragma unroll
-		for (int ks = 0; ks < MLP_H / 32; ks++) {
-			BF8 a;
+		for (int j = 0; j < 4; j++)
 			#pragma unroll
-			for (int e = 0; e < 8; e++) a.e[e] = sH[wave][lr][ks * 32 + lk * 8 + e];
-			#pragma unroll
-			for (int nt = 0; nt < 8; nt++) {
-				BF8 b; b.u = sW2[((nt * 16 + lr) * MLP_H + ks * 32 + lk * 8) / 8];
-				acc2[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc2[nt], 0, 0, 0);
+			for (int mt = 0; mt < 8; mt++) {
+				BF8 a; a.u = sW2[((mt * 16 + lr) * MLP_H + j * 32 + g * 8) / 8];
+				acc2[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, h[j].v, acc2[mt], 0, 0, 0);
 			}
-		}
-		__builtin_amdgcn_wave_barrier();
-		// head: y_m = sum_n GELU(Z2[m][n] + b2[n]) w3[n] + b3; lanes with equal l / 16 hold the same rows
-		float part[4] = {0.f, 0.f, 0.f, 0.f};
+		// head: y = sum_n GELU(Z2[n] + b2[n]) w3[n] + b3 over the lane's 32 units, then over the four lane groups
+		float part = 0.f;
 		#pragma unroll
-		for (int nt = 0; nt < 8; nt++) {
-			const int n = nt * 16 + lr;
-			#pragma unroll
-			for (int r = 0; r < 4; r++) part[r] += gelu_fast(acc2[nt][r] + b2[n]) * w3[n];
+		for (int mt = 0; mt < 8; mt++) {
+			const float4 bb = sB2[mt * 4 + g], ww = sW3[mt * 4 + g];
+			part += gelu_fast(acc2[mt][0] + bb.x) * ww.x + gelu_fast(acc2[mt][1] + bb.y) * ww.y
+			      + gelu_fast(acc2[mt][2] + bb.z) * ww.z + gelu_fast(acc2[mt][3] + bb.w) * ww.w;
 		}
-		#pragma unroll
-		for (int r = 0; r < 4; r++)
-			#pragma unroll
-			for (int off = 1; off < 16; off <<= 1) part[r] += __shfl_xor(part[r], off, 64);
-		if (lr == 0) {
-			#pragma unroll
-			for (int r = 0; r < 4; r++) {
-				const int qq = tile * 16 + lk * 4 + r;
-				if (qq < nrows) {
-					const float y = part[r] + b3;
-					if (yout) yout[qq] = y;
-					if (Pout) {
-						const Geo& g = X.g;
-						const double Kc = X.Kw[(qq % g.nx + GH) + (qq / g.nx + GH) * g.NXT];
-						Pout[qq] = cell_active(X, qq) ? Pplain[qq] + Kc * (double)(mu_t + sd_t * y) : Pplain[qq];
-					}
-				}
+		part += __shfl_xor(part, 16, 64);
+		part += __shfl_xor(part, 32, 64);
+		if (g == 0 && valid) {
+			const float y = part + b3;
+			if (yout) yout[q] = y;
+			if (Pout) {
+				const Geo& gg = X.g;
+				const double Kc = X.Kw[(q % gg.nx + GH) + (q / gg.nx + GH) * gg.NXT];
+				Pout[q] = cell_active(X, q) ? Pplain[q] + Kc * (double)(mu_t + sd_t * y) : Pplain[q];
 			}
 		}
 	}
@@ -394,8 +412,9 @@ void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, con
 	if (rows <= 0) return;
 	hipLaunchKernelGGL(k_dump, dim3(nb(rows)), dim3(256), 0, st, fx, Pconv, Pplain, off, stride, cap, X, Y, count);
 }
-void mlp_pack(const float* params, const float* mean, const float* istd, __bf16* W1t, __bf16* W2t, float* smean, float* sistd, hipStream_t st) {
-	hipLaunchKernelGGL(k_pack_bf16, dim3(nb(MLP_H * MLP_KPAD)), dim3(256), 0, st, params, mean, istd, W1t, W2t, smean, sistd);
+void mlp_pack(const float* params, const float* mean, const float* istd, __bf16* W1s, __bf16* W1f, __bf16* W2p, float* mub1, hipStream_t st) {
+	hipLaunchKernelGGL(k_pack_fold, dim3(1), dim3(MLP_H), 0, st, params, mean, istd, mub1);
+	hipLaunchKernelGGL(k_pack_bf16, dim3(nb(MLP_H * MLP_KPAD)), dim3(256), 0, st, params, istd, (const float*)mub1, W1s, W1f, W2p);
 }
 void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
                const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st) {
@@ -405,8 +424,8 @@ void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float*
 	int cus = 256; (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
 	const int tiles = (nrows + 15) / 16;
 	const int blocks = std::max(1, std::min(cus, (tiles + INF_WAVES - 1) / INF_WAVES));
-	hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * INF_WAVES), 0, st, fx, nrows, Xin, Xin ? 1 : 0, W.smean, W.sistd, W.W1t,
-	                   W.params + MLP_OFF_B1, W.W2t, W.params + MLP_OFF_B2, W.params + MLP_OFF_W3, W.b3, W.mu_t, W.sd_t, Pplain, Pout, yout);
+	hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * INF_WAVES), 0, st, fx, nrows, Xin, Xin ? 1 : 0, (const float*)W.mub1,
+	                   Xin ? W.W1s : W.W1f, W.W2p, W.params + MLP_OFF_B2, W.params + MLP_OFF_W3, W.b3, W.mu_t, W.sd_t, Pplain, Pout, yout);
 }
 
 // forward (and optionally backward) of the FP32 network on a batch Xb[Bn x 175]; grads into G (same layout as params)

@@ -61,7 +61,7 @@ def _gelu_tanh(x):
 
 
 def test_bf16_mfma_inference_matches_its_rounding_model():
-    """Inputs, W1, W2 and the first hidden layer are rounded to BF16; products accumulate in FP32 on MFMA; both GELUs
+    """Inputs, W1, b1, W2 and the first hidden layer are rounded to BF16; products accumulate in FP32 on MFMA; both GELUs
     take the tanh form (within 4.8e-4 of the erf form the network is trained with) and the second GELU and the head run
     in FP32. The kernel must agree with that model up to FP32 accumulation order, the native exp2 / reciprocal and rare
     BF16 ties of the hidden layer, and stay within BF16 accuracy of the FP64 network."""
@@ -69,7 +69,7 @@ def test_bf16_mfma_inference_matches_its_rounding_model():
     net, P, X, _ = _net_and_data(5, 1000)
     y = net.infer_bf16(X)
     W1, b1, W2, b2, w3, b3 = R.unpack(P.astype(np.float64))
-    Z1 = _bf16(X) @ _bf16(W1) + b1
+    Z1 = _bf16(X) @ _bf16(W1) + _bf16(b1)   # the bias enters the MFMA through a constant-1 input slot
     H1 = _bf16(_gelu_tanh(Z1).astype(np.float32))
     H2 = _gelu_tanh(H1 @ _bf16(W2) + b2)
     ym = H2 @ w3 + b3
