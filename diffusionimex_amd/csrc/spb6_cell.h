@@ -395,49 +395,85 @@ SPB_HD void fflux(const FaceIn& f, const FaceValsT<T>& v, double P_REF, T& M, T&
 	E = finc(f, v, 2) + finc(f, v, 3) + 0.5 * v.uf * v.uf * M + 0.5 * (f.utl * f.utl * ML + f.utr * f.utr * MR) + (P_REF + v.pf) * v.uf;
 }
 // closure inputs of one cell at the given pressures (cell_residual, PC:640-672): X = transported (alpha_1, alpha_2, m1,
-// m2, eps_1, eps_2), the volume change s, the momenta and the total energy
-template <class T>
-SPB_HD void cell_assemble(double P_REF, double rx, double ry, bool twoD, const CellIn& C, T Pc, T PW, T PE,
-                                              T PS, T PN, T X[6], T& s, T& mx, T& my, T& E) {
-	FaceValsT<T> fe, fw, fn, fs;
-	face_eval(C.fe, Pc, PE, fe); face_eval(C.fw, PW, Pc, fw);
-	if (twoD) { face_eval(C.fn, Pc, PN, fn); face_eval(C.fs, PS, Pc, fs); }
-	X[0] = T(C.xs[IA1]); X[1] = T(C.a2s); X[2] = T(C.xs[IM1]); X[3] = T(C.xs[IM2]); X[4] = T(C.xs[IE1]); X[5] = T(C.xs[IE2]);
-	#pragma unroll
-	for (int k = 2; k < 6; k++) {
-		X[k] -= rx * (finc(C.fe, fe, k - 2) - finc(C.fw, fw, k - 2));
-		if (twoD) X[k] -= ry * (finc(C.fn, fn, k - 2) - finc(C.fs, fs, k - 2));
+// m2, eps_1, eps_2), the volume change s, the momenta and the total energy. face_at(w, f) fills face w (0 east, 1 west,
+// 2 north, 3 south) when it is first needed: the x pair is evaluated and folded in before the y pair is read, with every
+// sum in the reference's order (each X[k] takes its x term, then its y term).
+template <class T, class FaceAt>
+SPB_HD void cell_assemble(double P_REF, double rx, double ry, bool twoD, const double xs[NV], FaceAt face_at, T Pc, T PW, T PE,
+                          T PS, T PN, T X[6], T& s, T& mx, T& my, T& E) {
+	X[0] = T(xs[IA1]); X[2] = T(xs[IM1]); X[3] = T(xs[IM2]); X[4] = T(xs[IE1]); X[5] = T(xs[IE2]);
+	T sx;
+	{
+		FaceIn fa, fb;
+		face_at(0, fa); face_at(1, fb);
+		FaceValsT<T> fe, fw;
+		face_eval(fa, Pc, PE, fe); face_eval(fb, PW, Pc, fw);
+		#pragma unroll
+		for (int k = 2; k < 6; k++) X[k] -= rx * (finc(fa, fe, k - 2) - finc(fb, fw, k - 2));
+		sx = rx * (fe.uf - fw.uf);
+		T Me, Mte, Ee, Mw, Mtw, Ew;
+		fflux(fa, fe, P_REF, Me, Mte, Ee); fflux(fb, fw, P_REF, Mw, Mtw, Ew);
+		mx = xs[IMU] - rx * (fe.pf - fw.pf) - rx * (fe.uf * Me - fw.uf * Mw);
+		my = xs[IMV] - rx * (Mte - Mtw);
+		E = xs[IEN] - rx * (Ee - Ew);
 	}
-	X[1] = 1.0 - X[0];
-	s = rx * (fe.uf - fw.uf) + (twoD ? ry * (fn.uf - fs.uf) : T(0.0));
-	X[0] *= 1.0 - s; X[1] *= 1.0 - s;
-	T Me, Mte, Ee, Mw, Mtw, Ew;
-	fflux(C.fe, fe, P_REF, Me, Mte, Ee); fflux(C.fw, fw, P_REF, Mw, Mtw, Ew);
-	mx = C.xs[IMU] - rx * (fe.pf - fw.pf) - rx * (fe.uf * Me - fw.uf * Mw);
-	my = C.xs[IMV] - rx * (Mte - Mtw);
-	E = C.xs[IEN] - rx * (Ee - Ew);
 	if (twoD) {
+		FaceIn fa, fb;
+		face_at(2, fa); face_at(3, fb);
+		FaceValsT<T> fn, fs;
+		face_eval(fa, Pc, PN, fn); face_eval(fb, PS, Pc, fs);
+		#pragma unroll
+		for (int k = 2; k < 6; k++) X[k] -= ry * (finc(fa, fn, k - 2) - finc(fb, fs, k - 2));
+		s = sx + ry * (fn.uf - fs.uf);
 		T Mn, Mtn, En, Ms, Mts, Es;
-		fflux(C.fn, fn, P_REF, Mn, Mtn, En); fflux(C.fs, fs, P_REF, Ms, Mts, Es);
+		fflux(fa, fn, P_REF, Mn, Mtn, En); fflux(fb, fs, P_REF, Ms, Mts, Es);
 		my -= ry * (fn.pf - fs.pf) + ry * (fn.uf * Mn - fs.uf * Ms);
 		mx -= ry * (Mtn - Mts);
 		E -= ry * (En - Es);
 	}
+	else s = sx + T(0.0);
+	X[1] = 1.0 - X[0];
+	X[0] *= 1.0 - s; X[1] *= 1.0 - s;
 }
-// G = (pbar_cl - P_REF) - Pc of one cell (cell_residual, PC:640-679)
-SPB_HD bool cell_res(const Phys& P, double P_REF, double rx, double ry, bool twoD, const CellIn& C,
-                                         double Pc, double PW, double PE, double PS, double PN, double& G, ResOut* out) {
+// G = (pbar_cl - P_REF) - Pc of one cell (cell_residual, PC:640-679), faces from face_at as in cell_assemble
+template <class FaceAt>
+SPB_HD bool cell_res(const Phys& P, double P_REF, double rx, double ry, bool twoD, const double xs[NV], double Q0, double Q1,
+                     FaceAt face_at, double Pc, double PW, double PE, double PS, double PN, double& G, ResOut* out) {
 	double X[6], s, mx, my, E;
-	cell_assemble(P_REF, rx, ry, twoD, C, Pc, PW, PE, PS, PN, X, s, mx, my, E);
+	cell_assemble(P_REF, rx, ry, twoD, xs, face_at, Pc, PW, PE, PS, PN, X, s, mx, my, E);
 	const double rho = X[2] + X[3];
 	if (!(rho > 0)) return false;
 	ClIn in; in.atr0 = X[0]; in.atr1 = X[1]; in.mtr0 = X[2]; in.mtr1 = X[3]; in.etr0 = X[4]; in.etr1 = X[5];
-	in.s = s; in.rhoeC = E - 0.5 * (mx * mx + my * my) / rho; in.pguess = P_REF + Pc; in.Q0 = C.Q0; in.Q1 = C.Q1;
+	in.s = s; in.rhoeC = E - 0.5 * (mx * mx + my * my) / rho; in.pguess = P_REF + Pc; in.Q0 = Q0; in.Q1 = Q1;
 	ClOut o; if (!closure_p(P, in, o)) return false;
 	G = (o.pbar - P_REF) - Pc;
 	if (out) { out->a0 = o.a0; out->a1 = o.a1; out->m0 = X[2]; out->m1 = X[3]; out->e0 = o.e0; out->e1 = o.e1; out->mx = mx; out->my = my; out->E = E; out->sig = closure_roundoff(P, in, o.pbar); }
 	return isfin(G);
 }
+// the same on a cell whose data is already loaded (the finite-difference Jacobian evaluates one cell five times)
+SPB_HD bool cell_res(const Phys& P, double P_REF, double rx, double ry, bool twoD, const CellIn& C,
+                     double Pc, double PW, double PE, double PS, double PN, double& G, ResOut* out) {
+	auto face_at = [&](int w, FaceIn& f) { f = w == 0 ? C.fe : w == 1 ? C.fw : w == 2 ? C.fn : C.fs; };
+	return cell_res(P, P_REF, rx, ry, twoD, C.xs, C.Q0, C.Q1, face_at, Pc, PW, PE, PS, PN, G, out);
+}
+// on-demand faces of interior row q (w = 0 east, 1 west, 2 north, 3 south) and its cell values; the scheduling
+// barrier keeps each face's loads next to their use instead of all ~64 face doubles live from the top
+struct CellFaces {
+	const Dev& D; int q, c, i, j;
+	__device__ CellFaces(const Dev& D_, int q_) : D(D_), q(q_) { i = q % D.g.nx + GH; j = q / D.g.nx + GH; c = i + j * D.g.NXT; }
+	__device__ void operator()(int w, FaceIn& f) const {
+		__builtin_amdgcn_sched_barrier(0);
+		if (w == 0) load_face(D, false, c, i, f);
+		else if (w == 1) load_face(D, false, c - 1, i - 1, f);
+		else if (w == 2) load_face(D, true, c, j, f);
+		else load_face(D, true, c - D.g.NXT, j - 1, f);
+	}
+	__device__ void cell(double xs[NV], double& Q0, double& Q1, double& Kw) const {
+		#pragma unroll
+		for (int k = 0; k < NV; k++) xs[k] = D.Ust[k * D.g.NC + c];
+		Q0 = D.Qph[c]; Q1 = D.Qph[D.g.NC + c]; Kw = D.Kw[c];
+	}
+};
 struct Nb { int W, E, S, N; };
 __device__ __forceinline__ Nb neighbours(const Geo& g, int q) {   // M_P of the four neighbours of interior row q
 	const int ii = q % g.nx, jj = q / g.nx; Nb b;
@@ -456,12 +492,13 @@ __device__ __forceinline__ bool residual_at(const Dev& D, const double* __restri
 	auto pv = [&](int r) { return dP ? P[r] + lam * dP[r] : P[r]; };
 	const double Pc = pv(q), PW = pv(b.W), PE = pv(b.E);
 	const double PS = D.g.twoD ? pv(b.S) : 0.0, PN = D.g.twoD ? pv(b.N) : 0.0;
-	CellIn C; load_cell(D, q, C);
+	const CellFaces F(D, q);
+	double xs[NV], Q0, Q1, Kw; F.cell(xs, Q0, Q1, Kw);
 	double g;
 	bool ok = true;
 	nv = 0.0;
-	if (!cell_res(D.ph, D.P_REF, D.rx, D.ry, D.g.twoD, C, Pc, PW, PE, PS, PN, g, nullptr)) { ok = false; g = dnan(); }
-	else nv = fabs(g) / (fabs(D.P_REF + Pc) + C.Kw + 1.0);
+	if (!cell_res(D.ph, D.P_REF, D.rx, D.ry, D.g.twoD, xs, Q0, Q1, F, Pc, PW, PE, PS, PN, g, nullptr)) { ok = false; g = dnan(); }
+	else nv = fabs(g) / (fabs(D.P_REF + Pc) + Kw + 1.0);
 	G[q] = g;
 	if (Pout) Pout[q] = Pc;
 	return ok;
@@ -547,23 +584,16 @@ SPB_HD bool jacobian_analytic_row(const Phys& Ph, double P_REF, double rx, doubl
 }
 __device__ __forceinline__ bool jacobian_analytic_at(const Dev& D, const double* __restrict__ P, const double* __restrict__ G,
                                                      double* __restrict__ J, double* __restrict__ rhs, int q) {
-	const Geo& g = D.g; const int n = g.n, NC = g.NC, i = q % g.nx + GH, j = q / g.nx + GH, c = i + j * g.NXT;
+	const Geo& g = D.g; const int n = g.n;
 	const bool twoD = g.twoD;
 	const Nb b = neighbours(g, q);
 	const double Pv[5] = {P[q], P[b.W], P[b.E], twoD ? P[b.S] : 0.0, twoD ? P[b.N] : 0.0};
-	double xs[NV];
-	#pragma unroll
-	for (int k = 0; k < NV; k++) xs[k] = D.Ust[k * NC + c];
+	const CellFaces F(D, q);
+	double xs[NV], Q0, Q1, Kw; F.cell(xs, Q0, Q1, Kw);
 	const double Gq = G[q];
 	rhs[q] = -Gq;
-	auto face_at = [&](int w, FaceIn& f) {
-		if (w == 0) load_face(D, false, c, i, f);
-		else if (w == 1) load_face(D, false, c - 1, i - 1, f);
-		else if (w == 2) load_face(D, true, c, j, f);
-		else load_face(D, true, c - g.NXT, j - 1, f);
-	};
 	double Jr[5];
-	const bool ok = jacobian_analytic_row(D.ph, D.P_REF, D.rx, D.ry, twoD, xs, D.Qph[c], D.Qph[NC + c], Pv, Gq, face_at, Jr);
+	const bool ok = jacobian_analytic_row(D.ph, D.P_REF, D.rx, D.ry, twoD, xs, Q0, Q1, Pv, Gq, F, Jr);
 	J[q] = Jr[0]; J[n + q] = Jr[1]; J[2 * n + q] = Jr[2];
 	if (twoD) { J[3 * n + q] = Jr[3]; J[4 * n + q] = Jr[4]; }
 	return ok;
@@ -571,21 +601,23 @@ __device__ __forceinline__ bool jacobian_analytic_at(const Dev& D, const double*
 // round-off floor acceptance test of row q (PC:834-842): false where the residual exceeds tol N_i + sigma_i
 __device__ __forceinline__ bool floor_ok_at(const Dev& D, const double* __restrict__ P, double tol, int q) {
 	const Nb b = neighbours(D.g, q);
-	CellIn C; load_cell(D, q, C);
+	const CellFaces F(D, q);
+	double xs[NV], Q0, Q1, Kw; F.cell(xs, Q0, Q1, Kw);
 	const bool twoD = D.g.twoD;
 	double g; ResOut o;
-	if (!cell_res(D.ph, D.P_REF, D.rx, D.ry, twoD, C, P[q], P[b.W], P[b.E], twoD ? P[b.S] : 0.0, twoD ? P[b.N] : 0.0, g, &o)) return false;
-	return !(fabs(g) > tol * (fabs(D.P_REF + P[q]) + C.Kw + 1.0) + o.sig);
+	if (!cell_res(D.ph, D.P_REF, D.rx, D.ry, twoD, xs, Q0, Q1, F, P[q], P[b.W], P[b.E], twoD ? P[b.S] : 0.0, twoD ? P[b.N] : 0.0, g, &o)) return false;
+	return !(fabs(g) > tol * (fabs(D.P_REF + P[q]) + Kw + 1.0) + o.sig);
 }
 // final stage state of row q at the converged pressure (PC:975-987) and the stage derivative Hs = (U_s - Qs) / adt
 // (PC:1276-1284); false where the closure fails
 __device__ __forceinline__ bool final_at(const Dev& D, const double* __restrict__ P, const double* __restrict__ Qs, double adt, double* __restrict__ H, int q) {
 	const Geo& g = D.g; const int NC = g.NC, c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
 	const Nb b = neighbours(g, q);
-	CellIn C; load_cell(D, q, C);
+	const CellFaces F(D, q);
+	double xs[NV], Q0, Q1, Kw; F.cell(xs, Q0, Q1, Kw);
 	const bool twoD = g.twoD;
 	double gg; ResOut o;
-	if (!cell_res(D.ph, D.P_REF, D.rx, D.ry, twoD, C, P[q], P[b.W], P[b.E], twoD ? P[b.S] : 0.0, twoD ? P[b.N] : 0.0, gg, &o)) return false;
+	if (!cell_res(D.ph, D.P_REF, D.rx, D.ry, twoD, xs, Q0, Q1, F, P[q], P[b.W], P[b.E], twoD ? P[b.S] : 0.0, twoD ? P[b.N] : 0.0, gg, &o)) return false;
 	const double u[NV] = {o.a0, o.m0, o.m1, o.mx, o.my, o.e0, o.e1, o.E};
 	#pragma unroll
 	for (int k = 0; k < NV; k++) { D.Ust[k * NC + c] = u[k]; H[k * NC + c] = (u[k] - Qs[k * NC + c]) / adt; }
