@@ -187,7 +187,7 @@ int krylov_run(spb6_ctx* c, const KrylovBufsT<T>& B, const T* b, double tol, int
 // sweep asking its FP32 solve for max(eta / 2, 5e-7) relative accuracy (the FP32 floor), so that one sweep meets
 // eta >= 1e-6. A step whose defect misses eta but has decreased is still returned: Newton's residual test and line
 // search judge it.
-int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res) {
+int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J) {
 	const long n = c->g.n;
 	if (!c->cfg.krylov_fp32) {
 		KrylovBufs B{c->dP, c->kr, c->kr0, c->kp, c->kv, c->ks_, c->kt, c->J, c->part, c->part_stride, c->ksd};
@@ -196,7 +196,7 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res) {
 	const double res = std::max(newton_res, 1e-300), eta = std::max(0.1 * c->cfg.newton_tol / res, std::min(1e-6, res));
 	{
 		TRegion tr(c, T_KAUX);
-		launch_to_f32(5 * n, c->J, c->J32, c->st);
+		if (new_J) launch_to_f32(5 * n, c->J, c->J32, c->st);   // a chord step keeps the previous FP32 copy
 		launch_to_f32(n, c->rhs, c->b32, c->st);
 		HIPCHK(hipMemsetAsync(c->dP, 0, n * sizeof(double), c->st));
 	}
@@ -251,22 +251,34 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 	if (!have_G) { rc = eval_all(c, D, c->Pst, nullptr, 0.0, nullptr, c->G, ok, res); if (rc) return rc; }
 	if (!ok) { iters = 0; resfinal = INFINITY; return SPB6_NEWTON_FAIL; }
 	const int verbose = getenv("SPB6_VERBOSE") ? atoi(getenv("SPB6_VERBOSE")) : 0;
+	// faster forms: the last step of a solve as a chord step (this solve's previous Jacobian): once the residual is
+	// within 100x of the tolerance after a full step that contracted at least 1000x, the Jacobian has moved by about that
+	// step and the chord step's extra error is ~res_prev x res. A chord step that does not contract 100x turns chord
+	// steps off for the rest of the solve (near the cavitation threshold the Jacobian can move much more than the step
+	// suggests; there chord steps from res ~1e-8 were measured to cost Newton iterations).
+	bool chord_ok = c->cfg.analytic_jacobian && c->cfg.krylov_fp32 && !tri;
+	bool have_J = false, full_step = false;
+	double res_prev = INFINITY;
 	for (int it = 0; it < c->cfg.newton_iter; it++) {
 		iters = it; resfinal = res;
 		if (verbose >= 3) fprintf(stderr, "    newton it %d res %.3e\n", it, res);   // the reference's verbose >= 3 trace (PC:781)
 		if (res < tol) return SPB6_OK;
 		rc = reset_scalars(c); if (rc) return rc;
+		const bool chord = chord_ok && have_J && full_step && res < 100.0 * tol && res <= 1e-3 * res_prev;
+		const double res_before = res;
 		{
 			TRegion tr(c, T_JACOBIAN);
-			launch_jacobian(D, c->Pst, c->G, c->sc, c->J, c->rhs, c->cfg.analytic_jacobian != 0, c->st);
+			if (chord) launch_chord_rhs(D, c->Pst, c->G, c->sc, c->rhs, c->st);
+			else launch_jacobian(D, c->Pst, c->G, c->sc, c->J, c->rhs, c->cfg.analytic_jacobian != 0, c->st);
 		}
+		have_J = true;
 		c->stats.linsolves++;
 		bool kok = true; int kit = 0;
 		if (tri) {
 			TRegion tr(c, T_OTHER);
 			launch_thomas(n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
 		} else {
-			rc = krylov(c, kit, kok, res); if (rc) return rc;
+			rc = krylov(c, kit, kok, res, !chord); if (rc) return rc;
 		}
 		launch_stepcrit(n, c->dP, c->Pst, c->sc, &c->nrm[64], c->st);
 		rc = fetch(c); if (rc) return rc;
@@ -290,6 +302,9 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 			if (ok && (rn < res * (1.0 - 1e-4 * lam) || rn < tol)) { acc = true; res = rn; break; }
 			lam *= 0.5;
 		}
+		full_step = acc && lam == 1.0;
+		if (chord && !(acc && res <= 1e-2 * res_before)) chord_ok = false;
+		res_prev = res_before;
 		if (!acc) {   // round-off floor acceptance (PC:834-842)
 			resfinal = res;
 			rc = reset_scalars(c); if (rc) return rc;
@@ -674,7 +689,7 @@ int spb6_linsolve(spb6_ctx* c, const double* val, const double* b, double* x, in
 	HIPCHK(hipMemcpyAsync(c->rhs, b, n * sizeof(double), hipMemcpyHostToDevice, c->st));
 	int it = 0; bool ok = true;
 	if (use_thomas) launch_thomas((int)n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
-	else { int rc = krylov(c, it, ok, 1.0); if (rc) return rc; }
+	else { int rc = krylov(c, it, ok, 1.0, true); if (rc) return rc; }
 	HIPCHK(hipMemcpyAsync(x, c->dP, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
 	HIPCHK(hipStreamSynchronize(c->st));
 	if (iters) *iters = it;

@@ -205,6 +205,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 	if (q >= D.g.n) return;
 	if (!jacobian_analytic_at(D, P, G, J, rhs, q)) flags[FLAG_JAC] = 1;
 }
+// right-hand side -G of a chord step (the faster forms' Newton step on the previous Jacobian)
+__global__ void __launch_bounds__(256) k_neg(int n, const double* __restrict__ G, double* __restrict__ rhs) {
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q < n) rhs[q] = -G[q];
+}
 // Newton step criterion max |dP| / (|P| + s) (PC:819-823)
 __global__ void __launch_bounds__(256) k_stepcrit(int n, const double* __restrict__ dP, const double* __restrict__ P, const double* __restrict__ sc, unsigned long long* __restrict__ nrm) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -644,6 +649,10 @@ void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc,
 	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc);
 	if (analytic) hipLaunchKernelGGL(k_jacobian_an, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, J, rhs, D.flags);
 	else hipLaunchKernelGGL(k_jacobian, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, (const double*)sc, J, rhs, D.flags);
+}
+void launch_chord_rhs(const Dev& D, const double* P, const double* G, double* sc, double* rhs, hipStream_t st) {
+	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc);
+	hipLaunchKernelGGL(k_neg, dim3(nblk(D.g.n)), dim3(256), 0, st, D.g.n, G, rhs);
 }
 void launch_fdscale(const Dev& D, const double* P, double* sc, hipStream_t st) {
 	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc);

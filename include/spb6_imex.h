@@ -67,7 +67,10 @@ typedef struct spb6_config {
     int krylov_fp32;                 /* 1: BiCGStab vectors/matrix in FP32 inside the FP64 Newton (defect correction) */
     int learned_guess;               /* 1: Newton initial guess from the MLP (spb6_mlp_*), plain guess as fallback */
     int analytic_jacobian;           /* 1: Newton Jacobian from the closure's implicit derivative at the residual's root
-                                        instead of five finite-difference closure solves per row (PC:790-804) */
+                                        instead of five finite-difference closure solves per row (PC:790-804); with
+                                        krylov_fp32 the last Newton step as a chord step (the solve's previous
+                                        Jacobian) when the residual is within 100x of the tolerance after a full step
+                                        that contracted 1000x */
 } spb6_config_t;
 
 /* run statistics: the counters of the reference's stats line (PC:1337-1339) plus Krylov counts */
