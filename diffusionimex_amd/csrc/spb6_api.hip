@@ -6,6 +6,7 @@
 // the scalar decisions the reference takes (norms, step criterion, line-search acceptance, reject-and-halve).
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -23,7 +24,9 @@ static thread_local std::string g_err;
 
 namespace {
 
-struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; };
+// the scalars the host reads back: norms (8 slots of 64 striped maxima), flags, the Krylov loop state and the FP32
+// form's defect sums. The device keeps the same struct in one allocation, so every read-back is one copy.
+struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; double dsum[2]; };
 
 // T_KRYLOV: BiCGStab iteration passes only; T_KAUX: the solves' setup, precision conversions and FP64 defects;
 // T_GUESS: the learned guess's features and inference
@@ -66,6 +69,8 @@ struct spb6_ctx {
 	// FP32 Krylov form: matrix, right-hand side and vectors in single precision
 	float *J32 = nullptr, *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *r032 = nullptr, *p32 = nullptr, *v32 = nullptr, *s32 = nullptr, *t32 = nullptr;
 	double* part = nullptr; int part_stride = 0; KS* ksd = nullptr;
+	double *part2 = nullptr, *dsum = nullptr;   // the FP32 form's defect partials and their two sums
+	HostScratch* mail = nullptr;               // device side of hs: nrm, flags, ksd and dsum point into it
 	unsigned long long* nrm = nullptr; int* flags = nullptr;
 	HostScratch* hs = nullptr;
 	std::vector<void*> allocs;
@@ -77,7 +82,6 @@ struct spb6_ctx {
 	long krylov_launched = 0;          // BiCGStab iteration passes launched since timing was (re)enabled
 	double tau_last = 0.0;
 	Timer timer;
-	std::vector<double> hpart;          // host copy of block partials (FP64 defect norms)
 	// learned guess and training dumps
 	spb6_mlp* model = nullptr;
 	float* F = nullptr;                 // 7 per-cell feature fields over the ghosted grid
@@ -128,15 +132,13 @@ struct TRegion {
 };
 
 int reset_scalars(spb6_ctx* c) {
-	HIPCHK(hipMemsetAsync(c->nrm, 0, 8 * 64 * sizeof(unsigned long long), c->st));
-	HIPCHK(hipMemsetAsync(c->flags, 0, NFLAGS * sizeof(int), c->st));
+	HIPCHK(hipMemsetAsync(c->mail, 0, offsetof(HostScratch, ks), c->st));   // nrm and flags (contiguous)
 	return SPB6_OK;
 }
 // copy norms, flags and Krylov scalars to pinned host memory and wait
 int fetch(spb6_ctx* c, bool with_ks = false) {
-	HIPCHK(hipMemcpyAsync(c->hs->nrm, c->nrm, 8 * 64 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
-	HIPCHK(hipMemcpyAsync(c->hs->flags, c->flags, NFLAGS * sizeof(int), hipMemcpyDeviceToHost, c->st));
-	if (with_ks) HIPCHK(hipMemcpyAsync(&c->hs->ks, c->ksd, sizeof(KS), hipMemcpyDeviceToHost, c->st));
+	(void)with_ks;
+	HIPCHK(hipMemcpyAsync(c->hs, c->mail, sizeof(HostScratch), hipMemcpyDeviceToHost, c->st));
 	HIPCHK(hipStreamSynchronize(c->st));
 	return SPB6_OK;
 }
@@ -187,7 +189,8 @@ int krylov_run(spb6_ctx* c, const KrylovBufsT<T>& B, const T* b, double tol, int
 // sweep asking its FP32 solve for max(eta / 2, 5e-7) relative accuracy (the FP32 floor), so that one sweep meets
 // eta >= 1e-6. A step whose defect misses eta but has decreased is still returned: Newton's residual test and line
 // search judge it.
-int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J) {
+int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, bool& crit) {
+	crit = false;
 	const long n = c->g.n;
 	if (!c->cfg.krylov_fp32) {
 		KrylovBufs B{c->dP, c->kr, c->kr0, c->kp, c->kv, c->ks_, c->kt, c->J, c->part, c->part_stride, c->ksd};
@@ -202,24 +205,35 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J) {
 	}
 	KrylovBufs32 B{c->x32, c->r32, c->r032, c->p32, c->v32, c->s32, c->t32, c->J32, c->part, c->part_stride, c->ksd};
 	iters = 0; ok = false;
-	const int nbl = (int)((n + 255) / 256);
+	const double tin = std::max(0.5 * eta, 5e-7);
+	const int maxit = c->cfg.krylov_iter;
+	const Geo& g = c->g;
 	for (int sweep = 0; sweep < 6; sweep++) {
-		int it = 0; bool kok = false;
-		const int rc = krylov_run<float>(c, B, c->b32, std::max(0.5 * eta, 5e-7), it, kok);
-		if (rc) return rc;
-		iters += it;
-		// FP64 defect of the accumulated step
-		{
-			TRegion tr(c, T_KAUX);
-			launch_add_f32(n, c->x32, c->dP, c->st);
-			launch_linres(c->g, c->J, c->rhs, c->dP, c->kr, c->part, c->part_stride, c->st);
+		{ TRegion tr(c, T_KAUX); launch_krylov_setup32(g, B, c->b32, c->st); }
+		// the sweep's BiCGStab passes in chunks sized by the previous solve; behind each chunk, in the same stream, the
+		// FP64 defect and the step criterion of dP + x, so that one synchronisation reads the solve's state, the defect
+		// and the criterion together (a solve that is not done yet continues; dP is only written once it is)
+		int chunk = std::max(4, c->last_krylov + 2), launched = 0;
+		for (;;) {
+			{
+				TRegion tr(c, T_KRYLOV);
+				for (int k = 0; k < chunk && launched <= maxit; k++, launched++) launch_krylov_iter32(g, B, c->st, launched, tin, maxit);
+			}
+			{
+				TRegion tr(c, T_KAUX);
+				launch_defect_x(g, c->J, c->rhs, c->dP, c->x32, c->kr, c->part2, c->part_stride, c->dsum, c->Pst, c->sc, &c->nrm[64], c->st);
+			}
+			HIPCHK(hipMemcpyAsync(c->hs, c->mail, sizeof(HostScratch), hipMemcpyDeviceToHost, c->st));
+			HIPCHK(hipStreamSynchronize(c->st));
+			if (c->hs->ks.done || launched > maxit) break;
+			chunk = 4;
 		}
-		std::vector<double>& hp = c->hpart;
-		hp.resize(2 * (size_t)c->part_stride);
-		HIPCHK(hipMemcpyAsync(hp.data(), c->part, 2 * (size_t)c->part_stride * sizeof(double), hipMemcpyDeviceToHost, c->st));
-		HIPCHK(hipStreamSynchronize(c->st));
-		double rr = 0.0, bb = 0.0;
-		for (int q = 0; q < nbl; q++) { rr += hp[q]; bb += hp[c->part_stride + q]; }
+		const int it = c->hs->ks.iters;
+		const bool kok = c->hs->ks.done && c->hs->ks.status == 0;
+		c->last_krylov = it; c->krylov_launched += launched; iters += it;
+		{ TRegion tr(c, T_KAUX); launch_add_f32(n, c->x32, c->dP, c->st); }
+		crit = true;   // the step criterion just read is that of dP as it now stands
+		const double rr = c->hs->dsum[0], bb = c->hs->dsum[1];
 		if (getenv("SPB6_VERBOSE") && atoi(getenv("SPB6_VERBOSE")) >= 3)
 			fprintf(stderr, "        sweep %d krylov %d defect %.3e eta %.3e\n", sweep, it, std::sqrt(rr / bb), eta);
 		if (std::sqrt(rr) <= eta * std::sqrt(bb) || !(bb > 0)) { ok = true; break; }
@@ -273,15 +287,17 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 		}
 		have_J = true;
 		c->stats.linsolves++;
-		bool kok = true; int kit = 0;
+		bool kok = true, crit = false; int kit = 0;
 		if (tri) {
 			TRegion tr(c, T_OTHER);
 			launch_thomas(n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
 		} else {
-			rc = krylov(c, kit, kok, res, !chord); if (rc) return rc;
+			rc = krylov(c, kit, kok, res, !chord, crit); if (rc) return rc;
 		}
-		launch_stepcrit(n, c->dP, c->Pst, c->sc, &c->nrm[64], c->st);
-		rc = fetch(c); if (rc) return rc;
+		if (!crit) {   // the FP32 form reads the step criterion and the flags with its last defect
+			launch_stepcrit(n, c->dP, c->Pst, c->sc, &c->nrm[64], c->st);
+			rc = fetch(c); if (rc) return rc;
+		}
 		if (c->hs->flags[FLAG_JAC]) return SPB6_NEWTON_FAIL;   // the reference returns before its linear solve (PC:806)
 		c->stats.krylov_it += kit;
 		if (!kok) return SPB6_KRYLOV_FAIL;
@@ -483,9 +499,11 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	}
 	c->part_stride = (int)((n + 255) / 256) + 1;   // partial-sum slots sized for 256-row blocks
 	if (dalloc(c, &c->part, (size_t)KRYLOV_SLOTS * c->part_stride)) return fail("alloc");
-	if (hipMalloc((void**)&c->ksd, sizeof(KS)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->ksd);
-	if (hipMalloc((void**)&c->nrm, 8 * 64 * sizeof(unsigned long long)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->nrm);
-	if (hipMalloc((void**)&c->flags, NFLAGS * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->flags);
+	if (dalloc(c, &c->part2, 2 * (size_t)c->part_stride)) return fail("alloc");
+	if (hipMalloc((void**)&c->mail, sizeof(HostScratch)) != hipSuccess) return fail("alloc");
+	c->allocs.push_back(c->mail);
+	if (hipMemset(c->mail, 0, sizeof(HostScratch)) != hipSuccess) return fail("alloc");
+	c->nrm = c->mail->nrm; c->flags = c->mail->flags; c->ksd = &c->mail->ks; c->dsum = c->mail->dsum;
 	if (hipHostMalloc((void**)&c->hs, sizeof(HostScratch), hipHostMallocDefault) != hipSuccess) return fail("host alloc");
 	if (dalloc(c, &c->Pplain, n) || dalloc(c, &c->Gplain, n)) return fail("alloc");
 	if (hipMalloc((void**)&c->slist, (n + 1) * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->slist);
@@ -689,7 +707,7 @@ int spb6_linsolve(spb6_ctx* c, const double* val, const double* b, double* x, in
 	HIPCHK(hipMemcpyAsync(c->rhs, b, n * sizeof(double), hipMemcpyHostToDevice, c->st));
 	int it = 0; bool ok = true;
 	if (use_thomas) launch_thomas((int)n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
-	else { int rc = krylov(c, it, ok, 1.0, true); if (rc) return rc; }
+	else { bool crit; int rc = krylov(c, it, ok, 1.0, true, crit); if (rc) return rc; }
 	HIPCHK(hipMemcpyAsync(x, c->dP, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
 	HIPCHK(hipStreamSynchronize(c->st));
 	if (iters) *iters = it;
