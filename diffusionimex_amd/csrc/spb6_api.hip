@@ -25,8 +25,8 @@ static thread_local std::string g_err;
 namespace {
 
 // the scalars the host reads back: norms (8 slots of 64 striped maxima), flags, the Krylov loop state and the FP32
-// form's defect sums. The device keeps the same struct in one allocation, so every read-back is one copy.
-struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; double dsum[2]; };
+// form's defect sums and step criterion. The device keeps the same struct in one allocation, so every read-back is one copy.
+struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; double dsum[3]; };
 
 // T_KRYLOV: BiCGStab iteration passes only; T_KAUX: the solves' setup, precision conversions and FP64 defects;
 // T_GUESS: the learned guess's features and inference
@@ -189,8 +189,8 @@ int krylov_run(spb6_ctx* c, const KrylovBufsT<T>& B, const T* b, double tol, int
 // sweep asking its FP32 solve for max(eta / 2, 5e-7) relative accuracy (the FP32 floor), so that one sweep meets
 // eta >= 1e-6. A step whose defect misses eta but has decreased is still returned: Newton's residual test and line
 // search judge it.
-int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, bool& crit) {
-	crit = false;
+int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, bool& crit, double& sn) {
+	crit = false; sn = 0.0;
 	const long n = c->g.n;
 	if (!c->cfg.krylov_fp32) {
 		KrylovBufs B{c->dP, c->kr, c->kr0, c->kp, c->kv, c->ks_, c->kt, c->J, c->part, c->part_stride, c->ksd};
@@ -221,7 +221,7 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, boo
 			}
 			{
 				TRegion tr(c, T_KAUX);
-				launch_defect_x(g, c->J, c->rhs, c->dP, c->x32, c->kr, c->part2, c->part_stride, c->dsum, c->Pst, c->sc, &c->nrm[64], c->st);
+				launch_defect_x(g, c->J, c->rhs, c->dP, c->x32, c->kr, c->part2, c->part_stride, c->dsum, c->Pst, c->sc, c->st);
 			}
 			HIPCHK(hipMemcpyAsync(c->hs, c->mail, sizeof(HostScratch), hipMemcpyDeviceToHost, c->st));
 			HIPCHK(hipStreamSynchronize(c->st));
@@ -232,7 +232,7 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, boo
 		const bool kok = c->hs->ks.done && c->hs->ks.status == 0;
 		c->last_krylov = it; c->krylov_launched += launched; iters += it;
 		{ TRegion tr(c, T_KAUX); launch_add_f32(n, c->x32, c->dP, c->st); }
-		crit = true;   // the step criterion just read is that of dP as it now stands
+		crit = true; sn = c->hs->dsum[2];   // the step criterion just read is that of dP as it now stands
 		const double rr = c->hs->dsum[0], bb = c->hs->dsum[1];
 		if (getenv("SPB6_VERBOSE") && atoi(getenv("SPB6_VERBOSE")) >= 3)
 			fprintf(stderr, "        sweep %d krylov %d defect %.3e eta %.3e\n", sweep, it, std::sqrt(rr / bb), eta);
@@ -287,12 +287,12 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 		}
 		have_J = true;
 		c->stats.linsolves++;
-		bool kok = true, crit = false; int kit = 0;
+		bool kok = true, crit = false; int kit = 0; double snk = 0.0;
 		if (tri) {
 			TRegion tr(c, T_OTHER);
 			launch_thomas(n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
 		} else {
-			rc = krylov(c, kit, kok, res, !chord, crit); if (rc) return rc;
+			rc = krylov(c, kit, kok, res, !chord, crit, snk); if (rc) return rc;
 		}
 		if (!crit) {   // the FP32 form reads the step criterion and the flags with its last defect
 			launch_stepcrit(n, c->dP, c->Pst, c->sc, &c->nrm[64], c->st);
@@ -301,7 +301,7 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 		if (c->hs->flags[FLAG_JAC]) return SPB6_NEWTON_FAIL;   // the reference returns before its linear solve (PC:806)
 		c->stats.krylov_it += kit;
 		if (!kok) return SPB6_KRYLOV_FAIL;
-		const double sn = nrm_val(c, 1);
+		const double sn = crit ? snk : nrm_val(c, 1);
 		if (verbose >= 3) fprintf(stderr, "      step %.3e krylov %d\n", sn, kit);
 		if (sn < tol) {   // step criterion (PC:819-826): accept P + dP
 			rc = eval_all(c, D, c->Pst, c->dP, 1.0, c->Pn, c->Gn, ok, res); if (rc) return rc;
@@ -499,7 +499,7 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	}
 	c->part_stride = (int)((n + 255) / 256) + 1;   // partial-sum slots sized for 256-row blocks
 	if (dalloc(c, &c->part, (size_t)KRYLOV_SLOTS * c->part_stride)) return fail("alloc");
-	if (dalloc(c, &c->part2, 2 * (size_t)c->part_stride)) return fail("alloc");
+	if (dalloc(c, &c->part2, 3 * (size_t)c->part_stride)) return fail("alloc");
 	if (hipMalloc((void**)&c->mail, sizeof(HostScratch)) != hipSuccess) return fail("alloc");
 	c->allocs.push_back(c->mail);
 	if (hipMemset(c->mail, 0, sizeof(HostScratch)) != hipSuccess) return fail("alloc");
@@ -707,7 +707,7 @@ int spb6_linsolve(spb6_ctx* c, const double* val, const double* b, double* x, in
 	HIPCHK(hipMemcpyAsync(c->rhs, b, n * sizeof(double), hipMemcpyHostToDevice, c->st));
 	int it = 0; bool ok = true;
 	if (use_thomas) launch_thomas((int)n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
-	else { bool crit; int rc = krylov(c, it, ok, 1.0, true, crit); if (rc) return rc; }
+	else { bool crit; double sn; int rc = krylov(c, it, ok, 1.0, true, crit, sn); if (rc) return rc; }
 	HIPCHK(hipMemcpyAsync(x, c->dP, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
 	HIPCHK(hipStreamSynchronize(c->st));
 	if (iters) *iters = it;

@@ -53,9 +53,9 @@ void launch_prep(const Dev& D, int limiter, int order, double tau, const double*
 void launch_shock(const Dev& D, int* list, int* count, int strong_blocks, hipStream_t st);
 void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st);
 void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, bool analytic, hipStream_t st);
-// FP64 defect b - J (dP + x) and the step criterion of dP + x for a running FP32 solve x, (r.r, b.b) into sums[2]
+// FP64 defect b - J (dP + x) and the step criterion of dP + x for a running FP32 solve x: (r.r, b.b, criterion) into sums[3]
 void launch_defect_x(const Geo& g, const double* J, const double* b, const double* dP, const float* x, double* r, double* part, int stride,
-                     double* sums, const double* P, const double* sc, unsigned long long* nrm, hipStream_t st);
+                     double* sums, const double* P, const double* sc, hipStream_t st);
 void launch_chord_rhs(const Dev& D, const double* P, const double* G, double* sc, double* rhs, hipStream_t st);
 void launch_fdscale(const Dev& D, const double* P, double* sc, hipStream_t st);
 void launch_stepcrit(int n, const double* dP, const double* P, const double* sc, unsigned long long* nrm, hipStream_t st);

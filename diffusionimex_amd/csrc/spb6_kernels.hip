@@ -533,14 +533,15 @@ __global__ void __launch_bounds__(256) k_linres(Geo g, const double* __restrict_
 	if (threadIdx.x == 0) { part[blockIdx.x] = t1; part[stride + blockIdx.x] = t2; }
 }
 // FP64 defect r = b - J (dP + x) of the accumulated step plus the running FP32 solution x, without writing dP (the FP32
-// solve may continue), with the block partials of r.r and b.b in part (slots 0 and 1 of stride); the values equal
-// k_linres after k_add_f32
+// solve may continue), and the Newton step criterion (PC:819-823) of dP + x: block partials of r.r and b.b (slots 0, 1
+// of stride) and block maxima of |dP + x| / (|P| + s) (slot 2); the values equal k_linres and k_stepcrit after k_add_f32
 __global__ void __launch_bounds__(256) k_linres_x(Geo g, const double* __restrict__ J, const double* __restrict__ b, const double* __restrict__ dP,
-                                                 const float* __restrict__ x, double* __restrict__ r, double* __restrict__ part, int stride) {
+                                                 const float* __restrict__ x, const double* __restrict__ P, const double* __restrict__ sc,
+                                                 double* __restrict__ r, double* __restrict__ part, int stride) {
 	__shared__ double sh[4];
 	const int n = g.n;
 	const int q = blockIdx.x * 256 + threadIdx.x;
-	double rr = 0.0, bb = 0.0;
+	double rr = 0.0, bb = 0.0, crit = 0.0;
 	if (q < n) {
 		const Nb nb = neighbours(g, q);
 		const int col[5] = {q, nb.W, nb.E, nb.S, nb.N}, ne = g.twoD ? 5 : 3;
@@ -548,25 +549,29 @@ __global__ void __launch_bounds__(256) k_linres_x(Geo g, const double* __restric
 		#pragma unroll
 		for (int k = 0; k < 5; k++) if (k < ne) y += J[k * n + q] * (dP[col[k]] + (double)x[col[k]]);
 		const double v = b[q] - y; r[q] = v; rr = v * v; bb = b[q] * b[q];
+		crit = fabs(dP[q] + (double)x[q]) / (fabs(P[q]) + sc[q]);
 	}
 	const double t1 = block_sum256(rr, sh), t2 = block_sum256(bb, sh);
-	if (threadIdx.x == 0) { part[blockIdx.x] = t1; part[stride + blockIdx.x] = t2; }
+	crit = wave_max(crit);
+	__syncthreads();
+	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = crit;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		part[blockIdx.x] = t1; part[stride + blockIdx.x] = t2;
+		part[2 * stride + blockIdx.x] = smax(smax(sh[0], sh[1]), smax(sh[2], sh[3]));
+	}
 }
-// the two partial slots summed in one block in the fixed order: out = (r.r, b.b)
+// the partial slots reduced in one block in the fixed order: out = (r.r, b.b, max criterion)
 __global__ void __launch_bounds__(256) k_sum_part2(const double* __restrict__ part, int nb, int stride, double* __restrict__ out) {
 	__shared__ double sh[4];
-	double s0 = 0.0, s1 = 0.0;
-	for (int c = threadIdx.x; c < nb; c += 256) { s0 += part[c]; s1 += part[stride + c]; }
+	double s0 = 0.0, s1 = 0.0, m = 0.0;
+	for (int c = threadIdx.x; c < nb; c += 256) { s0 += part[c]; s1 += part[stride + c]; m = smax(m, part[2 * stride + c]); }
 	s0 = block_sum256(s0, sh); s1 = block_sum256(s1, sh);
-	if (threadIdx.x == 0) { out[0] = s0; out[1] = s1; }
-}
-// Newton step criterion (PC:819-823) of dP + x, as k_stepcrit after k_add_f32
-__global__ void __launch_bounds__(256) k_stepcrit_x(int n, const double* __restrict__ dP, const float* __restrict__ x, const double* __restrict__ P,
-                                                   const double* __restrict__ sc, unsigned long long* __restrict__ nrm) {
-	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	double v = 0.0;
-	if (q < n) v = fabs(dP[q] + (double)x[q]) / (fabs(P[q]) + sc[q]);
-	atomic_max_nonneg(nrm, v);
+	m = wave_max(m);
+	__syncthreads();
+	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+	__syncthreads();
+	if (threadIdx.x == 0) { out[0] = s0; out[1] = s1; out[2] = smax(smax(sh[0], sh[1]), smax(sh[2], sh[3])); }
 }
 // x += (double) d
 __global__ void __launch_bounds__(256) k_add_f32(long n, const float* __restrict__ d, double* __restrict__ x) {
@@ -577,12 +582,10 @@ void launch_linres(const Geo& g, const double* J, const double* b, const double*
 	hipLaunchKernelGGL(k_linres, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, st, g, J, b, x, r, part, stride);
 }
 void launch_defect_x(const Geo& g, const double* J, const double* b, const double* dP, const float* x, double* r, double* part, int stride,
-                     double* sums, const double* P, const double* sc, unsigned long long* nrm, hipStream_t st) {
+                     double* sums, const double* P, const double* sc, hipStream_t st) {
 	const unsigned nb = (unsigned)((g.n + 255) / 256);
-	(void)hipMemsetAsync(nrm, 0, 64 * sizeof(unsigned long long), st);
-	hipLaunchKernelGGL(k_linres_x, dim3(nb), dim3(256), 0, st, g, J, b, dP, x, r, part, stride);
+	hipLaunchKernelGGL(k_linres_x, dim3(nb), dim3(256), 0, st, g, J, b, dP, x, P, sc, r, part, stride);
 	hipLaunchKernelGGL(k_sum_part2, dim3(1), dim3(256), 0, st, (const double*)part, (int)nb, stride, sums);
-	hipLaunchKernelGGL(k_stepcrit_x, dim3(nb), dim3(256), 0, st, g.n, dP, x, P, sc, nrm);
 }
 void launch_add_f32(long n, const float* d, double* x, hipStream_t st) { hipLaunchKernelGGL(k_add_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, d, x); }
 
