@@ -68,6 +68,7 @@ struct spb6_ctx {
 	double *kr, *kr0, *kp, *kv, *ks_, *kt;
 	// FP32 Krylov form: matrix, right-hand side and vectors in single precision
 	float *J32 = nullptr, *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *r032 = nullptr, *p32 = nullptr, *v32 = nullptr, *s32 = nullptr, *t32 = nullptr;
+	float *ph32 = nullptr, *sh32 = nullptr, *dg32 = nullptr;   // red-black preconditioner vectors of the FP32 solve
 	double* part = nullptr; int part_stride = 0; KS* ksd = nullptr;
 	double *part2 = nullptr, *dsum = nullptr;   // the FP32 form's defect partials and their two sums
 	HostScratch* mail = nullptr;               // device side of hs: nrm, flags, ksd and dsum point into it
@@ -203,7 +204,7 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, boo
 		launch_to_f32(n, c->rhs, c->b32, c->st);
 		HIPCHK(hipMemsetAsync(c->dP, 0, n * sizeof(double), c->st));
 	}
-	KrylovBufs32 B{c->x32, c->r32, c->r032, c->p32, c->v32, c->s32, c->t32, c->J32, c->part, c->part_stride, c->ksd};
+	KrylovBufs32 B{c->x32, c->r32, c->r032, c->p32, c->v32, c->s32, c->t32, c->J32, c->part, c->part_stride, c->ksd, c->ph32, c->sh32, c->dg32};
 	iters = 0; ok = false;
 	const double tin = std::max(0.5 * eta, 5e-7);
 	const int maxit = c->cfg.krylov_iter;
@@ -493,7 +494,7 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	for (auto p : vecs) if (dalloc(c, p, n)) return fail("alloc");
 	if (dalloc(c, &c->J, 5 * n)) return fail("alloc");
 	if (cfg->krylov_fp32) {
-		float** fv[] = {&c->b32, &c->x32, &c->r32, &c->r032, &c->p32, &c->v32, &c->s32, &c->t32};
+		float** fv[] = {&c->b32, &c->x32, &c->r32, &c->r032, &c->p32, &c->v32, &c->s32, &c->t32, &c->ph32, &c->sh32, &c->dg32};
 		for (auto p : fv) { if (hipMalloc((void**)p, n * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(*p); }
 		if (hipMalloc((void**)&c->J32, 5 * n * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->J32);
 	}

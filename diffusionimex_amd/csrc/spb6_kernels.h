@@ -37,6 +37,7 @@ template <typename T> struct KrylovBufsT {
 	T *x, *r, *r0, *p, *v, *s, *t;
 	const T* J;
 	double* part; int stride; KS* ks;
+	T *ph = nullptr, *sh = nullptr, *dg = nullptr;   // red-black sweep preconditioner: M^-1 p, M^-1 s, 1 / diagonal
 };
 using KrylovBufs = KrylovBufsT<double>;
 using KrylovBufs32 = KrylovBufsT<float>;

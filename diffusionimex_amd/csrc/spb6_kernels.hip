@@ -254,12 +254,13 @@ __global__ void __launch_bounds__(256) k_thomas(int n, const double* __restrict_
 // Rows: block b, thread t owns rows b*256*IT + it*256 + t (it = 0..IT-1), the fixed order of the sums. IT is a
 // compile-time count, and each thread loads the inputs of a chunk of its rows into registers before it computes them
 // (the first chunk before the partial re-reduction that opens the pass), so several loads per thread are in flight.
-template <typename T> struct KV { T *x, *r, *r0, *p, *v, *s, *t; const T* J; };
+template <typename T> struct KV { T *x, *r, *r0, *p, *v, *s, *t; const T* J; T *ph, *sh, *dg; };
 struct KP { double* part; int stride; int nb; };
 enum { SL_RR = 0, SL_R0R, SL_SP, SL_R0V, SL_SS, SL_SSUM, SL_TS, SL_TT, SL_E, SL_SB, SL_BB, NSLOTS };
 
 template <typename T>
 __device__ __forceinline__ T dinv_v(T d) { return d != T(0) ? T(1) / d : T(1); }
+__device__ __forceinline__ bool is_red(const Geo& g, int r) { return (((r % g.nx) + (r / g.nx)) & 1) == 0; }
 template <typename T>
 __device__ __forceinline__ T dinv_of(const T* __restrict__ J, int r) { return dinv_v(J[r]); }
 // the inputs of one row's product: entries (self, W, E, S, N), the vector and the diagonal at those columns
@@ -356,7 +357,7 @@ __global__ void __launch_bounds__(256) kb_setup3(Geo g, const T* __restrict__ b,
 	put_partial(P, SL_RR, srr, sh); put_partial(P, SL_R0R, sr0r, sh);
 }
 // I1 (iteration it): loop top (PC:741-744) from r.r and r0.r, then p = r + beta (p - omega v); partials of sum(p)
-template <typename T, int IT>
+template <typename T, int IT, bool RB>
 __global__ void __launch_bounds__(256) kb_p(Geo g, KV<T> V, KS* ks, KP P, int it, double tol, int maxit) {
 	if (ks->done) return;
 	__shared__ double sh[4];
@@ -385,40 +386,46 @@ __global__ void __launch_bounds__(256) kb_p(Geo g, KV<T> V, KS* ks, KP P, int it
 		#pragma unroll
 		for (int k = 0; k < CHV; k++) {
 			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
-			const T pn = rv[k] + beta * (pv[k] - omega * vv[k]); V.p[r] = pn; sp += (double)pn;
+			const T pn = rv[k] + beta * (pv[k] - omega * vv[k]); V.p[r] = pn;
+			if constexpr (RB) { if (is_red(g, r)) V.ph[r] = pn * V.dg[r]; }
+			else sp += (double)pn;
 		}
 	}
-	put_partial(P, SL_SP, sp, sh);
+	if constexpr (!RB) put_partial(P, SL_SP, sp, sh);
 }
 // I2: cc_p = sum(p) Einv; v = A M^-1 p; partials of r0.v
-template <typename T, int IT>
+template <typename T, int IT, bool RB>
 __global__ void __launch_bounds__(256) kb_v(Geo g, KV<T> V, KS* ks, KP P, int it) {
 	if (ks->done) return;
 	__shared__ double sh[4];
 	const int n = g.n, ne = g.twoD ? 5 : 3;
 	constexpr int CHS = chs<T>();
+	const T* z = RB ? (const T*)V.ph : (const T*)V.p;
 	SpIn<T> q[CHS]; T r0v[CHS];
 	#pragma unroll
-	for (int k = 0; k < CHS; k++) { const int r = krow(IT, k); if (k < IT && r < n) { sp_load(g, V.J, (const T*)V.p, r, true, q[k]); r0v[k] = V.r0[r]; } }
-	const T cc = (T)grid_total(P, SL_SP, sh) * (T)ks->Einv;
-	if (blockIdx.x == 0 && threadIdx.x == 0) ks->cc_p = (double)cc;
+	for (int k = 0; k < CHS; k++) { const int r = krow(IT, k); if (k < IT && r < n) { sp_load(g, V.J, z, r, !RB, q[k]); r0v[k] = V.r0[r]; } }
+	T cc = T(0);
+	if constexpr (!RB) {
+		cc = (T)grid_total(P, SL_SP, sh) * (T)ks->Einv;
+		if (blockIdx.x == 0 && threadIdx.x == 0) ks->cc_p = (double)cc;
+	}
 	double s = 0.0;
 	#pragma unroll
 	for (int c0 = 0; c0 < IT; c0 += CHS) {
 		if (c0 > 0) {
 			#pragma unroll
-			for (int k = 0; k < CHS; k++) { const int r = krow(IT, c0 + k); if (c0 + k < IT && r < n) { sp_load(g, V.J, (const T*)V.p, r, true, q[k]); r0v[k] = V.r0[r]; } }
+			for (int k = 0; k < CHS; k++) { const int r = krow(IT, c0 + k); if (c0 + k < IT && r < n) { sp_load(g, V.J, z, r, !RB, q[k]); r0v[k] = V.r0[r]; } }
 		}
 		#pragma unroll
 		for (int k = 0; k < CHS; k++) {
 			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
-			const T vv = sp_prec(q[k], ne, cc); V.v[r] = vv; s += (double)(r0v[k] * vv);
+			const T vv = RB ? sp_plain(q[k], ne) : sp_prec(q[k], ne, cc); V.v[r] = vv; s += (double)(r0v[k] * vv);
 		}
 	}
 	put_partial(P, SL_R0V, s, sh);
 }
 // I3: alpha = rho / r0.v; s = r - alpha v; partials of s.s and sum(s)
-template <typename T, int IT>
+template <typename T, int IT, bool RB>
 __global__ void __launch_bounds__(256) kb_s(Geo g, KV<T> V, KS* ks, KP P, int it) {
 	if (ks->done) return;
 	__shared__ double sh[4];
@@ -440,50 +447,56 @@ __global__ void __launch_bounds__(256) kb_s(Geo g, KV<T> V, KS* ks, KP P, int it
 		for (int k = 0; k < CHV; k++) {
 			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
 			const T sv = rv[k] - alpha * vv[k]; V.s[r] = sv; sss += (double)(sv * sv); ssum += (double)sv;
+			if constexpr (RB) { if (is_red(g, r)) V.sh[r] = sv * V.dg[r]; }
 		}
 	}
 	put_partial(P, SL_SS, sss, sh); put_partial(P, SL_SSUM, ssum, sh);
 }
 // I4: early exit if |s| <= tol |b| (x += alpha M^-1 p, PC:751); otherwise cc_s = sum(s) Einv, t = A M^-1 s; partials of
 // t.s and t.t
-template <typename T, int IT>
+template <typename T, int IT, bool RB>
 __global__ void __launch_bounds__(256) kb_t(Geo g, KV<T> V, KS* ks, KP P, int it, double tol) {
 	if (ks->done) return;
 	__shared__ double sh[4];
 	const int n = g.n, nxt = (it + 1) & 1, ne = g.twoD ? 5 : 3;
 	constexpr int CHS = chs<T>();
-	SpIn<T> q[CHS];
+	const T* z = RB ? (const T*)V.sh : (const T*)V.s;
+	SpIn<T> q[CHS]; T sv[CHS];
 	#pragma unroll
-	for (int k = 0; k < CHS; k++) { const int r = krow(IT, k); if (k < IT && r < n) sp_load(g, V.J, (const T*)V.s, r, true, q[k]); }
+	for (int k = 0; k < CHS; k++) { const int r = krow(IT, k); if (k < IT && r < n) { sp_load(g, V.J, z, r, !RB, q[k]); if constexpr (RB) sv[k] = V.s[r]; } }
 	const double s2 = grid_total(P, SL_SS, sh), Ss = grid_total(P, SL_SSUM, sh);
 	const T alpha = (T)ks->alpha[nxt];
 	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
 	if ((T)sqrt((double)(T)s2) <= (T)tol * (T)ks->bn) {
 		const T ccp = (T)ks->cc_p;
 		#pragma unroll
-		for (int k = 0; k < IT; k++) { const int r = krow(IT, k); if (r < n) V.x[r] += alpha * (dinv_of(V.J, r) * V.p[r] + ccp); }
+		for (int k = 0; k < IT; k++) {
+			const int r = krow(IT, k);
+			if (r < n) V.x[r] += alpha * (RB ? V.ph[r] : dinv_of(V.J, r) * V.p[r] + ccp);
+		}
 		if (lead) { ks->early = 1; ks->iters = it + 1; ks->status = 0; }
 		return;
 	}
-	const T cc = (T)Ss * (T)ks->Einv;
+	const T cc = RB ? T(0) : (T)Ss * (T)ks->Einv;
 	if (lead) ks->cc_s = (double)cc;
 	double sts = 0.0, stt = 0.0;
 	#pragma unroll
 	for (int c0 = 0; c0 < IT; c0 += CHS) {
 		if (c0 > 0) {
 			#pragma unroll
-			for (int k = 0; k < CHS; k++) { const int r = krow(IT, c0 + k); if (c0 + k < IT && r < n) sp_load(g, V.J, (const T*)V.s, r, true, q[k]); }
+			for (int k = 0; k < CHS; k++) { const int r = krow(IT, c0 + k); if (c0 + k < IT && r < n) { sp_load(g, V.J, z, r, !RB, q[k]); if constexpr (RB) sv[k] = V.s[r]; } }
 		}
 		#pragma unroll
 		for (int k = 0; k < CHS; k++) {
 			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
-			const T tv = sp_prec(q[k], ne, cc); V.t[r] = tv; sts += (double)(tv * q[k].z[0]); stt += (double)(tv * tv);
+			const T tv = RB ? sp_plain(q[k], ne) : sp_prec(q[k], ne, cc); V.t[r] = tv;
+			sts += (double)(tv * (RB ? sv[k] : q[k].z[0])); stt += (double)(tv * tv);
 		}
 	}
 	put_partial(P, SL_TS, sts, sh); put_partial(P, SL_TT, stt, sh);
 }
 // I5: omega = t.s / t.t; x += alpha M^-1 p + omega M^-1 s; r = s - omega t; partials of r.r and r0.r
-template <typename T, int IT>
+template <typename T, int IT, bool RB>
 __global__ void __launch_bounds__(256) kb_xr(Geo g, KV<T> V, KS* ks, KP P, int it) {
 	if (ks->done) return;
 	const int nxt = (it + 1) & 1;
@@ -496,7 +509,10 @@ __global__ void __launch_bounds__(256) kb_xr(Geo g, KV<T> V, KS* ks, KP P, int i
 		#pragma unroll
 		for (int k = 0; k < CHV; k++) {
 			const int r = krow(IT, c0 + k);
-			if (c0 + k < IT && r < n) { dv[k] = V.J[r]; pv[k] = V.p[r]; sv[k] = V.s[r]; tv[k] = V.t[r]; xv[k] = V.x[r]; r0v[k] = V.r0[r]; }
+			if (c0 + k < IT && r < n) {
+				if constexpr (RB) { dv[k] = V.sh[r]; pv[k] = V.ph[r]; } else { dv[k] = V.J[r]; pv[k] = V.p[r]; }
+				sv[k] = V.s[r]; tv[k] = V.t[r]; xv[k] = V.x[r]; r0v[k] = V.r0[r];
+			}
 		}
 	};
 	load(0);
@@ -511,12 +527,77 @@ __global__ void __launch_bounds__(256) kb_xr(Geo g, KV<T> V, KS* ks, KP P, int i
 		#pragma unroll
 		for (int k = 0; k < CHV; k++) {
 			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
-			const T di = dinv_v(dv[k]);
-			const T ph = di * pv[k] + ccp, shv = di * sv[k] + ccs;
+			T ph, shv;
+			if constexpr (RB) { ph = pv[k]; shv = dv[k]; }
+			else { const T di = dinv_v(dv[k]); ph = di * pv[k] + ccp; shv = di * sv[k] + ccs; }
 			V.x[r] = xv[k] + (alpha * ph + omega * shv);
 			const T rv = sv[k] - omega * tv[k]; V.r[r] = rv;
 			srr += (double)(rv * rv); sr0r += (double)(r0v[k] * rv);
 		}
+	}
+	put_partial(P, SL_RR, srr, sh); put_partial(P, SL_R0R, sr0r, sh);
+}
+
+// Red-black variant (the FP32 form): M^-1 is one forward red-black Gauss-Seidel sweep from zero -- red rows
+// ((i + j) even) y = z / d, black rows y = (z - sum over red neighbours of A_rq y_q) / d, d being the row's diagonal
+// with its boundary self-columns (a neighbour index equal to the row) added. Couplings between rows of one colour (an
+// odd periodic wrap) are left out, so M^-1 is a fixed linear operator. The red rows of M^-1 p and M^-1 s are written by
+// kb_p / kb_s, the black rows by kbr_black, and the products read them (no constant coarse space: with the sweep it
+// slowed convergence on every workload measured, e.g. 3 -> 6 iterations on the 2048x1024 shock-bubble system).
+// setup: dg = 1 / d; x = M^-1 b on the red rows (kbr_black completes it); p = v = 0; partials of b.b
+template <typename T, int IT>
+__global__ void __launch_bounds__(256) kbr_setup1(Geo g, const T* __restrict__ b, KV<T> V, KP P) {
+	__shared__ double sh[4];
+	const int n = g.n, ne = g.twoD ? 5 : 3;
+	double sbb = 0.0;
+	#pragma unroll
+	for (int it = 0; it < IT; it++) {
+		const int r = krow(IT, it); if (r >= n) continue;
+		const Nb nb = neighbours(g, r);
+		const int col[5] = {r, nb.W, nb.E, nb.S, nb.N};
+		T d = V.J[r];
+		#pragma unroll
+		for (int k = 1; k < 5; k++) if (k < ne && col[k] == r) d += V.J[k * n + r];
+		const T di = dinv_v(d);
+		V.dg[r] = di; V.x[r] = is_red(g, r) ? b[r] * di : T(0); V.p[r] = T(0); V.v[r] = T(0);
+		sbb += (double)(b[r] * b[r]);
+	}
+	put_partial(P, SL_BB, sbb, sh);
+}
+// black rows of y = M^-1 z from the red rows already in y; skipped once the solve is done (unless in the setup)
+template <typename T, int IT>
+__global__ void __launch_bounds__(256) kbr_black(Geo g, const T* __restrict__ z, T* __restrict__ y, const T* __restrict__ J,
+                                                 const T* __restrict__ dg, KS* ks, int setup) {
+	if (!setup && ks->done) return;
+	const int n = g.n, ne = g.twoD ? 5 : 3;
+	#pragma unroll
+	for (int it = 0; it < IT; it++) {
+		const int r = krow(IT, it); if (r >= n || is_red(g, r)) continue;
+		const Nb nb = neighbours(g, r);
+		const int col[5] = {r, nb.W, nb.E, nb.S, nb.N};
+		T acc = z[r];
+		#pragma unroll
+		for (int k = 1; k < 5; k++) if (k < ne && col[k] != r && is_red(g, col[k])) acc -= J[k * n + r] * y[col[k]];
+		y[r] = acc * dg[r];
+	}
+}
+// setup 2: loop scalars from b.b; r = b - A x, r0 = r; partials of r.r, r0.r
+template <typename T, int IT>
+__global__ void __launch_bounds__(256) kbr_setup2(Geo g, const T* __restrict__ b, KV<T> V, KS* ks, KP P) {
+	__shared__ double sh[4];
+	const int n = g.n;
+	const double bb = grid_total(P, SL_BB, sh);
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		ks->Einv = 0.0; ks->cc_p = 0.0; ks->cc_s = 0.0; ks->bn = (double)((T)sqrt((double)(T)bb) + (T)1e-300);
+		ks->rho[0] = 1; ks->alpha[0] = 1; ks->omega[0] = 1;
+		ks->done = 0; ks->iters = 0; ks->early = 0; ks->status = 0;
+	}
+	double srr = 0.0, sr0r = 0.0;
+	#pragma unroll
+	for (int it = 0; it < IT; it++) {
+		const int r = krow(IT, it); if (r >= n) continue;
+		const T rv = b[r] - spmv_plain(g, V.J, (const T*)V.x, r); V.r[r] = rv; V.r0[r] = rv;
+		srr += (double)(rv * rv); sr0r += (double)(rv * rv);
 	}
 	put_partial(P, SL_RR, srr, sh); put_partial(P, SL_R0R, sr0r, sh);
 }
@@ -628,9 +709,15 @@ template <typename T>
 static void krylov_setup_t(const Geo& g, const KrylovBufsT<T>& B, const T* b, hipStream_t st) {
 	const int items = items_of<T>(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
 	KP P{B.part, B.stride, nb};
-	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J};
+	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J, B.ph, B.sh, B.dg};
 	with_items(items, [&](auto I) {
 		constexpr int IT = decltype(I)::value;
+		if (B.dg) {
+			hipLaunchKernelGGL((kbr_setup1<T, IT>), dim3(nb), dim3(256), 0, st, g, b, V, P);
+			hipLaunchKernelGGL((kbr_black<T, IT>), dim3(nb), dim3(256), 0, st, g, b, B.x, B.J, (const T*)B.dg, B.ks, 1);
+			hipLaunchKernelGGL((kbr_setup2<T, IT>), dim3(nb), dim3(256), 0, st, g, b, V, B.ks, P);
+			return;
+		}
 		hipLaunchKernelGGL((kb_setup1<T, IT>), dim3(nb), dim3(256), 0, st, g, B.J, b, P);
 		hipLaunchKernelGGL((kb_setup2<T, IT>), dim3(nb), dim3(256), 0, st, g, B.J, b, V, B.ks, P);
 		hipLaunchKernelGGL((kb_setup3<T, IT>), dim3(nb), dim3(256), 0, st, g, b, V, P);
@@ -640,14 +727,24 @@ template <typename T>
 static void krylov_iter_t(const Geo& g, const KrylovBufsT<T>& B, hipStream_t st, int it, double tol, int maxit) {
 	const int items = items_of<T>(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
 	KP P{B.part, B.stride, nb};
-	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J};
+	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J, B.ph, B.sh, B.dg};
 	with_items(items, [&](auto I) {
 		constexpr int IT = decltype(I)::value;
-		hipLaunchKernelGGL((kb_p<T, IT>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol, maxit);
-		hipLaunchKernelGGL((kb_v<T, IT>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
-		hipLaunchKernelGGL((kb_s<T, IT>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
-		hipLaunchKernelGGL((kb_t<T, IT>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol);
-		hipLaunchKernelGGL((kb_xr<T, IT>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+		if (B.dg) {   // red-black sweep preconditioner (the FP32 form)
+			hipLaunchKernelGGL((kb_p<T, IT, true>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol, maxit);
+			hipLaunchKernelGGL((kbr_black<T, IT>), dim3(nb), dim3(256), 0, st, g, (const T*)B.p, B.ph, B.J, (const T*)B.dg, B.ks, 0);
+			hipLaunchKernelGGL((kb_v<T, IT, true>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+			hipLaunchKernelGGL((kb_s<T, IT, true>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+			hipLaunchKernelGGL((kbr_black<T, IT>), dim3(nb), dim3(256), 0, st, g, (const T*)B.s, B.sh, B.J, (const T*)B.dg, B.ks, 0);
+			hipLaunchKernelGGL((kb_t<T, IT, true>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol);
+			hipLaunchKernelGGL((kb_xr<T, IT, true>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+			return;
+		}
+		hipLaunchKernelGGL((kb_p<T, IT, false>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol, maxit);
+		hipLaunchKernelGGL((kb_v<T, IT, false>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+		hipLaunchKernelGGL((kb_s<T, IT, false>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
+		hipLaunchKernelGGL((kb_t<T, IT, false>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol);
+		hipLaunchKernelGGL((kb_xr<T, IT, false>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
 	});
 }
 void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hipStream_t st) { krylov_setup_t<double>(g, B, b, st); }
