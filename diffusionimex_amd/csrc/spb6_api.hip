@@ -190,14 +190,15 @@ int krylov_run(spb6_ctx* c, const KrylovBufsT<T>& B, const T* b, double tol, int
 // sweep asking its FP32 solve for max(eta / 2, 5e-7) relative accuracy (the FP32 floor), so that one sweep meets
 // eta >= 1e-6. A step whose defect misses eta but has decreased is still returned: Newton's residual test and line
 // search judge it.
-int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, bool& crit, double& sn) {
+int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, bool& crit, double& sn, bool finish) {
 	crit = false; sn = 0.0;
 	const long n = c->g.n;
 	if (!c->cfg.krylov_fp32) {
 		KrylovBufs B{c->dP, c->kr, c->kr0, c->kp, c->kv, c->ks_, c->kt, c->J, c->part, c->part_stride, c->ksd};
 		return krylov_run<double>(c, B, c->rhs, c->cfg.krylov_tol, iters, ok);
 	}
-	const double res = std::max(newton_res, 1e-300), eta = std::max(0.1 * c->cfg.newton_tol / res, std::min(1e-6, res));
+	const double res = std::max(newton_res, 1e-300);
+	const double eta = finish ? 0.1 * c->cfg.newton_tol / res : std::max(0.1 * c->cfg.newton_tol / res, std::min(1e-6, res));
 	{
 		TRegion tr(c, T_KAUX);
 		if (new_J) launch_to_f32(5 * n, c->J, c->J32, c->st);   // a chord step keeps the previous FP32 copy
@@ -293,7 +294,10 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 			TRegion tr(c, T_OTHER);
 			launch_thomas(n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
 		} else {
-			rc = krylov(c, kit, kok, res, !chord, crit, snk); if (rc) return rc;
+			// a step predicted to be the last (the observed contraction res / res_prev^2 puts the next residual below
+			// tol / 2) is solved to 0.1 tol / res, so that Newton finishes with it instead of taking one more step
+			const bool finish = chord_ok && full_step && res * res * res < 0.5 * tol * res_prev * res_prev;
+			rc = krylov(c, kit, kok, res, !chord, crit, snk, finish); if (rc) return rc;
 		}
 		if (!crit) {   // the FP32 form reads the step criterion and the flags with its last defect
 			launch_stepcrit(n, c->dP, c->Pst, c->sc, &c->nrm[64], c->st);
@@ -708,7 +712,7 @@ int spb6_linsolve(spb6_ctx* c, const double* val, const double* b, double* x, in
 	HIPCHK(hipMemcpyAsync(c->rhs, b, n * sizeof(double), hipMemcpyHostToDevice, c->st));
 	int it = 0; bool ok = true;
 	if (use_thomas) launch_thomas((int)n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
-	else { bool crit; double sn; int rc = krylov(c, it, ok, 1.0, true, crit, sn); if (rc) return rc; }
+	else { bool crit; double sn; int rc = krylov(c, it, ok, 1.0, true, crit, sn, false); if (rc) return rc; }
 	HIPCHK(hipMemcpyAsync(x, c->dP, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
 	HIPCHK(hipStreamSynchronize(c->st));
 	if (iters) *iters = it;
