@@ -112,8 +112,8 @@ def roofline_of(res, n, fp32, nx=2048, ny=1024):
         "prep": (kt["prep_ms"], kt["prep_n"], BYTES_2D["prep"] * n),
         "residual": (kt["residual_ms"], kt["residual_n"], BYTES_2D["residual"] * n),
         "jacobian": (kt["jacobian_ms"], kt["jacobian_n"], BYTES_2D["jacobian"] * n),
-        # per BiCGStab iteration taken: the region's time also holds the passes launched past a solve's end (chunks),
-        # which return at once, so this average is a lower bound on the bandwidth of an iteration
+        # per BiCGStab iteration taken: each launched iteration is timed by its own HIP events, and those launched past a
+        # solve's end (whole chunks are launched; they return at once) are kept apart in krylov_idle
         "krylov_iter": (kt["krylov_ms"], d["krylov_it"], kb * n),
         "final": (kt["final_ms"], kt["final_n"], BYTES_2D["final"] * n),
     }
@@ -121,6 +121,7 @@ def roofline_of(res, n, fp32, nx=2048, ny=1024):
     kernel_ms["shock"] = round(kt["shock_ms"], 3)
     kernel_ms["other"] = round(kt["other_ms"], 3)
     kernel_ms["krylov_aux"] = round(kt["krylov_aux_ms"], 3)
+    kernel_ms["krylov_idle"] = round(kt["krylov_idle_ms"], 3)
     kernel_ms["guess"] = round(kt["guess_ms"], 3)
     dom = max(cats, key=lambda k: cats[k][0])
     ms, cnt, byts = cats[dom]

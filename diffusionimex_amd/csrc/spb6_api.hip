@@ -30,10 +30,12 @@ struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; d
 
 // T_KRYLOV: BiCGStab iteration passes only; T_KAUX: the solves' setup, precision conversions and FP64 defects;
 // T_GUESS: the learned guess's features and inference
-enum TimerCat { T_EXPLICIT = 0, T_PREP, T_SHOCK, T_RESIDUAL, T_JACOBIAN, T_KRYLOV, T_FINAL, T_OTHER, T_KAUX, T_GUESS, T_NCAT };
+// T_KIDLE: BiCGStab iterations launched past a solve's end (in a chunk), which return at once
+enum TimerCat { T_EXPLICIT = 0, T_PREP, T_SHOCK, T_RESIDUAL, T_JACOBIAN, T_KRYLOV, T_FINAL, T_OTHER, T_KAUX, T_GUESS, T_KIDLE, T_NCAT };
 
 struct Timer {
 	bool on = false;
+	bool hold = false;   // events are held outside pend (the FP32 solve's per-iteration pairs): no resolve meanwhile
 	std::vector<hipEvent_t> pool;
 	struct Pend { int cat; hipEvent_t a, b; };
 	std::vector<Pend> pend;
@@ -68,7 +70,7 @@ struct spb6_ctx {
 	double *kr, *kr0, *kp, *kv, *ks_, *kt;
 	// FP32 Krylov form: matrix, right-hand side and vectors in single precision
 	float *J32 = nullptr, *b32 = nullptr, *x32 = nullptr, *r32 = nullptr, *r032 = nullptr, *p32 = nullptr, *v32 = nullptr, *s32 = nullptr, *t32 = nullptr;
-	float *ph32 = nullptr, *sh32 = nullptr, *dg32 = nullptr;   // red-black preconditioner vectors of the FP32 solve
+	float *ph32 = nullptr, *sh32 = nullptr, *dg32 = nullptr, *pp32 = nullptr;   // red-black preconditioner vectors of the FP32 solve
 	double* part = nullptr; int part_stride = 0; KS* ksd = nullptr;
 	double *part2 = nullptr, *dsum = nullptr;   // the FP32 form's defect partials and their two sums
 	HostScratch* mail = nullptr;               // device side of hs: nrm, flags, ksd and dsum point into it
@@ -129,7 +131,7 @@ Dev make_dev(const spb6_ctx* c) {
 struct TRegion {
 	spb6_ctx* c; int cat; hipEvent_t a = nullptr;
 	TRegion(spb6_ctx* c_, int cat_) : c(c_), cat(cat_) { if (c->timer.on) { a = c->timer.get(); hipEventRecord(a, c->st); } }
-	~TRegion() { if (c->timer.on) { hipEvent_t b = c->timer.get(); hipEventRecord(b, c->st); c->timer.pend.push_back({cat, a, b}); if (c->timer.pend.size() > 512) c->timer.resolve(); } }
+	~TRegion() { if (c->timer.on) { hipEvent_t b = c->timer.get(); hipEventRecord(b, c->st); c->timer.pend.push_back({cat, a, b}); if (c->timer.pend.size() > 512 && !c->timer.hold) c->timer.resolve(); } }
 };
 
 int reset_scalars(spb6_ctx* c) {
@@ -205,7 +207,7 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, boo
 		launch_to_f32(n, c->rhs, c->b32, c->st);
 		HIPCHK(hipMemsetAsync(c->dP, 0, n * sizeof(double), c->st));
 	}
-	KrylovBufs32 B{c->x32, c->r32, c->r032, c->p32, c->v32, c->s32, c->t32, c->J32, c->part, c->part_stride, c->ksd, c->ph32, c->sh32, c->dg32};
+	KrylovBufs32 B{c->x32, c->r32, c->r032, c->p32, c->v32, c->s32, c->t32, c->J32, c->part, c->part_stride, c->ksd, c->ph32, c->sh32, c->dg32, c->pp32};
 	iters = 0; ok = false;
 	const double tin = std::max(0.5 * eta, 5e-7);
 	const int maxit = c->cfg.krylov_iter;
@@ -215,11 +217,18 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, boo
 		// the sweep's BiCGStab passes in chunks sized by the previous solve; behind each chunk, in the same stream, the
 		// FP64 defect and the step criterion of dP + x, so that one synchronisation reads the solve's state, the defect
 		// and the criterion together (a solve that is not done yet continues; dP is only written once it is)
-		int chunk = std::max(4, c->last_krylov + 2), launched = 0;
+		int chunk = std::max(2, c->last_krylov + 1), launched = 0;
+		// with timing on, each launched iteration gets its own event pair; once the solve's iteration count is read,
+		// the iterations taken go to T_KRYLOV and those launched past the end to T_KIDLE
+		struct IterEv { int it; hipEvent_t a, b; };
+		std::vector<IterEv> iev;
+		c->timer.hold = c->timer.on;
 		for (;;) {
-			{
-				TRegion tr(c, T_KRYLOV);
-				for (int k = 0; k < chunk && launched <= maxit; k++, launched++) launch_krylov_iter32(g, B, c->st, launched, tin, maxit);
+			for (int k = 0; k < chunk && launched <= maxit; k++, launched++) {
+				hipEvent_t a = nullptr, b = nullptr;
+				if (c->timer.on) { a = c->timer.get(); hipEventRecord(a, c->st); }
+				launch_krylov_iter32(g, B, c->st, launched, tin, maxit);
+				if (c->timer.on) { b = c->timer.get(); hipEventRecord(b, c->st); iev.push_back({launched, a, b}); }
 			}
 			{
 				TRegion tr(c, T_KAUX);
@@ -232,6 +241,8 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, boo
 		}
 		const int it = c->hs->ks.iters;
 		const bool kok = c->hs->ks.done && c->hs->ks.status == 0;
+		for (const IterEv& e : iev) c->timer.pend.push_back({e.it < it ? T_KRYLOV : T_KIDLE, e.a, e.b});
+		c->timer.hold = false;
 		c->last_krylov = it; c->krylov_launched += launched; iters += it;
 		{ TRegion tr(c, T_KAUX); launch_add_f32(n, c->x32, c->dP, c->st); }
 		crit = true; sn = c->hs->dsum[2];   // the step criterion just read is that of dP as it now stands
@@ -498,7 +509,7 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	for (auto p : vecs) if (dalloc(c, p, n)) return fail("alloc");
 	if (dalloc(c, &c->J, 5 * n)) return fail("alloc");
 	if (cfg->krylov_fp32) {
-		float** fv[] = {&c->b32, &c->x32, &c->r32, &c->r032, &c->p32, &c->v32, &c->s32, &c->t32, &c->ph32, &c->sh32, &c->dg32};
+		float** fv[] = {&c->b32, &c->x32, &c->r32, &c->r032, &c->p32, &c->v32, &c->s32, &c->t32, &c->ph32, &c->sh32, &c->dg32, &c->pp32};
 		for (auto p : fv) { if (hipMalloc((void**)p, n * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(*p); }
 		if (hipMalloc((void**)&c->J32, 5 * n * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->J32);
 	}
@@ -767,6 +778,7 @@ int spb6_get_kernel_times(spb6_ctx* c, spb6_kernel_times_t* t) {
 	t->jacobian_n = T.cnt[T_JACOBIAN]; t->krylov_n = T.cnt[T_KRYLOV]; t->final_n = T.cnt[T_FINAL];
 	t->krylov_iters = c->stats.krylov_it;
 	t->krylov_aux_ms = T.ms[T_KAUX]; t->guess_ms = T.ms[T_GUESS]; t->krylov_launched = c->krylov_launched;
+	t->krylov_idle_ms = T.ms[T_KIDLE];
 	return SPB6_OK;
 }
 

@@ -38,6 +38,7 @@ template <typename T> struct KrylovBufsT {
 	const T* J;
 	double* part; int stride; KS* ks;
 	T *ph = nullptr, *sh = nullptr, *dg = nullptr;   // red-black sweep preconditioner: M^-1 p, M^-1 s, 1 / diagonal
+	T* p2 = nullptr;                                 // second buffer of p (the red-black form alternates p between p, p2)
 };
 using KrylovBufs = KrylovBufsT<double>;
 using KrylovBufs32 = KrylovBufsT<float>;

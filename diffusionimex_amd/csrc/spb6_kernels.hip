@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(256) k_thomas(int n, const double* __restrict_
 // Rows: block b, thread t owns rows b*256*IT + it*256 + t (it = 0..IT-1), the fixed order of the sums. IT is a
 // compile-time count, and each thread loads the inputs of a chunk of its rows into registers before it computes them
 // (the first chunk before the partial re-reduction that opens the pass), so several loads per thread are in flight.
-template <typename T> struct KV { T *x, *r, *r0, *p, *v, *s, *t; const T* J; T *ph, *sh, *dg; };
+template <typename T> struct KV { T *x, *r, *r0, *p, *v, *s, *t; const T* J; T *ph, *sh, *dg, *p2; };
 struct KP { double* part; int stride; int nb; };
 enum { SL_RR = 0, SL_R0R, SL_SP, SL_R0V, SL_SS, SL_SSUM, SL_TS, SL_TT, SL_E, SL_SB, SL_BB, NSLOTS };
 
@@ -310,6 +310,19 @@ __device__ __forceinline__ double grid_total(const KP& P, int slot, double* sh) 
 template <typename T> constexpr int chv() { return sizeof(T) == 4 ? 4 : 1; }
 template <typename T> constexpr int chs() { return sizeof(T) == 4 ? 2 : 1; }
 
+// row r of y = M^-1 z for the red-black sweep: z_r / d on a red row; on a black row (z_r - sum over red neighbours c
+// of A_rc z_c / d_c) / d, with the neighbours' z_c from zc(c) (the pass recomputes them from their own inputs)
+template <typename T, typename ZC>
+__device__ __forceinline__ T rb_row(const Geo& g, const KV<T>& V, int r, T zr, ZC zc) {
+	if (is_red(g, r)) return zr * V.dg[r];
+	const int n = g.n, ne = g.twoD ? 5 : 3;
+	const Nb nb = neighbours(g, r);
+	const int col[5] = {r, nb.W, nb.E, nb.S, nb.N};
+	T acc = zr;
+	#pragma unroll
+	for (int k = 1; k < 5; k++) if (k < ne && col[k] != r && is_red(g, col[k])) acc -= V.J[k * n + r] * (zc(col[k]) * V.dg[col[k]]);
+	return acc * V.dg[r];
+}
 // setup 1: partials of E = sum of all entries, sum(b), b.b (PC:726-738)
 template <typename T, int IT>
 __global__ void __launch_bounds__(256) kb_setup1(Geo g, const T* __restrict__ J, const T* __restrict__ b, KP P) {
@@ -386,9 +399,12 @@ __global__ void __launch_bounds__(256) kb_p(Geo g, KV<T> V, KS* ks, KP P, int it
 		#pragma unroll
 		for (int k = 0; k < CHV; k++) {
 			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
-			const T pn = rv[k] + beta * (pv[k] - omega * vv[k]); V.p[r] = pn;
-			if constexpr (RB) { if (is_red(g, r)) V.ph[r] = pn * V.dg[r]; }
-			else sp += (double)pn;
+			const T pn = rv[k] + beta * (pv[k] - omega * vv[k]);
+			if constexpr (RB) {   // new p into the other buffer: black rows read their red neighbours' old p
+				V.p2[r] = pn;
+				V.ph[r] = rb_row(g, V, r, pn, [&](int c) { return V.r[c] + beta * (V.p[c] - omega * V.v[c]); });
+			}
+			else { V.p[r] = pn; sp += (double)pn; }
 		}
 	}
 	if constexpr (!RB) put_partial(P, SL_SP, sp, sh);
@@ -447,7 +463,7 @@ __global__ void __launch_bounds__(256) kb_s(Geo g, KV<T> V, KS* ks, KP P, int it
 		for (int k = 0; k < CHV; k++) {
 			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
 			const T sv = rv[k] - alpha * vv[k]; V.s[r] = sv; sss += (double)(sv * sv); ssum += (double)sv;
-			if constexpr (RB) { if (is_red(g, r)) V.sh[r] = sv * V.dg[r]; }
+			if constexpr (RB) V.sh[r] = rb_row(g, V, r, sv, [&](int c) { return V.r[c] - alpha * V.v[c]; });
 		}
 	}
 	put_partial(P, SL_SS, sss, sh); put_partial(P, SL_SSUM, ssum, sh);
@@ -541,9 +557,9 @@ __global__ void __launch_bounds__(256) kb_xr(Geo g, KV<T> V, KS* ks, KP P, int i
 // Red-black variant (the FP32 form): M^-1 is one forward red-black Gauss-Seidel sweep from zero -- red rows
 // ((i + j) even) y = z / d, black rows y = (z - sum over red neighbours of A_rq y_q) / d, d being the row's diagonal
 // with its boundary self-columns (a neighbour index equal to the row) added. Couplings between rows of one colour (an
-// odd periodic wrap) are left out, so M^-1 is a fixed linear operator. The red rows of M^-1 p and M^-1 s are written by
-// kb_p / kb_s, the black rows by kbr_black, and the products read them (no constant coarse space: with the sweep it
-// slowed convergence on every workload measured, e.g. 3 -> 6 iterations on the 2048x1024 shock-bubble system).
+// odd periodic wrap) are left out, so M^-1 is a fixed linear operator. kb_p and kb_s write M^-1 p and M^-1 s (rb_row: a
+// black row recomputes its red neighbours' new p or s), and the products read them (no constant coarse space: with the
+// sweep it slowed convergence on every workload measured, e.g. 3 -> 6 iterations on the 2048x1024 shock-bubble system).
 // setup: dg = 1 / d; x = M^-1 b on the red rows (kbr_black completes it); p = v = 0; partials of b.b
 template <typename T, int IT>
 __global__ void __launch_bounds__(256) kbr_setup1(Geo g, const T* __restrict__ b, KV<T> V, KP P) {
@@ -709,7 +725,7 @@ template <typename T>
 static void krylov_setup_t(const Geo& g, const KrylovBufsT<T>& B, const T* b, hipStream_t st) {
 	const int items = items_of<T>(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
 	KP P{B.part, B.stride, nb};
-	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J, B.ph, B.sh, B.dg};
+	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J, B.ph, B.sh, B.dg, B.p2};
 	with_items(items, [&](auto I) {
 		constexpr int IT = decltype(I)::value;
 		if (B.dg) {
@@ -727,15 +743,14 @@ template <typename T>
 static void krylov_iter_t(const Geo& g, const KrylovBufsT<T>& B, hipStream_t st, int it, double tol, int maxit) {
 	const int items = items_of<T>(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
 	KP P{B.part, B.stride, nb};
-	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J, B.ph, B.sh, B.dg};
+	KV<T> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J, B.ph, B.sh, B.dg, B.p2};
 	with_items(items, [&](auto I) {
 		constexpr int IT = decltype(I)::value;
-		if (B.dg) {   // red-black sweep preconditioner (the FP32 form)
+		if (B.dg) {   // red-black sweep preconditioner (the FP32 form); p alternates between two buffers
+			if (it & 1) { V.p = B.p2; V.p2 = B.p; }
 			hipLaunchKernelGGL((kb_p<T, IT, true>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol, maxit);
-			hipLaunchKernelGGL((kbr_black<T, IT>), dim3(nb), dim3(256), 0, st, g, (const T*)B.p, B.ph, B.J, (const T*)B.dg, B.ks, 0);
 			hipLaunchKernelGGL((kb_v<T, IT, true>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
 			hipLaunchKernelGGL((kb_s<T, IT, true>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
-			hipLaunchKernelGGL((kbr_black<T, IT>), dim3(nb), dim3(256), 0, st, g, (const T*)B.s, B.sh, B.J, (const T*)B.dg, B.ks, 0);
 			hipLaunchKernelGGL((kb_t<T, IT, true>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it, tol);
 			hipLaunchKernelGGL((kb_xr<T, IT, true>), dim3(nb), dim3(256), 0, st, g, V, B.ks, P, it);
 			return;

@@ -48,7 +48,8 @@ class KernelTimes(C.Structure):
                                            "krylov_ms", "final_ms", "other_ms")] + \
                [(k, C.c_long) for k in ("explicit_n", "prep_n", "shock_n", "residual_n", "jacobian_n", "krylov_n",
                                          "final_n", "krylov_iters")] + \
-               [("krylov_aux_ms", C.c_double), ("guess_ms", C.c_double), ("krylov_launched", C.c_long)]
+               [("krylov_aux_ms", C.c_double), ("guess_ms", C.c_double), ("krylov_launched", C.c_long),
+                ("krylov_idle_ms", C.c_double)]
 
 
 # every symbol include/spb6_imex.h declares

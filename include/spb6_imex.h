@@ -142,6 +142,7 @@ typedef struct spb6_kernel_times {
     double krylov_aux_ms;        /* BiCGStab setup, FP32 conversions and FP64 defects (krylov_ms: iteration passes only) */
     double guess_ms;             /* learned-guess features and inference */
     long krylov_launched;        /* BiCGStab iterations launched (a pass launched after convergence returns at once) */
+    double krylov_idle_ms;       /* FP32 form: time of the iterations launched past a solve's end (krylov_ms: taken ones) */
 } spb6_kernel_times_t;
 int spb6_enable_timing(spb6_ctx* ctx, int on);
 int spb6_get_kernel_times(spb6_ctx* ctx, spb6_kernel_times_t* t);
