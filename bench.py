@@ -25,11 +25,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12   # MI355X HBM3E, B/s
 # algorithmic bytes per interior cell per launch (FP64, 2D): compulsory reads + writes of each kernel's data flow
 BYTES_2D = {"explicit": 232, "prep": 184, "residual": 256, "jacobian": 304, "krylov_iter": 296, "final": 376}
-# one BiCGStab iteration of the FP32 form (red-black sweep preconditioner, seven FP32 passes), bytes per row: kb_p 24
-# (r, p, v, 1/d in; p, red M^-1 p out), the black rows of M^-1 p 32 (own p, four off-diagonals, 1/d, red neighbours in;
-# out), kb_v 32 (five diagonals, M^-1 p, r0 in; v out), kb_s 20 (r, v, 1/d in; s, red M^-1 s out), black rows of M^-1 s
-# 32, kb_t 32 (five diagonals, M^-1 s, s in; t out), kb_xr 32 (M^-1 p, M^-1 s, s, t, x, r0 in; x, r out)
-BYTES_KRYLOV_FP32 = 204
+# one BiCGStab iteration of the FP32 form (red-black sweep preconditioner, five FP32 passes), bytes per row: kb_p 40
+# (r, p, v, 1/d and the four off-diagonals in; new p and M^-1 p out), kb_v 32 (five diagonals, M^-1 p, r0 in; v out),
+# kb_s 36 (r, v, 1/d and the four off-diagonals in; s, M^-1 s out), kb_t 32 (five diagonals, M^-1 s, s in; t out),
+# kb_xr 32 (M^-1 p, M^-1 s, s, t, x, r0 in; x, r out); a black row's red neighbours come through the caches
+BYTES_KRYLOV_FP32 = 172
 
 
 def parse():
@@ -98,7 +98,7 @@ def run_form(a, form, keep_state=False):
     return res, st, cfg
 
 
-# HBM bytes per FP32 BiCGStab iteration measured by rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over the seven
+# HBM bytes per FP32 BiCGStab iteration measured by rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over the five
 # passes (active dispatches), calibrated on the FP32 conversion of the 5n-entry Jacobian (known bytes: FETCH_SIZE reads
 # x 2.00, WRITE_SIZE x 1.00); commands and per-pass counts in profiles/krylov_fp32_iteration_2048x1024_traffic.md
 TRAFFIC = {(True, 2048, 1024): 456535936}
@@ -130,7 +130,10 @@ def roofline_of(res, n, fp32, nx=2048, ny=1024):
     traffic = TRAFFIC.get((fp32, nx, ny)) if dom == "krylov_iter" else None
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": traffic, "bytes_per_launch": byts, "avg_launch_ms": avg_s * 1e3,
-            "iterations": cnt}
+            "iterations": cnt,
+            # every category on its algorithmic bytes over its timed region (the region's small boundary-fill and
+            # reduction kernels included)
+            "by_kernel": {k: round(v[2] * max(v[1], 1) / (v[0] / 1e3) / HBM_PEAK, 4) for k, v in cats.items() if v[0] > 0}}
     step_bytes = (BYTES_2D["explicit"] + BYTES_2D["prep"] + BYTES_2D["final"]) * d["stages"] + BYTES_2D["jacobian"] * d["newton_it"] \
         + BYTES_2D["residual"] * (d["stages"] + d["newton_it"]) + kb * d["krylov_it"]
     return roof, kernel_ms, step_bytes * n / res["wall"] / HBM_PEAK

@@ -63,7 +63,8 @@ struct spb6_ctx {
 	Geo g; Phys ph;
 	// ghosted grids (8 fields each) and arrays
 	double *U = nullptr, *Ust = nullptr, *Qe = nullptr, *Qs = nullptr, *H[2] = {nullptr, nullptr};
-	double *wxL, *wxR, *wyL, *wyR, *phx, *phy, *a2s, *ubx, *uby, *vsc, *esc;
+	double *wxL, *wxR, *wyL, *wyR, *phx, *phy, *a2s, *ubx, *uby, *vsc, *esc, *vu, *vv;   // face grids: the donors (m1, m2, e1, e2) only
+	const double* qe_last = nullptr;   // Qe of the last explicit stage
 	double *ust, *vst, *cz, *Kw, *Pold, *ubsx, *kpx, *wLx, *pvx, *ubsy, *kpy, *wLy, *pvy, *Qph;
 	// pressure system (interior)
 	double *P, *Pst, *Pn, *G, *Gn, *dP, *rhs, *sc, *J, *cpw, *dpw;
@@ -120,7 +121,7 @@ Dev make_dev(const spb6_ctx* c) {
 	Dev D;
 	D.g = c->g; D.ph = c->ph; D.P_REF = c->P_REF; D.rx = 0; D.ry = 0; D.tau = 0;
 	D.Ust_c = c->Ust; D.Ust = c->Ust; D.a2s = c->a2s; D.ubx = c->ubx; D.uby = c->uby; D.phx = c->phx; D.phy = c->phy;
-	D.vsc = c->vsc; D.esc = c->esc; D.wxL = c->wxL; D.wxR = c->wxR; D.wyL = c->wyL; D.wyR = c->wyR;
+	D.vsc = c->vsc; D.esc = c->esc; D.vu = c->vu; D.vv = c->vv; D.wxL = c->wxL; D.wxR = c->wxR; D.wyL = c->wyL; D.wyR = c->wyR;
 	D.ust = c->ust; D.vst = c->vst; D.cz = c->cz; D.Kw = c->Kw; D.Pold = c->Pold;
 	D.ubsx = c->ubsx; D.kpx = c->kpx; D.wLx = c->wLx; D.pvx = c->pvx; D.ubsy = c->ubsy; D.kpy = c->kpy; D.wLy = c->wLy; D.pvy = c->pvy;
 	D.Qph = c->Qph; D.flags = c->flags;
@@ -361,6 +362,7 @@ Dev stage_dev(spb6_ctx* c, double adt) {
 int explicit_part(spb6_ctx* c, const double* Qe, const double* Qs, double adt) {
 	Dev D = stage_dev(c, adt);
 	TRegion tr(c, T_EXPLICIT);
+	c->qe_last = Qe;
 	launch_explicit(D, c->cfg.limiter, c->cfg.order, Qe, Qs, adt, c->st);
 	return SPB6_OK;
 }
@@ -501,7 +503,7 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	for (auto p : grids) if (dalloc(c, p, NC * NV)) return fail("alloc");
 	if (g.twoD) { if (dalloc(c, &c->wyL, NC * NV) || dalloc(c, &c->wyR, NC * NV)) return fail("alloc"); }
 	else { c->wyL = c->wxL; c->wyR = c->wxR; }
-	double** arrs[] = {&c->phx, &c->phy, &c->a2s, &c->ubx, &c->uby, &c->vsc, &c->esc, &c->ust, &c->vst, &c->cz, &c->Kw, &c->Pold,
+	double** arrs[] = {&c->phx, &c->phy, &c->a2s, &c->ubx, &c->uby, &c->vsc, &c->esc, &c->vu, &c->vv, &c->ust, &c->vst, &c->cz, &c->Kw, &c->Pold,
 	                   &c->ubsx, &c->kpx, &c->wLx, &c->pvx, &c->ubsy, &c->kpy, &c->wLy, &c->pvy};
 	for (auto p : arrs) if (dalloc(c, p, NC)) return fail("alloc");
 	if (dalloc(c, &c->Qph, 2 * NC)) return fail("alloc");
@@ -748,8 +750,19 @@ long spb6_get_field(spb6_ctx* c, const char* name, double* out, long capacity) {
 	if (grid) {
 		const long tot = (long)g.NC * NV; if (capacity < tot) return -1;
 		std::vector<double> h(tot);
-		if (hipMemcpyAsync(h.data(), grid, tot * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess) return -1;
-		if (hipStreamSynchronize(c->st) != hipSuccess) return -1;
+		double* tmp = nullptr;
+		if (grid == c->wxL || grid == c->wxR || grid == c->wyL || grid == c->wyR) {
+			// the explicit stage keeps only the face donors: the full face states of its Qe are rebuilt for inspection
+			if (!c->qe_last) return -1;
+			if (hipMalloc((void**)&tmp, 4 * (size_t)tot * sizeof(double)) != hipSuccess) return -1;
+			double *xL = tmp, *xR = tmp + tot, *yL = g.twoD ? tmp + 2 * tot : xL, *yR = g.twoD ? tmp + 3 * tot : xR;
+			launch_recon_full(make_dev(c), c->cfg.limiter, c->cfg.order, c->qe_last, xL, xR, yL, yR, c->st);
+			grid = grid == c->wxL ? xL : grid == c->wxR ? xR : grid == c->wyL ? yL : yR;
+		}
+		const bool ok = hipMemcpyAsync(h.data(), grid, tot * sizeof(double), hipMemcpyDeviceToHost, c->st) == hipSuccess &&
+		                hipStreamSynchronize(c->st) == hipSuccess;
+		if (tmp) (void)hipFree(tmp);
+		if (!ok) return -1;
 		for (long cc = 0; cc < g.NC; cc++) for (int k = 0; k < NV; k++) out[cc * NV + k] = h[(size_t)k * g.NC + cc];
 		return tot;
 	}

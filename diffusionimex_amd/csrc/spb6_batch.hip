@@ -110,7 +110,7 @@ __device__ __forceinline__ View make_view(const BArgs& A, const BProb& B, const 
 	Dev& D = v.D;
 	D.g = g; D.ph = B.ph; D.P_REF = S.P_REF; D.rx = 0; D.ry = 0; D.tau = 0;
 	D.Ust_c = v.Ust; D.Ust = v.Ust; D.a2s = f(A.a2s); D.ubx = f(A.ubx); D.uby = D.ubx; D.phx = f(A.phx); D.phy = D.phx;
-	D.vsc = f(A.vsc); D.esc = f(A.esc); D.wxL = f(A.wxL); D.wxR = f(A.wxR); D.wyL = D.wxL; D.wyR = D.wxR;
+	D.vsc = f(A.vsc); D.esc = f(A.esc); D.vu = nullptr; D.vv = nullptr; D.wxL = f(A.wxL); D.wxR = f(A.wxR); D.wyL = D.wxL; D.wyR = D.wxR;
 	D.ust = f(A.ust); D.vst = f(A.vst); D.cz = f(A.cz); D.Kw = f(A.Kw); D.Pold = f(A.Pold);
 	D.ubsx = f(A.ubsx); D.kpx = f(A.kpx); D.wLx = f(A.wLx); D.pvx = f(A.pvx);
 	D.ubsy = D.ubsx; D.kpy = D.kpx; D.wLy = D.wLx; D.pvy = D.pvx; D.Qph = f(A.Qph); D.flags = nullptr;

@@ -42,9 +42,10 @@ __device__ __forceinline__ void bcarr_x_at(const Geo& g, double* a, int j, int q
 }
 
 // ------------------------------------------------------------------------------------------------ explicit stage
-// reconstruction scales (PC:356-366) of cell c; false where rho <= 0 (the check of Update, PC:255-258)
+// reconstruction scales (PC:356-366) of cell c; false where rho <= 0 (the check of Update, PC:255-258). With vu, vv
+// given, also the cell velocities mu / rho, mv / rho (the quotients the reconstruction's velocity minmod takes, PC:428-433)
 __device__ __forceinline__ bool scales_at(const Geo& g, const Phys& P, const double* __restrict__ Q, double* __restrict__ vsc,
-                                          double* __restrict__ esc, int c) {
+                                          double* __restrict__ esc, int c, double* __restrict__ vu = nullptr, double* __restrict__ vv = nullptr) {
 	const int NC = g.NC;
 	const double a1 = Q[IA1 * NC + c], m1 = Q[IM1 * NC + c], m2 = Q[IM2 * NC + c], mu = Q[IMU * NC + c], mv = Q[IMV * NC + c];
 	const double e1 = Q[IE1 * NC + c], e2 = Q[IE2 * NC + c];
@@ -52,18 +53,31 @@ __device__ __forceinline__ bool scales_at(const Geo& g, const Phys& P, const dou
 	double cf;
 	vsc[c] = (rho > 0 ? mm / rho : 0.0) + (prim_cf(P, a1, m1, m2, e1, e2, cf) ? cf : 0.0);
 	esc[c] = fabs(e1) + fabs(e2) + (rho > 0 ? 0.5 * mm * mm / rho : 0.0);
+	if (vu) { vu[c] = mu / rho; vv[c] = mv / rho; }
 	return rho > 0;
 }
-// TVD reconstruction of one cell along one direction (PC:392-445): face values L, R (8 vars) and the limiter factor
-__device__ __forceinline__ double recon_cell(const Geo& g, const Phys& P, int limiter, const double* __restrict__ Q,
-                                             const double* __restrict__ vsc, const double* __restrict__ esc,
-                                             int ca, int cc, int cb, double* L, double* R) {
-	const int NC = g.NC;
+// sources of the reconstruction's inputs at a cell: vol(q, cell) the volume field VOL[q] (alpha1, m1, m2, e1, e2),
+// vsc / esc the scales, vel(d, cell, rho) the velocity component d (the quotient (rho u_d) / rho)
+struct ReconGlobal {   // the conserved grid, velocities divided out here
+	const double* __restrict__ Q; const double* __restrict__ vs; const double* __restrict__ es; int NC;
+	__device__ double vol(int q, int cell) const { return Q[(q == 0 ? IA1 : q == 1 ? IM1 : q == 2 ? IM2 : q == 3 ? IE1 : IE2) * NC + cell]; }
+	__device__ double vsc(int cell) const { return vs[cell]; }
+	__device__ double esc(int cell) const { return es[cell]; }
+	__device__ double vel(int d, int cell, double rho) const { return Q[(d == 0 ? IMU : IMV) * NC + cell] / rho; }
+};
+struct ReconGlobalV : ReconGlobal {   // the same with the velocities scales_at stored
+	const double* __restrict__ vu; const double* __restrict__ vv;
+	__device__ double vel(int d, int cell, double) const { return d == 0 ? vu[cell] : vv[cell]; }
+};
+// TVD reconstruction of one cell along one direction (PC:392-445) from the inputs of cells ca, cc, cb in S: face values
+// L, R (8 vars) and the limiter factor
+template <class Src>
+__device__ __forceinline__ double recon_cell_s(const Phys& P, int limiter, const Src& S, int ca, int cc, int cb, double* L, double* R) {
 	const int VOL[5] = {IA1, IM1, IM2, IE1, IE2};
 	double xa[5], xc[5], xb[5];
 	#pragma unroll
-	for (int q = 0; q < 5; q++) { xa[q] = Q[VOL[q] * NC + ca]; xc[q] = Q[VOL[q] * NC + cc]; xb[q] = Q[VOL[q] * NC + cb]; }
-	const double esum = esc[ca] + esc[cc] + esc[cb];
+	for (int q = 0; q < 5; q++) { xa[q] = S.vol(q, ca); xc[q] = S.vol(q, cc); xb[q] = S.vol(q, cb); }
+	const double esum = S.esc(ca) + S.esc(cc) + S.esc(cb);
 	double phi = 1.0;
 	#pragma unroll
 	for (int q = 0; q < 5; q++) {
@@ -98,12 +112,11 @@ __device__ __forceinline__ double recon_cell(const Geo& g, const Phys& P, int li
 		R[VOL[q]] = xc[q] + 0.5 * sig;
 	}
 	const double ra = xa[1] + xa[2], rc = xc[1] + xc[2], rb = xb[1] + xb[2];
-	const double tiny = 1e-12 * (vsc[ca] + vsc[cc] + vsc[cb]) + 1e-300;
+	const double tiny = 1e-12 * (S.vsc(ca) + S.vsc(cc) + S.vsc(cb)) + 1e-300;
 	double vL[2], vR[2], sm = 1.0;
 	#pragma unroll
 	for (int d = 0; d < 2; d++) {
-		const int k = d == 0 ? IMU : IMV;
-		const double va = Q[k * NC + ca] / ra, vc = Q[k * NC + cc] / rc, vb = Q[k * NC + cb] / rb, a = vc - va, b = vb - vc;
+		const double va = S.vel(d, ca, ra), vc = S.vel(d, cc, rc), vb = S.vel(d, cb, rb), a = vc - va, b = vb - vc;
 		double sl = 0.0;
 		if (!limiter) sl = 0.5 * (a + b);
 		else if (!(fabs(a) <= tiny && fabs(b) <= tiny)) {
@@ -116,6 +129,11 @@ __device__ __forceinline__ double recon_cell(const Geo& g, const Phys& P, int li
 	L[IMU] = rL * vL[0]; L[IMV] = rL * vL[1]; L[IEN] = L[IE1] + L[IE2] + 0.5 * rL * (vL[0] * vL[0] + vL[1] * vL[1]);
 	R[IMU] = rR * vR[0]; R[IMV] = rR * vR[1]; R[IEN] = R[IE1] + R[IE2] + 0.5 * rR * (vR[0] * vR[0] + vR[1] * vR[1]);
 	return smin(phi, sm);
+}
+__device__ __forceinline__ double recon_cell(const Geo& g, const Phys& P, int limiter, const double* __restrict__ Q,
+                                             const double* __restrict__ vsc, const double* __restrict__ esc,
+                                             int ca, int cc, int cb, double* L, double* R) {
+	return recon_cell_s(P, limiter, ReconGlobal{Q, vsc, esc, g.NC}, ca, cc, cb, L, R);
 }
 // x-direction face states of ghosted cell c (all rows; cells with both x neighbours, else first order)
 __device__ __forceinline__ void recon_x_at(const Geo& g, const Phys& P, int limiter, int order, const double* __restrict__ Q,
@@ -131,15 +149,45 @@ __device__ __forceinline__ void recon_x_at(const Geo& g, const Phys& P, int limi
 		for (int k = 0; k < NV; k++) { const double v = Q[k * NC + c]; wxL[k * NC + c] = v; wxR[k * NC + c] = v; }
 	}
 }
-// Rusanov flux with the local material speed between face states L (cell cL's side) and R (PC:224-246)
-__device__ __forceinline__ void rusanov_flux(const double* __restrict__ wR_, int cL, const double* __restrict__ wL_, int cR, int NC, int dirk, double* F) {
-	double QL[NV], QR[NV];
+// face states L, R and limiter factor of ghosted cell c along the direction of stride d (1: x, NXT: y): the TVD
+// reconstruction when second is set (order 2 and both neighbours on the line), else first order (L = R = Q, factor 0)
+// (velocities from the arrays vu, vv that scales_at wrote)
+__device__ __forceinline__ double recon_dir(const Geo& g, const Phys& P, int limiter, bool second, const double* __restrict__ Q,
+                                            const double* __restrict__ vsc, const double* __restrict__ esc, const double* __restrict__ vu,
+                                            const double* __restrict__ vv, int c, int d, double* L, double* R) {
+	if (second) {
+		ReconGlobalV S; S.Q = Q; S.vs = vsc; S.es = esc; S.NC = g.NC; S.vu = vu; S.vv = vv;
+		return recon_cell_s(P, limiter, S, c - d, c, c + d, L, R);
+	}
 	#pragma unroll
-	for (int k = 0; k < NV; k++) { QL[k] = wR_[k * NC + cL]; QR[k] = wL_[k * NC + cR]; }
+	for (int k = 0; k < NV; k++) { const double v = Q[k * g.NC + c]; L[k] = v; R[k] = v; }
+	return 0.0;
+}
+// a store that does not keep its line in the caches (outputs a kernel does not read again), so that the L2 keeps the
+// stencil's input lines for the neighbouring tiles
+__device__ __forceinline__ void st_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
+// a store that is nontemporal when NT (the grid kernels; the batch kernel reads its values back inside the workgroup)
+template <bool NT> __device__ __forceinline__ void st_out(double* p, double v) { if constexpr (NT) st_nt(p, v); else *p = v; }
+// the face donors the implicit stage reads (m1, m2, e1, e2 of both face states) of ghosted cell c
+__device__ __forceinline__ void put_donors(double* __restrict__ wL, double* __restrict__ wR, int NC, int c, const double* L, const double* R) {
+	st_nt(wL + IM1 * NC + c, L[IM1]); st_nt(wL + IM2 * NC + c, L[IM2]); st_nt(wL + IE1 * NC + c, L[IE1]); st_nt(wL + IE2 * NC + c, L[IE2]);
+	st_nt(wR + IM1 * NC + c, R[IM1]); st_nt(wR + IM2 * NC + c, R[IM2]); st_nt(wR + IE1 * NC + c, R[IE1]); st_nt(wR + IE2 * NC + c, R[IE2]);
+}
+// Rusanov flux with the local material speed between face states QL and QR (PC:224-246), and the face velocity
+// ub = (u_L + u_R) / 2 of the path-conservative alpha1 term (PC:300-304)
+__device__ __forceinline__ void rusanov_states(const double* QL, const double* QR, int dirk, double* F, double& ub) {
 	const double uL = QL[dirk] / (QL[IM1] + QL[IM2]), uR = QR[dirk] / (QR[IM1] + QR[IM2]);
 	const double a = smax(fabs(QL[dirk] / (QL[IM1] + QL[IM2])), fabs(QR[dirk] / (QR[IM1] + QR[IM2])));
 	#pragma unroll
 	for (int k = 0; k < NV; k++) { const double fL = QL[k] * uL, fR = QR[k] * uR; F[k] = 0.5 * (fL + fR) - 0.5 * a * (QR[k] - QL[k]); }
+	ub = 0.5 * (uL + uR);
+}
+// the same between the stored face states: L state at cL (its right face), R state at cR
+__device__ __forceinline__ void rusanov_flux(const double* __restrict__ wR_, int cL, const double* __restrict__ wL_, int cR, int NC, int dirk, double* F) {
+	double QL[NV], QR[NV], ub;
+	#pragma unroll
+	for (int k = 0; k < NV; k++) { QL[k] = wR_[k * NC + cL]; QR[k] = wL_[k * NC + cR]; }
+	rusanov_states(QL, QR, dirk, F, ub);
 }
 __device__ __forceinline__ double face_vel(const double* __restrict__ wR_, int cL, const double* __restrict__ wL_, int cR, int NC, int dirk) {
 	const double uL = wR_[dirk * NC + cL] / (wR_[IM1 * NC + cL] + wR_[IM2 * NC + cL]);
@@ -486,6 +534,7 @@ __device__ __forceinline__ Nb neighbours(const Geo& g, int q) {   // M_P of the 
 }
 // residual of interior row q at P + lam dP (dP may be null): writes G (NaN where the closure fails) and the trial pressure;
 // nv = |G| / (|P_REF + P| + K + 1) (PC:770); false where the closure fails
+template <bool NT = false>
 __device__ __forceinline__ bool residual_at(const Dev& D, const double* __restrict__ P, const double* __restrict__ dP, double lam,
                                             double* __restrict__ Pout, double* __restrict__ G, int q, double& nv) {
 	const Nb b = neighbours(D.g, q);
@@ -499,8 +548,8 @@ __device__ __forceinline__ bool residual_at(const Dev& D, const double* __restri
 	nv = 0.0;
 	if (!cell_res(D.ph, D.P_REF, D.rx, D.ry, D.g.twoD, xs, Q0, Q1, F, Pc, PW, PE, PS, PN, g, nullptr)) { ok = false; g = dnan(); }
 	else nv = fabs(g) / (fabs(D.P_REF + Pc) + Kw + 1.0);
-	G[q] = g;
-	if (Pout) Pout[q] = Pc;
+	st_out<NT>(G + q, g);
+	if (Pout) st_out<NT>(Pout + q, Pc);
 	return ok;
 }
 // P scale s_q = (|P| + K + 1) / (1 + K D) of the finite-difference steps (PC:783-789)
@@ -582,6 +631,7 @@ SPB_HD bool jacobian_analytic_row(const Phys& Ph, double P_REF, double rx, doubl
 	for (int j = 0; j < 5; j++) { J[j] = -phi.d[j] / dphidp - (j == 0 ? 1.0 : 0.0); ok = ok && isfin(J[j]); }
 	return ok;
 }
+template <bool NT = false>
 __device__ __forceinline__ bool jacobian_analytic_at(const Dev& D, const double* __restrict__ P, const double* __restrict__ G,
                                                      double* __restrict__ J, double* __restrict__ rhs, int q) {
 	const Geo& g = D.g; const int n = g.n;
@@ -591,11 +641,11 @@ __device__ __forceinline__ bool jacobian_analytic_at(const Dev& D, const double*
 	const CellFaces F(D, q);
 	double xs[NV], Q0, Q1, Kw; F.cell(xs, Q0, Q1, Kw);
 	const double Gq = G[q];
-	rhs[q] = -Gq;
+	st_out<NT>(rhs + q, -Gq);
 	double Jr[5];
 	const bool ok = jacobian_analytic_row(D.ph, D.P_REF, D.rx, D.ry, twoD, xs, Q0, Q1, Pv, Gq, F, Jr);
-	J[q] = Jr[0]; J[n + q] = Jr[1]; J[2 * n + q] = Jr[2];
-	if (twoD) { J[3 * n + q] = Jr[3]; J[4 * n + q] = Jr[4]; }
+	st_out<NT>(J + q, Jr[0]); st_out<NT>(J + n + q, Jr[1]); st_out<NT>(J + 2 * n + q, Jr[2]);
+	if (twoD) { st_out<NT>(J + 3 * n + q, Jr[3]); st_out<NT>(J + 4 * n + q, Jr[4]); }
 	return ok;
 }
 // round-off floor acceptance test of row q (PC:834-842): false where the residual exceeds tol N_i + sigma_i
@@ -610,6 +660,7 @@ __device__ __forceinline__ bool floor_ok_at(const Dev& D, const double* __restri
 }
 // final stage state of row q at the converged pressure (PC:975-987) and the stage derivative Hs = (U_s - Qs) / adt
 // (PC:1276-1284); false where the closure fails
+template <bool NT = false>
 __device__ __forceinline__ bool final_at(const Dev& D, const double* __restrict__ P, const double* __restrict__ Qs, double adt, double* __restrict__ H, int q) {
 	const Geo& g = D.g; const int NC = g.NC, c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
 	const Nb b = neighbours(g, q);
@@ -620,7 +671,7 @@ __device__ __forceinline__ bool final_at(const Dev& D, const double* __restrict_
 	if (!cell_res(D.ph, D.P_REF, D.rx, D.ry, twoD, xs, Q0, Q1, F, P[q], P[b.W], P[b.E], twoD ? P[b.S] : 0.0, twoD ? P[b.N] : 0.0, gg, &o)) return false;
 	const double u[NV] = {o.a0, o.m0, o.m1, o.mx, o.my, o.e0, o.e1, o.E};
 	#pragma unroll
-	for (int k = 0; k < NV; k++) { D.Ust[k * NC + c] = u[k]; H[k * NC + c] = (u[k] - Qs[k * NC + c]) / adt; }
+	for (int k = 0; k < NV; k++) { st_out<NT>(D.Ust + k * NC + c, u[k]); st_out<NT>(H + k * NC + c, (u[k] - Qs[k * NC + c]) / adt); }
 	return true;
 }
 // caldt terms of row q (PC:477-488): |u| + c and |v| + c (c only when acoustic) for admissible primitives, else 0

@@ -26,7 +26,7 @@ struct Dev {
 	Geo g; Phys ph;
 	double P_REF, rx, ry, tau;
 	const double *Ust_c;
-	double *Ust, *a2s, *ubx, *uby, *phx, *phy, *vsc, *esc;
+	double *Ust, *a2s, *ubx, *uby, *phx, *phy, *vsc, *esc, *vu, *vv;
 	double *wxL, *wxR, *wyL, *wyR;
 	double *ust, *vst, *cz, *Kw, *Pold;
 	double *ubsx, *kpx, *wLx, *pvx, *ubsy, *kpy, *wLy, *pvy, *Qph;
@@ -49,6 +49,9 @@ void launch_bc_grid(const Geo& g, double* u, hipStream_t st);
 void launch_bc_array(const Geo& g, double* a, hipStream_t st);
 void launch_combine(const Geo& g, const double* Un, const double* H0, const double* H1, int nr, double ae0, double ae1, double ai0, double ai1, double* Qe, double* Qs, hipStream_t st);
 void launch_explicit(const Dev& D, int limiter, int order, const double* Qe, const double* Qs, double adt, hipStream_t st);
+// full 8-field face states of Qe (TVDReconstruction) into the given grids, for inspection: the explicit stage itself
+// keeps only the face donors
+void launch_recon_full(const Dev& D, int limiter, int order, const double* Qe, double* wxL, double* wxR, double* wyL, double* wyR, hipStream_t st);
 void launch_prep(const Dev& D, int limiter, int order, double tau, const double* Pst, hipStream_t st);
 // shock budget in two passes; list holds n ints, count one int (both device scratch); strong_blocks: blocks of 64 for
 // the strong-shock pass (a few per compute unit)

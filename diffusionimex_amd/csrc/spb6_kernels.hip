@@ -13,6 +13,15 @@
 
 namespace spb {
 
+// ------------------------------------------------------------------------------------------------ workgroup order
+// Workgroups go to the 8 XCDs round robin (workgroup b to XCD b mod 8). xcd_band(b, nb) is a bijection of [0, nb) that
+// gives XCD x the contiguous band x q + min(x, m) ... (q = nb / 8, m = nb mod 8) in dispatch order, so that blocks
+// working on neighbouring parts of the grid share their XCD's L2 for the rows they read beyond their own.
+__device__ __forceinline__ int xcd_band(int b, int nb) {
+	const int x = b & 7, q = nb >> 3, m = nb & 7;
+	return x * q + (x < m ? x : m) + (b >> 3);
+}
+
 // ------------------------------------------------------------------------------------------------ reductions
 __device__ __forceinline__ double wave_bfly_sum(double v) {
 	#pragma unroll
@@ -108,10 +117,12 @@ __global__ void __launch_bounds__(256) k_combine(Geo g, const double* __restrict
 
 // ------------------------------------------------------------------------------------------------ explicit stage
 // reconstruction scales (PC:356-366) and the rho > 0 check of Update (PC:255-258) over all cells of Qe
-__global__ void __launch_bounds__(256) k_scales(Geo g, Phys P, const double* __restrict__ Q, double* __restrict__ vsc, double* __restrict__ esc, int* __restrict__ flags) {
+// (and the cell velocities into vu, vv)
+__global__ void __launch_bounds__(256) k_scales(Geo g, Phys P, const double* __restrict__ Q, double* __restrict__ vsc, double* __restrict__ esc,
+                                                double* __restrict__ vu, double* __restrict__ vv, int* __restrict__ flags) {
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c >= g.NC) return;
-	if (!scales_at(g, P, Q, vsc, esc, c)) flags[FLAG_RHO] = 1;
+	if (!scales_at(g, P, Q, vsc, esc, c, vu, vv)) flags[FLAG_RHO] = 1;
 }
 __global__ void __launch_bounds__(256) k_recon(Geo g, Phys P, int limiter, int order, const double* __restrict__ Q, const double* __restrict__ vsc,
                         const double* __restrict__ esc, double* __restrict__ wxL, double* __restrict__ wxR,
@@ -130,14 +141,182 @@ __global__ void __launch_bounds__(256) k_recon(Geo g, Phys P, int limiter, int o
 		for (int k = 0; k < NV; k++) { const double v = Q[k * NC + c]; wyL[k * NC + c] = v; wyR[k * NC + c] = v; }
 	}
 }
-// X* = Qs - adt div F(Qe) with the path-conservative alpha1 term (PC:251-326); interior cells
-__global__ void __launch_bounds__(256) k_update(Geo g, double rxd, double ryd, const double* __restrict__ Qe, const double* __restrict__ Qs,
-                         const double* __restrict__ wxL, const double* __restrict__ wxR, const double* __restrict__ wyL,
-                         const double* __restrict__ wyR, double* __restrict__ Ust, double* __restrict__ a2s,
-                         double* __restrict__ ubx, double* __restrict__ uby) {
-	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	if (q >= g.n) return;
-	update_at(g, rxd, ryd, Qe, Qs, wxL, wxR, wyL, wyR, Ust, a2s, ubx, uby, q);
+// TVDReconstruction + Update fused (PC:349-446, PC:251-326) on 1D grids: TX cells of one grid row per block, one cell
+// per thread. Each thread reconstructs its cell; the left face states go through LDS to the west neighbour, so each
+// face's Rusanov flux is computed once (by the cell west of it; the face west of the block by thread 64, which
+// reconstructs the cell west of the block, thread 65 the cell east of it for its left state) and handed back through
+// LDS. Face states never reach HBM: the kernel writes the limiter factors, the face donors the implicit stage reads
+// (put_donors), X*, alpha2* and the face velocities, each by the operations of recon_x_at / update_at.
+template <int TX>
+__global__ void __launch_bounds__(TX) k_explicit_1d(Geo g, Phys P, int limiter, int order, double rxd,
+		const double* __restrict__ Qe, const double* __restrict__ Qs, const double* __restrict__ vsc, const double* __restrict__ esc,
+		const double* __restrict__ vu, const double* __restrict__ vv, double* __restrict__ wxL, double* __restrict__ wxR,
+		double* __restrict__ phx, double* __restrict__ Ust, double* __restrict__ a2s, double* __restrict__ ubx, int tiles_y) {
+	constexpr int HX = 64;
+	static_assert(HX + 2 <= TX, "halo threads");
+	__shared__ double sh[std::max(NV * (TX + 2), (NV + 1) * (TX + 1))];
+	const int NXT = g.NXT, NYT = g.NYT, NC = g.NC, tx = threadIdx.x;
+	const int tile = xcd_band(blockIdx.x, gridDim.x), i0 = (tile / tiles_y) * TX, j = tile % tiles_y;
+	const int i = i0 + tx, c = i + j * NXT;
+	const bool valid = i < NXT && j < NYT;
+	const bool inner = i >= GH && i < g.nx + GH && j >= GH && j < g.ny + GH;
+	double L[NV], R[NV], F[NV], hR[NV], hF[NV], ub = 0.0, hub = 0.0;
+	// states [k][1 + column] (east halo at column TX + 1), then fluxes [k][face] (face f west of column f)
+	if (valid) {
+		phx[c] = recon_dir(g, P, limiter, order >= 2 && i >= 1 && i + 1 < NXT, Qe, vsc, esc, vu, vv, c, 1, L, R);
+		put_donors(wxL, wxR, NC, c, L, R);
+		#pragma unroll
+		for (int k = 0; k < NV; k++) sh[k * (TX + 2) + tx + 1] = L[k];
+	}
+	const int hx = tx - HX;
+	bool hw = false;
+	if (hx == 0 || hx == 1) {
+		const int hi = hx ? i0 + TX : i0 - 1;
+		if (hi >= 0 && hi < NXT && j < NYT) {
+			double hL[NV];
+			recon_dir(g, P, limiter, order >= 2 && hi >= 1 && hi + 1 < NXT, Qe, vsc, esc, vu, vv, hi + j * NXT, 1, hL, hR);
+			if (hx) {
+				#pragma unroll
+				for (int k = 0; k < NV; k++) sh[k * (TX + 2) + TX + 1] = hL[k];
+			} else hw = true;
+		}
+	}
+	__syncthreads();
+	const bool hasF = valid && i + 1 < NXT;
+	if (hasF) {
+		double QR[NV];
+		#pragma unroll
+		for (int k = 0; k < NV; k++) QR[k] = sh[k * (TX + 2) + tx + 2];
+		rusanov_states(R, QR, IMU, F, ub);
+	}
+	if (hw) {
+		double QR[NV];
+		#pragma unroll
+		for (int k = 0; k < NV; k++) QR[k] = sh[k * (TX + 2) + 1];
+		rusanov_states(hR, QR, IMU, hF, hub);
+	}
+	__syncthreads();
+	if (hasF) {
+		#pragma unroll
+		for (int k = 0; k < NV; k++) sh[k * (TX + 1) + tx + 1] = F[k];
+		sh[NV * (TX + 1) + tx + 1] = ub;
+	}
+	if (hw) {
+		#pragma unroll
+		for (int k = 0; k < NV; k++) sh[k * (TX + 1)] = hF[k];
+		sh[NV * (TX + 1)] = hub;
+	}
+	__syncthreads();
+	if (inner) {
+		double o[NV];
+		const double ubw = sh[NV * (TX + 1) + tx];
+		#pragma unroll
+		for (int k = 0; k < NV; k++) o[k] = Qs[k * NC + c] - rxd * (F[k] - sh[k * (TX + 1) + tx]);
+		o[IA1] += rxd * Qe[IA1 * NC + c] * (ub - ubw);
+		ubx[c] = ub; if (i == GH) ubx[c - 1] = ubw;
+		#pragma unroll
+		for (int k = 0; k < NV; k++) Ust[k * NC + c] = o[k];
+		a2s[c] = 1.0 - o[IA1];
+	}
+}
+
+// The 2D explicit stage with one barrier per direction: each thread reconstructs its cell (halo threads the cells
+// beside the tile) and puts both face states in LDS; after the barrier each cell takes its west / south neighbour's right
+// state and its east / north neighbour's left state from LDS and computes both of its face fluxes (each interior face
+// is computed by the two cells beside it, by the same operations, so both get the same value). Only the update is
+// held in registers across a barrier. Halo threads: the 2 TY x-halo cells on threads 64.., the 2 TX y-halo cells on the
+// last 2 TX threads.
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) k_explicit_2d(Geo g, Phys P, int limiter, int order, double rxd, double ryd,
+		const double* __restrict__ Qe, const double* __restrict__ Qs, const double* __restrict__ vsc, const double* __restrict__ esc,
+		const double* __restrict__ vu, const double* __restrict__ vv, double* __restrict__ wxL, double* __restrict__ wxR,
+		double* __restrict__ wyL, double* __restrict__ wyR, double* __restrict__ phx, double* __restrict__ phy, double* __restrict__ Ust,
+		double* __restrict__ a2s, double* __restrict__ ubx, double* __restrict__ uby, int tiles_y) {
+	constexpr int NT = TX * TY, HX = 64, HY = NT - 2 * TX;
+	static_assert(HY >= HX + 2 * TY, "halo threads");
+	// x: states [side][k][row][1 + column] (columns 0 and TX + 1 the halo); y: [side][k][1 + row][column]
+	constexpr int SX = NV * TY * (TX + 2), SY = NV * (TY + 2) * TX;
+	__shared__ double sh[2 * std::max(SX, SY)];
+	const int NXT = g.NXT, NYT = g.NYT, NC = g.NC, tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+	const int tile = xcd_band(blockIdx.x, gridDim.x), i0 = (tile / tiles_y) * TX, j0 = (tile % tiles_y) * TY;
+	const int i = i0 + tx, j = j0 + ty, c = i + j * NXT;
+	const bool valid = i < NXT && j < NYT;
+	const bool inner = i >= GH && i < g.nx + GH && j >= GH && j < g.ny + GH;
+	double o[NV];
+	// ---- x
+	{
+		auto at = [&](int side, int k, int row, int col) -> double& { return sh[side * SX + (k * TY + row) * (TX + 2) + col]; };
+		if (valid) {
+			double L[NV], R[NV];
+			st_nt(phx + c, recon_dir(g, P, limiter, order >= 2 && i >= 1 && i + 1 < NXT, Qe, vsc, esc, vu, vv, c, 1, L, R));
+			put_donors(wxL, wxR, NC, c, L, R);
+			#pragma unroll
+			for (int k = 0; k < NV; k++) { at(0, k, ty, tx + 1) = L[k]; at(1, k, ty, tx + 1) = R[k]; }
+		}
+		const int hx = tid - HX;
+		if (hx >= 0 && hx < 2 * TY) {
+			const int hr = hx >> 1, e = hx & 1, hi = e ? i0 + TX : i0 - 1, hj = j0 + hr;
+			if (hi >= 0 && hi < NXT && hj < NYT) {
+				double L[NV], R[NV];
+				recon_dir(g, P, limiter, order >= 2 && hi >= 1 && hi + 1 < NXT, Qe, vsc, esc, vu, vv, hi + hj * NXT, 1, L, R);
+				#pragma unroll
+				for (int k = 0; k < NV; k++) { at(0, k, hr, e ? TX + 1 : 0) = L[k]; at(1, k, hr, e ? TX + 1 : 0) = R[k]; }
+			}
+		}
+		__syncthreads();
+		if (inner) {
+			double A[NV], B[NV], Fw[NV], Fe[NV], ubw, ube;
+			#pragma unroll
+			for (int k = 0; k < NV; k++) { A[k] = at(1, k, ty, tx); B[k] = at(0, k, ty, tx + 1); }
+			rusanov_states(A, B, IMU, Fw, ubw);
+			#pragma unroll
+			for (int k = 0; k < NV; k++) { A[k] = at(1, k, ty, tx + 1); B[k] = at(0, k, ty, tx + 2); }
+			rusanov_states(A, B, IMU, Fe, ube);
+			#pragma unroll
+			for (int k = 0; k < NV; k++) o[k] = Qs[k * NC + c] - rxd * (Fe[k] - Fw[k]);
+			o[IA1] += rxd * Qe[IA1 * NC + c] * (ube - ubw);
+			st_nt(ubx + c, ube); if (i == GH) st_nt(ubx + c - 1, ubw);
+		}
+		__syncthreads();
+	}
+	// ---- y
+	{
+		auto at = [&](int side, int k, int row, int col) -> double& { return sh[side * SY + (k * (TY + 2) + row) * TX + col]; };
+		if (valid) {
+			double L[NV], R[NV];
+			st_nt(phy + c, recon_dir(g, P, limiter, order >= 2 && j >= 1 && j + 1 < NYT, Qe, vsc, esc, vu, vv, c, NXT, L, R));
+			put_donors(wyL, wyR, NC, c, L, R);
+			#pragma unroll
+			for (int k = 0; k < NV; k++) { at(0, k, ty + 1, tx) = L[k]; at(1, k, ty + 1, tx) = R[k]; }
+		}
+		const int hy = tid - HY;
+		if (hy >= 0) {
+			const int n = hy >= TX, hc = hy % TX, hi = i0 + hc, hj = n ? j0 + TY : j0 - 1;
+			if (hi < NXT && hj >= 0 && hj < NYT) {
+				double L[NV], R[NV];
+				recon_dir(g, P, limiter, order >= 2 && hj >= 1 && hj + 1 < NYT, Qe, vsc, esc, vu, vv, hi + hj * NXT, NXT, L, R);
+				#pragma unroll
+				for (int k = 0; k < NV; k++) { at(0, k, n ? TY + 1 : 0, hc) = L[k]; at(1, k, n ? TY + 1 : 0, hc) = R[k]; }
+			}
+		}
+		__syncthreads();
+		if (inner) {
+			double A[NV], B[NV], Fs[NV], Fn[NV], ubs, ubn;
+			#pragma unroll
+			for (int k = 0; k < NV; k++) { A[k] = at(1, k, ty, tx); B[k] = at(0, k, ty + 1, tx); }
+			rusanov_states(A, B, IMV, Fs, ubs);
+			#pragma unroll
+			for (int k = 0; k < NV; k++) { A[k] = at(1, k, ty + 1, tx); B[k] = at(0, k, ty + 2, tx); }
+			rusanov_states(A, B, IMV, Fn, ubn);
+			#pragma unroll
+			for (int k = 0; k < NV; k++) o[k] -= ryd * (Fn[k] - Fs[k]);
+			o[IA1] += ryd * Qe[IA1 * NC + c] * (ubn - ubs);
+			st_nt(uby + c, ubn); if (j == GH) st_nt(uby + c - NXT, ubs);
+			#pragma unroll
+			for (int k = 0; k < NV; k++) st_nt(Ust + k * NC + c, o[k]);
+			st_nt(a2s + c, 1.0 - o[IA1]);
+		}
+	}
 }
 
 // ------------------------------------------------------------------------------------------------ implicit stage preparation
@@ -181,7 +360,7 @@ __global__ void __launch_bounds__(256) k_residual(Dev D, const double* __restric
                            double* __restrict__ Pout, double* __restrict__ G, unsigned long long* __restrict__ nrm, int* __restrict__ flags, int flag_slot) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	double nv = 0.0;
-	if (q < D.g.n && !residual_at(D, P, dP, lam, Pout, G, q, nv)) flags[flag_slot] = 1;
+	if (q < D.g.n && !residual_at<true>(D, P, dP, lam, Pout, G, q, nv)) flags[flag_slot] = 1;
 	atomic_max_nonneg(nrm, nv);
 }
 // P scale s_q = (|P| + K + 1) / (1 + K D) of the finite-difference steps (PC:783-789)
@@ -203,7 +382,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                            double* __restrict__ J, double* __restrict__ rhs, int* __restrict__ flags) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= D.g.n) return;
-	if (!jacobian_analytic_at(D, P, G, J, rhs, q)) flags[FLAG_JAC] = 1;
+	if (!jacobian_analytic_at<true>(D, P, G, J, rhs, q)) flags[FLAG_JAC] = 1;
 }
 // right-hand side -G of a chord step (the faster forms' Newton step on the previous Jacobian)
 __global__ void __launch_bounds__(256) k_neg(int n, const double* __restrict__ G, double* __restrict__ rhs) {
@@ -292,11 +471,15 @@ template <typename T>
 __device__ __forceinline__ T spmv_plain(const Geo& g, const T* __restrict__ J, const T* __restrict__ z, int r) {
 	SpIn<T> s; sp_load(g, J, z, r, false, s); return sp_plain(s, g.twoD ? 5 : 3);
 }
-__device__ __forceinline__ int krow(int IT, int k) { return blockIdx.x * 256 * IT + k * 256 + threadIdx.x; }
+// logical block of a Krylov pass (xcd_band): the rows a block reads above and below its own (one grid line per block at
+// the common sizes) are those its XCD's other blocks are reading at about the same time. The partials are stored by
+// logical block, so every reduction sums in the same order as without the mapping.
+__device__ __forceinline__ int kblk() { return xcd_band(blockIdx.x, gridDim.x); }
+__device__ __forceinline__ int krow(int IT, int k) { return kblk() * 256 * IT + k * 256 + threadIdx.x; }
 // this block's partial of one slot (thread 0 writes)
 __device__ __forceinline__ void put_partial(const KP& P, int slot, double v, double* sh) {
 	const double tot = block_sum256(v, sh);
-	if (threadIdx.x == 0) P.part[slot * P.stride + blockIdx.x] = tot;
+	if (threadIdx.x == 0) P.part[slot * P.stride + kblk()] = tot;
 }
 // grid total of one slot written by the previous pass, in the fixed order (identical in every block)
 __device__ __forceinline__ double grid_total(const KP& P, int slot, double* sh) {
@@ -783,11 +966,27 @@ void launch_combine(const Geo& g, const double* Un, const double* H0, const doub
 }
 void launch_explicit(const Dev& D, int limiter, int order, const double* Qe, const double* Qs, double adt, hipStream_t st) {
 	const Geo& g = D.g;
-	hipLaunchKernelGGL(k_scales, dim3(nblk(g.NC)), dim3(256), 0, st, g, D.ph, Qe, D.vsc, D.esc, D.flags);
-	hipLaunchKernelGGL(k_recon, dim3(nblk(g.NC)), dim3(256), 0, st, g, D.ph, limiter, order, Qe, D.vsc, D.esc, D.wxL, D.wxR, D.wyL, D.wyR, D.phx, D.phy);
-	hipLaunchKernelGGL(k_update, dim3(nblk(g.n)), dim3(256), 0, st, g, adt / g.dx, adt / g.dy, Qe, Qs, D.wxL, D.wxR, D.wyL, D.wyR, D.Ust, D.a2s, D.ubx, D.uby);
+	hipLaunchKernelGGL(k_scales, dim3(nblk(g.NC)), dim3(256), 0, st, g, D.ph, Qe, D.vsc, D.esc, D.vu, D.vv, D.flags);
+	if (g.twoD) {
+		constexpr int TX = 32, TY = 8;
+		const int tx = (g.NXT + TX - 1) / TX, ty = (g.NYT + TY - 1) / TY;
+		hipLaunchKernelGGL((k_explicit_2d<TX, TY>), dim3(tx * ty), dim3(TX * TY), 0, st, g, D.ph, limiter, order, adt / g.dx, adt / g.dy, Qe, Qs,
+		                   (const double*)D.vsc, (const double*)D.esc, (const double*)D.vu, (const double*)D.vv, D.wxL, D.wxR, D.wyL, D.wyR, D.phx, D.phy,
+		                   D.Ust, D.a2s, D.ubx, D.uby, ty);
+	} else {
+		constexpr int TX = 256;
+		const int tx = (g.NXT + TX - 1) / TX, ty = g.NYT;
+		hipLaunchKernelGGL((k_explicit_1d<TX>), dim3(tx * ty), dim3(TX), 0, st, g, D.ph, limiter, order, adt / g.dx, Qe, Qs,
+		                   (const double*)D.vsc, (const double*)D.esc, (const double*)D.vu, (const double*)D.vv, D.wxL, D.wxR, D.phx,
+		                   D.Ust, D.a2s, D.ubx, ty);
+	}
 	launch_bc_grid(g, D.Ust, st);
 	launch_bc_array(g, D.a2s, st);
+}
+void launch_recon_full(const Dev& D, int limiter, int order, const double* Qe, double* wxL, double* wxR, double* wyL, double* wyR, hipStream_t st) {
+	const Geo& g = D.g;
+	hipLaunchKernelGGL(k_recon, dim3(nblk(g.NC)), dim3(256), 0, st, g, D.ph, limiter, order, Qe, (const double*)D.vsc, (const double*)D.esc,
+	                   wxL, wxR, wyL, wyR, D.phx, D.phy);
 }
 void launch_prep(const Dev& D, int limiter, int order, double tau, const double* Pst, hipStream_t st) {
 	const Geo& g = D.g;
