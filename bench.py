@@ -101,7 +101,7 @@ def run_form(a, form, keep_state=False):
 # HBM bytes per FP32 BiCGStab iteration measured by rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over the five
 # passes (active dispatches), calibrated on the FP32 conversion of the 5n-entry Jacobian (known bytes: FETCH_SIZE reads
 # x 2.00, WRITE_SIZE x 1.00); commands and per-pass counts in profiles/krylov_fp32_iteration_2048x1024_traffic.md
-TRAFFIC = {(True, 2048, 1024): 456535936}
+TRAFFIC = {(True, 2048, 1024): 349109376}
 
 
 def roofline_of(res, n, fp32, nx=2048, ny=1024):
