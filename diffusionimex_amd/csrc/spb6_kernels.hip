@@ -439,7 +439,7 @@ enum { SL_RR = 0, SL_R0R, SL_SP, SL_R0V, SL_SS, SL_SSUM, SL_TS, SL_TT, SL_E, SL_
 
 template <typename T>
 __device__ __forceinline__ T dinv_v(T d) { return d != T(0) ? T(1) / d : T(1); }
-__device__ __forceinline__ bool is_red(const Geo& g, int r) { return (((r % g.nx) + (r / g.nx)) & 1) == 0; }
+__device__ __forceinline__ bool is_red(const Geo& g, int r) { const int jj = r / g.nx; return ((r - jj * g.nx + jj) & 1) == 0; }
 template <typename T>
 __device__ __forceinline__ T dinv_of(const T* __restrict__ J, int r) { return dinv_v(J[r]); }
 // the inputs of one row's product: entries (self, W, E, S, N), the vector and the diagonal at those columns
@@ -494,16 +494,22 @@ template <typename T> constexpr int chv() { return sizeof(T) == 4 ? 4 : 1; }
 template <typename T> constexpr int chs() { return sizeof(T) == 4 ? 2 : 1; }
 
 // row r of y = M^-1 z for the red-black sweep: z_r / d on a red row; on a black row (z_r - sum over red neighbours c
-// of A_rc z_c / d_c) / d, with the neighbours' z_c from zc(c) (the pass recomputes them from their own inputs)
+// of A_rc z_c / d_c) / d, with the neighbours' z_c from zc(c) (the pass recomputes them from their own inputs). The
+// row's grid position takes one division; its neighbours are those of neighbours(), an interior one being red (the
+// other colour) and a periodic partner's colour following from its position, as is_red would give it.
 template <typename T, typename ZC>
 __device__ __forceinline__ T rb_row(const Geo& g, const KV<T>& V, int r, T zr, ZC zc) {
-	if (is_red(g, r)) return zr * V.dg[r];
+	const int nx = g.nx, ny = g.ny, jj = r / nx, ii = r - jj * nx;
+	if (((ii + jj) & 1) == 0) return zr * V.dg[r];
 	const int n = g.n, ne = g.twoD ? 5 : 3;
-	const Nb nb = neighbours(g, r);
-	const int col[5] = {r, nb.W, nb.E, nb.S, nb.N};
+	const bool yp = g.yper && ny > 1;
+	const int col[5] = {r, ii > 0 ? r - 1 : (g.xper ? r + nx - 1 : r), ii < nx - 1 ? r + 1 : (g.xper ? r - (nx - 1) : r),
+	                    jj > 0 ? r - nx : (yp ? r + (ny - 1) * nx : r), jj < ny - 1 ? r + nx : (yp ? r - (ny - 1) * nx : r)};
+	const bool red[5] = {false, ii > 0 || ((nx - 1 + jj) & 1) == 0, ii < nx - 1 || (jj & 1) == 0,
+	                     jj > 0 || ((ii + ny - 1) & 1) == 0, jj < ny - 1 || (ii & 1) == 0};
 	T acc = zr;
 	#pragma unroll
-	for (int k = 1; k < 5; k++) if (k < ne && col[k] != r && is_red(g, col[k])) acc -= V.J[k * n + r] * (zc(col[k]) * V.dg[col[k]]);
+	for (int k = 1; k < 5; k++) if (k < ne && col[k] != r && red[k]) acc -= V.J[k * n + r] * (zc(col[k]) * V.dg[col[k]]);
 	return acc * V.dg[r];
 }
 // setup 1: partials of E = sum of all entries, sum(b), b.b (PC:726-738)
