@@ -131,6 +131,8 @@ int spb6_linsolve(spb6_ctx* ctx, const double* val, const double* b, double* x, 
 
 /* Copy a named device field to the host (ghosted fields: (nx+4)*(ny+4) values, grids 8x that in AoS order).
  * Names: U Ust Qe Qs wxL wxR wyL wyR phx phy a2s ubx uby ust vst cz Kw ubsx kpx wLx pvx ubsy kpy wLy pvy Qph P Pst.
+ * The face states wxL wxR wyL wyR (TVDReconstruction's, PC:392-445) are rebuilt from the last explicit stage's Qe for
+ * the call (the stage itself keeps only the face donors the implicit stage reads); the call allocates their scratch.
  * Returns the number of doubles written, or -1. */
 long spb6_get_field(spb6_ctx* ctx, const char* name, double* out, long capacity);
 
