@@ -399,7 +399,8 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 		MlpDev W{M->P, M->W1s, M->W1f, M->W2p, M->mub1, M->b3, M->mu_t, M->sd_t};
 		{
 			TRegion tr(c, T_GUESS);
-			mlp_infer(&D, c->F, adt, (int)n, nullptr, W, c->Pplain, ok0 ? c->Gplain : nullptr, thr, c->Pn, nullptr, c->st);
+			// the shock budget's list is free again by now: it carries the active tiles
+			mlp_infer(&D, c->F, adt, (int)n, nullptr, W, c->Pplain, ok0 ? c->Gplain : nullptr, thr, c->Pn, nullptr, c->st, c->slist, c->scount);
 		}
 		bool okl; double resl;
 		rc = eval_all(c, D, c->Pn, nullptr, 0.0, nullptr, c->Gn, okl, resl); if (rc) return rc;

@@ -22,8 +22,10 @@ void mlp_feat_cell(const Dev& D, const double* Pplain, float* F, hipStream_t st)
 void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, const double* Pplain, const double* Gplain, double thr,
               int off, int stride, long cap, float* X, float* Y, unsigned long long* count, hipStream_t st);
 void mlp_pack(const float* params, const float* mean, const float* istd, __bf16* W1s, __bf16* W1f, __bf16* W2p, float* mub1, hipStream_t st);
+// the learned guess into Pout (D given) or the network's outputs for standardised rows Xin into yout; with an activity
+// mask (Gplain) and list / count (device scratch of nrows / 16 + 1 ints), the active tiles are listed first
 void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
-               const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st);
+               const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st, int* list = nullptr, int* count = nullptr);
 // a cell takes the network's guess (and is dumped for training) when its plain-guess residual is at least this fraction
 // of the plain residual's max norm: elsewhere the plain guess is kept, so quiescent cells are never perturbed
 constexpr double GUESS_ACTIVE = 1e-3;

@@ -284,10 +284,30 @@ __global__ void __launch_bounds__(256) k_pack_bf16(const float* __restrict__ par
 // each wave takes 16-cell tiles. mode 0: features from the stage (guess); mode 1: standardised features given in
 // Xin[rows][175] (tests).
 constexpr int INF_WAVES = 16;
+// first pass of the learned guess: every cell gets the plain guess, and the 16-cell tiles with an active cell are
+// listed (in any order: a tile's result depends only on its own cells) for the network pass, which then takes them
+// in turn from the list. Active cells are a few percent of a grid and cluster (the shock, the bubble), so taking the
+// tiles in grid order would leave most wavefronts idle while a few run many active tiles.
+__global__ void __launch_bounds__(256) k_guess_tiles(FeatCtx X, int nrows, const double* __restrict__ Pplain, double* __restrict__ Pout,
+                                                     int* __restrict__ list, int* __restrict__ count) {
+	const int q = blockIdx.x * 256 + threadIdx.x, lane = threadIdx.x & 63;
+	const bool valid = q < nrows;
+	if (valid) Pout[q] = Pplain[q];
+	const unsigned long long m = __ballot(valid && cell_active(X, q));
+	__shared__ int nt, base, tl[16];
+	if (threadIdx.x == 0) nt = 0;
+	__syncthreads();
+	if ((lane & 15) == 0 && ((m >> lane) & 0xFFFFull)) tl[atomicAdd(&nt, 1)] = q >> 4;
+	__syncthreads();
+	if (threadIdx.x == 0) base = nt ? atomicAdd(count, nt) : 0;
+	__syncthreads();
+	if (threadIdx.x < nt) list[base + threadIdx.x] = tl[threadIdx.x];
+}
 __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(FeatCtx X, int nrows, const float* __restrict__ Xin, int mode,
                                                    const float* __restrict__ mu_f, const __bf16* __restrict__ W1, const __bf16* __restrict__ W2p,
                                                    const float* __restrict__ b2, const float* __restrict__ w3, float b3, float mu_t, float sd_t,
-                                                   const double* __restrict__ Pplain, double* __restrict__ Pout, float* __restrict__ yout) {
+                                                   const double* __restrict__ Pplain, double* __restrict__ Pout, float* __restrict__ yout,
+                                                   const int* __restrict__ tiles, const int* __restrict__ ntl) {
 	__shared__ uint4 sW1[MLP_H * MLP_KPAD / 8];     // 56 KB
 	__shared__ uint4 sW2[MLP_H * MLP_H / 8];        // 32 KB
 	__shared__ float4 sB2[MLP_H / 4], sW3[MLP_H / 4];
@@ -304,14 +324,16 @@ __global__ void __launch_bounds__(64 * INF_WAVES) k_mlp_infer(FeatCtx X, int nro
 	int soff[7]; bool sin[7];
 	#pragma unroll
 	for (int m = 0; m < 7; m++) { const int sp = 4 * m + g; sin[m] = sp < 25; soff[m] = sin[m] ? (sp / 5) * 20 + sp % 5 : 0; }
-	const int ntiles = (nrows + 15) / 16;
+	const int ntiles = tiles ? *ntl : (nrows + 15) / 16;
 	const bool staged = mode == 0 && X.g.nx % 16 == 0 && nrows % 16 == 0;
 	const float* tileF = &sF[wave][0][0][0] + lr;
-	for (int tile = blockIdx.x * INF_WAVES + wave; tile < ntiles; tile += gridDim.x * INF_WAVES) {
+	for (int it = blockIdx.x * INF_WAVES + wave; it < ntiles; it += gridDim.x * INF_WAVES) {
+		const int tile = tiles ? tiles[it] : it;
 		const int q = tile * 16 + lr;           // this lane's cell
 		const bool valid = q < nrows;
-		// a tile none of whose cells is active keeps the plain guess and skips the network (wave-uniform branch)
-		if (mode == 0 && __ballot(valid && cell_active(X, q)) == 0ull) {
+		// a tile none of whose cells is active keeps the plain guess and skips the network (wave-uniform branch; listed
+		// tiles have an active cell)
+		if (!tiles && mode == 0 && __ballot(valid && cell_active(X, q)) == 0ull) {
 			if (g == 0 && valid && Pout) Pout[q] = Pplain[q];
 			continue;
 		}
@@ -417,15 +439,21 @@ void mlp_pack(const float* params, const float* mean, const float* istd, __bf16*
 	hipLaunchKernelGGL(k_pack_bf16, dim3(nb(MLP_H * MLP_KPAD)), dim3(256), 0, st, params, istd, (const float*)mub1, W1s, W1f, W2p);
 }
 void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
-               const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st) {
+               const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st, int* list, int* count) {
 	FeatCtx fx{};
 	if (D) { fx.g = D->g; fx.F = F; fx.Kw = D->Kw; fx.cz = D->cz; fx.tau_dx = tau / D->g.dx; fx.G = Gplain; fx.Pplain = Pplain; fx.P_REF = D->P_REF; fx.thr = thr; }
 	int dev = 0; (void)hipGetDevice(&dev);
 	int cus = 256; (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
 	const int tiles = (nrows + 15) / 16;
 	const int blocks = std::max(1, std::min(cus, (tiles + INF_WAVES - 1) / INF_WAVES));
+	const bool listed = D && Gplain && Pout && list && count;   // the guess with an activity mask: listed active tiles
+	if (listed) {
+		(void)hipMemsetAsync(count, 0, sizeof(int), st);
+		hipLaunchKernelGGL(k_guess_tiles, dim3(nb(nrows)), dim3(256), 0, st, fx, nrows, Pplain, Pout, list, count);
+	}
 	hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * INF_WAVES), 0, st, fx, nrows, Xin, Xin ? 1 : 0, (const float*)W.mub1,
-	                   Xin ? W.W1s : W.W1f, W.W2p, W.params + MLP_OFF_B2, W.params + MLP_OFF_W3, W.b3, W.mu_t, W.sd_t, Pplain, Pout, yout);
+	                   Xin ? W.W1s : W.W1f, W.W2p, W.params + MLP_OFF_B2, W.params + MLP_OFF_W3, W.b3, W.mu_t, W.sd_t, Pplain, Pout, yout,
+	                   listed ? (const int*)list : nullptr, listed ? (const int*)count : nullptr);
 }
 
 // forward (and optionally backward) of the FP32 network on a batch Xb[Bn x 175]; grads into G (same layout as params)
