@@ -135,7 +135,13 @@ struct TRegion {
 	~TRegion() { if (c->timer.on) { hipEvent_t b = c->timer.get(); hipEventRecord(b, c->st); c->timer.pend.push_back({cat, a, b}); if (c->timer.pend.size() > 512 && !c->timer.hold) c->timer.resolve(); } }
 };
 
+// clear the norms and the flags of one evaluation (the stage's admissibility flags stay)
 int reset_scalars(spb6_ctx* c) {
+	HIPCHK(hipMemsetAsync(c->mail, 0, offsetof(HostScratch, flags) + FLAG_RHO * sizeof(int), c->st));
+	return SPB6_OK;
+}
+// clear the norms and every flag (the start of a stage)
+int reset_stage(spb6_ctx* c) {
 	HIPCHK(hipMemsetAsync(c->mail, 0, offsetof(HostScratch, ks), c->st));   // nrm and flags (contiguous)
 	return SPB6_OK;
 }
@@ -376,17 +382,21 @@ int prep_part(spb6_ctx* c, double adt) {
 // PressureSubsystem (PC:851-993) after the explicit part; writes the stage state into Ust and H_rks
 int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& iters, double& res) {
 	int rc = prep_part(c, adt); if (rc) return rc;
-	rc = fetch(c); if (rc) return rc;
-	if (c->hs->flags[FLAG_STAGE]) return SPB6_STAGE_INADMISSIBLE;
 	Dev D = stage_dev(c, adt);
 	const long n = c->g.n;
 	const bool dumping = c->dX && c->dcap > 0;
 	const bool learned = c->cfg.learned_guess && c->model;
-	bool have_G = false, ok0 = false; double res0 = 0.0;
+	// the plain guess's residual (Newton's first, PC:764-770); the stage's admissibility flags, of the explicit part
+	// (rho > 0, PC:255-258) and of the preparation (PC:862), are read back with it: one synchronisation instead of
+	// three. What runs meanwhile on an inadmissible stage ends normally (every loop in the kernels is bounded) and is
+	// discarded with the attempt.
+	bool ok0 = false; double res0 = 0.0;
+	rc = eval_all(c, D, c->Pst, nullptr, 0.0, nullptr, c->G, ok0, res0); if (rc) return rc;
+	if (c->hs->flags[FLAG_RHO]) return SPB6_RHO_NONPOSITIVE;
+	if (c->hs->flags[FLAG_STAGE]) return SPB6_STAGE_INADMISSIBLE;
+	const bool have_G = true;
 	if (dumping || learned) {
 		// the plain guess and its residual: features, the activity mask and the judge of the learned guess
-		rc = eval_all(c, D, c->Pst, nullptr, 0.0, nullptr, c->G, ok0, res0); if (rc) return rc;
-		have_G = true;
 		HIPCHK(hipMemcpyAsync(c->Pplain, c->Pst, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 		HIPCHK(hipMemcpyAsync(c->Gplain, c->G, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 		TRegion tr(c, T_GUESS);
@@ -448,10 +458,8 @@ int attempt(spb6_ctx* c, double dt, int rk) {
 			Qe = c->Qe; Qs = c->Qs;
 		}
 		const double adt = c->Ai[rks][rks] * dt;
-		int rc = reset_scalars(c); if (rc) return rc;
-		rc = explicit_part(c, Qe, Qs, adt); if (rc) return rc;
-		rc = fetch(c); if (rc) return rc;
-		if (c->hs->flags[FLAG_RHO]) return SPB6_RHO_NONPOSITIVE;
+		int rc = reset_stage(c); if (rc) return rc;
+		rc = explicit_part(c, Qe, Qs, adt); if (rc) return rc;   // its flag is read with the first residual
 		int iters; double res;
 		rc = implicit_part(c, adt, Qs, c->H[rks], iters, res);
 		if (rc != SPB6_OK) return rc;
@@ -617,7 +625,7 @@ int spb6_explicit_stage(spb6_ctx* c, double adt) {
 	if (!c) return SPB6_BAD_ARG;
 	hipSetDevice(c->device);
 	HIPCHK(hipMemcpyAsync(c->Pst, c->P, (size_t)c->g.n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
-	int rc = reset_scalars(c); if (rc) return rc;
+	int rc = reset_stage(c); if (rc) return rc;
 	rc = explicit_part(c, c->U, c->U, adt); if (rc) return rc;
 	rc = fetch(c); if (rc) return rc;
 	return c->hs->flags[FLAG_RHO] ? SPB6_RHO_NONPOSITIVE : SPB6_OK;
@@ -625,7 +633,7 @@ int spb6_explicit_stage(spb6_ctx* c, double adt) {
 int spb6_stage_prep(spb6_ctx* c, double adt) {
 	if (!c) return SPB6_BAD_ARG;
 	hipSetDevice(c->device);
-	int rc = reset_scalars(c); if (rc) return rc;
+	int rc = reset_stage(c); if (rc) return rc;
 	rc = prep_part(c, adt); if (rc) return rc;
 	rc = fetch(c); if (rc) return rc;
 	return c->hs->flags[FLAG_STAGE] ? SPB6_STAGE_INADMISSIBLE : SPB6_OK;
@@ -634,6 +642,7 @@ int spb6_implicit_stage(spb6_ctx* c, double adt, int* newton_iters, double* resi
 	if (!c) return SPB6_BAD_ARG;
 	hipSetDevice(c->device);
 	int it = 0; double res = 0;
+	if (reset_stage(c)) return SPB6_HIP_ERROR;   // the explicit part's flag was read by spb6_explicit_stage
 	const int rc = implicit_part(c, adt, c->U, c->H[0], it, res);
 	if (newton_iters) *newton_iters = it;
 	if (residual) *residual = res;

@@ -5,8 +5,9 @@
 
 namespace spb {
 
-// error / event flags written by kernels (int slots in device memory)
-enum { FLAG_RHO = 0, FLAG_STAGE = 1, FLAG_RES = 2, FLAG_JAC = 3, FLAG_FLOOR = 4, FLAG_FINAL = 5, FLAG_RES2 = 6, NFLAGS = 8 };
+// error / event flags written by kernels (int slots in device memory): the flags of one evaluation first, then the
+// stage's admissibility flags, which a stage keeps until its first residual is read back
+enum { FLAG_RES = 0, FLAG_JAC = 1, FLAG_FLOOR = 2, FLAG_FINAL = 3, FLAG_RES2 = 4, FLAG_RHO = 5, FLAG_STAGE = 6, NFLAGS = 8 };
 
 struct Geo {
 	int nx, ny, NXT, NYT, NC, n;   // interior sizes, ghosted sizes, ghosted cell count, interior cell count
