@@ -523,8 +523,9 @@ struct CellFaces {
 	}
 };
 struct Nb { int W, E, S, N; };
-__device__ __forceinline__ Nb neighbours(const Geo& g, int q) {   // M_P of the four neighbours of interior row q
-	const int ii = q % g.nx, jj = q / g.nx; Nb b;
+// M_P of the four neighbours of interior row q at grid position (ii, jj)
+__device__ __forceinline__ Nb neighbours_at(const Geo& g, int q, int ii, int jj) {
+	Nb b;
 	b.W = ii > 0 ? q - 1 : (g.xper ? q + g.nx - 1 : q);
 	b.E = ii < g.nx - 1 ? q + 1 : (g.xper ? q - (g.nx - 1) : q);
 	const bool yp = g.yper && g.ny > 1;
@@ -532,6 +533,7 @@ __device__ __forceinline__ Nb neighbours(const Geo& g, int q) {   // M_P of the 
 	b.N = jj < g.ny - 1 ? q + g.nx : (yp ? q - (g.ny - 1) * g.nx : q);
 	return b;
 }
+__device__ __forceinline__ Nb neighbours(const Geo& g, int q) { const int jj = q / g.nx; return neighbours_at(g, q, q - jj * g.nx, jj); }
 // residual of interior row q at P + lam dP (dP may be null): writes G (NaN where the closure fails) and the trial pressure;
 // nv = |G| / (|P_REF + P| + K + 1) (PC:770); false where the closure fails
 template <bool NT = false>

@@ -444,9 +444,15 @@ template <typename T>
 __device__ __forceinline__ T dinv_of(const T* __restrict__ J, int r) { return dinv_v(J[r]); }
 // the inputs of one row's product: entries (self, W, E, S, N), the vector and the diagonal at those columns
 template <typename T> struct SpIn { T j[5], z[5], d[5]; };
+// grid position of a pass's rows: one division for a thread's first row, then krow's steps of 256 rows
+struct RowPos { int i, j; };
+__device__ __forceinline__ RowPos rowpos(const Geo& g, int r) { const int j = r / g.nx; return RowPos{r - j * g.nx, j}; }
+__device__ __forceinline__ RowPos rowpos_step(const Geo& g, RowPos p) { p.i += 256; while (p.i >= g.nx) { p.i -= g.nx; p.j++; } return p; }
+// the position of row r in a pass: the running one in the FP32 red-black passes (measured faster there), one division
+// per row in the FP64 passes (measured faster there)
 template <typename T>
-__device__ __forceinline__ void sp_load(const Geo& g, const T* __restrict__ J, const T* __restrict__ z, int r, bool diag, SpIn<T>& s) {
-	const int n = g.n; const Nb b = neighbours(g, r);
+__device__ __forceinline__ void sp_load(const Geo& g, const T* __restrict__ J, const T* __restrict__ z, int r, RowPos pr, bool diag, SpIn<T>& s) {
+	const int n = g.n; const Nb b = neighbours_at(g, r, pr.i, pr.j);
 	const int col[5] = {r, b.W, b.E, b.S, b.N};
 	const int ne = g.twoD ? 5 : 3;
 	#pragma unroll
@@ -469,7 +475,7 @@ __device__ __forceinline__ T sp_plain(const SpIn<T>& s, int ne) {
 }
 template <typename T>
 __device__ __forceinline__ T spmv_plain(const Geo& g, const T* __restrict__ J, const T* __restrict__ z, int r) {
-	SpIn<T> s; sp_load(g, J, z, r, false, s); return sp_plain(s, g.twoD ? 5 : 3);
+	SpIn<T> s; sp_load(g, J, z, r, rowpos(g, r), false, s); return sp_plain(s, g.twoD ? 5 : 3);
 }
 // logical block of a Krylov pass (xcd_band): the rows a block reads above and below its own (one grid line per block at
 // the common sizes) are those its XCD's other blocks are reading at about the same time. The partials are stored by
@@ -498,8 +504,8 @@ template <typename T> constexpr int chs() { return sizeof(T) == 4 ? 2 : 1; }
 // row's grid position takes one division; its neighbours are those of neighbours(), an interior one being red (the
 // other colour) and a periodic partner's colour following from its position, as is_red would give it.
 template <typename T, typename ZC>
-__device__ __forceinline__ T rb_row(const Geo& g, const KV<T>& V, int r, T zr, ZC zc) {
-	const int nx = g.nx, ny = g.ny, jj = r / nx, ii = r - jj * nx;
+__device__ __forceinline__ T rb_row(const Geo& g, const KV<T>& V, int r, RowPos pr, T zr, ZC zc) {
+	const int nx = g.nx, ny = g.ny, jj = pr.j, ii = pr.i;
 	if (((ii + jj) & 1) == 0) return zr * V.dg[r];
 	const int n = g.n, ne = g.twoD ? 5 : 3;
 	const bool yp = g.yper && ny > 1;
@@ -579,6 +585,7 @@ __global__ void __launch_bounds__(256) kb_p(Geo g, KV<T> V, KS* ks, KP P, int it
 	const T omega = (T)ks->omega[cur];
 	if (lead) ks->rho[nxt] = (double)rho1;
 	double sp = 0.0;
+	RowPos pos = RB ? rowpos(g, krow(IT, 0)) : RowPos{0, 0};
 	#pragma unroll
 	for (int c0 = 0; c0 < IT; c0 += CHV) {
 		if (c0 > 0) {
@@ -587,11 +594,12 @@ __global__ void __launch_bounds__(256) kb_p(Geo g, KV<T> V, KS* ks, KP P, int it
 		}
 		#pragma unroll
 		for (int k = 0; k < CHV; k++) {
+			const RowPos pk = pos; if (RB) pos = rowpos_step(g, pos);
 			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
 			const T pn = rv[k] + beta * (pv[k] - omega * vv[k]);
 			if constexpr (RB) {   // new p into the other buffer: black rows read their red neighbours' old p
 				V.p2[r] = pn;
-				V.ph[r] = rb_row(g, V, r, pn, [&](int c) { return V.r[c] + beta * (V.p[c] - omega * V.v[c]); });
+				V.ph[r] = rb_row(g, V, r, pk, pn, [&](int c) { return V.r[c] + beta * (V.p[c] - omega * V.v[c]); });
 			}
 			else { V.p[r] = pn; sp += (double)pn; }
 		}
@@ -607,8 +615,12 @@ __global__ void __launch_bounds__(256) kb_v(Geo g, KV<T> V, KS* ks, KP P, int it
 	constexpr int CHS = chs<T>();
 	const T* z = RB ? (const T*)V.ph : (const T*)V.p;
 	SpIn<T> q[CHS]; T r0v[CHS];
+	RowPos pos = rowpos(g, krow(IT, 0));
 	#pragma unroll
-	for (int k = 0; k < CHS; k++) { const int r = krow(IT, k); if (k < IT && r < n) { sp_load(g, V.J, z, r, !RB, q[k]); r0v[k] = V.r0[r]; } }
+	for (int k = 0; k < CHS; k++) {
+		const RowPos pk = pos; pos = rowpos_step(g, pos);
+		const int r = krow(IT, k); if (k < IT && r < n) { sp_load(g, V.J, z, r, RB ? pk : rowpos(g, r), !RB, q[k]); r0v[k] = V.r0[r]; }
+	}
 	T cc = T(0);
 	if constexpr (!RB) {
 		cc = (T)grid_total(P, SL_SP, sh) * (T)ks->Einv;
@@ -619,7 +631,10 @@ __global__ void __launch_bounds__(256) kb_v(Geo g, KV<T> V, KS* ks, KP P, int it
 	for (int c0 = 0; c0 < IT; c0 += CHS) {
 		if (c0 > 0) {
 			#pragma unroll
-			for (int k = 0; k < CHS; k++) { const int r = krow(IT, c0 + k); if (c0 + k < IT && r < n) { sp_load(g, V.J, z, r, !RB, q[k]); r0v[k] = V.r0[r]; } }
+			for (int k = 0; k < CHS; k++) {
+				const RowPos pk = pos; pos = rowpos_step(g, pos);
+				const int r = krow(IT, c0 + k); if (c0 + k < IT && r < n) { sp_load(g, V.J, z, r, RB ? pk : rowpos(g, r), !RB, q[k]); r0v[k] = V.r0[r]; }
+			}
 		}
 		#pragma unroll
 		for (int k = 0; k < CHS; k++) {
@@ -642,6 +657,7 @@ __global__ void __launch_bounds__(256) kb_s(Geo g, KV<T> V, KS* ks, KP P, int it
 	const T alpha = (T)ks->rho[nxt] / (T)grid_total(P, SL_R0V, sh);
 	if (blockIdx.x == 0 && threadIdx.x == 0) ks->alpha[nxt] = (double)alpha;
 	double sss = 0.0, ssum = 0.0;
+	RowPos pos = RB ? rowpos(g, krow(IT, 0)) : RowPos{0, 0};
 	#pragma unroll
 	for (int c0 = 0; c0 < IT; c0 += CHV) {
 		if (c0 > 0) {
@@ -650,9 +666,10 @@ __global__ void __launch_bounds__(256) kb_s(Geo g, KV<T> V, KS* ks, KP P, int it
 		}
 		#pragma unroll
 		for (int k = 0; k < CHV; k++) {
+			const RowPos pk = pos; if (RB) pos = rowpos_step(g, pos);
 			const int r = krow(IT, c0 + k); if (!(c0 + k < IT && r < n)) continue;
 			const T sv = rv[k] - alpha * vv[k]; V.s[r] = sv; sss += (double)(sv * sv); ssum += (double)sv;
-			if constexpr (RB) V.sh[r] = rb_row(g, V, r, sv, [&](int c) { return V.r[c] - alpha * V.v[c]; });
+			if constexpr (RB) V.sh[r] = rb_row(g, V, r, pk, sv, [&](int c) { return V.r[c] - alpha * V.v[c]; });
 		}
 	}
 	put_partial(P, SL_SS, sss, sh); put_partial(P, SL_SSUM, ssum, sh);
@@ -667,8 +684,12 @@ __global__ void __launch_bounds__(256) kb_t(Geo g, KV<T> V, KS* ks, KP P, int it
 	constexpr int CHS = chs<T>();
 	const T* z = RB ? (const T*)V.sh : (const T*)V.s;
 	SpIn<T> q[CHS]; T sv[CHS];
+	RowPos pos = rowpos(g, krow(IT, 0));
 	#pragma unroll
-	for (int k = 0; k < CHS; k++) { const int r = krow(IT, k); if (k < IT && r < n) { sp_load(g, V.J, z, r, !RB, q[k]); if constexpr (RB) sv[k] = V.s[r]; } }
+	for (int k = 0; k < CHS; k++) {
+		const RowPos pk = pos; pos = rowpos_step(g, pos);
+		const int r = krow(IT, k); if (k < IT && r < n) { sp_load(g, V.J, z, r, RB ? pk : rowpos(g, r), !RB, q[k]); if constexpr (RB) sv[k] = V.s[r]; }
+	}
 	const double s2 = grid_total(P, SL_SS, sh), Ss = grid_total(P, SL_SSUM, sh);
 	const T alpha = (T)ks->alpha[nxt];
 	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -689,7 +710,10 @@ __global__ void __launch_bounds__(256) kb_t(Geo g, KV<T> V, KS* ks, KP P, int it
 	for (int c0 = 0; c0 < IT; c0 += CHS) {
 		if (c0 > 0) {
 			#pragma unroll
-			for (int k = 0; k < CHS; k++) { const int r = krow(IT, c0 + k); if (c0 + k < IT && r < n) { sp_load(g, V.J, z, r, !RB, q[k]); if constexpr (RB) sv[k] = V.s[r]; } }
+			for (int k = 0; k < CHS; k++) {
+				const RowPos pk = pos; pos = rowpos_step(g, pos);
+				const int r = krow(IT, c0 + k); if (c0 + k < IT && r < n) { sp_load(g, V.J, z, r, RB ? pk : rowpos(g, r), !RB, q[k]); if constexpr (RB) sv[k] = V.s[r]; }
+			}
 		}
 		#pragma unroll
 		for (int k = 0; k < CHS; k++) {
