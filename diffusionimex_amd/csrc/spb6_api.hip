@@ -26,7 +26,7 @@ namespace {
 
 // the scalars the host reads back: norms (8 slots of 64 striped maxima), flags, the Krylov loop state and the FP32
 // form's defect sums and step criterion. The device keeps the same struct in one allocation, so every read-back is one copy.
-struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; double dsum[3]; };
+struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; double dsum[3]; int spec; };
 
 // T_KRYLOV: BiCGStab iteration passes only; T_KAUX: the solves' setup, precision conversions and FP64 defects;
 // T_GUESS: the learned guess's features and inference
@@ -199,8 +199,12 @@ int krylov_run(spb6_ctx* c, const KrylovBufsT<T>& B, const T* b, double tol, int
 // sweep asking its FP32 solve for max(eta / 2, 5e-7) relative accuracy (the FP32 floor), so that one sweep meets
 // eta >= 1e-6. A step whose defect misses eta but has decreased is still returned: Newton's residual test and line
 // search judge it.
-int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, bool& crit, double& sn, bool finish) {
+// Dspec (FP32 form): launch the Newton step's first trial residual (P + dP at lam 1, into Pn / Gn) behind each chunk,
+// gated on the device's close of the sweep; trial is set when it ran and the read-back holds its norm and flag
+int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, bool& crit, double& sn, bool finish,
+           const Dev* Dspec = nullptr, bool* trial = nullptr) {
 	crit = false; sn = 0.0;
+	if (trial) *trial = false;
 	const long n = c->g.n;
 	if (!c->cfg.krylov_fp32) {
 		KrylovBufs B{c->dP, c->kr, c->kr0, c->kp, c->kv, c->ks_, c->kt, c->J, c->part, c->part_stride, c->ksd};
@@ -241,6 +245,10 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, boo
 				TRegion tr(c, T_KAUX);
 				launch_defect_x(g, c->J, c->rhs, c->dP, c->x32, c->kr, c->part2, c->part_stride, c->dsum, c->Pst, c->sc, c->st);
 			}
+			if (Dspec) {
+				{ TRegion tr(c, T_KAUX); launch_spec_commit(n, c->x32, c->dP, c->ksd, c->dsum, eta, &c->mail->spec, &c->nrm[0], c->flags, c->st); }
+				{ TRegion tr(c, T_RESIDUAL); launch_residual(*Dspec, c->Pst, c->dP, 1.0, c->Pn, c->Gn, &c->nrm[0], FLAG_RES, c->st, &c->mail->spec); }
+			}
 			HIPCHK(hipMemcpyAsync(c->hs, c->mail, sizeof(HostScratch), hipMemcpyDeviceToHost, c->st));
 			HIPCHK(hipStreamSynchronize(c->st));
 			if (c->hs->ks.done || launched > maxit) break;
@@ -251,12 +259,17 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, boo
 		for (const IterEv& e : iev) c->timer.pend.push_back({e.it < it ? T_KRYLOV : T_KIDLE, e.a, e.b});
 		c->timer.hold = false;
 		c->last_krylov = it; c->krylov_launched += launched; iters += it;
-		{ TRegion tr(c, T_KAUX); launch_add_f32(n, c->x32, c->dP, c->st); }
+		const bool closed = Dspec && c->hs->ks.done;   // the device has added x and decided the defect test
+		if (!closed) { TRegion tr(c, T_KAUX); launch_add_f32(n, c->x32, c->dP, c->st); }
 		crit = true; sn = c->hs->dsum[2];   // the step criterion just read is that of dP as it now stands
 		const double rr = c->hs->dsum[0], bb = c->hs->dsum[1];
 		if (getenv("SPB6_VERBOSE") && atoi(getenv("SPB6_VERBOSE")) >= 3)
 			fprintf(stderr, "        sweep %d krylov %d defect %.3e eta %.3e\n", sweep, it, std::sqrt(rr / bb), eta);
-		if (std::sqrt(rr) <= eta * std::sqrt(bb) || !(bb > 0)) { ok = true; break; }
+		if (closed ? c->hs->spec != 0 : (std::sqrt(rr) <= eta * std::sqrt(bb) || !(bb > 0))) {
+			ok = true;
+			if (closed && trial) *trial = true;
+			break;
+		}
 		ok = std::isfinite(rr) && rr < bb;   // progress so far
 		if (!kok && sweep > 0) break;
 		{ TRegion tr(c, T_KAUX); launch_to_f32(n, c->kr, c->b32, c->st); }
@@ -307,7 +320,7 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 		}
 		have_J = true;
 		c->stats.linsolves++;
-		bool kok = true, crit = false; int kit = 0; double snk = 0.0;
+		bool kok = true, crit = false, trial = false; int kit = 0; double snk = 0.0;
 		if (tri) {
 			TRegion tr(c, T_OTHER);
 			launch_thomas(n, c->J, c->rhs, c->dP, c->cpw, c->dpw, c->st);
@@ -315,7 +328,7 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 			// a step predicted to be the last (the observed contraction res / res_prev^2 puts the next residual below
 			// tol / 2) is solved to 0.1 tol / res, so that Newton finishes with it instead of taking one more step
 			const bool finish = chord_ok && full_step && res * res * res < 0.5 * tol * res_prev * res_prev;
-			rc = krylov(c, kit, kok, res, !chord, crit, snk, finish); if (rc) return rc;
+			rc = krylov(c, kit, kok, res, !chord, crit, snk, finish, &D, &trial); if (rc) return rc;
 		}
 		if (!crit) {   // the FP32 form reads the step criterion and the flags with its last defect
 			launch_stepcrit(n, c->dP, c->Pst, c->sc, &c->nrm[64], c->st);
@@ -326,8 +339,14 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 		if (!kok) return SPB6_KRYLOV_FAIL;
 		const double sn = crit ? snk : nrm_val(c, 1);
 		if (verbose >= 3) fprintf(stderr, "      step %.3e krylov %d\n", sn, kit);
+		// the first evaluation of both branches below is P + dP at lam 1: with FP32 Krylov it has already run behind the
+		// solve's last chunk (trial), and its norm and flag came back with the solve's state
+		auto first_eval = [&](bool& okv, double& rv) -> int {
+			if (trial) { okv = c->hs->flags[FLAG_RES] == 0; rv = nrm_val(c, 0); trial = false; return SPB6_OK; }
+			return eval_all(c, D, c->Pst, c->dP, 1.0, c->Pn, c->Gn, okv, rv);
+		};
 		if (sn < tol) {   // step criterion (PC:819-826): accept P + dP
-			rc = eval_all(c, D, c->Pst, c->dP, 1.0, c->Pn, c->Gn, ok, res); if (rc) return rc;
+			rc = first_eval(ok, res); if (rc) return rc;
 			std::swap(c->Pst, c->Pn); std::swap(c->G, c->Gn);
 			c->stats.newton_it++; iters = it + 1;
 			if (!ok) return SPB6_NEWTON_FAIL;
@@ -336,7 +355,7 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 		double lam = 1.0; bool acc = false;
 		for (int ls = 0; ls < 40; ls++) {   // Armijo line search (PC:827-832)
 			double rn;
-			rc = eval_all(c, D, c->Pst, c->dP, lam, c->Pn, c->Gn, ok, rn); if (rc) return rc;
+			rc = ls == 0 ? first_eval(ok, rn) : eval_all(c, D, c->Pst, c->dP, lam, c->Pn, c->Gn, ok, rn); if (rc) return rc;
 			if (verbose >= 3) fprintf(stderr, "      line search lam %.3e res %.3e ok %d\n", lam, rn, (int)ok);
 			if (ok && (rn < res * (1.0 - 1e-4 * lam) || rn < tol)) { acc = true; res = rn; break; }
 			lam *= 0.5;

@@ -57,7 +57,13 @@ void launch_prep(const Dev& D, int limiter, int order, double tau, const double*
 // shock budget in two passes; list holds n ints, count one int (both device scratch); strong_blocks: blocks of 64 for
 // the strong-shock pass (a few per compute unit)
 void launch_shock(const Dev& D, int* list, int* count, int strong_blocks, hipStream_t st);
-void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st);
+// (gate: launched speculatively, the evaluation runs only where *gate is set)
+void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st,
+                     const int* gate = nullptr);
+// close of an FP32 Krylov sweep behind a chunk: dP += x once the solve is done, the defect test into *spec, the trial
+// residual's norm slots (nrm[0..63]) and flag cleared
+void launch_spec_commit(long n, const float* x, double* dP, const KS* ks, const double* dsum, double eta, int* spec, unsigned long long* nrm, int* flags,
+                        hipStream_t st);
 void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, bool analytic, hipStream_t st);
 // FP64 defect b - J (dP + x) and the step criterion of dP + x for a running FP32 solve x: (r.r, b.b, criterion) into sums[3]
 void launch_defect_x(const Geo& g, const double* J, const double* b, const double* dP, const float* x, double* r, double* part, int stride,

@@ -357,7 +357,9 @@ __global__ void __launch_bounds__(64) k_shock_strong(Geo g, Phys P, const double
 // residual of every interior cell at P + lam dP (dP may be null); writes the trial pressure and G, the max-norm
 // |G| / (|P_REF + P| + K + 1) (PC:770) and the closure-failure flag
 __global__ void __launch_bounds__(256) k_residual(Dev D, const double* __restrict__ P, const double* __restrict__ dP, double lam,
-                           double* __restrict__ Pout, double* __restrict__ G, unsigned long long* __restrict__ nrm, int* __restrict__ flags, int flag_slot) {
+                           double* __restrict__ Pout, double* __restrict__ G, unsigned long long* __restrict__ nrm, int* __restrict__ flags, int flag_slot,
+                           const int* __restrict__ gate) {
+	if (gate && !*gate) return;   // a speculative evaluation the device decided against (the same value in every block)
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	double nv = 0.0;
 	if (q < D.g.n && !residual_at<true>(D, P, dP, lam, Pout, G, q, nv)) flags[flag_slot] = 1;
@@ -884,6 +886,25 @@ __global__ void __launch_bounds__(256) k_sum_part2(const double* __restrict__ pa
 	if (threadIdx.x == 0) { out[0] = s0; out[1] = s1; out[2] = smax(smax(sh[0], sh[1]), smax(sh[2], sh[3])); }
 }
 // x += (double) d
+// the close of an FP32 Krylov sweep, launched behind each chunk so that the host reads the sweep's end and the Newton
+// step's first trial residual in one synchronisation: once the solve is done, dP += x (the host then adds nothing) and
+// the sweep's defect test |r| <= eta |b| (or b = 0), the host's test, is decided here into *spec; the trial residual
+// launched next is gated on it. Block 0 clears that residual's norm slots and flag (the other flags stay for the host).
+__global__ void __launch_bounds__(256) k_spec_commit(long n, const float* __restrict__ x, double* __restrict__ dP, const KS* __restrict__ ks,
+                                                     const double* __restrict__ dsum, double eta, int* __restrict__ spec,
+                                                     unsigned long long* __restrict__ nrm, int* __restrict__ flags) {
+	const bool done = ks->done != 0;
+	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (done && i < n) dP[i] += (double)x[i];
+	if (blockIdx.x == 0) {
+		if (threadIdx.x < 64) nrm[threadIdx.x] = 0ull;
+		if (threadIdx.x == 0) {
+			const double rr = dsum[0], bb = dsum[1];
+			*spec = done && (sqrt(rr) <= eta * sqrt(bb) || !(bb > 0)) ? 1 : 0;
+			flags[FLAG_RES] = 0;
+		}
+	}
+}
 __global__ void __launch_bounds__(256) k_add_f32(long n, const float* __restrict__ d, double* __restrict__ x) {
 	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) x[i] += (double)d[i];
@@ -1030,8 +1051,13 @@ void launch_shock(const Dev& D, int* list, int* count, int strong_blocks, hipStr
 	hipLaunchKernelGGL(k_shock_strong, dim3(std::max(1, std::min(nblk(D.g.n, 64), strong_blocks))), dim3(64), 0, st, D.g, D.ph,
 	                   (const double*)D.Ust, D.Qph, (const int*)list, (const int*)count);
 }
-void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st) {
-	hipLaunchKernelGGL(k_residual, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, dP, lam, Pout, G, nrm, D.flags, flag_slot);
+void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st,
+                     const int* gate) {
+	hipLaunchKernelGGL(k_residual, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, dP, lam, Pout, G, nrm, D.flags, flag_slot, gate);
+}
+void launch_spec_commit(long n, const float* x, double* dP, const KS* ks, const double* dsum, double eta, int* spec, unsigned long long* nrm, int* flags,
+                        hipStream_t st) {
+	hipLaunchKernelGGL(k_spec_commit, dim3(nblk(n)), dim3(256), 0, st, n, x, dP, ks, dsum, eta, spec, nrm, flags);
 }
 void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, bool analytic, hipStream_t st) {
 	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc);
