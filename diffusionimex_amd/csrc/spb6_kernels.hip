@@ -501,24 +501,38 @@ __device__ __forceinline__ double grid_total(const KP& P, int slot, double* sh) 
 template <typename T> constexpr int chv() { return sizeof(T) == 4 ? 4 : 1; }
 template <typename T> constexpr int chs() { return sizeof(T) == 4 ? 2 : 1; }
 
+// the neighbours of row r at grid position pr, as neighbours() gives them (col[1..4] = W, E, S, N; the row itself for a
+// missing one), and whether each is red: an interior neighbour has the other colour of a black row, a periodic
+// partner's colour follows from its position, as is_red would give it
+__device__ __forceinline__ void rb_nbrs(const Geo& g, int r, RowPos pr, int col[5], bool red[5]) {
+	const int nx = g.nx, ny = g.ny, ii = pr.i, jj = pr.j;
+	const bool yp = g.yper && ny > 1;
+	col[0] = r; red[0] = false;
+	col[1] = ii > 0 ? r - 1 : (g.xper ? r + nx - 1 : r);            red[1] = ii > 0 || ((nx - 1 + jj) & 1) == 0;
+	col[2] = ii < nx - 1 ? r + 1 : (g.xper ? r - (nx - 1) : r);     red[2] = ii < nx - 1 || (jj & 1) == 0;
+	col[3] = jj > 0 ? r - nx : (yp ? r + (ny - 1) * nx : r);        red[3] = jj > 0 || ((ii + ny - 1) & 1) == 0;
+	col[4] = jj < ny - 1 ? r + nx : (yp ? r - (ny - 1) * nx : r);   red[4] = jj < ny - 1 || (ii & 1) == 0;
+}
 // row r of y = M^-1 z for the red-black sweep: z_r / d on a red row; on a black row (z_r - sum over red neighbours c
 // of A_rc z_c / d_c) / d, with the neighbours' z_c from zc(c) (the pass recomputes them from their own inputs). The
 // row's grid position takes one division; its neighbours are those of neighbours(), an interior one being red (the
 // other colour) and a periodic partner's colour following from its position, as is_red would give it.
 template <typename T, typename ZC>
 __device__ __forceinline__ T rb_row(const Geo& g, const KV<T>& V, int r, RowPos pr, T zr, ZC zc) {
-	const int nx = g.nx, ny = g.ny, jj = pr.j, ii = pr.i;
-	if (((ii + jj) & 1) == 0) return zr * V.dg[r];
-	const int n = g.n, ne = g.twoD ? 5 : 3;
-	const bool yp = g.yper && ny > 1;
-	const int col[5] = {r, ii > 0 ? r - 1 : (g.xper ? r + nx - 1 : r), ii < nx - 1 ? r + 1 : (g.xper ? r - (nx - 1) : r),
-	                    jj > 0 ? r - nx : (yp ? r + (ny - 1) * nx : r), jj < ny - 1 ? r + nx : (yp ? r - (ny - 1) * nx : r)};
-	const bool red[5] = {false, ii > 0 || ((nx - 1 + jj) & 1) == 0, ii < nx - 1 || (jj & 1) == 0,
-	                     jj > 0 || ((ii + ny - 1) & 1) == 0, jj < ny - 1 || (ii & 1) == 0};
+	const int jj = pr.j, ii = pr.i, n = g.n, ne = g.twoD ? 5 : 3;
+	int col[5]; bool red[5];
+	rb_nbrs(g, r, pr, col, red);
+	// every neighbour's inputs are loaded first, for every lane (a missing neighbour is the row itself), so that the
+	// loads are in flight together; the sum then takes the red neighbours in the order W, E, S, N
+	T zd[5], jv[5];
+	#pragma unroll
+	for (int k = 1; k < 5; k++) if (k < ne) { zd[k] = zc(col[k]) * V.dg[col[k]]; jv[k] = V.J[k * n + r]; }
+	const T d = V.dg[r];
+	if (((ii + jj) & 1) == 0) return zr * d;
 	T acc = zr;
 	#pragma unroll
-	for (int k = 1; k < 5; k++) if (k < ne && col[k] != r && red[k]) acc -= V.J[k * n + r] * (zc(col[k]) * V.dg[col[k]]);
-	return acc * V.dg[r];
+	for (int k = 1; k < 5; k++) if (k < ne && col[k] != r && red[k]) acc -= jv[k] * zd[k];
+	return acc * d;
 }
 // setup 1: partials of E = sum of all entries, sum(b), b.b (PC:726-738)
 template <typename T, int IT>
@@ -786,9 +800,12 @@ __global__ void __launch_bounds__(256) kbr_setup1(Geo g, const T* __restrict__ b
 		const int r = krow(IT, it); if (r >= n) continue;
 		const Nb nb = neighbours(g, r);
 		const int col[5] = {r, nb.W, nb.E, nb.S, nb.N};
-		T d = V.J[r];
+		T jv[5];
 		#pragma unroll
-		for (int k = 1; k < 5; k++) if (k < ne && col[k] == r) d += V.J[k * n + r];
+		for (int k = 0; k < 5; k++) if (k < ne) jv[k] = V.J[k * n + r];   // loaded together, added below as before
+		T d = jv[0];
+		#pragma unroll
+		for (int k = 1; k < 5; k++) if (k < ne && col[k] == r) d += jv[k];
 		const T di = dinv_v(d);
 		V.dg[r] = di; V.x[r] = is_red(g, r) ? b[r] * di : T(0); V.p[r] = T(0); V.v[r] = T(0);
 		sbb += (double)(b[r] * b[r]);
@@ -803,12 +820,17 @@ __global__ void __launch_bounds__(256) kbr_black(Geo g, const T* __restrict__ z,
 	const int n = g.n, ne = g.twoD ? 5 : 3;
 	#pragma unroll
 	for (int it = 0; it < IT; it++) {
-		const int r = krow(IT, it); if (r >= n || is_red(g, r)) continue;
-		const Nb nb = neighbours(g, r);
-		const int col[5] = {r, nb.W, nb.E, nb.S, nb.N};
+		const int r = krow(IT, it); if (r >= n) continue;
+		const RowPos pr = rowpos(g, r);
+		if (((pr.i + pr.j) & 1) == 0) continue;   // red row
+		int col[5]; bool red[5];
+		rb_nbrs(g, r, pr, col, red);
+		T jv[5], yv[5];   // loaded together (a missing neighbour is the row itself), summed in the order W, E, S, N
+		#pragma unroll
+		for (int k = 1; k < 5; k++) if (k < ne) { jv[k] = J[k * n + r]; yv[k] = y[col[k]]; }
 		T acc = z[r];
 		#pragma unroll
-		for (int k = 1; k < 5; k++) if (k < ne && col[k] != r && is_red(g, col[k])) acc -= J[k * n + r] * y[col[k]];
+		for (int k = 1; k < 5; k++) if (k < ne && col[k] != r && red[k]) acc -= jv[k] * yv[k];
 		y[r] = acc * dg[r];
 	}
 }
