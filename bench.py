@@ -98,10 +98,11 @@ def run_form(a, form, keep_state=False):
     return res, st, cfg
 
 
-# HBM bytes per FP32 BiCGStab iteration measured by rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over the five
-# passes (active dispatches), calibrated on the FP32 conversion of the 5n-entry Jacobian (known bytes: FETCH_SIZE reads
-# x 2.00, WRITE_SIZE x 1.00); commands and per-pass counts in profiles/krylov_fp32_iteration_2048x1024_traffic.md
-TRAFFIC = {(True, 2048, 1024): 349109376}
+# HBM bytes per launch of the head form's candidate dominant kernels at 2048x1024, measured by rocprofv3 --pmc FETCH_SIZE
+# and --pmc WRITE_SIZE (active dispatches), calibrated on the FP32 conversion kernel (known bytes: FETCH_SIZE reads x 2.00,
+# WRITE_SIZE x 1.00): one FP32 BiCGStab iteration (five passes) and one residual evaluation; commands and per-pass counts
+# in profiles/krylov_fp32_iteration_2048x1024_traffic.md and profiles/residual_2048x1024_traffic.md
+TRAFFIC = {("krylov_iter", True, 2048, 1024): 342980352, ("residual", True, 2048, 1024): 773206656}
 
 
 def roofline_of(res, n, fp32, nx=2048, ny=1024):
@@ -127,7 +128,7 @@ def roofline_of(res, n, fp32, nx=2048, ny=1024):
     ms, cnt, byts = cats[dom]
     avg_s = (ms / 1e3) / max(cnt, 1)
     achieved = byts / avg_s if avg_s > 0 else 0.0
-    traffic = TRAFFIC.get((fp32, nx, ny)) if dom == "krylov_iter" else None
+    traffic = TRAFFIC.get((dom, fp32, nx, ny))
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": traffic, "bytes_per_launch": byts, "avg_launch_ms": avg_s * 1e3,
             "iterations": cnt,
