@@ -26,7 +26,7 @@ namespace {
 
 // the scalars the host reads back: norms (8 slots of 64 striped maxima), flags, the Krylov loop state and the FP32
 // form's defect sums and step criterion. The device keeps the same struct in one allocation, so every read-back is one copy.
-struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; double dsum[3]; int spec; };
+struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; double dsum[3]; int spec; double gthr; };
 
 // T_KRYLOV: BiCGStab iteration passes only; T_KAUX: the solves' setup, precision conversions and FP64 defects;
 // T_GUESS: the learned guess's features and inference
@@ -409,13 +409,36 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 	// (rho > 0, PC:255-258) and of the preparation (PC:862), are read back with it: one synchronisation instead of
 	// three. What runs meanwhile on an inadmissible stage ends normally (every loop in the kernels is bounded) and is
 	// discarded with the attempt.
-	bool ok0 = false; double res0 = 0.0;
-	rc = eval_all(c, D, c->Pst, nullptr, 0.0, nullptr, c->G, ok0, res0); if (rc) return rc;
+	bool ok0 = false; double res0 = 0.0, resl = 0.0; bool okl = false;
+	if (learned) {
+		// the plain guess's residual, the learned guess built from it (its activity threshold computed on the device
+		// from that residual's norm) and the learned guess's residual (norm slots 128.., FLAG_RES2), all read back in one
+		// synchronisation with the stage's flags
+		rc = reset_scalars(c); if (rc) return rc;
+		{ TRegion tr(c, T_RESIDUAL); launch_residual(D, c->Pst, nullptr, 0.0, nullptr, c->G, &c->nrm[0], FLAG_RES, c->st); }
+		HIPCHK(hipMemcpyAsync(c->Pplain, c->Pst, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+		HIPCHK(hipMemcpyAsync(c->Gplain, c->G, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+		const spb6_mlp* M = c->model;
+		MlpDev W{M->P, M->W1s, M->W1f, M->W2p, M->mub1, M->b3, M->mu_t, M->sd_t};
+		{
+			TRegion tr(c, T_GUESS);
+			mlp_feat_cell(D, c->Pplain, c->F, c->st);
+			// the shock budget's list is free again by now: it carries the active tiles
+			mlp_infer(&D, c->F, adt, (int)n, nullptr, W, c->Pplain, c->Gplain, 0.0, c->Pn, nullptr, c->st, c->slist, c->scount,
+			          &c->nrm[0], &c->flags[FLAG_RES], &c->mail->gthr);
+		}
+		{ TRegion tr(c, T_RESIDUAL); launch_residual(D, c->Pn, nullptr, 0.0, nullptr, c->Gn, &c->nrm[128], FLAG_RES2, c->st); }
+		rc = fetch(c); if (rc) return rc;
+		ok0 = c->hs->flags[FLAG_RES] == 0; res0 = nrm_val(c, 0);
+		okl = c->hs->flags[FLAG_RES2] == 0; resl = nrm_val(c, 2);
+	} else {
+		rc = eval_all(c, D, c->Pst, nullptr, 0.0, nullptr, c->G, ok0, res0); if (rc) return rc;
+	}
 	if (c->hs->flags[FLAG_RHO]) return SPB6_RHO_NONPOSITIVE;
 	if (c->hs->flags[FLAG_STAGE]) return SPB6_STAGE_INADMISSIBLE;
 	const bool have_G = true;
-	if (dumping || learned) {
-		// the plain guess and its residual: features, the activity mask and the judge of the learned guess
+	if (dumping && !learned) {
+		// the plain guess and its residual: features and the activity mask of the training dump
 		HIPCHK(hipMemcpyAsync(c->Pplain, c->Pst, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 		HIPCHK(hipMemcpyAsync(c->Gplain, c->G, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 		TRegion tr(c, T_GUESS);
@@ -424,15 +447,6 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 	const double thr = ok0 ? GUESS_ACTIVE * res0 : 0.0;
 	if (learned && !(ok0 && res0 < c->cfg.newton_tol)) {
 		// the plain guess judges the learned one: keep whichever has the smaller Newton residual norm
-		const spb6_mlp* M = c->model;
-		MlpDev W{M->P, M->W1s, M->W1f, M->W2p, M->mub1, M->b3, M->mu_t, M->sd_t};
-		{
-			TRegion tr(c, T_GUESS);
-			// the shock budget's list is free again by now: it carries the active tiles
-			mlp_infer(&D, c->F, adt, (int)n, nullptr, W, c->Pplain, ok0 ? c->Gplain : nullptr, thr, c->Pn, nullptr, c->st, c->slist, c->scount);
-		}
-		bool okl; double resl;
-		rc = eval_all(c, D, c->Pn, nullptr, 0.0, nullptr, c->Gn, okl, resl); if (rc) return rc;
 		if (okl && (!ok0 || resl < res0)) {
 			if (ok0 && resl > 0) c->stats.guess_log10_gain += std::log10(res0 / resl);
 			std::swap(c->Pst, c->Pn); std::swap(c->G, c->Gn); ok0 = true; res0 = resl; c->stats.guess_used++;

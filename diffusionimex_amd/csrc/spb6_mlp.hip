@@ -54,11 +54,21 @@ struct FeatCtx {
 	// activity mask: a cell takes part (is dumped, or takes the network's guess) iff |G_q| / (|P_REF + P_q| + K_q + 1)
 	// >= thr for the plain guess's residual G and pressure P; G == nullptr marks every cell active
 	const double* G; const double* Pplain; double P_REF; double thr;
+	const double* thr_dev;   // when given, thr from device memory (k_guess_thr) instead
 };
 __device__ __forceinline__ bool cell_active(const FeatCtx& X, int q) {
 	if (!X.G) return true;
 	const double K = X.Kw[(q % X.g.nx + GH) + (q / X.g.nx + GH) * X.g.NXT];
-	return fabs(X.G[q]) >= X.thr * (fabs(X.P_REF + X.Pplain[q]) + K + 1.0);
+	const double thr = X.thr_dev ? *X.thr_dev : X.thr;
+	return fabs(X.G[q]) >= thr * (fabs(X.P_REF + X.Pplain[q]) + K + 1.0);
+}
+// the activity threshold from the plain guess's residual on the device: GUESS_ACTIVE x its max-norm (the maximum of
+// the 64 slots the residual kernel fills with non-negative bit patterns), 0 if that residual failed (flag set)
+__global__ void __launch_bounds__(64) k_guess_thr(const unsigned long long* __restrict__ nrm, const int* __restrict__ flag, double* __restrict__ thr) {
+	unsigned long long m = nrm[threadIdx.x];
+	#pragma unroll
+	for (int off = 32; off >= 1; off >>= 1) { const unsigned long long o = __shfl_xor(m, off, 64); m = m > o ? m : o; }
+	if (threadIdx.x == 0) *thr = *flag ? 0.0 : GUESS_ACTIVE * __longlong_as_double((long long)m);
 }
 // centre-cell values a row's features are taken relative to
 struct Centre { float iK, icz, F1c, F2c, F3c, F5c; };
@@ -429,7 +439,7 @@ void mlp_feat_cell(const Dev& D, const double* Pplain, float* F, hipStream_t st)
 }
 void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, const double* Pplain, const double* Gplain, double thr,
               int off, int stride, long cap, float* X, float* Y, unsigned long long* count, hipStream_t st) {
-	FeatCtx fx{D.g, F, D.Kw, D.cz, tau / D.g.dx, Gplain, Pplain, D.P_REF, thr};
+	FeatCtx fx{D.g, F, D.Kw, D.cz, tau / D.g.dx, Gplain, Pplain, D.P_REF, thr, nullptr};
 	const long rows = (D.g.n - off + stride - 1) / stride;
 	if (rows <= 0) return;
 	hipLaunchKernelGGL(k_dump, dim3(nb(rows)), dim3(256), 0, st, fx, Pconv, Pplain, off, stride, cap, X, Y, count);
@@ -439,9 +449,14 @@ void mlp_pack(const float* params, const float* mean, const float* istd, __bf16*
 	hipLaunchKernelGGL(k_pack_bf16, dim3(nb(MLP_H * MLP_KPAD)), dim3(256), 0, st, params, istd, (const float*)mub1, W1s, W1f, W2p);
 }
 void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
-               const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st, int* list, int* count) {
+               const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st, int* list, int* count,
+               const unsigned long long* nrm, const int* flag, double* thr_dev) {
 	FeatCtx fx{};
 	if (D) { fx.g = D->g; fx.F = F; fx.Kw = D->Kw; fx.cz = D->cz; fx.tau_dx = tau / D->g.dx; fx.G = Gplain; fx.Pplain = Pplain; fx.P_REF = D->P_REF; fx.thr = thr; }
+	if (D && thr_dev) {   // the threshold from the plain residual's norm on the device
+		hipLaunchKernelGGL(k_guess_thr, dim3(1), dim3(64), 0, st, nrm, flag, thr_dev);
+		fx.thr_dev = thr_dev;
+	}
 	int dev = 0; (void)hipGetDevice(&dev);
 	int cus = 256; (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
 	const int tiles = (nrows + 15) / 16;
