@@ -63,7 +63,7 @@ struct spb6_ctx {
 	Geo g; Phys ph;
 	// ghosted grids (8 fields each) and arrays
 	double *U = nullptr, *Ust = nullptr, *Qe = nullptr, *Qs = nullptr, *H[2] = {nullptr, nullptr};
-	double *wxL, *wxR, *wyL, *wyR, *phx, *phy, *a2s, *ubx, *uby, *vsc, *esc, *vu, *vv;   // face grids: the donors (m1, m2, e1, e2) only
+	double *donx, *dony, *phx, *phy, *a2s, *ubx, *uby, *vsc, *esc, *vu, *vv;   // face donors: (left, right) pairs of m1, m2, e1, e2
 	const double* qe_last = nullptr;   // Qe of the last explicit stage
 	double *ust, *vst, *cz, *Kw, *Pold, *ubsx, *kpx, *wLx, *pvx, *ubsy, *kpy, *wLy, *pvy, *Qph;
 	// pressure system (interior)
@@ -121,7 +121,7 @@ Dev make_dev(const spb6_ctx* c) {
 	Dev D;
 	D.g = c->g; D.ph = c->ph; D.P_REF = c->P_REF; D.rx = 0; D.ry = 0; D.tau = 0;
 	D.Ust_c = c->Ust; D.Ust = c->Ust; D.a2s = c->a2s; D.ubx = c->ubx; D.uby = c->uby; D.phx = c->phx; D.phy = c->phy;
-	D.vsc = c->vsc; D.esc = c->esc; D.vu = c->vu; D.vv = c->vv; D.wxL = c->wxL; D.wxR = c->wxR; D.wyL = c->wyL; D.wyR = c->wyR;
+	D.vsc = c->vsc; D.esc = c->esc; D.vu = c->vu; D.vv = c->vv; D.wxL = D.wxR = D.wyL = D.wyR = nullptr; D.donx = c->donx; D.dony = c->dony;
 	D.ust = c->ust; D.vst = c->vst; D.cz = c->cz; D.Kw = c->Kw; D.Pold = c->Pold;
 	D.ubsx = c->ubsx; D.kpx = c->kpx; D.wLx = c->wLx; D.pvx = c->pvx; D.ubsy = c->ubsy; D.kpy = c->kpy; D.wLy = c->wLy; D.pvy = c->pvy;
 	D.Qph = c->Qph; D.flags = c->flags;
@@ -541,10 +541,10 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	for (int k = 0; k < 2; k++) { c->ph.g[k] = cfg->ph[k].gamma; c->ph.pi[k] = cfg->ph[k].pi_inf; c->ph.q[k] = cfg->ph[k].q; c->ph.b[k] = cfg->ph[k].b; c->ph.cv[k] = cfg->ph[k].cv; }
 	const size_t NC = g.NC, n = g.n;
 	g_err.clear();
-	double** grids[] = {&c->U, &c->Ust, &c->Qe, &c->Qs, &c->H[0], &c->H[1], &c->wxL, &c->wxR};
+	double** grids[] = {&c->U, &c->Ust, &c->Qe, &c->Qs, &c->H[0], &c->H[1], &c->donx};   // donors: 4 fields x 2 = NV per cell
 	for (auto p : grids) if (dalloc(c, p, NC * NV)) return fail("alloc");
-	if (g.twoD) { if (dalloc(c, &c->wyL, NC * NV) || dalloc(c, &c->wyR, NC * NV)) return fail("alloc"); }
-	else { c->wyL = c->wxL; c->wyR = c->wxR; }
+	if (g.twoD) { if (dalloc(c, &c->dony, NC * NV)) return fail("alloc"); }
+	else c->dony = c->donx;
 	double** arrs[] = {&c->phx, &c->phy, &c->a2s, &c->ubx, &c->uby, &c->vsc, &c->esc, &c->vu, &c->vv, &c->ust, &c->vst, &c->cz, &c->Kw, &c->Pold,
 	                   &c->ubsx, &c->kpx, &c->wLx, &c->pvx, &c->ubsy, &c->kpy, &c->wLy, &c->pvy};
 	for (auto p : arrs) if (dalloc(c, p, NC)) return fail("alloc");
@@ -781,8 +781,9 @@ long spb6_get_field(spb6_ctx* c, const char* name, double* out, long capacity) {
 	const std::string s(name);
 	const Geo& g = c->g;
 	const double* grid = nullptr; const double* arr = nullptr; long cnt = 0;
+	const int face = s == "wxL" ? 0 : s == "wxR" ? 1 : s == "wyL" ? 2 : s == "wyR" ? 3 : -1;
 	if (s == "U") grid = c->U; else if (s == "Ust") grid = c->Ust; else if (s == "Qe") grid = c->Qe; else if (s == "Qs") grid = c->Qs;
-	else if (s == "wxL") grid = c->wxL; else if (s == "wxR") grid = c->wxR; else if (s == "wyL") grid = c->wyL; else if (s == "wyR") grid = c->wyR;
+	else if (face >= 0) {}
 	else if (s == "phx") arr = c->phx; else if (s == "phy") arr = c->phy; else if (s == "a2s") arr = c->a2s; else if (s == "ubx") arr = c->ubx; else if (s == "uby") arr = c->uby;
 	else if (s == "ust") arr = c->ust; else if (s == "vst") arr = c->vst; else if (s == "cz") arr = c->cz; else if (s == "Kw") arr = c->Kw;
 	else if (s == "ubsx") arr = c->ubsx; else if (s == "kpx") arr = c->kpx; else if (s == "wLx") arr = c->wLx; else if (s == "pvx") arr = c->pvx;
@@ -790,17 +791,17 @@ long spb6_get_field(spb6_ctx* c, const char* name, double* out, long capacity) {
 	else if (s == "Qph") { arr = c->Qph; cnt = 2L * g.NC; }
 	else if (s == "P") { arr = c->P; cnt = g.n; } else if (s == "Pst") { arr = c->Pst; cnt = g.n; }
 	else return -1;
-	if (grid) {
+	if (grid || face >= 0) {
 		const long tot = (long)g.NC * NV; if (capacity < tot) return -1;
 		std::vector<double> h(tot);
 		double* tmp = nullptr;
-		if (grid == c->wxL || grid == c->wxR || grid == c->wyL || grid == c->wyR) {
+		if (face >= 0) {
 			// the explicit stage keeps only the face donors: the full face states of its Qe are rebuilt for inspection
 			if (!c->qe_last) return -1;
 			if (hipMalloc((void**)&tmp, 4 * (size_t)tot * sizeof(double)) != hipSuccess) return -1;
 			double *xL = tmp, *xR = tmp + tot, *yL = g.twoD ? tmp + 2 * tot : xL, *yR = g.twoD ? tmp + 3 * tot : xR;
 			launch_recon_full(make_dev(c), c->cfg.limiter, c->cfg.order, c->qe_last, xL, xR, yL, yR, c->st);
-			grid = grid == c->wxL ? xL : grid == c->wxR ? xR : grid == c->wyL ? yL : yR;
+			grid = face == 0 ? xL : face == 1 ? xR : face == 2 ? yL : yR;
 		}
 		const bool ok = hipMemcpyAsync(h.data(), grid, tot * sizeof(double), hipMemcpyDeviceToHost, c->st) == hipSuccess &&
 		                hipStreamSynchronize(c->st) == hipSuccess;

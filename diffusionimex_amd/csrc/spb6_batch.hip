@@ -47,7 +47,7 @@ struct BState {                // the problem's time loop (device resident)
 struct BArgs {
 	const BProb* prob; BState* st; const int* order; int np; int* qctr; int* nfin;
 	int NT;
-	double *U0, *U1, *H0, *H1, *Qe, *Qs, *wxL, *wxR;
+	double *U0, *U1, *H0, *H1, *Qe, *Qs, *wxL, *wxR, *donx;
 	double *vsc, *esc, *phx, *a2s, *ubx, *ust, *vst, *cz, *Kw, *Pold, *ubsx, *kpx, *wLx, *pvx, *Qph;
 	double *P0, *P1, *Pn, *G, *Gn, *dP, *sc, *rhs, *J, *cpw, *dpw;
 	int* slist;                // strong-shock cells of each problem (sum L ints)
@@ -111,6 +111,7 @@ __device__ __forceinline__ View make_view(const BArgs& A, const BProb& B, const 
 	D.g = g; D.ph = B.ph; D.P_REF = S.P_REF; D.rx = 0; D.ry = 0; D.tau = 0;
 	D.Ust_c = v.Ust; D.Ust = v.Ust; D.a2s = f(A.a2s); D.ubx = f(A.ubx); D.uby = D.ubx; D.phx = f(A.phx); D.phy = D.phx;
 	D.vsc = f(A.vsc); D.esc = f(A.esc); D.vu = nullptr; D.vv = nullptr; D.wxL = f(A.wxL); D.wxR = f(A.wxR); D.wyL = D.wxL; D.wyR = D.wxR;
+	D.donx = A.donx + 2 * sh; D.dony = D.donx;   // pairs: a shift of sh cells is 2 sh doubles
 	D.ust = f(A.ust); D.vst = f(A.vst); D.cz = f(A.cz); D.Kw = f(A.Kw); D.Pold = f(A.Pold);
 	D.ubsx = f(A.ubsx); D.kpx = f(A.kpx); D.wLx = f(A.wLx); D.pvx = f(A.pvx);
 	D.ubsy = D.ubsx; D.kpy = D.kpx; D.wLy = D.wLx; D.pvy = D.pvx; D.Qph = f(A.Qph); D.flags = nullptr;
@@ -223,7 +224,7 @@ __device__ int wg_stage(const BArgs& A, View& v, BState& S, int s, double dt, do
 	int bad = 0;
 	for (int i = threadIdx.x; i < NXT; i += T) if (!scales_at(g, D.ph, Qe, D.vsc, D.esc, i + GH * NXT)) bad = 1;
 	__syncthreads();
-	for (int i = threadIdx.x; i < NXT; i += T) recon_x_at(g, D.ph, A.limiter, A.order_, Qe, D.vsc, D.esc, D.wxL, D.wxR, D.phx, i + GH * NXT);
+	for (int i = threadIdx.x; i < NXT; i += T) recon_x_at(g, D.ph, A.limiter, A.order_, Qe, D.vsc, D.esc, D.wxL, D.wxR, D.phx, i + GH * NXT, D.donx);
 	__syncthreads();
 	for (int q = threadIdx.x; q < n; q += T) update_at(g, tau / g.dx, tau / g.dy, Qe, Qs, D.wxL, D.wxR, D.wyL, D.wyR, D.Ust, D.a2s, D.ubx, D.uby, q);
 	__syncthreads();
@@ -514,6 +515,8 @@ spb6_batch* spb6_batch_create(const spb6_batch_problem_t* probs, int nprob, cons
 	auto grid = [&](double** p, int fields) { return balloc(b, p, fields * b->NT, b->pad); };
 	if (grid(&A.U0, NV) || grid(&A.U1, NV) || grid(&A.H0, NV) || grid(&A.H1, NV) || grid(&A.Qe, NV) || grid(&A.Qs, NV) ||
 	    grid(&A.wxL, NV) || grid(&A.wxR, NV)) return fail();
+	// donor pairs: 4 fields x 2 per cell; a view's shift of sh cells moves pair addresses by 2 sh
+	if (balloc(b, &A.donx, 8L * b->NT, 2L * b->pad)) return fail();
 	double** arrs[] = {&A.vsc, &A.esc, &A.phx, &A.a2s, &A.ubx, &A.ust, &A.vst, &A.cz, &A.Kw, &A.Pold, &A.ubsx, &A.kpx, &A.wLx, &A.pvx};
 	for (auto p : arrs) if (grid(p, 1)) return fail();
 	if (grid(&A.Qph, 2)) return fail();

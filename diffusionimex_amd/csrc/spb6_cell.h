@@ -135,10 +135,25 @@ __device__ __forceinline__ double recon_cell(const Geo& g, const Phys& P, int li
                                              int ca, int cc, int cb, double* L, double* R) {
 	return recon_cell_s(P, limiter, ReconGlobal{Q, vsc, esc, g.NC}, ca, cc, cb, L, R);
 }
+// The face donors the implicit stage reads (m1, m2, e1, e2 of the left and right face states of a ghosted cell) are
+// kept per direction as four arrays of (left, right) pairs, pair c of field q at don + 2 (q NC + c): four 16-byte
+// stores per cell and direction, contiguous across a wavefront's lanes (half the streams of one array per value).
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+template <bool NT = true>
+__device__ __forceinline__ void put_donors(double* __restrict__ don, int NC, int c, const double* L, const double* R) {
+	const int F[4] = {IM1, IM2, IE1, IE2};
+	#pragma unroll
+	for (int q = 0; q < 4; q++) {
+		const dbl2 v = {L[F[q]], R[F[q]]};
+		dbl2* p = (dbl2*)(don + 2 * ((long)q * NC + c));
+		if constexpr (NT) __builtin_nontemporal_store(v, p); else *p = v;
+	}
+}
 // x-direction face states of ghosted cell c (all rows; cells with both x neighbours, else first order)
 __device__ __forceinline__ void recon_x_at(const Geo& g, const Phys& P, int limiter, int order, const double* __restrict__ Q,
                                            const double* __restrict__ vsc, const double* __restrict__ esc,
-                                           double* __restrict__ wxL, double* __restrict__ wxR, double* __restrict__ phx, int c) {
+                                           double* __restrict__ wxL, double* __restrict__ wxR, double* __restrict__ phx, int c,
+                                           double* __restrict__ donx = nullptr) {
 	const int NC = g.NC, i = c % g.NXT;
 	double L[NV], R[NV];
 	if (order >= 2 && i >= 1 && i + 1 < g.NXT) {
@@ -146,8 +161,9 @@ __device__ __forceinline__ void recon_x_at(const Geo& g, const Phys& P, int limi
 		for (int k = 0; k < NV; k++) { wxL[k * NC + c] = L[k]; wxR[k * NC + c] = R[k]; }
 	} else {
 		phx[c] = 0.0;
-		for (int k = 0; k < NV; k++) { const double v = Q[k * NC + c]; wxL[k * NC + c] = v; wxR[k * NC + c] = v; }
+		for (int k = 0; k < NV; k++) { const double v = Q[k * NC + c]; wxL[k * NC + c] = v; wxR[k * NC + c] = v; L[k] = v; R[k] = v; }
 	}
+	if (donx) put_donors<false>(donx, NC, c, L, R);
 }
 // face states L, R and limiter factor of ghosted cell c along the direction of stride d (1: x, NXT: y): the TVD
 // reconstruction when second is set (order 2 and both neighbours on the line), else first order (L = R = Q, factor 0)
@@ -168,11 +184,6 @@ __device__ __forceinline__ double recon_dir(const Geo& g, const Phys& P, int lim
 __device__ __forceinline__ void st_nt(double* p, double v) { __builtin_nontemporal_store(v, p); }
 // a store that is nontemporal when NT (the grid kernels; the batch kernel reads its values back inside the workgroup)
 template <bool NT> __device__ __forceinline__ void st_out(double* p, double v) { if constexpr (NT) st_nt(p, v); else *p = v; }
-// the face donors the implicit stage reads (m1, m2, e1, e2 of both face states) of ghosted cell c
-__device__ __forceinline__ void put_donors(double* __restrict__ wL, double* __restrict__ wR, int NC, int c, const double* L, const double* R) {
-	st_nt(wL + IM1 * NC + c, L[IM1]); st_nt(wL + IM2 * NC + c, L[IM2]); st_nt(wL + IE1 * NC + c, L[IE1]); st_nt(wL + IE2 * NC + c, L[IE2]);
-	st_nt(wR + IM1 * NC + c, R[IM1]); st_nt(wR + IM2 * NC + c, R[IM2]); st_nt(wR + IE1 * NC + c, R[IE1]); st_nt(wR + IE2 * NC + c, R[IE2]);
-}
 // Rusanov flux with the local material speed between face states QL and QR (PC:224-246), and the face velocity
 // ub = (u_L + u_R) / 2 of the path-conservative alpha1 term (PC:300-304)
 __device__ __forceinline__ void rusanov_states(const double* QL, const double* QR, int dirk, double* F, double& ub) {
@@ -404,11 +415,14 @@ __device__ __forceinline__ void load_face(const Dev& D, bool ydir, int c, int id
 	const Geo& g = D.g; const int NC = g.NC, cn = c + (ydir ? g.NXT : 1);
 	const double* wLf = ydir ? D.wLy : D.wLx; const double* pvf = ydir ? D.pvy : D.pvx;
 	const double* ubsf = ydir ? D.ubsy : D.ubsx; const double* kpf = ydir ? D.kpy : D.kpx; const double* ubef = ydir ? D.uby : D.ubx;
-	const double* wR = ydir ? D.wyR : D.wxR; const double* wLs = ydir ? D.wyL : D.wxL;
+	const double* don = ydir ? D.dony : D.donx;
 	const double* ut = ydir ? D.ust : D.vst;
 	f.wl = wLf[c]; f.pv = pvf[c]; f.ubs = ubsf[c]; f.kp = kpf[c]; f.ube = ubef[c];
-	f.cl[0] = wR[IM1 * NC + c]; f.cl[1] = wR[IM2 * NC + c]; f.cl[2] = wR[IE1 * NC + c]; f.cl[3] = wR[IE2 * NC + c];
-	f.cr[0] = wLs[IM1 * NC + cn]; f.cr[1] = wLs[IM2 * NC + cn]; f.cr[2] = wLs[IE1 * NC + cn]; f.cr[3] = wLs[IE2 * NC + cn];
+	#pragma unroll
+	for (int q = 0; q < 4; q++) {   // right state of c, left state of cn (put_donors' pairs)
+		f.cl[q] = don[2 * ((long)q * NC + c) + 1];
+		f.cr[q] = don[2 * ((long)q * NC + cn)];
+	}
 	f.utl = ut[c]; f.utr = ut[cn];
 	// wall faces (wall_x / wall_y, PC:594-595); idx is the face's i (x) or j (y) index
 	f.wall = -1;

@@ -28,7 +28,8 @@ struct Dev {
 	double P_REF, rx, ry, tau;
 	const double *Ust_c;
 	double *Ust, *a2s, *ubx, *uby, *phx, *phy, *vsc, *esc, *vu, *vv;
-	double *wxL, *wxR, *wyL, *wyR;
+	double *wxL, *wxR, *wyL, *wyR;   // full face states (the batch's update)
+	double *donx, *dony;             // face donors per direction: for each of m1, m2, e1, e2 an array of (left, right) pairs
 	double *ust, *vst, *cz, *Kw, *Pold;
 	double *ubsx, *kpx, *wLx, *pvx, *ubsy, *kpy, *wLy, *pvy, *Qph;
 	int* flags;

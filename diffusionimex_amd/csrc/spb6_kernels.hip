@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(256) k_recon(Geo g, Phys P, int limiter, int o
 template <int TX>
 __global__ void __launch_bounds__(TX) k_explicit_1d(Geo g, Phys P, int limiter, int order, double rxd,
 		const double* __restrict__ Qe, const double* __restrict__ Qs, const double* __restrict__ vsc, const double* __restrict__ esc,
-		const double* __restrict__ vu, const double* __restrict__ vv, double* __restrict__ wxL, double* __restrict__ wxR,
+		const double* __restrict__ vu, const double* __restrict__ vv, double* __restrict__ donx,
 		double* __restrict__ phx, double* __restrict__ Ust, double* __restrict__ a2s, double* __restrict__ ubx, int tiles_y) {
 	constexpr int HX = 64;
 	static_assert(HX + 2 <= TX, "halo threads");
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(TX) k_explicit_1d(Geo g, Phys P, int limiter, 
 	// states [k][1 + column] (east halo at column TX + 1), then fluxes [k][face] (face f west of column f)
 	if (valid) {
 		phx[c] = recon_dir(g, P, limiter, order >= 2 && i >= 1 && i + 1 < NXT, Qe, vsc, esc, vu, vv, c, 1, L, R);
-		put_donors(wxL, wxR, NC, c, L, R);
+		put_donors(donx, NC, c, L, R);
 		#pragma unroll
 		for (int k = 0; k < NV; k++) sh[k * (TX + 2) + tx + 1] = L[k];
 	}
@@ -229,8 +229,8 @@ __global__ void __launch_bounds__(TX) k_explicit_1d(Geo g, Phys P, int limiter, 
 template <int TX, int TY>
 __global__ void __launch_bounds__(TX * TY) k_explicit_2d(Geo g, Phys P, int limiter, int order, double rxd, double ryd,
 		const double* __restrict__ Qe, const double* __restrict__ Qs, const double* __restrict__ vsc, const double* __restrict__ esc,
-		const double* __restrict__ vu, const double* __restrict__ vv, double* __restrict__ wxL, double* __restrict__ wxR,
-		double* __restrict__ wyL, double* __restrict__ wyR, double* __restrict__ phx, double* __restrict__ phy, double* __restrict__ Ust,
+		const double* __restrict__ vu, const double* __restrict__ vv, double* __restrict__ donx,
+		double* __restrict__ dony, double* __restrict__ phx, double* __restrict__ phy, double* __restrict__ Ust,
 		double* __restrict__ a2s, double* __restrict__ ubx, double* __restrict__ uby, int tiles_y) {
 	constexpr int NT = TX * TY, HX = 64, HY = NT - 2 * TX;
 	static_assert(HY >= HX + 2 * TY, "halo threads");
@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(TX * TY) k_explicit_2d(Geo g, Phys P, int limi
 		if (valid) {
 			double L[NV], R[NV];
 			st_nt(phx + c, recon_dir(g, P, limiter, order >= 2 && i >= 1 && i + 1 < NXT, Qe, vsc, esc, vu, vv, c, 1, L, R));
-			put_donors(wxL, wxR, NC, c, L, R);
+			put_donors(donx, NC, c, L, R);
 			#pragma unroll
 			for (int k = 0; k < NV; k++) { at(0, k, ty, tx + 1) = L[k]; at(1, k, ty, tx + 1) = R[k]; }
 		}
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(TX * TY) k_explicit_2d(Geo g, Phys P, int limi
 		if (valid) {
 			double L[NV], R[NV];
 			st_nt(phy + c, recon_dir(g, P, limiter, order >= 2 && j >= 1 && j + 1 < NYT, Qe, vsc, esc, vu, vv, c, NXT, L, R));
-			put_donors(wyL, wyR, NC, c, L, R);
+			put_donors(dony, NC, c, L, R);
 			#pragma unroll
 			for (int k = 0; k < NV; k++) { at(0, k, ty + 1, tx) = L[k]; at(1, k, ty + 1, tx) = R[k]; }
 		}
@@ -1044,13 +1044,13 @@ void launch_explicit(const Dev& D, int limiter, int order, const double* Qe, con
 		constexpr int TX = 32, TY = 8;
 		const int tx = (g.NXT + TX - 1) / TX, ty = (g.NYT + TY - 1) / TY;
 		hipLaunchKernelGGL((k_explicit_2d<TX, TY>), dim3(tx * ty), dim3(TX * TY), 0, st, g, D.ph, limiter, order, adt / g.dx, adt / g.dy, Qe, Qs,
-		                   (const double*)D.vsc, (const double*)D.esc, (const double*)D.vu, (const double*)D.vv, D.wxL, D.wxR, D.wyL, D.wyR, D.phx, D.phy,
+		                   (const double*)D.vsc, (const double*)D.esc, (const double*)D.vu, (const double*)D.vv, D.donx, D.dony, D.phx, D.phy,
 		                   D.Ust, D.a2s, D.ubx, D.uby, ty);
 	} else {
 		constexpr int TX = 256;
 		const int tx = (g.NXT + TX - 1) / TX, ty = g.NYT;
 		hipLaunchKernelGGL((k_explicit_1d<TX>), dim3(tx * ty), dim3(TX), 0, st, g, D.ph, limiter, order, adt / g.dx, Qe, Qs,
-		                   (const double*)D.vsc, (const double*)D.esc, (const double*)D.vu, (const double*)D.vv, D.wxL, D.wxR, D.phx,
+		                   (const double*)D.vsc, (const double*)D.esc, (const double*)D.vu, (const double*)D.vv, D.donx, D.phx,
 		                   D.Ust, D.a2s, D.ubx, ty);
 	}
 	launch_bc_grid(g, D.Ust, st);
