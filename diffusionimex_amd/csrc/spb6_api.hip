@@ -65,6 +65,7 @@ struct spb6_ctx {
 	double *U = nullptr, *Ust = nullptr, *Qe = nullptr, *Qs = nullptr, *H[2] = {nullptr, nullptr};
 	double *donx, *dony, *phx, *phy, *a2s, *ubx, *uby, *vsc, *esc, *vu, *vv;   // face donors: (left, right) pairs of m1, m2, e1, e2
 	const double* qe_last = nullptr;   // Qe of the last explicit stage
+	bool final_pending = false;        // the previous stage's final-state flag (FLAG_FINAL_PREV) is not read back yet
 	double *ust, *vst, *cz, *Kw, *Pold, *ubsx, *kpx, *wLx, *pvx, *ubsy, *kpy, *wLy, *pvy, *Qph;
 	// pressure system (interior)
 	double *P, *Pst, *Pn, *G, *Gn, *dP, *rhs, *sc, *J, *cpw, *dpw;
@@ -140,9 +141,10 @@ int reset_scalars(spb6_ctx* c) {
 	HIPCHK(hipMemsetAsync(c->mail, 0, offsetof(HostScratch, flags) + FLAG_RHO * sizeof(int), c->st));
 	return SPB6_OK;
 }
-// clear the norms and every flag (the start of a stage)
-int reset_stage(spb6_ctx* c) {
-	HIPCHK(hipMemsetAsync(c->mail, 0, offsetof(HostScratch, ks), c->st));   // nrm and flags (contiguous)
+// clear the norms and the flags (the start of a stage); keep_prev keeps the previous stage's final-state flag
+int reset_stage(spb6_ctx* c, bool keep_prev = false) {
+	const size_t end = keep_prev ? offsetof(HostScratch, flags) + FLAG_FINAL_PREV * sizeof(int) : offsetof(HostScratch, ks);
+	HIPCHK(hipMemsetAsync(c->mail, 0, end, c->st));   // nrm and flags (contiguous)
 	return SPB6_OK;
 }
 // copy norms, flags and Krylov scalars to pinned host memory and wait
@@ -215,8 +217,7 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, boo
 	{
 		TRegion tr(c, T_KAUX);
 		if (new_J) launch_to_f32(5 * n, c->J, c->J32, c->st);   // a chord step keeps the previous FP32 copy
-		launch_to_f32(n, c->rhs, c->b32, c->st);
-		HIPCHK(hipMemsetAsync(c->dP, 0, n * sizeof(double), c->st));
+		launch_to_f32_clear(n, c->rhs, c->b32, c->dP, c->st);
 	}
 	KrylovBufs32 B{c->x32, c->r32, c->r032, c->p32, c->v32, c->s32, c->t32, c->J32, c->part, c->part_stride, c->ksd, c->ph32, c->sh32, c->dg32, c->pp32};
 	iters = 0; ok = false;
@@ -398,8 +399,10 @@ int prep_part(spb6_ctx* c, double adt) {
 	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->slist, c->scount, 16 * c->cus, c->st); }
 	return SPB6_OK;
 }
-// PressureSubsystem (PC:851-993) after the explicit part; writes the stage state into Ust and H_rks
-int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& iters, double& res) {
+// PressureSubsystem (PC:851-993) after the explicit part; writes the stage state into Ust and H_rks. defer_final
+// (a stage with another after it in the same attempt): the final state's closure flag is not waited for here but
+// read back with the next stage's first residual, which then reports the failure as this stage would have
+int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& iters, double& res, bool defer_final = false) {
 	int rc = prep_part(c, adt); if (rc) return rc;
 	Dev D = stage_dev(c, adt);
 	const long n = c->g.n;
@@ -434,6 +437,11 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 	} else {
 		rc = eval_all(c, D, c->Pst, nullptr, 0.0, nullptr, c->G, ok0, res0); if (rc) return rc;
 	}
+	if (c->final_pending) {   // the previous stage's final state, whose failure comes first
+		c->final_pending = false;
+		if (c->hs->flags[FLAG_FINAL_PREV]) return SPB6_CLOSURE_FAIL;
+		c->stats.stages++;
+	}
 	if (c->hs->flags[FLAG_RHO]) return SPB6_RHO_NONPOSITIVE;
 	if (c->hs->flags[FLAG_STAGE]) return SPB6_STAGE_INADMISSIBLE;
 	const bool have_G = true;
@@ -461,6 +469,12 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 		mlp_dump(D, c->F, adt, c->Pst, c->Pplain, c->Gplain, thr, off, c->dstride, c->dcap, c->dX, c->dY, c->dcount, c->st);
 	}
 	c->stats.maxres = std::max(c->stats.maxres, res);
+	if (defer_final) {
+		{ TRegion tr(c, T_FINAL); launch_final(D, c->Pst, Qs, adt, Hk, c->st, FLAG_FINAL_PREV); }
+		launch_bc_grid(c->g, c->Ust, c->st);
+		c->final_pending = true;
+		return SPB6_OK;
+	}
 	rc = reset_scalars(c); if (rc) return rc;
 	{ TRegion tr(c, T_FINAL); launch_final(D, c->Pst, Qs, adt, Hk, c->st); }
 	launch_bc_grid(c->g, c->Ust, c->st);
@@ -481,6 +495,7 @@ int attempt(spb6_ctx* c, double dt, int rk) {
 	set_tab(c, rk);
 	const size_t n = c->g.n;
 	HIPCHK(hipMemcpyAsync(c->Pst, c->P, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+	c->final_pending = false;
 	for (int rks = 0; rks < rk; rks++) {
 		const double *Qe = c->U, *Qs = c->U;
 		if (rks > 0) {
@@ -491,10 +506,10 @@ int attempt(spb6_ctx* c, double dt, int rk) {
 			Qe = c->Qe; Qs = c->Qs;
 		}
 		const double adt = c->Ai[rks][rks] * dt;
-		int rc = reset_stage(c); if (rc) return rc;
+		int rc = reset_stage(c, rks > 0); if (rc) return rc;
 		rc = explicit_part(c, Qe, Qs, adt); if (rc) return rc;   // its flag is read with the first residual
 		int iters; double res;
-		rc = implicit_part(c, adt, Qs, c->H[rks], iters, res);
+		rc = implicit_part(c, adt, Qs, c->H[rks], iters, res, rks + 1 < rk);
 		if (rc != SPB6_OK) return rc;
 	}
 	return SPB6_OK;
@@ -676,6 +691,7 @@ int spb6_implicit_stage(spb6_ctx* c, double adt, int* newton_iters, double* resi
 	hipSetDevice(c->device);
 	int it = 0; double res = 0;
 	if (reset_stage(c)) return SPB6_HIP_ERROR;   // the explicit part's flag was read by spb6_explicit_stage
+	c->final_pending = false;
 	const int rc = implicit_part(c, adt, c->U, c->H[0], it, res);
 	if (newton_iters) *newton_iters = it;
 	if (residual) *residual = res;

@@ -6,8 +6,9 @@
 namespace spb {
 
 // error / event flags written by kernels (int slots in device memory): the flags of one evaluation first, then the
-// stage's admissibility flags, which a stage keeps until its first residual is read back
-enum { FLAG_RES = 0, FLAG_JAC = 1, FLAG_FLOOR = 2, FLAG_FINAL = 3, FLAG_RES2 = 4, FLAG_RHO = 5, FLAG_STAGE = 6, NFLAGS = 8 };
+// stage's admissibility flags, which a stage keeps until its first residual is read back, then the closure failure of
+// the final state of an attempt's first stage, which the next stage reads back with its first residual
+enum { FLAG_RES = 0, FLAG_JAC = 1, FLAG_FLOOR = 2, FLAG_FINAL = 3, FLAG_RES2 = 4, FLAG_RHO = 5, FLAG_STAGE = 6, FLAG_FINAL_PREV = 7, NFLAGS = 8 };
 
 struct Geo {
 	int nx, ny, NXT, NYT, NC, n;   // interior sizes, ghosted sizes, ghosted cell count, interior cell count
@@ -73,7 +74,7 @@ void launch_chord_rhs(const Dev& D, const double* P, const double* G, double* sc
 void launch_fdscale(const Dev& D, const double* P, double* sc, hipStream_t st);
 void launch_stepcrit(int n, const double* dP, const double* P, const double* sc, unsigned long long* nrm, hipStream_t st);
 void launch_floor(const Dev& D, const double* P, double tol, hipStream_t st);
-void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, hipStream_t st);
+void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, hipStream_t st, int flag_slot = FLAG_FINAL);
 void launch_caldt(const Geo& g, const Phys& P, const double* U, int acoustic, unsigned long long* nrm, hipStream_t st);
 void launch_thomas(int n, const double* J, const double* rhs, double* x, double* cp, double* dp, hipStream_t st);
 void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hipStream_t st);
@@ -81,6 +82,7 @@ void launch_krylov_iter(const Geo& g, const KrylovBufs& B, hipStream_t st, int i
 void launch_linres(const Geo& g, const double* J, const double* b, const double* x, double* r, double* part, int stride, hipStream_t st);
 void launch_add_f32(long n, const float* d, double* x, hipStream_t st);
 void launch_to_f32(long n, const double* a, float* b, hipStream_t st);
+void launch_to_f32_clear(long n, const double* a, float* b, double* z, hipStream_t st);
 void launch_to_f64(long n, const float* a, double* b, hipStream_t st);
 void launch_krylov_setup32(const Geo& g, const KrylovBufs32& B, const float* b, hipStream_t st);
 void launch_krylov_iter32(const Geo& g, const KrylovBufs32& B, hipStream_t st, int it, double tol, int maxit);

@@ -379,17 +379,21 @@ __global__ void __launch_bounds__(256) k_jacobian(Dev D, const double* __restric
 	if (!jacobian_at(D, P, G, sc, J, rhs, q)) flags[FLAG_JAC] = 1;
 }
 // analytic Jacobian rows (the fast forms): one thread per row, no closure iterations. Two waves per SIMD (a few
-// registers spilled) measured 252 us at 2M rows against 289 us at the compiler's default of one.
+// registers spilled) measured 252 us at 2M rows against 289 us at the compiler's default of one. The row's P scale
+// stays a pass of its own (k_fdscale): computed here it measured 11 us slower than the two passes together.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_jacobian_an(Dev D, const double* __restrict__ P, const double* __restrict__ G,
                            double* __restrict__ J, double* __restrict__ rhs, int* __restrict__ flags) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= D.g.n) return;
 	if (!jacobian_analytic_at<true>(D, P, G, J, rhs, q)) flags[FLAG_JAC] = 1;
 }
-// right-hand side -G of a chord step (the faster forms' Newton step on the previous Jacobian)
-__global__ void __launch_bounds__(256) k_neg(int n, const double* __restrict__ G, double* __restrict__ rhs) {
+// right-hand side -G of a chord step (the faster forms' Newton step on the previous Jacobian) and the row's P scale
+__global__ void __launch_bounds__(256) k_chord_rhs(Dev D, const double* __restrict__ P, const double* __restrict__ G, double* __restrict__ sc,
+                                                   double* __restrict__ rhs) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	if (q < n) rhs[q] = -G[q];
+	if (q >= D.g.n) return;
+	fdscale_at(D, P, sc, q);
+	rhs[q] = -G[q];
 }
 // Newton step criterion max |dP| / (|P| + s) (PC:819-823)
 __global__ void __launch_bounds__(256) k_stepcrit(int n, const double* __restrict__ dP, const double* __restrict__ P, const double* __restrict__ sc, unsigned long long* __restrict__ nrm) {
@@ -405,10 +409,11 @@ __global__ void __launch_bounds__(256) k_floor(Dev D, const double* __restrict__
 	if (!floor_ok_at(D, P, tol, q)) flags[FLAG_FLOOR] = 1;
 }
 // final stage state at the converged pressure (PC:975-987) and the stage derivative Hs = (U_s - Qs) / adt (PC:1276-1284)
-__global__ void __launch_bounds__(256) k_final(Dev D, const double* __restrict__ P, const double* __restrict__ Qs, double adt, double* __restrict__ H, int* __restrict__ flags) {
+__global__ void __launch_bounds__(256) k_final(Dev D, const double* __restrict__ P, const double* __restrict__ Qs, double adt, double* __restrict__ H,
+                                              int* __restrict__ flags, int flag_slot) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= D.g.n) return;
-	if (!final_at(D, P, Qs, adt, H, q)) flags[FLAG_FINAL] = 1;
+	if (!final_at(D, P, Qs, adt, H, q)) flags[flag_slot] = 1;
 }
 // caldt maxima (PC:477-488): max |u| + c and |v| + c over interior cells with admissible primitives
 __global__ void __launch_bounds__(256) k_caldt(Geo g, Phys P, const double* __restrict__ U, int acoustic, unsigned long long* __restrict__ nrm) {
@@ -895,11 +900,23 @@ __global__ void __launch_bounds__(256) k_linres_x(Geo g, const double* __restric
 		part[2 * stride + blockIdx.x] = smax(smax(sh[0], sh[1]), smax(sh[2], sh[3]));
 	}
 }
-// the partial slots reduced in one block in the fixed order: out = (r.r, b.b, max criterion)
+// the partial slots reduced in one block in a fixed order: out = (r.r, b.b, max criterion); each thread issues the
+// loads of eight slots before it adds them, so the block waits on memory once per 2048 slots instead of once per 256
 __global__ void __launch_bounds__(256) k_sum_part2(const double* __restrict__ part, int nb, int stride, double* __restrict__ out) {
 	__shared__ double sh[4];
+	constexpr int U = 8;
 	double s0 = 0.0, s1 = 0.0, m = 0.0;
-	for (int c = threadIdx.x; c < nb; c += 256) { s0 += part[c]; s1 += part[stride + c]; m = smax(m, part[2 * stride + c]); }
+	for (int c0 = threadIdx.x; c0 < nb; c0 += 256 * U) {
+		double a[U], b[U], e[U];
+		#pragma unroll
+		for (int u = 0; u < U; u++) {
+			const int c = c0 + u * 256;
+			const bool in = c < nb;
+			a[u] = in ? part[c] : 0.0; b[u] = in ? part[stride + c] : 0.0; e[u] = in ? part[2 * stride + c] : 0.0;
+		}
+		#pragma unroll
+		for (int u = 0; u < U; u++) { s0 += a[u]; s1 += b[u]; m = smax(m, e[u]); }
+	}
 	s0 = block_sum256(s0, sh); s1 = block_sum256(s1, sh);
 	m = wave_max(m);
 	__syncthreads();
@@ -947,11 +964,19 @@ __global__ void __launch_bounds__(256) k_to_f32(long n, const double* __restrict
 	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) b[i] = (float)a[i];
 }
+// b = (float) a and z = 0 (the FP32 right-hand side of a Newton step's first sweep and its cleared FP64 step)
+__global__ void __launch_bounds__(256) k_to_f32_clear(long n, const double* __restrict__ a, float* __restrict__ b, double* __restrict__ z) {
+	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) { b[i] = (float)a[i]; z[i] = 0.0; }
+}
 __global__ void __launch_bounds__(256) k_to_f64(long n, const float* __restrict__ a, double* __restrict__ b) {
 	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) b[i] = (double)a[i];
 }
 void launch_to_f32(long n, const double* a, float* b, hipStream_t st) { hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b); }
+void launch_to_f32_clear(long n, const double* a, float* b, double* z, hipStream_t st) {
+	hipLaunchKernelGGL(k_to_f32_clear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b, z);
+}
 void launch_to_f64(long n, const float* a, double* b, hipStream_t st) { hipLaunchKernelGGL(k_to_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b); }
 
 // rows per thread of the Krylov passes: about a thousand blocks whatever n (4 .. 32 rows per thread)
@@ -1087,8 +1112,7 @@ void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc,
 	else hipLaunchKernelGGL(k_jacobian, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, (const double*)sc, J, rhs, D.flags);
 }
 void launch_chord_rhs(const Dev& D, const double* P, const double* G, double* sc, double* rhs, hipStream_t st) {
-	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc);
-	hipLaunchKernelGGL(k_neg, dim3(nblk(D.g.n)), dim3(256), 0, st, D.g.n, G, rhs);
+	hipLaunchKernelGGL(k_chord_rhs, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, sc, rhs);
 }
 void launch_fdscale(const Dev& D, const double* P, double* sc, hipStream_t st) {
 	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc);
@@ -1099,8 +1123,8 @@ void launch_stepcrit(int n, const double* dP, const double* P, const double* sc,
 void launch_floor(const Dev& D, const double* P, double tol, hipStream_t st) {
 	hipLaunchKernelGGL(k_floor, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, tol, D.flags);
 }
-void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, hipStream_t st) {
-	hipLaunchKernelGGL(k_final, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, Qs, adt, H, D.flags);
+void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, hipStream_t st, int flag_slot) {
+	hipLaunchKernelGGL(k_final, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, Qs, adt, H, D.flags, flag_slot);
 }
 void launch_caldt(const Geo& g, const Phys& P, const double* U, int acoustic, unsigned long long* nrm, hipStream_t st) {
 	hipLaunchKernelGGL(k_caldt, dim3(nblk(g.n)), dim3(256), 0, st, g, P, U, acoustic, nrm);
