@@ -311,13 +311,15 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 		iters = it; resfinal = res;
 		if (verbose >= 3) fprintf(stderr, "    newton it %d res %.3e\n", it, res);   // the reference's verbose >= 3 trace (PC:781)
 		if (res < tol) return SPB6_OK;
-		rc = reset_scalars(c); if (rc) return rc;
+		// the step's first pass clears its norm slots and flags (reset_scalars' bytes)
+		int* const clr = reinterpret_cast<int*>(c->mail);
+		const int ncl = (int)((offsetof(HostScratch, flags) + FLAG_RHO * sizeof(int)) / sizeof(int));
 		const bool chord = chord_ok && have_J && full_step && res < 100.0 * tol && res <= 1e-3 * res_prev;
 		const double res_before = res;
 		{
 			TRegion tr(c, T_JACOBIAN);
-			if (chord) launch_chord_rhs(D, c->Pst, c->G, c->sc, c->rhs, c->st);
-			else launch_jacobian(D, c->Pst, c->G, c->sc, c->J, c->rhs, c->cfg.analytic_jacobian != 0, c->st);
+			if (chord) launch_chord_rhs(D, c->Pst, c->G, c->sc, c->rhs, c->st, clr, ncl);
+			else launch_jacobian(D, c->Pst, c->G, c->sc, c->J, c->rhs, c->cfg.analytic_jacobian != 0, c->st, clr, ncl);
 		}
 		have_J = true;
 		c->stats.linsolves++;

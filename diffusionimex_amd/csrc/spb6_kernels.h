@@ -66,11 +66,13 @@ void launch_residual(const Dev& D, const double* P, const double* dP, double lam
 // residual's norm slots (nrm[0..63]) and flag cleared
 void launch_spec_commit(long n, const float* x, double* dP, const KS* ks, const double* dsum, double eta, int* spec, unsigned long long* nrm, int* flags,
                         hipStream_t st);
-void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, bool analytic, hipStream_t st);
+// clear (optional): nclear ints zeroed by the step's first pass (a Newton step's norm slots and flags)
+void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, bool analytic, hipStream_t st,
+                     int* clear = nullptr, int nclear = 0);
 // FP64 defect b - J (dP + x) and the step criterion of dP + x for a running FP32 solve x: (r.r, b.b, criterion) into sums[3]
 void launch_defect_x(const Geo& g, const double* J, const double* b, const double* dP, const float* x, double* r, double* part, int stride,
                      double* sums, const double* P, const double* sc, hipStream_t st);
-void launch_chord_rhs(const Dev& D, const double* P, const double* G, double* sc, double* rhs, hipStream_t st);
+void launch_chord_rhs(const Dev& D, const double* P, const double* G, double* sc, double* rhs, hipStream_t st, int* clear = nullptr, int nclear = 0);
 void launch_fdscale(const Dev& D, const double* P, double* sc, hipStream_t st);
 void launch_stepcrit(int n, const double* dP, const double* P, const double* sc, unsigned long long* nrm, hipStream_t st);
 void launch_floor(const Dev& D, const double* P, double tol, hipStream_t st);

@@ -366,7 +366,12 @@ __global__ void __launch_bounds__(256) k_residual(Dev D, const double* __restric
 	atomic_max_nonneg(nrm, nv);
 }
 // P scale s_q = (|P| + K + 1) / (1 + K D) of the finite-difference steps (PC:783-789)
-__global__ void __launch_bounds__(256) k_fdscale(Dev D, const double* __restrict__ P, double* __restrict__ sc) {
+// block 0 also zeroes clear[0, nclear) (a Newton step's norm slots and flags, in the pass that starts the step)
+__device__ __forceinline__ void clear_ints(int* __restrict__ clear, int nclear) {
+	if (blockIdx.x == 0) for (int k = threadIdx.x; k < nclear; k += blockDim.x) clear[k] = 0;
+}
+__global__ void __launch_bounds__(256) k_fdscale(Dev D, const double* __restrict__ P, double* __restrict__ sc, int* __restrict__ clear, int nclear) {
+	clear_ints(clear, nclear);
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= D.g.n) return;
 	fdscale_at(D, P, sc, q);
@@ -389,7 +394,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 }
 // right-hand side -G of a chord step (the faster forms' Newton step on the previous Jacobian) and the row's P scale
 __global__ void __launch_bounds__(256) k_chord_rhs(Dev D, const double* __restrict__ P, const double* __restrict__ G, double* __restrict__ sc,
-                                                   double* __restrict__ rhs) {
+                                                   double* __restrict__ rhs, int* __restrict__ clear, int nclear) {
+	clear_ints(clear, nclear);
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= D.g.n) return;
 	fdscale_at(D, P, sc, q);
@@ -873,22 +879,25 @@ __global__ void __launch_bounds__(256) k_linres(Geo g, const double* __restrict_
 }
 // FP64 defect r = b - J (dP + x) of the accumulated step plus the running FP32 solution x, without writing dP (the FP32
 // solve may continue), and the Newton step criterion (PC:819-823) of dP + x: block partials of r.r and b.b (slots 0, 1
-// of stride) and block maxima of |dP + x| / (|P| + s) (slot 2); the values equal k_linres and k_stepcrit after k_add_f32
+// of stride) and block maxima of |dP + x| / (|P| + s) (slot 2); the values equal k_linres and k_stepcrit after k_add_f32.
+// IT rows per thread in the Krylov passes' layout (krow: XCD bands, partials by logical block).
+template <int IT>
 __global__ void __launch_bounds__(256) k_linres_x(Geo g, const double* __restrict__ J, const double* __restrict__ b, const double* __restrict__ dP,
                                                  const float* __restrict__ x, const double* __restrict__ P, const double* __restrict__ sc,
                                                  double* __restrict__ r, double* __restrict__ part, int stride) {
 	__shared__ double sh[4];
 	const int n = g.n;
-	const int q = blockIdx.x * 256 + threadIdx.x;
 	double rr = 0.0, bb = 0.0, crit = 0.0;
-	if (q < n) {
+	#pragma unroll
+	for (int it = 0; it < IT; it++) {
+		const int q = krow(IT, it); if (q >= n) continue;
 		const Nb nb = neighbours(g, q);
 		const int col[5] = {q, nb.W, nb.E, nb.S, nb.N}, ne = g.twoD ? 5 : 3;
 		double y = 0.0;
 		#pragma unroll
 		for (int k = 0; k < 5; k++) if (k < ne) y += J[k * n + q] * (dP[col[k]] + (double)x[col[k]]);
-		const double v = b[q] - y; r[q] = v; rr = v * v; bb = b[q] * b[q];
-		crit = fabs(dP[q] + (double)x[q]) / (fabs(P[q]) + sc[q]);
+		const double bq = b[q], v = bq - y; r[q] = v; rr += v * v; bb += bq * bq;
+		crit = smax(crit, fabs(dP[q] + (double)x[q]) / (fabs(P[q]) + sc[q]));
 	}
 	const double t1 = block_sum256(rr, sh), t2 = block_sum256(bb, sh);
 	crit = wave_max(crit);
@@ -896,8 +905,9 @@ __global__ void __launch_bounds__(256) k_linres_x(Geo g, const double* __restric
 	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = crit;
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		part[blockIdx.x] = t1; part[stride + blockIdx.x] = t2;
-		part[2 * stride + blockIdx.x] = smax(smax(sh[0], sh[1]), smax(sh[2], sh[3]));
+		const int lb = kblk();
+		part[lb] = t1; part[stride + lb] = t2;
+		part[2 * stride + lb] = smax(smax(sh[0], sh[1]), smax(sh[2], sh[3]));
 	}
 }
 // the partial slots reduced in one block in a fixed order: out = (r.r, b.b, max criterion); each thread issues the
@@ -953,8 +963,9 @@ void launch_linres(const Geo& g, const double* J, const double* b, const double*
 }
 void launch_defect_x(const Geo& g, const double* J, const double* b, const double* dP, const float* x, double* r, double* part, int stride,
                      double* sums, const double* P, const double* sc, hipStream_t st) {
-	const unsigned nb = (unsigned)((g.n + 255) / 256);
-	hipLaunchKernelGGL(k_linres_x, dim3(nb), dim3(256), 0, st, g, J, b, dP, x, P, sc, r, part, stride);
+	constexpr int IT = 8;
+	const unsigned nb = (unsigned)((g.n + 256 * IT - 1) / (256 * IT));
+	hipLaunchKernelGGL(k_linres_x<IT>, dim3(nb), dim3(256), 0, st, g, J, b, dP, x, P, sc, r, part, stride);
 	hipLaunchKernelGGL(k_sum_part2, dim3(1), dim3(256), 0, st, (const double*)part, (int)nb, stride, sums);
 }
 void launch_add_f32(long n, const float* d, double* x, hipStream_t st) { hipLaunchKernelGGL(k_add_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, d, x); }
@@ -1106,16 +1117,17 @@ void launch_spec_commit(long n, const float* x, double* dP, const KS* ks, const 
                         hipStream_t st) {
 	hipLaunchKernelGGL(k_spec_commit, dim3(nblk(n)), dim3(256), 0, st, n, x, dP, ks, dsum, eta, spec, nrm, flags);
 }
-void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, bool analytic, hipStream_t st) {
-	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc);
+void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, bool analytic, hipStream_t st, int* clear,
+                     int nclear) {
+	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc, clear, nclear);
 	if (analytic) hipLaunchKernelGGL(k_jacobian_an, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, J, rhs, D.flags);
 	else hipLaunchKernelGGL(k_jacobian, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, (const double*)sc, J, rhs, D.flags);
 }
-void launch_chord_rhs(const Dev& D, const double* P, const double* G, double* sc, double* rhs, hipStream_t st) {
-	hipLaunchKernelGGL(k_chord_rhs, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, sc, rhs);
+void launch_chord_rhs(const Dev& D, const double* P, const double* G, double* sc, double* rhs, hipStream_t st, int* clear, int nclear) {
+	hipLaunchKernelGGL(k_chord_rhs, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, sc, rhs, clear, nclear);
 }
 void launch_fdscale(const Dev& D, const double* P, double* sc, hipStream_t st) {
-	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc);
+	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc, (int*)nullptr, 0);
 }
 void launch_stepcrit(int n, const double* dP, const double* P, const double* sc, unsigned long long* nrm, hipStream_t st) {
 	hipLaunchKernelGGL(k_stepcrit, dim3(nblk(n)), dim3(256), 0, st, n, dP, P, sc, nrm);
