@@ -93,6 +93,7 @@ struct spb6_ctx {
 	double* Pplain = nullptr;           // the plain guess of the current stage
 	double* Gplain = nullptr;           // its residual
 	int *slist = nullptr, *scount = nullptr;   // strong-shock cells of the shock budget's second pass
+	int *rlist = nullptr, *rcount = nullptr;   // cells whose residual the learned guess can change
 	int cus = 256;                      // compute units of the device
 	float *dX = nullptr, *dY = nullptr; unsigned long long* dcount = nullptr; long dcap = 0; int dstride = 0; unsigned dseq = 0;
 };
@@ -430,9 +431,11 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 			mlp_feat_cell(D, c->Pplain, c->F, c->st);
 			// the shock budget's list is free again by now: it carries the active tiles
 			mlp_infer(&D, c->F, adt, (int)n, nullptr, W, c->Pplain, c->Gplain, 0.0, c->Pn, nullptr, c->st, c->slist, c->scount,
-			          &c->nrm[0], &c->flags[FLAG_RES], &c->mail->gthr);
+			          &c->nrm[0], &c->flags[FLAG_RES], &c->mail->gthr, c->Gn, c->rlist, c->rcount, &c->nrm[128], &c->flags[FLAG_RES2]);
+			// the learned guess's residual: the plain one where no pressure it reads has changed (set above), evaluated
+			// on the listed cells
+			launch_residual_list(D, c->Pn, c->rlist, c->rcount, c->Gn, &c->nrm[128], FLAG_RES2, 4 * c->cus, c->st);
 		}
-		{ TRegion tr(c, T_RESIDUAL); launch_residual(D, c->Pn, nullptr, 0.0, nullptr, c->Gn, &c->nrm[128], FLAG_RES2, c->st); }
 		rc = fetch(c); if (rc) return rc;
 		ok0 = c->hs->flags[FLAG_RES] == 0; res0 = nrm_val(c, 0);
 		okl = c->hs->flags[FLAG_RES2] == 0; resl = nrm_val(c, 2);
@@ -585,6 +588,9 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (dalloc(c, &c->Pplain, n) || dalloc(c, &c->Gplain, n)) return fail("alloc");
 	if (hipMalloc((void**)&c->slist, (n + 1) * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->slist);
 	c->scount = c->slist + n;
+	if (hipMalloc((void**)&c->rlist, (n + 1) * sizeof(int)) != hipSuccess) return fail("alloc");
+	c->allocs.push_back(c->rlist);
+	c->rcount = c->rlist + n;
 	(void)hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device);
 	if (hipMalloc((void**)&c->F, 7 * NC * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->F);
 	if (hipDeviceSynchronize() != hipSuccess) return fail("sync");
@@ -759,6 +765,32 @@ int spb6_residual(spb6_ctx* c, const double* P, double* G) {
 	if (hipStreamSynchronize(c->st) != hipSuccess) return -1;
 	int bad = 0; for (size_t q = 0; q < n; q++) if (G[q] != G[q]) bad++;
 	return bad;
+}
+int spb6_guess_residual(spb6_ctx* c, const double* P, double* Pg, double* G, double* G_full, double* norms) {
+	if (!c || !P || !Pg || !G || !G_full || !norms || !c->model) return -1;
+	hipSetDevice(c->device);
+	const size_t n = c->g.n;
+	if (hipMemcpyAsync(c->Pplain, P, n * sizeof(double), hipMemcpyHostToDevice, c->st) != hipSuccess) return -1;
+	Dev D = stage_dev(c, c->tau_last);
+	if (reset_scalars(c)) return -1;
+	// as implicit_part's learned branch: the plain residual, the guess and its residual (norm slots 128..)
+	launch_residual(D, c->Pplain, nullptr, 0.0, nullptr, c->Gplain, &c->nrm[0], FLAG_RES, c->st);
+	const spb6_mlp* M = c->model;
+	MlpDev W{M->P, M->W1s, M->W1f, M->W2p, M->mub1, M->b3, M->mu_t, M->sd_t};
+	mlp_feat_cell(D, c->Pplain, c->F, c->st);
+	mlp_infer(&D, c->F, c->tau_last, (int)n, nullptr, W, c->Pplain, c->Gplain, 0.0, c->Pn, nullptr, c->st, c->slist, c->scount,
+	          &c->nrm[0], &c->flags[FLAG_RES], &c->mail->gthr, c->Gn, c->rlist, c->rcount, &c->nrm[128], &c->flags[FLAG_RES2]);
+	launch_residual_list(D, c->Pn, c->rlist, c->rcount, c->Gn, &c->nrm[128], FLAG_RES2, 4 * c->cus, c->st);
+	// the full evaluation at the guess (norm slots 192..)
+	launch_residual(D, c->Pn, nullptr, 0.0, nullptr, c->kr, &c->nrm[192], FLAG_FLOOR, c->st);
+	int cnt = 0;
+	if (hipMemcpyAsync(Pg, c->Pn, n * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess) return -1;
+	if (hipMemcpyAsync(G, c->Gn, n * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess) return -1;
+	if (hipMemcpyAsync(G_full, c->kr, n * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess) return -1;
+	if (hipMemcpyAsync(&cnt, c->rcount, sizeof(int), hipMemcpyDeviceToHost, c->st) != hipSuccess) return -1;
+	if (fetch(c)) return -1;
+	norms[0] = nrm_val(c, 2); norms[1] = nrm_val(c, 3);
+	return cnt;
 }
 int spb6_jacobian(spb6_ctx* c, const double* P, const double* G, double* val, double* sc) {
 	if (!c || !P || !G || !val || !sc) return -1;

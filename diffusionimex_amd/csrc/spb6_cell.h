@@ -392,6 +392,25 @@ __device__ __forceinline__ void shock_strong_at(const Geo& g, const Phys& P, con
 	}
 	Qph[c] = Q0; Qph[NC + c] = Q1;
 }
+// wavefront maximum (xor butterflies)
+__device__ __forceinline__ double wave_max(double v) {
+	#pragma unroll
+	for (int off = 32; off >= 1; off >>= 1) { const double o = __shfl_xor(v, off, 64); v = smax(v, o); }
+	return v;
+}
+// non-negative double max through its bit pattern (monotone for x >= 0): block maximum, then one atomic per block on one
+// of 64 striped slots (dst[0..63]); the host takes the maximum of the 64 slots (256-thread blocks)
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* dst, double v) {
+	__shared__ double shm[4];
+	v = wave_max(v);
+	__syncthreads();
+	if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		const double m = smax(smax(shm[0], shm[1]), smax(shm[2], shm[3]));
+		if (m > 0.0) atomicMax(&dst[blockIdx.x & 63], (unsigned long long)__double_as_longlong(m));
+	}
+}
 // append the cells of this wavefront whose flag is set to list (one atomic per wavefront); the list order is not
 // deterministic but each cell's result does not depend on it
 __device__ __forceinline__ void wave_append(bool flag, int q, int* __restrict__ list, int* __restrict__ count) {

@@ -64,6 +64,9 @@ void launch_residual(const Dev& D, const double* P, const double* dP, double lam
                      const int* gate = nullptr);
 // close of an FP32 Krylov sweep behind a chunk: dP += x once the solve is done, the defect test into *spec, the trial
 // residual's norm slots (nrm[0..63]) and flag cleared
+// the residual on the rows list[0, *count) only (blocks: the grid, each thread striding over the list)
+void launch_residual_list(const Dev& D, const double* P, const int* list, const int* count, double* G, unsigned long long* nrm, int flag_slot,
+                          int blocks, hipStream_t st);
 void launch_spec_commit(long n, const float* x, double* dP, const KS* ks, const double* dsum, double eta, int* spec, unsigned long long* nrm, int* flags,
                         hipStream_t st);
 // clear (optional): nclear ints zeroed by the step's first pass (a Newton step's norm slots and flags)

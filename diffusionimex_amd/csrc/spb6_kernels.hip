@@ -28,11 +28,6 @@ __device__ __forceinline__ double wave_bfly_sum(double v) {
 	for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
 	return v;
 }
-__device__ __forceinline__ double wave_max(double v) {
-	#pragma unroll
-	for (int off = 32; off >= 1; off >>= 1) { const double o = __shfl_xor(v, off, 64); v = smax(v, o); }
-	return v;
-}
 // 256-thread block total in the fixed order: xor butterflies in each wavefront, then (w0 + w1) + (w2 + w3)
 __device__ __forceinline__ double block_sum256(double v, double* sh) {
 	v = wave_bfly_sum(v);
@@ -40,19 +35,6 @@ __device__ __forceinline__ double block_sum256(double v, double* sh) {
 	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
 	__syncthreads();
 	return (sh[0] + sh[1]) + (sh[2] + sh[3]);
-}
-// non-negative double max through its bit pattern (monotone for x >= 0): block maximum, then one atomic per block on one
-// of 64 striped slots (dst[0..63]); the host takes the maximum of the 64 slots
-__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* dst, double v) {
-	__shared__ double shm[4];
-	v = wave_max(v);
-	__syncthreads();
-	if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = v;
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		const double m = smax(smax(shm[0], shm[1]), smax(shm[2], shm[3]));
-		if (m > 0.0) atomicMax(&dst[blockIdx.x & 63], (unsigned long long)__double_as_longlong(m));
-	}
 }
 
 // ------------------------------------------------------------------------------------------------ boundaries (PHY:926-957)
@@ -364,6 +346,21 @@ __global__ void __launch_bounds__(256) k_residual(Dev D, const double* __restric
 	double nv = 0.0;
 	if (q < D.g.n && !residual_at<true>(D, P, dP, lam, Pout, G, q, nv)) flags[flag_slot] = 1;
 	atomic_max_nonneg(nrm, nv);
+}
+// the same residual on the listed rows only (list[0, *count)), each thread taking rows at the grid's stride: the learned
+// guess's residual where it can differ from the plain guess's (the other rows come from k_guess_tiles)
+__global__ void __launch_bounds__(256) k_residual_list(Dev D, const double* __restrict__ P, const int* __restrict__ list, const int* __restrict__ count,
+                                                      double* __restrict__ G, unsigned long long* __restrict__ nrm, int* __restrict__ flags, int flag_slot) {
+	const int m = *count;
+	double vmax = 0.0;
+	bool bad = false;
+	for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
+		double nv;
+		if (!residual_at(D, P, nullptr, 0.0, nullptr, G, list[i], nv)) bad = true;
+		vmax = smax(vmax, nv);
+	}
+	if (bad) flags[flag_slot] = 1;
+	atomic_max_nonneg(nrm, vmax);
 }
 // P scale s_q = (|P| + K + 1) / (1 + K D) of the finite-difference steps (PC:783-789)
 // block 0 also zeroes clear[0, nclear) (a Newton step's norm slots and flags, in the pass that starts the step)
@@ -1112,6 +1109,10 @@ void launch_shock(const Dev& D, int* list, int* count, int strong_blocks, hipStr
 void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st,
                      const int* gate) {
 	hipLaunchKernelGGL(k_residual, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, dP, lam, Pout, G, nrm, D.flags, flag_slot, gate);
+}
+void launch_residual_list(const Dev& D, const double* P, const int* list, const int* count, double* G, unsigned long long* nrm, int flag_slot,
+                          int blocks, hipStream_t st) {
+	hipLaunchKernelGGL(k_residual_list, dim3(std::max(1, std::min(blocks, nblk(D.g.n)))), dim3(256), 0, st, D, P, list, count, G, nrm, D.flags, flag_slot);
 }
 void launch_spec_commit(long n, const float* x, double* dP, const KS* ks, const double* dsum, double eta, int* spec, unsigned long long* nrm, int* flags,
                         hipStream_t st) {

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include "spb6_device.h"
 #include "spb6_kernels.h"
+#include "spb6_cell.h"
 #include "spb6_mlp.h"
 #include <algorithm>
 
@@ -63,12 +64,18 @@ __device__ __forceinline__ bool cell_active(const FeatCtx& X, int q) {
 	return fabs(X.G[q]) >= thr * (fabs(X.P_REF + X.Pplain[q]) + K + 1.0);
 }
 // the activity threshold from the plain guess's residual on the device: GUESS_ACTIVE x its max-norm (the maximum of
-// the 64 slots the residual kernel fills with non-negative bit patterns), 0 if that residual failed (flag set)
-__global__ void __launch_bounds__(64) k_guess_thr(const unsigned long long* __restrict__ nrm, const int* __restrict__ flag, double* __restrict__ thr) {
+// the 64 slots the residual kernel fills with non-negative bit patterns), 0 if that residual failed (flag set); also
+// zeroes the list counters of the passes that follow (c0, c1 when given)
+__global__ void __launch_bounds__(64) k_guess_thr(const unsigned long long* __restrict__ nrm, const int* __restrict__ flag, double* __restrict__ thr,
+                                                  int* __restrict__ c0, int* __restrict__ c1) {
 	unsigned long long m = nrm[threadIdx.x];
 	#pragma unroll
 	for (int off = 32; off >= 1; off >>= 1) { const unsigned long long o = __shfl_xor(m, off, 64); m = m > o ? m : o; }
-	if (threadIdx.x == 0) *thr = *flag ? 0.0 : GUESS_ACTIVE * __longlong_as_double((long long)m);
+	if (threadIdx.x == 0) {
+		*thr = *flag ? 0.0 : GUESS_ACTIVE * __longlong_as_double((long long)m);
+		if (c0) *c0 = 0;
+		if (c1) *c1 = 0;
+	}
 }
 // centre-cell values a row's features are taken relative to
 struct Centre { float iK, icz, F1c, F2c, F3c, F5c; };
@@ -298,12 +305,36 @@ constexpr int INF_WAVES = 16;
 // listed (in any order: a tile's result depends only on its own cells) for the network pass, which then takes them
 // in turn from the list. Active cells are a few percent of a grid and cluster (the shock, the bubble), so taking the
 // tiles in grid order would leave most wavefronts idle while a few run many active tiles.
+// With rlist (the learned guess's residual evaluated only where it can differ from the plain guess's): a cell is
+// touched when it or one of its four neighbours (those its residual reads P at) is active, i.e. changes its pressure.
+// An untouched cell's residual inputs are the plain guess's, so its residual is the plain one, bit for bit: it is
+// copied into Gout, its norm term |G| / (|P_REF + P| + K + 1) taken into nrm2 (and a failed closure, G = NaN, into
+// *flag2) as the residual kernel would; touched cells are listed in rlist for k_residual_list.
 __global__ void __launch_bounds__(256) k_guess_tiles(FeatCtx X, int nrows, const double* __restrict__ Pplain, double* __restrict__ Pout,
-                                                     int* __restrict__ list, int* __restrict__ count) {
+                                                     int* __restrict__ list, int* __restrict__ count, double* __restrict__ Gout,
+                                                     int* __restrict__ rlist, int* __restrict__ rcount, unsigned long long* __restrict__ nrm2,
+                                                     int* __restrict__ flag2) {
 	const int q = blockIdx.x * 256 + threadIdx.x, lane = threadIdx.x & 63;
 	const bool valid = q < nrows;
 	if (valid) Pout[q] = Pplain[q];
-	const unsigned long long m = __ballot(valid && cell_active(X, q));
+	const bool act = valid && cell_active(X, q);
+	if (rlist) {
+		bool touched = false;
+		double nv = 0.0;
+		if (valid) {
+			const Nb b = neighbours(X.g, q);
+			touched = act || cell_active(X, b.W) || cell_active(X, b.E) || (X.g.twoD && (cell_active(X, b.S) || cell_active(X, b.N)));
+			if (!touched) {
+				const double gq = X.G[q];
+				Gout[q] = gq;
+				if (isfin(gq)) nv = fabs(gq) / (fabs(X.P_REF + X.Pplain[q]) + X.Kw[(q % X.g.nx + GH) + (q / X.g.nx + GH) * X.g.NXT] + 1.0);
+				else *flag2 = 1;
+			}
+		}
+		wave_append(touched, q, rlist, rcount);
+		atomic_max_nonneg(nrm2, nv);
+	}
+	const unsigned long long m = __ballot(act);
 	__shared__ int nt, base, tl[16];
 	if (threadIdx.x == 0) nt = 0;
 	__syncthreads();
@@ -450,11 +481,13 @@ void mlp_pack(const float* params, const float* mean, const float* istd, __bf16*
 }
 void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
                const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st, int* list, int* count,
-               const unsigned long long* nrm, const int* flag, double* thr_dev) {
+               const unsigned long long* nrm, const int* flag, double* thr_dev, double* Gout, int* rlist, int* rcount, unsigned long long* nrm2,
+               int* flag2) {
 	FeatCtx fx{};
 	if (D) { fx.g = D->g; fx.F = F; fx.Kw = D->Kw; fx.cz = D->cz; fx.tau_dx = tau / D->g.dx; fx.G = Gplain; fx.Pplain = Pplain; fx.P_REF = D->P_REF; fx.thr = thr; }
 	if (D && thr_dev) {   // the threshold from the plain residual's norm on the device
-		hipLaunchKernelGGL(k_guess_thr, dim3(1), dim3(64), 0, st, nrm, flag, thr_dev);
+		const bool lists = Gplain && Pout && list && count;
+		hipLaunchKernelGGL(k_guess_thr, dim3(1), dim3(64), 0, st, nrm, flag, thr_dev, lists ? count : nullptr, lists ? rcount : nullptr);
 		fx.thr_dev = thr_dev;
 	}
 	int dev = 0; (void)hipGetDevice(&dev);
@@ -463,8 +496,10 @@ void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float*
 	const int blocks = std::max(1, std::min(cus, (tiles + INF_WAVES - 1) / INF_WAVES));
 	const bool listed = D && Gplain && Pout && list && count;   // the guess with an activity mask: listed active tiles
 	if (listed) {
-		(void)hipMemsetAsync(count, 0, sizeof(int), st);
-		hipLaunchKernelGGL(k_guess_tiles, dim3(nb(nrows)), dim3(256), 0, st, fx, nrows, Pplain, Pout, list, count);
+		if (!fx.thr_dev) (void)hipMemsetAsync(count, 0, sizeof(int), st);   // else k_guess_thr has cleared the counters
+		const bool sparse = fx.thr_dev && Gout && rlist && rcount && nrm2 && flag2;
+		hipLaunchKernelGGL(k_guess_tiles, dim3(nb(nrows)), dim3(256), 0, st, fx, nrows, Pplain, Pout, list, count, Gout, sparse ? rlist : nullptr,
+		                   rcount, nrm2, flag2);
 	}
 	hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * INF_WAVES), 0, st, fx, nrows, Xin, Xin ? 1 : 0, (const float*)W.mub1,
 	                   Xin ? W.W1s : W.W1f, W.W2p, W.params + MLP_OFF_B2, W.params + MLP_OFF_W3, W.b3, W.mu_t, W.sd_t, Pplain, Pout, yout,

@@ -59,7 +59,7 @@ EXPORTS = ["spb6_create", "spb6_destroy", "spb6_last_error", "spb6_set_state", "
            "spb6_jacobian", "spb6_linsolve", "spb6_get_field", "spb6_enable_timing", "spb6_get_kernel_times",
            "spb6_mlp_create", "spb6_mlp_destroy", "spb6_mlp_nparams", "spb6_mlp_set_params", "spb6_mlp_get_params",
            "spb6_mlp_set_data", "spb6_mlp_train", "spb6_mlp_forward", "spb6_mlp_gradients", "spb6_mlp_infer_bf16",
-           "spb6_set_guess_model", "spb6_dump_enable", "spb6_dump_fetch",
+           "spb6_set_guess_model", "spb6_guess_residual", "spb6_dump_enable", "spb6_dump_fetch",
            "spb6_batch_create", "spb6_batch_destroy", "spb6_batch_last_error", "spb6_batch_cells", "spb6_batch_set_state",
            "spb6_batch_get_state", "spb6_batch_get_pressure", "spb6_batch_run", "spb6_batch_get_stats",
            "spb6_batch_set_launch", "spb6_batch_autotune"]
@@ -113,6 +113,7 @@ def load():
     L.spb6_mlp_gradients.argtypes = [vp, fp, fp, C.c_int, fp, fp]
     L.spb6_mlp_infer_bf16.argtypes = [vp, fp, C.c_int, fp]
     L.spb6_set_guess_model.argtypes = [vp, vp]
+    L.spb6_guess_residual.argtypes = [vp, dp, dp, dp, dp, dp]
     L.spb6_dump_enable.argtypes = [vp, C.c_long, C.c_int]
     L.spb6_dump_fetch.restype = C.c_long
     L.spb6_dump_fetch.argtypes = [vp, fp, fp, C.c_long]
@@ -260,6 +261,16 @@ class Stepper:
         if bad < 0:
             raise SPB6Error("residual failed")
         return G, bad
+
+    def guess_residual(self, P):
+        """learned guess at plain guess P after stage_prep: (guess, its residual as a stage computes it, the same residual
+        evaluated on every cell, their two max-norms, the number of cells the first evaluated)"""
+        P = np.ascontiguousarray(P, dtype=np.float64).ravel()
+        Pg, G, Gf, nrm = np.empty_like(P), np.empty_like(P), np.empty_like(P), np.zeros(2)
+        cnt = self.L.spb6_guess_residual(self.h, _dp(P), _dp(Pg), _dp(G), _dp(Gf), _dp(nrm))
+        if cnt < 0:
+            raise SPB6Error("guess_residual failed (no network attached?)")
+        return Pg, G, Gf, nrm, cnt
 
     def jacobian(self, P, G):
         P = np.ascontiguousarray(P, dtype=np.float64).ravel()
