@@ -30,11 +30,14 @@ struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; d
 
 // T_KRYLOV: BiCGStab iteration passes only; T_KAUX: the solves' setup, precision conversions and FP64 defects;
 // T_GUESS: the learned guess's features and inference
-// T_KIDLE: BiCGStab iterations launched past a solve's end (in a chunk), which return at once
+// T_KIDLE: BiCGStab iterations launched past a solve's end (in a chunk), which return at once, and speculative trial
+// residuals whose gate was off (they return at once too)
 enum TimerCat { T_EXPLICIT = 0, T_PREP, T_SHOCK, T_RESIDUAL, T_JACOBIAN, T_KRYLOV, T_FINAL, T_OTHER, T_KAUX, T_GUESS, T_KIDLE, T_NCAT };
 
 struct Timer {
 	bool on = false;
+	unsigned mask = ~0u;   // categories timed while on (bit k: category k); the others record no events
+	bool want(int cat) const { return on && ((mask >> cat) & 1u); }
 	bool hold = false;   // events are held outside pend (the FP32 solve's per-iteration pairs): no resolve meanwhile
 	std::vector<hipEvent_t> pool;
 	struct Pend { int cat; hipEvent_t a, b; };
@@ -133,8 +136,8 @@ Dev make_dev(const spb6_ctx* c) {
 // timed launch region
 struct TRegion {
 	spb6_ctx* c; int cat; hipEvent_t a = nullptr;
-	TRegion(spb6_ctx* c_, int cat_) : c(c_), cat(cat_) { if (c->timer.on) { a = c->timer.get(); hipEventRecord(a, c->st); } }
-	~TRegion() { if (c->timer.on) { hipEvent_t b = c->timer.get(); hipEventRecord(b, c->st); c->timer.pend.push_back({cat, a, b}); if (c->timer.pend.size() > 512 && !c->timer.hold) c->timer.resolve(); } }
+	TRegion(spb6_ctx* c_, int cat_) : c(c_), cat(cat_) { if (c->timer.want(cat)) { a = c->timer.get(); hipEventRecord(a, c->st); } }
+	~TRegion() { if (a) { hipEvent_t b = c->timer.get(); hipEventRecord(b, c->st); c->timer.pend.push_back({cat, a, b}); if (c->timer.pend.size() > 512 && !c->timer.hold) c->timer.resolve(); } }
 };
 
 // clear the norms and the flags of one evaluation (the stage's admissibility flags stay)
@@ -233,32 +236,45 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, boo
 		int chunk = std::max(2, c->last_krylov + 1), launched = 0;
 		// with timing on, each launched iteration gets its own event pair; once the solve's iteration count is read,
 		// the iterations taken go to T_KRYLOV and those launched past the end to T_KIDLE
+		// (and a speculative trial residual is filed once its gate is known: T_RESIDUAL when it ran, T_KIDLE otherwise)
 		struct IterEv { int it; hipEvent_t a, b; };
 		std::vector<IterEv> iev;
+		const bool iter_ev = c->timer.want(T_KRYLOV) || c->timer.want(T_KIDLE);
+		const bool spec_ev = c->timer.want(T_RESIDUAL) || c->timer.want(T_KIDLE);
 		c->timer.hold = c->timer.on;
 		for (;;) {
 			for (int k = 0; k < chunk && launched <= maxit; k++, launched++) {
 				hipEvent_t a = nullptr, b = nullptr;
-				if (c->timer.on) { a = c->timer.get(); hipEventRecord(a, c->st); }
+				if (iter_ev) { a = c->timer.get(); hipEventRecord(a, c->st); }
 				launch_krylov_iter32(g, B, c->st, launched, tin, maxit);
-				if (c->timer.on) { b = c->timer.get(); hipEventRecord(b, c->st); iev.push_back({launched, a, b}); }
+				if (iter_ev) { b = c->timer.get(); hipEventRecord(b, c->st); iev.push_back({launched, a, b}); }
 			}
 			{
 				TRegion tr(c, T_KAUX);
 				launch_defect_x(g, c->J, c->rhs, c->dP, c->x32, c->kr, c->part2, c->part_stride, c->dsum, c->Pst, c->sc, c->st);
 			}
+			hipEvent_t sa = nullptr, sb = nullptr;
 			if (Dspec) {
 				{ TRegion tr(c, T_KAUX); launch_spec_commit(n, c->x32, c->dP, c->ksd, c->dsum, eta, &c->mail->spec, &c->nrm[0], c->flags, c->st); }
-				{ TRegion tr(c, T_RESIDUAL); launch_residual(*Dspec, c->Pst, c->dP, 1.0, c->Pn, c->Gn, &c->nrm[0], FLAG_RES, c->st, &c->mail->spec); }
+				if (spec_ev) { sa = c->timer.get(); hipEventRecord(sa, c->st); }
+				launch_residual(*Dspec, c->Pst, c->dP, 1.0, c->Pn, c->Gn, &c->nrm[0], FLAG_RES, c->st, &c->mail->spec);
+				if (spec_ev) { sb = c->timer.get(); hipEventRecord(sb, c->st); }
 			}
 			HIPCHK(hipMemcpyAsync(c->hs, c->mail, sizeof(HostScratch), hipMemcpyDeviceToHost, c->st));
 			HIPCHK(hipStreamSynchronize(c->st));
+			if (sa) {
+				const int cat = c->hs->spec ? T_RESIDUAL : T_KIDLE;
+				if (c->timer.want(cat)) c->timer.pend.push_back({cat, sa, sb});
+			}
 			if (c->hs->ks.done || launched > maxit) break;
 			chunk = 4;
 		}
 		const int it = c->hs->ks.iters;
 		const bool kok = c->hs->ks.done && c->hs->ks.status == 0;
-		for (const IterEv& e : iev) c->timer.pend.push_back({e.it < it ? T_KRYLOV : T_KIDLE, e.a, e.b});
+		for (const IterEv& e : iev) {
+			const int cat = e.it < it ? T_KRYLOV : T_KIDLE;
+			if (c->timer.want(cat)) c->timer.pend.push_back({cat, e.a, e.b});
+		}
 		c->timer.hold = false;
 		c->last_krylov = it; c->krylov_launched += launched; iters += it;
 		const bool closed = Dspec && c->hs->ks.done;   // the device has added x and decided the defect test
@@ -867,6 +883,12 @@ long spb6_get_field(spb6_ctx* c, const char* name, double* out, long capacity) {
 	return cnt;
 }
 
+int spb6_set_timing_mask(spb6_ctx* c, unsigned mask) {
+	if (!c) return SPB6_BAD_ARG;
+	c->timer.resolve();
+	c->timer.mask = mask;
+	return SPB6_OK;
+}
 int spb6_enable_timing(spb6_ctx* c, int on) {
 	if (!c) return SPB6_BAD_ARG;
 	c->timer.resolve();

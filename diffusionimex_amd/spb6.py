@@ -56,7 +56,7 @@ class KernelTimes(C.Structure):
 EXPORTS = ["spb6_create", "spb6_destroy", "spb6_last_error", "spb6_set_state", "spb6_get_state", "spb6_get_pressure",
            "spb6_set_pressure", "spb6_caldt", "spb6_bc_fill", "spb6_explicit_stage", "spb6_implicit_stage",
            "spb6_stage_prep", "spb6_step", "spb6_run", "spb6_get_stats", "spb6_time", "spb6_residual",
-           "spb6_jacobian", "spb6_linsolve", "spb6_get_field", "spb6_enable_timing", "spb6_get_kernel_times",
+           "spb6_jacobian", "spb6_linsolve", "spb6_get_field", "spb6_enable_timing", "spb6_set_timing_mask", "spb6_get_kernel_times",
            "spb6_mlp_create", "spb6_mlp_destroy", "spb6_mlp_nparams", "spb6_mlp_set_params", "spb6_mlp_get_params",
            "spb6_mlp_set_data", "spb6_mlp_train", "spb6_mlp_forward", "spb6_mlp_gradients", "spb6_mlp_infer_bf16",
            "spb6_set_guess_model", "spb6_guess_residual", "spb6_dump_enable", "spb6_dump_fetch",
@@ -100,6 +100,7 @@ def load():
     L.spb6_get_field.restype = C.c_long
     L.spb6_get_field.argtypes = [vp, C.c_char_p, dp, C.c_long]
     L.spb6_enable_timing.argtypes = [vp, C.c_int]
+    L.spb6_set_timing_mask.argtypes = [vp, C.c_uint]
     L.spb6_get_kernel_times.argtypes = [vp, C.POINTER(KernelTimes)]
     fp = C.POINTER(C.c_float)
     L.spb6_mlp_create.restype = vp
@@ -298,7 +299,14 @@ class Stepper:
             raise SPB6Error(f"unknown field {name}")
         return out[:n].copy()
 
-    def enable_timing(self, on=True):
+    # timer categories, in the library's bit order (spb6_set_timing_mask)
+    TIMING_CATEGORIES = ("explicit", "prep", "shock", "residual", "jacobian", "krylov", "final", "other", "krylov_aux", "guess",
+                         "krylov_idle")
+
+    def enable_timing(self, on=True, categories=None):
+        """time the kernel categories (all, or the named ones: the others record no events) from now on, from zero"""
+        mask = 0xFFFFFFFF if categories is None else sum(1 << self.TIMING_CATEGORIES.index(k) for k in categories)
+        self._check(self.L.spb6_set_timing_mask(self.h, mask), "set_timing_mask")
         self._check(self.L.spb6_enable_timing(self.h, int(on)), "enable_timing")
 
     def kernel_times(self):

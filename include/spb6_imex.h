@@ -147,6 +147,10 @@ typedef struct spb6_kernel_times {
     double krylov_idle_ms;       /* FP32 form: time of the iterations launched past a solve's end (krylov_ms: taken ones) */
 } spb6_kernel_times_t;
 int spb6_enable_timing(spb6_ctx* ctx, int on);
+/* categories timed while timing is on (default all): bit k = the k-th category in the order explicit, prep, shock,
+ * residual, jacobian, krylov, final, other, krylov_aux, guess, krylov_idle; the others record no events, so the step
+ * runs with fewer of them (no counterpart in the reference: instrumentation of this library) */
+int spb6_set_timing_mask(spb6_ctx* ctx, unsigned mask);
 int spb6_get_kernel_times(spb6_ctx* ctx, spb6_kernel_times_t* t);
 
 /* ---- learned Newton guess (no counterpart in the reference; the reference's guess is the previous stage's pressure,
