@@ -497,11 +497,19 @@ __device__ __forceinline__ void put_partial(const KP& P, int slot, double v, dou
 	const double tot = block_sum256(v, sh);
 	if (threadIdx.x == 0) P.part[slot * P.stride + kblk()] = tot;
 }
-// grid total of one slot written by the previous pass, in the fixed order (identical in every block)
+// grid total of one slot written by the previous pass, in the fixed order (identical in every block): each thread adds
+// its slots c = tid, tid + 256, ... in that order, the loads of eight of them issued before their additions
 __device__ __forceinline__ double grid_total(const KP& P, int slot, double* sh) {
 	const double* q = P.part + slot * P.stride;
+	constexpr int U = 8;
 	double s = 0.0;
-	for (int c = threadIdx.x; c < P.nb; c += 256) s += q[c];
+	for (int c0 = threadIdx.x; c0 < P.nb; c0 += 256 * U) {
+		double a[U];
+		#pragma unroll
+		for (int u = 0; u < U; u++) { const int c = c0 + u * 256; a[u] = c < P.nb ? q[c] : 0.0; }
+		#pragma unroll
+		for (int u = 0; u < U; u++) if (c0 + u * 256 < P.nb) s += a[u];
+	}
 	return block_sum256(s, sh);
 }
 // rows per register chunk of the vector passes and of the product passes: FP32 passes gain from several rows' loads in
@@ -696,7 +704,8 @@ __global__ void __launch_bounds__(256) kb_s(Geo g, KV<T> V, KS* ks, KP P, int it
 			if constexpr (RB) V.sh[r] = rb_row(g, V, r, pk, sv, [&](int c) { return V.r[c] - alpha * V.v[c]; });
 		}
 	}
-	put_partial(P, SL_SS, sss, sh); put_partial(P, SL_SSUM, ssum, sh);
+	put_partial(P, SL_SS, sss, sh);
+	if constexpr (!RB) put_partial(P, SL_SSUM, ssum, sh);   // sum(s) feeds the coarse-space constant only
 }
 // I4: early exit if |s| <= tol |b| (x += alpha M^-1 p, PC:751); otherwise cc_s = sum(s) Einv, t = A M^-1 s; partials of
 // t.s and t.t
@@ -714,7 +723,7 @@ __global__ void __launch_bounds__(256) kb_t(Geo g, KV<T> V, KS* ks, KP P, int it
 		const RowPos pk = pos; pos = rowpos_step(g, pos);
 		const int r = krow(IT, k); if (k < IT && r < n) { sp_load(g, V.J, z, r, RB ? pk : rowpos(g, r), !RB, q[k]); if constexpr (RB) sv[k] = V.s[r]; }
 	}
-	const double s2 = grid_total(P, SL_SS, sh), Ss = grid_total(P, SL_SSUM, sh);
+	const double s2 = grid_total(P, SL_SS, sh), Ss = RB ? 0.0 : grid_total(P, SL_SSUM, sh);
 	const T alpha = (T)ks->alpha[nxt];
 	const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
 	if ((T)sqrt((double)(T)s2) <= (T)tol * (T)ks->bn) {
