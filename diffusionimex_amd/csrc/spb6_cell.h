@@ -516,10 +516,11 @@ SPB_HD void cell_assemble(double P_REF, double rx, double ry, bool twoD, const d
 	X[1] = 1.0 - X[0];
 	X[0] *= 1.0 - s; X[1] *= 1.0 - s;
 }
-// G = (pbar_cl - P_REF) - Pc of one cell (cell_residual, PC:640-679), faces from face_at as in cell_assemble
+// G = (pbar_cl - P_REF) - Pc of one cell (cell_residual, PC:640-679), faces from face_at as in cell_assemble; out (if
+// given) receives the closure's state and, with want_sig, its round-off bound sigma
 template <class FaceAt>
 SPB_HD bool cell_res(const Phys& P, double P_REF, double rx, double ry, bool twoD, const double xs[NV], double Q0, double Q1,
-                     FaceAt face_at, double Pc, double PW, double PE, double PS, double PN, double& G, ResOut* out) {
+                     FaceAt face_at, double Pc, double PW, double PE, double PS, double PN, double& G, ResOut* out, bool want_sig = true) {
 	double X[6], s, mx, my, E;
 	cell_assemble(P_REF, rx, ry, twoD, xs, face_at, Pc, PW, PE, PS, PN, X, s, mx, my, E);
 	const double rho = X[2] + X[3];
@@ -528,7 +529,7 @@ SPB_HD bool cell_res(const Phys& P, double P_REF, double rx, double ry, bool two
 	in.s = s; in.rhoeC = E - 0.5 * (mx * mx + my * my) / rho; in.pguess = P_REF + Pc; in.Q0 = Q0; in.Q1 = Q1;
 	ClOut o; if (!closure_p(P, in, o)) return false;
 	G = (o.pbar - P_REF) - Pc;
-	if (out) { out->a0 = o.a0; out->a1 = o.a1; out->m0 = X[2]; out->m1 = X[3]; out->e0 = o.e0; out->e1 = o.e1; out->mx = mx; out->my = my; out->E = E; out->sig = closure_roundoff(P, in, o.pbar); }
+	if (out) { out->a0 = o.a0; out->a1 = o.a1; out->m0 = X[2]; out->m1 = X[3]; out->e0 = o.e0; out->e1 = o.e1; out->mx = mx; out->my = my; out->E = E; out->sig = want_sig ? closure_roundoff(P, in, o.pbar) : 0.0; }
 	return isfin(G);
 }
 // the same on a cell whose data is already loaded (the finite-difference Jacobian evaluates one cell five times)
@@ -703,7 +704,8 @@ __device__ __forceinline__ bool final_at(const Dev& D, const double* __restrict_
 	double xs[NV], Q0, Q1, Kw; F.cell(xs, Q0, Q1, Kw);
 	const bool twoD = g.twoD;
 	double gg; ResOut o;
-	if (!cell_res(D.ph, D.P_REF, D.rx, D.ry, twoD, xs, Q0, Q1, F, P[q], P[b.W], P[b.E], twoD ? P[b.S] : 0.0, twoD ? P[b.N] : 0.0, gg, &o)) return false;
+	if (!cell_res(D.ph, D.P_REF, D.rx, D.ry, twoD, xs, Q0, Q1, F, P[q], P[b.W], P[b.E], twoD ? P[b.S] : 0.0, twoD ? P[b.N] : 0.0, gg, &o, false))
+		return false;
 	const double u[NV] = {o.a0, o.m0, o.m1, o.mx, o.my, o.e0, o.e1, o.E};
 	#pragma unroll
 	for (int k = 0; k < NV; k++) { st_out<NT>(D.Ust + k * NC + c, u[k]); st_out<NT>(H + k * NC + c, (u[k] - Qs[k * NC + c]) / adt); }
