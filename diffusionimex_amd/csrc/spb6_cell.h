@@ -339,19 +339,28 @@ __device__ __forceinline__ void prep_face_at(const Geo& g, int order, int limite
 __device__ __forceinline__ bool shock_inputs(const Geo& g, const Phys& P, const double* __restrict__ Xs, int q, double Y[2], double vpre[2],
                                              double& plo, double& phi, double& rho) {
 	const int NC = g.NC, c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
+	// the compressibility test first: the divergence takes only the four neighbours' velocities (stencil_state's
+	// st[5 + d], the same operations in the same order), so a cell that fails it skips the five mixture pressures
+	double div = 0.0;
+	for (int d = 0; d < (g.twoD ? 2 : 1); d++) for (int s = -1; s <= 1; s += 2) {
+		const int cc = c + (d == 0 ? s : s * g.NXT);
+		const double r = Xs[IM1 * NC + cc] + Xs[IM2 * NC + cc];
+		const double v = r > 0 ? Xs[(d == 0 ? IMU : IMV) * NC + cc] / r : 0.0;
+		if (s == 1) div += v; else div -= v;
+	}
+	if (!(div < 0.0)) return false;
 	auto load = [&](int cc, double* u) { for (int k = 0; k < NV; k++) u[k] = Xs[k * NC + cc]; };
-	double u[NV], c0[7], st[7], lo[7] = {0, 0, 0, 0, 0, 0, 0}, div = 0.0;
+	double u[NV], c0[7], st[7], lo[7] = {0, 0, 0, 0, 0, 0, 0};
 	plo = 1e300; phi = -1e300;
 	load(c, u); stencil_state(P, u, c0);
 	for (int d = 0; d < (g.twoD ? 2 : 1); d++) for (int s = -1; s <= 1; s++) {
 		if (d == 1 && s == 0) continue;
 		load(c + (d == 0 ? s : s * g.NXT), u); stencil_state(P, u, st);
-		if (s == 1) div += st[5 + d]; else if (s == -1) div -= st[5 + d];
 		if (!isfin(st[4])) continue;
 		if (st[4] < plo) { plo = st[4]; for (int n = 0; n < 7; n++) lo[n] = st[n]; }
 		if (st[4] > phi) phi = st[4];
 	}
-	if (!((div < 0.0) && (phi > plo))) return false;
+	if (!(phi > plo)) return false;
 	rho = c0[2] + c0[3];
 	if (!(rho > 0)) return false;
 	for (int k = 0; k < 2; k++) {
