@@ -976,23 +976,49 @@ void launch_defect_x(const Geo& g, const double* J, const double* b, const doubl
 }
 void launch_add_f32(long n, const float* d, double* x, hipStream_t st) { hipLaunchKernelGGL(k_add_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, d, x); }
 
-// precision conversions around the FP32 Krylov solve (the Newton system stays FP64)
+// precision conversions around the FP32 Krylov solve (the Newton system stays FP64). Two elements per thread (16-byte
+// FP64 accesses) where the host finds the arrays aligned for it, the odd last element by the thread after the pairs.
 __global__ void __launch_bounds__(256) k_to_f32(long n, const double* __restrict__ a, float* __restrict__ b) {
 	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) b[i] = (float)a[i];
+}
+__global__ void __launch_bounds__(256) k_to_f32x2(long n, const double* __restrict__ a, float* __restrict__ b) {
+	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x, h = n >> 1;
+	if (i < h) { const double2 v = reinterpret_cast<const double2*>(a)[i]; reinterpret_cast<float2*>(b)[i] = make_float2((float)v.x, (float)v.y); }
+	else if (i == h && (n & 1)) b[n - 1] = (float)a[n - 1];
 }
 // b = (float) a and z = 0 (the FP32 right-hand side of a Newton step's first sweep and its cleared FP64 step)
 __global__ void __launch_bounds__(256) k_to_f32_clear(long n, const double* __restrict__ a, float* __restrict__ b, double* __restrict__ z) {
 	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) { b[i] = (float)a[i]; z[i] = 0.0; }
 }
+__global__ void __launch_bounds__(256) k_to_f32_clear_x2(long n, const double* __restrict__ a, float* __restrict__ b, double* __restrict__ z) {
+	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x, h = n >> 1;
+	if (i < h) {
+		const double2 v = reinterpret_cast<const double2*>(a)[i];
+		reinterpret_cast<float2*>(b)[i] = make_float2((float)v.x, (float)v.y);
+		reinterpret_cast<double2*>(z)[i] = make_double2(0.0, 0.0);
+	}
+	else if (i == h && (n & 1)) { b[n - 1] = (float)a[n - 1]; z[n - 1] = 0.0; }
+}
 __global__ void __launch_bounds__(256) k_to_f64(long n, const float* __restrict__ a, double* __restrict__ b) {
 	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) b[i] = (double)a[i];
 }
-void launch_to_f32(long n, const double* a, float* b, hipStream_t st) { hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b); }
+static bool host_aligned(const void* p, unsigned long long al) { return (reinterpret_cast<unsigned long long>(p) & (al - 1)) == 0; }
+void launch_to_f32(long n, const double* a, float* b, hipStream_t st) {
+	if (host_aligned(a, 16) && host_aligned(b, 8)) {
+		const long t = (n >> 1) + (n & 1);
+		hipLaunchKernelGGL(k_to_f32x2, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, st, n, a, b);
+	}
+	else hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b);
+}
 void launch_to_f32_clear(long n, const double* a, float* b, double* z, hipStream_t st) {
-	hipLaunchKernelGGL(k_to_f32_clear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b, z);
+	if (host_aligned(a, 16) && host_aligned(b, 8) && host_aligned(z, 16)) {
+		const long t = (n >> 1) + (n & 1);
+		hipLaunchKernelGGL(k_to_f32_clear_x2, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, st, n, a, b, z);
+	}
+	else hipLaunchKernelGGL(k_to_f32_clear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b, z);
 }
 void launch_to_f64(long n, const float* a, double* b, hipStream_t st) { hipLaunchKernelGGL(k_to_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b); }
 
