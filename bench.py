@@ -112,10 +112,11 @@ def run_form(a, form, keep_state=False):
 
 
 # HBM bytes per launch of the head form's candidate dominant kernels at 2048x1024, measured by rocprofv3 --pmc FETCH_SIZE
-# and --pmc WRITE_SIZE (active dispatches), calibrated on the FP32 conversion kernel (known bytes: FETCH_SIZE reads x 2.00,
-# WRITE_SIZE x 1.00): one FP32 BiCGStab iteration (five passes) and one residual evaluation; commands and per-pass counts
-# in profiles/krylov_fp32_iteration_2048x1024_traffic.md and profiles/residual_2048x1024_traffic.md
-TRAFFIC = {("krylov_iter", True, 2048, 1024): 342980352, ("residual", True, 2048, 1024): 773206656}
+# and --pmc WRITE_SIZE, calibrated on the FP32 conversion kernel (known bytes: FETCH_SIZE reads x 2.00, WRITE_SIZE
+# x 1.00): one FP32 BiCGStab iteration (the five passes' counts over the run divided by the iterations taken) and one
+# residual evaluation writing its trial pressure (median active dispatch); commands and per-pass counts in
+# profiles/krylov_fp32_iteration_2048x1024_traffic.md and profiles/residual_2048x1024_traffic.md
+TRAFFIC = {("krylov_iter", True, 2048, 1024): 327036314, ("residual", True, 2048, 1024): 782086963}
 
 
 # timer categories (Stepper.TIMING_CATEGORIES) behind each roofline category
