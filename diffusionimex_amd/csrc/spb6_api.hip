@@ -444,10 +444,12 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 		MlpDev W{M->P, M->W1s, M->W1f, M->W2p, M->mub1, M->b3, M->mu_t, M->sd_t};
 		{
 			TRegion tr(c, T_GUESS);
-			mlp_feat_cell(D, c->Pplain, c->F, c->st);
-			// the shock budget's list is free again by now: it carries the active tiles
+			if (dumping) mlp_feat_cell(D, c->Pplain, c->F, c->st);   // the dump takes the features of every cell
+			// the shock budget's list is free again by now: it carries the active tiles; without a dump the features are
+			// computed in mlp_infer, on the listed tiles' stencils only
 			mlp_infer(&D, c->F, adt, (int)n, nullptr, W, c->Pplain, c->Gplain, 0.0, c->Pn, nullptr, c->st, c->slist, c->scount,
-			          &c->nrm[0], &c->flags[FLAG_RES], &c->mail->gthr, c->Gn, c->rlist, c->rcount, &c->nrm[128], &c->flags[FLAG_RES2]);
+			          &c->nrm[0], &c->flags[FLAG_RES], &c->mail->gthr, c->Gn, c->rlist, c->rcount, &c->nrm[128], &c->flags[FLAG_RES2],
+			          !dumping);
 			// the learned guess's residual: the plain one where no pressure it reads has changed (set above), evaluated
 			// on the listed cells
 			launch_residual_list(D, c->Pn, c->rlist, c->rcount, c->Gn, &c->nrm[128], FLAG_RES2, 4 * c->cus, c->st);
@@ -782,8 +784,8 @@ int spb6_residual(spb6_ctx* c, const double* P, double* G) {
 	int bad = 0; for (size_t q = 0; q < n; q++) if (G[q] != G[q]) bad++;
 	return bad;
 }
-int spb6_guess_residual(spb6_ctx* c, const double* P, double* Pg, double* G, double* G_full, double* norms) {
-	if (!c || !P || !Pg || !G || !G_full || !norms || !c->model) return -1;
+int spb6_guess_residual(spb6_ctx* c, const double* P, double* Pg, double* G, double* G_full, double* norms, double* Pg_full) {
+	if (!c || !P || !Pg || !G || !G_full || !norms || !Pg_full || !c->model) return -1;
 	hipSetDevice(c->device);
 	const size_t n = c->g.n;
 	if (hipMemcpyAsync(c->Pplain, P, n * sizeof(double), hipMemcpyHostToDevice, c->st) != hipSuccess) return -1;
@@ -793,9 +795,14 @@ int spb6_guess_residual(spb6_ctx* c, const double* P, double* Pg, double* G, dou
 	launch_residual(D, c->Pplain, nullptr, 0.0, nullptr, c->Gplain, &c->nrm[0], FLAG_RES, c->st);
 	const spb6_mlp* M = c->model;
 	MlpDev W{M->P, M->W1s, M->W1f, M->W2p, M->mub1, M->b3, M->mu_t, M->sd_t};
+	// the guess from the features of every cell, then (features cleared to NaN) as a stage computes it
 	mlp_feat_cell(D, c->Pplain, c->F, c->st);
 	mlp_infer(&D, c->F, c->tau_last, (int)n, nullptr, W, c->Pplain, c->Gplain, 0.0, c->Pn, nullptr, c->st, c->slist, c->scount,
 	          &c->nrm[0], &c->flags[FLAG_RES], &c->mail->gthr, c->Gn, c->rlist, c->rcount, &c->nrm[128], &c->flags[FLAG_RES2]);
+	if (hipMemcpyAsync(Pg_full, c->Pn, n * sizeof(double), hipMemcpyDeviceToHost, c->st) != hipSuccess) return -1;
+	if (hipMemsetAsync(c->F, 0xFF, (size_t)7 * c->g.NC * sizeof(float), c->st) != hipSuccess) return -1;
+	mlp_infer(&D, c->F, c->tau_last, (int)n, nullptr, W, c->Pplain, c->Gplain, 0.0, c->Pn, nullptr, c->st, c->slist, c->scount,
+	          &c->nrm[0], &c->flags[FLAG_RES], &c->mail->gthr, c->Gn, c->rlist, c->rcount, &c->nrm[128], &c->flags[FLAG_RES2], true);
 	launch_residual_list(D, c->Pn, c->rlist, c->rcount, c->Gn, &c->nrm[128], FLAG_RES2, 4 * c->cus, c->st);
 	// the full evaluation at the guess (norm slots 192..)
 	launch_residual(D, c->Pn, nullptr, 0.0, nullptr, c->kr, &c->nrm[192], FLAG_FLOOR, c->st);

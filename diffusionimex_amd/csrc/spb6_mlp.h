@@ -24,13 +24,15 @@ void mlp_dump(const Dev& D, const float* F, double tau, const double* Pconv, con
 void mlp_pack(const float* params, const float* mean, const float* istd, __bf16* W1s, __bf16* W1f, __bf16* W2p, float* mub1, hipStream_t st);
 // the learned guess into Pout (D given) or the network's outputs for standardised rows Xin into yout; with an activity
 // mask (Gplain) and list / count (device scratch of nrows / 16 + 1 ints), the active tiles are listed first
-void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
+void mlp_infer(const Dev* D, float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
                const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st, int* list = nullptr, int* count = nullptr,
                const unsigned long long* nrm = nullptr, const int* flag = nullptr, double* thr_dev = nullptr, double* Gout = nullptr,
-               int* rlist = nullptr, int* rcount = nullptr, unsigned long long* nrm2 = nullptr, int* flag2 = nullptr);
+               int* rlist = nullptr, int* rcount = nullptr, unsigned long long* nrm2 = nullptr, int* flag2 = nullptr, bool feats = false);
 // (nrm / flag / thr_dev: the activity threshold computed on the device from the plain residual's norm slots and flag;
 // Gout ... flag2, with thr_dev: the guess's residual outside the cells it can change -- the plain residual -- into Gout,
-// its norm into nrm2 and its failure into flag2, and the cells left to evaluate (launch_residual_list) into rlist)
+// its norm into nrm2 and its failure into flag2, and the cells left to evaluate (launch_residual_list) into rlist;
+// feats: the features F computed here (of the listed tiles' stencils where the tiles are staged), instead of by the
+// caller's mlp_feat_cell)
 // a cell takes the network's guess (and is dumped for training) when its plain-guess residual is at least this fraction
 // of the plain residual's max norm: elsewhere the plain guess is kept, so quiescent cells are never perturbed
 constexpr double GUESS_ACTIVE = 1e-3;

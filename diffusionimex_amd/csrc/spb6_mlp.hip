@@ -27,11 +27,9 @@ __device__ __forceinline__ int mapP(const Geo& g, int i, int j) {   // M_P (PC:6
 }
 // per-cell raw fields over the ghosted grid: F[f*NC + c], f = alpha1, log rho, u*, v*, pbar(X*) - P_REF, plain P,
 // log(1 + tau c_frozen / dx)
-__global__ void __launch_bounds__(256) k_feat_cell(Geo g, Phys P, double P_REF, const double* __restrict__ Xs, const double* __restrict__ Pg,
-                                                   const double* __restrict__ ust, const double* __restrict__ vst, const double* __restrict__ cz,
-                                                   float tau_dx, float* __restrict__ F) {
-	const int c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= g.NC) return;
+__device__ __forceinline__ void feat_at(const Geo& g, const Phys& P, double P_REF, const double* __restrict__ Xs, const double* __restrict__ Pg,
+                                        const double* __restrict__ ust, const double* __restrict__ vst, const double* __restrict__ cz,
+                                        float tau_dx, float* __restrict__ F, int c) {
 	const int NC = g.NC, i = c % g.NXT, j = c / g.NXT;
 	const double a1 = Xs[IA1 * NC + c], m1 = Xs[IM1 * NC + c], m2 = Xs[IM2 * NC + c], rho = m1 + m2;
 	const double pg = Pg[mapP(g, i, j)];
@@ -49,6 +47,24 @@ __global__ void __launch_bounds__(256) k_feat_cell(Geo g, Phys P, double P_REF, 
 	F[4 * NC + c] = (float)pb;
 	F[5 * NC + c] = (float)pg;
 	F[6 * NC + c] = log1pf(tau_dx * (float)cf);   // acoustic CFL of the cell, log-compressed
+}
+__global__ void __launch_bounds__(256) k_feat_cell(Geo g, Phys P, double P_REF, const double* __restrict__ Xs, const double* __restrict__ Pg,
+                                                   const double* __restrict__ ust, const double* __restrict__ vst, const double* __restrict__ cz,
+                                                   float tau_dx, float* __restrict__ F) {
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c < g.NC) feat_at(g, P, P_REF, Xs, Pg, ust, vst, cz, tau_dx, F, c);
+}
+// the same fields on the cells the inference stages for the listed tiles only: a tile's 16 cells lie in one grid row
+// (nx a multiple of 16), and their 5x5 stencils span the 20 x 5 ghosted cells from (i0, j0), the tile's first cell in
+// interior coordinates; overlapping tiles write the same values
+__global__ void __launch_bounds__(256) k_feat_tiles(Geo g, Phys P, double P_REF, const double* __restrict__ Xs, const double* __restrict__ Pg,
+                                                    const double* __restrict__ ust, const double* __restrict__ vst, const double* __restrict__ cz,
+                                                    float tau_dx, float* __restrict__ F, const int* __restrict__ tiles, const int* __restrict__ ntl) {
+	const long m = 100L * *ntl;
+	for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (long)gridDim.x * blockDim.x) {
+		const int q0 = tiles[e / 100] * 16, r = (int)(e % 100), dy = r / 20, dx = r - 20 * dy;
+		feat_at(g, P, P_REF, Xs, Pg, ust, vst, cz, tau_dx, F, (q0 % g.nx + dx) + (q0 / g.nx + dy) * g.NXT);
+	}
 }
 struct FeatCtx {
 	Geo g; const float* F; const double* Kw; const double* cz; double tau_dx;
@@ -479,10 +495,10 @@ void mlp_pack(const float* params, const float* mean, const float* istd, __bf16*
 	hipLaunchKernelGGL(k_pack_fold, dim3(1), dim3(MLP_H), 0, st, params, mean, istd, mub1);
 	hipLaunchKernelGGL(k_pack_bf16, dim3(nb(MLP_H * MLP_KPAD)), dim3(256), 0, st, params, istd, (const float*)mub1, W1s, W1f, W2p);
 }
-void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
+void mlp_infer(const Dev* D, float* F, double tau, int nrows, const float* Xin, const MlpDev& W, const double* Pplain,
                const double* Gplain, double thr, double* Pout, float* yout, hipStream_t st, int* list, int* count,
                const unsigned long long* nrm, const int* flag, double* thr_dev, double* Gout, int* rlist, int* rcount, unsigned long long* nrm2,
-               int* flag2) {
+               int* flag2, bool feats) {
 	FeatCtx fx{};
 	if (D) { fx.g = D->g; fx.F = F; fx.Kw = D->Kw; fx.cz = D->cz; fx.tau_dx = tau / D->g.dx; fx.G = Gplain; fx.Pplain = Pplain; fx.P_REF = D->P_REF; fx.thr = thr; }
 	if (D && thr_dev) {   // the threshold from the plain residual's norm on the device
@@ -495,11 +511,19 @@ void mlp_infer(const Dev* D, const float* F, double tau, int nrows, const float*
 	const int tiles = (nrows + 15) / 16;
 	const int blocks = std::max(1, std::min(cus, (tiles + INF_WAVES - 1) / INF_WAVES));
 	const bool listed = D && Gplain && Pout && list && count;   // the guess with an activity mask: listed active tiles
+	// feats: the features are computed here, on the listed tiles' staged stencils only where the inference stages the
+	// tiles (tiles in one grid row, as in k_mlp_infer), else on every cell first
+	const bool staged = D && !Xin && D->g.nx % 16 == 0 && nrows % 16 == 0;
+	if (feats && D && !(listed && staged)) mlp_feat_cell(*D, Pplain, F, st);
 	if (listed) {
 		if (!fx.thr_dev) (void)hipMemsetAsync(count, 0, sizeof(int), st);   // else k_guess_thr has cleared the counters
 		const bool sparse = fx.thr_dev && Gout && rlist && rcount && nrm2 && flag2;
 		hipLaunchKernelGGL(k_guess_tiles, dim3(nb(nrows)), dim3(256), 0, st, fx, nrows, Pplain, Pout, list, count, Gout, sparse ? rlist : nullptr,
 		                   rcount, nrm2, flag2);
+		if (feats && staged)
+			hipLaunchKernelGGL(k_feat_tiles, dim3(4 * cus), dim3(256), 0, st, D->g, D->ph, D->P_REF, (const double*)D->Ust, Pplain,
+			                   (const double*)D->ust, (const double*)D->vst, (const double*)D->cz, (float)(D->tau / D->g.dx), F,
+			                   (const int*)list, (const int*)count);
 	}
 	hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(64 * INF_WAVES), 0, st, fx, nrows, Xin, Xin ? 1 : 0, (const float*)W.mub1,
 	                   Xin ? W.W1s : W.W1f, W.W2p, W.params + MLP_OFF_B2, W.params + MLP_OFF_W3, W.b3, W.mu_t, W.sd_t, Pplain, Pout, yout,

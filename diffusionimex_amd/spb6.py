@@ -114,7 +114,7 @@ def load():
     L.spb6_mlp_gradients.argtypes = [vp, fp, fp, C.c_int, fp, fp]
     L.spb6_mlp_infer_bf16.argtypes = [vp, fp, C.c_int, fp]
     L.spb6_set_guess_model.argtypes = [vp, vp]
-    L.spb6_guess_residual.argtypes = [vp, dp, dp, dp, dp, dp]
+    L.spb6_guess_residual.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.spb6_dump_enable.argtypes = [vp, C.c_long, C.c_int]
     L.spb6_dump_fetch.restype = C.c_long
     L.spb6_dump_fetch.argtypes = [vp, fp, fp, C.c_long]
@@ -264,14 +264,15 @@ class Stepper:
         return G, bad
 
     def guess_residual(self, P):
-        """learned guess at plain guess P after stage_prep: (guess, its residual as a stage computes it, the same residual
-        evaluated on every cell, their two max-norms, the number of cells the first evaluated)"""
+        """learned guess at plain guess P after stage_prep: (the guess as a stage computes it, the same from the features
+        of every cell, its residual as a stage computes it, the same residual evaluated on every cell, their two
+        max-norms, the number of cells the first evaluated)"""
         P = np.ascontiguousarray(P, dtype=np.float64).ravel()
-        Pg, G, Gf, nrm = np.empty_like(P), np.empty_like(P), np.empty_like(P), np.zeros(2)
-        cnt = self.L.spb6_guess_residual(self.h, _dp(P), _dp(Pg), _dp(G), _dp(Gf), _dp(nrm))
+        Pg, Pgf, G, Gf, nrm = np.empty_like(P), np.empty_like(P), np.empty_like(P), np.empty_like(P), np.zeros(2)
+        cnt = self.L.spb6_guess_residual(self.h, _dp(P), _dp(Pg), _dp(G), _dp(Gf), _dp(nrm), _dp(Pgf))
         if cnt < 0:
             raise SPB6Error("guess_residual failed (no network attached?)")
-        return Pg, G, Gf, nrm, cnt
+        return Pg, Pgf, G, Gf, nrm, cnt
 
     def jacobian(self, P, G):
         P = np.ascontiguousarray(P, dtype=np.float64).ravel()

@@ -176,11 +176,12 @@ int spb6_mlp_infer_bf16(spb6_mlp* m, const float* X, int n, float* y);
 /* stepper side: use this network for the learned guess (the network must outlive the context's use of it) */
 int spb6_set_guess_model(spb6_ctx* ctx, spb6_mlp* m);
 /* learned guess at plain guess P after spb6_stage_prep (no counterpart in the reference: the check of the library's
- * learned guess): the guess into Pg, its residual as a stage computes it -- evaluated only on the cells whose residual
- * inputs it changes, the plain residual elsewhere -- into G, the same residual evaluated on every cell into G_full, and
- * the max-norms of the two (PC:770) into norms[2]; returns the number of cells evaluated (-1 on error, or without a
- * network) */
-int spb6_guess_residual(spb6_ctx* ctx, const double* P, double* Pg, double* G, double* G_full, double* norms);
+ * learned guess): the guess as a stage computes it (features of the listed tiles' stencils only) into Pg and the same
+ * guess from the features of every cell into Pg_full; the guess's residual as a stage computes it -- evaluated only on
+ * the cells whose residual inputs it changes, the plain residual elsewhere -- into G, the same residual evaluated on
+ * every cell into G_full, and the max-norms of the two (PC:770) into norms[2]; returns the number of cells evaluated
+ * (-1 on error, or without a network) */
+int spb6_guess_residual(spb6_ctx* ctx, const double* P, double* Pg, double* G, double* G_full, double* norms, double* Pg_full);
 /* record raw (features, target) pairs of every stage's cells q = off, off + stride, ... into a device buffer of
  * `capacity` rows (capacity 0 disables); spb6_dump_fetch copies up to max_rows rows to the host and empties it */
 int spb6_dump_enable(spb6_ctx* ctx, long capacity, int stride);
