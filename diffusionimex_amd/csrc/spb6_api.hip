@@ -437,9 +437,16 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 		// from that residual's norm) and the learned guess's residual (norm slots 128.., FLAG_RES2), all read back in one
 		// synchronisation with the stage's flags
 		rc = reset_scalars(c); if (rc) return rc;
-		{ TRegion tr(c, T_RESIDUAL); launch_residual(D, c->Pst, nullptr, 0.0, nullptr, c->G, &c->nrm[0], FLAG_RES, c->st); }
-		HIPCHK(hipMemcpyAsync(c->Pplain, c->Pst, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
-		HIPCHK(hipMemcpyAsync(c->Gplain, c->G, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+		// the plain residual and its pressure straight into Gplain / Pplain (G takes that buffer after the read-back); a
+		// training dump, which reads Gplain after Newton, keeps copies instead
+		if (dumping) {
+			{ TRegion tr(c, T_RESIDUAL); launch_residual(D, c->Pst, nullptr, 0.0, nullptr, c->G, &c->nrm[0], FLAG_RES, c->st); }
+			HIPCHK(hipMemcpyAsync(c->Pplain, c->Pst, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+			HIPCHK(hipMemcpyAsync(c->Gplain, c->G, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+		} else {
+			TRegion tr(c, T_RESIDUAL);
+			launch_residual(D, c->Pst, nullptr, 0.0, c->Pplain, c->Gplain, &c->nrm[0], FLAG_RES, c->st);
+		}
 		const spb6_mlp* M = c->model;
 		MlpDev W{M->P, M->W1s, M->W1f, M->W2p, M->mub1, M->b3, M->mu_t, M->sd_t};
 		{
@@ -455,6 +462,7 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 			launch_residual_list(D, c->Pn, c->rlist, c->rcount, c->Gn, &c->nrm[128], FLAG_RES2, 4 * c->cus, c->st);
 		}
 		rc = fetch(c); if (rc) return rc;
+		if (!dumping) std::swap(c->G, c->Gplain);   // G: the plain guess's residual, as eval_all leaves it
 		ok0 = c->hs->flags[FLAG_RES] == 0; res0 = nrm_val(c, 0);
 		okl = c->hs->flags[FLAG_RES2] == 0; resl = nrm_val(c, 2);
 	} else {
