@@ -76,7 +76,6 @@ void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc,
 void launch_defect_x(const Geo& g, const double* J, const double* b, const double* dP, const float* x, double* r, double* part, int stride,
                      double* sums, const double* P, const double* sc, hipStream_t st);
 void launch_chord_rhs(const Dev& D, const double* P, const double* G, double* sc, double* rhs, hipStream_t st, int* clear = nullptr, int nclear = 0);
-void launch_fdscale(const Dev& D, const double* P, double* sc, hipStream_t st);
 void launch_stepcrit(int n, const double* dP, const double* P, const double* sc, unsigned long long* nrm, hipStream_t st);
 void launch_floor(const Dev& D, const double* P, double tol, hipStream_t st);
 void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, hipStream_t st, int flag_slot = FLAG_FINAL);
@@ -84,11 +83,9 @@ void launch_caldt(const Geo& g, const Phys& P, const double* U, int acoustic, un
 void launch_thomas(int n, const double* J, const double* rhs, double* x, double* cp, double* dp, hipStream_t st);
 void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hipStream_t st);
 void launch_krylov_iter(const Geo& g, const KrylovBufs& B, hipStream_t st, int it, double tol, int maxit);
-void launch_linres(const Geo& g, const double* J, const double* b, const double* x, double* r, double* part, int stride, hipStream_t st);
 void launch_add_f32(long n, const float* d, double* x, hipStream_t st);
 void launch_to_f32(long n, const double* a, float* b, hipStream_t st);
 void launch_to_f32_clear(long n, const double* a, float* b, double* z, hipStream_t st);
-void launch_to_f64(long n, const float* a, double* b, hipStream_t st);
 void launch_krylov_setup32(const Geo& g, const KrylovBufs32& B, const float* b, hipStream_t st);
 void launch_krylov_iter32(const Geo& g, const KrylovBufs32& B, hipStream_t st, int it, double tol, int maxit);
 

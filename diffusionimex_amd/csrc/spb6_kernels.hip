@@ -872,20 +872,10 @@ __global__ void __launch_bounds__(256) kbr_setup2(Geo g, const T* __restrict__ b
 	put_partial(P, SL_RR, srr, sh); put_partial(P, SL_R0R, sr0r, sh);
 }
 
-// FP64 defect of the linear system r = b - J x, with partials of r.r (slot SL_RR) and b.b (slot SL_BB) per block
-__global__ void __launch_bounds__(256) k_linres(Geo g, const double* __restrict__ J, const double* __restrict__ b, const double* __restrict__ x,
-                                               double* __restrict__ r, double* __restrict__ part, int stride) {
-	__shared__ double sh[4];
-	const int n = g.n;
-	const int q = blockIdx.x * 256 + threadIdx.x;
-	double rr = 0.0, bb = 0.0;
-	if (q < n) { const double v = b[q] - spmv_plain(g, J, x, q); r[q] = v; rr = v * v; bb = b[q] * b[q]; }
-	const double t1 = block_sum256(rr, sh), t2 = block_sum256(bb, sh);
-	if (threadIdx.x == 0) { part[blockIdx.x] = t1; part[stride + blockIdx.x] = t2; }
-}
 // FP64 defect r = b - J (dP + x) of the accumulated step plus the running FP32 solution x, without writing dP (the FP32
 // solve may continue), and the Newton step criterion (PC:819-823) of dP + x: block partials of r.r and b.b (slots 0, 1
-// of stride) and block maxima of |dP + x| / (|P| + s) (slot 2); the values equal k_linres and k_stepcrit after k_add_f32.
+// of stride) and block maxima of |dP + x| / (|P| + s) (slot 2); the values equal those of r = b - J dP and k_stepcrit
+// after k_add_f32.
 // IT rows per thread in the Krylov passes' layout (krow: XCD bands, partials by logical block).
 template <int IT>
 __global__ void __launch_bounds__(256) k_linres_x(Geo g, const double* __restrict__ J, const double* __restrict__ b, const double* __restrict__ dP,
@@ -964,9 +954,6 @@ __global__ void __launch_bounds__(256) k_add_f32(long n, const float* __restrict
 	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) x[i] += (double)d[i];
 }
-void launch_linres(const Geo& g, const double* J, const double* b, const double* x, double* r, double* part, int stride, hipStream_t st) {
-	hipLaunchKernelGGL(k_linres, dim3((unsigned)((g.n + 255) / 256)), dim3(256), 0, st, g, J, b, x, r, part, stride);
-}
 void launch_defect_x(const Geo& g, const double* J, const double* b, const double* dP, const float* x, double* r, double* part, int stride,
                      double* sums, const double* P, const double* sc, hipStream_t st) {
 	constexpr int IT = 8;
@@ -1001,10 +988,6 @@ __global__ void __launch_bounds__(256) k_to_f32_clear_x2(long n, const double* _
 	}
 	else if (i == h && (n & 1)) { b[n - 1] = (float)a[n - 1]; z[n - 1] = 0.0; }
 }
-__global__ void __launch_bounds__(256) k_to_f64(long n, const float* __restrict__ a, double* __restrict__ b) {
-	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) b[i] = (double)a[i];
-}
 static bool host_aligned(const void* p, unsigned long long al) { return (reinterpret_cast<unsigned long long>(p) & (al - 1)) == 0; }
 void launch_to_f32(long n, const double* a, float* b, hipStream_t st) {
 	if (host_aligned(a, 16) && host_aligned(b, 8)) {
@@ -1020,7 +1003,6 @@ void launch_to_f32_clear(long n, const double* a, float* b, double* z, hipStream
 	}
 	else hipLaunchKernelGGL(k_to_f32_clear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b, z);
 }
-void launch_to_f64(long n, const float* a, double* b, hipStream_t st) { hipLaunchKernelGGL(k_to_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b); }
 
 // rows per thread of the Krylov passes: about a thousand blocks whatever n (4 .. 32 rows per thread)
 int krylov_items(int n) {
@@ -1161,9 +1143,6 @@ void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc,
 }
 void launch_chord_rhs(const Dev& D, const double* P, const double* G, double* sc, double* rhs, hipStream_t st, int* clear, int nclear) {
 	hipLaunchKernelGGL(k_chord_rhs, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, sc, rhs, clear, nclear);
-}
-void launch_fdscale(const Dev& D, const double* P, double* sc, hipStream_t st) {
-	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc, (int*)nullptr, 0);
 }
 void launch_stepcrit(int n, const double* dP, const double* P, const double* sc, unsigned long long* nrm, hipStream_t st) {
 	hipLaunchKernelGGL(k_stepcrit, dim3(nblk(n)), dim3(256), 0, st, n, dP, P, sc, nrm);
