@@ -74,3 +74,27 @@ def test_fp32_krylov_shock_bubble_within_scheme_spread():
     l1 = field_l1(st.state(), o.state())
     for k in range(8):
         assert l1[k] <= 10 * spread[k] + 1e-12, f"field {k}: {l1[k]:.2e} vs spread {spread[k]:.2e}"
+
+
+def test_timing_mask_times_only_the_named_categories():
+    """The kernel timer records events only for the categories in its mask (bench.py times its value steps with the
+    dominant category alone): a masked-out category reads zero, a named one a positive time and count, and the step
+    results do not depend on what is timed."""
+    from diffusionimex_amd import cases, spb6
+    runs = {}
+    for cats in (None, ("residual",), ("krylov", "krylov_idle")):
+        cfg, U = cases.shock_bubble(128, 64, seed=1, cfl=0.5, krylov_fp32=1, analytic_jacobian=1)
+        s = spb6.Stepper(cfg); s.set_state(U)
+        assert s.run(1.0, 20) == 0
+        s.enable_timing(True, categories=cats)
+        assert s.run(1.0, 3) == 0
+        kt = s.kernel_times()
+        runs[cats] = (kt, s.state().copy())
+        s.close()
+    full, res, kry = runs[None][0], runs[("residual",)][0], runs[("krylov", "krylov_idle")][0]
+    for k in ("explicit_ms", "residual_ms", "jacobian_ms", "krylov_ms", "final_ms"):
+        assert full[k] > 0, k
+    assert res["residual_ms"] > 0 and res["residual_n"] > 0
+    assert res["krylov_ms"] == 0 and res["explicit_ms"] == 0 and res["jacobian_ms"] == 0
+    assert kry["krylov_ms"] > 0 and kry["residual_ms"] == 0 and kry["final_ms"] == 0
+    assert np.array_equal(runs[None][1], runs[("residual",)][1]) and np.array_equal(runs[None][1], runs[("krylov", "krylov_idle")][1])
