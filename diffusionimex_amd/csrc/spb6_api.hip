@@ -218,18 +218,19 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, boo
 	}
 	const double res = std::max(newton_res, 1e-300);
 	const double eta = finish ? 0.1 * c->cfg.newton_tol / res : std::max(0.1 * c->cfg.newton_tol / res, std::min(1e-6, res));
-	{
-		TRegion tr(c, T_KAUX);
-		if (new_J) launch_to_f32(5 * n, c->J, c->J32, c->st);   // a chord step keeps the previous FP32 copy
-		launch_to_f32_clear(n, c->rhs, c->b32, c->dP, c->st);
-	}
 	KrylovBufs32 B{c->x32, c->r32, c->r032, c->p32, c->v32, c->s32, c->t32, c->J32, c->part, c->part_stride, c->ksd, c->ph32, c->sh32, c->dg32, c->pp32};
 	iters = 0; ok = false;
 	const double tin = std::max(0.5 * eta, 5e-7);
 	const int maxit = c->cfg.krylov_iter;
 	const Geo& g = c->g;
 	for (int sweep = 0; sweep < 6; sweep++) {
-		{ TRegion tr(c, T_KAUX); launch_krylov_setup32(g, B, c->b32, c->st); }
+		{
+			TRegion tr(c, T_KAUX);
+			// the first sweep converts the system in its setup's first pass: b and dP = 0 always, J when it is new (a
+			// chord step keeps the previous FP32 copy)
+			if (sweep == 0) launch_krylov_setup32_cvt(g, B, new_J ? c->J : nullptr, c->J32, c->rhs, c->b32, c->dP, c->st);
+			else launch_krylov_setup32(g, B, c->b32, c->st);
+		}
 		// the sweep's BiCGStab passes in chunks sized by the previous solve; behind each chunk, in the same stream, the
 		// FP64 defect and the step criterion of dP + x, so that one synchronisation reads the solve's state, the defect
 		// and the criterion together (a solve that is not done yet continues; dP is only written once it is)

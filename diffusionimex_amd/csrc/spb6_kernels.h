@@ -85,8 +85,11 @@ void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hip
 void launch_krylov_iter(const Geo& g, const KrylovBufs& B, hipStream_t st, int it, double tol, int maxit);
 void launch_add_f32(long n, const float* d, double* x, hipStream_t st);
 void launch_to_f32(long n, const double* a, float* b, hipStream_t st);
-void launch_to_f32_clear(long n, const double* a, float* b, double* z, hipStream_t st);
 void launch_krylov_setup32(const Geo& g, const KrylovBufs32& B, const float* b, hipStream_t st);
+// the red-black FP32 setup of a Newton step's first sweep straight from the FP64 system: b32 = (float) b64, dP = 0, and
+// with J64 (a new Jacobian) J32 = (float) J64, in the setup's first pass
+void launch_krylov_setup32_cvt(const Geo& g, const KrylovBufs32& B, const double* J64, float* J32, const double* b64, float* b32, double* dP,
+                               hipStream_t st);
 void launch_krylov_iter32(const Geo& g, const KrylovBufs32& B, hipStream_t st, int it, double tol, int maxit);
 
 }   // namespace spb

@@ -829,6 +829,36 @@ __global__ void __launch_bounds__(256) kbr_setup1(Geo g, const T* __restrict__ b
 	}
 	put_partial(P, SL_BB, sbb, sh);
 }
+// kbr_setup1 on the FP64 system of a Newton step's first sweep: b32 = (float) b64 and dP = 0, and with J64 (a new
+// Jacobian) J32 = (float) J64 on the row's ne diagonals, the values the separate conversions would write
+template <int IT>
+__global__ void __launch_bounds__(256) kbr_setup1_cvt(Geo g, const double* __restrict__ J64, float* __restrict__ J32, const double* __restrict__ b64,
+                                                      float* __restrict__ b32, double* __restrict__ dP, KV<float> V, KP P) {
+	__shared__ double sh[4];
+	const int n = g.n, ne = g.twoD ? 5 : 3;
+	double sbb = 0.0;
+	#pragma unroll
+	for (int it = 0; it < IT; it++) {
+		const int r = krow(IT, it); if (r >= n) continue;
+		const Nb nb = neighbours(g, r);
+		const int col[5] = {r, nb.W, nb.E, nb.S, nb.N};
+		float jv[5];
+		#pragma unroll
+		for (int k = 0; k < 5; k++) if (k < ne) jv[k] = J64 ? (float)J64[k * n + r] : J32[k * n + r];
+		if (J64) {
+			#pragma unroll
+			for (int k = 0; k < 5; k++) if (k < ne) J32[k * n + r] = jv[k];
+		}
+		float d = jv[0];
+		#pragma unroll
+		for (int k = 1; k < 5; k++) if (k < ne && col[k] == r) d += jv[k];
+		const float di = dinv_v(d), bf = (float)b64[r];
+		b32[r] = bf; dP[r] = 0.0;
+		V.dg[r] = di; V.x[r] = is_red(g, r) ? bf * di : 0.f; V.p[r] = 0.f; V.v[r] = 0.f;
+		sbb += (double)(bf * bf);
+	}
+	put_partial(P, SL_BB, sbb, sh);
+}
 // black rows of y = M^-1 z from the red rows already in y; skipped once the solve is done (unless in the setup)
 template <typename T, int IT>
 __global__ void __launch_bounds__(256) kbr_black(Geo g, const T* __restrict__ z, T* __restrict__ y, const T* __restrict__ J,
@@ -974,20 +1004,6 @@ __global__ void __launch_bounds__(256) k_to_f32x2(long n, const double* __restri
 	if (i < h) { const double2 v = reinterpret_cast<const double2*>(a)[i]; reinterpret_cast<float2*>(b)[i] = make_float2((float)v.x, (float)v.y); }
 	else if (i == h && (n & 1)) b[n - 1] = (float)a[n - 1];
 }
-// b = (float) a and z = 0 (the FP32 right-hand side of a Newton step's first sweep and its cleared FP64 step)
-__global__ void __launch_bounds__(256) k_to_f32_clear(long n, const double* __restrict__ a, float* __restrict__ b, double* __restrict__ z) {
-	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) { b[i] = (float)a[i]; z[i] = 0.0; }
-}
-__global__ void __launch_bounds__(256) k_to_f32_clear_x2(long n, const double* __restrict__ a, float* __restrict__ b, double* __restrict__ z) {
-	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x, h = n >> 1;
-	if (i < h) {
-		const double2 v = reinterpret_cast<const double2*>(a)[i];
-		reinterpret_cast<float2*>(b)[i] = make_float2((float)v.x, (float)v.y);
-		reinterpret_cast<double2*>(z)[i] = make_double2(0.0, 0.0);
-	}
-	else if (i == h && (n & 1)) { b[n - 1] = (float)a[n - 1]; z[n - 1] = 0.0; }
-}
 static bool host_aligned(const void* p, unsigned long long al) { return (reinterpret_cast<unsigned long long>(p) & (al - 1)) == 0; }
 void launch_to_f32(long n, const double* a, float* b, hipStream_t st) {
 	if (host_aligned(a, 16) && host_aligned(b, 8)) {
@@ -995,13 +1011,6 @@ void launch_to_f32(long n, const double* a, float* b, hipStream_t st) {
 		hipLaunchKernelGGL(k_to_f32x2, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, st, n, a, b);
 	}
 	else hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b);
-}
-void launch_to_f32_clear(long n, const double* a, float* b, double* z, hipStream_t st) {
-	if (host_aligned(a, 16) && host_aligned(b, 8) && host_aligned(z, 16)) {
-		const long t = (n >> 1) + (n & 1);
-		hipLaunchKernelGGL(k_to_f32_clear_x2, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, st, n, a, b, z);
-	}
-	else hipLaunchKernelGGL(k_to_f32_clear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, a, b, z);
 }
 
 // rows per thread of the Krylov passes: about a thousand blocks whatever n (4 .. 32 rows per thread)
@@ -1071,6 +1080,18 @@ static void krylov_iter_t(const Geo& g, const KrylovBufsT<T>& B, hipStream_t st,
 void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hipStream_t st) { krylov_setup_t<double>(g, B, b, st); }
 void launch_krylov_iter(const Geo& g, const KrylovBufs& B, hipStream_t st, int it, double tol, int maxit) { krylov_iter_t<double>(g, B, st, it, tol, maxit); }
 void launch_krylov_setup32(const Geo& g, const KrylovBufs32& B, const float* b, hipStream_t st) { krylov_setup_t<float>(g, B, b, st); }
+void launch_krylov_setup32_cvt(const Geo& g, const KrylovBufs32& B, const double* J64, float* J32, const double* b64, float* b32, double* dP,
+                               hipStream_t st) {
+	const int items = items_of<float>(g.n), nb = (g.n + 256 * items - 1) / (256 * items);
+	KP P{B.part, B.stride, nb};
+	KV<float> V{B.x, B.r, B.r0, B.p, B.v, B.s, B.t, B.J, B.ph, B.sh, B.dg, B.p2};
+	with_items(items, [&](auto I) {
+		constexpr int IT = decltype(I)::value;
+		hipLaunchKernelGGL((kbr_setup1_cvt<IT>), dim3(nb), dim3(256), 0, st, g, J64, J32, b64, b32, dP, V, P);
+		hipLaunchKernelGGL((kbr_black<float, IT>), dim3(nb), dim3(256), 0, st, g, (const float*)b32, B.x, B.J, (const float*)B.dg, B.ks, 1);
+		hipLaunchKernelGGL((kbr_setup2<float, IT>), dim3(nb), dim3(256), 0, st, g, (const float*)b32, V, B.ks, P);
+	});
+}
 void launch_krylov_iter32(const Geo& g, const KrylovBufs32& B, hipStream_t st, int it, double tol, int maxit) { krylov_iter_t<float>(g, B, st, it, tol, maxit); }
 
 // ------------------------------------------------------------------------------------------------ host launchers
