@@ -95,9 +95,11 @@ struct spb6_ctx {
 	float* F = nullptr;                 // 7 per-cell feature fields over the ghosted grid
 	double* Pplain = nullptr;           // the plain guess of the current stage
 	double* Gplain = nullptr;           // its residual
-	int *slist = nullptr, *scount = nullptr;   // strong-shock cells of the shock budget's second pass
+	int *slist = nullptr, *scount = nullptr;   // strong-shock cells of the shock budget's second pass (scount[1]: its cursor)
+	double* srec = nullptr;                    // their profile problems (SHOCK_REC x n)
 	int *rlist = nullptr, *rcount = nullptr;   // cells whose residual the learned guess can change
 	int cus = 256;                      // compute units of the device
+	int shock_blocks = 1024;            // resident 256-thread blocks of the strong-shock pass
 	float *dX = nullptr, *dY = nullptr; unsigned long long* dcount = nullptr; long dcap = 0; int dstride = 0; unsigned dseq = 0;
 };
 
@@ -416,7 +418,7 @@ int prep_part(spb6_ctx* c, double adt) {
 	c->tau_last = adt;
 	Dev D = stage_dev(c, adt);
 	{ TRegion tr(c, T_PREP); launch_prep(D, c->cfg.limiter, c->cfg.order, adt, c->Pst, c->st); }
-	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->slist, c->scount, 16 * c->cus, c->st); }
+	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->slist, c->scount, c->srec, c->shock_blocks, c->st); }
 	return SPB6_OK;
 }
 // PressureSubsystem (PC:851-993) after the explicit part; writes the stage state into Ust and H_rks. defer_final
@@ -613,12 +615,14 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	c->nrm = c->mail->nrm; c->flags = c->mail->flags; c->ksd = &c->mail->ks; c->dsum = c->mail->dsum;
 	if (hipHostMalloc((void**)&c->hs, sizeof(HostScratch), hipHostMallocDefault) != hipSuccess) return fail("host alloc");
 	if (dalloc(c, &c->Pplain, n) || dalloc(c, &c->Gplain, n)) return fail("alloc");
-	if (hipMalloc((void**)&c->slist, (n + 1) * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->slist);
+	if (hipMalloc((void**)&c->slist, (n + 2) * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->slist);
+	if (dalloc(c, &c->srec, (size_t)SHOCK_REC * n)) return fail("alloc");
 	c->scount = c->slist + n;
 	if (hipMalloc((void**)&c->rlist, (n + 1) * sizeof(int)) != hipSuccess) return fail("alloc");
 	c->allocs.push_back(c->rlist);
 	c->rcount = c->rlist + n;
 	(void)hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device);
+	c->shock_blocks = c->cus * shock_strong_blocks_per_cu();
 	if (hipMalloc((void**)&c->F, 7 * NC * sizeof(float)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->F);
 	if (hipDeviceSynchronize() != hipSuccess) return fail("sync");
 	return c;

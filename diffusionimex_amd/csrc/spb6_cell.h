@@ -401,6 +401,28 @@ __device__ __forceinline__ void shock_strong_at(const Geo& g, const Phys& P, con
 	}
 	Qph[c] = Q0; Qph[NC + c] = Q1;
 }
+// the grid path's first pass with records: settles every cell without a strong shock, including those whose profile
+// problem has no share (share_setup false), and returns true for a strong-shock cell with its record: the profile
+// problem S, phase 0's Hugoniot volume, rho and the start of the shooting bracket (SHOCK_REC doubles, in this order)
+__device__ __forceinline__ bool shock_classify_rec_at(const Geo& g, const Phys& P, const double* __restrict__ Xs, double* __restrict__ Qph, int q,
+                                                      double rec[SHOCK_REC]) {
+	const int NC = g.NC, c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
+	double Q0 = 0.0, Q1 = 0.0, Y[2], vpre[2], plo, phi, rho, lam[2], qq, K[2], G[2];
+	if (shock_inputs(g, P, Xs, q, Y, vpre, plo, phi, rho)) {
+		const int m = share_pre(P, Y, vpre, plo, phi, K, G, lam, qq);
+		if (m == 2) {
+			Prof S; double vH0, t0;
+			if (share_setup(P, Y, vpre, plo, phi, K, G, S, vH0, t0)) {
+				rec[0] = S.Y0; rec[1] = S.Y1; rec[2] = S.vk00; rec[3] = S.p0; rec[4] = S.j2; rec[5] = S.v0; rec[6] = S.vend;
+				rec[7] = vH0; rec[8] = rho; rec[9] = t0;
+				return true;
+			}
+		}
+		if (m == 1) { Q0 = lam[0] * rho * qq; Q1 = lam[1] * rho * qq; }
+	}
+	Qph[c] = Q0; Qph[NC + c] = Q1;
+	return false;
+}
 // wavefront maximum (xor butterflies)
 __device__ __forceinline__ double wave_max(double v) {
 	#pragma unroll
@@ -420,16 +442,19 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* dst, doubl
 		if (m > 0.0) atomicMax(&dst[blockIdx.x & 63], (unsigned long long)__double_as_longlong(m));
 	}
 }
-// append the cells of this wavefront whose flag is set to list (one atomic per wavefront); the list order is not
-// deterministic but each cell's result does not depend on it
-__device__ __forceinline__ void wave_append(bool flag, int q, int* __restrict__ list, int* __restrict__ count) {
+// append the cells of this wavefront whose flag is set to list (one atomic per wavefront) and return each flagged
+// lane's slot (-1 for the others); the list order is not deterministic but each cell's result does not depend on it
+__device__ __forceinline__ int wave_append(bool flag, int q, int* __restrict__ list, int* __restrict__ count) {
 	const unsigned long long m = __ballot(flag);
-	if (m == 0ull) return;
+	if (m == 0ull) return -1;
 	const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
 	int base = 0;
 	if (lane == leader) base = atomicAdd(count, __popcll(m));
 	base = __shfl(base, leader, 64);
-	if (flag) list[base + __popcll(m & ((1ull << lane) - 1ull))] = q;
+	if (!flag) return -1;
+	const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+	list[slot] = q;
+	return slot;
 }
 
 // ------------------------------------------------------------------------------------------------ cell residual (PC:584-705)

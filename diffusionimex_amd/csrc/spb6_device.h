@@ -179,22 +179,33 @@ SPB_HD double hug_v(const Phys& P, int k, double p0, double p1, double v0) {    
 	const double x0 = p0 + P.pi[k], x1 = p1 + P.pi[k];
 	return P.b[k] + (v0 - P.b[k]) * ((P.g[k] - 1.0) * x1 + (P.g[k] + 1.0) * x0) / ((P.g[k] + 1.0) * x1 + (P.g[k] - 1.0) * x0);
 }
-SPB_HD bool prof_rhs(const Phys& P, const Prof& S, double c, double v, const double* y, double* dy) {   // PHY:302-310
+// right-hand side of the profile ODE in v (PHY:302-310); lr0 = lam_0 / Y_0 and lr1 = lam_1 / Y_1 are the integration's
+// constant quotients (the reference evaluates lam_k / Y_k * (p - P) left to right, so hoisting the quotient keeps every
+// result). Branch-free: the admissibility tests only decide the returned flag (lanes of one wavefront keep together).
+SPB_HD bool prof_rhs(const Phys& P, const Prof& S, double lr0, double lr1, double v, const double* y, double* dy) {
 	const double p = y[0], v1 = y[1], v2 = (v - S.Y0 * v1) / S.Y1, Pr = S.p0 + S.j2 * (S.v0 - v);
-	if (!eos_ok(P, 0, p, v1) || !eos_ok(P, 1, p, v2)) return false;
-	const double lam0 = c, lam1 = 1.0 - c;
+	const bool adm = eos_ok(P, 0, p, v1) && eos_ok(P, 1, p, v2);
 	double num = -1.0, den = 0.0;
 	const double ep0 = (v1 - P.b[0]) / (P.g[0] - 1.0), evp0 = P.g[0] * (p + P.pi[0]) / (P.g[0] - 1.0);
-	const double r0 = lam0 / S.Y0 * (p - Pr);
+	const double r0 = lr0 * (p - Pr);
 	num += S.Y0 * r0 / evp0; den += S.Y0 * ep0 / evp0;
 	const double ep1 = (v2 - P.b[1]) / (P.g[1] - 1.0), evp1 = P.g[1] * (p + P.pi[1]) / (P.g[1] - 1.0);
-	const double r1 = lam1 / S.Y1 * (p - Pr);
+	const double r1 = lr1 * (p - Pr);
 	num += S.Y1 * r1 / evp1; den += S.Y1 * ep1 / evp1;
 	dy[0] = num / den; dy[1] = (r0 - ep0 * dy[0]) / evp0; dy[2] = p - Pr;
-	return isfin(dy[0]) && isfin(dy[1]);
+	return adm && isfin(dy[0]) && isfin(dy[1]);
 }
-// adaptive Cash-Karp RK4(5) integration of the relaxed shock profile (PHY:311-341)
-__device__ __forceinline__ bool prof_integrate(const Phys& P, const Prof& S, double c, double& v1end, double& qend) {
+// adaptive Cash-Karp RK4(5) integration of the relaxed shock profile (PHY:311-341), written as a start and a single
+// pass of the integration loop so that a caller can interleave the passes of many integrations (the strong-shock
+// kernel advances one integration per lane, one pass at a time); prof_integrate runs the passes back to back
+struct ProfRun { double y0, y1, y2, v, h, lr0, lr1; int it; };
+// start the integration at share c (lam = (c, 1 - c))
+SPB_HD void prof_begin(const Prof& S, double c, ProfRun& R) {
+	R.y0 = S.p0; R.y1 = S.vk00; R.y2 = 0.0; R.v = S.v0; R.h = (S.vend - S.v0) / 32.0; R.it = 0;
+	R.lr0 = c / S.Y0; R.lr1 = (1.0 - c) / S.Y1;
+}
+// one pass of the loop body: 0 = not done yet, 1 = reached v_end (v1end = R.y1, qend = R.y2), -1 = failed
+SPB_HD int prof_step(const Phys& P, const Prof& S, ProfRun& R) {
 	constexpr double A1 = 0.2, A2 = 0.3, A3 = 0.6, A4 = 1.0, A5 = 0.875;
 	constexpr double B10 = 0.2;
 	constexpr double B20 = 3.0 / 40, B21 = 9.0 / 40;
@@ -203,53 +214,73 @@ __device__ __forceinline__ bool prof_integrate(const Phys& P, const Prof& S, dou
 	constexpr double B50 = 1631.0 / 55296, B51 = 175.0 / 512, B52 = 575.0 / 13824, B53 = 44275.0 / 110592, B54 = 253.0 / 4096;
 	constexpr double C50 = 37.0 / 378, C52 = 250.0 / 621, C53 = 125.0 / 594, C55 = 512.0 / 1771;
 	constexpr double C40 = 2825.0 / 27648, C42 = 18575.0 / 48384, C43 = 13525.0 / 55296, C44 = 277.0 / 14336, C45 = 0.25;
-	double y[3] = {S.p0, S.vk00, 0.0}, v = S.v0;
+	if (R.it >= 4000) return -1;
+	R.it++;
 	const double L = S.vend - S.v0;
 	const double sc[3] = {fabs(S.p0) + S.j2 * fabs(L), S.vk00, S.j2 * L * L + 1e-300};
-	double h = L / 32.0;
-	for (int it = 0; it < 4000; it++) {
-		const bool last = (L < 0 && v + h <= S.vend) || (L > 0 && v + h >= S.vend);
-		if (last) h = S.vend - v;
-		double k0[3], k1[3], k2[3], k3[3], k4[3], k5[3], yt[3];
-		bool ok = prof_rhs(P, S, c, v + 0.0 * h, y, k0);
-		// stage inputs are accumulated term by term in the same order as the reference's loop over q
-		if (ok) { for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B10 * k0[n]; } ok = prof_rhs(P, S, c, v + A1 * h, yt, k1); }
-		if (ok) { for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B20 * k0[n]; yt[n] += h * B21 * k1[n]; } ok = prof_rhs(P, S, c, v + A2 * h, yt, k2); }
-		if (ok) { for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B30 * k0[n]; yt[n] += h * B31 * k1[n]; yt[n] += h * B32 * k2[n]; } ok = prof_rhs(P, S, c, v + A3 * h, yt, k3); }
-		if (ok) { for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B40 * k0[n]; yt[n] += h * B41 * k1[n]; yt[n] += h * B42 * k2[n]; yt[n] += h * B43 * k3[n]; } ok = prof_rhs(P, S, c, v + A4 * h, yt, k4); }
-		if (ok) { for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B50 * k0[n]; yt[n] += h * B51 * k1[n]; yt[n] += h * B52 * k2[n]; yt[n] += h * B53 * k3[n]; yt[n] += h * B54 * k4[n]; } ok = prof_rhs(P, S, c, v + A5 * h, yt, k5); }
-		double err = 0.0, y5[3];
-		if (ok) for (int n = 0; n < 3; n++) {
-			double d5 = 0, d4 = 0;
-			d5 += C50 * k0[n]; d4 += C40 * k0[n];
-			d5 += 0.0 * k1[n]; d4 += 0.0 * k1[n];
-			d5 += C52 * k2[n]; d4 += C42 * k2[n];
-			d5 += C53 * k3[n]; d4 += C43 * k3[n];
-			d5 += 0.0 * k4[n]; d4 += C44 * k4[n];
-			d5 += C55 * k5[n]; d4 += C45 * k5[n];
-			y5[n] = y[n] + h * d5; err = smax(err, fabs(h * (d5 - d4)) / sc[n]);
-		}
-		if (!ok || !(err <= 1e-8)) {
-			h *= ok ? smax(0.1, 0.9 * ppow(1e-8 / err, 0.25)) : 0.25;
-			if (fabs(h) < 1e-14 * fabs(L)) return false;
-			continue;
-		}
-		for (int n = 0; n < 3; n++) y[n] = y5[n];
-		if (last) { v1end = y[1]; qend = y[2]; return true; }
-		v += h;
-		h *= smin(5.0, 0.9 * ppow(1e-8 / smax(err, 1e-30), 0.2));
+	const double y[3] = {R.y0, R.y1, R.y2}, v = R.v;
+	double h = R.h;
+	const bool last = (L < 0 && v + h <= S.vend) || (L > 0 && v + h >= S.vend);
+	if (last) h = S.vend - v;
+	// The six stages are evaluated unconditionally: where a stage is inadmissible the reference stops evaluating them
+	// and rejects the step, and so does this code (the later stages' values are not used), without a branch per stage.
+	// Stage inputs are accumulated term by term in the reference's order over q. v + 0.0 * h is v (h is finite).
+	const double lr0 = R.lr0, lr1 = R.lr1;
+	double k0[3], k1[3], k2[3], k3[3], k4[3], k5[3], yt[3];
+	bool ok = prof_rhs(P, S, lr0, lr1, v, y, k0);
+	for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B10 * k0[n]; }
+	ok = prof_rhs(P, S, lr0, lr1, v + A1 * h, yt, k1) && ok;
+	for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B20 * k0[n]; yt[n] += h * B21 * k1[n]; }
+	ok = prof_rhs(P, S, lr0, lr1, v + A2 * h, yt, k2) && ok;
+	for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B30 * k0[n]; yt[n] += h * B31 * k1[n]; yt[n] += h * B32 * k2[n]; }
+	ok = prof_rhs(P, S, lr0, lr1, v + A3 * h, yt, k3) && ok;
+	for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B40 * k0[n]; yt[n] += h * B41 * k1[n]; yt[n] += h * B42 * k2[n]; yt[n] += h * B43 * k3[n]; }
+	ok = prof_rhs(P, S, lr0, lr1, v + A4 * h, yt, k4) && ok;
+	for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B50 * k0[n]; yt[n] += h * B51 * k1[n]; yt[n] += h * B52 * k2[n]; yt[n] += h * B53 * k3[n]; yt[n] += h * B54 * k4[n]; }
+	ok = prof_rhs(P, S, lr0, lr1, v + A5 * h, yt, k5) && ok;
+	// fifth- and fourth-order sums in the reference's order; its zero-weight terms (0.0 * k1, 0.0 * k4 in the fifth-order
+	// sum, 0.0 * k1 in the fourth) are left out: where the step is admissible every k is finite, so each such term is a
+	// signed zero, and a sum that starts at +0.0 is never -0.0, so adding a signed zero leaves it unchanged
+	double err = 0.0, y5[3];
+	for (int n = 0; n < 3; n++) {
+		double d5 = 0, d4 = 0;
+		d5 += C50 * k0[n]; d4 += C40 * k0[n];
+		d5 += C52 * k2[n]; d4 += C42 * k2[n];
+		d5 += C53 * k3[n]; d4 += C43 * k3[n];
+		d4 += C44 * k4[n];
+		d5 += C55 * k5[n]; d4 += C45 * k5[n];
+		y5[n] = y[n] + h * d5; err = smax(err, fabs(h * (d5 - d4)) / sc[n]);
 	}
-	return false;
+	const bool acc = ok && (err <= 1e-8);
+	// the step-size factor of both outcomes from one power evaluation (reject: (1e-8 / err)^(1/4), accept:
+	// (1e-8 / max(err, 1e-30))^(1/5)), so that lanes taking either branch share it
+	const double pw = ppow(acc ? 1e-8 / smax(err, 1e-30) : (ok ? 1e-8 / err : 1.0), acc ? 0.2 : (ok ? 0.25 : 0.0));
+	if (!acc) {
+		h *= ok ? smax(0.1, 0.9 * pw) : 0.25;
+		R.h = h;
+		return fabs(h) < 1e-14 * fabs(L) ? -1 : 0;
+	}
+	R.y0 = y5[0]; R.y1 = y5[1]; R.y2 = y5[2];
+	if (last) return 1;
+	R.v = v + h;
+	R.h = h * smin(5.0, 0.9 * pw);
+	return 0;
 }
-__device__ __forceinline__ double share_F(const Phys& P, const Prof& S, double vH0, double vpre0, double t, double& qq) {
-	const double c = 1.0 / (1.0 + pexp(-t)); double v1e;
-	if (!prof_integrate(P, S, c, v1e, qq)) return dnan();
-	return (v1e - vH0) / vpre0;
+SPB_HD bool prof_integrate(const Phys& P, const Prof& S, double c, double& v1end, double& qend) {
+	ProfRun R; prof_begin(S, c, R);
+	for (;;) {
+		const int r = prof_step(P, S, R);
+		if (r < 0) return false;
+		if (r > 0) { v1end = R.y1; qend = R.y2; return true; }
+	}
 }
+// the share's logistic parameter t -> c = 1 / (1 + exp(-t))
+SPB_HD double share_c(double t) { return 1.0 / (1.0 + pexp(-t)); }
 // phase shares lam and mixture dissipation q of a shock from (p0, vpre, Y) to p1 (PHY:343-383, PHO start bracket),
-// in two parts: share_pre computes the phase stiffnesses K, G and settles the cheap cases -- 0: no share (shock_share
-// false), 1: the weak-shock closed form (lam, q set; shock_share true) -- or returns 2 when the strong branch
-// (share_strong: shooting over the adaptive RK45 profile) is needed
+// in three parts: share_pre computes the phase stiffnesses K, G and settles the cheap cases -- 0: no share (shock_share
+// false), 1: the weak-shock closed form (lam, q set; shock_share true) -- or returns 2 when the strong branch is needed;
+// share_setup then builds the profile problem (false: no share), and the shooting over the adaptive RK45 profile runs
+// as the state machine Shoot (shoot_begin, then shoot_next with each F(t) until the phase is SH_DONE or SH_FAIL)
 SPB_HD int share_pre(const Phys& P, const double Y[2], const double vpre[2], double p0, double p1, double K[2], double G[2],
                      double lam[2], double& q) {
 	double Kmin = dinf();
@@ -268,44 +299,121 @@ SPB_HD int share_pre(const Phys& P, const double Y[2], const double vpre[2], dou
 	}
 	return 2;
 }
-__device__ __noinline__ bool share_strong(const Phys& P, const double Y[2], const double vpre[2], double p0, double p1, const double K[2],
-                                          const double G[2], double lam[2], double& q) {
+// the strong branch's profile problem S, phase 0's Hugoniot volume vH0 and the start t0 of the shooting bracket (the
+// weak-shock share's logit, PHO:365-374); false where the reference returns no share before shooting (PHY:359-364)
+SPB_HD bool share_setup(const Phys& P, const double Y[2], const double vpre[2], double p0, double p1, const double K[2], const double G[2],
+                        Prof& S, double& vH0, double& t0) {
 	const double dp = p1 - p0;
-	Prof S; S.p0 = p0; S.Y0 = Y[0]; S.Y1 = Y[1]; S.vk00 = vpre[0];
-	const double vH0 = hug_v(P, 0, p0, p1, vpre[0]), vH1 = hug_v(P, 1, p0, p1, vpre[1]);
+	S.p0 = p0; S.Y0 = Y[0]; S.Y1 = Y[1]; S.vk00 = vpre[0];
+	vH0 = hug_v(P, 0, p0, p1, vpre[0]);
+	const double vH1 = hug_v(P, 1, p0, p1, vpre[1]);
 	if (!isfin(vH0) || !isfin(vH1)) return false;
 	S.v0 = Y[0] * vpre[0] + Y[1] * vpre[1]; S.vend = Y[0] * vH0 + Y[1] * vH1;
 	if (!(S.vend < S.v0)) return false;
 	S.j2 = dp / (S.v0 - S.vend);
-	double t0;
-	{ double ww0 = Y[0] * vpre[0] * G[0] / (K[0] * K[0]), ww1 = Y[1] * vpre[1] * G[1] / (K[1] * K[1]);
-	  const double c0 = smin(1.0 - 1e-12, smax(1e-12, ww0 / (ww0 + ww1)));
-	  t0 = isfin(c0) ? plog(c0 / (1.0 - c0)) : 0.0; }
-	double ta = smax(-40.0, t0 - 1.0), tb = smin(40.0, t0 + 1.0), qa, qb;
-	double fa = share_F(P, S, vH0, vpre[0], ta, qa), fb = share_F(P, S, vH0, vpre[0], tb, qb);
-	if (!isfin(fa) || !isfin(fb)) return false;
-	for (double st = 2.0; fa >= 0 && ta > -40.0; st *= 2.0) { tb = ta; fb = fa; qb = qa; ta = smax(-40.0, ta - st); fa = share_F(P, S, vH0, vpre[0], ta, qa); if (!isfin(fa)) return false; }
-	for (double st = 2.0; fb <= 0 && tb < 40.0; st *= 2.0) { ta = tb; fa = fb; qa = qb; tb = smin(40.0, tb + st); fb = share_F(P, S, vH0, vpre[0], tb, qb); if (!isfin(fb)) return false; }
-	double t, qt;
-	if (fa >= 0) { t = ta; qt = qa; }
-	else if (fb <= 0) { t = tb; qt = qb; }
-	else {
-		int side = 0; t = ta; qt = qa;
-		for (int it = 0; it < 100; it++) {
-			t = (ta * fb - tb * fa) / (fb - fa);
-			if (!(t > ta && t < tb)) t = 0.5 * (ta + tb);
-			const double ft = share_F(P, S, vH0, vpre[0], t, qt);
-			if (!isfin(ft)) return false;
-			if (fabs(ft) < 1e-10 || tb - ta < 1e-10) break;
-			if (ft < 0) { ta = t; fa = ft; if (side == -1) fb *= 0.5; side = -1; }
-			else { tb = t; fb = ft; if (side == 1) fa *= 0.5; side = 1; }
-		}
+	const double ww0 = Y[0] * vpre[0] * G[0] / (K[0] * K[0]), ww1 = Y[1] * vpre[1] * G[1] / (K[1] * K[1]);
+	const double c0 = smin(1.0 - 1e-12, smax(1e-12, ww0 / (ww0 + ww1)));
+	t0 = isfin(c0) ? plog(c0 / (1.0 - c0)) : 0.0;
+	return true;
+}
+// F(t) of the shooting: the relative miss of phase 0's end volume, (v1(v_end; c(t)) - vH0) / vpre0, NaN where the
+// integration fails
+SPB_HD double share_miss(const ProfRun& R, int r, double vH0, double vpre0) { return r > 0 ? (R.y1 - vH0) / vpre0 : dnan(); }
+// Illinois shooting on t = logit(c) (PHY:366-379) with the PHO start bracket t0 -+ 1 expanded by doubling steps
+enum { SH_FA = 0, SH_FB, SH_EXPL, SH_EXPR, SH_ILL, SH_DONE, SH_FAIL };
+constexpr int SHOCK_REC = 10;   // doubles per strong-shock record of the grid path (spb6_cell.h)
+struct Shoot {
+	double ta, tb, fa, fb, qa, qb, st, t, qt;
+	int phase, side, it;
+};
+// returns the first t to evaluate F at
+SPB_HD double shoot_begin(Shoot& H, double t0) {
+	H.ta = smax(-40.0, t0 - 1.0); H.tb = smin(40.0, t0 + 1.0); H.phase = SH_FA; H.side = 0; H.it = 0; H.st = 2.0;
+	H.t = H.ta; H.qt = 0.0; H.fa = H.fb = H.qa = H.qb = 0.0;
+	return H.ta;
+}
+// F at the t asked for last is f with dissipation q: the next t to evaluate (phase < SH_DONE), or the end of the
+// shooting (SH_DONE: the share's parameter H.t and dissipation H.qt; SH_FAIL: no share)
+SPB_HD double shoot_next(Shoot& H, double f, double q) {
+	bool right = false;
+	switch (H.phase) {
+	case SH_FA: H.fa = f; H.qa = q; H.phase = SH_FB; return H.tb;
+	case SH_FB:
+		H.fb = f; H.qb = q;
+		if (!isfin(H.fa) || !isfin(H.fb)) { H.phase = SH_FAIL; return 0.0; }
+		H.st = 2.0;
+		break;
+	case SH_EXPL:
+		H.fa = f; H.qa = q;
+		if (!isfin(H.fa)) { H.phase = SH_FAIL; return 0.0; }
+		H.st *= 2.0;
+		break;
+	case SH_EXPR:
+		H.fb = f; H.qb = q;
+		if (!isfin(H.fb)) { H.phase = SH_FAIL; return 0.0; }
+		H.st *= 2.0;
+		right = true;
+		break;
+	case SH_ILL: {
+		const double ft = f;
+		if (!isfin(ft)) { H.phase = SH_FAIL; return 0.0; }
+		H.qt = q;
+		if (fabs(ft) < 1e-10 || H.tb - H.ta < 1e-10) { H.phase = SH_DONE; return 0.0; }
+		// ft < 0: ta = t, fa = ft, and fb halved if the previous point fell on the same side; else tb = t, fb = ft, fa
+		// halved likewise (written as selects, which keeps the bracket in registers)
+		const bool neg = ft < 0;
+		const double fa = neg ? ft : (H.side == 1 ? H.fa * 0.5 : H.fa);
+		const double fb = neg ? (H.side == -1 ? H.fb * 0.5 : H.fb) : ft;
+		H.ta = neg ? H.t : H.ta; H.tb = neg ? H.tb : H.t; H.fa = fa; H.fb = fb; H.side = neg ? -1 : 1;
+		if (++H.it >= 100) { H.phase = SH_DONE; return 0.0; }
+		H.t = (H.ta * H.fb - H.tb * H.fa) / (H.fb - H.fa);
+		if (!(H.t > H.ta && H.t < H.tb)) H.t = 0.5 * (H.ta + H.tb);
+		return H.t;
 	}
-	const double c = 1.0 / (1.0 + pexp(-t));
-	lam[0] = c; lam[1] = 1.0 - c; q = qt;
+	default: return 0.0;
+	}
+	if (!right) {   // expand the bracket to the left while F(ta) >= 0
+		if (H.fa >= 0 && H.ta > -40.0) {
+			H.tb = H.ta; H.fb = H.fa; H.qb = H.qa; H.ta = smax(-40.0, H.ta - H.st); H.phase = SH_EXPL;
+			return H.ta;
+		}
+		H.st = 2.0;
+	}
+	if (H.fb <= 0 && H.tb < 40.0) {   // then to the right while F(tb) <= 0
+		H.ta = H.tb; H.fa = H.fb; H.qa = H.qb; H.tb = smin(40.0, H.tb + H.st); H.phase = SH_EXPR;
+		return H.tb;
+	}
+	if (H.fa >= 0) { H.t = H.ta; H.qt = H.qa; H.phase = SH_DONE; return 0.0; }
+	if (H.fb <= 0) { H.t = H.tb; H.qt = H.qb; H.phase = SH_DONE; return 0.0; }
+	H.side = 0; H.it = 0;
+	H.t = (H.ta * H.fb - H.tb * H.fa) / (H.fb - H.fa);
+	if (!(H.t > H.ta && H.t < H.tb)) H.t = 0.5 * (H.ta + H.tb);
+	H.phase = SH_ILL;
+	return H.t;
+}
+// the share of a finished shooting (PHY:380-382): lam = (c, 1 - c), q; false without a share
+SPB_HD bool shoot_result(const Shoot& H, double lam[2], double& q) {
+	if (H.phase != SH_DONE) return false;
+	const double c = share_c(H.t);
+	lam[0] = c; lam[1] = 1.0 - c; q = H.qt;
 	return q > 0;
 }
-__device__ __forceinline__ bool shock_share(const Phys& P, const double Y[2], const double vpre[2], double p0, double p1, double lam[2], double& q) {
+// the strong branch in one call: shooting with each F(t) integrated at once
+SPB_HD bool share_strong(const Phys& P, const double Y[2], const double vpre[2], double p0, double p1, const double K[2],
+                         const double G[2], double lam[2], double& q) {
+	Prof S; double vH0, t0;
+	if (!share_setup(P, Y, vpre, p0, p1, K, G, S, vH0, t0)) return false;
+	Shoot H;
+	double t = shoot_begin(H, t0);
+	while (H.phase < SH_DONE) {
+		ProfRun R; prof_begin(S, share_c(t), R);
+		int r;
+		do r = prof_step(P, S, R); while (r == 0);
+		t = shoot_next(H, share_miss(R, r, vH0, vpre[0]), R.y2);
+	}
+	return shoot_result(H, lam, q);
+}
+SPB_HD bool shock_share(const Phys& P, const double Y[2], const double vpre[2], double p0, double p1, double lam[2], double& q) {
 	double K[2], G[2];
 	const int m = share_pre(P, Y, vpre, p0, p1, K, G, lam, q);
 	if (m != 2) return m == 1;

@@ -56,9 +56,12 @@ void launch_explicit(const Dev& D, int limiter, int order, const double* Qe, con
 // keeps only the face donors
 void launch_recon_full(const Dev& D, int limiter, int order, const double* Qe, double* wxL, double* wxR, double* wyL, double* wyR, hipStream_t st);
 void launch_prep(const Dev& D, int limiter, int order, double tau, const double* Pst, hipStream_t st);
-// shock budget in two passes; list holds n ints, count one int (both device scratch); strong_blocks: blocks of 64 for
-// the strong-shock pass (a few per compute unit)
-void launch_shock(const Dev& D, int* list, int* count, int strong_blocks, hipStream_t st);
+// shock budget in two passes; list holds n ints, count two ints (the list's length and the strong pass's cursor), rec
+// SHOCK_REC * n doubles (the strong cells' profile problems), all device scratch; strong_blocks: persistent 256-thread
+// blocks of the strong-shock pass (as many as are resident at once)
+void launch_shock(const Dev& D, int* list, int* count, double* rec, int strong_blocks, hipStream_t st);
+// 256-thread blocks of the strong-shock pass resident on one compute unit at once
+int shock_strong_blocks_per_cu();
 // (gate: launched speculatively, the evaluation runs only where *gate is set)
 void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st,
                      const int* gate = nullptr);

@@ -321,18 +321,88 @@ __global__ void __launch_bounds__(256) k_prep_face(Geo g, int order, int limiter
 	prep_face_at(g, order, limiter, tau, Xs, ust, vst, cz, Pold, phx, phy, ubsx, kpx, wLx, pvx, ubsy, kpy, wLy, pvy, c);
 }
 // shock budget of each interior cell on the +-1 cross stencil of X* (PHY:395-418), in two passes: every cell is
-// classified and all but the strong-shock cells are settled; the strong-shock cells (adaptive RK45 shooting) are then
-// taken from a compact list, so a wavefront's lanes all run the expensive branch instead of waiting on one that does
+// classified and all but the strong-shock cells are settled, each strong-shock cell's profile problem going to a record
+// of the list (SoA, field k of slot i at rec[k * cap + i]); the second pass then runs the strong cells' shootings
 __global__ void __launch_bounds__(256) k_shock_classify(Geo g, Phys P, const double* __restrict__ Xs, double* __restrict__ Qph,
-                                                       int* __restrict__ list, int* __restrict__ count) {
+                                                       int* __restrict__ list, int* __restrict__ count, double* __restrict__ rec, long cap) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	const bool strong = q < g.n && shock_classify_at(g, P, Xs, Qph, q);
-	wave_append(strong, q, list, count);
+	double r[SHOCK_REC];
+	const bool strong = q < g.n && shock_classify_rec_at(g, P, Xs, Qph, q, r);
+	const int slot = wave_append(strong, q, list, count);
+	if (slot >= 0) {
+		#pragma unroll
+		for (int k = 0; k < SHOCK_REC; k++) rec[k * cap + slot] = r[k];
+	}
 }
-__global__ void __launch_bounds__(64) k_shock_strong(Geo g, Phys P, const double* __restrict__ Xs, double* __restrict__ Qph,
-                                                    const int* __restrict__ list, const int* __restrict__ count) {
+// second pass: persistent lanes, each running one strong cell's shooting at a time. A shooting is a sequence of
+// adaptive RK45 profile integrations (share_strong's loop) whose lengths vary several-fold from cell to cell (12 to
+// ~300 RK steps per cell on a cavitating flow), so a lane advances its integration by one step per pass of the loop,
+// moves its shooting on when an integration ends, and takes the next cell when its cell is done: the lanes of a
+// wavefront stay busy until the list is empty instead of waiting for the longest cell of a fixed group. A wavefront
+// takes cells from a chunk of SHOCK_CHUNK consecutive list slots and claims the next chunk with one atomic on the
+// cursor (an atomic per refill, from every wavefront on the one cursor, was measured to serialise: 2.5x slower). Per pass, the lanes that
+// start an integration (a new cell's first, a shooting's next) or finish a cell share one tail (the logistic of the
+// shooting parameter, then the integration's start or the cell's share), so the divergent work per pass stays small.
+// Every cell runs the same operations as share_strong, so the result does not depend on the order or the launch
+// shape. Every lane's work is bounded (4000 steps per integration, 100 Illinois iterations), so every wavefront
+// reaches the exit.
+constexpr int SHOCK_CHUNK = 128;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_shock_strong(Geo g, Phys P, const double* __restrict__ rec, long cap,
+                                                     double* __restrict__ Qph, const int* __restrict__ list, const int* __restrict__ count,
+                                                     int* __restrict__ cursor) {
 	const int m = *count;
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) shock_strong_at(g, P, Xs, Qph, list[i]);
+	const int lane = threadIdx.x & 63;
+	bool active = false, more = true;
+	int slot = 0;
+	int qn = 0, qe = 0;   // the wavefront's chunk of list slots not yet taken, [qn, qe)
+	Prof S; double vH0 = 0.0, rho = 0.0; Shoot H; ProfRun R;
+	for (;;) {
+		bool tail = false;   // this lane starts an integration or finishes its cell in this pass
+		double t = 0.0;
+		if (more) {   // refill the idle lanes (more, qn and qe are wavefront-uniform)
+			const unsigned long long idle = __ballot(!active);
+			if (idle) {
+				if (qn >= qe) {
+					int nb = 0;
+					if (lane == 0) nb = atomicAdd(cursor, SHOCK_CHUNK);
+					nb = __shfl(nb, 0, 64);
+					if (nb >= m) more = false;
+					else { qn = nb; qe = min(m, nb + SHOCK_CHUNK); }
+				}
+				if (qn < qe) {
+					const int avail = qe - qn, rank = __popcll(idle & ((1ull << lane) - 1ull));
+					if (!active && rank < avail) {
+						const int i = qn + rank;
+						S.Y0 = rec[i]; S.Y1 = rec[cap + i]; S.vk00 = rec[2 * cap + i]; S.p0 = rec[3 * cap + i]; S.j2 = rec[4 * cap + i];
+						S.v0 = rec[5 * cap + i]; S.vend = rec[6 * cap + i]; vH0 = rec[7 * cap + i]; rho = rec[8 * cap + i];
+						t = shoot_begin(H, rec[9 * cap + i]);
+						slot = i; active = true; tail = true;
+					}
+					qn += min(__popcll(idle), avail);
+				}
+			}
+		}
+		if (__ballot(active) == 0ull) {
+			if (!more) break;
+			continue;
+		}
+		if (active && !tail) {
+			const int r = prof_step(P, S, R);
+			if (r != 0) { t = shoot_next(H, share_miss(R, r, vH0, S.vk00), R.y2); tail = true; }
+		}
+		if (tail) {
+			const bool fin = H.phase >= SH_DONE;
+			const double c = share_c(fin ? H.t : t);
+			if (!fin) prof_begin(S, c, R);
+			else {   // the share of the finished shooting (shoot_result: lam = (c, 1 - c), q)
+				double Q0 = 0.0, Q1 = 0.0;
+				if (H.phase == SH_DONE && H.qt > 0) { Q0 = c * rho * H.qt; Q1 = (1.0 - c) * rho * H.qt; }
+				const int q = list[slot], cc = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
+				Qph[cc] = Q0; Qph[g.NC + cc] = Q1;
+				active = false;
+			}
+		}
+	}
 }
 
 // ------------------------------------------------------------------------------------------------ cell residual (PC:584-705)
@@ -1138,11 +1208,16 @@ void launch_prep(const Dev& D, int limiter, int order, double tau, const double*
 	hipLaunchKernelGGL(k_prep_face, dim3(nblk(g.NC)), dim3(256), 0, st, g, order, limiter, tau, D.Ust, D.ust, D.vst, D.cz, D.Pold, D.phx, D.phy,
 	                   D.ubsx, D.kpx, D.wLx, D.pvx, D.ubsy, D.kpy, D.wLy, D.pvy);
 }
-void launch_shock(const Dev& D, int* list, int* count, int strong_blocks, hipStream_t st) {
-	(void)hipMemsetAsync(count, 0, sizeof(int), st);
-	hipLaunchKernelGGL(k_shock_classify, dim3(nblk(D.g.n)), dim3(256), 0, st, D.g, D.ph, D.Ust, D.Qph, list, count);
-	hipLaunchKernelGGL(k_shock_strong, dim3(std::max(1, std::min(nblk(D.g.n, 64), strong_blocks))), dim3(64), 0, st, D.g, D.ph,
-	                   (const double*)D.Ust, D.Qph, (const int*)list, (const int*)count);
+int shock_strong_blocks_per_cu() {
+	int b = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(k_shock_strong), 256, 0) != hipSuccess || b < 1) b = 2;
+	return b;
+}
+void launch_shock(const Dev& D, int* list, int* count, double* rec, int strong_blocks, hipStream_t st) {
+	(void)hipMemsetAsync(count, 0, 2 * sizeof(int), st);   // the count and the strong pass's cursor
+	hipLaunchKernelGGL(k_shock_classify, dim3(nblk(D.g.n)), dim3(256), 0, st, D.g, D.ph, D.Ust, D.Qph, list, count, rec, (long)D.g.n);
+	hipLaunchKernelGGL(k_shock_strong, dim3(std::max(1, std::min(nblk(D.g.n), strong_blocks))), dim3(256), 0, st, D.g, D.ph,
+	                   (const double*)rec, (long)D.g.n, D.Qph, (const int*)list, (const int*)count, count + 1);
 }
 void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st,
                      const int* gate) {

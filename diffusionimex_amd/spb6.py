@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libspb6_hip.so")
 BC_TRANSMISSIVE, BC_WALL, BC_PERIODIC = 0, 1, 2
 STATUS = {0: "OK", 1: "RHO_NONPOSITIVE", 2: "STAGE_INADMISSIBLE", 3: "CLOSURE_FAIL", 4: "NEWTON_FAIL",
           5: "KRYLOV_FAIL", 6: "DT_UNDERFLOW", 7: "STEP_FAILED", 8: "BAD_ARG", 9: "HIP_ERROR"}
+STATUS_CODE = {v: k for k, v in STATUS.items()}
 
 
 class Phase(C.Structure):

@@ -221,10 +221,11 @@ class Oracle:
         return np.ctypeslib.as_array(p, shape=(n,)).copy()
 
 
-def case_init(test, nx, **kw):
-    """Reference test case `test` (1D stiffened-gas cases 0, 4, 5, 80-88) on nx cells: (cfg, U[nx, 8], t_end)."""
-    cfg = default_cfg(nx, 1, **kw)
-    U = np.zeros((nx, 8))
+def case_init(test, nx, ny=1, **kw):
+    """Reference test case `test` (stiffened-gas cases 0, 4, 5, 80-88 in 1D, 70 in 2D) on nx x ny cells:
+    (cfg, U[ny * nx, 8], t_end)."""
+    cfg = default_cfg(nx, ny, **kw)
+    U = np.zeros((nx * ny, 8))
     tend = lib().ora_case_init(int(test), C.byref(cfg), _dp(U))
     if tend < 0:
         raise ValueError(f"unknown case {test}")

@@ -1148,11 +1148,15 @@ static bool case_setup(int test, Cfg& C, double& tend) {                        
 	case 81: case 85: set_SG(C, 0, 4.4, 6e8); set_SG(C, 1, 4.4, 6e8); tend = test == 81 ? 1e-4 : 0.15e-3; break;
 	case 86: case 87: set_SG(C, 0, 4.4, 6e8); set_SG(C, 1, 1.4, 0); tend = test == 86 ? 0.25 : 241e-6; break;
 	case 88: set_SG(C, 0, 2.43, 5.3e9); set_SG(C, 1, 1.62, 141e9); tend = 30e-6; break;
+	case 70:   // PHY:998-999: 0.3 x 0.08 m, air / helium ideal gases, walls at the bottom and top, 600 x 160 cells
+		set_SG(C, 0, 1.4, 0); set_SG(C, 1, 1.67, 0); C.x1 = 0.3; C.y1 = 0.08; C.bc[2] = C.bc[3] = BC_WALL; tend = 600e-6; break;
 	default: return false;
 	}
 	return true;
 }
-static void case_state(const Model& M, int test, double x, double* Q) {                           // PHY:1028-1100 (SG 1D cases)
+// sub-cell averaging of the initial state (TestSetup::sub, PHY:898-905): 4 x 4 for case 70, none for the 1D cases
+static int case_sub(int test) { return test == 70 ? 4 : 1; }
+static void case_state(const Model& M, int test, double x, double y, double* Q) {                 // PHY:1028-1100 (SG cases)
 	auto setcell = [&](double a1, double r1, double r2, double u, double v, double p1, double p2) {
 		Prim w; w.a1 = a1; w.r1 = r1; w.r2 = r2; w.u = u; w.v = v; w.p1 = p1; w.p2 = p2; prim2cell(M, w, Q); };
 	switch (test) {
@@ -1174,6 +1178,13 @@ static void case_state(const Model& M, int test, double x, double* Q) {         
 		const bool adv = test == 86, L = x < (adv ? 0.25 : 0.7);
 		setcell(L ? 1 - 1e-7 : 1e-7, 1000.0, adv ? 0.01 : 50.0, adv ? 1.0 : 0.0, 0, adv ? 1e5 : (L ? 1e9 : 1e5), adv ? 1e5 : (L ? 1e9 : 1e5)); break; }
 	case 88: { const double p = x < 0.6 ? 1e10 : 1e5; setcell(0.5954, 1185.0, 3622.0, 0, 0, p, p); break; }
+	case 70: {   // PHY:1042-1048: Mach 1.5 shock in air moving left, its post-shock state right of x = 0.221 m; helium bubble
+		const double g = 1.4, Ms = 1.5, p0 = 101325.0, r0 = 1.29, c0 = std::sqrt(g * p0 / r0);
+		const double p1 = p0 * (1 + 2 * g / (g + 1) * (Ms * Ms - 1)), r1 = r0 * (g + 1) * Ms * Ms / ((g - 1) * Ms * Ms + 2), u1 = -Ms * c0 * (1 - r0 / r1);
+		const bool in = (x - 0.2) * (x - 0.2) + (y - 0.04) * (y - 0.04) < 0.02 * 0.02;
+		if (x > 0.221) setcell(1 - 1e-6, r1, 0.167, u1, 0, p1, p1);
+		else setcell(in ? 1e-6 : 1 - 1e-6, r0, 0.167, 0, 0, p0, p0);
+		break; }
 	}
 }
 static void cell_diag(const Model& M, const double* Q, double* o) {                                 // PHY:1105-1116 (columns p u v rho alpha2 Y2 rho1 rho2 cf cw T1 T2 p1 p2)
@@ -1293,12 +1304,25 @@ const double* ora_field(void* h, const char* name) {
 }
 double ora_pref(void* h) { return ((Ctx*)h)->P_REF; }
 
-// reference test cases (1D SG): fills cfg geometry/EOS and the interior state U[nx*8]; returns t_end (<0: unknown case)
+// reference test cases (SG): fills cfg geometry, boundaries and EOS and the interior state U[(i + j nx) * 8] at the cfg's
+// nx x ny cells (u_initialization, PC:450-465: cell centres, or the average of the conserved variables over SUB x SUB
+// sub-points where the case asks for it); returns t_end (< 0: unknown case)
 double ora_case_init(int test, Cfg* C, double* U) {
 	double tend; if (!case_setup(test, *C, tend)) return -1.0;
 	Model M; for (int k = 0; k < 2; k++) M.PH[k] = Phase{C->ph[k][0], C->ph[k][1], C->ph[k][2], C->ph[k][3], C->ph[k][4], C->ph[k][5]};
-	const double dx = (C->x1 - C->x0) / C->nx;
-	for (int i = 0; i < C->nx; i++) case_state(M, test, C->x0 + (i + 0.5) * dx, &U[(size_t)i * NV]);
+	const double dx = (C->x1 - C->x0) / C->nx, dy = (C->y1 - C->y0) / C->ny;
+	const int sub = case_sub(test);
+	for (int j = 0; j < C->ny; j++) for (int i = 0; i < C->nx; i++) {
+		const double x = C->x0 + (i + 0.5) * dx, y = C->y0 + (j + 0.5) * dy;
+		double* Q = &U[((size_t)i + (size_t)j * C->nx) * NV];
+		if (sub <= 1) { case_state(M, test, x, y, Q); continue; }
+		for (int k = 0; k < NV; k++) Q[k] = 0.0;
+		for (int a = 0; a < sub; a++) for (int b = 0; b < sub; b++) {
+			double Qs[NV];
+			case_state(M, test, x + ((a + 0.5) / sub - 0.5) * dx, y + ((b + 0.5) / sub - 0.5) * dy, Qs);
+			for (int k = 0; k < NV; k++) Q[k] += Qs[k] / (sub * sub);
+		}
+	}
 	return tend;
 }
 // profile columns p u v rho alpha2 Y2 rho1 rho2 cf cw T1 T2 p1 p2 (14 per cell)
