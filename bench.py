@@ -1,15 +1,23 @@
 """Benchmark: 2D IMEX cell-updates/s of the SPB 6+1 semi-implicit time step on one MI355X.
 
-Workload: 2D shock-bubble, 2048 x 1024 cells on 0.2 x 0.1 m (Mach-1.22 shock in gas gamma 1.4 onto a gamma-1.648
-light-gas bubble, reflective top/bottom walls), FP64, material CFL 0.5, LSDIRK2, Newton tol 1e-13, BiCGStab tol 1e-13.
-A "step" is one accepted LSDIRK2 time step (2 stages with all their Newton and BiCGStab work). --warmup steps run first
-and cover the start-up ramp (the first 17 steps of this workload are first-order steps whose dt grows from the acoustic
-step to the material CFL step), then exactly --steps full LSDIRK2 steps are timed; every stepper call returns after its HIP stream
-synchronises, so the timed region is bracketed by device synchronisation on both sides.
+Headline workload: 2D low-Mach cavitation, 4096 x 4096 cells on [0, 1]^2 (water gamma 4.4, p_inf 6e8 Pa, with gas
+volume fraction 1e-2, gamma 1.4; p0 = 1e5 Pa; u = 100 sign(x - 0.5), v = 100 sign(y - 0.5), each with +-1% per-cell
+perturbations from the seed; transmissive boundaries), FP64 state, material CFL 0.5, LSDIRK2, Newton tolerance 1e-13
+and BiCGStab tolerance 1e-13 (the reference defaults), in three forms: plain (the reference's algorithm, FP64
+BiCGStab with the finite-difference Jacobian), mixed (FP32 BiCGStab with FP64 defect correction, analytic Jacobian)
+and learned (mixed + the learned Newton guess). A "step" is one accepted LSDIRK2 time step (2 stages with all their
+Newton and BiCGStab work). --warmup steps run first and cover the start-up ramp (the first 17 steps of this workload
+are first-order steps whose dt grows from the acoustic to the material CFL step), then --breakdown steps with every
+kernel category timed, then exactly --steps full LSDIRK2 steps are timed; every stepper call returns after its HIP
+stream synchronises, so the timed region is bracketed by device synchronisation on both sides, and the state stays
+resident in HBM throughout.
 
-Prints one JSON line with the metric, the roofline of the dominant kernel (algorithmic bytes per launch / average launch
-duration from HIP events on the stepper's stream) and the CPU baseline (the oracle's PC_omp-path restatement on the host
-cores, a bounded sample of the same workload resumed from the GPU's warmed state).
+Prints one JSON line with the metric (value: the learned form's cell-updates/s), the three forms' rates, their per-field
+relative L1 against the plain FP64 form after the same steps, a closed-box conservation check of each form, the
+roofline of the head form's dominant kernel, every category's fraction of its roofline, and the CPU baseline (the
+oracle's PC_omp-path restatement on the host's physical cores, one step of the same grid resumed from the GPU state).
+Legs: the 2048 x 1024 shock-bubble (a window after the shock has reached the bubble), the 4096 x 2048 perturbed
+interface, and the 4096-problem batched 1D sweep.
 """
 import argparse
 import json
@@ -22,79 +30,91 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK = 8.0e12   # MI355X HBM3E, B/s
-# algorithmic bytes per interior cell per launch (FP64, 2D): compulsory reads + writes of each kernel's data flow
+HBM_PEAK = 8.0e12     # MI355X HBM3E, B/s
+FP64_PEAK = 78.6e12   # MI355X FP64, FLOP/s
+# algorithmic bytes per interior cell per launch (FP64, 2D): compulsory reads + writes of the reference's data flow
+# (face donors recomputed from Qe, not stored): explicit stage 232, stage preparation 184, residual 256, residual +
+# Jacobian 304, BiCGStab iteration 296 (FP32 vectors and matrix: 148), final state + stage derivative 376
 BYTES_2D = {"explicit": 232, "prep": 184, "residual": 256, "jacobian": 304, "krylov_iter": 296, "final": 376}
-# one BiCGStab iteration of the FP32 form (red-black sweep preconditioner, five FP32 passes), bytes per row: kb_p 40
-# (r, p, v, 1/d and the four off-diagonals in; new p and M^-1 p out), kb_v 32 (five diagonals, M^-1 p, r0 in; v out),
-# kb_s 36 (r, v, 1/d and the four off-diagonals in; s, M^-1 s out), kb_t 32 (five diagonals, M^-1 s, s in; t out),
-# kb_xr 32 (M^-1 p, M^-1 s, s, t, x, r0 in; x, r out); a black row's red neighbours come through the caches
-BYTES_KRYLOV_FP32 = 172
+BYTES_KRYLOV_FP32 = 148
+# the FP32 form's own iteration (red-black sweep preconditioner, five FP32 passes), bytes per row: kb_p 40 (r, p, v,
+# 1/d and the four off-diagonals in; new p and M^-1 p out), kb_v 32 (five diagonals, M^-1 p, r0 in; v out), kb_s 36
+# (r, v, 1/d and the four off-diagonals in; s, M^-1 s out), kb_t 32 (five diagonals, M^-1 s, s in; t out), kb_xr 32
+# (M^-1 p, M^-1 s, s, t, x, r0 in; x, r out); a black row's red neighbours come through the caches
+BYTES_KRYLOV_FP32_RB = 172
+# FP64 operations of one RK45 step of the shock budget's profile integration (prof_step, divisions counted as one
+# operation): six right-hand sides of 43 (three divisions per phase, v2, num / den, dy1, the pressure line), stage
+# inputs 9 + 18 + 27 + 36 + 45 plus 2 each, the error and fifth-order sums 69, the step-size power 72 (log 36, exp 34,
+# the division and the product), the scales 6, the end test 2 and the step update 3
+SHOCK_RK_FLOPS = 555
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=30, help="timed LSDIRK2 steps of each form of the headline workload")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--nx", type=int, default=2048)
-    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--breakdown", type=int, default=4, help="steps with every kernel category timed, before the timed ones")
+    ap.add_argument("--n", type=int, default=4096, help="headline grid: n x n cells")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: one thread per physical core available (at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU legs; legs default to the 2D one)")
+    ap.add_argument("--profile", action="store_true", help="short run for rocprofv3 (no CPU legs, headline only)")
     ap.add_argument("--legs", default=None,
-                    help="comma list of shock-bubble (the headline 2048x1024 leg), interface, batch; default all three "
-                         "(shock-bubble only with --profile)")
-    ap.add_argument("--interface-steps", type=int, default=6,
-                    help="timed steps of the 4096x2048 perturbed-interface leg, plain vs learned + FP32 Krylov (0: skip)")
-    ap.add_argument("--interface-warmup", type=int, default=20,
-                    help="warm-up steps of the interface leg (its start-up ramp is 15 first-order steps)")
+                    help="comma list of cavitation (the headline), bubble, interface, batch; default all (cavitation "
+                         "only with --profile)")
+    ap.add_argument("--forms", default=None, help="comma list of plain, mixed, learned (default all three)")
+    ap.add_argument("--bubble-warmup", type=int, default=260,
+                    help="warm-up steps of the 2048x1024 shock-bubble leg (the shock reaches the bubble near step 250)")
+    ap.add_argument("--bubble-steps", type=int, default=20)
+    ap.add_argument("--interface-warmup", type=int, default=40)
+    ap.add_argument("--interface-steps", type=int, default=10)
     ap.add_argument("--batch-steps", type=int, default=8, help="timed steps of the batched 1D leg (0: skip the leg)")
     ap.add_argument("--batch-warmup", type=int, default=12)
     ap.add_argument("--batch-cpu-problems", type=int, default=64)
-    ap.add_argument("--forms", default=None,
-                    help="comma list of plain, mixed (FP32 Krylov), learned (MLP guess + FP32 Krylov); default all three "
-                         "when the trained network is present, else plain,mixed")
     return ap.parse_args()
 
 
-def run_form(a, form, keep_state=False):
-    """warm up and time one form of the stepper on the workload; returns (result dict, stepper, cfg)"""
-    from diffusionimex_amd import spb6, cases
-    kw = {}
-    if form in ("mixed", "learned"):
-        kw["krylov_fp32"] = 1
-        kw["analytic_jacobian"] = 1
-    if form == "learned":
-        kw["learned_guess"] = 1
-    cfg, U = cases.shock_bubble(a.nx, a.ny, seed=a.seed, cfl=0.5, **kw)
-    n = a.nx * a.ny
+FORM_KW = {"plain": {}, "mixed": {"krylov_fp32": 1, "analytic_jacobian": 1},
+           "learned": {"krylov_fp32": 1, "analytic_jacobian": 1, "learned_guess": 1}}
+STAT_KEYS = ("steps", "stages", "newton_it", "krylov_it", "linsolves", "rejects", "guess_used", "guess_fallback", "guess_skipped")
+
+
+def make_stepper(make, form):
+    from diffusionimex_amd import spb6
+    cfg, U = make(**FORM_KW[form])
     st = spb6.Stepper(cfg, device=0)
     if form == "learned":
         from diffusionimex_amd import mlp
         mlp.load_into(st)
-    st.set_state(U)
-    rc = st.run(1.0, a.warmup)
+    st.set_state(U.reshape(cfg.ny, cfg.nx, 8))
+    return st, cfg, U
+
+
+def run_form(make, form, warmup, steps, breakdown):
+    """warm up, time `breakdown` steps category by category, then `steps` steps with only the dominant category's
+    launches bracketed by events; returns the result dict and the stepper"""
+    from diffusionimex_amd import spb6
+    st, cfg, U0 = make_stepper(make, form)
+    n = cfg.nx * cfg.ny
+    rc = st.run(1.0, warmup)
     if rc != 0:
-        raise SystemExit(f"{form}: warmup failed: {spb6.STATUS.get(rc, rc)}")
-    keys = ("steps", "stages", "newton_it", "krylov_it", "linsolves", "guess_used", "guess_fallback")
-    # the per-category breakdown: K steps with every kernel category bracketed by HIP events (some 180 event records a
-    # step, which cost about 8% of the step), ahead of the timed region
+        raise SystemExit(f"{form}: warm-up failed: {spb6.STATUS.get(rc, rc)}")
     s0 = st.stats()
-    st.enable_timing(True)
-    rc = st.run(1.0, a.steps)
-    if rc != 0:
-        raise SystemExit(f"{form}: breakdown run failed: {spb6.STATUS.get(rc, rc)}")
-    kt_all = st.kernel_times()
+    kt_all = None
+    if breakdown > 0:
+        st.enable_timing(True)
+        rc = st.run(1.0, breakdown)
+        if rc != 0:
+            raise SystemExit(f"{form}: breakdown run failed: {spb6.STATUS.get(rc, rc)}")
+        kt_all = st.kernel_times()
     s1 = st.stats()
-    d_all = {k: s1[k] - s0[k] for k in keys}
-    dom = dominant(kt_all)
-    # the timed region: K more steps, only the dominant category's launches bracketed by events (its live duration)
-    st.enable_timing(True, categories=TIMER_CATS[dom])
+    dom = dominant(kt_all) if kt_all else None
+    if dom:
+        st.enable_timing(True, categories=TIMER_CATS[dom])
     t0 = time.perf_counter()
-    rc = st.run(1.0, a.steps)
+    rc = st.run(1.0, steps)
     t1 = time.perf_counter()
     if rc != 0:
         raise SystemExit(f"{form}: timed run failed: {spb6.STATUS.get(rc, rc)}")
@@ -102,30 +122,24 @@ def run_form(a, form, keep_state=False):
     kt_live = st.kernel_times()
     st.enable_timing(False)
     wall = t1 - t0
-    d = {k: s2[k] - s1[k] for k in keys}
+    d = {k: s2[k] - s1[k] for k in STAT_KEYS}
+    d_all = {k: s1[k] - s0[k] for k in STAT_KEYS}
     res = {"value": n * d["steps"] / wall, "wall": wall, "ms_per_step": 1e3 * wall / max(d["steps"], 1),
            "newton_per_stage": d["newton_it"] / max(d["stages"], 1), "krylov_per_newton": d["krylov_it"] / max(d["linsolves"], 1),
-           "krylov_per_step": d["krylov_it"] / max(d["steps"], 1), "d": d, "kt": kt_all, "d_all": d_all, "dom": dom, "kt_live": kt_live}
+           "krylov_per_step": d["krylov_it"] / max(d["steps"], 1), "d": d, "kt": kt_all, "d_all": d_all, "dom": dom,
+           "kt_live": kt_live, "t": st.t, "n": n}
     if form == "learned":
-        res["guess_used"], res["guess_fallback"] = d["guess_used"], d["guess_fallback"]
-    return res, st, cfg
-
-
-# HBM bytes per launch of the head form's candidate dominant kernels at 2048x1024, measured by rocprofv3 --pmc FETCH_SIZE
-# and --pmc WRITE_SIZE, calibrated on the FP32 conversion kernel (known bytes: FETCH_SIZE reads x 2.00, WRITE_SIZE
-# x 1.00): one FP32 BiCGStab iteration (the five passes' counts over the run divided by the iterations taken) and one
-# residual evaluation writing its trial pressure (median active dispatch); commands and per-pass counts in
-# profiles/krylov_fp32_iteration_2048x1024_traffic.md and profiles/residual_2048x1024_traffic.md
-TRAFFIC = {("krylov_iter", True, 2048, 1024): 327036314, ("residual", True, 2048, 1024): 782086963}
+        res["guess"] = {k: d[k] for k in ("guess_used", "guess_fallback", "guess_skipped")}
+    return res, st
 
 
 # timer categories (Stepper.TIMING_CATEGORIES) behind each roofline category
-TIMER_CATS = {"explicit": ("explicit",), "prep": ("prep",), "residual": ("residual",), "jacobian": ("jacobian",),
-              "krylov_iter": ("krylov", "krylov_idle"), "final": ("final",)}
+TIMER_CATS = {"explicit": ("explicit",), "prep": ("prep",), "shock": ("shock",), "residual": ("residual",),
+              "jacobian": ("jacobian",), "krylov_iter": ("krylov", "krylov_idle"), "final": ("final",)}
 
 
 def categories(kt, krylov_it, n, fp32):
-    """roofline category -> (timed ms, launches, algorithmic bytes per launch)"""
+    """HBM-bound roofline category -> (timed ms, launches, algorithmic bytes per launch)"""
     kb = BYTES_KRYLOV_FP32 if fp32 else BYTES_2D["krylov_iter"]
     return {
         "explicit": (kt["explicit_ms"], kt["explicit_n"], BYTES_2D["explicit"] * n),
@@ -140,145 +154,202 @@ def categories(kt, krylov_it, n, fp32):
 
 
 def dominant(kt):
-    """the category with the most kernel time"""
-    cats = categories(kt, 1, 1, False)
-    return max(cats, key=lambda k: cats[k][0])
+    """the category with the most kernel time (the shock budget included)"""
+    t = {k: v[0] for k, v in categories(kt, 1, 1, False).items()}
+    t["shock"] = kt["shock_ms"]
+    return max(t, key=lambda k: t[k])
 
 
-def roofline_of(res, n, fp32, nx=2048, ny=1024):
+def roofline_of(res, n, fp32, traffic=None):
+    """roofline of the dominant category over its live launches in the timed region, and every category's fraction
+    over the breakdown steps"""
     kt, d = res["kt"], res["d"]
-    kb = BYTES_KRYLOV_FP32 if fp32 else BYTES_2D["krylov_iter"]
     cats = categories(kt, res["d_all"]["krylov_it"], n, fp32)
     kernel_ms = {k: round(v[0], 3) for k, v in cats.items()}
-    kernel_ms["shock"] = round(kt["shock_ms"], 3)
-    kernel_ms["other"] = round(kt["other_ms"], 3)
-    kernel_ms["krylov_aux"] = round(kt["krylov_aux_ms"], 3)
-    kernel_ms["krylov_idle"] = round(kt["krylov_idle_ms"], 3)
-    kernel_ms["guess"] = round(kt["guess_ms"], 3)
-    # the dominant category's launches as timed live in the timed region
-    dom = res["dom"]
-    ms, cnt, byts = categories(res["kt_live"], d["krylov_it"], n, fp32)[dom]
-    avg_s = (ms / 1e3) / max(cnt, 1)
-    achieved = byts / avg_s if avg_s > 0 else 0.0
-    traffic = TRAFFIC.get((dom, fp32, nx, ny))
-    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": traffic, "bytes_per_launch": byts, "avg_launch_ms": avg_s * 1e3,
-            "iterations": cnt,
-            # every category on its algorithmic bytes over its timed region (the region's small boundary-fill and
-            # reduction kernels included)
-            "by_kernel": {k: round(v[2] * max(v[1], 1) / (v[0] / 1e3) / HBM_PEAK, 4) for k, v in cats.items() if v[0] > 0}}
+    for k in ("shock", "other", "krylov_aux", "krylov_idle", "guess"):
+        kernel_ms[k] = round(kt[f"{k}_ms"], 3)
+    by_kernel = {k: round(v[2] * max(v[1], 1) / (v[0] / 1e3) / HBM_PEAK, 4) for k, v in cats.items() if v[0] > 0}
+    if fp32 and kt["krylov_ms"] > 0:
+        by_kernel["krylov_iter_rb172"] = round(BYTES_KRYLOV_FP32_RB * n * res["d_all"]["krylov_it"] / (kt["krylov_ms"] / 1e3) / HBM_PEAK, 4)
+    if kt["shock_ms"] > 0:
+        by_kernel["shock_fp64"] = round(kt["shock_rk_steps"] * SHOCK_RK_FLOPS / (kt["shock_ms"] / 1e3) / FP64_PEAK, 4)
+    dom, live = res["dom"], res["kt_live"]
+    if dom == "shock":
+        ms, cnt = live["shock_ms"], live["shock_n"]
+        avg_s = (ms / 1e3) / max(cnt, 1)
+        flops = live["shock_rk_steps"] * SHOCK_RK_FLOPS / max(cnt, 1)
+        achieved = flops / avg_s if avg_s > 0 else 0.0
+        roof = {"bound": "fp64", "kernel": "shock", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK, "traffic": traffic, "flops_per_launch": flops, "rk_steps_per_launch": live["shock_rk_steps"] / max(cnt, 1),
+                "avg_launch_ms": avg_s * 1e3, "launches": cnt,
+                "note": "FP64 VALU issue-bound: a correctly rounded FP64 division costs about 7.4 FMA issue slots and the "
+                        "profile right-hand sides are division-dominated, so the FP64 peak (FMA = 2 operations) is not reachable"}
+    else:
+        ms, cnt, byts = categories(live, d["krylov_it"], n, fp32)[dom]
+        avg_s = (ms / 1e3) / max(cnt, 1)
+        achieved = byts / avg_s if avg_s > 0 else 0.0
+        roof = {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "bytes_per_launch": byts, "avg_launch_ms": avg_s * 1e3,
+                "launches": cnt}
+    roof["by_kernel"] = by_kernel
+    kb = BYTES_KRYLOV_FP32 if fp32 else BYTES_2D["krylov_iter"]
     step_bytes = (BYTES_2D["explicit"] + BYTES_2D["prep"] + BYTES_2D["final"]) * d["stages"] + BYTES_2D["jacobian"] * d["newton_it"] \
         + BYTES_2D["residual"] * (d["stages"] + d["newton_it"]) + kb * d["krylov_it"]
     return roof, kernel_ms, step_bytes * n / res["wall"] / HBM_PEAK
 
 
+def field_l1(A, B):
+    A, B = A.reshape(-1, 8), B.reshape(-1, 8)
+    return [float(np.abs(A[:, k] - B[:, k]).sum() / max(np.abs(B[:, k]).sum(), 1e-300)) for k in range(8)]
+
+
+def totals(U):
+    """sum m1, sum m2, sum rho u, sum rho v, sum E over the interior cells (the reference's conserved_totals, PC:1104-1107,
+    with the momenta)"""
+    U = U.reshape(-1, 8)
+    return np.array([U[:, k].sum(dtype=np.float64) for k in (1, 2, 3, 4, 7)])
+
+
+def conservation_check(a, forms, n=1024, steps=30):
+    """closed box: the headline's initial state on an n x n periodic grid (every flux leaves one cell and enters another)
+    -- relative change of sum m1, m2, rho u, rho v, E over `steps` steps for each form"""
+    from diffusionimex_amd import cases
+    from diffusionimex_amd.spb6 import BC_PERIODIC
+    out = {}
+    for form in forms:
+        st, cfg, U0 = make_stepper(lambda **kw: cases.cavitation_2d(n, n, seed=a.seed, cfl=0.5, bc=(BC_PERIODIC,) * 4, **kw), form)
+        T0 = totals(U0)
+        rc = st.run(1.0, steps)
+        T1 = totals(st.state())
+        st.close()
+        scale = np.array([abs(T0[0]), abs(T0[1]), np.abs(U0.reshape(-1, 8)[:, 3]).sum(), np.abs(U0.reshape(-1, 8)[:, 4]).sum(), abs(T0[4])])
+        out[form] = {"rc": rc, "rel_change": {k: float((T1[i] - T0[i]) / scale[i]) for i, k in enumerate(("m1", "m2", "rho_u", "rho_v", "E"))}}
+    out["workload"] = f"{n}x{n} periodic cavitation, {steps} steps (momenta relative to sum |rho u|, sum |rho v|)"
+    return out
+
+
 def main():
     a = parse()
-    if a.legs is None:
-        a.legs = "shock-bubble" if a.profile else "shock-bubble,interface,batch"
-    legs = set(x for x in a.legs.split(",") if x)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); multi-GPU sharding is design-only", file=sys.stderr)
         sys.exit(2)
-    out = run_headline(a) if "shock-bubble" in legs else {"legs": sorted(legs)}
-    if a.interface_steps > 0 and "interface" in legs:
+    _pin_cpu_threads(a)
+    if a.legs is None:
+        a.legs = "cavitation" if a.profile else "cavitation,bubble,interface,batch"
+    legs = set(x for x in a.legs.split(",") if x)
+    out = run_headline(a) if "cavitation" in legs else {"legs": sorted(legs)}
+    if "bubble" in legs and not a.profile:
+        out["bubble"] = run_bubble(a)
+    if "interface" in legs and a.interface_steps > 0 and not a.profile:
         out["interface"] = run_interface(a)
-    if a.batch_steps > 0 and "batch" in legs:
+    if "batch" in legs and a.batch_steps > 0 and not a.profile:
         out["batch"] = run_batch(a)
     print(json.dumps(out))
 
 
+def _forms(a):
+    from diffusionimex_amd import mlp
+    forms = [f for f in (a.forms or "plain,mixed,learned").split(",") if f]
+    if "learned" in forms and not os.path.exists(mlp.WEIGHTS):
+        forms.remove("learned")
+    return forms
+
+
 def run_headline(a):
-    """The 2048x1024 shock-bubble leg in each form (plain FP64, FP32 Krylov, learned guess + FP32 Krylov): the JSON
-    contract fields, with the head form's dominant-kernel roofline and the oracle CPU baseline."""
-    n = a.nx * a.ny
-    if a.forms is None:
-        from diffusionimex_amd import mlp
-        a.forms = "plain,mixed,learned" if os.path.exists(mlp.WEIGHTS) else "plain,mixed"
-    forms = [f for f in a.forms.split(",") if f]
+    """The 4096^2 cavitation workload in each form: the JSON contract fields, the forms' rates and errors against the
+    plain FP64 form, the head form's dominant-kernel roofline and the oracle CPU baseline."""
+    from diffusionimex_amd import cases
+    N = a.n
+    make = lambda **kw: cases.cavitation_2d(N, N, seed=a.seed, cfl=0.5, **kw)
+    forms = _forms(a)
     results, states = {}, {}
-    st_plain = cfg = None
+    st_plain = cfg_plain = None
     for form in forms:
-        res, st, c = run_form(a, form)
+        res, st = run_form(make, form, a.warmup, a.steps, a.breakdown)
         states[form] = st.state()
         results[form] = res
         if form == "plain":
-            st_plain, cfg = st, c
+            st_plain, cfg_plain = st, st.cfg
         else:
             st.close()
-        if cfg is None:
-            cfg = c
-    head = "learned" if "learned" in results else ("mixed" if "mixed" in results and "plain" not in results else "plain")
+    head = "learned" if "learned" in results else ("mixed" if "mixed" in results else "plain")
     r = results[head]
-    roof, kernel_ms, hbm_frac = roofline_of(r, n, head != "plain", a.nx, a.ny)
+    n = N * N
+    roof, kernel_ms, hbm_frac = roofline_of(r, n, head != "plain")
     forms_out = {}
     for form, res in results.items():
-        fo = {k: res[k] for k in ("value", "ms_per_step", "newton_per_stage", "krylov_per_newton", "krylov_per_step")}
+        fo = {k: round(res[k], 4) if isinstance(res[k], float) else res[k]
+              for k in ("value", "ms_per_step", "newton_per_stage", "krylov_per_newton", "krylov_per_step")}
+        fo["kernel_ms_per_step"] = {k: round(v / max(res["d_all"]["steps"], 1), 3) for k, v in roofline_of(res, n, form != "plain")[1].items()} \
+            if res["kt"] else None
         if "plain" in states and form != "plain":
-            A, B = states[form].reshape(-1, 8), states["plain"].reshape(-1, 8)
-            fo["field_l1_vs_fp64"] = [float(np.abs(A[:, k] - B[:, k]).sum() / max(np.abs(B[:, k]).sum(), 1e-300)) for k in range(8)]
+            fo["field_l1_vs_fp64"] = field_l1(states[form], states["plain"])
+            fo["max_field_l1_vs_fp64"] = max(fo["field_l1_vs_fp64"])
             fo["speedup_vs_plain"] = res["value"] / results["plain"]["value"]
-        if form == "learned":
-            fo["guess_used"], fo["guess_fallback"] = res["guess_used"], res["guess_fallback"]
+        if "guess" in res:
+            fo["guess"] = res["guess"]
         forms_out[form] = fo
     out = {
         "metric": "2D IMEX cell-updates/s + BiCGStab iters/step, learned+mixed vs plain FP64",
-        "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": r["d"]["steps"], "warmup": a.warmup,
+        "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": r["d"]["steps"], "warmup": a.warmup + a.breakdown,
         "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if head == "plain" else "f64 (FP32 Krylov)", "data": "synthetic",
-        "config": {"workload": f"2D shock-bubble {a.nx}x{a.ny} ({head} form; Mach 1.22, gamma 1.4 / 1.648, walls top/bottom)",
-                   "cells": n, "seed": a.seed, "cfl": 0.5, "newton_tol": 1e-13, "krylov_tol": 1e-13, "form": head},
+        "dtype": "f64" if head == "plain" else "f64 (FP32 BiCGStab inner iterations)", "data": "synthetic",
+        "config": {"workload": f"2D low-Mach cavitation {N}x{N} on [0,1]^2 ({head} form; water 4.4/6e8 with gas fraction 1e-2, "
+                               f"u, v = +-100 m/s diverging with 1% seeded perturbations, transmissive)",
+                   "cells": n, "seed": a.seed, "cfl": 0.5, "newton_tol": 1e-13, "krylov_tol": 1e-13, "form": head,
+                   "timed_steps": f"{a.warmup + a.breakdown + 1}..{a.warmup + a.breakdown + r['d']['steps']} of 300 "
+                                  f"(t = {r['t']:.4e} s at the end)"},
         "newton_per_stage": r["newton_per_stage"], "krylov_per_newton": r["krylov_per_newton"], "krylov_per_step": r["krylov_per_step"],
         "hbm_step_frac": hbm_frac, "kernel_ms": kernel_ms, "forms": forms_out,
         "roofline": roof,
     }
+    if not a.profile:
+        out["conservation"] = conservation_check(a, forms)
     if not a.no_cpu and not a.profile:
         if st_plain is None:
-            _, st_plain, cfg = run_form(a, "plain")
-        out["cpu_baseline"] = cpu_baseline(a, cfg, st_plain)
+            _, st_plain = run_form(make, "plain", a.warmup, 0, 0)
+            cfg_plain = st_plain.cfg
+        out["cpu_baseline"] = cpu_baseline(a, cfg_plain, st_plain)
     if st_plain is not None:
         st_plain.close()
     return out
 
 
-def run_interface(a):
-    """4096x2048 Mach-1.5 shock on a perturbed water-gas interface (seeded amplitude and mode), Newton tolerance 1e-10:
-    the plain FP64 form against the learned guess + FP32 Krylov form, same warm-up and timed steps."""
-    from diffusionimex_amd import spb6, cases, mlp
+def _leg(a, make, warmup, steps, workload):
+    """one leg: every form over the same window, rates and field errors against the plain FP64 form"""
     res, states = {}, {}
-    forms = ["plain", "learned"] if os.path.exists(mlp.WEIGHTS) else ["plain", "mixed"]
-    for form in forms:
-        kw = {} if form == "plain" else {"krylov_fp32": 1, "analytic_jacobian": 1}
-        if form == "learned":
-            kw["learned_guess"] = 1
-        cfg, U = cases.perturbed_interface(4096, 2048, seed=a.seed, cfl=0.5, newton_tol=1e-10, **kw)
-        st = spb6.Stepper(cfg, device=0)
-        if form == "learned":
-            mlp.load_into(st)
-        st.set_state(U.reshape(cfg.ny, cfg.nx, 8))
-        if st.run(1.0, a.interface_warmup) != 0:
-            raise SystemExit(f"interface {form}: warm-up failed")
-        s0 = st.stats()
-        t0 = time.perf_counter()
-        rc = st.run(1.0, a.interface_steps)
-        wall = time.perf_counter() - t0
-        s1 = st.stats()
-        if rc != 0:
-            raise SystemExit(f"interface {form}: timed run failed")
-        d = {k: s1[k] - s0[k] for k in ("steps", "stages", "newton_it", "krylov_it", "linsolves")}
-        res[form] = {"value": cfg.nx * cfg.ny * d["steps"] / wall, "ms_per_step": 1e3 * wall / d["steps"],
-                     "newton_per_stage": d["newton_it"] / max(d["stages"], 1),
-                     "krylov_per_newton": d["krylov_it"] / max(d["linsolves"], 1),
-                     "krylov_per_step": d["krylov_it"] / max(d["steps"], 1)}
-        states[form] = st.state().reshape(-1, 8)
+    for form in _forms(a):
+        r, st = run_form(make, form, warmup, steps, 0)
+        states[form] = st.state()
         st.close()
-    other = forms[1]
-    A, B = states[other], states["plain"]
-    res[other]["field_l1_vs_plain"] = [float(np.abs(A[:, k] - B[:, k]).sum() / max(np.abs(B[:, k]).sum(), 1e-300)) for k in range(8)]
-    res[other]["speedup_vs_plain"] = res[other]["value"] / res["plain"]["value"]
-    res["workload"] = f"4096x2048 shock on a perturbed water-gas interface, Newton tol 1e-10, {a.interface_warmup} warm-up steps"
-    res["steps"] = a.interface_steps
+        res[form] = {k: round(r[k], 4) if isinstance(r[k], float) else r[k]
+                     for k in ("value", "ms_per_step", "newton_per_stage", "krylov_per_newton", "krylov_per_step")}
+        if "guess" in r:
+            res[form]["guess"] = r["guess"]
+    for form in res:
+        if form != "plain" and "plain" in states:
+            res[form]["field_l1_vs_fp64"] = field_l1(states[form], states["plain"])
+            res[form]["speedup_vs_plain"] = res[form]["value"] / res["plain"]["value"]
+    res["workload"] = workload
+    res["window"] = f"steps {warmup + 1}..{warmup + steps}"
     return res
+
+
+def run_bubble(a):
+    """2048x1024 shock-bubble (Mach 1.22 in gas gamma 1.4 onto a gamma-1.648 light-gas bubble, walls top and bottom),
+    Newton tolerance 1e-13, timed in a window after the shock has reached the bubble"""
+    from diffusionimex_amd import cases
+    return _leg(a, lambda **kw: cases.shock_bubble(2048, 1024, seed=a.seed, cfl=0.5, **kw), a.bubble_warmup, a.bubble_steps,
+                "2D shock-bubble 2048x1024 on 0.2x0.1 m, Newton tol 1e-13 (the shock reaches the bubble near step 250)")
+
+
+def run_interface(a):
+    """4096x2048 Mach-1.5 shock on a perturbed water-gas interface (seeded amplitude and mode), Newton tolerance 1e-10"""
+    from diffusionimex_amd import cases
+    return _leg(a, lambda **kw: cases.perturbed_interface(4096, 2048, seed=a.seed, cfl=0.5, newton_tol=1e-10, **kw),
+                a.interface_warmup, a.interface_steps,
+                "4096x2048 shock on a perturbed water-gas interface, Newton tol 1e-10 (the shock travels in gas during the "
+                "500 steps of the workload)")
 
 
 def run_batch(a):
@@ -314,6 +385,36 @@ def run_batch(a):
     return res
 
 
+def _physical_cores():
+    """(physical cores among the CPUs this process may run on, one CPU id per core)"""
+    try:
+        allowed = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = list(range(os.cpu_count() or 1))
+    seen, pick = set(), []
+    for c in allowed:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            key = (open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip())
+        except OSError:
+            key = ("?", str(c))
+        if key not in seen:
+            seen.add(key)
+            pick.append(c)
+    return len(pick), pick
+
+
+def _pin_cpu_threads(a):
+    """one OpenMP thread per physical core (at most 16, this box's CPU share for one GPU), pinned to cores; set before
+    the oracle's OpenMP runtime starts"""
+    ncore, pick = _physical_cores()
+    threads = a.cpu_threads or max(1, min(16, ncore))
+    a.cpu_threads_used = threads
+    a.cpu_cores_available = ncore
+    os.environ["OMP_PLACES"] = "{" + "},{".join(str(c) for c in pick[:threads]) + "}"
+    os.environ["OMP_PROC_BIND"] = "close"
+
+
 def batch_cpu_baseline(a, scheme, probs, b, per):
     """The oracle (PC_omp paths) on the host cores, one problem per thread: a sample of the batch's problems resumed from
     the GPU's warmed state, batch-steps steps each."""
@@ -321,7 +422,7 @@ def batch_cpu_baseline(a, scheme, probs, b, per):
     from oracle import oracle as O
     from tests.helpers import oracle_cfg_from
     from diffusionimex_amd import spb6
-    threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = a.cpu_threads_used
     O.lib().ora_set_threads(1)
     U = b.state()
     P, pref = b.pressure()
@@ -352,11 +453,11 @@ def batch_cpu_baseline(a, scheme, probs, b, per):
 
 
 def cpu_baseline(a, cfg, st):
-    """The oracle (PC_omp paths, B = 1024 block sums, libm) on the host cores: cpu-steps LSDIRK2 steps of the same grid
-    resumed from the GPU's warmed state (same t, dt history, pressure)."""
+    """The oracle (PC_omp paths, B = 1024 block sums, libm) on the host's physical cores, threads pinned: cpu-steps
+    LSDIRK2 steps of the same grid resumed from the GPU's warmed plain-form state (same t, dt history, pressure)."""
     from oracle import oracle as O
     from tests.helpers import oracle_cfg_from
-    threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = a.cpu_threads_used
     L = O.lib()
     L.ora_set_threads(threads)
     oc = oracle_cfg_from(cfg, O)
@@ -373,9 +474,9 @@ def cpu_baseline(a, cfg, st):
     n = cfg.nx * cfg.ny
     so = o.stats()
     return {"value": n * so["steps"] / (t1 - t0), "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{so['steps']} LSDIRK2 steps of the {cfg.nx}x{cfg.ny} workload resumed from the GPU state after "
-                      f"{a.warmup + 2 * a.steps} steps ({t1 - t0:.1f} s, rc {rc}, {so['newton_it']} Newton, {so['krylov_it']} BiCGStab)",
-            "cpu": _cpu_model()}
+            "sample": f"{so['steps']} LSDIRK2 step(s) of the {cfg.nx}x{cfg.ny} workload resumed from the GPU plain-form state "
+                      f"after {s['steps']} steps ({t1 - t0:.1f} s, rc {rc}, {so['newton_it']} Newton, {so['krylov_it']} BiCGStab)",
+            "cpu": _cpu_model(), "physical_cores_available": a.cpu_cores_available, "pinned": os.environ.get("OMP_PROC_BIND") == "close"}
 
 
 def _cpu_model():

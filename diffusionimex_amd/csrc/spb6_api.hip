@@ -89,6 +89,9 @@ struct spb6_ctx {
 	int last_krylov = 0;
 	long krylov_launched = 0;          // BiCGStab iteration passes launched since timing was (re)enabled
 	double tau_last = 0.0;
+	// learned guess back-off: after two stages in a row where the plain guess judged the network's guess worse, the
+	// network is skipped for 1, 2, 4, ... 32 stages (guess_skip) before it is tried again; a win resets the back-off
+	int guess_skip = 0, guess_losses = 0, guess_backoff = 0;
 	Timer timer;
 	// learned guess and training dumps
 	spb6_mlp* model = nullptr;
@@ -97,6 +100,7 @@ struct spb6_ctx {
 	double* Gplain = nullptr;           // its residual
 	int *slist = nullptr, *scount = nullptr;   // strong-shock cells of the shock budget's second pass (scount[1]: its cursor)
 	double* srec = nullptr;                    // their profile problems (SHOCK_REC x n)
+	unsigned long long* shock_steps = nullptr; // RK45 steps of the strong-shock passes since timing was (re)enabled
 	int *rlist = nullptr, *rcount = nullptr;   // cells whose residual the learned guess can change
 	int cus = 256;                      // compute units of the device
 	int shock_blocks = 1024;            // resident 256-thread blocks of the strong-shock pass
@@ -418,7 +422,7 @@ int prep_part(spb6_ctx* c, double adt) {
 	c->tau_last = adt;
 	Dev D = stage_dev(c, adt);
 	{ TRegion tr(c, T_PREP); launch_prep(D, c->cfg.limiter, c->cfg.order, adt, c->Pst, c->st); }
-	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->slist, c->scount, c->srec, c->shock_blocks, c->st); }
+	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->slist, c->scount, c->srec, c->shock_steps, c->shock_blocks, c->st); }
 	return SPB6_OK;
 }
 // PressureSubsystem (PC:851-993) after the explicit part; writes the stage state into Ust and H_rks. defer_final
@@ -429,7 +433,11 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 	Dev D = stage_dev(c, adt);
 	const long n = c->g.n;
 	const bool dumping = c->dX && c->dcap > 0;
-	const bool learned = c->cfg.learned_guess && c->model;
+	bool learned = c->cfg.learned_guess && c->model;
+	if (learned && !dumping && c->guess_skip > 0) {   // backed off: the plain guess, without evaluating the network
+		c->guess_skip--; c->stats.guess_skipped++; c->stats.guess_fallback++;
+		learned = false;
+	}
 	// the plain guess's residual (Newton's first, PC:764-770); the stage's admissibility flags, of the explicit part
 	// (rho > 0, PC:255-258) and of the preparation (PC:862), are read back with it: one synchronisation instead of
 	// three. What runs meanwhile on an inadmissible stage ends normally (every loop in the kernels is bounded) and is
@@ -492,8 +500,12 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 		if (okl && (!ok0 || resl < res0)) {
 			if (ok0 && resl > 0) c->stats.guess_log10_gain += std::log10(res0 / resl);
 			std::swap(c->Pst, c->Pn); std::swap(c->G, c->Gn); ok0 = true; res0 = resl; c->stats.guess_used++;
+			c->guess_losses = 0; c->guess_backoff = 0;
 		}
-		else c->stats.guess_fallback++;
+		else {
+			c->stats.guess_fallback++;
+			if (++c->guess_losses >= 2) { c->guess_skip = 1 << std::min(c->guess_backoff, 5); c->guess_backoff++; }
+		}
 	}
 	else if (learned) c->stats.guess_fallback++;   // the plain guess has already converged
 	rc = newton(c, D, iters, res, have_G, ok0, res0);
@@ -617,6 +629,9 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (dalloc(c, &c->Pplain, n) || dalloc(c, &c->Gplain, n)) return fail("alloc");
 	if (hipMalloc((void**)&c->slist, (n + 2) * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->slist);
 	if (dalloc(c, &c->srec, (size_t)SHOCK_REC * n)) return fail("alloc");
+	if (hipMalloc((void**)&c->shock_steps, sizeof(unsigned long long)) != hipSuccess) return fail("alloc");
+	c->allocs.push_back(c->shock_steps);
+	if (hipMemset(c->shock_steps, 0, sizeof(unsigned long long)) != hipSuccess) return fail("alloc");
 	c->scount = c->slist + n;
 	if (hipMalloc((void**)&c->rlist, (n + 1) * sizeof(int)) != hipSuccess) return fail("alloc");
 	c->allocs.push_back(c->rlist);
@@ -660,6 +675,7 @@ int spb6_set_state(spb6_ctx* c, const double* U_aos) {
 	HIPCHK(hipMemcpyAsync(c->Pst, P.data(), P.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
 	HIPCHK(hipStreamSynchronize(c->st));
 	c->t = 0.0; c->dtprev = -1.0; c->startup = true; c->last_krylov = 0;
+	c->guess_skip = c->guess_losses = c->guess_backoff = 0;
 	std::memset(&c->stats, 0, sizeof c->stats);
 	return SPB6_OK;
 }
@@ -915,6 +931,7 @@ int spb6_enable_timing(spb6_ctx* c, int on) {
 	c->timer.on = on != 0;
 	for (int k = 0; k < T_NCAT; k++) { c->timer.ms[k] = 0; c->timer.cnt[k] = 0; }
 	c->krylov_launched = 0;
+	HIPCHK(hipMemsetAsync(c->shock_steps, 0, sizeof(unsigned long long), c->st));
 	return SPB6_OK;
 }
 int spb6_get_kernel_times(spb6_ctx* c, spb6_kernel_times_t* t) {
@@ -928,6 +945,10 @@ int spb6_get_kernel_times(spb6_ctx* c, spb6_kernel_times_t* t) {
 	t->krylov_iters = c->stats.krylov_it;
 	t->krylov_aux_ms = T.ms[T_KAUX]; t->guess_ms = T.ms[T_GUESS]; t->krylov_launched = c->krylov_launched;
 	t->krylov_idle_ms = T.ms[T_KIDLE];
+	unsigned long long rk = 0;
+	HIPCHK(hipMemcpyAsync(&rk, c->shock_steps, sizeof rk, hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	t->shock_rk_steps = (long)rk;
 	return SPB6_OK;
 }
 

@@ -349,12 +349,13 @@ __global__ void __launch_bounds__(256) k_shock_classify(Geo g, Phys P, const dou
 constexpr int SHOCK_CHUNK = 128;
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_shock_strong(Geo g, Phys P, const double* __restrict__ rec, long cap,
                                                      double* __restrict__ Qph, const int* __restrict__ list, const int* __restrict__ count,
-                                                     int* __restrict__ cursor) {
+                                                     int* __restrict__ cursor, unsigned long long* __restrict__ rk_steps) {
 	const int m = *count;
 	const int lane = threadIdx.x & 63;
 	bool active = false, more = true;
 	int slot = 0;
 	int qn = 0, qe = 0;   // the wavefront's chunk of list slots not yet taken, [qn, qe)
+	int nsteps = 0;       // RK45 steps this lane took (added to *rk_steps at the exit: the pass's work count)
 	Prof S; double vH0 = 0.0, rho = 0.0; Shoot H; ProfRun R;
 	for (;;) {
 		bool tail = false;   // this lane starts an integration or finishes its cell in this pass
@@ -383,10 +384,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 			}
 		}
 		if (__ballot(active) == 0ull) {
-			if (!more) break;
+			if (!more) {
+				#pragma unroll
+				for (int off = 32; off >= 1; off >>= 1) nsteps += __shfl_xor(nsteps, off, 64);
+				if (lane == 0) atomicAdd(rk_steps, (unsigned long long)nsteps);
+				break;
+			}
 			continue;
 		}
 		if (active && !tail) {
+			nsteps++;
 			const int r = prof_step(P, S, R);
 			if (r != 0) { t = shoot_next(H, share_miss(R, r, vH0, S.vk00), R.y2); tail = true; }
 		}
@@ -1213,11 +1220,11 @@ int shock_strong_blocks_per_cu() {
 	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(k_shock_strong), 256, 0) != hipSuccess || b < 1) b = 2;
 	return b;
 }
-void launch_shock(const Dev& D, int* list, int* count, double* rec, int strong_blocks, hipStream_t st) {
+void launch_shock(const Dev& D, int* list, int* count, double* rec, unsigned long long* rk_steps, int strong_blocks, hipStream_t st) {
 	(void)hipMemsetAsync(count, 0, 2 * sizeof(int), st);   // the count and the strong pass's cursor
 	hipLaunchKernelGGL(k_shock_classify, dim3(nblk(D.g.n)), dim3(256), 0, st, D.g, D.ph, D.Ust, D.Qph, list, count, rec, (long)D.g.n);
 	hipLaunchKernelGGL(k_shock_strong, dim3(std::max(1, std::min(nblk(D.g.n), strong_blocks))), dim3(256), 0, st, D.g, D.ph,
-	                   (const double*)rec, (long)D.g.n, D.Qph, (const int*)list, (const int*)count, count + 1);
+	                   (const double*)rec, (long)D.g.n, D.Qph, (const int*)list, (const int*)count, count + 1, rk_steps);
 }
 void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st,
                      const int* gate) {

@@ -41,7 +41,8 @@ class Stats(C.Structure):
     _fields_ = [("steps", C.c_long), ("rk1", C.c_long), ("rejects", C.c_long), ("floor_acc", C.c_long),
                 ("stages", C.c_long), ("newton_it", C.c_long), ("krylov_it", C.c_long), ("linsolves", C.c_long),
                 ("maxres", C.c_double), ("t", C.c_double), ("dt_last", C.c_double), ("status", C.c_int),
-                ("guess_used", C.c_long), ("guess_fallback", C.c_long), ("guess_log10_gain", C.c_double)]
+                ("guess_used", C.c_long), ("guess_fallback", C.c_long), ("guess_log10_gain", C.c_double),
+                ("guess_skipped", C.c_long)]
 
 
 class KernelTimes(C.Structure):
@@ -50,7 +51,7 @@ class KernelTimes(C.Structure):
                [(k, C.c_long) for k in ("explicit_n", "prep_n", "shock_n", "residual_n", "jacobian_n", "krylov_n",
                                          "final_n", "krylov_iters")] + \
                [("krylov_aux_ms", C.c_double), ("guess_ms", C.c_double), ("krylov_launched", C.c_long),
-                ("krylov_idle_ms", C.c_double)]
+                ("krylov_idle_ms", C.c_double), ("shock_rk_steps", C.c_long)]
 
 
 # every symbol include/spb6_imex.h declares

@@ -80,6 +80,9 @@ typedef struct spb6_stats {
     int status;
     long guess_used, guess_fallback;  /* learned-guess stages, and stages where the plain guess was kept */
     double guess_log10_gain;          /* sum over stages of log10(plain residual norm / chosen guess's residual norm) */
+    long guess_skipped;               /* stages of guess_fallback where the network was not evaluated: after two
+                                         stages in a row where the plain guess won, 1, 2, 4, ... 32 stages are skipped
+                                         before the network is tried again (a win resets the back-off) */
 } spb6_stats_t;
 
 typedef struct spb6_ctx spb6_ctx;
@@ -145,6 +148,7 @@ typedef struct spb6_kernel_times {
     double guess_ms;             /* learned-guess features and inference */
     long krylov_launched;        /* BiCGStab iterations launched (a pass launched after convergence returns at once) */
     double krylov_idle_ms;       /* FP32 form: time of the iterations launched past a solve's end (krylov_ms: taken ones) */
+    long shock_rk_steps;         /* RK45 profile steps taken by the shock budget's strong-shock passes (its work count) */
 } spb6_kernel_times_t;
 int spb6_enable_timing(spb6_ctx* ctx, int on);
 /* categories timed while timing is on (default all): bit k = the k-th category in the order explicit, prep, shock,
