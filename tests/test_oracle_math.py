@@ -248,12 +248,14 @@ def test_portable_math_against_libm(oracle_lib):
             assert abs(L.ora_ppow(x, y) - x ** y) <= (4 + 2 * abs(y * math.log(x))) * ulp(x ** y)
 
 
-def test_conservation_closed_box(oracle_lib):
-    """All-wall 2D box with a pressure bump: sum m1, sum m2, sum E and sum rho v conserved to round-off (PC:1299)."""
+@pytest.mark.parametrize("bc", [1, 2])
+def test_conservation_closed_box(oracle_lib, bc):
+    """2D box with a pressure bump, walls (bc 1) or periodic (bc 2) on every side: sum m1, sum m2 and sum E conserved to
+    round-off (PC:1299); in the periodic box also both momenta (walls exert pressure forces, so there they change)."""
     O = oracle_lib
     nx, ny = 24, 16
     ph = [[4.4, 6e8, 0, 0, 1000.0, 0], [1.4, 0, 0, 0, 1000.0, 0]]
-    cfg = O.default_cfg(nx, ny, bc=[1, 1, 1, 1], ph=ph, cfl=0.5)
+    cfg = O.default_cfg(nx, ny, bc=[bc] * 4, ph=ph, cfl=0.5)
     x = (np.arange(nx) + 0.5) / nx
     y = (np.arange(ny) + 0.5) / ny
     X, Y = np.meshgrid(x, y)
@@ -271,6 +273,10 @@ def test_conservation_closed_box(oracle_lib):
     U1 = o.state()
     for k in (1, 2, 7):
         assert abs(U1[..., k].sum() - U[..., k].sum()) <= 2e-15 * nx * ny * np.abs(U[..., k]).max()
+    if bc == 2:
+        for k in (3, 4):   # the momenta start at zero: their sums stay zero to round-off of the moved momentum
+            assert np.abs(U1[..., k]).max() > 0
+            assert abs(U1[..., k].sum()) <= 2e-15 * nx * ny * np.abs(U1[..., k]).max()
 
 
 def test_well_balanced_interface_advection(oracle_lib):
