@@ -65,7 +65,8 @@ struct spb6_ctx {
 	hipStream_t st = nullptr;
 	Geo g; Phys ph;
 	// ghosted grids (8 fields each) and arrays
-	double *U = nullptr, *Ust = nullptr, *Qe = nullptr, *Qs = nullptr, *H[2] = {nullptr, nullptr};
+	double *U = nullptr, *Ust = nullptr, *Qe = nullptr, *Qs = nullptr, *H[3] = {nullptr, nullptr, nullptr};   // H[2]: rk = 3 only
+	const double *qe_cur = nullptr, *qs_cur = nullptr;   // the stage operands of the per-stage entry points
 	double *donx, *dony, *phx, *phy, *a2s, *ubx, *uby, *vsc, *esc, *vu, *vv;   // face donors: (left, right) pairs of m1, m2, e1, e2
 	const double* qe_last = nullptr;   // Qe of the last explicit stage
 	bool final_pending = false;        // the previous stage's final-state flag (FLAG_FINAL_PREV) is not read back yet
@@ -530,10 +531,15 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 	return SPB6_OK;
 }
 
-void set_tab(spb6_ctx* c, int rk) {   // PC:1195-1218
+void set_tab(spb6_ctx* c, int rk) {   // PC:1195-1218: 1 backward Euler, 2 LSDIRK2(2,2,2), 3 SSP2(3,3,2)
 	for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) c->Ae[a][b] = c->Ai[a][b] = 0;
 	if (rk == 1) c->Ai[0][0] = 1.0;
 	else if (rk == 2) { const double bb = 1.0 - 1.0 / std::sqrt(2.0), cc = 1.0 / (2.0 * bb); c->Ae[1][0] = cc; c->Ai[0][0] = bb; c->Ai[1][0] = 1 - bb; c->Ai[1][1] = bb; }
+	else if (rk == 3) {
+		const double r12 = 0.5, r13 = 1.0 / 3.0, r14 = 1.0 / 4.0;
+		c->Ae[1][0] = r12; c->Ae[2][0] = r12; c->Ae[2][1] = r12;
+		c->Ai[0][0] = r14; c->Ai[1][1] = r14; c->Ai[2][0] = r13; c->Ai[2][1] = r13; c->Ai[2][2] = r13;
+	}
 }
 
 // one attempt (PC:1243-1285): on success Ust / Pst hold the new state and pressure
@@ -587,7 +593,7 @@ const char* spb6_last_error(void) { return g_err.c_str(); }
 spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (!cfg || cfg->nx < 2 || cfg->ny < 1) { g_err = "bad config"; return nullptr; }
 	if (cfg->rus_local != 1) { g_err = "only the local Rusanov speed (rus_local = 1) is supported"; return nullptr; }
-	if (cfg->rk < 1 || cfg->rk > 2) { g_err = "rk must be 1 or 2"; return nullptr; }
+	if (cfg->rk < 1 || cfg->rk > 3) { g_err = "rk must be 1, 2 or 3"; return nullptr; }
 	spb6_ctx* c = new spb6_ctx();
 	c->cfg = *cfg; c->device = device;
 	std::memset(&c->stats, 0, sizeof c->stats);
@@ -604,6 +610,7 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	g_err.clear();
 	double** grids[] = {&c->U, &c->Ust, &c->Qe, &c->Qs, &c->H[0], &c->H[1], &c->donx};   // donors: 4 fields x 2 = NV per cell
 	for (auto p : grids) if (dalloc(c, p, NC * NV)) return fail("alloc");
+	if (cfg->rk == 3 && dalloc(c, &c->H[2], NC * NV)) return fail("alloc");
 	if (g.twoD) { if (dalloc(c, &c->dony, NC * NV)) return fail("alloc"); }
 	else c->dony = c->donx;
 	double** arrs[] = {&c->phx, &c->phy, &c->a2s, &c->ubx, &c->uby, &c->vsc, &c->esc, &c->vu, &c->vv, &c->ust, &c->vst, &c->cz, &c->Kw, &c->Pold,
@@ -753,11 +760,56 @@ int spb6_implicit_stage(spb6_ctx* c, double adt, int* newton_iters, double* resi
 	return rc;
 }
 int spb6_step(spb6_ctx* c, double dt, int rk) {
-	if (!c || !(dt > 0) || rk < 1 || rk > 2) return SPB6_BAD_ARG;
+	if (!c || !(dt > 0) || rk < 1 || rk > 3 || (rk == 3 && !c->H[2])) return SPB6_BAD_ARG;
 	hipSetDevice(c->device);
 	const int rc = attempt(c, dt, rk);
 	if (rc == SPB6_OK) { std::swap(c->U, c->Ust); std::swap(c->P, c->Pst); c->t += dt; c->stats.steps++; }
 	return rc;
+}
+
+int spb6_attempt_begin(spb6_ctx* c) {
+	if (!c) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	HIPCHK(hipMemcpyAsync(c->Pst, c->P, (size_t)c->g.n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+	c->final_pending = false;
+	c->qe_cur = c->qs_cur = c->U;
+	return SPB6_OK;
+}
+int spb6_stage_combine(spb6_ctx* c, int rks, double dt, const double* ae, const double* ai) {
+	if (!c || rks < 0 || rks > 2 || (rks > 0 && (!ae || !ai))) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	if (rks == 0) { c->qe_cur = c->qs_cur = c->U; return SPB6_OK; }
+	launch_combine(c->g, c->U, c->H[0], c->H[1], rks, ae[0] * dt, rks > 1 ? ae[1] * dt : 0.0, ai[0] * dt, rks > 1 ? ai[1] * dt : 0.0,
+	               c->Qe, c->Qs, c->st);
+	launch_bc_grid(c->g, c->Qe, c->st); launch_bc_grid(c->g, c->Qs, c->st);
+	c->qe_cur = c->Qe; c->qs_cur = c->Qs;
+	return SPB6_OK;
+}
+int spb6_stage_update(spb6_ctx* c, double adt) {
+	if (!c || !c->qe_cur || !(adt > 0)) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	int rc = reset_stage(c); if (rc) return rc;
+	rc = explicit_part(c, c->qe_cur, c->qs_cur, adt); if (rc) return rc;
+	rc = fetch(c); if (rc) return rc;
+	return c->hs->flags[FLAG_RHO] ? SPB6_RHO_NONPOSITIVE : SPB6_OK;
+}
+int spb6_stage_pressure(spb6_ctx* c, double adt, int rks, int* newton_iters, double* residual) {
+	if (!c || !c->qs_cur || !(adt > 0) || rks < 0 || rks > 2 || !c->H[rks]) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	int it = 0; double res = 0;
+	if (reset_stage(c)) return SPB6_HIP_ERROR;   // the explicit part's flag was read by spb6_stage_update
+	c->final_pending = false;
+	const int rc = implicit_part(c, adt, c->qs_cur, c->H[rks], it, res);
+	if (newton_iters) *newton_iters = it;
+	if (residual) *residual = res;
+	return rc;
+}
+int spb6_attempt_accept(spb6_ctx* c, double dt) {
+	if (!c) return SPB6_BAD_ARG;
+	std::swap(c->U, c->Ust); std::swap(c->P, c->Pst);
+	c->t = c->t + dt; c->stats.steps++; c->stats.dt_last = dt; c->stats.t = c->t;
+	c->qe_cur = c->qs_cur = nullptr;
+	return SPB6_OK;
 }
 
 int spb6_run(spb6_ctx* c, double t_end, long max_steps) {

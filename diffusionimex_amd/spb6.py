@@ -57,7 +57,8 @@ class KernelTimes(C.Structure):
 # every symbol include/spb6_imex.h declares
 EXPORTS = ["spb6_create", "spb6_destroy", "spb6_last_error", "spb6_set_state", "spb6_get_state", "spb6_get_pressure",
            "spb6_set_pressure", "spb6_caldt", "spb6_bc_fill", "spb6_explicit_stage", "spb6_implicit_stage",
-           "spb6_stage_prep", "spb6_step", "spb6_run", "spb6_get_stats", "spb6_time", "spb6_residual",
+           "spb6_stage_prep", "spb6_step", "spb6_attempt_begin", "spb6_stage_combine", "spb6_stage_update",
+           "spb6_stage_pressure", "spb6_attempt_accept", "spb6_run", "spb6_get_stats", "spb6_time", "spb6_residual",
            "spb6_jacobian", "spb6_linsolve", "spb6_get_field", "spb6_enable_timing", "spb6_set_timing_mask", "spb6_get_kernel_times",
            "spb6_mlp_create", "spb6_mlp_destroy", "spb6_mlp_nparams", "spb6_mlp_set_params", "spb6_mlp_get_params",
            "spb6_mlp_set_data", "spb6_mlp_train", "spb6_mlp_forward", "spb6_mlp_gradients", "spb6_mlp_infer_bf16",
@@ -92,6 +93,11 @@ def load():
     L.spb6_implicit_stage.argtypes = [vp, C.c_double, ip, dp]
     L.spb6_stage_prep.argtypes = [vp, C.c_double]
     L.spb6_step.argtypes = [vp, C.c_double, C.c_int]
+    L.spb6_attempt_begin.argtypes = [vp]
+    L.spb6_stage_combine.argtypes = [vp, C.c_int, C.c_double, dp, dp]
+    L.spb6_stage_update.argtypes = [vp, C.c_double]
+    L.spb6_stage_pressure.argtypes = [vp, C.c_double, C.c_int, ip, dp]
+    L.spb6_attempt_accept.argtypes = [vp, C.c_double]
     L.spb6_run.argtypes = [vp, C.c_double, C.c_long]
     L.spb6_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.spb6_time.restype = C.c_double

@@ -53,7 +53,7 @@ typedef struct spb6_config {
     int bc[4];                       /* BCS[4] (PHY:908) */
     double bcu[4];                   /* BCU[4]: wall normal velocity (PHY:908) */
     spb6_phase_t ph[2];              /* PH[2] (PHY:54) */
-    int rk;                          /* RKs: 1 backward Euler, 2 LSDIRK2 (PC:57) */
+    int rk;                          /* RKs: 1 backward Euler, 2 LSDIRK2, 3 SSP2(3,3,2) (PC:57, PC:1195-1218) */
     double cfl, cfl_ac;              /* CFL, CFL_ac (PC:59-60) */
     int ramp;                        /* RAMP (PC:63) */
     double dtfix;                    /* DTFIX (PC:65) */
@@ -107,16 +107,45 @@ int spb6_caldt(spb6_ctx* ctx, int acoustic, double* inv_rate);
 /* bound_cond_u on the state (PHY:926-957). */
 int spb6_bc_fill(spb6_ctx* ctx);
 
-/* Explicit part of one stage with Qe = Qs = U and step adt: TVDReconstruction (PC:349) + Update (PC:251). */
+/* Explicit part of the first stage of an attempt (Qe = Qs = U) with step adt: TVDReconstruction (PC:349) + Update
+ * (PC:251); a later stage's operands go through spb6_stage_combine / spb6_stage_update below. */
 int spb6_explicit_stage(spb6_ctx* ctx, double adt);
-/* Implicit part of that stage: PressureSubsystem (PC:851-993) = stage prep, shock budget, Newton_Solver (PC:761) with
- * Thomas (PC:807-812) or BiCGStab (PC:725-758), final state. The stage's Newton guess is the current pressure. */
+/* Implicit part of that first stage (Qs = U, stage derivative Hs[0]): PressureSubsystem (PC:851-993) = stage prep, shock
+ * budget, Newton_Solver (PC:761) with Thomas (PC:807-812) or BiCGStab (PC:725-758), final state. The stage's Newton
+ * guess is the current pressure. Any stage: spb6_stage_pressure below. */
 int spb6_implicit_stage(spb6_ctx* ctx, double adt, int* newton_iters, double* residual);
 /* Only the stage preparation and shock budget of PressureSubsystem (PC:859-969), for per-kernel checks. */
 int spb6_stage_prep(spb6_ctx* ctx, double adt);
 /* One attempt of a step of size dt with an rk-stage table (body of the attempt loop, PC:1243-1285); on success the
  * state and pressure advance. */
 int spb6_step(spb6_ctx* ctx, double dt, int rk);
+/* ---- one attempt of the reference's step, stage by stage (the attempt body PC:1243-1285), for a caller that keeps its
+ * own time loop (PC:1230-1296). Ae, Ai are the caller's Butcher tables (set_tab, PC:1195-1218: rk 1, 2 or 3 stages):
+ *
+ *   spb6_attempt_begin(ctx);                                     u_con_n = u_con, Pst = P             (PC:1247-1250)
+ *   for (rks = 0; rks < rk; rks++) {
+ *       spb6_stage_combine(ctx, rks, dt, Ae[rks], Ai[rks]);       Qe, Qs from the stage derivatives     (PC:1252-1266)
+ *       ok = spb6_stage_update(ctx, Ai[rks][rks] * dt) == 0;      TVDReconstruction(Qe); Update(Ust, Qe, Qs)   (PC:1268-1271)
+ *       ok = ok && spb6_stage_pressure(ctx, Ai[rks][rks] * dt, rks, &it, &res) == 0;
+ *                                                                  PressureSubsystem(Ust, Qs, Qe, Pst, adt, rks) and
+ *                                                                  Hs[rks] = (Ust - Qs) / adt          (PC:1273-1284)
+ *   }
+ *   ok ? spb6_attempt_accept(ctx, dt) : halve dt and retry         u_con = Ust, P = Pst, t += dt       (PC:1286-1291)
+ *
+ * Each call waits for its own kernels and reports its own failure, so a replay gives the bits of spb6_run. */
+/* u_con_n = u_con and Pst = P: the stage operands start from the state of the step (PC:1247-1250). */
+int spb6_attempt_begin(spb6_ctx* ctx);
+/* Qe = U + dt sum_{r<rks} ae[r] Hs[r], Qs = U + dt sum_{r<rks} ai[r] Hs[r], then bound_cond_u on both (PC:1252-1266);
+ * rks = 0: Qe = Qs = U. rks <= 2; Hs[2] exists only in a context created with rk = 3. */
+int spb6_stage_combine(spb6_ctx* ctx, int rks, double dt, const double* ae, const double* ai);
+/* TVDReconstruction(Qe) (PC:349) + Update(Ust, Qe, Qs) (PC:251) with step adt into X*; SPB6_RHO_NONPOSITIVE where the
+ * reference's Update returns false (PC:255-258). */
+int spb6_stage_update(spb6_ctx* ctx, double adt);
+/* PressureSubsystem(Ust, Qs, Qe, Pst, adt, rks) (PC:851-993): stage data, shock budget, Newton from the current Pst,
+ * final state into Ust, and the stage derivative Hs[rks] = (Ust - Qs) / adt (PC:1276-1284). */
+int spb6_stage_pressure(spb6_ctx* ctx, double adt, int rks, int* newton_iters, double* residual);
+/* u_con = Ust, P = Pst, t += dt, steps + 1 (PC:1286, PC:1291). */
+int spb6_attempt_accept(spb6_ctx* ctx, double dt);
 /* The time loop (PC:1230-1296): dt from caldt with the start-up ramp, reject-and-halve, until t_end or max_steps
  * accepted steps (max_steps < 0: no cap). */
 int spb6_run(spb6_ctx* ctx, double t_end, long max_steps);

@@ -147,3 +147,21 @@ def test_dt_underflow(S, O):
     assert rco == 2 and o.stats()["status"] == 3
     assert rcs == S.STATUS_CODE["DT_UNDERFLOW"]
     assert st.stats()["steps"] == 0 and st.stats()["status"] == S.STATUS_CODE["DT_UNDERFLOW"]
+
+
+@pytest.mark.parametrize("which", ["tube", "bubble"])
+def test_ssp2_332_table_bitwise(S, O, which):
+    """rk = 3, the reference's SSP2(3,3,2) table (PC:1206-1216): three stages per step, the third combining both earlier
+    stage derivatives, bit for bit against the oracle"""
+    from diffusionimex_amd import cases
+    if which == "tube":
+        cfg, U = cases.water_air_tube(200, cfl=0.2, newton_tol=1e-13, rk=3)
+        steps = 40
+    else:
+        cfg, U = cases.shock_bubble(48, 24, seed=3, cfl=0.5, rk=3)
+        steps = 25
+    o, st, rco, rcs = _run_both(S, O, cfg, U, steps)
+    assert rco == 0 and rcs == 0
+    s = st.stats()
+    assert s["rejects"] == 0 and s["steps"] > s["rk1"] and s["stages"] == s["rk1"] + 3 * (s["steps"] - s["rk1"])
+    _assert_same(f"{which}_rk3", o, st)
