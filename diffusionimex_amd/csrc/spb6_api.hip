@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 #include "spb6_kernels.h"
 #include "spb6_mlp.h"
 #include "../../include/spb6_imex.h"
@@ -583,6 +584,35 @@ double pbar_host(const Phys& P, const double* Q) {
 	return a1 * p_ame(P, 0, a1, Q[IM1], Q[IE1]) + (1 - a1) * p_ame(P, 1, 1 - a1, Q[IM2], Q[IE2]);
 }
 
+// ---- the allocation-free linear solve (spb6_linsolve_device): its work vectors live in a file-scope device array
+// of a fixed size, resolved once per device; see include/spb6_imex.h
+constexpr long NA_ROWS = SPB6_LINSOLVE_DEVICE_MAX_ROWS;
+constexpr long NA_STRIDE = NA_ROWS / 256 + 2;             // partial-sum slots of the 256-row blocks, as spb6_create sizes them
+constexpr long NA_KS = 6 * NA_ROWS + KRYLOV_SLOTS * NA_STRIDE;   // offset of the loop scalars (doubles)
+constexpr long NA_DOUBLES = NA_KS + (long)((sizeof(KS) + sizeof(double) - 1) / sizeof(double));
+
+}   // namespace
+
+// six BiCGStab work vectors (r, r0, p, v, s, t; Thomas uses the first two), the partial sums and the loop scalars:
+// about 12.7 MB per device, held by every device the library runs on until the process ends
+static __device__ __align__(256) double scratch[NA_DOUBLES];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[64] = {};
+// the device address of scratch on the current device (hipGetSymbolAddress once per device, then cached); the array's
+// name in host code is only a host-side stand-in and must never be passed to a kernel
+double* scratch_ptr(int& dev) {
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) { g_err = "spb6_linsolve_device: no current device"; return nullptr; }
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	if (!g_scratch_ptr[dev]) {
+		void* p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p) { g_err = std::string("spb6_linsolve_device: hipGetSymbolAddress: ") + hipGetErrorString(e); return nullptr; }
+		g_scratch_ptr[dev] = static_cast<double*>(p);
+	}
+	return g_scratch_ptr[dev];
+}
 }   // namespace
 
 // ================================================================================================ C ABI
@@ -927,6 +957,48 @@ int spb6_linsolve(spb6_ctx* c, const double* val, const double* b, double* x, in
 	HIPCHK(hipStreamSynchronize(c->st));
 	if (iters) *iters = it;
 	return ok ? SPB6_OK : SPB6_KRYLOV_FAIL;
+}
+
+int spb6_linsolve_device(int nx, int ny, const int* bc, const double* J, const double* b, double* x, double tol, int maxit,
+                         int use_thomas, int* iters, void* stream) {
+	if (nx < 1 || ny < 1 || !bc || !J || !b || !x || maxit < 0) { g_err = "spb6_linsolve_device: bad argument"; return SPB6_BAD_ARG; }
+	const long n = (long)nx * ny;
+	if (n > NA_ROWS) {
+		g_err = "spb6_linsolve_device: " + std::to_string(n) + " rows exceed the maximum of " + std::to_string(NA_ROWS);
+		return SPB6_BAD_ARG;
+	}
+	if (use_thomas && ny != 1) { g_err = "spb6_linsolve_device: Thomas needs a 1D row (ny = 1)"; return SPB6_BAD_ARG; }
+	int dev = 0;
+	double* S = scratch_ptr(dev);
+	if (!S) return SPB6_HIP_ERROR;
+	hipStream_t st = static_cast<hipStream_t>(stream);
+	Geo g{};
+	g.nx = nx; g.ny = ny; g.NXT = nx + 2 * GH; g.NYT = ny + 2 * GH; g.NC = g.NXT * g.NYT; g.n = (int)n;
+	g.twoD = ny > 1; g.xper = bc[0] == SPB6_BC_PERIODIC; g.yper = bc[2] == SPB6_BC_PERIODIC;
+	for (int q = 0; q < 4; q++) g.bc[q] = bc[q];
+	if (use_thomas) {
+		launch_thomas((int)n, J, b, x, S, S + NA_ROWS, st);
+		HIPCHK(hipStreamSynchronize(st));
+		if (iters) *iters = 0;
+		return SPB6_OK;
+	}
+	// every buffer is written before it is read: setup 1 writes the partials setup 2 reads, setup 2 sets the loop scalars
+	// (done, iteration count, rho, alpha, omega of iteration 0) and x, p, v, setup 3 r and r0; nothing from an earlier call
+	// is used
+	KS* ks = reinterpret_cast<KS*>(S + NA_KS);
+	KrylovBufs B{x, S, S + NA_ROWS, S + 2 * NA_ROWS, S + 3 * NA_ROWS, S + 4 * NA_ROWS, S + 5 * NA_ROWS, J, S + 6 * NA_ROWS,
+	             (int)((n + 255) / 256) + 1, ks};
+	launch_krylov_setup(g, B, b, st);
+	KS hk{};
+	int launched = 0;
+	for (;;) {
+		for (int k = 0; k < 4 && launched <= maxit; k++, launched++) launch_krylov_iter(g, B, st, launched, tol, maxit);
+		HIPCHK(hipMemcpyAsync(&hk, ks, sizeof(KS), hipMemcpyDeviceToHost, st));
+		HIPCHK(hipStreamSynchronize(st));
+		if (hk.done || launched > maxit) break;
+	}
+	if (iters) *iters = hk.iters;
+	return hk.done && hk.status == 0 ? SPB6_OK : SPB6_KRYLOV_FAIL;
 }
 
 long spb6_get_field(spb6_ctx* c, const char* name, double* out, long capacity) {

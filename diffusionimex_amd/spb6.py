@@ -59,7 +59,7 @@ EXPORTS = ["spb6_create", "spb6_destroy", "spb6_last_error", "spb6_set_state", "
            "spb6_set_pressure", "spb6_caldt", "spb6_bc_fill", "spb6_explicit_stage", "spb6_implicit_stage",
            "spb6_stage_prep", "spb6_step", "spb6_attempt_begin", "spb6_stage_combine", "spb6_stage_update",
            "spb6_stage_pressure", "spb6_attempt_accept", "spb6_run", "spb6_get_stats", "spb6_time", "spb6_residual",
-           "spb6_jacobian", "spb6_linsolve", "spb6_get_field", "spb6_enable_timing", "spb6_set_timing_mask", "spb6_get_kernel_times",
+           "spb6_jacobian", "spb6_linsolve", "spb6_linsolve_device", "spb6_get_field", "spb6_enable_timing", "spb6_set_timing_mask", "spb6_get_kernel_times",
            "spb6_mlp_create", "spb6_mlp_destroy", "spb6_mlp_nparams", "spb6_mlp_set_params", "spb6_mlp_get_params",
            "spb6_mlp_set_data", "spb6_mlp_train", "spb6_mlp_forward", "spb6_mlp_gradients", "spb6_mlp_infer_bf16",
            "spb6_set_guess_model", "spb6_guess_residual", "spb6_dump_enable", "spb6_dump_fetch",
@@ -105,6 +105,7 @@ def load():
     L.spb6_residual.argtypes = [vp, dp, dp]
     L.spb6_jacobian.argtypes = [vp, dp, dp, dp, dp]
     L.spb6_linsolve.argtypes = [vp, dp, dp, dp, C.c_int, ip]
+    L.spb6_linsolve_device.argtypes = [C.c_int, C.c_int, ip, vp, vp, vp, C.c_double, C.c_int, C.c_int, ip, vp]
     L.spb6_get_field.restype = C.c_long
     L.spb6_get_field.argtypes = [vp, C.c_char_p, dp, C.c_long]
     L.spb6_enable_timing.argtypes = [vp, C.c_int]

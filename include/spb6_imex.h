@@ -161,6 +161,20 @@ int spb6_jacobian(spb6_ctx* ctx, const double* P, const double* G, double* val, 
  * (PC:807-812). */
 int spb6_linsolve(spb6_ctx* ctx, const double* val, const double* b, double* x, int use_thomas, int* iters);
 
+/* No counterpart in the reference: an allocation-free form of spb6_linsolve for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The caller passes device
+ * buffers: J[5 * n] the five diagonals (self, W, E, S, N; J[w * n + r]) of the 5-point (2D) / 3-point (1D) rows as
+ * spb6_jacobian gives them, b[n], x[n] the solution, on an nx x ny grid with boundary kinds bc[4] (periodic sides wrap).
+ * BiCGStab (use_thomas = 0: tolerance tol |b|, at most maxit iterations, the reference's Jacobi + coarse-space
+ * preconditioner, GMRES_Solver PC:725-758) or Thomas (use_thomas = 1, ny = 1, PC:807-812); the same kernels and sums as
+ * spb6_linsolve, so the same bits. Work vectors live in a fixed device array of the library (about 12.7 MB per device,
+ * held until the process ends): at most SPB6_LINSOLVE_DEVICE_MAX_ROWS rows (SPB6_BAD_ARG above it, no work done). No
+ * hipMalloc, hipFree or other allocation. stream: a hipStream_t (NULL: the null stream); the call waits for it. Calls
+ * on one device share the array: calls that overlap on one device (other threads, other streams) race. */
+#define SPB6_LINSOLVE_DEVICE_MAX_ROWS 262144
+int spb6_linsolve_device(int nx, int ny, const int* bc, const double* J, const double* b, double* x, double tol, int maxit,
+                         int use_thomas, int* iters, void* stream);
+
 /* Copy a named device field to the host (ghosted fields: (nx+4)*(ny+4) values, grids 8x that in AoS order).
  * Names: U Ust Qe Qs wxL wxR wyL wyR phx phy a2s ubx uby ust vst cz Kw ubsx kpx wLx pvx ubsy kpy wLy pvy Qph P Pst.
  * The face states wxL wxR wyL wyR (TVDReconstruction's, PC:392-445) are rebuilt from the last explicit stage's Qe for
