@@ -103,6 +103,8 @@ struct spb6_ctx {
 	int *slist = nullptr, *scount = nullptr;   // strong-shock cells of the shock budget's second pass (scount[1]: its cursor)
 	double* srec = nullptr;                    // their profile problems (SHOCK_REC x n)
 	unsigned long long* shock_steps = nullptr; // RK45 steps of the strong-shock passes since timing was (re)enabled
+	unsigned long long* tmax = nullptr;        // running maxima of the phase temperatures (2 x 64 striped slots)
+	double M0[3] = {0, 0, 0};                  // conserved totals of the state set last (PC:1166)
 	int *rlist = nullptr, *rcount = nullptr;   // cells whose residual the learned guess can change
 	int cus = 256;                      // compute units of the device
 	int shock_blocks = 1024;            // resident 256-thread blocks of the strong-shock pass
@@ -483,7 +485,7 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 	}
 	if (c->final_pending) {   // the previous stage's final state, whose failure comes first
 		c->final_pending = false;
-		if (c->hs->flags[FLAG_FINAL_PREV]) return SPB6_CLOSURE_FAIL;
+		if (c->hs->flags[FLAG_FINAL_PREV]) { c->stats.closure_fails++; return SPB6_CLOSURE_FAIL; }
 		c->stats.stages++;
 	}
 	if (c->hs->flags[FLAG_RHO]) return SPB6_RHO_NONPOSITIVE;
@@ -518,16 +520,16 @@ int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& it
 	}
 	c->stats.maxres = std::max(c->stats.maxres, res);
 	if (defer_final) {
-		{ TRegion tr(c, T_FINAL); launch_final(D, c->Pst, Qs, adt, Hk, c->st, FLAG_FINAL_PREV); }
+		{ TRegion tr(c, T_FINAL); launch_final(D, c->Pst, Qs, adt, Hk, c->tmax, c->st, FLAG_FINAL_PREV); }
 		launch_bc_grid(c->g, c->Ust, c->st);
 		c->final_pending = true;
 		return SPB6_OK;
 	}
 	rc = reset_scalars(c); if (rc) return rc;
-	{ TRegion tr(c, T_FINAL); launch_final(D, c->Pst, Qs, adt, Hk, c->st); }
+	{ TRegion tr(c, T_FINAL); launch_final(D, c->Pst, Qs, adt, Hk, c->tmax, c->st); }
 	launch_bc_grid(c->g, c->Ust, c->st);
 	rc = fetch(c); if (rc) return rc;
-	if (c->hs->flags[FLAG_FINAL]) return SPB6_CLOSURE_FAIL;
+	if (c->hs->flags[FLAG_FINAL]) { c->stats.closure_fails++; return SPB6_CLOSURE_FAIL; }
 	c->stats.stages++;
 	return SPB6_OK;
 }
@@ -635,7 +637,7 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	g.twoD = cfg->ny > 1; g.xper = cfg->bc[0] == SPB6_BC_PERIODIC; g.yper = cfg->bc[2] == SPB6_BC_PERIODIC;
 	for (int q = 0; q < 4; q++) { g.bc[q] = cfg->bc[q]; g.bcu[q] = cfg->bcu[q]; }
 	g.dx = (cfg->x1 - cfg->x0) / cfg->nx; g.dy = (cfg->y1 - cfg->y0) / cfg->ny;
-	for (int k = 0; k < 2; k++) { c->ph.g[k] = cfg->ph[k].gamma; c->ph.pi[k] = cfg->ph[k].pi_inf; c->ph.q[k] = cfg->ph[k].q; c->ph.b[k] = cfg->ph[k].b; c->ph.cv[k] = cfg->ph[k].cv; }
+	for (int k = 0; k < 2; k++) { c->ph.g[k] = cfg->ph[k].gamma; c->ph.pi[k] = cfg->ph[k].pi_inf; c->ph.q[k] = cfg->ph[k].q; c->ph.b[k] = cfg->ph[k].b; c->ph.cv[k] = cfg->ph[k].cv; c->ph.qp[k] = cfg->ph[k].qp; }
 	const size_t NC = g.NC, n = g.n;
 	g_err.clear();
 	double** grids[] = {&c->U, &c->Ust, &c->Qe, &c->Qs, &c->H[0], &c->H[1], &c->donx};   // donors: 4 fields x 2 = NV per cell
@@ -666,6 +668,8 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (dalloc(c, &c->Pplain, n) || dalloc(c, &c->Gplain, n)) return fail("alloc");
 	if (hipMalloc((void**)&c->slist, (n + 2) * sizeof(int)) != hipSuccess) return fail("alloc"); c->allocs.push_back(c->slist);
 	if (dalloc(c, &c->srec, (size_t)SHOCK_REC * n)) return fail("alloc");
+	if (hipMalloc((void**)&c->tmax, 128 * sizeof(unsigned long long)) != hipSuccess) return fail("alloc");
+	c->allocs.push_back(c->tmax);
 	if (hipMalloc((void**)&c->shock_steps, sizeof(unsigned long long)) != hipSuccess) return fail("alloc");
 	c->allocs.push_back(c->shock_steps);
 	if (hipMemset(c->shock_steps, 0, sizeof(unsigned long long)) != hipSuccess) return fail("alloc");
@@ -712,6 +716,12 @@ int spb6_set_state(spb6_ctx* c, const double* U_aos) {
 	HIPCHK(hipMemcpyAsync(c->Pst, P.data(), P.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
 	HIPCHK(hipStreamSynchronize(c->st));
 	c->t = 0.0; c->dtprev = -1.0; c->startup = true; c->last_krylov = 0;
+	HIPCHK(hipMemset(c->tmax, 0, 128 * sizeof(unsigned long long)));
+	c->M0[0] = c->M0[1] = c->M0[2] = 0.0;   // conserved_totals (PC:1104-1107) in the reference's cell order
+	for (int j = 0; j < g.ny; j++) for (int i = 0; i < g.nx; i++) {
+		const double* Q = &U_aos[((size_t)i + (size_t)j * g.nx) * NV];
+		c->M0[0] += Q[IM1]; c->M0[1] += Q[IM2]; c->M0[2] += Q[IEN];
+	}
 	c->guess_skip = c->guess_losses = c->guess_backoff = 0;
 	std::memset(&c->stats, 0, sizeof c->stats);
 	return SPB6_OK;
@@ -879,8 +889,81 @@ int spb6_run(spb6_ctx* c, double t_end, long max_steps) {
 	return SPB6_OK;
 }
 
-int spb6_get_stats(spb6_ctx* c, spb6_stats_t* s) { if (!c || !s) return SPB6_BAD_ARG; *s = c->stats; s->t = c->t; return SPB6_OK; }
+int spb6_get_stats(spb6_ctx* c, spb6_stats_t* s) {
+	if (!c || !s) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	unsigned long long tm[128];
+	HIPCHK(hipMemcpyAsync(tm, c->tmax, sizeof tm, hipMemcpyDeviceToHost, c->st));
+	HIPCHK(hipStreamSynchronize(c->st));
+	*s = c->stats; s->t = c->t;
+	for (int k = 0; k < 2; k++) {
+		unsigned long long m = 0;
+		for (int q = 0; q < 64; q++) m = std::max(m, tm[64 * k + q]);
+		std::memcpy(&s->tmax[k], &m, sizeof m);
+	}
+	return SPB6_OK;
+}
 double spb6_time(spb6_ctx* c) { return c ? c->t : 0.0; }
+
+int spb6_get_profile(spb6_ctx* c, double* out) {
+	if (!c || !out) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	const Geo& g = c->g;
+	const size_t n = g.n;
+	// an output routine: its 15 n doubles of device memory are allocated for the call (as spb6_get_field's face states)
+	double* tmp = nullptr;
+	HIPCHK(hipMalloc((void**)&tmp, n * 15 * sizeof(double)));
+	launch_cell_diag(g, c->ph, c->U, tmp, c->st);
+	std::vector<double> d(n * 15);
+	const hipError_t e1 = hipMemcpyAsync(d.data(), tmp, d.size() * sizeof(double), hipMemcpyDeviceToHost, c->st);
+	const hipError_t e2 = hipStreamSynchronize(c->st);
+	(void)hipFree(tmp);
+	HIPCHK(e1); HIPCHK(e2);
+	for (int j = 0; j < g.ny; j++) for (int i = 0; i < g.nx; i++) {
+		const size_t q = (size_t)i + (size_t)j * g.nx;
+		double* o = out + q * 17;
+		o[0] = c->cfg.x0 + (i + 0.5) * g.dx; o[1] = c->cfg.y0 + (j + 0.5) * g.dy;
+		for (int k = 0; k < 15; k++) o[2 + k] = d[q * 15 + k];
+	}
+	return SPB6_OK;
+}
+int spb6_write_profile(spb6_ctx* c, const char* path, int prec) {
+	if (!c || !path || prec < 1 || prec > 17) return SPB6_BAD_ARG;
+	std::vector<double> o((size_t)c->g.n * 17);
+	int rc = spb6_get_profile(c, o.data()); if (rc) return rc;
+	FILE* f = std::fopen(path, "w");
+	if (!f) { g_err = std::string("cannot open ") + path; return SPB6_BAD_ARG; }
+	const int P = prec, Q = std::min(prec, 8), R = std::min(prec, 10);
+	std::fprintf(f, "# x y p u v rho alpha2 Y2 rho1 rho2 c_frozen c_wood T1 T2 p1 p2 g1-g2\n");
+	for (size_t q = 0; q < (size_t)c->g.n; q++) {
+		const double* v = &o[q * 17];
+		std::fprintf(f, "%.10e %.10e %.*e %.*e %.*e %.*e %.*e %.*e %.*e %.*e %.*e %.*e %.*e %.*e %.*e %.*e %.*e\n", v[0], v[1], P, v[2], P, v[3],
+		             P, v[4], P, v[5], P, v[6], P, v[7], P, v[8], P, v[9], Q, v[10], Q, v[11], R, v[12], R, v[13], P, v[14], P, v[15], Q, v[16]);
+	}
+	return std::fclose(f) == 0 ? SPB6_OK : SPB6_BAD_ARG;
+}
+int spb6_conserved_totals(spb6_ctx* c, double M[3]) {
+	if (!c || !M) return SPB6_BAD_ARG;
+	std::vector<double> U((size_t)c->g.n * NV);
+	int rc = spb6_get_state(c, U.data()); if (rc) return rc;
+	M[0] = M[1] = M[2] = 0.0;
+	for (size_t q = 0; q < (size_t)c->g.n; q++) { M[0] += U[q * NV + IM1]; M[1] += U[q * NV + IM2]; M[2] += U[q * NV + IEN]; }
+	return SPB6_OK;
+}
+int spb6_stats_line(spb6_ctx* c, int case_id, double wall_s, char* buf, long len) {
+	if (!c || !buf || len < 1) return SPB6_BAD_ARG;
+	spb6_stats_t s;
+	int rc = spb6_get_stats(c, &s); if (rc) return rc;
+	double M1[3];
+	rc = spb6_conserved_totals(c, M1); if (rc) return rc;
+	std::snprintf(buf, (size_t)len,
+	              "case %d mode 1 order %d rk %d N %d CFL %g | steps %ld (rk1 %ld) rejects %ld floor %ld | stages %ld newton it %ld (%.2f/stage) "
+	              "maxres %.2e Tmax %.4g %.4g | closure fails %ld | dM1/M %.2e dM2/M %.2e dE/E %.2e | wall %.2f s",
+	              case_id, c->cfg.order, c->cfg.rk, c->cfg.nx, c->cfg.cfl, s.steps, s.rk1, s.rejects, s.floor_acc, s.stages, s.newton_it,
+	              s.stages ? (double)s.newton_it / s.stages : 0.0, s.maxres, s.tmax[0], s.tmax[1], s.closure_fails,
+	              (M1[0] - c->M0[0]) / c->M0[0], (M1[1] - c->M0[1]) / c->M0[1], (M1[2] - c->M0[2]) / c->M0[2], wall_s);
+	return SPB6_OK;
+}
 
 int spb6_residual(spb6_ctx* c, const double* P, double* G) {
 	if (!c || !P || !G) return -1;

@@ -501,7 +501,7 @@ spb6_batch* spb6_batch_create(const spb6_batch_problem_t* probs, int nprob, cons
 		B.L = q.nx; B.off = (int)off; B.poff = (int)poff;
 		B.bc[0] = q.bc[0]; B.bc[1] = q.bc[1]; B.bcu[0] = q.bcu[0]; B.bcu[1] = q.bcu[1];
 		B.dx = (q.x1 - q.x0) / q.nx;
-		for (int k = 0; k < 2; k++) { B.ph.g[k] = q.ph[k].gamma; B.ph.pi[k] = q.ph[k].pi_inf; B.ph.q[k] = q.ph[k].q; B.ph.b[k] = q.ph[k].b; B.ph.cv[k] = q.ph[k].cv; }
+		for (int k = 0; k < 2; k++) { B.ph.g[k] = q.ph[k].gamma; B.ph.pi[k] = q.ph[k].pi_inf; B.ph.q[k] = q.ph[k].q; B.ph.b[k] = q.ph[k].b; B.ph.cv[k] = q.ph[k].cv; B.ph.qp[k] = q.ph[k].qp; }
 		off += q.nx + 2 * GH; poff += q.nx; Lmax = std::max(Lmax, q.nx);
 	}
 	if (off >= (1L << 31) / NV) return fail();

@@ -462,7 +462,7 @@ struct FaceIn { double wl, pv, ubs, kp, ube, cl[4], cr[4], utl, utr, bw; int wal
 struct CellIn { double xs[NV], a2s, Q0, Q1, Kw; FaceIn fe, fw, fn, fs; };
 template <class T> struct FaceValsT { T uf, pf, wL, wR; };
 using FaceVals = FaceValsT<double>;
-struct ResOut { double a0, a1, m0, m1, e0, e1, mx, my, E, sig; };
+struct ResOut { double a0, a1, m0, m1, e0, e1, mx, my, E, sig, p; };
 
 __device__ __forceinline__ void load_face(const Dev& D, bool ydir, int c, int idx, FaceIn& f) {
 	const Geo& g = D.g; const int NC = g.NC, cn = c + (ydir ? g.NXT : 1);
@@ -563,7 +563,7 @@ SPB_HD bool cell_res(const Phys& P, double P_REF, double rx, double ry, bool two
 	in.s = s; in.rhoeC = E - 0.5 * (mx * mx + my * my) / rho; in.pguess = P_REF + Pc; in.Q0 = Q0; in.Q1 = Q1;
 	ClOut o; if (!closure_p(P, in, o)) return false;
 	G = (o.pbar - P_REF) - Pc;
-	if (out) { out->a0 = o.a0; out->a1 = o.a1; out->m0 = X[2]; out->m1 = X[3]; out->e0 = o.e0; out->e1 = o.e1; out->mx = mx; out->my = my; out->E = E; out->sig = want_sig ? closure_roundoff(P, in, o.pbar) : 0.0; }
+	if (out) { out->a0 = o.a0; out->a1 = o.a1; out->m0 = X[2]; out->m1 = X[3]; out->e0 = o.e0; out->e1 = o.e1; out->mx = mx; out->my = my; out->E = E; out->sig = want_sig ? closure_roundoff(P, in, o.pbar) : 0.0; out->p = o.pbar; }
 	return isfin(G);
 }
 // the same on a cell whose data is already loaded (the finite-difference Jacobian evaluates one cell five times)
@@ -729,9 +729,11 @@ __device__ __forceinline__ bool floor_ok_at(const Dev& D, const double* __restri
 	return !(fabs(g) > tol * (fabs(D.P_REF + P[q]) + Kw + 1.0) + o.sig);
 }
 // final stage state of row q at the converged pressure (PC:975-987) and the stage derivative Hs = (U_s - Qs) / adt
-// (PC:1276-1284); false where the closure fails
+// (PC:1276-1284); false where the closure fails. T0, T1: the phase temperatures of the closure (PHY:491) where the phase
+// is present and T finite -- the reference's T_max statistics, PC:986-989 -- else 0
 template <bool NT = false>
-__device__ __forceinline__ bool final_at(const Dev& D, const double* __restrict__ P, const double* __restrict__ Qs, double adt, double* __restrict__ H, int q) {
+__device__ __forceinline__ bool final_at(const Dev& D, const double* __restrict__ P, const double* __restrict__ Qs, double adt, double* __restrict__ H, int q,
+                                         double* T0 = nullptr, double* T1 = nullptr) {
 	const Geo& g = D.g; const int NC = g.NC, c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
 	const Nb b = neighbours(g, q);
 	const CellFaces F(D, q);
@@ -743,6 +745,11 @@ __device__ __forceinline__ bool final_at(const Dev& D, const double* __restrict_
 	const double u[NV] = {o.a0, o.m0, o.m1, o.mx, o.my, o.e0, o.e1, o.E};
 	#pragma unroll
 	for (int k = 0; k < NV; k++) { st_out<NT>(D.Ust + k * NC + c, u[k]); st_out<NT>(H + k * NC + c, (u[k] - Qs[k * NC + c]) / adt); }
+	if (T0) {
+		const double t0 = T_amp(D.ph, 0, o.a0, o.m0, o.p), t1 = T_amp(D.ph, 1, o.a1, o.m1, o.p);
+		*T0 = (o.a0 > 1e-9 && isfin(t0)) ? smax(t0, 0.0) : 0.0;
+		*T1 = (o.a1 > 1e-9 && isfin(t1)) ? smax(t1, 0.0) : 0.0;
+	}
 	return true;
 }
 // caldt terms of row q (PC:477-488): |u| + c and |v| + c (c only when acoustic) for admissible primitives, else 0

@@ -135,7 +135,7 @@ SPB_HD double phypot(double a, double b) {
 
 // ------------------------------------------------------------------ phases (stiffened gas / NASG covolume b)
 struct Phys {
-	double g[2], pi[2], q[2], b[2], cv[2];
+	double g[2], pi[2], q[2], b[2], cv[2], qp[2];
 };
 
 SPB_HD double eps_amp(const Phys& P, int k, double a, double m, double p) {          // PHY:69-71
@@ -158,6 +158,40 @@ SPB_HD double p_mix(const Phys& P, double a1, double m1, double m2, double rhoe)
 	return num / den;
 }
 SPB_HD bool p_adm(const Phys& P, int k, double p) { return p + P.pi[k] > 0; }       // PHY:101-104 (SG)
+
+// phase temperature T(alpha, m, p) (PHY:75-77, SG/NASG)
+SPB_HD double T_amp(const Phys& P, int k, double a, double m, double p) {
+	return (a / m - P.b[k]) * (p + P.pi[k]) / ((P.g[k] - 1.0) * P.cv[k]); }
+// the profile columns of cell_diag (PHY:1105-1116) from the 8 conserved values: p u v rho alpha2 Y2 rho1 rho2 c_frozen
+// c_wood T1 T2 p1 p2 g1-g2, NaN where prim_of (PHY:837-847) fails. The Gibbs difference g1 - g2 (g_pT, PHY:78-81) takes
+// the device log; every other column is the reference's arithmetic operation for operation.
+SPB_HD void cell_diag(const Phys& P, const double* Q, double* o) {
+	const double a1 = Q[IA1], a2 = 1 - a1, m1 = Q[IM1], m2 = Q[IM2], rho = m1 + m2;
+	const double nan = dnan();
+	bool ok = a1 > 0 && a2 > 0 && m1 > 0 && m2 > 0;
+	double r1 = nan, r2 = nan, u = nan, v = nan, p1 = nan, p2 = nan, cf = nan;
+	if (ok) {
+		r1 = m1 / a1; r2 = m2 / a2; u = Q[IMU] / rho; v = Q[IMV] / rho;
+		p1 = p_ame(P, 0, a1, m1, Q[IE1]); p2 = p_ame(P, 1, a2, m2, Q[IE2]);
+		ok = p_adm(P, 0, p1) && p_adm(P, 1, p2);
+		if (ok) cf = sqrt((m1 * c2_rp(P, 0, r1, p1) + m2 * c2_rp(P, 1, r2, p2)) / rho);
+	}
+	double cw = nan, T1 = nan, T2 = nan, dg = nan;
+	if (ok) {
+		cw = sqrt(1.0 / (rho * (a1 / (r1 * c2_rp(P, 0, r1, p1)) + (1 - a1) / (r2 * c2_rp(P, 1, r2, p2)))));
+		T1 = T_amp(P, 0, a1, m1, p1); T2 = T_amp(P, 1, 1 - a1, m2, p2);
+		if (T1 > 0 && T2 > 0 && P.cv[0] > 0) {
+			auto gpt = [&](int k, double pp, double T) {
+				const double sk = P.cv[k] * (P.g[k] * log(T) - (P.g[k] - 1.0) * log(pp + P.pi[k])) + P.qp[k];
+				return P.g[k] * P.cv[k] * T + P.b[k] * pp + P.q[k] - T * sk;
+			};
+			dg = gpt(0, p1, T1) - gpt(1, p2, T2);
+		}
+	}
+	o[0] = ok ? a1 * p1 + (1 - a1) * p2 : nan; o[1] = ok ? u : nan; o[2] = ok ? v : nan; o[3] = rho; o[4] = 1 - a1; o[5] = m2 / rho;
+	o[6] = ok ? r1 : nan; o[7] = ok ? r2 : nan; o[8] = ok ? cf : nan; o[9] = cw; o[10] = T1; o[11] = T2; o[12] = ok ? p1 : nan;
+	o[13] = ok ? p2 : nan; o[14] = dg;
+}
 
 // frozen sound speed of prim_of (PHY:837-847); false where prim_of fails
 SPB_HD bool prim_cf(const Phys& P, double a1, double m1, double m2, double e1, double e2, double& cf) {

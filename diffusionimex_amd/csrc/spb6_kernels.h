@@ -82,8 +82,13 @@ void launch_defect_x(const Geo& g, const double* J, const double* b, const doubl
 void launch_chord_rhs(const Dev& D, const double* P, const double* G, double* sc, double* rhs, hipStream_t st, int* clear = nullptr, int nclear = 0);
 void launch_stepcrit(int n, const double* dP, const double* P, const double* sc, unsigned long long* nrm, hipStream_t st);
 void launch_floor(const Dev& D, const double* P, double tol, hipStream_t st);
-void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, hipStream_t st, int flag_slot = FLAG_FINAL);
+// tmax: 128 slots, the running maxima of the two phase temperatures (bit patterns of non-negative doubles, 64 striped
+// slots each)
+void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, unsigned long long* tmax, hipStream_t st,
+                  int flag_slot = FLAG_FINAL);
 void launch_caldt(const Geo& g, const Phys& P, const double* U, int acoustic, unsigned long long* nrm, hipStream_t st);
+// cell_diag's 15 columns of every interior cell of U into out[q * 15 + k]
+void launch_cell_diag(const Geo& g, const Phys& P, const double* U, double* out, hipStream_t st);
 void launch_thomas(int n, const double* J, const double* rhs, double* x, double* cp, double* dp, hipStream_t st);
 void launch_krylov_setup(const Geo& g, const KrylovBufs& B, const double* b, hipStream_t st);
 void launch_krylov_iter(const Geo& g, const KrylovBufs& B, hipStream_t st, int it, double tol, int maxit);

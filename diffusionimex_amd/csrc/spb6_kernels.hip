@@ -490,10 +490,24 @@ __global__ void __launch_bounds__(256) k_floor(Dev D, const double* __restrict__
 }
 // final stage state at the converged pressure (PC:975-987) and the stage derivative Hs = (U_s - Qs) / adt (PC:1276-1284)
 __global__ void __launch_bounds__(256) k_final(Dev D, const double* __restrict__ P, const double* __restrict__ Qs, double adt, double* __restrict__ H,
-                                              int* __restrict__ flags, int flag_slot) {
+                                              int* __restrict__ flags, int flag_slot, unsigned long long* __restrict__ tmax) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	if (q >= D.g.n) return;
-	if (!final_at(D, P, Qs, adt, H, q)) flags[flag_slot] = 1;
+	double T0 = 0.0, T1 = 0.0;
+	if (q < D.g.n && !final_at(D, P, Qs, adt, H, q, &T0, &T1)) flags[flag_slot] = 1;
+	atomic_max_nonneg(tmax, T0);      // the running maxima of the phase temperatures (slots 0..63 and 64..127)
+	atomic_max_nonneg(tmax + 64, T1);
+}
+// write_profile's cell columns (cell_diag, PHY:1105-1116) of every interior cell: out[q * 15 + k]
+__global__ void __launch_bounds__(256) k_cell_diag(Geo g, Phys P, const double* __restrict__ U, double* __restrict__ out) {
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= g.n) return;
+	const int c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
+	double Q[NV], o[15];
+	#pragma unroll
+	for (int k = 0; k < NV; k++) Q[k] = U[k * g.NC + c];
+	cell_diag(P, Q, o);
+	#pragma unroll
+	for (int k = 0; k < 15; k++) out[(size_t)q * 15 + k] = o[k];
 }
 // caldt maxima (PC:477-488): max |u| + c and |v| + c over interior cells with admissible primitives
 __global__ void __launch_bounds__(256) k_caldt(Geo g, Phys P, const double* __restrict__ U, int acoustic, unsigned long long* __restrict__ nrm) {
@@ -1253,8 +1267,11 @@ void launch_stepcrit(int n, const double* dP, const double* P, const double* sc,
 void launch_floor(const Dev& D, const double* P, double tol, hipStream_t st) {
 	hipLaunchKernelGGL(k_floor, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, tol, D.flags);
 }
-void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, hipStream_t st, int flag_slot) {
-	hipLaunchKernelGGL(k_final, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, Qs, adt, H, D.flags, flag_slot);
+void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, unsigned long long* tmax, hipStream_t st, int flag_slot) {
+	hipLaunchKernelGGL(k_final, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, Qs, adt, H, D.flags, flag_slot, tmax);
+}
+void launch_cell_diag(const Geo& g, const Phys& P, const double* U, double* out, hipStream_t st) {
+	hipLaunchKernelGGL(k_cell_diag, dim3(nblk(g.n)), dim3(256), 0, st, g, P, U, out);
 }
 void launch_caldt(const Geo& g, const Phys& P, const double* U, int acoustic, unsigned long long* nrm, hipStream_t st) {
 	hipLaunchKernelGGL(k_caldt, dim3(nblk(g.n)), dim3(256), 0, st, g, P, U, acoustic, nrm);

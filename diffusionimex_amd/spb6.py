@@ -42,7 +42,7 @@ class Stats(C.Structure):
                 ("stages", C.c_long), ("newton_it", C.c_long), ("krylov_it", C.c_long), ("linsolves", C.c_long),
                 ("maxres", C.c_double), ("t", C.c_double), ("dt_last", C.c_double), ("status", C.c_int),
                 ("guess_used", C.c_long), ("guess_fallback", C.c_long), ("guess_log10_gain", C.c_double),
-                ("guess_skipped", C.c_long)]
+                ("guess_skipped", C.c_long), ("tmax", C.c_double * 2), ("closure_fails", C.c_long)]
 
 
 class KernelTimes(C.Structure):
@@ -58,7 +58,8 @@ class KernelTimes(C.Structure):
 EXPORTS = ["spb6_create", "spb6_destroy", "spb6_last_error", "spb6_set_state", "spb6_get_state", "spb6_get_pressure",
            "spb6_set_pressure", "spb6_caldt", "spb6_bc_fill", "spb6_explicit_stage", "spb6_implicit_stage",
            "spb6_stage_prep", "spb6_step", "spb6_attempt_begin", "spb6_stage_combine", "spb6_stage_update",
-           "spb6_stage_pressure", "spb6_attempt_accept", "spb6_run", "spb6_get_stats", "spb6_time", "spb6_residual",
+           "spb6_stage_pressure", "spb6_attempt_accept", "spb6_run", "spb6_get_profile", "spb6_write_profile",
+           "spb6_conserved_totals", "spb6_stats_line", "spb6_get_stats", "spb6_time", "spb6_residual",
            "spb6_jacobian", "spb6_linsolve", "spb6_linsolve_device", "spb6_get_field", "spb6_enable_timing", "spb6_set_timing_mask", "spb6_get_kernel_times",
            "spb6_mlp_create", "spb6_mlp_destroy", "spb6_mlp_nparams", "spb6_mlp_set_params", "spb6_mlp_get_params",
            "spb6_mlp_set_data", "spb6_mlp_train", "spb6_mlp_forward", "spb6_mlp_gradients", "spb6_mlp_infer_bf16",
@@ -98,6 +99,10 @@ def load():
     L.spb6_stage_update.argtypes = [vp, C.c_double]
     L.spb6_stage_pressure.argtypes = [vp, C.c_double, C.c_int, ip, dp]
     L.spb6_attempt_accept.argtypes = [vp, C.c_double]
+    L.spb6_get_profile.argtypes = [vp, dp]
+    L.spb6_write_profile.argtypes = [vp, C.c_char_p, C.c_int]
+    L.spb6_conserved_totals.argtypes = [vp, dp]
+    L.spb6_stats_line.argtypes = [vp, C.c_int, C.c_double, C.c_char_p, C.c_long]
     L.spb6_run.argtypes = [vp, C.c_double, C.c_long]
     L.spb6_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.spb6_time.restype = C.c_double
@@ -257,7 +262,31 @@ class Stepper:
     def stats(self):
         s = Stats()
         self._check(self.L.spb6_get_stats(self.h, C.byref(s)), "get_stats")
-        return {k: getattr(s, k) for k, _ in Stats._fields_}
+        out = {k: getattr(s, k) for k, _ in Stats._fields_}
+        out["tmax"] = (s.tmax[0], s.tmax[1])
+        return out
+
+    # --- output (write_profile PC:1002-1013, conserved_totals PC:1104-1107, stats line PC:1337-1339)
+    PROFILE_COLUMNS = ("x", "y", "p", "u", "v", "rho", "alpha2", "Y2", "rho1", "rho2", "c_frozen", "c_wood", "T1", "T2", "p1", "p2",
+                       "g1-g2")
+
+    def profile(self):
+        out = np.empty((self.ny * self.nx, 17))
+        self._check(self.L.spb6_get_profile(self.h, _dp(out)), "get_profile")
+        return out
+
+    def write_profile(self, path, prec=16):
+        self._check(self.L.spb6_write_profile(self.h, str(path).encode(), int(prec)), "write_profile")
+
+    def conserved_totals(self):
+        M = np.zeros(3)
+        self._check(self.L.spb6_conserved_totals(self.h, _dp(M)), "conserved_totals")
+        return M
+
+    def stats_line(self, case_id, wall_s=0.0):
+        buf = C.create_string_buffer(512)
+        self._check(self.L.spb6_stats_line(self.h, int(case_id), float(wall_s), buf, 512), "stats_line")
+        return buf.value.decode()
 
     @property
     def t(self):

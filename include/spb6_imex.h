@@ -83,6 +83,9 @@ typedef struct spb6_stats {
     long guess_skipped;               /* stages of guess_fallback where the network was not evaluated: after two
                                          stages in a row where the plain guess won, 1, 2, 4, ... 32 stages are skipped
                                          before the network is tried again (a win resets the back-off) */
+    double tmax[2];                   /* the phases' largest temperatures in the stages' final states where the phase is
+                                         present (alpha > 1e-9), the reference's st_Tmax (PC:986-989) */
+    long closure_fails;               /* stages whose final state failed in the closure (an attempt rejected for it) */
 } spb6_stats_t;
 
 typedef struct spb6_ctx spb6_ctx;
@@ -181,6 +184,20 @@ int spb6_linsolve_device(int nx, int ny, const int* bc, const double* J, const d
  * the call (the stage itself keeps only the face donors the implicit stage reads); the call allocates their scratch.
  * Returns the number of doubles written, or -1. */
 long spb6_get_field(spb6_ctx* ctx, const char* name, double* out, long capacity);
+
+/* ---- output (the reference's write_profile, conserved_totals and stats line) */
+/* write_profile's columns (PC:1002-1013) of every interior cell of the current state, out[(i + j nx) * 17 + k]:
+ * x y p u v rho alpha2 Y2 rho1 rho2 c_frozen c_wood T1 T2 p1 p2 g1-g2 (cell_diag, PHY:1105-1116, computed on the device;
+ * NaN where prim_of fails; g1 - g2 takes the device log, every other column is the reference's arithmetic). */
+int spb6_get_profile(spb6_ctx* ctx, double* out);
+/* write_profile (PC:1002-1013): the file with the reference's header line and columns, prec significant decimals after
+ * the point for the state columns (OUTPREC, default 16 in the reference; 8 for c_frozen, c_wood, g1-g2; 10 for T1, T2). */
+int spb6_write_profile(spb6_ctx* ctx, const char* path, int prec);
+/* conserved_totals (PC:1104-1107): sum m1, sum m2, sum E over the interior cells in the reference's cell order. */
+int spb6_conserved_totals(spb6_ctx* ctx, double M[3]);
+/* The reference's run statistics line (PC:1337-1339) into buf: case id and wall time from the caller, the counters of
+ * spb6_get_stats, and dM1/M, dM2/M, dE/E against the totals of the state set last. */
+int spb6_stats_line(spb6_ctx* ctx, int case_id, double wall_s, char* buf, long len);
 
 /* ---- per-launch timing of the dominant kernels (HIP events on the context's stream) */
 typedef struct spb6_kernel_times {
