@@ -1,4 +1,5 @@
-"""Builds the gfx950 HIP library libspb6_hip.so in-tree with hipcc.
+"""Builds the gfx950 HIP library libspb6_hip.so in-tree with hipcc, and beside it the GPU self-check programs of the
+test suite (tests/bin/), with the same flags.
 
 Flags: -ffp-contract=off keeps every FP64 operation separately rounded (the reference's results are reproduced bit for bit);
 the packed-FP32 target feature is disabled so that no v_pk_*_f32 instruction is generated.
@@ -12,6 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspb6_hip.so")
 SOURCES = ["spb6_kernels.hip", "spb6_mlp.hip", "spb6_batch.hip", "spb6_api.hip"]
 ARCH = os.environ.get("SPB6_ARCH", "gfx950")
+TESTS = os.path.join(HERE, "..", "tests")
+CHECKS = [("quotient_check.hip", "quotient_check")]   # tests/<source> -> tests/bin/<program>
 
 
 def hipcc():
@@ -36,6 +39,14 @@ def build(verbose=False):
             print(" ".join(cmd))
         procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT), s))
         objs.append(o)
+    os.makedirs(os.path.join(TESTS, "bin"), exist_ok=True)
+    for src, exe in CHECKS:
+        s = os.path.join(TESTS, src)
+        if os.path.exists(s):
+            cmd = [hipcc()] + flags() + ["-o", os.path.join(TESTS, "bin", exe), s]
+            if verbose:
+                print(" ".join(cmd))
+            procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT), s))
     for p, s in procs:
         out, _ = p.communicate()
         if p.returncode != 0:

@@ -23,6 +23,8 @@ SPB_HD double dnan() { return __builtin_nan(""); }
 SPB_HD double smax(double a, double b) { return (a < b) ? b : a; }
 SPB_HD double smin(double a, double b) { return (b < a) ? b : a; }
 SPB_HD bool isfin(double x) { return __builtin_isfinite(x); }
+SPB_HD uint32_t umin(uint32_t a, uint32_t b) { return b < a ? b : a; }
+SPB_HD uint32_t umax(uint32_t a, uint32_t b) { return a < b ? b : a; }
 
 // forward-mode derivatives along N directions (value v, derivatives d), for the analytic Newton Jacobian. Division takes
 // one reciprocal (values need not match the double path bit for bit here); smax / smin with a constant take the right
@@ -229,6 +231,85 @@ SPB_HD bool prof_rhs(const Phys& P, const Prof& S, double lr0, double lr1, doubl
 	dy[0] = num / den; dy[1] = (r0 - ep0 * dy[0]) / evp0; dy[2] = p - Pr;
 	return adm && isfin(dy[0]) && isfin(dy[1]);
 }
+// Shared-reciprocal quotients (the strong-shock kernel's profile right-hand side). The compiler divides in FP64 by
+// div_scale, rcp, two Newton-Raphson steps on the reciprocal r, q = a * r, e = fma(-b, q, a), div_fmas and div_fixup.
+// When |a| and |b| both lie in [2^-384, 2^384) the two scale steps are identities (the exponent gap stays below 768,
+// neither operand nor the quotient comes near the subnormal or the overflow range), div_fmas is the plain fma and
+// div_fixup passes the value through, so quot(a, b, rcp_nr(b)) is the same correctly rounded a / b. rcp_nr depends on
+// b alone: a divisor with several numerators pays for one reciprocal, and each quotient then costs a multiply and two
+// fmas. A zero a gives a zero whose sign may differ from a / b's (-0 / b with b > 0 gives +0).
+SPB_HD double rcp_nr(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	double r = __builtin_amdgcn_rcp(b);
+	double e = fma(-b, r, 1.0);
+	r = fma(r, e, r);
+	e = fma(-b, r, 1.0);
+	return fma(r, e, r);
+#else
+	return 1.0 / b;   // the host never takes the shared-reciprocal path
+#endif
+}
+SPB_HD double quot(double a, double b, double r) { const double q = a * r; return fma(fma(-b, q, a), r, q); }
+// true in every lane of the wavefront when x is true in any (the value itself on the host)
+SPB_HD bool wave_any(bool x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+	return __ballot(x) != 0ull;
+#else
+	return x;
+#endif
+}
+// The exponent field of x where it lies in the high word (bits 20..30; the sign cleared). Bounds on |x| are taken as
+// bounds on this field: 2^-k <= |x| < 2^(k+1) exactly when expo_lo(k) <= field <= expo_hi(k); zeros and subnormals
+// have field 0, infinities and NaN 0x7ff00000, so they fall outside every band.
+SPB_HD uint32_t expo_field(double x) { return (uint32_t)(__builtin_bit_cast(uint64_t, x) >> 32) & 0x7ff00000u; }
+constexpr uint32_t expo_lo(int k) { return (uint32_t)(1023 - k) << 20; }
+constexpr uint32_t expo_hi(int k) { return (uint32_t)(1023 + k) << 20; }
+SPB_HD bool in_band(double x, int k) { const uint32_t e = expo_field(x); return e >= expo_lo(k) && e <= expo_hi(k); }
+constexpr int BAND_LEAF = 120, BAND_CONST = 8;
+// what the strong-shock pass holds for all its cells: 1 / (g_k - 1) and whether g_k and g_k - 1 lie in band 8
+struct ShockRcp { double rg0, rg1; bool ok; };
+SPB_HD ShockRcp shock_rcp(const Phys& P) {
+	ShockRcp c;
+	c.rg0 = rcp_nr(P.g[0] - 1.0); c.rg1 = rcp_nr(P.g[1] - 1.0);
+	c.ok = in_band(P.g[0], BAND_CONST) && in_band(P.g[1], BAND_CONST) && in_band(P.g[0] - 1.0, BAND_CONST) &&
+	       in_band(P.g[1] - 1.0, BAND_CONST);
+	return c;
+}
+// prof_rhs with the ten quotients by Y1, g0 - 1, g1 - 1, evp0 and evp1 taken from shared reciprocals (rY1 = rcp_nr(Y1)
+// is the step's, rc the pass's); num / den stays a division. Band k means 2^-k <= |x| < 2^(k+1). When rc.ok, Y0, Y1,
+// lr0, lr1 lie in band 120 (both checked by the caller) and the leaves v - Y0 v1, v1 - b0, v2 - b1, p + pi0, p + pi1,
+// r0 - ep0 dy0 and p - P (which may also be zero) lie in band 120, every numerator lies within [2^-360, 2^363) (the
+// widest, Y_k lr_k (p - P), is a product of three such factors; ep_k and g_k (p + pi_k) lie within [2^-128, 2^130)) and
+// every divisor (Y1, g_k - 1, evp_k) within [2^-136, 2^139): both operands of each quotient lie in [2^-384, 2^384), so
+// it is a / b bit for bit. At p = P the two lam terms have zero numerators; their quotients, zeros of either sign, are
+// added to num = -1 and leave it unchanged. Where a leaf is out of its band (NaN and infinities included), fast is
+// cleared and the caller evaluates the step's stages again with prof_rhs. The band test reads the exponent fields
+// with integer operations (half the issue cost of FP64 compares).
+SPB_HD bool prof_rhs_rcp(const Phys& P, const Prof& S, const ShockRcp& rc, double rY1, double lr0, double lr1, double v,
+                         const double* y, double* dy, bool& fast) {
+	const double p = y[0], v1 = y[1], a1 = v - S.Y0 * v1;
+	const double v2 = quot(a1, S.Y1, rY1), Pr = S.p0 + S.j2 * (S.v0 - v), dp = p - Pr;
+	const bool adm = eos_ok(P, 0, p, v1) && eos_ok(P, 1, p, v2);
+	const double g0m = P.g[0] - 1.0, g1m = P.g[1] - 1.0;
+	const double n0 = v1 - P.b[0], pp0 = p + P.pi[0];
+	const double ep0 = quot(n0, g0m, rc.rg0), evp0 = quot(P.g[0] * pp0, g0m, rc.rg0), re0 = rcp_nr(evp0);
+	const double r0 = lr0 * dp;
+	double num = -1.0, den = 0.0;
+	num += quot(S.Y0 * r0, evp0, re0); den += quot(S.Y0 * ep0, evp0, re0);
+	const double n1 = v2 - P.b[1], pp1 = p + P.pi[1];
+	const double ep1 = quot(n1, g1m, rc.rg1), evp1 = quot(P.g[1] * pp1, g1m, rc.rg1), re1 = rcp_nr(evp1);
+	const double r1 = lr1 * dp;
+	num += quot(S.Y1 * r1, evp1, re1); den += quot(S.Y1 * ep1, evp1, re1);
+	dy[0] = num / den;
+	const double n2 = r0 - ep0 * dy[0];
+	dy[1] = quot(n2, evp0, re0); dy[2] = dp;
+	const uint32_t ea = expo_field(a1), eb = expo_field(n0), ec = expo_field(n1), ed = expo_field(pp0);
+	const uint32_t ee = expo_field(pp1), ef = expo_field(n2), eg = expo_field(dp), egl = dp == 0.0 ? expo_lo(BAND_LEAF) : eg;
+	const uint32_t lo = umin(umin(umin(ea, eb), umin(ec, ed)), umin(umin(ee, ef), egl));
+	const uint32_t hi = umax(umax(umax(ea, eb), umax(ec, ed)), umax(umax(ee, ef), eg));
+	fast = fast && lo >= expo_lo(BAND_LEAF) && hi <= expo_hi(BAND_LEAF);
+	return adm && isfin(dy[0]) && isfin(dy[1]);
+}
 // adaptive Cash-Karp RK4(5) integration of the relaxed shock profile (PHY:311-341), written as a start and a single
 // pass of the integration loop so that a caller can interleave the passes of many integrations (the strong-shock
 // kernel advances one integration per lane, one pass at a time); prof_integrate runs the passes back to back
@@ -238,8 +319,11 @@ SPB_HD void prof_begin(const Prof& S, double c, ProfRun& R) {
 	R.y0 = S.p0; R.y1 = S.vk00; R.y2 = 0.0; R.v = S.v0; R.h = (S.vend - S.v0) / 32.0; R.it = 0;
 	R.lr0 = c / S.Y0; R.lr1 = (1.0 - c) / S.Y1;
 }
-// one pass of the loop body: 0 = not done yet, 1 = reached v_end (v1end = R.y1, qend = R.y2), -1 = failed
-SPB_HD int prof_step(const Phys& P, const Prof& S, ProfRun& R) {
+// one pass of the loop body: 0 = not done yet, 1 = reached v_end (v1end = R.y1, qend = R.y2), -1 = failed. RCP takes
+// the right-hand side's quotients from shared reciprocals (prof_rhs_rcp with the pass's rc), with the same results, and
+// may return 2 in every lane of the wavefront, R unchanged: then the step is to be taken with prof_step<false>.
+template <bool RCP = false>
+SPB_HD int prof_step(const Phys& P, const Prof& S, ProfRun& R, const ShockRcp* rc = nullptr) {
 	constexpr double A1 = 0.2, A2 = 0.3, A3 = 0.6, A4 = 1.0, A5 = 0.875;
 	constexpr double B10 = 0.2;
 	constexpr double B20 = 3.0 / 40, B21 = 9.0 / 40;
@@ -261,17 +345,31 @@ SPB_HD int prof_step(const Phys& P, const Prof& S, ProfRun& R) {
 	// Stage inputs are accumulated term by term in the reference's order over q. v + 0.0 * h is v (h is finite).
 	const double lr0 = R.lr0, lr1 = R.lr1;
 	double k0[3], k1[3], k2[3], k3[3], k4[3], k5[3], yt[3];
-	bool ok = prof_rhs(P, S, lr0, lr1, v, y, k0);
-	for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B10 * k0[n]; }
-	ok = prof_rhs(P, S, lr0, lr1, v + A1 * h, yt, k1) && ok;
-	for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B20 * k0[n]; yt[n] += h * B21 * k1[n]; }
-	ok = prof_rhs(P, S, lr0, lr1, v + A2 * h, yt, k2) && ok;
-	for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B30 * k0[n]; yt[n] += h * B31 * k1[n]; yt[n] += h * B32 * k2[n]; }
-	ok = prof_rhs(P, S, lr0, lr1, v + A3 * h, yt, k3) && ok;
-	for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B40 * k0[n]; yt[n] += h * B41 * k1[n]; yt[n] += h * B42 * k2[n]; yt[n] += h * B43 * k3[n]; }
-	ok = prof_rhs(P, S, lr0, lr1, v + A4 * h, yt, k4) && ok;
-	for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B50 * k0[n]; yt[n] += h * B51 * k1[n]; yt[n] += h * B52 * k2[n]; yt[n] += h * B53 * k3[n]; yt[n] += h * B54 * k4[n]; }
-	ok = prof_rhs(P, S, lr0, lr1, v + A5 * h, yt, k5) && ok;
+	auto stages = [&](auto rhs) {
+		bool ok = rhs(v, y, k0);
+		for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B10 * k0[n]; }
+		ok = rhs(v + A1 * h, yt, k1) && ok;
+		for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B20 * k0[n]; yt[n] += h * B21 * k1[n]; }
+		ok = rhs(v + A2 * h, yt, k2) && ok;
+		for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B30 * k0[n]; yt[n] += h * B31 * k1[n]; yt[n] += h * B32 * k2[n]; }
+		ok = rhs(v + A3 * h, yt, k3) && ok;
+		for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B40 * k0[n]; yt[n] += h * B41 * k1[n]; yt[n] += h * B42 * k2[n]; yt[n] += h * B43 * k3[n]; }
+		ok = rhs(v + A4 * h, yt, k4) && ok;
+		for (int n = 0; n < 3; n++) { yt[n] = y[n]; yt[n] += h * B50 * k0[n]; yt[n] += h * B51 * k1[n]; yt[n] += h * B52 * k2[n]; yt[n] += h * B53 * k3[n]; yt[n] += h * B54 * k4[n]; }
+		return rhs(v + A5 * h, yt, k5) && ok;
+	};
+	auto plain = [&](double vv, const double* yy, double* kk) { return prof_rhs(P, S, lr0, lr1, vv, yy, kk); };
+	bool ok;
+	if constexpr (RCP) {
+		const double rY1 = rcp_nr(S.Y1);
+		bool fast = rc->ok && in_band(S.Y0, BAND_LEAF) && in_band(S.Y1, BAND_LEAF) && in_band(lr0, BAND_LEAF) && in_band(lr1, BAND_LEAF);
+		ok = stages([&](double vv, const double* yy, double* kk) { return prof_rhs_rcp(P, S, *rc, rY1, lr0, lr1, vv, yy, kk, fast); });
+		// A bound failed somewhere in the step: every lane of the wavefront returns 2 with R as it was, and the caller
+		// takes the step again with prof_step<false> (the lanes within bounds get the same results again)
+		if (wave_any(!fast)) { R.it--; return 2; }
+	} else {
+		ok = stages(plain);
+	}
 	// fifth- and fourth-order sums in the reference's order; its zero-weight terms (0.0 * k1, 0.0 * k4 in the fifth-order
 	// sum, 0.0 * k1 in the fourth) are left out: where the step is admissible every k is finite, so each such term is a
 	// signed zero, and a sum that starts at +0.0 is never -0.0, so adding a signed zero leaves it unchanged

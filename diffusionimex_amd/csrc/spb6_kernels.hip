@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(256) k_shock_classify(Geo g, Phys P, const dou
 // shape. Every lane's work is bounded (4000 steps per integration, 100 Illinois iterations), so every wavefront
 // reaches the exit.
 constexpr int SHOCK_CHUNK = 128;
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_shock_strong(Geo g, Phys P, const double* __restrict__ rec, long cap,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_shock_strong(Geo g, Phys P, const double* __restrict__ rec, long cap,
                                                      double* __restrict__ Qph, const int* __restrict__ list, const int* __restrict__ count,
                                                      int* __restrict__ cursor, unsigned long long* __restrict__ rk_steps) {
 	const int m = *count;
@@ -356,6 +356,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 	int slot = 0;
 	int qn = 0, qe = 0;   // the wavefront's chunk of list slots not yet taken, [qn, qe)
 	int nsteps = 0;       // RK45 steps this lane took (added to *rk_steps at the exit: the pass's work count)
+	const ShockRcp rc = shock_rcp(P);
 	Prof S; double vH0 = 0.0, rho = 0.0; Shoot H; ProfRun R;
 	for (;;) {
 		bool tail = false;   // this lane starts an integration or finishes its cell in this pass
@@ -394,7 +395,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 		}
 		if (active && !tail) {
 			nsteps++;
-			const int r = prof_step(P, S, R);
+			int r = prof_step<true>(P, S, R, &rc);
+			if (r == 2) r = prof_step(P, S, R);   // (wavefront-uniform) a quotient's operand out of the shortcut's range
 			if (r != 0) { t = shoot_next(H, share_miss(R, r, vH0, S.vk00), R.y2); tail = true; }
 		}
 		if (tail) {
