@@ -81,6 +81,11 @@ FORM_KW = {"plain": {}, "mixed": {"krylov_fp32": 1, "analytic_jacobian": 1},
 STAT_KEYS = ("steps", "stages", "newton_it", "krylov_it", "linsolves", "rejects", "guess_used", "guess_fallback", "guess_skipped")
 
 
+def log(msg):
+    """progress on stderr (the JSON line alone goes to stdout)"""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def make_stepper(make, form):
     from diffusionimex_amd import spb6
     cfg, U = make(**FORM_KW[form])
@@ -98,6 +103,7 @@ def run_form(make, form, warmup, steps, breakdown):
     from diffusionimex_amd import spb6
     st, cfg, U0 = make_stepper(make, form)
     n = cfg.nx * cfg.ny
+    log(f"{form}: {cfg.nx}x{cfg.ny}, {warmup} warm-up steps")
     rc = st.run(1.0, warmup)
     if rc != 0:
         raise SystemExit(f"{form}: warm-up failed: {spb6.STATUS.get(rc, rc)}")
@@ -122,6 +128,7 @@ def run_form(make, form, warmup, steps, breakdown):
     kt_live = st.kernel_times()
     st.enable_timing(False)
     wall = t1 - t0
+    log(f"{form}: {steps} timed steps in {wall:.2f} s")
     d = {k: s2[k] - s1[k] for k in STAT_KEYS}
     d_all = {k: s1[k] - s0[k] for k in STAT_KEYS}
     res = {"value": n * d["steps"] / wall, "wall": wall, "ms_per_step": 1e3 * wall / max(d["steps"], 1),
@@ -239,11 +246,15 @@ def main():
     legs = set(x for x in a.legs.split(",") if x)
     out = run_headline(a) if "cavitation" in legs else {"legs": sorted(legs)}
     if "bubble" in legs and not a.profile:
+        log("shock-bubble leg")
         out["bubble"] = run_bubble(a)
     if "interface" in legs and a.interface_steps > 0 and not a.profile:
+        log("interface leg")
         out["interface"] = run_interface(a)
     if "batch" in legs and a.batch_steps > 0 and not a.profile:
+        log("batched 1D leg")
         out["batch"] = run_batch(a)
+    log("done")
     print(json.dumps(out))
 
 
@@ -275,7 +286,7 @@ def run_headline(a):
     head = "learned" if "learned" in results else ("mixed" if "mixed" in results else "plain")
     r = results[head]
     n = N * N
-    roof, kernel_ms, hbm_frac = roofline_of(r, n, head != "plain")
+    roof, kernel_ms, hbm_frac = roofline_of(r, n, head != "plain") if r["kt"] else (None, None, None)
     forms_out = {}
     for form, res in results.items():
         fo = {k: round(res[k], 4) if isinstance(res[k], float) else res[k]
@@ -304,11 +315,13 @@ def run_headline(a):
         "roofline": roof,
     }
     if not a.profile:
+        log("conservation check")
         out["conservation"] = conservation_check(a, forms)
     if not a.no_cpu and not a.profile:
         if st_plain is None:
             _, st_plain = run_form(make, "plain", a.warmup, 0, 0)
             cfg_plain = st_plain.cfg
+        log("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(a, cfg_plain, st_plain)
     if st_plain is not None:
         st_plain.close()

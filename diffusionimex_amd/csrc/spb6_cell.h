@@ -442,17 +442,24 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* dst, doubl
 		if (m > 0.0) atomicMax(&dst[blockIdx.x & 63], (unsigned long long)__double_as_longlong(m));
 	}
 }
-// append the cells of this wavefront whose flag is set to list (one atomic per wavefront) and return each flagged
-// lane's slot (-1 for the others); the list order is not deterministic but each cell's result does not depend on it
-__device__ __forceinline__ int wave_append(bool flag, int q, int* __restrict__ list, int* __restrict__ count) {
+// append the cells of a 256-thread block whose flag is set to list, with one atomic per block, and return each flagged
+// thread's slot (-1 for the others); every thread of the block calls it. The list order is not deterministic but each
+// cell's result does not depend on it. (One device-scope atomic per wavefront on the one counter, over a 16M-cell grid,
+// serialises at the memory side: 3.0 ms against 0.8 ms for the strong-shock classification of a 4096 x 4096 grid.)
+__device__ __forceinline__ int block_append(bool flag, int q, int* __restrict__ list, int* __restrict__ count) {
+	__shared__ int s_cnt[4], s_base;
 	const unsigned long long m = __ballot(flag);
-	if (m == 0ull) return -1;
-	const int lane = threadIdx.x & 63, leader = __ffsll((long long)m) - 1;
-	int base = 0;
-	if (lane == leader) base = atomicAdd(count, __popcll(m));
-	base = __shfl(base, leader, 64);
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (lane == 0) s_cnt[w] = __popcll(m);
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		const int tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+		s_base = tot ? atomicAdd(count, tot) : 0;
+	}
+	__syncthreads();
 	if (!flag) return -1;
-	const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+	int slot = s_base + __popcll(m & ((1ull << lane) - 1ull));
+	for (int k = 0; k < w; k++) slot += s_cnt[k];
 	list[slot] = q;
 	return slot;
 }

@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(256) k_shock_classify(Geo g, Phys P, const dou
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	double r[SHOCK_REC];
 	const bool strong = q < g.n && shock_classify_rec_at(g, P, Xs, Qph, q, r);
-	const int slot = wave_append(strong, q, list, count);
+	const int slot = block_append(strong, q, list, count);
 	if (slot >= 0) {
 		#pragma unroll
 		for (int k = 0; k < SHOCK_REC; k++) rec[k * cap + slot] = r[k];

@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(256) k_guess_tiles(FeatCtx X, int nrows, const
 				else *flag2 = 1;
 			}
 		}
-		wave_append(touched, q, rlist, rcount);
+		block_append(touched, q, rlist, rcount);
 		atomic_max_nonneg(nrm2, nv);
 	}
 	const unsigned long long m = __ballot(act);
