@@ -332,9 +332,12 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 	// step and the chord step's extra error is ~res_prev x res. A chord step that does not contract 100x turns chord
 	// steps off for the rest of the solve (near the cavitation threshold the Jacobian can move much more than the step
 	// suggests; there chord steps from res ~1e-8 were measured to cost Newton iterations).
-	bool chord_ok = c->cfg.analytic_jacobian && c->cfg.krylov_fp32 && !tri;
+	// jreuse (modified Newton, PCO:796-888) takes the place of the chord step
+	const bool jreuse = c->cfg.jreuse != 0;
+	bool chord_ok = c->cfg.analytic_jacobian && c->cfg.krylov_fp32 && !tri && !jreuse;
 	bool have_J = false, full_step = false;
 	double res_prev = INFINITY;
+	bool fresh = false; double ratio = 1.0, lamprev = 1.0;   // jreuse: the last step's contraction and damping (PCO:825)
 	for (int it = 0; it < c->cfg.newton_iter; it++) {
 		iters = it; resfinal = res;
 		if (verbose >= 3) fprintf(stderr, "    newton it %d res %.3e\n", it, res);   // the reference's verbose >= 3 trace (PC:781)
@@ -343,12 +346,18 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 		int* const clr = reinterpret_cast<int*>(c->mail);
 		const int ncl = (int)((offsetof(HostScratch, flags) + FLAG_RHO * sizeof(int)) / sizeof(int));
 		const bool chord = chord_ok && have_J && full_step && res < 100.0 * tol && res <= 1e-3 * res_prev;
+		// jreuse: rebuild at the stage's first iteration and after a damped step or one that contracted less than 4x
+		// (fresh_ok, PCO:797, PCO:828); a reused Jacobian keeps the scale s of its build too
+		const bool reuse = jreuse && it > 0 && lamprev == 1.0 && ratio <= 0.25;
 		const double res_before = res;
 		{
 			TRegion tr(c, T_JACOBIAN);
-			if (chord) launch_chord_rhs(D, c->Pst, c->G, c->sc, c->rhs, c->st, clr, ncl);
+			if (reuse) launch_chord_rhs(D, c->Pst, c->G, nullptr, c->rhs, c->st, clr, ncl);
+			else if (chord) launch_chord_rhs(D, c->Pst, c->G, c->sc, c->rhs, c->st, clr, ncl);
 			else launch_jacobian(D, c->Pst, c->G, c->sc, c->J, c->rhs, c->cfg.analytic_jacobian != 0, c->st, clr, ncl);
 		}
+		if (!reuse && !chord) c->stats.jac_builds++;
+		fresh = !reuse;
 		have_J = true;
 		c->stats.linsolves++;
 		bool kok = true, crit = false, trial = false; int kit = 0; double snk = 0.0;
@@ -359,7 +368,7 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 			// a step predicted to be the last (the observed contraction res / res_prev^2 puts the next residual below
 			// tol / 2) is solved to 0.1 tol / res, so that Newton finishes with it instead of taking one more step
 			const bool finish = chord_ok && full_step && res * res * res < 0.5 * tol * res_prev * res_prev;
-			rc = krylov(c, kit, kok, res, !chord, crit, snk, finish, &D, &trial); if (rc) return rc;
+			rc = krylov(c, kit, kok, res, !chord && !reuse, crit, snk, finish, &D, &trial); if (rc) return rc;
 		}
 		if (!crit) {   // the FP32 form reads the step criterion and the flags with its last defect
 			launch_stepcrit(n, c->dP, c->Pst, c->sc, &c->nrm[64], c->st);
@@ -394,6 +403,8 @@ int newton(spb6_ctx* c, const Dev& D, int& iters, double& resfinal, bool have_G 
 		full_step = acc && lam == 1.0;
 		if (chord && !(acc && res <= 1e-2 * res_before)) chord_ok = false;
 		res_prev = res_before;
+		if (jreuse && !acc && !fresh) { ratio = 1.0; lamprev = 0.0; continue; }   // rebuild, retake the step from P (PCO:833)
+		ratio = res / res_before; lamprev = lam;
 		if (!acc) {   // round-off floor acceptance (PC:834-842)
 			resfinal = res;
 			rc = reset_scalars(c); if (rc) return rc;

@@ -485,7 +485,7 @@ const char* spb6_batch_last_error(spb6_batch* b) { return b ? b->err.c_str() : "
 
 spb6_batch* spb6_batch_create(const spb6_batch_problem_t* probs, int nprob, const spb6_config_t* cfg, int device) {
 	if (!probs || nprob < 1 || !cfg) return nullptr;
-	if (cfg->rus_local != 1 || cfg->rk < 1 || cfg->rk > 2) return nullptr;
+	if (cfg->rus_local != 1 || cfg->rk < 1 || cfg->rk > 2 || cfg->jreuse) return nullptr;   // the batch's Newton has no jreuse
 	spb6_batch* b = new spb6_batch();
 	b->device = device; b->cfg = *cfg;
 	auto fail = [&]() -> spb6_batch* { spb6_batch_destroy(b); return nullptr; };

@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(256) k_chord_rhs(Dev D, const double* __restri
 	clear_ints(clear, nclear);
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= D.g.n) return;
-	fdscale_at(D, P, sc, q);
+	if (sc) fdscale_at(D, P, sc, q);   // a reused Jacobian (jreuse) keeps the scale of its build
 	rhs[q] = -G[q];
 }
 // Newton step criterion max |dP| / (|P| + s) (PC:819-823)

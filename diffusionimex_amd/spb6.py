@@ -34,6 +34,7 @@ class Config(C.Structure):
         ("krylov_tol", C.c_double), ("krylov_iter", C.c_int),
         ("limiter", C.c_int), ("order", C.c_int), ("rus_local", C.c_int),
         ("krylov_fp32", C.c_int), ("learned_guess", C.c_int), ("analytic_jacobian", C.c_int),
+        ("jreuse", C.c_int),
     ]
 
 
@@ -42,7 +43,8 @@ class Stats(C.Structure):
                 ("stages", C.c_long), ("newton_it", C.c_long), ("krylov_it", C.c_long), ("linsolves", C.c_long),
                 ("maxres", C.c_double), ("t", C.c_double), ("dt_last", C.c_double), ("status", C.c_int),
                 ("guess_used", C.c_long), ("guess_fallback", C.c_long), ("guess_log10_gain", C.c_double),
-                ("guess_skipped", C.c_long), ("tmax", C.c_double * 2), ("closure_fails", C.c_long)]
+                ("guess_skipped", C.c_long), ("tmax", C.c_double * 2), ("closure_fails", C.c_long),
+                ("jac_builds", C.c_long)]
 
 
 class KernelTimes(C.Structure):
@@ -157,7 +159,7 @@ def _dp(a):
 def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0, 1.0), bc=(0, 0, 0, 0), bcu=(0, 0, 0, 0),
                 rk=2, cfl=0.2, cfl_ac=0.4, ramp=1, dtfix=0.0, acdt=0, newton_tol=1e-13, newton_iter=60,
                 krylov_tol=1e-13, krylov_iter=4000, limiter=1, order=2, rus_local=1, krylov_fp32=0, learned_guess=0,
-                analytic_jacobian=0):
+                analytic_jacobian=0, jreuse=0):
     """Config with the reference defaults (PC:57-80): phases are (gamma, p_inf[, q, qp, cv, b])."""
     c = Config()
     c.nx, c.ny = nx, ny
@@ -174,6 +176,7 @@ def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0
     c.newton_tol, c.newton_iter, c.krylov_tol, c.krylov_iter = newton_tol, newton_iter, krylov_tol, krylov_iter
     c.limiter, c.order, c.rus_local, c.krylov_fp32, c.learned_guess = limiter, order, rus_local, krylov_fp32, learned_guess
     c.analytic_jacobian = analytic_jacobian
+    c.jreuse = jreuse
     return c
 
 

@@ -71,6 +71,11 @@ typedef struct spb6_config {
                                         krylov_fp32 the last Newton step as a chord step (the solve's previous
                                         Jacobian) when the residual is within 100x of the tolerance after a full step
                                         that contracted 1000x */
+    int jreuse;                      /* JREUSE (src/SPB6_PC_IMEX_omp.cpp:83, :796-888): modified Newton; the Jacobian (and
+                                        its scale s) is rebuilt at a stage's first iteration and after a damped step or a
+                                        step that contracted the residual less than 4x, and a line search that fails with
+                                        a reused Jacobian rebuilds it and retakes the step. Single-grid stepper only (the
+                                        batch rejects it) */
 } spb6_config_t;
 
 /* run statistics: the counters of the reference's stats line (PC:1337-1339) plus Krylov counts */
@@ -86,6 +91,8 @@ typedef struct spb6_stats {
     double tmax[2];                   /* the phases' largest temperatures in the stages' final states where the phase is
                                          present (alpha > 1e-9), the reference's st_Tmax (PC:986-989) */
     long closure_fails;               /* stages whose final state failed in the closure (an attempt rejected for it) */
+    long jac_builds;                  /* Newton Jacobians built (finite-difference or analytic; jreuse and chord steps
+                                         reuse one) */
 } spb6_stats_t;
 
 typedef struct spb6_ctx spb6_ctx;

@@ -27,7 +27,7 @@ class Cfg(C.Structure):
         ("krylov_tol", C.c_double), ("krylov_iter", C.c_int),
         ("limiter", C.c_int), ("order", C.c_int), ("rus_local", C.c_int),
         ("closure_path", C.c_int), ("share_start", C.c_int), ("dot_order", C.c_int),
-        ("portable_math", C.c_int), ("nthreads", C.c_int),
+        ("portable_math", C.c_int), ("nthreads", C.c_int), ("jreuse", C.c_int),
     ]
 
 
@@ -36,6 +36,7 @@ class Stats(C.Structure):
         ("steps", C.c_long), ("rk1", C.c_long), ("rejects", C.c_long), ("floor_acc", C.c_long),
         ("stages", C.c_long), ("newton_it", C.c_long), ("krylov_it", C.c_long), ("linsolves", C.c_long),
         ("maxres", C.c_double), ("t", C.c_double), ("dt_last", C.c_double), ("status", C.c_int),
+        ("jac_builds", C.c_long),
     ]
 
 
@@ -127,6 +128,7 @@ def default_cfg(nx, ny=1, **kw):
     c.limiter, c.order, c.rus_local = 1, 2, 1
     c.closure_path, c.share_start, c.dot_order, c.portable_math = 1, 1, 2, 1
     c.nthreads = 0
+    c.jreuse = 0
     for k, v in kw.items():
         if k == "bc":
             for q in range(4):

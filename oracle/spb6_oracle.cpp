@@ -486,8 +486,10 @@ struct Cfg {
 	int limiter, order, rus_local;
 	int closure_path, share_start, dot_order, portable_math;
 	int nthreads;
+	int jreuse;   // JREUSE (PCO:83): modified Newton, the Jacobian rebuilt only as PCO:826-828 decides
 };
-struct Stats { long steps, rk1, rejects, floor_acc, stages, newton_it, krylov_it, linsolves; double maxres, t, dt_last; int status; };
+struct Stats { long steps, rk1, rejects, floor_acc, stages, newton_it, krylov_it, linsolves; double maxres, t, dt_last; int status;
+               long jac_builds; };
 
 struct SpRow { int n; int col[5]; double val[5]; };                                                  // PC:708
 
@@ -946,7 +948,10 @@ static void thomas(const std::vector<SpRow>& A, const std::vector<double>& rhs, 
 	dP[n - 1] = dp[n - 1]; for (int r = n - 2; r >= 0; r--) dP[r] = dp[r] - cp[r] * dP[r + 1];
 }
 
-// ------------------------------------------------------------------------------------------------ Newton (PC:761-847, PCO:799-904 with jreuse=0)
+// ------------------------------------------------------------------------------------------------ Newton (PC:761-847, PCO:799-904)
+// jreuse (PCO:796-888): the Jacobian and its scale s are rebuilt at a stage's first iteration and whenever the previous
+// step was damped or contracted the residual by less than 4x (fresh_ok, PCO:797); a line search that fails with a reused
+// Jacobian rebuilds it and takes the step again from the same P (PCO:833).
 struct NewtonTrace { std::vector<double>* J5; };   // optional dump of the first Jacobian (tests)
 static bool newton(Ctx& X, std::vector<double>& P, int& iters, double& resfinal) {
 	const int nx = X.nx, ny = X.ny, n = nx * ny; const double tol = X.C.newton_tol;
@@ -966,9 +971,13 @@ static bool newton(Ctx& X, std::vector<double>& P, int& iters, double& resfinal)
 	if (!evalAll(P, G)) { iters = 0; resfinal = INF; return false; }
 	double res = norm(G, P);
 	const bool tri = (ny == 1 && X.C.bc[0] != BC_PERIODIC);
+	bool fresh = false; double ratio = 1.0, lamprev = 1.0;
 	for (int it = 0; it < X.C.newton_iter; it++) {
 		iters = it; resfinal = res;
 		if (res < tol) return true;
+		const bool build = !X.C.jreuse || it == 0 || !(lamprev == 1.0 && ratio <= 0.25);
+		if (build) {
+		X.st.jac_builds++;
 		#pragma omp parallel for schedule(static)
 		for (int q = 0; q < n; q++) {
 			const int I = q % nx + GH, J = q / nx + GH; const double K = X.Kw[X.id(I, J)];
@@ -992,6 +1001,8 @@ static bool newton(Ctx& X, std::vector<double>& P, int& iters, double& resfinal)
 			}
 		}
 		if (!jok) return false;
+		}
+		fresh = build;
 		for (int q = 0; q < n; q++) rhs[q] = -G[q];
 		X.st.linsolves++;
 		if (tri) thomas(A, rhs, dP);
@@ -1004,12 +1015,14 @@ static bool newton(Ctx& X, std::vector<double>& P, int& iters, double& resfinal)
 			if (!evalAll(P, G)) return false;
 			resfinal = norm(G, P); return true;
 		}
-		double lam = 1.0; bool acc = false;
+		double lam = 1.0; bool acc = false; const double res0 = res;
 		for (int ls = 0; ls < 40; ls++) {
 			for (int q = 0; q < n; q++) Pn[q] = P[q] + lam * dP[q];
 			if (evalAll(Pn, Gn)) { const double rn = norm(Gn, Pn); if (rn < res * (1.0 - 1e-4 * lam) || rn < tol) { acc = true; res = rn; break; } }
 			lam *= 0.5;
 		}
+		if (!acc && !fresh) { ratio = 1.0; lamprev = 0.0; continue; }   // PCO:833: rebuild and retake the step from the same P
+		ratio = res / res0; lamprev = lam;
 		if (!acc) {   // round-off floor acceptance (PC:834-842)
 			resfinal = res; bool floor_ok = true;
 			#pragma omp parallel for schedule(static) reduction(&&:floor_ok)
