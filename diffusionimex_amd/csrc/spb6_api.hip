@@ -27,7 +27,7 @@ namespace {
 
 // the scalars the host reads back: norms (8 slots of 64 striped maxima), flags, the Krylov loop state and the FP32
 // form's defect sums and step criterion. The device keeps the same struct in one allocation, so every read-back is one copy.
-struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; double dsum[3]; int spec; double gthr; };
+struct HostScratch { unsigned long long nrm[8 * 64]; int flags[NFLAGS]; KS ks; double dsum[3]; int spec; double gthr; int hxf; };
 
 // T_KRYLOV: BiCGStab iteration passes only; T_KAUX: the solves' setup, precision conversions and FP64 defects;
 // T_GUESS: the learned guess's features and inference
@@ -109,6 +109,8 @@ struct spb6_ctx {
 	int cus = 256;                      // compute units of the device
 	int shock_blocks = 1024;            // resident 256-thread blocks of the strong-shock pass
 	float *dX = nullptr, *dY = nullptr; unsigned long long* dcount = nullptr; long dcap = 0; int dstride = 0; unsigned dseq = 0;
+	// the explicit HLLC scheme (allocated at its first call): cell primitives and the x / y face values
+	double *hxp = nullptr, *hfx = nullptr, *hfy = nullptr;
 };
 
 struct spb6_mlp {
@@ -591,6 +593,63 @@ int caldt(spb6_ctx* c, bool acoustic, double& inv_rate) {   // PC:477-488
 	return SPB6_OK;
 }
 
+// ---- the reference's fully explicit scheme (src/SPB6_explicit.cpp, "EX") at relaxation level 0 on the context's grids:
+// the attempt's working state in Ust (V), the update's output in Qe; the stage data of the IMEX path are free meanwhile
+int hllc_alloc(spb6_ctx* c) {
+	if (c->hxp) return SPB6_OK;
+	const size_t NC = c->g.NC;
+	if (dalloc(c, &c->hxp, NC * hllc_prim_fields()) || dalloc(c, &c->hfx, NC * hllc_face_fields())) return SPB6_HIP_ERROR;
+	if (c->g.twoD) { if (dalloc(c, &c->hfy, NC * hllc_face_fields())) return SPB6_HIP_ERROR; }
+	else c->hfy = c->hfx;
+	return SPB6_OK;
+}
+// RelaxationSubsystem (EX:344-361): ghost cells, the shock budget on the +-1 cross stencil, the per-cell relaxation
+void hllc_relax(spb6_ctx* c, double* U) {
+	launch_bc_grid(c->g, U, c->st);
+	Dev D = make_dev(c); D.Ust = U; D.Ust_c = U;
+	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->slist, c->scount, c->srec, c->shock_steps, c->shock_blocks, c->st); }
+	{ TRegion tr(c, T_FINAL); launch_hllc_relax(c->g, c->ph, U, c->Qph, c->flags, &c->mail->hxf, c->st); }
+	launch_bc_grid(c->g, U, c->st);
+}
+// explicit_stage (EX:363-372): reconstruction, HLLC fluxes and the Godunov update of V into N, then its relaxation; V and
+// N trade places
+void hllc_stage(spb6_ctx* c, double*& V, double*& N, double h) {
+	const Geo& g = c->g;
+	launch_bc_grid(g, V, c->st);
+	{
+		TRegion tr(c, T_EXPLICIT);
+		launch_hllc_prim(g, c->ph, V, c->hxp, c->flags, c->st);
+		launch_hllc_flux(g, c->ph, c->hxp, c->cfg.order, c->cfg.limiter, c->hfx, c->hfy, c->flags, c->st);
+		launch_hllc_update(g, V, c->hxp, c->hfx, c->hfy, h / g.dx, h / g.dy, N, c->flags, c->st);
+	}
+	hllc_relax(c, N);
+	std::swap(V, N);
+}
+// one attempt of a step (EX:546-556): first order one stage; second order Heun, U2 = R(U1 + dt L(U1)) after U1 =
+// R(U + dt L(U)), then (U + U2) / 2 relaxed. The passes stop at the first failure, as the reference does; one
+// synchronisation reads the outcome. On success the new state becomes U.
+int hllc_attempt(spb6_ctx* c, double dt) {
+	int rc = reset_stage(c); if (rc) return rc;
+	HIPCHK(hipMemsetAsync(&c->mail->hxf, 0, sizeof(int), c->st));
+	const size_t bytes = (size_t)NV * c->g.NC * sizeof(double);
+	double *V = c->Ust, *N = c->Qe;
+	HIPCHK(hipMemcpyAsync(V, c->U, bytes, hipMemcpyDeviceToDevice, c->st));
+	if (c->cfg.order < 2) hllc_stage(c, V, N, dt);
+	else {
+		hllc_stage(c, V, N, dt);
+		hllc_stage(c, V, N, dt);
+		launch_hllc_average((long)NV * c->g.NC, V, c->U, c->flags, c->st);
+		hllc_relax(c, V);
+	}
+	rc = fetch(c); if (rc) return rc;
+	c->stats.closure_fails += c->hs->hxf;
+	if (c->hs->flags[FLAG_STAGE]) return SPB6_STAGE_INADMISSIBLE;
+	if (c->hs->flags[FLAG_FINAL]) return SPB6_CLOSURE_FAIL;
+	double* old = c->U;
+	c->U = V; c->Ust = old; c->Qe = N;   // the three grids stay distinct
+	return SPB6_OK;
+}
+
 // pbar_of (PHY:855-857) on the host
 double pbar_host(const Phys& P, const double* Q) {
 	const double a1 = Q[IA1];
@@ -895,6 +954,39 @@ int spb6_run(spb6_ctx* c, double t_end, long max_steps) {
 		if (!ok) { c->stats.status = SPB6_STEP_FAILED; return SPB6_STEP_FAILED; }
 		c->t = c->t + dt; c->dtprev = dt; c->stats.steps++; c->stats.dt_last = dt; done_steps++;
 		if (!ramping) c->startup = false;
+	}
+	c->stats.t = c->t;
+	return SPB6_OK;
+}
+
+int spb6_hllc_step(spb6_ctx* c, double dt) {
+	if (!c || !(dt > 0)) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	int rc = hllc_alloc(c); if (rc) return rc;
+	rc = hllc_attempt(c, dt);
+	if (rc == SPB6_OK) { c->t += dt; c->stats.steps++; c->stats.dt_last = dt; c->stats.t = c->t; }
+	return rc;
+}
+int spb6_hllc_run(spb6_ctx* c, double t_end, long max_steps, double cfl) {
+	if (!c || !(cfl > 0)) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	int rc = hllc_alloc(c); if (rc) return rc;
+	long done_steps = 0;
+	while (c->t < t_end * (1 - 1e-12) && (max_steps < 0 || done_steps < max_steps)) {
+		double ir;
+		rc = caldt(c, true, ir); if (rc) return rc;
+		double dt = cfl * ir;
+		dt = ((t_end - c->t) < dt) ? (t_end - c->t) : dt;
+		if (!(dt > 1e-12 * t_end)) { c->stats.status = SPB6_DT_UNDERFLOW; return SPB6_DT_UNDERFLOW; }
+		bool ok = false;
+		for (int att = 0; att < 30 && !ok; att++) {
+			rc = hllc_attempt(c, dt);
+			if (rc == SPB6_HIP_ERROR || rc == SPB6_BAD_ARG) return rc;
+			if (rc == SPB6_OK) ok = true;
+			else { c->stats.rejects++; dt *= 0.5; }
+		}
+		if (!ok) { c->stats.status = SPB6_STEP_FAILED; return SPB6_STEP_FAILED; }
+		c->t = c->t + dt; c->stats.steps++; c->stats.dt_last = dt; done_steps++;
 	}
 	c->stats.t = c->t;
 	return SPB6_OK;

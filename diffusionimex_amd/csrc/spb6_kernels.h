@@ -101,4 +101,17 @@ void launch_krylov_setup32_cvt(const Geo& g, const KrylovBufs32& B, const double
                                hipStream_t st);
 void launch_krylov_iter32(const Geo& g, const KrylovBufs32& B, hipStream_t st, int it, double tol, int maxit);
 
+// ---- the explicit HLLC scheme (spb6_hllc.hip): cell primitives (hllc_prim_fields arrays over the ghosted grid), face
+// values (hllc_face_fields arrays per direction, at the face's low cell); every pass returns at once once flags
+// FLAG_STAGE or FLAG_FINAL is set
+int hllc_prim_fields();
+int hllc_face_fields();
+void launch_hllc_prim(const Geo& g, const Phys& P, const double* U, double* W, int* flags, hipStream_t st);
+void launch_hllc_flux(const Geo& g, const Phys& P, const double* W, int order, int limiter, double* FX, double* FY, const int* flags, hipStream_t st);
+void launch_hllc_update(const Geo& g, const double* U, const double* W, const double* FX, const double* FY, double rx, double ry, double* Uo,
+                        int* flags, hipStream_t st);
+// fails: closure failures counted (one int)
+void launch_hllc_relax(const Geo& g, const Phys& P, double* U, const double* Qph, int* flags, int* fails, hipStream_t st);
+void launch_hllc_average(long n, double* V, const double* W, const int* flags, hipStream_t st);
+
 }   // namespace spb

@@ -60,7 +60,8 @@ class KernelTimes(C.Structure):
 EXPORTS = ["spb6_create", "spb6_destroy", "spb6_last_error", "spb6_set_state", "spb6_get_state", "spb6_get_pressure",
            "spb6_set_pressure", "spb6_caldt", "spb6_bc_fill", "spb6_explicit_stage", "spb6_implicit_stage",
            "spb6_stage_prep", "spb6_step", "spb6_attempt_begin", "spb6_stage_combine", "spb6_stage_update",
-           "spb6_stage_pressure", "spb6_attempt_accept", "spb6_run", "spb6_get_profile", "spb6_write_profile",
+           "spb6_stage_pressure", "spb6_attempt_accept", "spb6_run", "spb6_hllc_step", "spb6_hllc_run",
+           "spb6_get_profile", "spb6_write_profile",
            "spb6_conserved_totals", "spb6_stats_line", "spb6_get_stats", "spb6_time", "spb6_residual",
            "spb6_jacobian", "spb6_linsolve", "spb6_linsolve_device", "spb6_get_field", "spb6_enable_timing", "spb6_set_timing_mask", "spb6_get_kernel_times",
            "spb6_mlp_create", "spb6_mlp_destroy", "spb6_mlp_nparams", "spb6_mlp_set_params", "spb6_mlp_get_params",
@@ -106,6 +107,8 @@ def load():
     L.spb6_conserved_totals.argtypes = [vp, dp]
     L.spb6_stats_line.argtypes = [vp, C.c_int, C.c_double, C.c_char_p, C.c_long]
     L.spb6_run.argtypes = [vp, C.c_double, C.c_long]
+    L.spb6_hllc_step.argtypes = [vp, C.c_double]
+    L.spb6_hllc_run.argtypes = [vp, C.c_double, C.c_long, C.c_double]
     L.spb6_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.spb6_time.restype = C.c_double
     L.spb6_time.argtypes = [vp]
@@ -261,6 +264,14 @@ class Stepper:
 
     def run(self, t_end=1e300, max_steps=-1):
         return self.L.spb6_run(self.h, float(t_end), int(max_steps))
+
+    def hllc_step(self, dt):
+        """One step of the reference's explicit HLLC scheme (src/SPB6_explicit.cpp) of size dt."""
+        return self.L.spb6_hllc_step(self.h, float(dt))
+
+    def hllc_run(self, t_end=1e300, max_steps=-1, cfl=0.4):
+        """The explicit HLLC scheme's time loop at acoustic CFL cfl (EX:537-566)."""
+        return self.L.spb6_hllc_run(self.h, float(t_end), int(max_steps), float(cfl))
 
     def stats(self):
         s = Stats()

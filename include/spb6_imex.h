@@ -159,6 +159,19 @@ int spb6_attempt_accept(spb6_ctx* ctx, double dt);
 /* The time loop (PC:1230-1296): dt from caldt with the start-up ramp, reject-and-halve, until t_end or max_steps
  * accepted steps (max_steps < 0: no cap). */
 int spb6_run(spb6_ctx* ctx, double t_end, long max_steps);
+/* ---- the reference's fully explicit scheme, src/SPB6_explicit.cpp ("EX"), at relaxation level 0, on the same context
+ * and state (an independent cross-check of the IMEX path on shock problems; it leaves the pressure unknown alone):
+ * primitive-variable minmod reconstruction (EX:228-256; cfg.order, cfg.limiter), HLLC fluxes with the phase Hugoniot
+ * states (EX:145-192), the Godunov update (EX:194-225), the mechanical relaxation by the p closure with the shock
+ * budget's defect split (EX:305-361), Heun's two stages (EX:546-556). The first call allocates the scheme's work arrays
+ * (cell primitives and face values: 27 doubles per ghosted cell in 2D). */
+/* One step of size dt (EX:546-556): SPB6_OK, SPB6_STAGE_INADMISSIBLE (a cell prim_of rejects, or alpha_1, m_1, m_2 out
+ * of range after an update) or SPB6_CLOSURE_FAIL (the relaxation's closure; counted in stats.closure_fails). The state
+ * changes only on success. */
+int spb6_hllc_step(spb6_ctx* ctx, double dt);
+/* Its time loop (EX:537-566): dt = min(cfl * caldt(acoustic), t_end - t) (the reference's cfl is 0.4), failed steps
+ * halved up to 30 times (stats.rejects), SPB6_DT_UNDERFLOW below 1e-12 t_end, SPB6_STEP_FAILED after 30 halvings. */
+int spb6_hllc_run(spb6_ctx* ctx, double t_end, long max_steps, double cfl);
 int spb6_get_stats(spb6_ctx* ctx, spb6_stats_t* stats);
 double spb6_time(spb6_ctx* ctx);
 

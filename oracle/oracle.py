@@ -79,6 +79,9 @@ def lib():
         L.ora_stage_explicit.argtypes = [vp, C.c_double]
         L.ora_stage_prep.argtypes = [vp, C.c_double]
         L.ora_stage_newton.argtypes = [vp, ip, dp]
+        L.ora_hllc_run.argtypes = [vp, C.c_double, C.c_long, C.c_double]
+        L.ora_hllc_closure_fails.restype = C.c_long
+        L.ora_hllc_step.argtypes = [vp, C.c_double]
         L.ora_residual_all.argtypes = [vp, dp, dp]
         L.ora_jacobian.argtypes = [vp, dp, dp, dp, dp]
         L.ora_linsolve.argtypes = [vp, dp, dp, dp, C.c_int, ip]
@@ -207,6 +210,14 @@ class Oracle:
 
     def run(self, t_end=1e300, max_steps=-1):
         return self.L.ora_run(self.h, float(t_end), int(max_steps))
+
+    def hllc_run(self, t_end=1e300, max_steps=-1, cfl=0.4):
+        """The reference's explicit HLLC scheme (src/SPB6_explicit.cpp) on this context's state, acoustic CFL cfl."""
+        return self.L.ora_hllc_run(self.h, float(t_end), int(max_steps), float(cfl))
+
+    def hllc_step(self, dt):
+        """One step of the explicit HLLC scheme of size dt (0: accepted)."""
+        return self.L.ora_hllc_step(self.h, float(dt))
 
     def stats(self):
         s = Stats()

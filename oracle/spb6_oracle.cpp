@@ -1149,6 +1149,210 @@ static int run(Ctx& X, double t_end, long max_steps) {
 	return 0;
 }
 
+// ================================================================================================ explicit HLLC scheme
+// The reference's fully explicit scheme, src/SPB6_explicit.cpp ("EX"), at relaxation level 0: minmod reconstruction of
+// the primitive variables (EX:228-256), HLLC fluxes with the phase Hugoniot states in the star region (EX:145-192), the
+// Godunov update with the non-conservative alpha_1 and phase-energy terms (EX:194-225), then per cell the mechanical
+// relaxation by the p closure with the shock budget's defect split (EX:305-361), in Heun's two-stage scheme with the
+// reject-and-halve time loop at the acoustic CFL (EX:537-566). The IMEX run's statistics (steps, rejects) are reused;
+// hllc_cl_fail counts the relaxation's closure failures (st_cl_fail).
+struct HFace { double F[5], ustar, ua1, ue[2]; };                                                       // EX:145
+static double hugoniot_p(const Model& M, int k, double p, double v, double vs) {                     // EX:147-157 (SG / NASG)
+	const Phase& P = M.PH[k];
+	const double num = (p + P.g * P.pi) * (v - P.b) - P.g * P.pi * (vs - P.b) - 0.5 * (P.g - 1.0) * p * (vs - v);
+	const double den = (vs - P.b) + 0.5 * (P.g - 1.0) * (vs - v);
+	return num / den;
+}
+static HFace hllc(const Model& M, const Prim& L, const Prim& R, int dir) {                          // EX:159-192
+	auto un = [&](const Prim& w) { return dir == 0 ? w.u : w.v; };
+	auto ut = [&](const Prim& w) { return dir == 0 ? w.v : w.u; };
+	const double uL = un(L), uR = un(R), pL = L.a1 * L.p1 + (1 - L.a1) * L.p2, pR = R.a1 * R.p1 + (1 - R.a1) * R.p2;
+	const double SL = smin(uL - L.cf, uR - R.cf), SR = smax(uL + L.cf, uR + R.cf);
+	const double SM = (pR - pL + L.rho * uL * (SL - uL) - R.rho * uR * (SR - uR)) / (L.rho * (SL - uL) - R.rho * (SR - uR));
+	auto EK = [&](const Prim& w) { const double a2 = 1 - w.a1;
+		return eps_amp(M, 0, w.a1, w.a1 * w.r1, w.p1) + eps_amp(M, 1, a2, a2 * w.r2, w.p2) + 0.5 * w.rho * (w.u * w.u + w.v * w.v); };
+	auto flux = [&](const Prim& w, double* F) { const double u = un(w), p = w.a1 * w.p1 + (1 - w.a1) * w.p2;
+		F[0] = w.a1 * w.r1 * u; F[1] = (1 - w.a1) * w.r2 * u; F[2] = w.rho * u * u + p; F[3] = w.rho * u * ut(w); F[4] = (EK(w) + p) * u; };
+	HFace f;
+	const bool left = SM >= 0; const Prim& K = left ? L : R;
+	const double uK = left ? uL : uR, SK = left ? SL : SR, pK = left ? pL : pR;
+	double FK[5]; flux(K, FK);
+	if ((left && SL >= 0) || (!left && SR <= 0)) {
+		for (int q = 0; q < 5; q++) f.F[q] = FK[q];
+		f.ustar = uK; f.ua1 = uK * K.a1;
+		f.ue[0] = eps_amp(M, 0, K.a1, K.a1 * K.r1, K.p1) * uK; f.ue[1] = eps_amp(M, 1, 1 - K.a1, (1 - K.a1) * K.r2, K.p2) * uK;
+		return f;
+	}
+	const double fac = (SK - uK) / (SK - SM), EKv = EK(K);
+	const double rs = K.rho * fac;
+	const double Es = fac * (EKv + (SM - uK) * (K.rho * SM + pK / (SK - uK)));
+	const double UK[5] = {K.a1 * K.r1, (1 - K.a1) * K.r2, K.rho * uK, K.rho * ut(K), EKv};
+	const double Us[5] = {UK[0] * fac, UK[1] * fac, rs * SM, rs * ut(K), Es};
+	for (int q = 0; q < 5; q++) f.F[q] = FK[q] + SK * (Us[q] - UK[q]);
+	f.ustar = SM; f.ua1 = SM * K.a1;
+	const double ak[2] = {K.a1, 1 - K.a1}, rk[2] = {K.r1, K.r2}, pk[2] = {K.p1, K.p2};
+	for (int k = 0; k < 2; k++) {
+		const double v0 = 1.0 / rk[k], vs = 1.0 / (rk[k] * fac);
+		double ps = hugoniot_p(M, k, pk[k], v0, vs);
+		if (!std::isfinite(ps) || !p_admissible(M, k, ps)) ps = pk[k];
+		f.ue[k] = eps_amp(M, k, ak[k], ak[k] * rk[k] * fac, ps) * SM;
+	}
+	return f;
+}
+// cell primitives of every ghosted cell, then the minmod face states per direction (EX:228-256); false where prim_of fails
+static bool hllc_recon(Ctx& X, const std::vector<double>& U, std::vector<Prim>& w, std::vector<Prim>* WL, std::vector<Prim>* WR) {
+	const int NXT = X.NXT, NYT = X.NYT; const size_t NC = (size_t)NXT * NYT;
+	w.assign(NC, Prim{}); bool ok = true;
+	#pragma omp parallel for schedule(static) reduction(&&:ok)
+	for (size_t c = 0; c < NC; c++) if (!prim_of(X.M, &U[NV * c], w[c])) ok = false;
+	if (!ok) return false;
+	for (int d = 0; d < 2; d++) { WL[d] = w; WR[d] = w; }
+	if (X.C.order < 2) return true;
+	auto mm = [&](double a, double b) { if (!X.C.limiter) return 0.5 * (a + b); return a * b > 0 ? (std::fabs(a) < std::fabs(b) ? a : b) : 0.0; };
+	for (int dir = 0; dir < (X.ny > 1 ? 2 : 1); dir++)
+		#pragma omp parallel for schedule(static)
+		for (int j = 0; j < NYT; j++) for (int i = 0; i < NXT; i++) {
+			int im = i, ip = i, jm = j, jp = j;
+			if (dir == 0) { im = i - 1; ip = i + 1; if (im < 0 || ip >= NXT) continue; }
+			else { jm = j - 1; jp = j + 1; if (jm < 0 || jp >= NYT) continue; }
+			const Prim &c = w[X.id(i, j)], &a = w[X.id(im, jm)], &b = w[X.id(ip, jp)];
+			const double cv[7] = {c.a1, c.r1, c.r2, c.u, c.v, c.p1, c.p2}, av[7] = {a.a1, a.r1, a.r2, a.u, a.v, a.p1, a.p2};
+			const double bv[7] = {b.a1, b.r1, b.r2, b.u, b.v, b.p1, b.p2};
+			double l[7], r[7];
+			for (int q = 0; q < 7; q++) { const double sl = mm(cv[q] - av[q], bv[q] - cv[q]); l[q] = cv[q] - 0.5 * sl; r[q] = cv[q] + 0.5 * sl; }
+			Prim PL = c, PR = c;
+			PL.a1 = l[0]; PL.r1 = l[1]; PL.r2 = l[2]; PL.u = l[3]; PL.v = l[4]; PL.p1 = l[5]; PL.p2 = l[6];
+			PR.a1 = r[0]; PR.r1 = r[1]; PR.r2 = r[2]; PR.u = r[3]; PR.v = r[4]; PR.p1 = r[5]; PR.p2 = r[6];
+			for (Prim* P : {&PL, &PR}) {
+				const double m1 = P->a1 * P->r1, m2 = (1 - P->a1) * P->r2; P->rho = m1 + m2;
+				P->cf = std::sqrt((m1 * c2_rp(X.M, 0, P->r1, P->p1) + m2 * c2_rp(X.M, 1, P->r2, P->p2)) / P->rho);
+			}
+			WL[dir][X.id(i, j)] = PL; WR[dir][X.id(i, j)] = PR;
+		}
+	return true;
+}
+// Godunov update u_new = u + dU of the interior cells (EX:194-225); false where alpha_1, m_1 or m_2 leave their range
+static bool hllc_update(Ctx& X, std::vector<double>& Unew, const std::vector<double>& U, const std::vector<Prim>& w,
+                        const std::vector<Prim>* WL, const std::vector<Prim>* WR, double h) {
+	const int nx = X.nx, ny = X.ny, NXT = X.NXT, NYT = X.NYT;
+	const double rx = h / X.dx, ry = h / X.dy;
+	std::vector<HFace> FX((size_t)NXT * NYT), FY;
+	#pragma omp parallel for schedule(static)
+	for (int j = GH; j < ny + GH; j++) for (int i = GH - 1; i < nx + GH; i++) FX[X.id(i, j)] = hllc(X.M, WR[0][X.id(i, j)], WL[0][X.id(i + 1, j)], 0);
+	if (ny > 1) {
+		FY.resize((size_t)NXT * NYT);
+		#pragma omp parallel for schedule(static)
+		for (int j = GH - 1; j < ny + GH; j++) for (int i = GH; i < nx + GH; i++) FY[X.id(i, j)] = hllc(X.M, WR[1][X.id(i, j)], WL[1][X.id(i, j + 1)], 1);
+	}
+	Unew = U;
+	bool ok = true;
+	#pragma omp parallel for schedule(static) reduction(&&:ok)
+	for (int j = GH; j < ny + GH; j++) for (int i = GH; i < nx + GH; i++) {
+		const Prim& wc = w[X.id(i, j)];
+		const HFace &fp = FX[X.id(i, j)], &fm = FX[X.id(i - 1, j)];
+		double dU[NV] = {0};
+		const double ak[2] = {wc.a1, 1 - wc.a1}, pk[2] = {wc.p1, wc.p2};
+		dU[IM1] = -rx * (fp.F[0] - fm.F[0]); dU[IM2] = -rx * (fp.F[1] - fm.F[1]); dU[IMU] = -rx * (fp.F[2] - fm.F[2]);
+		dU[IMV] = -rx * (fp.F[3] - fm.F[3]); dU[IEN] = -rx * (fp.F[4] - fm.F[4]);
+		dU[IA1] = -rx * ((fp.ua1 - fm.ua1) - wc.a1 * (fp.ustar - fm.ustar));
+		for (int k = 0; k < 2; k++) dU[IE1 + k] = -rx * ((fp.ue[k] - fm.ue[k]) + ak[k] * pk[k] * (fp.ustar - fm.ustar));
+		if (ny > 1) {
+			const HFace &gp = FY[X.id(i, j)], &gm = FY[X.id(i, j - 1)];
+			dU[IM1] -= ry * (gp.F[0] - gm.F[0]); dU[IM2] -= ry * (gp.F[1] - gm.F[1]); dU[IMV] -= ry * (gp.F[2] - gm.F[2]);
+			dU[IMU] -= ry * (gp.F[3] - gm.F[3]); dU[IEN] -= ry * (gp.F[4] - gm.F[4]);
+			dU[IA1] -= ry * ((gp.ua1 - gm.ua1) - wc.a1 * (gp.ustar - gm.ustar));
+			for (int k = 0; k < 2; k++) dU[IE1 + k] -= ry * ((gp.ue[k] - gm.ue[k]) + ak[k] * pk[k] * (gp.ustar - gm.ustar));
+		}
+		double* Q = &Unew[NV * X.id(i, j)];
+		for (int k = 0; k < NV; k++) Q[k] = U[NV * X.id(i, j) + k] + dU[k];
+		if (!(Q[IA1] > 0 && Q[IA1] < 1 && Q[IM1] > 0 && Q[IM2] > 0)) ok = false;
+	}
+	return ok;
+}
+// mechanical relaxation of one cell (EX:305-343 at relaxation level 0, mu = inf): a phase whose transported state is
+// inadmissible restarts from the mixture pressure, then the p closure (s = 0, the cell's shock budget) sets alpha and eps
+static bool hllc_relax_cell(const Model& M, double* Q, const double Qph[2]) {
+	const double rho = Q[IM1] + Q[IM2], rhoe = Q[IEN] - 0.5 * (Q[IMU] * Q[IMU] + Q[IMV] * Q[IMV]) / rho;
+	double a1 = Q[IA1], m1 = Q[IM1], m2 = Q[IM2], e1 = Q[IE1], e2 = Q[IE2];
+	{
+		const double ak[2] = {a1, 1 - a1}, mk[2] = {m1, m2}; double* ek[2] = {&e1, &e2};
+		for (int k = 0; k < 2; k++)
+			if (!p_admissible(M, k, p_ame(M, k, ak[k], mk[k], *ek[k]))) {
+				const double pm = p_mix(M, a1, m1, m2, rhoe);
+				if (p_admissible(M, k, pm)) *ek[k] = eps_amp(M, k, ak[k], mk[k], pm);
+			}
+	}
+	ClIn in{}; in.atr[0] = a1; in.atr[1] = 1 - a1; in.mtr[0] = m1; in.mtr[1] = m2; in.etr[0] = e1; in.etr[1] = e2;
+	in.s = 0; in.rhoeC = rhoe; in.pguess = smax(pbar_of(M, Q), 1.0); in.Qph[0] = Qph[0]; in.Qph[1] = Qph[1];
+	ClOut o; if (!closure(M, in, o)) return false;
+	Q[IA1] = o.a[0]; Q[IM1] = o.m[0]; Q[IM2] = o.m[1]; Q[IE1] = o.e[0]; Q[IE2] = o.e[1];
+	return true;
+}
+static long hllc_cl_fail = 0;
+static bool hllc_relax(Ctx& X, std::vector<double>& U) {                                             // EX:344-361
+	bc_apply(X, U);
+	const int nx = X.nx, n = nx * X.ny;
+	std::vector<double> Qph((size_t)n * 2);
+	#pragma omp parallel for schedule(static)
+	for (int q = 0; q < n; q++) {
+		const int i = q % nx + GH, j = q / nx + GH;
+		auto cell = [&](int di, int dj) { return &U[NV * X.id(i + di, j + dj)]; };
+		shock_budget(X.M, cell, X.twoD, &Qph[(size_t)q * 2]);
+	}
+	bool ok = true; long fails = 0;
+	#pragma omp parallel for schedule(static) reduction(&&:ok) reduction(+:fails)
+	for (int q = 0; q < n; q++)
+		if (!hllc_relax_cell(X.M, &U[NV * X.id(q % nx + GH, q / nx + GH)], &Qph[(size_t)q * 2])) { ok = false; fails++; }
+	hllc_cl_fail += fails;
+	if (!ok) return false;
+	bc_apply(X, U);
+	return true;
+}
+static bool hllc_stage(Ctx& X, std::vector<double>& U, double h) {                                   // EX:363-372
+	bc_apply(X, U);
+	std::vector<Prim> w, WL[2], WR[2];
+	if (!hllc_recon(X, U, w, WL, WR)) return false;
+	std::vector<double> Unew;
+	if (!hllc_update(X, Unew, U, w, WL, WR, h)) return false;
+	if (!hllc_relax(X, Unew)) return false;
+	U.swap(Unew);
+	return true;
+}
+// one step (EX:546-556): first order, one stage; second order, Heun U2 = R(U1 + dt L(U1)) after U1, then (U + U2) / 2 relaxed
+static bool hllc_attempt(Ctx& X, double dt) {
+	std::vector<double> V = X.U;
+	bool done;
+	if (X.C.order < 2) done = hllc_stage(X, V, dt);
+	else {
+		const std::vector<double> W = V;
+		done = hllc_stage(X, V, dt) && hllc_stage(X, V, dt);
+		if (done) {
+			#pragma omp parallel for schedule(static)
+			for (size_t q = 0; q < V.size(); q++) V[q] = 0.5 * (V[q] + W[q]);
+			done = hllc_relax(X, V);
+		}
+	}
+	if (done) X.U = V;
+	return done;
+}
+// the time loop (EX:537-566): dt = min(cfl * caldt(acoustic), t_end - t), 30 halvings at most
+static int hllc_run(Ctx& X, double t_end, long max_steps, double cfl) {
+	long done_steps = 0;
+	while (X.t < t_end * (1 - 1e-12) && (max_steps < 0 || done_steps < max_steps)) {
+		double dt = smin(cfl * caldt(X, true), t_end - X.t);
+		if (!(dt > 1e-12 * t_end)) { X.st.status = 3; return 2; }
+		bool ok = false;
+		for (int att = 0; att < 30 && !ok; att++) {
+			ok = hllc_attempt(X, dt);
+			if (!ok) { X.st.rejects++; dt *= 0.5; }
+		}
+		if (!ok) { X.st.status = 2; return 2; }
+		X.t = X.t + dt; X.st.steps++; X.st.dt_last = dt; done_steps++;
+	}
+	X.st.t = X.t;
+	return 0;
+}
+
 // ------------------------------------------------------------------------------------------------ reference cases (PHY:961-1100 subset)
 static void set_SG(Cfg& C, int k, double g, double pi) { C.ph[k][0] = g; C.ph[k][1] = pi; C.ph[k][2] = 0; C.ph[k][3] = 0; C.ph[k][4] = 1000.0; C.ph[k][5] = 0; }   // PHY:958
 static bool case_setup(int test, Cfg& C, double& tend) {                                           // PHY:961-1022 (SG cases)
@@ -1245,6 +1449,17 @@ void ora_get_state(void* h, double* Uout) {
 }
 void ora_get_pressure(void* h, double* P, double* pref) { Ctx& X = *(Ctx*)h; std::memcpy(P, X.P.data(), X.P.size() * sizeof(double)); *pref = X.P_REF; }
 int ora_run(void* h, double t_end, long max_steps) { return run(*(Ctx*)h, t_end, max_steps); }
+// the explicit HLLC scheme's time loop on the context's state (cfl: the acoustic CFL, 0.4 in the reference); the
+// relaxation's closure failures accumulate in ora_hllc_closure_fails
+int ora_hllc_run(void* h, double t_end, long max_steps, double cfl) { return hllc_run(*(Ctx*)h, t_end, max_steps, cfl); }
+long ora_hllc_closure_fails(void) { return hllc_cl_fail; }
+// one step of size dt (0: accepted; 1: the attempt failed, the state is unchanged)
+int ora_hllc_step(void* h, double dt) {
+	Ctx& X = *(Ctx*)h;
+	if (!hllc_attempt(X, dt)) return 1;
+	X.t = X.t + dt; X.st.steps++; X.st.dt_last = dt; X.st.t = X.t;
+	return 0;
+}
 // resume from another solver's state: pressure unknown and P_REF, and the driver clock (t, previous dt, start-up flag)
 void ora_set_pressure(void* h, const double* P, double pref) { Ctx& X = *(Ctx*)h; std::memcpy(X.P.data(), P, X.P.size() * sizeof(double)); X.P_REF = pref; }
 void ora_set_clock(void* h, double t, double dtprev, int startup) { Ctx& X = *(Ctx*)h; X.t = t; X.dtprev = dtprev; X.startup = startup != 0; }
