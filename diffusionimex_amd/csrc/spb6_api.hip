@@ -111,6 +111,10 @@ struct spb6_ctx {
 	float *dX = nullptr, *dY = nullptr; unsigned long long* dcount = nullptr; long dcap = 0; int dstride = 0; unsigned dseq = 0;
 	// the explicit HLLC scheme (allocated at its first call): cell primitives and the x / y face values
 	double *hxp = nullptr, *hfx = nullptr, *hfy = nullptr;
+	// an IMEX variant's arrays (cfg.variant != 0): the full face states, the TV-type variants' eTV_k and ETV, BB's
+	// alpha_1 step, TV's alpha_1 face donors
+	double *fwxL = nullptr, *fwxR = nullptr, *fwyL = nullptr, *fwyR = nullptr;
+	double *eTV1 = nullptr, *eTV2 = nullptr, *ETV = nullptr, *dA = nullptr, *donax = nullptr, *donay = nullptr;
 };
 
 struct spb6_mlp {
@@ -142,6 +146,9 @@ Dev make_dev(const spb6_ctx* c) {
 	D.ust = c->ust; D.vst = c->vst; D.cz = c->cz; D.Kw = c->Kw; D.Pold = c->Pold;
 	D.ubsx = c->ubsx; D.kpx = c->kpx; D.wLx = c->wLx; D.pvx = c->pvx; D.ubsy = c->ubsy; D.kpy = c->kpy; D.wLy = c->wLy; D.pvy = c->pvy;
 	D.Qph = c->Qph; D.flags = c->flags;
+	D.variant = c->cfg.variant;
+	D.eTV1 = c->eTV1; D.eTV2 = c->eTV2; D.ETV = c->ETV; D.dA = c->dA; D.donax = c->donax; D.donay = c->donay;
+	if (D.variant != VAR_PC) { D.wxL = c->fwxL; D.wxR = c->fwxR; D.wyL = c->fwyL; D.wyR = c->fwyR; }
 	return D;
 }
 
@@ -696,6 +703,9 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (!cfg || cfg->nx < 2 || cfg->ny < 1) { g_err = "bad config"; return nullptr; }
 	if (cfg->rus_local != 1) { g_err = "only the local Rusanov speed (rus_local = 1) is supported"; return nullptr; }
 	if (cfg->rk < 1 || cfg->rk > 3) { g_err = "rk must be 1, 2 or 3"; return nullptr; }
+	if (cfg->variant < SPB6_VARIANT_PC || cfg->variant > SPB6_VARIANT_BB) { g_err = "variant out of range"; return nullptr; }
+	if (cfg->variant != SPB6_VARIANT_PC && (cfg->analytic_jacobian || cfg->learned_guess)) {
+		g_err = "the IMEX variants run with the finite-difference Jacobian and the plain Newton guess"; return nullptr; }
 	spb6_ctx* c = new spb6_ctx();
 	c->cfg = *cfg; c->device = device;
 	std::memset(&c->stats, 0, sizeof c->stats);
@@ -715,6 +725,13 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (cfg->rk == 3 && dalloc(c, &c->H[2], NC * NV)) return fail("alloc");
 	if (g.twoD) { if (dalloc(c, &c->dony, NC * NV)) return fail("alloc"); }
 	else c->dony = c->donx;
+	if (cfg->variant != SPB6_VARIANT_PC) {
+		double** vg[] = {&c->fwxL, &c->fwxR, &c->fwyL, &c->fwyR};
+		for (auto p : vg) if (dalloc(c, p, NC * NV)) return fail("alloc");
+		double** va[] = {&c->eTV1, &c->eTV2, &c->ETV, &c->dA};
+		for (auto p : va) if (dalloc(c, p, NC)) return fail("alloc");
+		if (dalloc(c, &c->donax, 2 * NC) || dalloc(c, &c->donay, 2 * NC)) return fail("alloc");
+	}
 	double** arrs[] = {&c->phx, &c->phy, &c->a2s, &c->ubx, &c->uby, &c->vsc, &c->esc, &c->vu, &c->vv, &c->ust, &c->vst, &c->cz, &c->Kw, &c->Pold,
 	                   &c->ubsx, &c->kpx, &c->wLx, &c->pvx, &c->ubsy, &c->kpy, &c->wLy, &c->pvy};
 	for (auto p : arrs) if (dalloc(c, p, NC)) return fail("alloc");

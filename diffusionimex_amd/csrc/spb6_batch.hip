@@ -485,7 +485,8 @@ const char* spb6_batch_last_error(spb6_batch* b) { return b ? b->err.c_str() : "
 
 spb6_batch* spb6_batch_create(const spb6_batch_problem_t* probs, int nprob, const spb6_config_t* cfg, int device) {
 	if (!probs || nprob < 1 || !cfg) return nullptr;
-	if (cfg->rus_local != 1 || cfg->rk < 1 || cfg->rk > 2 || cfg->jreuse) return nullptr;   // the batch's Newton has no jreuse
+	// the batch runs the reference scheme (PC) with full Newton: no jreuse, no IMEX variant
+	if (cfg->rus_local != 1 || cfg->rk < 1 || cfg->rk > 2 || cfg->jreuse || cfg->variant) return nullptr;
 	spb6_batch* b = new spb6_batch();
 	b->device = device; b->cfg = *cfg;
 	auto fail = [&]() -> spb6_batch* { spb6_batch_destroy(b); return nullptr; };

@@ -759,6 +759,176 @@ __device__ __forceinline__ bool final_at(const Dev& D, const double* __restrict_
 	}
 	return true;
 }
+// ------------------------------------------------------------------------------------------------ IMEX variants
+// The reference's sibling programs change the implicit stage's cell assembly (cell_residual): TV (SPB6_TV_IMEX.cpp:
+// 641-646) transports alpha_1 and alpha_2 with the increments too and takes s = 1 - alpha_1 - alpha_2; the TV-type
+// variants (SPB6_TVpure / TVbb / BB_IMEX.cpp:655-675) move the phase energies by the upwind flux eps_k u_f from the
+// explicit step's eTV_k and the total energy by the enthalpy flux (sum eps_k + p_f) u_f from its ETV, mass and
+// momentum staying with the explicit step; TVinc / TVsplit (SPB6_TVinc_IMEX.cpp:655-687) add the mass increments, the
+// momentum terms and the increments' kinetic energy. A variant face also carries its alpha_1 donors (TV).
+struct FaceInV : FaceIn { double cla, cra; };
+struct VarCell { double a2s, eT1, eT2, ET; };
+SPB_HD double var_inc(const FaceInV& f, const FaceVals& v, int k) {   // inc (PC:617) over the 6-vector (alpha1, 1 - alpha1, m1, m2, e1, e2)
+	if (k == 0) return f.cla * v.wL + f.cra * v.wR;
+	if (k == 1) return (1.0 - f.cla) * v.wL + (1.0 - f.cra) * v.wR;
+	return finc(f, v, k - 2);
+}
+SPB_HD double var_up(const FaceInV& f, const FaceVals& v, int k) { return v.uf > 0.0 ? f.cl[k - 2] : f.cr[k - 2]; }   // up (TVpure:656)
+template <class FaceAt>
+SPB_HD void cell_assemble_var(int var, double P_REF, double rx, double ry, bool twoD, const double xs[NV], const VarCell& vc, FaceAt face_at,
+                              double Pc, double PW, double PE, double PS, double PN, double X[6], double& s, double& mx, double& my, double& E) {
+	FaceInV fa, fb, fc, fd;
+	FaceVals fe, fw, fn, fs;
+	face_at(0, fa); face_at(1, fb);
+	face_eval(fa, Pc, PE, fe); face_eval(fb, PW, Pc, fw);
+	if (twoD) { face_at(2, fc); face_at(3, fd); face_eval(fc, Pc, PN, fn); face_eval(fd, PS, Pc, fs); }
+	const bool tvx = var >= VAR_TVPURE, tvinc = var == VAR_TVINC || var == VAR_TVSPLIT;
+	X[0] = xs[IA1]; X[1] = vc.a2s; X[2] = xs[IM1]; X[3] = xs[IM2]; X[4] = tvx ? vc.eT1 : xs[IE1]; X[5] = tvx ? vc.eT2 : xs[IE2];
+	if (tvx)
+		for (int k = 4; k < 6; k++) {
+			X[k] -= rx * (var_up(fa, fe, k) * fe.uf - var_up(fb, fw, k) * fw.uf);
+			if (twoD) X[k] -= ry * (var_up(fc, fn, k) * fn.uf - var_up(fd, fs, k) * fs.uf);
+		}
+	const int k0 = var == VAR_TV ? 0 : 2, k1 = !tvx ? 6 : tvinc ? 4 : 2;
+	for (int k = k0; k < k1; k++) {
+		X[k] -= rx * (var_inc(fa, fe, k) - var_inc(fb, fw, k));
+		if (twoD) X[k] -= ry * (var_inc(fc, fn, k) - var_inc(fd, fs, k));
+	}
+	if (var == VAR_TV) s = 1.0 - X[0] - X[1];
+	else {
+		X[1] = 1.0 - X[0];
+		s = rx * (fe.uf - fw.uf) + (twoD ? ry * (fn.uf - fs.uf) : 0.0);
+		X[0] *= 1.0 - s; X[1] *= 1.0 - s;
+	}
+	auto hflux = [&](const FaceInV& f, const FaceVals& v) { return (var_up(f, v, 4) + var_up(f, v, 5) + P_REF + v.pf) * v.uf; };
+	auto kinc = [&](const FaceInV& f, const FaceVals& v, double M) {
+		return 0.5 * v.uf * v.uf * M + 0.5 * (f.utl * f.utl * (f.cl[0] + f.cl[1]) * v.wL + f.utr * f.utr * (f.cr[0] + f.cr[1]) * v.wR); };
+	double Me, Mte, Ee, Mw, Mtw, Ew;
+	fflux(fa, fe, P_REF, Me, Mte, Ee); fflux(fb, fw, P_REF, Mw, Mtw, Ew);
+	const bool mom_inc = !tvx || tvinc;
+	if (mom_inc) { mx = xs[IMU] - rx * (fe.pf - fw.pf) - rx * (fe.uf * Me - fw.uf * Mw); my = xs[IMV] - rx * (Mte - Mtw); }
+	else { mx = xs[IMU] - rx * (fe.pf - fw.pf); my = xs[IMV]; }
+	if (!tvx) E = xs[IEN] - rx * (Ee - Ew);
+	else if (tvinc) E = vc.ET - rx * (hflux(fa, fe) + kinc(fa, fe, Me) - hflux(fb, fw) - kinc(fb, fw, Mw));
+	else E = vc.ET - rx * (hflux(fa, fe) - hflux(fb, fw));
+	if (twoD) {
+		double Mn, Mtn, En, Ms, Mts, Es;
+		fflux(fc, fn, P_REF, Mn, Mtn, En); fflux(fd, fs, P_REF, Ms, Mts, Es);
+		if (mom_inc) { my -= ry * (fn.pf - fs.pf) + ry * (fn.uf * Mn - fs.uf * Ms); mx -= ry * (Mtn - Mts); }
+		else my -= ry * (fn.pf - fs.pf);
+		if (!tvx) E -= ry * (En - Es);
+		else if (tvinc) E -= ry * (hflux(fc, fn) + kinc(fc, fn, Mn) - hflux(fd, fs) - kinc(fd, fs, Ms));
+		else E -= ry * (hflux(fc, fn) - hflux(fd, fs));
+	}
+}
+template <class FaceAt>
+SPB_HD bool cell_res_var(int var, const Phys& P, double P_REF, double rx, double ry, bool twoD, const double xs[NV], const VarCell& vc, double Q0,
+                         double Q1, FaceAt face_at, double Pc, double PW, double PE, double PS, double PN, double& G, ResOut* out, bool want_sig = true) {
+	double X[6], s, mx, my, E;
+	cell_assemble_var(var, P_REF, rx, ry, twoD, xs, vc, face_at, Pc, PW, PE, PS, PN, X, s, mx, my, E);
+	const double rho = X[2] + X[3];
+	if (!(rho > 0)) return false;
+	ClIn in; in.atr0 = X[0]; in.atr1 = X[1]; in.mtr0 = X[2]; in.mtr1 = X[3]; in.etr0 = X[4]; in.etr1 = X[5];
+	in.s = s; in.rhoeC = E - 0.5 * (mx * mx + my * my) / rho; in.pguess = P_REF + Pc; in.Q0 = Q0; in.Q1 = Q1;
+	ClOut o; if (!closure_p(P, in, o)) return false;
+	G = (o.pbar - P_REF) - Pc;
+	if (out) { out->a0 = o.a0; out->a1 = o.a1; out->m0 = X[2]; out->m1 = X[3]; out->e0 = o.e0; out->e1 = o.e1; out->mx = mx; out->my = my; out->E = E; out->sig = want_sig ? closure_roundoff(P, in, o.pbar) : 0.0; out->p = o.pbar; }
+	return isfin(G);
+}
+// the faces and cell values of interior row q for a variant (TV: the alpha_1 donors too)
+struct CellFacesV {
+	const Dev& D; int q, c, i, j;
+	__device__ CellFacesV(const Dev& D_, int q_) : D(D_), q(q_) { i = q % D.g.nx + GH; j = q / D.g.nx + GH; c = i + j * D.g.NXT; }
+	__device__ void operator()(int w, FaceInV& f) const {
+		const bool y = w >= 2;
+		const int cf = w == 0 ? c : w == 1 ? c - 1 : w == 2 ? c : c - D.g.NXT;
+		load_face(D, y, cf, w == 0 ? i : w == 1 ? i - 1 : w == 2 ? j : j - 1, f);
+		if (D.variant == VAR_TV) {
+			const double* da = y ? D.donay : D.donax;
+			f.cla = da[2 * cf + 1]; f.cra = da[2 * (cf + (y ? D.g.NXT : 1))];
+		} else { f.cla = 0.0; f.cra = 0.0; }
+	}
+	__device__ void cell(double xs[NV], VarCell& vc, double& Q0, double& Q1, double& Kw) const {
+		#pragma unroll
+		for (int k = 0; k < NV; k++) xs[k] = D.Ust[k * D.g.NC + c];
+		vc.a2s = D.a2s[c];
+		if (D.variant >= VAR_TVPURE) { vc.eT1 = D.eTV1[c]; vc.eT2 = D.eTV2[c]; vc.ET = D.ETV[c]; } else { vc.eT1 = vc.eT2 = vc.ET = 0.0; }
+		Q0 = D.Qph[c]; Q1 = D.Qph[D.g.NC + c]; Kw = D.Kw[c];
+	}
+};
+// residual_at, jacobian_at, floor_ok_at and final_at of a variant (the same outputs and operation order, the variant's
+// cell assembly)
+__device__ __forceinline__ bool residual_var_at(const Dev& D, const double* __restrict__ P, const double* __restrict__ dP, double lam,
+                                                double* __restrict__ Pout, double* __restrict__ G, int q, double& nv) {
+	const Nb b = neighbours(D.g, q);
+	auto pv = [&](int r) { return dP ? P[r] + lam * dP[r] : P[r]; };
+	const double Pc = pv(q), PW = pv(b.W), PE = pv(b.E);
+	const double PS = D.g.twoD ? pv(b.S) : 0.0, PN = D.g.twoD ? pv(b.N) : 0.0;
+	const CellFacesV F(D, q);
+	double xs[NV], Q0, Q1, Kw; VarCell vc; F.cell(xs, vc, Q0, Q1, Kw);
+	double g;
+	bool ok = true;
+	nv = 0.0;
+	if (!cell_res_var(D.variant, D.ph, D.P_REF, D.rx, D.ry, D.g.twoD, xs, vc, Q0, Q1, F, Pc, PW, PE, PS, PN, g, nullptr)) { ok = false; g = dnan(); }
+	else nv = fabs(g) / (fabs(D.P_REF + Pc) + Kw + 1.0);
+	G[q] = g;
+	if (Pout) Pout[q] = Pc;
+	return ok;
+}
+__device__ __forceinline__ bool jacobian_var_at(const Dev& D, const double* __restrict__ P, const double* __restrict__ G, const double* __restrict__ sc,
+                                                double* __restrict__ J, double* __restrict__ rhs, int q) {
+	const int n = D.g.n; const bool twoD = D.g.twoD;
+	const Nb b = neighbours(D.g, q);
+	const CellFacesV F(D, q);
+	double xs[NV], Q0, Q1, Kw; VarCell vc; F.cell(xs, vc, Q0, Q1, Kw);
+	const double Pc = P[q], PW = P[b.W], PE = P[b.E], PS = twoD ? P[b.S] : 0.0, PN = twoD ? P[b.N] : 0.0, Gq = G[q];
+	rhs[q] = -Gq;
+	auto res = [&](double a, double w, double e, double s_, double nn, double& g) {
+		return cell_res_var(D.variant, D.ph, D.P_REF, D.rx, D.ry, twoD, xs, vc, Q0, Q1, F, a, w, e, s_, nn, g, nullptr); };
+	const double h = 1e-7 * sc[q];
+	double g1; bool ok = res(Pc + h, PW, PE, PS, PN, g1);
+	double v0 = 0, vW = 0, vE = 0, vS = 0, vN = 0;
+	if (ok) {
+		v0 = (g1 - Gq) / h;
+		double hn = 1e-7 * sc[b.W], g2;
+		ok = res(Pc, PW + hn, PE, PS, PN, g2); if (ok) vW = (g2 - Gq) / hn;
+		if (ok) { hn = 1e-7 * sc[b.E]; ok = res(Pc, PW, PE + hn, PS, PN, g2); if (ok) vE = (g2 - Gq) / hn; }
+		if (ok && twoD) { hn = 1e-7 * sc[b.S]; ok = res(Pc, PW, PE, PS + hn, PN, g2); if (ok) vS = (g2 - Gq) / hn; }
+		if (ok && twoD) { hn = 1e-7 * sc[b.N]; ok = res(Pc, PW, PE, PS, PN + hn, g2); if (ok) vN = (g2 - Gq) / hn; }
+	}
+	J[q] = v0; J[n + q] = vW; J[2 * n + q] = vE;
+	if (twoD) { J[3 * n + q] = vS; J[4 * n + q] = vN; }
+	return ok;
+}
+__device__ __forceinline__ bool floor_ok_var_at(const Dev& D, const double* __restrict__ P, double tol, int q) {
+	const Nb b = neighbours(D.g, q);
+	const CellFacesV F(D, q);
+	double xs[NV], Q0, Q1, Kw; VarCell vc; F.cell(xs, vc, Q0, Q1, Kw);
+	const bool twoD = D.g.twoD;
+	double g; ResOut o;
+	if (!cell_res_var(D.variant, D.ph, D.P_REF, D.rx, D.ry, twoD, xs, vc, Q0, Q1, F, P[q], P[b.W], P[b.E], twoD ? P[b.S] : 0.0, twoD ? P[b.N] : 0.0, g, &o))
+		return false;
+	return !(fabs(g) > tol * (fabs(D.P_REF + P[q]) + Kw + 1.0) + o.sig);
+}
+__device__ __forceinline__ bool final_var_at(const Dev& D, const double* __restrict__ P, const double* __restrict__ Qs, double adt, double* __restrict__ H,
+                                             int q, double* T0, double* T1) {
+	const Geo& g = D.g; const int NC = g.NC, c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
+	const Nb b = neighbours(g, q);
+	const CellFacesV F(D, q);
+	double xs[NV], Q0, Q1, Kw; VarCell vc; F.cell(xs, vc, Q0, Q1, Kw);
+	const bool twoD = g.twoD;
+	double gg; ResOut o;
+	if (!cell_res_var(D.variant, D.ph, D.P_REF, D.rx, D.ry, twoD, xs, vc, Q0, Q1, F, P[q], P[b.W], P[b.E], twoD ? P[b.S] : 0.0, twoD ? P[b.N] : 0.0,
+	                  gg, &o, false))
+		return false;
+	const double u[NV] = {o.a0, o.m0, o.m1, o.mx, o.my, o.e0, o.e1, o.E};
+	#pragma unroll
+	for (int k = 0; k < NV; k++) { D.Ust[k * NC + c] = u[k]; H[k * NC + c] = (u[k] - Qs[k * NC + c]) / adt; }
+	const double t0 = T_amp(D.ph, 0, o.a0, o.m0, o.p), t1 = T_amp(D.ph, 1, o.a1, o.m1, o.p);
+	*T0 = (o.a0 > 1e-9 && isfin(t0)) ? smax(t0, 0.0) : 0.0;
+	*T1 = (o.a1 > 1e-9 && isfin(t1)) ? smax(t1, 0.0) : 0.0;
+	return true;
+}
 // caldt terms of row q (PC:477-488): |u| + c and |v| + c (c only when acoustic) for admissible primitives, else 0
 __device__ __forceinline__ void caldt_at(const Geo& g, const Phys& P, const double* __restrict__ U, int acoustic, int q, double& ux, double& uy) {
 	ux = 0.0; uy = 0.0;

@@ -1204,6 +1204,7 @@ void launch_combine(const Geo& g, const double* Un, const double* H0, const doub
 void launch_explicit(const Dev& D, int limiter, int order, const double* Qe, const double* Qs, double adt, hipStream_t st) {
 	const Geo& g = D.g;
 	hipLaunchKernelGGL(k_scales, dim3(nblk(g.NC)), dim3(256), 0, st, g, D.ph, Qe, D.vsc, D.esc, D.vu, D.vv, D.flags);
+	if (D.variant != VAR_PC) { launch_explicit_var(D, limiter, order, Qe, Qs, adt, st); return; }
 	if (g.twoD) {
 		constexpr int TX = 32, TY = 8;
 		const int tx = (g.NXT + TX - 1) / TX, ty = (g.NYT + TY - 1) / TY;
@@ -1244,6 +1245,7 @@ void launch_shock(const Dev& D, int* list, int* count, double* rec, unsigned lon
 }
 void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st,
                      const int* gate) {
+	if (D.variant != VAR_PC) { launch_residual_var(D, P, dP, lam, Pout, G, nrm, flag_slot, st, gate); return; }
 	hipLaunchKernelGGL(k_residual, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, dP, lam, Pout, G, nrm, D.flags, flag_slot, gate);
 }
 void launch_residual_list(const Dev& D, const double* P, const int* list, const int* count, double* G, unsigned long long* nrm, int flag_slot,
@@ -1257,7 +1259,8 @@ void launch_spec_commit(long n, const float* x, double* dP, const KS* ks, const 
 void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, bool analytic, hipStream_t st, int* clear,
                      int nclear) {
 	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc, clear, nclear);
-	if (analytic) hipLaunchKernelGGL(k_jacobian_an, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, J, rhs, D.flags);
+	if (D.variant != VAR_PC) launch_jacobian_var(D, P, G, sc, J, rhs, st);
+	else if (analytic) hipLaunchKernelGGL(k_jacobian_an, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, J, rhs, D.flags);
 	else hipLaunchKernelGGL(k_jacobian, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, (const double*)sc, J, rhs, D.flags);
 }
 void launch_chord_rhs(const Dev& D, const double* P, const double* G, double* sc, double* rhs, hipStream_t st, int* clear, int nclear) {
@@ -1267,9 +1270,11 @@ void launch_stepcrit(int n, const double* dP, const double* P, const double* sc,
 	hipLaunchKernelGGL(k_stepcrit, dim3(nblk(n)), dim3(256), 0, st, n, dP, P, sc, nrm);
 }
 void launch_floor(const Dev& D, const double* P, double tol, hipStream_t st) {
+	if (D.variant != VAR_PC) { launch_floor_var(D, P, tol, st); return; }
 	hipLaunchKernelGGL(k_floor, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, tol, D.flags);
 }
 void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, unsigned long long* tmax, hipStream_t st, int flag_slot) {
+	if (D.variant != VAR_PC) { launch_final_var(D, P, Qs, adt, H, tmax, st, flag_slot); return; }
 	hipLaunchKernelGGL(k_final, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, Qs, adt, H, D.flags, flag_slot, tmax);
 }
 void launch_cell_diag(const Geo& g, const Phys& P, const double* U, double* out, hipStream_t st) {

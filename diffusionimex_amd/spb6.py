@@ -17,6 +17,8 @@ BC_TRANSMISSIVE, BC_WALL, BC_PERIODIC = 0, 1, 2
 STATUS = {0: "OK", 1: "RHO_NONPOSITIVE", 2: "STAGE_INADMISSIBLE", 3: "CLOSURE_FAIL", 4: "NEWTON_FAIL",
           5: "KRYLOV_FAIL", 6: "DT_UNDERFLOW", 7: "STEP_FAILED", 8: "BAD_ARG", 9: "HIP_ERROR"}
 STATUS_CODE = {v: k for k, v in STATUS.items()}
+# IMEX variants (spb6_variant): the reference's sibling programs SPB6_<name>_IMEX.cpp
+VARIANTS = {"PC": 0, "TV": 1, "TVpure": 2, "TVinc": 3, "TVsplit": 4, "TVbb": 5, "BB": 6}
 
 
 class Phase(C.Structure):
@@ -34,7 +36,7 @@ class Config(C.Structure):
         ("krylov_tol", C.c_double), ("krylov_iter", C.c_int),
         ("limiter", C.c_int), ("order", C.c_int), ("rus_local", C.c_int),
         ("krylov_fp32", C.c_int), ("learned_guess", C.c_int), ("analytic_jacobian", C.c_int),
-        ("jreuse", C.c_int),
+        ("jreuse", C.c_int), ("variant", C.c_int),
     ]
 
 
@@ -162,7 +164,7 @@ def _dp(a):
 def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0, 1.0), bc=(0, 0, 0, 0), bcu=(0, 0, 0, 0),
                 rk=2, cfl=0.2, cfl_ac=0.4, ramp=1, dtfix=0.0, acdt=0, newton_tol=1e-13, newton_iter=60,
                 krylov_tol=1e-13, krylov_iter=4000, limiter=1, order=2, rus_local=1, krylov_fp32=0, learned_guess=0,
-                analytic_jacobian=0, jreuse=0):
+                analytic_jacobian=0, jreuse=0, variant=0):
     """Config with the reference defaults (PC:57-80): phases are (gamma, p_inf[, q, qp, cv, b])."""
     c = Config()
     c.nx, c.ny = nx, ny
@@ -180,6 +182,7 @@ def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0
     c.limiter, c.order, c.rus_local, c.krylov_fp32, c.learned_guess = limiter, order, rus_local, krylov_fp32, learned_guess
     c.analytic_jacobian = analytic_jacobian
     c.jreuse = jreuse
+    c.variant = variant
     return c
 
 

@@ -41,6 +41,18 @@ enum spb6_status {
     SPB6_HIP_ERROR = 9
 };
 
+/* IMEX variants: the reference's sibling programs, the same time step with another explicit update and cell residual
+ * (src/SPB6_<name>_IMEX.cpp; docs/devlog.md section 23.6 tabulates them) */
+enum spb6_variant {
+    SPB6_VARIANT_PC = 0,       /* SPB6_PC_IMEX.cpp: the paper scheme */
+    SPB6_VARIANT_TV = 1,       /* SPB6_TV_IMEX.cpp: alpha moved by its flux and the increments, s = 1 - alpha_1 - alpha_2 */
+    SPB6_VARIANT_TVPURE = 2,   /* SPB6_TVpure_IMEX.cpp: TV split of the energy (kinetic explicit, enthalpy flux implicit) */
+    SPB6_VARIANT_TVINC = 3,    /* SPB6_TVinc_IMEX.cpp: TVpure + mass / momentum increments */
+    SPB6_VARIANT_TVSPLIT = 4,  /* SPB6_TVsplit_IMEX.cpp: TVinc + explicit phase-energy dissipation */
+    SPB6_VARIANT_TVBB = 5,     /* SPB6_TVbb_IMEX.cpp: TVpure + explicit phase-energy dissipation */
+    SPB6_VARIANT_BB = 6        /* SPB6_BB_IMEX.cpp: TVpure with the face states moved to the explicit new alpha_1 */
+};
+
 /* Phase EOS parameters (PHY:49-54 Phase{g, pi, q, qp, cv, b}); stiffened gas when b = 0 */
 typedef struct spb6_phase {
     double gamma, pi_inf, q, qp, cv, b;
@@ -76,6 +88,8 @@ typedef struct spb6_config {
                                         step that contracted the residual less than 4x, and a line search that fails with
                                         a reused Jacobian rebuilds it and retakes the step. Single-grid stepper only (the
                                         batch rejects it) */
+    int variant;                     /* spb6_variant: which of the reference's IMEX programs the step follows (single-grid
+                                        stepper, finite-difference Jacobian and plain guess only; the batch rejects it) */
 } spb6_config_t;
 
 /* run statistics: the counters of the reference's stats line (PC:1337-1339) plus Krylov counts */

@@ -27,7 +27,7 @@ class Cfg(C.Structure):
         ("krylov_tol", C.c_double), ("krylov_iter", C.c_int),
         ("limiter", C.c_int), ("order", C.c_int), ("rus_local", C.c_int),
         ("closure_path", C.c_int), ("share_start", C.c_int), ("dot_order", C.c_int),
-        ("portable_math", C.c_int), ("nthreads", C.c_int), ("jreuse", C.c_int),
+        ("portable_math", C.c_int), ("nthreads", C.c_int), ("jreuse", C.c_int), ("variant", C.c_int),
     ]
 
 
@@ -132,6 +132,7 @@ def default_cfg(nx, ny=1, **kw):
     c.closure_path, c.share_start, c.dot_order, c.portable_math = 1, 1, 2, 1
     c.nthreads = 0
     c.jreuse = 0
+    c.variant = 0
     for k, v in kw.items():
         if k == "bc":
             for q in range(4):

@@ -487,6 +487,7 @@ struct Cfg {
 	int closure_path, share_start, dot_order, portable_math;
 	int nthreads;
 	int jreuse;   // JREUSE (PCO:83): modified Newton, the Jacobian rebuilt only as PCO:826-828 decides
+	int variant;  // IMEX variant: 0 PC (SPB6_PC_IMEX.cpp), 1 TV, 2 TVpure, 3 TVinc, 4 TVsplit, 5 TVbb, 6 BB (SPB6_<name>_IMEX.cpp)
 };
 struct Stats { long steps, rk1, rejects, floor_acc, stages, newton_it, krylov_it, linsolves; double maxres, t, dt_last; int status;
                long jac_builds; };
@@ -500,6 +501,7 @@ struct Ctx {
 	// grids (AoS, PC:112-131), arrays (PC:133-152)
 	std::vector<double> U, Un, Qe, Qs, Ust, wxL, wxR, wyL, wyR, H[3];
 	std::vector<double> phx, phy, a2s, ubx, uby;
+	std::vector<double> eTV1, eTV2, ETV, dA;   // the TV-type variants' explicit phase energies and total energy; BB's alpha_1 step
 	// stage data (PC:565-582)
 	double tau, rx, ry;
 	std::vector<double> ust, vst, ubsx, kpx, wLx, pvx, ubsy, kpy, wLy, pvy, Kw, Qph, cz, Pold;
@@ -522,7 +524,8 @@ static void init_ctx(Ctx& X, const Cfg& C) {
 	X.dx = (C.x1 - C.x0) / C.nx; X.dy = (C.y1 - C.y0) / C.ny; X.twoD = C.ny > 1;
 	const size_t nc = (size_t)X.NXT * X.NYT;
 	for (auto* g : {&X.U, &X.Un, &X.Qe, &X.Qs, &X.Ust, &X.wxL, &X.wxR, &X.wyL, &X.wyR, &X.H[0], &X.H[1], &X.H[2]}) g->assign(nc * NV, 0.0);
-	for (auto* a : {&X.phx, &X.phy, &X.a2s, &X.ubx, &X.uby, &X.ust, &X.vst, &X.ubsx, &X.kpx, &X.pvx, &X.ubsy, &X.kpy, &X.pvy, &X.Kw, &X.cz, &X.Pold}) a->assign(nc, 0.0);
+	for (auto* a : {&X.phx, &X.phy, &X.a2s, &X.ubx, &X.uby, &X.ust, &X.vst, &X.ubsx, &X.kpx, &X.pvx, &X.ubsy, &X.kpy, &X.pvy, &X.Kw, &X.cz, &X.Pold,
+	                &X.eTV1, &X.eTV2, &X.ETV, &X.dA}) a->assign(nc, 0.0);
 	X.wLx.assign(nc, 0.5); X.wLy.assign(nc, 0.5); X.Qph.assign(nc * 2, 0.0);
 	X.P.assign((size_t)C.nx * C.ny, 0.0); X.Pst = X.P;
 }
@@ -665,6 +668,33 @@ static void rusanov(const double* QL, const double* QR, int dirk, double* flux) 
 	const double a = smax(std::fabs(QL[dirk] / (QL[IM1] + QL[IM2])), std::fabs(QR[dirk] / (QR[IM1] + QR[IM2])));
 	for (int k = 0; k < NV; k++) flux[k] = 0.5 * (fL[k] + fR[k]) - 0.5 * a * (QR[k] - QL[k]);
 }
+// IMEX variants (the reference's sibling programs): TV (SPB6_TV_IMEX.cpp), TVpure, TVinc, TVsplit, TVbb, BB
+enum { VAR_PC = 0, VAR_TV, VAR_TVPURE, VAR_TVINC, VAR_TVSPLIT, VAR_TVBB, VAR_BB };
+// explicit kinetic-energy flux of the TV-type variants: k = E - eps_1 - eps_2 transported with u, Rusanov dissipation on
+// its jump (SPB6_TVpure_IMEX.cpp:252-258)
+static double kin_flux(const double* QL, const double* QR, int IM) {
+	const double rL = QL[IM1] + QL[IM2], rR = QR[IM1] + QR[IM2], uL = QL[IM] / rL, uR = QR[IM] / rR;
+	const double kL = QL[IEN] - QL[IE1] - QL[IE2], kR = QR[IEN] - QR[IE1] - QR[IE2];
+	const double a = smax(std::fabs(uL), std::fabs(uR));
+	return 0.5 * (kL * uL + kR * uR) - 0.5 * a * (kR - kL);
+}
+// the explicit dissipation of phase energy k beyond the face velocity's own (SPB6_TVsplit_IMEX.cpp:259-263)
+static double eps_diss(const double* QL, const double* QR, int IM, int k, double ube) {
+	const double uL = QL[IM] / (QL[IM1] + QL[IM2]), uR = QR[IM] / (QR[IM1] + QR[IM2]);
+	const double a = smax(std::fabs(uL), std::fabs(uR));
+	return -0.5 * smax(a - std::fabs(ube), 0.0) * (QR[k] - QL[k]);
+}
+// BB: the face states of cell c moved to its explicit new alpha_1 (SPB6_BB_IMEX.cpp:310-318): phase masses and energies
+// scaled by the volume fraction ratio, velocity kept
+static void bb_rescale(double* w, double d) {
+	const double a1 = w[IA1], a2 = 1.0 - a1;
+	if (!(a1 > 0 && a2 > 0)) return;
+	const double r1 = smax(a1 + d, 0.0) / a1, r2 = smax(a2 - d, 0.0) / a2;
+	const double rho = w[IM1] + w[IM2], u = w[IMU] / rho, v = w[IMV] / rho;
+	w[IM1] *= r1; w[IE1] *= r1; w[IM2] *= r2; w[IE2] *= r2;
+	const double rn = w[IM1] + w[IM2];
+	w[IMU] = rn * u; w[IMV] = rn * v; w[IEN] = w[IE1] + w[IE2] + 0.5 * rn * (u * u + v * v);
+}
 static bool explicit_update(Ctx& X, double adt) {                                                    // PC:251-326 (rus=1)
 	const int g = GH, nx = X.nx, ny = X.ny, NXT = X.NXT, NYT = X.NYT;
 	bool rok = true;
@@ -680,23 +710,70 @@ static bool explicit_update(Ctx& X, double adt) {                               
 		X.uby[X.id(i, j)] = 0.5 * (vL + vR);
 	}
 	const double rxd = adt / X.dx, ryd = adt / X.dy;
+	const int V = X.C.variant;
+	if (V == VAR_BB) {   // SPB6_BB_IMEX.cpp:289-324: alpha_1's explicit step from the unscaled faces, then every face state rescaled
+		std::fill(X.dA.begin(), X.dA.end(), 0.0);
+		#pragma omp parallel for schedule(static)
+		for (int q = 0; q < nx * ny; q++) {
+			const int i = q % nx + g, j = q / nx + g;
+			double fp[NV], fm[NV];
+			rusanov(&X.wxR[NV * X.id(i, j)], &X.wxL[NV * X.id(i + 1, j)], IMU, fp);
+			rusanov(&X.wxR[NV * X.id(i - 1, j)], &X.wxL[NV * X.id(i, j)], IMU, fm);
+			double a = X.u(X.Qs, i, j, IA1) - rxd * (fp[IA1] - fm[IA1]) + rxd * X.u(X.Qe, i, j, IA1) * (X.ubx[X.id(i, j)] - X.ubx[X.id(i - 1, j)]);
+			if (ny > 1) {
+				rusanov(&X.wyR[NV * X.id(i, j)], &X.wyL[NV * X.id(i, j + 1)], IMV, fp);
+				rusanov(&X.wyR[NV * X.id(i, j - 1)], &X.wyL[NV * X.id(i, j)], IMV, fm);
+				a += -ryd * (fp[IA1] - fm[IA1]) + ryd * X.u(X.Qe, i, j, IA1) * (X.uby[X.id(i, j)] - X.uby[X.id(i, j - 1)]);
+			}
+			X.dA[X.id(i, j)] = a - X.u(X.Qe, i, j, IA1);
+		}
+		#pragma omp parallel for schedule(static)
+		for (int c = 0; c < NXT * NYT; c++) {
+			const double d = X.dA[c];
+			if (d == 0.0) continue;
+			bb_rescale(&X.wxL[NV * (size_t)c], d); bb_rescale(&X.wxR[NV * (size_t)c], d);
+			if (ny > 1) { bb_rescale(&X.wyL[NV * (size_t)c], d); bb_rescale(&X.wyR[NV * (size_t)c], d); }
+		}
+	}
 	#pragma omp parallel for schedule(static)
 	for (int q = 0; q < nx * ny; q++) {
 		const int i = q % nx + g, j = q / nx + g;
+		const size_t c = X.id(i, j);
 		double fp[NV], fm[NV];
 		rusanov(&X.wxR[NV * X.id(i, j)], &X.wxL[NV * X.id(i + 1, j)], IMU, fp);
 		rusanov(&X.wxR[NV * X.id(i - 1, j)], &X.wxL[NV * X.id(i, j)], IMU, fm);
 		double* o = &X.Ust[NV * X.id(i, j)];
 		for (int k = 0; k < NV; k++) o[k] = X.u(X.Qs, i, j, k) - rxd * (fp[k] - fm[k]);
-		o[IA1] += rxd * X.u(X.Qe, i, j, IA1) * (X.ubx[X.id(i, j)] - X.ubx[X.id(i - 1, j)]);
+		double a2 = 0.0;
+		const double *Le = &X.wxR[NV * c], *Re = &X.wxL[NV * X.id(i + 1, j)], *Lw = &X.wxR[NV * X.id(i - 1, j)], *Rw = &X.wxL[NV * c];
+		if (V == VAR_TV)                                                                             // TV:298
+			a2 = (1.0 - X.u(X.Qs, i, j, IA1)) - rxd * ((X.ubx[c] - fp[IA1]) - (X.ubx[X.id(i - 1, j)] - fm[IA1]));
+		else if (V == VAR_TVSPLIT || V == VAR_TVBB) {                                                // TVsplit:313-316
+			const double d1 = eps_diss(Le, Re, IMU, IE1, X.ubx[c]) - eps_diss(Lw, Rw, IMU, IE1, X.ubx[X.id(i - 1, j)]);
+			const double d2 = eps_diss(Le, Re, IMU, IE2, X.ubx[c]) - eps_diss(Lw, Rw, IMU, IE2, X.ubx[X.id(i - 1, j)]);
+			X.eTV1[c] = X.u(X.Qs, i, j, IE1) - rxd * d1; X.eTV2[c] = X.u(X.Qs, i, j, IE2) - rxd * d2;
+			X.ETV[c] = X.u(X.Qs, i, j, IEN) - rxd * (kin_flux(Le, Re, IMU) - kin_flux(Lw, Rw, IMU) + d1 + d2);
+		} else if (V != VAR_PC) {                                                                     // TVpure:306-307
+			X.eTV1[c] = X.u(X.Qs, i, j, IE1); X.eTV2[c] = X.u(X.Qs, i, j, IE2);
+			X.ETV[c] = X.u(X.Qs, i, j, IEN) - rxd * (kin_flux(Le, Re, IMU) - kin_flux(Lw, Rw, IMU));
+		}
+		if (V != VAR_TV) o[IA1] += rxd * X.u(X.Qe, i, j, IA1) * (X.ubx[X.id(i, j)] - X.ubx[X.id(i - 1, j)]);
 		if (ny > 1) {
 			double gp[NV], gm[NV];
 			rusanov(&X.wyR[NV * X.id(i, j)], &X.wyL[NV * X.id(i, j + 1)], IMV, gp);
 			rusanov(&X.wyR[NV * X.id(i, j - 1)], &X.wyL[NV * X.id(i, j)], IMV, gm);
 			for (int k = 0; k < NV; k++) o[k] -= ryd * (gp[k] - gm[k]);
-			o[IA1] += ryd * X.u(X.Qe, i, j, IA1) * (X.uby[X.id(i, j)] - X.uby[X.id(i, j - 1)]);
+			const double *Ln = &X.wyR[NV * c], *Rn = &X.wyL[NV * X.id(i, j + 1)], *Ls = &X.wyR[NV * X.id(i, j - 1)], *Rs = &X.wyL[NV * c];
+			if (V == VAR_TV) a2 -= ryd * ((X.uby[c] - gp[IA1]) - (X.uby[X.id(i, j - 1)] - gm[IA1]));
+			else if (V == VAR_TVSPLIT || V == VAR_TVBB) {
+				const double d1 = eps_diss(Ln, Rn, IMV, IE1, X.uby[c]) - eps_diss(Ls, Rs, IMV, IE1, X.uby[X.id(i, j - 1)]);
+				const double d2 = eps_diss(Ln, Rn, IMV, IE2, X.uby[c]) - eps_diss(Ls, Rs, IMV, IE2, X.uby[X.id(i, j - 1)]);
+				X.eTV1[c] -= ryd * d1; X.eTV2[c] -= ryd * d2;
+				X.ETV[c] -= ryd * (kin_flux(Ln, Rn, IMV) - kin_flux(Ls, Rs, IMV) + d1 + d2);
+			} else if (V != VAR_PC) X.ETV[c] -= ryd * (kin_flux(Ln, Rn, IMV) - kin_flux(Ls, Rs, IMV));
+			if (V != VAR_TV) o[IA1] += ryd * X.u(X.Qe, i, j, IA1) * (X.uby[X.id(i, j)] - X.uby[X.id(i, j - 1)]);
 		}
-		X.a2s[X.id(i, j)] = 1.0 - o[IA1];
+		X.a2s[X.id(i, j)] = V == VAR_TV ? a2 : 1.0 - o[IA1];
 	}
 	bc_apply(X, X.Ust);
 	bound_cond_array(X, X.a2s);
@@ -843,7 +920,8 @@ static inline void face_y(const Ctx& X, int i, int j, double PL, double PR, Face
 	face_weights(f, X.uby[c]);
 }
 // donor component k of the reference's 6-vector (alpha1, 1 - alpha1, m1, m2, e1, e2) read from an 8-variable face state
-static inline double dk(const double* w, int k) { return k == 2 ? w[IM1] : k == 3 ? w[IM2] : k == 4 ? w[IE1] : w[IE2]; }
+static inline double dk(const double* w, int k) {
+	return k == 0 ? w[IA1] : k == 1 ? 1.0 - w[IA1] : k == 2 ? w[IM1] : k == 3 ? w[IM2] : k == 4 ? w[IE1] : w[IE2]; }
 static inline double inc(const FaceVals& f, int k) { return dk(f.cL, k) * f.wL + dk(f.cR, k) * f.wR; }   // PC:617
 static inline FaceFlux face_flux(const Ctx& X, const FaceVals& f) {                                 // PC:620-625
 	const double ML = (f.cL[IM1] + f.cL[IM2]) * f.wL, MR = (f.cR[IM1] + f.cR[IM2]) * f.wR, Mm = ML + MR;
@@ -859,23 +937,47 @@ static bool cell_residual(const Ctx& X, int i, int j, double Pc, double PW, doub
 	const bool twoD = X.twoD;
 	if (twoD) { face_y(X, i, j, Pc, PN, fn); face_y(X, i, j - 1, PS, Pc, fs); }
 	const size_t c = X.id(i, j); const double* xs = &X.Ust[NV * c];
-	double Xv[6] = {xs[IA1], X.a2s[c], xs[IM1], xs[IM2], xs[IE1], xs[IE2]};
-	for (int k = 2; k < 6; k++) {
+	const int V = X.C.variant;
+	// the TV-type variants: phase energies from the explicit step without transport, enthalpy flux (sum eps_k + p_f) u_f
+	// with upwind donors (TVpure:655-675); TVinc / TVsplit add the mass increments and their kinetic energy (TVinc:662-687)
+	const bool tvx = V >= VAR_TVPURE, tvinc = V == VAR_TVINC || V == VAR_TVSPLIT;
+	auto up = [](const FaceVals& f, int k) { return f.uf > 0.0 ? dk(f.cL, k) : dk(f.cR, k); };
+	auto hflux = [&](const FaceVals& f) { return (up(f, 4) + up(f, 5) + X.P_REF + f.pf) * f.uf; };
+	auto kinc = [](const FaceVals& f, const FaceFlux& F) {
+		return 0.5 * f.uf * f.uf * F.M + 0.5 * (f.utL * f.utL * (f.cL[IM1] + f.cL[IM2]) * f.wL + f.utR * f.utR * (f.cR[IM1] + f.cR[IM2]) * f.wR); };
+	double Xv[6] = {xs[IA1], X.a2s[c], xs[IM1], xs[IM2], tvx ? X.eTV1[c] : xs[IE1], tvx ? X.eTV2[c] : xs[IE2]};
+	if (tvx)
+		for (int k = 4; k < 6; k++) {
+			Xv[k] -= X.rx * (up(fe, k) * fe.uf - up(fw, k) * fw.uf);
+			if (twoD) Xv[k] -= X.ry * (up(fn, k) * fn.uf - up(fs, k) * fs.uf);
+		}
+	const int k0 = V == VAR_TV ? 0 : 2, k1 = !tvx ? 6 : tvinc ? 4 : 2;
+	for (int k = k0; k < k1; k++) {
 		Xv[k] -= X.rx * (inc(fe, k) - inc(fw, k));
 		if (twoD) Xv[k] -= X.ry * (inc(fn, k) - inc(fs, k));
 	}
-	Xv[1] = 1.0 - Xv[0];
-	const double s = X.rx * (fe.uf - fw.uf) + (twoD ? X.ry * (fn.uf - fs.uf) : 0.0);
-	Xv[0] *= 1.0 - s; Xv[1] *= 1.0 - s;
+	double s;
+	if (V == VAR_TV) s = 1.0 - Xv[0] - Xv[1];                                                        // TV:646
+	else {
+		Xv[1] = 1.0 - Xv[0];
+		s = X.rx * (fe.uf - fw.uf) + (twoD ? X.ry * (fn.uf - fs.uf) : 0.0);
+		Xv[0] *= 1.0 - s; Xv[1] *= 1.0 - s;
+	}
 	const FaceFlux FEe = face_flux(X, fe), FEw = face_flux(X, fw);
-	double mx = xs[IMU] - X.rx * (fe.pf - fw.pf) - X.rx * (fe.uf * FEe.M - fw.uf * FEw.M);
-	double my = xs[IMV] - X.rx * (FEe.Mt - FEw.Mt);
-	double E = xs[IEN] - X.rx * (FEe.E - FEw.E);
+	const bool mom_inc = !tvx || tvinc;   // TVpure / TVbb / BB: mass and momentum transported in the explicit step only
+	double mx = mom_inc ? xs[IMU] - X.rx * (fe.pf - fw.pf) - X.rx * (fe.uf * FEe.M - fw.uf * FEw.M) : xs[IMU] - X.rx * (fe.pf - fw.pf);
+	double my = mom_inc ? xs[IMV] - X.rx * (FEe.Mt - FEw.Mt) : xs[IMV];
+	double E = !tvx ? xs[IEN] - X.rx * (FEe.E - FEw.E)
+	         : tvinc ? X.ETV[c] - X.rx * (hflux(fe) + kinc(fe, FEe) - hflux(fw) - kinc(fw, FEw)) : X.ETV[c] - X.rx * (hflux(fe) - hflux(fw));
 	if (twoD) {
 		const FaceFlux FEn = face_flux(X, fn), FEs = face_flux(X, fs);
-		my -= X.ry * (fn.pf - fs.pf) + X.ry * (fn.uf * FEn.M - fs.uf * FEs.M);
-		mx -= X.ry * (FEn.Mt - FEs.Mt);
-		E -= X.ry * (FEn.E - FEs.E);
+		if (mom_inc) {
+			my -= X.ry * (fn.pf - fs.pf) + X.ry * (fn.uf * FEn.M - fs.uf * FEs.M);
+			mx -= X.ry * (FEn.Mt - FEs.Mt);
+		} else my -= X.ry * (fn.pf - fs.pf);
+		if (!tvx) E -= X.ry * (FEn.E - FEs.E);
+		else if (tvinc) E -= X.ry * (hflux(fn) + kinc(fn, FEn) - hflux(fs) - kinc(fs, FEs));
+		else E -= X.ry * (hflux(fn) - hflux(fs));
 	}
 	const double rho = Xv[2] + Xv[3];
 	if (!(rho > 0)) return false;

@@ -23,7 +23,7 @@ def oracle_cfg_from(scfg, O, **over):
     c = O.default_cfg(scfg.nx, scfg.ny, bc=list(scfg.bc), bcu=list(scfg.bcu), ph=ph)
     c.x0, c.x1, c.y0, c.y1 = scfg.x0, scfg.x1, scfg.y0, scfg.y1
     for k in ("rk", "cfl", "cfl_ac", "ramp", "dtfix", "acdt", "newton_tol", "newton_iter", "krylov_tol", "krylov_iter",
-              "limiter", "order", "rus_local", "jreuse"):
+              "limiter", "order", "rus_local", "jreuse", "variant"):
         setattr(c, k, getattr(scfg, k))
     for k, v in over.items():
         setattr(c, k, v)
