@@ -36,21 +36,23 @@ def eps_amp(g, pi, a, m, p, b=0.0, q=0.0):
 
 
 def prim2cell(ph, a1, r1, r2, u, v, p1, p2):
-    """conserved state from primitives (PHY:849-854); arrays broadcast."""
+    """conserved state from primitives (PHY:849-854); arrays broadcast. ph[k] = (gamma, p_inf[, q, q', cv, b])."""
     a1 = np.asarray(a1, dtype=np.float64)
     a2 = 1 - a1
     m1 = a1 * r1
     m2 = a2 * r2
     rho = m1 + m2
     (g0, pi0), (g1, pi1) = ph[0][:2], ph[1][:2]
+    q0, q1 = (ph[0][2] if len(ph[0]) > 2 else 0.0), (ph[1][2] if len(ph[1]) > 2 else 0.0)
+    b0, b1 = (ph[0][5] if len(ph[0]) > 5 else 0.0), (ph[1][5] if len(ph[1]) > 5 else 0.0)
     U = np.empty(np.broadcast(a1, r1, r2, u, v, p1, p2).shape + (8,))
     U[..., 0] = a1
     U[..., 1] = m1
     U[..., 2] = m2
     U[..., 3] = rho * u
     U[..., 4] = rho * v
-    U[..., 5] = eps_amp(g0, pi0, a1, m1, p1)
-    U[..., 6] = eps_amp(g1, pi1, a2, m2, p2)
+    U[..., 5] = eps_amp(g0, pi0, a1, m1, p1, b0, q0)
+    U[..., 6] = eps_amp(g1, pi1, a2, m2, p2, b1, q1)
     U[..., 7] = U[..., 5] + U[..., 6] + 0.5 * rho * (u * u + v * v)
     return U
 

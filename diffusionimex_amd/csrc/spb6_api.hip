@@ -115,6 +115,8 @@ struct spb6_ctx {
 	// alpha_1 step, TV's alpha_1 face donors
 	double *fwxL = nullptr, *fwxR = nullptr, *fwyL = nullptr, *fwyR = nullptr;
 	double *eTV1 = nullptr, *eTV2 = nullptr, *ETV = nullptr, *dA = nullptr, *donax = nullptr, *donay = nullptr;
+	int* ctype = nullptr;   // relaxation level 1: the closure type of each interior cell for the step (STEP_TYPE)
+	double eps_i = 1e-4;    // EPS_I (PHY:38)
 };
 
 struct spb6_mlp {
@@ -149,6 +151,7 @@ Dev make_dev(const spb6_ctx* c) {
 	D.variant = c->cfg.variant;
 	D.eTV1 = c->eTV1; D.eTV2 = c->eTV2; D.ETV = c->ETV; D.dA = c->dA; D.donax = c->donax; D.donay = c->donay;
 	if (D.variant != VAR_PC) { D.wxL = c->fwxL; D.wxR = c->fwxR; D.wyL = c->fwyL; D.wyR = c->fwyR; }
+	D.ctype = c->ctype;
 	return D;
 }
 
@@ -568,6 +571,7 @@ void set_tab(spb6_ctx* c, int rk) {   // PC:1195-1218: 1 backward Euler, 2 LSDIR
 // one attempt (PC:1243-1285): on success Ust / Pst hold the new state and pressure
 int attempt(spb6_ctx* c, double dt, int rk) {
 	set_tab(c, rk);
+	if (c->ctype) launch_relax_activate(c->g, c->ph, c->U, c->eps_i, c->ctype, c->st);   // PC:1247
 	const size_t n = c->g.n;
 	HIPCHK(hipMemcpyAsync(c->Pst, c->P, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 	c->final_pending = false;
@@ -706,6 +710,9 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (cfg->variant < SPB6_VARIANT_PC || cfg->variant > SPB6_VARIANT_BB) { g_err = "variant out of range"; return nullptr; }
 	if (cfg->variant != SPB6_VARIANT_PC && (cfg->analytic_jacobian || cfg->learned_guess)) {
 		g_err = "the IMEX variants run with the finite-difference Jacobian and the plain Newton guess"; return nullptr; }
+	if (cfg->relax < 0 || cfg->relax > 1) { g_err = "relaxation level must be 0 (p) or 1 (p-pT)"; return nullptr; }
+	if (cfg->relax > 0 && (cfg->analytic_jacobian || cfg->learned_guess)) {
+		g_err = "relaxation level 1 runs with the finite-difference Jacobian and the plain Newton guess"; return nullptr; }
 	spb6_ctx* c = new spb6_ctx();
 	c->cfg = *cfg; c->device = device;
 	std::memset(&c->stats, 0, sizeof c->stats);
@@ -731,6 +738,12 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 		double** va[] = {&c->eTV1, &c->eTV2, &c->ETV, &c->dA};
 		for (auto p : va) if (dalloc(c, p, NC)) return fail("alloc");
 		if (dalloc(c, &c->donax, 2 * NC) || dalloc(c, &c->donay, 2 * NC)) return fail("alloc");
+	}
+	if (cfg->relax > 0) {
+		if (hipMalloc((void**)&c->ctype, n * sizeof(int)) != hipSuccess) return fail("alloc");
+		c->allocs.push_back(c->ctype);
+		if (hipMemset(c->ctype, 0, n * sizeof(int)) != hipSuccess) return fail("alloc");
+		if (cfg->eps_i > 0) c->eps_i = cfg->eps_i;
 	}
 	double** arrs[] = {&c->phx, &c->phy, &c->a2s, &c->ubx, &c->uby, &c->vsc, &c->esc, &c->vu, &c->vv, &c->ust, &c->vst, &c->cz, &c->Kw, &c->Pold,
 	                   &c->ubsx, &c->kpx, &c->wLx, &c->pvx, &c->ubsy, &c->kpy, &c->wLy, &c->pvy};
@@ -897,6 +910,7 @@ int spb6_step(spb6_ctx* c, double dt, int rk) {
 int spb6_attempt_begin(spb6_ctx* c) {
 	if (!c) return SPB6_BAD_ARG;
 	hipSetDevice(c->device);
+	if (c->ctype) launch_relax_activate(c->g, c->ph, c->U, c->eps_i, c->ctype, c->st);   // PC:1247
 	HIPCHK(hipMemcpyAsync(c->Pst, c->P, (size_t)c->g.n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 	c->final_pending = false;
 	c->qe_cur = c->qs_cur = c->U;
@@ -977,7 +991,7 @@ int spb6_run(spb6_ctx* c, double t_end, long max_steps) {
 }
 
 int spb6_hllc_step(spb6_ctx* c, double dt) {
-	if (!c || !(dt > 0)) return SPB6_BAD_ARG;
+	if (!c || !(dt > 0) || c->ctype) return SPB6_BAD_ARG;   // relaxation level 0 only
 	hipSetDevice(c->device);
 	int rc = hllc_alloc(c); if (rc) return rc;
 	rc = hllc_attempt(c, dt);
@@ -985,7 +999,7 @@ int spb6_hllc_step(spb6_ctx* c, double dt) {
 	return rc;
 }
 int spb6_hllc_run(spb6_ctx* c, double t_end, long max_steps, double cfl) {
-	if (!c || !(cfl > 0)) return SPB6_BAD_ARG;
+	if (!c || !(cfl > 0) || c->ctype) return SPB6_BAD_ARG;   // relaxation level 0 only
 	hipSetDevice(c->device);
 	int rc = hllc_alloc(c); if (rc) return rc;
 	long done_steps = 0;
@@ -1006,6 +1020,15 @@ int spb6_hllc_run(spb6_ctx* c, double t_end, long max_steps, double cfl) {
 		c->t = c->t + dt; c->stats.steps++; c->stats.dt_last = dt; done_steps++;
 	}
 	c->stats.t = c->t;
+	return SPB6_OK;
+}
+
+int spb6_relax_activate(spb6_ctx* c) {
+	if (!c) return SPB6_BAD_ARG;
+	hipSetDevice(c->device);
+	if (!c->ctype) return SPB6_OK;
+	launch_relax_activate(c->g, c->ph, c->U, c->eps_i, c->ctype, c->st);
+	HIPCHK(hipStreamSynchronize(c->st));
 	return SPB6_OK;
 }
 

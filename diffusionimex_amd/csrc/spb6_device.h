@@ -688,4 +688,93 @@ SPB_HD double closure_roundoff(const Phys& P, const ClIn& in, double p) {
 	return dphi < 0 ? en * sens / -dphi : 0.0;
 }
 
+// ------------------------------------------------------------------ relaxation level 1: the pT closure (PHY:183-230, PHY:660-672)
+// closure types (PHY:401): mechanical (p) and thermal (pT) equilibrium
+enum { CL_P = 0, CL_PT = 2 };
+// Brent's method on a sign-changing bracket (PHY:183-204), operation for operation
+template <class F>
+SPB_HD bool brent_root(F f, double a, double b, double fa, double fb, double tol, double& root) {
+	if (!(fa * fb <= 0)) return false;
+	if (fa == 0) { root = a; return true; }
+	if (fb == 0) { root = b; return true; }
+	double c = a, fc = fa, d = b - a, e = d;
+	for (int it = 0; it < 200; it++) {
+		if (fb * fc > 0) { c = a; fc = fa; d = b - a; e = d; }
+		if (fabs(fc) < fabs(fb)) { a = b; b = c; c = a; fa = fb; fb = fc; fc = fa; }
+		const double tol1 = 2.0 * 1e-16 * fabs(b) + 0.5 * tol, xm = 0.5 * (c - b);
+		if (fabs(xm) <= tol1 || fb == 0) { root = b; return true; }
+		if (fabs(e) >= tol1 && fabs(fa) > fabs(fb)) {
+			const double sv = fb / fa;
+			double pv, qv;
+			if (a == c) { pv = 2 * xm * sv; qv = 1 - sv; }
+			else { const double qq = fa / fc, r = fb / fc; pv = sv * (2 * xm * qq * (qq - r) - (b - a) * (r - 1)); qv = (qq - 1) * (r - 1) * (sv - 1); }
+			if (pv > 0) qv = -qv;
+			pv = fabs(pv);
+			if (2 * pv < smin(3 * xm * qv - fabs(tol1 * qv), fabs(e * qv))) { e = d; d = pv / qv; } else { d = xm; e = d; }
+		} else { d = xm; e = d; }
+		a = b; fa = fb;
+		b += (fabs(d) > tol1) ? d : (xm > 0 ? tol1 : -tol1);
+		fb = f(b); if (!isfin(fb)) return false;
+	}
+	root = b; return true;
+}
+// a root of f above plo: bracket search on y = ln(p - plo) from the guess, then Brent (PHY:205-225); every loop bounded
+template <class F>
+SPB_HD bool root_above(F f, double plo, double guess, double& root) {
+	auto Fy = [&](double y) { return f(plo + pexp(y)); };
+	const double y0 = plog(smax(guess - plo, 1e-6 * (fabs(plo) + 1.0) + 1e-12));
+	const double f0 = Fy(y0);
+	if (f0 == 0.0) { root = plo + pexp(y0); return true; }
+	double ya = y0, yb = y0, fa = f0, fb = f0, st = 0.1;
+	if (!isfin(f0)) {
+		for (int k = 0; k < 200 && !isfin(fb); k++) { yb += 1.0; fb = Fy(yb); }
+		if (!isfin(fb)) return false;
+		ya = yb; fa = fb;
+	}
+	if (fa > 0) { for (int k = 0; k < 400 && !(fb < 0); k++) { ya = yb; fa = fb; yb += st; st *= 1.6; fb = Fy(yb); if (!isfin(fb)) return false; } }
+	else {
+		for (int k = 0; k < 400 && !(fa > 0); k++) {
+			yb = ya; fb = fa; ya -= st; st *= 1.6; fa = Fy(ya);
+			if (!isfin(fa)) {
+				ya = yb; fa = fb;
+				for (int q = 0; q < 60; q++) {
+					const double ym = ya - st / ldexp(1.0, q + 1), fm = Fy(ym);
+					if (isfin(fm)) { if (fm > 0) { ya = ym; fa = fm; break; } }
+				}
+				break;
+			}
+		}
+	}
+	if (fa == 0.0) { root = plo + pexp(ya); return true; }
+	if (fb == 0.0) { root = plo + pexp(yb); return true; }
+	if (!(fa > 0 && fb < 0)) return false;
+	double y; if (!brent_root(Fy, ya, yb, fa, fb, 1e-15, y)) return false;
+	root = plo + pexp(y); return true;
+}
+SPB_HD double v_pT(const Phys& P, int k, double p, double T) { return (P.g[k] - 1.0) * P.cv[k] * T / (p + P.pi[k]) + P.b[k]; }   // PHY:78
+SPB_HD double e_pT(const Phys& P, int k, double p, double T) { return P.cv[k] * T * (p + P.g[k] * P.pi[k]) / (p + P.pi[k]) + P.q[k]; }   // PHY:79
+// the pT closure (B6.3): mechanical and thermal equilibrium at the transported masses, T(p) from the energy, the root of
+// sum m_k v_k(p, T(p)) = 1 above max(-pi_k); T_out: the common temperature. closure()'s input checks (PHY:787-790) first.
+SPB_HD bool closure_pT(const Phys& P, const ClIn& in, ClOut& o, double* T_out = nullptr) {
+	if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0)) return false;
+	if (!(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
+	if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
+	const double m0 = in.mtr0, m1 = in.mtr1;
+	const double Eth = in.rhoeC - m0 * P.q[0] - m1 * P.q[1];
+	if (!(Eth > 0)) return false;
+	auto Tof = [&](double p) {
+		double d = 0;
+		d += m0 * P.cv[0] * (p + P.g[0] * P.pi[0]) / (p + P.pi[0]);
+		d += m1 * P.cv[1] * (p + P.g[1] * P.pi[1]) / (p + P.pi[1]);
+		return Eth / d;
+	};
+	auto Phi = [&](double p) { const double T = Tof(p); return m0 * v_pT(P, 0, p, T) + m1 * v_pT(P, 1, p, T) - 1.0; };
+	double p; if (!root_above(Phi, smax(-P.pi[0], -P.pi[1]), in.pguess, p)) return false;
+	const double T = Tof(p);
+	o.a0 = m0 * v_pT(P, 0, p, T); o.a1 = m1 * v_pT(P, 1, p, T); o.e0 = m0 * e_pT(P, 0, p, T); o.e1 = m1 * e_pT(P, 1, p, T);
+	o.pbar = p;
+	if (T_out) *T_out = T;
+	return T > 0 && isfin(o.pbar);
+}
+
 }   // namespace spb

@@ -38,9 +38,12 @@ struct Dev {
 	// alpha_1 face donors per direction as (left, right) pairs; null for the reference's PC scheme
 	int variant;
 	double *eTV1, *eTV2, *ETV, *dA, *donax, *donay;
+	const int* ctype;   // relaxation level 1: each interior row's closure type of the step (CL_P / CL_PT); null at level 0
 };
 // the IMEX variants of the reference (its sibling programs SPB6_<name>_IMEX.cpp); VAR_PC is SPB6_PC_IMEX.cpp
 enum { VAR_PC = 0, VAR_TV = 1, VAR_TVPURE = 2, VAR_TVINC = 3, VAR_TVSPLIT = 4, VAR_TVBB = 5, VAR_BB = 6 };
+// the implicit stage takes the general cell assembly (cell_assemble_var) for an IMEX variant or relaxation level 1
+inline bool general_path(const Dev& D) { return D.variant != VAR_PC || D.ctype != nullptr; }
 
 template <typename T> struct KrylovBufsT {
 	T *x, *r, *r0, *p, *v, *s, *t;
@@ -108,15 +111,19 @@ void launch_krylov_setup32_cvt(const Geo& g, const KrylovBufs32& B, const double
 void launch_krylov_iter32(const Geo& g, const KrylovBufs32& B, hipStream_t st, int it, double tol, int maxit);
 
 // ---- the IMEX variants (spb6_variants.hip): the explicit stage from the full face states wxL.. (launch_recon_full),
-// then the variant's update; the implicit-stage kernels of a Dev with variant != VAR_PC (residual, finite-difference
-// Jacobian, floor test, final state) take the variant's cell residual. launch_explicit, launch_residual,
-// launch_jacobian, launch_floor and launch_final route a variant's Dev here.
+// then the variant's update; the implicit-stage kernels of a Dev on the general path (an IMEX variant or relaxation
+// level 1: residual, finite-difference Jacobian, floor test, final state) take the general cell residual.
+// launch_explicit routes a variant's Dev here, and launch_residual, launch_jacobian, launch_floor and launch_final a
+// Dev on the general path.
 void launch_explicit_var(const Dev& D, int limiter, int order, const double* Qe, const double* Qs, double adt, hipStream_t st);
 void launch_residual_var(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot,
                          hipStream_t st, const int* gate);
 void launch_jacobian_var(const Dev& D, const double* P, const double* G, const double* sc, double* J, double* rhs, hipStream_t st);
 void launch_floor_var(const Dev& D, const double* P, double tol, hipStream_t st);
 void launch_final_var(const Dev& D, const double* P, const double* Qs, double adt, double* H, unsigned long long* tmax, hipStream_t st, int flag_slot);
+// relaxation level 1 (RelaxationActivation, PC:537-560): each interior cell's closure type into ctype, the cells of
+// type pT projected onto the pT equilibrium in U (a failed projection keeps the cell mechanical), then U's ghosts
+void launch_relax_activate(const Geo& g, const Phys& P, double* U, double eps_i, int* ctype, hipStream_t st);
 
 // ---- the explicit HLLC scheme (spb6_hllc.hip): cell primitives (hllc_prim_fields arrays over the ghosted grid), face
 // values (hllc_face_fields arrays per direction, at the face's low cell); every pass returns at once once flags

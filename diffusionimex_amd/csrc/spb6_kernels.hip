@@ -1245,7 +1245,7 @@ void launch_shock(const Dev& D, int* list, int* count, double* rec, unsigned lon
 }
 void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st,
                      const int* gate) {
-	if (D.variant != VAR_PC) { launch_residual_var(D, P, dP, lam, Pout, G, nrm, flag_slot, st, gate); return; }
+	if (general_path(D)) { launch_residual_var(D, P, dP, lam, Pout, G, nrm, flag_slot, st, gate); return; }
 	hipLaunchKernelGGL(k_residual, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, dP, lam, Pout, G, nrm, D.flags, flag_slot, gate);
 }
 void launch_residual_list(const Dev& D, const double* P, const int* list, const int* count, double* G, unsigned long long* nrm, int flag_slot,
@@ -1259,7 +1259,7 @@ void launch_spec_commit(long n, const float* x, double* dP, const KS* ks, const 
 void launch_jacobian(const Dev& D, const double* P, const double* G, double* sc, double* J, double* rhs, bool analytic, hipStream_t st, int* clear,
                      int nclear) {
 	hipLaunchKernelGGL(k_fdscale, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, sc, clear, nclear);
-	if (D.variant != VAR_PC) launch_jacobian_var(D, P, G, sc, J, rhs, st);
+	if (general_path(D)) launch_jacobian_var(D, P, G, sc, J, rhs, st);
 	else if (analytic) hipLaunchKernelGGL(k_jacobian_an, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, J, rhs, D.flags);
 	else hipLaunchKernelGGL(k_jacobian, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, G, (const double*)sc, J, rhs, D.flags);
 }
@@ -1270,11 +1270,11 @@ void launch_stepcrit(int n, const double* dP, const double* P, const double* sc,
 	hipLaunchKernelGGL(k_stepcrit, dim3(nblk(n)), dim3(256), 0, st, n, dP, P, sc, nrm);
 }
 void launch_floor(const Dev& D, const double* P, double tol, hipStream_t st) {
-	if (D.variant != VAR_PC) { launch_floor_var(D, P, tol, st); return; }
+	if (general_path(D)) { launch_floor_var(D, P, tol, st); return; }
 	hipLaunchKernelGGL(k_floor, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, tol, D.flags);
 }
 void launch_final(const Dev& D, const double* P, const double* Qs, double adt, double* H, unsigned long long* tmax, hipStream_t st, int flag_slot) {
-	if (D.variant != VAR_PC) { launch_final_var(D, P, Qs, adt, H, tmax, st, flag_slot); return; }
+	if (general_path(D)) { launch_final_var(D, P, Qs, adt, H, tmax, st, flag_slot); return; }
 	hipLaunchKernelGGL(k_final, dim3(nblk(D.g.n)), dim3(256), 0, st, D, P, Qs, adt, H, D.flags, flag_slot, tmax);
 }
 void launch_cell_diag(const Geo& g, const Phys& P, const double* U, double* out, hipStream_t st) {

@@ -36,7 +36,7 @@ class Config(C.Structure):
         ("krylov_tol", C.c_double), ("krylov_iter", C.c_int),
         ("limiter", C.c_int), ("order", C.c_int), ("rus_local", C.c_int),
         ("krylov_fp32", C.c_int), ("learned_guess", C.c_int), ("analytic_jacobian", C.c_int),
-        ("jreuse", C.c_int), ("variant", C.c_int),
+        ("jreuse", C.c_int), ("variant", C.c_int), ("relax", C.c_int), ("eps_i", C.c_double),
     ]
 
 
@@ -62,7 +62,7 @@ class KernelTimes(C.Structure):
 EXPORTS = ["spb6_create", "spb6_destroy", "spb6_last_error", "spb6_set_state", "spb6_get_state", "spb6_get_pressure",
            "spb6_set_pressure", "spb6_caldt", "spb6_bc_fill", "spb6_explicit_stage", "spb6_implicit_stage",
            "spb6_stage_prep", "spb6_step", "spb6_attempt_begin", "spb6_stage_combine", "spb6_stage_update",
-           "spb6_stage_pressure", "spb6_attempt_accept", "spb6_run", "spb6_hllc_step", "spb6_hllc_run",
+           "spb6_stage_pressure", "spb6_attempt_accept", "spb6_run", "spb6_relax_activate", "spb6_hllc_step", "spb6_hllc_run",
            "spb6_get_profile", "spb6_write_profile",
            "spb6_conserved_totals", "spb6_stats_line", "spb6_get_stats", "spb6_time", "spb6_residual",
            "spb6_jacobian", "spb6_linsolve", "spb6_linsolve_device", "spb6_get_field", "spb6_enable_timing", "spb6_set_timing_mask", "spb6_get_kernel_times",
@@ -110,6 +110,7 @@ def load():
     L.spb6_stats_line.argtypes = [vp, C.c_int, C.c_double, C.c_char_p, C.c_long]
     L.spb6_run.argtypes = [vp, C.c_double, C.c_long]
     L.spb6_hllc_step.argtypes = [vp, C.c_double]
+    L.spb6_relax_activate.argtypes = [vp]
     L.spb6_hllc_run.argtypes = [vp, C.c_double, C.c_long, C.c_double]
     L.spb6_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.spb6_time.restype = C.c_double
@@ -164,7 +165,7 @@ def _dp(a):
 def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0, 1.0), bc=(0, 0, 0, 0), bcu=(0, 0, 0, 0),
                 rk=2, cfl=0.2, cfl_ac=0.4, ramp=1, dtfix=0.0, acdt=0, newton_tol=1e-13, newton_iter=60,
                 krylov_tol=1e-13, krylov_iter=4000, limiter=1, order=2, rus_local=1, krylov_fp32=0, learned_guess=0,
-                analytic_jacobian=0, jreuse=0, variant=0):
+                analytic_jacobian=0, jreuse=0, variant=0, relax=0, eps_i=0.0):
     """Config with the reference defaults (PC:57-80): phases are (gamma, p_inf[, q, qp, cv, b])."""
     c = Config()
     c.nx, c.ny = nx, ny
@@ -183,6 +184,7 @@ def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0
     c.analytic_jacobian = analytic_jacobian
     c.jreuse = jreuse
     c.variant = variant
+    c.relax, c.eps_i = relax, eps_i
     return c
 
 
@@ -267,6 +269,10 @@ class Stepper:
 
     def run(self, t_end=1e300, max_steps=-1):
         return self.L.spb6_run(self.h, float(t_end), int(max_steps))
+
+    def relax_activate(self):
+        """RelaxationActivation on the state (PC:537-560); the reference applies it before its output (PC:1300)."""
+        return self.L.spb6_relax_activate(self.h)
 
     def hllc_step(self, dt):
         """One step of the reference's explicit HLLC scheme (src/SPB6_explicit.cpp) of size dt."""

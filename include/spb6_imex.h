@@ -90,6 +90,11 @@ typedef struct spb6_config {
                                         batch rejects it) */
     int variant;                     /* spb6_variant: which of the reference's IMEX programs the step follows (single-grid
                                         stepper, finite-difference Jacobian and plain guess only; the batch rejects it) */
+    int relax;                       /* RELAX_LEVEL (PHY:37): 0 p (mechanical equilibrium), 1 p-pT (thermal equilibrium too
+                                        in the cells with eps_i <= alpha_1 <= 1 - eps_i: the pT closure, PHY:660-672,
+                                        and the step's RelaxationActivation, PC:537-560). Finite-difference Jacobian and
+                                        plain guess only; the batch and the explicit HLLC scheme reject level 1 */
+    double eps_i;                    /* EPS_I (PHY:38); <= 0: the reference's 1e-4 */
 } spb6_config_t;
 
 /* run statistics: the counters of the reference's stats line (PC:1337-1339) plus Krylov counts */
@@ -173,6 +178,10 @@ int spb6_attempt_accept(spb6_ctx* ctx, double dt);
 /* The time loop (PC:1230-1296): dt from caldt with the start-up ramp, reject-and-halve, until t_end or max_steps
  * accepted steps (max_steps < 0: no cap). */
 int spb6_run(spb6_ctx* ctx, double t_end, long max_steps);
+/* RelaxationActivation (PC:537-560) on the state: at relaxation level 1 each attempt starts with it; the reference's
+ * main applies it once more before its output (PC:1300), which a caller mirrors by calling this before
+ * spb6_get_state / spb6_write_profile. No-op at level 0. */
+int spb6_relax_activate(spb6_ctx* ctx);
 /* ---- the reference's fully explicit scheme, src/SPB6_explicit.cpp ("EX"), at relaxation level 0, on the same context
  * and state (an independent cross-check of the IMEX path on shock problems; it leaves the pressure unknown alone):
  * primitive-variable minmod reconstruction (EX:228-256; cfg.order, cfg.limiter), HLLC fluxes with the phase Hugoniot

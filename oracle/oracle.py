@@ -28,6 +28,7 @@ class Cfg(C.Structure):
         ("limiter", C.c_int), ("order", C.c_int), ("rus_local", C.c_int),
         ("closure_path", C.c_int), ("share_start", C.c_int), ("dot_order", C.c_int),
         ("portable_math", C.c_int), ("nthreads", C.c_int), ("jreuse", C.c_int), ("variant", C.c_int),
+        ("relax", C.c_int), ("eps_i", C.c_double),
     ]
 
 
@@ -82,6 +83,7 @@ def lib():
         L.ora_hllc_run.argtypes = [vp, C.c_double, C.c_long, C.c_double]
         L.ora_hllc_closure_fails.restype = C.c_long
         L.ora_hllc_step.argtypes = [vp, C.c_double]
+        L.ora_relax_activate.argtypes = [vp]
         L.ora_residual_all.argtypes = [vp, dp, dp]
         L.ora_jacobian.argtypes = [vp, dp, dp, dp, dp]
         L.ora_linsolve.argtypes = [vp, dp, dp, dp, C.c_int, ip]
@@ -133,6 +135,7 @@ def default_cfg(nx, ny=1, **kw):
     c.nthreads = 0
     c.jreuse = 0
     c.variant = 0
+    c.relax, c.eps_i = 0, 1e-4
     for k, v in kw.items():
         if k == "bc":
             for q in range(4):

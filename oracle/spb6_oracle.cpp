@@ -314,7 +314,8 @@ template <class F> static void shock_budget(const Model& M, F cellptr, bool twoD
 }
 
 // ------------------------------------------------------------------------------------------------ p closure (PHY:420-494, PHO:472-548)
-struct ClIn { double atr[2], mtr[2], etr[2], s, rhoeC, pguess, Qph[2]; };
+enum { CL_P = 0, CL_PT = 2 };   // closure types of PHY:401 used here: mechanical (p) and thermal (pT) equilibrium
+struct ClIn { double atr[2], mtr[2], etr[2], s, rhoeC, pguess, Qph[2]; int type = CL_P; };
 struct ClOut { double a[2], m[2], e[2], p[2], T[2], pbar; };
 struct PCoef { double A[2], B[2], N[2], lam[2]; };
 static double E_above(const Model& M, const ClIn& in, int k) {                                     // PHY:420-423 (SG)
@@ -423,13 +424,86 @@ static bool closure_p_pho(const Model& M, const ClIn& in, ClOut& o) {           
 	if (!conv) return false;
 	return closure_out(M, in, p, o);
 }
-static bool closure(const Model& M, const ClIn& in, ClOut& o) {                                    // PHY:786-810 (CL_P)
+// ---- scalar root finding (PHY:183-230): Brent's method on a sign-changing bracket, and the bracket search above a
+// lower bound on y = ln(p - plo)
+template <class F> static bool brent(F f, double a, double b, double fa, double fb, double tol, double& root) {   // PHY:183-204
+	if (!(fa * fb <= 0)) return false;
+	if (fa == 0) { root = a; return true; }
+	if (fb == 0) { root = b; return true; }
+	double c = a, fc = fa, d = b - a, e = d;
+	for (int it = 0; it < 200; it++) {
+		if (fb * fc > 0) { c = a; fc = fa; d = b - a; e = d; }
+		if (std::fabs(fc) < std::fabs(fb)) { a = b; b = c; c = a; fa = fb; fb = fc; fc = fa; }
+		const double tol1 = 2.0 * 1e-16 * std::fabs(b) + 0.5 * tol, xm = 0.5 * (c - b);
+		if (std::fabs(xm) <= tol1 || fb == 0) { root = b; return true; }
+		if (std::fabs(e) >= tol1 && std::fabs(fa) > std::fabs(fb)) {
+			const double sv = fb / fa;
+			double pv, qv;
+			if (a == c) { pv = 2 * xm * sv; qv = 1 - sv; }
+			else { const double qq = fa / fc, r = fb / fc; pv = sv * (2 * xm * qq * (qq - r) - (b - a) * (r - 1)); qv = (qq - 1) * (r - 1) * (sv - 1); }
+			if (pv > 0) qv = -qv;
+			pv = std::fabs(pv);
+			if (2 * pv < smin(3 * xm * qv - std::fabs(tol1 * qv), std::fabs(e * qv))) { e = d; d = pv / qv; } else { d = xm; e = d; }
+		} else { d = xm; e = d; }
+		a = b; fa = fb;
+		b += (std::fabs(d) > tol1) ? d : (xm > 0 ? tol1 : -tol1);
+		fb = f(b); if (!std::isfinite(fb)) return false;
+	}
+	root = b; return true;
+}
+template <class F> static bool root_above(const Model& M, F f, double plo, double guess, double& root) {   // PHY:205-225
+	auto Fy = [&](double y) { return f(plo + M.exp_(y)); };
+	const double y0 = M.log_(smax(guess - plo, 1e-6 * (std::fabs(plo) + 1.0) + 1e-12));
+	const double f0 = Fy(y0);
+	if (f0 == 0.0) { root = plo + M.exp_(y0); return true; }
+	double ya = y0, yb = y0, fa = f0, fb = f0, st = 0.1;
+	if (!std::isfinite(f0)) {
+		for (int k = 0; k < 200 && !std::isfinite(fb); k++) { yb += 1.0; fb = Fy(yb); }
+		if (!std::isfinite(fb)) return false;
+		ya = yb; fa = fb;
+	}
+	if (fa > 0) { for (int k = 0; k < 400 && !(fb < 0); k++) { ya = yb; fa = fb; yb += st; st *= 1.6; fb = Fy(yb); if (!std::isfinite(fb)) return false; } }
+	else {
+		for (int k = 0; k < 400 && !(fa > 0); k++) {
+			yb = ya; fb = fa; ya -= st; st *= 1.6; fa = Fy(ya);
+			if (!std::isfinite(fa)) {
+				ya = yb; fa = fb;
+				for (int q = 0; q < 60; q++) {
+					const double ym = ya - st / std::ldexp(1.0, q + 1), fm = Fy(ym);   // 2^(q+1), as the reference's pow(2, q + 1)
+					if (std::isfinite(fm)) { if (fm > 0) { ya = ym; fa = fm; break; } }
+				}
+				break;
+			}
+		}
+	}
+	if (fa == 0.0) { root = plo + M.exp_(ya); return true; }
+	if (fb == 0.0) { root = plo + M.exp_(yb); return true; }
+	if (!(fa > 0 && fb < 0)) return false;
+	double y; if (!brent(Fy, ya, yb, fa, fb, 1e-15, y)) return false;
+	root = plo + M.exp_(y); return true;
+}
+// pT closure (B6.3, PHY:660-672): mechanical and thermal equilibrium at the phases' masses; T(p) from the energy
+static inline double v_pT(const Model& M, int k, double p, double T) { const Phase& P = M.PH[k]; return (P.g - 1.0) * P.cv * T / (p + P.pi) + P.b; }   // PHY:78
+static inline double e_pT(const Model& M, int k, double p, double T) { const Phase& P = M.PH[k]; return P.cv * T * (p + P.g * P.pi) / (p + P.pi) + P.q; }   // PHY:79
+static bool closure_pT(const Model& M, const ClIn& in, ClOut& o) {
+	const double m[2] = {in.mtr[0], in.mtr[1]};
+	const double Eth = in.rhoeC - m[0] * M.PH[0].q - m[1] * M.PH[1].q;
+	if (!(Eth > 0)) return false;
+	auto Tof = [&](double p) { double d = 0; for (int k = 0; k < 2; k++) d += m[k] * M.PH[k].cv * (p + M.PH[k].g * M.PH[k].pi) / (p + M.PH[k].pi); return Eth / d; };
+	auto Phi = [&](double p) { const double T = Tof(p); return m[0] * v_pT(M, 0, p, T) + m[1] * v_pT(M, 1, p, T) - 1.0; };
+	double p; if (!root_above(M, Phi, plo_mix(M), in.pguess, p)) return false;
+	const double T = Tof(p);
+	for (int k = 0; k < 2; k++) { o.m[k] = m[k]; o.a[k] = m[k] * v_pT(M, k, p, T); o.e[k] = m[k] * e_pT(M, k, p, T); o.p[k] = p; o.T[k] = T; }
+	o.pbar = p; return T > 0;
+}
+static bool closure(const Model& M, const ClIn& in, ClOut& o) {                                    // PHY:786-810 (CL_P, CL_PT)
 	for (int k = 0; k < 2; k++) if (!(in.atr[k] > 0) || !(in.mtr[k] > 0) || !std::isfinite(in.etr[k])) return false;
 	if (!std::isfinite(in.rhoeC) || !std::isfinite(in.s)) return false;
-	const bool ok = M.closure_path == 0 ? closure_p_phy(M, in, o) : closure_p_pho(M, in, o);
+	const bool ok = in.type == CL_PT ? closure_pT(M, in, o) : M.closure_path == 0 ? closure_p_phy(M, in, o) : closure_p_pho(M, in, o);
 	return ok && std::isfinite(o.pbar);
 }
 static double closure_roundoff(const Model& M, const ClIn& in, double p) {                         // PC:631-638
+	if (in.type != CL_P) return 0.0;
 	PCoef c; if (!p_coef(M, in, c, p)) return 0.0;
 	double dphi = 0.0, sens = 0.0;
 	for (int k = 0; k < 2; k++) { dphi += dalpha_p(M, c, k, p); sens += std::fabs(c.lam[k]) * (M.PH[k].g - 1.0) / (M.PH[k].g * (p + M.PH[k].pi)); }
@@ -488,6 +562,7 @@ struct Cfg {
 	int nthreads;
 	int jreuse;   // JREUSE (PCO:83): modified Newton, the Jacobian rebuilt only as PCO:826-828 decides
 	int variant;  // IMEX variant: 0 PC (SPB6_PC_IMEX.cpp), 1 TV, 2 TVpure, 3 TVinc, 4 TVsplit, 5 TVbb, 6 BB (SPB6_<name>_IMEX.cpp)
+	int relax; double eps_i;   // RELAX_LEVEL (PHY:37; 0 p, 1 p-pT) and EPS_I (PHY:38)
 };
 struct Stats { long steps, rk1, rejects, floor_acc, stages, newton_it, krylov_it, linsolves; double maxres, t, dt_last; int status;
                long jac_builds; };
@@ -502,6 +577,7 @@ struct Ctx {
 	std::vector<double> U, Un, Qe, Qs, Ust, wxL, wxR, wyL, wyR, H[3];
 	std::vector<double> phx, phy, a2s, ubx, uby;
 	std::vector<double> eTV1, eTV2, ETV, dA;   // the TV-type variants' explicit phase energies and total energy; BB's alpha_1 step
+	std::vector<int> ctype;                    // STEP_TYPE (PC:537-560): each interior cell's closure type of the step
 	// stage data (PC:565-582)
 	double tau, rx, ry;
 	std::vector<double> ust, vst, ubsx, kpx, wLx, pvx, ubsy, kpy, wLy, pvy, Kw, Qph, cz, Pold;
@@ -983,6 +1059,7 @@ static bool cell_residual(const Ctx& X, int i, int j, double Pc, double PW, doub
 	if (!(rho > 0)) return false;
 	ClIn in; in.atr[0] = Xv[0]; in.atr[1] = Xv[1]; in.mtr[0] = Xv[2]; in.mtr[1] = Xv[3]; in.etr[0] = Xv[4]; in.etr[1] = Xv[5];
 	in.s = s; in.rhoeC = E - 0.5 * (mx * mx + my * my) / rho; in.pguess = X.P_REF + Pc; in.Qph[0] = X.Qph[c * 2]; in.Qph[1] = X.Qph[c * 2 + 1];
+	in.type = X.ctype.empty() ? CL_P : X.ctype[(size_t)(i - GH) + (size_t)(j - GH) * X.nx];   // S.type (PC:951)
 	ClOut o; if (!closure(M, in, o)) return false;
 	G = (o.pbar - X.P_REF) - Pc;
 	if (out) { out->cl = o; out->mx = mx; out->my = my; out->E = E; out->s = s; out->sig = closure_roundoff(M, in, o.pbar); }
@@ -1189,8 +1266,35 @@ static void init_pressure(Ctx& X) {                                             
 	for (int j = 0; j < X.ny; j++) for (int i = 0; i < X.nx; i++) X.P[(size_t)i + (size_t)j * X.nx] = pbar_of(X.M, &X.U[NV * X.id(i + GH, j + GH)]) - X.P_REF;
 }
 // one attempt of a step of size dt with an rk-stage table; false = inadmissible (PC:1243-1285)
+// ---- relaxation level 1, p-pT (PC:513-560, PHY:814-830): each cell's closure type for the step (thermal equilibrium
+// where eps_I <= alpha_1 <= 1 - eps_I, else mechanical), with U^n projected onto the pT equilibrium in those cells
+static bool project_cell(const Model& M, double* Q, int type) {                                      // PC:524-535
+	if (type != CL_PT) return true;
+	const double rho = Q[IM1] + Q[IM2];
+	ClIn in{}; in.atr[0] = Q[IA1]; in.atr[1] = 1.0 - Q[IA1]; in.mtr[0] = Q[IM1]; in.mtr[1] = Q[IM2];
+	in.etr[0] = Q[IE1]; in.etr[1] = Q[IE2]; in.s = 0.0;
+	in.rhoeC = Q[IEN] - 0.5 * (Q[IMU] * Q[IMU] + Q[IMV] * Q[IMV]) / rho;
+	in.type = type; in.pguess = smax(p_mix(M, in.atr[0], in.mtr[0], in.mtr[1], in.rhoeC), 1.0);
+	ClOut o; if (!closure(M, in, o)) return false;
+	Q[IA1] = o.a[0]; Q[IM1] = o.m[0]; Q[IM2] = o.m[1]; Q[IE1] = o.e[0]; Q[IE2] = o.e[1];
+	return true;
+}
+static void relaxation_activation(Ctx& X, std::vector<double>& U) {                                // PC:537-560
+	const int nx = X.nx, n = nx * X.ny;
+	X.ctype.assign((size_t)n, CL_P);
+	#pragma omp parallel for schedule(static)
+	for (int q = 0; q < n; q++) {
+		double* Q = &U[NV * X.id(q % nx + GH, q / nx + GH)];
+		const double a1 = Q[IA1];
+		int ty = (X.C.relax > 0 && a1 >= X.C.eps_i && a1 <= 1.0 - X.C.eps_i) ? CL_PT : CL_P;   // closure_type_of at level 1
+		if (!project_cell(X.M, Q, ty)) ty = CL_P;
+		X.ctype[(size_t)q] = ty;
+	}
+	bc_apply(X, U);
+}
 static bool attempt(Ctx& X, double dt, int rk) {
 	set_tab(X, rk);
+	if (X.C.relax > 0) relaxation_activation(X, X.U);   // PC:1247, before u_con_n = u_con
 	X.Un = X.U; X.Pst = X.P;
 	const int nx = X.nx, ny = X.ny;
 	bool ok = true;
@@ -1461,6 +1565,10 @@ static bool case_setup(int test, Cfg& C, double& tend) {                        
 	C.x0 = 0; C.x1 = 1; C.y0 = 0; C.y1 = 1; for (int q = 0; q < 4; q++) { C.bc[q] = BC_TRANS; C.bcu[q] = 0; }
 	switch (test) {
 	case 0: set_SG(C, 0, 4.4, 6e8); set_SG(C, 1, 1.4, 0); tend = 1.85e-3; break;
+	case 1: case 2:   // PHY:965-967: Pelanti-Shyue water cavitation tubes, liquid and vapour with their thermal parameters
+		set_SG(C, 0, 2.35, 1e9); C.ph[0][2] = -1167e3; C.ph[0][4] = 1816.0;
+		set_SG(C, 1, 1.43, 0.0); C.ph[1][2] = 2030e3; C.ph[1][3] = -23.4e3; C.ph[1][4] = 1040.0;
+		tend = test == 1 ? 3.2e-3 : 0.58e-3; break;
 	case 4: set_SG(C, 0, 4.4, 6e8); set_SG(C, 1, 1.4, 0); tend = 2.79e-3; break;
 	case 5: set_SG(C, 0, 4.4, 6e8); set_SG(C, 1, 1.4, 0); tend = 240e-6; break;
 	case 80: case 82: case 83: case 84: set_SG(C, 0, 1.4, 0); set_SG(C, 1, 1.4, 0); tend = test == 80 ? 0.5 : test == 82 ? 0.14 : test == 83 ? 2.5e-6 : 0.15; break;
@@ -1480,6 +1588,9 @@ static void case_state(const Model& M, int test, double x, double y, double* Q) 
 		Prim w; w.a1 = a1; w.r1 = r1; w.r2 = r2; w.u = u; w.v = v; w.p1 = p1; w.p2 = p2; prim2cell(M, w, Q); };
 	switch (test) {
 	case 0: { const double u = x < 0.5 ? -100.0 : 100.0; setcell(1 - 1e-2, 1000.0, 1.0, u, 0, 1e5, 1e5); break; }
+	case 1: case 2: {   // PHY:1034-1036: T = 354.728 K, p = 1e5 Pa, u = -U0 | +U0 about x = 0.5, vapour fraction 1e-2
+		const double T0 = 354.728, U0 = test == 1 ? 2.0 : 500.0, u = x < 0.5 ? -U0 : U0;
+		setcell(1 - 1e-2, 1.0 / v_pT(M, 0, 1e5, T0), 1.0 / v_pT(M, 1, 1e5, T0), u, 0, 1e5, 1e5); break; }
 	case 4: { const bool L = x < 0.5; setcell(L ? 1 - 1e-8 : 1e-8, 1000.0, 10.0, 100.0, 0, 1e5, 1e5); break; }
 	case 5: { const bool L = x < 0.75; setcell(L ? 1 - 1e-6 : 1e-6, 1000.0, 1.0, 0, 0, L ? 1e9 : 1e5, L ? 1e9 : 1e5); break; }
 	case 80: case 81: case 82: case 83: case 84: case 85: {
@@ -1551,6 +1662,8 @@ void ora_get_state(void* h, double* Uout) {
 }
 void ora_get_pressure(void* h, double* P, double* pref) { Ctx& X = *(Ctx*)h; std::memcpy(P, X.P.data(), X.P.size() * sizeof(double)); *pref = X.P_REF; }
 int ora_run(void* h, double t_end, long max_steps) { return run(*(Ctx*)h, t_end, max_steps); }
+// RelaxationActivation on the state (the reference's main applies it once more before its output, PC:1300)
+void ora_relax_activate(void* h) { Ctx& X = *(Ctx*)h; if (X.C.relax > 0) relaxation_activation(X, X.U); }
 // the explicit HLLC scheme's time loop on the context's state (cfl: the acoustic CFL, 0.4 in the reference); the
 // relaxation's closure failures accumulate in ora_hllc_closure_fails
 int ora_hllc_run(void* h, double t_end, long max_steps, double cfl) { return hllc_run(*(Ctx*)h, t_end, max_steps, cfl); }

@@ -3,7 +3,8 @@
 Source files (data, not code) under the reference tree: data/Schropff2026_Riemann_problems/c{80..88}_200.dat,
 data/SPB2009_4.3.1_water_air/water_air_N100_t240us.dat (case 5), data/SPB2009_4.1_interface_advection/
 interface_advection_t2.79ms.dat (case 4, N = 200). Columns kept (17 significant digits in the source): p u v rho alpha2
-Y2 rho1 rho2 p1 p2 (write_profile, PC:1002-1013). Run: python tests/golden/make_ref_profiles.py /root/reference
+Y2 rho1 rho2 p1 p2 (write_profile, PC:1002-1013); and data/PS2014_water_cavitation_tube_{2mps,500mps}/ (cases 1 and 2,
+5000 cells, relaxation levels 0 and 1). Run: python tests/golden/make_ref_profiles.py /root/reference
 """
 import os
 import sys
@@ -14,6 +15,12 @@ COLS = {"p": 2, "u": 3, "v": 4, "rho": 5, "alpha2": 6, "Y2": 7, "rho1": 8, "rho2
 FILES = {f"c{c}_200": (c, 200, f"data/Schropff2026_Riemann_problems/c{c}_200.dat") for c in range(80, 89)}
 FILES["c5_100"] = (5, 100, "data/SPB2009_4.3.1_water_air/water_air_N100_t240us.dat")
 FILES["c4_200"] = (4, 200, "data/SPB2009_4.1_interface_advection/interface_advection_t2.79ms.dat")
+# Pelanti-Shyue water cavitation tubes (cases 1, 2; 5000 cells) at relaxation levels 0 (p) and 1 (p-pT): the columns
+# p u rho alpha2 p1 p2 (the others follow from them or carry fewer printed digits)
+RELAX_COLS = ["p", "u", "rho", "alpha2", "p1", "p2"]
+RELAX_FILES = {"c1_5000_r0": (1, 5000, 0, "data/PS2014_water_cavitation_tube_2mps/p_relax.dat"),
+               "c1_5000_r1": (1, 5000, 1, "data/PS2014_water_cavitation_tube_2mps/p-pT_relax.dat"),
+               "c2_5000_r1": (2, 5000, 1, "data/PS2014_water_cavitation_tube_500mps/p-pT_relax.dat")}
 
 
 def main(ref):
@@ -24,6 +31,12 @@ def main(ref):
         for name, col in COLS.items():
             out[f"{key}/{name}"] = a[:, col]
         out[f"{key}/meta"] = np.array([case, n], dtype=np.int64)
+    for key, (case, n, relax, rel) in RELAX_FILES.items():
+        a = np.loadtxt(os.path.join(ref, rel))
+        assert a.shape[0] == n, (key, a.shape)
+        for name in RELAX_COLS:
+            out[f"{key}/{name}"] = a[:, COLS[name]]
+        out[f"{key}/meta"] = np.array([case, n, relax], dtype=np.int64)
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_profiles.npz")
     np.savez_compressed(dst, **out)
     print(dst, len(out))
