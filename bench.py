@@ -65,6 +65,7 @@ def parse():
                     help="comma list of cavitation (the headline), bubble, interface, batch; default all (cavitation "
                          "only with --profile)")
     ap.add_argument("--forms", default=None, help="comma list of plain, mixed, learned (default all three)")
+    ap.add_argument("--graph", action="store_true", help="graph mode: stage heads and Krylov chunks as captured hipGraphs")
     ap.add_argument("--bubble-warmup", type=int, default=260,
                     help="warm-up steps of the 2048x1024 shock-bubble leg (the shock reaches the bubble near step 250)")
     ap.add_argument("--bubble-steps", type=int, default=20)
@@ -86,9 +87,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+GRAPH = {}   # {"graph": 1} in graph mode (--graph)
+
+
 def make_stepper(make, form):
     from diffusionimex_amd import spb6
-    cfg, U = make(**FORM_KW[form])
+    cfg, U = make(**FORM_KW[form], **GRAPH)
     st = spb6.Stepper(cfg, device=0)
     if form == "learned":
         from diffusionimex_amd import mlp
@@ -241,6 +245,8 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); multi-GPU sharding is design-only", file=sys.stderr)
         sys.exit(2)
     _pin_cpu_threads(a)
+    if a.graph:
+        GRAPH["graph"] = 1
     if a.legs is None:
         a.legs = "cavitation" if a.profile else "cavitation,bubble,interface,batch"
     legs = set(x for x in a.legs.split(",") if x)
@@ -254,6 +260,8 @@ def main():
     if "batch" in legs and a.batch_steps > 0 and not a.profile:
         log("batched 1D leg")
         out["batch"] = run_batch(a)
+    if a.graph:
+        out["graph_mode"] = True
     log("done")
     print(json.dumps(out))
 

@@ -117,6 +117,13 @@ struct spb6_ctx {
 	double *eTV1 = nullptr, *eTV2 = nullptr, *ETV = nullptr, *dA = nullptr, *donax = nullptr, *donay = nullptr;
 	int* ctype = nullptr;   // relaxation level 1: the closure type of each interior cell for the step (STEP_TYPE)
 	double eps_i = 1e-4;    // EPS_I (PHY:38)
+	// graph mode (cfg.graph): a stage's fixed head (combination, explicit part, preparation, shock budget) and each
+	// Krylov chunk (its BiCGStab passes, the FP32 form's defect, sweep close and trial residual) run as captured graphs,
+	// re-captured per launch and applied to the instantiated graph of the same shape with hipGraphExecUpdate
+	bool capturing = false;
+	hipGraphExec_t g_head[3] = {nullptr, nullptr, nullptr};
+	hipGraphExec_t g_chunk[2][33] = {};   // [trial residual behind the chunk][chunk length]
+	long graph_launches = 0, graph_rebuilds = 0;
 };
 
 struct spb6_mlp {
@@ -158,19 +165,19 @@ Dev make_dev(const spb6_ctx* c) {
 // timed launch region
 struct TRegion {
 	spb6_ctx* c; int cat; hipEvent_t a = nullptr;
-	TRegion(spb6_ctx* c_, int cat_) : c(c_), cat(cat_) { if (c->timer.want(cat)) { a = c->timer.get(); hipEventRecord(a, c->st); } }
+	TRegion(spb6_ctx* c_, int cat_) : c(c_), cat(cat_) { if (c->timer.want(cat) && !c->capturing) { a = c->timer.get(); hipEventRecord(a, c->st); } }
 	~TRegion() { if (a) { hipEvent_t b = c->timer.get(); hipEventRecord(b, c->st); c->timer.pend.push_back({cat, a, b}); if (c->timer.pend.size() > 512 && !c->timer.hold) c->timer.resolve(); } }
 };
 
 // clear the norms and the flags of one evaluation (the stage's admissibility flags stay)
 int reset_scalars(spb6_ctx* c) {
-	HIPCHK(hipMemsetAsync(c->mail, 0, offsetof(HostScratch, flags) + FLAG_RHO * sizeof(int), c->st));
+	launch_zero_ints(reinterpret_cast<int*>(c->mail), (long)((offsetof(HostScratch, flags) + FLAG_RHO * sizeof(int)) / sizeof(int)), c->st);
 	return SPB6_OK;
 }
 // clear the norms and the flags (the start of a stage); keep_prev keeps the previous stage's final-state flag
 int reset_stage(spb6_ctx* c, bool keep_prev = false) {
 	const size_t end = keep_prev ? offsetof(HostScratch, flags) + FLAG_FINAL_PREV * sizeof(int) : offsetof(HostScratch, ks);
-	HIPCHK(hipMemsetAsync(c->mail, 0, end, c->st));   // nrm and flags (contiguous)
+	launch_zero_ints(reinterpret_cast<int*>(c->mail), (long)(end / sizeof(int)), c->st);   // nrm and flags (contiguous)
 	return SPB6_OK;
 }
 // copy norms, flags and Krylov scalars to pinned host memory and wait
@@ -185,6 +192,31 @@ inline double nrm_val(const spb6_ctx* c, int k) {
 	unsigned long long m = 0;
 	for (int q = 0; q < 64; q++) m = std::max(m, c->hs->nrm[64 * k + q]);
 	double v; std::memcpy(&v, &m, sizeof v); return v;
+}
+
+// graph mode: the launches of f captured into a graph and run through the instantiated graph in slot (updated in place
+// when the shape matches, instantiated anew when it does not); without graph mode f launches directly
+template <class F>
+int run_graph(spb6_ctx* c, hipGraphExec_t& slot, F f) {
+	if (!c->cfg.graph) { f(); return SPB6_OK; }
+	HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+	c->capturing = true;
+	f();
+	c->capturing = false;
+	hipGraph_t gr = nullptr;
+	HIPCHK(hipStreamEndCapture(c->st, &gr));
+	if (slot) {
+		hipGraphNode_t bad = nullptr; hipGraphExecUpdateResult res;
+		if (hipGraphExecUpdate(slot, gr, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+			(void)hipGetLastError();
+			(void)hipGraphExecDestroy(slot); slot = nullptr;
+		}
+	}
+	if (!slot) { c->graph_rebuilds++; HIPCHK(hipGraphInstantiate(&slot, gr, nullptr, nullptr, 0)); }
+	HIPCHK(hipGraphDestroy(gr));
+	HIPCHK(hipGraphLaunch(slot, c->st));
+	c->graph_launches++;
+	return SPB6_OK;
 }
 
 // BiCGStab (PC:725-758) on the Jacobian c->J with right-hand side c->rhs; solution in c->dP. With krylov_fp32 the
@@ -202,10 +234,15 @@ int krylov_run(spb6_ctx* c, const KrylovBufsT<T>& B, const T* b, double tol, int
 	for (;;) {
 		{
 			TRegion tr(c, T_KRYLOV);
-			for (int k = 0; k < chunk && launched <= maxit; k++, launched++) {
-				if constexpr (sizeof(T) == 8) launch_krylov_iter(g, B, c->st, launched, tol, maxit);
-				else launch_krylov_iter32(g, B, c->st, launched, tol, maxit);
-			}
+			const int nk = std::min(chunk, maxit + 1 - launched), l0 = launched;
+			int rc = run_graph(c, c->g_chunk[0][std::min(nk, 32)], [&] {
+				for (int k = 0; k < nk; k++) {
+					if constexpr (sizeof(T) == 8) launch_krylov_iter(g, B, c->st, l0 + k, tol, maxit);
+					else launch_krylov_iter32(g, B, c->st, l0 + k, tol, maxit);
+				}
+			});
+			if (rc) return rc;
+			launched += nk;
 		}
 		HIPCHK(hipMemcpyAsync(&c->hs->ks, c->ksd, sizeof(KS), hipMemcpyDeviceToHost, c->st));
 		HIPCHK(hipStreamSynchronize(c->st));
@@ -262,27 +299,32 @@ int krylov(spb6_ctx* c, int& iters, bool& ok, double newton_res, bool new_J, boo
 		// (and a speculative trial residual is filed once its gate is known: T_RESIDUAL when it ran, T_KIDLE otherwise)
 		struct IterEv { int it; hipEvent_t a, b; };
 		std::vector<IterEv> iev;
-		const bool iter_ev = c->timer.want(T_KRYLOV) || c->timer.want(T_KIDLE);
-		const bool spec_ev = c->timer.want(T_RESIDUAL) || c->timer.want(T_KIDLE);
+		const bool iter_ev = (c->timer.want(T_KRYLOV) || c->timer.want(T_KIDLE)) && !c->cfg.graph;
+		const bool spec_ev = (c->timer.want(T_RESIDUAL) || c->timer.want(T_KIDLE)) && !c->cfg.graph;
 		c->timer.hold = c->timer.on;
 		for (;;) {
-			for (int k = 0; k < chunk && launched <= maxit; k++, launched++) {
-				hipEvent_t a = nullptr, b = nullptr;
-				if (iter_ev) { a = c->timer.get(); hipEventRecord(a, c->st); }
-				launch_krylov_iter32(g, B, c->st, launched, tin, maxit);
-				if (iter_ev) { b = c->timer.get(); hipEventRecord(b, c->st); iev.push_back({launched, a, b}); }
-			}
-			{
-				TRegion tr(c, T_KAUX);
-				launch_defect_x(g, c->J, c->rhs, c->dP, c->x32, c->kr, c->part2, c->part_stride, c->dsum, c->Pst, c->sc, c->st);
-			}
 			hipEvent_t sa = nullptr, sb = nullptr;
-			if (Dspec) {
-				{ TRegion tr(c, T_KAUX); launch_spec_commit(n, c->x32, c->dP, c->ksd, c->dsum, eta, &c->mail->spec, &c->nrm[0], c->flags, c->st); }
-				if (spec_ev) { sa = c->timer.get(); hipEventRecord(sa, c->st); }
-				launch_residual(*Dspec, c->Pst, c->dP, 1.0, c->Pn, c->Gn, &c->nrm[0], FLAG_RES, c->st, &c->mail->spec);
-				if (spec_ev) { sb = c->timer.get(); hipEventRecord(sb, c->st); }
-			}
+			const int nk = std::min(chunk, maxit + 1 - launched), l0 = launched;
+			int grc = run_graph(c, c->g_chunk[Dspec ? 1 : 0][std::min(nk, 32)], [&] {
+				for (int k = 0; k < nk; k++) {
+					hipEvent_t a = nullptr, b = nullptr;
+					if (iter_ev) { a = c->timer.get(); hipEventRecord(a, c->st); }
+					launch_krylov_iter32(g, B, c->st, l0 + k, tin, maxit);
+					if (iter_ev) { b = c->timer.get(); hipEventRecord(b, c->st); iev.push_back({l0 + k, a, b}); }
+				}
+				{
+					TRegion tr(c, T_KAUX);
+					launch_defect_x(g, c->J, c->rhs, c->dP, c->x32, c->kr, c->part2, c->part_stride, c->dsum, c->Pst, c->sc, c->st);
+				}
+				if (Dspec) {
+					{ TRegion tr(c, T_KAUX); launch_spec_commit(n, c->x32, c->dP, c->ksd, c->dsum, eta, &c->mail->spec, &c->nrm[0], c->flags, c->st); }
+					if (spec_ev) { sa = c->timer.get(); hipEventRecord(sa, c->st); }
+					launch_residual(*Dspec, c->Pst, c->dP, 1.0, c->Pn, c->Gn, &c->nrm[0], FLAG_RES, c->st, &c->mail->spec);
+					if (spec_ev) { sb = c->timer.get(); hipEventRecord(sb, c->st); }
+				}
+			});
+			if (grc) return grc;
+			launched += nk;
 			HIPCHK(hipMemcpyAsync(c->hs, c->mail, sizeof(HostScratch), hipMemcpyDeviceToHost, c->st));
 			HIPCHK(hipStreamSynchronize(c->st));
 			if (sa) {
@@ -455,8 +497,8 @@ int prep_part(spb6_ctx* c, double adt) {
 // PressureSubsystem (PC:851-993) after the explicit part; writes the stage state into Ust and H_rks. defer_final
 // (a stage with another after it in the same attempt): the final state's closure flag is not waited for here but
 // read back with the next stage's first residual, which then reports the failure as this stage would have
-int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& iters, double& res, bool defer_final = false) {
-	int rc = prep_part(c, adt); if (rc) return rc;
+int implicit_part(spb6_ctx* c, double adt, const double* Qs, double* Hk, int& iters, double& res, bool defer_final = false, bool prepped = false) {
+	int rc = prepped ? SPB6_OK : prep_part(c, adt); if (rc) return rc;
 	Dev D = stage_dev(c, adt);
 	const long n = c->g.n;
 	const bool dumping = c->dX && c->dcap > 0;
@@ -576,19 +618,24 @@ int attempt(spb6_ctx* c, double dt, int rk) {
 	HIPCHK(hipMemcpyAsync(c->Pst, c->P, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 	c->final_pending = false;
 	for (int rks = 0; rks < rk; rks++) {
-		const double *Qe = c->U, *Qs = c->U;
-		if (rks > 0) {
-			TRegion tr(c, T_OTHER);
-			launch_combine(c->g, c->U, c->H[0], c->H[1], rks, c->Ae[rks][0] * dt, rks > 1 ? c->Ae[rks][1] * dt : 0.0,
-			               c->Ai[rks][0] * dt, rks > 1 ? c->Ai[rks][1] * dt : 0.0, c->Qe, c->Qs, c->st);
-			launch_bc_grid(c->g, c->Qe, c->st); launch_bc_grid(c->g, c->Qs, c->st);
-			Qe = c->Qe; Qs = c->Qs;
-		}
+		const double *Qe = rks > 0 ? c->Qe : c->U, *Qs = rks > 0 ? c->Qs : c->U;
 		const double adt = c->Ai[rks][rks] * dt;
-		int rc = reset_stage(c, rks > 0); if (rc) return rc;
-		rc = explicit_part(c, Qe, Qs, adt); if (rc) return rc;   // its flag is read with the first residual
+		// the stage's fixed head: the stage operands, the explicit part (its flag is read with the first residual), the
+		// preparation and the shock budget -- one graph in graph mode
+		int rc = run_graph(c, c->g_head[rks], [&] {
+			if (rks > 0) {
+				TRegion tr(c, T_OTHER);
+				launch_combine(c->g, c->U, c->H[0], c->H[1], rks, c->Ae[rks][0] * dt, rks > 1 ? c->Ae[rks][1] * dt : 0.0,
+				               c->Ai[rks][0] * dt, rks > 1 ? c->Ai[rks][1] * dt : 0.0, c->Qe, c->Qs, c->st);
+				launch_bc_grid(c->g, c->Qe, c->st); launch_bc_grid(c->g, c->Qs, c->st);
+			}
+			reset_stage(c, rks > 0);
+			explicit_part(c, Qe, Qs, adt);
+			prep_part(c, adt);
+		});
+		if (rc) return rc;
 		int iters; double res;
-		rc = implicit_part(c, adt, Qs, c->H[rks], iters, res, rks + 1 < rk);
+		rc = implicit_part(c, adt, Qs, c->H[rks], iters, res, rks + 1 < rk, true);
 		if (rc != SPB6_OK) return rc;
 	}
 	return SPB6_OK;
@@ -785,6 +832,10 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 }
 
 void spb6_destroy(spb6_ctx* c) {
+	if (c) {
+		for (auto& e : c->g_head) if (e) (void)hipGraphExecDestroy(e);
+		for (auto& row : c->g_chunk) for (auto& e : row) if (e) (void)hipGraphExecDestroy(e);
+	}
 	if (!c) return;
 	if (c->st) hipStreamSynchronize(c->st);
 	for (void* p : c->allocs) hipFree(p);
@@ -1299,6 +1350,7 @@ int spb6_get_kernel_times(spb6_ctx* c, spb6_kernel_times_t* t) {
 	HIPCHK(hipMemcpyAsync(&rk, c->shock_steps, sizeof rk, hipMemcpyDeviceToHost, c->st));
 	HIPCHK(hipStreamSynchronize(c->st));
 	t->shock_rk_steps = (long)rk;
+	t->graph_launches = c->graph_launches; t->graph_rebuilds = c->graph_rebuilds;
 	return SPB6_OK;
 }
 

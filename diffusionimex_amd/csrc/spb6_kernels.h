@@ -58,6 +58,8 @@ constexpr int KRYLOV_SLOTS = 11;   // partial-sum slots of the Krylov passes
 int krylov_items(int n);
 
 void launch_bc_grid(const Geo& g, double* u, hipStream_t st);
+// zero n ints with a kernel (capturable in a graph)
+void launch_zero_ints(int* p, long n, hipStream_t st);
 void launch_bc_array(const Geo& g, double* a, hipStream_t st);
 void launch_combine(const Geo& g, const double* Un, const double* H0, const double* H1, int nr, double ae0, double ae1, double ai0, double ai1, double* Qe, double* Qs, hipStream_t st);
 void launch_explicit(const Dev& D, int limiter, int order, const double* Qe, const double* Qs, double adt, hipStream_t st);

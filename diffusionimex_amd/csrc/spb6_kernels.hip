@@ -1237,8 +1237,15 @@ int shock_strong_blocks_per_cu() {
 	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(k_shock_strong), 256, 0) != hipSuccess || b < 1) b = 2;
 	return b;
 }
+// zero n ints (a kernel, so that it can be captured in a graph, where memset nodes are avoided)
+__global__ void __launch_bounds__(256) k_zero_ints(int* __restrict__ p, long n) {
+	for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) p[k] = 0;
+}
+void launch_zero_ints(int* p, long n, hipStream_t st) {
+	hipLaunchKernelGGL(k_zero_ints, dim3(std::max(1, std::min(nblk(n), 64))), dim3(256), 0, st, p, n);
+}
 void launch_shock(const Dev& D, int* list, int* count, double* rec, unsigned long long* rk_steps, int strong_blocks, hipStream_t st) {
-	(void)hipMemsetAsync(count, 0, 2 * sizeof(int), st);   // the count and the strong pass's cursor
+	launch_zero_ints(count, 2, st);   // the count and the strong pass's cursor
 	hipLaunchKernelGGL(k_shock_classify, dim3(nblk(D.g.n)), dim3(256), 0, st, D.g, D.ph, D.Ust, D.Qph, list, count, rec, (long)D.g.n);
 	hipLaunchKernelGGL(k_shock_strong, dim3(std::max(1, std::min(nblk(D.g.n), strong_blocks))), dim3(256), 0, st, D.g, D.ph,
 	                   (const double*)rec, (long)D.g.n, D.Qph, (const int*)list, (const int*)count, count + 1, rk_steps);

@@ -36,7 +36,7 @@ class Config(C.Structure):
         ("krylov_tol", C.c_double), ("krylov_iter", C.c_int),
         ("limiter", C.c_int), ("order", C.c_int), ("rus_local", C.c_int),
         ("krylov_fp32", C.c_int), ("learned_guess", C.c_int), ("analytic_jacobian", C.c_int),
-        ("jreuse", C.c_int), ("variant", C.c_int), ("relax", C.c_int), ("eps_i", C.c_double),
+        ("jreuse", C.c_int), ("variant", C.c_int), ("relax", C.c_int), ("eps_i", C.c_double), ("graph", C.c_int),
     ]
 
 
@@ -55,7 +55,8 @@ class KernelTimes(C.Structure):
                [(k, C.c_long) for k in ("explicit_n", "prep_n", "shock_n", "residual_n", "jacobian_n", "krylov_n",
                                          "final_n", "krylov_iters")] + \
                [("krylov_aux_ms", C.c_double), ("guess_ms", C.c_double), ("krylov_launched", C.c_long),
-                ("krylov_idle_ms", C.c_double), ("shock_rk_steps", C.c_long)]
+                ("krylov_idle_ms", C.c_double), ("shock_rk_steps", C.c_long), ("graph_launches", C.c_long),
+                ("graph_rebuilds", C.c_long)]
 
 
 # every symbol include/spb6_imex.h declares
@@ -165,7 +166,7 @@ def _dp(a):
 def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0, 1.0), bc=(0, 0, 0, 0), bcu=(0, 0, 0, 0),
                 rk=2, cfl=0.2, cfl_ac=0.4, ramp=1, dtfix=0.0, acdt=0, newton_tol=1e-13, newton_iter=60,
                 krylov_tol=1e-13, krylov_iter=4000, limiter=1, order=2, rus_local=1, krylov_fp32=0, learned_guess=0,
-                analytic_jacobian=0, jreuse=0, variant=0, relax=0, eps_i=0.0):
+                analytic_jacobian=0, jreuse=0, variant=0, relax=0, eps_i=0.0, graph=0):
     """Config with the reference defaults (PC:57-80): phases are (gamma, p_inf[, q, qp, cv, b])."""
     c = Config()
     c.nx, c.ny = nx, ny
@@ -185,6 +186,7 @@ def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0
     c.jreuse = jreuse
     c.variant = variant
     c.relax, c.eps_i = relax, eps_i
+    c.graph = graph
     return c
 
 

@@ -95,6 +95,9 @@ typedef struct spb6_config {
                                         and the step's RelaxationActivation, PC:537-560). Finite-difference Jacobian and
                                         plain guess only; the batch and the explicit HLLC scheme reject level 1 */
     double eps_i;                    /* EPS_I (PHY:38); <= 0: the reference's 1e-4 */
+    int graph;                       /* 1: each stage's fixed head (stage operands, explicit part, preparation, shock budget)
+                                        and each Krylov chunk (BiCGStab passes; the FP32 form's defect, sweep close and
+                                        trial residual) launched as a captured hipGraph; the same kernels, the same bits */
 } spb6_config_t;
 
 /* run statistics: the counters of the reference's stats line (PC:1337-1339) plus Krylov counts */
@@ -252,6 +255,8 @@ typedef struct spb6_kernel_times {
     long krylov_launched;        /* BiCGStab iterations launched (a pass launched after convergence returns at once) */
     double krylov_idle_ms;       /* FP32 form: time of the iterations launched past a solve's end (krylov_ms: taken ones) */
     long shock_rk_steps;         /* RK45 profile steps taken by the shock budget's strong-shock passes (its work count) */
+    long graph_launches;         /* graph mode: captured graphs launched, and graphs instantiated anew (a shape that */
+    long graph_rebuilds;         /* hipGraphExecUpdate could not apply to the graph of its slot) */
 } spb6_kernel_times_t;
 int spb6_enable_timing(spb6_ctx* ctx, int on);
 /* categories timed while timing is on (default all): bit k = the k-th category in the order explicit, prep, shock,
