@@ -42,11 +42,15 @@ BYTES_KRYLOV_FP32 = 148
 # (r, v, 1/d and the four off-diagonals in; s, M^-1 s out), kb_t 32 (five diagonals, M^-1 s, s in; t out), kb_xr 32
 # (M^-1 p, M^-1 s, s, t, x, r0 in; x, r out); a black row's red neighbours come through the caches
 BYTES_KRYLOV_FP32_RB = 172
-# FP64 operations of one RK45 step of the shock budget's profile integration (prof_step, divisions counted as one
-# operation): six right-hand sides of 43 (three divisions per phase, v2, num / den, dy1, the pressure line), stage
-# inputs 9 + 18 + 27 + 36 + 45 plus 2 each, the error and fifth-order sums 69, the step-size power 72 (log 36, exp 34,
-# the division and the product), the scales 6, the end test 2 and the step update 3
-SHOCK_RK_FLOPS = 555
+# FP64 operations of one RK45 step of the shock budget's profile integration (divisions counted as one operation, an
+# FMA as two). The reference's arithmetic (prof_step): six right-hand sides of 43 (three divisions per phase, v2,
+# num / den, dy1, the pressure line), stage inputs 9 + 18 + 27 + 36 + 45 plus 2 each, the error and fifth-order sums
+# 69, the step-size power 72 (log 36, exp 34, the division and the product), the scales 6, the end test 2 and the step
+# update 3: 555. The fused form (prof_step_fused, the bench's mixed and learned forms): six right-hand sides of 33 (v2
+# 3, the pressure line 4, n_k 2, x_k 4, den 5, num 6, the one reciprocal and dy0, dy1 9), stage inputs 15 products
+# h B plus 45 FMAs over three components and the 5 abscissae (2 each), sums, y5 and error 66, the power 72, the end
+# test 2 and the step update 3 (the scales are per cell): 456.
+SHOCK_RK_FLOPS = {0: 555, 1: 456}
 
 
 def parse():
@@ -77,8 +81,8 @@ def parse():
     return ap.parse_args()
 
 
-FORM_KW = {"plain": {}, "mixed": {"krylov_fp32": 1, "analytic_jacobian": 1},
-           "learned": {"krylov_fp32": 1, "analytic_jacobian": 1, "learned_guess": 1}}
+FORM_KW = {"plain": {}, "mixed": {"krylov_fp32": 1, "analytic_jacobian": 1, "shock_fused": 1},
+           "learned": {"krylov_fp32": 1, "analytic_jacobian": 1, "learned_guess": 1, "shock_fused": 1}}
 STAT_KEYS = ("steps", "stages", "newton_it", "krylov_it", "linsolves", "rejects", "guess_used", "guess_fallback", "guess_skipped")
 
 
@@ -138,7 +142,7 @@ def run_form(make, form, warmup, steps, breakdown):
     res = {"value": n * d["steps"] / wall, "wall": wall, "ms_per_step": 1e3 * wall / max(d["steps"], 1),
            "newton_per_stage": d["newton_it"] / max(d["stages"], 1), "krylov_per_newton": d["krylov_it"] / max(d["linsolves"], 1),
            "krylov_per_step": d["krylov_it"] / max(d["steps"], 1), "d": d, "kt": kt_all, "d_all": d_all, "dom": dom,
-           "kt_live": kt_live, "t": st.t, "n": n}
+           "kt_live": kt_live, "t": st.t, "n": n, "fused": int(cfg.shock_fused)}
     if form == "learned":
         res["guess"] = {k: d[k] for k in ("guess_used", "guess_fallback", "guess_skipped")}
     return res, st
@@ -183,12 +187,12 @@ def roofline_of(res, n, fp32, traffic=None):
     if fp32 and kt["krylov_ms"] > 0:
         by_kernel["krylov_iter_rb172"] = round(BYTES_KRYLOV_FP32_RB * n * res["d_all"]["krylov_it"] / (kt["krylov_ms"] / 1e3) / HBM_PEAK, 4)
     if kt["shock_ms"] > 0:
-        by_kernel["shock_fp64"] = round(kt["shock_rk_steps"] * SHOCK_RK_FLOPS / (kt["shock_ms"] / 1e3) / FP64_PEAK, 4)
+        by_kernel["shock_fp64"] = round(kt["shock_rk_steps"] * SHOCK_RK_FLOPS[res["fused"]] / (kt["shock_ms"] / 1e3) / FP64_PEAK, 4)
     dom, live = res["dom"], res["kt_live"]
     if dom == "shock":
         ms, cnt = live["shock_ms"], live["shock_n"]
         avg_s = (ms / 1e3) / max(cnt, 1)
-        flops = live["shock_rk_steps"] * SHOCK_RK_FLOPS / max(cnt, 1)
+        flops = live["shock_rk_steps"] * SHOCK_RK_FLOPS[res["fused"]] / max(cnt, 1)
         achieved = flops / avg_s if avg_s > 0 else 0.0
         roof = {"bound": "fp64", "kernel": "shock", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK, "traffic": traffic, "flops_per_launch": flops, "rk_steps_per_launch": live["shock_rk_steps"] / max(cnt, 1),

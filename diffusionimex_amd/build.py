@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libspb6_hip.so")
 SOURCES = ["spb6_kernels.hip", "spb6_variants.hip", "spb6_hllc.hip", "spb6_mlp.hip", "spb6_batch.hip", "spb6_api.hip"]
 ARCH = os.environ.get("SPB6_ARCH", "gfx950")
 TESTS = os.path.join(HERE, "..", "tests")
-CHECKS = [("quotient_check.hip", "quotient_check")]   # tests/<source> -> tests/bin/<program>
+CHECKS = [("quotient_check.hip", "quotient_check"), ("shoot_check.hip", "shoot_check")]   # tests/<source> -> tests/bin/<program>
 
 
 def hipcc():

@@ -491,7 +491,7 @@ int prep_part(spb6_ctx* c, double adt) {
 	c->tau_last = adt;
 	Dev D = stage_dev(c, adt);
 	{ TRegion tr(c, T_PREP); launch_prep(D, c->cfg.limiter, c->cfg.order, adt, c->Pst, c->st); }
-	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->slist, c->scount, c->srec, c->shock_steps, c->shock_blocks, c->st); }
+	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->slist, c->scount, c->srec, c->shock_steps, c->shock_blocks, c->cfg.shock_fused, c->st); }
 	return SPB6_OK;
 }
 // PressureSubsystem (PC:851-993) after the explicit part; writes the stage state into Ust and H_rks. defer_final
@@ -665,7 +665,7 @@ int hllc_alloc(spb6_ctx* c) {
 void hllc_relax(spb6_ctx* c, double* U) {
 	launch_bc_grid(c->g, U, c->st);
 	Dev D = make_dev(c); D.Ust = U; D.Ust_c = U;
-	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->slist, c->scount, c->srec, c->shock_steps, c->shock_blocks, c->st); }
+	{ TRegion tr(c, T_SHOCK); launch_shock(D, c->slist, c->scount, c->srec, c->shock_steps, c->shock_blocks, c->cfg.shock_fused, c->st); }
 	{ TRegion tr(c, T_FINAL); launch_hllc_relax(c->g, c->ph, U, c->Qph, c->flags, &c->mail->hxf, c->st); }
 	launch_bc_grid(c->g, U, c->st);
 }
@@ -758,6 +758,7 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (cfg->variant != SPB6_VARIANT_PC && (cfg->analytic_jacobian || cfg->learned_guess)) {
 		g_err = "the IMEX variants run with the finite-difference Jacobian and the plain Newton guess"; return nullptr; }
 	if (cfg->relax < 0 || cfg->relax > 1) { g_err = "relaxation level must be 0 (p) or 1 (p-pT)"; return nullptr; }
+	if (cfg->shock_fused < 0 || cfg->shock_fused > 1) { g_err = "shock_fused must be 0 or 1"; return nullptr; }
 	if (cfg->relax > 0 && (cfg->analytic_jacobian || cfg->learned_guess)) {
 		g_err = "relaxation level 1 runs with the finite-difference Jacobian and the plain Newton guess"; return nullptr; }
 	spb6_ctx* c = new spb6_ctx();

@@ -485,8 +485,9 @@ const char* spb6_batch_last_error(spb6_batch* b) { return b ? b->err.c_str() : "
 
 spb6_batch* spb6_batch_create(const spb6_batch_problem_t* probs, int nprob, const spb6_config_t* cfg, int device) {
 	if (!probs || nprob < 1 || !cfg) return nullptr;
-	// the batch runs the reference scheme (PC) with full Newton at relaxation level 0: no jreuse, variant or relaxation
-	if (cfg->rus_local != 1 || cfg->rk < 1 || cfg->rk > 2 || cfg->jreuse || cfg->variant || cfg->relax) return nullptr;
+	// the batch runs the reference scheme (PC) with full Newton at relaxation level 0 and the reference's profile
+	// arithmetic: no jreuse, variant, relaxation or fused shock integration
+	if (cfg->rus_local != 1 || cfg->rk < 1 || cfg->rk > 2 || cfg->jreuse || cfg->variant || cfg->relax || cfg->shock_fused) return nullptr;
 	spb6_batch* b = new spb6_batch();
 	b->device = device; b->cfg = *cfg;
 	auto fail = [&]() -> spb6_batch* { spb6_batch_destroy(b); return nullptr; };

@@ -406,6 +406,95 @@ SPB_HD bool prof_integrate(const Phys& P, const Prof& S, double c, double& v1end
 		if (r > 0) { v1end = R.y1; qend = R.y2; return true; }
 	}
 }
+// The profile integration in fused form (the strong-shock kernel with shock_fused = 1): the same ODE and the same
+// adaptive Cash-Karp step as prof_rhs / prof_step, rounded differently. With x_k = g_k (p + pi_k), n_k = v_k - b_k,
+// the integration's quotients lr_k = lam_k / Y_k (prof_begin), e_k = Y_k ((g_k - 1) lr_k) and E_0 = (g_0 - 1) lr_0,
+// the two phase terms of PHY:302-310 over their common denominator give
+//     dp/dv  = (-x0 x1 + (p - P)(e0 x1 + e1 x0)) / (Y0 n0 x1 + Y1 n1 x0),
+//     dv1/dv = (E0 (p - P) den - n0 num) / (den x0),
+// so a right-hand side takes one correctly rounded division (1 / (den x0)) where PHY:302-310 takes ten quotients; v2
+// multiplies by 1 / Y1, the error norm by 1 / sc_k (both once per cell), and the stage inputs, the fifth- and
+// fourth-order sums and the products feeding them are FMA chains in the reference's term order. Every operation is a
+// correctly rounded IEEE one (fma included), so the oracle's restatement gives the same bits on the CPU.
+struct ProfF { double rY1, rs0, rs1, rs2, E0, e0, e1; };
+// the per-cell part: 1 / Y1 and the reciprocal error scales 1 / sc_k
+SPB_HD void proff_cell(const Prof& S, ProfF& F) {
+	const double L = S.vend - S.v0;
+	F.rY1 = 1.0 / S.Y1;
+	F.rs0 = 1.0 / (fabs(S.p0) + S.j2 * fabs(L)); F.rs1 = 1.0 / S.vk00; F.rs2 = 1.0 / (S.j2 * L * L + 1e-300);
+}
+// the per-integration part, from the integration's lam_k / Y_k (prof_begin)
+SPB_HD void proff_begin(const Phys& P, const Prof& S, const ProfRun& R, ProfF& F) {
+	F.E0 = (P.g[0] - 1.0) * R.lr0;
+	F.e0 = S.Y0 * F.E0;
+	F.e1 = S.Y1 * ((P.g[1] - 1.0) * R.lr1);
+}
+SPB_HD bool prof_rhs_fused(const Phys& P, const Prof& S, const ProfF& F, double v, const double* y, double* dy) {
+	const double p = y[0], v1 = y[1];
+	const double v2 = fma(-S.Y0, v1, v) * F.rY1;
+	const double Pr = fma(S.j2, S.v0 - v, S.p0), dp = p - Pr;
+	const bool adm = eos_ok(P, 0, p, v1) && eos_ok(P, 1, p, v2);
+	const double n0 = v1 - P.b[0], n1 = v2 - P.b[1];
+	const double x0 = P.g[0] * (p + P.pi[0]), x1 = P.g[1] * (p + P.pi[1]);
+	const double den = fma(S.Y0 * n0, x1, (S.Y1 * n1) * x0);
+	const double num = fma(dp, fma(F.e0, x1, F.e1 * x0), -(x0 * x1));
+	const double D = 1.0 / (den * x0);
+	dy[0] = (num * x0) * D;
+	dy[1] = fma(F.E0 * dp, den, -(n0 * num)) * D;
+	dy[2] = dp;
+	return adm && isfin(dy[0]) && isfin(dy[1]);
+}
+// one pass of the fused integration loop, with prof_step's return codes (0 not done, 1 reached v_end, -1 failed) and
+// its step control (the same factors, from the same portable power)
+SPB_HD int prof_step_fused(const Phys& P, const Prof& S, const ProfF& F, ProfRun& R) {
+	constexpr double A1 = 0.2, A2 = 0.3, A3 = 0.6, A4 = 1.0, A5 = 0.875;
+	constexpr double B10 = 0.2;
+	constexpr double B20 = 3.0 / 40, B21 = 9.0 / 40;
+	constexpr double B30 = 0.3, B31 = -0.9, B32 = 1.2;
+	constexpr double B40 = -11.0 / 54, B41 = 2.5, B42 = -70.0 / 27, B43 = 35.0 / 27;
+	constexpr double B50 = 1631.0 / 55296, B51 = 175.0 / 512, B52 = 575.0 / 13824, B53 = 44275.0 / 110592, B54 = 253.0 / 4096;
+	constexpr double C50 = 37.0 / 378, C52 = 250.0 / 621, C53 = 125.0 / 594, C55 = 512.0 / 1771;
+	constexpr double C40 = 2825.0 / 27648, C42 = 18575.0 / 48384, C43 = 13525.0 / 55296, C44 = 277.0 / 14336, C45 = 0.25;
+	if (R.it >= 4000) return -1;
+	R.it++;
+	const double L = S.vend - S.v0;
+	const double y[3] = {R.y0, R.y1, R.y2}, v = R.v;
+	double h = R.h;
+	const bool last = (L < 0 && v + h <= S.vend) || (L > 0 && v + h >= S.vend);
+	if (last) h = S.vend - v;
+	double k0[3], k1[3], k2[3], k3[3], k4[3], k5[3], yt[3];
+	bool ok = prof_rhs_fused(P, S, F, v, y, k0);
+	for (int n = 0; n < 3; n++) yt[n] = fma(h * B10, k0[n], y[n]);
+	ok = prof_rhs_fused(P, S, F, v + A1 * h, yt, k1) && ok;
+	for (int n = 0; n < 3; n++) yt[n] = fma(h * B21, k1[n], fma(h * B20, k0[n], y[n]));
+	ok = prof_rhs_fused(P, S, F, v + A2 * h, yt, k2) && ok;
+	for (int n = 0; n < 3; n++) yt[n] = fma(h * B32, k2[n], fma(h * B31, k1[n], fma(h * B30, k0[n], y[n])));
+	ok = prof_rhs_fused(P, S, F, v + A3 * h, yt, k3) && ok;
+	for (int n = 0; n < 3; n++) yt[n] = fma(h * B43, k3[n], fma(h * B42, k2[n], fma(h * B41, k1[n], fma(h * B40, k0[n], y[n]))));
+	ok = prof_rhs_fused(P, S, F, v + A4 * h, yt, k4) && ok;
+	for (int n = 0; n < 3; n++)
+		yt[n] = fma(h * B54, k4[n], fma(h * B53, k3[n], fma(h * B52, k2[n], fma(h * B51, k1[n], fma(h * B50, k0[n], y[n])))));
+	ok = prof_rhs_fused(P, S, F, v + A5 * h, yt, k5) && ok;
+	const double rs[3] = {F.rs0, F.rs1, F.rs2};
+	double err = 0.0, y5[3];
+	for (int n = 0; n < 3; n++) {
+		const double d5 = fma(C55, k5[n], fma(C53, k3[n], fma(C52, k2[n], C50 * k0[n])));
+		const double d4 = fma(C45, k5[n], fma(C44, k4[n], fma(C43, k3[n], fma(C42, k2[n], C40 * k0[n]))));
+		y5[n] = fma(h, d5, y[n]); err = smax(err, fabs(h * (d5 - d4)) * rs[n]);
+	}
+	const bool acc = ok && (err <= 1e-8);
+	const double pw = ppow(acc ? 1e-8 / smax(err, 1e-30) : (ok ? 1e-8 / err : 1.0), acc ? 0.2 : (ok ? 0.25 : 0.0));
+	if (!acc) {
+		h *= ok ? smax(0.1, 0.9 * pw) : 0.25;
+		R.h = h;
+		return fabs(h) < 1e-14 * fabs(L) ? -1 : 0;
+	}
+	R.y0 = y5[0]; R.y1 = y5[1]; R.y2 = y5[2];
+	if (last) return 1;
+	R.v = v + h;
+	R.h = h * smin(5.0, 0.9 * pw);
+	return 0;
+}
 // the share's logistic parameter t -> c = 1 / (1 + exp(-t))
 SPB_HD double share_c(double t) { return 1.0 / (1.0 + pexp(-t)); }
 // phase shares lam and mixture dissipation q of a shock from (p0, vpre, Y) to p1 (PHY:343-383, PHO start bracket),
@@ -522,6 +611,55 @@ SPB_HD double shoot_next(Shoot& H, double f, double q) {
 	if (!(H.t > H.ta && H.t < H.tb)) H.t = 0.5 * (H.ta + H.tb);
 	H.phase = SH_ILL;
 	return H.t;
+}
+// shoot_next as straight-line code with selects, for the strong-shock kernel, where the lanes of a wavefront are in
+// different phases: each lane evaluates every case's arithmetic once and keeps the fields of its own case, so a
+// wavefront pays for one pass and one regula falsi quotient instead of one per phase present and the state is not
+// copied between branches. Field by field the same values as shoot_next for every state and input (the regula falsi
+// point is taken from the bracket that the lane's case continues from: the one just expanded, or the Illinois update).
+SPB_HD double shoot_next_sel(Shoot& H, double f, double q) {
+	const int ph = H.phase;
+	if (ph < SH_FA || ph >= SH_DONE) return 0.0;
+	const bool isFA = ph == SH_FA, isFB = ph == SH_FB, isEL = ph == SH_EXPL, isER = ph == SH_EXPR, isIL = ph == SH_ILL;
+	// F and q go to the bracket end the phase evaluated
+	const double fa = (isFA || isEL) ? f : H.fa, qa = (isFA || isEL) ? q : H.qa;
+	const double fb = (isFB || isER) ? f : H.fb, qb = (isFB || isER) ? q : H.qb;
+	const bool fail = (isFB && !(isfin(fa) && isfin(fb))) || (isEL && !isfin(fa)) || (isER && !isfin(fb)) || (isIL && !isfin(f));
+	// bracket expansion (FB, EXPL, EXPR): L1 to the left, else R1 to the right, else an end with F of the right sign
+	// (D1, D2), else the Illinois start
+	const double st1 = isFB ? 2.0 : H.st * 2.0;
+	const bool L1 = !isER && fa >= 0 && H.ta > -40.0;
+	const double st2 = (!isER && !L1) ? 2.0 : st1;
+	const bool R1 = !L1 && fb <= 0 && H.tb < 40.0;
+	const bool D1 = !L1 && !R1 && fa >= 0, D2 = !L1 && !R1 && !D1 && fb <= 0;
+	// Illinois update (ILL)
+	const bool neg = f < 0, conv = fabs(f) < 1e-10 || H.tb - H.ta < 1e-10;
+	const double fai = neg ? f : (H.side == 1 ? H.fa * 0.5 : H.fa), fbi = neg ? (H.side == -1 ? H.fb * 0.5 : H.fb) : f;
+	const double tai = neg ? H.t : H.ta, tbi = neg ? H.tb : H.t;
+	const int iti = H.it + 1;
+	const double A = isIL ? tai : H.ta, B = isIL ? tbi : H.tb, FA = isIL ? fai : fa, FB = isIL ? fbi : fb;
+	double tn = (A * FB - B * FA) / (FB - FA);
+	if (!(tn > A && tn < B)) tn = 0.5 * (A + B);
+	const bool ex = (isFB || isEL || isER) && !fail, il = isIL && !fail, ilc = il && !conv;
+	const bool I1 = ex && !L1 && !R1 && !D1 && !D2, ilgo = ilc && iti < 100;
+	double ret = isFA ? H.tb : 0.0;
+	int nph = isFA ? SH_FB : (fail ? SH_FAIL : ph);
+	if (ex) nph = L1 ? SH_EXPL : R1 ? SH_EXPR : (D1 || D2) ? SH_DONE : SH_ILL;
+	if (il) nph = ilgo ? SH_ILL : SH_DONE;
+	const double nta = (ex && L1) ? smax(-40.0, H.ta - st1) : (ex && R1) ? H.tb : ilc ? tai : H.ta;
+	const double ntb = (ex && L1) ? H.ta : (ex && R1) ? smin(40.0, H.tb + st2) : ilc ? tbi : H.tb;
+	ret = (ex && L1) ? nta : (ex && R1) ? ntb : (I1 || ilgo) ? tn : ret;
+	const double nt = (ex && D1) ? H.ta : (ex && D2) ? H.tb : (I1 || ilgo) ? tn : H.t;
+	const double nqt = (ex && D1) ? qa : (ex && D2) ? qb : il ? q : H.qt;
+	H.fa = (ex && R1) ? fb : ilc ? fai : fa;
+	H.qa = (ex && R1) ? qb : qa;
+	H.fb = (ex && L1) ? fa : ilc ? fbi : fb;
+	H.qb = (ex && L1) ? qa : qb;
+	H.st = ex ? (L1 ? st1 : st2) : H.st;
+	H.side = I1 ? 0 : ilc ? (neg ? -1 : 1) : H.side;
+	H.it = I1 ? 0 : ilc ? iti : H.it;
+	H.ta = nta; H.tb = ntb; H.t = nt; H.qt = nqt; H.phase = nph;
+	return ret;
 }
 // the share of a finished shooting (PHY:380-382): lam = (c, 1 - c), q; false without a share
 SPB_HD bool shoot_result(const Shoot& H, double lam[2], double& q) {

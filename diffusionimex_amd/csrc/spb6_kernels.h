@@ -71,7 +71,7 @@ void launch_prep(const Dev& D, int limiter, int order, double tau, const double*
 // SHOCK_REC * n doubles (the strong cells' profile problems), all device scratch; rk_steps accumulates the strong
 // pass's RK45 steps (its work count); strong_blocks: persistent 256-thread blocks of the strong-shock pass (as many as
 // are resident at once)
-void launch_shock(const Dev& D, int* list, int* count, double* rec, unsigned long long* rk_steps, int strong_blocks, hipStream_t st);
+void launch_shock(const Dev& D, int* list, int* count, double* rec, unsigned long long* rk_steps, int strong_blocks, int fused, hipStream_t st);
 // 256-thread blocks of the strong-shock pass resident on one compute unit at once
 int shock_strong_blocks_per_cu();
 // (gate: launched speculatively, the evaluation runs only where *gate is set)

@@ -335,83 +335,110 @@ __global__ void __launch_bounds__(256) k_shock_classify(Geo g, Phys P, const dou
 	}
 }
 // second pass: persistent lanes, each running one strong cell's shooting at a time. A shooting is a sequence of
-// adaptive RK45 profile integrations (share_strong's loop) whose lengths vary several-fold from cell to cell (12 to
-// ~300 RK steps per cell on a cavitating flow), so a lane advances its integration by one step per pass of the loop,
-// moves its shooting on when an integration ends, and takes the next cell when its cell is done: the lanes of a
-// wavefront stay busy until the list is empty instead of waiting for the longest cell of a fixed group. A wavefront
-// takes cells from a chunk of SHOCK_CHUNK consecutive list slots and claims the next chunk with one atomic on the
-// cursor (an atomic per refill, from every wavefront on the one cursor, was measured to serialise: 2.5x slower). Per pass, the lanes that
-// start an integration (a new cell's first, a shooting's next) or finish a cell share one tail (the logistic of the
-// shooting parameter, then the integration's start or the cell's share), so the divergent work per pass stays small.
-// Every cell runs the same operations as share_strong, so the result does not depend on the order or the launch
-// shape. Every lane's work is bounded (4000 steps per integration, 100 Illinois iterations), so every wavefront
-// reaches the exit.
+// adaptive RK45 profile integrations (share_strong's loop) whose lengths vary from cell to cell (on a cavitating flow
+// ~10 RK steps per integration and ~3.6 integrations per cell on average, up to ~300 steps per cell), so a lane
+// advances its integration by one step per pass of the loop and the lanes of a wavefront stay busy until the list is
+// empty instead of waiting for the longest cell of a fixed group. Between integrations a lane needs service: the
+// shooting's next point (shoot_next_sel), the logistic of the shooting parameter, the next integration's start or the
+// cell's share, and a new cell when its cell is done. Service runs for all waiting lanes of the wavefront together,
+// in a pass where at least SHOCK_SERVE lanes wait (or no lane is integrating), and the lanes it starts take their
+// first step in the same pass: each pass then carries one RK step for most lanes, and the service's divergent code
+// runs every few passes for many lanes instead of every pass for a few. A wavefront takes cells from a chunk of
+// SHOCK_CHUNK consecutive list slots and claims the next chunk with one atomic on the cursor (an atomic per refill,
+// from every wavefront on the one cursor, was measured to serialise: 2.5x slower). The shooting state, read only at
+// service, lives in LDS. FUSED takes the profile integration in its fused form (prof_step_fused, one division per
+// right-hand side). Every cell runs the same operations as share_strong (or, FUSED, as share_strong with the fused
+// integration), so the result does not depend on the order or the launch shape. Every lane's work is bounded (4000 steps per integration, 100 Illinois iterations), so
+// every wavefront reaches the exit.
 constexpr int SHOCK_CHUNK = 128;
+constexpr int SHOCK_SERVE = 16;
+template <bool FUSED>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_shock_strong(Geo g, Phys P, const double* __restrict__ rec, long cap,
                                                      double* __restrict__ Qph, const int* __restrict__ list, const int* __restrict__ count,
                                                      int* __restrict__ cursor, unsigned long long* __restrict__ rk_steps) {
+	__shared__ Shoot Hs[256];
+	__shared__ double vH0s[256], rhos[256];
+	__shared__ int slots[256];
+	Shoot& H = Hs[threadIdx.x];
+	double& vH0 = vH0s[threadIdx.x];
+	double& rho = rhos[threadIdx.x];
+	int& slot = slots[threadIdx.x];
 	const int m = *count;
 	const int lane = threadIdx.x & 63;
-	bool active = false, more = true;
-	int slot = 0;
-	int qn = 0, qe = 0;   // the wavefront's chunk of list slots not yet taken, [qn, qe)
-	int nsteps = 0;       // RK45 steps this lane took (added to *rk_steps at the exit: the pass's work count)
+	bool more = true;    // list slots are left (wavefront-uniform, as are qn and qe)
+	bool has = false;    // the lane holds a cell whose shooting is not finished
+	bool run = false;    // its profile integration is in progress
+	bool ended = false;  // the integration has ended with r_end and the shooting has not taken the result yet
+	int r_end = 0;
+	int qn = 0, qe = 0;  // the wavefront's chunk of list slots not yet taken, [qn, qe)
+	int nsteps = 0;      // RK45 steps this lane took (added to *rk_steps at the exit: the pass's work count)
 	const ShockRcp rc = shock_rcp(P);
-	Prof S; double vH0 = 0.0, rho = 0.0; Shoot H; ProfRun R;
+	Prof S; ProfRun R; ProfF F;
 	for (;;) {
-		bool tail = false;   // this lane starts an integration or finishes its cell in this pass
-		double t = 0.0;
-		if (more) {   // refill the idle lanes (more, qn and qe are wavefront-uniform)
-			const unsigned long long idle = __ballot(!active);
-			if (idle) {
-				if (qn >= qe) {
-					int nb = 0;
-					if (lane == 0) nb = atomicAdd(cursor, SHOCK_CHUNK);
-					nb = __shfl(nb, 0, 64);
-					if (nb >= m) more = false;
-					else { qn = nb; qe = min(m, nb + SHOCK_CHUNK); }
+		const unsigned long long running = __ballot(run), waiting = __ballot(ended || (!has && more));
+		if (running == 0ull && waiting == 0ull) break;
+		if (waiting != 0ull && (running == 0ull || __popcll(waiting) >= SHOCK_SERVE)) {
+			double t = 0.0;
+			if (ended) { t = shoot_next_sel(H, share_miss(R, r_end, vH0, S.vk00), R.y2); ended = false; }
+			if (has && !run) {   // the shooting's next integration, or its end: the share (shoot_result: lam = (c, 1 - c), q)
+				const bool fin = H.phase >= SH_DONE;
+				const double c = share_c(fin ? H.t : t);
+				if (!fin) {
+					prof_begin(S, c, R);
+					if (FUSED) proff_begin(P, S, R, F);
+					run = true;
+				} else {
+					double Q0 = 0.0, Q1 = 0.0;
+					if (H.phase == SH_DONE && H.qt > 0) { Q0 = c * rho * H.qt; Q1 = (1.0 - c) * rho * H.qt; }
+					const int q = list[slot], cc = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
+					Qph[cc] = Q0; Qph[g.NC + cc] = Q1;
+					has = false;
 				}
-				if (qn < qe) {
-					const int avail = qe - qn, rank = __popcll(idle & ((1ull << lane) - 1ull));
-					if (!active && rank < avail) {
-						const int i = qn + rank;
-						S.Y0 = rec[i]; S.Y1 = rec[cap + i]; S.vk00 = rec[2 * cap + i]; S.p0 = rec[3 * cap + i]; S.j2 = rec[4 * cap + i];
-						S.v0 = rec[5 * cap + i]; S.vend = rec[6 * cap + i]; vH0 = rec[7 * cap + i]; rho = rec[8 * cap + i];
-						t = shoot_begin(H, rec[9 * cap + i]);
-						slot = i; active = true; tail = true;
+			}
+			if (more) {   // the idle lanes take the next cells of the chunk
+				const unsigned long long idle = __ballot(!has);
+				bool fresh = false;
+				if (idle) {
+					if (qn >= qe) {
+						int nb = 0;
+						if (lane == 0) nb = atomicAdd(cursor, SHOCK_CHUNK);
+						nb = __shfl(nb, 0, 64);
+						if (nb >= m) more = false;
+						else { qn = nb; qe = min(m, nb + SHOCK_CHUNK); }
 					}
-					qn += min(__popcll(idle), avail);
+					if (qn < qe) {
+						const int avail = qe - qn, rank = __popcll(idle & ((1ull << lane) - 1ull));
+						if (!has && rank < avail) {
+							const int i = qn + rank;
+							S.Y0 = rec[i]; S.Y1 = rec[cap + i]; S.vk00 = rec[2 * cap + i]; S.p0 = rec[3 * cap + i]; S.j2 = rec[4 * cap + i];
+							S.v0 = rec[5 * cap + i]; S.vend = rec[6 * cap + i]; vH0 = rec[7 * cap + i]; rho = rec[8 * cap + i];
+							t = shoot_begin(H, rec[9 * cap + i]);
+							slot = i; has = true; fresh = true;
+						}
+						qn += min(__popcll(idle), avail);
+					}
+				}
+				if (fresh) {
+					prof_begin(S, share_c(t), R);
+					if (FUSED) { proff_cell(S, F); proff_begin(P, S, R, F); }
+					run = true;
 				}
 			}
 		}
-		if (__ballot(active) == 0ull) {
-			if (!more) {
-				#pragma unroll
-				for (int off = 32; off >= 1; off >>= 1) nsteps += __shfl_xor(nsteps, off, 64);
-				if (lane == 0) atomicAdd(rk_steps, (unsigned long long)nsteps);
-				break;
-			}
-			continue;
-		}
-		if (active && !tail) {
+		if (run) {
 			nsteps++;
-			int r = prof_step<true>(P, S, R, &rc);
-			if (r == 2) r = prof_step(P, S, R);   // (wavefront-uniform) a quotient's operand out of the shortcut's range
-			if (r != 0) { t = shoot_next(H, share_miss(R, r, vH0, S.vk00), R.y2); tail = true; }
-		}
-		if (tail) {
-			const bool fin = H.phase >= SH_DONE;
-			const double c = share_c(fin ? H.t : t);
-			if (!fin) prof_begin(S, c, R);
-			else {   // the share of the finished shooting (shoot_result: lam = (c, 1 - c), q)
-				double Q0 = 0.0, Q1 = 0.0;
-				if (H.phase == SH_DONE && H.qt > 0) { Q0 = c * rho * H.qt; Q1 = (1.0 - c) * rho * H.qt; }
-				const int q = list[slot], cc = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
-				Qph[cc] = Q0; Qph[g.NC + cc] = Q1;
-				active = false;
+			int r;
+			if (FUSED) r = prof_step_fused(P, S, F, R);
+			else {
+				r = prof_step<true>(P, S, R, &rc);
+				if (r == 2) r = prof_step(P, S, R);   // (wavefront-uniform) a quotient's operand out of the shortcut's range
 			}
+			if (r != 0) { run = false; ended = true; r_end = r; }
 		}
 	}
+	#pragma unroll
+	for (int off = 32; off >= 1; off >>= 1) nsteps += __shfl_xor(nsteps, off, 64);
+	if (lane == 0) atomicAdd(rk_steps, (unsigned long long)nsteps);
 }
 
 // ------------------------------------------------------------------------------------------------ cell residual (PC:584-705)
@@ -1234,7 +1261,7 @@ void launch_prep(const Dev& D, int limiter, int order, double tau, const double*
 }
 int shock_strong_blocks_per_cu() {
 	int b = 0;
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(k_shock_strong), 256, 0) != hipSuccess || b < 1) b = 2;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void*>(k_shock_strong<false>), 256, 0) != hipSuccess || b < 1) b = 2;
 	return b;
 }
 // zero n ints (a kernel, so that it can be captured in a graph, where memset nodes are avoided)
@@ -1244,11 +1271,14 @@ __global__ void __launch_bounds__(256) k_zero_ints(int* __restrict__ p, long n) 
 void launch_zero_ints(int* p, long n, hipStream_t st) {
 	hipLaunchKernelGGL(k_zero_ints, dim3(std::max(1, std::min(nblk(n), 64))), dim3(256), 0, st, p, n);
 }
-void launch_shock(const Dev& D, int* list, int* count, double* rec, unsigned long long* rk_steps, int strong_blocks, hipStream_t st) {
+void launch_shock(const Dev& D, int* list, int* count, double* rec, unsigned long long* rk_steps, int strong_blocks, int fused, hipStream_t st) {
 	launch_zero_ints(count, 2, st);   // the count and the strong pass's cursor
 	hipLaunchKernelGGL(k_shock_classify, dim3(nblk(D.g.n)), dim3(256), 0, st, D.g, D.ph, D.Ust, D.Qph, list, count, rec, (long)D.g.n);
-	hipLaunchKernelGGL(k_shock_strong, dim3(std::max(1, std::min(nblk(D.g.n), strong_blocks))), dim3(256), 0, st, D.g, D.ph,
-	                   (const double*)rec, (long)D.g.n, D.Qph, (const int*)list, (const int*)count, count + 1, rk_steps);
+	const dim3 grid(std::max(1, std::min(nblk(D.g.n), strong_blocks)));
+	if (fused) hipLaunchKernelGGL(k_shock_strong<true>, grid, dim3(256), 0, st, D.g, D.ph, (const double*)rec, (long)D.g.n, D.Qph,
+	                              (const int*)list, (const int*)count, count + 1, rk_steps);
+	else hipLaunchKernelGGL(k_shock_strong<false>, grid, dim3(256), 0, st, D.g, D.ph, (const double*)rec, (long)D.g.n, D.Qph,
+	                        (const int*)list, (const int*)count, count + 1, rk_steps);
 }
 void launch_residual(const Dev& D, const double* P, const double* dP, double lam, double* Pout, double* G, unsigned long long* nrm, int flag_slot, hipStream_t st,
                      const int* gate) {

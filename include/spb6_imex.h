@@ -98,6 +98,11 @@ typedef struct spb6_config {
     int graph;                       /* 1: each stage's fixed head (stage operands, explicit part, preparation, shock budget)
                                         and each Krylov chunk (BiCGStab passes; the FP32 form's defect, sweep close and
                                         trial residual) launched as a captured hipGraph; the same kernels, the same bits */
+    int shock_fused;                 /* 1: the strong-shock profile integrations (PHY:302-341) in fused form: the ODE's two
+                                        phase terms over one denominator (one division per right-hand side instead of
+                                        ten) and FMA stage sums; the same equation and step control, rounded differently
+                                        (the shares agree with the reference arithmetic's to ~1e-15 relative). Grid
+                                        stepper only (the batch rejects it) */
 } spb6_config_t;
 
 /* run statistics: the counters of the reference's stats line (PC:1337-1339) plus Krylov counts */

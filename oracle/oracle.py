@@ -28,7 +28,7 @@ class Cfg(C.Structure):
         ("limiter", C.c_int), ("order", C.c_int), ("rus_local", C.c_int),
         ("closure_path", C.c_int), ("share_start", C.c_int), ("dot_order", C.c_int),
         ("portable_math", C.c_int), ("nthreads", C.c_int), ("jreuse", C.c_int), ("variant", C.c_int),
-        ("relax", C.c_int), ("eps_i", C.c_double),
+        ("relax", C.c_int), ("eps_i", C.c_double), ("shock_fused", C.c_int),
     ]
 
 
@@ -43,7 +43,7 @@ class Stats(C.Structure):
 
 class OraModel(C.Structure):
     _fields_ = [("ph", (C.c_double * 6) * 2), ("closure_path", C.c_int), ("share_start", C.c_int),
-                ("dot_order", C.c_int), ("portable_math", C.c_int)]
+                ("dot_order", C.c_int), ("portable_math", C.c_int), ("shock_fused", C.c_int)]
 
 
 def build(force=False):
@@ -103,6 +103,7 @@ def lib():
         L.ora_hug_v.restype = C.c_double
         L.ora_hug_v.argtypes = [C.POINTER(OraModel), C.c_int, C.c_double, C.c_double, C.c_double]
         L.ora_prof_integrate.argtypes = [C.POINTER(OraModel), dp, dp, C.c_double, C.c_double, C.c_double, dp, dp]
+        L.ora_prof_rhs.argtypes = [C.POINTER(OraModel), dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
         L.ora_shock_budget.argtypes = [C.POINTER(OraModel), dp, C.c_int, dp]
         for f in ("ora_pexp", "ora_plog"):
             getattr(L, f).restype = C.c_double
@@ -136,6 +137,7 @@ def default_cfg(nx, ny=1, **kw):
     c.jreuse = 0
     c.variant = 0
     c.relax, c.eps_i = 0, 1e-4
+    c.shock_fused = 0
     for k, v in kw.items():
         if k == "bc":
             for q in range(4):
@@ -158,6 +160,7 @@ def model_of(cfg):
         for q in range(6):
             m.ph[p][q] = cfg.ph[p][q]
     m.closure_path, m.share_start, m.dot_order, m.portable_math = cfg.closure_path, cfg.share_start, cfg.dot_order, cfg.portable_math
+    m.shock_fused = cfg.shock_fused
     return m
 
 

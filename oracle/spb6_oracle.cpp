@@ -111,6 +111,7 @@ struct Phase { double g, pi, q, qp, cv, b; };   // PHY:49 (SG / NASG subset)
 struct Model {
 	Phase PH[2];
 	int closure_path = 1, share_start = 1, dot_order = 2, portable = 1;
+	int shock_fused = 0;   // the strong-shock profile integrations in fused form (prof_integrate_fused)
 	double exp_(double x) const { return portable ? pexp(x) : std::exp(x); }
 	double log_(double x) const { return portable ? plog(x) : std::log(x); }
 	double pow_(double x, double y) const { return portable ? ppow(x, y) : std::pow(x, y); }
@@ -226,6 +227,69 @@ static bool prof_integrate(const Model& M, const ShockProf& S, double c, double&
 	}
 	return false;
 }
+// The fused form of the profile integration (shock_fused = 1), as the strong-shock kernel's prof_step_fused computes
+// it: the ODE of PHY:302-310 with its two phase terms over their common denominator, x_k = g_k (p + pi_k),
+// n_k = v_k - b_k, lr_k = lam_k / Y_k, e_k = Y_k ((g_k - 1) lr_k), E0 = (g_0 - 1) lr_0:
+//     dp/dv = num / den,  num = -x0 x1 + (p - P)(e0 x1 + e1 x0),  den = Y0 n0 x1 + Y1 n1 x0,
+//     dv1/dv = (E0 (p - P) den - n0 num) / (den x0),
+// both from the one correctly rounded division 1 / (den x0); v2 = (v - Y0 v1) (1 / Y1); the stage inputs, the fifth-
+// and fourth-order sums (zero weights left out) and y5 as FMA chains in the Cash-Karp term order; the error norm
+// multiplies by 1 / sc_k. The step control is prof_integrate's. Not the reference's rounding: this form is held to the
+// HIP kernel bit for bit and to prof_rhs / prof_integrate within round-off (tests/test_oracle_shock_fused.py).
+struct ProfFused { double rY1, rs[3], E0, e0, e1; };
+static bool prof_rhs_fused(const Model& M, const ShockProf& S, const ProfFused& F, double v, const double* y, double* dy) {
+	const Phase &A = M.PH[0], &B = M.PH[1];
+	const double p = y[0], v1 = y[1], v2 = std::fma(-S.Y[0], v1, v) * F.rY1;
+	const double P = std::fma(S.j2, S.v0 - v, S.p0), dp = p - P;
+	const bool adm = eos_ok(M, 0, p, v1) && eos_ok(M, 1, p, v2);
+	const double n0 = v1 - A.b, n1 = v2 - B.b, x0 = A.g * (p + A.pi), x1 = B.g * (p + B.pi);
+	const double den = std::fma(S.Y[0] * n0, x1, (S.Y[1] * n1) * x0);
+	const double num = std::fma(dp, std::fma(F.e0, x1, F.e1 * x0), -(x0 * x1));
+	const double D = 1.0 / (den * x0);
+	dy[0] = (num * x0) * D; dy[1] = std::fma(F.E0 * dp, den, -(n0 * num)) * D; dy[2] = dp;
+	return adm && std::isfinite(dy[0]) && std::isfinite(dy[1]);
+}
+static void prof_fused_setup(const Model& M, const ShockProf& S, double c, ProfFused& F) {
+	const double L = S.vend - S.v0;
+	F.rY1 = 1.0 / S.Y[1];
+	F.rs[0] = 1.0 / (std::fabs(S.p0) + S.j2 * std::fabs(L)); F.rs[1] = 1.0 / S.vk0[0]; F.rs[2] = 1.0 / (S.j2 * L * L + 1e-300);
+	const double lr0 = c / S.Y[0], lr1 = (1.0 - c) / S.Y[1];
+	F.E0 = (M.PH[0].g - 1.0) * lr0; F.e0 = S.Y[0] * F.E0; F.e1 = S.Y[1] * ((M.PH[1].g - 1.0) * lr1);
+}
+static bool prof_integrate_fused(const Model& M, const ShockProf& S, double c, double& v1end, double& qend) {
+	ProfFused F; prof_fused_setup(M, S, c, F);
+	double y[3] = {S.p0, S.vk0[0], 0.0}, v = S.v0;
+	const double L = S.vend - S.v0;
+	double h = L / 32.0;
+	static const int W5[3] = {2, 3, 5}, W4[4] = {2, 3, 4, 5};   // the nonzero weights after the first
+	for (int it = 0; it < 4000; it++) {
+		const bool last = (L < 0 && v + h <= S.vend) || (L > 0 && v + h >= S.vend);
+		if (last) h = S.vend - v;
+		double k[6][3], yt[3];
+		bool ok = true;
+		for (int st = 0; st < 6; st++) {
+			for (int n = 0; n < 3; n++) { yt[n] = y[n]; for (int q = 0; q < st; q++) yt[n] = std::fma(h * CK_B[st][q], k[q][n], yt[n]); }
+			ok = prof_rhs_fused(M, S, F, v + CK_A[st] * h, yt, k[st]) && ok;
+		}
+		double err = 0.0, y5[3];
+		for (int n = 0; n < 3; n++) {
+			double d5 = CK_C5[0] * k[0][n], d4 = CK_C4[0] * k[0][n];
+			for (int st : W5) d5 = std::fma(CK_C5[st], k[st][n], d5);
+			for (int st : W4) d4 = std::fma(CK_C4[st], k[st][n], d4);
+			y5[n] = std::fma(h, d5, y[n]); err = smax(err, std::fabs(h * (d5 - d4)) * F.rs[n]);
+		}
+		if (!ok || !(err <= 1e-8)) {
+			h *= ok ? smax(0.1, 0.9 * M.pow_(1e-8 / err, 0.25)) : 0.25;
+			if (std::fabs(h) < 1e-14 * std::fabs(L)) return false;
+			continue;
+		}
+		for (int n = 0; n < 3; n++) y[n] = y5[n];
+		if (last) { v1end = y[1]; qend = y[2]; return true; }
+		v += h;
+		h *= smin(5.0, 0.9 * M.pow_(1e-8 / smax(err, 1e-30), 0.2));
+	}
+	return false;
+}
 static bool shock_share(const Model& M, const double Y[2], const double vpre[2], double p0, double p1, double lam[2], double& q) {   // PHY:343-383, PHO:365-374
 	double K[2], G[2], w[2], sw = 0.0, Kmin = INF;
 	for (int k = 0; k < 2; k++) { K_G(M, k, p0, vpre[k], K[k], G[k]); if (!(K[k] > 0)) return false; Kmin = smin(Kmin, K[k]); }
@@ -245,7 +309,7 @@ static bool shock_share(const Model& M, const double Y[2], const double vpre[2],
 	S.j2 = dp / (S.v0 - S.vend);
 	auto F = [&](double t, double& qq) -> double {
 		const double c = 1.0 / (1.0 + M.exp_(-t)); double v1e;
-		if (!prof_integrate(M, S, c, v1e, qq)) return NAN;
+		if (!(M.shock_fused ? prof_integrate_fused(M, S, c, v1e, qq) : prof_integrate(M, S, c, v1e, qq))) return NAN;
 		return (v1e - vH[0]) / vpre[0]; };
 	double ta, tb, qa, qb, fa, fb;
 	if (M.share_start == 0) { ta = -40.0; tb = 40.0; fa = F(ta, qa); fb = F(tb, qb); if (!std::isfinite(fa) || !std::isfinite(fb)) return false; }
@@ -563,6 +627,7 @@ struct Cfg {
 	int jreuse;   // JREUSE (PCO:83): modified Newton, the Jacobian rebuilt only as PCO:826-828 decides
 	int variant;  // IMEX variant: 0 PC (SPB6_PC_IMEX.cpp), 1 TV, 2 TVpure, 3 TVinc, 4 TVsplit, 5 TVbb, 6 BB (SPB6_<name>_IMEX.cpp)
 	int relax; double eps_i;   // RELAX_LEVEL (PHY:37; 0 p, 1 p-pT) and EPS_I (PHY:38)
+	int shock_fused;           // 1: the strong-shock profile integrations in fused form (prof_integrate_fused)
 };
 struct Stats { long steps, rk1, rejects, floor_acc, stages, newton_it, krylov_it, linsolves; double maxres, t, dt_last; int status;
                long jac_builds; };
@@ -596,6 +661,7 @@ static void init_ctx(Ctx& X, const Cfg& C) {
 	X.C = C;
 	for (int k = 0; k < 2; k++) X.M.PH[k] = Phase{C.ph[k][0], C.ph[k][1], C.ph[k][2], C.ph[k][3], C.ph[k][4], C.ph[k][5]};
 	X.M.closure_path = C.closure_path; X.M.share_start = C.share_start; X.M.dot_order = C.dot_order; X.M.portable = C.portable_math;
+	X.M.shock_fused = C.shock_fused;
 	X.nx = C.nx; X.ny = C.ny; X.NXT = C.nx + 2 * GH; X.NYT = C.ny + 2 * GH;
 	X.dx = (C.x1 - C.x0) / C.nx; X.dy = (C.y1 - C.y0) / C.ny; X.twoD = C.ny > 1;
 	const size_t nc = (size_t)X.NXT * X.NYT;
@@ -1635,10 +1701,11 @@ static void cell_diag(const Model& M, const double* Q, double* o) {             
 using namespace ora;
 extern "C" {
 
-typedef struct { double ph[2][6]; int closure_path, share_start, dot_order, portable_math; } OraModel;
+typedef struct { double ph[2][6]; int closure_path, share_start, dot_order, portable_math, shock_fused; } OraModel;
 static Model mk_model(const OraModel* m) {
 	Model M; for (int k = 0; k < 2; k++) M.PH[k] = Phase{m->ph[k][0], m->ph[k][1], m->ph[k][2], m->ph[k][3], m->ph[k][4], m->ph[k][5]};
-	M.closure_path = m->closure_path; M.share_start = m->share_start; M.dot_order = m->dot_order; M.portable = m->portable_math; return M;
+	M.closure_path = m->closure_path; M.share_start = m->share_start; M.dot_order = m->dot_order; M.portable = m->portable_math;
+	M.shock_fused = m->shock_fused; return M;
 }
 
 int ora_version(void) { return 1; }
@@ -1793,7 +1860,16 @@ int ora_prof_integrate(const OraModel* m, const double* Y, const double* vpre, d
 	const Model M = mk_model(m); ShockProf S; S.p0 = p0; double vH[2];
 	for (int k = 0; k < 2; k++) { S.Y[k] = Y[k]; S.vk0[k] = vpre[k]; vH[k] = hug_v(M, k, p0, p1, vpre[k]); }
 	S.v0 = Y[0] * vpre[0] + Y[1] * vpre[1]; S.vend = Y[0] * vH[0] + Y[1] * vH[1]; S.j2 = (p1 - p0) / (S.v0 - S.vend);
-	return prof_integrate(M, S, c, *v1end, *qend) ? 1 : 0;
+	return (M.shock_fused ? prof_integrate_fused(M, S, c, *v1end, *qend) : prof_integrate(M, S, c, *v1end, *qend)) ? 1 : 0;
+}
+// the profile right-hand side at (v, y) of the problem ora_prof_integrate sets up, in the model's form; returns its flag
+int ora_prof_rhs(const OraModel* m, const double* Y, const double* vpre, double p0, double p1, double c, double v, const double* y, double* dy) {
+	const Model M = mk_model(m); ShockProf S; S.p0 = p0; double vH[2];
+	for (int k = 0; k < 2; k++) { S.Y[k] = Y[k]; S.vk0[k] = vpre[k]; vH[k] = hug_v(M, k, p0, p1, vpre[k]); }
+	S.v0 = Y[0] * vpre[0] + Y[1] * vpre[1]; S.vend = Y[0] * vH[0] + Y[1] * vH[1]; S.j2 = (p1 - p0) / (S.v0 - S.vend);
+	if (!M.shock_fused) return prof_rhs(M, S, c, v, y, dy) ? 1 : 0;
+	ProfFused F; prof_fused_setup(M, S, c, F);
+	return prof_rhs_fused(M, S, F, v, y, dy) ? 1 : 0;
 }
 // shock budget of one cell from its cross stencil: cells[5*8] = centre, W, E, S, N (S/N ignored when twoD = 0)
 void ora_shock_budget(const OraModel* m, const double* cells, int twoD, double* Q) {

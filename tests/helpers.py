@@ -12,7 +12,8 @@ def config_from_oracle(ocfg, **over):
                     rk=ocfg.rk, cfl=ocfg.cfl, cfl_ac=ocfg.cfl_ac, ramp=ocfg.ramp, dtfix=ocfg.dtfix, acdt=ocfg.acdt,
                     newton_tol=ocfg.newton_tol, newton_iter=ocfg.newton_iter, krylov_tol=ocfg.krylov_tol,
                     krylov_iter=ocfg.krylov_iter, limiter=ocfg.limiter, order=ocfg.order, rus_local=ocfg.rus_local,
-                    jreuse=ocfg.jreuse, variant=ocfg.variant, relax=ocfg.relax, eps_i=ocfg.eps_i)
+                    jreuse=ocfg.jreuse, variant=ocfg.variant, relax=ocfg.relax, eps_i=ocfg.eps_i,
+                    shock_fused=ocfg.shock_fused)
     for k, v in over.items():
         setattr(c, k, v)
     return c
@@ -24,7 +25,7 @@ def oracle_cfg_from(scfg, O, **over):
     c = O.default_cfg(scfg.nx, scfg.ny, bc=list(scfg.bc), bcu=list(scfg.bcu), ph=ph)
     c.x0, c.x1, c.y0, c.y1 = scfg.x0, scfg.x1, scfg.y0, scfg.y1
     for k in ("rk", "cfl", "cfl_ac", "ramp", "dtfix", "acdt", "newton_tol", "newton_iter", "krylov_tol", "krylov_iter",
-              "limiter", "order", "rus_local", "jreuse", "variant", "relax"):
+              "limiter", "order", "rus_local", "jreuse", "variant", "relax", "shock_fused"):
         setattr(c, k, getattr(scfg, k))
     c.eps_i = scfg.eps_i if scfg.eps_i > 0 else 1e-4
     for k, v in over.items():

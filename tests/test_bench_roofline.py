@@ -35,7 +35,7 @@ def test_hbm_roofline_uses_the_live_measure_and_the_iterations_taken():
     live = _kt(krylov=(1.59, 200))   # split pairs count in krylov_n; the taken iterations come from the step stats
     res = {"kt": full, "kt_live": live, "dom": "krylov_iter", "wall": 0.0713,
            "d": {"krylov_it": 20, "stages": 20, "newton_it": 53, "steps": 10},
-           "d_all": {"krylov_it": 186, "stages": 20, "newton_it": 53, "steps": 10}}
+           "d_all": {"krylov_it": 186, "stages": 20, "newton_it": 53, "steps": 10}, "fused": 1}
     roof, kernel_ms, frac = bench.roofline_of(res, n, True)
     avg = 1.59e-3 / 20
     assert roof["kernel"] == "krylov_iter" and roof["bound"] == "hbm" and abs(roof["avg_launch_ms"] - avg * 1e3) < 1e-12
@@ -48,15 +48,20 @@ def test_hbm_roofline_uses_the_live_measure_and_the_iterations_taken():
     assert kernel_ms["krylov_iter"] == 14.4 and 0 < frac < 1
 
 
-def test_shock_roofline_counts_fp64_operations_per_rk_step():
+import pytest  # noqa: E402
+
+
+@pytest.mark.parametrize("fused,ops", [(0, 555), (1, 456)])
+def test_shock_roofline_counts_fp64_operations_per_rk_step(fused, ops):
     n = 4096 * 4096
     full = _kt(explicit=(4.0, 2), shock=(50.0, 2), residual=(9.0, 7), shock_rk_steps=600_000_000)
     live = _kt(shock=(100.0, 4), shock_rk_steps=1_300_000_000)
     res = {"kt": full, "kt_live": live, "dom": "shock", "wall": 0.5,
            "d": {"krylov_it": 10, "stages": 8, "newton_it": 20, "steps": 4},
-           "d_all": {"krylov_it": 5, "stages": 2, "newton_it": 5, "steps": 1}}
+           "d_all": {"krylov_it": 5, "stages": 2, "newton_it": 5, "steps": 1}, "fused": fused}
     roof, _, _ = bench.roofline_of(res, n, True)
     assert roof["bound"] == "fp64" and roof["kernel"] == "shock" and roof["unit"] == "TFLOP/s"
-    achieved = 1_300_000_000 / 4 * bench.SHOCK_RK_FLOPS / (100.0e-3 / 4)
+    assert bench.SHOCK_RK_FLOPS[fused] == ops
+    achieved = 1_300_000_000 / 4 * ops / (100.0e-3 / 4)
     assert abs(roof["achieved"] - achieved / 1e12) < 1e-9 and abs(roof["frac"] - achieved / bench.FP64_PEAK) < 1e-12
-    assert abs(roof["by_kernel"]["shock_fp64"] - 600_000_000 * bench.SHOCK_RK_FLOPS / 0.05 / bench.FP64_PEAK) < 1e-4
+    assert abs(roof["by_kernel"]["shock_fp64"] - 600_000_000 * ops / 0.05 / bench.FP64_PEAK) < 1e-4
