@@ -84,6 +84,9 @@ def lib():
         L.ora_hllc_closure_fails.restype = C.c_long
         L.ora_hllc_step.argtypes = [vp, C.c_double]
         L.ora_relax_activate.argtypes = [vp]
+        L.ora_tsat.restype = C.c_double
+        L.ora_tsat.argtypes = [vp, C.c_double]
+        L.ora_sat_info.argtypes = [vp, dp]
         L.ora_residual_all.argtypes = [vp, dp, dp]
         L.ora_jacobian.argtypes = [vp, dp, dp, dp, dp]
         L.ora_linsolve.argtypes = [vp, dp, dp, dp, C.c_int, ip]

@@ -108,8 +108,12 @@ static double phypot(double a, double b) { return std::sqrt(a * a + b * b); }
 // ------------------------------------------------------------------------------------------------ options / model
 struct Phase { double g, pi, q, qp, cv, b; };   // PHY:49 (SG / NASG subset)
 
+// the saturation line T_sat(p) of the two phases (PHY:111-163): ln p on a uniform grid and the root T there
+struct SatTable { std::vector<double> lp, T; double pcrit = std::numeric_limits<double>::infinity(), vcrit = 0.0, pmin = 0.0; };
+
 struct Model {
 	Phase PH[2];
+	const SatTable* sat = nullptr;   // built for relaxation levels 2-3 (T_sat of the pTg closure)
 	int closure_path = 1, share_start = 1, dot_order = 2, portable = 1;
 	int shock_fused = 0;   // the strong-shock profile integrations in fused form (prof_integrate_fused)
 	double exp_(double x) const { return portable ? pexp(x) : std::exp(x); }
@@ -378,8 +382,9 @@ template <class F> static void shock_budget(const Model& M, F cellptr, bool twoD
 }
 
 // ------------------------------------------------------------------------------------------------ p closure (PHY:420-494, PHO:472-548)
-enum { CL_P = 0, CL_PT = 2 };   // closure types of PHY:401 used here: mechanical (p) and thermal (pT) equilibrium
-struct ClIn { double atr[2], mtr[2], etr[2], s, rhoeC, pguess, Qph[2]; int type = CL_P; };
+enum { CL_P = 0, CL_PT = 2, CL_PTG = 5 };   // closure types of PHY:226 used here: mechanical (p), thermal (pT), chemical (pTg) equilibrium
+// nodelta (PHY:234): the p closure without the energy defect, for the p-pTg activation's prediction only
+struct ClIn { double atr[2], mtr[2], etr[2], s, rhoeC, pguess, Qph[2]; int type = CL_P; bool nodelta = false; };
 struct ClOut { double a[2], m[2], e[2], p[2], T[2], pbar; };
 struct PCoef { double A[2], B[2], N[2], lam[2]; };
 static double E_above(const Model& M, const ClIn& in, int k) {                                     // PHY:420-423 (SG)
@@ -396,10 +401,10 @@ static void defect_split(const Model& M, const ClIn& in, double delta, double la
 	for (int k = 0; k < 2; k++) { lam[k] = lE[k]; extra[k] = in.Qph[k] - Q * lE[k]; }
 }
 static bool p_coef(const Model& M, const ClIn& in, PCoef& c, double pseg) {                       // PHY:443-454
-	if (!(in.mtr[0] + in.mtr[1] > 0)) return false;
-	const double D0 = in.rhoeC - in.etr[0] - in.etr[1];
+	if (!in.nodelta && !(in.mtr[0] + in.mtr[1] > 0)) return false;
+	const double D0 = in.nodelta ? 0.0 : in.rhoeC - in.etr[0] - in.etr[1];
 	double ex[2] = {0.0, 0.0};
-	defect_split(M, in, D0 + pseg * in.s, c.lam, ex);
+	if (in.nodelta) c.lam[0] = c.lam[1] = 0.0; else defect_split(M, in, D0 + pseg * in.s, c.lam, ex);
 	for (int k = 0; k < 2; k++) {
 		c.A[k] = in.etr[k] - in.mtr[k] * M.PH[k].q + c.lam[k] * D0 + ex[k];
 		c.B[k] = in.atr[k] + c.lam[k] * in.s;
@@ -445,9 +450,9 @@ static bool closure_p_phy(const Model& M, const ClIn& in, ClOut& o) {           
 	return closure_out(M, in, p, o);
 }
 static bool closure_p_pho(const Model& M, const ClIn& in, ClOut& o) {                              // PHO:472-548
-	if (!(in.mtr[0] + in.mtr[1] > 0)) return false;
+	if (!in.nodelta && !(in.mtr[0] + in.mtr[1] > 0)) return false;
 	const double plo = plo_mix(M);
-	const double D0 = in.rhoeC - in.etr[0] - in.etr[1];
+	const double D0 = in.nodelta ? 0.0 : in.rhoeC - in.etr[0] - in.etr[1];
 	double lE[2], A0[2], sm = in.mtr[0] + in.mtr[1];
 	{ double E[2], sE = 0.0; for (int k = 0; k < 2; k++) { E[k] = E_above(M, in, k); sE += E[k]; }
 	  if (sE > 0) for (int k = 0; k < 2; k++) lE[k] = E[k] / sE;
@@ -457,7 +462,8 @@ static bool closure_p_pho(const Model& M, const ClIn& in, ClOut& o) {           
 	auto PhiD = [&](double p, double& df) {
 		double lam[2], ex[2] = {0.0, 0.0};
 		const double delta = D0 + p * in.s;
-		if (!(delta > 0.0) || !(Qb > 0.0)) { lam[0] = lE[0]; lam[1] = lE[1]; }
+		if (in.nodelta) lam[0] = lam[1] = 0.0;
+		else if (!(delta > 0.0) || !(Qb > 0.0)) { lam[0] = lE[0]; lam[1] = lE[1]; }
 		else if (delta < Qb) { lam[0] = in.Qph[0] / Qb; lam[1] = in.Qph[1] / Qb; }
 		else for (int k = 0; k < 2; k++) { lam[k] = lE[k]; ex[k] = in.Qph[k] - Qb * lE[k]; }
 		double f = 0.0; df = 0.0;
@@ -560,10 +566,115 @@ static bool closure_pT(const Model& M, const ClIn& in, ClOut& o) {
 	for (int k = 0; k < 2; k++) { o.m[k] = m[k]; o.a[k] = m[k] * v_pT(M, k, p, T); o.e[k] = m[k] * e_pT(M, k, p, T); o.p[k] = p; o.T[k] = T; }
 	o.pbar = p; return T > 0;
 }
-static bool closure(const Model& M, const ClIn& in, ClOut& o) {                                    // PHY:786-810 (CL_P, CL_PT)
+// ---- the saturation line and the pTg closure (relaxation levels 2-3)
+static inline double s_pT(const Model& M, int k, double p, double T) { const Phase& P = M.PH[k];            // PHY:80
+	return P.cv * (P.g * M.log_(T) - (P.g - 1.0) * M.log_(p + P.pi)) + P.qp; }
+static inline double g_pT(const Model& M, int k, double p, double T) { const Phase& P = M.PH[k];            // PHY:81
+	return P.g * P.cv * T + P.b * p + P.q - T * s_pT(M, k, p, T); }
+static double sat_f(const Model& M, double p, double T) { return g_pT(M, 0, p, T) - g_pT(M, 1, p, T); }   // PHY:113
+// Newton on T for g_1 = g_2 at fixed p, the step kept within [T/2, 2T] (PHY:114-124)
+static double sat_newton(const Model& M, double p, double Tg, bool& ok) {
+	double T = Tg; ok = false;
+	for (int it = 0; it < 60; it++) {
+		const double f = sat_f(M, p, T), d = s_pT(M, 1, p, T) - s_pT(M, 0, p, T);
+		if (!(d > 0) || !std::isfinite(f)) return T;
+		double Tn = T - f / d; if (Tn < 0.5 * T) Tn = 0.5 * T; if (Tn > 2.0 * T) Tn = 2.0 * T;
+		if (std::fabs(Tn - T) < 1e-14 * T) { ok = true; return Tn; }
+		T = Tn;
+	}
+	ok = std::fabs(sat_f(M, p, T)) < 1e-8 * std::fabs(g_pT(M, 0, p, T)) + 1e-8; return T;
+}
+// the table (PHY:125-152): 1600 points in ln p over [ln 0.1, ln 5e9]; the first root by a scan of T in [20, 5000] K
+// and bisection, each next one by Newton from the previous; it ends where no root is found or where the liquid volume
+// reaches the vapour's (the critical point, refined by bisection in p)
+static void build_sat(const Model& M, SatTable& S) {
+	S = SatTable();
+	const int NP = 1600; const double l0 = M.log_(1e-1), l1 = M.log_(5e9);
+	double Tprev = -1.0;
+	for (int n = 0; n < NP; n++) {
+		const double lp = l0 + (l1 - l0) * n / (NP - 1), p = M.exp_(lp);
+		double Troot = -1.0;
+		if (Tprev < 0) {
+			double Ta = 20.0, fa = sat_f(M, p, Ta);
+			for (int m = 1; m <= 4000; m++) {
+				const double Tb = 20.0 * M.pow_(250.0, m / 4000.0), fb = sat_f(M, p, Tb);
+				if (fa < 0 && fb >= 0) {
+					double a = Ta, b = Tb;
+					for (int it = 0; it < 200; it++) { const double c = 0.5 * (a + b); if (sat_f(M, p, c) < 0) a = c; else b = c; }
+					Troot = 0.5 * (a + b); break;
+				}
+				Ta = Tb; fa = fb;
+			}
+		} else { bool ok; const double T = sat_newton(M, p, Tprev, ok); if (ok) Troot = T; }
+		if (Troot < 0) { if (n > 0) S.pcrit = M.exp_(S.lp.back()); break; }
+		if (!(v_pT(M, 1, p, Troot) > v_pT(M, 0, p, Troot))) { S.pcrit = p; S.vcrit = v_pT(M, 0, p, Troot); break; }
+		S.lp.push_back(lp); S.T.push_back(Troot); Tprev = Troot;
+	}
+	if (!S.lp.empty()) S.pmin = M.exp_(S.lp.front());
+	if (S.vcrit == 0.0 && !S.T.empty()) S.vcrit = v_pT(M, 0, M.exp_(S.lp.back()), S.T.back());
+	if (std::isfinite(S.pcrit) && !S.lp.empty()) {
+		double a = M.exp_(S.lp.back()), b = S.pcrit, Ta = S.T.back();
+		for (int it = 0; it < 100; it++) {
+			const double c = std::sqrt(a * b); bool ok; const double T = sat_newton(M, c, Ta, ok);
+			if (ok && v_pT(M, 1, c, T) > v_pT(M, 0, c, T)) { a = c; Ta = T; } else b = c;
+		}
+		S.pcrit = a; S.vcrit = v_pT(M, 0, a, Ta);
+	}
+}
+// T_sat(p) (PHY:154-162): Newton from the table's linear interpolation in ln p; NaN outside (0, pcrit) or on failure
+static double Tsat(const Model& M, double p) {
+	const SatTable& S = *M.sat;
+	if (!(p > 0) || !(p < S.pcrit) || S.lp.empty()) return NAN;
+	const double lp = M.log_(p); double Tg;
+	if (lp <= S.lp.front()) Tg = S.T.front();
+	else if (lp >= S.lp.back()) Tg = S.T.back();
+	else {
+		const double h = S.lp[1] - S.lp[0];
+		const int n = std::min((int)((lp - S.lp[0]) / h), (int)S.lp.size() - 2);
+		const double w = (lp - S.lp[n]) / h; Tg = (1 - w) * S.T[n] + w * S.T[n + 1];
+	}
+	bool ok; const double T = sat_newton(M, p, Tg, ok); return ok ? T : NAN;
+}
+static const double YMIN = 1e-10;                                                                    // PHY:242
+// T at p and vapour mass fraction Y from the mixture's specific energy (PHY:701-704)
+static inline double T_from_energy(const Model& M, double p, double Y, double eC) {
+	const Phase &A = M.PH[0], &B = M.PH[1];
+	const double d = (1 - Y) * A.cv * (p + A.g * A.pi) / (p + A.pi) + Y * B.cv * (p + B.g * B.pi) / (p + B.pi);
+	return (eC - (1 - Y) * A.q - Y * B.q) / d;
+}
+// the pTg state at p (PHY:757-767): T = T_sat(p) and Y from the energy, clamped to [YMIN, 1 - YMIN] (then T from the
+// energy); at p >= pcrit a single phase by v against vcrit
+static bool pTg_state(const Model& M, double p, double rho, double eC, double& Y, double& T) {
+	const double Ts = Tsat(M, p);
+	if (std::isfinite(Ts)) {
+		const double el = e_pT(M, 0, p, Ts), ev = e_pT(M, 1, p, Ts);
+		Y = (eC - el) / (ev - el);
+		if (Y > YMIN && Y < 1 - YMIN) { T = Ts; return true; }
+		Y = smin(smax(Y, YMIN), 1 - YMIN);
+	} else {
+		if (!(p >= M.sat->pcrit)) return false;
+		Y = (1.0 / rho < M.sat->vcrit) ? YMIN : 1 - YMIN;
+	}
+	T = T_from_energy(M, p, Y, eC); return T > 0;
+}
+// pTg closure (B6.6, PHY:768-778): mechanical, thermal and chemical equilibrium; the root of rho sum_k Y_k v_k = 1
+static bool closure_pTg(const Model& M, const ClIn& in, ClOut& o) {
+	const double rho = in.mtr[0] + in.mtr[1], eC = in.rhoeC / rho;
+	auto Phi = [&](double p) {
+		double Y, T; if (!pTg_state(M, p, rho, eC, Y, T)) return (double)NAN;
+		return rho * ((1 - Y) * v_pT(M, 0, p, T) + Y * v_pT(M, 1, p, T)) - 1.0; };
+	double p; if (!root_above(M, Phi, plo_mix(M), in.pguess, p)) return false;
+	double Y, T; if (!pTg_state(M, p, rho, eC, Y, T)) return false;
+	o.m[0] = rho * (1 - Y); o.m[1] = rho * Y;
+	for (int k = 0; k < 2; k++) { o.a[k] = o.m[k] * v_pT(M, k, p, T); o.e[k] = o.m[k] * e_pT(M, k, p, T); o.p[k] = p; o.T[k] = T; }
+	o.pbar = p; return true;
+}
+static bool closure_p(const Model& M, const ClIn& in, ClOut& o) {
+	return M.closure_path == 0 ? closure_p_phy(M, in, o) : closure_p_pho(M, in, o); }
+static bool closure(const Model& M, const ClIn& in, ClOut& o) {                                    // PHY:786-810 (CL_P, CL_PT, CL_PTG)
 	for (int k = 0; k < 2; k++) if (!(in.atr[k] > 0) || !(in.mtr[k] > 0) || !std::isfinite(in.etr[k])) return false;
 	if (!std::isfinite(in.rhoeC) || !std::isfinite(in.s)) return false;
-	const bool ok = in.type == CL_PT ? closure_pT(M, in, o) : M.closure_path == 0 ? closure_p_phy(M, in, o) : closure_p_pho(M, in, o);
+	const bool ok = in.type == CL_PT ? closure_pT(M, in, o) : in.type == CL_PTG ? closure_pTg(M, in, o) : closure_p(M, in, o);
 	return ok && std::isfinite(o.pbar);
 }
 static double closure_roundoff(const Model& M, const ClIn& in, double p) {                         // PC:631-638
@@ -643,6 +754,7 @@ struct Ctx {
 	std::vector<double> phx, phy, a2s, ubx, uby;
 	std::vector<double> eTV1, eTV2, ETV, dA;   // the TV-type variants' explicit phase energies and total energy; BB's alpha_1 step
 	std::vector<int> ctype;                    // STEP_TYPE (PC:537-560): each interior cell's closure type of the step
+	SatTable SAT;                              // the saturation line (relaxation levels 2-3; PC:1151 builds it before the run)
 	// stage data (PC:565-582)
 	double tau, rx, ry;
 	std::vector<double> ust, vst, ubsx, kpx, wLx, pvx, ubsy, kpy, wLy, pvy, Kw, Qph, cz, Pold;
@@ -662,6 +774,7 @@ static void init_ctx(Ctx& X, const Cfg& C) {
 	for (int k = 0; k < 2; k++) X.M.PH[k] = Phase{C.ph[k][0], C.ph[k][1], C.ph[k][2], C.ph[k][3], C.ph[k][4], C.ph[k][5]};
 	X.M.closure_path = C.closure_path; X.M.share_start = C.share_start; X.M.dot_order = C.dot_order; X.M.portable = C.portable_math;
 	X.M.shock_fused = C.shock_fused;
+	if (C.relax >= 2) { build_sat(X.M, X.SAT); X.M.sat = &X.SAT; }
 	X.nx = C.nx; X.ny = C.ny; X.NXT = C.nx + 2 * GH; X.NYT = C.ny + 2 * GH;
 	X.dx = (C.x1 - C.x0) / C.nx; X.dy = (C.y1 - C.y0) / C.ny; X.twoD = C.ny > 1;
 	const size_t nc = (size_t)X.NXT * X.NYT;
@@ -1332,10 +1445,27 @@ static void init_pressure(Ctx& X) {                                             
 	for (int j = 0; j < X.ny; j++) for (int i = 0; i < X.nx; i++) X.P[(size_t)i + (size_t)j * X.nx] = pbar_of(X.M, &X.U[NV * X.id(i + GH, j + GH)]) - X.P_REF;
 }
 // one attempt of a step of size dt with an rk-stage table; false = inadmissible (PC:1243-1285)
-// ---- relaxation level 1, p-pT (PC:513-560, PHY:814-830): each cell's closure type for the step (thermal equilibrium
-// where eps_I <= alpha_1 <= 1 - eps_I, else mechanical), with U^n projected onto the pT equilibrium in those cells
+// ---- relaxation levels 1-3 (PC:513-560, PHY:812-830): each cell's closure type for the step, with U^n projected onto
+// that closure's equilibrium (pT, pTg). Only cells with eps_I <= alpha_1 <= 1 - eps_I relax beyond p. Level 1 (p-pT):
+// thermal equilibrium there. Level 2 (p-pT-pTg): chemical where the pT equilibrium's T_1 exceeds T_sat(p), else
+// thermal. Level 3 (p-pTg): chemical where either phase's T at the mechanical equilibrium (without the energy defect)
+// exceeds T_sat(p), else mechanical. The rates are instantaneous (mu = theta = nu = inf, the reference's defaults).
+static int closure_type_of(const Model& M, int level, double eps_i, double a1, double m1, double m2, double e1, double e2, double rhoe) {   // PHY:814-830
+	if (level == 0) return CL_P;
+	if (!(a1 >= eps_i && a1 <= 1.0 - eps_i)) return CL_P;
+	ClIn in{}; in.atr[0] = a1; in.atr[1] = 1 - a1; in.mtr[0] = m1; in.mtr[1] = m2; in.etr[0] = e1; in.etr[1] = e2; in.s = 0; in.rhoeC = rhoe;
+	in.type = CL_PT; in.pguess = smax(p_mix(M, a1, m1, m2, rhoe), 1.0);
+	if (level == 3) {
+		in.type = CL_P; in.nodelta = true; ClOut o; if (!closure_p(M, in, o)) return CL_P;
+		const double Ts = Tsat(M, o.p[0]); if (std::isfinite(Ts) && (o.T[0] > Ts || o.T[1] > Ts)) return CL_PTG; return CL_P;
+	}
+	if (level == 2) {
+		ClOut o; if (closure_pT(M, in, o)) { const double Ts = Tsat(M, o.p[0]); if (std::isfinite(Ts) && o.T[0] > Ts) return CL_PTG; }
+	}
+	return CL_PT;
+}
 static bool project_cell(const Model& M, double* Q, int type) {                                      // PC:524-535
-	if (type != CL_PT) return true;
+	if (!(type == CL_PT || type == CL_PTG)) return true;
 	const double rho = Q[IM1] + Q[IM2];
 	ClIn in{}; in.atr[0] = Q[IA1]; in.atr[1] = 1.0 - Q[IA1]; in.mtr[0] = Q[IM1]; in.mtr[1] = Q[IM2];
 	in.etr[0] = Q[IE1]; in.etr[1] = Q[IE2]; in.s = 0.0;
@@ -1351,8 +1481,8 @@ static void relaxation_activation(Ctx& X, std::vector<double>& U) {             
 	#pragma omp parallel for schedule(static)
 	for (int q = 0; q < n; q++) {
 		double* Q = &U[NV * X.id(q % nx + GH, q / nx + GH)];
-		const double a1 = Q[IA1];
-		int ty = (X.C.relax > 0 && a1 >= X.C.eps_i && a1 <= 1.0 - X.C.eps_i) ? CL_PT : CL_P;   // closure_type_of at level 1
+		const double rho = Q[IM1] + Q[IM2], rhoe = Q[IEN] - 0.5 * (Q[IMU] * Q[IMU] + Q[IMV] * Q[IMV]) / rho;   // cell_type_U (PC:517-521)
+		int ty = closure_type_of(X.M, X.C.relax, X.C.eps_i, Q[IA1], Q[IM1], Q[IM2], Q[IE1], Q[IE2], rhoe);
 		if (!project_cell(X.M, Q, ty)) ty = CL_P;
 		X.ctype[(size_t)q] = ty;
 	}
@@ -1731,6 +1861,12 @@ void ora_get_pressure(void* h, double* P, double* pref) { Ctx& X = *(Ctx*)h; std
 int ora_run(void* h, double t_end, long max_steps) { return run(*(Ctx*)h, t_end, max_steps); }
 // RelaxationActivation on the state (the reference's main applies it once more before its output, PC:1300)
 void ora_relax_activate(void* h) { Ctx& X = *(Ctx*)h; if (X.C.relax > 0) relaxation_activation(X, X.U); }
+// the context's saturation line (relaxation levels 2-3): T_sat(p), and (pmin, pcrit, vcrit, table points); NaN / -1 without it
+double ora_tsat(void* h, double p) { Ctx& X = *(Ctx*)h; return X.M.sat ? Tsat(X.M, p) : NAN; }
+int ora_sat_info(void* h, double* out) {
+	Ctx& X = *(Ctx*)h; if (!X.M.sat) return -1;
+	out[0] = X.SAT.pmin; out[1] = X.SAT.pcrit; out[2] = X.SAT.vcrit; return (int)X.SAT.lp.size();
+}
 // the explicit HLLC scheme's time loop on the context's state (cfl: the acoustic CFL, 0.4 in the reference); the
 // relaxation's closure failures accumulate in ora_hllc_closure_fails
 int ora_hllc_run(void* h, double t_end, long max_steps, double cfl) { return hllc_run(*(Ctx*)h, t_end, max_steps, cfl); }

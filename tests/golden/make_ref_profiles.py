@@ -4,7 +4,7 @@ Source files (data, not code) under the reference tree: data/Schropff2026_Rieman
 data/SPB2009_4.3.1_water_air/water_air_N100_t240us.dat (case 5), data/SPB2009_4.1_interface_advection/
 interface_advection_t2.79ms.dat (case 4, N = 200). Columns kept (17 significant digits in the source): p u v rho alpha2
 Y2 rho1 rho2 p1 p2 (write_profile, PC:1002-1013); and data/PS2014_water_cavitation_tube_{2mps,500mps}/ (cases 1 and 2,
-5000 cells, relaxation levels 0 and 1). Run: python tests/golden/make_ref_profiles.py /root/reference
+5000 cells, relaxation levels 0 to 3). Run: python tests/golden/make_ref_profiles.py /root/reference
 """
 import os
 import sys
@@ -15,12 +15,15 @@ COLS = {"p": 2, "u": 3, "v": 4, "rho": 5, "alpha2": 6, "Y2": 7, "rho1": 8, "rho2
 FILES = {f"c{c}_200": (c, 200, f"data/Schropff2026_Riemann_problems/c{c}_200.dat") for c in range(80, 89)}
 FILES["c5_100"] = (5, 100, "data/SPB2009_4.3.1_water_air/water_air_N100_t240us.dat")
 FILES["c4_200"] = (4, 200, "data/SPB2009_4.1_interface_advection/interface_advection_t2.79ms.dat")
-# Pelanti-Shyue water cavitation tubes (cases 1, 2; 5000 cells) at relaxation levels 0 (p) and 1 (p-pT): the columns
-# p u rho alpha2 p1 p2 (the others follow from them or carry fewer printed digits)
-RELAX_COLS = ["p", "u", "rho", "alpha2", "p1", "p2"]
+# Pelanti-Shyue water cavitation tubes (cases 1, 2; 5000 cells) at relaxation levels 0 (p), 1 (p-pT), 2 (p-pT-pTg) and
+# 3 (p-pTg): the columns p u rho alpha2 Y2 p1 p2 (the others follow from them or carry fewer printed digits)
+RELAX_COLS = ["p", "u", "rho", "alpha2", "Y2", "p1", "p2"]
 RELAX_FILES = {"c1_5000_r0": (1, 5000, 0, "data/PS2014_water_cavitation_tube_2mps/p_relax.dat"),
                "c1_5000_r1": (1, 5000, 1, "data/PS2014_water_cavitation_tube_2mps/p-pT_relax.dat"),
-               "c2_5000_r1": (2, 5000, 1, "data/PS2014_water_cavitation_tube_500mps/p-pT_relax.dat")}
+               "c1_5000_r2": (1, 5000, 2, "data/PS2014_water_cavitation_tube_2mps/p-pT-pTg_relax.dat"),
+               "c1_5000_r3": (1, 5000, 3, "data/PS2014_water_cavitation_tube_2mps/p-pTg_relax.dat"),
+               "c2_5000_r1": (2, 5000, 1, "data/PS2014_water_cavitation_tube_500mps/p-pT_relax.dat"),
+               "c2_5000_r2": (2, 5000, 2, "data/PS2014_water_cavitation_tube_500mps/p-pT-pTg_relax.dat")}
 
 
 def main(ref):

@@ -1,16 +1,21 @@
-"""Relaxation level 1 (p-pT: thermal equilibrium in the mixture cells, the pT closure PHY:660-672 with its root finders
-PHY:183-225, and the step's RelaxationActivation PC:537-560) in the oracle, against the reference's archived
+"""Relaxation levels 1-3 in the oracle: level 1 (p-pT: thermal equilibrium in the mixture cells, the pT closure
+PHY:660-672 with its root finders PHY:183-225), levels 2 (p-pT-pTg) and 3 (p-pTg: chemical equilibrium where the phases
+exceed the saturation temperature, the pTg closure PHY:756-778 on the saturation line T_sat(p) PHY:111-163, the
+activation criteria PHY:814-830), and the step's RelaxationActivation PC:537-560, against the reference's archived
 full-precision profiles of the Pelanti-Shyue water cavitation tubes (data/PS2014_water_cavitation_tube_2mps and _500mps,
-5000 cells; extracted into tests/golden/ref_profiles.npz). In its PC mode the oracle reproduces them bit for bit, with
-the archived step, first-order-step and Newton counts (data/*/note.md)."""
+5000 cells; extracted into tests/golden/ref_profiles.npz). In its PC mode the oracle reproduces them bit for bit (the
+vapour mass fraction Y2, which the pTg closure changes, included), with the archived step, first-order-step and Newton
+counts (data/*/note.md). The saturation line itself is checked against what thermodynamics fixes: equal Gibbs energies
+at T_sat(p), the vapour branch (s2 > s1), and the Clausius-Clapeyron slope dp/dT = (s2 - s1) / (v2 - v1)."""
 import os
 
 import numpy as np
 import pytest
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_profiles.npz")
-DIAG = {"p": 0, "u": 1, "rho": 3, "alpha2": 4, "p1": 12, "p2": 13}
-NOTES = {"c1_5000_r0": (188, 33, 719), "c1_5000_r1": (188, 33, 695), "c2_5000_r1": (7252, 4, 14504)}
+DIAG = {"p": 0, "u": 1, "rho": 3, "alpha2": 4, "Y2": 5, "p1": 12, "p2": 13}
+NOTES = {"c1_5000_r0": (188, 33, 719), "c1_5000_r1": (188, 33, 695), "c1_5000_r2": (188, 33, 791),
+         "c1_5000_r3": (188, 33, 883), "c2_5000_r1": (7252, 4, 14504), "c2_5000_r2": (7252, 4, 15990)}
 
 
 def _check(O, gold, key):
@@ -30,15 +35,16 @@ def _check(O, gold, key):
     assert (st["steps"], st["rk1"], st["newton_it"]) == NOTES[key]
 
 
-@pytest.mark.parametrize("key", ["c1_5000_r0", "c1_5000_r1"])
+@pytest.mark.parametrize("key", ["c1_5000_r0", "c1_5000_r1", "c1_5000_r2", "c1_5000_r3"])
 def test_cavitation_tube_reproduced_bit_for_bit(oracle_lib, key):
     _check(oracle_lib, np.load(GOLD), key)
 
 
 @pytest.mark.slow
 @pytest.mark.skipif(not os.environ.get("SPB6_SLOW"), reason="7252 steps of 5000 cells (about a minute); SPB6_SLOW=1 runs it")
-def test_fast_cavitation_tube_reproduced_bit_for_bit(oracle_lib):
-    _check(oracle_lib, np.load(GOLD), "c2_5000_r1")
+@pytest.mark.parametrize("key", ["c2_5000_r1", "c2_5000_r2"])
+def test_fast_cavitation_tube_reproduced_bit_for_bit(oracle_lib, key):
+    _check(oracle_lib, np.load(GOLD), key)
 
 
 def test_level1_types_and_projection(oracle_lib):
@@ -55,3 +61,67 @@ def test_level1_types_and_projection(oracle_lib):
     T1, T2, p1, p2 = D[:, 10], D[:, 11], D[:, 12], D[:, 13]
     assert np.max(np.abs(T1 - T2) / T1) < 1e-12
     assert np.max(np.abs(p1 - p2) / (np.abs(p1) + 1e9)) < 1e-12
+
+
+def _gibbs(ph, p, T):
+    """g_k, s_k, v_k of a stiffened-gas phase (PHY:78-81) in extended precision"""
+    g, pi, q, qp, cv, b = (np.longdouble(x) for x in ph)
+    p, T = np.longdouble(p), np.longdouble(T)
+    s = cv * (g * np.log(T) - (g - 1) * np.log(p + pi)) + qp
+    return g * cv * T + b * p + q - T * s, s, (g - 1) * cv * T / (p + pi) + b
+
+
+def test_saturation_line_thermodynamics(oracle_lib):
+    """T_sat(p) of the water-vapour pair of the cavitation tubes: g1 = g2 at (p, T_sat) to round-off of g, the vapour
+    branch (s2 > s1, v2 > v1), T_sat increasing, and the Clausius-Clapeyron slope dp/dT = (s2 - s1) / (v2 - v1) against
+    central differences of T_sat (1e-5 relative); the table spans [0.1 Pa, pcrit) with pcrit where v1 = v2."""
+    O = oracle_lib
+    cfg, U, _ = O.case_init(1, 200)
+    cfg.relax = 2
+    o = O.Oracle(cfg)
+    o.set_state(U)
+    info = np.zeros(3)
+    npts = o.L.ora_sat_info(o.h, O._dp(info))
+    pmin, pcrit, vcrit = info
+    assert npts > 1000 and abs(pmin - 0.1) < 1e-12 and 1e6 < pcrit < 5e9
+    ph = [[cfg.ph[k][q] for q in range(6)] for k in range(2)]
+    Tprev = 0.0
+    for p in np.geomspace(1.0, 0.99 * pcrit, 60):
+        T = o.L.ora_tsat(o.h, p)
+        assert np.isfinite(T) and T > Tprev
+        Tprev = T
+        g1, s1, v1 = _gibbs(ph[0], p, T)
+        g2, s2, v2 = _gibbs(ph[1], p, T)
+        assert abs(float(g1 - g2)) <= 1e-11 * abs(float(g1)) + 1e-8, (p, T, float(g1 - g2))
+        assert s2 > s1 and v2 > v1
+        h = 1e-5 * p
+        dTdp = (o.L.ora_tsat(o.h, p + h) - o.L.ora_tsat(o.h, p - h)) / (2 * h)
+        assert abs(float((v2 - v1) / (s2 - s1)) / dTdp - 1) < 1e-5, (p, dTdp, float((v2 - v1) / (s2 - s1)))
+    assert np.isnan(o.L.ora_tsat(o.h, 1.01 * pcrit))
+    _, _, v1c = _gibbs(ph[0], pcrit, o.L.ora_tsat(o.h, pcrit * (1 - 1e-9)))
+    assert abs(float(v1c) / vcrit - 1) < 1e-6
+
+
+def test_level2_chemical_cells_sit_on_the_saturation_line(oracle_lib):
+    """After 40 steps and an activation at level 2 the tube's rarefaction holds cells where vapour has formed (Y2 above
+    twice its initial 5.5e-6): each is in chemical equilibrium, both phases at T_sat(p) and g1 = g2, to the round-off of
+    recomputing p and T from the conserved state (the liquid's p_inf = 6e8 against p ~ 5e4 Pa costs four digits)."""
+    O = oracle_lib
+    cfg, U, tend = O.case_init(1, 400)
+    cfg.relax = 2
+    o = O.Oracle(cfg)
+    o.set_state(U)
+    Y0 = O.cell_diag(cfg, U)[:, 5].max()
+    assert o.run(tend, 40) == 0
+    o.L.ora_relax_activate(o.h)
+    D = O.cell_diag(cfg, o.state())
+    p, Y2, T1, T2 = D[:, 0], D[:, 5], D[:, 10], D[:, 11]
+    ph = [[cfg.ph[k][q] for q in range(6)] for k in range(2)]
+    chem = np.where(Y2 > 2 * Y0)[0]
+    assert len(chem) > 5, (Y0, Y2.max())
+    for i in chem:
+        Ts = o.L.ora_tsat(o.h, p[i])
+        assert abs(T1[i] / Ts - 1) < 1e-10 and abs(T2[i] / Ts - 1) < 1e-10 and abs(T1[i] / T2[i] - 1) < 1e-14, (i, T1[i], T2[i], Ts)
+        g1, _, _ = _gibbs(ph[0], p[i], T1[i])
+        g2, _, _ = _gibbs(ph[1], p[i], T1[i])
+        assert abs(float(g1 - g2)) <= 1e-11 * abs(float(g1)), (i, float(g1 - g2), float(g1))
