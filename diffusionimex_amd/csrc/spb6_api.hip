@@ -115,8 +115,9 @@ struct spb6_ctx {
 	// alpha_1 step, TV's alpha_1 face donors
 	double *fwxL = nullptr, *fwxR = nullptr, *fwyL = nullptr, *fwyR = nullptr;
 	double *eTV1 = nullptr, *eTV2 = nullptr, *ETV = nullptr, *dA = nullptr, *donax = nullptr, *donay = nullptr;
-	int* ctype = nullptr;   // relaxation level 1: the closure type of each interior cell for the step (STEP_TYPE)
+	int* ctype = nullptr;   // relaxation levels 1-3: the closure type of each interior cell for the step (STEP_TYPE)
 	double eps_i = 1e-4;    // EPS_I (PHY:38)
+	SatDev sat{nullptr, nullptr, 0, 0.0, 0.0};   // the saturation table on the device (relaxation levels 2-3)
 	// graph mode (cfg.graph): a stage's fixed head (combination, explicit part, preparation, shock budget) and each
 	// Krylov chunk (its BiCGStab passes, the FP32 form's defect, sweep close and trial residual) run as captured graphs,
 	// re-captured per launch and applied to the instantiated graph of the same shape with hipGraphExecUpdate
@@ -159,7 +160,45 @@ Dev make_dev(const spb6_ctx* c) {
 	D.eTV1 = c->eTV1; D.eTV2 = c->eTV2; D.ETV = c->ETV; D.dA = c->dA; D.donax = c->donax; D.donay = c->donay;
 	if (D.variant != VAR_PC) { D.wxL = c->fwxL; D.wxR = c->fwxR; D.wyL = c->fwyL; D.wyR = c->fwyR; }
 	D.ctype = c->ctype;
+	D.sat = c->sat;
 	return D;
+}
+// the saturation table T_sat(ln p) (PHY:125-152), built on the host with the device's portable exp, log and pow: 1600
+// points in ln p over [ln 0.1, ln 5e9]; the first root by a scan of T in [20, 5000] K and bisection, each next one by
+// Newton from the previous; it ends where no root is found or where the liquid's volume reaches the vapour's (the
+// critical point, refined by bisection in p)
+void build_sat(const Phys& P, std::vector<double>& lpv, std::vector<double>& Tv, double& pcrit, double& vcrit) {
+	lpv.clear(); Tv.clear(); pcrit = INFINITY; vcrit = 0.0;
+	const int NP = 1600; const double l0 = plog(1e-1), l1 = plog(5e9);
+	double Tprev = -1.0;
+	for (int n = 0; n < NP; n++) {
+		const double lp = l0 + (l1 - l0) * n / (NP - 1), p = pexp(lp);
+		double Troot = -1.0;
+		if (Tprev < 0) {
+			double Ta = 20.0, fa = sat_f(P, p, Ta);
+			for (int m = 1; m <= 4000; m++) {
+				const double Tb = 20.0 * ppow(250.0, m / 4000.0), fb = sat_f(P, p, Tb);
+				if (fa < 0 && fb >= 0) {
+					double a = Ta, b = Tb;
+					for (int it = 0; it < 200; it++) { const double cm = 0.5 * (a + b); if (sat_f(P, p, cm) < 0) a = cm; else b = cm; }
+					Troot = 0.5 * (a + b); break;
+				}
+				Ta = Tb; fa = fb;
+			}
+		} else { bool ok; const double T = sat_newton(P, p, Tprev, ok); if (ok) Troot = T; }
+		if (Troot < 0) { if (n > 0) pcrit = pexp(lpv.back()); break; }
+		if (!(v_pT(P, 1, p, Troot) > v_pT(P, 0, p, Troot))) { pcrit = p; vcrit = v_pT(P, 0, p, Troot); break; }
+		lpv.push_back(lp); Tv.push_back(Troot); Tprev = Troot;
+	}
+	if (vcrit == 0.0 && !Tv.empty()) vcrit = v_pT(P, 0, pexp(lpv.back()), Tv.back());
+	if (std::isfinite(pcrit) && !lpv.empty()) {
+		double a = pexp(lpv.back()), b = pcrit, Ta = Tv.back();
+		for (int it = 0; it < 100; it++) {
+			const double cm = std::sqrt(a * b); bool ok; const double T = sat_newton(P, cm, Ta, ok);
+			if (ok && v_pT(P, 1, cm, T) > v_pT(P, 0, cm, T)) { a = cm; Ta = T; } else b = cm;
+		}
+		pcrit = a; vcrit = v_pT(P, 0, a, Ta);
+	}
 }
 
 // timed launch region
@@ -613,7 +652,7 @@ void set_tab(spb6_ctx* c, int rk) {   // PC:1195-1218: 1 backward Euler, 2 LSDIR
 // one attempt (PC:1243-1285): on success Ust / Pst hold the new state and pressure
 int attempt(spb6_ctx* c, double dt, int rk) {
 	set_tab(c, rk);
-	if (c->ctype) launch_relax_activate(c->g, c->ph, c->U, c->eps_i, c->ctype, c->st);   // PC:1247
+	if (c->ctype) launch_relax_activate(c->g, c->ph, c->sat, c->cfg.relax, c->U, c->eps_i, c->ctype, c->st);   // PC:1247
 	const size_t n = c->g.n;
 	HIPCHK(hipMemcpyAsync(c->Pst, c->P, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 	c->final_pending = false;
@@ -757,10 +796,10 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (cfg->variant < SPB6_VARIANT_PC || cfg->variant > SPB6_VARIANT_BB) { g_err = "variant out of range"; return nullptr; }
 	if (cfg->variant != SPB6_VARIANT_PC && (cfg->analytic_jacobian || cfg->learned_guess)) {
 		g_err = "the IMEX variants run with the finite-difference Jacobian and the plain Newton guess"; return nullptr; }
-	if (cfg->relax < 0 || cfg->relax > 1) { g_err = "relaxation level must be 0 (p) or 1 (p-pT)"; return nullptr; }
+	if (cfg->relax < 0 || cfg->relax > 3) { g_err = "relaxation level must be 0 (p), 1 (p-pT), 2 (p-pT-pTg) or 3 (p-pTg)"; return nullptr; }
 	if (cfg->shock_fused < 0 || cfg->shock_fused > 1) { g_err = "shock_fused must be 0 or 1"; return nullptr; }
 	if (cfg->relax > 0 && (cfg->analytic_jacobian || cfg->learned_guess)) {
-		g_err = "relaxation level 1 runs with the finite-difference Jacobian and the plain Newton guess"; return nullptr; }
+		g_err = "relaxation levels 1-3 run with the finite-difference Jacobian and the plain Newton guess"; return nullptr; }
 	spb6_ctx* c = new spb6_ctx();
 	c->cfg = *cfg; c->device = device;
 	std::memset(&c->stats, 0, sizeof c->stats);
@@ -792,6 +831,16 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 		c->allocs.push_back(c->ctype);
 		if (hipMemset(c->ctype, 0, n * sizeof(int)) != hipSuccess) return fail("alloc");
 		if (cfg->eps_i > 0) c->eps_i = cfg->eps_i;
+	}
+	if (cfg->relax >= 2) {   // the saturation line before the run (PC:1151)
+		std::vector<double> lpv, Tv;
+		build_sat(c->ph, lpv, Tv, c->sat.pcrit, c->sat.vcrit);
+		if (lpv.size() < 2) return fail("the phases have no saturation line (relaxation levels 2-3 need one)");
+		double *dl = nullptr, *dT = nullptr;
+		if (dalloc(c, &dl, lpv.size()) || dalloc(c, &dT, Tv.size())) return fail("alloc");
+		if (hipMemcpy(dl, lpv.data(), lpv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+		    hipMemcpy(dT, Tv.data(), Tv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return fail("upload");
+		c->sat.lp = dl; c->sat.T = dT; c->sat.n = (int)lpv.size();
 	}
 	double** arrs[] = {&c->phx, &c->phy, &c->a2s, &c->ubx, &c->uby, &c->vsc, &c->esc, &c->vu, &c->vv, &c->ust, &c->vst, &c->cz, &c->Kw, &c->Pold,
 	                   &c->ubsx, &c->kpx, &c->wLx, &c->pvx, &c->ubsy, &c->kpy, &c->wLy, &c->pvy};
@@ -962,7 +1011,7 @@ int spb6_step(spb6_ctx* c, double dt, int rk) {
 int spb6_attempt_begin(spb6_ctx* c) {
 	if (!c) return SPB6_BAD_ARG;
 	hipSetDevice(c->device);
-	if (c->ctype) launch_relax_activate(c->g, c->ph, c->U, c->eps_i, c->ctype, c->st);   // PC:1247
+	if (c->ctype) launch_relax_activate(c->g, c->ph, c->sat, c->cfg.relax, c->U, c->eps_i, c->ctype, c->st);   // PC:1247
 	HIPCHK(hipMemcpyAsync(c->Pst, c->P, (size_t)c->g.n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 	c->final_pending = false;
 	c->qe_cur = c->qs_cur = c->U;
@@ -1079,7 +1128,7 @@ int spb6_relax_activate(spb6_ctx* c) {
 	if (!c) return SPB6_BAD_ARG;
 	hipSetDevice(c->device);
 	if (!c->ctype) return SPB6_OK;
-	launch_relax_activate(c->g, c->ph, c->U, c->eps_i, c->ctype, c->st);
+	launch_relax_activate(c->g, c->ph, c->sat, c->cfg.relax, c->U, c->eps_i, c->ctype, c->st);
 	HIPCHK(hipStreamSynchronize(c->st));
 	return SPB6_OK;
 }

@@ -702,7 +702,9 @@ SPB_HD void stencil_state(const Phys& P, const double* c, double* st) {
 
 // ------------------------------------------------------------------ p closure (PHY:420-494, PHO:472-548)
 struct ClIn { double atr0, atr1, mtr0, mtr1, etr0, etr1, s, rhoeC, pguess, Q0, Q1; };
-struct ClOut { double a0, a1, e0, e1, pbar; };
+// the closure's phase contents: volume fractions, masses (the transported ones but where the pTg closure moves mass)
+// and energies, and the pressure
+struct ClOut { double a0, a1, m0, m1, e0, e1, pbar; };
 
 SPB_HD double E_above(const Phys& P, int k, double atr, double mtr, double etr) {    // PHY:420-423 (SG)
 	return smax(0.0, etr - mtr * P.q[k] - (atr - mtr * P.b[k]) * P.pi[k]); }
@@ -717,14 +719,18 @@ SPB_HD void defect_seg(double delta, double Qb, double Q0, double Q1, double lE0
 }
 // closure_p along the PHO iteration path: safeguarded Newton from the guess, Phi and Phi' together (PHO:472-548).
 // Returns false where the reference's closure returns false; on success fills o (alpha, eps per phase and pbar).
+// NODELTA (PHY:234): without the energy defect (no defect share, D0 = 0), for the p-pTg activation's prediction only.
+template <bool NODELTA = false>
 SPB_HD bool closure_p(const Phys& P, const ClIn& in, ClOut& o) {
-	// closure() input checks (PHY:787-790)
-	if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0)) return false;
-	if (!(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
-	if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
-	if (!(in.mtr0 + in.mtr1 > 0)) return false;
+	// closure() input checks (PHY:787-790; the activation's prediction calls closure_p itself, without them)
+	if (!NODELTA) {
+		if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0)) return false;
+		if (!(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
+		if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
+		if (!(in.mtr0 + in.mtr1 > 0)) return false;
+	}
 	const double plo = smax(-P.pi[0], -P.pi[1]);
-	const double D0 = in.rhoeC - in.etr0 - in.etr1;
+	const double D0 = NODELTA ? 0.0 : in.rhoeC - in.etr0 - in.etr1;
 	const double sm = in.mtr0 + in.mtr1;
 	double lE0, lE1;
 	{ const double E0 = E_above(P, 0, in.atr0, in.mtr0, in.etr0), E1 = E_above(P, 1, in.atr1, in.mtr1, in.etr1);
@@ -739,7 +745,8 @@ SPB_HD bool closure_p(const Phys& P, const ClIn& in, ClOut& o) {
 	bool conv = false;
 	for (int it = 0; it < 400; it++) {
 		double lam0, lam1, ex0, ex1;
-		defect_seg(D0 + p * in.s, Qb, in.Q0, in.Q1, lE0, lE1, lam0, lam1, ex0, ex1);
+		if (NODELTA) { lam0 = lam1 = ex0 = ex1 = 0.0; }
+		else defect_seg(D0 + p * in.s, Qb, in.Q0, in.Q1, lE0, lE1, lam0, lam1, ex0, ex1);
 		const double A0 = A00 + lam0 * D0 + ex0, B0 = in.atr0 + lam0 * in.s, N0 = A0 - (B0 - in.mtr0 * b0) * pi0;
 		const double A1 = A01 + lam1 * D0 + ex1, B1 = in.atr1 + lam1 * in.s, N1 = A1 - (B1 - in.mtr1 * b1) * pi1;
 		double f = 0.0, df = 0.0;
@@ -763,7 +770,8 @@ SPB_HD bool closure_p(const Phys& P, const ClIn& in, ClOut& o) {
 	if (!conv) return false;
 	// outputs (PHY:487-493)
 	double lam0, lam1, ex0, ex1;
-	defect_seg(D0 + p * in.s, Qb, in.Q0, in.Q1, lE0, lE1, lam0, lam1, ex0, ex1);
+	if (NODELTA) { lam0 = lam1 = ex0 = ex1 = 0.0; }
+	else defect_seg(D0 + p * in.s, Qb, in.Q0, in.Q1, lE0, lE1, lam0, lam1, ex0, ex1);
 	const double A0 = in.etr0 - in.mtr0 * P.q[0] + lam0 * D0 + ex0, B0 = in.atr0 + lam0 * in.s;
 	const double A1 = in.etr1 - in.mtr1 * P.q[1] + lam1 * D0 + ex1, B1 = in.atr1 + lam1 * in.s;
 	o.a0 = ((g0 - 1.0) * (A0 + B0 * p) + (p + g0 * pi0) * in.mtr0 * b0) / (g0 * (p + pi0));
@@ -772,6 +780,7 @@ SPB_HD bool closure_p(const Phys& P, const ClIn& in, ClOut& o) {
 	o.a1 = ((g1 - 1.0) * (A1 + B1 * p) + (p + g1 * pi1) * in.mtr1 * b1) / (g1 * (p + pi1));
 	if (!(o.a1 > in.mtr1 * b1) || !(p + pi1 > 0)) return false;
 	o.e1 = eps_amp(P, 1, o.a1, in.mtr1, p);
+	o.m0 = in.mtr0; o.m1 = in.mtr1;
 	o.pbar = p;
 	return isfin(p);
 }
@@ -827,8 +836,8 @@ SPB_HD double closure_roundoff(const Phys& P, const ClIn& in, double p) {
 }
 
 // ------------------------------------------------------------------ relaxation level 1: the pT closure (PHY:183-230, PHY:660-672)
-// closure types (PHY:401): mechanical (p) and thermal (pT) equilibrium
-enum { CL_P = 0, CL_PT = 2 };
+// closure types (PHY:226): mechanical (p), thermal (pT) and chemical (pTg) equilibrium
+enum { CL_P = 0, CL_PT = 2, CL_PTG = 5 };
 // Brent's method on a sign-changing bracket (PHY:183-204), operation for operation
 template <class F>
 SPB_HD bool brent_root(F f, double a, double b, double fa, double fb, double tol, double& root) {
@@ -910,9 +919,109 @@ SPB_HD bool closure_pT(const Phys& P, const ClIn& in, ClOut& o, double* T_out = 
 	double p; if (!root_above(Phi, smax(-P.pi[0], -P.pi[1]), in.pguess, p)) return false;
 	const double T = Tof(p);
 	o.a0 = m0 * v_pT(P, 0, p, T); o.a1 = m1 * v_pT(P, 1, p, T); o.e0 = m0 * e_pT(P, 0, p, T); o.e1 = m1 * e_pT(P, 1, p, T);
+	o.m0 = m0; o.m1 = m1;
 	o.pbar = p;
 	if (T_out) *T_out = T;
 	return T > 0 && isfin(o.pbar);
+}
+
+// ------------------------------------------------------------------ relaxation levels 2-3: the saturation line and the pTg closure
+// (PHY:111-163, PHY:756-778), with the portable log of the device
+// the saturation table (built on the host by the same arithmetic, PHY:125-152): n points of ln p on a uniform grid and
+// T_sat there, and the critical point (pcrit, vcrit)
+struct SatDev { const double* lp; const double* T; int n; double pcrit, vcrit; };
+SPB_HD double s_pT(const Phys& P, int k, double p, double T) {                      // PHY:80
+	return P.cv[k] * (P.g[k] * plog(T) - (P.g[k] - 1.0) * plog(p + P.pi[k])) + P.qp[k]; }
+SPB_HD double g_pT(const Phys& P, int k, double p, double T) {                      // PHY:81
+	return P.g[k] * P.cv[k] * T + P.b[k] * p + P.q[k] - T * s_pT(P, k, p, T); }
+SPB_HD double sat_f(const Phys& P, double p, double T) { return g_pT(P, 0, p, T) - g_pT(P, 1, p, T); }   // PHY:113
+// Newton on T for g_1 = g_2 at fixed p, the step kept within [T/2, 2T] (PHY:114-124)
+SPB_HD double sat_newton(const Phys& P, double p, double Tg, bool& ok) {
+	double T = Tg; ok = false;
+	for (int it = 0; it < 60; it++) {
+		const double f = sat_f(P, p, T), d = s_pT(P, 1, p, T) - s_pT(P, 0, p, T);
+		if (!(d > 0) || !isfin(f)) return T;
+		double Tn = T - f / d; if (Tn < 0.5 * T) Tn = 0.5 * T; if (Tn > 2.0 * T) Tn = 2.0 * T;
+		if (fabs(Tn - T) < 1e-14 * T) { ok = true; return Tn; }
+		T = Tn;
+	}
+	ok = fabs(sat_f(P, p, T)) < 1e-8 * fabs(g_pT(P, 0, p, T)) + 1e-8; return T;
+}
+// T_sat(p) (PHY:154-162): Newton from the table's linear interpolation in ln p; NaN outside (0, pcrit) or on failure
+SPB_HD double Tsat(const Phys& P, const SatDev& S, double p) {
+	if (!(p > 0) || !(p < S.pcrit) || S.n < 1) return dnan();
+	const double lp = plog(p); double Tg;
+	if (lp <= S.lp[0]) Tg = S.T[0];
+	else if (lp >= S.lp[S.n - 1]) Tg = S.T[S.n - 1];
+	else {
+		const double h = S.lp[1] - S.lp[0];
+		const int m = (int)((lp - S.lp[0]) / h), n = m < S.n - 2 ? m : S.n - 2;
+		const double w = (lp - S.lp[n]) / h; Tg = (1 - w) * S.T[n] + w * S.T[n + 1];
+	}
+	bool ok; const double T = sat_newton(P, p, Tg, ok); return ok ? T : dnan();
+}
+constexpr double YMIN = 1e-10;                                                        // PHY:242
+// T at p and vapour mass fraction Y from the mixture's specific energy (PHY:701-704)
+SPB_HD double T_from_energy(const Phys& P, double p, double Y, double eC) {
+	const double d = (1 - Y) * P.cv[0] * (p + P.g[0] * P.pi[0]) / (p + P.pi[0]) + Y * P.cv[1] * (p + P.g[1] * P.pi[1]) / (p + P.pi[1]);
+	return (eC - (1 - Y) * P.q[0] - Y * P.q[1]) / d;
+}
+// the pTg state at p (PHY:757-767): T = T_sat(p) and Y from the energy, clamped to [YMIN, 1 - YMIN] (then T from the
+// energy); at p >= pcrit a single phase by v against vcrit
+SPB_HD bool pTg_state(const Phys& P, const SatDev& S, double p, double rho, double eC, double& Y, double& T) {
+	const double Ts = Tsat(P, S, p);
+	if (isfin(Ts)) {
+		const double el = e_pT(P, 0, p, Ts), ev = e_pT(P, 1, p, Ts);
+		Y = (eC - el) / (ev - el);
+		if (Y > YMIN && Y < 1 - YMIN) { T = Ts; return true; }
+		Y = smin(smax(Y, YMIN), 1 - YMIN);
+	} else {
+		if (!(p >= S.pcrit)) return false;
+		Y = (1.0 / rho < S.vcrit) ? YMIN : 1 - YMIN;
+	}
+	T = T_from_energy(P, p, Y, eC); return T > 0;
+}
+// the pTg closure (B6.6, PHY:768-778): mechanical, thermal and chemical equilibrium, the root of rho sum_k Y_k v_k = 1
+// above max(-pi_k); the masses move (vapour fraction Y). closure()'s input checks (PHY:787-790) first.
+SPB_HD bool closure_pTg(const Phys& P, const SatDev& S, const ClIn& in, ClOut& o) {
+	if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0)) return false;
+	if (!(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
+	if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
+	const double rho = in.mtr0 + in.mtr1, eC = in.rhoeC / rho;
+	auto Phi = [&](double p) {
+		double Y, T; if (!pTg_state(P, S, p, rho, eC, Y, T)) return dnan();
+		return rho * ((1 - Y) * v_pT(P, 0, p, T) + Y * v_pT(P, 1, p, T)) - 1.0; };
+	double p; if (!root_above(Phi, smax(-P.pi[0], -P.pi[1]), in.pguess, p)) return false;
+	double Y, T; if (!pTg_state(P, S, p, rho, eC, Y, T)) return false;
+	o.m0 = rho * (1 - Y); o.m1 = rho * Y;
+	o.a0 = o.m0 * v_pT(P, 0, p, T); o.a1 = o.m1 * v_pT(P, 1, p, T); o.e0 = o.m0 * e_pT(P, 0, p, T); o.e1 = o.m1 * e_pT(P, 1, p, T);
+	o.pbar = p;
+	return isfin(o.pbar);
+}
+// the closure of type ty (CL_P, CL_PT, CL_PTG; PHY:786-810)
+SPB_HD bool closure_of(const Phys& P, const SatDev* S, int ty, const ClIn& in, ClOut& o) {
+	return ty == CL_PT ? closure_pT(P, in, o) : ty == CL_PTG ? closure_pTg(P, *S, in, o) : closure_p(P, in, o);
+}
+// the cell's closure type of the step at relaxation level `level` (PHY:814-830), from its content and specific energy:
+// beyond p only where eps_I <= alpha_1 <= 1 - eps_I; level 1 thermal; level 2 chemical where the pT equilibrium's T
+// exceeds T_sat(p), else thermal; level 3 chemical where a phase's T at the mechanical equilibrium without the energy
+// defect exceeds T_sat(p), else mechanical
+SPB_HD int closure_type_of(const Phys& P, const SatDev* S, int level, double eps_i, double a1, double m1, double m2, double e1, double e2, double rhoe) {
+	if (level == 0) return CL_P;
+	if (!(a1 >= eps_i && a1 <= 1.0 - eps_i)) return CL_P;
+	ClIn in; in.atr0 = a1; in.atr1 = 1 - a1; in.mtr0 = m1; in.mtr1 = m2; in.etr0 = e1; in.etr1 = e2; in.s = 0; in.rhoeC = rhoe;
+	in.Q0 = 0.0; in.Q1 = 0.0; in.pguess = smax(p_mix(P, a1, m1, m2, rhoe), 1.0);
+	if (level == 3) {
+		ClOut o; if (!closure_p<true>(P, in, o)) return CL_P;
+		const double Ts = Tsat(P, *S, o.pbar);
+		const double T0 = T_amp(P, 0, o.a0, o.m0, o.pbar), T1 = T_amp(P, 1, o.a1, o.m1, o.pbar);
+		return (isfin(Ts) && (T0 > Ts || T1 > Ts)) ? CL_PTG : CL_P;
+	}
+	if (level == 2) {
+		ClOut o; double T;
+		if (closure_pT(P, in, o, &T)) { const double Ts = Tsat(P, *S, o.pbar); if (isfin(Ts) && T > Ts) return CL_PTG; }
+	}
+	return CL_PT;
 }
 
 }   // namespace spb

@@ -202,33 +202,35 @@ __global__ void __launch_bounds__(256) k_final_var(Dev D, const double* __restri
 	atomic_max_nonneg(tmax + 64, T1);
 }
 
-// RelaxationActivation at level 1 (PC:513-560): closure_type_of (PHY:814-830) gives pT where eps_I <= alpha_1 <= 1 -
-// eps_I, and project_cell moves such a cell onto the pT equilibrium (s = 0, the guess max(p_mix, 1)); a cell whose
-// projection fails stays mechanical and unprojected
-__global__ void __launch_bounds__(256) k_relax_activate(Geo g, Phys P, double* __restrict__ U, double eps_i, int* __restrict__ ctype) {
+// RelaxationActivation at levels 1-3 (PC:513-560): closure_type_of (PHY:814-830) gives each cell's type from its
+// content and specific energy (cell_type_U, PC:517-521), and project_cell moves a pT or pTg cell onto that closure's
+// equilibrium (s = 0, the guess max(p_mix, 1); the pTg closure moves mass); a cell whose projection fails stays
+// mechanical and unprojected
+__global__ void __launch_bounds__(256) k_relax_activate(Geo g, Phys P, SatDev S, int level, double* __restrict__ U, double eps_i,
+                                                       int* __restrict__ ctype) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= g.n) return;
 	const int NC = g.NC, c = (q % g.nx + GH) + (q / g.nx + GH) * g.NXT;
-	const double a1 = U[IA1 * NC + c];
-	int ty = (a1 >= eps_i && a1 <= 1.0 - eps_i) ? CL_PT : CL_P;
-	if (ty == CL_PT) {
-		const double m1 = U[IM1 * NC + c], m2 = U[IM2 * NC + c], mu = U[IMU * NC + c], mv = U[IMV * NC + c];
-		const double rho = m1 + m2;
+	const double a1 = U[IA1 * NC + c], m1 = U[IM1 * NC + c], m2 = U[IM2 * NC + c], mu = U[IMU * NC + c], mv = U[IMV * NC + c];
+	const double e1 = U[IE1 * NC + c], e2 = U[IE2 * NC + c];
+	const double rho = m1 + m2, rhoe = U[IEN * NC + c] - 0.5 * (mu * mu + mv * mv) / rho;
+	int ty = closure_type_of(P, &S, level, eps_i, a1, m1, m2, e1, e2, rhoe);
+	if (ty == CL_PT || ty == CL_PTG) {
 		ClIn in;
-		in.atr0 = a1; in.atr1 = 1.0 - a1; in.mtr0 = m1; in.mtr1 = m2; in.etr0 = U[IE1 * NC + c]; in.etr1 = U[IE2 * NC + c]; in.s = 0.0;
-		in.rhoeC = U[IEN * NC + c] - 0.5 * (mu * mu + mv * mv) / rho;
+		in.atr0 = a1; in.atr1 = 1.0 - a1; in.mtr0 = m1; in.mtr1 = m2; in.etr0 = e1; in.etr1 = e2; in.s = 0.0; in.rhoeC = rhoe;
 		in.pguess = smax(p_mix(P, in.atr0, in.mtr0, in.mtr1, in.rhoeC), 1.0); in.Q0 = 0.0; in.Q1 = 0.0;
 		ClOut o;
-		if (closure_pT(P, in, o)) { U[IA1 * NC + c] = o.a0; U[IE1 * NC + c] = o.e0; U[IE2 * NC + c] = o.e1; }
-		else ty = CL_P;
+		if (closure_of(P, &S, ty, in, o)) {
+			U[IA1 * NC + c] = o.a0; U[IM1 * NC + c] = o.m0; U[IM2 * NC + c] = o.m1; U[IE1 * NC + c] = o.e0; U[IE2 * NC + c] = o.e1;
+		} else ty = CL_P;
 	}
 	ctype[q] = ty;
 }
 
 }   // namespace
 
-void launch_relax_activate(const Geo& g, const Phys& P, double* U, double eps_i, int* ctype, hipStream_t st) {
-	hipLaunchKernelGGL(k_relax_activate, dim3(nb(g.n)), dim3(256), 0, st, g, P, U, eps_i, ctype);
+void launch_relax_activate(const Geo& g, const Phys& P, const SatDev& S, int level, double* U, double eps_i, int* ctype, hipStream_t st) {
+	hipLaunchKernelGGL(k_relax_activate, dim3(nb(g.n)), dim3(256), 0, st, g, P, S, level, U, eps_i, ctype);
 	launch_bc_grid(g, U, st);
 }
 

@@ -90,10 +90,15 @@ typedef struct spb6_config {
                                         batch rejects it) */
     int variant;                     /* spb6_variant: which of the reference's IMEX programs the step follows (single-grid
                                         stepper, finite-difference Jacobian and plain guess only; the batch rejects it) */
-    int relax;                       /* RELAX_LEVEL (PHY:37): 0 p (mechanical equilibrium), 1 p-pT (thermal equilibrium too
-                                        in the cells with eps_i <= alpha_1 <= 1 - eps_i: the pT closure, PHY:660-672,
-                                        and the step's RelaxationActivation, PC:537-560). Finite-difference Jacobian and
-                                        plain guess only; the batch and the explicit HLLC scheme reject level 1 */
+    int relax;                       /* RELAX_LEVEL (PHY:37): 0 p (mechanical equilibrium); in the cells with
+                                        eps_i <= alpha_1 <= 1 - eps_i also 1 p-pT (thermal equilibrium: the pT closure,
+                                        PHY:660-672), 2 p-pT-pTg (chemical equilibrium where the pT state exceeds the
+                                        saturation temperature, else thermal), 3 p-pTg (chemical where a phase at the
+                                        mechanical equilibrium exceeds it, else mechanical): the pTg closure on the
+                                        saturation line T_sat(p), PHY:111-163 and PHY:756-778, and the step's
+                                        RelaxationActivation, PC:537-560, with the reference's instantaneous rates.
+                                        Finite-difference Jacobian and plain guess only; the batch and the explicit
+                                        HLLC scheme reject levels 1-3 */
     double eps_i;                    /* EPS_I (PHY:38); <= 0: the reference's 1e-4 */
     int graph;                       /* 1: each stage's fixed head (stage operands, explicit part, preparation, shock budget)
                                         and each Krylov chunk (BiCGStab passes; the FP32 form's defect, sweep close and
@@ -186,7 +191,7 @@ int spb6_attempt_accept(spb6_ctx* ctx, double dt);
 /* The time loop (PC:1230-1296): dt from caldt with the start-up ramp, reject-and-halve, until t_end or max_steps
  * accepted steps (max_steps < 0: no cap). */
 int spb6_run(spb6_ctx* ctx, double t_end, long max_steps);
-/* RelaxationActivation (PC:537-560) on the state: at relaxation level 1 each attempt starts with it; the reference's
+/* RelaxationActivation (PC:537-560) on the state: at relaxation levels 1-3 each attempt starts with it; the reference's
  * main applies it once more before its output (PC:1300), which a caller mirrors by calling this before
  * spb6_get_state / spb6_write_profile. No-op at level 0. */
 int spb6_relax_activate(spb6_ctx* ctx);

@@ -1,6 +1,9 @@
-"""Relaxation level 1 (p-pT) on the GPU: whole runs bit-identical to the oracle in its device mode (state, pressure and
-counters; the pT closure's root finders use the portable exp / log shared with the device), and the reference's archived
-level-1 profile of the Pelanti-Shyue water cavitation tube (5000 cells, t = 3.2 ms) reproduced."""
+"""Relaxation levels 1-3 on the GPU: whole runs bit-identical to the oracle in its device mode (state, pressure and
+counters; the pT and pTg closures' root finders and the saturation line use the portable exp / log shared with the
+device; the saturation table is built on the host with them), and the reference's archived profiles of the
+Pelanti-Shyue water cavitation tube (5000 cells, t = 3.2 ms) reproduced: level 1 to round-off, levels 2-3 within the
+spread the oracle measures live for the reference's own variants (its OpenMP program's paths, and its serial paths with
+the portable exp and log): the phase-change decisions flip on last-ulp changes (test_oracle_relax)."""
 import os
 
 import numpy as np
@@ -87,3 +90,81 @@ def test_level1_archived_profile():
     for name, col in {"p": 0, "u": 1, "rho": 3, "alpha2": 4, "p1": 12, "p2": 13}.items():
         a, b = D[:, col], gold[f"c1_5000_r1/{name}"]
         assert np.abs(a - b).sum() / np.abs(b).sum() <= 1e-11, name
+
+
+@pytest.mark.parametrize("relax", [2, 3])
+def test_levels23_tube_bitwise(relax):
+    """the whole 5000-cell tube (188 steps; vapour forms in the rarefaction) at level 2 (p-pT-pTg) and 3 (p-pTg)"""
+    from oracle import oracle as O
+    from diffusionimex_amd import spb6
+    ocfg, U, tend = O.case_init(1, 5000)
+    scfg = config_from_oracle(ocfg, relax=relax)
+    o = O.Oracle(oracle_cfg_from(scfg, O))
+    o.set_state(U)
+    assert o.run(tend) == 0
+    st = spb6.Stepper(scfg)
+    st.set_state(U)
+    assert st.run(tend) == 0
+    o.L.ora_relax_activate(o.h)
+    assert st.relax_activate() == 0
+    assert st.stats()["steps"] == 188
+    _check(o, st, f"tube5000_r{relax}")
+
+
+@pytest.mark.parametrize("relax", [2, 3])
+def test_levels23_2d_bitwise(relax):
+    """the 2D vapour pocket of the level-1 test under expansion, at levels 2 and 3"""
+    from diffusionimex_amd import cases, spb6
+    from oracle import oracle as O
+    nx, ny = 40, 28
+    ocfg, _, _ = O.case_init(1, nx, ny)
+    scfg = config_from_oracle(ocfg, relax=relax)
+    scfg.bc[2] = scfg.bc[3] = spb6.BC_WALL
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y)
+    r2 = (X - 0.5) ** 2 + (Y - 0.5) ** 2
+    a1 = np.where(r2 < 0.04, 0.9, 1 - 1e-2)
+    ph = [[ocfg.ph[k][q] for q in range(6)] for k in range(2)]
+    T0 = 354.728
+    v = lambda k, p, T: (ph[k][0] - 1) * ph[k][4] * T / (p + ph[k][1]) + ph[k][5]
+    U = cases.prim2cell(ph, a1, 1.0 / v(0, 1e5, T0), 1.0 / v(1, 1e5, T0), 30.0 * (X - 0.5), 30.0 * (Y - 0.5), 1e5, 1e5)
+    o, st = _pair_run(scfg, U, 25, activate=True)
+    _check(o, st, f"pocket40x28_r{relax}")
+
+
+@pytest.mark.parametrize("relax", [2, 3])
+def test_levels23_archived_profiles(relax):
+    """The archived p-pT-pTg and p-pTg profiles: per column within twice the larger of the reference's own variants'
+    distances to the archive (the oracle in the OpenMP program's paths with libm, and in the serial program's paths with
+    the portable exp and log), measured here on the CPU, plus 1e-12; the steps and first-order steps exactly."""
+    from diffusionimex_amd import spb6
+    from oracle import oracle as O
+    gold = np.load(os.path.join(HERE, "golden", "ref_profiles.npz"))
+    key = f"c1_5000_r{relax}"
+    cols = {"p": 0, "u": 1, "rho": 3, "alpha2": 4, "Y2": 5, "p1": 12, "p2": 13}
+    ocfg, U, tend = O.case_init(1, 5000)
+    st = spb6.Stepper(config_from_oracle(ocfg, relax=relax))
+    st.set_state(U)
+    assert st.run(tend) == 0
+    assert st.relax_activate() == 0
+    s = st.stats()
+    assert (s["steps"], s["rk1"]) == (188, 33)
+    D = O.cell_diag(ocfg, st.state())
+
+    def spread(setup):
+        cfg, U0, t_end = O.case_init(1, 5000)
+        setup(cfg)
+        cfg.relax = relax
+        o = O.Oracle(cfg)
+        o.set_state(U0)
+        assert o.run(t_end) == 0
+        o.L.ora_relax_activate(o.h)
+        Do = O.cell_diag(cfg, o.state())
+        return {k: np.abs(Do[:, c] - gold[f"{key}/{k}"]).sum() / np.abs(gold[f"{key}/{k}"]).sum() for k, c in cols.items()}
+
+    omp = spread(lambda c: O.reference_mode(c, "PC_omp"))
+    port = spread(lambda c: (O.reference_mode(c, "PC"), setattr(c, "portable_math", 1)))
+    for k, c in cols.items():
+        got = np.abs(D[:, c] - gold[f"{key}/{k}"]).sum() / np.abs(gold[f"{key}/{k}"]).sum()
+        assert got <= 2.0 * max(omp[k], port[k]) + 1e-12, (relax, k, got, omp[k], port[k])

@@ -125,3 +125,41 @@ def test_level2_chemical_cells_sit_on_the_saturation_line(oracle_lib):
         g1, _, _ = _gibbs(ph[0], p[i], T1[i])
         g2, _, _ = _gibbs(ph[1], p[i], T1[i])
         assert abs(float(g1 - g2)) <= 1e-11 * abs(float(g1)), (i, float(g1 - g2), float(g1))
+
+
+@pytest.mark.parametrize("relax", [2, 3])
+def test_levels23_device_mode_spread_is_the_references_own(oracle_lib, relax):
+    """What separates the device mode from the archive at levels 2-3. The phase-change decisions (T > T_sat(p), the
+    Y clamps) flip on last-ulp changes: the serial archive is reproduced bit for bit by the oracle in the OpenMP
+    program's iteration paths and the device's summation tree as long as libm's exp and log are used (level 2: exactly;
+    level 3: the OpenMP paths alone already move it, by the reference's own PC / PC_omp spread); swapping in the
+    portable exp and log of the device, in the serial program's paths, moves it by about as much as the device mode does.
+    Measured (relative L1, p / alpha2 / Y2): level 2, portable in PC paths 1.2e-5 / 6.2e-3 / 1.3e-2, device mode
+    1.2e-5 / 6.2e-3 / 1.3e-2; level 3, PC_omp 7.2e-6 / 7.0e-3 / 1.7e-2, device mode 7.5e-6 / 8.5e-3 / 2.5e-2."""
+    O = oracle_lib
+    gold = np.load(GOLD)
+    key = f"c1_5000_r{relax}"
+
+    def run(setup):
+        cfg, U, tend = O.case_init(1, 5000)
+        setup(cfg)
+        cfg.relax = relax
+        o = O.Oracle(cfg)
+        o.set_state(U)
+        assert o.run(tend) == 0
+        o.L.ora_relax_activate(o.h)
+        D = O.cell_diag(cfg, o.state())
+        return {name: np.abs(D[:, col] - gold[f"{key}/{name}"]).sum() / np.abs(gold[f"{key}/{name}"]).sum()
+                for name, col in DIAG.items()}
+
+    libm_device_paths = run(lambda c: setattr(c, "portable_math", 0))
+    portable_pc_paths = run(lambda c: (O.reference_mode(c, "PC"), setattr(c, "portable_math", 1)))
+    omp = run(lambda c: O.reference_mode(c, "PC_omp"))
+    device = run(lambda c: None)
+    if relax == 2:
+        assert max(libm_device_paths.values()) == 0.0 and max(omp.values()) == 0.0
+    else:
+        assert all(libm_device_paths[k] == omp[k] for k in DIAG)
+    ref = {k: max(portable_pc_paths[k], omp[k]) for k in DIAG}
+    for k in DIAG:
+        assert device[k] <= 2.0 * ref[k] + 1e-12, (relax, k, device[k], ref)
