@@ -4,7 +4,7 @@ Source files (data, not code) under the reference tree: data/Schropff2026_Rieman
 data/SPB2009_4.3.1_water_air/water_air_N100_t240us.dat (case 5), data/SPB2009_4.1_interface_advection/
 interface_advection_t2.79ms.dat (case 4, N = 200). Columns kept (17 significant digits in the source): p u v rho alpha2
 Y2 rho1 rho2 p1 p2 (write_profile, PC:1002-1013); and data/PS2014_water_cavitation_tube_{2mps,500mps}/ (cases 1 and 2,
-5000 cells, relaxation levels 0 to 3). Run: python tests/golden/make_ref_profiles.py /root/reference
+5000 cells, relaxation levels 0 to 3) and data/Pelanti2022_8.2.1_water_cavitation/ (case 30, levels 0 to 2). Run: python tests/golden/make_ref_profiles.py /root/reference
 """
 import os
 import sys
@@ -23,7 +23,11 @@ RELAX_FILES = {"c1_5000_r0": (1, 5000, 0, "data/PS2014_water_cavitation_tube_2mp
                "c1_5000_r2": (1, 5000, 2, "data/PS2014_water_cavitation_tube_2mps/p-pT-pTg_relax.dat"),
                "c1_5000_r3": (1, 5000, 3, "data/PS2014_water_cavitation_tube_2mps/p-pTg_relax.dat"),
                "c2_5000_r1": (2, 5000, 1, "data/PS2014_water_cavitation_tube_500mps/p-pT_relax.dat"),
-               "c2_5000_r2": (2, 5000, 2, "data/PS2014_water_cavitation_tube_500mps/p-pT-pTg_relax.dat")}
+               "c2_5000_r2": (2, 5000, 2, "data/PS2014_water_cavitation_tube_500mps/p-pT-pTg_relax.dat"),
+               # Pelanti's water cavitation tube (case 30, NASG liquid with covolume), levels 0-2
+               "c30_5000_r0": (30, 5000, 0, "data/Pelanti2022_8.2.1_water_cavitation/p_relax.dat"),
+               "c30_5000_r1": (30, 5000, 1, "data/Pelanti2022_8.2.1_water_cavitation/pT_relax.dat"),
+               "c30_5000_r2": (30, 5000, 2, "data/Pelanti2022_8.2.1_water_cavitation/pTg_relax.dat")}
 
 
 def main(ref):

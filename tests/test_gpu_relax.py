@@ -168,3 +168,24 @@ def test_levels23_archived_profiles(relax):
     for k, c in cols.items():
         got = np.abs(D[:, c] - gold[f"{key}/{k}"]).sum() / np.abs(gold[f"{key}/{k}"]).sum()
         assert got <= 2.0 * max(omp[k], port[k]) + 1e-12, (relax, k, got, omp[k], port[k])
+
+
+@pytest.mark.parametrize("relax", [0, 1, 2])
+def test_nasg_tube_bitwise(relax):
+    """Pelanti's water cavitation tube (case 30: NASG liquid with covolume, vapour; 5000 cells, 178 steps) at levels
+    0-2, bit-identical to the oracle's device mode"""
+    from diffusionimex_amd import spb6
+    from oracle import oracle as O
+    ocfg, U, tend = O.case_init(30, 5000)
+    scfg = config_from_oracle(ocfg, relax=relax)
+    o = O.Oracle(oracle_cfg_from(scfg, O))
+    o.set_state(U)
+    assert o.run(tend) == 0
+    st = spb6.Stepper(scfg)
+    st.set_state(U)
+    assert st.run(tend) == 0
+    if relax:
+        o.L.ora_relax_activate(o.h)
+        assert st.relax_activate() == 0
+    assert st.stats()["steps"] == 178
+    _check(o, st, f"nasg5000_r{relax}")
