@@ -29,6 +29,7 @@ class Cfg(C.Structure):
         ("closure_path", C.c_int), ("share_start", C.c_int), ("dot_order", C.c_int),
         ("portable_math", C.c_int), ("nthreads", C.c_int), ("jreuse", C.c_int), ("variant", C.c_int),
         ("relax", C.c_int), ("eps_i", C.c_double), ("shock_fused", C.c_int),
+        ("theta_rate", C.c_double), ("nu_rate", C.c_double),
     ]
 
 
@@ -141,6 +142,7 @@ def default_cfg(nx, ny=1, **kw):
     c.variant = 0
     c.relax, c.eps_i = 0, 1e-4
     c.shock_fused = 0
+    c.theta_rate, c.nu_rate = -1.0, -1.0
     for k, v in kw.items():
         if k == "bc":
             for q in range(4):

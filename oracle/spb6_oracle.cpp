@@ -116,6 +116,8 @@ struct Model {
 	const SatTable* sat = nullptr;   // built for relaxation levels 2-3 (T_sat of the pTg closure)
 	int closure_path = 1, share_start = 1, dot_order = 2, portable = 1;
 	int shock_fused = 0;   // the strong-shock profile integrations in fused form (prof_integrate_fused)
+	// the thermal and chemical relaxation rates (PHY:33; resolved by level as PC:1148-1149: inf = instantaneous)
+	double theta_rate = 0.0, nu_rate = 0.0;
 	double exp_(double x) const { return portable ? pexp(x) : std::exp(x); }
 	double log_(double x) const { return portable ? plog(x) : std::log(x); }
 	double pow_(double x, double y) const { return portable ? ppow(x, y) : std::pow(x, y); }
@@ -382,9 +384,11 @@ template <class F> static void shock_budget(const Model& M, F cellptr, bool twoD
 }
 
 // ------------------------------------------------------------------------------------------------ p closure (PHY:420-494, PHO:472-548)
-enum { CL_P = 0, CL_PT = 2, CL_PTG = 5 };   // closure types of PHY:226 used here: mechanical (p), thermal (pT), chemical (pTg) equilibrium
-// nodelta (PHY:234): the p closure without the energy defect, for the p-pTg activation's prediction only
-struct ClIn { double atr[2], mtr[2], etr[2], s, rhoeC, pguess, Qph[2]; int type = CL_P; bool nodelta = false; };
+// closure types of PHY:226 used here: mechanical (p), thermal (pT; finite rate theta), chemical (pTg; finite rate nu)
+enum { CL_P = 0, CL_PT = 2, CL_THETA = 3, CL_NU = 4, CL_PTG = 5 };
+// nodelta (PHY:234): the p closure without the energy defect, for the p-pTg activation's prediction only; tau: the
+// stage's implicit step a_ii dt (the finite-rate closures)
+struct ClIn { double atr[2], mtr[2], etr[2], s, rhoeC, pguess, Qph[2]; int type = CL_P; bool nodelta = false; double tau = 0.0; };
 struct ClOut { double a[2], m[2], e[2], p[2], T[2], pbar; };
 struct PCoef { double A[2], B[2], N[2], lam[2]; };
 static double E_above(const Model& M, const ClIn& in, int k) {                                     // PHY:420-423 (SG)
@@ -671,10 +675,71 @@ static bool closure_pTg(const Model& M, const ClIn& in, ClOut& o) {
 }
 static bool closure_p(const Model& M, const ClIn& in, ClOut& o) {
 	return M.closure_path == 0 ? closure_p_phy(M, in, o) : closure_p_pho(M, in, o); }
-static bool closure(const Model& M, const ClIn& in, ClOut& o) {                                    // PHY:786-810 (CL_P, CL_PT, CL_PTG)
+// ---- finite relaxation rates, backward-Euler form (RELAX_FORM 0, the reference's default)
+// finite theta (B6.4, PHY:679-700): at fixed p the phases' temperatures are linear in the exchanged heat, and
+// dQ = tau theta (T2 - T1) / (1 + tau theta S), S = 1/Cp1 + 1/Cp2, shifts the p closure's volume fractions; the
+// coefficient segment is the one at the guess (p_coef once), the root of sum alpha_k = 1 by the bracket search
+static bool closure_theta(const Model& M, const ClIn& in, ClOut& o) {
+	PCoef c; if (!p_coef(M, in, c, in.pguess)) return false;
+	const double th = M.theta_rate, tt = std::isinf(th) ? INF : in.tau * th;
+	if (tt > 0 && !std::isinf(tt)) for (int k = 0; k < 2; k++) if (!(c.N[k] > 0)) return false;
+	const Phase &A = M.PH[0], &B = M.PH[1];
+	const double Cp[2] = {A.g * A.cv * in.mtr[0], B.g * B.cv * in.mtr[1]};
+	const double S = 1.0 / Cp[0] + 1.0 / Cp[1];
+	auto eval = [&](double p, double a[2], double& dQ) {
+		double a0[2], T0[2];
+		for (int k = 0; k < 2; k++) { a0[k] = alpha_p(M, c, in, k, p); T0[k] = T_amp(M, k, a0[k], in.mtr[k], p); }
+		if (std::isinf(tt)) dQ = (T0[1] - T0[0]) / S;
+		else dQ = tt * (T0[1] - T0[0]) / (1.0 + tt * (1.0 / Cp[0] + 1.0 / Cp[1]));
+		a[0] = a0[0] + (A.g - 1.0) * dQ / (A.g * (p + A.pi));
+		a[1] = a0[1] - (B.g - 1.0) * dQ / (B.g * (p + B.pi));
+	};
+	auto Phi = [&](double p) { double a[2], dQ; eval(p, a, dQ); return a[0] + a[1] - 1.0; };
+	double p; if (!root_above(M, Phi, plo_mix(M), in.pguess, p)) return false;
+	double a[2], dQ; eval(p, a, dQ);
+	for (int k = 0; k < 2; k++) {
+		o.a[k] = a[k]; o.m[k] = in.mtr[k]; if (!(o.a[k] > o.m[k] * M.PH[k].b)) return false;
+		o.e[k] = eps_amp(M, k, o.a[k], o.m[k], p); o.p[k] = p; o.T[k] = T_amp(M, k, o.a[k], o.m[k], p);
+	}
+	o.pbar = p; return true;
+}
+// finite nu (B6.5, PHY:716-753): at each p the vapour fraction Y solves rho Y - m2 + tau nu (g2 - g1)(p, T(p, Y)) = 0 on
+// its admissible interval (clamped at the ends), T from the mixture energy; the root of rho sum_k Y_k v_k = 1
+static bool closure_nu(const Model& M, const ClIn& in, ClOut& o) {
+	const Phase &A = M.PH[0], &B = M.PH[1];
+	const double rho = in.mtr[0] + in.mtr[1], eC = in.rhoeC / rho, nr = M.nu_rate, tn = std::isinf(nr) ? INF : in.tau * nr;
+	auto Yof = [&](double p, double& Y, double& T) -> bool {
+		auto R = [&](double y) {
+			const double TT = T_from_energy(M, p, y, eC); if (!(TT > 0)) return (double)NAN;
+			if (std::isinf(tn)) return g_pT(M, 1, p, TT) - g_pT(M, 0, p, TT);
+			return rho * y - in.mtr[1] + tn * (g_pT(M, 1, p, TT) - g_pT(M, 0, p, TT)); };
+		double ya = YMIN, yb = 1.0 - YMIN;
+		const double dq = B.q - A.q, yT = dq != 0.0 ? (eC - A.q) / dq : INF;
+		if (dq > 0) yb = smin(yb, yT * (1.0 - 1e-12)); else if (dq < 0) ya = smax(ya, yT * (1.0 + 1e-12));
+		if (!(ya < yb)) return false;
+		const double Ra = R(ya), Rb = R(yb);
+		if (!std::isfinite(Ra) || !std::isfinite(Rb)) return false;
+		if (Ra >= 0) Y = ya; else if (Rb <= 0) Y = yb; else if (!brent(R, ya, yb, Ra, Rb, 1e-16, Y)) return false;
+		T = T_from_energy(M, p, Y, eC); return T > 0;
+	};
+	auto Phi = [&](double p) { double Y, T; if (!Yof(p, Y, T)) return (double)NAN; return rho * ((1 - Y) * v_pT(M, 0, p, T) + Y * v_pT(M, 1, p, T)) - 1.0; };
+	double p; if (!root_above(M, Phi, plo_mix(M), in.pguess, p)) return false;
+	double Y, T; if (!Yof(p, Y, T)) return false;
+	o.m[0] = rho * (1 - Y); o.m[1] = rho * Y;
+	for (int k = 0; k < 2; k++) { o.a[k] = o.m[k] * v_pT(M, k, p, T); o.e[k] = o.m[k] * e_pT(M, k, p, T); o.p[k] = p; o.T[k] = T; }
+	o.pbar = p; return true;
+}
+static bool closure(const Model& M, const ClIn& in, ClOut& o) {                                    // PHY:786-810
 	for (int k = 0; k < 2; k++) if (!(in.atr[k] > 0) || !(in.mtr[k] > 0) || !std::isfinite(in.etr[k])) return false;
 	if (!std::isfinite(in.rhoeC) || !std::isfinite(in.s)) return false;
-	const bool ok = in.type == CL_PT ? closure_pT(M, in, o) : in.type == CL_PTG ? closure_pTg(M, in, o) : closure_p(M, in, o);
+	bool ok;
+	switch (in.type) {
+	case CL_PT: ok = closure_pT(M, in, o); break;
+	case CL_THETA: ok = closure_theta(M, in, o); break;
+	case CL_NU: ok = closure_nu(M, in, o); break;
+	case CL_PTG: ok = closure_pTg(M, in, o); break;
+	default: ok = closure_p(M, in, o); break;
+	}
 	return ok && std::isfinite(o.pbar);
 }
 static double closure_roundoff(const Model& M, const ClIn& in, double p) {                         // PC:631-638
@@ -739,6 +804,7 @@ struct Cfg {
 	int variant;  // IMEX variant: 0 PC (SPB6_PC_IMEX.cpp), 1 TV, 2 TVpure, 3 TVinc, 4 TVsplit, 5 TVbb, 6 BB (SPB6_<name>_IMEX.cpp)
 	int relax; double eps_i;   // RELAX_LEVEL (PHY:37; 0 p, 1 p-pT) and EPS_I (PHY:38)
 	int shock_fused;           // 1: the strong-shock profile integrations in fused form (prof_integrate_fused)
+	double theta_rate, nu_rate;   // theta_rate, nu_rate (PHY:33); < 0: the level's default (PC:1148-1149)
 };
 struct Stats { long steps, rk1, rejects, floor_acc, stages, newton_it, krylov_it, linsolves; double maxres, t, dt_last; int status;
                long jac_builds; };
@@ -774,6 +840,8 @@ static void init_ctx(Ctx& X, const Cfg& C) {
 	for (int k = 0; k < 2; k++) X.M.PH[k] = Phase{C.ph[k][0], C.ph[k][1], C.ph[k][2], C.ph[k][3], C.ph[k][4], C.ph[k][5]};
 	X.M.closure_path = C.closure_path; X.M.share_start = C.share_start; X.M.dot_order = C.dot_order; X.M.portable = C.portable_math;
 	X.M.shock_fused = C.shock_fused;
+	X.M.theta_rate = C.theta_rate < 0 ? (C.relax >= 1 ? INF : 0.0) : C.theta_rate;                 // PC:1148-1149
+	X.M.nu_rate = C.nu_rate < 0 ? ((C.relax == 2 || C.relax == 3) ? INF : 0.0) : C.nu_rate;
 	if (C.relax >= 2) { build_sat(X.M, X.SAT); X.M.sat = &X.SAT; }
 	X.nx = C.nx; X.ny = C.ny; X.NXT = C.nx + 2 * GH; X.NYT = C.ny + 2 * GH;
 	X.dx = (C.x1 - C.x0) / C.nx; X.dy = (C.y1 - C.y0) / C.ny; X.twoD = C.ny > 1;
@@ -1239,6 +1307,7 @@ static bool cell_residual(const Ctx& X, int i, int j, double Pc, double PW, doub
 	ClIn in; in.atr[0] = Xv[0]; in.atr[1] = Xv[1]; in.mtr[0] = Xv[2]; in.mtr[1] = Xv[3]; in.etr[0] = Xv[4]; in.etr[1] = Xv[5];
 	in.s = s; in.rhoeC = E - 0.5 * (mx * mx + my * my) / rho; in.pguess = X.P_REF + Pc; in.Qph[0] = X.Qph[c * 2]; in.Qph[1] = X.Qph[c * 2 + 1];
 	in.type = X.ctype.empty() ? CL_P : X.ctype[(size_t)(i - GH) + (size_t)(j - GH) * X.nx];   // S.type (PC:951)
+	in.tau = X.tau;
 	ClOut o; if (!closure(M, in, o)) return false;
 	G = (o.pbar - X.P_REF) - Pc;
 	if (out) { out->cl = o; out->mx = mx; out->my = my; out->E = E; out->s = s; out->sig = closure_roundoff(M, in, o.pbar); }
@@ -1455,14 +1524,15 @@ static int closure_type_of(const Model& M, int level, double eps_i, double a1, d
 	if (!(a1 >= eps_i && a1 <= 1.0 - eps_i)) return CL_P;
 	ClIn in{}; in.atr[0] = a1; in.atr[1] = 1 - a1; in.mtr[0] = m1; in.mtr[1] = m2; in.etr[0] = e1; in.etr[1] = e2; in.s = 0; in.rhoeC = rhoe;
 	in.type = CL_PT; in.pguess = smax(p_mix(M, a1, m1, m2, rhoe), 1.0);
+	const int chem = std::isinf(M.nu_rate) ? CL_PTG : CL_NU, therm = std::isinf(M.theta_rate) ? CL_PT : CL_THETA;
 	if (level == 3) {
 		in.type = CL_P; in.nodelta = true; ClOut o; if (!closure_p(M, in, o)) return CL_P;
-		const double Ts = Tsat(M, o.p[0]); if (std::isfinite(Ts) && (o.T[0] > Ts || o.T[1] > Ts)) return CL_PTG; return CL_P;
+		const double Ts = Tsat(M, o.p[0]); if (std::isfinite(Ts) && (o.T[0] > Ts || o.T[1] > Ts)) return chem; return CL_P;
 	}
 	if (level == 2) {
-		ClOut o; if (closure_pT(M, in, o)) { const double Ts = Tsat(M, o.p[0]); if (std::isfinite(Ts) && o.T[0] > Ts) return CL_PTG; }
+		ClOut o; if (closure_pT(M, in, o)) { const double Ts = Tsat(M, o.p[0]); if (std::isfinite(Ts) && o.T[0] > Ts) return chem; }
 	}
-	return CL_PT;
+	return therm;
 }
 static bool project_cell(const Model& M, double* Q, int type) {                                      // PC:524-535
 	if (!(type == CL_PT || type == CL_PTG)) return true;

@@ -27,7 +27,11 @@ RELAX_FILES = {"c1_5000_r0": (1, 5000, 0, "data/PS2014_water_cavitation_tube_2mp
                # Pelanti's water cavitation tube (case 30, NASG liquid with covolume), levels 0-2
                "c30_5000_r0": (30, 5000, 0, "data/Pelanti2022_8.2.1_water_cavitation/p_relax.dat"),
                "c30_5000_r1": (30, 5000, 1, "data/Pelanti2022_8.2.1_water_cavitation/pT_relax.dat"),
-               "c30_5000_r2": (30, 5000, 2, "data/Pelanti2022_8.2.1_water_cavitation/pTg_relax.dat")}
+               "c30_5000_r2": (30, 5000, 2, "data/Pelanti2022_8.2.1_water_cavitation/pTg_relax.dat"),
+               # and its finite-rate runs: level 1 with theta = 2000, level 2 with nu = 1e-4 (meta: case, n, level, theta, nu)
+               "c30_5000_theta2000": (30, 5000, 1, "data/Pelanti2022_8.2.1_water_cavitation/pT_finite_theta2000.dat"),
+               "c30_5000_nu1e-4": (30, 5000, 2, "data/Pelanti2022_8.2.1_water_cavitation/pTg_finite_nu1e-4.dat")}
+RATES = {"c30_5000_theta2000": (2000.0, -1.0), "c30_5000_nu1e-4": (-1.0, 1e-4)}
 
 
 def main(ref):
@@ -44,6 +48,8 @@ def main(ref):
         for name in RELAX_COLS:
             out[f"{key}/{name}"] = a[:, COLS[name]]
         out[f"{key}/meta"] = np.array([case, n, relax], dtype=np.int64)
+        if key in RATES:
+            out[f"{key}/rates"] = np.array(RATES[key])
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_profiles.npz")
     np.savez_compressed(dst, **out)
     print(dst, len(out))

@@ -16,7 +16,8 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_p
 DIAG = {"p": 0, "u": 1, "rho": 3, "alpha2": 4, "Y2": 5, "p1": 12, "p2": 13}
 NOTES = {"c1_5000_r0": (188, 33, 719), "c1_5000_r1": (188, 33, 695), "c1_5000_r2": (188, 33, 791),
          "c1_5000_r3": (188, 33, 883), "c2_5000_r1": (7252, 4, 14504), "c2_5000_r2": (7252, 4, 15990),
-         "c30_5000_r0": (178, 33, 867), "c30_5000_r1": (178, 33, 666), "c30_5000_r2": (178, 33, 786)}
+         "c30_5000_r0": (178, 33, 867), "c30_5000_r1": (178, 33, 666), "c30_5000_r2": (178, 33, 786),
+         "c30_5000_theta2000": (178, 33, 866), "c30_5000_nu1e-4": (178, 33, 666)}
 
 
 def _check(O, gold, key, bound=0.0):
@@ -24,6 +25,8 @@ def _check(O, gold, key, bound=0.0):
     cfg, U, tend = O.case_init(case, n)
     O.reference_mode(cfg, "PC")
     cfg.relax = relax
+    if f"{key}/rates" in gold.files:
+        cfg.theta_rate, cfg.nu_rate = (float(v) for v in gold[f"{key}/rates"])
     o = O.Oracle(cfg)
     o.set_state(U)
     assert o.run(tend) == 0
@@ -44,10 +47,11 @@ def test_cavitation_tube_reproduced_bit_for_bit(oracle_lib, key):
     _check(oracle_lib, np.load(GOLD), key)
 
 
-@pytest.mark.parametrize("key", ["c30_5000_r1", "c30_5000_r2"])
+@pytest.mark.parametrize("key", ["c30_5000_r1", "c30_5000_r2", "c30_5000_theta2000", "c30_5000_nu1e-4"])
 def test_nasg_cavitation_tube_reproduced_bit_for_bit(oracle_lib, key):
-    """Pelanti's water cavitation tube (case 30: NASG liquid with covolume b = 6.61e-4, vapour): levels 1 and 2 bit for
-    bit with their Newton counts."""
+    """Pelanti's water cavitation tube (case 30: NASG liquid with covolume b = 6.61e-4, vapour): levels 1 and 2, and the
+    finite rates (level 1 with theta = 2000: the theta closure, PHY:679-700; level 2 with nu = 1e-4: the nu closure,
+    PHY:716-753; backward-Euler form), bit for bit with their Newton counts."""
     _check(oracle_lib, np.load(GOLD), key)
 
 
