@@ -117,7 +117,7 @@ struct spb6_ctx {
 	double *eTV1 = nullptr, *eTV2 = nullptr, *ETV = nullptr, *dA = nullptr, *donax = nullptr, *donay = nullptr;
 	int* ctype = nullptr;   // relaxation levels 1-3: the closure type of each interior cell for the step (STEP_TYPE)
 	double eps_i = 1e-4;    // EPS_I (PHY:38)
-	SatDev sat{nullptr, nullptr, 0, 0.0, 0.0};   // the saturation table on the device (relaxation levels 2-3)
+	RelaxDev rel{nullptr, nullptr, 0, 0.0, 0.0, 0.0, 0.0};   // the saturation table on the device (levels 2-3) and the rates
 	// graph mode (cfg.graph): a stage's fixed head (combination, explicit part, preparation, shock budget) and each
 	// Krylov chunk (its BiCGStab passes, the FP32 form's defect, sweep close and trial residual) run as captured graphs,
 	// re-captured per launch and applied to the instantiated graph of the same shape with hipGraphExecUpdate
@@ -160,7 +160,7 @@ Dev make_dev(const spb6_ctx* c) {
 	D.eTV1 = c->eTV1; D.eTV2 = c->eTV2; D.ETV = c->ETV; D.dA = c->dA; D.donax = c->donax; D.donay = c->donay;
 	if (D.variant != VAR_PC) { D.wxL = c->fwxL; D.wxR = c->fwxR; D.wyL = c->fwyL; D.wyR = c->fwyR; }
 	D.ctype = c->ctype;
-	D.sat = c->sat;
+	D.rel = c->rel;
 	return D;
 }
 // the saturation table T_sat(ln p) (PHY:125-152), built on the host with the device's portable exp, log and pow: 1600
@@ -652,7 +652,7 @@ void set_tab(spb6_ctx* c, int rk) {   // PC:1195-1218: 1 backward Euler, 2 LSDIR
 // one attempt (PC:1243-1285): on success Ust / Pst hold the new state and pressure
 int attempt(spb6_ctx* c, double dt, int rk) {
 	set_tab(c, rk);
-	if (c->ctype) launch_relax_activate(c->g, c->ph, c->sat, c->cfg.relax, c->U, c->eps_i, c->ctype, c->st);   // PC:1247
+	if (c->ctype) launch_relax_activate(c->g, c->ph, c->rel, c->cfg.relax, c->U, c->eps_i, c->ctype, c->st);   // PC:1247
 	const size_t n = c->g.n;
 	HIPCHK(hipMemcpyAsync(c->Pst, c->P, n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 	c->final_pending = false;
@@ -832,15 +832,17 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 		if (hipMemset(c->ctype, 0, n * sizeof(int)) != hipSuccess) return fail("alloc");
 		if (cfg->eps_i > 0) c->eps_i = cfg->eps_i;
 	}
+	c->rel.theta = cfg->theta_rate > 0 ? cfg->theta_rate : (cfg->relax >= 1 ? INFINITY : 0.0);                 // PC:1148-1149
+	c->rel.nu = cfg->nu_rate > 0 ? cfg->nu_rate : ((cfg->relax == 2 || cfg->relax == 3) ? INFINITY : 0.0);
 	if (cfg->relax >= 2) {   // the saturation line before the run (PC:1151)
 		std::vector<double> lpv, Tv;
-		build_sat(c->ph, lpv, Tv, c->sat.pcrit, c->sat.vcrit);
+		build_sat(c->ph, lpv, Tv, c->rel.pcrit, c->rel.vcrit);
 		if (lpv.size() < 2) return fail("the phases have no saturation line (relaxation levels 2-3 need one)");
 		double *dl = nullptr, *dT = nullptr;
 		if (dalloc(c, &dl, lpv.size()) || dalloc(c, &dT, Tv.size())) return fail("alloc");
 		if (hipMemcpy(dl, lpv.data(), lpv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
 		    hipMemcpy(dT, Tv.data(), Tv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return fail("upload");
-		c->sat.lp = dl; c->sat.T = dT; c->sat.n = (int)lpv.size();
+		c->rel.lp = dl; c->rel.T = dT; c->rel.n = (int)lpv.size();
 	}
 	double** arrs[] = {&c->phx, &c->phy, &c->a2s, &c->ubx, &c->uby, &c->vsc, &c->esc, &c->vu, &c->vv, &c->ust, &c->vst, &c->cz, &c->Kw, &c->Pold,
 	                   &c->ubsx, &c->kpx, &c->wLx, &c->pvx, &c->ubsy, &c->kpy, &c->wLy, &c->pvy};
@@ -1011,7 +1013,7 @@ int spb6_step(spb6_ctx* c, double dt, int rk) {
 int spb6_attempt_begin(spb6_ctx* c) {
 	if (!c) return SPB6_BAD_ARG;
 	hipSetDevice(c->device);
-	if (c->ctype) launch_relax_activate(c->g, c->ph, c->sat, c->cfg.relax, c->U, c->eps_i, c->ctype, c->st);   // PC:1247
+	if (c->ctype) launch_relax_activate(c->g, c->ph, c->rel, c->cfg.relax, c->U, c->eps_i, c->ctype, c->st);   // PC:1247
 	HIPCHK(hipMemcpyAsync(c->Pst, c->P, (size_t)c->g.n * sizeof(double), hipMemcpyDeviceToDevice, c->st));
 	c->final_pending = false;
 	c->qe_cur = c->qs_cur = c->U;
@@ -1128,7 +1130,7 @@ int spb6_relax_activate(spb6_ctx* c) {
 	if (!c) return SPB6_BAD_ARG;
 	hipSetDevice(c->device);
 	if (!c->ctype) return SPB6_OK;
-	launch_relax_activate(c->g, c->ph, c->sat, c->cfg.relax, c->U, c->eps_i, c->ctype, c->st);
+	launch_relax_activate(c->g, c->ph, c->rel, c->cfg.relax, c->U, c->eps_i, c->ctype, c->st);
 	HIPCHK(hipStreamSynchronize(c->st));
 	return SPB6_OK;
 }

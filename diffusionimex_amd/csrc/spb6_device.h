@@ -701,7 +701,8 @@ SPB_HD void stencil_state(const Phys& P, const double* c, double* st) {
 }
 
 // ------------------------------------------------------------------ p closure (PHY:420-494, PHO:472-548)
-struct ClIn { double atr0, atr1, mtr0, mtr1, etr0, etr1, s, rhoeC, pguess, Q0, Q1; };
+// tau: the stage's implicit step a_ii dt (read by the finite-rate closures only)
+struct ClIn { double atr0, atr1, mtr0, mtr1, etr0, etr1, s, rhoeC, pguess, Q0, Q1; double tau = 0.0; };
 // the closure's phase contents: volume fractions, masses (the transported ones but where the pTg closure moves mass)
 // and energies, and the pressure
 struct ClOut { double a0, a1, m0, m1, e0, e1, pbar; };
@@ -836,8 +837,8 @@ SPB_HD double closure_roundoff(const Phys& P, const ClIn& in, double p) {
 }
 
 // ------------------------------------------------------------------ relaxation level 1: the pT closure (PHY:183-230, PHY:660-672)
-// closure types (PHY:226): mechanical (p), thermal (pT) and chemical (pTg) equilibrium
-enum { CL_P = 0, CL_PT = 2, CL_PTG = 5 };
+// closure types (PHY:226): mechanical (p), thermal (pT; finite rate theta) and chemical (pTg; finite rate nu) equilibrium
+enum { CL_P = 0, CL_PT = 2, CL_THETA = 3, CL_NU = 4, CL_PTG = 5 };
 // Brent's method on a sign-changing bracket (PHY:183-204), operation for operation
 template <class F>
 SPB_HD bool brent_root(F f, double a, double b, double fa, double fb, double tol, double& root) {
@@ -928,8 +929,9 @@ SPB_HD bool closure_pT(const Phys& P, const ClIn& in, ClOut& o, double* T_out = 
 // ------------------------------------------------------------------ relaxation levels 2-3: the saturation line and the pTg closure
 // (PHY:111-163, PHY:756-778), with the portable log of the device
 // the saturation table (built on the host by the same arithmetic, PHY:125-152): n points of ln p on a uniform grid and
-// T_sat there, and the critical point (pcrit, vcrit)
-struct SatDev { const double* lp; const double* T; int n; double pcrit, vcrit; };
+// T_sat there, and the critical point (pcrit, vcrit); and the thermal and chemical relaxation rates theta, nu (PHY:33,
+// resolved by level: inf = instantaneous)
+struct RelaxDev { const double* lp; const double* T; int n; double pcrit, vcrit, theta, nu; };
 SPB_HD double s_pT(const Phys& P, int k, double p, double T) {                      // PHY:80
 	return P.cv[k] * (P.g[k] * plog(T) - (P.g[k] - 1.0) * plog(p + P.pi[k])) + P.qp[k]; }
 SPB_HD double g_pT(const Phys& P, int k, double p, double T) {                      // PHY:81
@@ -948,7 +950,7 @@ SPB_HD double sat_newton(const Phys& P, double p, double Tg, bool& ok) {
 	ok = fabs(sat_f(P, p, T)) < 1e-8 * fabs(g_pT(P, 0, p, T)) + 1e-8; return T;
 }
 // T_sat(p) (PHY:154-162): Newton from the table's linear interpolation in ln p; NaN outside (0, pcrit) or on failure
-SPB_HD double Tsat(const Phys& P, const SatDev& S, double p) {
+SPB_HD double Tsat(const Phys& P, const RelaxDev& S, double p) {
 	if (!(p > 0) || !(p < S.pcrit) || S.n < 1) return dnan();
 	const double lp = plog(p); double Tg;
 	if (lp <= S.lp[0]) Tg = S.T[0];
@@ -968,7 +970,7 @@ SPB_HD double T_from_energy(const Phys& P, double p, double Y, double eC) {
 }
 // the pTg state at p (PHY:757-767): T = T_sat(p) and Y from the energy, clamped to [YMIN, 1 - YMIN] (then T from the
 // energy); at p >= pcrit a single phase by v against vcrit
-SPB_HD bool pTg_state(const Phys& P, const SatDev& S, double p, double rho, double eC, double& Y, double& T) {
+SPB_HD bool pTg_state(const Phys& P, const RelaxDev& S, double p, double rho, double eC, double& Y, double& T) {
 	const double Ts = Tsat(P, S, p);
 	if (isfin(Ts)) {
 		const double el = e_pT(P, 0, p, Ts), ev = e_pT(P, 1, p, Ts);
@@ -983,7 +985,7 @@ SPB_HD bool pTg_state(const Phys& P, const SatDev& S, double p, double rho, doub
 }
 // the pTg closure (B6.6, PHY:768-778): mechanical, thermal and chemical equilibrium, the root of rho sum_k Y_k v_k = 1
 // above max(-pi_k); the masses move (vapour fraction Y). closure()'s input checks (PHY:787-790) first.
-SPB_HD bool closure_pTg(const Phys& P, const SatDev& S, const ClIn& in, ClOut& o) {
+SPB_HD bool closure_pTg(const Phys& P, const RelaxDev& S, const ClIn& in, ClOut& o) {
 	if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0)) return false;
 	if (!(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
 	if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
@@ -998,30 +1000,117 @@ SPB_HD bool closure_pTg(const Phys& P, const SatDev& S, const ClIn& in, ClOut& o
 	o.pbar = p;
 	return isfin(o.pbar);
 }
-// the closure of type ty (CL_P, CL_PT, CL_PTG; PHY:786-810)
-SPB_HD bool closure_of(const Phys& P, const SatDev* S, int ty, const ClIn& in, ClOut& o) {
-	return ty == CL_PT ? closure_pT(P, in, o) : ty == CL_PTG ? closure_pTg(P, *S, in, o) : closure_p(P, in, o);
+// ---- finite relaxation rates, backward-Euler form (RELAX_FORM 0, the reference's default)
+// the p closure's coefficients of the defect segment at pseg (p_coef, PHY:443-454): A_k, B_k, N_k
+SPB_HD bool p_coef_at(const Phys& P, const ClIn& in, double pseg, double A[2], double B[2], double N[2]) {
+	if (!(in.mtr0 + in.mtr1 > 0)) return false;
+	const double D0 = in.rhoeC - in.etr0 - in.etr1, sm = in.mtr0 + in.mtr1;
+	const double E0 = E_above(P, 0, in.atr0, in.mtr0, in.etr0), E1 = E_above(P, 1, in.atr1, in.mtr1, in.etr1);
+	double sE = 0.0; sE += E0; sE += E1;
+	double lE0, lE1;
+	if (sE > 0) { lE0 = E0 / sE; lE1 = E1 / sE; }
+	else { lE0 = sm > 0 ? in.mtr0 / sm : 0.5; lE1 = sm > 0 ? in.mtr1 / sm : 0.5; }
+	double lam0, lam1, ex0, ex1;
+	defect_seg(D0 + pseg * in.s, in.Q0 + in.Q1, in.Q0, in.Q1, lE0, lE1, lam0, lam1, ex0, ex1);
+	A[0] = in.etr0 - in.mtr0 * P.q[0] + lam0 * D0 + ex0; B[0] = in.atr0 + lam0 * in.s; N[0] = A[0] - (B[0] - in.mtr0 * P.b[0]) * P.pi[0];
+	A[1] = in.etr1 - in.mtr1 * P.q[1] + lam1 * D0 + ex1; B[1] = in.atr1 + lam1 * in.s; N[1] = A[1] - (B[1] - in.mtr1 * P.b[1]) * P.pi[1];
+	return true;
+}
+// finite theta (B6.4, PHY:679-700): at fixed p the phases' temperatures are linear in the exchanged heat, and
+// dQ = tau theta (T2 - T1) / (1 + tau theta S), S = 1/Cp1 + 1/Cp2, shifts the p closure's volume fractions; the
+// coefficient segment is the one at the guess, the root of sum alpha_k = 1 by the bracket search above max(-pi_k)
+SPB_HD bool closure_theta(const Phys& P, double theta, const ClIn& in, ClOut& o) {
+	if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0)) return false;
+	if (!(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
+	if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
+	double A[2], B[2], N[2];
+	if (!p_coef_at(P, in, in.pguess, A, B, N)) return false;
+	const double tt = __builtin_isinf(theta) ? dinf() : in.tau * theta;
+	if (tt > 0 && !__builtin_isinf(tt)) { if (!(N[0] > 0) || !(N[1] > 0)) return false; }
+	const double m[2] = {in.mtr0, in.mtr1};
+	const double Cp0 = P.g[0] * P.cv[0] * m[0], Cp1 = P.g[1] * P.cv[1] * m[1];
+	const double S = 1.0 / Cp0 + 1.0 / Cp1;
+	auto eval = [&](double p, double& a0, double& a1) {
+		double al[2], T0[2];
+		for (int k = 0; k < 2; k++) {
+			al[k] = ((P.g[k] - 1.0) * (A[k] + B[k] * p) + (p + P.g[k] * P.pi[k]) * m[k] * P.b[k]) / (P.g[k] * (p + P.pi[k]));   // alpha_p
+			T0[k] = T_amp(P, k, al[k], m[k], p);
+		}
+		double dQ;
+		if (__builtin_isinf(tt)) dQ = (T0[1] - T0[0]) / S;
+		else dQ = tt * (T0[1] - T0[0]) / (1.0 + tt * (1.0 / Cp0 + 1.0 / Cp1));
+		a0 = al[0] + (P.g[0] - 1.0) * dQ / (P.g[0] * (p + P.pi[0]));
+		a1 = al[1] - (P.g[1] - 1.0) * dQ / (P.g[1] * (p + P.pi[1]));
+	};
+	auto Phi = [&](double p) { double a0, a1; eval(p, a0, a1); return a0 + a1 - 1.0; };
+	double p; if (!root_above(Phi, smax(-P.pi[0], -P.pi[1]), in.pguess, p)) return false;
+	eval(p, o.a0, o.a1);
+	if (!(o.a0 > m[0] * P.b[0]) || !(o.a1 > m[1] * P.b[1])) return false;
+	o.m0 = m[0]; o.m1 = m[1];
+	o.e0 = eps_amp(P, 0, o.a0, m[0], p); o.e1 = eps_amp(P, 1, o.a1, m[1], p);
+	o.pbar = p;
+	return isfin(o.pbar);
+}
+// finite nu (B6.5, PHY:716-753): at each p the vapour fraction Y solves rho Y - m2 + tau nu (g2 - g1)(p, T(p, Y)) = 0 on
+// its admissible interval (clamped at the ends), T from the mixture energy; the root of rho sum_k Y_k v_k = 1
+SPB_HD bool closure_nu(const Phys& P, double nu, const ClIn& in, ClOut& o) {
+	if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0)) return false;
+	if (!(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
+	if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
+	const double rho = in.mtr0 + in.mtr1, eC = in.rhoeC / rho, tn = __builtin_isinf(nu) ? dinf() : in.tau * nu;
+	auto Yof = [&](double p, double& Y, double& T) -> bool {
+		auto R = [&](double y) {
+			const double TT = T_from_energy(P, p, y, eC); if (!(TT > 0)) return dnan();
+			if (__builtin_isinf(tn)) return g_pT(P, 1, p, TT) - g_pT(P, 0, p, TT);
+			return rho * y - in.mtr1 + tn * (g_pT(P, 1, p, TT) - g_pT(P, 0, p, TT)); };
+		double ya = YMIN, yb = 1.0 - YMIN;
+		const double dq = P.q[1] - P.q[0], yT = dq != 0.0 ? (eC - P.q[0]) / dq : dinf();
+		if (dq > 0) yb = smin(yb, yT * (1.0 - 1e-12)); else if (dq < 0) ya = smax(ya, yT * (1.0 + 1e-12));
+		if (!(ya < yb)) return false;
+		const double Ra = R(ya), Rb = R(yb);
+		if (!isfin(Ra) || !isfin(Rb)) return false;
+		if (Ra >= 0) Y = ya; else if (Rb <= 0) Y = yb; else if (!brent_root(R, ya, yb, Ra, Rb, 1e-16, Y)) return false;
+		T = T_from_energy(P, p, Y, eC); return T > 0;
+	};
+	auto Phi = [&](double p) { double Y, T; if (!Yof(p, Y, T)) return dnan(); return rho * ((1 - Y) * v_pT(P, 0, p, T) + Y * v_pT(P, 1, p, T)) - 1.0; };
+	double p; if (!root_above(Phi, smax(-P.pi[0], -P.pi[1]), in.pguess, p)) return false;
+	double Y, T; if (!Yof(p, Y, T)) return false;
+	o.m0 = rho * (1 - Y); o.m1 = rho * Y;
+	o.a0 = o.m0 * v_pT(P, 0, p, T); o.a1 = o.m1 * v_pT(P, 1, p, T); o.e0 = o.m0 * e_pT(P, 0, p, T); o.e1 = o.m1 * e_pT(P, 1, p, T);
+	o.pbar = p;
+	return isfin(o.pbar);
+}
+// the closure of type ty (CL_P, CL_PT, CL_THETA, CL_NU, CL_PTG; PHY:786-810)
+SPB_HD bool closure_of(const Phys& P, const RelaxDev* S, int ty, const ClIn& in, ClOut& o) {
+	switch (ty) {
+	case CL_PT: return closure_pT(P, in, o);
+	case CL_THETA: return closure_theta(P, S->theta, in, o);
+	case CL_NU: return closure_nu(P, S->nu, in, o);
+	case CL_PTG: return closure_pTg(P, *S, in, o);
+	default: return closure_p(P, in, o);
+	}
 }
 // the cell's closure type of the step at relaxation level `level` (PHY:814-830), from its content and specific energy:
 // beyond p only where eps_I <= alpha_1 <= 1 - eps_I; level 1 thermal; level 2 chemical where the pT equilibrium's T
 // exceeds T_sat(p), else thermal; level 3 chemical where a phase's T at the mechanical equilibrium without the energy
 // defect exceeds T_sat(p), else mechanical
-SPB_HD int closure_type_of(const Phys& P, const SatDev* S, int level, double eps_i, double a1, double m1, double m2, double e1, double e2, double rhoe) {
+SPB_HD int closure_type_of(const Phys& P, const RelaxDev* S, int level, double eps_i, double a1, double m1, double m2, double e1, double e2, double rhoe) {
 	if (level == 0) return CL_P;
 	if (!(a1 >= eps_i && a1 <= 1.0 - eps_i)) return CL_P;
 	ClIn in; in.atr0 = a1; in.atr1 = 1 - a1; in.mtr0 = m1; in.mtr1 = m2; in.etr0 = e1; in.etr1 = e2; in.s = 0; in.rhoeC = rhoe;
 	in.Q0 = 0.0; in.Q1 = 0.0; in.pguess = smax(p_mix(P, a1, m1, m2, rhoe), 1.0);
+	const int chem = __builtin_isinf(S->nu) ? CL_PTG : CL_NU, therm = __builtin_isinf(S->theta) ? CL_PT : CL_THETA;
 	if (level == 3) {
 		ClOut o; if (!closure_p<true>(P, in, o)) return CL_P;
 		const double Ts = Tsat(P, *S, o.pbar);
 		const double T0 = T_amp(P, 0, o.a0, o.m0, o.pbar), T1 = T_amp(P, 1, o.a1, o.m1, o.pbar);
-		return (isfin(Ts) && (T0 > Ts || T1 > Ts)) ? CL_PTG : CL_P;
+		return (isfin(Ts) && (T0 > Ts || T1 > Ts)) ? chem : CL_P;
 	}
 	if (level == 2) {
 		ClOut o; double T;
-		if (closure_pT(P, in, o, &T)) { const double Ts = Tsat(P, *S, o.pbar); if (isfin(Ts) && T > Ts) return CL_PTG; }
+		if (closure_pT(P, in, o, &T)) { const double Ts = Tsat(P, *S, o.pbar); if (isfin(Ts) && T > Ts) return chem; }
 	}
-	return CL_PT;
+	return therm;
 }
 
 }   // namespace spb

@@ -39,7 +39,7 @@ struct Dev {
 	int variant;
 	double *eTV1, *eTV2, *ETV, *dA, *donax, *donay;
 	const int* ctype;   // relaxation levels 1-3: each interior row's closure type of the step (CL_P / CL_PT / CL_PTG); null at level 0
-	SatDev sat;         // the saturation table (relaxation levels 2-3; n = 0 otherwise)
+	RelaxDev rel;         // the saturation table (relaxation levels 2-3; n = 0 otherwise) and the finite rates
 };
 // the IMEX variants of the reference (its sibling programs SPB6_<name>_IMEX.cpp); VAR_PC is SPB6_PC_IMEX.cpp
 enum { VAR_PC = 0, VAR_TV = 1, VAR_TVPURE = 2, VAR_TVINC = 3, VAR_TVSPLIT = 4, VAR_TVBB = 5, VAR_BB = 6 };
@@ -126,7 +126,7 @@ void launch_floor_var(const Dev& D, const double* P, double tol, hipStream_t st)
 void launch_final_var(const Dev& D, const double* P, const double* Qs, double adt, double* H, unsigned long long* tmax, hipStream_t st, int flag_slot);
 // relaxation level 1 (RelaxationActivation, PC:537-560): each interior cell's closure type into ctype, the cells of
 // type pT projected onto the pT equilibrium in U (a failed projection keeps the cell mechanical), then U's ghosts
-void launch_relax_activate(const Geo& g, const Phys& P, const SatDev& S, int level, double* U, double eps_i, int* ctype, hipStream_t st);
+void launch_relax_activate(const Geo& g, const Phys& P, const RelaxDev& S, int level, double* U, double eps_i, int* ctype, hipStream_t st);
 
 // ---- the explicit HLLC scheme (spb6_hllc.hip): cell primitives (hllc_prim_fields arrays over the ghosted grid), face
 // values (hllc_face_fields arrays per direction, at the face's low cell); every pass returns at once once flags

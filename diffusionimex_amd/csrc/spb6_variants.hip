@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(256) k_final_var(Dev D, const double* __restri
 // content and specific energy (cell_type_U, PC:517-521), and project_cell moves a pT or pTg cell onto that closure's
 // equilibrium (s = 0, the guess max(p_mix, 1); the pTg closure moves mass); a cell whose projection fails stays
 // mechanical and unprojected
-__global__ void __launch_bounds__(256) k_relax_activate(Geo g, Phys P, SatDev S, int level, double* __restrict__ U, double eps_i,
+__global__ void __launch_bounds__(256) k_relax_activate(Geo g, Phys P, RelaxDev S, int level, double* __restrict__ U, double eps_i,
                                                        int* __restrict__ ctype) {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= g.n) return;
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(256) k_relax_activate(Geo g, Phys P, SatDev S,
 
 }   // namespace
 
-void launch_relax_activate(const Geo& g, const Phys& P, const SatDev& S, int level, double* U, double eps_i, int* ctype, hipStream_t st) {
+void launch_relax_activate(const Geo& g, const Phys& P, const RelaxDev& S, int level, double* U, double eps_i, int* ctype, hipStream_t st) {
 	hipLaunchKernelGGL(k_relax_activate, dim3(nb(g.n)), dim3(256), 0, st, g, P, S, level, U, eps_i, ctype);
 	launch_bc_grid(g, U, st);
 }

@@ -37,7 +37,7 @@ class Config(C.Structure):
         ("limiter", C.c_int), ("order", C.c_int), ("rus_local", C.c_int),
         ("krylov_fp32", C.c_int), ("learned_guess", C.c_int), ("analytic_jacobian", C.c_int),
         ("jreuse", C.c_int), ("variant", C.c_int), ("relax", C.c_int), ("eps_i", C.c_double), ("graph", C.c_int),
-        ("shock_fused", C.c_int),
+        ("shock_fused", C.c_int), ("theta_rate", C.c_double), ("nu_rate", C.c_double),
     ]
 
 
@@ -167,7 +167,7 @@ def _dp(a):
 def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0, 1.0), bc=(0, 0, 0, 0), bcu=(0, 0, 0, 0),
                 rk=2, cfl=0.2, cfl_ac=0.4, ramp=1, dtfix=0.0, acdt=0, newton_tol=1e-13, newton_iter=60,
                 krylov_tol=1e-13, krylov_iter=4000, limiter=1, order=2, rus_local=1, krylov_fp32=0, learned_guess=0,
-                analytic_jacobian=0, jreuse=0, variant=0, relax=0, eps_i=0.0, graph=0, shock_fused=0):
+                analytic_jacobian=0, jreuse=0, variant=0, relax=0, eps_i=0.0, graph=0, shock_fused=0, theta_rate=0.0, nu_rate=0.0):
     """Config with the reference defaults (PC:57-80): phases are (gamma, p_inf[, q, qp, cv, b])."""
     c = Config()
     c.nx, c.ny = nx, ny
@@ -189,6 +189,7 @@ def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0
     c.relax, c.eps_i = relax, eps_i
     c.graph = graph
     c.shock_fused = shock_fused
+    c.theta_rate, c.nu_rate = theta_rate, nu_rate
     return c
 
 

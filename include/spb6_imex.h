@@ -108,6 +108,10 @@ typedef struct spb6_config {
                                         ten) and FMA stage sums; the same equation and step control, rounded differently
                                         (the shares agree with the reference arithmetic's to ~1e-15 relative). Grid
                                         stepper only (the batch rejects it) */
+    double theta_rate, nu_rate;      /* theta_rate, nu_rate (PHY:33): finite thermal / chemical relaxation rates at
+                                        relaxation levels >= 1 / 2-3 (the theta closure PHY:679-700, the nu closure
+                                        PHY:716-753, backward-Euler form); <= 0: the level's default, instantaneous
+                                        (PC:1148-1149) */
 } spb6_config_t;
 
 /* run statistics: the counters of the reference's stats line (PC:1337-1339) plus Krylov counts */

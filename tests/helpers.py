@@ -13,7 +13,7 @@ def config_from_oracle(ocfg, **over):
                     newton_tol=ocfg.newton_tol, newton_iter=ocfg.newton_iter, krylov_tol=ocfg.krylov_tol,
                     krylov_iter=ocfg.krylov_iter, limiter=ocfg.limiter, order=ocfg.order, rus_local=ocfg.rus_local,
                     jreuse=ocfg.jreuse, variant=ocfg.variant, relax=ocfg.relax, eps_i=ocfg.eps_i,
-                    shock_fused=ocfg.shock_fused)
+                    shock_fused=ocfg.shock_fused, theta_rate=max(ocfg.theta_rate, 0.0), nu_rate=max(ocfg.nu_rate, 0.0))
     for k, v in over.items():
         setattr(c, k, v)
     return c
@@ -28,6 +28,8 @@ def oracle_cfg_from(scfg, O, **over):
               "limiter", "order", "rus_local", "jreuse", "variant", "relax", "shock_fused"):
         setattr(c, k, getattr(scfg, k))
     c.eps_i = scfg.eps_i if scfg.eps_i > 0 else 1e-4
+    c.theta_rate = scfg.theta_rate if scfg.theta_rate > 0 else -1.0
+    c.nu_rate = scfg.nu_rate if scfg.nu_rate > 0 else -1.0
     for k, v in over.items():
         setattr(c, k, v)
     return c

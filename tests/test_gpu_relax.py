@@ -189,3 +189,25 @@ def test_nasg_tube_bitwise(relax):
         assert st.relax_activate() == 0
     assert st.stats()["steps"] == 178
     _check(o, st, f"nasg5000_r{relax}")
+
+
+@pytest.mark.parametrize("rates", [(1, 2000.0, 0.0), (2, 0.0, 1e-4)], ids=["theta2000", "nu1e-4"])
+def test_finite_rates_bitwise(rates):
+    """the finite-rate closures on Pelanti's tube (case 30): level 1 with theta = 2000 and level 2 with nu = 1e-4 (the
+    runs the reference archived), bit-identical to the oracle's device mode"""
+    from diffusionimex_amd import spb6
+    from oracle import oracle as O
+    relax, th, nu = rates
+    ocfg, U, tend = O.case_init(30, 5000)
+    scfg = config_from_oracle(ocfg, relax=relax, theta_rate=th, nu_rate=nu)
+    oc = oracle_cfg_from(scfg, O)
+    assert (oc.theta_rate, oc.nu_rate) == (th if th > 0 else -1.0, nu if nu > 0 else -1.0)
+    o = O.Oracle(oc)
+    o.set_state(U)
+    assert o.run(tend) == 0
+    st = spb6.Stepper(scfg)
+    st.set_state(U)
+    assert st.run(tend) == 0
+    o.L.ora_relax_activate(o.h)
+    assert st.relax_activate() == 0
+    _check(o, st, f"nasg5000_r{relax}_th{th}_nu{nu}")
