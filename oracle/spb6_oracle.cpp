@@ -1835,6 +1835,14 @@ static bool case_setup(int test, Cfg& C, double& tend) {                        
 		set_SG(C, 0, 2.35, 1e9); C.ph[0][2] = -1167e3; C.ph[0][4] = 1816.0;
 		set_SG(C, 1, 1.43, 0.0); C.ph[1][2] = 2030e3; C.ph[1][3] = -23.4e3; C.ph[1][4] = 1040.0;
 		tend = test == 1 ? 3.2e-3 : 0.58e-3; break;
+	case 3:   // PHY:968-969: Pelanti-Shyue dodecane liquid-vapour shock tube (SG), material CFL 0.05
+		set_SG(C, 0, 2.35, 4e8); C.ph[0][2] = -775.269e3; C.ph[0][4] = 1077.7;
+		set_SG(C, 1, 1.025, 0.0); C.ph[1][2] = -237.547e3; C.ph[1][3] = -24.4e3; C.ph[1][4] = 1956.45;
+		C.cfl = 0.05; tend = 473e-6; break;
+	case 31: {   // PHY:980: Pelanti's dodecane shock tube with the NASG liquid (covolume 7.51e-4)
+		const double l[6] = {1.206, 1681e5, -996054.0, 0.0, 2532.0, 7.51e-4}, v[6] = {1.021, 0.0, -384592.0, -4301.0, 2274.0, 0.0};
+		for (int q = 0; q < 6; q++) { C.ph[0][q] = l[q]; C.ph[1][q] = v[q]; }
+		tend = 473e-6; break; }
 	case 30: {   // PHY:979: Pelanti's water cavitation tube, NASG liquid (covolume 6.61e-4) and vapour
 		const double l[6] = {1.187, 7028e5, -1177788.0, 0.0, 3610.0, 6.61e-4}, v[6] = {1.467, 0.0, 2077616.0, 14317.0, 955.0, 0.0};
 		for (int q = 0; q < 6; q++) { C.ph[0][q] = l[q]; C.ph[1][q] = v[q]; }
@@ -1861,6 +1869,8 @@ static void case_state(const Model& M, int test, double x, double y, double* Q) 
 	case 1: case 2: {   // PHY:1034-1036: T = 354.728 K, p = 1e5 Pa, u = -U0 | +U0 about x = 0.5, vapour fraction 1e-2
 		const double T0 = 354.728, U0 = test == 1 ? 2.0 : 500.0, u = x < 0.5 ? -U0 : U0;
 		setcell(1 - 1e-2, 1.0 / v_pT(M, 0, 1e5, T0), 1.0 / v_pT(M, 1, 1e5, T0), u, 0, 1e5, 1e5); break; }
+	case 3: case 31: {   // PHY:1039: liquid at 1e8 Pa left of x = 0.75, vapour at 1e5 Pa right of it, at rest
+		const bool L = x < 0.75; setcell(L ? 1 - 1e-8 : 1e-8, 500.0, 2.0, 0, 0, L ? 1e8 : 1e5, L ? 1e8 : 1e5); break; }
 	case 30: {   // PHY:1085: T = 353 K, p = 1e5 Pa, u = -2 | +2 m/s about x = 0.5, vapour fraction 1e-2
 		const double T0 = 353.0, u = x < 0.5 ? -2.0 : 2.0;
 		setcell(1 - 1e-2, 1.0 / v_pT(M, 0, 1e5, T0), 1.0 / v_pT(M, 1, 1e5, T0), u, 0, 1e5, 1e5); break; }

@@ -4,7 +4,9 @@ Source files (data, not code) under the reference tree: data/Schropff2026_Rieman
 data/SPB2009_4.3.1_water_air/water_air_N100_t240us.dat (case 5), data/SPB2009_4.1_interface_advection/
 interface_advection_t2.79ms.dat (case 4, N = 200). Columns kept (17 significant digits in the source): p u v rho alpha2
 Y2 rho1 rho2 p1 p2 (write_profile, PC:1002-1013); and data/PS2014_water_cavitation_tube_{2mps,500mps}/ (cases 1 and 2,
-5000 cells, relaxation levels 0 to 3) and data/Pelanti2022_8.2.1_water_cavitation/ (case 30, levels 0 to 2). Run: python tests/golden/make_ref_profiles.py /root/reference
+5000 cells, relaxation levels 0 to 3), data/Pelanti2022_8.2.1_water_cavitation/ (case 30, levels 0 to 2),
+data/PS2014_dodecane_shock_tube/ (case 3, 2000 cells, levels 0, 2, 3) and data/Pelanti2022_8.2.2_dodecane_NASG/
+(case 31, levels 0, 2). Run: python tests/golden/make_ref_profiles.py /root/reference
 """
 import os
 import sys
@@ -30,7 +32,14 @@ RELAX_FILES = {"c1_5000_r0": (1, 5000, 0, "data/PS2014_water_cavitation_tube_2mp
                "c30_5000_r2": (30, 5000, 2, "data/Pelanti2022_8.2.1_water_cavitation/pTg_relax.dat"),
                # and its finite-rate runs: level 1 with theta = 2000, level 2 with nu = 1e-4 (meta: case, n, level, theta, nu)
                "c30_5000_theta2000": (30, 5000, 1, "data/Pelanti2022_8.2.1_water_cavitation/pT_finite_theta2000.dat"),
-               "c30_5000_nu1e-4": (30, 5000, 2, "data/Pelanti2022_8.2.1_water_cavitation/pTg_finite_nu1e-4.dat")}
+               "c30_5000_nu1e-4": (30, 5000, 2, "data/Pelanti2022_8.2.1_water_cavitation/pTg_finite_nu1e-4.dat"),
+               # the dodecane liquid / vapour shock tubes (2000 cells): Pelanti-Shyue's SG form (case 3, material CFL
+               # 0.05) at levels 0, 2 and 3, and Pelanti's NASG form (case 31) at levels 0 and 2
+               "c3_2000_r0": (3, 2000, 0, "data/PS2014_dodecane_shock_tube/p_relax.dat"),
+               "c3_2000_r2": (3, 2000, 2, "data/PS2014_dodecane_shock_tube/p-pT-pTg_relax.dat"),
+               "c3_2000_r3": (3, 2000, 3, "data/PS2014_dodecane_shock_tube/p-pTg_relax.dat"),
+               "c31_2000_r0": (31, 2000, 0, "data/Pelanti2022_8.2.2_dodecane_NASG/p_relax.dat"),
+               "c31_2000_r2": (31, 2000, 2, "data/Pelanti2022_8.2.2_dodecane_NASG/pTg_relax.dat")}
 RATES = {"c30_5000_theta2000": (2000.0, -1.0), "c30_5000_nu1e-4": (-1.0, 1e-4)}
 
 

@@ -211,3 +211,24 @@ def test_finite_rates_bitwise(rates):
     o.L.ora_relax_activate(o.h)
     assert st.relax_activate() == 0
     _check(o, st, f"nasg5000_r{relax}_th{th}_nu{nu}")
+
+
+@pytest.mark.parametrize("case,relax", [(3, 0), (31, 0), (31, 2)], ids=["dodecane_sg_r0", "dodecane_nasg_r0", "dodecane_nasg_r2"])
+def test_dodecane_tube_bitwise(case, relax):
+    """the dodecane liquid / vapour shock tubes (2000 cells, t = 473 us): the SG form (case 3, material CFL 0.05, 2675
+    steps) whose strong shock into the vapour runs the profile shooting in every shocked cell, and the NASG form (case 31,
+    covolume 7.51e-4) at levels 0 and 2, bit-identical to the oracle's device mode"""
+    from diffusionimex_amd import spb6
+    from oracle import oracle as O
+    ocfg, U, tend = O.case_init(case, 2000)
+    scfg = config_from_oracle(ocfg, relax=relax)
+    o = O.Oracle(oracle_cfg_from(scfg, O))
+    o.set_state(U)
+    assert o.run(tend) == 0
+    st = spb6.Stepper(scfg)
+    st.set_state(U)
+    assert st.run(tend) == 0
+    if relax:
+        o.L.ora_relax_activate(o.h)
+        assert st.relax_activate() == 0
+    _check(o, st, f"dodecane{case}_r{relax}")
