@@ -1,5 +1,7 @@
-"""Builds the gfx950 HIP library libspb6_hip.so in-tree with hipcc, and beside it the GPU self-check programs of the
-test suite (tests/bin/), with the same flags.
+"""Builds the gfx950 HIP library libspb6_hip.so in-tree with hipcc, its Mie-Grueneisen build libspb6_hip_mg.so (the same
+sources with SPB6_MG=1: the EOS branches of Cochran-Chan and JWL phases compiled in, which the NASG-only library folds
+away to keep its kernels' registers), and beside them the GPU self-check programs of the test suite (tests/bin/), with
+the same flags.
 
 Flags: -ffp-contract=off keeps every FP64 operation separately rounded (the reference's results are reproduced bit for bit);
 the packed-FP32 target feature is disabled so that no v_pk_*_f32 instruction is generated.
@@ -11,6 +13,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspb6_hip.so")
+LIB_MG = os.path.join(HERE, "libspb6_hip_mg.so")
 SOURCES = ["spb6_kernels.hip", "spb6_variants.hip", "spb6_hllc.hip", "spb6_mlp.hip", "spb6_batch.hip", "spb6_api.hip"]
 ARCH = os.environ.get("SPB6_ARCH", "gfx950")
 TESTS = os.path.join(HERE, "..", "tests")
@@ -30,15 +33,17 @@ def flags():
 
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    objs = []
+    libs = [(LIB, [], ""), (LIB_MG, ["-DSPB6_MG=1"], ".mg")]
+    objs = {lib: [] for lib, _, _ in libs}
     procs = []
-    for s in srcs:
-        o = os.path.join(CSRC, os.path.basename(s) + ".o")
-        cmd = [hipcc()] + flags() + ["-c", s, "-o", o]
-        if verbose:
-            print(" ".join(cmd))
-        procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT), s))
-        objs.append(o)
+    for lib, defs, tag in libs:
+        for s in srcs:
+            o = os.path.join(CSRC, os.path.basename(s) + tag + ".o")
+            cmd = [hipcc()] + flags() + defs + ["-c", s, "-o", o]
+            if verbose:
+                print(" ".join(cmd))
+            procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT), s))
+            objs[lib].append(o)
     os.makedirs(os.path.join(TESTS, "bin"), exist_ok=True)
     for src, exe in CHECKS:
         s = os.path.join(TESTS, src)
@@ -52,15 +57,16 @@ def build(verbose=False):
         if p.returncode != 0:
             sys.stderr.write(out.decode(errors="replace"))
             raise RuntimeError(f"hipcc failed on {s}")
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + flags() + ["-shared", "-o", tmp] + objs
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout.decode(errors="replace"))
-        raise RuntimeError("hipcc link failed")
-    os.replace(tmp, LIB)
-    for o in objs:
-        os.remove(o)
+    for lib, _, _ in libs:
+        tmp = lib + ".tmp"
+        cmd = [hipcc()] + flags() + ["-shared", "-o", tmp] + objs[lib]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout.decode(errors="replace"))
+            raise RuntimeError("hipcc link failed")
+        os.replace(tmp, lib)
+        for o in objs[lib]:
+            os.remove(o)
     return LIB
 
 

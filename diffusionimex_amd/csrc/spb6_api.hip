@@ -789,6 +789,20 @@ extern "C" {
 
 const char* spb6_last_error(void) { return g_err.c_str(); }
 
+// the Mie-Grueneisen parameters of the phases (PHY:49-53) into the kernels' phase block
+static void set_mg(Phys& P, const spb6_config_t* cfg) {
+#if !SPB6_MG
+	(void)P; (void)cfg;
+#else
+	P.mg = 0;
+	for (int k = 0; k < 2; k++) {
+		const double* m = cfg->mg[k];
+		P.kind[k] = cfg->eos_kind[k];
+		if (P.kind[k]) P.mg = 1;
+		P.G[k] = m[0]; P.c1[k] = m[1]; P.c2[k] = m[2]; P.e1[k] = m[3]; P.e2[k] = m[4]; P.rr[k] = m[5]; P.e0[k] = m[6];
+	}
+#endif
+}
 spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (!cfg || cfg->nx < 2 || cfg->ny < 1) { g_err = "bad config"; return nullptr; }
 	if (cfg->rus_local != 1) { g_err = "only the local Rusanov speed (rus_local = 1) is supported"; return nullptr; }
@@ -800,6 +814,16 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (cfg->shock_fused < 0 || cfg->shock_fused > 1) { g_err = "shock_fused must be 0 or 1"; return nullptr; }
 	if (cfg->relax > 0 && (cfg->analytic_jacobian || cfg->learned_guess)) {
 		g_err = "relaxation levels 1-3 run with the finite-difference Jacobian and the plain Newton guess"; return nullptr; }
+	for (int k = 0; k < 2; k++) {
+		if (cfg->eos_kind[k] < 0 || cfg->eos_kind[k] > 2) { g_err = "eos_kind must be 0 (NASG), 1 (Cochran-Chan) or 2 (JWL)"; return nullptr; }
+		if (cfg->eos_kind[k] && !(cfg->mg[k][0] > 0 && cfg->mg[k][5] > 0)) { g_err = "a Mie-Grueneisen phase needs G > 0 and rr > 0"; return nullptr; }
+	}
+	if ((cfg->eos_kind[0] || cfg->eos_kind[1]) && !SPB6_MG) {
+		g_err = "Mie-Grueneisen phases run in libspb6_hip_mg.so (the same sources built with SPB6_MG=1)"; return nullptr; }
+	if ((cfg->eos_kind[0] || cfg->eos_kind[1]) && (cfg->relax || cfg->variant != SPB6_VARIANT_PC || cfg->shock_fused ||
+	                                               cfg->analytic_jacobian || cfg->learned_guess || cfg->krylov_fp32)) {
+		g_err = "a Mie-Grueneisen phase runs at relaxation level 0 with the reference scheme, the finite-difference Jacobian, "
+		        "the FP64 Krylov solve, the plain guess and the exact profile form"; return nullptr; }
 	spb6_ctx* c = new spb6_ctx();
 	c->cfg = *cfg; c->device = device;
 	std::memset(&c->stats, 0, sizeof c->stats);
@@ -812,6 +836,7 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	for (int q = 0; q < 4; q++) { g.bc[q] = cfg->bc[q]; g.bcu[q] = cfg->bcu[q]; }
 	g.dx = (cfg->x1 - cfg->x0) / cfg->nx; g.dy = (cfg->y1 - cfg->y0) / cfg->ny;
 	for (int k = 0; k < 2; k++) { c->ph.g[k] = cfg->ph[k].gamma; c->ph.pi[k] = cfg->ph[k].pi_inf; c->ph.q[k] = cfg->ph[k].q; c->ph.b[k] = cfg->ph[k].b; c->ph.cv[k] = cfg->ph[k].cv; c->ph.qp[k] = cfg->ph[k].qp; }
+	set_mg(c->ph, cfg);
 	const size_t NC = g.NC, n = g.n;
 	g_err.clear();
 	double** grids[] = {&c->U, &c->Ust, &c->Qe, &c->Qs, &c->H[0], &c->H[1], &c->donx};   // donors: 4 fields x 2 = NV per cell
@@ -1094,7 +1119,7 @@ int spb6_run(spb6_ctx* c, double t_end, long max_steps) {
 }
 
 int spb6_hllc_step(spb6_ctx* c, double dt) {
-	if (!c || !(dt > 0) || c->ctype) return SPB6_BAD_ARG;   // relaxation level 0 only
+	if (!c || !(dt > 0) || c->ctype || mg_any(c->ph)) return SPB6_BAD_ARG;   // relaxation level 0, NASG phases only
 	hipSetDevice(c->device);
 	int rc = hllc_alloc(c); if (rc) return rc;
 	rc = hllc_attempt(c, dt);
@@ -1102,7 +1127,7 @@ int spb6_hllc_step(spb6_ctx* c, double dt) {
 	return rc;
 }
 int spb6_hllc_run(spb6_ctx* c, double t_end, long max_steps, double cfl) {
-	if (!c || !(cfl > 0) || c->ctype) return SPB6_BAD_ARG;   // relaxation level 0 only
+	if (!c || !(cfl > 0) || c->ctype || mg_any(c->ph)) return SPB6_BAD_ARG;   // relaxation level 0, NASG phases only
 	hipSetDevice(c->device);
 	int rc = hllc_alloc(c); if (rc) return rc;
 	long done_steps = 0;

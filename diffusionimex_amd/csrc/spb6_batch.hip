@@ -486,8 +486,9 @@ const char* spb6_batch_last_error(spb6_batch* b) { return b ? b->err.c_str() : "
 spb6_batch* spb6_batch_create(const spb6_batch_problem_t* probs, int nprob, const spb6_config_t* cfg, int device) {
 	if (!probs || nprob < 1 || !cfg) return nullptr;
 	// the batch runs the reference scheme (PC) with full Newton at relaxation level 0 and the reference's profile
-	// arithmetic: no jreuse, variant, relaxation or fused shock integration
-	if (cfg->rus_local != 1 || cfg->rk < 1 || cfg->rk > 2 || cfg->jreuse || cfg->variant || cfg->relax || cfg->shock_fused) return nullptr;
+	// arithmetic: no jreuse, variant, relaxation, fused shock integration or Mie-Grueneisen phase
+	if (cfg->rus_local != 1 || cfg->rk < 1 || cfg->rk > 2 || cfg->jreuse || cfg->variant || cfg->relax || cfg->shock_fused ||
+	    cfg->eos_kind[0] || cfg->eos_kind[1]) return nullptr;
 	spb6_batch* b = new spb6_batch();
 	b->device = device; b->cfg = *cfg;
 	auto fail = [&]() -> spb6_batch* { spb6_batch_destroy(b); return nullptr; };
@@ -503,6 +504,7 @@ spb6_batch* spb6_batch_create(const spb6_batch_problem_t* probs, int nprob, cons
 		B.L = q.nx; B.off = (int)off; B.poff = (int)poff;
 		B.bc[0] = q.bc[0]; B.bc[1] = q.bc[1]; B.bcu[0] = q.bcu[0]; B.bcu[1] = q.bcu[1];
 		B.dx = (q.x1 - q.x0) / q.nx;
+		B.ph = Phys{};
 		for (int k = 0; k < 2; k++) { B.ph.g[k] = q.ph[k].gamma; B.ph.pi[k] = q.ph[k].pi_inf; B.ph.q[k] = q.ph[k].q; B.ph.b[k] = q.ph[k].b; B.ph.cv[k] = q.ph[k].cv; B.ph.qp[k] = q.ph[k].qp; }
 		off += q.nx + 2 * GH; poff += q.nx; Lmax = std::max(Lmax, q.nx);
 	}

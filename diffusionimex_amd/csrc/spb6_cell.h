@@ -95,6 +95,7 @@ __device__ __forceinline__ double recon_cell_s(const Phys& P, int limiter, const
 	if (limiter && phi > 0.0) {   // face admissibility cap (PC:400-412)
 		#pragma unroll
 		for (int ph = 0; ph < 2; ph++) {
+			if (is_mg(P, ph)) continue;   // SG / NASG phases only (PC:394)
 			double L1[3], L2[3];
 			const double av[3] = {ph == 0 ? xa[0] : 1.0 - xa[0], ph == 0 ? xc[0] : 1.0 - xc[0], ph == 0 ? xb[0] : 1.0 - xb[0]};
 			const double mv[3] = {ph == 0 ? xa[1] : xa[2], ph == 0 ? xc[1] : xc[2], ph == 0 ? xb[1] : xb[2]};
@@ -259,14 +260,14 @@ __device__ __forceinline__ bool prep_cell_at(const Geo& g, const Phys& P, const 
 	double czv = dinf(), K = smax(P.pi[0], P.pi[1]), PE = 0.0;
 	if (a1 > 0 && a2 > 0 && m1 > 0 && m2 > 0) {
 		const double p1 = p_ame(P, 0, a1, m1, e1), p2 = p_ame(P, 1, a2, m2, e2);
-		if (p_adm(P, 0, p1) && p_adm(P, 1, p2)) {
+		if (p_adm(P, 0, a1, m1, p1) && p_adm(P, 1, a2, m2, p2)) {
 			const double c21 = c2_rp(P, 0, m1 / a1, p1), c22 = c2_rp(P, 1, m2 / a2, p2);
 			const double c2 = (m1 * c21 + m2 * c22) / (m1 + m2);
 			czv = c2 > 0 ? sqrt(c2) : dinf();
 			K = 1.0 / ((a1 / sa) / (m1 / a1 * c21) + (a2 / sa) / (m2 / a2 * c22));
 		}
 		const double dd = dedp_am(P, 0, a1, m1) + dedp_am(P, 1, a2, m2);
-		if (dd > 0) PE = (fabs(e1) + fabs(m1 * P.q[0]) + fabs(e2) + fabs(m2 * P.q[1])) / dd;
+		if (dd > 0) PE = (fabs(e1) + eps_round(P, 0, a1, m1) + fabs(e2) + eps_round(P, 1, a2, m2)) / dd;
 	}
 	cz[c] = czv; Kw[c] = K + PE;
 	Pold[c] = Pst[map_P(g, i, j)];

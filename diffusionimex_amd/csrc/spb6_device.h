@@ -135,34 +135,89 @@ SPB_HD double phypot(double a, double b) {
 	return sqrt(a * a + b * b);
 }
 
-// ------------------------------------------------------------------ phases (stiffened gas / NASG covolume b)
+// ------------------------------------------------------------------ phases (stiffened gas / NASG covolume b; Mie-Grueneisen)
+// kind[k] 0: NASG (b = 0: SG) with g, pi, q, qp, cv, b; 1 (Cochran-Chan) and 2 (JWL): a Mie-Grueneisen phase
+// p = G rho (e - er(rho)) + pr(rho) with G, c1, c2, e1, e2, rr (reference density) and e0 (PHY:49-53); mg: either phase
+// is one. Every MG branch below tests kind, a kernel argument, so a launch takes one side for all its lanes.
+// The Mie-Grueneisen fields and branches exist only in the build with SPB6_MG = 1 (libspb6_hip_mg.so): with SPB6_MG = 0
+// (libspb6_hip.so) the phase block is the NASG one, is_mg is the constant false, the branches fold away and the kernels
+// keep their NASG-only registers and private memory.
+#ifndef SPB6_MG
+#define SPB6_MG 0
+#endif
 struct Phys {
 	double g[2], pi[2], q[2], b[2], cv[2], qp[2];
+#if SPB6_MG
+	int kind[2], mg;
+	double G[2], c1[2], c2[2], e1[2], e2[2], rr[2], e0[2];
+#endif
 };
+#if SPB6_MG
+SPB_HD bool is_mg(const Phys& P, int k) { return P.kind[k] != 0; }
+SPB_HD bool mg_any(const Phys& P) { return P.mg != 0; }
+SPB_HD double mg_G(const Phys& P, int k) { return P.G[k]; }
+#else
+SPB_HD constexpr bool is_mg(const Phys&, int) { return false; }
+SPB_HD constexpr bool mg_any(const Phys&) { return false; }
+SPB_HD constexpr double mg_G(const Phys&, int) { return 1.0; }
+#endif
+// the Mie-Grueneisen reference curve at density r: pr, pr' and er with er' = pr / r^2 (PHY:58-66), through the portable
+// exp and pow (Cochran-Chan: pr = c1 x^e1 - c2 x^e2, x = r / rr; JWL: pr = c1 exp(-e1 rr / r) + c2 exp(-e2 rr / r))
+SPB_HD void mg_ref(const Phys& P, int k, double r, double& pr, double& dpr, double& er) {
+#if !SPB6_MG
+	(void)P; (void)k; (void)r; pr = dpr = er = dnan();
+#else
+	if (P.kind[k] == 1) {
+		const double x = r / P.rr[k], t1 = P.c1[k] * ppow(x, P.e1[k]), t2 = P.c2[k] * ppow(x, P.e2[k]);
+		pr = t1 - t2; dpr = (P.e1[k] * t1 - P.e2[k] * t2) / r; er = (t1 / (P.e1[k] - 1.0) - t2 / (P.e2[k] - 1.0)) / r;
+	} else {
+		const double t1 = P.c1[k] * pexp(-P.e1[k] * P.rr[k] / r), t2 = P.c2[k] * pexp(-P.e2[k] * P.rr[k] / r);
+		pr = t1 + t2; dpr = (P.e1[k] * t1 + P.e2[k] * t2) * P.rr[k] / (r * r);
+		er = t1 / (P.e1[k] * P.rr[k]) + t2 / (P.e2[k] * P.rr[k]) - P.e0[k];
+	}
+#endif
+}
 
 SPB_HD double eps_amp(const Phys& P, int k, double a, double m, double p) {          // PHY:69-71
+	if (is_mg(P, k)) { double pr, dpr, er; mg_ref(P, k, m / a, pr, dpr, er); return m * er + a * (p - pr) / mg_G(P, k); }
 	return (p + P.g[k] * P.pi[k]) * (a - m * P.b[k]) / (P.g[k] - 1.0) + m * P.q[k]; }
 SPB_HD double p_ame(const Phys& P, int k, double a, double m, double e) {            // PHY:72-74
+	if (is_mg(P, k)) { double pr, dpr, er; mg_ref(P, k, m / a, pr, dpr, er); return mg_G(P, k) * (e - m * er) / a + pr; }
 	return (e - m * P.q[k]) * (P.g[k] - 1.0) / (a - m * P.b[k]) - P.g[k] * P.pi[k]; }
 SPB_HD double c2_rp(const Phys& P, int k, double r, double p) {                      // PHY:83-85
+	if (is_mg(P, k)) { double pr, dpr, er; mg_ref(P, k, r, pr, dpr, er); return (mg_G(P, k) + 1.0) * (p - pr) / r + dpr; }
 	return P.g[k] * (p + P.pi[k]) / (r * (1.0 - r * P.b[k])); }
 SPB_HD double dedp_am(const Phys& P, int k, double a, double m) {                    // PHY:86-88
+	if (is_mg(P, k)) return a / mg_G(P, k);
 	return (a - m * P.b[k]) / (P.g[k] - 1.0); }
 SPB_HD double p_mix(const Phys& P, double a1, double m1, double m2, double rhoe) {  // PHY:90-99
 	const double a0 = a1, aa1 = 1.0 - a1;
 	double num = rhoe, den = 0.0;
 	const double d0 = dedp_am(P, 0, a0, m1);
-	num -= m1 * P.q[0] + P.g[0] * P.pi[0] * d0;
+	if (is_mg(P, 0)) { double pr, dpr, er; mg_ref(P, 0, m1 / a0, pr, dpr, er); num -= m1 * er - a0 * pr / mg_G(P, 0); }
+	else num -= m1 * P.q[0] + P.g[0] * P.pi[0] * d0;
 	den += d0;
 	const double d1 = dedp_am(P, 1, aa1, m2);
-	num -= m2 * P.q[1] + P.g[1] * P.pi[1] * d1;
+	if (is_mg(P, 1)) { double pr, dpr, er; mg_ref(P, 1, m2 / aa1, pr, dpr, er); num -= m2 * er - aa1 * pr / mg_G(P, 1); }
+	else num -= m2 * P.q[1] + P.g[1] * P.pi[1] * d1;
 	den += d1;
 	return num / den;
 }
-SPB_HD bool p_adm(const Phys& P, int k, double p) { return p + P.pi[k] > 0; }       // PHY:101-104 (SG)
+// phase admissibility (PHY:101-104): p + pi > 0 for NASG, c^2 > 0 for a Mie-Grueneisen phase
+SPB_HD bool p_adm(const Phys& P, int k, double a, double m, double p) {
+	if (is_mg(P, k)) return isfin(p) && c2_rp(P, k, m / a, p) > 0;
+	return p + P.pi[k] > 0; }
+SPB_HD bool p_adm_sg(const Phys& P, int k, double p) { return p + P.pi[k] > 0; }    // the explicit HLLC scheme's (NASG)
+// the part of eps_k's round-off scale besides |eps_k| (PHY:106-109): |m q| for NASG, |m er| + |alpha pr / G| for MG
+SPB_HD double eps_round(const Phys& P, int k, double a, double m) {
+	if (is_mg(P, k)) { double pr, dpr, er; mg_ref(P, k, m / a, pr, dpr, er); return fabs(m * er) + fabs(a * pr / mg_G(P, k)); }
+	return fabs(m * P.q[k]); }
 
-// phase temperature T(alpha, m, p) (PHY:75-77, SG/NASG)
+// phase temperature T(alpha, m, p) (PHY:75-77); NaN for a Mie-Grueneisen phase without cv
 SPB_HD double T_amp(const Phys& P, int k, double a, double m, double p) {
+	if (is_mg(P, k)) {
+		if (!(P.cv[k] > 0)) return dnan();
+		double pr, dpr, er; mg_ref(P, k, m / a, pr, dpr, er); return (a * (p - pr) / (mg_G(P, k) * m)) / P.cv[k]; }
 	return (a / m - P.b[k]) * (p + P.pi[k]) / ((P.g[k] - 1.0) * P.cv[k]); }
 // the profile columns of cell_diag (PHY:1105-1116) from the 8 conserved values: p u v rho alpha2 Y2 rho1 rho2 c_frozen
 // c_wood T1 T2 p1 p2 g1-g2, NaN where prim_of (PHY:837-847) fails. The Gibbs difference g1 - g2 (g_pT, PHY:78-81) takes
@@ -175,14 +230,14 @@ SPB_HD void cell_diag(const Phys& P, const double* Q, double* o) {
 	if (ok) {
 		r1 = m1 / a1; r2 = m2 / a2; u = Q[IMU] / rho; v = Q[IMV] / rho;
 		p1 = p_ame(P, 0, a1, m1, Q[IE1]); p2 = p_ame(P, 1, a2, m2, Q[IE2]);
-		ok = p_adm(P, 0, p1) && p_adm(P, 1, p2);
+		ok = p_adm(P, 0, a1, m1, p1) && p_adm(P, 1, a2, m2, p2);
 		if (ok) cf = sqrt((m1 * c2_rp(P, 0, r1, p1) + m2 * c2_rp(P, 1, r2, p2)) / rho);
 	}
 	double cw = nan, T1 = nan, T2 = nan, dg = nan;
 	if (ok) {
 		cw = sqrt(1.0 / (rho * (a1 / (r1 * c2_rp(P, 0, r1, p1)) + (1 - a1) / (r2 * c2_rp(P, 1, r2, p2)))));
 		T1 = T_amp(P, 0, a1, m1, p1); T2 = T_amp(P, 1, 1 - a1, m2, p2);
-		if (T1 > 0 && T2 > 0 && P.cv[0] > 0) {
+		if (T1 > 0 && T2 > 0 && P.cv[0] > 0 && !mg_any(P)) {
 			auto gpt = [&](int k, double pp, double T) {
 				const double sk = P.cv[k] * (P.g[k] * log(T) - (P.g[k] - 1.0) * log(pp + P.pi[k])) + P.qp[k];
 				return P.g[k] * P.cv[k] * T + P.b[k] * pp + P.q[k] - T * sk;
@@ -201,19 +256,44 @@ SPB_HD bool prim_cf(const Phys& P, double a1, double m1, double m2, double e1, d
 	if (!(a1 > 0 && a2 > 0 && m1 > 0 && m2 > 0)) return false;
 	const double rho = m1 + m2, r1 = m1 / a1, r2 = m2 / a2;
 	const double p1 = p_ame(P, 0, a1, m1, e1), p2 = p_ame(P, 1, a2, m2, e2);
-	if (!(p_adm(P, 0, p1) && p_adm(P, 1, p2))) return false;
+	if (!(p_adm(P, 0, a1, m1, p1) && p_adm(P, 1, a2, m2, p2))) return false;
 	cf = sqrt((m1 * c2_rp(P, 0, r1, p1) + m2 * c2_rp(P, 1, r2, p2)) / rho);
 	return true;
 }
 
 // ------------------------------------------------------------------ shock budget (PHY:260-418 with PHO:365-374)
 struct Prof { double Y0, Y1, vk00, v0, p0, j2, vend; };
-SPB_HD bool eos_ok(const Phys& P, int k, double p, double v) {                       // PHY:271-275 (SG)
+SPB_HD bool eos_ok(const Phys& P, int k, double p, double v) {                       // PHY:271-275
 	if (!(v > 0) || !isfin(p)) return false;
+	if (is_mg(P, k)) return c2_rp(P, k, 1.0 / v, p) > 0;
 	return v > P.b[k] && p + P.pi[k] > 0; }
-SPB_HD double hug_v(const Phys& P, int k, double p0, double p1, double v0) {         // PHY:277-282 (SG)
-	const double x0 = p0 + P.pi[k], x1 = p1 + P.pi[k];
-	return P.b[k] + (v0 - P.b[k]) * ((P.g[k] - 1.0) * x1 + (P.g[k] + 1.0) * x0) / ((P.g[k] + 1.0) * x1 + (P.g[k] - 1.0) * x0);
+// phase k's Hugoniot volume from (p0, v0) to p1 (PHY:277-292): closed form for NASG; for a Mie-Grueneisen phase the
+// root of e(p1, v) - e(p0, v0) + (p0 + p1) / 2 (v - v0) below v0, bracketed by steps of 0.9 v, bisected to 1e-14 v0
+SPB_HD double hug_v(const Phys& P, int k, double p0, double p1, double v0) {
+	if (!is_mg(P, k)) {
+		const double x0 = p0 + P.pi[k], x1 = p1 + P.pi[k];
+		return P.b[k] + (v0 - P.b[k]) * ((P.g[k] - 1.0) * x1 + (P.g[k] + 1.0) * x0) / ((P.g[k] + 1.0) * x1 + (P.g[k] - 1.0) * x0);
+	}
+	auto e_pv = [&](double p, double v) { double pr, dpr, er; mg_ref(P, k, 1.0 / v, pr, dpr, er); return er + v * (p - pr) / mg_G(P, k); };
+	const double e0 = e_pv(p0, v0);
+	auto hg = [&](double v) { return e_pv(p1, v) - e0 + 0.5 * (p0 + p1) * (v - v0); };
+	double hi = v0, lo = v0, gl = hg(v0);
+	if (!(gl > 0)) return v0;
+	for (int it = 0; it < 80 && gl > 0; it++) { hi = lo; lo *= 0.9; gl = hg(lo); if (!isfin(gl)) return dnan(); }
+	if (gl > 0) return dnan();
+	for (int it = 0; it < 100; it++) {
+		const double mid = 0.5 * (lo + hi), gm = hg(mid);
+		if (gm > 0) hi = mid; else lo = mid;
+		if (hi - lo <= 1e-14 * v0) break;
+	}
+	return 0.5 * (lo + hi);
+}
+// de/dp at fixed v and de/dv at fixed p plus p, of phase k at (p, v) (PHY:266-270)
+SPB_HD void eos_ep_ev(const Phys& P, int k, double p, double v, double& ep, double& evp) {
+	if (is_mg(P, k)) {
+		double pr, dpr, er; const double r = 1.0 / v; mg_ref(P, k, r, pr, dpr, er);
+		ep = v / mg_G(P, k); evp = (p - pr) * (1.0 + 1.0 / mg_G(P, k)) + r * dpr / mg_G(P, k); return; }
+	ep = (v - P.b[k]) / (P.g[k] - 1.0); evp = P.g[k] * (p + P.pi[k]) / (P.g[k] - 1.0);
 }
 // right-hand side of the profile ODE in v (PHY:302-310); lr0 = lam_0 / Y_0 and lr1 = lam_1 / Y_1 are the integration's
 // constant quotients (the reference evaluates lam_k / Y_k * (p - P) left to right, so hoisting the quotient keeps every
@@ -221,11 +301,11 @@ SPB_HD double hug_v(const Phys& P, int k, double p0, double p1, double v0) {    
 SPB_HD bool prof_rhs(const Phys& P, const Prof& S, double lr0, double lr1, double v, const double* y, double* dy) {
 	const double p = y[0], v1 = y[1], v2 = (v - S.Y0 * v1) / S.Y1, Pr = S.p0 + S.j2 * (S.v0 - v);
 	const bool adm = eos_ok(P, 0, p, v1) && eos_ok(P, 1, p, v2);
-	double num = -1.0, den = 0.0;
-	const double ep0 = (v1 - P.b[0]) / (P.g[0] - 1.0), evp0 = P.g[0] * (p + P.pi[0]) / (P.g[0] - 1.0);
+	double num = -1.0, den = 0.0, ep0, evp0, ep1, evp1;
+	eos_ep_ev(P, 0, p, v1, ep0, evp0);
 	const double r0 = lr0 * (p - Pr);
 	num += S.Y0 * r0 / evp0; den += S.Y0 * ep0 / evp0;
-	const double ep1 = (v2 - P.b[1]) / (P.g[1] - 1.0), evp1 = P.g[1] * (p + P.pi[1]) / (P.g[1] - 1.0);
+	eos_ep_ev(P, 1, p, v2, ep1, evp1);
 	const double r1 = lr1 * (p - Pr);
 	num += S.Y1 * r1 / evp1; den += S.Y1 * ep1 / evp1;
 	dy[0] = num / den; dy[1] = (r0 - ep0 * dy[0]) / evp0; dy[2] = p - Pr;
@@ -271,7 +351,7 @@ struct ShockRcp { double rg0, rg1; bool ok; };
 SPB_HD ShockRcp shock_rcp(const Phys& P) {
 	ShockRcp c;
 	c.rg0 = rcp_nr(P.g[0] - 1.0); c.rg1 = rcp_nr(P.g[1] - 1.0);
-	c.ok = in_band(P.g[0], BAND_CONST) && in_band(P.g[1], BAND_CONST) && in_band(P.g[0] - 1.0, BAND_CONST) &&
+	c.ok = !mg_any(P) && in_band(P.g[0], BAND_CONST) && in_band(P.g[1], BAND_CONST) && in_band(P.g[0] - 1.0, BAND_CONST) &&
 	       in_band(P.g[1] - 1.0, BAND_CONST);
 	return c;
 }
@@ -506,8 +586,11 @@ SPB_HD int share_pre(const Phys& P, const double Y[2], const double vpre[2], dou
                      double lam[2], double& q) {
 	double Kmin = dinf();
 	for (int k = 0; k < 2; k++) {
-		K[k] = c2_rp(P, k, 1.0 / vpre[k], p0) / vpre[k];
-		G[k] = 0.5 * (P.g[k] + 1.0); if (P.b[k] != 0.0) G[k] *= vpre[k] / (vpre[k] - P.b[k]);
+		K[k] = c2_rp(P, k, 1.0 / vpre[k], p0) / vpre[k];   // K and G (PHY:294-299): MG by a difference along the isentrope
+		if (is_mg(P, k)) {
+			const double dp = 1e-6 * K[k], vv = vpre[k] * (1.0 - dp / K[k]), K2 = c2_rp(P, k, 1.0 / vv, p0 + dp) / vv;
+			G[k] = 0.5 * (1.0 + (K2 - K[k]) / dp);
+		} else { G[k] = 0.5 * (P.g[k] + 1.0); if (P.b[k] != 0.0) G[k] *= vpre[k] / (vpre[k] - P.b[k]); }
 		if (!(K[k] > 0)) return 0; Kmin = smin(Kmin, K[k]);
 	}
 	const double dp = p1 - p0;
@@ -696,7 +779,7 @@ SPB_HD void stencil_state(const Phys& P, const double* c, double* st) {
 	st[5] = r > 0 ? c[IMU] / r : 0.0; st[6] = r > 0 ? c[IMV] / r : 0.0;
 	if (a1 > 0 && a1 < 1 && m1 > 0 && m2 > 0) {
 		const double p = p_mix(P, a1, m1, m2, c[IE1] + c[IE2]);
-		if (p_adm(P, 0, p) && p_adm(P, 1, p)) st[4] = p;
+		if (p_adm(P, 0, a1, m1, p) && p_adm(P, 1, 1.0 - a1, m2, p)) st[4] = p;
 	}
 }
 
@@ -707,7 +790,10 @@ struct ClIn { double atr0, atr1, mtr0, mtr1, etr0, etr1, s, rhoeC, pguess, Q0, Q
 // and energies, and the pressure
 struct ClOut { double a0, a1, m0, m1, e0, e1, pbar; };
 
-SPB_HD double E_above(const Phys& P, int k, double atr, double mtr, double etr) {    // PHY:420-423 (SG)
+SPB_HD double E_above(const Phys& P, int k, double atr, double mtr, double etr) {    // PHY:420-423
+	if (is_mg(P, k)) {
+		if (!(atr > 0 && mtr > 0)) return 0.0;
+		double pr, dpr, er; mg_ref(P, k, mtr / atr, pr, dpr, er); return smax(0.0, etr - mtr * er); }
 	return smax(0.0, etr - mtr * P.q[k] - (atr - mtr * P.b[k]) * P.pi[k]); }
 
 // segment of the piecewise-affine defect share at delta (PHY:428-439): lam, extra
@@ -717,6 +803,133 @@ SPB_HD void defect_seg(double delta, double Qb, double Q0, double Q1, double lE0
 	if (!(delta > 0.0) || !(Qb > 0.0)) { lam0 = lE0; lam1 = lE1; }
 	else if (delta < Qb) { lam0 = Q0 / Qb; lam1 = Q1 / Qb; }
 	else { lam0 = lE0; lam1 = lE1; ex0 = Q0 - Qb * lE0; ex1 = Q1 - Qb * lE1; }
+}
+// ---- the p closure with a Mie-Grueneisen phase along the PHO iteration paths (alpha_mg PHO:552-583, closure_p_mg
+// PHO:585-628). An MG phase's alpha(p) is the root of f(a) = m er(m / a) + a (p - pr) / G + p a - A - B p, increasing where
+// c^2 > 0 (f' = rho c^2 / G): the guess moves up past the c^2 <= 0 side, the root is bracketed by doubling / halving (a
+// halving across c^2 = 0 bisects geometrically for a point of the admissible branch with f <= 0), then safeguarded
+// Newton from the guess when it lies in the bracket, a converged Newton step taken before the bracket test.
+SPB_HD bool alpha_mg(const Phys& P, int k, double m, double A, double B, double p, double guess, double& al, double& dal) {
+	const double G = mg_G(P, k);
+	auto F = [&](double a, double& df) {
+		double pr, dpr, er; const double r = m / a; mg_ref(P, k, r, pr, dpr, er);
+		df = ((G + 1.0) * (p - pr) + r * dpr) / G;
+		return m * er + a * (p - pr) / G + p * a - A - B * p; };
+	double lo = guess, hi = guess, d; double flo = F(lo, d), fhi = flo;
+	if (!isfin(flo)) return false;
+	if (!(d > 0)) {
+		for (int it = 0; it < 200 && !(d > 0); it++) { guess *= 2.0; flo = F(guess, d); if (!isfin(flo)) return false; }
+		if (!(d > 0)) return false;
+		lo = hi = guess; fhi = flo;
+	}
+	if (flo < 0) {
+		for (int it = 0; it < 200 && fhi < 0; it++) { lo = hi; flo = fhi; hi *= 2.0; fhi = F(hi, d); if (!isfin(fhi)) return false; }
+		if (fhi < 0) return false;
+	} else {
+		for (int it = 0; it < 200 && flo > 0; it++) {
+			hi = lo; fhi = flo; lo *= 0.5; flo = F(lo, d); if (!isfin(flo)) return false;
+			if (d <= 0) {
+				bool found = false;
+				for (int b = 0; b < 200; b++) {
+					const double mid = sqrt(lo * hi); double dm; const double fm = F(mid, dm);
+					if (!isfin(fm)) return false;
+					if (!(dm > 0)) lo = mid;
+					else if (fm > 0) { hi = mid; fhi = fm; }
+					else { lo = mid; flo = fm; found = true; break; }
+					if (hi - lo <= 1e-15 * hi) break;
+				}
+				if (!found) return false;
+				break;
+			}
+		}
+		if (flo > 0) return false;
+	}
+	double a = (flo == 0) ? lo : (fhi == 0 ? hi : (guess >= lo && guess <= hi ? guess : sqrt(lo * hi)));
+	for (int it = 0; it < 200; it++) {
+		double df; const double fa = F(a, df);
+		if (fa == 0) break;
+		if (fa < 0) lo = a; else hi = a;
+		double an = (df > 0) ? a - fa / df : -1.0;
+		if (an > 0 && fabs(an - a) <= 4e-16 * a) { a = an; break; }
+		if (!(an > lo && an < hi)) an = sqrt(lo * hi);
+		if (fabs(an - a) <= 4e-16 * a || hi - lo <= 4e-16 * hi) { a = an; break; }
+		a = an;
+	}
+	double df; F(a, df); if (!(df > 0)) return false;
+	al = a; dal = -(a / G + a - B) / df; return true;
+}
+// Phi(p) = sum_k alpha_k(p) - 1 on the guess's defect segment (p_coef, PHY:443-454), each MG phase's alpha from alpha_mg
+// warm-started from the last accepted alphas, a NASG phase's in closed form (PHY:455-456); bracketed from the guess by
+// doubling steps, then bisection-safeguarded Newton
+SPB_HD bool closure_p_mg(const Phys& P, const ClIn& in, ClOut& o) {
+	const double D0 = in.rhoeC - in.etr0 - in.etr1, sm = in.mtr0 + in.mtr1;
+	double lE0, lE1;
+	{ const double E0 = E_above(P, 0, in.atr0, in.mtr0, in.etr0), E1 = E_above(P, 1, in.atr1, in.mtr1, in.etr1);
+	  double sE = 0.0; sE += E0; sE += E1;
+	  if (sE > 0) { lE0 = E0 / sE; lE1 = E1 / sE; }
+	  else { lE0 = sm > 0 ? in.mtr0 / sm : 0.5; lE1 = sm > 0 ? in.mtr1 / sm : 0.5; } }
+	double lam0, lam1, ex0, ex1;
+	defect_seg(D0 + in.pguess * in.s, in.Q0 + in.Q1, in.Q0, in.Q1, lE0, lE1, lam0, lam1, ex0, ex1);
+	const double mt[2] = {in.mtr0, in.mtr1};
+	const double A[2] = {in.etr0 - in.mtr0 * P.q[0] + lam0 * D0 + ex0, in.etr1 - in.mtr1 * P.q[1] + lam1 * D0 + ex1};
+	const double B[2] = {in.atr0 + lam0 * in.s, in.atr1 + lam1 * in.s};
+	const double N[2] = {A[0] - (B[0] - in.mtr0 * P.b[0]) * P.pi[0], A[1] - (B[1] - in.mtr1 * P.b[1]) * P.pi[1]};
+	double ag[2] = {in.atr0 / (in.atr0 + in.atr1), in.atr1 / (in.atr0 + in.atr1)};
+	auto Phi = [&](double p, double& dPhi, bool upd) -> double {
+		double sum = -1.0, an[2]; dPhi = 0.0;
+		for (int k = 0; k < 2; k++) {
+			double a, da;
+			if (is_mg(P, k)) { if (!alpha_mg(P, k, mt[k], A[k], B[k], p, ag[k], a, da)) return dnan(); }
+			else {
+				if (!(p + P.pi[k] > 0) || !(N[k] > 0)) return dnan();
+				a = ((P.g[k] - 1.0) * (A[k] + B[k] * p) + (p + P.g[k] * P.pi[k]) * mt[k] * P.b[k]) / (P.g[k] * (p + P.pi[k]));
+				da = -(P.g[k] - 1.0) * N[k] / (P.g[k] * (p + P.pi[k]) * (p + P.pi[k]));
+			}
+			an[k] = a; sum += a; dPhi += da;
+		}
+		if (upd) { ag[0] = an[0]; ag[1] = an[1]; }
+		return sum; };
+	double p = in.pguess, d;
+	const double f = Phi(p, d, true);
+	double scale = 1e-6 * (fabs(p) + 1e5);
+	double lo, hi, flo, fhi;
+	if (!isfin(f) || f > 0) {
+		lo = p; flo = f; hi = p;
+		for (int it = 0; it < 300; it++) {
+			hi = p + scale; fhi = Phi(hi, d, false);
+			if (isfin(fhi) && fhi <= 0) break;
+			if (isfin(fhi)) { lo = hi; flo = fhi; }
+			scale *= 2.0;
+		}
+		if (!(isfin(fhi) && fhi <= 0)) return false;
+	} else {
+		hi = p; fhi = f; lo = p;
+		for (int it = 0; it < 300; it++) {
+			lo = p - scale; flo = Phi(lo, d, false);
+			if (isfin(flo) && flo >= 0) break;
+			if (!isfin(flo)) { scale *= 0.5; if (scale < 1e-300) return false; continue; }
+			hi = lo; fhi = flo; scale *= 2.0;
+		}
+		if (!(isfin(flo) && flo >= 0)) return false;
+	}
+	p = (isfin(f) && in.pguess >= lo && in.pguess <= hi) ? in.pguess : 0.5 * (lo + hi);
+	for (int it = 0; it < 300; it++) {
+		double df; const double fp = Phi(p, df, true);
+		if (fp == 0.0) break;
+		if (!isfin(fp) || fp > 0) lo = p; else hi = p;
+		double pn = (isfin(fp) && df < 0) ? p - fp / df : 0.5 * (lo + hi);
+		if (isfin(fp) && df < 0 && fabs(pn - p) <= 2e-16 * (fabs(p) + fabs(pn))) { p = pn; break; }
+		if (!(pn > lo && pn < hi)) pn = 0.5 * (lo + hi);
+		if (fabs(pn - p) <= 2e-16 * (fabs(p) + fabs(pn)) || hi - lo <= 4e-16 * (fabs(hi) + fabs(lo))) { p = pn; break; }
+		p = pn;
+	}
+	double df; if (!isfin(Phi(p, df, true))) return false;
+	if (!p_adm(P, 0, ag[0], in.mtr0, p)) return false;
+	o.e0 = eps_amp(P, 0, ag[0], in.mtr0, p);
+	if (!p_adm(P, 1, ag[1], in.mtr1, p)) return false;
+	o.e1 = eps_amp(P, 1, ag[1], in.mtr1, p);
+	o.a0 = ag[0]; o.a1 = ag[1]; o.m0 = in.mtr0; o.m1 = in.mtr1; o.pbar = p;
+	return isfin(p);
 }
 // closure_p along the PHO iteration path: safeguarded Newton from the guess, Phi and Phi' together (PHO:472-548).
 // Returns false where the reference's closure returns false; on success fills o (alpha, eps per phase and pbar).
@@ -729,6 +942,7 @@ SPB_HD bool closure_p(const Phys& P, const ClIn& in, ClOut& o) {
 		if (!(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
 		if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
 		if (!(in.mtr0 + in.mtr1 > 0)) return false;
+		if (mg_any(P)) return closure_p_mg(P, in, o);   // PHY:463, PHO:473
 	}
 	const double plo = smax(-P.pi[0], -P.pi[1]);
 	const double D0 = NODELTA ? 0.0 : in.rhoeC - in.etr0 - in.etr1;
@@ -816,6 +1030,7 @@ SPB_HD T closure_phi(const Phys& P, const T X[6], const T& s, const T& rhoeC, do
 }
 // round-off floor of the closure pressure (PC:631-638)
 SPB_HD double closure_roundoff(const Phys& P, const ClIn& in, double p) {
+	if (mg_any(P)) return 0.0;   // NASG / SG closures only (PC:632)
 	const double D0 = in.rhoeC - in.etr0 - in.etr1;
 	const double sm = in.mtr0 + in.mtr1;
 	const double E0 = E_above(P, 0, in.atr0, in.mtr0, in.etr0), E1 = E_above(P, 1, in.atr1, in.mtr1, in.etr1);

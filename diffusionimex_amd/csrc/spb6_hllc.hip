@@ -108,7 +108,7 @@ __device__ __forceinline__ void hllc_face(const Phys& P, const HPrim& L, const H
 	for (int k = 0; k < 2; k++) {
 		const double v0 = 1.0 / rk[k], vs = 1.0 / (rk[k] * fac);
 		double ps = hugoniot_p(P, k, pk[k], v0, vs);
-		if (!isfin(ps) || !p_adm(P, k, ps)) ps = pk[k];
+		if (!isfin(ps) || !p_adm_sg(P, k, ps)) ps = pk[k];
 		f[HF_UE0 + k] = eps_amp(P, k, ak[k], ak[k] * rk[k] * fac, ps) * SM;
 	}
 }
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(256) k_hx_prim(Geo g, Phys P, const double* __
 	if (!(a1 > 0 && a2 > 0 && m1 > 0 && m2 > 0)) { flags[FLAG_STAGE] = 1; return; }
 	const double rho = m1 + m2, r1 = m1 / a1, r2 = m2 / a2, u = U[IMU * NC + c] / rho, v = U[IMV * NC + c] / rho;
 	const double p1 = p_ame(P, 0, a1, m1, U[IE1 * NC + c]), p2 = p_ame(P, 1, a2, m2, U[IE2 * NC + c]);
-	if (!(p_adm(P, 0, p1) && p_adm(P, 1, p2))) { flags[FLAG_STAGE] = 1; return; }
+	if (!(p_adm_sg(P, 0, p1) && p_adm_sg(P, 1, p2))) { flags[FLAG_STAGE] = 1; return; }
 	const double cf = sqrt((m1 * c2_rp(P, 0, r1, p1) + m2 * c2_rp(P, 1, r2, p2)) / rho);
 	W[HP_A1 * NC + c] = a1; W[HP_R1 * NC + c] = r1; W[HP_R2 * NC + c] = r2; W[HP_U * NC + c] = u; W[HP_V * NC + c] = v;
 	W[HP_P1 * NC + c] = p1; W[HP_P2 * NC + c] = p2; W[HP_RHO * NC + c] = rho; W[HP_CF * NC + c] = cf;
@@ -198,9 +198,9 @@ __global__ void __launch_bounds__(256) k_hx_relax(Geo g, Phys P, double* __restr
 	const double ak[2] = {a1, 1 - a1}, mk[2] = {m1, m2};
 	#pragma unroll
 	for (int k = 0; k < 2; k++)
-		if (!p_adm(P, k, p_ame(P, k, ak[k], mk[k], ek[k]))) {
+		if (!p_adm_sg(P, k, p_ame(P, k, ak[k], mk[k], ek[k]))) {
 			const double pm = p_mix(P, a1, m1, m2, rhoe);
-			if (p_adm(P, k, pm)) ek[k] = eps_amp(P, k, ak[k], mk[k], pm);
+			if (p_adm_sg(P, k, pm)) ek[k] = eps_amp(P, k, ak[k], mk[k], pm);
 		}
 	const double pbar = a1 * p_ame(P, 0, a1, m1, e1o) + (1 - a1) * p_ame(P, 1, 1 - a1, m2, e2o);
 	ClIn in;

@@ -429,6 +429,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 			nsteps++;
 			int r;
 			if (FUSED) r = prof_step_fused(P, S, F, R);
+			else if (mg_any(P)) r = prof_step(P, S, R);   // Mie-Grueneisen phases: the plain right-hand side
 			else {
 				r = prof_step<true>(P, S, R, &rc);
 				if (r == 2) r = prof_step(P, S, R);   // (wavefront-uniform) a quotient's operand out of the shortcut's range

@@ -12,6 +12,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libspb6_hip.so")
+LIB_MG_PATH = os.path.join(HERE, "libspb6_hip_mg.so")   # the same library built with the Mie-Grueneisen EOS branches
 
 BC_TRANSMISSIVE, BC_WALL, BC_PERIODIC = 0, 1, 2
 STATUS = {0: "OK", 1: "RHO_NONPOSITIVE", 2: "STAGE_INADMISSIBLE", 3: "CLOSURE_FAIL", 4: "NEWTON_FAIL",
@@ -38,6 +39,7 @@ class Config(C.Structure):
         ("krylov_fp32", C.c_int), ("learned_guess", C.c_int), ("analytic_jacobian", C.c_int),
         ("jreuse", C.c_int), ("variant", C.c_int), ("relax", C.c_int), ("eps_i", C.c_double), ("graph", C.c_int),
         ("shock_fused", C.c_int), ("theta_rate", C.c_double), ("nu_rate", C.c_double),
+        ("eos_kind", C.c_int * 2), ("mg", (C.c_double * 7) * 2),
     ]
 
 
@@ -75,17 +77,18 @@ EXPORTS = ["spb6_create", "spb6_destroy", "spb6_last_error", "spb6_set_state", "
            "spb6_batch_get_state", "spb6_batch_get_pressure", "spb6_batch_run", "spb6_batch_get_stats",
            "spb6_batch_set_launch", "spb6_batch_autotune"]
 
-_lib = None
+_libs = {}
 
 
-def load():
-    """Load libspb6_hip.so (built by diffusionimex_amd/build.py); raises if it is missing."""
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"HIP library {LIB_PATH} is missing: run `python diffusionimex_amd/build.py` (no CPU fallback)")
-    L = C.CDLL(LIB_PATH)
+def load(mg=False):
+    """Load libspb6_hip.so, or with mg its Mie-Grueneisen build libspb6_hip_mg.so (both built by
+    diffusionimex_amd/build.py); raises if it is missing."""
+    path = LIB_MG_PATH if mg else LIB_PATH
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library {path} is missing: run `python diffusionimex_amd/build.py` (no CPU fallback)")
+    L = C.CDLL(path)
     vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
     L.spb6_create.restype = vp
     L.spb6_create.argtypes = [C.POINTER(Config), C.c_int]
@@ -156,7 +159,7 @@ def load():
     L.spb6_batch_get_stats.argtypes = [vp, C.POINTER(Stats), C.POINTER(Stats)]
     L.spb6_batch_set_launch.argtypes = [vp, C.c_int, C.c_int]
     L.spb6_batch_autotune.argtypes = [vp, C.c_long, vp]
-    _lib = L
+    _libs[path] = L
     return L
 
 
@@ -167,8 +170,10 @@ def _dp(a):
 def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0, 1.0), bc=(0, 0, 0, 0), bcu=(0, 0, 0, 0),
                 rk=2, cfl=0.2, cfl_ac=0.4, ramp=1, dtfix=0.0, acdt=0, newton_tol=1e-13, newton_iter=60,
                 krylov_tol=1e-13, krylov_iter=4000, limiter=1, order=2, rus_local=1, krylov_fp32=0, learned_guess=0,
-                analytic_jacobian=0, jreuse=0, variant=0, relax=0, eps_i=0.0, graph=0, shock_fused=0, theta_rate=0.0, nu_rate=0.0):
-    """Config with the reference defaults (PC:57-80): phases are (gamma, p_inf[, q, qp, cv, b])."""
+                analytic_jacobian=0, jreuse=0, variant=0, relax=0, eps_i=0.0, graph=0, shock_fused=0, theta_rate=0.0, nu_rate=0.0,
+                eos_kind=(0, 0), mg=((0.0,) * 7, (0.0,) * 7)):
+    """Config with the reference defaults (PC:57-80): phases are (gamma, p_inf[, q, qp, cv, b]); eos_kind[k] 1 or 2 makes
+    phase k a Cochran-Chan or JWL phase with mg[k] = (G, c1, c2, e1, e2, rr, e0)."""
     c = Config()
     c.nx, c.ny = nx, ny
     c.x0, c.x1, c.y0, c.y1 = domain
@@ -190,6 +195,10 @@ def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0
     c.graph = graph
     c.shock_fused = shock_fused
     c.theta_rate, c.nu_rate = theta_rate, nu_rate
+    for k in range(2):
+        c.eos_kind[k] = eos_kind[k]
+        for q in range(7):
+            c.mg[k][q] = mg[k][q]
     return c
 
 
@@ -201,7 +210,7 @@ class Stepper:
     """GPU stepper context: owns all device buffers for one grid (PC:1157-1163, StageData PC:565-582)."""
 
     def __init__(self, cfg, device=0):
-        self.L = load()
+        self.L = load(mg=bool(cfg.eos_kind[0] or cfg.eos_kind[1]))
         self.cfg = cfg
         self.h = self.L.spb6_create(C.byref(cfg), int(device))
         if not self.h:

@@ -112,6 +112,15 @@ typedef struct spb6_config {
                                         relaxation levels >= 1 / 2-3 (the theta closure PHY:679-700, the nu closure
                                         PHY:716-753, backward-Euler form); <= 0: the level's default, instantaneous
                                         (PC:1148-1149) */
+    int eos_kind[2];                 /* Phase::kind (PHY:50-53): 0 NASG / SG with ph[k]; 1 Cochran-Chan, 2 JWL: a
+                                        Mie-Grueneisen phase p = G rho (e - er(rho)) + pr(rho) with mg[k] (ph[k] unused).
+                                        Relaxation level 0 (the reference's MG phases relax mechanically only, PHY:790),
+                                        the reference scheme, the finite-difference Jacobian, the FP64 Krylov solve, the
+                                        plain guess and the exact profile form only; the batch and the explicit HLLC
+                                        scheme reject it */
+    double mg[2][7];                 /* a Mie-Grueneisen phase's G, c1, c2, e1, e2, rr (reference density), e0 (JWL's
+                                        energy shift): Cochran-Chan pr = c1 x^e1 - c2 x^e2 with x = rho / rr, JWL
+                                        pr = c1 exp(-e1 rr / rho) + c2 exp(-e2 rr / rho) (PHY:58-66) */
 } spb6_config_t;
 
 /* run statistics: the counters of the reference's stats line (PC:1337-1339) plus Krylov counts */

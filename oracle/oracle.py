@@ -30,6 +30,7 @@ class Cfg(C.Structure):
         ("portable_math", C.c_int), ("nthreads", C.c_int), ("jreuse", C.c_int), ("variant", C.c_int),
         ("relax", C.c_int), ("eps_i", C.c_double), ("shock_fused", C.c_int),
         ("theta_rate", C.c_double), ("nu_rate", C.c_double),
+        ("eos_kind", C.c_int * 2), ("mg", (C.c_double * 7) * 2),
     ]
 
 
@@ -44,7 +45,8 @@ class Stats(C.Structure):
 
 class OraModel(C.Structure):
     _fields_ = [("ph", (C.c_double * 6) * 2), ("closure_path", C.c_int), ("share_start", C.c_int),
-                ("dot_order", C.c_int), ("portable_math", C.c_int), ("shock_fused", C.c_int)]
+                ("dot_order", C.c_int), ("portable_math", C.c_int), ("shock_fused", C.c_int),
+                ("eos_kind", C.c_int * 2), ("mg", (C.c_double * 7) * 2)]
 
 
 def build(force=False):
@@ -65,6 +67,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
         L.ora_create.restype = vp
+        L.ora_eos.argtypes = [C.POINTER(OraModel), C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.ora_create.argtypes = [C.POINTER(Cfg)]
         L.ora_destroy.argtypes = [vp]
         L.ora_cfg_size.restype = C.c_size_t
@@ -166,6 +169,10 @@ def model_of(cfg):
             m.ph[p][q] = cfg.ph[p][q]
     m.closure_path, m.share_start, m.dot_order, m.portable_math = cfg.closure_path, cfg.share_start, cfg.dot_order, cfg.portable_math
     m.shock_fused = cfg.shock_fused
+    for p in range(2):
+        m.eos_kind[p] = cfg.eos_kind[p]
+        for q in range(7):
+            m.mg[p][q] = cfg.mg[p][q]
     return m
 
 
@@ -187,6 +194,8 @@ class Oracle:
         self.cfg = cfg
         self.L = lib()
         self.h = self.L.ora_create(C.byref(cfg))
+        if not self.h:
+            raise ValueError("a Mie-Grueneisen phase runs at relaxation level 0 with the reference scheme and the exact profile form only")
         self.nx, self.ny = cfg.nx, cfg.ny
         self.NXT, self.NYT = cfg.nx + 4, cfg.ny + 4
 
@@ -255,7 +264,7 @@ def case_init(test, nx, ny=1, **kw):
     if tend < 0:
         raise ValueError(f"unknown case {test}")
     for k, v in kw.items():   # explicit overrides win over the case defaults
-        if k not in ("bc", "bcu", "ph"):
+        if k not in ("bc", "bcu", "ph", "eos_kind", "mg"):
             setattr(cfg, k, v)
     return cfg, U, tend
 

@@ -106,7 +106,9 @@ static double ppow(double x, double y) { return pexp(y * plog(x)); }
 static double phypot(double a, double b) { return std::sqrt(a * a + b * b); }
 
 // ------------------------------------------------------------------------------------------------ options / model
-struct Phase { double g, pi, q, qp, cv, b; };   // PHY:49 (SG / NASG subset)
+// PHY:49-53: kind 0 is NASG (b = 0: SG) with g, pi, q, qp, cv, b; kind 1 (Cochran-Chan) and 2 (JWL) are Mie-Grueneisen
+// phases p = G rho (e - er(rho)) + pr(rho) with G, c1, c2, e1, e2, rr (reference density) and e0 (JWL's energy shift)
+struct Phase { double g, pi, q, qp, cv, b; int kind = 0; double G = 0, c1 = 0, c2 = 0, e1 = 0, e2 = 0, rr = 0, e0 = 0; };
 
 // the saturation line T_sat(p) of the two phases (PHY:111-163): ln p on a uniform grid and the root T there
 struct SatTable { std::vector<double> lp, T; double pcrit = std::numeric_limits<double>::infinity(), vcrit = 0.0, pmin = 0.0; };
@@ -122,30 +124,55 @@ struct Model {
 	double log_(double x) const { return portable ? plog(x) : std::log(x); }
 	double pow_(double x, double y) const { return portable ? ppow(x, y) : std::pow(x, y); }
 	double hypot_(double a, double b) const { return portable ? phypot(a, b) : std::hypot(a, b); }
+	bool is_mg(int k) const { return PH[k].kind != 0; }                                            // PHY:55-56
+	bool mg_any() const { return is_mg(0) || is_mg(1); }
 };
+// the Mie-Grueneisen reference curve at density r: pr(r), pr'(r), er(r) (PHY:58-66). Cochran-Chan: pr = c1 x^e1 - c2 x^e2
+// with x = r / rr, er its integral with er' = pr / r^2; JWL: pr = c1 exp(-e1 rr / r) + c2 exp(-e2 rr / r), er shifted by e0
+static void mg_ref(const Model& M, int k, double r, double& pr, double& dpr, double& er) {
+	const Phase& P = M.PH[k];
+	if (P.kind == 1) {
+		const double x = r / P.rr, t1 = P.c1 * M.pow_(x, P.e1), t2 = P.c2 * M.pow_(x, P.e2);
+		pr = t1 - t2; dpr = (P.e1 * t1 - P.e2 * t2) / r; er = (t1 / (P.e1 - 1.0) - t2 / (P.e2 - 1.0)) / r;
+	} else {
+		const double t1 = P.c1 * M.exp_(-P.e1 * P.rr / r), t2 = P.c2 * M.exp_(-P.e2 * P.rr / r);
+		pr = t1 + t2; dpr = (P.e1 * t1 + P.e2 * t2) * P.rr / (r * r); er = t1 / (P.e1 * P.rr) + t2 / (P.e2 * P.rr) - P.e0;
+	}
+}
 
 // ------------------------------------------------------------------------------------------------ EOS (PHY:69-109)
-static inline double eps_amp(const Model& M, int k, double a, double m, double p) { const Phase& P = M.PH[k];
-	return (p + P.g * P.pi) * (a - m * P.b) / (P.g - 1.0) + m * P.q; }                               // PHY:69-71
-static inline double p_ame(const Model& M, int k, double a, double m, double e) { const Phase& P = M.PH[k];
-	return (e - m * P.q) * (P.g - 1.0) / (a - m * P.b) - P.g * P.pi; }                               // PHY:72-74
-static inline double T_amp(const Model& M, int k, double a, double m, double p) { const Phase& P = M.PH[k];
-	return (a / m - P.b) * (p + P.pi) / ((P.g - 1.0) * P.cv); }                                      // PHY:75-77
-static inline double c2_rp(const Model& M, int k, double r, double p) { const Phase& P = M.PH[k];
-	return P.g * (p + P.pi) / (r * (1.0 - r * P.b)); }                                               // PHY:83-85
-static inline double dedp_am(const Model& M, int k, double a, double m) { const Phase& P = M.PH[k];
-	return (a - m * P.b) / (P.g - 1.0); }                                                             // PHY:86-88
+static inline double eps_amp(const Model& M, int k, double a, double m, double p) { const Phase& P = M.PH[k];   // PHY:69-71
+	if (P.kind) { double pr, dpr, er; mg_ref(M, k, m / a, pr, dpr, er); return m * er + a * (p - pr) / P.G; }
+	return (p + P.g * P.pi) * (a - m * P.b) / (P.g - 1.0) + m * P.q; }
+static inline double p_ame(const Model& M, int k, double a, double m, double e) { const Phase& P = M.PH[k];     // PHY:72-74
+	if (P.kind) { double pr, dpr, er; mg_ref(M, k, m / a, pr, dpr, er); return P.G * (e - m * er) / a + pr; }
+	return (e - m * P.q) * (P.g - 1.0) / (a - m * P.b) - P.g * P.pi; }
+static inline double T_amp(const Model& M, int k, double a, double m, double p) { const Phase& P = M.PH[k];     // PHY:75-77
+	if (P.kind) { if (!(P.cv > 0)) return NAN; double pr, dpr, er; mg_ref(M, k, m / a, pr, dpr, er); return (a * (p - pr) / (P.G * m)) / P.cv; }
+	return (a / m - P.b) * (p + P.pi) / ((P.g - 1.0) * P.cv); }
+static inline double c2_rp(const Model& M, int k, double r, double p) { const Phase& P = M.PH[k];               // PHY:83-85
+	if (P.kind) { double pr, dpr, er; mg_ref(M, k, r, pr, dpr, er); return (P.G + 1.0) * (p - pr) / r + dpr; }
+	return P.g * (p + P.pi) / (r * (1.0 - r * P.b)); }
+static inline double dedp_am(const Model& M, int k, double a, double m) { const Phase& P = M.PH[k];             // PHY:86-88
+	if (P.kind) return a / P.G;
+	return (a - m * P.b) / (P.g - 1.0); }
 static double p_mix(const Model& M, double a1, double m1, double m2, double rhoe) {                 // PHY:90-99
 	const double a[2] = {a1, 1.0 - a1}, m[2] = {m1, m2};
 	double num = rhoe, den = 0.0;
 	for (int k = 0; k < 2; k++) {
-		num -= m[k] * M.PH[k].q + M.PH[k].g * M.PH[k].pi * dedp_am(M, k, a[k], m[k]);
+		if (M.is_mg(k)) { double pr, dpr, er; mg_ref(M, k, m[k] / a[k], pr, dpr, er); num -= m[k] * er - a[k] * pr / M.PH[k].G; }
+		else num -= m[k] * M.PH[k].q + M.PH[k].g * M.PH[k].pi * dedp_am(M, k, a[k], m[k]);
 		den += dedp_am(M, k, a[k], m[k]);
 	}
 	return num / den;
 }
-static inline bool p_admissible(const Model& M, int k, double p) { return p + M.PH[k].pi > 0; }     // PHY:101-104 (SG)
-static inline double eps_round(const Model& M, int k, double m) { return std::fabs(m * M.PH[k].q); } // PHY:106-109 (SG)
+static inline bool p_admissible(const Model& M, int k, double a, double m, double p) {              // PHY:101-104
+	if (M.is_mg(k)) return std::isfinite(p) && c2_rp(M, k, m / a, p) > 0;
+	return p + M.PH[k].pi > 0; }
+static inline bool p_adm_sg(const Model& M, int k, double p) { return p + M.PH[k].pi > 0; }         // the explicit scheme's (SG / NASG)
+static inline double eps_round(const Model& M, int k, double a, double m) {                         // PHY:106-109
+	if (M.is_mg(k)) { double pr, dpr, er; mg_ref(M, k, m / a, pr, dpr, er); return std::fabs(m * er) + std::fabs(a * pr / M.PH[k].G); }
+	return std::fabs(m * M.PH[k].q); }
 
 // ------------------------------------------------------------------------------------------------ primitives (PHY:835-857)
 struct Prim { double a1, r1, r2, u, v, p1, p2, p, rho, cf; };
@@ -155,7 +182,7 @@ static bool prim_of(const Model& M, const double* Q, Prim& w) {                 
 	w.a1 = a1; w.rho = m1 + m2; w.r1 = m1 / a1; w.r2 = m2 / a2; w.u = Q[IMU] / w.rho; w.v = Q[IMV] / w.rho;
 	w.p = p_mix(M, a1, m1, m2, Q[IEN] - 0.5 * w.rho * (w.u * w.u + w.v * w.v));
 	w.p1 = p_ame(M, 0, a1, m1, Q[IE1]); w.p2 = p_ame(M, 1, a2, m2, Q[IE2]);
-	if (!(p_admissible(M, 0, w.p1) && p_admissible(M, 1, w.p2))) return false;
+	if (!(p_admissible(M, 0, a1, m1, w.p1) && p_admissible(M, 1, a2, m2, w.p2))) return false;
 	w.cf = std::sqrt((m1 * c2_rp(M, 0, w.r1, w.p1) + m2 * c2_rp(M, 1, w.r2, w.p2)) / w.rho);
 	return true;
 }
@@ -171,18 +198,44 @@ static inline double pbar_of(const Model& M, const double* Q) {                 
 }
 
 // ------------------------------------------------------------------------------------------------ shock budget (PHY:260-418)
-static inline void eos_ep_ev(const Model& M, int k, double p, double v, double& ep, double& evp) {  // PHY:266-270 (SG)
-	const Phase& P = M.PH[k]; ep = (v - P.b) / (P.g - 1.0); evp = P.g * (p + P.pi) / (P.g - 1.0); }
-static inline bool eos_ok(const Model& M, int k, double p, double v) {                             // PHY:271-275 (SG)
+// de/dp at fixed v and de/dv at fixed p plus p, of phase k at (p, v) (PHY:266-270)
+static inline void eos_ep_ev(const Model& M, int k, double p, double v, double& ep, double& evp) {
+	const Phase& P = M.PH[k];
+	if (P.kind) { double pr, dpr, er; const double r = 1.0 / v; mg_ref(M, k, r, pr, dpr, er); ep = v / P.G; evp = (p - pr) * (1.0 + 1.0 / P.G) + r * dpr / P.G; return; }
+	ep = (v - P.b) / (P.g - 1.0); evp = P.g * (p + P.pi) / (P.g - 1.0); }
+static inline bool eos_ok(const Model& M, int k, double p, double v) {                             // PHY:271-275
 	if (!(v > 0) || !std::isfinite(p)) return false;
+	if (M.is_mg(k)) return c2_rp(M, k, 1.0 / v, p) > 0;
 	return v > M.PH[k].b && p + M.PH[k].pi > 0; }
-static double hug_v(const Model& M, int k, double p0, double p1, double v0) {                      // PHY:277-282 (SG)
-	const Phase& P = M.PH[k]; const double x0 = p0 + P.pi, x1 = p1 + P.pi;
-	return P.b + (v0 - P.b) * ((P.g - 1.0) * x1 + (P.g + 1.0) * x0) / ((P.g + 1.0) * x1 + (P.g - 1.0) * x0);
+// phase k's Hugoniot from (p0, v0) to p1 (PHY:277-292): closed form for NASG; for a Mie-Grueneisen phase the root of
+// e(p1, v) - e(p0, v0) + (p0 + p1) / 2 (v - v0) below v0, bracketed by steps of 0.9 v and bisected to 1e-14 v0
+static double hug_v(const Model& M, int k, double p0, double p1, double v0) {
+	const Phase& P = M.PH[k];
+	if (!P.kind) {
+		const double x0 = p0 + P.pi, x1 = p1 + P.pi;
+		return P.b + (v0 - P.b) * ((P.g - 1.0) * x1 + (P.g + 1.0) * x0) / ((P.g + 1.0) * x1 + (P.g - 1.0) * x0);
+	}
+	auto e_pv = [&](double p, double v) { double pr, dpr, er; mg_ref(M, k, 1.0 / v, pr, dpr, er); return er + v * (p - pr) / P.G; };
+	const double e0 = e_pv(p0, v0);
+	auto hg = [&](double v) { return e_pv(p1, v) - e0 + 0.5 * (p0 + p1) * (v - v0); };
+	double hi = v0, lo = v0, gl = hg(v0);
+	if (!(gl > 0)) return v0;
+	for (int it = 0; it < 80 && gl > 0; it++) { hi = lo; lo *= 0.9; gl = hg(lo); if (!std::isfinite(gl)) return NAN; }
+	if (gl > 0) return NAN;
+	for (int it = 0; it < 100; it++) {
+		const double mid = 0.5 * (lo + hi), gm = hg(mid);
+		if (gm > 0) hi = mid; else lo = mid;
+		if (hi - lo <= 1e-14 * v0) break;
+	}
+	return 0.5 * (lo + hi);
 }
-static void K_G(const Model& M, int k, double p, double v, double& K, double& G) {                 // PHY:294-299 (SG)
+// isentropic bulk modulus K and fundamental derivative G = (1 + dK/dp|_s) / 2 (PHY:294-299): closed form for NASG, a
+// one-sided difference along the isentrope dv = -v dp / K (dp = 1e-6 K) for a Mie-Grueneisen phase
+static void K_G(const Model& M, int k, double p, double v, double& K, double& G) {
 	K = c2_rp(M, k, 1.0 / v, p) / v;
-	G = 0.5 * (M.PH[k].g + 1.0); if (M.PH[k].b != 0.0) G *= v / (v - M.PH[k].b);
+	if (!M.is_mg(k)) { G = 0.5 * (M.PH[k].g + 1.0); if (M.PH[k].b != 0.0) G *= v / (v - M.PH[k].b); return; }
+	const double dp = 1e-6 * K, K2 = c2_rp(M, k, 1.0 / (v * (1.0 - dp / K)), p + dp) / (v * (1.0 - dp / K));
+	G = 0.5 * (1.0 + (K2 - K) / dp);
 }
 struct ShockProf { double Y[2], vk0[2], v0, p0, j2, vend; };
 static bool prof_rhs(const Model& M, const ShockProf& S, double c, double v, const double* y, double* dy) {   // PHY:302-310
@@ -354,7 +407,7 @@ static void stencil_state(const Model& M, const double* c, double* st) {        
 	st[5] = r > 0 ? c[IMU] / r : 0.0; st[6] = r > 0 ? c[IMV] / r : 0.0;
 	if (a1 > 0 && a1 < 1 && m1 > 0 && m2 > 0) {
 		const double p = p_mix(M, a1, m1, m2, c[IE1] + c[IE2]);
-		if (p_admissible(M, 0, p) && p_admissible(M, 1, p)) st[4] = p;
+		if (p_admissible(M, 0, a1, m1, p) && p_admissible(M, 1, 1.0 - a1, m2, p)) st[4] = p;
 	}
 }
 // cellptr(di, dj) returns a pointer to the 8 conserved values of the stencil cell
@@ -391,7 +444,10 @@ enum { CL_P = 0, CL_PT = 2, CL_THETA = 3, CL_NU = 4, CL_PTG = 5 };
 struct ClIn { double atr[2], mtr[2], etr[2], s, rhoeC, pguess, Qph[2]; int type = CL_P; bool nodelta = false; double tau = 0.0; };
 struct ClOut { double a[2], m[2], e[2], p[2], T[2], pbar; };
 struct PCoef { double A[2], B[2], N[2], lam[2]; };
-static double E_above(const Model& M, const ClIn& in, int k) {                                     // PHY:420-423 (SG)
+static double E_above(const Model& M, const ClIn& in, int k) {                                     // PHY:420-423
+	if (M.is_mg(k)) {
+		if (!(in.atr[k] > 0 && in.mtr[k] > 0)) return 0.0;
+		double pr, dpr, er; mg_ref(M, k, in.mtr[k] / in.atr[k], pr, dpr, er); return smax(0.0, in.etr[k] - in.mtr[k] * er); }
 	return smax(0.0, in.etr[k] - in.mtr[k] * M.PH[k].q - (in.atr[k] - in.mtr[k] * M.PH[k].b) * M.PH[k].pi); }
 static void defect_split(const Model& M, const ClIn& in, double delta, double lam[2], double extra[2]) {   // PHY:428-439 (SHARE_MODE 0)
 	double E[2], sE = 0.0; for (int k = 0; k < 2; k++) { E[k] = E_above(M, in, k); sE += E[k]; }
@@ -673,7 +729,125 @@ static bool closure_pTg(const Model& M, const ClIn& in, ClOut& o) {
 	for (int k = 0; k < 2; k++) { o.a[k] = o.m[k] * v_pT(M, k, p, T); o.e[k] = o.m[k] * e_pT(M, k, p, T); o.p[k] = p; o.T[k] = T; }
 	o.pbar = p; return true;
 }
+// ---- the p closure with a Mie-Grueneisen phase (alpha_mg PHY:498-528, closure_p_mg PHY:530-573; PHO:552-628 differ
+// only in the start and the two early exits marked)
+// alpha_k(p) of an MG phase is the root of f(a) = m er(m / a) + a (p - pr) / G + p a - A - B p, increasing where
+// c^2 > 0 (f' = rho c^2 / G): the guess is first moved up past the c^2 <= 0 side, then the root is bracketed by
+// doubling / halving (a step across c^2 = 0 bisects geometrically for a point of the admissible branch with f <= 0)
+// and polished by safeguarded Newton. Returns alpha and d alpha / d p.
+static bool alpha_mg(const Model& M, int k, double m, double A, double B, double p, double guess, double& al, double& dal) {
+	const Phase& P = M.PH[k];
+	const bool pho = M.closure_path != 0;
+	auto F = [&](double a, double& df) {
+		double pr, dpr, er; const double r = m / a; mg_ref(M, k, r, pr, dpr, er);
+		df = ((P.G + 1.0) * (p - pr) + r * dpr) / P.G;
+		return m * er + a * (p - pr) / P.G + p * a - A - B * p; };
+	double lo = guess, hi = guess, d; double flo = F(lo, d), fhi = flo;
+	if (!std::isfinite(flo)) return false;
+	if (!(d > 0)) {
+		for (int it = 0; it < 200 && !(d > 0); it++) { guess *= 2.0; flo = F(guess, d); if (!std::isfinite(flo)) return false; }
+		if (!(d > 0)) return false;
+		lo = hi = guess; fhi = flo;
+	}
+	if (flo < 0) {
+		for (int it = 0; it < 200 && fhi < 0; it++) { lo = hi; flo = fhi; hi *= 2.0; fhi = F(hi, d); if (!std::isfinite(fhi)) return false; }
+		if (fhi < 0) return false;
+	} else {
+		for (int it = 0; it < 200 && flo > 0; it++) {
+			hi = lo; fhi = flo; lo *= 0.5; flo = F(lo, d); if (!std::isfinite(flo)) return false;
+			if (d <= 0) {
+				bool found = false;
+				for (int b = 0; b < 200; b++) {
+					const double mid = std::sqrt(lo * hi); double dm; const double fm = F(mid, dm);
+					if (!std::isfinite(fm)) return false;
+					if (!(dm > 0)) lo = mid;
+					else if (fm > 0) { hi = mid; fhi = fm; }
+					else { lo = mid; flo = fm; found = true; break; }
+					if (hi - lo <= 1e-15 * hi) break;
+				}
+				if (!found) return false;
+				break;
+			}
+		}
+		if (flo > 0) return false;
+	}
+	const bool gin = pho ? (guess >= lo && guess <= hi) : (guess > lo && guess < hi);   // PHO:571 starts from an endpoint guess too
+	double a = (flo == 0) ? lo : (fhi == 0 ? hi : (gin ? guess : std::sqrt(lo * hi)));
+	for (int it = 0; it < 200; it++) {
+		double df; const double fa = F(a, df);
+		if (fa == 0) break;
+		if (fa < 0) lo = a; else hi = a;
+		double an = (df > 0) ? a - fa / df : -1.0;
+		if (pho && an > 0 && std::fabs(an - a) <= 4e-16 * a) { a = an; break; }   // PHO:577: a converged Newton step first
+		if (!(an > lo && an < hi)) an = std::sqrt(lo * hi);
+		if (std::fabs(an - a) <= 4e-16 * a || hi - lo <= 4e-16 * hi) { a = an; break; }
+		a = an;
+	}
+	double df; F(a, df); if (!(df > 0)) return false;
+	al = a; dal = -(a / P.G + a - B) / df; return true;
+}
+// Phi(p) = sum_k alpha_k(p) - 1 on the guess's defect segment, each MG phase's alpha from alpha_mg (warm-started from the
+// last accepted alphas), the NASG phases' in closed form; bracketed from the guess by doubling steps, then bisection-
+// safeguarded Newton
+static bool closure_p_mg(const Model& M, const ClIn& in, ClOut& o) {
+	PCoef c; if (!p_coef(M, in, c, in.pguess)) return false;
+	const bool pho = M.closure_path != 0;
+	double ag[2] = {in.atr[0] / (in.atr[0] + in.atr[1]), in.atr[1] / (in.atr[0] + in.atr[1])};
+	auto Phi = [&](double p, double& dPhi, bool upd) -> double {
+		double sum = -1.0; dPhi = 0.0; double an[2];
+		for (int k = 0; k < 2; k++) {
+			double a, da;
+			if (M.is_mg(k)) { if (!alpha_mg(M, k, in.mtr[k], c.A[k], c.B[k], p, ag[k], a, da)) return NAN; }
+			else { if (!(p + M.PH[k].pi > 0) || !(c.N[k] > 0)) return NAN; a = alpha_p(M, c, in, k, p); da = dalpha_p(M, c, k, p); }
+			an[k] = a; sum += a; dPhi += da;
+		}
+		if (upd) { ag[0] = an[0]; ag[1] = an[1]; }
+		return sum; };
+	double p = in.pguess, d;
+	const double f = Phi(p, d, true);
+	double scale = 1e-6 * (std::fabs(p) + 1e5);
+	double lo, hi, flo, fhi;
+	if (!std::isfinite(f) || f > 0) {
+		lo = p; flo = f; hi = p;
+		for (int it = 0; it < 300; it++) {
+			hi = p + scale; fhi = Phi(hi, d, false);
+			if (std::isfinite(fhi) && fhi <= 0) break;
+			if (std::isfinite(fhi)) { lo = hi; flo = fhi; }
+			scale *= 2.0;
+		}
+		if (!(std::isfinite(fhi) && fhi <= 0)) return false;
+	} else {
+		hi = p; fhi = f; lo = p;
+		for (int it = 0; it < 300; it++) {
+			lo = p - scale; flo = Phi(lo, d, false);
+			if (std::isfinite(flo) && flo >= 0) break;
+			if (!std::isfinite(flo)) { scale *= 0.5; if (scale < 1e-300) return false; continue; }
+			hi = lo; fhi = flo; scale *= 2.0;
+		}
+		if (!(std::isfinite(flo) && flo >= 0)) return false;
+	}
+	(void)flo; (void)fhi;
+	p = (std::isfinite(f) && in.pguess >= lo && in.pguess <= hi) ? in.pguess : 0.5 * (lo + hi);
+	for (int it = 0; it < 300; it++) {
+		double df; const double fp = Phi(p, df, true);
+		if (fp == 0.0) break;
+		if (!std::isfinite(fp) || fp > 0) lo = p; else hi = p;
+		double pn = (std::isfinite(fp) && df < 0) ? p - fp / df : 0.5 * (lo + hi);
+		if (pho && std::isfinite(fp) && df < 0 && std::fabs(pn - p) <= 2e-16 * (std::fabs(p) + std::fabs(pn))) { p = pn; break; }   // PHO:617
+		if (!(pn > lo && pn < hi)) pn = 0.5 * (lo + hi);
+		if (std::fabs(pn - p) <= 2e-16 * (std::fabs(p) + std::fabs(pn)) || hi - lo <= 4e-16 * (std::fabs(hi) + std::fabs(lo))) { p = pn; break; }
+		p = pn;
+	}
+	double df; if (!std::isfinite(Phi(p, df, true))) return false;
+	for (int k = 0; k < 2; k++) {
+		o.a[k] = ag[k]; o.m[k] = in.mtr[k];
+		if (!p_admissible(M, k, o.a[k], o.m[k], p)) return false;
+		o.e[k] = eps_amp(M, k, o.a[k], o.m[k], p); o.p[k] = p; o.T[k] = T_amp(M, k, o.a[k], o.m[k], p);
+	}
+	o.pbar = p; return std::isfinite(p);
+}
 static bool closure_p(const Model& M, const ClIn& in, ClOut& o) {
+	if (M.mg_any()) return closure_p_mg(M, in, o);                                                  // PHY:463, PHO:473
 	return M.closure_path == 0 ? closure_p_phy(M, in, o) : closure_p_pho(M, in, o); }
 // ---- finite relaxation rates, backward-Euler form (RELAX_FORM 0, the reference's default)
 // finite theta (B6.4, PHY:679-700): at fixed p the phases' temperatures are linear in the exchanged heat, and
@@ -743,7 +917,7 @@ static bool closure(const Model& M, const ClIn& in, ClOut& o) {                 
 	return ok && std::isfinite(o.pbar);
 }
 static double closure_roundoff(const Model& M, const ClIn& in, double p) {                         // PC:631-638
-	if (in.type != CL_P) return 0.0;
+	if (in.type != CL_P || M.mg_any()) return 0.0;
 	PCoef c; if (!p_coef(M, in, c, p)) return 0.0;
 	double dphi = 0.0, sens = 0.0;
 	for (int k = 0; k < 2; k++) { dphi += dalpha_p(M, c, k, p); sens += std::fabs(c.lam[k]) * (M.PH[k].g - 1.0) / (M.PH[k].g * (p + M.PH[k].pi)); }
@@ -805,7 +979,15 @@ struct Cfg {
 	int relax; double eps_i;   // RELAX_LEVEL (PHY:37; 0 p, 1 p-pT) and EPS_I (PHY:38)
 	int shock_fused;           // 1: the strong-shock profile integrations in fused form (prof_integrate_fused)
 	double theta_rate, nu_rate;   // theta_rate, nu_rate (PHY:33); < 0: the level's default (PC:1148-1149)
+	int eos_kind[2];              // Phase::kind (PHY:50-53): 0 NASG / SG, 1 Cochran-Chan, 2 JWL
+	double mg[2][7];              // a Mie-Grueneisen phase's G, c1, c2, e1, e2, rr, e0
 };
+static Phase phase_of(const double* ph, int kind, const double* mg) {
+	Phase P{ph[0], ph[1], ph[2], ph[3], ph[4], ph[5]};
+	P.kind = kind;
+	if (kind) { P.G = mg[0]; P.c1 = mg[1]; P.c2 = mg[2]; P.e1 = mg[3]; P.e2 = mg[4]; P.rr = mg[5]; P.e0 = mg[6]; }
+	return P;
+}
 struct Stats { long steps, rk1, rejects, floor_acc, stages, newton_it, krylov_it, linsolves; double maxres, t, dt_last; int status;
                long jac_builds; };
 
@@ -837,7 +1019,7 @@ struct Ctx {
 
 static void init_ctx(Ctx& X, const Cfg& C) {
 	X.C = C;
-	for (int k = 0; k < 2; k++) X.M.PH[k] = Phase{C.ph[k][0], C.ph[k][1], C.ph[k][2], C.ph[k][3], C.ph[k][4], C.ph[k][5]};
+	for (int k = 0; k < 2; k++) X.M.PH[k] = phase_of(C.ph[k], C.eos_kind[k], C.mg[k]);
 	X.M.closure_path = C.closure_path; X.M.share_start = C.share_start; X.M.dot_order = C.dot_order; X.M.portable = C.portable_math;
 	X.M.shock_fused = C.shock_fused;
 	X.M.theta_rate = C.theta_rate < 0 ? (C.relax >= 1 ? INF : 0.0) : C.theta_rate;                 // PC:1148-1149
@@ -944,6 +1126,7 @@ static void tvd_reconstruction(Ctx& X, const std::vector<double>& Q) {          
 			if (lim && phi > 0.0) {   // face admissibility cap (SG/NASG phases)
 				const size_t cs[3] = {ca, cc, cb};
 				for (int ph = 0; ph < 2; ph++) {
+					if (M.is_mg(ph)) continue;
 					double L1[3], L2[3];
 					for (int n = 0; n < 3; n++) {
 						const double a = ph == 0 ? U(cs[n], IA1) : 1.0 - U(cs[n], IA1), m = U(cs[n], ph == 0 ? IM1 : IM2), e = U(cs[n], ph == 0 ? IE1 : IE2);
@@ -1135,7 +1318,7 @@ static bool stage_prep(Ctx& X) {                                                
 		double cz = INF;
 		if (a1 > 0 && a2 > 0 && m1 > 0 && m2 > 0) {
 			const double p1 = p_ame(M, 0, a1, m1, XS(i, j, IE1)), p2 = p_ame(M, 1, a2, m2, XS(i, j, IE2));
-			if (p_admissible(M, 0, p1) && p_admissible(M, 1, p2)) {
+			if (p_admissible(M, 0, a1, m1, p1) && p_admissible(M, 1, a2, m2, p2)) {
 				const double c21 = c2_rp(M, 0, m1 / a1, p1), c22 = c2_rp(M, 1, m2 / a2, p2);
 				const double c2 = (m1 * c21 + m2 * c22) / (m1 + m2);
 				cz = c2 > 0 ? std::sqrt(c2) : INF;
@@ -1146,9 +1329,9 @@ static bool stage_prep(Ctx& X) {                                                
 		double K = smax(M.PH[0].pi, M.PH[1].pi), PE = 0.0;
 		if (a1 > 0 && a2 > 0 && m1 > 0 && m2 > 0) {
 			const double p1 = p_ame(M, 0, a1, m1, XS(i, j, IE1)), p2 = p_ame(M, 1, a2, m2, XS(i, j, IE2));
-			if (p_admissible(M, 0, p1) && p_admissible(M, 1, p2)) K = 1.0 / ((a1 / sa) / (m1 / a1 * c2_rp(M, 0, m1 / a1, p1)) + (a2 / sa) / (m2 / a2 * c2_rp(M, 1, m2 / a2, p2)));
+			if (p_admissible(M, 0, a1, m1, p1) && p_admissible(M, 1, a2, m2, p2)) K = 1.0 / ((a1 / sa) / (m1 / a1 * c2_rp(M, 0, m1 / a1, p1)) + (a2 / sa) / (m2 / a2 * c2_rp(M, 1, m2 / a2, p2)));
 			const double dd = dedp_am(M, 0, a1, m1) + dedp_am(M, 1, a2, m2);
-			if (dd > 0) PE = (std::fabs(XS(i, j, IE1)) + eps_round(M, 0, m1) + std::fabs(XS(i, j, IE2)) + eps_round(M, 1, m2)) / dd;
+			if (dd > 0) PE = (std::fabs(XS(i, j, IE1)) + eps_round(M, 0, a1, m1) + std::fabs(XS(i, j, IE2)) + eps_round(M, 1, a2, m2)) / dd;
 		}
 		X.Kw[c] = K + PE;
 		X.Pold[c] = X.Pst[M_P(X, i, j)];   // PC:899-900: the stage pressure (Newton guess)
@@ -1666,7 +1849,7 @@ static HFace hllc(const Model& M, const Prim& L, const Prim& R, int dir) {      
 	for (int k = 0; k < 2; k++) {
 		const double v0 = 1.0 / rk[k], vs = 1.0 / (rk[k] * fac);
 		double ps = hugoniot_p(M, k, pk[k], v0, vs);
-		if (!std::isfinite(ps) || !p_admissible(M, k, ps)) ps = pk[k];
+		if (!std::isfinite(ps) || !p_adm_sg(M, k, ps)) ps = pk[k];
 		f.ue[k] = eps_amp(M, k, ak[k], ak[k] * rk[k] * fac, ps) * SM;
 	}
 	return f;
@@ -1749,9 +1932,9 @@ static bool hllc_relax_cell(const Model& M, double* Q, const double Qph[2]) {
 	{
 		const double ak[2] = {a1, 1 - a1}, mk[2] = {m1, m2}; double* ek[2] = {&e1, &e2};
 		for (int k = 0; k < 2; k++)
-			if (!p_admissible(M, k, p_ame(M, k, ak[k], mk[k], *ek[k]))) {
+			if (!p_adm_sg(M, k, p_ame(M, k, ak[k], mk[k], *ek[k]))) {
 				const double pm = p_mix(M, a1, m1, m2, rhoe);
-				if (p_admissible(M, k, pm)) *ek[k] = eps_amp(M, k, ak[k], mk[k], pm);
+				if (p_adm_sg(M, k, pm)) *ek[k] = eps_amp(M, k, ak[k], mk[k], pm);
 			}
 	}
 	ClIn in{}; in.atr[0] = a1; in.atr[1] = 1 - a1; in.mtr[0] = m1; in.mtr[1] = m2; in.etr[0] = e1; in.etr[1] = e2;
@@ -1847,6 +2030,20 @@ static bool case_setup(int test, Cfg& C, double& tend) {                        
 		const double l[6] = {1.187, 7028e5, -1177788.0, 0.0, 3610.0, 6.61e-4}, v[6] = {1.467, 0.0, 2077616.0, 14317.0, 955.0, 0.0};
 		for (int q = 0; q < 6; q++) { C.ph[0][q] = l[q]; C.ph[1][q] = v[q]; }
 		tend = 3e-3; break; }
+	case 60: case 61: {   // PHY:991-997: Saurel-Petitpas-Berry's nitromethane tube (both phases one Cochran-Chan fluid);
+		// Pelanti's detonation gases (JWL) | NASG water tube on [-10, 10] m
+		auto mgp = [&](int k, int kind, double G, double c1, double c2, double e1, double e2, double rr, double e0, double g) {
+			for (int q = 0; q < 6; q++) C.ph[k][q] = 0.0;
+			C.ph[k][0] = g; C.eos_kind[k] = kind;
+			const double v[7] = {G, c1, c2, e1, e2, rr, e0}; for (int q = 0; q < 7; q++) C.mg[k][q] = v[q]; };
+		if (test == 60) { for (int k = 0; k < 2; k++) mgp(k, 1, 1.19, 0.819181e9, 1.50835e9, 4.52969, 1.42144, 1134.0, 0.0, 2.19); tend = 67e-6; }
+		else {
+			mgp(0, 2, 0.25, 854.5e9, 20.5e9, 4.6, 1.35, 1840.0, 0.0, 1.25);
+			const double w[6] = {1.187, 7028e5, -1177788.0, 0.0, 3610.0, 6.61e-4};
+			for (int q = 0; q < 6; q++) C.ph[1][q] = w[q];
+			C.x0 = -10.0; C.x1 = 10.0; tend = 0.9e-3;
+		}
+		break; }
 	case 4: set_SG(C, 0, 4.4, 6e8); set_SG(C, 1, 1.4, 0); tend = 2.79e-3; break;
 	case 5: set_SG(C, 0, 4.4, 6e8); set_SG(C, 1, 1.4, 0); tend = 240e-6; break;
 	case 80: case 82: case 83: case 84: set_SG(C, 0, 1.4, 0); set_SG(C, 1, 1.4, 0); tend = test == 80 ? 0.5 : test == 82 ? 0.14 : test == 83 ? 2.5e-6 : 0.15; break;
@@ -1875,6 +2072,8 @@ static void case_state(const Model& M, int test, double x, double y, double* Q) 
 		const double T0 = 353.0, u = x < 0.5 ? -2.0 : 2.0;
 		setcell(1 - 1e-2, 1.0 / v_pT(M, 0, 1e5, T0), 1.0 / v_pT(M, 1, 1e5, T0), u, 0, 1e5, 1e5); break; }
 	case 4: { const bool L = x < 0.5; setcell(L ? 1 - 1e-8 : 1e-8, 1000.0, 10.0, 100.0, 0, 1e5, 1e5); break; }
+	case 60: { const bool L = x < 0.5; setcell(L ? 1 - 1e-8 : 1e-8, 1134.0, 1200.0, 0, 0, L ? 20e9 : 0.2e6, L ? 20e9 : 0.2e6); break; }   // PHY:1040
+	case 61: { const bool L = x < 0.0; setcell(L ? 1 - 1e-8 : 1e-8, 2000.0, 1044.0, 0, 0, L ? 4.6406e10 : 1e5, L ? 4.6406e10 : 1e5); break; }   // PHY:1041
 	case 5: { const bool L = x < 0.75; setcell(L ? 1 - 1e-6 : 1e-6, 1000.0, 1.0, 0, 0, L ? 1e9 : 1e5, L ? 1e9 : 1e5); break; }
 	case 80: case 81: case 82: case 83: case 84: case 85: {
 		double xi = 0.5, rL = 0, uL = 0, pL = 0, rR = 0, uR = 0, pR = 0;
@@ -1918,9 +2117,9 @@ static void cell_diag(const Model& M, const double* Q, double* o) {             
 using namespace ora;
 extern "C" {
 
-typedef struct { double ph[2][6]; int closure_path, share_start, dot_order, portable_math, shock_fused; } OraModel;
+typedef struct { double ph[2][6]; int closure_path, share_start, dot_order, portable_math, shock_fused; int eos_kind[2]; double mg[2][7]; } OraModel;
 static Model mk_model(const OraModel* m) {
-	Model M; for (int k = 0; k < 2; k++) M.PH[k] = Phase{m->ph[k][0], m->ph[k][1], m->ph[k][2], m->ph[k][3], m->ph[k][4], m->ph[k][5]};
+	Model M; for (int k = 0; k < 2; k++) M.PH[k] = phase_of(m->ph[k], m->eos_kind[k], m->mg[k]);
 	M.closure_path = m->closure_path; M.share_start = m->share_start; M.dot_order = m->dot_order; M.portable = m->portable_math;
 	M.shock_fused = m->shock_fused; return M;
 }
@@ -1928,7 +2127,11 @@ static Model mk_model(const OraModel* m) {
 int ora_version(void) { return 1; }
 int ora_set_threads(int n) { if (n > 0) omp_set_num_threads(n); return omp_get_max_threads(); }
 
-void* ora_create(const Cfg* C) { Ctx* X = new Ctx(); init_ctx(*X, *C); return X; }
+// NULL for a Mie-Grueneisen phase with a relaxation level above 0 (the reference fails every non-mechanical closure of
+// such a phase, PHY:790), an IMEX variant or the fused profile form (both written for NASG phases only)
+void* ora_create(const Cfg* C) {
+	if ((C->eos_kind[0] || C->eos_kind[1]) && (C->relax || C->variant || C->shock_fused)) return nullptr;
+	Ctx* X = new Ctx(); init_ctx(*X, *C); return X; }
 void ora_destroy(void* h) { delete (Ctx*)h; }
 size_t ora_cfg_size(void) { return sizeof(Cfg); }
 size_t ora_stats_size(void) { return sizeof(Stats); }
@@ -2042,7 +2245,9 @@ double ora_pref(void* h) { return ((Ctx*)h)->P_REF; }
 // sub-points where the case asks for it); returns t_end (< 0: unknown case)
 double ora_case_init(int test, Cfg* C, double* U) {
 	double tend; if (!case_setup(test, *C, tend)) return -1.0;
-	Model M; for (int k = 0; k < 2; k++) M.PH[k] = Phase{C->ph[k][0], C->ph[k][1], C->ph[k][2], C->ph[k][3], C->ph[k][4], C->ph[k][5]};
+	// the initial state as the reference's test_state builds it: libm in the Mie-Grueneisen reference curves
+	Model M; for (int k = 0; k < 2; k++) M.PH[k] = phase_of(C->ph[k], C->eos_kind[k], C->mg[k]);
+	M.portable = 0;
 	const double dx = (C->x1 - C->x0) / C->nx, dy = (C->y1 - C->y0) / C->ny;
 	const int sub = case_sub(test);
 	for (int j = 0; j < C->ny; j++) for (int i = 0; i < C->nx; i++) {
@@ -2059,6 +2264,15 @@ double ora_case_init(int test, Cfg* C, double* U) {
 	return tend;
 }
 // profile columns p u v rho alpha2 Y2 rho1 rho2 cf cw T1 T2 p1 p2 (14 per cell)
+// phase k's EOS at (alpha, m) for the tests: out = {eps_amp(p), p_ame(e), c2_rp(m / alpha, p), dedp_am, T_amp(p),
+// pr, pr', er of the Mie-Grueneisen reference curve at m / alpha (NaN for NASG)}
+void ora_eos(const OraModel* mo, int k, double a, double m, double p, double e, double* out) {
+	const Model M = mk_model(mo);
+	out[0] = eps_amp(M, k, a, m, p); out[1] = p_ame(M, k, a, m, e); out[2] = c2_rp(M, k, m / a, p); out[3] = dedp_am(M, k, a, m);
+	out[4] = T_amp(M, k, a, m, p);
+	out[5] = out[6] = out[7] = NAN;
+	if (M.is_mg(k)) mg_ref(M, k, m / a, out[5], out[6], out[7]);
+}
 void ora_cell_diag(const OraModel* m, const double* U, int ncell, double* out) { const Model M = mk_model(m); for (int c = 0; c < ncell; c++) cell_diag(M, &U[(size_t)c * NV], &out[(size_t)c * 14]); }
 
 // ---- per-function entry points (unit tests)

@@ -5,8 +5,9 @@ data/SPB2009_4.3.1_water_air/water_air_N100_t240us.dat (case 5), data/SPB2009_4.
 interface_advection_t2.79ms.dat (case 4, N = 200). Columns kept (17 significant digits in the source): p u v rho alpha2
 Y2 rho1 rho2 p1 p2 (write_profile, PC:1002-1013); and data/PS2014_water_cavitation_tube_{2mps,500mps}/ (cases 1 and 2,
 5000 cells, relaxation levels 0 to 3), data/Pelanti2022_8.2.1_water_cavitation/ (case 30, levels 0 to 2),
-data/PS2014_dodecane_shock_tube/ (case 3, 2000 cells, levels 0, 2, 3) and data/Pelanti2022_8.2.2_dodecane_NASG/
-(case 31, levels 0, 2). Run: python tests/golden/make_ref_profiles.py /root/reference
+data/PS2014_dodecane_shock_tube/ (case 3, 2000 cells, levels 0, 2, 3), data/Pelanti2022_8.2.2_dodecane_NASG/
+(case 31, levels 0, 2), data/SPB2009_4.2_nitromethane/ (case 60) and data/Pelanti2022_8.1.1_JWL_water/ (case 61).
+Run: python tests/golden/make_ref_profiles.py /root/reference
 """
 import os
 import sys
@@ -39,7 +40,11 @@ RELAX_FILES = {"c1_5000_r0": (1, 5000, 0, "data/PS2014_water_cavitation_tube_2mp
                "c3_2000_r2": (3, 2000, 2, "data/PS2014_dodecane_shock_tube/p-pT-pTg_relax.dat"),
                "c3_2000_r3": (3, 2000, 3, "data/PS2014_dodecane_shock_tube/p-pTg_relax.dat"),
                "c31_2000_r0": (31, 2000, 0, "data/Pelanti2022_8.2.2_dodecane_NASG/p_relax.dat"),
-               "c31_2000_r2": (31, 2000, 2, "data/Pelanti2022_8.2.2_dodecane_NASG/pTg_relax.dat")}
+               "c31_2000_r2": (31, 2000, 2, "data/Pelanti2022_8.2.2_dodecane_NASG/pTg_relax.dat"),
+               # the Mie-Grueneisen tubes (500 cells, level 0): Saurel-Petitpas-Berry's nitromethane (Cochran-Chan, case 60)
+               # and Pelanti's detonation gases (JWL) | water (NASG) (case 61)
+               "c60_500_r0": (60, 500, 0, "data/SPB2009_4.2_nitromethane/nitromethane_t67us.dat"),
+               "c61_500_r0": (61, 500, 0, "data/Pelanti2022_8.1.1_JWL_water/JWL_water_t0.9ms.dat")}
 RATES = {"c30_5000_theta2000": (2000.0, -1.0), "c30_5000_nu1e-4": (-1.0, 1e-4)}
 
 

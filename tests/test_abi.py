@@ -23,9 +23,11 @@ def test_header_declares_entry_points():
     assert "spb6_create" in names and "spb6_run" in names and len(names) >= 20
 
 
-def test_library_exports_every_declared_symbol():
+@pytest.mark.parametrize("mg", [False, True], ids=["nasg", "mie_grueneisen"])
+def test_library_exports_every_declared_symbol(mg):
+    """both builds, libspb6_hip.so and its Mie-Grueneisen build libspb6_hip_mg.so, export the whole C ABI"""
     from diffusionimex_amd import spb6
-    lib = spb6.load()
+    lib = spb6.load(mg=mg)
     missing = [n for n in _declared() if not hasattr(lib, n)]
     assert not missing, f"missing exports: {missing}"
     assert set(spb6.EXPORTS) <= set(_declared())
@@ -55,7 +57,19 @@ def test_python_structs_match_header_sizes():
 
 def test_missing_library_fails_loudly(monkeypatch):
     from diffusionimex_amd import spb6
-    monkeypatch.setattr(spb6, "_lib", None)
+    monkeypatch.setattr(spb6, "_libs", {})
     monkeypatch.setattr(spb6, "LIB_PATH", "/nonexistent/libspb6_hip.so")
     with pytest.raises(RuntimeError):
         spb6.load()
+
+
+def test_nasg_library_rejects_mie_grueneisen_phases():
+    """libspb6_hip.so is built without the Mie-Grueneisen branches: spb6_create refuses an MG phase (before any device
+    call) and names the library that runs it"""
+    from diffusionimex_amd import spb6
+    lib = spb6.load(mg=False)
+    cfg = spb6.make_config(16, eos_kind=(2, 0), mg=((0.25, 854.5e9, 20.5e9, 4.6, 1.35, 1840.0, 0.0), (0.0,) * 7))
+    assert not lib.spb6_create(ctypes.byref(cfg), 0)
+    assert b"libspb6_hip_mg.so" in lib.spb6_last_error()
+    bad = spb6.make_config(16, eos_kind=(2, 0), mg=((0.25, 854.5e9, 20.5e9, 4.6, 1.35, 1840.0, 0.0), (0.0,) * 7), relax=1)
+    assert not spb6.load(mg=True).spb6_create(ctypes.byref(bad), 0)
