@@ -120,6 +120,7 @@ struct Model {
 	int shock_fused = 0;   // the strong-shock profile integrations in fused form (prof_integrate_fused)
 	// the thermal and chemical relaxation rates (PHY:33; resolved by level as PC:1148-1149: inf = instantaneous)
 	double theta_rate = 0.0, nu_rate = 0.0;
+	double mu_rate = std::numeric_limits<double>::infinity();   // the mechanical relaxation rate (PHY:33: inf = instantaneous)
 	double exp_(double x) const { return portable ? pexp(x) : std::exp(x); }
 	double log_(double x) const { return portable ? plog(x) : std::log(x); }
 	double pow_(double x, double y) const { return portable ? ppow(x, y) : std::pow(x, y); }
@@ -438,10 +439,11 @@ template <class F> static void shock_budget(const Model& M, F cellptr, bool twoD
 
 // ------------------------------------------------------------------------------------------------ p closure (PHY:420-494, PHO:472-548)
 // closure types of PHY:226 used here: mechanical (p), thermal (pT; finite rate theta), chemical (pTg; finite rate nu)
-enum { CL_P = 0, CL_PT = 2, CL_THETA = 3, CL_NU = 4, CL_PTG = 5 };
+enum { CL_P = 0, CL_MU = 1, CL_PT = 2, CL_THETA = 3, CL_NU = 4, CL_PTG = 5 };
 // nodelta (PHY:234): the p closure without the energy defect, for the p-pTg activation's prediction only; tau: the
 // stage's implicit step a_ii dt (the finite-rate closures)
-struct ClIn { double atr[2], mtr[2], etr[2], s, rhoeC, pguess, Qph[2]; int type = CL_P; bool nodelta = false; double tau = 0.0; };
+struct ClIn { double atr[2], mtr[2], etr[2], s, rhoeC, pguess, Qph[2]; int type = CL_P; bool nodelta = false; double tau = 0.0;
+               double Z[2] = {1.0, 1.0}; };   // Z: the phases' frozen acoustic impedances (finite mu, PC:956-958)
 struct ClOut { double a[2], m[2], e[2], p[2], T[2], pbar; };
 struct PCoef { double A[2], B[2], N[2], lam[2]; };
 static double E_above(const Model& M, const ClIn& in, int k) {                                     // PHY:420-423
@@ -849,6 +851,59 @@ static bool closure_p_mg(const Model& M, const ClIn& in, ClOut& o) {
 static bool closure_p(const Model& M, const ClIn& in, ClOut& o) {
 	if (M.mg_any()) return closure_p_mg(M, in, o);                                                  // PHY:463, PHO:473
 	return M.closure_path == 0 ? closure_p_phy(M, in, o) : closure_p_pho(M, in, o); }
+// ---- finite mu (B6.2, PHY:575-656), backward-Euler form (RELAX_FORM 0): for a volume transfer R the phase pressures
+// (p1, p2) solve a 2x2 linear system (each phase's energy with its own pressure work, the defect share on the guess's
+// segment, the impedance weights w_k = Z_other / (Z0 + Z1) splitting the transfer's work); R is the root of
+// F(R) = R - tau mu (p1 - p2)(R), which the relaxation keeps between 0 and tau mu dp(0): bracketed from F(0) by steps of
+// the first estimate (an inadmissible trial halves back towards the last good point), then Brent's method.
+static inline int mech_type(const Model& M) { return std::isinf(M.mu_rate) ? CL_P : CL_MU; }      // PHY:815
+static bool closure_mu(const Model& M, const ClIn& in, ClOut& o) {
+	const double sa = in.atr[0] + in.atr[1];
+	const double om[2] = {in.atr[0] / sa, in.atr[1] / sa}, ak[2] = {in.atr[0] / sa, in.atr[1] / sa};
+	const double D0 = in.rhoeC - in.etr[0] - in.etr[1];
+	double lam[2], ex[2]; defect_split(M, in, D0 + in.pguess * in.s, lam, ex);
+	const double w1 = in.Z[0] + in.Z[1] > 0 ? in.Z[1] / (in.Z[0] + in.Z[1]) : 0.5, w2 = 1.0 - w1;
+	const double tm = std::isinf(M.mu_rate) ? INF : in.tau * M.mu_rate;
+	const Phase &A = M.PH[0], &B = M.PH[1];
+	auto solve = [&](double R, double& p1, double& p2) -> bool {
+		const double a1 = ak[0] + R, a2 = ak[1] - R;
+		const double C1 = dedp_am(M, 0, a1, in.mtr[0]) + om[0] * in.s, C2 = dedp_am(M, 1, a2, in.mtr[1]) + om[1] * in.s;
+		const double M11 = C1 + R * w1 - lam[0] * in.s * om[0], M12 = R * w2 - lam[0] * in.s * om[1];
+		const double M21 = -R * w1 - lam[1] * in.s * om[0], M22 = C2 - R * w2 - lam[1] * in.s * om[1];
+		const double r1 = in.etr[0] - in.mtr[0] * A.q - A.g * A.pi * dedp_am(M, 0, a1, in.mtr[0]) + lam[0] * D0 + ex[0];
+		const double r2 = in.etr[1] - in.mtr[1] * B.q - B.g * B.pi * dedp_am(M, 1, a2, in.mtr[1]) + lam[1] * D0 + ex[1];
+		const double det = M11 * M22 - M12 * M21; if (!(std::fabs(det) > 0)) return false;
+		p1 = (r1 * M22 - M12 * r2) / det; p2 = (M11 * r2 - M21 * r1) / det; return std::isfinite(p1) && std::isfinite(p2);
+	};
+	double R = 0.0, p1, p2;
+	if (tm > 0) {
+		auto Fa = [&](double r, double& f) -> bool {
+			double q1, q2; if (!solve(r, q1, q2)) return false;
+			const double a1 = ak[0] + r, a2 = ak[1] - r;
+			if (!(a1 > in.mtr[0] * A.b && a2 > in.mtr[1] * B.b)) return false;
+			if (!(q1 + A.pi > 0 && q2 + B.pi > 0)) return false;
+			f = r - tm * (q1 - q2); return std::isfinite(f); };
+		double f0; if (!Fa(0.0, f0)) return false;
+		if (f0 != 0.0) {
+			double hi = -f0, lo = 0.0, flo = f0, fhi = NAN; bool ok = false;
+			for (int it = 0; it < 200; it++) {
+				if (Fa(hi, fhi)) { if (fhi * flo < 0) { ok = true; break; } lo = hi; flo = fhi; hi = lo + (lo == 0.0 ? -f0 : lo); }
+				else hi = 0.5 * (lo + hi);
+			}
+			if (!ok) return false;
+			auto F = [&](double r) { double f; return Fa(r, f) ? f : (double)NAN; };
+			if (!brent(F, smin(lo, hi), smax(lo, hi), lo < hi ? flo : fhi, lo < hi ? fhi : flo, 1e-16 * (std::fabs(lo) + std::fabs(hi)) + 1e-300, R)) return false;
+		}
+	}
+	if (!solve(R, p1, p2)) return false;
+	const double pp[2] = {p1, p2};
+	for (int k = 0; k < 2; k++) {
+		o.a[k] = ak[k] + (k == 0 ? R : -R); o.m[k] = in.mtr[k]; o.p[k] = pp[k];
+		if (!(o.a[k] > o.m[k] * M.PH[k].b) || !(pp[k] + M.PH[k].pi > 0)) return false;
+		o.e[k] = eps_amp(M, k, o.a[k], o.m[k], pp[k]); o.T[k] = T_amp(M, k, o.a[k], o.m[k], pp[k]);
+	}
+	o.pbar = o.a[0] * p1 + o.a[1] * p2; return true;
+}
 // ---- finite relaxation rates, backward-Euler form (RELAX_FORM 0, the reference's default)
 // finite theta (B6.4, PHY:679-700): at fixed p the phases' temperatures are linear in the exchanged heat, and
 // dQ = tau theta (T2 - T1) / (1 + tau theta S), S = 1/Cp1 + 1/Cp2, shifts the p closure's volume fractions; the
@@ -908,6 +963,7 @@ static bool closure(const Model& M, const ClIn& in, ClOut& o) {                 
 	if (!std::isfinite(in.rhoeC) || !std::isfinite(in.s)) return false;
 	bool ok;
 	switch (in.type) {
+	case CL_MU: ok = closure_mu(M, in, o); break;
 	case CL_PT: ok = closure_pT(M, in, o); break;
 	case CL_THETA: ok = closure_theta(M, in, o); break;
 	case CL_NU: ok = closure_nu(M, in, o); break;
@@ -981,6 +1037,7 @@ struct Cfg {
 	double theta_rate, nu_rate;   // theta_rate, nu_rate (PHY:33); < 0: the level's default (PC:1148-1149)
 	int eos_kind[2];              // Phase::kind (PHY:50-53): 0 NASG / SG, 1 Cochran-Chan, 2 JWL
 	double mg[2][7];              // a Mie-Grueneisen phase's G, c1, c2, e1, e2, rr, e0
+	double mu_rate;               // mu_rate (PHY:33); <= 0: instantaneous (inf, the default)
 };
 static Phase phase_of(const double* ph, int kind, const double* mg) {
 	Phase P{ph[0], ph[1], ph[2], ph[3], ph[4], ph[5]};
@@ -1006,6 +1063,7 @@ struct Ctx {
 	// stage data (PC:565-582)
 	double tau, rx, ry;
 	std::vector<double> ust, vst, ubsx, kpx, wLx, pvx, ubsy, kpy, wLy, pvy, Kw, Qph, cz, Pold;
+	std::vector<double> Z;   // the phases' acoustic impedances of the stage's finite-mu cells (PC:956-958; 1 elsewhere)
 	// pressure
 	std::vector<double> P, Pst; double P_REF = 0.0;
 	double t = 0.0, dtprev = -1.0; bool startup = true;
@@ -1024,6 +1082,7 @@ static void init_ctx(Ctx& X, const Cfg& C) {
 	X.M.shock_fused = C.shock_fused;
 	X.M.theta_rate = C.theta_rate < 0 ? (C.relax >= 1 ? INF : 0.0) : C.theta_rate;                 // PC:1148-1149
 	X.M.nu_rate = C.nu_rate < 0 ? ((C.relax == 2 || C.relax == 3) ? INF : 0.0) : C.nu_rate;
+	X.M.mu_rate = C.mu_rate > 0 ? C.mu_rate : INF;
 	if (C.relax >= 2) { build_sat(X.M, X.SAT); X.M.sat = &X.SAT; }
 	X.nx = C.nx; X.ny = C.ny; X.NXT = C.nx + 2 * GH; X.NYT = C.ny + 2 * GH;
 	X.dx = (C.x1 - C.x0) / C.nx; X.dy = (C.y1 - C.y0) / C.ny; X.twoD = C.ny > 1;
@@ -1031,7 +1090,7 @@ static void init_ctx(Ctx& X, const Cfg& C) {
 	for (auto* g : {&X.U, &X.Un, &X.Qe, &X.Qs, &X.Ust, &X.wxL, &X.wxR, &X.wyL, &X.wyR, &X.H[0], &X.H[1], &X.H[2]}) g->assign(nc * NV, 0.0);
 	for (auto* a : {&X.phx, &X.phy, &X.a2s, &X.ubx, &X.uby, &X.ust, &X.vst, &X.ubsx, &X.kpx, &X.pvx, &X.ubsy, &X.kpy, &X.pvy, &X.Kw, &X.cz, &X.Pold,
 	                &X.eTV1, &X.eTV2, &X.ETV, &X.dA}) a->assign(nc, 0.0);
-	X.wLx.assign(nc, 0.5); X.wLy.assign(nc, 0.5); X.Qph.assign(nc * 2, 0.0);
+	X.wLx.assign(nc, 0.5); X.wLy.assign(nc, 0.5); X.Qph.assign(nc * 2, 0.0); X.Z.assign(nc * 2, 1.0);
 	X.P.assign((size_t)C.nx * C.ny, 0.0); X.Pst = X.P;
 }
 
@@ -1335,6 +1394,18 @@ static bool stage_prep(Ctx& X) {                                                
 		}
 		X.Kw[c] = K + PE;
 		X.Pold[c] = X.Pst[M_P(X, i, j)];   // PC:899-900: the stage pressure (Newton guess)
+		// the finite-mu cells' frozen impedances Z_k = rho_k c_k of the stage's content per unit volume (PC:956-958)
+		X.Z[2 * c] = X.Z[2 * c + 1] = 1.0;
+		if (i >= GH && i < X.nx + GH && j >= GH && j < X.ny + GH && sa > 0) {
+			const int ty = X.ctype.empty() ? mech_type(M) : X.ctype[(size_t)(i - GH) + (size_t)(j - GH) * X.nx];
+			if (ty == CL_MU) {
+				const double cc[6] = {a1 / sa, a2 / sa, m1 / sa, m2 / sa, XS(i, j, IE1) / sa, XS(i, j, IE2) / sa};
+				for (int k = 0; k < 2; k++) {
+					const double a = cc[k], m = cc[2 + k], pk = p_ame(M, k, a, m, cc[4 + k]), r = m / a;
+					X.Z[2 * c + k] = r * std::sqrt(smax(c2_rp(M, k, r, pk), 1e-30));
+				}
+			}
+		}
 	}
 	const double tau = X.tau, dx = X.dx, dy = X.dy;
 	auto rhoS = [&](int i, int j) { return XS(i, j, IM1) + XS(i, j, IM2); };
@@ -1489,8 +1560,9 @@ static bool cell_residual(const Ctx& X, int i, int j, double Pc, double PW, doub
 	if (!(rho > 0)) return false;
 	ClIn in; in.atr[0] = Xv[0]; in.atr[1] = Xv[1]; in.mtr[0] = Xv[2]; in.mtr[1] = Xv[3]; in.etr[0] = Xv[4]; in.etr[1] = Xv[5];
 	in.s = s; in.rhoeC = E - 0.5 * (mx * mx + my * my) / rho; in.pguess = X.P_REF + Pc; in.Qph[0] = X.Qph[c * 2]; in.Qph[1] = X.Qph[c * 2 + 1];
-	in.type = X.ctype.empty() ? CL_P : X.ctype[(size_t)(i - GH) + (size_t)(j - GH) * X.nx];   // S.type (PC:951)
+	in.type = X.ctype.empty() ? mech_type(M) : X.ctype[(size_t)(i - GH) + (size_t)(j - GH) * X.nx];   // S.type (PC:951)
 	in.tau = X.tau;
+	if (in.type == CL_MU) { in.Z[0] = X.Z[2 * c]; in.Z[1] = X.Z[2 * c + 1]; }
 	ClOut o; if (!closure(M, in, o)) return false;
 	G = (o.pbar - X.P_REF) - Pc;
 	if (out) { out->cl = o; out->mx = mx; out->my = my; out->E = E; out->s = s; out->sig = closure_roundoff(M, in, o.pbar); }
@@ -1703,14 +1775,15 @@ static void init_pressure(Ctx& X) {                                             
 // thermal. Level 3 (p-pTg): chemical where either phase's T at the mechanical equilibrium (without the energy defect)
 // exceeds T_sat(p), else mechanical. The rates are instantaneous (mu = theta = nu = inf, the reference's defaults).
 static int closure_type_of(const Model& M, int level, double eps_i, double a1, double m1, double m2, double e1, double e2, double rhoe) {   // PHY:814-830
-	if (level == 0) return CL_P;
-	if (!(a1 >= eps_i && a1 <= 1.0 - eps_i)) return CL_P;
+	const int mech = mech_type(M);
+	if (level == 0) return mech;
+	if (!(a1 >= eps_i && a1 <= 1.0 - eps_i)) return mech;
 	ClIn in{}; in.atr[0] = a1; in.atr[1] = 1 - a1; in.mtr[0] = m1; in.mtr[1] = m2; in.etr[0] = e1; in.etr[1] = e2; in.s = 0; in.rhoeC = rhoe;
 	in.type = CL_PT; in.pguess = smax(p_mix(M, a1, m1, m2, rhoe), 1.0);
 	const int chem = std::isinf(M.nu_rate) ? CL_PTG : CL_NU, therm = std::isinf(M.theta_rate) ? CL_PT : CL_THETA;
 	if (level == 3) {
-		in.type = CL_P; in.nodelta = true; ClOut o; if (!closure_p(M, in, o)) return CL_P;
-		const double Ts = Tsat(M, o.p[0]); if (std::isfinite(Ts) && (o.T[0] > Ts || o.T[1] > Ts)) return chem; return CL_P;
+		in.type = CL_P; in.nodelta = true; ClOut o; if (!closure_p(M, in, o)) return mech;
+		const double Ts = Tsat(M, o.p[0]); if (std::isfinite(Ts) && (o.T[0] > Ts || o.T[1] > Ts)) return chem; return mech;
 	}
 	if (level == 2) {
 		ClOut o; if (closure_pT(M, in, o)) { const double Ts = Tsat(M, o.p[0]); if (std::isfinite(Ts) && o.T[0] > Ts) return chem; }
@@ -1730,13 +1803,13 @@ static bool project_cell(const Model& M, double* Q, int type) {                 
 }
 static void relaxation_activation(Ctx& X, std::vector<double>& U) {                                // PC:537-560
 	const int nx = X.nx, n = nx * X.ny;
-	X.ctype.assign((size_t)n, CL_P);
+	X.ctype.assign((size_t)n, mech_type(X.M));
 	#pragma omp parallel for schedule(static)
 	for (int q = 0; q < n; q++) {
 		double* Q = &U[NV * X.id(q % nx + GH, q / nx + GH)];
 		const double rho = Q[IM1] + Q[IM2], rhoe = Q[IEN] - 0.5 * (Q[IMU] * Q[IMU] + Q[IMV] * Q[IMV]) / rho;   // cell_type_U (PC:517-521)
 		int ty = closure_type_of(X.M, X.C.relax, X.C.eps_i, Q[IA1], Q[IM1], Q[IM2], Q[IE1], Q[IE2], rhoe);
-		if (!project_cell(X.M, Q, ty)) ty = CL_P;
+		if (!project_cell(X.M, Q, ty)) ty = mech_type(X.M);   // PC:544
 		X.ctype[(size_t)q] = ty;
 	}
 	bc_apply(X, U);
@@ -2130,7 +2203,7 @@ int ora_set_threads(int n) { if (n > 0) omp_set_num_threads(n); return omp_get_m
 // NULL for a Mie-Grueneisen phase with a relaxation level above 0 (the reference fails every non-mechanical closure of
 // such a phase, PHY:790), an IMEX variant or the fused profile form (both written for NASG phases only)
 void* ora_create(const Cfg* C) {
-	if ((C->eos_kind[0] || C->eos_kind[1]) && (C->relax || C->variant || C->shock_fused)) return nullptr;
+	if ((C->eos_kind[0] || C->eos_kind[1]) && (C->relax || C->variant || C->shock_fused || C->mu_rate > 0)) return nullptr;
 	Ctx* X = new Ctx(); init_ctx(*X, *C); return X; }
 void ora_destroy(void* h) { delete (Ctx*)h; }
 size_t ora_cfg_size(void) { return sizeof(Cfg); }
@@ -2292,6 +2365,18 @@ double ora_closure_phi(const OraModel* m, const double* in, double p) {
 	PCoef q; p_coef(M, c, q, p); return alpha_p(M, q, c, 0, p) + alpha_p(M, q, c, 1, p) - 1.0;
 }
 int ora_shock_share(const OraModel* m, const double* Y, const double* vpre, double p0, double p1, double* lam, double* q) { return shock_share(mk_model(m), Y, vpre, p0, p1, lam, *q) ? 1 : 0; }
+// the closure of a given type at finite rates for the tests: in as ora_closure; type (CL_P 0, CL_MU 1, ...), tau, the
+// impedances Z0, Z1 and mu_rate (<= 0: inf); out: a0 a1 m0 m1 e0 e1 pbar p0 p1
+int ora_closure_typed(const OraModel* m, const double* in, int type, double tau, double Z0, double Z1, double mu, double* out) {
+	Model M = mk_model(m); M.mu_rate = mu > 0 ? mu : INF;
+	ClIn c; c.atr[0] = in[0]; c.atr[1] = in[1]; c.mtr[0] = in[2]; c.mtr[1] = in[3]; c.etr[0] = in[4]; c.etr[1] = in[5];
+	c.s = in[6]; c.rhoeC = in[7]; c.pguess = in[8]; c.Qph[0] = in[9]; c.Qph[1] = in[10];
+	c.type = type; c.tau = tau; c.Z[0] = Z0; c.Z[1] = Z1;
+	ClOut o; if (!closure(M, c, o)) return 0;
+	out[0] = o.a[0]; out[1] = o.a[1]; out[2] = o.m[0]; out[3] = o.m[1]; out[4] = o.e[0]; out[5] = o.e[1]; out[6] = o.pbar;
+	out[7] = o.p[0]; out[8] = o.p[1];
+	return 1;
+}
 double ora_hug_v(const OraModel* m, int k, double p0, double p1, double v0) { return hug_v(mk_model(m), k, p0, p1, v0); }
 int ora_prof_integrate(const OraModel* m, const double* Y, const double* vpre, double p0, double p1, double c, double* v1end, double* qend) {
 	const Model M = mk_model(m); ShockProf S; S.p0 = p0; double vH[2];
