@@ -117,7 +117,8 @@ struct spb6_ctx {
 	double *eTV1 = nullptr, *eTV2 = nullptr, *ETV = nullptr, *dA = nullptr, *donax = nullptr, *donay = nullptr;
 	int* ctype = nullptr;   // relaxation levels 1-3: the closure type of each interior cell for the step (STEP_TYPE)
 	double eps_i = 1e-4;    // EPS_I (PHY:38)
-	RelaxDev rel{nullptr, nullptr, 0, 0.0, 0.0, 0.0, 0.0};   // the saturation table on the device (levels 2-3) and the rates
+	RelaxDev rel{nullptr, nullptr, 0, 0.0, 0.0, 0.0, 0.0, INFINITY};   // the saturation table on the device (levels 2-3) and the rates
+	double* Z = nullptr;    // finite mu: the stage's impedances of the CL_MU cells (2 NC)
 	// graph mode (cfg.graph): a stage's fixed head (combination, explicit part, preparation, shock budget) and each
 	// Krylov chunk (its BiCGStab passes, the FP32 form's defect, sweep close and trial residual) run as captured graphs,
 	// re-captured per launch and applied to the instantiated graph of the same shape with hipGraphExecUpdate
@@ -161,6 +162,7 @@ Dev make_dev(const spb6_ctx* c) {
 	if (D.variant != VAR_PC) { D.wxL = c->fwxL; D.wxR = c->fwxR; D.wyL = c->fwyL; D.wyR = c->fwyR; }
 	D.ctype = c->ctype;
 	D.rel = c->rel;
+	D.Z = c->Z;
 	return D;
 }
 // the saturation table T_sat(ln p) (PHY:125-152), built on the host with the device's portable exp, log and pow: 1600
@@ -818,6 +820,9 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 		if (cfg->eos_kind[k] < 0 || cfg->eos_kind[k] > 2) { g_err = "eos_kind must be 0 (NASG), 1 (Cochran-Chan) or 2 (JWL)"; return nullptr; }
 		if (cfg->eos_kind[k] && !(cfg->mg[k][0] > 0 && cfg->mg[k][5] > 0)) { g_err = "a Mie-Grueneisen phase needs G > 0 and rr > 0"; return nullptr; }
 	}
+	if (cfg->mu_rate > 0 && (cfg->analytic_jacobian || cfg->learned_guess || cfg->variant != SPB6_VARIANT_PC || cfg->eos_kind[0] ||
+	                         cfg->eos_kind[1])) {
+		g_err = "a finite mu runs with the reference scheme, the finite-difference Jacobian, the plain guess and NASG phases"; return nullptr; }
 	if ((cfg->eos_kind[0] || cfg->eos_kind[1]) && !SPB6_MG) {
 		g_err = "Mie-Grueneisen phases run in libspb6_hip_mg.so (the same sources built with SPB6_MG=1)"; return nullptr; }
 	if ((cfg->eos_kind[0] || cfg->eos_kind[1]) && (cfg->relax || cfg->variant != SPB6_VARIANT_PC || cfg->shock_fused ||
@@ -851,12 +856,17 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 		for (auto p : va) if (dalloc(c, p, NC)) return fail("alloc");
 		if (dalloc(c, &c->donax, 2 * NC) || dalloc(c, &c->donay, 2 * NC)) return fail("alloc");
 	}
-	if (cfg->relax > 0) {
+	c->rel.mu = cfg->mu_rate > 0 ? cfg->mu_rate : INFINITY;   // PHY:33 (inf: instantaneous)
+	if (cfg->relax > 0 || cfg->mu_rate > 0) {
+		// the cells' closure types: at level 0 with a finite mu every cell is CL_MU (PHY:815), so the stage takes the
+		// general per-cell-type path; each attempt's activation rewrites them
 		if (hipMalloc((void**)&c->ctype, n * sizeof(int)) != hipSuccess) return fail("alloc");
 		c->allocs.push_back(c->ctype);
-		if (hipMemset(c->ctype, 0, n * sizeof(int)) != hipSuccess) return fail("alloc");
+		std::vector<int> t0(n, cfg->mu_rate > 0 ? CL_MU : CL_P);
+		if (hipMemcpy(c->ctype, t0.data(), n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return fail("upload");
 		if (cfg->eps_i > 0) c->eps_i = cfg->eps_i;
 	}
+	if (cfg->mu_rate > 0 && dalloc(c, &c->Z, 2 * NC)) return fail("alloc");
 	c->rel.theta = cfg->theta_rate > 0 ? cfg->theta_rate : (cfg->relax >= 1 ? INFINITY : 0.0);                 // PC:1148-1149
 	c->rel.nu = cfg->nu_rate > 0 ? cfg->nu_rate : ((cfg->relax == 2 || cfg->relax == 3) ? INFINITY : 0.0);
 	if (cfg->relax >= 2) {   // the saturation line before the run (PC:1151)

@@ -488,7 +488,7 @@ spb6_batch* spb6_batch_create(const spb6_batch_problem_t* probs, int nprob, cons
 	// the batch runs the reference scheme (PC) with full Newton at relaxation level 0 and the reference's profile
 	// arithmetic: no jreuse, variant, relaxation, fused shock integration or Mie-Grueneisen phase
 	if (cfg->rus_local != 1 || cfg->rk < 1 || cfg->rk > 2 || cfg->jreuse || cfg->variant || cfg->relax || cfg->shock_fused ||
-	    cfg->eos_kind[0] || cfg->eos_kind[1]) return nullptr;
+	    cfg->eos_kind[0] || cfg->eos_kind[1] || cfg->mu_rate > 0) return nullptr;
 	spb6_batch* b = new spb6_batch();
 	b->device = device; b->cfg = *cfg;
 	auto fail = [&]() -> spb6_batch* { spb6_batch_destroy(b); return nullptr; };

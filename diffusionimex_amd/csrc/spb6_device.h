@@ -785,7 +785,7 @@ SPB_HD void stencil_state(const Phys& P, const double* c, double* st) {
 
 // ------------------------------------------------------------------ p closure (PHY:420-494, PHO:472-548)
 // tau: the stage's implicit step a_ii dt (read by the finite-rate closures only)
-struct ClIn { double atr0, atr1, mtr0, mtr1, etr0, etr1, s, rhoeC, pguess, Q0, Q1; double tau = 0.0; };
+struct ClIn { double atr0, atr1, mtr0, mtr1, etr0, etr1, s, rhoeC, pguess, Q0, Q1; double tau = 0.0; double Z0 = 1.0, Z1 = 1.0; };
 // the closure's phase contents: volume fractions, masses (the transported ones but where the pTg closure moves mass)
 // and energies, and the pressure
 struct ClOut { double a0, a1, m0, m1, e0, e1, pbar; };
@@ -1053,7 +1053,7 @@ SPB_HD double closure_roundoff(const Phys& P, const ClIn& in, double p) {
 
 // ------------------------------------------------------------------ relaxation level 1: the pT closure (PHY:183-230, PHY:660-672)
 // closure types (PHY:226): mechanical (p), thermal (pT; finite rate theta) and chemical (pTg; finite rate nu) equilibrium
-enum { CL_P = 0, CL_PT = 2, CL_THETA = 3, CL_NU = 4, CL_PTG = 5 };
+enum { CL_P = 0, CL_MU = 1, CL_PT = 2, CL_THETA = 3, CL_NU = 4, CL_PTG = 5 };
 // Brent's method on a sign-changing bracket (PHY:183-204), operation for operation
 template <class F>
 SPB_HD bool brent_root(F f, double a, double b, double fa, double fb, double tol, double& root) {
@@ -1146,7 +1146,7 @@ SPB_HD bool closure_pT(const Phys& P, const ClIn& in, ClOut& o, double* T_out = 
 // the saturation table (built on the host by the same arithmetic, PHY:125-152): n points of ln p on a uniform grid and
 // T_sat there, and the critical point (pcrit, vcrit); and the thermal and chemical relaxation rates theta, nu (PHY:33,
 // resolved by level: inf = instantaneous)
-struct RelaxDev { const double* lp; const double* T; int n; double pcrit, vcrit, theta, nu; };
+struct RelaxDev { const double* lp; const double* T; int n; double pcrit, vcrit, theta, nu, mu; };
 SPB_HD double s_pT(const Phys& P, int k, double p, double T) {                      // PHY:80
 	return P.cv[k] * (P.g[k] * plog(T) - (P.g[k] - 1.0) * plog(p + P.pi[k])) + P.qp[k]; }
 SPB_HD double g_pT(const Phys& P, int k, double p, double T) {                      // PHY:81
@@ -1296,8 +1296,68 @@ SPB_HD bool closure_nu(const Phys& P, double nu, const ClIn& in, ClOut& o) {
 	return isfin(o.pbar);
 }
 // the closure of type ty (CL_P, CL_PT, CL_THETA, CL_NU, CL_PTG; PHY:786-810)
+// finite mu (B6.2, PHY:575-656), backward-Euler form: for a volume transfer R the phase pressures (p1, p2) solve a 2x2
+// linear system (each phase's energy with its own pressure work, the defect share on the guess's segment, the frozen
+// impedance weights w_k splitting the transfer's work); R is the root of F(R) = R - tau mu (p1 - p2)(R), bracketed from
+// F(0) by steps of the first estimate (an inadmissible trial halves back towards the last good point), then Brent.
+// pbar = alpha_1 p1 + alpha_2 p2.
+SPB_HD bool closure_mu(const Phys& P, double mu, const ClIn& in, ClOut& o) {
+	if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0) || !(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
+	if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
+	const double sa = in.atr0 + in.atr1;
+	const double om0 = in.atr0 / sa, om1 = in.atr1 / sa, ak0 = in.atr0 / sa, ak1 = in.atr1 / sa;
+	const double D0 = in.rhoeC - in.etr0 - in.etr1, sm = in.mtr0 + in.mtr1;
+	double lE0, lE1;
+	{ const double E0 = E_above(P, 0, in.atr0, in.mtr0, in.etr0), E1 = E_above(P, 1, in.atr1, in.mtr1, in.etr1);
+	  double sE = 0.0; sE += E0; sE += E1;
+	  if (sE > 0) { lE0 = E0 / sE; lE1 = E1 / sE; }
+	  else { lE0 = sm > 0 ? in.mtr0 / sm : 0.5; lE1 = sm > 0 ? in.mtr1 / sm : 0.5; } }
+	double lam0, lam1, ex0, ex1;
+	defect_seg(D0 + in.pguess * in.s, in.Q0 + in.Q1, in.Q0, in.Q1, lE0, lE1, lam0, lam1, ex0, ex1);
+	const double w1 = in.Z0 + in.Z1 > 0 ? in.Z1 / (in.Z0 + in.Z1) : 0.5, w2 = 1.0 - w1;
+	const double tm = __builtin_isinf(mu) ? dinf() : in.tau * mu;
+	auto solve = [&](double R, double& p1, double& p2) -> bool {
+		const double a1 = ak0 + R, a2 = ak1 - R;
+		const double C1 = dedp_am(P, 0, a1, in.mtr0) + om0 * in.s, C2 = dedp_am(P, 1, a2, in.mtr1) + om1 * in.s;
+		const double M11 = C1 + R * w1 - lam0 * in.s * om0, M12 = R * w2 - lam0 * in.s * om1;
+		const double M21 = -R * w1 - lam1 * in.s * om0, M22 = C2 - R * w2 - lam1 * in.s * om1;
+		const double r1 = in.etr0 - in.mtr0 * P.q[0] - P.g[0] * P.pi[0] * dedp_am(P, 0, a1, in.mtr0) + lam0 * D0 + ex0;
+		const double r2 = in.etr1 - in.mtr1 * P.q[1] - P.g[1] * P.pi[1] * dedp_am(P, 1, a2, in.mtr1) + lam1 * D0 + ex1;
+		const double det = M11 * M22 - M12 * M21; if (!(fabs(det) > 0)) return false;
+		p1 = (r1 * M22 - M12 * r2) / det; p2 = (M11 * r2 - M21 * r1) / det; return isfin(p1) && isfin(p2);
+	};
+	double R = 0.0, p1, p2;
+	if (tm > 0) {
+		auto Fa = [&](double r, double& f) -> bool {
+			double q1, q2; if (!solve(r, q1, q2)) return false;
+			const double a1 = ak0 + r, a2 = ak1 - r;
+			if (!(a1 > in.mtr0 * P.b[0] && a2 > in.mtr1 * P.b[1])) return false;
+			if (!(q1 + P.pi[0] > 0 && q2 + P.pi[1] > 0)) return false;
+			f = r - tm * (q1 - q2); return isfin(f); };
+		double f0; if (!Fa(0.0, f0)) return false;
+		if (f0 != 0.0) {
+			double hi = -f0, lo = 0.0, flo = f0, fhi = dnan(); bool ok = false;
+			for (int it = 0; it < 200; it++) {
+				if (Fa(hi, fhi)) { if (fhi * flo < 0) { ok = true; break; } lo = hi; flo = fhi; hi = lo + (lo == 0.0 ? -f0 : lo); }
+				else hi = 0.5 * (lo + hi);
+			}
+			if (!ok) return false;
+			auto F = [&](double r) { double f; return Fa(r, f) ? f : dnan(); };
+			if (!brent_root(F, smin(lo, hi), smax(lo, hi), lo < hi ? flo : fhi, lo < hi ? fhi : flo, 1e-16 * (fabs(lo) + fabs(hi)) + 1e-300, R)) return false;
+		}
+	}
+	if (!solve(R, p1, p2)) return false;
+	o.a0 = ak0 + R; o.a1 = ak1 - R; o.m0 = in.mtr0; o.m1 = in.mtr1;
+	if (!(o.a0 > o.m0 * P.b[0]) || !(p1 + P.pi[0] > 0)) return false;
+	o.e0 = eps_amp(P, 0, o.a0, o.m0, p1);
+	if (!(o.a1 > o.m1 * P.b[1]) || !(p2 + P.pi[1] > 0)) return false;
+	o.e1 = eps_amp(P, 1, o.a1, o.m1, p2);
+	o.pbar = o.a0 * p1 + o.a1 * p2;
+	return isfin(o.pbar);
+}
 SPB_HD bool closure_of(const Phys& P, const RelaxDev* S, int ty, const ClIn& in, ClOut& o) {
 	switch (ty) {
+	case CL_MU: return closure_mu(P, S->mu, in, o);
 	case CL_PT: return closure_pT(P, in, o);
 	case CL_THETA: return closure_theta(P, S->theta, in, o);
 	case CL_NU: return closure_nu(P, S->nu, in, o);
@@ -1310,16 +1370,17 @@ SPB_HD bool closure_of(const Phys& P, const RelaxDev* S, int ty, const ClIn& in,
 // exceeds T_sat(p), else thermal; level 3 chemical where a phase's T at the mechanical equilibrium without the energy
 // defect exceeds T_sat(p), else mechanical
 SPB_HD int closure_type_of(const Phys& P, const RelaxDev* S, int level, double eps_i, double a1, double m1, double m2, double e1, double e2, double rhoe) {
-	if (level == 0) return CL_P;
-	if (!(a1 >= eps_i && a1 <= 1.0 - eps_i)) return CL_P;
+	const int mech = __builtin_isinf(S->mu) ? CL_P : CL_MU;   // PHY:815
+	if (level == 0) return mech;
+	if (!(a1 >= eps_i && a1 <= 1.0 - eps_i)) return mech;
 	ClIn in; in.atr0 = a1; in.atr1 = 1 - a1; in.mtr0 = m1; in.mtr1 = m2; in.etr0 = e1; in.etr1 = e2; in.s = 0; in.rhoeC = rhoe;
 	in.Q0 = 0.0; in.Q1 = 0.0; in.pguess = smax(p_mix(P, a1, m1, m2, rhoe), 1.0);
 	const int chem = __builtin_isinf(S->nu) ? CL_PTG : CL_NU, therm = __builtin_isinf(S->theta) ? CL_PT : CL_THETA;
 	if (level == 3) {
-		ClOut o; if (!closure_p<true>(P, in, o)) return CL_P;
+		ClOut o; if (!closure_p<true>(P, in, o)) return mech;
 		const double Ts = Tsat(P, *S, o.pbar);
 		const double T0 = T_amp(P, 0, o.a0, o.m0, o.pbar), T1 = T_amp(P, 1, o.a1, o.m1, o.pbar);
-		return (isfin(Ts) && (T0 > Ts || T1 > Ts)) ? chem : CL_P;
+		return (isfin(Ts) && (T0 > Ts || T1 > Ts)) ? chem : mech;
 	}
 	if (level == 2) {
 		ClOut o; double T;

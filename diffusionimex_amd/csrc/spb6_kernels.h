@@ -40,6 +40,7 @@ struct Dev {
 	double *eTV1, *eTV2, *ETV, *dA, *donax, *donay;
 	const int* ctype;   // relaxation levels 1-3: each interior row's closure type of the step (CL_P / CL_PT / CL_PTG); null at level 0
 	RelaxDev rel;         // the saturation table (relaxation levels 2-3; n = 0 otherwise) and the finite rates
+	double* Z;            // finite mu: the phases' acoustic impedances of the stage's CL_MU cells (2 NC; null otherwise)
 };
 // the IMEX variants of the reference (its sibling programs SPB6_<name>_IMEX.cpp); VAR_PC is SPB6_PC_IMEX.cpp
 enum { VAR_PC = 0, VAR_TV = 1, VAR_TVPURE = 2, VAR_TVINC = 3, VAR_TVSPLIT = 4, VAR_TVBB = 5, VAR_BB = 6 };

@@ -1254,8 +1254,31 @@ void launch_recon_full(const Dev& D, int limiter, int order, const double* Qe, d
 	hipLaunchKernelGGL(k_recon, dim3(nblk(g.NC)), dim3(256), 0, st, g, D.ph, limiter, order, Qe, (const double*)D.vsc, (const double*)D.esc,
 	                   wxL, wxR, wyL, wyR, D.phx, D.phy);
 }
+// finite mu: the frozen acoustic impedances Z_k = rho_k c_k of each CL_MU cell's stage content per unit volume
+// (PC:956-958), 1 elsewhere
+__global__ void __launch_bounds__(256) k_mu_impedance(Geo g, Phys P, const double* __restrict__ Xs, const double* __restrict__ a2s,
+                                                      const int* __restrict__ ctype, double* __restrict__ Z) {
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= g.NC) return;
+	const int i = c % g.NXT, j = c / g.NXT, NC = g.NC;
+	double z0 = 1.0, z1 = 1.0;
+	if (i >= GH && i < g.nx + GH && j >= GH && j < g.ny + GH && ctype[(i - GH) + (j - GH) * g.nx] == CL_MU) {
+		const double a1 = Xs[IA1 * NC + c], a2 = a2s[c], sa = a1 + a2;
+		if (sa > 0) {
+			const double cc[6] = {a1 / sa, a2 / sa, Xs[IM1 * NC + c] / sa, Xs[IM2 * NC + c] / sa, Xs[IE1 * NC + c] / sa, Xs[IE2 * NC + c] / sa};
+			double zz[2];
+			for (int k = 0; k < 2; k++) {
+				const double a = cc[k], m = cc[2 + k], pk = p_ame(P, k, a, m, cc[4 + k]), r = m / a;
+				zz[k] = r * sqrt(smax(c2_rp(P, k, r, pk), 1e-30));
+			}
+			z0 = zz[0]; z1 = zz[1];
+		}
+	}
+	Z[c] = z0; Z[NC + c] = z1;
+}
 void launch_prep(const Dev& D, int limiter, int order, double tau, const double* Pst, hipStream_t st) {
 	const Geo& g = D.g;
+	if (D.Z) hipLaunchKernelGGL(k_mu_impedance, dim3(nblk(g.NC)), dim3(256), 0, st, g, D.ph, D.Ust, D.a2s, D.ctype, D.Z);
 	hipLaunchKernelGGL(k_prep_cell, dim3(nblk(g.NC)), dim3(256), 0, st, g, D.ph, D.Ust, D.a2s, Pst, D.ust, D.vst, D.cz, D.Kw, D.Pold, D.flags);
 	hipLaunchKernelGGL(k_prep_face, dim3(nblk(g.NC)), dim3(256), 0, st, g, order, limiter, tau, D.Ust, D.ust, D.vst, D.cz, D.Pold, D.phx, D.phy,
 	                   D.ubsx, D.kpx, D.wLx, D.pvx, D.ubsy, D.kpy, D.wLy, D.pvy);

@@ -121,6 +121,11 @@ typedef struct spb6_config {
     double mg[2][7];                 /* a Mie-Grueneisen phase's G, c1, c2, e1, e2, rr (reference density), e0 (JWL's
                                         energy shift): Cochran-Chan pr = c1 x^e1 - c2 x^e2 with x = rho / rr, JWL
                                         pr = c1 exp(-e1 rr / rho) + c2 exp(-e2 rr / rho) (PHY:58-66) */
+    double mu_rate;                  /* mu_rate (PHY:33): a finite mechanical relaxation rate (the mu closure, PHY:575-656,
+                                        backward-Euler form, with the stage's frozen impedances PC:956-958) in the cells of
+                                        mechanical type; <= 0: instantaneous (inf, the default). The reference scheme, the
+                                        finite-difference Jacobian, the plain guess and NASG phases only; the batch and the
+                                        explicit HLLC scheme reject it */
 } spb6_config_t;
 
 /* run statistics: the counters of the reference's stats line (PC:1337-1339) plus Krylov counts */

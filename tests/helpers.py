@@ -14,7 +14,8 @@ def config_from_oracle(ocfg, **over):
                     krylov_iter=ocfg.krylov_iter, limiter=ocfg.limiter, order=ocfg.order, rus_local=ocfg.rus_local,
                     jreuse=ocfg.jreuse, variant=ocfg.variant, relax=ocfg.relax, eps_i=ocfg.eps_i,
                     shock_fused=ocfg.shock_fused, theta_rate=max(ocfg.theta_rate, 0.0), nu_rate=max(ocfg.nu_rate, 0.0),
-                    eos_kind=tuple(ocfg.eos_kind), mg=[tuple(ocfg.mg[k][q] for q in range(7)) for k in range(2)])
+                    eos_kind=tuple(ocfg.eos_kind), mg=[tuple(ocfg.mg[k][q] for q in range(7)) for k in range(2)],
+                    mu_rate=max(ocfg.mu_rate, 0.0))
     for k, v in over.items():
         setattr(c, k, v)
     return c
@@ -31,6 +32,7 @@ def oracle_cfg_from(scfg, O, **over):
     c.eps_i = scfg.eps_i if scfg.eps_i > 0 else 1e-4
     c.theta_rate = scfg.theta_rate if scfg.theta_rate > 0 else -1.0
     c.nu_rate = scfg.nu_rate if scfg.nu_rate > 0 else -1.0
+    c.mu_rate = scfg.mu_rate if scfg.mu_rate > 0 else -1.0
     for k in range(2):
         c.eos_kind[k] = scfg.eos_kind[k]
         for q in range(7):

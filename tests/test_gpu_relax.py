@@ -232,3 +232,39 @@ def test_dodecane_tube_bitwise(case, relax):
         o.L.ora_relax_activate(o.h)
         assert st.relax_activate() == 0
     _check(o, st, f"dodecane{case}_r{relax}")
+
+
+@pytest.mark.parametrize("relax,mu,steps", [(0, 1e3, 60), (0, 1e8, 40), (1, 1e3, 40)], ids=["r0_mu1e3", "r0_mu1e8", "r1_mu1e3"])
+def test_finite_mu_bitwise(relax, mu, steps):
+    """the finite mechanical rate (the mu closure with the stage's frozen impedances) on the 2 m/s cavitation tube (800
+    cells): at level 0 every cell is of the mechanical type CL_MU; at level 1 the cells outside eps_I <= alpha_1 <=
+    1 - eps_I are. With mu = 1e3 the run takes rejected attempts (trial transfers that leave the vapour without pressure)
+    as the oracle does; bit-identical to the oracle's device mode, counters included"""
+    from oracle import oracle as O
+    ocfg, U, _ = O.case_init(1, 800)
+    scfg = config_from_oracle(ocfg, relax=relax, mu_rate=mu)
+    oc = oracle_cfg_from(scfg, O)
+    assert oc.mu_rate == mu
+    o, st = _pair_run(scfg, U, steps, activate=relax > 0)
+    _check(o, st, f"tube800_r{relax}_mu{mu}")
+
+
+def test_finite_mu_2d_bitwise():
+    """a 2D vapour pocket under expansion (40 x 28, walls in y) at level 0 with mu = 1e4: the general path's 2D stage
+    with CL_MU in every cell"""
+    from diffusionimex_amd import cases, spb6
+    from oracle import oracle as O
+    nx, ny = 40, 28
+    ocfg, _, _ = O.case_init(1, nx, ny)
+    scfg = config_from_oracle(ocfg, mu_rate=1e4)
+    scfg.bc[2] = scfg.bc[3] = spb6.BC_WALL
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    X, Y = np.meshgrid(x, y)
+    a1 = np.where((X - 0.5) ** 2 + (Y - 0.5) ** 2 < 0.04, 0.9, 1 - 1e-2)
+    ph = [[ocfg.ph[k][q] for q in range(6)] for k in range(2)]
+    T0 = 354.728
+    v = lambda k, p, T: (ph[k][0] - 1) * ph[k][4] * T / (p + ph[k][1]) + ph[k][5]
+    U = cases.prim2cell(ph, a1, 1.0 / v(0, 1e5, T0), 1.0 / v(1, 1e5, T0), 3.0 * (X - 0.5), 3.0 * (Y - 0.5), 1e5, 1.2e5)
+    o, st = _pair_run(scfg, U, 20)
+    _check(o, st, "pocket40x28_mu1e4")
