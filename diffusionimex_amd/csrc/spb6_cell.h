@@ -832,9 +832,9 @@ SPB_HD bool cell_res_var(int var, const Phys& P, double P_REF, double rx, double
 	const double rho = X[2] + X[3];
 	if (!(rho > 0)) return false;
 	ClIn in; in.atr0 = X[0]; in.atr1 = X[1]; in.mtr0 = X[2]; in.mtr1 = X[3]; in.etr0 = X[4]; in.etr1 = X[5];
-	in.s = s; in.rhoeC = E - 0.5 * (mx * mx + my * my) / rho; in.pguess = P_REF + Pc; in.Q0 = Q0; in.Q1 = Q1; in.tau = vc.tau; in.Z0 = vc.Z0; in.Z1 = vc.Z1;
+	in.s = s; in.rhoeC = E - 0.5 * (mx * mx + my * my) / rho; in.pguess = P_REF + Pc; in.Q0 = Q0; in.Q1 = Q1; in.tau = vc.tau;
 	ClOut o;
-	if (!closure_of(P, vc.rel, vc.type, in, o)) return false;
+	if (!closure_of(P, vc.rel, vc.type, in, o, vc.Z0, vc.Z1)) return false;
 	G = (o.pbar - P_REF) - Pc;
 	if (out) {
 		out->a0 = o.a0; out->a1 = o.a1; out->m0 = o.m0; out->m1 = o.m1; out->e0 = o.e0; out->e1 = o.e1; out->mx = mx; out->my = my; out->E = E;

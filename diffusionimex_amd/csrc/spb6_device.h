@@ -785,7 +785,7 @@ SPB_HD void stencil_state(const Phys& P, const double* c, double* st) {
 
 // ------------------------------------------------------------------ p closure (PHY:420-494, PHO:472-548)
 // tau: the stage's implicit step a_ii dt (read by the finite-rate closures only)
-struct ClIn { double atr0, atr1, mtr0, mtr1, etr0, etr1, s, rhoeC, pguess, Q0, Q1; double tau = 0.0; double Z0 = 1.0, Z1 = 1.0; };
+struct ClIn { double atr0, atr1, mtr0, mtr1, etr0, etr1, s, rhoeC, pguess, Q0, Q1; double tau = 0.0; };
 // the closure's phase contents: volume fractions, masses (the transported ones but where the pTg closure moves mass)
 // and energies, and the pressure
 struct ClOut { double a0, a1, m0, m1, e0, e1, pbar; };
@@ -1300,8 +1300,8 @@ SPB_HD bool closure_nu(const Phys& P, double nu, const ClIn& in, ClOut& o) {
 // linear system (each phase's energy with its own pressure work, the defect share on the guess's segment, the frozen
 // impedance weights w_k splitting the transfer's work); R is the root of F(R) = R - tau mu (p1 - p2)(R), bracketed from
 // F(0) by steps of the first estimate (an inadmissible trial halves back towards the last good point), then Brent.
-// pbar = alpha_1 p1 + alpha_2 p2.
-SPB_HD bool closure_mu(const Phys& P, double mu, const ClIn& in, ClOut& o) {
+// pbar = alpha_1 p1 + alpha_2 p2. Z0, Z1: the cell's frozen impedances.
+SPB_HD bool closure_mu(const Phys& P, double mu, const ClIn& in, double Z0, double Z1, ClOut& o) {
 	if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0) || !(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
 	if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
 	const double sa = in.atr0 + in.atr1;
@@ -1314,7 +1314,7 @@ SPB_HD bool closure_mu(const Phys& P, double mu, const ClIn& in, ClOut& o) {
 	  else { lE0 = sm > 0 ? in.mtr0 / sm : 0.5; lE1 = sm > 0 ? in.mtr1 / sm : 0.5; } }
 	double lam0, lam1, ex0, ex1;
 	defect_seg(D0 + in.pguess * in.s, in.Q0 + in.Q1, in.Q0, in.Q1, lE0, lE1, lam0, lam1, ex0, ex1);
-	const double w1 = in.Z0 + in.Z1 > 0 ? in.Z1 / (in.Z0 + in.Z1) : 0.5, w2 = 1.0 - w1;
+	const double w1 = Z0 + Z1 > 0 ? Z1 / (Z0 + Z1) : 0.5, w2 = 1.0 - w1;
 	const double tm = __builtin_isinf(mu) ? dinf() : in.tau * mu;
 	auto solve = [&](double R, double& p1, double& p2) -> bool {
 		const double a1 = ak0 + R, a2 = ak1 - R;
@@ -1355,9 +1355,9 @@ SPB_HD bool closure_mu(const Phys& P, double mu, const ClIn& in, ClOut& o) {
 	o.pbar = o.a0 * p1 + o.a1 * p2;
 	return isfin(o.pbar);
 }
-SPB_HD bool closure_of(const Phys& P, const RelaxDev* S, int ty, const ClIn& in, ClOut& o) {
+SPB_HD bool closure_of(const Phys& P, const RelaxDev* S, int ty, const ClIn& in, ClOut& o, double Z0 = 1.0, double Z1 = 1.0) {
 	switch (ty) {
-	case CL_MU: return closure_mu(P, S->mu, in, o);
+	case CL_MU: return closure_mu(P, S->mu, in, Z0, Z1, o);
 	case CL_PT: return closure_pT(P, in, o);
 	case CL_THETA: return closure_theta(P, S->theta, in, o);
 	case CL_NU: return closure_nu(P, S->nu, in, o);
