@@ -31,6 +31,7 @@ class Cfg(C.Structure):
         ("relax", C.c_int), ("eps_i", C.c_double), ("shock_fused", C.c_int),
         ("theta_rate", C.c_double), ("nu_rate", C.c_double),
         ("eos_kind", C.c_int * 2), ("mg", (C.c_double * 7) * 2), ("mu_rate", C.c_double),
+        ("nu_law", C.c_int), ("nu_cr", C.c_double), ("nu_pc", C.c_double),
     ]
 
 
@@ -67,6 +68,11 @@ def lib():
         L = C.CDLL(LIB_PATH)
         vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
         L.ora_create.restype = vp
+        L.ora_psat.restype = C.c_double
+        L.ora_psat.argtypes = [vp, C.c_double]
+        L.ora_nu_law.restype = C.c_double
+        L.ora_nu_law.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+        L.ora_cell_nu.argtypes = [vp, dp]
         L.ora_closure_typed.argtypes = [C.POINTER(OraModel), dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.ora_eos.argtypes = [C.POINTER(OraModel), C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.ora_create.argtypes = [C.POINTER(Cfg)]

@@ -121,6 +121,7 @@ struct Model {
 	// the thermal and chemical relaxation rates (PHY:33; resolved by level as PC:1148-1149: inf = instantaneous)
 	double theta_rate = 0.0, nu_rate = 0.0;
 	double mu_rate = std::numeric_limits<double>::infinity();   // the mechanical relaxation rate (PHY:33: inf = instantaneous)
+	int nu_law = 0; double nu_cr = 1.0, nu_pc = 22.064e6;           // the state-dependent transfer rate (NU_LAW, NU_CR, NU_PC; PHY:36)
 	double exp_(double x) const { return portable ? pexp(x) : std::exp(x); }
 	double log_(double x) const { return portable ? plog(x) : std::log(x); }
 	double pow_(double x, double y) const { return portable ? ppow(x, y) : std::pow(x, y); }
@@ -443,7 +444,8 @@ enum { CL_P = 0, CL_MU = 1, CL_PT = 2, CL_THETA = 3, CL_NU = 4, CL_PTG = 5 };
 // nodelta (PHY:234): the p closure without the energy defect, for the p-pTg activation's prediction only; tau: the
 // stage's implicit step a_ii dt (the finite-rate closures)
 struct ClIn { double atr[2], mtr[2], etr[2], s, rhoeC, pguess, Qph[2]; int type = CL_P; bool nodelta = false; double tau = 0.0;
-               double Z[2] = {1.0, 1.0}; };   // Z: the phases' frozen acoustic impedances (finite mu, PC:956-958)
+               double Z[2] = {1.0, 1.0}; double nu = -1.0; };   // Z: the phases' frozen acoustic impedances (finite mu,
+// PC:956-958); nu: the cell's transfer rate of the step (>= 0 replaces the model's, the nu law; PHY:241)
 struct ClOut { double a[2], m[2], e[2], p[2], T[2], pbar; };
 struct PCoef { double A[2], B[2], N[2], lam[2]; };
 static double E_above(const Model& M, const ClIn& in, int k) {                                     // PHY:420-423
@@ -697,6 +699,24 @@ static double Tsat(const Model& M, double p) {
 	}
 	bool ok; const double T = sat_newton(M, p, Tg, ok); return ok ? T : NAN;
 }
+// the saturation pressure psat(T), the inverse of T_sat on the table's range (PHY:162-171): bisection in ln p between
+// the table's lowest pressure and the critical one; 0 below the table's T, inf above the critical T
+static double psat(const Model& M, double T) {
+	if (!M.sat || M.sat->lp.empty()) return NAN;
+	double a = M.sat->pmin, b = M.sat->pcrit * (1 - 1e-12);
+	const double Ta = Tsat(M, a), Tb = Tsat(M, b);
+	if (!(T > Ta)) return 0.0;
+	if (!(T < Tb)) return INF;
+	for (int it = 0; it < 80; it++) { const double c = std::sqrt(a * b); if (Tsat(M, c) < T) a = c; else b = c; if (b / a - 1 < 1e-14) break; }
+	return std::sqrt(a * b);
+}
+// Pelanti 2022 eq. (84) (PHY:172-177): nu = c_r alpha_v^0.6 ((psat(T) - p) / (p_c - psat(T)))^1.76 in superheated liquid
+// (p < psat(T)), else 0
+static double nu_law(const Model& M, double p, double T, double av) {
+	const double ps = psat(M, T);
+	if (!std::isfinite(ps) || !(ps > p) || !(ps < M.nu_pc) || !(av > 0)) return 0.0;
+	return M.nu_cr * M.pow_(av, 0.6) * M.pow_((ps - p) / (M.nu_pc - ps), 1.76);
+}
 static const double YMIN = 1e-10;                                                                    // PHY:242
 // T at p and vapour mass fraction Y from the mixture's specific energy (PHY:701-704)
 static inline double T_from_energy(const Model& M, double p, double Y, double eC) {
@@ -936,7 +956,7 @@ static bool closure_theta(const Model& M, const ClIn& in, ClOut& o) {
 // its admissible interval (clamped at the ends), T from the mixture energy; the root of rho sum_k Y_k v_k = 1
 static bool closure_nu(const Model& M, const ClIn& in, ClOut& o) {
 	const Phase &A = M.PH[0], &B = M.PH[1];
-	const double rho = in.mtr[0] + in.mtr[1], eC = in.rhoeC / rho, nr = M.nu_rate, tn = std::isinf(nr) ? INF : in.tau * nr;
+	const double rho = in.mtr[0] + in.mtr[1], eC = in.rhoeC / rho, nr = in.nu >= 0 ? in.nu : M.nu_rate, tn = std::isinf(nr) ? INF : in.tau * nr;
 	auto Yof = [&](double p, double& Y, double& T) -> bool {
 		auto R = [&](double y) {
 			const double TT = T_from_energy(M, p, y, eC); if (!(TT > 0)) return (double)NAN;
@@ -1038,6 +1058,7 @@ struct Cfg {
 	int eos_kind[2];              // Phase::kind (PHY:50-53): 0 NASG / SG, 1 Cochran-Chan, 2 JWL
 	double mg[2][7];              // a Mie-Grueneisen phase's G, c1, c2, e1, e2, rr, e0
 	double mu_rate;               // mu_rate (PHY:33); <= 0: instantaneous (inf, the default)
+	int nu_law; double nu_cr, nu_pc;   // NU_LAW, NU_CR, NU_PC (PHY:36); nu_cr, nu_pc <= 0: 1, 22.064e6
 };
 static Phase phase_of(const double* ph, int kind, const double* mg) {
 	Phase P{ph[0], ph[1], ph[2], ph[3], ph[4], ph[5]};
@@ -1059,6 +1080,7 @@ struct Ctx {
 	std::vector<double> phx, phy, a2s, ubx, uby;
 	std::vector<double> eTV1, eTV2, ETV, dA;   // the TV-type variants' explicit phase energies and total energy; BB's alpha_1 step
 	std::vector<int> ctype;                    // STEP_TYPE (PC:537-560): each interior cell's closure type of the step
+	std::vector<double> cnu;                   // STEP_NU (PC:162): the nu law's rate of each interior cell for the step
 	SatTable SAT;                              // the saturation line (relaxation levels 2-3; PC:1151 builds it before the run)
 	// stage data (PC:565-582)
 	double tau, rx, ry;
@@ -1083,6 +1105,8 @@ static void init_ctx(Ctx& X, const Cfg& C) {
 	X.M.theta_rate = C.theta_rate < 0 ? (C.relax >= 1 ? INF : 0.0) : C.theta_rate;                 // PC:1148-1149
 	X.M.nu_rate = C.nu_rate < 0 ? ((C.relax == 2 || C.relax == 3) ? INF : 0.0) : C.nu_rate;
 	X.M.mu_rate = C.mu_rate > 0 ? C.mu_rate : INF;
+	X.M.nu_law = C.nu_law; X.M.nu_cr = C.nu_cr > 0 ? C.nu_cr : 1.0; X.M.nu_pc = C.nu_pc > 0 ? C.nu_pc : 22.064e6;
+	if (C.nu_law) X.M.nu_rate = 0.0;   // PC:1150: the chemical cells take finite-rate closures, each with its own rate
 	if (C.relax >= 2) { build_sat(X.M, X.SAT); X.M.sat = &X.SAT; }
 	X.nx = C.nx; X.ny = C.ny; X.NXT = C.nx + 2 * GH; X.NYT = C.ny + 2 * GH;
 	X.dx = (C.x1 - C.x0) / C.nx; X.dy = (C.y1 - C.y0) / C.ny; X.twoD = C.ny > 1;
@@ -1563,6 +1587,7 @@ static bool cell_residual(const Ctx& X, int i, int j, double Pc, double PW, doub
 	in.type = X.ctype.empty() ? mech_type(M) : X.ctype[(size_t)(i - GH) + (size_t)(j - GH) * X.nx];   // S.type (PC:951)
 	in.tau = X.tau;
 	if (in.type == CL_MU) { in.Z[0] = X.Z[2 * c]; in.Z[1] = X.Z[2 * c + 1]; }
+	if (!X.cnu.empty()) in.nu = X.cnu[(size_t)(i - GH) + (size_t)(j - GH) * X.nx];   // PC:671
 	ClOut o; if (!closure(M, in, o)) return false;
 	G = (o.pbar - X.P_REF) - Pc;
 	if (out) { out->cl = o; out->mx = mx; out->my = my; out->E = E; out->s = s; out->sig = closure_roundoff(M, in, o.pbar); }
@@ -1804,6 +1829,7 @@ static bool project_cell(const Model& M, double* Q, int type) {                 
 static void relaxation_activation(Ctx& X, std::vector<double>& U) {                                // PC:537-560
 	const int nx = X.nx, n = nx * X.ny;
 	X.ctype.assign((size_t)n, mech_type(X.M));
+	if (X.M.nu_law) X.cnu.assign((size_t)n, 0.0);
 	#pragma omp parallel for schedule(static)
 	for (int q = 0; q < n; q++) {
 		double* Q = &U[NV * X.id(q % nx + GH, q / nx + GH)];
@@ -1811,6 +1837,13 @@ static void relaxation_activation(Ctx& X, std::vector<double>& U) {             
 		int ty = closure_type_of(X.M, X.C.relax, X.C.eps_i, Q[IA1], Q[IM1], Q[IM2], Q[IE1], Q[IE2], rhoe);
 		if (!project_cell(X.M, Q, ty)) ty = mech_type(X.M);   // PC:544
 		X.ctype[(size_t)q] = ty;
+		if (X.M.nu_law && ty == CL_NU) {   // PC:546-556: the rate from the cell's pT equilibrium (p, T, vapour fraction)
+			ClIn in{}; in.atr[0] = Q[IA1]; in.atr[1] = 1.0 - Q[IA1]; in.mtr[0] = Q[IM1]; in.mtr[1] = Q[IM2];
+			in.etr[0] = Q[IE1]; in.etr[1] = Q[IE2]; in.s = 0.0; in.tau = 0.0; in.type = CL_PT; in.Qph[0] = in.Qph[1] = 0.0;
+			in.rhoeC = Q[IEN] - 0.5 * (Q[IMU] * Q[IMU] + Q[IMV] * Q[IMV]) / rho;
+			in.pguess = smax(p_mix(X.M, in.atr[0], in.mtr[0], in.mtr[1], in.rhoeC), 1.0);
+			ClOut o; if (closure(X.M, in, o)) X.cnu[(size_t)q] = nu_law(X.M, o.p[0], o.T[0], o.a[1]);
+		}
 	}
 	bc_apply(X, U);
 }
@@ -2377,6 +2410,11 @@ int ora_closure_typed(const OraModel* m, const double* in, int type, double tau,
 	out[7] = o.p[0]; out[8] = o.p[1];
 	return 1;
 }
+// the context's saturation pressure psat(T) and the nu law's rate at (p, T, alpha_v) (NaN / 0 without the table)
+double ora_psat(void* h, double T) { Ctx& X = *(Ctx*)h; return psat(X.M, T); }
+double ora_nu_law(void* h, double p, double T, double av) { Ctx& X = *(Ctx*)h; return nu_law(X.M, p, T, av); }
+// the nu law's rates of the last activation (n interior cells; -1 without the law)
+int ora_cell_nu(void* h, double* out) { Ctx& X = *(Ctx*)h; if (X.cnu.empty()) return -1; std::memcpy(out, X.cnu.data(), X.cnu.size() * sizeof(double)); return (int)X.cnu.size(); }
 double ora_hug_v(const OraModel* m, int k, double p0, double p1, double v0) { return hug_v(mk_model(m), k, p0, p1, v0); }
 int ora_prof_integrate(const OraModel* m, const double* Y, const double* vpre, double p0, double p1, double c, double* v1end, double* qend) {
 	const Model M = mk_model(m); ShockProf S; S.p0 = p0; double vH[2];
