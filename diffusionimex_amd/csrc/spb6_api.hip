@@ -117,7 +117,7 @@ struct spb6_ctx {
 	double *eTV1 = nullptr, *eTV2 = nullptr, *ETV = nullptr, *dA = nullptr, *donax = nullptr, *donay = nullptr;
 	int* ctype = nullptr;   // relaxation levels 1-3: the closure type of each interior cell for the step (STEP_TYPE)
 	double eps_i = 1e-4;    // EPS_I (PHY:38)
-	RelaxDev rel{nullptr, nullptr, 0, 0.0, 0.0, 0.0, 0.0, INFINITY};   // the saturation table on the device (levels 2-3) and the rates
+	RelaxDev rel{nullptr, nullptr, 0, 0.0, 0.0, 0.0, 0.0, INFINITY, 0, 1.0, 22.064e6, nullptr};   // the saturation table on the device (levels 2-3) and the rates
 	double* Z = nullptr;    // finite mu: the stage's impedances of the CL_MU cells (2 NC)
 	// graph mode (cfg.graph): a stage's fixed head (combination, explicit part, preparation, shock budget) and each
 	// Krylov chunk (its BiCGStab passes, the FP32 form's defect, sweep close and trial residual) run as captured graphs,
@@ -820,6 +820,7 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 		if (cfg->eos_kind[k] < 0 || cfg->eos_kind[k] > 2) { g_err = "eos_kind must be 0 (NASG), 1 (Cochran-Chan) or 2 (JWL)"; return nullptr; }
 		if (cfg->eos_kind[k] && !(cfg->mg[k][0] > 0 && cfg->mg[k][5] > 0)) { g_err = "a Mie-Grueneisen phase needs G > 0 and rr > 0"; return nullptr; }
 	}
+	if (cfg->nu_law && !(cfg->relax == 2 || cfg->relax == 3)) { g_err = "the nu law needs relaxation level 2 or 3"; return nullptr; }
 	if (cfg->mu_rate > 0 && (cfg->analytic_jacobian || cfg->learned_guess || cfg->variant != SPB6_VARIANT_PC || cfg->eos_kind[0] ||
 	                         cfg->eos_kind[1])) {
 		g_err = "a finite mu runs with the reference scheme, the finite-difference Jacobian, the plain guess and NASG phases"; return nullptr; }
@@ -869,6 +870,11 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 	if (cfg->mu_rate > 0 && dalloc(c, &c->Z, 2 * NC)) return fail("alloc");
 	c->rel.theta = cfg->theta_rate > 0 ? cfg->theta_rate : (cfg->relax >= 1 ? INFINITY : 0.0);                 // PC:1148-1149
 	c->rel.nu = cfg->nu_rate > 0 ? cfg->nu_rate : ((cfg->relax == 2 || cfg->relax == 3) ? INFINITY : 0.0);
+	if (cfg->nu_law) {   // PC:1150: the chemical cells take the nu closure, each with the rate of the law
+		c->rel.nu_law = 1; c->rel.nu = 0.0;
+		c->rel.nu_cr = cfg->nu_cr > 0 ? cfg->nu_cr : 1.0; c->rel.nu_pc = cfg->nu_pc > 0 ? cfg->nu_pc : 22.064e6;
+		if (dalloc(c, &c->rel.cnu, n)) return fail("alloc");
+	}
 	if (cfg->relax >= 2) {   // the saturation line before the run (PC:1151)
 		std::vector<double> lpv, Tv;
 		build_sat(c->ph, lpv, Tv, c->rel.pcrit, c->rel.vcrit);

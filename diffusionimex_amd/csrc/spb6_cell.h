@@ -770,7 +770,7 @@ __device__ __forceinline__ bool final_at(const Dev& D, const double* __restrict_
 struct FaceInV : FaceIn { double cla, cra; };
 // type: the cell's closure type (CL_P; CL_PT, CL_THETA, CL_NU, CL_PTG at relaxation levels 1-3); rel: the saturation
 // table and the finite rates; tau: the stage's implicit step a_ii dt
-struct VarCell { double a2s, eT1, eT2, ET; int type; const RelaxDev* rel; double tau; double Z0 = 1.0, Z1 = 1.0; };
+struct VarCell { double a2s, eT1, eT2, ET; int type; const RelaxDev* rel; double tau; double Z0 = 1.0, Z1 = 1.0, nu = -1.0; };
 SPB_HD double var_inc(const FaceInV& f, const FaceVals& v, int k) {   // inc (PC:617) over the 6-vector (alpha1, 1 - alpha1, m1, m2, e1, e2)
 	if (k == 0) return f.cla * v.wL + f.cra * v.wR;
 	if (k == 1) return (1.0 - f.cla) * v.wL + (1.0 - f.cra) * v.wR;
@@ -834,7 +834,7 @@ SPB_HD bool cell_res_var(int var, const Phys& P, double P_REF, double rx, double
 	ClIn in; in.atr0 = X[0]; in.atr1 = X[1]; in.mtr0 = X[2]; in.mtr1 = X[3]; in.etr0 = X[4]; in.etr1 = X[5];
 	in.s = s; in.rhoeC = E - 0.5 * (mx * mx + my * my) / rho; in.pguess = P_REF + Pc; in.Q0 = Q0; in.Q1 = Q1; in.tau = vc.tau;
 	ClOut o;
-	if (!closure_of(P, vc.rel, vc.type, in, o, vc.Z0, vc.Z1)) return false;
+	if (!closure_of(P, vc.rel, vc.type, in, o, vc.Z0, vc.Z1, vc.nu)) return false;
 	G = (o.pbar - P_REF) - Pc;
 	if (out) {
 		out->a0 = o.a0; out->a1 = o.a1; out->m0 = o.m0; out->m1 = o.m1; out->e0 = o.e0; out->e1 = o.e1; out->mx = mx; out->my = my; out->E = E;
@@ -863,6 +863,7 @@ struct CellFacesV {
 		if (D.variant >= VAR_TVPURE) { vc.eT1 = D.eTV1[c]; vc.eT2 = D.eTV2[c]; vc.ET = D.ETV[c]; } else { vc.eT1 = vc.eT2 = vc.ET = 0.0; }
 		vc.type = D.ctype ? D.ctype[q] : CL_P; vc.rel = &D.rel; vc.tau = D.tau;
 		if (vc.type == CL_MU) { vc.Z0 = D.Z[c]; vc.Z1 = D.Z[D.g.NC + c]; }
+		if (D.rel.cnu) vc.nu = D.rel.cnu[q];   // PC:671
 		Q0 = D.Qph[c]; Q1 = D.Qph[D.g.NC + c]; Kw = D.Kw[c];
 	}
 };

@@ -1146,7 +1146,9 @@ SPB_HD bool closure_pT(const Phys& P, const ClIn& in, ClOut& o, double* T_out = 
 // the saturation table (built on the host by the same arithmetic, PHY:125-152): n points of ln p on a uniform grid and
 // T_sat there, and the critical point (pcrit, vcrit); and the thermal and chemical relaxation rates theta, nu (PHY:33,
 // resolved by level: inf = instantaneous)
-struct RelaxDev { const double* lp; const double* T; int n; double pcrit, vcrit, theta, nu, mu; };
+// the saturation table and the rates; nu_law: Pelanti's state-dependent transfer rate (c_r, p_c; PHY:34-36) with each
+// interior cell's rate of the step in cnu (STEP_NU, PC:162)
+struct RelaxDev { const double* lp; const double* T; int n; double pcrit, vcrit, theta, nu, mu; int nu_law; double nu_cr, nu_pc; double* cnu; };
 SPB_HD double s_pT(const Phys& P, int k, double p, double T) {                      // PHY:80
 	return P.cv[k] * (P.g[k] * plog(T) - (P.g[k] - 1.0) * plog(p + P.pi[k])) + P.qp[k]; }
 SPB_HD double g_pT(const Phys& P, int k, double p, double T) {                      // PHY:81
@@ -1355,12 +1357,30 @@ SPB_HD bool closure_mu(const Phys& P, double mu, const ClIn& in, double Z0, doub
 	o.pbar = o.a0 * p1 + o.a1 * p2;
 	return isfin(o.pbar);
 }
-SPB_HD bool closure_of(const Phys& P, const RelaxDev* S, int ty, const ClIn& in, ClOut& o, double Z0 = 1.0, double Z1 = 1.0) {
+// the saturation pressure psat(T), the inverse of T_sat on the table (PHY:162-171): bisection in ln p between the table's
+// lowest pressure and the critical one; 0 below the table's T, inf above the critical T
+SPB_HD double psat(const Phys& P, const RelaxDev& S, double T) {
+	if (S.n < 1) return dnan();
+	double a = pexp(S.lp[0]), b = S.pcrit * (1 - 1e-12);
+	const double Ta = Tsat(P, S, a), Tb = Tsat(P, S, b);
+	if (!(T > Ta)) return 0.0;
+	if (!(T < Tb)) return dinf();
+	for (int it = 0; it < 80; it++) { const double c = sqrt(a * b); if (Tsat(P, S, c) < T) a = c; else b = c; if (b / a - 1 < 1e-14) break; }
+	return sqrt(a * b);
+}
+// Pelanti 2022 eq. (84) (PHY:172-177): nu = c_r alpha_v^0.6 ((psat(T) - p) / (p_c - psat(T)))^1.76 where p < psat(T)
+SPB_HD double nu_law(const Phys& P, const RelaxDev& S, double p, double T, double av) {
+	const double ps = psat(P, S, T);
+	if (!isfin(ps) || !(ps > p) || !(ps < S.nu_pc) || !(av > 0)) return 0.0;
+	return S.nu_cr * ppow(av, 0.6) * ppow((ps - p) / (S.nu_pc - ps), 1.76);
+}
+// nu: the cell's own transfer rate of the step (>= 0 replaces the context's; the nu law)
+SPB_HD bool closure_of(const Phys& P, const RelaxDev* S, int ty, const ClIn& in, ClOut& o, double Z0 = 1.0, double Z1 = 1.0, double nu = -1.0) {
 	switch (ty) {
 	case CL_MU: return closure_mu(P, S->mu, in, Z0, Z1, o);
 	case CL_PT: return closure_pT(P, in, o);
 	case CL_THETA: return closure_theta(P, S->theta, in, o);
-	case CL_NU: return closure_nu(P, S->nu, in, o);
+	case CL_NU: return closure_nu(P, nu >= 0 ? nu : S->nu, in, o);
 	case CL_PTG: return closure_pTg(P, *S, in, o);
 	default: return closure_p(P, in, o);
 	}

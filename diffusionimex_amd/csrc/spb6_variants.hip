@@ -222,9 +222,20 @@ __global__ void __launch_bounds__(256) k_relax_activate(Geo g, Phys P, RelaxDev 
 		ClOut o;
 		if (closure_of(P, &S, ty, in, o)) {
 			U[IA1 * NC + c] = o.a0; U[IM1 * NC + c] = o.m0; U[IM2 * NC + c] = o.m1; U[IE1 * NC + c] = o.e0; U[IE2 * NC + c] = o.e1;
-		} else ty = CL_P;
+		} else ty = __builtin_isinf(S.mu) ? CL_P : CL_MU;   // PC:544
 	}
 	ctype[q] = ty;
+	if (S.nu_law) {   // PC:546-556: the chemical cell's rate from its pT equilibrium (p, T, vapour fraction)
+		double nu = 0.0;
+		if (ty == CL_NU && a1 > 0 && 1.0 - a1 > 0 && m1 > 0 && m2 > 0 && isfin(e1) && isfin(e2) && isfin(rhoe)) {
+			ClIn in;
+			in.atr0 = a1; in.atr1 = 1.0 - a1; in.mtr0 = m1; in.mtr1 = m2; in.etr0 = e1; in.etr1 = e2; in.s = 0.0; in.rhoeC = rhoe;
+			in.pguess = smax(p_mix(P, in.atr0, in.mtr0, in.mtr1, in.rhoeC), 1.0); in.Q0 = 0.0; in.Q1 = 0.0;
+			ClOut o; double T;
+			if (closure_pT(P, in, o, &T)) nu = nu_law(P, S, o.pbar, T, o.a1);
+		}
+		S.cnu[q] = nu;
+	}
 }
 
 }   // namespace

@@ -126,6 +126,11 @@ typedef struct spb6_config {
                                         mechanical type; <= 0: instantaneous (inf, the default). The reference scheme, the
                                         finite-difference Jacobian, the plain guess and NASG phases only; the batch and the
                                         explicit HLLC scheme reject it */
+    int nu_law;                      /* NU_LAW (PHY:34-36, the command line's nulaw=1): at levels 2-3 the chemical cells take
+                                        the nu closure with Pelanti's state-dependent rate nu = nu_cr alpha_v^0.6
+                                        ((psat(T) - p) / (nu_pc - psat(T)))^1.76 (PHY:172-177), frozen per cell at each
+                                        attempt's activation from the cell's pT equilibrium (PC:546-556) */
+    double nu_cr, nu_pc;             /* NU_CR, NU_PC (PHY:36); <= 0: 1 and 22.064e6 */
 } spb6_config_t;
 
 /* run statistics: the counters of the reference's stats line (PC:1337-1339) plus Krylov counts */

@@ -268,3 +268,15 @@ def test_finite_mu_2d_bitwise():
     U = cases.prim2cell(ph, a1, 1.0 / v(0, 1e5, T0), 1.0 / v(1, 1e5, T0), 3.0 * (X - 0.5), 3.0 * (Y - 0.5), 1e5, 1.2e5)
     o, st = _pair_run(scfg, U, 20)
     _check(o, st, "pocket40x28_mu1e4")
+
+
+@pytest.mark.parametrize("relax", [2, 3])
+def test_nu_law_bitwise(relax):
+    """Pelanti's state-dependent transfer rate (the nu law, psat from the device's saturation table) on the NASG water
+    tube (case 30, 1000 cells, 60 steps) at levels 2 and 3: each attempt's activation freezes the chemical cells' rates
+    from their pT equilibria; bit-identical to the oracle's device mode"""
+    from oracle import oracle as O
+    ocfg, U, _ = O.case_init(30, 1000)
+    scfg = config_from_oracle(ocfg, relax=relax, nu_law=1)
+    o, st = _pair_run(scfg, U, 60, activate=True)
+    _check(o, st, f"nasg1000_r{relax}_nulaw")
