@@ -117,7 +117,7 @@ struct spb6_ctx {
 	double *eTV1 = nullptr, *eTV2 = nullptr, *ETV = nullptr, *dA = nullptr, *donax = nullptr, *donay = nullptr;
 	int* ctype = nullptr;   // relaxation levels 1-3: the closure type of each interior cell for the step (STEP_TYPE)
 	double eps_i = 1e-4;    // EPS_I (PHY:38)
-	RelaxDev rel{nullptr, nullptr, 0, 0.0, 0.0, 0.0, 0.0, INFINITY, 0, 1.0, 22.064e6, nullptr};   // the saturation table on the device (levels 2-3) and the rates
+	RelaxDev rel{nullptr, nullptr, 0, 0.0, 0.0, 0.0, 0.0, INFINITY, 0, 1.0, 22.064e6, nullptr, 0};   // the saturation table on the device (levels 2-3) and the rates
 	double* Z = nullptr;    // finite mu: the stage's impedances of the CL_MU cells (2 NC)
 	// graph mode (cfg.graph): a stage's fixed head (combination, explicit part, preparation, shock budget) and each
 	// Krylov chunk (its BiCGStab passes, the FP32 form's defect, sweep close and trial residual) run as captured graphs,
@@ -820,6 +820,8 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 		if (cfg->eos_kind[k] < 0 || cfg->eos_kind[k] > 2) { g_err = "eos_kind must be 0 (NASG), 1 (Cochran-Chan) or 2 (JWL)"; return nullptr; }
 		if (cfg->eos_kind[k] && !(cfg->mg[k][0] > 0 && cfg->mg[k][5] > 0)) { g_err = "a Mie-Grueneisen phase needs G > 0 and rr > 0"; return nullptr; }
 	}
+	if (cfg->relax_form < 0 || cfg->relax_form > 1 || (cfg->relax_form == 1 && cfg->mu_rate > 0)) {
+		g_err = "relax_form is 0 (backward Euler) or 1 (the exponential form of the theta and nu closures; not with a finite mu)"; return nullptr; }
 	if (cfg->nu_law && !(cfg->relax == 2 || cfg->relax == 3)) { g_err = "the nu law needs relaxation level 2 or 3"; return nullptr; }
 	if (cfg->mu_rate > 0 && (cfg->analytic_jacobian || cfg->learned_guess || cfg->variant != SPB6_VARIANT_PC || cfg->eos_kind[0] ||
 	                         cfg->eos_kind[1])) {
@@ -858,6 +860,7 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 		if (dalloc(c, &c->donax, 2 * NC) || dalloc(c, &c->donay, 2 * NC)) return fail("alloc");
 	}
 	c->rel.mu = cfg->mu_rate > 0 ? cfg->mu_rate : INFINITY;   // PHY:33 (inf: instantaneous)
+	c->rel.form = cfg->relax_form;
 	if (cfg->relax > 0 || cfg->mu_rate > 0) {
 		// the cells' closure types: at level 0 with a finite mu every cell is CL_MU (PHY:815), so the stage takes the
 		// general per-cell-type path; each attempt's activation rewrites them

@@ -129,6 +129,15 @@ SPB_HD double plog(double x) {
 	return de * 6.93147180369123816490e-01 + (logm + de * 1.90821492927058770002e-10);
 }
 SPB_HD double ppow(double x, double y) { return pexp(y * plog(x)); }
+// e^x - 1: the Taylor series to x^17 / 17! by Horner on |x| < 0.5 (the next term is below 2^-70 relative), else
+// pexp(x) - 1 (no cancellation beyond 2 bits there)
+SPB_HD double pexpm1(double x) {
+	if (!(fabs(x) < 0.5)) return pexp(x) - 1.0;
+	double f = 1.0; for (int k = 2; k <= 17; k++) f *= (double)k;   // 17! is exact in FP64
+	double q = 1.0 / f;
+	for (int k = 16; k >= 1; k--) { f /= (double)(k + 1); q = q * x + 1.0 / f; }
+	return q * x;
+}
 SPB_HD double phypot(double a, double b) {
 	if (b == 0.0) return fabs(a);
 	if (a == 0.0) return fabs(b);
@@ -1148,7 +1157,7 @@ SPB_HD bool closure_pT(const Phys& P, const ClIn& in, ClOut& o, double* T_out = 
 // resolved by level: inf = instantaneous)
 // the saturation table and the rates; nu_law: Pelanti's state-dependent transfer rate (c_r, p_c; PHY:34-36) with each
 // interior cell's rate of the step in cnu (STEP_NU, PC:162)
-struct RelaxDev { const double* lp; const double* T; int n; double pcrit, vcrit, theta, nu, mu; int nu_law; double nu_cr, nu_pc; double* cnu; };
+struct RelaxDev { const double* lp; const double* T; int n; double pcrit, vcrit, theta, nu, mu; int nu_law; double nu_cr, nu_pc; double* cnu; int form; };
 SPB_HD double s_pT(const Phys& P, int k, double p, double T) {                      // PHY:80
 	return P.cv[k] * (P.g[k] * plog(T) - (P.g[k] - 1.0) * plog(p + P.pi[k])) + P.qp[k]; }
 SPB_HD double g_pT(const Phys& P, int k, double p, double T) {                      // PHY:81
@@ -1236,7 +1245,7 @@ SPB_HD bool p_coef_at(const Phys& P, const ClIn& in, double pseg, double A[2], d
 // finite theta (B6.4, PHY:679-700): at fixed p the phases' temperatures are linear in the exchanged heat, and
 // dQ = tau theta (T2 - T1) / (1 + tau theta S), S = 1/Cp1 + 1/Cp2, shifts the p closure's volume fractions; the
 // coefficient segment is the one at the guess, the root of sum alpha_k = 1 by the bracket search above max(-pi_k)
-SPB_HD bool closure_theta(const Phys& P, double theta, const ClIn& in, ClOut& o) {
+SPB_HD bool closure_theta(const Phys& P, double theta, const ClIn& in, ClOut& o, int form = 0) {
 	if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0)) return false;
 	if (!(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
 	if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
@@ -1254,7 +1263,8 @@ SPB_HD bool closure_theta(const Phys& P, double theta, const ClIn& in, ClOut& o)
 			T0[k] = T_amp(P, k, al[k], m[k], p);
 		}
 		double dQ;
-		if (__builtin_isinf(tt)) dQ = (T0[1] - T0[0]) / S;
+		if (form == 1) dQ = (T0[1] - T0[0]) * (-pexpm1(-tt * S)) / S;   // the exponential form (PHY:685)
+		else if (__builtin_isinf(tt)) dQ = (T0[1] - T0[0]) / S;
 		else dQ = tt * (T0[1] - T0[0]) / (1.0 + tt * (1.0 / Cp0 + 1.0 / Cp1));
 		a0 = al[0] + (P.g[0] - 1.0) * dQ / (P.g[0] * (p + P.pi[0]));
 		a1 = al[1] - (P.g[1] - 1.0) * dQ / (P.g[1] * (p + P.pi[1]));
@@ -1270,23 +1280,37 @@ SPB_HD bool closure_theta(const Phys& P, double theta, const ClIn& in, ClOut& o)
 }
 // finite nu (B6.5, PHY:716-753): at each p the vapour fraction Y solves rho Y - m2 + tau nu (g2 - g1)(p, T(p, Y)) = 0 on
 // its admissible interval (clamped at the ends), T from the mixture energy; the root of rho sum_k Y_k v_k = 1
-SPB_HD bool closure_nu(const Phys& P, double nu, const ClIn& in, ClOut& o) {
+SPB_HD bool closure_nu(const Phys& P, double nu, const ClIn& in, ClOut& o, int form = 0) {
 	if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0)) return false;
 	if (!(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
 	if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
 	const double rho = in.mtr0 + in.mtr1, eC = in.rhoeC / rho, tn = __builtin_isinf(nu) ? dinf() : in.tau * nu;
 	auto Yof = [&](double p, double& Y, double& T) -> bool {
+		auto G = [&](double y) { const double TT = T_from_energy(P, p, y, eC); if (!(TT > 0)) return dnan(); return g_pT(P, 1, p, TT) - g_pT(P, 0, p, TT); };
 		auto R = [&](double y) {
 			const double TT = T_from_energy(P, p, y, eC); if (!(TT > 0)) return dnan();
-			if (__builtin_isinf(tn)) return g_pT(P, 1, p, TT) - g_pT(P, 0, p, TT);
+			if (form == 1 || __builtin_isinf(tn)) return g_pT(P, 1, p, TT) - g_pT(P, 0, p, TT);
 			return rho * y - in.mtr1 + tn * (g_pT(P, 1, p, TT) - g_pT(P, 0, p, TT)); };
 		double ya = YMIN, yb = 1.0 - YMIN;
 		const double dq = P.q[1] - P.q[0], yT = dq != 0.0 ? (eC - P.q[0]) / dq : dinf();
 		if (dq > 0) yb = smin(yb, yT * (1.0 - 1e-12)); else if (dq < 0) ya = smax(ya, yT * (1.0 + 1e-12));
 		if (!(ya < yb)) return false;
+		const double Y0 = in.mtr1 / rho;
+		if (form == 1 && tn == 0.0) { Y = smin(smax(Y0, ya), yb); T = T_from_energy(P, p, Y, eC); return T > 0; }
 		const double Ra = R(ya), Rb = R(yb);
 		if (!isfin(Ra) || !isfin(Rb)) return false;
 		if (Ra >= 0) Y = ya; else if (Rb <= 0) Y = yb; else if (!brent_root(R, ya, yb, Ra, Rb, 1e-16, Y)) return false;
+		if (form == 1 && !__builtin_isinf(tn)) {   // the exponential form (PHY:733-745): along the secant D of G from Y_eq
+			const double Yeq = Y, Ys = smin(smax(Y0, ya), yb);
+			const double D = Ys != Yeq ? G(Ys) / (Ys - Yeq) : 0.0;
+			if (D > 0 && isfin(D)) Y = smin(smax(Yeq + (Y0 - Yeq) * pexp(-(tn * D) / rho), ya), yb);
+			else if (fabs(Ys - Yeq) <= 1e-14) Y = Yeq;
+			else {
+				auto Rbe = [&](double y) { const double g = G(y); return isfin(g) ? rho * y - in.mtr1 + tn * g : dnan(); };
+				const double ra = Rbe(ya), rb = Rbe(yb); if (!isfin(ra) || !isfin(rb)) return false;
+				if (ra >= 0) Y = ya; else if (rb <= 0) Y = yb; else if (!brent_root(Rbe, ya, yb, ra, rb, 1e-16, Y)) return false;
+			}
+		}
 		T = T_from_energy(P, p, Y, eC); return T > 0;
 	};
 	auto Phi = [&](double p) { double Y, T; if (!Yof(p, Y, T)) return dnan(); return rho * ((1 - Y) * v_pT(P, 0, p, T) + Y * v_pT(P, 1, p, T)) - 1.0; };
@@ -1379,8 +1403,8 @@ SPB_HD bool closure_of(const Phys& P, const RelaxDev* S, int ty, const ClIn& in,
 	switch (ty) {
 	case CL_MU: return closure_mu(P, S->mu, in, Z0, Z1, o);
 	case CL_PT: return closure_pT(P, in, o);
-	case CL_THETA: return closure_theta(P, S->theta, in, o);
-	case CL_NU: return closure_nu(P, nu >= 0 ? nu : S->nu, in, o);
+	case CL_THETA: return closure_theta(P, S->theta, in, o, S->form);
+	case CL_NU: return closure_nu(P, nu >= 0 ? nu : S->nu, in, o, S->form);
 	case CL_PTG: return closure_pTg(P, *S, in, o);
 	default: return closure_p(P, in, o);
 	}

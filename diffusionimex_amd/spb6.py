@@ -40,7 +40,7 @@ class Config(C.Structure):
         ("jreuse", C.c_int), ("variant", C.c_int), ("relax", C.c_int), ("eps_i", C.c_double), ("graph", C.c_int),
         ("shock_fused", C.c_int), ("theta_rate", C.c_double), ("nu_rate", C.c_double),
         ("eos_kind", C.c_int * 2), ("mg", (C.c_double * 7) * 2), ("mu_rate", C.c_double),
-        ("nu_law", C.c_int), ("nu_cr", C.c_double), ("nu_pc", C.c_double),
+        ("nu_law", C.c_int), ("nu_cr", C.c_double), ("nu_pc", C.c_double), ("relax_form", C.c_int),
     ]
 
 
@@ -173,7 +173,7 @@ def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0
                 krylov_tol=1e-13, krylov_iter=4000, limiter=1, order=2, rus_local=1, krylov_fp32=0, learned_guess=0,
                 analytic_jacobian=0, jreuse=0, variant=0, relax=0, eps_i=0.0, graph=0, shock_fused=0, theta_rate=0.0, nu_rate=0.0,
                 eos_kind=(0, 0), mg=((0.0,) * 7, (0.0,) * 7), mu_rate=0.0,
-                nu_law=0, nu_cr=0.0, nu_pc=0.0):
+                nu_law=0, nu_cr=0.0, nu_pc=0.0, relax_form=0):
     """Config with the reference defaults (PC:57-80): phases are (gamma, p_inf[, q, qp, cv, b]); eos_kind[k] 1 or 2 makes
     phase k a Cochran-Chan or JWL phase with mg[k] = (G, c1, c2, e1, e2, rr, e0)."""
     c = Config()
@@ -199,6 +199,7 @@ def make_config(nx, ny=1, phases=((4.4, 6e8), (1.4, 0.0)), domain=(0.0, 1.0, 0.0
     c.theta_rate, c.nu_rate = theta_rate, nu_rate
     c.mu_rate = mu_rate
     c.nu_law, c.nu_cr, c.nu_pc = nu_law, nu_cr, nu_pc
+    c.relax_form = relax_form
     for k in range(2):
         c.eos_kind[k] = eos_kind[k]
         for q in range(7):

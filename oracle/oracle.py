@@ -32,6 +32,7 @@ class Cfg(C.Structure):
         ("theta_rate", C.c_double), ("nu_rate", C.c_double),
         ("eos_kind", C.c_int * 2), ("mg", (C.c_double * 7) * 2), ("mu_rate", C.c_double),
         ("nu_law", C.c_int), ("nu_cr", C.c_double), ("nu_pc", C.c_double),
+        ("relax_form", C.c_int),
     ]
 
 
@@ -68,6 +69,8 @@ def lib():
         L = C.CDLL(LIB_PATH)
         vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
         L.ora_create.restype = vp
+        L.ora_pexpm1.restype = C.c_double
+        L.ora_pexpm1.argtypes = [C.c_double]
         L.ora_psat.restype = C.c_double
         L.ora_psat.argtypes = [vp, C.c_double]
         L.ora_nu_law.restype = C.c_double

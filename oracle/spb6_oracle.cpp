@@ -103,6 +103,15 @@ static double plog(double x) {
 	return de * LN2_HI + (logm + de * LN2_LO);
 }
 static double ppow(double x, double y) { return pexp(y * plog(x)); }
+// e^x - 1: the Taylor series to x^17 / 17! by Horner on |x| < 0.5 (the next term is below 2^-70 relative), else
+// pexp(x) - 1 (no cancellation beyond 2 bits there); shared bit for bit with the device
+static double pexpm1(double x) {
+	if (!(std::fabs(x) < 0.5)) return pexp(x) - 1.0;
+	double f = 1.0; for (int k = 2; k <= 17; k++) f *= (double)k;   // 17! is exact in FP64
+	double q = 1.0 / f;
+	for (int k = 16; k >= 1; k--) { f /= (double)(k + 1); q = q * x + 1.0 / f; }
+	return q * x;
+}
 static double phypot(double a, double b) { return std::sqrt(a * a + b * b); }
 
 // ------------------------------------------------------------------------------------------------ options / model
@@ -122,7 +131,9 @@ struct Model {
 	double theta_rate = 0.0, nu_rate = 0.0;
 	double mu_rate = std::numeric_limits<double>::infinity();   // the mechanical relaxation rate (PHY:33: inf = instantaneous)
 	int nu_law = 0; double nu_cr = 1.0, nu_pc = 22.064e6;           // the state-dependent transfer rate (NU_LAW, NU_CR, NU_PC; PHY:36)
+	int relax_form = 0;   // RELAX_FORM (PHY:39-41): 0 backward Euler on the stage's step, 1 the exponential solution
 	double exp_(double x) const { return portable ? pexp(x) : std::exp(x); }
+	double expm1_(double x) const { return portable ? pexpm1(x) : std::expm1(x); }
 	double log_(double x) const { return portable ? plog(x) : std::log(x); }
 	double pow_(double x, double y) const { return portable ? ppow(x, y) : std::pow(x, y); }
 	double hypot_(double a, double b) const { return portable ? phypot(a, b) : std::hypot(a, b); }
@@ -938,7 +949,8 @@ static bool closure_theta(const Model& M, const ClIn& in, ClOut& o) {
 	auto eval = [&](double p, double a[2], double& dQ) {
 		double a0[2], T0[2];
 		for (int k = 0; k < 2; k++) { a0[k] = alpha_p(M, c, in, k, p); T0[k] = T_amp(M, k, a0[k], in.mtr[k], p); }
-		if (std::isinf(tt)) dQ = (T0[1] - T0[0]) / S;
+		if (M.relax_form == 1) dQ = (T0[1] - T0[0]) * (-M.expm1_(-tt * S)) / S;   // PHY:685: T2 - T1 decays by exp(-tt S)
+		else if (std::isinf(tt)) dQ = (T0[1] - T0[0]) / S;
 		else dQ = tt * (T0[1] - T0[0]) / (1.0 + tt * (1.0 / Cp[0] + 1.0 / Cp[1]));
 		a[0] = a0[0] + (A.g - 1.0) * dQ / (A.g * (p + A.pi));
 		a[1] = a0[1] - (B.g - 1.0) * dQ / (B.g * (p + B.pi));
@@ -958,17 +970,31 @@ static bool closure_nu(const Model& M, const ClIn& in, ClOut& o) {
 	const Phase &A = M.PH[0], &B = M.PH[1];
 	const double rho = in.mtr[0] + in.mtr[1], eC = in.rhoeC / rho, nr = in.nu >= 0 ? in.nu : M.nu_rate, tn = std::isinf(nr) ? INF : in.tau * nr;
 	auto Yof = [&](double p, double& Y, double& T) -> bool {
+		auto G = [&](double y) { const double TT = T_from_energy(M, p, y, eC); if (!(TT > 0)) return (double)NAN; return g_pT(M, 1, p, TT) - g_pT(M, 0, p, TT); };
 		auto R = [&](double y) {
 			const double TT = T_from_energy(M, p, y, eC); if (!(TT > 0)) return (double)NAN;
-			if (std::isinf(tn)) return g_pT(M, 1, p, TT) - g_pT(M, 0, p, TT);
+			if (M.relax_form == 1 || std::isinf(tn)) return g_pT(M, 1, p, TT) - g_pT(M, 0, p, TT);   // the exponential form's Y_eq first
 			return rho * y - in.mtr[1] + tn * (g_pT(M, 1, p, TT) - g_pT(M, 0, p, TT)); };
 		double ya = YMIN, yb = 1.0 - YMIN;
 		const double dq = B.q - A.q, yT = dq != 0.0 ? (eC - A.q) / dq : INF;
 		if (dq > 0) yb = smin(yb, yT * (1.0 - 1e-12)); else if (dq < 0) ya = smax(ya, yT * (1.0 + 1e-12));
 		if (!(ya < yb)) return false;
+		const double Y0 = in.mtr[1] / rho;
+		if (M.relax_form == 1 && tn == 0.0) { Y = smin(smax(Y0, ya), yb); T = T_from_energy(M, p, Y, eC); return T > 0; }   // no transfer
 		const double Ra = R(ya), Rb = R(yb);
 		if (!std::isfinite(Ra) || !std::isfinite(Rb)) return false;
 		if (Ra >= 0) Y = ya; else if (Rb <= 0) Y = yb; else if (!brent(R, ya, yb, Ra, Rb, 1e-16, Y)) return false;
+		if (M.relax_form == 1 && !std::isinf(tn)) {   // PHY:733-745: Y = Y_eq + (Y0 - Y_eq) exp(-tn D / rho) along the secant D of G
+			const double Yeq = Y, Ys = smin(smax(Y0, ya), yb);
+			const double D = Ys != Yeq ? G(Ys) / (Ys - Yeq) : 0.0;
+			if (D > 0 && std::isfinite(D)) Y = smin(smax(Yeq + (Y0 - Yeq) * M.exp_(-(tn * D) / rho), ya), yb);
+			else if (std::fabs(Ys - Yeq) <= 1e-14) Y = Yeq;
+			else {   // G not monotone (outside B6.5's assumption): the backward-Euler root
+				auto Rbe = [&](double y) { const double g = G(y); return std::isfinite(g) ? rho * y - in.mtr[1] + tn * g : (double)NAN; };
+				const double ra = Rbe(ya), rb = Rbe(yb); if (!std::isfinite(ra) || !std::isfinite(rb)) return false;
+				if (ra >= 0) Y = ya; else if (rb <= 0) Y = yb; else if (!brent(Rbe, ya, yb, ra, rb, 1e-16, Y)) return false;
+			}
+		}
 		T = T_from_energy(M, p, Y, eC); return T > 0;
 	};
 	auto Phi = [&](double p) { double Y, T; if (!Yof(p, Y, T)) return (double)NAN; return rho * ((1 - Y) * v_pT(M, 0, p, T) + Y * v_pT(M, 1, p, T)) - 1.0; };
@@ -1059,6 +1085,7 @@ struct Cfg {
 	double mg[2][7];              // a Mie-Grueneisen phase's G, c1, c2, e1, e2, rr, e0
 	double mu_rate;               // mu_rate (PHY:33); <= 0: instantaneous (inf, the default)
 	int nu_law; double nu_cr, nu_pc;   // NU_LAW, NU_CR, NU_PC (PHY:36); nu_cr, nu_pc <= 0: 1, 22.064e6
+	int relax_form;               // RELAX_FORM (PHY:41): 1 the exponential form of the theta and nu closures
 };
 static Phase phase_of(const double* ph, int kind, const double* mg) {
 	Phase P{ph[0], ph[1], ph[2], ph[3], ph[4], ph[5]};
@@ -1105,6 +1132,7 @@ static void init_ctx(Ctx& X, const Cfg& C) {
 	X.M.theta_rate = C.theta_rate < 0 ? (C.relax >= 1 ? INF : 0.0) : C.theta_rate;                 // PC:1148-1149
 	X.M.nu_rate = C.nu_rate < 0 ? ((C.relax == 2 || C.relax == 3) ? INF : 0.0) : C.nu_rate;
 	X.M.mu_rate = C.mu_rate > 0 ? C.mu_rate : INF;
+	X.M.relax_form = C.relax_form;
 	X.M.nu_law = C.nu_law; X.M.nu_cr = C.nu_cr > 0 ? C.nu_cr : 1.0; X.M.nu_pc = C.nu_pc > 0 ? C.nu_pc : 22.064e6;
 	if (C.nu_law) X.M.nu_rate = 0.0;   // PC:1150: the chemical cells take finite-rate closures, each with its own rate
 	if (C.relax >= 2) { build_sat(X.M, X.SAT); X.M.sat = &X.SAT; }
@@ -2237,6 +2265,7 @@ int ora_set_threads(int n) { if (n > 0) omp_set_num_threads(n); return omp_get_m
 // such a phase, PHY:790), an IMEX variant or the fused profile form (both written for NASG phases only)
 void* ora_create(const Cfg* C) {
 	if ((C->eos_kind[0] || C->eos_kind[1]) && (C->relax || C->variant || C->shock_fused || C->mu_rate > 0)) return nullptr;
+	if (C->relax_form == 1 && C->mu_rate > 0) return nullptr;   // the mu closure's exponential form is not restated
 	Ctx* X = new Ctx(); init_ctx(*X, *C); return X; }
 void ora_destroy(void* h) { delete (Ctx*)h; }
 size_t ora_cfg_size(void) { return sizeof(Cfg); }
@@ -2411,6 +2440,7 @@ int ora_closure_typed(const OraModel* m, const double* in, int type, double tau,
 	return 1;
 }
 // the context's saturation pressure psat(T) and the nu law's rate at (p, T, alpha_v) (NaN / 0 without the table)
+double ora_pexpm1(double x) { return pexpm1(x); }
 double ora_psat(void* h, double T) { Ctx& X = *(Ctx*)h; return psat(X.M, T); }
 double ora_nu_law(void* h, double p, double T, double av) { Ctx& X = *(Ctx*)h; return nu_law(X.M, p, T, av); }
 // the nu law's rates of the last activation (n interior cells; -1 without the law)

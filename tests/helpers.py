@@ -15,7 +15,8 @@ def config_from_oracle(ocfg, **over):
                     jreuse=ocfg.jreuse, variant=ocfg.variant, relax=ocfg.relax, eps_i=ocfg.eps_i,
                     shock_fused=ocfg.shock_fused, theta_rate=max(ocfg.theta_rate, 0.0), nu_rate=max(ocfg.nu_rate, 0.0),
                     eos_kind=tuple(ocfg.eos_kind), mg=[tuple(ocfg.mg[k][q] for q in range(7)) for k in range(2)],
-                    mu_rate=max(ocfg.mu_rate, 0.0), nu_law=ocfg.nu_law, nu_cr=ocfg.nu_cr, nu_pc=ocfg.nu_pc)
+                    mu_rate=max(ocfg.mu_rate, 0.0), nu_law=ocfg.nu_law, nu_cr=ocfg.nu_cr, nu_pc=ocfg.nu_pc,
+                    relax_form=ocfg.relax_form)
     for k, v in over.items():
         setattr(c, k, v)
     return c
@@ -34,6 +35,7 @@ def oracle_cfg_from(scfg, O, **over):
     c.nu_rate = scfg.nu_rate if scfg.nu_rate > 0 else -1.0
     c.mu_rate = scfg.mu_rate if scfg.mu_rate > 0 else -1.0
     c.nu_law, c.nu_cr, c.nu_pc = scfg.nu_law, scfg.nu_cr, scfg.nu_pc
+    c.relax_form = scfg.relax_form
     for k in range(2):
         c.eos_kind[k] = scfg.eos_kind[k]
         for q in range(7):

@@ -280,3 +280,23 @@ def test_nu_law_bitwise(relax):
     scfg = config_from_oracle(ocfg, relax=relax, nu_law=1)
     o, st = _pair_run(scfg, U, 60, activate=True)
     _check(o, st, f"nasg1000_r{relax}_nulaw")
+
+
+@pytest.mark.parametrize("rates", [(1, 2000.0, 0.0), (2, 0.0, 1e-4)], ids=["theta2000", "nu1e-4"])
+def test_exponential_form_bitwise(rates):
+    """the exponential form of the theta and nu closures (RELAX_FORM 1) on Pelanti's NASG tube (5000 cells, to t_end),
+    the portable e^x - 1 and e^x shared with the oracle: bit-identical to the oracle's device mode"""
+    from diffusionimex_amd import spb6
+    from oracle import oracle as O
+    relax, th, nu = rates
+    ocfg, U, tend = O.case_init(30, 5000)
+    scfg = config_from_oracle(ocfg, relax=relax, theta_rate=th, nu_rate=nu, relax_form=1)
+    o = O.Oracle(oracle_cfg_from(scfg, O))
+    o.set_state(U)
+    assert o.run(tend) == 0
+    st = spb6.Stepper(scfg)
+    st.set_state(U)
+    assert st.run(tend) == 0
+    o.L.ora_relax_activate(o.h)
+    assert st.relax_activate() == 0
+    _check(o, st, f"nasg5000_form1_r{relax}")
