@@ -76,7 +76,7 @@ def lib():
         L.ora_nu_law.restype = C.c_double
         L.ora_nu_law.argtypes = [vp, C.c_double, C.c_double, C.c_double]
         L.ora_cell_nu.argtypes = [vp, dp]
-        L.ora_closure_typed.argtypes = [C.POINTER(OraModel), dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp]
+        L.ora_closure_typed.argtypes = [C.POINTER(OraModel), dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, dp]
         L.ora_eos.argtypes = [C.POINTER(OraModel), C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.ora_create.argtypes = [C.POINTER(Cfg)]
         L.ora_destroy.argtypes = [vp]

@@ -896,6 +896,7 @@ static bool closure_mu(const Model& M, const ClIn& in, ClOut& o) {
 	const double w1 = in.Z[0] + in.Z[1] > 0 ? in.Z[1] / (in.Z[0] + in.Z[1]) : 0.5, w2 = 1.0 - w1;
 	const double tm = std::isinf(M.mu_rate) ? INF : in.tau * M.mu_rate;
 	const Phase &A = M.PH[0], &B = M.PH[1];
+	double det = 0.0;   // the last solve's determinant
 	auto solve = [&](double R, double& p1, double& p2) -> bool {
 		const double a1 = ak[0] + R, a2 = ak[1] - R;
 		const double C1 = dedp_am(M, 0, a1, in.mtr[0]) + om[0] * in.s, C2 = dedp_am(M, 1, a2, in.mtr[1]) + om[1] * in.s;
@@ -903,27 +904,56 @@ static bool closure_mu(const Model& M, const ClIn& in, ClOut& o) {
 		const double M21 = -R * w1 - lam[1] * in.s * om[0], M22 = C2 - R * w2 - lam[1] * in.s * om[1];
 		const double r1 = in.etr[0] - in.mtr[0] * A.q - A.g * A.pi * dedp_am(M, 0, a1, in.mtr[0]) + lam[0] * D0 + ex[0];
 		const double r2 = in.etr[1] - in.mtr[1] * B.q - B.g * B.pi * dedp_am(M, 1, a2, in.mtr[1]) + lam[1] * D0 + ex[1];
-		const double det = M11 * M22 - M12 * M21; if (!(std::fabs(det) > 0)) return false;
+		det = M11 * M22 - M12 * M21; if (!(std::fabs(det) > 0)) return false;
 		p1 = (r1 * M22 - M12 * r2) / det; p2 = (M11 * r2 - M21 * r1) / det; return std::isfinite(p1) && std::isfinite(p2);
 	};
 	double R = 0.0, p1, p2;
+	// the exponential form first finds the equilibrium transfer R_eq, dp(R_eq) = 0 (PHY:601-643), from a start r0 inside
+	// the covolume bounds, its bracket not crossing the 2x2 system's singularity (det keeps its sign)
+	const bool eq = M.relax_form == 1 || std::isinf(tm);
 	if (tm > 0) {
+		double det0 = 0.0, r0 = 0.0;
+		if (eq) {
+			const double Rlo = in.mtr[0] * A.b - ak[0], Rhi = ak[1] - in.mtr[1] * B.b;
+			if (!(Rlo < Rhi)) return false;
+			if (!(Rlo < 0.0 && 0.0 < Rhi)) r0 = Rlo >= 0.0 ? Rlo + 1e-9 * (Rhi - Rlo) : Rhi - 1e-9 * (Rhi - Rlo);
+		}
 		auto Fa = [&](double r, double& f) -> bool {
 			double q1, q2; if (!solve(r, q1, q2)) return false;
 			const double a1 = ak[0] + r, a2 = ak[1] - r;
 			if (!(a1 > in.mtr[0] * A.b && a2 > in.mtr[1] * B.b)) return false;
-			if (!(q1 + A.pi > 0 && q2 + B.pi > 0)) return false;
-			f = r - tm * (q1 - q2); return std::isfinite(f); };
-		double f0; if (!Fa(0.0, f0)) return false;
+			if (eq) { if (r != r0 && !(det * det0 > 0)) return false; }
+			else if (!(q1 + A.pi > 0 && q2 + B.pi > 0)) return false;
+			f = eq ? q2 - q1 : r - tm * (q1 - q2); return std::isfinite(f); };
+		double f0; if (!Fa(r0, f0)) return false;
+		det0 = det;
+		if (eq && std::fabs(f0) <= 1e-13 * (std::fabs(in.pguess) + A.g * std::fabs(A.pi) + B.g * std::fabs(B.pi))) f0 = 0.0;
+		R = r0;
 		if (f0 != 0.0) {
-			double hi = -f0, lo = 0.0, flo = f0, fhi = NAN; bool ok = false;
+			double hi = -f0;
+			if (eq) {   // the acoustic linearisation dp(0) / sum_k rho_k c_k^2 / a_k (half the compressed phase's volume if inadmissible)
+				double q[2], k = 0.0; solve(r0, q[0], q[1]);
+				for (int jj = 0; jj < 2; jj++) { const double a = ak[jj] + (jj == 0 ? r0 : -r0), r = in.mtr[jj] / a; k += r * c2_rp(M, jj, r, q[jj]) / a; }
+				hi = -f0 / k;
+				if (!(std::isfinite(hi) && hi * -f0 > 0)) hi = f0 < 0 ? 0.5 * (ak[1] - r0 - in.mtr[1] * B.b) : -0.5 * (ak[0] + r0 - in.mtr[0] * A.b);
+				hi += r0;
+			}
+			double lo = r0, flo = f0, fhi = NAN; bool ok = false, exact = false;
 			for (int it = 0; it < 200; it++) {
-				if (Fa(hi, fhi)) { if (fhi * flo < 0) { ok = true; break; } lo = hi; flo = fhi; hi = lo + (lo == 0.0 ? -f0 : lo); }
-				else hi = 0.5 * (lo + hi);
+				if (Fa(hi, fhi)) {
+					if (eq && fhi == 0.0) { ok = exact = true; break; }
+					if (fhi * flo < 0) { ok = true; break; }
+					lo = hi; flo = fhi; hi = lo + (lo - r0 == 0 ? -f0 : (lo - r0));
+				} else hi = 0.5 * (lo + hi);
 			}
 			if (!ok) return false;
 			auto F = [&](double r) { double f; return Fa(r, f) ? f : (double)NAN; };
-			if (!brent(F, smin(lo, hi), smax(lo, hi), lo < hi ? flo : fhi, lo < hi ? fhi : flo, 1e-16 * (std::fabs(lo) + std::fabs(hi)) + 1e-300, R)) return false;
+			if (exact) R = hi;
+			else if (!brent(F, smin(lo, hi), smax(lo, hi), lo < hi ? flo : fhi, lo < hi ? fhi : flo, 1e-16 * (std::fabs(lo) + std::fabs(hi)) + 1e-300, R)) return false;
+		}
+		if (M.relax_form == 1 && !std::isinf(tm)) {   // PHY:644-647: R = R_eq (1 - exp(-tm D)) along the secant D = dp(0) / R_eq
+			const double D = R != 0.0 && r0 == 0.0 ? -f0 / R : 0.0;
+			if (D > 0 && std::isfinite(D)) R = -R * M.expm1_(-tm * D);
 		}
 	}
 	if (!solve(R, p1, p2)) return false;
@@ -1085,7 +1115,7 @@ struct Cfg {
 	double mg[2][7];              // a Mie-Grueneisen phase's G, c1, c2, e1, e2, rr, e0
 	double mu_rate;               // mu_rate (PHY:33); <= 0: instantaneous (inf, the default)
 	int nu_law; double nu_cr, nu_pc;   // NU_LAW, NU_CR, NU_PC (PHY:36); nu_cr, nu_pc <= 0: 1, 22.064e6
-	int relax_form;               // RELAX_FORM (PHY:41): 1 the exponential form of the theta and nu closures
+	int relax_form;               // RELAX_FORM (PHY:41): 1 the exponential form of the finite-rate closures
 };
 static Phase phase_of(const double* ph, int kind, const double* mg) {
 	Phase P{ph[0], ph[1], ph[2], ph[3], ph[4], ph[5]};
@@ -2265,7 +2295,6 @@ int ora_set_threads(int n) { if (n > 0) omp_set_num_threads(n); return omp_get_m
 // such a phase, PHY:790), an IMEX variant or the fused profile form (both written for NASG phases only)
 void* ora_create(const Cfg* C) {
 	if ((C->eos_kind[0] || C->eos_kind[1]) && (C->relax || C->variant || C->shock_fused || C->mu_rate > 0)) return nullptr;
-	if (C->relax_form == 1 && C->mu_rate > 0) return nullptr;   // the mu closure's exponential form is not restated
 	Ctx* X = new Ctx(); init_ctx(*X, *C); return X; }
 void ora_destroy(void* h) { delete (Ctx*)h; }
 size_t ora_cfg_size(void) { return sizeof(Cfg); }
@@ -2428,9 +2457,9 @@ double ora_closure_phi(const OraModel* m, const double* in, double p) {
 }
 int ora_shock_share(const OraModel* m, const double* Y, const double* vpre, double p0, double p1, double* lam, double* q) { return shock_share(mk_model(m), Y, vpre, p0, p1, lam, *q) ? 1 : 0; }
 // the closure of a given type at finite rates for the tests: in as ora_closure; type (CL_P 0, CL_MU 1, ...), tau, the
-// impedances Z0, Z1 and mu_rate (<= 0: inf); out: a0 a1 m0 m1 e0 e1 pbar p0 p1
-int ora_closure_typed(const OraModel* m, const double* in, int type, double tau, double Z0, double Z1, double mu, double* out) {
-	Model M = mk_model(m); M.mu_rate = mu > 0 ? mu : INF;
+// impedances Z0, Z1, mu_rate (<= 0: inf) and the relaxation form; out: a0 a1 m0 m1 e0 e1 pbar p0 p1
+int ora_closure_typed(const OraModel* m, const double* in, int type, double tau, double Z0, double Z1, double mu, int form, double* out) {
+	Model M = mk_model(m); M.mu_rate = mu > 0 ? mu : INF; M.relax_form = form;
 	ClIn c; c.atr[0] = in[0]; c.atr[1] = in[1]; c.mtr[0] = in[2]; c.mtr[1] = in[3]; c.etr[0] = in[4]; c.etr[1] = in[5];
 	c.s = in[6]; c.rhoeC = in[7]; c.pguess = in[8]; c.Qph[0] = in[9]; c.Qph[1] = in[10];
 	c.type = type; c.tau = tau; c.Z[0] = Z0; c.Z[1] = Z1;

@@ -6,14 +6,15 @@ R satisfies R = tau mu (p1 - p2) with the two phase pressures it returns, the en
 leaves the transported state (R = 0), and mu -> inf converges to the p closure's equilibrium; whole runs converge to
 the instantaneous run as mu grows."""
 import ctypes as C
+import math
 
 import numpy as np
 import pytest
 
 
-def _closure(O, m, inp, typ, tau=0.0, Z=(1.0, 1.0), mu=-1.0):
+def _closure(O, m, inp, typ, tau=0.0, Z=(1.0, 1.0), mu=-1.0, form=0):
     out = (C.c_double * 9)()
-    ok = O.lib().ora_closure_typed(C.byref(m), (C.c_double * 11)(*inp), typ, tau, Z[0], Z[1], mu, out)
+    ok = O.lib().ora_closure_typed(C.byref(m), (C.c_double * 11)(*inp), typ, tau, Z[0], Z[1], mu, form, out)
     return ok, np.array(out[:])
 
 
@@ -102,3 +103,25 @@ def test_mu_whole_run_converges_to_instantaneous(oracle_lib):
     assert dists[2] < 1e-8
     D = run(1e3)
     assert np.max(np.abs(D[:, 12] - D[:, 13]) / np.abs(D[:, 0])) > 1e-6
+
+
+def test_mu_closure_exponential_form(oracle_lib):
+    """the exponential form (PHY:601-647): the transfer is R = R_eq (1 - exp(-tau mu D)) along the secant D = dp(0) / R_eq,
+    R_eq the equilibrium transfer (the p closure's, with no shock budget and no strain), dp(0) the transported state's
+    pressure difference; it tends to R_eq as mu grows and to tau mu dp(0) as mu vanishes"""
+    O = oracle_lib
+    m = _model(O)
+    rng = np.random.default_rng(11)
+    for _ in range(30):
+        inp = _state(O, m, rng)
+        okp, op = _closure(O, m, inp, 0)
+        assert okp
+        Req = op[0] - inp[0]
+        ok0, o0 = _closure(O, m, inp, 1, 1e-6, (1.5e6, 408.0), 1e-12)   # R ~ 0: the transported pressures
+        dp0 = o0[7] - o0[8]
+        for mu in (1e2, 1e4, 1e6):
+            ok, o = _closure(O, m, inp, 1, 1e-6, (1.5e6, 408.0), mu, form=1)
+            assert ok
+            tmD = 1e-6 * mu * dp0 / Req
+            want = Req * -math.expm1(-tmD)
+            assert abs((o[0] - inp[0]) - want) <= 1e-9 * abs(Req) + 1e-15, (mu, o[0] - inp[0], want)
