@@ -820,8 +820,8 @@ spb6_ctx* spb6_create(const spb6_config_t* cfg, int device) {
 		if (cfg->eos_kind[k] < 0 || cfg->eos_kind[k] > 2) { g_err = "eos_kind must be 0 (NASG), 1 (Cochran-Chan) or 2 (JWL)"; return nullptr; }
 		if (cfg->eos_kind[k] && !(cfg->mg[k][0] > 0 && cfg->mg[k][5] > 0)) { g_err = "a Mie-Grueneisen phase needs G > 0 and rr > 0"; return nullptr; }
 	}
-	if (cfg->relax_form < 0 || cfg->relax_form > 1 || (cfg->relax_form == 1 && cfg->mu_rate > 0)) {
-		g_err = "relax_form is 0 (backward Euler) or 1 (the exponential form of the theta and nu closures; not with a finite mu)"; return nullptr; }
+	if (cfg->relax_form < 0 || cfg->relax_form > 1) {
+		g_err = "relax_form is 0 (backward Euler) or 1 (the exponential form of the finite-rate closures)"; return nullptr; }
 	if (cfg->nu_law && !(cfg->relax == 2 || cfg->relax == 3)) { g_err = "the nu law needs relaxation level 2 or 3"; return nullptr; }
 	if (cfg->mu_rate > 0 && (cfg->analytic_jacobian || cfg->learned_guess || cfg->variant != SPB6_VARIANT_PC || cfg->eos_kind[0] ||
 	                         cfg->eos_kind[1])) {

@@ -1327,7 +1327,7 @@ SPB_HD bool closure_nu(const Phys& P, double nu, const ClIn& in, ClOut& o, int f
 // impedance weights w_k splitting the transfer's work); R is the root of F(R) = R - tau mu (p1 - p2)(R), bracketed from
 // F(0) by steps of the first estimate (an inadmissible trial halves back towards the last good point), then Brent.
 // pbar = alpha_1 p1 + alpha_2 p2. Z0, Z1: the cell's frozen impedances.
-SPB_HD bool closure_mu(const Phys& P, double mu, const ClIn& in, double Z0, double Z1, ClOut& o) {
+SPB_HD bool closure_mu(const Phys& P, double mu, const ClIn& in, double Z0, double Z1, ClOut& o, int form = 0) {
 	if (!(in.atr0 > 0) || !(in.mtr0 > 0) || !isfin(in.etr0) || !(in.atr1 > 0) || !(in.mtr1 > 0) || !isfin(in.etr1)) return false;
 	if (!isfin(in.rhoeC) || !isfin(in.s)) return false;
 	const double sa = in.atr0 + in.atr1;
@@ -1342,6 +1342,7 @@ SPB_HD bool closure_mu(const Phys& P, double mu, const ClIn& in, double Z0, doub
 	defect_seg(D0 + in.pguess * in.s, in.Q0 + in.Q1, in.Q0, in.Q1, lE0, lE1, lam0, lam1, ex0, ex1);
 	const double w1 = Z0 + Z1 > 0 ? Z1 / (Z0 + Z1) : 0.5, w2 = 1.0 - w1;
 	const double tm = __builtin_isinf(mu) ? dinf() : in.tau * mu;
+	double det = 0.0;   // the last solve's determinant
 	auto solve = [&](double R, double& p1, double& p2) -> bool {
 		const double a1 = ak0 + R, a2 = ak1 - R;
 		const double C1 = dedp_am(P, 0, a1, in.mtr0) + om0 * in.s, C2 = dedp_am(P, 1, a2, in.mtr1) + om1 * in.s;
@@ -1349,27 +1350,57 @@ SPB_HD bool closure_mu(const Phys& P, double mu, const ClIn& in, double Z0, doub
 		const double M21 = -R * w1 - lam1 * in.s * om0, M22 = C2 - R * w2 - lam1 * in.s * om1;
 		const double r1 = in.etr0 - in.mtr0 * P.q[0] - P.g[0] * P.pi[0] * dedp_am(P, 0, a1, in.mtr0) + lam0 * D0 + ex0;
 		const double r2 = in.etr1 - in.mtr1 * P.q[1] - P.g[1] * P.pi[1] * dedp_am(P, 1, a2, in.mtr1) + lam1 * D0 + ex1;
-		const double det = M11 * M22 - M12 * M21; if (!(fabs(det) > 0)) return false;
+		det = M11 * M22 - M12 * M21; if (!(fabs(det) > 0)) return false;
 		p1 = (r1 * M22 - M12 * r2) / det; p2 = (M11 * r2 - M21 * r1) / det; return isfin(p1) && isfin(p2);
 	};
 	double R = 0.0, p1, p2;
+	// the exponential form first finds the equilibrium transfer R_eq, dp(R_eq) = 0 (PHY:601-643), from a start r0 inside
+	// the covolume bounds, its bracket not crossing the 2x2 system's singularity (det keeps its sign)
+	const bool eq = form == 1 || __builtin_isinf(tm);
 	if (tm > 0) {
+		double det0 = 0.0, r0 = 0.0;
+		if (eq) {
+			const double Rlo = in.mtr0 * P.b[0] - ak0, Rhi = ak1 - in.mtr1 * P.b[1];
+			if (!(Rlo < Rhi)) return false;
+			if (!(Rlo < 0.0 && 0.0 < Rhi)) r0 = Rlo >= 0.0 ? Rlo + 1e-9 * (Rhi - Rlo) : Rhi - 1e-9 * (Rhi - Rlo);
+		}
 		auto Fa = [&](double r, double& f) -> bool {
 			double q1, q2; if (!solve(r, q1, q2)) return false;
 			const double a1 = ak0 + r, a2 = ak1 - r;
 			if (!(a1 > in.mtr0 * P.b[0] && a2 > in.mtr1 * P.b[1])) return false;
-			if (!(q1 + P.pi[0] > 0 && q2 + P.pi[1] > 0)) return false;
-			f = r - tm * (q1 - q2); return isfin(f); };
-		double f0; if (!Fa(0.0, f0)) return false;
+			if (eq) { if (r != r0 && !(det * det0 > 0)) return false; }
+			else if (!(q1 + P.pi[0] > 0 && q2 + P.pi[1] > 0)) return false;
+			f = eq ? q2 - q1 : r - tm * (q1 - q2); return isfin(f); };
+		double f0; if (!Fa(r0, f0)) return false;
+		det0 = det;
+		if (eq && fabs(f0) <= 1e-13 * (fabs(in.pguess) + P.g[0] * fabs(P.pi[0]) + P.g[1] * fabs(P.pi[1]))) f0 = 0.0;
+		R = r0;
 		if (f0 != 0.0) {
-			double hi = -f0, lo = 0.0, flo = f0, fhi = dnan(); bool ok = false;
+			double hi = -f0;
+			if (eq) {   // the acoustic linearisation dp(0) / sum_k rho_k c_k^2 / a_k (half the compressed phase's volume if inadmissible)
+				double q0, q1, k = 0.0; solve(r0, q0, q1);
+				{ const double a = ak0 + r0, r = in.mtr0 / a; k += r * c2_rp(P, 0, r, q0) / a; }
+				{ const double a = ak1 - r0, r = in.mtr1 / a; k += r * c2_rp(P, 1, r, q1) / a; }
+				hi = -f0 / k;
+				if (!(isfin(hi) && hi * -f0 > 0)) hi = f0 < 0 ? 0.5 * (ak1 - r0 - in.mtr1 * P.b[1]) : -0.5 * (ak0 + r0 - in.mtr0 * P.b[0]);
+				hi += r0;
+			}
+			double lo = r0, flo = f0, fhi = dnan(); bool ok = false, exact = false;
 			for (int it = 0; it < 200; it++) {
-				if (Fa(hi, fhi)) { if (fhi * flo < 0) { ok = true; break; } lo = hi; flo = fhi; hi = lo + (lo == 0.0 ? -f0 : lo); }
-				else hi = 0.5 * (lo + hi);
+				if (Fa(hi, fhi)) {
+					if (eq && fhi == 0.0) { ok = exact = true; break; }
+					if (fhi * flo < 0) { ok = true; break; }
+					lo = hi; flo = fhi; hi = lo + (lo - r0 == 0 ? -f0 : (lo - r0));
+				} else hi = 0.5 * (lo + hi);
 			}
 			if (!ok) return false;
 			auto F = [&](double r) { double f; return Fa(r, f) ? f : dnan(); };
-			if (!brent_root(F, smin(lo, hi), smax(lo, hi), lo < hi ? flo : fhi, lo < hi ? fhi : flo, 1e-16 * (fabs(lo) + fabs(hi)) + 1e-300, R)) return false;
+			if (exact) R = hi;
+			else if (!brent_root(F, smin(lo, hi), smax(lo, hi), lo < hi ? flo : fhi, lo < hi ? fhi : flo, 1e-16 * (fabs(lo) + fabs(hi)) + 1e-300, R)) return false;
+		}
+		if (form == 1 && !__builtin_isinf(tm)) {   // PHY:644-647: R = R_eq (1 - exp(-tm D)) along the secant D = dp(0) / R_eq
+			const double D = R != 0.0 && r0 == 0.0 ? -f0 / R : 0.0;
+			if (D > 0 && isfin(D)) R = -R * pexpm1(-tm * D);
 		}
 	}
 	if (!solve(R, p1, p2)) return false;
@@ -1401,7 +1432,7 @@ SPB_HD double nu_law(const Phys& P, const RelaxDev& S, double p, double T, doubl
 // nu: the cell's own transfer rate of the step (>= 0 replaces the context's; the nu law)
 SPB_HD bool closure_of(const Phys& P, const RelaxDev* S, int ty, const ClIn& in, ClOut& o, double Z0 = 1.0, double Z1 = 1.0, double nu = -1.0) {
 	switch (ty) {
-	case CL_MU: return closure_mu(P, S->mu, in, Z0, Z1, o);
+	case CL_MU: return closure_mu(P, S->mu, in, Z0, Z1, o, S->form);
 	case CL_PT: return closure_pT(P, in, o);
 	case CL_THETA: return closure_theta(P, S->theta, in, o, S->form);
 	case CL_NU: return closure_nu(P, nu >= 0 ? nu : S->nu, in, o, S->form);

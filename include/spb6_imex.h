@@ -132,9 +132,9 @@ typedef struct spb6_config {
                                         attempt's activation from the cell's pT equilibrium (PC:546-556) */
     double nu_cr, nu_pc;             /* NU_CR, NU_PC (PHY:36); <= 0: 1 and 22.064e6 */
     int relax_form;                  /* RELAX_FORM (PHY:39-41): 0 backward Euler over the stage's step (the default); 1 the
-                                        exponential solution of the theta and nu closures (PHY:685, PHY:721-745: T2 - T1
-                                        decays by exp(-tau theta S); Y moves towards Y_eq along the secant of g2 - g1);
-                                        not with a finite mu */
+                                        exponential solution of the finite-rate closures (PHY:685, PHY:721-745, PHY:644-647:
+                                        T2 - T1 decays by exp(-tau theta S); Y and the volume transfer move towards their
+                                        equilibria along secants) */
 } spb6_config_t;
 
 /* run statistics: the counters of the reference's stats line (PC:1337-1339) plus Krylov counts */

@@ -300,3 +300,13 @@ def test_exponential_form_bitwise(rates):
     o.L.ora_relax_activate(o.h)
     assert st.relax_activate() == 0
     _check(o, st, f"nasg5000_form1_r{relax}")
+
+
+def test_mu_exponential_form_bitwise():
+    """the mu closure's exponential form (the equilibrium transfer, then R = R_eq (1 - exp(-tau mu D))) at level 0 with
+    mu = 1e3 on the 2 m/s cavitation tube (800 cells, 60 steps): bit-identical to the oracle's device mode"""
+    from oracle import oracle as O
+    ocfg, U, _ = O.case_init(1, 800)
+    scfg = config_from_oracle(ocfg, mu_rate=1e3, relax_form=1)
+    o, st = _pair_run(scfg, U, 60)
+    _check(o, st, "tube800_mu1e3_form1")
